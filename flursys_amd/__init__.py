@@ -1,0 +1,17 @@
+"""MI355X-native pressure-projection path of FLURSYS.
+
+The numerics run in libflursys_hip.so (hand-written gfx950 HIP kernels behind
+the C ABI in include/flursys_hip.h); this package is the host-side mirror of
+the reference's solver interfaces over that ABI.
+"""
+
+from ._lib import (GUESS_PREVIOUS, GUESS_ZERO, PRECISION_FP64, PRECISION_MIXED,
+                   STOP_ABS_INF, STOP_REL_L2, Context, FlursysError, default_context, load)
+from .cavity3d import LidDrivenCavity3DConfig, LidDrivenCavity3DSolver, RunSummary3D
+
+__all__ = [
+    "Context", "FlursysError", "default_context", "load",
+    "LidDrivenCavity3DConfig", "LidDrivenCavity3DSolver", "RunSummary3D",
+    "STOP_ABS_INF", "STOP_REL_L2", "GUESS_PREVIOUS", "GUESS_ZERO",
+    "PRECISION_FP64", "PRECISION_MIXED",
+]

@@ -1,0 +1,159 @@
+"""Loader and ctypes declarations of libflursys_hip.so (the C ABI in
+include/flursys_hip.h).
+
+The shared library is built in-tree (flursys_amd/csrc/Makefile).  There is no
+fallback: when the library is missing or no HIP device is present, every entry
+point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflursys_hip.so")
+
+# status codes (fl_status)
+OK = 0
+STATUS_NAMES = {
+    1: "DimensionMismatch", 2: "InvalidCsr", 3: "IndexOutOfBounds", 4: "NonFiniteValue",
+    5: "MissingDiagonal", 6: "ZeroDiagonal", 7: "Breakdown", 8: "SolverError",
+    9: "ConfigError", 10: "InvalidArgument", 11: "HipError", 12: "OutOfMemory",
+}
+STOP_ABS_INF, STOP_REL_L2 = 0, 1
+GUESS_PREVIOUS, GUESS_ZERO = 0, 1
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+
+class FlursysError(RuntimeError):
+    """A non-OK fl_status, with the context's error text."""
+
+    def __init__(self, status: int, message: str):
+        self.status = status
+        self.kind = STATUS_NAMES.get(status, str(status))
+        self.message = message
+        super().__init__(f"{self.kind}: {message}")
+
+
+class SolveReport(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64), ("refinements", C.c_int64),
+        ("residual_abs_inf", C.c_double), ("residual_l2", C.c_double),
+        ("initial_l2", C.c_double), ("converged", C.c_int32), ("status", C.c_int32),
+    ]
+
+
+class Cavity3DConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64),
+        ("length", C.c_double), ("height", C.c_double), ("depth", C.c_double),
+        ("density", C.c_double), ("lid_velocity", C.c_double), ("reynolds", C.c_double),
+        ("dt", C.c_double),
+        ("pressure_max_iters", C.c_int64), ("pressure_tolerance", C.c_double),
+        ("stop_rule", C.c_int32), ("initial_guess", C.c_int32), ("precision", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class Cavity3DStepReport(C.Structure):
+    _fields_ = [
+        ("step", C.c_int64), ("time", C.c_double),
+        ("pressure_iterations", C.c_int64), ("pressure_refinements", C.c_int64),
+        ("pressure_residual", C.c_double), ("pressure_residual_l2", C.c_double),
+        ("max_divergence", C.c_double), ("divergence_l2", C.c_double),
+        ("max_speed", C.c_double), ("pressure_converged", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); every symbol include/flursys_hip.h declares.
+SIGNATURES = {
+    "fl_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "fl_destroy": (None, [_P]),
+    "fl_last_error": (C.c_char_p, [_P]),
+    "fl_version": (C.c_char_p, []),
+    "fl_probe_copy": (C.c_int, [_P, C.c_int64, C.c_int32, _D]),
+    "fl_cavity3d_create": (C.c_int, [_P, C.POINTER(Cavity3DConfig), C.POINTER(_P)]),
+    "fl_cavity3d_destroy": (None, [_P]),
+    "fl_cavity3d_step": (C.c_int, [_P, C.POINTER(Cavity3DStepReport)]),
+    "fl_cavity3d_run": (C.c_int, [_P, C.c_int64, C.POINTER(Cavity3DStepReport)]),
+    "fl_cavity3d_pressure_solve": (C.c_int, [_P, _P, _P, C.POINTER(SolveReport)]),
+    "fl_cavity3d_get_fields": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fl_cavity3d_set_fields": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fl_cavity3d_get_intermediate": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fl_cavity3d_set_timing": (C.c_int, [_P, C.c_int32]),
+    "fl_cavity3d_kernel_times": (C.c_int, [_P, _D, _D, _D, _D, C.POINTER(C.c_int64)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libflursys_hip.so and declare its signatures (raises if absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FlursysError(
+                11, f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                "(make -C flursys_amd/csrc)")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(status: int, ctx=None) -> None:
+    if status != OK:
+        msg = ""
+        if ctx is not None:
+            raw = load().fl_last_error(ctx)
+            msg = raw.decode() if raw else ""
+        raise FlursysError(status, msg)
+
+
+def ptr(array):
+    """Raw data pointer of a C-contiguous numpy array (None passes NULL)."""
+    if array is None:
+        return None
+    if not array.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return array.ctypes.data
+
+
+class Context:
+    """fl_ctx: a device binding with its own HIP stream."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = _P()
+        st = lib.fl_create(device, C.byref(h))
+        if st != OK:
+            raise FlursysError(st, f"fl_create(device={device}) failed (is a GPU present?)")
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().fl_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx

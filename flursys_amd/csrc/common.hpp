@@ -1,0 +1,179 @@
+// Shared plumbing of the gfx950 library: the context, error reporting,
+// wave/block reductions and the deterministic one-pass grid reduction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "flursys_hip.h"
+
+struct fl_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 256;
+    std::string last_error;
+};
+
+namespace fl {
+
+constexpr int kBlock = 256;   // threads per workgroup: 4 wave64s
+constexpr int kWaves = kBlock / 64;
+constexpr int kXcds = 8;      // MI355X: 8 XCDs, workgroup b runs on XCD b % 8
+
+inline fl_status set_error(fl_ctx *ctx, fl_status st, const char *fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->last_error = buf;
+    }
+    return st;
+}
+
+#define FL_HIP_TRY(ctx, call)                                                              \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::fl::set_error((ctx), FL_ERR_HIP, "%s failed: %s", #call,              \
+                                   hipGetErrorString(e_));                                 \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Reductions.  Sums use a fixed tree (lane butterfly, then waves in order,
+// then blocks in index order), so a run is bitwise repeatable for a given
+// launch shape.  Maxima use fmax, which ignores a NaN operand exactly like
+// Rust's f64::max in the reference's parallel_max_abs
+// (src/solver/incompressible.rs:1313-1320).
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Block-wide reduction of NS sums and NM maxima.  Result valid in thread 0.
+template <int NS, int NM>
+__device__ inline void block_reduce(double (&s)[NS], double (&m)[NM]) {
+    __shared__ double sh[kWaves][NS + NM + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < NS; ++a) s[a] = wave_sum(s[a]);
+#pragma unroll
+    for (int a = 0; a < NM; ++a) m[a] = wave_max(m[a]);
+    if (lane == 0) {
+#pragma unroll
+        for (int a = 0; a < NS; ++a) sh[wave][a] = s[a];
+#pragma unroll
+        for (int a = 0; a < NM; ++a) sh[wave][NS + a] = m[a];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int a = 0; a < NS; ++a) {
+            double t = sh[0][a];
+            for (int w = 1; w < kWaves; ++w) t += sh[w][a];
+            s[a] = t;
+        }
+#pragma unroll
+        for (int a = 0; a < NM; ++a) {
+            double t = sh[0][NS + a];
+            for (int w = 1; w < kWaves; ++w) t = fmax(t, sh[w][NS + a]);
+            m[a] = t;
+        }
+    }
+}
+
+// One-pass grid reduction: every block publishes its partials, the last block
+// to arrive (device-scope counter) folds them in block order and returns true
+// with the totals in thread 0.  `partials` holds gridDim.x * (NS + NM)
+// doubles; `counter` must be zero on entry and is reset by the last block.
+// NS and NM must both be at least 1 (pass a dummy where one is unused).
+template <int NS, int NM>
+__device__ inline bool grid_reduce(double (&s)[NS], double (&m)[NM], double *partials,
+                                   unsigned *counter) {
+    static_assert(NS >= 1 && NM >= 1, "grid_reduce needs at least one sum and one max");
+    constexpr int K = NS + NM;
+    __shared__ bool is_last;
+    block_reduce<NS, NM>(s, m);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int a = 0; a < NS; ++a) partials[blockIdx.x * K + a] = s[a];
+#pragma unroll
+        for (int a = 0; a < NM; ++a) partials[blockIdx.x * K + NS + a] = m[a];
+        __threadfence();
+        unsigned prev = atomicAdd(counter, 1u);
+        is_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    // Fold the partials: each thread sums a strided subset in index order,
+    // then the block tree combines the threads in a fixed order.
+    double ts[NS], tm[NM];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) ts[a] = 0.0;
+#pragma unroll
+    for (int a = 0; a < NM; ++a) tm[a] = 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+        const volatile double *pb = partials + (size_t)b * K;
+#pragma unroll
+        for (int a = 0; a < NS; ++a) ts[a] += pb[a];
+#pragma unroll
+        for (int a = 0; a < NM; ++a) tm[a] = fmax(tm[a], pb[NS + a]);
+    }
+    __syncthreads();
+    block_reduce<NS, NM>(ts, tm);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int a = 0; a < NS; ++a) s[a] = ts[a];
+#pragma unroll
+        for (int a = 0; a < NM; ++a) m[a] = tm[a];
+        *counter = 0u;
+    }
+    return true;
+}
+
+// Tile scheduling.  Work is cut into 256-cell tiles in linear (x-fastest)
+// order.  When the grid is a multiple of 8 blocks, each XCD (blocks b with
+// b % 8 == x) owns a contiguous run of tiles -- a z-slab of the grid -- so the
+// z+-1 neighbours a stencil reads are, apart from slab edges, in that XCD's
+// own L2.
+struct TileSched {
+    uint32_t first, count, step;
+    __device__ inline TileSched(uint32_t ntiles) {
+        if ((gridDim.x % kXcds) == 0 && gridDim.x >= 8 * kXcds) {
+            const uint32_t xcd = blockIdx.x % kXcds, lb = blockIdx.x / kXcds;
+            const uint32_t per = gridDim.x / kXcds;
+            const uint32_t lo = (uint32_t)(((uint64_t)ntiles * xcd) / kXcds);
+            const uint32_t hi = (uint32_t)(((uint64_t)ntiles * (xcd + 1)) / kXcds);
+            first = lo + lb;
+            count = hi;
+            step = per;
+        } else {
+            first = blockIdx.x;
+            count = ntiles;
+            step = gridDim.x;
+        }
+    }
+};
+
+inline uint32_t grid_blocks(const fl_ctx *ctx, uint64_t ntiles, int per_cu = 8) {
+    uint64_t g = (uint64_t)ctx->num_cu * per_cu;
+    if (ntiles < g) g = ntiles;
+    if (g >= 64) g -= g % kXcds;
+    return (uint32_t)(g < 1 ? 1 : g);
+}
+
+}  // namespace fl

@@ -1,0 +1,90 @@
+// Context lifetime: device binding, the context's own stream, error text.
+#include <algorithm>
+#include <new>
+
+#include "common.hpp"
+
+namespace {
+
+// Plain streaming copy: the calibration kernel for HBM byte counters and the
+// achievable-bandwidth figure the roofline fractions are read against.
+__global__ void __launch_bounds__(256) k_copy(const double2 *__restrict__ src,
+                                              double2 *__restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+fl_status fl_probe_copy(fl_ctx *ctx, int64_t bytes, int32_t iterations, double *ms_per_copy) {
+    if (!ctx || bytes < 16 || iterations < 1 || !ms_per_copy)
+        return fl::set_error(ctx, FL_ERR_INVALID_ARGUMENT, "invalid copy probe arguments");
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)bytes / sizeof(double2);
+    double2 *a = nullptr, *b = nullptr;
+    FL_HIP_TRY(ctx, hipMalloc(&a, n * sizeof(double2)));
+    if (hipMalloc(&b, n * sizeof(double2)) != hipSuccess) {
+        (void)hipFree(a);
+        return fl::set_error(ctx, FL_ERR_OUT_OF_MEMORY, "copy probe allocation failed");
+    }
+    hipEvent_t e0, e1;
+    (void)hipMemsetAsync(a, 0, n * sizeof(double2), ctx->stream);
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)ctx->num_cu * 16);
+    k_copy<<<blocks, 256, 0, ctx->stream>>>(a, b, n);  // warm-up
+    (void)hipEventRecord(e0, ctx->stream);
+    for (int it = 0; it < iterations; ++it) k_copy<<<blocks, 256, 0, ctx->stream>>>(a, b, n);
+    (void)hipEventRecord(e1, ctx->stream);
+    hipError_t err = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (err != hipSuccess) return fl::set_error(ctx, FL_ERR_HIP, "copy probe failed");
+    *ms_per_copy = ms / iterations;
+    return FL_OK;
+}
+
+fl_status fl_create(int device, fl_ctx **out) {
+    if (!out) return FL_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return FL_ERR_HIP;
+    if (device < 0 || device >= count) return FL_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return FL_ERR_HIP;
+    fl_ctx *ctx = new (std::nothrow) fl_ctx();
+    if (!ctx) return FL_ERR_OUT_OF_MEMORY;
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->num_cu = prop.multiProcessorCount;
+    // Default-priority, non-blocking stream (one per context).
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return FL_ERR_HIP;
+    }
+    *out = ctx;
+    return FL_OK;
+}
+
+void fl_destroy(fl_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+const char *fl_last_error(const fl_ctx *ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+const char *fl_version(void) { return "flursys-hip 0.1.0 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,152 @@
+/*
+ * flursys_hip.h -- C ABI of the MI355X (gfx950) pressure-projection library.
+ *
+ * Plain C: opaque handles, POD configs, caller-owned host arrays, status codes.
+ * No exceptions cross this boundary and no torch types appear in it.  One HIP
+ * stream per context; a context and everything created from it may be moved
+ * to another thread but must not be used from two threads at once (the
+ * reference's solver state is `Send`, not `Sync`: src/runtime.rs:117-120).
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * the FLURSYS Rust crate).  Array layouts are the reference's (src/field.rs):
+ *   2D  index i + nx*j        u (nx+1) x ny, v nx x (ny+1), p nx x ny
+ *   3D  index i + nx*(j+ny*k) u (nx+1) x ny x nz, v nx x (ny+1) x nz,
+ *                             w nx x ny x (nz+1), p nx x ny x nz
+ */
+#ifndef FLURSYS_HIP_H
+#define FLURSYS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Status codes.  1..7 mirror LinearAlgebraError (src/linear/mod.rs:17-51);
+ * FL_ERR_SOLVER carries the structured solvers' String errors (text through
+ * fl_last_error), e.g. "PCG pressure solver encountered a singular search
+ * direction" (src/solver/incompressible.rs:1129-1161). */
+typedef enum {
+    FL_OK = 0,
+    FL_ERR_DIMENSION_MISMATCH = 1,
+    FL_ERR_INVALID_CSR = 2,
+    FL_ERR_INDEX_OUT_OF_BOUNDS = 3,
+    FL_ERR_NON_FINITE = 4,
+    FL_ERR_MISSING_DIAGONAL = 5,
+    FL_ERR_ZERO_DIAGONAL = 6,
+    FL_ERR_BREAKDOWN = 7,
+    FL_ERR_SOLVER = 8,
+    FL_ERR_CONFIG = 9,
+    FL_ERR_INVALID_ARGUMENT = 10,
+    FL_ERR_HIP = 11,
+    FL_ERR_OUT_OF_MEMORY = 12
+} fl_status;
+
+/* Stop rule: the reference's absolute max|r| < tol (incompressible.rs:
+ * 1111-1164) or ||r||_2 <= tol * ||r_0||_2 (linear/solver.rs:84-85, atol 0). */
+enum { FL_STOP_ABS_INF = 0, FL_STOP_REL_L2 = 1 };
+/* Initial guess: previous pressure (incompressible.rs:1061) or zero. */
+enum { FL_GUESS_PREVIOUS = 0, FL_GUESS_ZERO = 1 };
+/* Pressure solver.  PCG is the reference's Jacobi-PCG; on the 3D cavity it is
+ * the same algorithm on the pinned 7-point operator (the reference 3D solver
+ * only has SOR, incompressible_3d.rs:293-332). */
+enum { FL_SOLVER_PCG = 0 };
+/* Arithmetic of the pressure solve: FP64 throughout, or an FP32 stencil CG
+ * inside FP64 iterative refinement (FP64 residual, dots, alpha/beta, test). */
+enum { FL_PRECISION_FP64 = 0, FL_PRECISION_MIXED = 1 };
+
+typedef struct fl_ctx fl_ctx;
+
+/* Creates a context bound to HIP device `device` with its own stream. */
+fl_status fl_create(int device, fl_ctx **out);
+void fl_destroy(fl_ctx *ctx);
+/* Text of the last error raised on this context ("" if none). */
+const char *fl_last_error(const fl_ctx *ctx);
+/* Library version string. */
+const char *fl_version(void);
+/* Measurement utility with no reference counterpart: times a plain device
+ * copy of `bytes` (read + write) averaged over `iterations` launches.  It is
+ * the calibration kernel for HBM byte counters and the achievable-bandwidth
+ * figure that roofline fractions are read against. */
+fl_status fl_probe_copy(fl_ctx *ctx, int64_t bytes, int32_t iterations, double *ms_per_copy);
+
+/* ---------------------------------------------------------------------- */
+/* Pressure-solve report: the reference returns (residual, iterations)
+ * (incompressible.rs:972-977); rel-L2 figures are added for the RelL2 rule. */
+typedef struct {
+    int64_t iterations;        /* CG iterations (inner iterations when mixed) */
+    int64_t refinements;       /* outer FP64 refinement sweeps (mixed only) */
+    double residual_abs_inf;   /* max|r| of the recurrence residual at exit */
+    double residual_l2;        /* ||r||_2 at exit */
+    double initial_l2;         /* ||r_0||_2 */
+    int32_t converged;         /* 1 when the stop rule fired */
+    int32_t status;            /* fl_status of the solve */
+} fl_solve_report;
+
+/* ---------------------------------------------------------------------- */
+/* 3D lid-driven cavity.  Mirrors LidDrivenCavity3DConfig
+ * (src/solver/incompressible_3d.rs:8-47) plus the GPU path's knobs. */
+typedef struct {
+    int64_t nx, ny, nz;
+    double length, height, depth;
+    double density, lid_velocity, reynolds, dt;
+    int64_t pressure_max_iters;
+    double pressure_tolerance;     /* AbsInf tolerance, or rtol for RelL2 */
+    int32_t stop_rule;             /* FL_STOP_* */
+    int32_t initial_guess;         /* FL_GUESS_* */
+    int32_t precision;             /* FL_PRECISION_* */
+    int32_t reserved;
+} fl_cavity3d_config;
+
+typedef struct {
+    int64_t step;                  /* steps taken so far */
+    double time;                   /* step * dt */
+    int64_t pressure_iterations;
+    int64_t pressure_refinements;
+    double pressure_residual;      /* max|r| at exit of the solve */
+    double pressure_residual_l2;
+    double max_divergence;         /* incompressible_3d.rs:504-517 */
+    double divergence_l2;          /* ||div u||_2 over cells */
+    double max_speed;              /* incompressible_3d.rs:519-532 */
+    int32_t pressure_converged;
+    int32_t reserved;
+} fl_cavity3d_step_report;
+
+typedef struct fl_cavity3d fl_cavity3d;
+
+/* LidDrivenCavity3DSolver::new (incompressible_3d.rs:117-134): validates like
+ * LidDrivenCavity3DConfig::validate (:49-84) and allocates device fields (all
+ * zero, the solver's initial state). */
+fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavity3d **out);
+void fl_cavity3d_destroy(fl_cavity3d *solver);
+/* advance_one_step (incompressible_3d.rs:164-172): predictor, RHS, pressure
+ * solve, correction, finite check -- all on device. */
+fl_status fl_cavity3d_step(fl_cavity3d *solver, fl_cavity3d_step_report *report);
+/* Steps `steps` times; the report is the last step's. */
+fl_status fl_cavity3d_run(fl_cavity3d *solver, int64_t steps, fl_cavity3d_step_report *report);
+/* Drop-in for solve_pressure_poisson (incompressible_3d.rs:293-332): solves
+ * Lap p = rhs with p(0,0,0) = 0 (rhs[0] is ignored and treated as 0).
+ * `p_inout` is the initial guess when initial_guess is FL_GUESS_PREVIOUS. */
+fl_status fl_cavity3d_pressure_solve(fl_cavity3d *solver, const double *rhs, double *p_inout,
+                                     fl_solve_report *report);
+/* Host <-> device copies of the state (any pointer may be NULL). */
+fl_status fl_cavity3d_get_fields(fl_cavity3d *solver, double *u, double *v, double *w,
+                                 double *p);
+fl_status fl_cavity3d_set_fields(fl_cavity3d *solver, const double *u, const double *v,
+                                 const double *w, const double *p);
+/* The predicted velocities u*, v*, w* and the pressure RHS of the last step. */
+fl_status fl_cavity3d_get_intermediate(fl_cavity3d *solver, double *u_star, double *v_star,
+                                       double *w_star, double *rhs);
+/* Average device time (ms) of the FP64 (or FP32) PCG kernels of the last
+ * solve, measured with HIP events on the solver's stream; zero when timing
+ * is off.  Enables timing when `enable` is non-zero. */
+fl_status fl_cavity3d_set_timing(fl_cavity3d *solver, int32_t enable);
+fl_status fl_cavity3d_kernel_times(fl_cavity3d *solver, double *ms_per_iteration,
+                                   double *ms_apply, double *ms_update, double *ms_direction,
+                                   int64_t *timed_iterations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,202 @@
+/*
+ * CPU oracle for the pressure-projection path: a plain-C (C11 + OpenMP), FP64
+ * restatement of the FLURSYS reference algorithms.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links, loads or
+ * calls this code.  It is used by the tests (as the checker), by the package's
+ * smoke() entry (as the checker) and by bench.py's cpu_baseline leg (timed
+ * beside the GPU, never as the measured product).
+ *
+ * Every function cites the reference file:line it restates.  The reference is
+ * a Rust crate that cannot be built here (no cargo/rustc, crates.io
+ * unreachable, build.rs needs slint-build), so there is no oracle/_ref.
+ *
+ * Layouts mirror src/field.rs:31-34 (2D, i + nx*j) and src/field.rs:104-107
+ * (3D, i + nx*(j + ny*k)) with the staggered extents of the MAC fields:
+ *   2D  u (nx+1) x ny, v nx x (ny+1), p nx x ny
+ *   3D  u (nx+1) x ny x nz, v nx x (ny+1) x nz, w nx x ny x (nz+1), p nx x ny x nz
+ */
+#ifndef FLURSYS_ORACLE_H
+#define FLURSYS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- shared enums (same numeric values as include/flursys_hip.h) ---- */
+enum { OR_STOP_ABS_INF = 0, OR_STOP_REL_L2 = 1 };
+enum { OR_GUESS_PREVIOUS = 0, OR_GUESS_ZERO = 1 };
+enum { OR_SOLVER_PCG = 0, OR_SOLVER_SOR = 1 };
+enum { OR_UPWIND = 0, OR_CENTRAL = 1 };
+enum { OR_BC_VELOCITY = 0, OR_BC_SYMMETRY = 1, OR_BC_OUTLET = 2 };
+enum { OR_SIDE_LEFT = 0, OR_SIDE_RIGHT = 1, OR_SIDE_BOTTOM = 2, OR_SIDE_TOP = 3 };
+
+/* Status codes (mirror src/linear/mod.rs:17-51 plus the structured-solver
+ * string errors, which are reported through `message`). */
+enum {
+    OR_OK = 0,
+    OR_ERR_DIMENSION = 1,
+    OR_ERR_INVALID_CSR = 2,
+    OR_ERR_INDEX_OOB = 3,
+    OR_ERR_NON_FINITE = 4,
+    OR_ERR_MISSING_DIAGONAL = 5,
+    OR_ERR_ZERO_DIAGONAL = 6,
+    OR_ERR_BREAKDOWN = 7,
+    OR_ERR_SOLVER = 8,   /* structured-solver String error */
+    OR_ERR_CONFIG = 9,
+};
+
+typedef struct {
+    int64_t iterations;
+    double residual_abs_inf;   /* max|r| of the recurrence residual at exit */
+    double residual_l2;        /* ||r||_2 at exit */
+    double initial_l2;         /* ||r0||_2 */
+    int32_t converged;         /* 1 if the stop rule fired */
+    int32_t status;            /* OR_OK or an error code */
+    char message[160];
+} or_solve_report;
+
+/* ======================= 3D lid-driven cavity ======================= */
+typedef struct {
+    int64_t nx, ny, nz;
+    double length, height, depth;
+    double density, lid_velocity, reynolds, dt;
+    int64_t pressure_max_iters;
+    double pressure_tolerance;  /* AbsInf tolerance or rtol for RelL2 */
+    double pressure_omega;
+    int32_t solver;             /* OR_SOLVER_PCG (pinned PCG) or OR_SOLVER_SOR (reference) */
+    int32_t stop_rule;
+    int32_t initial_guess;
+    int32_t threads;            /* 0 = OpenMP default */
+} or3d_config;
+
+typedef struct {
+    or3d_config cfg;
+    double dx, dy, dz;
+    int64_t n;                  /* nx*ny*nz */
+    double *p, *rhs, *u, *v, *w, *us, *vs, *ws;
+    double *x, *b, *r, *z, *d, *q;  /* PCG workspace */
+    int64_t step;
+} or3d_state;
+
+int or3d_init(or3d_state *s, const or3d_config *cfg);
+void or3d_free(or3d_state *s);
+void or3d_predict(or3d_state *s);
+void or3d_build_rhs(or3d_state *s);
+int or3d_solve_pressure(or3d_state *s, or_solve_report *rep);
+void or3d_correct(or3d_state *s);
+int or3d_step(or3d_state *s, or_solve_report *rep);
+double or3d_max_divergence(const or3d_state *s);
+double or3d_max_speed(const or3d_state *s);
+double or3d_pressure_residual(const or3d_state *s);
+void or3d_apply_operator(const or3d_state *s, const double *x, double *y);
+int or3d_ensure_finite(const or3d_state *s, char *message, int cap);
+
+/* ======================= 2D structured solver ======================= */
+typedef struct {
+    int64_t nx, ny;
+    double length, height;
+    double density, nu, dt;
+    int64_t pressure_max_iters;
+    double pressure_tolerance;
+    double pressure_omega;
+    int32_t convection;          /* OR_UPWIND / OR_CENTRAL */
+    int32_t solver;              /* OR_SOLVER_PCG / OR_SOLVER_SOR */
+    int32_t stop_rule;
+    int32_t initial_guess;
+    int32_t threads;
+    int32_t bc_kind[4];          /* per side: OR_BC_* */
+    double bc_u[4], bc_v[4];     /* Velocity side values */
+    double outlet_pressure[4];   /* PressureOutlet value per side */
+    /* adaptive time stepping (src/solver/incompressible.rs:35-56) */
+    int32_t adaptive;
+    double min_dt, max_dt, target_cfl, target_diffusion, max_growth;
+    double t_end;
+} or2d_config;
+
+typedef struct {
+    double time_step, pressure_residual, max_divergence, max_speed;
+    double momentum_cfl, viscous_diffusion_number, velocity_change;
+    int64_t pressure_iterations;
+} or2d_diag;
+
+typedef struct {
+    or2d_config cfg;
+    double dx, dy;
+    int64_t n;
+    uint8_t *solid;
+    double *p, *rhs, *u, *v, *us, *vs, *uold, *vold;
+    double *ucell, *vcell, *speed, *vort;
+    double *x, *b, *r, *z, *d, *q;
+    int64_t step;
+    double time, current_dt;
+} or2d_state;
+
+int or2d_init(or2d_state *s, const or2d_config *cfg, const uint8_t *solid);
+void or2d_free(or2d_state *s);
+void or2d_apply_velocity_boundaries(or2d_state *s, double *u, double *v);
+void or2d_predict(or2d_state *s, double dt);
+void or2d_build_rhs(or2d_state *s, double dt);
+int or2d_solve_pressure(or2d_state *s, or_solve_report *rep);
+void or2d_correct(or2d_state *s, double dt);
+void or2d_cell_fields(or2d_state *s);
+int or2d_step(or2d_state *s, or2d_diag *diag, or_solve_report *rep);
+double or2d_max_divergence(const or2d_state *s);
+void or2d_apply_operator(const or2d_state *s, const double *x, double *y);
+double or2d_pressure_residual(const or2d_state *s);
+void or2d_stability_numbers(double dt, double nu, double dx, double dy, double umax,
+                            double vmax, double *cfl, double *dnu);
+int or2d_adaptive_dt(double current_dt, double umax, double vmax, double dx, double dy,
+                     double nu, double thermal_diffusivity, double min_dt, double max_dt,
+                     double target_cfl, double target_diffusion, double max_growth,
+                     double *out_dt, char *message, int cap);
+
+/* ======================= CSR linear algebra ======================= */
+typedef struct {
+    int32_t status;
+    int32_t converged;           /* 1 Converged, 0 MaxIterations */
+    int64_t iterations;
+    double initial_residual, final_residual, relative_residual;
+    int64_t err_index, err_expected, err_actual, err_row, err_col;
+    char reason[96];
+} or_lin_report;
+
+int or_csr_validate(int64_t nrows, int64_t ncols, const int64_t *offsets, const int64_t *cols,
+                    const double *vals, int64_t nnz, or_lin_report *rep);
+int or_csr_spmv(int64_t n, const int64_t *offsets, const int64_t *cols, const double *vals,
+                const double *x, double *y, or_lin_report *rep);
+int or_csr_finalize(int64_t nrows, int64_t ncols, int64_t ntrip, const int64_t *trow,
+                    const int64_t *tcol, const double *tval, int64_t *offsets, int64_t *cols,
+                    double *vals, int64_t *nnz_out, or_lin_report *rep);
+int or_jacobi_inverse(int64_t n, const int64_t *offsets, const int64_t *cols,
+                      const double *vals, double *inv_diag, or_lin_report *rep);
+int or_cg(int64_t n, const int64_t *offsets, const int64_t *cols, const double *vals,
+          const double *rhs, double *x, double atol, double rtol, int64_t max_iter,
+          int32_t jacobi, or_lin_report *rep);
+int or_bicgstab(int64_t n, const int64_t *offsets, const int64_t *cols, const double *vals,
+                const double *rhs, double *x, double atol, double rtol, int64_t max_iter,
+                or_lin_report *rep);
+
+/* ============ unstructured triangle-mesh Poisson (config-3 producer) ============ */
+/* Builds the jittered structured triangulation of the unit square with m x m
+ * squares (2*m*m triangles), assembles the two-point Poisson matrix with
+ * ZeroGradient boundaries and Gamma=1, pins `ref_cell` to 0 and returns CSR
+ * arrays (caller frees with or_free).  See mesh.c for the cited reference lines. */
+int or_mesh_poisson(int64_t m, double jitter, uint64_t seed, int64_t ref_cell,
+                    int64_t *ncells_out, int64_t *nnz_out, int64_t **offsets,
+                    int64_t **cols, double **vals, double **volumes, int64_t *nfaces_out,
+                    int64_t **face_owner, int64_t **face_neigh, double **face_coef,
+                    char *message, int cap);
+void or_free(void *ptr);
+
+/* Deterministic PRNG shared with the GPU-side input generators. */
+uint64_t or_splitmix64(uint64_t *state);
+double or_uniform(uint64_t *state);  /* [0,1) with 53 random bits */
+double or_normal(uint64_t *state);   /* Box-Muller, consumes two uniforms */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,522 @@
+"""ctypes front-end of the C oracle (liboracle.so built by oracle/Makefile).
+
+TEST INFRASTRUCTURE ONLY: the checker for the GPU path and the timed CPU
+baseline.  Each wrapper names the C function it calls; the C sources cite the
+reference lines they restate.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+STOP_ABS_INF, STOP_REL_L2 = 0, 1
+GUESS_PREVIOUS, GUESS_ZERO = 0, 1
+SOLVER_PCG, SOLVER_SOR = 0, 1
+UPWIND, CENTRAL = 0, 1
+BC_VELOCITY, BC_SYMMETRY, BC_OUTLET = 0, 1, 2
+LEFT, RIGHT, BOTTOM, TOP = 0, 1, 2, 3
+
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+class SolveReport(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64),
+        ("residual_abs_inf", C.c_double),
+        ("residual_l2", C.c_double),
+        ("initial_l2", C.c_double),
+        ("converged", C.c_int32),
+        ("status", C.c_int32),
+        ("message", C.c_char * 160),
+    ]
+
+
+class Config3D(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64),
+        ("length", C.c_double), ("height", C.c_double), ("depth", C.c_double),
+        ("density", C.c_double), ("lid_velocity", C.c_double), ("reynolds", C.c_double),
+        ("dt", C.c_double),
+        ("pressure_max_iters", C.c_int64),
+        ("pressure_tolerance", C.c_double),
+        ("pressure_omega", C.c_double),
+        ("solver", C.c_int32), ("stop_rule", C.c_int32), ("initial_guess", C.c_int32),
+        ("threads", C.c_int32),
+    ]
+
+
+class State3D(C.Structure):
+    _fields_ = [
+        ("cfg", Config3D),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("n", C.c_int64),
+        ("p", C.POINTER(C.c_double)), ("rhs", C.POINTER(C.c_double)),
+        ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)),
+        ("w", C.POINTER(C.c_double)), ("us", C.POINTER(C.c_double)),
+        ("vs", C.POINTER(C.c_double)), ("ws", C.POINTER(C.c_double)),
+        ("x", C.POINTER(C.c_double)), ("b", C.POINTER(C.c_double)),
+        ("r", C.POINTER(C.c_double)), ("z", C.POINTER(C.c_double)),
+        ("d", C.POINTER(C.c_double)), ("q", C.POINTER(C.c_double)),
+        ("step", C.c_int64),
+    ]
+
+
+class Config2D(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int64), ("ny", C.c_int64),
+        ("length", C.c_double), ("height", C.c_double),
+        ("density", C.c_double), ("nu", C.c_double), ("dt", C.c_double),
+        ("pressure_max_iters", C.c_int64),
+        ("pressure_tolerance", C.c_double),
+        ("pressure_omega", C.c_double),
+        ("convection", C.c_int32), ("solver", C.c_int32),
+        ("stop_rule", C.c_int32), ("initial_guess", C.c_int32), ("threads", C.c_int32),
+        ("bc_kind", C.c_int32 * 4),
+        ("bc_u", C.c_double * 4), ("bc_v", C.c_double * 4),
+        ("outlet_pressure", C.c_double * 4),
+        ("adaptive", C.c_int32),
+        ("min_dt", C.c_double), ("max_dt", C.c_double), ("target_cfl", C.c_double),
+        ("target_diffusion", C.c_double), ("max_growth", C.c_double),
+        ("t_end", C.c_double),
+    ]
+
+
+class Diag2D(C.Structure):
+    _fields_ = [
+        ("time_step", C.c_double), ("pressure_residual", C.c_double),
+        ("max_divergence", C.c_double), ("max_speed", C.c_double),
+        ("momentum_cfl", C.c_double), ("viscous_diffusion_number", C.c_double),
+        ("velocity_change", C.c_double), ("pressure_iterations", C.c_int64),
+    ]
+
+
+class State2D(C.Structure):
+    _fields_ = [
+        ("cfg", Config2D),
+        ("dx", C.c_double), ("dy", C.c_double),
+        ("n", C.c_int64),
+        ("solid", C.POINTER(C.c_uint8)),
+    ] + [(name, C.POINTER(C.c_double)) for name in (
+        "p", "rhs", "u", "v", "us", "vs", "uold", "vold", "ucell", "vcell", "speed", "vort",
+        "x", "b", "r", "z", "d", "q")] + [
+        ("step", C.c_int64), ("time", C.c_double), ("current_dt", C.c_double),
+    ]
+
+
+class LinReport(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("converged", C.c_int32), ("iterations", C.c_int64),
+        ("initial_residual", C.c_double), ("final_residual", C.c_double),
+        ("relative_residual", C.c_double),
+        ("err_index", C.c_int64), ("err_expected", C.c_int64), ("err_actual", C.c_int64),
+        ("err_row", C.c_int64), ("err_col", C.c_int64),
+        ("reason", C.c_char * 96),
+    ]
+
+
+def build() -> None:
+    """Compile liboracle.so from the C sources (make, in-tree)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.or3d_init.argtypes = [C.POINTER(State3D), C.POINTER(Config3D)]
+        L.or3d_free.argtypes = [C.POINTER(State3D)]
+        for name in ("or3d_predict", "or3d_build_rhs", "or3d_correct"):
+            getattr(L, name).argtypes = [C.POINTER(State3D)]
+        L.or3d_solve_pressure.argtypes = [C.POINTER(State3D), C.POINTER(SolveReport)]
+        L.or3d_step.argtypes = [C.POINTER(State3D), C.POINTER(SolveReport)]
+        for name in ("or3d_max_divergence", "or3d_max_speed", "or3d_pressure_residual"):
+            getattr(L, name).argtypes = [C.POINTER(State3D)]
+            getattr(L, name).restype = C.c_double
+        L.or3d_apply_operator.argtypes = [C.POINTER(State3D), _f64p, _f64p]
+        L.or2d_init.argtypes = [C.POINTER(State2D), C.POINTER(Config2D), C.c_void_p]
+        L.or2d_free.argtypes = [C.POINTER(State2D)]
+        L.or2d_step.argtypes = [C.POINTER(State2D), C.POINTER(Diag2D), C.POINTER(SolveReport)]
+        L.or2d_solve_pressure.argtypes = [C.POINTER(State2D), C.POINTER(SolveReport)]
+        L.or2d_predict.argtypes = [C.POINTER(State2D), C.c_double]
+        L.or2d_build_rhs.argtypes = [C.POINTER(State2D), C.c_double]
+        L.or2d_correct.argtypes = [C.POINTER(State2D), C.c_double]
+        L.or2d_apply_velocity_boundaries.argtypes = [
+            C.POINTER(State2D), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or2d_cell_fields.argtypes = [C.POINTER(State2D)]
+        for name in ("or2d_max_divergence", "or2d_pressure_residual"):
+            getattr(L, name).argtypes = [C.POINTER(State2D)]
+            getattr(L, name).restype = C.c_double
+        L.or2d_apply_operator.argtypes = [C.POINTER(State2D), _f64p, _f64p]
+        L.or2d_stability_numbers.argtypes = [C.c_double] * 6 + [C.POINTER(C.c_double)] * 2
+        L.or2d_adaptive_dt.argtypes = [C.c_double] * 12 + [
+            C.POINTER(C.c_double), C.c_char_p, C.c_int]
+        L.or_csr_validate.argtypes = [C.c_int64, C.c_int64, _i64p, _i64p, _f64p, C.c_int64,
+                                      C.POINTER(LinReport)]
+        L.or_csr_spmv.argtypes = [C.c_int64, _i64p, _i64p, _f64p, _f64p, _f64p,
+                                  C.POINTER(LinReport)]
+        L.or_csr_finalize.argtypes = [C.c_int64, C.c_int64, C.c_int64, _i64p, _i64p, _f64p,
+                                      _i64p, _i64p, _f64p, C.POINTER(C.c_int64),
+                                      C.POINTER(LinReport)]
+        L.or_jacobi_inverse.argtypes = [C.c_int64, _i64p, _i64p, _f64p, _f64p,
+                                        C.POINTER(LinReport)]
+        L.or_cg.argtypes = [C.c_int64, _i64p, _i64p, _f64p, _f64p, _f64p, C.c_double,
+                            C.c_double, C.c_int64, C.c_int32, C.POINTER(LinReport)]
+        L.or_bicgstab.argtypes = [C.c_int64, _i64p, _i64p, _f64p, _f64p, _f64p, C.c_double,
+                                  C.c_double, C.c_int64, C.POINTER(LinReport)]
+        P64 = C.POINTER(C.POINTER(C.c_int64))
+        PD = C.POINTER(C.POINTER(C.c_double))
+        L.or_mesh_poisson.argtypes = [
+            C.c_int64, C.c_double, C.c_uint64, C.c_int64, C.POINTER(C.c_int64),
+            C.POINTER(C.c_int64), P64, P64, PD, PD, C.POINTER(C.c_int64), P64, P64, PD,
+            C.c_char_p, C.c_int]
+        L.or_free.argtypes = [C.c_void_p]
+        L.or_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
+        L.or_splitmix64.restype = C.c_uint64
+        L.or_uniform.argtypes = [C.POINTER(C.c_uint64)]
+        L.or_uniform.restype = C.c_double
+        L.or_normal.argtypes = [C.POINTER(C.c_uint64)]
+        L.or_normal.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _arr(ptr, n):
+    return np.ctypeslib.as_array(ptr, shape=(n,))
+
+
+# ----------------------------------------------------------------------------
+# 3D cavity
+# ----------------------------------------------------------------------------
+class Cavity3D:
+    """Oracle mirror of LidDrivenCavity3DSolver (src/solver/incompressible_3d.rs)."""
+
+    def __init__(self, nx, ny=None, nz=None, *, length=1.0, height=1.0, depth=1.0,
+                 density=1.0, lid_velocity=1.0, reynolds=100.0, dt=5.0e-4,
+                 pressure_max_iters=2000, pressure_tolerance=1.0e-6, pressure_omega=1.7,
+                 solver=SOLVER_PCG, stop_rule=STOP_ABS_INF, initial_guess=GUESS_PREVIOUS,
+                 threads=0):
+        ny = nx if ny is None else ny
+        nz = nx if nz is None else nz
+        self.cfg = Config3D(nx, ny, nz, length, height, depth, density, lid_velocity,
+                            reynolds, dt, pressure_max_iters, pressure_tolerance,
+                            pressure_omega, solver, stop_rule, initial_guess, threads)
+        self.s = State3D()
+        st = lib().or3d_init(C.byref(self.s), C.byref(self.cfg))
+        if st != 0:
+            raise ValueError(f"invalid 3D cavity configuration (status {st})")
+        self.nx, self.ny, self.nz = nx, ny, nz
+
+    def __del__(self):
+        try:
+            lib().or3d_free(C.byref(self.s))
+        except Exception:
+            pass
+
+    # field views (live, writable)
+    def field(self, name):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        shapes = {"p": (nz, ny, nx), "rhs": (nz, ny, nx), "u": (nz, ny, nx + 1),
+                  "us": (nz, ny, nx + 1), "v": (nz, ny + 1, nx), "vs": (nz, ny + 1, nx),
+                  "w": (nz + 1, ny, nx), "ws": (nz + 1, ny, nx)}
+        shp = shapes[name]
+        return _arr(getattr(self.s, name), int(np.prod(shp))).reshape(shp)
+
+    def step(self):
+        rep = SolveReport()
+        st = lib().or3d_step(C.byref(self.s), C.byref(rep))
+        if st != 0:
+            raise RuntimeError(rep.message.decode())
+        return rep
+
+    def predict(self):
+        lib().or3d_predict(C.byref(self.s))
+
+    def build_rhs(self):
+        lib().or3d_build_rhs(C.byref(self.s))
+
+    def solve_pressure(self):
+        rep = SolveReport()
+        st = lib().or3d_solve_pressure(C.byref(self.s), C.byref(rep))
+        if st != 0:
+            raise RuntimeError(rep.message.decode())
+        return rep
+
+    def correct(self):
+        lib().or3d_correct(C.byref(self.s))
+
+    def apply_operator(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        y = np.empty_like(x)
+        lib().or3d_apply_operator(C.byref(self.s), x, y)
+        return y
+
+    def max_divergence(self):
+        return lib().or3d_max_divergence(C.byref(self.s))
+
+    def max_speed(self):
+        return lib().or3d_max_speed(C.byref(self.s))
+
+    def pressure_residual(self):
+        return lib().or3d_pressure_residual(C.byref(self.s))
+
+
+# ----------------------------------------------------------------------------
+# 2D solver
+# ----------------------------------------------------------------------------
+class Solver2D:
+    """Oracle mirror of IncompressibleSolver (src/solver/incompressible.rs), projection."""
+
+    def __init__(self, nx, ny=None, *, length=1.0, height=1.0, density=1.0, nu=0.01,
+                 dt=1.0e-3, t_end=1.0e30, pressure_max_iters=800, pressure_tolerance=1.0e-6,
+                 pressure_omega=1.7, convection=UPWIND, solver=SOLVER_PCG,
+                 stop_rule=STOP_ABS_INF, initial_guess=GUESS_PREVIOUS, threads=0,
+                 bc_kind=(BC_VELOCITY,) * 4, bc_u=(0.0, 0.0, 0.0, 1.0), bc_v=(0.0,) * 4,
+                 outlet_pressure=(0.0,) * 4, adaptive=False, min_dt=1.0e-6, max_dt=1.0e-2,
+                 target_cfl=0.5, target_diffusion=0.25, max_growth=1.1, solid=None):
+        ny = nx if ny is None else ny
+        cfg = Config2D()
+        cfg.nx, cfg.ny, cfg.length, cfg.height = nx, ny, length, height
+        cfg.density, cfg.nu, cfg.dt = density, nu, dt
+        cfg.pressure_max_iters, cfg.pressure_tolerance = pressure_max_iters, pressure_tolerance
+        cfg.pressure_omega, cfg.convection, cfg.solver = pressure_omega, convection, solver
+        cfg.stop_rule, cfg.initial_guess, cfg.threads = stop_rule, initial_guess, threads
+        for i in range(4):
+            cfg.bc_kind[i] = bc_kind[i]
+            cfg.bc_u[i] = bc_u[i]
+            cfg.bc_v[i] = bc_v[i]
+            cfg.outlet_pressure[i] = outlet_pressure[i]
+        cfg.adaptive = int(adaptive)
+        cfg.min_dt, cfg.max_dt, cfg.target_cfl = min_dt, max_dt, target_cfl
+        cfg.target_diffusion, cfg.max_growth, cfg.t_end = target_diffusion, max_growth, t_end
+        self.cfg = cfg
+        self.s = State2D()
+        self._solid = None
+        sp = None
+        if solid is not None:
+            self._solid = np.ascontiguousarray(solid, dtype=np.uint8).ravel()
+            sp = self._solid.ctypes.data
+        st = lib().or2d_init(C.byref(self.s), C.byref(cfg), sp)
+        if st != 0:
+            raise ValueError(f"invalid 2D configuration (status {st})")
+        self.nx, self.ny = nx, ny
+
+    def __del__(self):
+        try:
+            lib().or2d_free(C.byref(self.s))
+        except Exception:
+            pass
+
+    def field(self, name):
+        nx, ny = self.nx, self.ny
+        shp = {"u": (ny, nx + 1), "us": (ny, nx + 1), "uold": (ny, nx + 1),
+               "v": (ny + 1, nx), "vs": (ny + 1, nx), "vold": (ny + 1, nx)}.get(name, (ny, nx))
+        return _arr(getattr(self.s, name), shp[0] * shp[1]).reshape(shp)
+
+    def step(self):
+        d, rep = Diag2D(), SolveReport()
+        st = lib().or2d_step(C.byref(self.s), C.byref(d), C.byref(rep))
+        if st != 0:
+            raise RuntimeError(rep.message.decode())
+        return d, rep
+
+    def solve_pressure(self):
+        rep = SolveReport()
+        st = lib().or2d_solve_pressure(C.byref(self.s), C.byref(rep))
+        if st != 0:
+            raise RuntimeError(rep.message.decode())
+        return rep
+
+    def apply_operator(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        y = np.empty_like(x)
+        lib().or2d_apply_operator(C.byref(self.s), x, y)
+        return y
+
+    def max_divergence(self):
+        return lib().or2d_max_divergence(C.byref(self.s))
+
+    def pressure_residual(self):
+        return lib().or2d_pressure_residual(C.byref(self.s))
+
+    @property
+    def time(self):
+        return self.s.time
+
+
+def stability_numbers(dt, nu, dx, dy, umax, vmax):
+    """momentum_stability_numbers (src/solver/incompressible.rs:2294-2306)."""
+    a, b = C.c_double(), C.c_double()
+    lib().or2d_stability_numbers(dt, nu, dx, dy, umax, vmax, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def adaptive_time_step(current_dt, umax, vmax, dx, dy, nu, thermal_diffusivity=None, *,
+                       min_dt, max_dt, target_cfl, target_diffusion, max_growth):
+    """adaptive_time_step (src/solver/incompressible.rs:2308-2336)."""
+    out = C.c_double()
+    msg = C.create_string_buffer(256)
+    td = -1.0 if thermal_diffusivity is None else thermal_diffusivity
+    st = lib().or2d_adaptive_dt(current_dt, umax, vmax, dx, dy, nu, td, min_dt, max_dt,
+                                target_cfl, target_diffusion, max_growth, C.byref(out), msg,
+                                256)
+    if st != 0:
+        raise RuntimeError(msg.value.decode())
+    return out.value
+
+
+# ----------------------------------------------------------------------------
+# CSR
+# ----------------------------------------------------------------------------
+class LinearAlgebraError(Exception):
+    def __init__(self, kind, rep):
+        self.kind = kind
+        self.reason = rep.reason.decode()
+        self.index = rep.err_index
+        self.row = rep.err_row
+        super().__init__(f"{kind}: {self.reason}")
+
+
+_KINDS = {1: "DimensionMismatch", 2: "InvalidCsr", 3: "IndexOutOfBounds",
+          4: "NonFiniteValue", 5: "MissingDiagonal", 6: "ZeroDiagonal", 7: "Breakdown"}
+
+
+def _check(st, rep):
+    if st != 0:
+        raise LinearAlgebraError(_KINDS.get(st, str(st)), rep)
+
+
+class Csr:
+    def __init__(self, nrows, ncols, offsets, cols, vals, validate=True):
+        self.nrows, self.ncols = nrows, ncols
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.cols = np.ascontiguousarray(cols, dtype=np.int64)
+        self.vals = np.ascontiguousarray(vals, dtype=np.float64)
+        if validate:
+            rep = LinReport()
+            if len(self.offsets) != nrows + 1:
+                raise LinearAlgebraError("InvalidCsr", rep)
+            if len(self.cols) != len(self.vals):
+                raise LinearAlgebraError("InvalidCsr", rep)
+            _check(lib().or_csr_validate(nrows, ncols, self.offsets, self.cols, self.vals,
+                                         len(self.vals), C.byref(rep)), rep)
+
+    @staticmethod
+    def from_triplets(nrows, ncols, rows, cols, vals):
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        cols = np.ascontiguousarray(cols, dtype=np.int64)
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        n = len(vals)
+        off = np.zeros(nrows + 1, np.int64)
+        oc = np.zeros(max(n, 1), np.int64)
+        ov = np.zeros(max(n, 1), np.float64)
+        nnz = C.c_int64()
+        rep = LinReport()
+        _check(lib().or_csr_finalize(nrows, ncols, n, rows, cols, vals, off, oc, ov,
+                                     C.byref(nnz), C.byref(rep)), rep)
+        return Csr(nrows, ncols, off, oc[:nnz.value].copy(), ov[:nnz.value].copy())
+
+    @staticmethod
+    def from_dense(rows):
+        a = np.asarray(rows, dtype=np.float64)
+        r, c = np.nonzero(a)
+        return Csr.from_triplets(a.shape[0], a.shape[1], r, c, a[r, c])
+
+    @property
+    def nnz(self):
+        return len(self.vals)
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.nrows)
+        rep = LinReport()
+        _check(lib().or_csr_spmv(self.nrows, self.offsets, self.cols, self.vals, x, y,
+                                 C.byref(rep)), rep)
+        return y
+
+    def jacobi_inverse(self):
+        inv = np.zeros(self.nrows)
+        rep = LinReport()
+        _check(lib().or_jacobi_inverse(self.nrows, self.offsets, self.cols, self.vals, inv,
+                                       C.byref(rep)), rep)
+        return inv
+
+
+def _solve(kind, A, rhs, x, atol, rtol, max_iter):
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    rep = LinReport()
+    if kind == "bicgstab":
+        st = lib().or_bicgstab(A.nrows, A.offsets, A.cols, A.vals, rhs, x, atol, rtol,
+                               max_iter, C.byref(rep))
+    else:
+        st = lib().or_cg(A.nrows, A.offsets, A.cols, A.vals, rhs, x, atol, rtol, max_iter,
+                         1 if kind == "pcg" else 0, C.byref(rep))
+    _check(st, rep)
+    return rep
+
+
+def cg(A, rhs, x, atol=1e-10, rtol=1e-8, max_iter=1000):
+    """cg (src/linear/solver.rs:43-50); x is updated in place."""
+    return _solve("cg", A, rhs, x, atol, rtol, max_iter)
+
+
+def pcg(A, rhs, x, atol=1e-10, rtol=1e-8, max_iter=1000):
+    """pcg with JacobiPreconditioner (src/linear/solver.rs:52-60)."""
+    return _solve("pcg", A, rhs, x, atol, rtol, max_iter)
+
+
+def bicgstab(A, rhs, x, atol=1e-10, rtol=1e-8, max_iter=1000):
+    """bicgstab (src/linear/solver.rs:155-256)."""
+    return _solve("bicgstab", A, rhs, x, atol, rtol, max_iter)
+
+
+def mesh_poisson(m, jitter=0.3, seed=0, ref_cell=0):
+    """Jittered triangulation + pinned two-point Poisson CSR (oracle/mesh.c)."""
+    L = lib()
+    nc, nnz, nf = C.c_int64(), C.c_int64(), C.c_int64()
+    po, pc, pfo, pfn = (C.POINTER(C.c_int64)() for _ in range(4))
+    pv, pvol, pfc = (C.POINTER(C.c_double)() for _ in range(3))
+    msg = C.create_string_buffer(256)
+    st = L.or_mesh_poisson(m, jitter, seed, ref_cell, C.byref(nc), C.byref(nnz), C.byref(po),
+                           C.byref(pc), C.byref(pv), C.byref(pvol), C.byref(nf), C.byref(pfo),
+                           C.byref(pfn), C.byref(pfc), msg, 256)
+    if st != 0:
+        raise RuntimeError(msg.value.decode())
+    n, z, f = nc.value, nnz.value, nf.value
+    out = dict(
+        offsets=_arr(po, n + 1).copy(), cols=_arr(pc, z).copy(), vals=_arr(pv, z).copy(),
+        volumes=_arr(pvol, n).copy(), face_owner=_arr(pfo, f).copy(),
+        face_neigh=_arr(pfn, f).copy(), face_coef=_arr(pfc, f).copy())
+    for p in (po, pc, pv, pvol, pfo, pfn, pfc):
+        L.or_free(p)
+    out["A"] = Csr(n, n, out["offsets"], out["cols"], out["vals"], validate=False)
+    return out
+
+
+class Rng:
+    """splitmix64 stream shared with the library's input generators."""
+
+    def __init__(self, seed):
+        self.state = C.c_uint64(seed)
+
+    def next_u64(self):
+        return lib().or_splitmix64(C.byref(self.state))
+
+    def uniform(self, n):
+        return np.array([lib().or_uniform(C.byref(self.state)) for _ in range(n)])
+
+    def normal(self, n):
+        return np.array([lib().or_normal(C.byref(self.state)) for _ in range(n)])

@@ -1,0 +1,181 @@
+"""GPU parity of the 3D cavity path (through the C ABI) against the oracle.
+
+Bars (FP64 path):
+* pressure solve: same iteration count as the oracle's pinned PCG to within
+  1 iteration, p within 1e-9 of max|p| (the two differ only by FMA
+  contraction and reduction order), and within 1e-10 of the DCT closed form
+  when solved to rel 1e-13;
+* whole steps: u, v, w, p within 1e-9 relative (north_star asks 1e-6 in FP64
+  mode), u* and rhs within 1e-12;
+* smoke thresholds of tests/smoke.rs:113-130;
+* size-independent properties at 128^3: true relative residual and gauge.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+import flursys_amd as fa
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    scale = max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() / scale
+
+
+def make(ctx, nx, ny, nz, **kw):
+    cfg = fa.LidDrivenCavity3DConfig(nx=nx, ny=ny, nz=nz, **kw)
+    return fa.LidDrivenCavity3DSolver(cfg, ctx)
+
+
+@pytest.mark.parametrize("key", ["8x8x8", "16x16x16", "12x10x14"])
+def test_pressure_solve_matches_dct(gpu_ctx, key):
+    d = np.load(os.path.join(GOLD, "neumann_dct.npz"))
+    nz, ny, nx = map(int, key.split("x"))
+    print(f"[gpu] pressure solve vs DCT {key}", flush=True)
+    s = make(gpu_ctx, nx, ny, nz, pressure_tolerance=1e-13, stop_rule=fa.STOP_REL_L2,
+             initial_guess=fa.GUESS_ZERO, pressure_max_iters=10000)
+    p = np.zeros((nz, ny, nx))
+    rep = s.pressure_solve(d[f"rhs3d_{key}"], p)
+    assert rep.converged and rep.status == 0
+    assert rel(p, d[f"p3d_{key}"]) < 1e-10
+    assert p[0, 0, 0] == 0.0
+
+
+@pytest.mark.parametrize("stop,tol,guess", [(fa.STOP_ABS_INF, 1e-6, fa.GUESS_PREVIOUS),
+                                            (fa.STOP_REL_L2, 1e-8, fa.GUESS_ZERO)])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (14, 10, 12), (33, 20, 9)])
+def test_pressure_solve_matches_oracle(gpu_ctx, shape, stop, tol, guess):
+    nx, ny, nz = shape
+    print(f"[gpu] pressure solve vs oracle {shape} stop={stop}", flush=True)
+    rng = np.random.default_rng(7)
+    rhs = rng.standard_normal((nz, ny, nx)) * 50.0
+    rhs[0, 0, 0] = 0.0
+    p0 = rng.standard_normal((nz, ny, nx)) * 0.01
+    p0[0, 0, 0] = 0.0
+    o = O.Cavity3D(nx, ny, nz, pressure_tolerance=tol, stop_rule=stop, initial_guess=guess,
+                   pressure_max_iters=5000)
+    o.field("rhs")[:] = rhs
+    o.field("p")[:] = p0
+    orep = o.solve_pressure()
+    s = make(gpu_ctx, nx, ny, nz, pressure_tolerance=tol, stop_rule=stop, initial_guess=guess,
+             pressure_max_iters=5000)
+    p = p0.copy()
+    rep = s.pressure_solve(rhs, p)
+    assert rep.converged and orep.converged
+    assert abs(rep.iterations - orep.iterations) <= 1
+    assert rel(p, o.field("p")) < 1e-9
+    assert abs(rep.initial_l2 - orep.initial_l2) <= 1e-12 * orep.initial_l2
+
+
+@pytest.mark.parametrize("shape,re", [((16, 16, 16), 400.0), ((12, 10, 14), 1000.0)])
+@pytest.mark.parametrize("stop,tol,guess", [(fa.STOP_ABS_INF, 1e-6, fa.GUESS_PREVIOUS),
+                                            (fa.STOP_REL_L2, 1e-8, fa.GUESS_ZERO)])
+def test_steps_match_oracle(gpu_ctx, shape, re, stop, tol, guess):
+    nx, ny, nz = shape
+    print(f"[gpu] cavity steps vs oracle {shape} Re={re} stop={stop}", flush=True)
+    kw = dict(reynolds=re, dt=5e-4, pressure_tolerance=tol, stop_rule=stop,
+              initial_guess=guess, pressure_max_iters=5000)
+    o = O.Cavity3D(nx, ny, nz, **kw)
+    s = make(gpu_ctx, nx, ny, nz, **kw)
+    for step in range(6):
+        orep = o.step()
+        rep = s.advance_one_step()
+        assert rep.pressure_converged
+        assert abs(rep.pressure_iterations - orep.iterations) <= 1
+    f = s.fields()
+    for name in ("u", "v", "w", "p"):
+        assert rel(f[name], o.field(name)) < 1e-9, name
+    inter = s.intermediate()
+    assert rel(inter["us"], o.field("us")) < 1e-12
+    assert rel(inter["rhs"], o.field("rhs")) < 1e-11
+    assert abs(rep.max_divergence - o.max_divergence()) <= 1e-9 * max(o.max_divergence(), 1e-12) + 1e-12
+    assert abs(rep.max_speed - o.max_speed()) <= 1e-12 * o.max_speed()
+
+
+def test_smoke_threshold(gpu_ctx):
+    """tests/smoke.rs:113-130 on the GPU path."""
+    s = make(gpu_ctx, 8, 8, 8, dt=5e-4)
+    summary = s.run(2)
+    assert summary.steps == 2
+    assert np.isfinite(summary.max_divergence) and summary.max_divergence < 1e-5
+    assert summary.max_speed > 0.0
+
+
+def test_gpu_matches_reference_sor_solution(gpu_ctx):
+    """The reference's own 3D solver is SOR; at a tight tolerance both solvers
+    reach the same pinned solution, so a 2-step run agrees with it."""
+    o = O.Cavity3D(8, solver=O.SOLVER_SOR, pressure_tolerance=1e-11, pressure_max_iters=50000)
+    s = make(gpu_ctx, 8, 8, 8, pressure_tolerance=1e-11, pressure_max_iters=50000)
+    for _ in range(2):
+        o.step()
+        s.advance_one_step()
+    f = s.fields()
+    for name in ("u", "v", "w", "p"):
+        assert rel(f[name], o.field(name)) < 1e-8, name
+
+
+def test_repeatable_bitwise(gpu_ctx):
+    out = []
+    for _ in range(2):
+        s = make(gpu_ctx, 20, 18, 16, reynolds=400.0)
+        s.run(3)
+        out.append(s.fields())
+    for name in ("u", "v", "w", "p"):
+        assert np.array_equal(out[0][name], out[1][name]), name
+
+
+def test_max_iterations_is_a_status(gpu_ctx):
+    s = make(gpu_ctx, 16, 16, 16, pressure_max_iters=3, pressure_tolerance=1e-12,
+             stop_rule=fa.STOP_REL_L2)
+    rhs = np.random.default_rng(3).standard_normal((16, 16, 16))
+    p = np.zeros_like(rhs)
+    rep = s.pressure_solve(rhs, p)
+    assert rep.iterations == 3 and not rep.converged and rep.status == 0
+
+
+def test_error_paths(gpu_ctx):
+    with pytest.raises(fa.FlursysError, match="at least 4") as e:
+        make(gpu_ctx, 3, 8, 8)
+    assert e.value.kind == "ConfigError"
+    with pytest.raises(fa.FlursysError, match="dt must be finite"):
+        make(gpu_ctx, 8, 8, 8, dt=float("nan"))
+    s = make(gpu_ctx, 8, 8, 8)
+    rhs = np.zeros((8, 8, 8))
+    rhs[2, 3, 4] = np.inf
+    with pytest.raises(fa.FlursysError) as e:
+        s.pressure_solve(rhs, np.zeros_like(rhs))
+    assert e.value.kind == "NonFiniteValue"
+
+
+def test_zero_rhs_takes_zero_iterations(gpu_ctx):
+    s = make(gpu_ctx, 8, 8, 8, stop_rule=fa.STOP_REL_L2, initial_guess=fa.GUESS_ZERO)
+    p = np.ones((8, 8, 8))
+    rep = s.pressure_solve(np.zeros((8, 8, 8)), p)
+    assert rep.iterations == 0 and rep.converged and np.all(p == 0.0)
+
+
+def test_large_grid_properties(gpu_ctx):
+    """128^3 at rel 1e-8: the true residual of the returned p (recomputed on
+    the CPU) meets the tolerance and the gauge holds."""
+    n = 128
+    print("[gpu] 128^3 property solve", flush=True)
+    s = make(gpu_ctx, n, n, n, pressure_tolerance=1e-8, stop_rule=fa.STOP_REL_L2,
+             initial_guess=fa.GUESS_ZERO, pressure_max_iters=5000)
+    rng = np.random.default_rng(11)
+    rhs = rng.standard_normal((n, n, n))
+    rhs[0, 0, 0] = 0.0
+    p = np.zeros_like(rhs)
+    rep = s.pressure_solve(rhs, p)
+    assert rep.converged
+    o = O.Cavity3D(n, n, n)
+    b = -rhs.ravel()
+    b[0] = 0.0
+    r = b - o.apply_operator(p.ravel())
+    assert np.linalg.norm(r) <= 1.5e-8 * np.linalg.norm(b)
+    assert p[0, 0, 0] == 0.0
