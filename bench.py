@@ -8,6 +8,13 @@ projection step (predictor, RHS, pressure solve, correction, diagnostics) on
 device-resident fields; the flow starts from rest (synthetic data: no
 dataset is needed).
 
+Variants timed back to back on the same workload:
+  plain_fp64_pcg  FP64 Jacobi-PCG (the judge)
+  mixed_ir        FP32 stencil CG inside FP64 iterative refinement (FP64
+                  residual, dots, alpha/beta and stop test)
+`value` is the steps/s of the fastest variant that meets the same FP64
+rel-res 1e-8 stop test; every variant's figures are in `config.variants`.
+
 Output: one JSON line with steps/s, CG iterations/step, the HBM roofline of
 the dominant kernel (timed live with HIP events on the library's stream) and
 a CPU baseline (the oracle timed on this host on a bounded sample).
@@ -60,22 +67,75 @@ def cpu_baseline(n, re, dt, iters_per_step, sample_iters):
         "sample": (f"C oracle (OpenMP FP64 restatement of the reference path) at {n}^3: "
                    f"predictor+RHS+correction once and {rep.iterations} Jacobi-PCG "
                    f"iterations timed ({t3 - t0:.1f} s), extrapolated to "
-                   f"{iters_per_step:.0f} iterations/step; {per_iter * 1e3:.1f} ms/iteration"),
+                   f"{iters_per_step:.0f} iterations/step (the plain FP64 GPU count); "
+                   f"{per_iter * 1e3:.1f} ms/iteration"),
+    }
+
+
+def run_variant(fa, ctx, name, precision, args):
+    n = args.n
+    cfg = fa.LidDrivenCavity3DConfig(nx=n, ny=n, nz=n, reynolds=args.reynolds, dt=args.dt,
+                                     pressure_tolerance=args.tol, stop_rule=fa.STOP_REL_L2,
+                                     initial_guess=fa.GUESS_ZERO, pressure_max_iters=50000,
+                                     precision=precision, inner_tolerance=args.inner_tol)
+    log(f"[bench] {name}: {n}^3 Re={args.reynolds} dt={args.dt} tol={args.tol}")
+    s = fa.LidDrivenCavity3DSolver(cfg, ctx)
+    for k in range(args.warmup):
+        rep = s.advance_one_step()
+        log(f"[bench] {name} warmup {k + 1}: {rep.pressure_iterations} iterations")
+    s.set_timing(True)
+    iters, refs = [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        rep = s.advance_one_step()  # returns after its stream has drained
+        iters.append(rep.pressure_iterations)
+        refs.append(rep.pressure_refinements)
+        if not rep.pressure_converged:
+            raise SystemExit(f"{name}: pressure solve did not converge")
+        log(f"[bench] {name} step {k + 1}/{args.steps}: {rep.pressure_iterations} iterations, "
+            f"{rep.pressure_refinements} refinements, div {rep.max_divergence:.3e}")
+    elapsed = time.perf_counter() - t0
+    kt = s.kernel_times()
+    s.close()
+    cells = n ** 3
+    word = 4 if precision == fa.PRECISION_MIXED else 8
+    # Algorithmic HBM bytes per launch: dirapply reads r, d and writes d', q
+    # (4 passes); update reads x, d', r, q and writes x, r (6 passes).
+    kernels = {
+        "k_cg_update": (6 * word * cells, kt["ms_update"]),
+        "k_cg_dirapply": (4 * word * cells, kt["ms_apply"]),
+    }
+    return {
+        "steps_per_s": args.steps / elapsed,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "cg_iters_per_step": sum(iters) / len(iters),
+        "cg_iters_per_step_all": iters,
+        "refinements_per_step": sum(refs) / len(refs),
+        "ms_per_cg_iteration": kt["ms_per_iteration"],
+        "cg_iteration_GBps": (10 * word * cells / (kt["ms_per_iteration"] * 1e-3) / 1e9
+                              if kt["ms_per_iteration"] else None),
+        "kernels": {k: {"ms": v[1], "bytes": v[0],
+                        "GBps": v[0] / (v[1] * 1e-3) / 1e9 if v[1] else None}
+                    for k, v in kernels.items()},
+        "max_divergence": rep.max_divergence,
+        "dtype": "f32+f64" if precision == fa.PRECISION_MIXED else "f64",
     }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256, help="grid cells per axis")
     ap.add_argument("--reynolds", type=float, default=1000.0)
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--inner-tol", type=float, default=0.0)
+    ap.add_argument("--variants", default="plain_fp64_pcg,mixed_ir")
     ap.add_argument("--cpu-sample-iters", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--copy-probe", action="store_true", default=True)
+    ap.add_argument("--no-copy-probe", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -87,95 +147,66 @@ def main():
     import flursys_amd as fa
     from flursys_amd import _lib
 
-    n = args.n
     ctx = fa.Context(0)
-    cfg = fa.LidDrivenCavity3DConfig(nx=n, ny=n, nz=n, reynolds=args.reynolds, dt=args.dt,
-                                     pressure_tolerance=args.tol, stop_rule=fa.STOP_REL_L2,
-                                     initial_guess=fa.GUESS_ZERO, pressure_max_iters=20000)
-    log(f"[bench] building solver {n}^3 Re={args.reynolds} dt={args.dt} tol={args.tol}")
-    s = fa.LidDrivenCavity3DSolver(cfg, ctx)
-    for k in range(args.warmup):
-        rep = s.advance_one_step()
-        log(f"[bench] warmup step {k + 1}: {rep.pressure_iterations} iterations")
-    s.set_timing(True)
-    iters = []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        rep = s.advance_one_step()  # returns after a device synchronise
-        iters.append(rep.pressure_iterations)
-        log(f"[bench] step {k + 1}/{args.steps}: {rep.pressure_iterations} iterations, "
-            f"div {rep.max_divergence:.3e}")
-    elapsed = time.perf_counter() - t0
-    kt = s.kernel_times()
-    ms_step = elapsed * 1e3 / args.steps
-    iters_per_step = sum(iters) / len(iters)
-    cells = n ** 3
-    # Algorithmic HBM bytes per launch (FP64): update reads x, d, r, q and
-    # writes x, r (6 passes); apply reads d, writes q (2); direction reads r,
-    # d, writes d (3).
-    kernels = {
-        "k_pcg_update": (6 * 8 * cells, kt["ms_update"]),
-        "k_pcg_apply": (2 * 8 * cells, kt["ms_apply"]),
-        "k_pcg_direction": (3 * 8 * cells, kt["ms_direction"]),
-    }
-    dom = max(kernels, key=lambda k: kernels[k][1])
-    bytes_dom, ms_dom = kernels[dom]
-    achieved = bytes_dom / (ms_dom * 1e-3) / 1e9 if ms_dom > 0 else 0.0
-    iter_gbs = 11 * 8 * cells / (kt["ms_per_iteration"] * 1e-3) / 1e9 if kt["ms_per_iteration"] else 0
+    precisions = {"plain_fp64_pcg": fa.PRECISION_FP64, "mixed_ir": fa.PRECISION_MIXED}
+    results = {}
+    for name in args.variants.split(","):
+        results[name] = run_variant(fa, ctx, name, precisions[name], args)
+    best = max(results, key=lambda k: results[k]["steps_per_s"])
+    b = results[best]
+    dom = max(b["kernels"], key=lambda k: b["kernels"][k]["ms"])
+    dk = b["kernels"][dom]
     copy = None
-    if args.copy_probe:
+    if not args.no_copy_probe:
         import ctypes as C
 
         ms = C.c_double()
         nbytes = 1 << 30
         _lib.check(_lib.load().fl_probe_copy(ctx.handle, nbytes, 20, C.byref(ms)), ctx.handle)
         copy = 2 * nbytes / (ms.value * 1e-3) / 1e9
+    n = args.n
     out = {
         "metric": "steps/s + CG iters/step, 256^3 cavity: learned FCG vs plain PCG @ rel-res 1e-8",
-        "value": 1e3 / ms_step,
+        "value": b["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": b["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": b["dtype"],
         "data": "synthetic: lid-driven cavity started from rest (no dataset)",
         "config": {
             "workload": f"3D lid-driven cavity {n}^3, Re {args.reynolds:g}, dt {args.dt:g}, "
-                        f"plain FP64 Jacobi-PCG (pinned p(0,0,0)) to rel-res {args.tol:g} "
-                        "from a zero guess",
+                        f"pressure to rel-res {args.tol:g} (FP64 test) from a zero guess, "
+                        "p(0,0,0) pinned",
             "grid": [n, n, n],
-            "variant": "plain_fp64_pcg",
-            "cg_iters_per_step": iters_per_step,
-            "cg_iters_per_step_all": iters,
-            "ms_per_cg_iteration": kt["ms_per_iteration"],
-            "cg_iteration_hbm_GBps": iter_gbs,
+            "variant": best,
+            "cg_iters_per_step": b["cg_iters_per_step"],
+            "variants": results,
             "copy_probe_GBps": copy,
-            "max_divergence": rep.max_divergence,
             "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": dom,
-            "achieved": achieved,
+            "kernel": f"{dom} ({b['dtype']})",
+            "achieved": dk["GBps"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": dk["GBps"] / HBM_PEAK_GBS if dk["GBps"] else None,
             "traffic": None,
-            "algorithmic_bytes_per_launch": bytes_dom,
-            "ms_per_launch": ms_dom,
-            "per_kernel_ms": {k: v[1] for k, v in kernels.items()},
+            "algorithmic_bytes_per_launch": dk["bytes"],
+            "ms_per_launch": dk["ms"],
         },
         "cpu_baseline": None,
     }
-    s.close()
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle) sample")
-        out["cpu_baseline"] = cpu_baseline(n, args.reynolds, args.dt, iters_per_step,
-                                           args.cpu_sample_iters)
+        plain = results.get("plain_fp64_pcg", b)
+        out["cpu_baseline"] = cpu_baseline(n, args.reynolds, args.dt,
+                                           plain["cg_iters_per_step"], args.cpu_sample_iters)
     print(json.dumps(out), flush=True)
     ctx.close()
 

@@ -52,7 +52,7 @@ class Cavity3DConfig(C.Structure):
         ("dt", C.c_double),
         ("pressure_max_iters", C.c_int64), ("pressure_tolerance", C.c_double),
         ("stop_rule", C.c_int32), ("initial_guess", C.c_int32), ("precision", C.c_int32),
-        ("reserved", C.c_int32),
+        ("reserved", C.c_int32), ("inner_tolerance", C.c_double),
     ]
 
 
@@ -66,8 +66,26 @@ class Cavity3DStepReport(C.Structure):
     ]
 
 
+class LinOpts(C.Structure):
+    _fields_ = [("absolute_tolerance", C.c_double), ("relative_tolerance", C.c_double),
+                ("max_iterations", C.c_int64)]
+
+
+class LinReport(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("reserved", C.c_int32), ("iterations", C.c_int64),
+                ("initial_residual", C.c_double), ("final_residual", C.c_double),
+                ("relative_residual", C.c_double), ("device_ms", C.c_double)]
+
+
+class ErrDetail(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("index", C.c_int64),
+                ("expected", C.c_int64), ("actual", C.c_int64), ("row", C.c_int64),
+                ("column", C.c_int64), ("context", C.c_char * 96)]
+
+
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
+_LIN = [_P, _P, _P, C.POINTER(LinOpts), C.POINTER(LinReport), C.POINTER(ErrDetail)]
 
 # name -> (restype, argtypes); every symbol include/flursys_hip.h declares.
 SIGNATURES = {
@@ -86,6 +104,13 @@ SIGNATURES = {
     "fl_cavity3d_get_intermediate": (C.c_int, [_P, _P, _P, _P, _P]),
     "fl_cavity3d_set_timing": (C.c_int, [_P, C.c_int32]),
     "fl_cavity3d_kernel_times": (C.c_int, [_P, _D, _D, _D, _D, C.POINTER(C.c_int64)]),
+    "fl_csr_upload": (C.c_int, [_P, C.c_int64, C.c_int64, _P, _P, _P, C.c_int64,
+                                C.POINTER(_P), C.POINTER(ErrDetail)]),
+    "fl_csr_destroy": (None, [_P]),
+    "fl_spmv": (C.c_int, [_P, _P, _P, C.POINTER(ErrDetail)]),
+    "fl_cg": (C.c_int, _LIN),
+    "fl_pcg_jacobi": (C.c_int, _LIN),
+    "fl_bicgstab": (C.c_int, _LIN),
 }
 
 _lib = None

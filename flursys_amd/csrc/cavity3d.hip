@@ -3,26 +3,34 @@
 // Replaces LidDrivenCavity3DSolver (src/solver/incompressible_3d.rs):
 //   predict_momentum :174-273   -> k_predict
 //   build_pressure_rhs :275-291 -> k_rhs
-//   solve_pressure_poisson :293-332 -> pinned Jacobi-PCG (k_pcg_*); the
-//       reference only has SOR here, the PCG loop is the 2D solver's
+//   solve_pressure_poisson :293-332 -> pinned Jacobi-PCG; the reference only
+//       has SOR here, the PCG loop is the 2D solver's
 //       (src/solver/incompressible.rs:1048-1214) on the 7-point operator with
 //       cell (0,0,0) eliminated (src/poisson.rs:303-337 style)
 //   correct_velocity :334-363   -> k_correct
 //   max_divergence / max_speed / ensure_finite :504-554 -> k_diag, k_finite
 //
 // Data layout in HBM: the reference's own (src/field.rs:104-107), one flat
-// array per field, x fastest.  The PCG vectors x (= p), r, d, q live beside
-// the fields; z = D^-1 r is never stored (recomputed from r where needed).
+// array per field, x fastest.  The PCG vectors x (= p), r, q and two
+// direction buffers d0/d1 (ping-pong) live beside the fields; z = D^-1 r is
+// never stored.
 //
-// Per PCG iteration three bandwidth-bound kernels run, each a grid-stride
-// loop over 256-cell tiles with an XCD-aware slab mapping and a one-pass
-// grid reduction finished by the last block:
-//   k_pcg_apply   q = A d,  d.q           -> alpha       (reads d, writes q)
-//   k_pcg_update  x += a d, r -= a q,
-//                 r.z, r.r, max|r|        -> stop test, beta (reads x d r q, writes x r)
-//   k_pcg_direction  d = D^-1 r + beta d                     (reads r d, writes d)
-// 11 vector passes per iteration; all scalars (alpha, beta, r.z, the stop
-// test) stay on the device, so the host only polls a small state block.
+// One PCG iteration is two bandwidth-bound kernels (10 vector passes):
+//   k_cg_dirapply  d' = D^-1 r + beta d,  q = A d',  d'.q     (reads r d, writes d' q)
+//   k_cg_update    x += alpha d', r -= alpha q,  r.z, r.r, max|r|  (reads x d' r q, writes x r)
+// The direction update is fused into the operator apply: the halo values of
+// d' are recomputed from r and d, so d' is written once and never re-read
+// for the stencil.  Scalars never leave the device: each kernel folds the
+// other kernel's per-block partials itself (common.hpp) and block 0 records
+// the state the host polls.  Stencil loads use clamped indices (the
+// reference's ghost = centre rule makes a missing neighbour contribute
+// nothing), so all seven loads of a cell are in flight together.
+//
+// Mixed precision (FL_PRECISION_MIXED): the same two kernels run on FP32
+// vectors (dots, alpha, beta and the stop test stay FP64) as the inner
+// solver of FP64 iterative refinement: r = b - A x is recomputed in FP64
+// (k_residual), the inner CG solves A e = r to a relative tolerance, and
+// x += e in FP64, until the outer FP64 stop test holds.
 
 #include <algorithm>
 #include <cstring>
@@ -34,215 +42,262 @@
 namespace fl {
 namespace c3 {
 
+// The Jacobi diagonal of the Neumann operator is the sum of the coefficients
+// of the neighbours that exist (ghost = centre, incompressible_3d.rs:450-474).
+// Every axis contributes 1 or 2 neighbours, so there are eight diagonals;
+// their reciprocals are tabulated (bit a of the index set = two neighbours
+// along axis a) and z = D^-1 r is a multiply.
 struct Grid {
     uint32_t nx, ny, nz, nxy, n;
     double ix2, iy2, iz2;
+    FastDiv fx, fxy;
+    double inv[8];
+    float invf[8];
 };
 
 __device__ inline void cell_ijk(const Grid &g, uint32_t c, uint32_t &i, uint32_t &j,
                                 uint32_t &k) {
-    k = c / g.nxy;
+    k = g.fxy.div(c);
     const uint32_t rem = c - k * g.nxy;
-    j = rem / g.nx;
+    j = g.fx.div(rem);
     i = rem - j * g.nx;
 }
 
-// Neumann operator row at an interior (non-pinned) cell: the reference's
-// ghost = centre rule (pressure_neighbor, incompressible_3d.rs:450-474) is
-// the same as dropping the missing neighbour from both diagonal and sum.
-template <typename T>
-__device__ inline T apply_row(const Grid &g, const T *__restrict__ x, uint32_t c, uint32_t i,
-                              uint32_t j, uint32_t k, T xc, T &diag) {
-    const T cx = (T)g.ix2, cy = (T)g.iy2, cz = (T)g.iz2;
-    T dg = 0, s = 0;
-    if (i + 1 < g.nx) { dg += cx; s += cx * x[c + 1]; }
-    if (i > 0) { dg += cx; s += cx * x[c - 1]; }
-    if (j + 1 < g.ny) { dg += cy; s += cy * x[c + g.nx]; }
-    if (j > 0) { dg += cy; s += cy * x[c - g.nx]; }
-    if (k + 1 < g.nz) { dg += cz; s += cz * x[c + g.nxy]; }
-    if (k > 0) { dg += cz; s += cz * x[c - g.nxy]; }
-    diag = dg;
-    return dg * xc - s;
+__device__ inline uint32_t diag_index(const Grid &g, uint32_t i, uint32_t j, uint32_t k) {
+    return (uint32_t)(i > 0 && i + 1 < g.nx) | ((uint32_t)(j > 0 && j + 1 < g.ny) << 1) |
+           ((uint32_t)(k > 0 && k + 1 < g.nz) << 2);
+}
+template <typename T> __device__ inline T inv_diag(const Grid &g, uint32_t idx);
+template <> __device__ inline double inv_diag<double>(const Grid &g, uint32_t idx) { return g.inv[idx]; }
+template <> __device__ inline float inv_diag<float>(const Grid &g, uint32_t idx) { return g.invf[idx]; }
+
+// Neighbour offsets clamped at the walls (a missing neighbour reads the
+// centre, contributing c * (x_c - x_c) = 0).
+struct Nbr {
+    uint32_t e, w, n, s, f, b;
+};
+__device__ inline Nbr neighbours(const Grid &g, uint32_t c, uint32_t i, uint32_t j, uint32_t k) {
+    Nbr o;
+    o.e = i + 1 < g.nx ? c + 1 : c;
+    o.w = i > 0 ? c - 1 : c;
+    o.n = j + 1 < g.ny ? c + g.nx : c;
+    o.s = j > 0 ? c - g.nx : c;
+    o.f = k + 1 < g.nz ? c + g.nxy : c;
+    o.b = k > 0 ? c - g.nxy : c;
+    return o;
 }
 
-template <typename T>
-__device__ inline T diag_at(const Grid &g, uint32_t i, uint32_t j, uint32_t k) {
-    const T cx = (T)g.ix2, cy = (T)g.iy2, cz = (T)g.iz2;
-    T dg = 0;
-    if (i + 1 < g.nx) dg += cx;
-    if (i > 0) dg += cx;
-    if (j + 1 < g.ny) dg += cy;
-    if (j > 0) dg += cy;
-    if (k + 1 < g.nz) dg += cz;
-    if (k > 0) dg += cz;
-    return dg;
-}
-
-// Device-resident scalars of one PCG solve.
-struct PcgState {
-    double rz, alpha, beta;
-    double rr0, thresh, tol;
-    double rmax, rr;
-    long long iter, max_iter;
-    int done, converged, status, stop_rule;
-    unsigned counters[4];
+// Host-polled state.  SA is written only by block 0 of k_cg_dirapply, SB
+// only by block 0 of k_cg_update (and the host before a solve); a kernel
+// never reads a field of its own struct that its block 0 may change, except
+// `done`, which block 0 raises only when every block reaches the same
+// verdict from the same folded partials.
+struct CgStateA {
+    double rz, beta, rr0, rr, rmax;
+    int done, converged, status, pad;
+};
+struct CgStateB {
+    double rz_prev, alpha;
+    long long iters;
+    int done, status, first, pad;
+};
+struct CgParams {
+    double tol;
+    long long max_iter;
+    int stop_rule;
 };
 enum { ST_OK = 0, ST_SINGULAR = 1, ST_ALPHA = 2, ST_RESIDUAL = 3 };
 
-// r = b - A x with b = -rhs (b_0 = 0), z = D^-1 r, d = z; r.r, r.z, max|r|.
-__global__ void __launch_bounds__(kBlock) k_pcg_init(Grid g, const double *__restrict__ rhs,
+// r = b - A x with b = -rhs (b_0 = 0, row 0 the identity of the pin);
+// publishes r.z, r.r, max|r| as the "update" partials of iteration 0.  For
+// the mixed path the residual is computed in FP64 and stored as FP32 for the
+// inner solver, with the partials taken from the stored values.
+template <typename TR>
+__global__ void __launch_bounds__(kBlock) k_residual(Grid g, const double *__restrict__ rhs,
                                                      const double *__restrict__ x,
-                                                     double *__restrict__ r,
-                                                     double *__restrict__ d, double *partials,
-                                                     PcgState *st) {
+                                                     TR *__restrict__ r, double *partials) {
     double s[2] = {0.0, 0.0}, m[1] = {0.0};
-    const uint32_t ntiles = (g.n + kBlock - 1) / kBlock;
-    TileSched ts(ntiles);
+    TileSched ts((g.n + kBlock - 1) / kBlock);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const uint32_t c = t * kBlock + threadIdx.x;
         if (c >= g.n) continue;
-        double rc = 0.0, z = 0.0;
+        TR rc = 0;
+        double inv = 1.0;
         if (c != 0) {
             uint32_t i, j, k;
             cell_ijk(g, c, i, j, k);
-            double dg;
-            const double ax = apply_row<double>(g, x, c, i, j, k, x[c], dg);
-            rc = -rhs[c] - ax;
-            z = rc / dg;
+            const Nbr o = neighbours(g, c, i, j, k);
+            const double xc = x[c];
+            const double ax = g.ix2 * ((xc - x[o.e]) + (xc - x[o.w])) +
+                              g.iy2 * ((xc - x[o.n]) + (xc - x[o.s])) +
+                              g.iz2 * ((xc - x[o.f]) + (xc - x[o.b]));
+            rc = (TR)(-rhs[c] - ax);
+            inv = (double)inv_diag<TR>(g, diag_index(g, i, j, k));
         }
         r[c] = rc;
-        d[c] = z;
-        s[0] += rc * rc;
-        s[1] += rc * z;
-        m[0] = fmax(m[0], fabs(rc));
+        const double rd = (double)rc;
+        s[0] += rd * (rd * inv);
+        s[1] += rd * rd;
+        m[0] = fmax(m[0], fabs(rd));
     }
-    if (grid_reduce<2, 1>(s, m, partials, &st->counters[0]) && threadIdx.x == 0) {
-        st->rr0 = s[0];
-        st->rr = s[0];
-        st->rz = s[1];
-        st->rmax = m[0];
-        st->iter = 0;
-        st->thresh = st->tol * sqrt(s[0]);
-        const bool conv = st->stop_rule == FL_STOP_REL_L2 ? (sqrt(s[0]) <= st->thresh)
-                                                          : (m[0] < st->tol);
-        st->converged = conv;
-        st->done = conv;
-        st->status = ST_OK;
-    }
+    publish_partials<2, 1>(s, m, partials);
 }
 
-// q = A d and d.q; the last block forms alpha = (r.z) / (d.q).
-__global__ void __launch_bounds__(kBlock) k_pcg_apply(Grid g, const double *__restrict__ d,
-                                                      double *__restrict__ q, double *partials,
-                                                      PcgState *st) {
-    if (st->done) return;
-    double s[1] = {0.0}, m[1] = {0.0};
-    const uint32_t ntiles = (g.n + kBlock - 1) / kBlock;
-    TileSched ts(ntiles);
-    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
-        const uint32_t c = t * kBlock + threadIdx.x;
-        if (c >= g.n) continue;
-        const double dc = d[c];
-        double qc = dc;
-        if (c != 0) {
-            uint32_t i, j, k;
-            cell_ijk(g, c, i, j, k);
-            double dg;
-            qc = apply_row<double>(g, d, c, i, j, k, dc, dg);
-        }
-        q[c] = qc;
-        s[0] += dc * qc;
-    }
-    if (grid_reduce<1, 1>(s, m, partials, &st->counters[1]) && threadIdx.x == 0) {
-        const double den = s[0];
-        if (fabs(den) < 1.0e-30 || !isfinite(den)) {
-            st->status = ST_SINGULAR;
-            st->done = 1;
-            return;
-        }
-        const double alpha = st->rz / den;
-        if (!isfinite(alpha)) {
-            st->status = ST_ALPHA;
-            st->done = 1;
-            return;
-        }
-        st->alpha = alpha;
-    }
-}
-
-// x += alpha d, r -= alpha q; r.z, r.r, max|r|; the last block runs the stop
-// test and forms beta, in the reference's order (incompressible.rs:1141-1185).
-__global__ void __launch_bounds__(kBlock) k_pcg_update(Grid g, double *__restrict__ x,
-                                                       double *__restrict__ r,
-                                                       const double *__restrict__ d,
-                                                       const double *__restrict__ q,
-                                                       double *partials, PcgState *st) {
-    if (st->done) return;
-    const double alpha = st->alpha;
-    double s[2] = {0.0, 0.0}, m[1] = {0.0};
-    const uint32_t ntiles = (g.n + kBlock - 1) / kBlock;
-    TileSched ts(ntiles);
-    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
-        const uint32_t c = t * kBlock + threadIdx.x;
-        if (c >= g.n) continue;
-        x[c] += alpha * d[c];
-        const double rc = r[c] - alpha * q[c];
-        r[c] = rc;
-        double z = rc;
-        if (c != 0) {
-            uint32_t i, j, k;
-            cell_ijk(g, c, i, j, k);
-            z = rc / diag_at<double>(g, i, j, k);
-        }
-        s[0] += rc * z;
-        s[1] += rc * rc;
-        m[0] = fmax(m[0], fabs(rc));
-    }
-    if (grid_reduce<2, 1>(s, m, partials, &st->counters[2]) && threadIdx.x == 0) {
-        st->iter += 1;
-        st->rmax = m[0];
-        st->rr = s[1];
-        if (!isfinite(m[0])) {
-            st->status = ST_RESIDUAL;
-            st->done = 1;
-            return;
-        }
-        const bool conv = st->stop_rule == FL_STOP_REL_L2 ? (sqrt(s[1]) <= st->thresh)
-                                                          : (m[0] < st->tol);
+// d' = z + beta d (z = D^-1 r), q = A d', partial d'.q.  Folds the update
+// partials first: stop test, breakdown test and beta, in the reference's
+// order (incompressible.rs:1154-1185).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cg_dirapply(
+    Grid g, CgParams prm, const T *__restrict__ r, const T *__restrict__ dold,
+    T *__restrict__ dnew, T *__restrict__ q, const double *__restrict__ pin,
+    double *__restrict__ pout, CgStateA *SA, const CgStateB *SB) {
+    if (SA->done || SB->done) return;
+    double f[2], fm[1];
+    fold_partials<2, 1>(pin, gridDim.x, f, fm);
+    const double rz = f[0], rr = f[1], rmax = fm[0];
+    const bool first = SB->first != 0;
+    const double rr0 = first ? rr : SA->rr0;
+    const long long iters = SB->iters;
+    int done = 0, converged = 0, status = ST_OK;
+    double beta = 0.0;
+    if (!first && !isfinite(rmax)) {
+        status = ST_RESIDUAL;
+        done = 1;
+    } else {
+        const bool conv = prm.stop_rule == FL_STOP_REL_L2 ? (sqrt(rr) <= prm.tol * sqrt(rr0))
+                                                          : (rmax < prm.tol);
         if (conv) {
-            st->converged = 1;
-            st->done = 1;
-            return;
+            converged = done = 1;
+        } else if (!first) {
+            if (fabs(SB->rz_prev) < 1.0e-30) done = 1;  // the silent break (:1171-1173)
+            else if (iters >= prm.max_iter) done = 1;   // MaxIterations
+            else beta = rz / SB->rz_prev;
         }
-        if (fabs(st->rz) < 1.0e-30) {  // the reference's silent break (:1171-1173)
-            st->done = 1;
-            return;
-        }
-        st->beta = s[0] / st->rz;
-        st->rz = s[0];
-        if (st->iter >= st->max_iter) st->done = 1;
     }
-}
-
-// d = D^-1 r + beta d.
-__global__ void __launch_bounds__(kBlock) k_pcg_direction(Grid g, const double *__restrict__ r,
-                                                          double *__restrict__ d,
-                                                          const PcgState *st) {
-    if (st->done) return;
-    const double beta = st->beta;
-    const uint32_t ntiles = (g.n + kBlock - 1) / kBlock;
-    TileSched ts(ntiles);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        SA->rz = rz;
+        SA->beta = beta;
+        SA->rr0 = rr0;
+        SA->rr = rr;
+        SA->rmax = rmax;
+        SA->converged = converged;
+        SA->status = status;
+        SA->done = done;
+    }
+    if (done) return;
+    const T tb = (T)beta;
+    double s[1] = {0.0}, m[1] = {0.0};
+    TileSched ts((g.n + kBlock - 1) / kBlock);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const uint32_t c = t * kBlock + threadIdx.x;
         if (c >= g.n) continue;
-        double z = r[c];
+        if (c == 0) {  // pinned row: identity; r_0 = d_0 = 0 throughout
+            const T d0 = first ? r[0] : (T)(r[0] + tb * dold[0]);
+            dnew[0] = d0;
+            q[0] = d0;
+            s[0] += (double)d0 * (double)d0;
+            continue;
+        }
+        uint32_t i, j, k;
+        cell_ijk(g, c, i, j, k);
+        const Nbr o = neighbours(g, c, i, j, k);
+        const uint32_t idx[7] = {c, o.e, o.w, o.n, o.s, o.f, o.b};
+        T rv[7], dv[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int a = 0; a < 7; ++a) rv[a] = r[idx[a]];
+        if (!first) {
+#pragma unroll
+            for (int a = 0; a < 7; ++a) dv[a] = dold[idx[a]];
+        }
+        // reciprocal diagonals of the centre and of the six neighbours: bit
+        // a of the index is "two neighbours along axis a"
+        const uint32_t bx = i > 0 && i + 1 < g.nx, by = j > 0 && j + 1 < g.ny,
+                       bz = k > 0 && k + 1 < g.nz;
+        const uint32_t ic = bx | by << 1 | bz << 2;
+        const uint32_t ixe = (i + 2 < g.nx) | by << 1 | bz << 2, ixw = (i > 1) | by << 1 | bz << 2;
+        const uint32_t iyn = bx | (j + 2 < g.ny) << 1 | bz << 2, iys = bx | (j > 1) << 1 | bz << 2;
+        const uint32_t izf = bx | by << 1 | (k + 2 < g.nz) << 2, izb = bx | by << 1 | (k > 1) << 2;
+        const uint32_t di[7] = {ic, ixe, ixw, iyn, iys, izf, izb};
+        const T cx = (T)g.ix2, cy = (T)g.iy2, cz = (T)g.iz2;
+        T dn[7];
+#pragma unroll
+        for (int a = 0; a < 7; ++a) {
+            // a clamped neighbour is the centre itself; the pinned cell's row
+            // is the identity
+            const T inv = idx[a] == 0 ? (T)1 : inv_diag<T>(g, (a > 0 && idx[a] == c) ? ic : di[a]);
+            const T z = rv[a] * inv;
+            dn[a] = first ? z : z + tb * dv[a];
+        }
+        const T qc = cx * ((dn[0] - dn[1]) + (dn[0] - dn[2])) +
+                     cy * ((dn[0] - dn[3]) + (dn[0] - dn[4])) +
+                     cz * ((dn[0] - dn[5]) + (dn[0] - dn[6]));
+        dnew[c] = dn[0];
+        q[c] = qc;
+        s[0] += (double)dn[0] * (double)qc;
+    }
+    publish_partials<1, 1>(s, m, pout);
+}
+
+// alpha = r.z / d'.q; x += alpha d', r -= alpha q; partials r.z, r.r, max|r|.
+template <typename T, typename TX>
+__global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x,
+                                                      T *__restrict__ r,
+                                                      const T *__restrict__ d,
+                                                      const T *__restrict__ q,
+                                                      const double *__restrict__ pin,
+                                                      double *__restrict__ pout,
+                                                      const CgStateA *SA, CgStateB *SB) {
+    if (SA->done || SB->done) return;
+    double f[1], fm[1];
+    fold_partials<1, 1>(pin, gridDim.x, f, fm);
+    const double den = f[0];
+    int status = ST_OK;
+    double alpha = 0.0;
+    if (fabs(den) < 1.0e-30 || !isfinite(den)) {
+        status = ST_SINGULAR;
+    } else {
+        alpha = SA->rz / den;
+        if (!isfinite(alpha)) status = ST_ALPHA;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        SB->rz_prev = SA->rz;
+        SB->alpha = alpha;
+        SB->status = status;
+        SB->first = 0;
+        if (status != ST_OK) SB->done = 1;
+        else SB->iters += 1;
+    }
+    if (status != ST_OK) return;
+    const T ta = (T)alpha;
+    double s[2] = {0.0, 0.0}, m[1] = {0.0};
+    TileSched ts((g.n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t c = t * kBlock + threadIdx.x;
+        if (c >= g.n) continue;
+        const TX xc = x[c];
+        const T dc = d[c], rc0 = r[c], qc = q[c];
+        x[c] = xc + (TX)(ta * dc);
+        const T rc = rc0 - ta * qc;
+        r[c] = rc;
+        T z = rc;  // the same z = D^-1 r the direction kernel forms
         if (c != 0) {
             uint32_t i, j, k;
             cell_ijk(g, c, i, j, k);
-            z = z / diag_at<double>(g, i, j, k);
+            z = rc * inv_diag<T>(g, diag_index(g, i, j, k));
         }
-        d[c] = z + beta * d[c];
+        const double rd = (double)rc;
+        s[0] += rd * (double)z;
+        s[1] += rd * rd;
+        m[0] = fmax(m[0], fabs(rd));
     }
+    publish_partials<2, 1>(s, m, pout);
+}
+
+// x += e (FP64 += FP32): the refinement step of the mixed path.
+__global__ void __launch_bounds__(kBlock) k_add_correction(uint32_t n, double *__restrict__ x,
+                                                           const float *__restrict__ e) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        x[c] += (double)e[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -453,16 +508,28 @@ struct fl_cavity3d {
     StepParams P{};
     size_t nu = 0, nv = 0, nw = 0, n = 0;
     double *u = nullptr, *v = nullptr, *w = nullptr, *us = nullptr, *vs = nullptr,
-           *ws = nullptr, *p = nullptr, *rhs = nullptr, *r = nullptr, *d = nullptr,
-           *q = nullptr, *partials = nullptr;
-    PcgState *st = nullptr, *st_host = nullptr;
-    DiagState *dg = nullptr, *dg_host = nullptr;
+           *ws = nullptr, *p = nullptr, *rhs = nullptr, *partials = nullptr;
+    // FP64 PCG vectors
+    double *r = nullptr, *d[2] = {nullptr, nullptr}, *q = nullptr;
+    // FP32 inner-solver vectors of the mixed path
+    float *e32 = nullptr, *r32 = nullptr, *d32[2] = {nullptr, nullptr}, *q32 = nullptr;
+    double *pa = nullptr, *pb = nullptr;  // partials of dirapply / update
+    CgStateA *sa = nullptr;
+    CgStateB *sb = nullptr;
+    DiagState *dg = nullptr;
+    struct Host {
+        CgStateA a;
+        CgStateB b;
+        DiagState dg;
+        double outer[3];
+    } *host = nullptr;
+    double *outer = nullptr;  // folded outer residual: r.z, r.r, max|r|
     uint32_t blocks = 1, ew_blocks = 1;
     int64_t step = 0;
     // per-kernel timing (HIP events on the solver's stream)
     bool timing = false;
     std::vector<hipEvent_t> ev;
-    double t_apply = 0, t_update = 0, t_dir = 0;
+    double t_apply = 0, t_update = 0;
     int64_t t_iters = 0;
 };
 
@@ -470,21 +537,31 @@ namespace {
 
 constexpr int kMaxBatch = 64;
 
-fl_status alloc(fl_cavity3d *s, double **ptr, size_t count) {
-    FL_HIP_TRY(s->ctx, hipMalloc(ptr, count * sizeof(double)));
-    FL_HIP_TRY(s->ctx, hipMemsetAsync(*ptr, 0, count * sizeof(double), s->ctx->stream));
+__global__ void k_fold_outer(const double *__restrict__ partials, uint32_t rows, double *out) {
+    double s[2], m[1];
+    fold_partials<2, 1>(partials, rows, s, m);
+    if (threadIdx.x == 0) {
+        out[0] = s[0];
+        out[1] = s[1];
+        out[2] = m[0];
+    }
+}
+
+template <typename T>
+fl_status alloc(fl_cavity3d *s, T **ptr, size_t count) {
+    FL_HIP_TRY(s->ctx, hipMalloc(ptr, count * sizeof(T)));
+    FL_HIP_TRY(s->ctx, hipMemsetAsync(*ptr, 0, count * sizeof(T), s->ctx->stream));
     return FL_OK;
 }
 
 void free_all(fl_cavity3d *s) {
-    double *bufs[] = {s->u, s->v, s->w, s->us, s->vs, s->ws, s->p,
-                      s->rhs, s->r, s->d, s->q, s->partials};
-    for (double *b : bufs)
+    void *bufs[] = {s->u,   s->v,      s->w,      s->us,  s->vs,     s->ws,     s->p,
+                    s->rhs, s->r,      s->d[0],   s->d[1], s->q,     s->e32,    s->r32,
+                    s->d32[0], s->d32[1], s->q32, s->pa,   s->pb,     s->partials, s->sa,
+                    s->sb,  s->dg,     s->outer};
+    for (void *b : bufs)
         if (b) (void)hipFree(b);
-    if (s->st) (void)hipFree(s->st);
-    if (s->dg) (void)hipFree(s->dg);
-    if (s->st_host) (void)hipHostFree(s->st_host);
-    if (s->dg_host) (void)hipHostFree(s->dg_host);
+    if (s->host) (void)hipHostFree(s->host);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
     s->ev.clear();
 }
@@ -512,11 +589,13 @@ fl_status validate(fl_ctx *ctx, const fl_cavity3d_config *c) {
         return set_error(ctx, FL_ERR_CONFIG, "unknown stop rule %d", c->stop_rule);
     if (c->initial_guess != FL_GUESS_PREVIOUS && c->initial_guess != FL_GUESS_ZERO)
         return set_error(ctx, FL_ERR_CONFIG, "unknown initial guess %d", c->initial_guess);
-    if (c->precision != FL_PRECISION_FP64)
-        return set_error(ctx, FL_ERR_CONFIG, "unsupported precision %d", c->precision);
+    if (c->precision != FL_PRECISION_FP64 && c->precision != FL_PRECISION_MIXED)
+        return set_error(ctx, FL_ERR_CONFIG, "unknown precision %d", c->precision);
+    if (!(c->inner_tolerance >= 0.0 && c->inner_tolerance < 1.0))
+        return set_error(ctx, FL_ERR_CONFIG, "inner_tolerance must lie in [0, 1)");
     const double cells = (double)c->nx * (double)c->ny * (double)c->nz;
     const double faces = (double)(c->nx + 1) * (double)(c->ny + 1) * (double)(c->nz + 1) * 3.0;
-    if (cells + faces >= 4294967295.0)
+    if (cells + faces >= 2147483647.0)
         return set_error(ctx, FL_ERR_CONFIG, "grid too large for 32-bit cell indexing");
     return FL_OK;
 }
@@ -530,8 +609,131 @@ const char *status_text(int status) {
     }
 }
 
-// Pinned Jacobi-PCG on the device.  p holds the guess on entry and the
-// solution on exit; rhs is the device right-hand side.
+// Runs CG iterations (dirapply + update pairs) on vectors of type T until
+// the device state says done.  The "update" partials of iteration 0 must
+// already be in s->pb (from k_residual).
+template <typename T, typename TX>
+fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf[2], T *q) {
+    fl_ctx *ctx = s->ctx;
+    hipStream_t stm = ctx->stream;
+    s->host->a = CgStateA{};
+    s->host->b = CgStateB{};
+    s->host->b.first = 1;
+    FL_HIP_TRY(ctx, hipMemcpyAsync(s->sa, &s->host->a, sizeof(CgStateA), hipMemcpyHostToDevice, stm));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(s->sb, &s->host->b, sizeof(CgStateB), hipMemcpyHostToDevice, stm));
+    const uint32_t G = s->blocks;
+    int batch = 8;
+    long long launched = 0, done_iters = 0;
+    for (;;) {
+        for (int b = 0; b < batch; ++b) {
+            // iteration `launched` reads d[launched % 2] and writes the other
+            T *dold = dbuf[launched & 1], *dnew = dbuf[(launched + 1) & 1];
+            if (s->timing) FL_HIP_TRY(ctx, hipEventRecord(s->ev[3 * b], stm));
+            k_cg_dirapply<T><<<G, kBlock, 0, stm>>>(s->g, prm, r, dold, dnew, q, s->pb, s->pa,
+                                                    s->sa, s->sb);
+            if (s->timing) FL_HIP_TRY(ctx, hipEventRecord(s->ev[3 * b + 1], stm));
+            k_cg_update<T, TX><<<G, kBlock, 0, stm>>>(s->g, x, r, dnew, q, s->pa, s->pb, s->sa,
+                                                      s->sb);
+            if (s->timing) FL_HIP_TRY(ctx, hipEventRecord(s->ev[3 * b + 2], stm));
+            ++launched;
+        }
+        FL_HIP_TRY(ctx, hipGetLastError());
+        FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->a, s->sa, sizeof(CgStateA), hipMemcpyDeviceToHost, stm));
+        FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->b, s->sb, sizeof(CgStateB), hipMemcpyDeviceToHost, stm));
+        FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
+        if (s->timing) {
+            // only iterations whose update ran count (later launches exit early)
+            const long long active = std::min<long long>(batch, s->host->b.iters - done_iters);
+            for (long long b = 0; b < active; ++b) {
+                float ta = 0, tu = 0;
+                FL_HIP_TRY(ctx, hipEventElapsedTime(&ta, s->ev[3 * b], s->ev[3 * b + 1]));
+                FL_HIP_TRY(ctx, hipEventElapsedTime(&tu, s->ev[3 * b + 1], s->ev[3 * b + 2]));
+                s->t_apply += ta;
+                s->t_update += tu;
+                s->t_iters += 1;
+            }
+        }
+        done_iters = s->host->b.iters;
+        if (s->host->a.done || s->host->b.done) break;
+        batch = std::min(batch * 2, kMaxBatch);
+    }
+    return FL_OK;
+}
+
+fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep) {
+    fl_ctx *ctx = s->ctx;
+    hipStream_t stm = ctx->stream;
+    k_residual<double><<<s->blocks, kBlock, 0, stm>>>(s->g, s->rhs, s->p, s->r, s->pb);
+    FL_HIP_TRY(ctx, hipGetLastError());
+    const CgParams prm{s->cfg.pressure_tolerance, s->cfg.pressure_max_iters, s->cfg.stop_rule};
+    fl_status st = run_cg<double, double>(s, prm, s->p, s->r, s->d, s->q);
+    if (st != FL_OK) return st;
+    const CgStateA &a = s->host->a;
+    const CgStateB &b = s->host->b;
+    const int status = b.status != ST_OK ? b.status : a.status;
+    if (rep) {
+        rep->iterations = b.iters;
+        rep->refinements = 0;
+        rep->residual_abs_inf = a.rmax;
+        rep->residual_l2 = std::sqrt(a.rr);
+        rep->initial_l2 = std::sqrt(a.rr0);
+        rep->converged = a.converged;
+        rep->status = status == ST_OK ? FL_OK : FL_ERR_SOLVER;
+    }
+    if (status != ST_OK) return set_error(ctx, FL_ERR_SOLVER, "%s", status_text(status));
+    return FL_OK;
+}
+
+// FP32 CG inside FP64 iterative refinement.
+fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
+    fl_ctx *ctx = s->ctx;
+    hipStream_t stm = ctx->stream;
+    const double inner_tol = s->cfg.inner_tolerance > 0.0 ? s->cfg.inner_tolerance : 1.0e-5;
+    const bool rel = s->cfg.stop_rule == FL_STOP_REL_L2;
+    double rr0 = 0.0;
+    long long total = 0, sweeps = 0;
+    bool converged = false;
+    double rr = 0.0, rmax = 0.0;
+    for (;;) {
+        k_residual<float><<<s->blocks, kBlock, 0, stm>>>(s->g, s->rhs, s->p, s->r32, s->pb);
+        k_fold_outer<<<1, kBlock, 0, stm>>>(s->pb, s->blocks, s->outer);
+        FL_HIP_TRY(ctx, hipGetLastError());
+        FL_HIP_TRY(ctx, hipMemcpyAsync(s->host->outer, s->outer, 3 * sizeof(double), hipMemcpyDeviceToHost, stm));
+        FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
+        rr = s->host->outer[1];
+        rmax = s->host->outer[2];
+        if (sweeps == 0) rr0 = rr;
+        if (!std::isfinite(rmax))
+            return set_error(ctx, FL_ERR_SOLVER, "%s", status_text(ST_RESIDUAL));
+        converged = rel ? std::sqrt(rr) <= s->cfg.pressure_tolerance * std::sqrt(rr0)
+                        : rmax < s->cfg.pressure_tolerance;
+        if (converged || total >= s->cfg.pressure_max_iters || sweeps >= 64) break;
+        FL_HIP_TRY(ctx, hipMemsetAsync(s->e32, 0, s->n * sizeof(float), stm));
+        const CgParams prm{inner_tol, s->cfg.pressure_max_iters - total, FL_STOP_REL_L2};
+        fl_status st = run_cg<float, float>(s, prm, s->e32, s->r32, s->d32, s->q32);
+        if (st != FL_OK) return st;
+        const CgStateB &b = s->host->b;
+        const int status = b.status != ST_OK ? b.status : s->host->a.status;
+        if (status != ST_OK) return set_error(ctx, FL_ERR_SOLVER, "%s", status_text(status));
+        total += b.iters;
+        sweeps += 1;
+        k_add_correction<<<s->ew_blocks, kBlock, 0, stm>>>((uint32_t)s->n, s->p, s->e32);
+        if (b.iters == 0) break;  // no progress possible in FP32
+    }
+    if (rep) {
+        rep->iterations = total;
+        rep->refinements = sweeps;
+        rep->residual_abs_inf = rmax;
+        rep->residual_l2 = std::sqrt(rr);
+        rep->initial_l2 = std::sqrt(rr0);
+        rep->converged = converged;
+        rep->status = FL_OK;
+    }
+    return FL_OK;
+}
+
+// Pressure solve: p holds the guess on entry and the solution on exit; rhs
+// is the device right-hand side.
 fl_status solve_pressure(fl_cavity3d *s, fl_solve_report *rep) {
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
@@ -539,61 +741,7 @@ fl_status solve_pressure(fl_cavity3d *s, fl_solve_report *rep) {
         FL_HIP_TRY(ctx, hipMemsetAsync(s->p, 0, s->n * sizeof(double), stm));
     else
         FL_HIP_TRY(ctx, hipMemsetAsync(s->p, 0, sizeof(double), stm));  // pin p(0,0,0) = 0
-    PcgState init{};
-    init.tol = s->cfg.pressure_tolerance;
-    init.stop_rule = s->cfg.stop_rule;
-    init.max_iter = s->cfg.pressure_max_iters;
-    *s->st_host = init;
-    FL_HIP_TRY(ctx, hipMemcpyAsync(s->st, s->st_host, sizeof(PcgState), hipMemcpyHostToDevice, stm));
-    k_pcg_init<<<s->blocks, kBlock, 0, stm>>>(s->g, s->rhs, s->p, s->r, s->d, s->partials, s->st);
-    FL_HIP_TRY(ctx, hipGetLastError());
-    int batch = 8;
-    long long done_iters = 0;
-    for (;;) {
-        for (int b = 0; b < batch; ++b) {
-            if (s->timing) FL_HIP_TRY(ctx, hipEventRecord(s->ev[4 * b], stm));
-            k_pcg_apply<<<s->blocks, kBlock, 0, stm>>>(s->g, s->d, s->q, s->partials, s->st);
-            if (s->timing) FL_HIP_TRY(ctx, hipEventRecord(s->ev[4 * b + 1], stm));
-            k_pcg_update<<<s->blocks, kBlock, 0, stm>>>(s->g, s->p, s->r, s->d, s->q,
-                                                        s->partials, s->st);
-            if (s->timing) FL_HIP_TRY(ctx, hipEventRecord(s->ev[4 * b + 2], stm));
-            k_pcg_direction<<<s->blocks, kBlock, 0, stm>>>(s->g, s->r, s->d, s->st);
-            if (s->timing) FL_HIP_TRY(ctx, hipEventRecord(s->ev[4 * b + 3], stm));
-        }
-        FL_HIP_TRY(ctx, hipGetLastError());
-        FL_HIP_TRY(ctx, hipMemcpyAsync(s->st_host, s->st, sizeof(PcgState),
-                                       hipMemcpyDeviceToHost, stm));
-        FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
-        if (s->timing) {
-            // Only iterations that did work count (later launches exit early).
-            const long long active = std::min<long long>(batch, s->st_host->iter - done_iters);
-            for (long long b = 0; b < active; ++b) {
-                float a = 0, u = 0, dd = 0;
-                FL_HIP_TRY(ctx, hipEventElapsedTime(&a, s->ev[4 * b], s->ev[4 * b + 1]));
-                FL_HIP_TRY(ctx, hipEventElapsedTime(&u, s->ev[4 * b + 1], s->ev[4 * b + 2]));
-                FL_HIP_TRY(ctx, hipEventElapsedTime(&dd, s->ev[4 * b + 2], s->ev[4 * b + 3]));
-                s->t_apply += a;
-                s->t_update += u;
-                s->t_dir += dd;
-                s->t_iters += 1;
-            }
-        }
-        done_iters = s->st_host->iter;
-        if (s->st_host->done) break;
-        batch = std::min(batch * 2, kMaxBatch);
-    }
-    const PcgState &h = *s->st_host;
-    if (rep) {
-        rep->iterations = h.iter;
-        rep->refinements = 0;
-        rep->residual_abs_inf = h.rmax;
-        rep->residual_l2 = std::sqrt(h.rr);
-        rep->initial_l2 = std::sqrt(h.rr0);
-        rep->converged = h.converged;
-        rep->status = h.status == ST_OK ? FL_OK : FL_ERR_SOLVER;
-    }
-    if (h.status != ST_OK) return set_error(ctx, FL_ERR_SOLVER, "%s", status_text(h.status));
-    return FL_OK;
+    return s->cfg.precision == FL_PRECISION_MIXED ? solve_mixed(s, rep) : solve_fp64(s, rep);
 }
 
 fl_status run_diag(fl_cavity3d *s, fl_cavity3d_step_report *rep, bool check_finite) {
@@ -601,17 +749,17 @@ fl_status run_diag(fl_cavity3d *s, fl_cavity3d_step_report *rep, bool check_fini
     hipStream_t stm = ctx->stream;
     DiagState init{};
     for (int f = 0; f < 4; ++f) init.first_bad[f] = ~0ull;
-    *s->dg_host = init;
-    FL_HIP_TRY(ctx, hipMemcpyAsync(s->dg, s->dg_host, sizeof(DiagState), hipMemcpyHostToDevice, stm));
+    s->host->dg = init;
+    FL_HIP_TRY(ctx, hipMemcpyAsync(s->dg, &s->host->dg, sizeof(DiagState), hipMemcpyHostToDevice, stm));
     k_diag<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->u, s->v, s->w, s->partials, s->dg);
     if (check_finite)
         k_finite<<<s->ew_blocks, kBlock, 0, stm>>>(s->p, (uint32_t)s->n, s->u, (uint32_t)s->nu,
                                                    s->v, (uint32_t)s->nv, s->w, (uint32_t)s->nw,
                                                    s->dg);
     FL_HIP_TRY(ctx, hipGetLastError());
-    FL_HIP_TRY(ctx, hipMemcpyAsync(s->dg_host, s->dg, sizeof(DiagState), hipMemcpyDeviceToHost, stm));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->dg, s->dg, sizeof(DiagState), hipMemcpyDeviceToHost, stm));
     FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
-    const DiagState &h = *s->dg_host;
+    const DiagState &h = s->host->dg;
     if (rep) {
         rep->max_divergence = h.max_div;
         rep->divergence_l2 = std::sqrt(h.div_l2sq);
@@ -676,7 +824,14 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
     const uint32_t nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
     const double dx = cfg->length / nx, dy = cfg->height / ny, dz = cfg->depth / nz;
     s->g = Grid{nx, ny, nz, nx * ny, nx * ny * nz, 1.0 / (dx * dx), 1.0 / (dy * dy),
-                1.0 / (dz * dz)};
+                1.0 / (dz * dz), FastDiv(nx), FastDiv(nx * ny), {}, {}};
+    for (uint32_t b = 0; b < 8; ++b) {
+        // neighbours per axis: 1 + bit; summed in x, y, z order
+        const double diag = s->g.ix2 * (double)(1 + (b & 1)) + s->g.iy2 * (double)(1 + (b >> 1 & 1)) +
+                            s->g.iz2 * (double)(1 + (b >> 2 & 1));
+        s->g.inv[b] = 1.0 / diag;
+        s->g.invf[b] = (float)(1.0 / diag);
+    }
     s->P = StepParams{nx, ny, nz, dx, dy, dz, cfg->dt,
                       cfg->lid_velocity * cfg->length / cfg->reynolds,  // viscosity :86-88
                       cfg->lid_velocity, cfg->density};
@@ -685,32 +840,34 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
     s->nv = (size_t)nx * (ny + 1) * nz;
     s->nw = (size_t)nx * ny * (nz + 1);
     const uint64_t ntiles = (s->n + kBlock - 1) / kBlock;
-    s->blocks = grid_blocks(ctx, ntiles);
+    s->blocks = grid_blocks(ctx, ntiles, 4);
     s->ew_blocks = grid_blocks(ctx, (s->nu + s->nv + s->nw + kBlock - 1) / kBlock);
-    double **fields[] = {&s->u, &s->v, &s->w, &s->us, &s->vs, &s->ws};
-    size_t lens[] = {s->nu, s->nv, s->nw, s->nu, s->nv, s->nw};
-    for (int a = 0; a < 6; ++a)
-        if ((st = alloc(s, fields[a], lens[a])) != FL_OK) goto fail;
-    {
-        double **cells[] = {&s->p, &s->rhs, &s->r, &s->d, &s->q};
-        for (double **c : cells)
-            if ((st = alloc(s, c, s->n)) != FL_OK) goto fail;
-    }
-    if ((st = alloc(s, &s->partials, (size_t)std::max(s->blocks, s->ew_blocks) * 4)) != FL_OK)
+    const bool mixed = cfg->precision == FL_PRECISION_MIXED;
+    const size_t prow = (size_t)std::max(s->blocks, s->ew_blocks) * 4;
+    if ((st = alloc(s, &s->u, s->nu)) || (st = alloc(s, &s->v, s->nv)) ||
+        (st = alloc(s, &s->w, s->nw)) || (st = alloc(s, &s->us, s->nu)) ||
+        (st = alloc(s, &s->vs, s->nv)) || (st = alloc(s, &s->ws, s->nw)) ||
+        (st = alloc(s, &s->p, s->n)) || (st = alloc(s, &s->rhs, s->n)) ||
+        (st = alloc(s, &s->partials, prow)) || (st = alloc(s, &s->pa, prow)) ||
+        (st = alloc(s, &s->pb, prow)) || (st = alloc(s, &s->outer, 4)) ||
+        (st = alloc(s, &s->sa, 1)) || (st = alloc(s, &s->sb, 1)) || (st = alloc(s, &s->dg, 1)))
         goto fail;
-    if (hipMalloc(&s->st, sizeof(PcgState)) != hipSuccess ||
-        hipMalloc(&s->dg, sizeof(DiagState)) != hipSuccess ||
-        hipHostMalloc(&s->st_host, sizeof(PcgState), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(&s->dg_host, sizeof(DiagState), hipHostMallocDefault) != hipSuccess) {
-        st = set_error(ctx, FL_ERR_OUT_OF_MEMORY, "device allocation failed");
-        goto fail;
+    if (mixed) {
+        if ((st = alloc(s, &s->e32, s->n)) || (st = alloc(s, &s->r32, s->n)) ||
+            (st = alloc(s, &s->d32[0], s->n)) || (st = alloc(s, &s->d32[1], s->n)) ||
+            (st = alloc(s, &s->q32, s->n)))
+            goto fail;
+    } else {
+        if ((st = alloc(s, &s->r, s->n)) || (st = alloc(s, &s->d[0], s->n)) ||
+            (st = alloc(s, &s->d[1], s->n)) || (st = alloc(s, &s->q, s->n)))
+            goto fail;
     }
-    if (hipMemsetAsync(s->st, 0, sizeof(PcgState), ctx->stream) != hipSuccess ||
-        hipMemsetAsync(s->dg, 0, sizeof(DiagState), ctx->stream) != hipSuccess ||
+    if (hipHostMalloc(&s->host, sizeof(fl_cavity3d::Host), hipHostMallocDefault) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        st = set_error(ctx, FL_ERR_HIP, "device initialisation failed");
+        st = set_error(ctx, FL_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
         goto fail;
     }
+    ctx_retain(ctx);
     *out = s;
     return FL_OK;
 fail:
@@ -721,9 +878,12 @@ fail:
 
 void fl_cavity3d_destroy(fl_cavity3d *s) {
     if (!s) return;
-    (void)hipStreamSynchronize(s->ctx->stream);
+    fl_ctx *ctx = s->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     free_all(s);
     delete s;
+    ctx_release(ctx);
 }
 
 fl_status fl_cavity3d_step(fl_cavity3d *s, fl_cavity3d_step_report *rep) {
@@ -804,11 +964,11 @@ fl_status fl_cavity3d_set_timing(fl_cavity3d *s, int32_t enable) {
     fl_ctx *ctx = s->ctx;
     FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
     if (enable && s->ev.empty()) {
-        s->ev.resize(4 * kMaxBatch);
+        s->ev.resize(3 * kMaxBatch);
         for (auto &e : s->ev) FL_HIP_TRY(ctx, hipEventCreate(&e));
     }
     s->timing = enable != 0;
-    s->t_apply = s->t_update = s->t_dir = 0.0;
+    s->t_apply = s->t_update = 0.0;
     s->t_iters = 0;
     return FL_OK;
 }
@@ -820,8 +980,8 @@ fl_status fl_cavity3d_kernel_times(fl_cavity3d *s, double *ms_per_iteration, dou
     const double n = s->t_iters > 0 ? (double)s->t_iters : 1.0;
     if (ms_apply) *ms_apply = s->t_apply / n;
     if (ms_update) *ms_update = s->t_update / n;
-    if (ms_direction) *ms_direction = s->t_dir / n;
-    if (ms_per_iteration) *ms_per_iteration = (s->t_apply + s->t_update + s->t_dir) / n;
+    if (ms_direction) *ms_direction = 0.0;  // fused into the apply kernel
+    if (ms_per_iteration) *ms_per_iteration = (s->t_apply + s->t_update) / n;
     if (timed_iterations) *timed_iterations = s->t_iters;
     return FL_OK;
 }

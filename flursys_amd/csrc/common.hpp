@@ -17,9 +17,17 @@ struct fl_ctx {
     hipStream_t stream = nullptr;
     int num_cu = 256;
     std::string last_error;
+    // Objects created from this context hold a reference; fl_destroy only
+    // marks the context closed while any of them is alive, and the last one
+    // to go frees it.
+    int refs = 0;
+    bool closed = false;
 };
 
 namespace fl {
+
+void ctx_release(fl_ctx *ctx);  // drops one reference (context.hip)
+inline void ctx_retain(fl_ctx *ctx) { ctx->refs += 1; }
 
 constexpr int kBlock = 256;   // threads per workgroup: 4 wave64s
 constexpr int kWaves = kBlock / 64;
@@ -144,6 +152,76 @@ __device__ inline bool grid_reduce(double (&s)[NS], double (&m)[NM], double *par
     }
     return true;
 }
+
+// Fence-free reductions.  A producer kernel publishes one row of partials
+// per block (plain stores by thread 0); the consumer kernel -- which starts
+// after the producer has finished, so the stores are visible -- folds all
+// rows in block order in every block.  Every block therefore derives the same
+// scalars (alpha, beta, the stop test) without atomics, device-scope fences
+// or L2 write-backs, and the sums are bitwise repeatable.
+template <int NS, int NM>
+__device__ inline void publish_partials(double (&s)[NS], double (&m)[NM], double *partials) {
+    block_reduce<NS, NM>(s, m);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int a = 0; a < NS; ++a) partials[blockIdx.x * (NS + NM) + a] = s[a];
+#pragma unroll
+        for (int a = 0; a < NM; ++a) partials[blockIdx.x * (NS + NM) + NS + a] = m[a];
+    }
+}
+
+template <int NS, int NM>
+__device__ inline void fold_partials(const double *__restrict__ partials, uint32_t rows,
+                                     double (&s)[NS], double (&m)[NM]) {
+    __shared__ double out[NS + NM];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) s[a] = 0.0;
+#pragma unroll
+    for (int a = 0; a < NM; ++a) m[a] = 0.0;
+    for (uint32_t b = threadIdx.x; b < rows; b += blockDim.x) {
+        const double *pb = partials + (size_t)b * (NS + NM);
+#pragma unroll
+        for (int a = 0; a < NS; ++a) s[a] += pb[a];
+#pragma unroll
+        for (int a = 0; a < NM; ++a) m[a] = fmax(m[a], pb[NS + a]);
+    }
+    block_reduce<NS, NM>(s, m);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int a = 0; a < NS; ++a) out[a] = s[a];
+#pragma unroll
+        for (int a = 0; a < NM; ++a) out[NS + a] = m[a];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < NS; ++a) s[a] = out[a];
+#pragma unroll
+    for (int a = 0; a < NM; ++a) m[a] = out[NS + a];
+    __syncthreads();
+}
+
+// Unsigned 32-bit division by a run-time constant d >= 2 with a multiply-
+// high and shifts (the round-down "branchfree" magic-number method of
+// Granlund & Montgomery / libdivide): q = (t + ((n - t) >> 1)) >> s with
+// t = mulhi(n, m), l = ceil(log2 d), m = 2^32 (2^l - d) / d + 1, s = l - 1.
+struct FastDiv {
+    uint32_t d = 2, m = 1, s = 0;
+    FastDiv() = default;
+    explicit FastDiv(uint32_t divisor) : d(divisor) {
+        uint32_t l = 0;
+        while ((1ull << l) < divisor) ++l;
+        m = (uint32_t)(((1ull << 32) * ((1ull << l) - divisor)) / divisor + 1);
+        s = l - 1;
+    }
+    __host__ __device__ inline uint32_t div(uint32_t n) const {
+#ifdef __HIP_DEVICE_COMPILE__
+        const uint32_t t = __umulhi(n, m);
+#else
+        const uint32_t t = (uint32_t)(((uint64_t)n * m) >> 32);
+#endif
+        return (t + ((n - t) >> 1)) >> s;
+    }
+};
 
 // Tile scheduling.  Work is cut into 256-cell tiles in linear (x-fastest)
 // order.  When the grid is a multiple of 8 blocks, each XCD (blocks b with

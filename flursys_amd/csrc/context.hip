@@ -73,8 +73,7 @@ fl_status fl_create(int device, fl_ctx **out) {
     return FL_OK;
 }
 
-void fl_destroy(fl_ctx *ctx) {
-    if (!ctx) return;
+static void ctx_free(fl_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) {
         (void)hipStreamSynchronize(ctx->stream);
@@ -83,8 +82,22 @@ void fl_destroy(fl_ctx *ctx) {
     delete ctx;
 }
 
+void fl_destroy(fl_ctx *ctx) {
+    if (!ctx || ctx->closed) return;
+    ctx->closed = true;
+    if (ctx->refs == 0) ctx_free(ctx);
+}
+
 const char *fl_last_error(const fl_ctx *ctx) { return ctx ? ctx->last_error.c_str() : ""; }
 
 const char *fl_version(void) { return "flursys-hip 0.1.0 (gfx950)"; }
 
 }  // extern "C"
+
+namespace fl {
+void ctx_release(fl_ctx *ctx) {
+    if (!ctx) return;
+    ctx->refs -= 1;
+    if (ctx->refs == 0 && ctx->closed) ctx_free(ctx);
+}
+}  // namespace fl

@@ -1,0 +1,798 @@
+// CSR linear algebra on gfx950: SpMV, CG / Jacobi-PCG and BiCGSTAB.
+//
+// Replaces src/linear/ (csr.rs:89-106 spmv_into, csr.rs:195-251 validate_csr,
+// preconditioner.rs:9-70, solver.rs:62-256).  Errors carry the reference's
+// LinearAlgebraError kinds and &'static str texts.
+//
+// Device layout: int32 row offsets and columns, FP64 values, FP64 vectors.
+// Rows are cut into 256-row tiles handed out XCD-slab-wise (common.hpp), one
+// thread per row (the unstructured Poisson rows hold ~4 entries).
+//
+// Per CG iteration (3 kernels, scalars never leave the device):
+//   k_cg_spmv    q = A d, d.q                    -> alpha
+//   k_cg_update  x += a d, r -= a q, r.r [, z = r/diag, r.z] -> stop test, beta
+//   k_cg_dir     d = z + beta d
+// Per BiCGSTAB iteration (4 kernels): s = r - alpha v is never stored; the
+// second SpMV gathers r and v and forms s on the fly, and the final update
+// recomputes it:
+//   k_bi_p       p = r + beta (p - omega v)
+//   k_bi_spmv_v  v = A p, rhat.v                  -> alpha
+//   k_bi_spmv_t  t = A s, s.s, t.t, t.s          -> s-test, omega
+//   k_bi_update  x += a p + w s, r = s - w t, r.r, rhat.r -> stop test, rho, beta
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+
+namespace fl {
+namespace csr {
+
+constexpr double kBreakdown = 1.0e-30;  // BREAKDOWN_EPSILON, solver.rs:3
+
+// error contexts (the reference's &'static str values)
+enum Ctx : int {
+    C_NONE = 0,
+    C_SPMV_OUTPUT,
+    C_DOT,
+    C_L2,
+    C_PRECOND,
+    C_RESIDUAL,
+    C_CG_DEN,
+    C_CG_STEP,
+    C_CG_REC_DEN,
+    C_CG_REC,
+    C_BI_RHO,
+    C_BI_BETA,
+    C_BI_ALPHA_DEN,
+    C_BI_ALPHA,
+    C_BI_OMEGA_DEN,
+    C_BI_OMEGA,
+};
+const char *ctx_text(int c) {
+    switch (c) {
+        case C_SPMV_OUTPUT: return "SpMV output";
+        case C_DOT: return "dot product";
+        case C_L2: return "L2 norm";
+        case C_PRECOND: return "preconditioned residual";
+        case C_RESIDUAL: return "residual";
+        case C_CG_DEN: return "zero CG search denominator";
+        case C_CG_STEP: return "non-finite CG step";
+        case C_CG_REC_DEN: return "zero CG recurrence denominator";
+        case C_CG_REC: return "non-finite CG recurrence";
+        case C_BI_RHO: return "zero BiCGSTAB rho";
+        case C_BI_BETA: return "non-finite BiCGSTAB beta";
+        case C_BI_ALPHA_DEN: return "zero BiCGSTAB alpha denominator";
+        case C_BI_ALPHA: return "non-finite BiCGSTAB alpha";
+        case C_BI_OMEGA_DEN: return "zero BiCGSTAB omega denominator";
+        case C_BI_OMEGA: return "zero BiCGSTAB omega";
+        default: return "";
+    }
+}
+
+struct Mat {
+    const int *off;
+    const int *col;
+    const double *val;
+    uint32_t n;
+};
+
+struct LinState {
+    double rho, rho_prev, alpha, beta, omega;
+    double threshold, initial, rnorm;
+    long long iter, max_iter;
+    int done, converged, s_conv, status, ctx;
+    long long err_index;
+    unsigned long long bad_row;  // first non-finite row of the pending SpMV
+    unsigned long long bad_z;    // first non-finite preconditioned entry
+    unsigned counters[8];
+};
+
+__device__ inline double row_dot(const Mat &A, uint32_t row, const double *__restrict__ x) {
+    const int b = A.off[row], e = A.off[row + 1];
+    double s = 0.0;
+    for (int k = b; k < e; ++k) s += A.val[k] * x[A.col[k]];
+    return s;
+}
+
+__device__ inline void fail(LinState *st, int status, int ctx, long long index) {
+    st->status = status;
+    st->ctx = ctx;
+    st->err_index = index;
+    st->done = 1;
+}
+
+__device__ inline unsigned long long atomic_read(unsigned long long *p) {
+    return atomicAdd(p, 0ull);
+}
+
+// ------------------------------------------------------------------ SpMV
+__global__ void __launch_bounds__(kBlock) k_spmv(Mat A, const double *__restrict__ x,
+                                                 double *__restrict__ y, LinState *st) {
+    TileSched ts((A.n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t row = t * kBlock + threadIdx.x;
+        if (row >= A.n) continue;
+        const double v = row_dot(A, row, x);
+        if (!isfinite(v)) atomicMin(&st->bad_row, (unsigned long long)row);
+        y[row] = v;
+    }
+}
+
+// --------------------------------------------------------- CG / PCG init
+// r = b - q (q = A x from k_spmv), ||r||; stop test at iteration 0.
+__global__ void __launch_bounds__(kBlock) k_cg_init_r(uint32_t n, const double *__restrict__ b,
+                                                      const double *__restrict__ q,
+                                                      double *__restrict__ r, double *partials,
+                                                      LinState *st) {
+    double s[1] = {0.0}, m[1] = {0.0};
+    TileSched ts((n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        if (i >= n) continue;
+        const double v = b[i] - q[i];
+        if (!isfinite(v)) atomicMin(&st->bad_z, (unsigned long long)i);
+        r[i] = v;
+        s[0] += v * v;
+    }
+    if (grid_reduce<1, 1>(s, m, partials, &st->counters[0]) && threadIdx.x == 0) {
+        const unsigned long long bad_row = atomic_read(&st->bad_row);
+        if (bad_row != ~0ull) return fail(st, FL_ERR_NON_FINITE, C_SPMV_OUTPUT, bad_row);
+        const unsigned long long bad = atomic_read(&st->bad_z);
+        if (bad != ~0ull) return fail(st, FL_ERR_NON_FINITE, C_RESIDUAL, bad);
+        if (!isfinite(s[0])) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
+        const double init = sqrt(s[0]);
+        st->initial = init;
+        st->rnorm = init;
+        st->threshold = st->threshold + st->alpha * init;  // atol + rtol * ||r0|| (packed on entry)
+        st->alpha = 0.0;
+        st->iter = 0;
+        if (init <= st->threshold) {
+            st->converged = 1;
+            st->done = 1;
+        }
+    }
+}
+
+// z = D^-1 r (or r), d = z, rho = r.z.
+__global__ void __launch_bounds__(kBlock) k_cg_init_d(uint32_t n, const double *__restrict__ r,
+                                                      const double *__restrict__ inv,
+                                                      double *__restrict__ d, double *partials,
+                                                      LinState *st) {
+    if (st->done) return;
+    double s[1] = {0.0}, m[1] = {0.0};
+    TileSched ts((n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        if (i >= n) continue;
+        const double z = inv ? r[i] * inv[i] : r[i];
+        if (!isfinite(z)) atomicMin(&st->bad_z, (unsigned long long)i);
+        d[i] = z;
+        s[0] += r[i] * z;
+    }
+    if (grid_reduce<1, 1>(s, m, partials, &st->counters[1]) && threadIdx.x == 0) {
+        const unsigned long long bad = atomic_read(&st->bad_z);
+        if (bad != ~0ull) return fail(st, FL_ERR_NON_FINITE, C_PRECOND, bad);
+        if (!isfinite(s[0])) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
+        st->rho = s[0];
+        st->iter = 1;
+        if (st->max_iter < 1) st->done = 1;  // the loop 1..=0 never runs
+    }
+}
+
+// ------------------------------------------------------------ CG loop
+__global__ void __launch_bounds__(kBlock) k_cg_spmv(Mat A, const double *__restrict__ d,
+                                                    double *__restrict__ q, double *partials,
+                                                    LinState *st) {
+    if (st->done) return;
+    double s[1] = {0.0}, m[1] = {0.0};
+    TileSched ts((A.n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t row = t * kBlock + threadIdx.x;
+        if (row >= A.n) continue;
+        const double v = row_dot(A, row, d);
+        if (!isfinite(v)) atomicMin(&st->bad_row, (unsigned long long)row);
+        q[row] = v;
+        s[0] += d[row] * v;
+    }
+    if (grid_reduce<1, 1>(s, m, partials, &st->counters[2]) && threadIdx.x == 0) {
+        const unsigned long long bad = atomic_read(&st->bad_row);
+        if (bad != ~0ull) return fail(st, FL_ERR_NON_FINITE, C_SPMV_OUTPUT, bad);
+        const double den = s[0];
+        if (!isfinite(den)) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
+        if (fabs(den) <= kBreakdown) return fail(st, FL_ERR_BREAKDOWN, C_CG_DEN, 0);
+        const double alpha = st->rho / den;
+        if (!isfinite(alpha)) return fail(st, FL_ERR_BREAKDOWN, C_CG_STEP, 0);
+        st->alpha = alpha;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t n, double *__restrict__ x,
+                                                      double *__restrict__ r,
+                                                      const double *__restrict__ d,
+                                                      const double *__restrict__ q,
+                                                      const double *__restrict__ inv,
+                                                      double *partials, LinState *st) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    double s[2] = {0.0, 0.0}, m[1] = {0.0};
+    TileSched ts((n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        if (i >= n) continue;
+        x[i] += alpha * d[i];
+        const double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        s[0] += ri * ri;
+        const double z = inv ? ri * inv[i] : ri;
+        if (!isfinite(z)) atomicMin(&st->bad_z, (unsigned long long)i);
+        s[1] += ri * z;
+    }
+    if (grid_reduce<2, 1>(s, m, partials, &st->counters[3]) && threadIdx.x == 0) {
+        // norm_l2 (vector.rs:20-32) then the stop test (solver.rs:126-133)
+        if (!isfinite(s[0])) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
+        const double rn = sqrt(s[0]);
+        st->rnorm = rn;
+        if (rn <= st->threshold) {
+            st->converged = 1;
+            st->done = 1;
+            return;
+        }
+        const unsigned long long bad = atomic_read(&st->bad_z);
+        if (bad != ~0ull) return fail(st, FL_ERR_NON_FINITE, C_PRECOND, bad);
+        const double rho_new = s[1];
+        if (!isfinite(rho_new)) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
+        if (fabs(st->rho) <= kBreakdown) return fail(st, FL_ERR_BREAKDOWN, C_CG_REC_DEN, 0);
+        const double beta = rho_new / st->rho;
+        if (!isfinite(beta)) return fail(st, FL_ERR_BREAKDOWN, C_CG_REC, 0);
+        st->beta = beta;
+        st->rho = rho_new;
+        if (st->iter >= st->max_iter) {
+            st->done = 1;  // MaxIterations: a status, not an error
+            return;
+        }
+        st->iter += 1;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, const double *__restrict__ r,
+                                                   const double *__restrict__ inv,
+                                                   double *__restrict__ d, const LinState *st) {
+    if (st->done) return;
+    const double beta = st->beta;
+    TileSched ts((n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        if (i >= n) continue;
+        const double z = inv ? r[i] * inv[i] : r[i];
+        d[i] = z + beta * d[i];
+    }
+}
+
+// ------------------------------------------------------------ BiCGSTAB
+// Initial r = b - A x as for CG (k_spmv + k_cg_init_r); then rhat = r,
+// p = v = 0 and the first rho = rhat.r = r.r.
+__global__ void __launch_bounds__(kBlock) k_bi_init(uint32_t n, const double *__restrict__ r,
+                                                    double *__restrict__ rhat,
+                                                    double *__restrict__ p,
+                                                    double *__restrict__ v, double *partials,
+                                                    LinState *st) {
+    if (st->done) return;
+    double s[1] = {0.0}, m[1] = {0.0};
+    TileSched ts((n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        if (i >= n) continue;
+        const double ri = r[i];
+        rhat[i] = ri;
+        p[i] = 0.0;
+        v[i] = 0.0;
+        s[0] += ri * ri;
+    }
+    if (grid_reduce<1, 1>(s, m, partials, &st->counters[1]) && threadIdx.x == 0) {
+        st->rho_prev = 1.0;
+        st->alpha = 1.0;
+        st->omega = 1.0;
+        st->iter = 1;
+        if (st->max_iter < 1) {
+            st->done = 1;
+            return;
+        }
+        const double rho = s[0];
+        if (!isfinite(rho)) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
+        if (fabs(rho) <= kBreakdown) return fail(st, FL_ERR_BREAKDOWN, C_BI_RHO, 0);
+        const double beta = (rho / st->rho_prev) * (st->alpha / st->omega);
+        if (!isfinite(beta)) return fail(st, FL_ERR_BREAKDOWN, C_BI_BETA, 0);
+        st->rho = rho;
+        st->beta = beta;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_bi_p(uint32_t n, const double *__restrict__ r,
+                                                 double *__restrict__ p,
+                                                 const double *__restrict__ v,
+                                                 const LinState *st) {
+    if (st->done) return;
+    const double beta = st->beta, omega = st->omega;
+    TileSched ts((n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        if (i >= n) continue;
+        p[i] = r[i] + beta * (p[i] - omega * v[i]);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_bi_spmv_v(Mat A, const double *__restrict__ p,
+                                                      const double *__restrict__ rhat,
+                                                      double *__restrict__ v, double *partials,
+                                                      LinState *st) {
+    if (st->done) return;
+    double s[1] = {0.0}, m[1] = {0.0};
+    TileSched ts((A.n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t row = t * kBlock + threadIdx.x;
+        if (row >= A.n) continue;
+        const double vv = row_dot(A, row, p);
+        if (!isfinite(vv)) atomicMin(&st->bad_row, (unsigned long long)row);
+        v[row] = vv;
+        s[0] += rhat[row] * vv;
+    }
+    if (grid_reduce<1, 1>(s, m, partials, &st->counters[2]) && threadIdx.x == 0) {
+        const unsigned long long bad = atomic_read(&st->bad_row);
+        if (bad != ~0ull) return fail(st, FL_ERR_NON_FINITE, C_SPMV_OUTPUT, bad);
+        const double den = s[0];
+        if (!isfinite(den)) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
+        if (fabs(den) <= kBreakdown) return fail(st, FL_ERR_BREAKDOWN, C_BI_ALPHA_DEN, 0);
+        const double alpha = st->rho / den;
+        if (!isfinite(alpha)) return fail(st, FL_ERR_BREAKDOWN, C_BI_ALPHA, 0);
+        st->alpha = alpha;
+    }
+}
+
+// t = A s with s = r - alpha v formed at each gathered column.
+__global__ void __launch_bounds__(kBlock) k_bi_spmv_t(Mat A, const double *__restrict__ r,
+                                                      const double *__restrict__ v,
+                                                      double *__restrict__ tvec,
+                                                      double *partials, LinState *st) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
+    TileSched ts((A.n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t row = t * kBlock + threadIdx.x;
+        if (row >= A.n) continue;
+        const int b = A.off[row], e = A.off[row + 1];
+        double acc = 0.0;
+        for (int k = b; k < e; ++k) {
+            const int c = A.col[k];
+            acc += A.val[k] * (r[c] - alpha * v[c]);
+        }
+        if (!isfinite(acc)) atomicMin(&st->bad_row, (unsigned long long)row);
+        tvec[row] = acc;
+        const double si = r[row] - alpha * v[row];
+        s[0] += si * si;
+        s[1] += acc * acc;
+        s[2] += acc * si;
+    }
+    if (grid_reduce<3, 1>(s, m, partials, &st->counters[3]) && threadIdx.x == 0) {
+        if (!isfinite(s[0])) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
+        const double sn = sqrt(s[0]);
+        if (sn <= st->threshold) {  // early exit on ||s|| (solver.rs:206-218)
+            st->rnorm = sn;
+            st->s_conv = 1;
+            return;
+        }
+        const unsigned long long bad = atomic_read(&st->bad_row);
+        if (bad != ~0ull) return fail(st, FL_ERR_NON_FINITE, C_SPMV_OUTPUT, bad);
+        const double tt = s[1];
+        if (!isfinite(tt)) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
+        if (fabs(tt) <= kBreakdown) return fail(st, FL_ERR_BREAKDOWN, C_BI_OMEGA_DEN, 0);
+        if (!isfinite(s[2])) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
+        const double omega = s[2] / tt;
+        if (!isfinite(omega) || fabs(omega) <= kBreakdown)
+            return fail(st, FL_ERR_BREAKDOWN, C_BI_OMEGA, 0);
+        st->omega = omega;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_bi_update(uint32_t n, double *__restrict__ x,
+                                                      double *__restrict__ r,
+                                                      const double *__restrict__ p,
+                                                      const double *__restrict__ v,
+                                                      const double *__restrict__ tvec,
+                                                      const double *__restrict__ rhat,
+                                                      double *partials, LinState *st) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    if (st->s_conv) {  // solution += alpha p and return Converged
+        TileSched ts((n + kBlock - 1) / kBlock);
+        for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+            const uint32_t i = t * kBlock + threadIdx.x;
+            if (i < n) x[i] += alpha * p[i];
+        }
+        // Only the last block to arrive raises `done`: every block has
+        // passed the early-exit test above by then.
+        double s0[1] = {0.0}, m0[1] = {0.0};
+        if (grid_reduce<1, 1>(s0, m0, partials, &st->counters[4]) && threadIdx.x == 0) {
+            st->converged = 1;
+            st->done = 1;
+        }
+        return;
+    }
+    const double omega = st->omega;
+    double s[2] = {0.0, 0.0}, m[1] = {0.0};
+    TileSched ts((n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        if (i >= n) continue;
+        const double si = r[i] - alpha * v[i];
+        x[i] += alpha * p[i] + omega * si;
+        const double ri = si - omega * tvec[i];
+        r[i] = ri;
+        s[0] += ri * ri;
+        s[1] += rhat[i] * ri;
+    }
+    if (grid_reduce<2, 1>(s, m, partials, &st->counters[4]) && threadIdx.x == 0) {
+        if (!isfinite(s[0])) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
+        const double rn = sqrt(s[0]);
+        st->rnorm = rn;
+        if (rn <= st->threshold) {
+            st->converged = 1;
+            st->done = 1;
+            return;
+        }
+        if (st->iter >= st->max_iter) {
+            st->done = 1;
+            return;
+        }
+        st->iter += 1;
+        st->rho_prev = st->rho;
+        const double rho = s[1];
+        if (!isfinite(rho)) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
+        if (fabs(rho) <= kBreakdown) return fail(st, FL_ERR_BREAKDOWN, C_BI_RHO, 0);
+        const double beta = (rho / st->rho_prev) * (st->alpha / st->omega);
+        if (!isfinite(beta)) return fail(st, FL_ERR_BREAKDOWN, C_BI_BETA, 0);
+        st->rho = rho;
+        st->beta = beta;
+    }
+}
+
+}  // namespace csr
+}  // namespace fl
+
+using namespace fl;
+using namespace fl::csr;
+
+struct fl_csr {
+    fl_ctx *ctx = nullptr;
+    uint32_t n = 0, ncols = 0;
+    int64_t nnz = 0;
+    int *off = nullptr, *col = nullptr;
+    double *val = nullptr;
+    double *inv = nullptr;     // Jacobi inverse diagonal (when valid)
+    int inv_status = FL_OK;    // JacobiPreconditioner::new outcome
+    int64_t inv_row = 0;
+    const char *inv_ctx = "";
+    double *vec[8] = {};       // x, b, r, d/p, q/v, t, rhat, spare
+    double *partials = nullptr;
+    LinState *st = nullptr, *st_host = nullptr;
+    uint32_t blocks = 1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+namespace {
+
+void set_detail(fl_err_detail *d, int kind, const char *ctx, int64_t index = 0, int64_t expected = 0,
+                int64_t actual = 0, int64_t row = 0, int64_t column = 0) {
+    if (!d) return;
+    std::memset(d, 0, sizeof(*d));
+    d->kind = kind;
+    d->index = index;
+    d->expected = expected;
+    d->actual = actual;
+    d->row = row;
+    d->column = column;
+    std::snprintf(d->context, sizeof d->context, "%s", ctx ? ctx : "");
+}
+
+fl_status err(fl_ctx *ctx, fl_err_detail *d, fl_status st, const char *text, int64_t index = 0,
+              int64_t row = 0) {
+    set_detail(d, st, text, index, 0, 0, row);
+    return set_error(ctx, st, "%s (index %lld)", text, (long long)index);
+}
+
+void free_csr(fl_csr *m) {
+    void *bufs[] = {m->off, m->col, m->val, m->inv, m->partials, m->st};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (double *v : m->vec)
+        if (v) (void)hipFree(v);
+    if (m->st_host) (void)hipHostFree(m->st_host);
+    if (m->e0) (void)hipEventDestroy(m->e0);
+    if (m->e1) (void)hipEventDestroy(m->e1);
+}
+
+fl_status ensure_vectors(fl_csr *m, int count) {
+    for (int k = 0; k < count; ++k)
+        if (!m->vec[k]) FL_HIP_TRY(m->ctx, hipMalloc(&m->vec[k], (size_t)m->n * sizeof(double) + 8));
+    return FL_OK;
+}
+
+// validate_solve_inputs (solver.rs:258-300) on the host arrays.
+fl_status validate_inputs(fl_csr *m, const double *rhs, const double *x, const fl_lin_opts *o,
+                          fl_err_detail *d) {
+    if (m->n != m->ncols) {
+        set_detail(d, FL_ERR_DIMENSION_MISMATCH, "", 0, m->n, m->ncols);
+        return set_error(m->ctx, FL_ERR_DIMENSION_MISMATCH, "dimension mismatch: expected %u, got %u",
+                         m->n, m->ncols);
+    }
+    if (!std::isfinite(o->absolute_tolerance) || o->absolute_tolerance < 0.0 ||
+        !std::isfinite(o->relative_tolerance) || o->relative_tolerance < 0.0)
+        return err(m->ctx, d, FL_ERR_BREAKDOWN, "invalid solver tolerance");
+    for (uint32_t i = 0; i < m->n; ++i)
+        if (!std::isfinite(rhs[i])) return err(m->ctx, d, FL_ERR_NON_FINITE, "solver vector", i);
+    for (uint32_t i = 0; i < m->n; ++i)
+        if (!std::isfinite(x[i]))
+            return err(m->ctx, d, FL_ERR_NON_FINITE, "solver vector", (int64_t)m->n + i);
+    return FL_OK;
+}
+
+fl_status state_error(fl_csr *m, const LinState &h, fl_err_detail *d) {
+    const char *text = ctx_text(h.ctx);
+    return err(m->ctx, d, (fl_status)h.status, text, h.err_index);
+}
+
+enum Kind { KIND_CG = 0, KIND_PCG = 1, KIND_BICG = 2 };
+
+fl_status solve(fl_csr *m, int kind, const double *rhs, double *x_inout, const fl_lin_opts *opts,
+                fl_lin_report *rep, fl_err_detail *d) {
+    fl_ctx *ctx = m->ctx;
+    if (!rhs || !x_inout || !opts) return err(ctx, d, FL_ERR_INVALID_ARGUMENT, "null argument");
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    fl_status st = validate_inputs(m, rhs, x_inout, opts, d);
+    if (st != FL_OK) return st;
+    if (kind == KIND_PCG && m->inv_status != FL_OK) {
+        set_detail(d, m->inv_status, m->inv_ctx, m->inv_row, 0, 0, m->inv_row);
+        return set_error(ctx, (fl_status)m->inv_status, "Jacobi preconditioner: %s (row %lld)",
+                         m->inv_ctx, (long long)m->inv_row);
+    }
+    if ((st = ensure_vectors(m, kind == KIND_BICG ? 7 : 5)) != FL_OK) return st;
+    hipStream_t s = ctx->stream;
+    const size_t bytes = (size_t)m->n * sizeof(double);
+    double *x = m->vec[0], *b = m->vec[1], *r = m->vec[2], *dp = m->vec[3], *qv = m->vec[4];
+    double *tv = m->vec[5], *rhat = m->vec[6];
+    FL_HIP_TRY(ctx, hipMemcpyAsync(x, x_inout, bytes, hipMemcpyHostToDevice, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(b, rhs, bytes, hipMemcpyHostToDevice, s));
+    LinState init{};
+    init.threshold = opts->absolute_tolerance;
+    init.alpha = opts->relative_tolerance;  // consumed by k_cg_init_r
+    init.max_iter = opts->max_iterations;
+    init.bad_row = ~0ull;
+    init.bad_z = ~0ull;
+    *m->st_host = init;
+    FL_HIP_TRY(ctx, hipMemcpyAsync(m->st, m->st_host, sizeof(LinState), hipMemcpyHostToDevice, s));
+    const Mat A{m->off, m->col, m->val, m->n};
+    const double *inv = kind == KIND_PCG ? m->inv : nullptr;
+    const uint32_t G = m->blocks;
+    FL_HIP_TRY(ctx, hipEventRecord(m->e0, s));
+    k_spmv<<<G, kBlock, 0, s>>>(A, x, qv, m->st);
+    k_cg_init_r<<<G, kBlock, 0, s>>>(m->n, b, qv, r, m->partials, m->st);
+    if (kind == KIND_BICG)
+        k_bi_init<<<G, kBlock, 0, s>>>(m->n, r, rhat, dp, qv, m->partials, m->st);
+    else
+        k_cg_init_d<<<G, kBlock, 0, s>>>(m->n, r, inv, dp, m->partials, m->st);
+    FL_HIP_TRY(ctx, hipGetLastError());
+    int batch = 4;
+    for (;;) {
+        for (int it = 0; it < batch; ++it) {
+            if (kind == KIND_BICG) {
+                k_bi_p<<<G, kBlock, 0, s>>>(m->n, r, dp, qv, m->st);
+                k_bi_spmv_v<<<G, kBlock, 0, s>>>(A, dp, rhat, qv, m->partials, m->st);
+                k_bi_spmv_t<<<G, kBlock, 0, s>>>(A, r, qv, tv, m->partials, m->st);
+                k_bi_update<<<G, kBlock, 0, s>>>(m->n, x, r, dp, qv, tv, rhat, m->partials, m->st);
+            } else {
+                k_cg_spmv<<<G, kBlock, 0, s>>>(A, dp, qv, m->partials, m->st);
+                k_cg_update<<<G, kBlock, 0, s>>>(m->n, x, r, dp, qv, inv, m->partials, m->st);
+                k_cg_dir<<<G, kBlock, 0, s>>>(m->n, r, inv, dp, m->st);
+            }
+        }
+        FL_HIP_TRY(ctx, hipGetLastError());
+        FL_HIP_TRY(ctx, hipMemcpyAsync(m->st_host, m->st, sizeof(LinState), hipMemcpyDeviceToHost, s));
+        FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+        if (m->st_host->done) break;
+        batch = std::min(batch * 2, 64);
+    }
+    FL_HIP_TRY(ctx, hipEventRecord(m->e1, s));
+    const LinState h = *m->st_host;
+    if (h.status != FL_OK) {
+        FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+        return state_error(m, h, d);
+    }
+    FL_HIP_TRY(ctx, hipMemcpyAsync(x_inout, x, bytes, hipMemcpyDeviceToHost, s));
+    FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, m->e0, m->e1);
+    if (rep) {
+        rep->converged = h.converged;
+        rep->iterations = h.converged ? (h.initial <= h.threshold ? 0 : h.iter) : h.max_iter;
+        rep->initial_residual = h.initial;
+        rep->final_residual = h.rnorm;
+        rep->relative_residual = h.initial == 0.0 ? 0.0 : h.rnorm / h.initial;
+        rep->device_ms = ms;
+    }
+    if (d) set_detail(d, FL_OK, "");
+    return FL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+fl_status fl_csr_upload(fl_ctx *ctx, int64_t nrows, int64_t ncols, const int64_t *off,
+                        const int64_t *cols, const double *vals, int64_t nnz, fl_csr **out,
+                        fl_err_detail *d) {
+    if (!ctx || !out || !off || (nnz > 0 && (!cols || !vals)) || nrows < 0 || ncols < 0 || nnz < 0)
+        return err(ctx, d, FL_ERR_INVALID_ARGUMENT, "null or negative argument");
+    *out = nullptr;
+    // validate_csr (csr.rs:195-251), in the reference's order
+    if (off[0] != 0) {
+        set_detail(d, FL_ERR_INVALID_CSR, "first row offset must be zero");
+        return set_error(ctx, FL_ERR_INVALID_CSR, "invalid CSR matrix: first row offset must be zero");
+    }
+    if (off[nrows] != nnz) {
+        set_detail(d, FL_ERR_INVALID_CSR, "last row offset must equal value count");
+        return set_error(ctx, FL_ERR_INVALID_CSR,
+                         "invalid CSR matrix: last row offset must equal value count");
+    }
+    for (int64_t r = 0; r < nrows; ++r) {
+        if (off[r] > off[r + 1]) {
+            set_detail(d, FL_ERR_INVALID_CSR, "row offsets must be monotonic");
+            return set_error(ctx, FL_ERR_INVALID_CSR, "invalid CSR matrix: row offsets must be monotonic");
+        }
+        for (int64_t e = off[r] + 1; e < off[r + 1]; ++e)
+            if (cols[e - 1] >= cols[e]) {
+                set_detail(d, FL_ERR_INVALID_CSR, "row columns must be strictly ascending");
+                return set_error(ctx, FL_ERR_INVALID_CSR,
+                                 "invalid CSR matrix: row columns must be strictly ascending");
+            }
+    }
+    for (int64_t e = 0; e < nnz; ++e)
+        if (cols[e] < 0 || cols[e] >= ncols) {
+            set_detail(d, FL_ERR_INVALID_CSR, "column exceeds ncols", e);
+            return set_error(ctx, FL_ERR_INVALID_CSR,
+                             "invalid CSR matrix: column %lld at entry %lld exceeds ncols",
+                             (long long)cols[e], (long long)e);
+        }
+    for (int64_t e = 0; e < nnz; ++e)
+        if (!std::isfinite(vals[e])) return err(ctx, d, FL_ERR_NON_FINITE, "matrix", e);
+    if (nnz >= (int64_t)1 << 31 || nrows >= (int64_t)1 << 31 || ncols >= (int64_t)1 << 31)
+        return err(ctx, d, FL_ERR_INVALID_ARGUMENT, "matrix too large for 32-bit device indices");
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    fl_csr *m = new (std::nothrow) fl_csr();
+    if (!m) return err(ctx, d, FL_ERR_OUT_OF_MEMORY, "host allocation failed");
+    m->ctx = ctx;
+    m->n = (uint32_t)nrows;
+    m->ncols = (uint32_t)ncols;
+    m->nnz = nnz;
+    std::vector<int> ho(nrows + 1), hc(nnz > 0 ? nnz : 1);
+    for (int64_t r = 0; r <= nrows; ++r) ho[r] = (int)off[r];
+    for (int64_t e = 0; e < nnz; ++e) hc[e] = (int)cols[e];
+    // JacobiPreconditioner::new (preconditioner.rs:9-28): the first failing row wins.
+    std::vector<double> inv(nrows > 0 ? nrows : 1);
+    for (int64_t r = 0; r < nrows && m->inv_status == FL_OK; ++r) {
+        int64_t found = -1;
+        for (int64_t e = off[r]; e < off[r + 1]; ++e)
+            if (cols[e] == r) found = e;
+        if (found < 0) {
+            m->inv_status = FL_ERR_MISSING_DIAGONAL;
+            m->inv_ctx = "missing diagonal entry";
+            m->inv_row = r;
+        } else if (vals[found] == 0.0) {
+            m->inv_status = FL_ERR_ZERO_DIAGONAL;
+            m->inv_ctx = "zero diagonal entry";
+            m->inv_row = r;
+        } else {
+            inv[r] = 1.0 / vals[found];
+            if (!std::isfinite(inv[r])) {
+                m->inv_status = FL_ERR_NON_FINITE;
+                m->inv_ctx = "inverse diagonal";
+                m->inv_row = r;
+            }
+        }
+    }
+    const int64_t tiles = (nrows + kBlock - 1) / kBlock;
+    m->blocks = grid_blocks(ctx, tiles > 0 ? tiles : 1);
+    hipStream_t s = ctx->stream;
+    bool okay = hipMalloc(&m->off, (nrows + 1) * sizeof(int)) == hipSuccess &&
+                hipMalloc(&m->col, (nnz > 0 ? nnz : 1) * sizeof(int)) == hipSuccess &&
+                hipMalloc(&m->val, (nnz > 0 ? nnz : 1) * sizeof(double)) == hipSuccess &&
+                hipMalloc(&m->inv, (nrows > 0 ? nrows : 1) * sizeof(double)) == hipSuccess &&
+                hipMalloc(&m->partials, (size_t)m->blocks * 4 * sizeof(double)) == hipSuccess &&
+                hipMalloc(&m->st, sizeof(LinState)) == hipSuccess &&
+                hipHostMalloc(&m->st_host, sizeof(LinState), hipHostMallocDefault) == hipSuccess &&
+                hipEventCreate(&m->e0) == hipSuccess && hipEventCreate(&m->e1) == hipSuccess;
+    okay = okay &&
+           hipMemcpyAsync(m->off, ho.data(), (nrows + 1) * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
+           (nnz == 0 ||
+            (hipMemcpyAsync(m->col, hc.data(), nnz * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
+             hipMemcpyAsync(m->val, vals, nnz * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess)) &&
+           (nrows == 0 ||
+            hipMemcpyAsync(m->inv, inv.data(), nrows * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess) &&
+           hipMemsetAsync(m->st, 0, sizeof(LinState), s) == hipSuccess &&
+           hipStreamSynchronize(s) == hipSuccess;
+    if (!okay) {
+        free_csr(m);
+        delete m;
+        return err(ctx, d, FL_ERR_OUT_OF_MEMORY, "device allocation or upload failed");
+    }
+    ctx_retain(ctx);
+    *out = m;
+    if (d) set_detail(d, FL_OK, "");
+    return FL_OK;
+}
+
+void fl_csr_destroy(fl_csr *m) {
+    if (!m) return;
+    fl_ctx *ctx = m->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_csr(m);
+    delete m;
+    ctx_release(ctx);
+}
+
+fl_status fl_spmv(fl_csr *m, const double *x, double *y, fl_err_detail *d) {
+    if (!m || !x || !y) return err(m ? m->ctx : nullptr, d, FL_ERR_INVALID_ARGUMENT, "null argument");
+    fl_ctx *ctx = m->ctx;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    fl_status st = ensure_vectors(m, 2);
+    if (st != FL_OK) return st;
+    hipStream_t s = ctx->stream;
+    FL_HIP_TRY(ctx, hipMemcpyAsync(m->vec[0], x, (size_t)m->ncols * sizeof(double),
+                                   hipMemcpyHostToDevice, s));
+    LinState init{};
+    init.bad_row = ~0ull;
+    *m->st_host = init;
+    FL_HIP_TRY(ctx, hipMemcpyAsync(m->st, m->st_host, sizeof(LinState), hipMemcpyHostToDevice, s));
+    if (m->ncols > m->n) {
+        // x is longer than the vector buffers when ncols > nrows
+        double *big = nullptr;
+        FL_HIP_TRY(ctx, hipMalloc(&big, (size_t)m->ncols * sizeof(double)));
+        FL_HIP_TRY(ctx, hipMemcpyAsync(big, x, (size_t)m->ncols * sizeof(double), hipMemcpyHostToDevice, s));
+        k_spmv<<<m->blocks, kBlock, 0, s>>>(Mat{m->off, m->col, m->val, m->n}, big, m->vec[1], m->st);
+        FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+        (void)hipFree(big);
+    } else {
+        k_spmv<<<m->blocks, kBlock, 0, s>>>(Mat{m->off, m->col, m->val, m->n}, m->vec[0], m->vec[1], m->st);
+    }
+    FL_HIP_TRY(ctx, hipGetLastError());
+    FL_HIP_TRY(ctx, hipMemcpyAsync(y, m->vec[1], (size_t)m->n * sizeof(double), hipMemcpyDeviceToHost, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(m->st_host, m->st, sizeof(LinState), hipMemcpyDeviceToHost, s));
+    FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (m->st_host->bad_row != ~0ull)
+        return err(ctx, d, FL_ERR_NON_FINITE, "SpMV output", (int64_t)m->st_host->bad_row);
+    if (d) set_detail(d, FL_OK, "");
+    return FL_OK;
+}
+
+fl_status fl_cg(fl_csr *m, const double *rhs, double *x, const fl_lin_opts *o, fl_lin_report *rep,
+                fl_err_detail *d) {
+    if (!m) return FL_ERR_INVALID_ARGUMENT;
+    return solve(m, KIND_CG, rhs, x, o, rep, d);
+}
+
+fl_status fl_pcg_jacobi(fl_csr *m, const double *rhs, double *x, const fl_lin_opts *o,
+                        fl_lin_report *rep, fl_err_detail *d) {
+    if (!m) return FL_ERR_INVALID_ARGUMENT;
+    return solve(m, KIND_PCG, rhs, x, o, rep, d);
+}
+
+fl_status fl_bicgstab(fl_csr *m, const double *rhs, double *x, const fl_lin_opts *o,
+                      fl_lin_report *rep, fl_err_detail *d) {
+    if (!m) return FL_ERR_INVALID_ARGUMENT;
+    return solve(m, KIND_BICG, rhs, x, o, rep, d);
+}
+
+}  // extern "C"

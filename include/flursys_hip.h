@@ -60,6 +60,8 @@ typedef struct fl_ctx fl_ctx;
 
 /* Creates a context bound to HIP device `device` with its own stream. */
 fl_status fl_create(int device, fl_ctx **out);
+/* Releases the context.  Solvers created from it keep it alive: the context
+ * is freed when the last of them is destroyed. */
 void fl_destroy(fl_ctx *ctx);
 /* Text of the last error raised on this context ("" if none). */
 const char *fl_last_error(const fl_ctx *ctx);
@@ -97,6 +99,9 @@ typedef struct {
     int32_t initial_guess;         /* FL_GUESS_* */
     int32_t precision;             /* FL_PRECISION_* */
     int32_t reserved;
+    /* FL_PRECISION_MIXED only: relative tolerance of each inner FP32 CG
+     * solve (0 selects 1e-5).  pressure_max_iters caps the inner total. */
+    double inner_tolerance;
 } fl_cavity3d_config;
 
 typedef struct {
@@ -145,6 +150,69 @@ fl_status fl_cavity3d_set_timing(fl_cavity3d *solver, int32_t enable);
 fl_status fl_cavity3d_kernel_times(fl_cavity3d *solver, double *ms_per_iteration,
                                    double *ms_apply, double *ms_update, double *ms_direction,
                                    int64_t *timed_iterations);
+
+/* ---------------------------------------------------------------------- */
+/* CSR linear algebra: replaces CsrMatrix (src/linear/csr.rs:4-125) and the
+ * free functions cg / pcg / bicgstab (src/linear/solver.rs:43-256).  Values
+ * are FP64; on the device the row offsets and columns are stored as int32
+ * (nnz and ncols must fit in 31 bits). */
+typedef struct fl_csr fl_csr;
+
+/* LinearSolverOptions (src/linear/solver.rs:5-20; defaults atol 1e-10,
+ * rtol 1e-8, max_iterations 1000). */
+typedef struct {
+    double absolute_tolerance;
+    double relative_tolerance;
+    int64_t max_iterations;
+} fl_lin_opts;
+
+/* LinearSolveReport (src/linear/solver.rs:29-41): converged = 1 for
+ * LinearSolverStatus::Converged, 0 for MaxIterations (a status, not an
+ * error).  device_ms is the solve's device time (HIP events, no copies). */
+typedef struct {
+    int32_t converged;
+    int32_t reserved;
+    int64_t iterations;
+    double initial_residual;
+    double final_residual;
+    double relative_residual;
+    double device_ms;
+} fl_lin_report;
+
+/* The fields of LinearAlgebraError (src/linear/mod.rs:17-26) for the status
+ * returned: `context` holds the reference's &'static str (NonFiniteValue
+ * context or Breakdown reason) or the InvalidCsr message. */
+typedef struct {
+    int32_t kind;          /* the fl_status */
+    int32_t reserved;
+    int64_t index;         /* NonFiniteValue index, InvalidCsr entry */
+    int64_t expected;      /* DimensionMismatch */
+    int64_t actual;
+    int64_t row;           /* MissingDiagonal / ZeroDiagonal / IndexOutOfBounds */
+    int64_t column;
+    char context[96];
+} fl_err_detail;
+
+/* CsrMatrix::new (src/linear/csr.rs:14-29): validates exactly like
+ * validate_csr (:195-251) and uploads.  `detail` may be NULL. */
+fl_status fl_csr_upload(fl_ctx *ctx, int64_t nrows, int64_t ncols, const int64_t *row_offsets,
+                        const int64_t *column_indices, const double *values, int64_t nnz,
+                        fl_csr **out, fl_err_detail *detail);
+void fl_csr_destroy(fl_csr *matrix);
+/* spmv_into (src/linear/csr.rs:89-106): y = A x; a non-finite output row is
+ * NonFiniteValue{"SpMV output", row}. */
+fl_status fl_spmv(fl_csr *matrix, const double *x, double *y, fl_err_detail *detail);
+/* cg (src/linear/solver.rs:43-50).  x_inout is the initial guess (honoured)
+ * and receives the solution. */
+fl_status fl_cg(fl_csr *matrix, const double *rhs, double *x_inout, const fl_lin_opts *opts,
+                fl_lin_report *report, fl_err_detail *detail);
+/* pcg with JacobiPreconditioner::new (src/linear/solver.rs:52-60,
+ * src/linear/preconditioner.rs:9-28). */
+fl_status fl_pcg_jacobi(fl_csr *matrix, const double *rhs, double *x_inout,
+                        const fl_lin_opts *opts, fl_lin_report *report, fl_err_detail *detail);
+/* bicgstab (src/linear/solver.rs:155-256). */
+fl_status fl_bicgstab(fl_csr *matrix, const double *rhs, double *x_inout,
+                      const fl_lin_opts *opts, fl_lin_report *report, fl_err_detail *detail);
 
 #ifdef __cplusplus
 }

@@ -89,8 +89,12 @@ def test_steps_match_oracle(gpu_ctx, shape, re, stop, tol, guess):
         assert rep.pressure_converged
         assert abs(rep.pressure_iterations - orep.iterations) <= 1
     f = s.fields()
-    for name in ("u", "v", "w", "p"):
-        assert rel(f[name], o.field(name)) < 1e-9, name
+    # velocity components share one scale (w is ~1e-10 of u in the first
+    # steps, so a per-component relative bar would measure round-off of 0)
+    uscale = max(np.abs(o.field(c)).max() for c in ("u", "v", "w"))
+    for name in ("u", "v", "w"):
+        assert np.abs(f[name] - o.field(name)).max() < 1e-9 * uscale, name
+    assert rel(f["p"], o.field("p")) < 1e-9
     inter = s.intermediate()
     assert rel(inter["us"], o.field("us")) < 1e-12
     assert rel(inter["rhs"], o.field("rhs")) < 1e-11
@@ -116,8 +120,10 @@ def test_gpu_matches_reference_sor_solution(gpu_ctx):
         o.step()
         s.advance_one_step()
     f = s.fields()
-    for name in ("u", "v", "w", "p"):
-        assert rel(f[name], o.field(name)) < 1e-8, name
+    uscale = max(np.abs(o.field(c)).max() for c in ("u", "v", "w"))
+    for name in ("u", "v", "w"):
+        assert np.abs(f[name] - o.field(name)).max() < 1e-8 * uscale, name
+    assert rel(f["p"], o.field("p")) < 1e-8
 
 
 def test_repeatable_bitwise(gpu_ctx):
@@ -179,3 +185,52 @@ def test_large_grid_properties(gpu_ctx):
     r = b - o.apply_operator(p.ravel())
     assert np.linalg.norm(r) <= 1.5e-8 * np.linalg.norm(b)
     assert p[0, 0, 0] == 0.0
+
+
+# ----------------------------------------------------------- mixed precision
+# FP32 stencil CG inside FP64 iterative refinement.  Stated tolerance: the
+# outer FP64 stop test is the same rel-res 1e-8, so the pressure agrees with
+# the FP64 solution to ~cond(A) * 1e-8 relative; the bars below are 1e-6 for
+# p and 1e-7 of the velocity scale for u, v, w after 6 steps.
+def test_mixed_pressure_solve_meets_fp64_tolerance(gpu_ctx):
+    n = 32
+    rng = np.random.default_rng(5)
+    rhs = rng.standard_normal((n, n, n)) * 100.0
+    rhs[0, 0, 0] = 0.0
+    out = {}
+    for prec in (fa.PRECISION_FP64, fa.PRECISION_MIXED):
+        s = make(gpu_ctx, n, n, n, pressure_tolerance=1e-8, stop_rule=fa.STOP_REL_L2,
+                 initial_guess=fa.GUESS_ZERO, pressure_max_iters=20000, precision=prec)
+        p = np.zeros_like(rhs)
+        rep = s.pressure_solve(rhs, p)
+        assert rep.converged
+        out[prec] = (p, rep)
+    p64, r64 = out[fa.PRECISION_FP64]
+    p32, r32 = out[fa.PRECISION_MIXED]
+    assert r32.refinements >= 1
+    assert r32.residual_l2 <= 1e-8 * r32.initial_l2
+    o = O.Cavity3D(n, n, n)
+    b = -rhs.ravel()
+    b[0] = 0.0
+    true = np.linalg.norm(b - o.apply_operator(p32.ravel())) / np.linalg.norm(b)
+    assert true <= 1.5e-8
+    assert rel(p32, p64) < 1e-6
+    assert p32[0, 0, 0] == 0.0
+
+
+def test_mixed_steps_match_oracle(gpu_ctx):
+    nx, ny, nz = 16, 14, 12
+    kw = dict(reynolds=400.0, dt=5e-4, pressure_tolerance=1e-8, stop_rule=fa.STOP_REL_L2,
+              initial_guess=fa.GUESS_ZERO, pressure_max_iters=5000)
+    o = O.Cavity3D(nx, ny, nz, **kw)
+    s = make(gpu_ctx, nx, ny, nz, precision=fa.PRECISION_MIXED, **kw)
+    for _ in range(6):
+        o.step()
+        rep = s.advance_one_step()
+        assert rep.pressure_converged and rep.pressure_refinements >= 1
+    f = s.fields()
+    uscale = max(np.abs(o.field(c)).max() for c in ("u", "v", "w"))
+    for name in ("u", "v", "w"):
+        assert np.abs(f[name] - o.field(name)).max() < 1e-7 * uscale, name
+    assert rel(f["p"], o.field("p")) < 1e-6
+    assert rep.max_divergence < 1e-5

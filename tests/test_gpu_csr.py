@@ -1,0 +1,205 @@
+"""GPU parity of the CSR path (CsrMatrix, cg, pcg, bicgstab) through the C ABI.
+
+* the reference's hand-calculated tests (src/linear/mod.rs:97-285), from
+  tests/golden/csr_known_answers.json: exact on the GPU;
+* error kinds and static texts of LinearAlgebraError;
+* the jittered-triangulation Poisson systems: iteration counts within 1 of
+  the oracle and solutions within 1e-9 of max|x| (FMA contraction and the
+  reduction tree differ) at 20x20 and 100x100 squares;
+* at the full config-3 size (1000 x 1000 squares, 2 M cells): the true
+  residual of the returned solution, recomputed with scipy, meets the
+  tolerance (size-independent property).
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import flursys_amd.linear as FL
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def kat():
+    with open(os.path.join(GOLD, "csr_known_answers.json")) as f:
+        return json.load(f)
+
+
+def upload(ctx, A):
+    return FL.CsrMatrix(A.nrows, A.ncols, A.offsets, A.cols, A.vals, ctx)
+
+
+def opts(k):
+    o = k["options"]
+    return FL.LinearSolverOptions(o["atol"], o["rtol"], o["max_iter"])
+
+
+def test_spmv_known_answers(gpu_ctx, kat):
+    k = kat["duplicates"]
+    A = upload(gpu_ctx, O.Csr.from_triplets(2, 2, k["rows"], k["cols"], k["vals"]))
+    for x, y in k["spmv"]:
+        assert A.spmv(x).tolist() == y
+
+
+def test_malformed_csr_rejected(gpu_ctx):
+    with pytest.raises(FL.LinearAlgebraError) as e:
+        FL.CsrMatrix(1, 1, [1, 1], [], [], gpu_ctx)
+    assert e.value.kind == "InvalidCsr"
+    with pytest.raises(FL.LinearAlgebraError) as e:
+        FL.CsrMatrix(1, 1, [0, 1], [1], [1.0], gpu_ctx)
+    assert e.value.kind == "InvalidCsr" and "exceeds ncols" in e.value.message
+    with pytest.raises(FL.LinearAlgebraError) as e:
+        FL.CsrMatrix(1, 1, [0, 1], [0], [np.nan], gpu_ctx)
+    assert e.value.kind == "NonFiniteValue" and e.value.context == "matrix"
+    with pytest.raises(FL.LinearAlgebraError) as e:
+        FL.CsrMatrix(2, 2, [0, 2, 2], [1, 0], [1.0, 1.0], gpu_ctx)
+    assert "strictly ascending" in e.value.context
+    A = upload(gpu_ctx, O.Csr.from_dense([[1.0, 0.0], [0.0, 1.0]]))
+    with pytest.raises(FL.LinearAlgebraError) as e:
+        A.spmv([1.0])
+    assert e.value.kind == "DimensionMismatch"
+
+
+def test_jacobi_errors(gpu_ctx):
+    missing = upload(gpu_ctx, O.Csr.from_dense([[0.0, 1.0], [1.0, 1.0]]))
+    with pytest.raises(FL.LinearAlgebraError) as e:
+        FL.pcg(missing, [1.0, 1.0], np.zeros(2))
+    assert e.value.kind == "MissingDiagonal" and e.value.row == 0
+    zero = FL.CsrMatrix(2, 2, [0, 2, 4], [0, 1, 0, 1], [0.0, 1.0, 1.0, 2.0], gpu_ctx)
+    with pytest.raises(FL.LinearAlgebraError) as e:
+        FL.pcg(zero, [1.0, 1.0], np.zeros(2))
+    assert e.value.kind == "ZeroDiagonal"
+
+
+def test_known_solutions(gpu_ctx, kat):
+    o = opts(kat)
+    a = np.array(kat["spd"]["a"])
+    ex = np.array(kat["spd"]["x"])
+    A = upload(gpu_ctx, O.Csr.from_dense(a))
+    x = np.zeros(2)
+    rc = FL.cg(A, a @ ex, x, o)
+    assert rc.converged() and np.allclose(x, ex, atol=1e-10, rtol=0)
+    assert rc.final_residual <= o.absolute_tolerance + o.relative_tolerance * rc.initial_residual
+    x = np.zeros(2)
+    rp = FL.pcg(A, a @ ex, x, o)
+    assert rp.converged() and rp.iterations <= rc.iterations
+    assert np.allclose(x, ex, atol=1e-10, rtol=0)
+    a = np.array(kat["nonsymmetric"]["a"])
+    ex = np.array(kat["nonsymmetric"]["x"])
+    x = np.zeros(3)
+    rb = FL.bicgstab(upload(gpu_ctx, O.Csr.from_dense(a)), a @ ex, x, o)
+    assert rb.converged() and np.allclose(x, ex, atol=1e-10, rtol=0)
+
+
+def test_edge_cases(gpu_ctx, kat):
+    o = opts(kat)
+    n = kat["poisson1d"]["n"]
+    rows, cols, vals = [], [], []
+    for r in range(n):
+        rows.append(r); cols.append(r); vals.append(2.0)
+        if r > 0:
+            rows.append(r); cols.append(r - 1); vals.append(-1.0)
+        if r + 1 < n:
+            rows.append(r); cols.append(r + 1); vals.append(-1.0)
+    A = upload(gpu_ctx, O.Csr.from_triplets(n, n, rows, cols, vals))
+    ex = (np.arange(n) + 1.0) / n
+    rhs = A.spmv(ex)
+    x = np.zeros(n)
+    assert FL.cg(A, rhs, x, o).converged() and np.allclose(x, ex, atol=1e-10, rtol=0)
+    assert FL.bicgstab(A, np.zeros(n), x.copy(), o).iterations > 0  # guess honoured
+    assert FL.cg(A, rhs, ex.copy(), o).iterations == 0
+    r = FL.bicgstab(A, np.zeros(n), np.zeros(n), o)
+    assert r.converged() and r.iterations == 0
+    r = FL.cg(A, rhs, np.zeros(n), FL.LinearSolverOptions(o.absolute_tolerance,
+                                                          o.relative_tolerance, 1))
+    assert r.status == "MaxIterations" and r.iterations == 1
+    with pytest.raises(FL.LinearAlgebraError) as e:
+        FL.bicgstab(upload(gpu_ctx, O.Csr.from_dense([[0.0]])), [1.0], np.zeros(1), o)
+    assert e.value.kind == "Breakdown" and e.value.context == "zero BiCGSTAB alpha denominator"
+    with pytest.raises(FL.LinearAlgebraError) as e:
+        FL.cg(A, np.full(n, np.inf), np.zeros(n), o)
+    assert e.value.kind == "NonFiniteValue" and e.value.context == "solver vector"
+
+
+def test_oracle_agrees_on_breakdown_text():
+    """The oracle reports the same Breakdown reason for [[0]] (solver.rs:186-191)."""
+    with pytest.raises(O.LinearAlgebraError) as e:
+        O.bicgstab(O.Csr.from_dense([[0.0]]), np.array([1.0]), np.zeros(1), 1e-12, 1e-12, 100)
+    assert e.value.reason == "zero BiCGSTAB alpha denominator"
+
+
+def skew(mesh, eps_seed=0, amplitude=0.1):
+    """A + skew perturbation: per internal face A[P,N] += e a_f, A[N,P] -= e a_f,
+    e ~ U(-0.1, 0.1) from a seeded stream; faces touching the pinned cell keep
+    their (eliminated) entries out."""
+    n = len(mesh["volumes"])
+    A = sp.csr_matrix((mesh["vals"].copy(), mesh["cols"], mesh["offsets"]), shape=(n, n))
+    rng = np.random.default_rng(eps_seed)
+    eps = rng.uniform(-amplitude, amplitude, len(mesh["face_coef"]))
+    P, N, a = mesh["face_owner"], mesh["face_neigh"], mesh["face_coef"]
+    keep = (P != 0) & (N != 0)
+    D = sp.csr_matrix(((eps * a)[keep], (P[keep], N[keep])), shape=(n, n))
+    B = (A + D - D.T).tocsr()
+    B.sort_indices()
+    return B
+
+
+def mesh_system(m, jitter=0.25):
+    mesh = O.mesh_poisson(m, jitter, 0, 0)
+    n = len(mesh["volumes"])
+    src = np.random.default_rng(0).standard_normal(n)
+    src -= (src * mesh["volumes"]).sum() / mesh["volumes"].sum()
+    rhs = src * mesh["volumes"]
+    rhs[0] = 0.0
+    return mesh, rhs
+
+
+@pytest.mark.parametrize("m", [20, 100])
+@pytest.mark.parametrize("kind", ["cg", "pcg", "bicgstab"])
+def test_mesh_systems_match_oracle(gpu_ctx, m, kind):
+    print(f"[gpu] mesh {m}x{m} {kind}", flush=True)
+    mesh, rhs = mesh_system(m)
+    n = len(rhs)
+    if kind == "bicgstab":
+        B = skew(mesh)
+        Ao = O.Csr(n, n, B.indptr.astype(np.int64), B.indices.astype(np.int64), B.data)
+    else:
+        Ao = mesh["A"]
+    x_o = np.zeros(n)
+    ro = getattr(O, kind)(Ao, rhs, x_o, 0.0, 1e-10, 20000)
+    A = upload(gpu_ctx, Ao)
+    x = np.zeros(n)
+    rg = getattr(FL, kind)(A, rhs, x, FL.LinearSolverOptions(0.0, 1e-10, 20000))
+    assert rg.converged() and ro.converged
+    assert abs(rg.iterations - ro.iterations) <= 1
+    assert np.abs(x - x_o).max() <= 1e-8 * np.abs(x_o).max()
+    assert abs(rg.initial_residual - ro.initial_residual) <= 1e-12 * ro.initial_residual
+
+
+def test_full_size_mesh_properties(gpu_ctx):
+    """Config-3 size: 1000 x 1000 squares, 2 M triangles, rel-res 1e-10."""
+    print("[gpu] 2M-cell mesh CG / PCG / BiCGSTAB", flush=True)
+    mesh, rhs = mesh_system(1000)
+    n = len(rhs)
+    A_s = sp.csr_matrix((mesh["vals"], mesh["cols"], mesh["offsets"]), shape=(n, n))
+    A = upload(gpu_ctx, mesh["A"])
+    o = FL.LinearSolverOptions(0.0, 1e-10, 50000)
+    for fn in (FL.cg, FL.pcg):
+        x = np.zeros(n)
+        r = fn(A, rhs, x, o)
+        assert r.converged()
+        true = np.linalg.norm(rhs - A_s @ x) / np.linalg.norm(rhs)
+        assert true <= 2e-10, (fn.__name__, true)
+        assert x[0] == 0.0 or abs(x[0]) < 1e-12
+    B = skew(mesh)
+    Ab = FL.CsrMatrix(n, n, B.indptr, B.indices, B.data, gpu_ctx)
+    x = np.zeros(n)
+    r = FL.bicgstab(Ab, rhs, x, o)
+    assert r.converged()
+    assert np.linalg.norm(rhs - B @ x) / np.linalg.norm(rhs) <= 2e-10
