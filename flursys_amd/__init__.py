@@ -6,12 +6,13 @@ the reference's solver interfaces over that ABI.
 """
 
 from ._lib import (GUESS_PREVIOUS, GUESS_ZERO, PRECISION_FP64, PRECISION_MIXED,
-                   STOP_ABS_INF, STOP_REL_L2, Context, FlursysError, default_context, load)
+                   PRECOND_JACOBI, PRECOND_MULTIGRID, STOP_ABS_INF, STOP_REL_L2, Context,
+                   FlursysError, default_context, load)
 from .cavity3d import LidDrivenCavity3DConfig, LidDrivenCavity3DSolver, RunSummary3D
 
 __all__ = [
     "Context", "FlursysError", "default_context", "load",
     "LidDrivenCavity3DConfig", "LidDrivenCavity3DSolver", "RunSummary3D",
     "STOP_ABS_INF", "STOP_REL_L2", "GUESS_PREVIOUS", "GUESS_ZERO",
-    "PRECISION_FP64", "PRECISION_MIXED",
+    "PRECISION_FP64", "PRECISION_MIXED", "PRECOND_JACOBI", "PRECOND_MULTIGRID",
 ]

@@ -24,6 +24,7 @@ STATUS_NAMES = {
 STOP_ABS_INF, STOP_REL_L2 = 0, 1
 GUESS_PREVIOUS, GUESS_ZERO = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
 
 
 class FlursysError(RuntimeError):
@@ -39,6 +40,7 @@ class FlursysError(RuntimeError):
 class SolveReport(C.Structure):
     _fields_ = [
         ("iterations", C.c_int64), ("refinements", C.c_int64),
+        ("fallback_iterations", C.c_int64), ("fallback", C.c_int32), ("reserved", C.c_int32),
         ("residual_abs_inf", C.c_double), ("residual_l2", C.c_double),
         ("initial_l2", C.c_double), ("converged", C.c_int32), ("status", C.c_int32),
     ]
@@ -52,7 +54,7 @@ class Cavity3DConfig(C.Structure):
         ("dt", C.c_double),
         ("pressure_max_iters", C.c_int64), ("pressure_tolerance", C.c_double),
         ("stop_rule", C.c_int32), ("initial_guess", C.c_int32), ("precision", C.c_int32),
-        ("reserved", C.c_int32), ("inner_tolerance", C.c_double),
+        ("preconditioner", C.c_int32), ("inner_tolerance", C.c_double),
     ]
 
 
@@ -62,7 +64,8 @@ class Cavity3DStepReport(C.Structure):
         ("pressure_iterations", C.c_int64), ("pressure_refinements", C.c_int64),
         ("pressure_residual", C.c_double), ("pressure_residual_l2", C.c_double),
         ("max_divergence", C.c_double), ("divergence_l2", C.c_double),
-        ("max_speed", C.c_double), ("pressure_converged", C.c_int32), ("reserved", C.c_int32),
+        ("max_speed", C.c_double), ("pressure_converged", C.c_int32),
+        ("pressure_fallback", C.c_int32),
     ]
 
 
@@ -83,9 +86,34 @@ class ErrDetail(C.Structure):
                 ("column", C.c_int64), ("context", C.c_char * 96)]
 
 
+class Grid2DConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int64), ("ny", C.c_int64), ("length", C.c_double), ("height", C.c_double),
+        ("density", C.c_double), ("kinematic_viscosity", C.c_double), ("dt", C.c_double),
+        ("t_end", C.c_double), ("pressure_max_iters", C.c_int64),
+        ("pressure_tolerance", C.c_double), ("convection", C.c_int32),
+        ("stop_rule", C.c_int32), ("initial_guess", C.c_int32), ("adaptive", C.c_int32),
+        ("bc_kind", C.c_int32 * 4), ("bc_u", C.c_double * 4), ("bc_v", C.c_double * 4),
+        ("outlet_pressure", C.c_double * 4), ("min_dt", C.c_double), ("max_dt", C.c_double),
+        ("target_cfl", C.c_double), ("target_diffusion_number", C.c_double),
+        ("max_growth_factor", C.c_double),
+    ]
+
+
+class Grid2DStepReport(C.Structure):
+    _fields_ = [
+        ("iteration", C.c_int64), ("time", C.c_double), ("time_step", C.c_double),
+        ("continuity_residual", C.c_double), ("pressure_residual", C.c_double),
+        ("pressure_iterations", C.c_int64), ("momentum_residual", C.c_double),
+        ("max_speed", C.c_double), ("momentum_cfl", C.c_double),
+        ("viscous_diffusion_number", C.c_double), ("pressure_converged", C.c_int32),
+        ("reached_end_time", C.c_int32),
+    ]
+
+
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
-_LIN = [_P, _P, _P, C.POINTER(LinOpts), C.POINTER(LinReport), C.POINTER(ErrDetail)]
+_LIN =[_P, _P, _P, C.POINTER(LinOpts), C.POINTER(LinReport), C.POINTER(ErrDetail)]
 
 # name -> (restype, argtypes); every symbol include/flursys_hip.h declares.
 SIGNATURES = {
@@ -104,7 +132,18 @@ SIGNATURES = {
     "fl_cavity3d_get_intermediate": (C.c_int, [_P, _P, _P, _P, _P]),
     "fl_cavity3d_set_timing": (C.c_int, [_P, C.c_int32]),
     "fl_cavity3d_kernel_times": (C.c_int, [_P, _D, _D, _D, _D, C.POINTER(C.c_int64)]),
-    "fl_csr_upload": (C.c_int, [_P, C.c_int64, C.c_int64, _P, _P, _P, C.c_int64,
+    "fl_grid2d_create": (C.c_int, [_P, C.POINTER(Grid2DConfig), _P, C.POINTER(_P)]),
+    "fl_grid2d_destroy": (None, [_P]),
+    "fl_grid2d_step": (C.c_int, [_P, C.POINTER(Grid2DStepReport)]),
+    "fl_grid2d_pressure_solve": (C.c_int, [_P, _P, _P, C.POINTER(SolveReport)]),
+    "fl_grid2d_get_fields": (C.c_int, [_P, _P, _P, _P]),
+    "fl_grid2d_set_fields": (C.c_int, [_P, _P, _P, _P]),
+    "fl_grid2d_cell_fields": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fl_batch2d_create": (C.c_int, [_P, C.c_int32, C.POINTER(Grid2DConfig), C.POINTER(_P)]),
+    "fl_batch2d_destroy": (None, [_P]),
+    "fl_batch2d_step": (C.c_int, [_P, C.POINTER(Grid2DStepReport)]),
+    "fl_batch2d_get_fields": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
+    "fl_csr_upload": (C.c_int,[_P, C.c_int64, C.c_int64, _P, _P, _P, C.c_int64,
                                 C.POINTER(_P), C.POINTER(ErrDetail)]),
     "fl_csr_destroy": (None, [_P]),
     "fl_spmv": (C.c_int, [_P, _P, _P, C.POINTER(ErrDetail)]),

@@ -15,7 +15,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import (GUESS_PREVIOUS, PRECISION_FP64, STOP_ABS_INF, Cavity3DConfig,
+from ._lib import (GUESS_PREVIOUS, PRECISION_FP64, PRECOND_JACOBI, STOP_ABS_INF, Cavity3DConfig,
                    Cavity3DStepReport, SolveReport, check, load, ptr)
 
 
@@ -39,14 +39,15 @@ class LidDrivenCavity3DConfig:
     stop_rule: int = STOP_ABS_INF
     initial_guess: int = GUESS_PREVIOUS
     precision: int = PRECISION_FP64
+    preconditioner: int = PRECOND_JACOBI
     inner_tolerance: float = 0.0
 
     def to_c(self) -> Cavity3DConfig:
         return Cavity3DConfig(self.nx, self.ny, self.nz, self.length, self.height, self.depth,
                               self.density, self.lid_velocity, self.reynolds, self.dt,
                               self.pressure_max_iters, self.pressure_tolerance, self.stop_rule,
-                              self.initial_guess, self.precision, 0,
-                              self.inner_tolerance)
+                              self.initial_guess, self.precision,
+                              self.preconditioner, self.inner_tolerance)
 
     def viscosity(self) -> float:
         """viscosity (incompressible_3d.rs:86-88)."""

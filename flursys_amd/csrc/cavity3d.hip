@@ -105,6 +105,7 @@ struct CgParams {
     double tol;
     long long max_iter;
     int stop_rule;
+    double rr0_ref;  // > 0: the rel-L2 test is against this ||r0||^2
 };
 enum { ST_OK = 0, ST_SINGULAR = 1, ST_ALPHA = 2, ST_RESIDUAL = 3 };
 
@@ -116,7 +117,7 @@ template <typename TR>
 __global__ void __launch_bounds__(kBlock) k_residual(Grid g, const double *__restrict__ rhs,
                                                      const double *__restrict__ x,
                                                      TR *__restrict__ r, double *partials) {
-    double s[2] = {0.0, 0.0}, m[1] = {0.0};
+    double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
     TileSched ts((g.n + kBlock - 1) / kBlock);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const uint32_t c = t * kBlock + threadIdx.x;
@@ -138,9 +139,10 @@ __global__ void __launch_bounds__(kBlock) k_residual(Grid g, const double *__res
         const double rd = (double)rc;
         s[0] += rd * (rd * inv);
         s[1] += rd * rd;
+        s[2] += rd;
         m[0] = fmax(m[0], fabs(rd));
     }
-    publish_partials<2, 1>(s, m, partials);
+    publish_partials<3, 1>(s, m, partials);
 }
 
 // d' = z + beta d (z = D^-1 r), q = A d', partial d'.q.  Folds the update
@@ -152,11 +154,11 @@ __global__ void __launch_bounds__(kBlock) k_cg_dirapply(
     T *__restrict__ dnew, T *__restrict__ q, const double *__restrict__ pin,
     double *__restrict__ pout, CgStateA *SA, const CgStateB *SB) {
     if (SA->done || SB->done) return;
-    double f[2], fm[1];
-    fold_partials<2, 1>(pin, gridDim.x, f, fm);
+    double f[3], fm[1];
+    fold_partials<3, 1>(pin, gridDim.x, f, fm);
     const double rz = f[0], rr = f[1], rmax = fm[0];
     const bool first = SB->first != 0;
-    const double rr0 = first ? rr : SA->rr0;
+    const double rr0 = prm.rr0_ref > 0.0 ? prm.rr0_ref : (first ? rr : SA->rr0);
     const long long iters = SB->iters;
     int done = 0, converged = 0, status = ST_OK;
     double beta = 0.0;
@@ -269,7 +271,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x
     }
     if (status != ST_OK) return;
     const T ta = (T)alpha;
-    double s[2] = {0.0, 0.0}, m[1] = {0.0};
+    double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
     TileSched ts((g.n + kBlock - 1) / kBlock);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const uint32_t c = t * kBlock + threadIdx.x;
@@ -288,9 +290,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x
         const double rd = (double)rc;
         s[0] += rd * (double)z;
         s[1] += rd * rd;
+        s[2] += rd;
         m[0] = fmax(m[0], fabs(rd));
     }
-    publish_partials<2, 1>(s, m, pout);
+    publish_partials<3, 1>(s, m, pout);
 }
 
 // x += e (FP64 += FP32): the refinement step of the mixed path.
@@ -298,6 +301,377 @@ __global__ void __launch_bounds__(kBlock) k_add_correction(uint32_t n, double *_
                                                            const float *__restrict__ e) {
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
         x[c] += (double)e[c];
+}
+
+// ---------------------------------------------------------------------------
+// Multigrid-preconditioned flexible CG (FP32 vectors, FP64 dots and scalars).
+//
+// The preconditioner is one V-cycle of cell-centred geometric multigrid for
+// the Neumann operator: 3 damped-Jacobi sweeps (omega = 6/7) before and after
+// the coarse correction, restriction = mean of the 8 children, prolongation =
+// piecewise constant scaled by 1.8, Galerkin-consistent coarse coefficients
+// (each axis coefficient halves per level).  Levels down to <= 4096 cells run
+// in one single-workgroup kernel out of LDS.  The pinned system is handed to
+// the Neumann cycle in its exact equivalent form: r_0 is replaced by
+// -sum_{k != 0} r_k (which makes the right-hand side compatible) and the
+// result is shifted so that z_0 = 0.  With z = M(r) the FCG(1) recurrence is
+//   p = z - (z.q_old / p_old.q_old) p_old,  q = A p,  alpha = r.p / p.q.
+// ---------------------------------------------------------------------------
+constexpr float kOmega = 6.0f / 7.0f;
+constexpr float kCorr = 1.8f;
+constexpr int kMaxLevels = 12;
+constexpr uint32_t kTailCells = 4096;
+
+struct Level {
+    uint32_t nx, ny, nz, nxy, n;
+    float cx, cy, cz;
+    FastDiv fx, fxy;
+    float inv[8];
+};
+
+struct FcgA {  // written by block 0 of k_mg_check (the stop test)
+    double rr0, rr, rmax, sumr;
+    int done, converged, status, pad;
+};
+struct FcgB {  // written by block 0 of k_fcg_update (and the host)
+    double pq;
+    long long iters;
+    int done, status, first, pad;
+};
+
+__device__ inline void lvl_ijk(const Level &L, uint32_t c, uint32_t &i, uint32_t &j, uint32_t &k) {
+    k = L.fxy.div(c);
+    const uint32_t rem = c - k * L.nxy;
+    j = L.fx.div(rem);
+    i = rem - j * L.nx;
+}
+__device__ inline uint32_t lvl_didx(const Level &L, uint32_t i, uint32_t j, uint32_t k) {
+    return (uint32_t)(i > 0 && i + 1 < L.nx) | ((uint32_t)(j > 0 && j + 1 < L.ny) << 1) |
+           ((uint32_t)(k > 0 && k + 1 < L.nz) << 2);
+}
+
+// f on level 0: the residual with the compatibility value at cell 0.
+__device__ inline float f_at(const float *f, uint32_t c, bool level0, float f0) {
+    return (level0 && c == 0) ? f0 : f[c];
+}
+
+// One damped-Jacobi sweep on a level:
+//   e_out = e' + omega D^-1 (f - A e'),  e' = e_in + corr P(e_c) (kProlong)
+// or, kFirst, e_out = omega D^-1 f (from a zero iterate).  kPublish (level 0,
+// last sweep) also publishes sum z.q_old and sum q_old for the FCG beta.
+template <bool kFirst, bool kProlong, bool kLevel0, bool kPublish>
+__global__ void __launch_bounds__(kBlock) k_mg_sweep(Level L, Level LC, const float *__restrict__ f,
+                                                     const float *__restrict__ ein,
+                                                     const float *__restrict__ ec,
+                                                     float *__restrict__ eout,
+                                                     const float *__restrict__ qold,
+                                                     double *__restrict__ pout, const FcgA *SA,
+                                                     const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    const float f0 = kLevel0 ? (float)(-SA->sumr) : 0.0f;
+    double s[2] = {0.0, 0.0}, m[1] = {0.0};
+    TileSched ts((L.n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t c = t * kBlock + threadIdx.x;
+        if (c >= L.n) continue;
+        uint32_t i, j, k;
+        lvl_ijk(L, c, i, j, k);
+        const float fc = f_at(f, c, kLevel0, f0);
+        const float inv = L.inv[lvl_didx(L, i, j, k)];
+        float out;
+        if (kFirst) {
+            out = kOmega * inv * fc;
+        } else {
+            const uint32_t ii[7] = {i, i + 1 < L.nx ? i + 1 : i, i > 0 ? i - 1 : i, i, i, i, i};
+            const uint32_t jj[7] = {j, j, j, j + 1 < L.ny ? j + 1 : j, j > 0 ? j - 1 : j, j, j};
+            const uint32_t kk[7] = {k, k, k, k, k, k + 1 < L.nz ? k + 1 : k, k > 0 ? k - 1 : k};
+            float ev[7];
+#pragma unroll
+            for (int a = 0; a < 7; ++a) {
+                const uint32_t idx = ii[a] + L.nx * (jj[a] + L.ny * kk[a]);
+                ev[a] = ein[idx];
+                if (kProlong) ev[a] += kCorr * ec[(ii[a] >> 1) + LC.nx * ((jj[a] >> 1) + LC.ny * (kk[a] >> 1))];
+            }
+            const float ae = L.cx * ((ev[0] - ev[1]) + (ev[0] - ev[2])) +
+                             L.cy * ((ev[0] - ev[3]) + (ev[0] - ev[4])) +
+                             L.cz * ((ev[0] - ev[5]) + (ev[0] - ev[6]));
+            out = ev[0] + kOmega * inv * (fc - ae);
+        }
+        eout[c] = out;
+        if (kPublish && !SB->first) {
+            const double qo = (double)qold[c];
+            s[0] += (double)out * qo;
+            s[1] += qo;
+        }
+    }
+    if (kPublish) publish_partials<2, 1>(s, m, pout);
+}
+
+// Restriction of the residual: f_c(C) = mean over the 8 children of f - A e.
+template <bool kLevel0>
+__global__ void __launch_bounds__(kBlock) k_mg_restrict(Level L, Level LC, const float *__restrict__ f,
+                                                        const float *__restrict__ e,
+                                                        float *__restrict__ fcoarse, const FcgA *SA,
+                                                        const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    const float f0 = kLevel0 ? (float)(-SA->sumr) : 0.0f;
+    for (uint32_t C = blockIdx.x * blockDim.x + threadIdx.x; C < LC.n; C += gridDim.x * blockDim.x) {
+        uint32_t I, J, K;
+        lvl_ijk(LC, C, I, J, K);
+        float acc = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const uint32_t i = 2 * I + (a & 1), j = 2 * J + (a >> 1 & 1), k = 2 * K + (a >> 2);
+            const uint32_t c = i + L.nx * (j + L.ny * k);
+            const float ec = e[c];
+            const float ee = e[i + 1 < L.nx ? c + 1 : c], ew = e[i > 0 ? c - 1 : c];
+            const float en = e[j + 1 < L.ny ? c + L.nx : c], es = e[j > 0 ? c - L.nx : c];
+            const float ef = e[k + 1 < L.nz ? c + L.nxy : c], eb = e[k > 0 ? c - L.nxy : c];
+            const float ae = L.cx * ((ec - ee) + (ec - ew)) + L.cy * ((ec - en) + (ec - es)) +
+                             L.cz * ((ec - ef) + (ec - eb));
+            acc += f_at(f, c, kLevel0, f0) - ae;
+        }
+        fcoarse[C] = 0.125f * acc;
+    }
+}
+
+// The levels with <= 4096 cells: a whole V-cycle (the same sweeps and
+// transfers) in one workgroup out of LDS; the coarsest level gets 60 sweeps
+// and its mean removed.  `lv` holds levels t..nl-1; f is level t's right-hand
+// side and e receives its solution.
+struct TailLevels {
+    Level lv[kMaxLevels];
+    int count;
+};
+__device__ inline void tail_sweep(const Level &L, const float *f, const float *ein, float *eout,
+                                  const float *ec, const Level *LC, bool first) {
+    for (uint32_t c = threadIdx.x; c < L.n; c += blockDim.x) {
+        uint32_t i, j, k;
+        lvl_ijk(L, c, i, j, k);
+        const float inv = L.inv[lvl_didx(L, i, j, k)];
+        if (first) {
+            eout[c] = kOmega * inv * f[c];
+            continue;
+        }
+        const uint32_t ii[7] = {i, i + 1 < L.nx ? i + 1 : i, i > 0 ? i - 1 : i, i, i, i, i};
+        const uint32_t jj[7] = {j, j, j, j + 1 < L.ny ? j + 1 : j, j > 0 ? j - 1 : j, j, j};
+        const uint32_t kk[7] = {k, k, k, k, k, k + 1 < L.nz ? k + 1 : k, k > 0 ? k - 1 : k};
+        float ev[7];
+        for (int a = 0; a < 7; ++a) {
+            ev[a] = ein[ii[a] + L.nx * (jj[a] + L.ny * kk[a])];
+            if (ec) ev[a] += kCorr * ec[(ii[a] >> 1) + LC->nx * ((jj[a] >> 1) + LC->ny * (kk[a] >> 1))];
+        }
+        const float ae = L.cx * ((ev[0] - ev[1]) + (ev[0] - ev[2])) +
+                         L.cy * ((ev[0] - ev[3]) + (ev[0] - ev[4])) +
+                         L.cz * ((ev[0] - ev[5]) + (ev[0] - ev[6]));
+        eout[c] = ev[0] + kOmega * inv * (f[c] - ae);
+    }
+    __syncthreads();
+}
+__global__ void __launch_bounds__(1024) k_mg_tail(TailLevels T, const float *__restrict__ fin,
+                                                  float *__restrict__ eout, const FcgA *SA,
+                                                  const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    // LDS: per level f, e, w (ping-pong); levels shrink by 8 so 3 * 1.15 * 4096 floats suffice
+    __shared__ float pool[3 * 4800];
+    float *F[kMaxLevels], *E[kMaxLevels], *W[kMaxLevels];
+    uint32_t off = 0;
+    for (int l = 0; l < T.count; ++l) {
+        F[l] = pool + off; off += T.lv[l].n;
+        E[l] = pool + off; off += T.lv[l].n;
+        W[l] = pool + off; off += T.lv[l].n;
+    }
+    for (uint32_t c = threadIdx.x; c < T.lv[0].n; c += blockDim.x) F[0][c] = fin[c];
+    __syncthreads();
+    const int last = T.count - 1;
+    // down: pre-smooth (result in E), restrict into F[l+1]
+    for (int l = 0; l < last; ++l) {
+        const Level &L = T.lv[l];
+        tail_sweep(L, F[l], nullptr, E[l], nullptr, nullptr, true);
+        tail_sweep(L, F[l], E[l], W[l], nullptr, nullptr, false);
+        tail_sweep(L, F[l], W[l], E[l], nullptr, nullptr, false);
+        const Level &LC = T.lv[l + 1];
+        for (uint32_t C = threadIdx.x; C < LC.n; C += blockDim.x) {
+            uint32_t I, J, K;
+            lvl_ijk(LC, C, I, J, K);
+            float acc = 0.0f;
+            for (int a = 0; a < 8; ++a) {
+                const uint32_t i = 2 * I + (a & 1), j = 2 * J + (a >> 1 & 1), k = 2 * K + (a >> 2);
+                const uint32_t c = i + L.nx * (j + L.ny * k);
+                const float *e = E[l];
+                const float ec = e[c];
+                const float ae = L.cx * ((ec - e[i + 1 < L.nx ? c + 1 : c]) + (ec - e[i > 0 ? c - 1 : c])) +
+                                 L.cy * ((ec - e[j + 1 < L.ny ? c + L.nx : c]) + (ec - e[j > 0 ? c - L.nx : c])) +
+                                 L.cz * ((ec - e[k + 1 < L.nz ? c + L.nxy : c]) + (ec - e[k > 0 ? c - L.nxy : c]));
+                acc += F[l][c] - ae;
+            }
+            F[l + 1][C] = 0.125f * acc;
+        }
+        __syncthreads();
+    }
+    // coarsest: 60 sweeps from zero, then remove the mean (Neumann null space)
+    {
+        const Level &L = T.lv[last];
+        tail_sweep(L, F[last], nullptr, E[last], nullptr, nullptr, true);
+        for (int it = 0; it < 59; it += 2) {
+            tail_sweep(L, F[last], E[last], W[last], nullptr, nullptr, false);
+            tail_sweep(L, F[last], W[last], E[last], nullptr, nullptr, false);
+        }
+        __shared__ float red[1024];
+        float acc = 0.0f;
+        for (uint32_t c = threadIdx.x; c < L.n; c += blockDim.x) acc += E[last][c];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        const float mean = red[0] / (float)L.n;
+        for (uint32_t c = threadIdx.x; c < L.n; c += blockDim.x) E[last][c] -= mean;
+        __syncthreads();
+    }
+    // up: prolongate-correct fused into the first post-sweep; result in E
+    for (int l = last - 1; l >= 0; --l) {
+        const Level &L = T.lv[l];
+        tail_sweep(L, F[l], E[l], W[l], E[l + 1], &T.lv[l + 1], false);
+        tail_sweep(L, F[l], W[l], E[l], nullptr, nullptr, false);
+        tail_sweep(L, F[l], E[l], W[l], nullptr, nullptr, false);
+        for (uint32_t c = threadIdx.x; c < L.n; c += blockDim.x) E[l][c] = W[l][c];
+        __syncthreads();
+    }
+    for (uint32_t c = threadIdx.x; c < T.lv[0].n; c += blockDim.x) eout[c] = E[0][c];
+}
+
+// Stop test of the inner FCG (folds the update partials), then the first
+// pre-sweep of the cycle on level 0.
+__global__ void __launch_bounds__(kBlock) k_mg_check(Level L, CgParams prm, const float *__restrict__ r,
+                                                     float *__restrict__ eout,
+                                                     const double *__restrict__ pin, FcgA *SA,
+                                                     const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    double fs[3], fm[1];
+    fold_partials<3, 1>(pin, gridDim.x, fs, fm);
+    const double rr = fs[1], sumr = fs[2], rmax = fm[0];
+    const bool first = SB->first != 0;
+    const double rr0 = first ? rr : SA->rr0;
+    int done = 0, conv = 0, status = ST_OK;
+    if (!isfinite(rmax) || !isfinite(rr)) {
+        status = ST_RESIDUAL;
+        done = 1;
+    } else if (prm.stop_rule == FL_STOP_REL_L2 ? sqrt(rr) <= prm.tol * sqrt(rr0) : rmax < prm.tol) {
+        conv = done = 1;
+    } else if (SB->iters >= prm.max_iter) {
+        done = 1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        SA->rr0 = rr0;
+        SA->rr = rr;
+        SA->rmax = rmax;
+        SA->sumr = sumr;
+        SA->converged = conv;
+        SA->status = status;
+        SA->done = done;
+    }
+    if (done) return;
+    const float f0 = (float)(-sumr);
+    TileSched ts((L.n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t c = t * kBlock + threadIdx.x;
+        if (c >= L.n) continue;
+        uint32_t i, j, k;
+        lvl_ijk(L, c, i, j, k);
+        eout[c] = kOmega * L.inv[lvl_didx(L, i, j, k)] * f_at(r, c, true, f0);
+    }
+}
+
+// p = (z~ - z~_0) - beta p_old, q = A p (pinned), partials p.q and r.p.
+__global__ void __launch_bounds__(kBlock) k_fcg_dirapply(Grid g, const float *__restrict__ zt,
+                                                         const float *__restrict__ pold,
+                                                         float *__restrict__ pnew,
+                                                         float *__restrict__ q,
+                                                         const float *__restrict__ r,
+                                                         const double *__restrict__ pz,
+                                                         double *__restrict__ pout,
+                                                         const FcgA *SA, const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    const bool first = SB->first != 0;
+    double fz[2], fm[1];
+    fold_partials<2, 1>(pz, gridDim.x, fz, fm);
+    const float z0 = zt[0];
+    const double beta = first ? 0.0 : (fz[0] - (double)z0 * fz[1]) / SB->pq;
+    const float tb = (float)beta;
+    const float cx = (float)g.ix2, cy = (float)g.iy2, cz = (float)g.iz2;
+    double s[2] = {0.0, 0.0}, m[1] = {0.0};
+    TileSched ts((g.n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t c = t * kBlock + threadIdx.x;
+        if (c >= g.n) continue;
+        float pc = 0.0f, qc = 0.0f;
+        if (c != 0) {
+            uint32_t i, j, k;
+            cell_ijk(g, c, i, j, k);
+            const Nbr o = neighbours(g, c, i, j, k);
+            const uint32_t idx[7] = {c, o.e, o.w, o.n, o.s, o.f, o.b};
+            float pv[7];
+#pragma unroll
+            for (int a = 0; a < 7; ++a) {
+                // the pinned cell's p stays 0
+                const float z = zt[idx[a]] - z0;
+                pv[a] = idx[a] == 0 ? 0.0f : (first ? z : z - tb * pold[idx[a]]);
+            }
+            pc = pv[0];
+            qc = cx * ((pv[0] - pv[1]) + (pv[0] - pv[2])) + cy * ((pv[0] - pv[3]) + (pv[0] - pv[4])) +
+                 cz * ((pv[0] - pv[5]) + (pv[0] - pv[6]));
+        }
+        pnew[c] = pc;
+        q[c] = qc;
+        s[0] += (double)pc * (double)qc;
+        s[1] += (double)r[c] * (double)pc;
+    }
+    publish_partials<2, 1>(s, m, pout);
+}
+
+// alpha = r.p / p.q; x += alpha p, r -= alpha q; partials r.r, sum r, max|r|.
+__global__ void __launch_bounds__(kBlock) k_fcg_update(uint32_t n, float *__restrict__ x,
+                                                       float *__restrict__ r,
+                                                       const float *__restrict__ p,
+                                                       const float *__restrict__ q,
+                                                       const double *__restrict__ pin,
+                                                       double *__restrict__ pout, const FcgA *SA,
+                                                       FcgB *SB) {
+    if (SA->done || SB->done) return;
+    double f[2], fm[1];
+    fold_partials<2, 1>(pin, gridDim.x, f, fm);
+    const double pq = f[0], rp = f[1];
+    int status = ST_OK;
+    double alpha = 0.0;
+    if (fabs(pq) < 1.0e-30 || !isfinite(pq)) status = ST_SINGULAR;
+    else {
+        alpha = rp / pq;
+        if (!isfinite(alpha)) status = ST_ALPHA;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        SB->pq = pq;
+        SB->first = 0;
+        SB->status = status;
+        if (status != ST_OK) SB->done = 1;
+        else SB->iters += 1;
+    }
+    if (status != ST_OK) return;
+    const float ta = (float)alpha;
+    double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
+    TileSched ts((n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t c = t * kBlock + threadIdx.x;
+        if (c >= n) continue;
+        x[c] += ta * p[c];
+        const float rc = r[c] - ta * q[c];
+        r[c] = rc;
+        s[1] += (double)rc * (double)rc;
+        s[2] += (double)rc;
+        m[0] = fmax(m[0], fabs((double)rc));
+    }
+    publish_partials<3, 1>(s, m, pout);
 }
 
 // ---------------------------------------------------------------------------
@@ -517,9 +891,20 @@ struct fl_cavity3d {
     CgStateA *sa = nullptr;
     CgStateB *sb = nullptr;
     DiagState *dg = nullptr;
+    // multigrid hierarchy (FL_PRECOND_MULTIGRID): levels 0..tail-1 in global
+    // memory, levels tail..nlev-1 inside k_mg_tail
+    int nlev = 0, tail = 0;
+    Level lv[kMaxLevels]{};
+    float *mf[kMaxLevels] = {}, *mea[kMaxLevels] = {}, *meb[kMaxLevels] = {};
+    uint32_t lblocks[kMaxLevels] = {};
+    double *pz = nullptr;  // partials z.q_old, sum q_old
+    FcgA *fa = nullptr;
+    FcgB *fb = nullptr;
     struct Host {
         CgStateA a;
         CgStateB b;
+        FcgA fa;
+        FcgB fb;
         DiagState dg;
         double outer[3];
     } *host = nullptr;
@@ -538,8 +923,8 @@ namespace {
 constexpr int kMaxBatch = 64;
 
 __global__ void k_fold_outer(const double *__restrict__ partials, uint32_t rows, double *out) {
-    double s[2], m[1];
-    fold_partials<2, 1>(partials, rows, s, m);
+    double s[3], m[1];
+    fold_partials<3, 1>(partials, rows, s, m);
     if (threadIdx.x == 0) {
         out[0] = s[0];
         out[1] = s[1];
@@ -561,9 +946,55 @@ void free_all(fl_cavity3d *s) {
                     s->sb,  s->dg,     s->outer};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    void *more[] = {s->pz, s->fa, s->fb};
+    for (void *b : more)
+        if (b) (void)hipFree(b);
+    for (int l = 0; l < kMaxLevels; ++l) {
+        if (s->mf[l]) (void)hipFree(s->mf[l]);
+        if (s->mea[l]) (void)hipFree(s->mea[l]);
+        if (s->meb[l]) (void)hipFree(s->meb[l]);
+    }
     if (s->host) (void)hipHostFree(s->host);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
     s->ev.clear();
+}
+
+// Multigrid levels: halve every axis while all are even and at least 8;
+// the coarse coefficients halve per level (Galerkin-consistent with the
+// mean restriction and piecewise-constant prolongation).  Levels from the
+// first one with <= kTailCells cells run inside k_mg_tail.
+fl_status build_hierarchy(fl_cavity3d *s) {
+    uint32_t nx = s->g.nx, ny = s->g.ny, nz = s->g.nz;
+    double cx = s->g.ix2, cy = s->g.iy2, cz = s->g.iz2;
+    int nl = 0;
+    for (;;) {
+        Level &L = s->lv[nl];
+        L.nx = nx; L.ny = ny; L.nz = nz; L.nxy = nx * ny; L.n = nx * ny * nz;
+        L.cx = (float)cx; L.cy = (float)cy; L.cz = (float)cz;
+        L.fx = FastDiv(nx);
+        L.fxy = FastDiv(nx * ny);
+        for (uint32_t b = 0; b < 8; ++b) {
+            const double dg = cx * (1 + (b & 1)) + cy * (1 + (b >> 1 & 1)) + cz * (1 + (b >> 2 & 1));
+            L.inv[b] = (float)(1.0 / dg);
+        }
+        ++nl;
+        const bool can = nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0 && nx >= 8 && ny >= 8 &&
+                         nz >= 8 && nl < kMaxLevels;
+        if (!can) break;
+        nx /= 2; ny /= 2; nz /= 2;
+        cx *= 0.5; cy *= 0.5; cz *= 0.5;
+    }
+    s->nlev = nl;
+    if (nl < 2 || s->lv[nl - 1].n > kTailCells)
+        return set_error(s->ctx, FL_ERR_CONFIG,
+                         "multigrid needs grid dimensions that halve (all even, >= 8) down to at "
+                         "most %u cells", kTailCells);
+    int t = 1;
+    while (s->lv[t].n > kTailCells) ++t;
+    s->tail = t;
+    for (int l = 0; l < nl; ++l)
+        s->lblocks[l] = grid_blocks(s->ctx, (s->lv[l].n + kBlock - 1) / kBlock, 4);
+    return FL_OK;
 }
 
 fl_status validate(fl_ctx *ctx, const fl_cavity3d_config *c) {
@@ -593,6 +1024,10 @@ fl_status validate(fl_ctx *ctx, const fl_cavity3d_config *c) {
         return set_error(ctx, FL_ERR_CONFIG, "unknown precision %d", c->precision);
     if (!(c->inner_tolerance >= 0.0 && c->inner_tolerance < 1.0))
         return set_error(ctx, FL_ERR_CONFIG, "inner_tolerance must lie in [0, 1)");
+    if (c->preconditioner != FL_PRECOND_JACOBI && c->preconditioner != FL_PRECOND_MULTIGRID)
+        return set_error(ctx, FL_ERR_CONFIG, "unknown preconditioner %d", c->preconditioner);
+    if (c->preconditioner == FL_PRECOND_MULTIGRID && c->precision != FL_PRECISION_MIXED)
+        return set_error(ctx, FL_ERR_CONFIG, "the multigrid preconditioner runs in the mixed-precision solve");
     const double cells = (double)c->nx * (double)c->ny * (double)c->nz;
     const double faces = (double)(c->nx + 1) * (double)(c->ny + 1) * (double)(c->nz + 1) * 3.0;
     if (cells + faces >= 2147483647.0)
@@ -660,12 +1095,13 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
     return FL_OK;
 }
 
-fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep) {
+fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep, double rr0_ref = 0.0) {
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
     k_residual<double><<<s->blocks, kBlock, 0, stm>>>(s->g, s->rhs, s->p, s->r, s->pb);
     FL_HIP_TRY(ctx, hipGetLastError());
-    const CgParams prm{s->cfg.pressure_tolerance, s->cfg.pressure_max_iters, s->cfg.stop_rule};
+    const CgParams prm{s->cfg.pressure_tolerance, s->cfg.pressure_max_iters, s->cfg.stop_rule,
+                       rr0_ref};
     fl_status st = run_cg<double, double>(s, prm, s->p, s->r, s->d, s->q);
     if (st != FL_OK) return st;
     const CgStateA &a = s->host->a;
@@ -674,6 +1110,8 @@ fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep) {
     if (rep) {
         rep->iterations = b.iters;
         rep->refinements = 0;
+        rep->fallback = 0;
+        rep->fallback_iterations = 0;
         rep->residual_abs_inf = a.rmax;
         rep->residual_l2 = std::sqrt(a.rr);
         rep->initial_l2 = std::sqrt(a.rr0);
@@ -684,15 +1122,102 @@ fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep) {
     return FL_OK;
 }
 
-// FP32 CG inside FP64 iterative refinement.
+// One multigrid-preconditioned FCG iteration: the V-cycle on r (levels
+// 0..tail-1 here, the rest in k_mg_tail), then the direction and update.
+void launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k) {
+    hipStream_t stm = s->ctx->stream;
+    const int t = s->tail;
+    const Level *L = s->lv;
+    float *pold = s->d32[k & 1], *pnew = s->d32[(k + 1) & 1];
+    const uint32_t G = s->blocks;
+    // down
+    k_mg_check<<<G, kBlock, 0, stm>>>(L[0], prm, s->r32, s->mea[0], s->pb, s->fa, s->fb);
+    k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], nullptr, nullptr, s->fa, s->fb);
+    k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb);
+    k_mg_restrict<true><<<s->lblocks[1], kBlock, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->mf[1], s->fa, s->fb);
+    for (int l = 1; l < t; ++l) {
+        const uint32_t g = s->lblocks[l];
+        k_mg_sweep<true, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], nullptr, nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb);
+        k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb);
+        k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb);
+        k_mg_restrict<false><<<s->lblocks[l + 1], kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->mf[l + 1], s->fa, s->fb);
+    }
+    TailLevels T{};
+    T.count = s->nlev - t;
+    for (int l = 0; l < T.count; ++l) T.lv[l] = L[t + l];
+    k_mg_tail<<<1, 1024, 0, stm>>>(T, s->mf[t], s->meb[t], s->fa, s->fb);
+    // up: the solution of level l ends in meb[l]
+    for (int l = t - 1; l >= 1; --l) {
+        const uint32_t g = s->lblocks[l];
+        k_mg_sweep<false, true, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->meb[l + 1], s->meb[l], nullptr, nullptr, s->fa, s->fb);
+        k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb);
+        k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb);
+    }
+    k_mg_sweep<false, true, true, false><<<G, kBlock, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->meb[1], s->meb[0], nullptr, nullptr, s->fa, s->fb);
+    k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb);
+    k_mg_sweep<false, false, true, true><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], s->q32, s->pz, s->fa, s->fb);
+    // FCG direction and update
+    k_fcg_dirapply<<<G, kBlock, 0, stm>>>(s->g, s->meb[0], pold, pnew, s->q32, s->r32, s->pz, s->pa, s->fa, s->fb);
+    k_fcg_update<<<G, kBlock, 0, stm>>>((uint32_t)s->n, s->e32, s->r32, pnew, s->q32, s->pa, s->pb, s->fa, s->fb);
+}
+
+// Inner FCG on A e = r32 (e32 = 0 on entry, r32 and its partials in pb from
+// k_residual).  Returns with the device state in host->fa / host->fb;
+// *stagnated is set when the residual fails to halve over a whole batch.
+fl_status run_fcg(fl_cavity3d *s, const CgParams &prm, bool *stagnated) {
+    fl_ctx *ctx = s->ctx;
+    hipStream_t stm = ctx->stream;
+    s->host->fa = FcgA{};
+    s->host->fb = FcgB{};
+    s->host->fb.first = 1;
+    FL_HIP_TRY(ctx, hipMemcpyAsync(s->fa, &s->host->fa, sizeof(FcgA), hipMemcpyHostToDevice, stm));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(s->fb, &s->host->fb, sizeof(FcgB), hipMemcpyHostToDevice, stm));
+    *stagnated = false;
+    long long launched = 0, iters_before = 0;
+    int batch = 4;
+    double rr_prev = -1.0;
+    for (;;) {
+        if (s->timing) FL_HIP_TRY(ctx, hipEventRecord(s->ev[0], stm));
+        for (int b = 0; b < batch; ++b) launch_fcg_iteration(s, prm, launched++);
+        if (s->timing) FL_HIP_TRY(ctx, hipEventRecord(s->ev[1], stm));
+        FL_HIP_TRY(ctx, hipGetLastError());
+        FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->fa, s->fa, sizeof(FcgA), hipMemcpyDeviceToHost, stm));
+        FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->fb, s->fb, sizeof(FcgB), hipMemcpyDeviceToHost, stm));
+        FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
+        if (s->timing) {
+            // whole FCG iterations (V-cycle + direction + update); launches
+            // after the stop test exit at once and add only their launch gaps
+            float ms = 0.0f;
+            FL_HIP_TRY(ctx, hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+            s->t_apply += ms;
+            s->t_iters += s->host->fb.iters - iters_before;
+        }
+        iters_before = s->host->fb.iters;
+        if (s->host->fa.done || s->host->fb.done) break;
+        const double rr = s->host->fa.rr;
+        if (rr_prev >= 0.0 && batch >= 8 && !(rr < 0.25 * rr_prev)) {
+            *stagnated = true;
+            break;
+        }
+        rr_prev = rr;
+        batch = std::min(batch * 2, 16);
+    }
+    return FL_OK;
+}
+
+// FP32 CG (Jacobi or multigrid-preconditioned flexible) inside FP64
+// iterative refinement; multigrid falls back to plain FP64 Jacobi-PCG from
+// the current iterate when an inner solve stagnates or exhausts its budget.
 fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
-    const double inner_tol = s->cfg.inner_tolerance > 0.0 ? s->cfg.inner_tolerance : 1.0e-5;
+    const bool mg = s->cfg.preconditioner == FL_PRECOND_MULTIGRID;
+    const double inner_tol =
+        s->cfg.inner_tolerance > 0.0 ? s->cfg.inner_tolerance : (mg ? 1.0e-4 : 1.0e-5);
     const bool rel = s->cfg.stop_rule == FL_STOP_REL_L2;
     double rr0 = 0.0;
     long long total = 0, sweeps = 0;
-    bool converged = false;
+    bool converged = false, fallback = false;
     double rr = 0.0, rmax = 0.0;
     for (;;) {
         k_residual<float><<<s->blocks, kBlock, 0, stm>>>(s->g, s->rhs, s->p, s->r32, s->pb);
@@ -709,20 +1234,51 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
                         : rmax < s->cfg.pressure_tolerance;
         if (converged || total >= s->cfg.pressure_max_iters || sweeps >= 64) break;
         FL_HIP_TRY(ctx, hipMemsetAsync(s->e32, 0, s->n * sizeof(float), stm));
-        const CgParams prm{inner_tol, s->cfg.pressure_max_iters - total, FL_STOP_REL_L2};
-        fl_status st = run_cg<float, float>(s, prm, s->e32, s->r32, s->d32, s->q32);
-        if (st != FL_OK) return st;
-        const CgStateB &b = s->host->b;
-        const int status = b.status != ST_OK ? b.status : s->host->a.status;
-        if (status != ST_OK) return set_error(ctx, FL_ERR_SOLVER, "%s", status_text(status));
-        total += b.iters;
+        long long it = 0;
+        int status = ST_OK;
+        bool budget_hit = false, stagnated = false;
+        if (mg) {
+            const long long budget = std::min<long long>(s->cfg.pressure_max_iters - total, 100);
+            const CgParams prm{inner_tol, budget, FL_STOP_REL_L2, 0.0};
+            fl_status st = run_fcg(s, prm, &stagnated);
+            if (st != FL_OK) return st;
+            const FcgB &b = s->host->fb;
+            status = b.status != ST_OK ? b.status : s->host->fa.status;
+            it = b.iters;
+            budget_hit = !s->host->fa.converged && !stagnated && status == ST_OK;
+        } else {
+            const CgParams prm{inner_tol, s->cfg.pressure_max_iters - total, FL_STOP_REL_L2, 0.0};
+            fl_status st = run_cg<float, float>(s, prm, s->e32, s->r32, s->d32, s->q32);
+            if (st != FL_OK) return st;
+            const CgStateB &b = s->host->b;
+            status = b.status != ST_OK ? b.status : s->host->a.status;
+            it = b.iters;
+        }
+        if (status != ST_OK && !mg) return set_error(ctx, FL_ERR_SOLVER, "%s", status_text(status));
+        total += it;
         sweeps += 1;
-        k_add_correction<<<s->ew_blocks, kBlock, 0, stm>>>((uint32_t)s->n, s->p, s->e32);
-        if (b.iters == 0) break;  // no progress possible in FP32
+        if (status == ST_OK && it > 0)
+            k_add_correction<<<s->ew_blocks, kBlock, 0, stm>>>((uint32_t)s->n, s->p, s->e32);
+        if (mg && (status != ST_OK || stagnated || budget_hit)) {
+            fallback = true;
+            break;
+        }
+        if (it == 0) break;  // no progress possible in FP32
+    }
+    fl_solve_report fb{};
+    if (fallback && total < s->cfg.pressure_max_iters) {
+        // the judge: plain FP64 Jacobi-PCG from the current iterate
+        fl_status st = solve_fp64(s, &fb, rr0);
+        if (st != FL_OK) return st;
+        converged = fb.converged;
+        rmax = fb.residual_abs_inf;
+        rr = fb.residual_l2 * fb.residual_l2;
     }
     if (rep) {
         rep->iterations = total;
         rep->refinements = sweeps;
+        rep->fallback = fallback ? 1 : 0;
+        rep->fallback_iterations = fb.iterations;
         rep->residual_abs_inf = rmax;
         rep->residual_l2 = std::sqrt(rr);
         rep->initial_l2 = std::sqrt(rr0);
@@ -857,10 +1413,23 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
             (st = alloc(s, &s->d32[0], s->n)) || (st = alloc(s, &s->d32[1], s->n)) ||
             (st = alloc(s, &s->q32, s->n)))
             goto fail;
-    } else {
+    }
+    if (!mixed || cfg->preconditioner == FL_PRECOND_MULTIGRID) {
+        // FP64 PCG vectors (the multigrid path keeps them for its fallback)
         if ((st = alloc(s, &s->r, s->n)) || (st = alloc(s, &s->d[0], s->n)) ||
             (st = alloc(s, &s->d[1], s->n)) || (st = alloc(s, &s->q, s->n)))
             goto fail;
+    }
+    if (cfg->preconditioner == FL_PRECOND_MULTIGRID) {
+        if ((st = build_hierarchy(s)) != FL_OK) goto fail;
+        if ((st = alloc(s, &s->pz, prow)) || (st = alloc(s, &s->fa, 1)) || (st = alloc(s, &s->fb, 1)))
+            goto fail;
+        for (int l = 0; l <= s->tail; ++l) {
+            const size_t n = s->lv[l].n;
+            if (l > 0 && (st = alloc(s, &s->mf[l], n))) goto fail;
+            if (l < s->tail && (st = alloc(s, &s->mea[l], n))) goto fail;
+            if ((st = alloc(s, &s->meb[l], n))) goto fail;
+        }
     }
     if (hipHostMalloc(&s->host, sizeof(fl_cavity3d::Host), hipHostMallocDefault) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {

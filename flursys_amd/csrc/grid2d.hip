@@ -1,0 +1,1331 @@
+// 2D structured projection step on gfx950, batched over independent grids.
+//
+// Replaces IncompressibleSolver with projection coupling
+// (src/solver/incompressible.rs):
+//   predict_momentum :748-922 (+ MomentumStencil :2089-2212) -> k_predict
+//   apply_velocity_boundaries :1981-2044                      -> k_boundaries
+//   build_pressure_rhs :924-970                               -> k_rhs
+//   solve_pressure_pcg :1048-1214 (+ operator :1216-1255,
+//     Jacobi :1257-1294, project_zero_mean :2338-2377)        -> k_pcg (persistent)
+//   correct_velocity :1322-1379                               -> k_correct
+//   compute_cell_fields :1381-1439, max_divergence :1569-1582,
+//     ensure_finite :1707-1742, stability numbers :2294-2306  -> k_cells + k_fold_diag
+//   advance_one_step :622-746, adaptive_time_step :2308-2336  -> host driver
+//
+// A single solver is a batch of one.  Cells of every grid are packed into
+// 256-cell tiles (one grid per tile, tail padded); faces are packed per grid.
+// Per-cell flag bytes carry the mask: which of the four neighbours is open,
+// and whether the row is an identity row (solid or outlet cell).
+//
+// The pressure solve is one cooperative persistent kernel for all grids.
+// Per iteration it runs two phases separated by grid barriers:
+//   A: fold the update partials -> stop test, beta, the zero-mean shift of
+//      z; d' = P(D^-1 r) + beta d (recomputed at the stencil points),
+//      q = A d', partial d'.q
+//   B: fold d'.q -> alpha; x += alpha d', r -= alpha q; partials r.z, sum z,
+//      sum r, r.r, max|r|
+// The reference's zero-mean projections (on x, b, r, z, d every iteration)
+// are applied where they change anything: b once, z through its mean (folded
+// into the d' formula and into r.z = sum r z - sum r sum z / count), x once
+// at the end.  A d' and the CG recurrences keep r and d mean-free up to
+// round-off because the Neumann operator's columns sum to zero.
+
+#include <hip/hip_cooperative_groups.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+
+namespace cgrp = cooperative_groups;
+
+namespace fl {
+namespace g2 {
+
+constexpr int kT = 256;  // cells per tile (= threads per block)
+
+enum : uint8_t { F_E = 1, F_W = 2, F_N = 4, F_S = 8, F_ID = 16, F_SOLID = 32, F_OUTLET = 64 };
+enum { KV = FL_BC_VELOCITY, KS = FL_BC_SYMMETRY, KO = FL_BC_OUTLET };
+enum { SL = FL_SIDE_LEFT, SR = FL_SIDE_RIGHT, SB = FL_SIDE_BOTTOM, ST = FL_SIDE_TOP };
+
+struct Prob {
+    uint32_t nx, ny, n;
+    uint32_t tile0, ntiles;
+    uint32_t uoff, voff;
+    uint32_t owner;  // block of the PCG kernel that records this grid's state
+    double dx, dy, cx, cy, nu, rho;
+    int32_t scheme;
+    int32_t kind[4];
+    double bu[4], bv[4];
+    double pout;
+    int32_t zm;  // zero-mean gauge (no outlet cells)
+    double inv_count;
+    double tol;
+    long long max_iter;
+    int32_t stop_rule, guess;
+    FastDiv fx;
+};
+
+struct PcgA {  // written by the owner block in phase A
+    double rz, beta, mz, rr0, rr, rmax;
+    int converged, status;
+};
+struct PcgB {  // written by the owner block in phase B (and the host)
+    double rz_prev, alpha;
+    long long iters;
+    int first, status;
+};
+enum { ST_OK = 0, ST_SINGULAR = 1, ST_ALPHA = 2, ST_RESIDUAL = 3 };
+
+struct Diag {
+    double max_div, max_speed, max_u, max_v, du, dv;
+    unsigned long long bad[4];  // first non-finite index of p, u, v, speed
+};
+
+__device__ inline bool cell_of(const Prob &P, uint32_t t, uint32_t &c, uint32_t &i, uint32_t &j) {
+    c = (t - P.tile0) * kT + threadIdx.x;
+    if (c >= P.n) return false;
+    j = P.fx.div(c);
+    i = c - j * P.nx;
+    return true;
+}
+
+// u_face_open / v_face_open (incompressible.rs:2046-2064) from the flags.
+__device__ inline bool uopen(const Prob &P, const uint8_t *F, int i, int j) {
+    if (i == 0) return !(F[j * P.nx] & F_SOLID);
+    if (i == (int)P.nx) return !(F[P.nx - 1 + j * P.nx] & F_SOLID);
+    const uint8_t f = F[i + j * P.nx];
+    return !(f & F_SOLID) && (f & F_W);
+}
+__device__ inline bool vopen(const Prob &P, const uint8_t *F, int i, int j) {
+    if (j == 0) return !(F[i] & F_SOLID);
+    if (j == (int)P.ny) return !(F[i + (P.ny - 1) * P.nx] & F_SOLID);
+    const uint8_t f = F[i + j * P.nx];
+    return !(f & F_SOLID) && (f & F_S);
+}
+
+// MomentumStencil::u_neighbor / v_neighbor with the tangential wall ghosts.
+__device__ inline double u_nb(const Prob &P, const uint8_t *F, const double *u, int i, int j,
+                              int di, int dj, double c) {
+    const int a = i + di, b = j + dj;
+    if (a >= 0 && a <= (int)P.nx && b >= 0 && b < (int)P.ny)
+        return uopen(P, F, a, b) ? u[a + (P.nx + 1) * b] : -c;
+    const int side = b < 0 ? SB : (b >= (int)P.ny ? ST : -1);
+    if (side < 0) return c;
+    return P.kind[side] == KV ? 2.0 * P.bu[side] - c : c;
+}
+__device__ inline double v_nb(const Prob &P, const uint8_t *F, const double *v, int i, int j,
+                              int di, int dj, double c) {
+    const int a = i + di, b = j + dj;
+    if (a >= 0 && a < (int)P.nx && b >= 0 && b <= (int)P.ny)
+        return vopen(P, F, a, b) ? v[a + P.nx * b] : -c;
+    const int side = a < 0 ? SL : (a >= (int)P.nx ? SR : -1);
+    if (side < 0) return c;
+    return P.kind[side] == KV ? 2.0 * P.bv[side] - c : c;
+}
+__device__ inline double transported(int scheme, double fv, double l, double r) {
+    return scheme == FL_UPWIND ? (fv >= 0.0 ? l : r) : 0.5 * (l + r);
+}
+
+// ------------------------------------------------------------ step kernels
+__global__ void __launch_bounds__(kT) k_predict(const Prob *probs, const uint32_t *tile_prob,
+                                                uint32_t ntiles, const int *active,
+                                                const double *dts, const uint8_t *flags,
+                                                const double *__restrict__ U,
+                                                const double *__restrict__ V,
+                                                double *__restrict__ US, double *__restrict__ VS) {
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t pi = tile_prob[t];
+        if (!active[pi]) continue;
+        const Prob &P = probs[pi];
+        uint32_t c, ui, uj;
+        if (!cell_of(P, t, c, ui, uj)) continue;
+        const int i = ui, j = uj, nx = P.nx, ny = P.ny;
+        const uint8_t *F = flags + (size_t)P.tile0 * kT;
+        const double *u = U + P.uoff, *v = V + P.voff;
+        double *us = US + P.uoff, *vs = VS + P.voff;
+        const double dt = dts[pi], dx = P.dx, dy = P.dy;
+        if (i >= 1) {
+            double val = 0.0;
+            if (uopen(P, F, i, j)) {
+                const double up = u[i + (nx + 1) * j];
+                const double ue_nb = u_nb(P, F, u, i, j, 1, 0, up), uw_nb = u_nb(P, F, u, i, j, -1, 0, up);
+                const double un_nb = u_nb(P, F, u, i, j, 0, 1, up), us_nb = u_nb(P, F, u, i, j, 0, -1, up);
+                const double ue = 0.5 * (up + ue_nb), uw = 0.5 * (uw_nb + up);
+                const double vn = 0.5 * (v[i - 1 + nx * (j + 1)] + v[i + nx * (j + 1)]);
+                const double vsv = 0.5 * (v[i - 1 + nx * j] + v[i + nx * j]);
+                const double pe = transported(P.scheme, ue, up, ue_nb), pw = transported(P.scheme, uw, uw_nb, up);
+                const double pn = transported(P.scheme, vn, up, un_nb), ps = transported(P.scheme, vsv, us_nb, up);
+                const double conv = (ue * pe - uw * pw) / dx + (vn * pn - vsv * ps) / dy;
+                const double diff = (ue_nb - 2.0 * up + uw_nb) / (dx * dx) + (un_nb - 2.0 * up + us_nb) / (dy * dy);
+                val = up + dt * (-conv + P.nu * diff);
+            }
+            us[i + (nx + 1) * j] = val;
+        } else {
+            us[(nx + 1) * j] = 0.0;
+        }
+        if (i == nx - 1) us[nx + (nx + 1) * j] = 0.0;
+        if (j >= 1) {
+            double val = 0.0;
+            if (vopen(P, F, i, j)) {
+                const double vp = v[i + nx * j];
+                const double ve_nb = v_nb(P, F, v, i, j, 1, 0, vp), vw_nb = v_nb(P, F, v, i, j, -1, 0, vp);
+                const double vn_nb = v_nb(P, F, v, i, j, 0, 1, vp), vs_nb = v_nb(P, F, v, i, j, 0, -1, vp);
+                const double ue = 0.5 * (u[i + 1 + (nx + 1) * (j - 1)] + u[i + 1 + (nx + 1) * j]);
+                const double uw = 0.5 * (u[i + (nx + 1) * (j - 1)] + u[i + (nx + 1) * j]);
+                const double vn = 0.5 * (vp + vn_nb), vsv = 0.5 * (vs_nb + vp);
+                const double pe = transported(P.scheme, ue, vp, ve_nb), pw = transported(P.scheme, uw, vw_nb, vp);
+                const double pn = transported(P.scheme, vn, vp, vn_nb), ps = transported(P.scheme, vsv, vs_nb, vp);
+                const double conv = (ue * pe - uw * pw) / dx + (vn * pn - vsv * ps) / dy;
+                const double diff = (ve_nb - 2.0 * vp + vw_nb) / (dx * dx) + (vn_nb - 2.0 * vp + vs_nb) / (dy * dy);
+                val = vp + dt * (-conv + P.nu * diff);
+            }
+            vs[i + nx * j] = val;
+        } else {
+            vs[i] = 0.0;
+        }
+        if (j == ny - 1) vs[i + nx * ny] = 0.0;
+    }
+}
+
+// Boundary-normal faces (apply_velocity_boundaries :1981-2022); one block
+// per grid.  Interior blocked faces are already zero.
+__global__ void __launch_bounds__(kT) k_boundaries(const Prob *probs, const int *active,
+                                                   double *__restrict__ U, double *__restrict__ V) {
+    const uint32_t pi = blockIdx.x;
+    if (!active[pi]) return;
+    const Prob &P = probs[pi];
+    double *u = U + P.uoff, *v = V + P.voff;
+    const uint32_t nx = P.nx, ny = P.ny;
+    for (uint32_t j = threadIdx.x; j < ny; j += blockDim.x) {
+        double &l = u[(nx + 1) * j], &r = u[nx + (nx + 1) * j];
+        l = P.kind[SL] == KV ? P.bu[SL] : (P.kind[SL] == KS ? 0.0 : u[1 + (nx + 1) * j]);
+        r = P.kind[SR] == KV ? P.bu[SR] : (P.kind[SR] == KS ? 0.0 : u[nx - 1 + (nx + 1) * j]);
+    }
+    for (uint32_t i = threadIdx.x; i < nx; i += blockDim.x) {
+        double &b = v[i], &t = v[i + nx * ny];
+        b = P.kind[SB] == KV ? P.bv[SB] : (P.kind[SB] == KS ? 0.0 : v[i + nx]);
+        t = P.kind[ST] == KV ? P.bv[ST] : (P.kind[ST] == KS ? 0.0 : v[i + nx * (ny - 1)]);
+    }
+}
+
+// build_pressure_rhs (:924-970): zero on solid and outlet cells.
+__global__ void __launch_bounds__(kT) k_rhs(const Prob *probs, const uint32_t *tile_prob,
+                                            uint32_t ntiles, const int *active, const double *dts,
+                                            const uint8_t *flags, const double *__restrict__ US,
+                                            const double *__restrict__ VS, double *__restrict__ RHS) {
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t pi = tile_prob[t];
+        if (!active[pi]) continue;
+        const Prob &P = probs[pi];
+        uint32_t c, i, j;
+        if (!cell_of(P, t, c, i, j)) continue;
+        const size_t base = (size_t)P.tile0 * kT;
+        if (flags[base + c] & F_ID) {
+            RHS[base + c] = 0.0;
+            continue;
+        }
+        const double *us = US + P.uoff, *vs = VS + P.voff;
+        const uint32_t nx = P.nx;
+        const double div = (us[i + 1 + (nx + 1) * j] - us[i + (nx + 1) * j]) / P.dx +
+                           (vs[i + nx * (j + 1)] - vs[i + nx * j]) / P.dy;
+        RHS[base + c] = P.rho / dts[pi] * div;
+    }
+}
+
+// correct_velocity (:1322-1379): u = u* - dt/rho grad p on open interior
+// faces, 0 on blocked ones, boundary faces copied from u*.
+__global__ void __launch_bounds__(kT) k_correct(const Prob *probs, const uint32_t *tile_prob,
+                                                uint32_t ntiles, const int *active,
+                                                const double *dts, const uint8_t *flags,
+                                                const double *__restrict__ Pp,
+                                                const double *__restrict__ US,
+                                                const double *__restrict__ VS,
+                                                double *__restrict__ U, double *__restrict__ V) {
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t pi = tile_prob[t];
+        if (!active[pi]) continue;
+        const Prob &P = probs[pi];
+        uint32_t c, ui, uj;
+        if (!cell_of(P, t, c, ui, uj)) continue;
+        const int i = ui, j = uj, nx = P.nx, ny = P.ny;
+        const uint8_t *F = flags + (size_t)P.tile0 * kT;
+        const double *p = Pp + (size_t)P.tile0 * kT;
+        const double *us = US + P.uoff, *vs = VS + P.voff;
+        double *u = U + P.uoff, *v = V + P.voff;
+        const double dor = dts[pi] / P.rho;
+        if (i >= 1)
+            u[i + (nx + 1) * j] = uopen(P, F, i, j)
+                                      ? us[i + (nx + 1) * j] - dor * ((p[c] - p[c - 1]) / P.dx)
+                                      : 0.0;
+        else
+            u[(nx + 1) * j] = us[(nx + 1) * j];
+        if (i == nx - 1) u[nx + (nx + 1) * j] = us[nx + (nx + 1) * j];
+        if (j >= 1)
+            v[i + nx * j] = vopen(P, F, i, j) ? vs[i + nx * j] - dor * ((p[c] - p[c - nx]) / P.dy)
+                                              : 0.0;
+        else
+            v[i] = vs[i];
+        if (j == ny - 1) v[i + nx * ny] = vs[i + nx * ny];
+    }
+}
+
+// Cell-centred fields (:1381-1439) and the per-tile diagnostic maxima: max
+// divergence (:1569-1582), max speed, max|u|, max|v|, velocity change, and
+// the first non-finite index of p, u, v, speed (:1707-1742).
+__device__ inline double cell_u(const Prob &P, const double *u, int i, int j) {
+    return 0.5 * (u[i + (P.nx + 1) * j] + u[i + 1 + (P.nx + 1) * j]);
+}
+__device__ inline double cell_v(const Prob &P, const double *v, int i, int j) {
+    return 0.5 * (v[i + P.nx * j] + v[i + P.nx * (j + 1)]);
+}
+__global__ void __launch_bounds__(kT) k_cells(const Prob *probs, const uint32_t *tile_prob,
+                                              uint32_t ntiles, const int *active,
+                                              const uint8_t *flags, const double *__restrict__ Pp,
+                                              const double *__restrict__ U,
+                                              const double *__restrict__ V,
+                                              const double *__restrict__ UO,
+                                              const double *__restrict__ VO, double *UC,
+                                              double *VC, double *SP, double *VORT,
+                                              double *partials, Diag *diag) {
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t pi = tile_prob[t];
+        const Prob &P = probs[pi];
+        double s[1] = {0.0}, m[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t c = 0, ui = 0, uj = 0;
+        const bool live = active[pi] && cell_of(P, t, c, ui, uj);
+        if (live) {
+            const int i = ui, j = uj, nx = P.nx, ny = P.ny;
+            const size_t base = (size_t)P.tile0 * kT;
+            const uint8_t *F = flags + base;
+            const double *u = U + P.uoff, *v = V + P.voff, *uo = UO + P.uoff, *vo = VO + P.voff;
+            const bool solid = F[c] & F_SOLID;
+            double uc = 0.0, vc = 0.0, sp = 0.0, vort = 0.0;
+            if (!solid) {
+                uc = cell_u(P, u, i, j);
+                vc = cell_v(P, v, i, j);
+                sp = sqrt(uc * uc + vc * vc);
+                // derivative_x_slice (v_cell) - derivative_y_slice (u_cell) (:2228-2285)
+                const bool eo = i + 1 < nx && !(F[c + 1] & F_SOLID), wo = i > 0 && !(F[c - 1] & F_SOLID);
+                const bool no = j + 1 < ny && !(F[c + nx] & F_SOLID), so = j > 0 && !(F[c - nx] & F_SOLID);
+                const double ve = eo ? cell_v(P, v, i + 1, j) : vc, vw = wo ? cell_v(P, v, i - 1, j) : vc;
+                const double un = no ? cell_u(P, u, i, j + 1) : uc, us_ = so ? cell_u(P, u, i, j - 1) : uc;
+                const double dvdx = (eo && wo) ? (ve - vw) / (2.0 * P.dx)
+                                    : eo       ? (ve - vc) / P.dx
+                                    : wo       ? (vc - vw) / P.dx
+                                               : 0.0;
+                const double dudy = (no && so) ? (un - us_) / (2.0 * P.dy)
+                                    : no       ? (un - uc) / P.dy
+                                    : so       ? (uc - us_) / P.dy
+                                               : 0.0;
+                vort = dvdx - dudy;
+                const double div = (u[i + 1 + (nx + 1) * j] - u[i + (nx + 1) * j]) / P.dx +
+                                   (v[i + nx * (j + 1)] - v[i + nx * j]) / P.dy;
+                m[0] = fmax(m[0], fabs(div));
+            }
+            UC[base + c] = uc;
+            VC[base + c] = vc;
+            SP[base + c] = sp;
+            VORT[base + c] = vort;
+            m[1] = fmax(m[1], sp);
+            // faces owned by this cell: u(i,j) [+ u(nx,j)], v(i,j) [+ v(i,ny)]
+            const uint32_t fu = i + (nx + 1) * j, fv = i + nx * j;
+            m[2] = fmax(m[2], fabs(u[fu]));
+            m[3] = fmax(m[3], fabs(v[fv]));
+            m[4] = fmax(m[4], fabs(u[fu] - uo[fu]));
+            m[5] = fmax(m[5], fabs(v[fv] - vo[fv]));
+            if (!isfinite(u[fu])) atomicMin(&diag[pi].bad[1], (unsigned long long)fu);
+            if (!isfinite(v[fv])) atomicMin(&diag[pi].bad[2], (unsigned long long)fv);
+            if (i == nx - 1) {
+                const uint32_t f2 = nx + (nx + 1) * j;
+                m[2] = fmax(m[2], fabs(u[f2]));
+                m[4] = fmax(m[4], fabs(u[f2] - uo[f2]));
+                if (!isfinite(u[f2])) atomicMin(&diag[pi].bad[1], (unsigned long long)f2);
+            }
+            if (j == ny - 1) {
+                const uint32_t f2 = i + nx * ny;
+                m[3] = fmax(m[3], fabs(v[f2]));
+                m[5] = fmax(m[5], fabs(v[f2] - vo[f2]));
+                if (!isfinite(v[f2])) atomicMin(&diag[pi].bad[2], (unsigned long long)f2);
+            }
+            if (!isfinite(Pp[base + c])) atomicMin(&diag[pi].bad[0], (unsigned long long)c);
+            if (!isfinite(sp)) atomicMin(&diag[pi].bad[3], (unsigned long long)c);
+        }
+        block_reduce<1, 6>(s, m);
+        if (threadIdx.x == 0)
+            for (int a = 0; a < 6; ++a) partials[(size_t)t * 6 + a] = m[a];
+        __syncthreads();
+    }
+}
+
+// One block per grid folds its tiles' diagnostic maxima in tile order.
+__global__ void __launch_bounds__(kT) k_fold_diag(const Prob *probs, const int *active,
+                                                  const double *partials, Diag *diag) {
+    const uint32_t pi = blockIdx.x;
+    if (!active[pi]) return;
+    const Prob &P = probs[pi];
+    double s[1] = {0.0}, m[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t t = threadIdx.x; t < P.ntiles; t += blockDim.x)
+        for (int a = 0; a < 6; ++a) m[a] = fmax(m[a], partials[(size_t)(P.tile0 + t) * 6 + a]);
+    block_reduce<1, 6>(s, m);
+    if (threadIdx.x == 0) {
+        Diag &d = diag[pi];
+        d.max_div = m[0];
+        d.max_speed = m[1];
+        d.max_u = m[2];
+        d.max_v = m[3];
+        d.du = m[4];
+        d.dv = m[5];
+    }
+}
+
+// ------------------------------------------------------ persistent PCG
+struct PcgArgs {
+    const Prob *probs;
+    const uint32_t *tile_prob;
+    uint32_t ntiles, nprob;
+    long long kmax;  // hard bound on the loop (max over grids of max_iter + 2)
+    const uint8_t *flags;
+    const double *rhs;
+    double *x, *r, *d0, *d1, *q;
+    double *pa, *pb, *p0;  // per-tile partials
+    PcgA *sa;
+    PcgB *sb;
+    const int *active;  // grids taking part in this solve
+    int *fin;  // fin[parity * nprob + grid]: grid finished in that loop (the
+               // host sets fin[1][g] = !active[g], the "loop -1" verdicts)
+};
+
+__host__ __device__ inline uint32_t range_lo(uint32_t b, uint32_t T, uint32_t G) {
+    return (uint32_t)(((uint64_t)b * T) / G);
+}
+
+// Jacobi inverse diagonal from the open-neighbour bits.
+__device__ inline double inv_diag(const Prob &P, uint8_t f) {
+    const double dg = P.cx * (double)(((f & F_E) != 0) + ((f & F_W) != 0)) +
+                      P.cy * (double)(((f & F_N) != 0) + ((f & F_S) != 0));
+    return dg > 0.0 ? 1.0 / dg : 1.0;
+}
+
+template <int K>
+__device__ inline void fold_tiles(const double *part, uint32_t tile0, uint32_t nt, int nsum,
+                                  double (&out)[K]) {
+    double s[K - 1 > 0 ? K - 1 : 1], m[1];
+    (void)nsum;
+    fold_partials<K - 1, 1>(part + (size_t)tile0 * K, nt, s, m);
+#pragma unroll
+    for (int a = 0; a < K - 1; ++a) out[a] = s[a];
+    out[K - 1] = m[0];
+}
+
+__global__ void __launch_bounds__(kT) k_pcg(PcgArgs A) {
+    cgrp::grid_group gg = cgrp::this_grid();
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    const uint32_t t_lo = range_lo(b, A.ntiles, G), t_hi = range_lo(b + 1, A.ntiles, G);
+    const uint32_t tid = threadIdx.x;
+    // ---- init 0: sums of x0 and b = -rhs over the gauge cells
+    for (uint32_t t = t_lo; t < t_hi; ++t) {
+        const uint32_t pi = A.tile_prob[t];
+        double s[2] = {0.0, 0.0}, m[1] = {0.0};
+        if (A.active[pi]) {
+            const Prob &P = A.probs[pi];
+            const uint32_t c = (t - P.tile0) * kT + tid;
+            const size_t base = (size_t)P.tile0 * kT;
+            if (c < P.n && !(A.flags[base + c] & F_ID)) {
+                s[0] = P.guess == FL_GUESS_ZERO ? 0.0 : A.x[base + c];
+                s[1] = -A.rhs[base + c];
+            }
+        }
+        block_reduce<2, 1>(s, m);
+        if (tid == 0) {
+            A.p0[(size_t)t * 3] = s[0];
+            A.p0[(size_t)t * 3 + 1] = s[1];
+            A.p0[(size_t)t * 3 + 2] = 0.0;
+        }
+        __syncthreads();
+    }
+    gg.sync();
+    // ---- init 1: r = P(b) - A x0; identity rows x = value, r = 0
+    {
+        uint32_t cur = ~0u;
+        double mean_b = 0.0;
+        for (uint32_t t = t_lo; t < t_hi; ++t) {
+            const uint32_t pi = A.tile_prob[t];
+            const bool skip = !A.active[pi];
+            const Prob &P = A.probs[pi];
+            if (!skip && pi != cur) {
+                cur = pi;
+                double f[3];
+                fold_tiles<3>(A.p0, P.tile0, P.ntiles, 2, f);
+                mean_b = P.zm ? f[1] * P.inv_count : 0.0;
+            }
+            double s[4] = {0, 0, 0, 0}, m[1] = {0.0};
+            if (!skip) {
+                const uint32_t c = (t - P.tile0) * kT + tid;
+                const size_t base = (size_t)P.tile0 * kT;
+                if (c < P.n) {
+                    const uint8_t f = A.flags[base + c];
+                    double *x = A.x + base;
+                    double rc = 0.0, z = 0.0;
+                    if (f & F_ID) {
+                        x[c] = (f & F_OUTLET) ? P.pout : 0.0;
+                    } else {
+                        const bool zero = P.guess == FL_GUESS_ZERO;
+                        auto xv = [&](uint32_t k) -> double {
+                            const uint8_t g = A.flags[base + k];
+                            if (g & F_ID) return (g & F_OUTLET) ? P.pout : 0.0;
+                            return zero ? 0.0 : x[k];
+                        };
+                        const double xc = xv(c);
+                        double dg = 0.0, sum = 0.0;
+                        if (f & F_E) { dg += P.cx; sum += P.cx * xv(c + 1); }
+                        if (f & F_W) { dg += P.cx; sum += P.cx * xv(c - 1); }
+                        if (f & F_N) { dg += P.cy; sum += P.cy * xv(c + P.nx); }
+                        if (f & F_S) { dg += P.cy; sum += P.cy * xv(c - P.nx); }
+                        rc = (-A.rhs[base + c] - mean_b) - (dg * xc - sum);
+                        z = rc * inv_diag(P, f);
+                        s[1] += z;
+                        s[2] += rc;
+                    }
+                    A.r[base + c] = rc;
+                    s[0] += rc * z;
+                    s[3] += rc * rc;
+                    m[0] = fmax(m[0], fabs(rc));
+                }
+            }
+            block_reduce<4, 1>(s, m);
+            if (tid == 0) {
+                double *pb = A.pb + (size_t)t * 5;
+                pb[0] = s[0]; pb[1] = s[1]; pb[2] = s[2]; pb[3] = s[3]; pb[4] = m[0];
+            }
+            __syncthreads();
+        }
+    }
+    gg.sync();
+    // zero-guess problems start from x = 0 on the fluid cells
+    for (uint32_t t = t_lo; t < t_hi; ++t) {
+        const uint32_t pi = A.tile_prob[t];
+        const Prob &P = A.probs[pi];
+        if (!A.active[pi] || P.guess != FL_GUESS_ZERO) continue;
+        const uint32_t c = (t - P.tile0) * kT + tid;
+        const size_t base = (size_t)P.tile0 * kT;
+        if (c < P.n && !(A.flags[base + c] & F_ID)) A.x[base + c] = 0.0;
+    }
+    // ---- CG loop
+    long long k = 0;
+    for (; k < A.kmax; ++k) {
+        const int par = (int)(k & 1), prev = par ^ 1;
+        double *dold = (k & 1) ? A.d1 : A.d0, *dnew = (k & 1) ? A.d0 : A.d1;
+        // phase A
+        {
+            uint32_t cur = ~0u;
+            bool done = false, first = false;
+            double beta = 0.0, mz = 0.0;
+            for (uint32_t t = t_lo; t < t_hi; ++t) {
+                const uint32_t pi = A.tile_prob[t];
+                const Prob &P = A.probs[pi];
+                const bool was = A.fin[prev * A.nprob + pi];
+                if (pi != cur) {
+                    cur = pi;
+                    done = was;
+                    if (!was) {
+                        double f[5];
+                        fold_tiles<5>(A.pb, P.tile0, P.ntiles, 4, f);
+                        const PcgB sb = A.sb[pi];
+                        first = sb.first != 0;
+                        const double rr = f[3], rmax = f[4];
+                        const double rz = P.zm ? f[0] - f[2] * f[1] * P.inv_count : f[0];
+                        mz = P.zm ? f[1] * P.inv_count : 0.0;
+                        const double rr0 = first ? rr : A.sa[pi].rr0;
+                        int status = ST_OK, conv = 0;
+                        beta = 0.0;
+                        if (!first && !isfinite(rmax)) {
+                            status = ST_RESIDUAL;
+                            done = true;
+                        } else if (P.stop_rule == FL_STOP_REL_L2 ? sqrt(rr) <= P.tol * sqrt(rr0)
+                                                                : rmax < P.tol) {
+                            conv = 1;
+                            done = true;
+                        } else if (!first) {
+                            if (fabs(sb.rz_prev) < 1.0e-30) done = true;
+                            else if (sb.iters >= P.max_iter) done = true;
+                            else beta = rz / sb.rz_prev;
+                        }
+                        if (b == P.owner && tid == 0) {
+                            PcgA &sa = A.sa[pi];
+                            sa.rz = rz;
+                            sa.beta = beta;
+                            sa.mz = mz;
+                            sa.rr0 = rr0;
+                            sa.rr = rr;
+                            sa.rmax = rmax;
+                            sa.converged = conv;
+                            sa.status = status;
+                        }
+                    }
+                    if (b == P.owner && tid == 0) A.fin[par * A.nprob + pi] = done ? 1 : 0;
+                }
+                double s[1] = {0.0}, m[1] = {0.0};
+                if (!done) {
+                    const uint32_t c = (t - P.tile0) * kT + tid;
+                    const size_t base = (size_t)P.tile0 * kT;
+                    if (c < P.n) {
+                        const uint8_t f = A.flags[base + c];
+                        const double *r = A.r + base, *dv = dold + base;
+                        auto dn = [&](uint32_t kk) -> double {
+                            const uint8_t g = A.flags[base + kk];
+                            const double z = (g & F_ID) ? r[kk] : r[kk] * inv_diag(P, g) - mz;
+                            return first ? z : z + beta * dv[kk];
+                        };
+                        const double dc = dn(c);
+                        double qc = dc;
+                        if (!(f & F_ID)) {
+                            double dg = 0.0, sum = 0.0;
+                            if (f & F_E) { dg += P.cx; sum += P.cx * dn(c + 1); }
+                            if (f & F_W) { dg += P.cx; sum += P.cx * dn(c - 1); }
+                            if (f & F_N) { dg += P.cy; sum += P.cy * dn(c + P.nx); }
+                            if (f & F_S) { dg += P.cy; sum += P.cy * dn(c - P.nx); }
+                            qc = dg * dc - sum;
+                        }
+                        dnew[base + c] = dc;
+                        A.q[base + c] = qc;
+                        s[0] = dc * qc;
+                    }
+                }
+                block_reduce<1, 1>(s, m);
+                if (tid == 0) {
+                    A.pa[(size_t)t * 2] = s[0];
+                    A.pa[(size_t)t * 2 + 1] = 0.0;
+                }
+                __syncthreads();
+            }
+        }
+        gg.sync();
+        // phase B
+        {
+            uint32_t cur = ~0u;
+            bool skip = false;
+            double alpha = 0.0;
+            for (uint32_t t = t_lo; t < t_hi; ++t) {
+                const uint32_t pi = A.tile_prob[t];
+                const Prob &P = A.probs[pi];
+                if (pi != cur) {
+                    cur = pi;
+                    skip = A.fin[par * A.nprob + pi] != 0;
+                    if (!skip) {
+                        double s[1], m[1];
+                        fold_partials<1, 1>(A.pa + (size_t)P.tile0 * 2, P.ntiles, s, m);
+                        const double den = s[0];
+                        int status = ST_OK;
+                        alpha = 0.0;
+                        if (fabs(den) < 1.0e-30 || !isfinite(den)) status = ST_SINGULAR;
+                        else {
+                            alpha = A.sa[pi].rz / den;
+                            if (!isfinite(alpha)) status = ST_ALPHA;
+                        }
+                        if (b == P.owner && tid == 0) {
+                            PcgB &sb = A.sb[pi];
+                            sb.rz_prev = A.sa[pi].rz;
+                            sb.alpha = alpha;
+                            sb.first = 0;
+                            sb.status = status;
+                            if (status == ST_OK) sb.iters += 1;
+                            else A.fin[par * A.nprob + pi] = 1;
+                        }
+                        skip = status != ST_OK;
+                    }
+                }
+                double s[4] = {0, 0, 0, 0}, m[1] = {0.0};
+                if (!skip) {
+                    const uint32_t c = (t - P.tile0) * kT + tid;
+                    const size_t base = (size_t)P.tile0 * kT;
+                    if (c < P.n) {
+                        const uint8_t f = A.flags[base + c];
+                        A.x[base + c] += alpha * dnew[base + c];
+                        const double rc = A.r[base + c] - alpha * A.q[base + c];
+                        A.r[base + c] = rc;
+                        if (!(f & F_ID)) {
+                            const double z = rc * inv_diag(P, f);
+                            s[0] += rc * z;
+                            s[1] += z;
+                            s[2] += rc;
+                        } else {
+                            s[0] += rc * rc;
+                        }
+                        s[3] += rc * rc;
+                        m[0] = fmax(m[0], fabs(rc));
+                    }
+                }
+                block_reduce<4, 1>(s, m);
+                if (tid == 0) {
+                    double *pb = A.pb + (size_t)t * 5;
+                    pb[0] = s[0]; pb[1] = s[1]; pb[2] = s[2]; pb[3] = s[3]; pb[4] = m[0];
+                }
+                __syncthreads();
+            }
+        }
+        gg.sync();
+        bool all = true;
+        for (uint32_t p = 0; p < A.nprob; ++p) all = all && A.fin[par * A.nprob + p];
+        if (all) break;
+    }
+    // ---- final: project x to the zero-mean gauge (or write identity values)
+    for (uint32_t t = t_lo; t < t_hi; ++t) {
+        const uint32_t pi = A.tile_prob[t];
+        double s[2] = {0.0, 0.0}, m[1] = {0.0};
+        if (A.active[pi]) {
+            const Prob &P = A.probs[pi];
+            const uint32_t c = (t - P.tile0) * kT + tid;
+            const size_t base = (size_t)P.tile0 * kT;
+            if (c < P.n && !(A.flags[base + c] & F_ID)) s[0] = A.x[base + c];
+        }
+        block_reduce<2, 1>(s, m);
+        if (tid == 0) {
+            A.p0[(size_t)t * 3] = s[0];
+            A.p0[(size_t)t * 3 + 1] = 0.0;
+            A.p0[(size_t)t * 3 + 2] = 0.0;
+        }
+        __syncthreads();
+    }
+    gg.sync();
+    uint32_t cur = ~0u;
+    double mean_x = 0.0;
+    for (uint32_t t = t_lo; t < t_hi; ++t) {
+        const uint32_t pi = A.tile_prob[t];
+        const Prob &P = A.probs[pi];
+        if (!A.active[pi]) continue;
+        if (pi != cur) {
+            cur = pi;
+            double f[3];
+            fold_tiles<3>(A.p0, P.tile0, P.ntiles, 2, f);
+            mean_x = P.zm ? f[0] * P.inv_count : 0.0;
+        }
+        const uint32_t c = (t - P.tile0) * kT + tid;
+        const size_t base = (size_t)P.tile0 * kT;
+        if (c < P.n) {
+            const uint8_t f = A.flags[base + c];
+            if (f & F_ID) A.x[base + c] = (f & F_OUTLET) ? P.pout : 0.0;
+            else A.x[base + c] -= mean_x;
+        }
+    }
+}
+
+}  // namespace g2
+}  // namespace fl
+
+// =========================================================== host driver
+using namespace fl;
+using namespace fl::g2;
+
+namespace {
+
+struct Batch2 {
+    fl_ctx *ctx = nullptr;
+    int count = 0;
+    std::vector<fl_grid2d_config> cfg;
+    std::vector<Prob> hp;
+    uint32_t ntiles = 0;
+    size_t ncell = 0, nuf = 0, nvf = 0;
+    Prob *probs = nullptr;
+    uint32_t *tile_prob = nullptr;
+    uint8_t *flags = nullptr;
+    double *x = nullptr, *rhs = nullptr, *r = nullptr, *d0 = nullptr, *d1 = nullptr, *q = nullptr;
+    double *uc = nullptr, *vc = nullptr, *sp = nullptr, *vort = nullptr;
+    double *u = nullptr, *v = nullptr, *us = nullptr, *vs = nullptr, *uo = nullptr, *vo = nullptr;
+    double *pa = nullptr, *pb = nullptr, *p0 = nullptr, *dpart = nullptr;
+    PcgA *sa = nullptr;
+    PcgB *sb = nullptr;
+    int *fin = nullptr, *active = nullptr;
+    double *dts = nullptr;
+    Diag *diag = nullptr;
+    // pinned host mirrors
+    int *h_active = nullptr, *h_fin = nullptr;
+    double *h_dts = nullptr;
+    PcgA *h_sa = nullptr;
+    PcgB *h_sb = nullptr;
+    Diag *h_diag = nullptr;
+    // host-side solver state per grid
+    std::vector<int64_t> step;
+    std::vector<double> time, current_dt, max_u, max_v;
+    uint32_t pcg_blocks = 1;
+    long long kmax = 1;
+};
+
+template <typename T>
+fl_status dalloc(Batch2 *B, T **p, size_t n) {
+    FL_HIP_TRY(B->ctx, hipMalloc(p, (n > 0 ? n : 1) * sizeof(T)));
+    FL_HIP_TRY(B->ctx, hipMemsetAsync(*p, 0, (n > 0 ? n : 1) * sizeof(T), B->ctx->stream));
+    return FL_OK;
+}
+template <typename T>
+fl_status halloc(Batch2 *B, T **p, size_t n) {
+    if (hipHostMalloc(p, (n > 0 ? n : 1) * sizeof(T), hipHostMallocDefault) != hipSuccess)
+        return set_error(B->ctx, FL_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
+    std::memset(*p, 0, (n > 0 ? n : 1) * sizeof(T));
+    return FL_OK;
+}
+
+void free_batch(Batch2 *B) {
+    void *d[] = {B->probs, B->tile_prob, B->flags, B->x,  B->rhs, B->r,  B->d0,    B->d1,
+                 B->q,     B->uc,        B->vc,    B->sp, B->vort, B->u, B->v,     B->us,
+                 B->vs,    B->uo,        B->vo,    B->pa, B->pb,   B->p0, B->dpart, B->sa,
+                 B->sb,    B->fin,       B->active, B->dts, B->diag};
+    for (void *p : d)
+        if (p) (void)hipFree(p);
+    void *h[] = {B->h_active, B->h_fin, B->h_dts, B->h_sa, B->h_sb, B->h_diag};
+    for (void *p : h)
+        if (p) (void)hipHostFree(p);
+}
+
+// SimulationConfig::validate (incompressible.rs:93-191), flow-only parts.
+fl_status validate2(fl_ctx *ctx, const fl_grid2d_config &c) {
+    if (c.nx < 4 || c.ny < 4) return set_error(ctx, FL_ERR_CONFIG, "nx and ny must be at least 4");
+    if (!(c.length > 0.0 && c.height > 0.0))
+        return set_error(ctx, FL_ERR_CONFIG, "Domain dimensions must be positive");
+    if (!std::isfinite(c.dt) || c.dt <= 0.0)
+        return set_error(ctx, FL_ERR_CONFIG, "dt must be finite and positive");
+    if (!std::isfinite(c.t_end) || c.t_end < 0.0)
+        return set_error(ctx, FL_ERR_CONFIG, "t_end must be finite and non-negative");
+    if (c.t_end == 0.0)
+        return set_error(ctx, FL_ERR_CONFIG, "t_end must be positive for projection coupling");
+    if (c.pressure_max_iters <= 0)
+        return set_error(ctx, FL_ERR_CONFIG, "pressure_max_iters must be positive");
+    if (!(std::isfinite(c.pressure_tolerance) && c.pressure_tolerance > 0.0))
+        return set_error(ctx, FL_ERR_CONFIG, "pressure_tolerance must be finite and positive");
+    if (!(std::isfinite(c.density) && c.density > 0.0 && std::isfinite(c.kinematic_viscosity) &&
+          c.kinematic_viscosity > 0.0))
+        return set_error(ctx, FL_ERR_CONFIG,
+                         "fluid density and kinematic viscosity must be finite and positive");
+    if (c.convection != FL_UPWIND && c.convection != FL_CENTRAL)
+        return set_error(ctx, FL_ERR_CONFIG, "unknown convection scheme %d", c.convection);
+    if (c.stop_rule != FL_STOP_ABS_INF && c.stop_rule != FL_STOP_REL_L2)
+        return set_error(ctx, FL_ERR_CONFIG, "unknown stop rule %d", c.stop_rule);
+    if (c.initial_guess != FL_GUESS_PREVIOUS && c.initial_guess != FL_GUESS_ZERO)
+        return set_error(ctx, FL_ERR_CONFIG, "unknown initial guess %d", c.initial_guess);
+    for (int s = 0; s < 4; ++s)
+        if (c.bc_kind[s] < FL_BC_VELOCITY || c.bc_kind[s] > FL_BC_OUTLET)
+            return set_error(ctx, FL_ERR_CONFIG, "unknown boundary kind %d", c.bc_kind[s]);
+    if (c.adaptive) {
+        if (!std::isfinite(c.min_dt) || c.min_dt <= 0.0)
+            return set_error(ctx, FL_ERR_CONFIG, "adaptive min_dt must be finite and positive");
+        if (!std::isfinite(c.max_dt) || c.max_dt <= 0.0)
+            return set_error(ctx, FL_ERR_CONFIG, "adaptive max_dt must be finite and positive");
+        if (c.min_dt > c.max_dt)
+            return set_error(ctx, FL_ERR_CONFIG, "adaptive min_dt must not exceed max_dt");
+        if (c.dt < c.min_dt || c.dt > c.max_dt)
+            return set_error(ctx, FL_ERR_CONFIG, "adaptive initial dt must lie between min_dt and max_dt");
+        if (!std::isfinite(c.target_cfl) || c.target_cfl <= 0.0 || c.target_cfl > 1.0)
+            return set_error(ctx, FL_ERR_CONFIG, "adaptive target_cfl must lie between 0 (exclusive) and 1");
+        if (!std::isfinite(c.target_diffusion_number) || c.target_diffusion_number <= 0.0 ||
+            c.target_diffusion_number > 0.5)
+            return set_error(ctx, FL_ERR_CONFIG,
+                             "adaptive target_diffusion_number must lie between 0 (exclusive) and 0.5");
+        if (!std::isfinite(c.max_growth_factor) || c.max_growth_factor < 1.0)
+            return set_error(ctx, FL_ERR_CONFIG, "adaptive max_growth_factor must be finite and at least 1");
+    }
+    if ((double)c.nx * (double)c.ny > 1.0e9)
+        return set_error(ctx, FL_ERR_CONFIG, "grid too large");
+    return FL_OK;
+}
+
+// adaptive_time_step (incompressible.rs:2308-2336), flow only.
+fl_status adaptive_dt(fl_ctx *ctx, const fl_grid2d_config &c, double current, double umax,
+                      double vmax, double dx, double dy, double *out) {
+    const double rate = umax / dx + vmax / dy;
+    const double cfl_dt = rate > 2.220446049250313e-16 ? c.target_cfl / rate : c.max_dt;
+    const double inv = 1.0 / (dx * dx) + 1.0 / (dy * dy);
+    const double diff_dt = c.target_diffusion_number / (c.kinematic_viscosity * inv);
+    const double stable = std::min(std::min(c.max_dt, cfl_dt), diff_dt);
+    if (stable < c.min_dt)
+        return set_error(ctx, FL_ERR_SOLVER,
+                         "required stable time step %.3e is below configured min_dt %.3e; CFL "
+                         "constraint=%.3e, diffusion constraint=%.3e",
+                         stable, c.min_dt, cfl_dt, diff_dt);
+    const double g = std::min(stable, current * c.max_growth_factor);
+    *out = std::min(std::max(g, c.min_dt), c.max_dt);
+    return FL_OK;
+}
+
+bool reached_end(const Batch2 *B, int p) {
+    const double t_end = B->cfg[p].t_end;
+    const double tol = 2.220446049250313e-16 * std::max(std::fabs(t_end), 1.0) * 16.0;
+    return B->time[p] >= t_end - tol;
+}
+
+fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
+                const uint8_t *const *solids) {
+    B->ctx = ctx;
+    B->count = count;
+    B->cfg.assign(cfgs, cfgs + count);
+    fl_status st;
+    for (int p = 0; p < count; ++p)
+        if ((st = validate2(ctx, cfgs[p])) != FL_OK) return st;
+    // pack grids into tiles and faces
+    uint32_t tile = 0;
+    size_t uo = 0, vo = 0;
+    B->hp.resize(count);
+    std::vector<uint8_t> hflags;
+    std::vector<uint32_t> htp;
+    for (int p = 0; p < count; ++p) {
+        const fl_grid2d_config &c = cfgs[p];
+        Prob &P = B->hp[p];
+        std::memset(&P, 0, sizeof(P));
+        P.nx = (uint32_t)c.nx;
+        P.ny = (uint32_t)c.ny;
+        P.n = P.nx * P.ny;
+        P.tile0 = tile;
+        P.ntiles = (P.n + kT - 1) / kT;
+        P.uoff = (uint32_t)uo;
+        P.voff = (uint32_t)vo;
+        P.dx = c.length / c.nx;
+        P.dy = c.height / c.ny;
+        P.cx = 1.0 / (P.dx * P.dx);
+        P.cy = 1.0 / (P.dy * P.dy);
+        P.nu = c.kinematic_viscosity;
+        P.rho = c.density;
+        P.scheme = c.convection;
+        for (int s = 0; s < 4; ++s) {
+            P.kind[s] = c.bc_kind[s];
+            // SolverBoundaryOverrides::velocity returns (0, 0) for Symmetry and
+            // PressureOutlet sides (src/preprocess.rs:1384-1391)
+            P.bu[s] = c.bc_kind[s] == FL_BC_VELOCITY ? c.bc_u[s] : 0.0;
+            P.bv[s] = c.bc_kind[s] == FL_BC_VELOCITY ? c.bc_v[s] : 0.0;
+        }
+        const bool outlet = c.bc_kind[FL_SIDE_RIGHT] == FL_BC_OUTLET;
+        P.pout = outlet ? c.outlet_pressure[FL_SIDE_RIGHT] : 0.0;
+        P.zm = outlet ? 0 : 1;
+        P.tol = c.pressure_tolerance;
+        P.max_iter = c.pressure_max_iters;
+        P.stop_rule = c.stop_rule;
+        P.guess = c.initial_guess;
+        P.fx = FastDiv(P.nx);
+        const uint8_t *sol = solids ? solids[p] : nullptr;
+        auto solid = [&](uint32_t i, uint32_t j) { return sol && sol[i + P.nx * j]; };
+        hflags.resize((size_t)(tile + P.ntiles) * kT, 0);
+        uint64_t mean_cells = 0;
+        for (uint32_t j = 0; j < P.ny; ++j)
+            for (uint32_t i = 0; i < P.nx; ++i) {
+                uint8_t f = 0;
+                const bool s = solid(i, j);
+                const bool out = outlet && i + 1 == P.nx && !s;  // pressure_outlet_cell
+                if (i + 1 < P.nx && !solid(i + 1, j)) f |= F_E;
+                if (i > 0 && !solid(i - 1, j)) f |= F_W;
+                if (j + 1 < P.ny && !solid(i, j + 1)) f |= F_N;
+                if (j > 0 && !solid(i, j - 1)) f |= F_S;
+                if (s) f |= F_SOLID | F_ID;
+                if (out) f |= F_OUTLET | F_ID;
+                if (!(f & F_ID)) ++mean_cells;
+                hflags[(size_t)tile * kT + i + P.nx * j] = f;
+            }
+        P.inv_count = mean_cells ? 1.0 / (double)mean_cells : 0.0;
+        for (uint32_t t = 0; t < P.ntiles; ++t) htp.push_back(p);
+        tile += P.ntiles;
+        uo += (size_t)(P.nx + 1) * P.ny;
+        vo += (size_t)P.nx * (P.ny + 1);
+    }
+    B->ntiles = tile;
+    B->ncell = (size_t)tile * kT;
+    B->nuf = uo;
+    B->nvf = vo;
+    // persistent-kernel grid: all blocks co-resident (cooperative launch)
+    int per_cu = 0;
+    FL_HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg, kT, 0));
+    uint64_t G = (uint64_t)std::max(per_cu, 1) * ctx->num_cu;
+    G = std::min<uint64_t>(G, (uint64_t)ctx->num_cu * 2);
+    G = std::min<uint64_t>(G, B->ntiles);
+    B->pcg_blocks = (uint32_t)std::max<uint64_t>(G, 1);
+    for (int p = 0; p < count; ++p) {
+        Prob &P = B->hp[p];
+        uint32_t owner = 0;
+        for (uint32_t b = 0; b < B->pcg_blocks; ++b)
+            if (range_lo(b, B->ntiles, B->pcg_blocks) <= P.tile0) owner = b;
+        P.owner = owner;
+    }
+    if ((st = dalloc(B, &B->probs, count)) || (st = dalloc(B, &B->tile_prob, B->ntiles)) ||
+        (st = dalloc(B, &B->flags, B->ncell)) || (st = dalloc(B, &B->x, B->ncell)) ||
+        (st = dalloc(B, &B->rhs, B->ncell)) || (st = dalloc(B, &B->r, B->ncell)) ||
+        (st = dalloc(B, &B->d0, B->ncell)) || (st = dalloc(B, &B->d1, B->ncell)) ||
+        (st = dalloc(B, &B->q, B->ncell)) || (st = dalloc(B, &B->uc, B->ncell)) ||
+        (st = dalloc(B, &B->vc, B->ncell)) || (st = dalloc(B, &B->sp, B->ncell)) ||
+        (st = dalloc(B, &B->vort, B->ncell)) || (st = dalloc(B, &B->u, B->nuf)) ||
+        (st = dalloc(B, &B->v, B->nvf)) || (st = dalloc(B, &B->us, B->nuf)) ||
+        (st = dalloc(B, &B->vs, B->nvf)) || (st = dalloc(B, &B->uo, B->nuf)) ||
+        (st = dalloc(B, &B->vo, B->nvf)) || (st = dalloc(B, &B->pa, (size_t)B->ntiles * 2)) ||
+        (st = dalloc(B, &B->pb, (size_t)B->ntiles * 5)) ||
+        (st = dalloc(B, &B->p0, (size_t)B->ntiles * 3)) ||
+        (st = dalloc(B, &B->dpart, (size_t)B->ntiles * 6)) || (st = dalloc(B, &B->sa, count)) ||
+        (st = dalloc(B, &B->sb, count)) || (st = dalloc(B, &B->fin, 2 * count)) ||
+        (st = dalloc(B, &B->active, count)) || (st = dalloc(B, &B->dts, count)) ||
+        (st = dalloc(B, &B->diag, count)))
+        return st;
+    if ((st = halloc(B, &B->h_active, count)) || (st = halloc(B, &B->h_fin, 2 * count)) ||
+        (st = halloc(B, &B->h_dts, count)) || (st = halloc(B, &B->h_sa, count)) ||
+        (st = halloc(B, &B->h_sb, count)) || (st = halloc(B, &B->h_diag, count)))
+        return st;
+    hipStream_t s = ctx->stream;
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->probs, B->hp.data(), count * sizeof(Prob), hipMemcpyHostToDevice, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->tile_prob, htp.data(), htp.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->flags, hflags.data(), hflags.size(), hipMemcpyHostToDevice, s));
+    B->step.assign(count, 0);
+    B->time.assign(count, 0.0);
+    B->current_dt.resize(count);
+    B->max_u.assign(count, 0.0);
+    B->max_v.assign(count, 0.0);
+    for (int p = 0; p < count; ++p) {
+        B->current_dt[p] = cfgs[p].dt;
+        B->h_active[p] = 1;
+    }
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->active, B->h_active, count * sizeof(int), hipMemcpyHostToDevice, s));
+    // initialize_velocity (:588-620): zero field, then boundary values at t = 0
+    k_boundaries<<<count, kT, 0, s>>>(B->probs, B->active, B->u, B->v);
+    FL_HIP_TRY(ctx, hipGetLastError());
+    FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+    return FL_OK;
+}
+
+uint32_t ew_blocks(const Batch2 *B) { return std::min<uint32_t>(B->ntiles, B->ctx->num_cu * 16); }
+
+// Cell fields and per-grid diagnostics into h_diag (after a synchronise).
+fl_status run_cells(Batch2 *B) {
+    fl_ctx *ctx = B->ctx;
+    hipStream_t s = ctx->stream;
+    for (int p = 0; p < B->count; ++p) {
+        B->h_diag[p] = Diag{};
+        for (int f = 0; f < 4; ++f) B->h_diag[p].bad[f] = ~0ull;
+    }
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->diag, B->h_diag, B->count * sizeof(Diag), hipMemcpyHostToDevice, s));
+    k_cells<<<ew_blocks(B), kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->flags, B->x,
+                                        B->u, B->v, B->uo, B->vo, B->uc, B->vc, B->sp, B->vort,
+                                        B->dpart, B->diag);
+    k_fold_diag<<<B->count, kT, 0, s>>>(B->probs, B->active, B->dpart, B->diag);
+    FL_HIP_TRY(ctx, hipGetLastError());
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->h_diag, B->diag, B->count * sizeof(Diag), hipMemcpyDeviceToHost, s));
+    FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+    return FL_OK;
+}
+
+// The persistent PCG over the active grids; states land in h_sa / h_sb.
+fl_status run_pcg(Batch2 *B) {
+    fl_ctx *ctx = B->ctx;
+    hipStream_t s = ctx->stream;
+    long long kmax = 1;
+    for (int p = 0; p < B->count; ++p) {
+        B->h_fin[p] = 0;
+        B->h_fin[B->count + p] = B->h_active[p] ? 0 : 1;
+        B->h_sb[p] = PcgB{};
+        B->h_sb[p].first = 1;
+        if (B->h_active[p]) kmax = std::max<long long>(kmax, B->hp[p].max_iter + 2);
+    }
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->fin, B->h_fin, 2 * B->count * sizeof(int), hipMemcpyHostToDevice, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->sb, B->h_sb, B->count * sizeof(PcgB), hipMemcpyHostToDevice, s));
+    PcgArgs a{B->probs, B->tile_prob, B->ntiles, (uint32_t)B->count, kmax, B->flags, B->rhs,
+              B->x, B->r, B->d0, B->d1, B->q, B->pa, B->pb, B->p0, B->sa, B->sb,
+              B->active, B->fin};
+    void *args[] = {&a};
+    FL_HIP_TRY(ctx, hipLaunchCooperativeKernel((const void *)k_pcg, dim3(B->pcg_blocks), dim3(kT),
+                                               args, 0, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->h_sa, B->sa, B->count * sizeof(PcgA), hipMemcpyDeviceToHost, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->h_sb, B->sb, B->count * sizeof(PcgB), hipMemcpyDeviceToHost, s));
+    FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+    return FL_OK;
+}
+
+const char *pcg_text(int status) {
+    switch (status) {
+        case ST_SINGULAR: return "PCG pressure solver encountered a singular search direction";
+        case ST_ALPHA: return "PCG pressure solver produced a non-finite step length";
+        case ST_RESIDUAL: return "PCG pressure solver produced a non-finite residual";
+        default: return "";
+    }
+}
+
+fl_status pcg_status(Batch2 *B, int p) {
+    const int st = B->h_sb[p].status != ST_OK ? B->h_sb[p].status : B->h_sa[p].status;
+    if (st != ST_OK) return set_error(B->ctx, FL_ERR_SOLVER, "%s", pcg_text(st));
+    return FL_OK;
+}
+
+// advance_one_step (incompressible.rs:622-746) for every grid that has not
+// reached t_end.  `strict` makes a grid at t_end an error (single solver).
+fl_status step_batch(Batch2 *B, fl_grid2d_step_report *reps, bool strict) {
+    fl_ctx *ctx = B->ctx;
+    hipStream_t s = ctx->stream;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int nact = 0;
+    for (int p = 0; p < B->count; ++p) {
+        const fl_grid2d_config &c = B->cfg[p];
+        B->h_active[p] = 0;
+        if (reached_end(B, p)) {
+            if (strict) return set_error(ctx, FL_ERR_SOLVER, "projection simulation has already reached t_end");
+            continue;
+        }
+        double dt = c.dt;
+        if (c.adaptive) {
+            fl_status st = adaptive_dt(ctx, c, B->current_dt[p], B->max_u[p], B->max_v[p],
+                                       B->hp[p].dx, B->hp[p].dy, &dt);
+            if (st != FL_OK) return st;
+        }
+        dt = std::min(dt, c.t_end - B->time[p]);
+        if (dt <= 0.0) {
+            if (strict) return set_error(ctx, FL_ERR_SOLVER, "projection simulation has already reached t_end");
+            continue;
+        }
+        if (c.adaptive) B->current_dt[p] = dt;
+        B->h_dts[p] = dt;
+        B->h_active[p] = 1;
+        ++nact;
+    }
+    if (nact == 0) return FL_OK;
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->active, B->h_active, B->count * sizeof(int), hipMemcpyHostToDevice, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->dts, B->h_dts, B->count * sizeof(double), hipMemcpyHostToDevice, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->uo, B->u, B->nuf * sizeof(double), hipMemcpyDeviceToDevice, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->vo, B->v, B->nvf * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const uint32_t G = ew_blocks(B);
+    k_predict<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->dts, B->flags,
+                               B->u, B->v, B->us, B->vs);
+    k_boundaries<<<B->count, kT, 0, s>>>(B->probs, B->active, B->us, B->vs);
+    k_rhs<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->dts, B->flags, B->us,
+                           B->vs, B->rhs);
+    FL_HIP_TRY(ctx, hipGetLastError());
+    fl_status st = run_pcg(B);
+    if (st != FL_OK) return st;
+    for (int p = 0; p < B->count; ++p)
+        if (B->h_active[p] && (st = pcg_status(B, p)) != FL_OK) return st;
+    k_correct<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->dts, B->flags, B->x,
+                               B->us, B->vs, B->u, B->v);
+    k_boundaries<<<B->count, kT, 0, s>>>(B->probs, B->active, B->u, B->v);
+    FL_HIP_TRY(ctx, hipGetLastError());
+    if ((st = run_cells(B)) != FL_OK) return st;
+    for (int p = 0; p < B->count; ++p) {
+        if (!B->h_active[p]) continue;
+        const Prob &P = B->hp[p];
+        const Diag &dg = B->h_diag[p];
+        B->step[p] += 1;
+        B->time[p] += B->h_dts[p];
+        // ensure_finite (:1707-1742): pressure, u, v, speed in that order
+        const char *names[4] = {"pressure", "u", "v", "speed"};
+        double *arrs[4] = {B->x + (size_t)P.tile0 * kT, B->u + P.uoff, B->v + P.voff,
+                           B->sp + (size_t)P.tile0 * kT};
+        for (int f = 0; f < 4; ++f)
+            if (dg.bad[f] != ~0ull) {
+                double val = 0.0;
+                FL_HIP_TRY(ctx, hipMemcpy(&val, arrs[f] + dg.bad[f], sizeof(double), hipMemcpyDeviceToHost));
+                return set_error(ctx, FL_ERR_SOLVER, "Non-finite %s value at flat index %llu: %g (step %lld)",
+                                 names[f], dg.bad[f], val, (long long)B->step[p]);
+            }
+        B->max_u[p] = dg.max_u;
+        B->max_v[p] = dg.max_v;
+        if (reps) {
+            fl_grid2d_step_report &r = reps[p];
+            const double dt = B->h_dts[p];
+            r.iteration = B->step[p];
+            r.time = B->time[p];
+            r.time_step = dt;
+            r.continuity_residual = dg.max_div;
+            r.pressure_residual = B->h_sa[p].rmax;
+            r.pressure_iterations = B->h_sb[p].iters;
+            r.momentum_residual = std::max(dg.du, dg.dv);
+            r.max_speed = dg.max_speed;
+            r.momentum_cfl = dt * (dg.max_u / P.dx + dg.max_v / P.dy);
+            r.viscous_diffusion_number = P.nu * dt * (1.0 / (P.dx * P.dx) + 1.0 / (P.dy * P.dy));
+            r.pressure_converged = B->h_sa[p].converged;
+            r.reached_end_time = reached_end(B, p);
+        }
+    }
+    return FL_OK;
+}
+
+fl_status create_common(fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
+                        const uint8_t *const *solids, Batch2 **out) {
+    if (!ctx || !cfgs || !out || count < 1)
+        return set_error(ctx, FL_ERR_INVALID_ARGUMENT, "null argument or empty batch");
+    *out = nullptr;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    Batch2 *B = new (std::nothrow) Batch2();
+    if (!B) return set_error(ctx, FL_ERR_OUT_OF_MEMORY, "host allocation failed");
+    fl_status st = build(B, ctx, count, cfgs, solids);
+    if (st == FL_OK) st = run_cells(B);  // cell fields and |u|, |v| of the initial state
+    if (st != FL_OK) {
+        free_batch(B);
+        delete B;
+        return st;
+    }
+    ctx_retain(ctx);
+    *out = B;
+    return FL_OK;
+}
+
+void destroy_common(Batch2 *B) {
+    if (!B) return;
+    fl_ctx *ctx = B->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_batch(B);
+    delete B;
+    ctx_release(ctx);
+}
+
+fl_status get_fields(Batch2 *B, int p, double *u, double *v, double *pp) {
+    fl_ctx *ctx = B->ctx;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const Prob &P = B->hp[p];
+    hipStream_t s = ctx->stream;
+    if (u) FL_HIP_TRY(ctx, hipMemcpyAsync(u, B->u + P.uoff, (size_t)(P.nx + 1) * P.ny * 8, hipMemcpyDeviceToHost, s));
+    if (v) FL_HIP_TRY(ctx, hipMemcpyAsync(v, B->v + P.voff, (size_t)P.nx * (P.ny + 1) * 8, hipMemcpyDeviceToHost, s));
+    if (pp) FL_HIP_TRY(ctx, hipMemcpyAsync(pp, B->x + (size_t)P.tile0 * kT, (size_t)P.n * 8, hipMemcpyDeviceToHost, s));
+    FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+    return FL_OK;
+}
+
+}  // namespace
+
+struct fl_grid2d {
+    Batch2 *b;
+};
+struct fl_batch2d {
+    Batch2 *b;
+};
+
+extern "C" {
+
+fl_status fl_grid2d_create(fl_ctx *ctx, const fl_grid2d_config *cfg, const uint8_t *solid,
+                           fl_grid2d **out) {
+    if (!out) return set_error(ctx, FL_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    const uint8_t *solids[1] = {solid};
+    Batch2 *B = nullptr;
+    fl_status st = create_common(ctx, 1, cfg, solids, &B);
+    if (st != FL_OK) return st;
+    fl_grid2d *g = new (std::nothrow) fl_grid2d{B};
+    if (!g) {
+        destroy_common(B);
+        return set_error(ctx, FL_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    *out = g;
+    return FL_OK;
+}
+
+void fl_grid2d_destroy(fl_grid2d *g) {
+    if (!g) return;
+    destroy_common(g->b);
+    delete g;
+}
+
+fl_status fl_grid2d_step(fl_grid2d *g, fl_grid2d_step_report *rep) {
+    if (!g) return FL_ERR_INVALID_ARGUMENT;
+    return step_batch(g->b, rep, true);
+}
+
+fl_status fl_grid2d_pressure_solve(fl_grid2d *g, const double *rhs, double *p_inout,
+                                   fl_solve_report *rep) {
+    if (!g || !rhs || !p_inout) return FL_ERR_INVALID_ARGUMENT;
+    Batch2 *B = g->b;
+    fl_ctx *ctx = B->ctx;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const Prob &P = B->hp[0];
+    for (uint32_t c = 0; c < P.n; ++c)
+        if (!std::isfinite(rhs[c]))
+            return set_error(ctx, FL_ERR_NON_FINITE, "non-finite right-hand side value at index %u", c);
+    hipStream_t s = ctx->stream;
+    B->h_active[0] = 1;
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->active, B->h_active, sizeof(int), hipMemcpyHostToDevice, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->rhs, rhs, (size_t)P.n * 8, hipMemcpyHostToDevice, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->x, p_inout, (size_t)P.n * 8, hipMemcpyHostToDevice, s));
+    fl_status st = run_pcg(B);
+    if (st != FL_OK) return st;
+    FL_HIP_TRY(ctx, hipMemcpyAsync(p_inout, B->x, (size_t)P.n * 8, hipMemcpyDeviceToHost, s));
+    FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (rep) {
+        rep->iterations = B->h_sb[0].iters;
+        rep->refinements = 0;
+        rep->fallback = 0;
+        rep->fallback_iterations = 0;
+        rep->residual_abs_inf = B->h_sa[0].rmax;
+        rep->residual_l2 = std::sqrt(B->h_sa[0].rr);
+        rep->initial_l2 = std::sqrt(B->h_sa[0].rr0);
+        rep->converged = B->h_sa[0].converged;
+        rep->status = FL_OK;
+    }
+    st = pcg_status(B, 0);
+    if (rep && st != FL_OK) rep->status = st;
+    return st;
+}
+
+fl_status fl_grid2d_get_fields(fl_grid2d *g, double *u, double *v, double *p) {
+    if (!g) return FL_ERR_INVALID_ARGUMENT;
+    return get_fields(g->b, 0, u, v, p);
+}
+
+fl_status fl_grid2d_set_fields(fl_grid2d *g, const double *u, const double *v, const double *p) {
+    if (!g) return FL_ERR_INVALID_ARGUMENT;
+    Batch2 *B = g->b;
+    fl_ctx *ctx = B->ctx;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const Prob &P = B->hp[0];
+    hipStream_t s = ctx->stream;
+    if (u) FL_HIP_TRY(ctx, hipMemcpyAsync(B->u + P.uoff, u, (size_t)(P.nx + 1) * P.ny * 8, hipMemcpyHostToDevice, s));
+    if (v) FL_HIP_TRY(ctx, hipMemcpyAsync(B->v + P.voff, v, (size_t)P.nx * (P.ny + 1) * 8, hipMemcpyHostToDevice, s));
+    if (p) FL_HIP_TRY(ctx, hipMemcpyAsync(B->x + (size_t)P.tile0 * kT, p, (size_t)P.n * 8, hipMemcpyHostToDevice, s));
+    B->h_active[0] = 1;
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->active, B->h_active, sizeof(int), hipMemcpyHostToDevice, s));
+    fl_status st = run_cells(B);
+    if (st == FL_OK) {
+        B->max_u[0] = B->h_diag[0].max_u;
+        B->max_v[0] = B->h_diag[0].max_v;
+    }
+    return st;
+}
+
+fl_status fl_grid2d_cell_fields(fl_grid2d *g, double *u_cell, double *v_cell, double *speed,
+                                double *vorticity) {
+    if (!g) return FL_ERR_INVALID_ARGUMENT;
+    Batch2 *B = g->b;
+    fl_ctx *ctx = B->ctx;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const Prob &P = B->hp[0];
+    hipStream_t s = ctx->stream;
+    const size_t base = (size_t)P.tile0 * kT, bytes = (size_t)P.n * 8;
+    if (u_cell) FL_HIP_TRY(ctx, hipMemcpyAsync(u_cell, B->uc + base, bytes, hipMemcpyDeviceToHost, s));
+    if (v_cell) FL_HIP_TRY(ctx, hipMemcpyAsync(v_cell, B->vc + base, bytes, hipMemcpyDeviceToHost, s));
+    if (speed) FL_HIP_TRY(ctx, hipMemcpyAsync(speed, B->sp + base, bytes, hipMemcpyDeviceToHost, s));
+    if (vorticity) FL_HIP_TRY(ctx, hipMemcpyAsync(vorticity, B->vort + base, bytes, hipMemcpyDeviceToHost, s));
+    FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+    return FL_OK;
+}
+
+fl_status fl_batch2d_create(fl_ctx *ctx, int32_t count, const fl_grid2d_config *cfgs,
+                            fl_batch2d **out) {
+    if (!out) return set_error(ctx, FL_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    Batch2 *B = nullptr;
+    fl_status st = create_common(ctx, count, cfgs, nullptr, &B);
+    if (st != FL_OK) return st;
+    fl_batch2d *b = new (std::nothrow) fl_batch2d{B};
+    if (!b) {
+        destroy_common(B);
+        return set_error(ctx, FL_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    *out = b;
+    return FL_OK;
+}
+
+void fl_batch2d_destroy(fl_batch2d *b) {
+    if (!b) return;
+    destroy_common(b->b);
+    delete b;
+}
+
+fl_status fl_batch2d_step(fl_batch2d *b, fl_grid2d_step_report *reps) {
+    if (!b) return FL_ERR_INVALID_ARGUMENT;
+    return step_batch(b->b, reps, false);
+}
+
+fl_status fl_batch2d_get_fields(fl_batch2d *b, int32_t index, double *u, double *v, double *p) {
+    if (!b || index < 0 || index >= b->b->count) return FL_ERR_INVALID_ARGUMENT;
+    return get_fields(b->b, index, u, v, p);
+}
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+"""Host-side mirror of the 2D IncompressibleSolver (projection coupling) over
+the C ABI, plus the batched uneven-cavity driver.
+
+Names follow src/solver/incompressible.rs: ``SimulationConfig`` (:58-90, the
+flow-only knobs), ``TimeStepSettings`` (:35-56), ``ConvectionScheme``
+(:14-18), ``IncompressibleSolver.advance`` (:523-555) returning a
+``SolverStep`` (:196-214), ``field_update`` (:563-586).  Boundaries are
+SolverBoundaryOverrides-style per-side kinds with constant velocities
+(src/preprocess.rs:1362-1407); the lid-driven cavity is the default.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (GUESS_PREVIOUS, STOP_ABS_INF, Grid2DConfig, Grid2DStepReport, SolveReport,
+                   check, load, ptr)
+
+FIRST_ORDER_UPWIND, CENTRAL = 0, 1
+BC_VELOCITY, BC_SYMMETRY, BC_PRESSURE_OUTLET = 0, 1, 2
+LEFT, RIGHT, BOTTOM, TOP = 0, 1, 2, 3
+
+
+@dataclass
+class TimeStepSettings:
+    """TimeStepSettings (incompressible.rs:35-56)."""
+
+    adaptive: bool = False
+    min_dt: float = 1.0e-6
+    max_dt: float = 1.0e-2
+    target_cfl: float = 0.5
+    target_diffusion_number: float = 0.25
+    max_growth_factor: float = 1.1
+
+
+@dataclass
+class SimulationConfig:
+    """SimulationConfig (incompressible.rs:58-90), projection coupling.
+
+    The default case is the lid-driven cavity (src/cases/cavity.rs): unit
+    square, lid u = 1 on top, nu = U L / Re."""
+
+    nx: int = 64
+    ny: int = 64
+    length: float = 1.0
+    height: float = 1.0
+    density: float = 1.0
+    kinematic_viscosity: float = 0.01
+    dt: float = 1.0e-3
+    t_end: float = 1.0e30
+    time_step: TimeStepSettings = field(default_factory=TimeStepSettings)
+    convection: int = FIRST_ORDER_UPWIND
+    pressure_max_iters: int = 1200
+    pressure_tolerance: float = 1.0e-5
+    stop_rule: int = STOP_ABS_INF
+    initial_guess: int = GUESS_PREVIOUS
+    bc_kind: tuple = (BC_VELOCITY,) * 4
+    bc_u: tuple = (0.0, 0.0, 0.0, 1.0)
+    bc_v: tuple = (0.0, 0.0, 0.0, 0.0)
+    outlet_pressure: tuple = (0.0, 0.0, 0.0, 0.0)
+
+    @staticmethod
+    def cavity(n, reynolds=100.0, lid_velocity=1.0, **kw):
+        """The lid-driven cavity (CavityCase, src/cases/cavity.rs:11-25)."""
+        kw.setdefault("kinematic_viscosity", lid_velocity * 1.0 / reynolds)
+        kw.setdefault("bc_u", (0.0, 0.0, 0.0, lid_velocity))
+        return SimulationConfig(nx=n, ny=n, **kw)
+
+    def to_c(self) -> Grid2DConfig:
+        c = Grid2DConfig()
+        c.nx, c.ny, c.length, c.height = self.nx, self.ny, self.length, self.height
+        c.density, c.kinematic_viscosity = self.density, self.kinematic_viscosity
+        c.dt, c.t_end = self.dt, self.t_end
+        c.pressure_max_iters, c.pressure_tolerance = self.pressure_max_iters, self.pressure_tolerance
+        c.convection, c.stop_rule, c.initial_guess = self.convection, self.stop_rule, self.initial_guess
+        ts = self.time_step
+        c.adaptive = int(ts.adaptive)
+        for s in range(4):
+            c.bc_kind[s] = self.bc_kind[s]
+            c.bc_u[s] = self.bc_u[s]
+            c.bc_v[s] = self.bc_v[s]
+            c.outlet_pressure[s] = self.outlet_pressure[s]
+        c.min_dt, c.max_dt, c.target_cfl = ts.min_dt, ts.max_dt, ts.target_cfl
+        c.target_diffusion_number, c.max_growth_factor = (ts.target_diffusion_number,
+                                                          ts.max_growth_factor)
+        return c
+
+
+@dataclass
+class SolverStep:
+    """SolverStep (incompressible.rs:196-214), flow-only fields."""
+
+    iteration: int
+    time: float
+    time_step: float
+    continuity_residual: float
+    pressure_residual: float
+    pressure_iterations: int
+    momentum_residual: float
+    max_speed: float
+    momentum_cfl: float
+    viscous_diffusion_number: float
+    pressure_converged: bool
+    reached_end_time: bool
+
+    @staticmethod
+    def from_c(r: Grid2DStepReport) -> "SolverStep":
+        return SolverStep(r.iteration, r.time, r.time_step, r.continuity_residual,
+                          r.pressure_residual, r.pressure_iterations, r.momentum_residual,
+                          r.max_speed, r.momentum_cfl, r.viscous_diffusion_number,
+                          bool(r.pressure_converged), bool(r.reached_end_time))
+
+
+class IncompressibleSolver:
+    """GPU IncompressibleSolver (projection coupling, PCG pressure)."""
+
+    def __init__(self, cfg: SimulationConfig, solid=None, ctx: _lib.Context | None = None):
+        self.cfg = cfg
+        self.ctx = ctx or _lib.default_context()
+        self._solid = None
+        if solid is not None:
+            self._solid = np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
+            if self._solid.size != cfg.nx * cfg.ny:
+                raise ValueError("solid mask must hold nx*ny cells")
+        h = C.c_void_p()
+        c = cfg.to_c()
+        check(load().fl_grid2d_create(self.ctx.handle, C.byref(c), ptr(self._solid), C.byref(h)),
+              self.ctx.handle)
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().fl_grid2d_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def advance(self) -> SolverStep:
+        r = Grid2DStepReport()
+        check(load().fl_grid2d_step(self.handle, C.byref(r)), self.ctx.handle)
+        return SolverStep.from_c(r)
+
+    def pressure_solve(self, rhs, p):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        n = self.cfg.nx * self.cfg.ny
+        if rhs.size != n or p.size != n or p.dtype != np.float64 or not p.flags["C_CONTIGUOUS"]:
+            raise ValueError("rhs and p must hold nx*ny float64 cells")
+        rep = SolveReport()
+        check(load().fl_grid2d_pressure_solve(self.handle, ptr(rhs), ptr(p), C.byref(rep)),
+              self.ctx.handle)
+        return rep
+
+    def fields(self):
+        nx, ny = self.cfg.nx, self.cfg.ny
+        u, v, p = np.empty((ny, nx + 1)), np.empty((ny + 1, nx)), np.empty((ny, nx))
+        check(load().fl_grid2d_get_fields(self.handle, ptr(u), ptr(v), ptr(p)), self.ctx.handle)
+        return {"u": u, "v": v, "p": p}
+
+    def set_fields(self, u=None, v=None, p=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, p)]
+        check(load().fl_grid2d_set_fields(self.handle, *[ptr(a) for a in arrs]), self.ctx.handle)
+
+    def field_update(self):
+        """Cell-centred fields (FieldUpdate, incompressible.rs:226-239)."""
+        nx, ny = self.cfg.nx, self.cfg.ny
+        out = {k: np.empty((ny, nx)) for k in ("velocity_x", "velocity_y", "speed", "vorticity")}
+        check(load().fl_grid2d_cell_fields(self.handle, ptr(out["velocity_x"]),
+                                           ptr(out["velocity_y"]), ptr(out["speed"]),
+                                           ptr(out["vorticity"])), self.ctx.handle)
+        out["pressure"] = self.fields()["p"]
+        return out
+
+
+class CavityBatch:
+    """Many independent 2D problems advanced together on one GPU."""
+
+    def __init__(self, cfgs, ctx: _lib.Context | None = None):
+        self.cfgs = list(cfgs)
+        self.ctx = ctx or _lib.default_context()
+        arr = (Grid2DConfig * len(self.cfgs))(*[c.to_c() for c in self.cfgs])
+        h = C.c_void_p()
+        check(load().fl_batch2d_create(self.ctx.handle, len(self.cfgs), arr, C.byref(h)),
+              self.ctx.handle)
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().fl_batch2d_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self):
+        reps = (Grid2DStepReport * len(self.cfgs))()
+        check(load().fl_batch2d_step(self.handle, reps), self.ctx.handle)
+        return [SolverStep.from_c(r) for r in reps]
+
+    def fields(self, index):
+        c = self.cfgs[index]
+        u, v, p = np.empty((c.ny, c.nx + 1)), np.empty((c.ny + 1, c.nx)), np.empty((c.ny, c.nx))
+        check(load().fl_batch2d_get_fields(self.handle, index, ptr(u), ptr(v), ptr(p)),
+              self.ctx.handle)
+        return {"u": u, "v": v, "p": p}

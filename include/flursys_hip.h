@@ -55,6 +55,13 @@ enum { FL_SOLVER_PCG = 0 };
 /* Arithmetic of the pressure solve: FP64 throughout, or an FP32 stencil CG
  * inside FP64 iterative refinement (FP64 residual, dots, alpha/beta, test). */
 enum { FL_PRECISION_FP64 = 0, FL_PRECISION_MIXED = 1 };
+/* Preconditioner of the inner solve.  JACOBI is the reference's.  MULTIGRID
+ * (no reference counterpart) runs flexible CG preconditioned by one
+ * cell-centred geometric-multigrid V-cycle (3+3 damped-Jacobi sweeps per
+ * level, piecewise-constant transfers with a 1.8 coarse-correction factor);
+ * when an inner solve exceeds its budget or stagnates, the solve falls back
+ * to plain FP64 Jacobi-PCG from the current iterate (counted in the report). */
+enum { FL_PRECOND_JACOBI = 0, FL_PRECOND_MULTIGRID = 1 };
 
 typedef struct fl_ctx fl_ctx;
 
@@ -79,6 +86,9 @@ fl_status fl_probe_copy(fl_ctx *ctx, int64_t bytes, int32_t iterations, double *
 typedef struct {
     int64_t iterations;        /* CG iterations (inner iterations when mixed) */
     int64_t refinements;       /* outer FP64 refinement sweeps (mixed only) */
+    int64_t fallback_iterations; /* plain FP64 PCG iterations after a fallback */
+    int32_t fallback;          /* 1 when the plain PCG fallback ran */
+    int32_t reserved;
     double residual_abs_inf;   /* max|r| of the recurrence residual at exit */
     double residual_l2;        /* ||r||_2 at exit */
     double initial_l2;         /* ||r_0||_2 */
@@ -98,9 +108,10 @@ typedef struct {
     int32_t stop_rule;             /* FL_STOP_* */
     int32_t initial_guess;         /* FL_GUESS_* */
     int32_t precision;             /* FL_PRECISION_* */
-    int32_t reserved;
-    /* FL_PRECISION_MIXED only: relative tolerance of each inner FP32 CG
-     * solve (0 selects 1e-5).  pressure_max_iters caps the inner total. */
+    int32_t preconditioner;        /* FL_PRECOND_* (multigrid needs FL_PRECISION_MIXED) */
+    /* FL_PRECISION_MIXED only: relative tolerance of each inner FP32 solve
+     * (0 selects 1e-5 for Jacobi-CG, 1e-4 for multigrid FCG).
+     * pressure_max_iters caps the inner total. */
     double inner_tolerance;
 } fl_cavity3d_config;
 
@@ -115,7 +126,7 @@ typedef struct {
     double divergence_l2;          /* ||div u||_2 over cells */
     double max_speed;              /* incompressible_3d.rs:519-532 */
     int32_t pressure_converged;
-    int32_t reserved;
+    int32_t pressure_fallback;     /* 1 when the plain PCG fallback ran */
 } fl_cavity3d_step_report;
 
 typedef struct fl_cavity3d fl_cavity3d;
@@ -150,6 +161,87 @@ fl_status fl_cavity3d_set_timing(fl_cavity3d *solver, int32_t enable);
 fl_status fl_cavity3d_kernel_times(fl_cavity3d *solver, double *ms_per_iteration,
                                    double *ms_apply, double *ms_update, double *ms_direction,
                                    int64_t *timed_iterations);
+
+/* ---------------------------------------------------------------------- */
+/* 2D structured projection solver: IncompressibleSolver with projection
+ * coupling (src/solver/incompressible.rs:293-746) on a staggered MAC grid
+ * with an optional solid mask.  Boundaries follow SolverBoundaryOverrides
+ * (src/preprocess.rs:1362-1407): per side a kind and constant (u, v); only
+ * the right side's PressureOutlet makes outlet cells (incompressible.rs:
+ * 2066-2082), as in the reference. */
+enum { FL_BC_VELOCITY = 0, FL_BC_SYMMETRY = 1, FL_BC_OUTLET = 2 };
+enum { FL_SIDE_LEFT = 0, FL_SIDE_RIGHT = 1, FL_SIDE_BOTTOM = 2, FL_SIDE_TOP = 3 };
+/* ConvectionScheme (incompressible.rs:14-18). */
+enum { FL_UPWIND = 0, FL_CENTRAL = 1 };
+
+/* SimulationConfig (incompressible.rs:58-90), the fields this path uses. */
+typedef struct {
+    int64_t nx, ny;
+    double length, height;
+    double density, kinematic_viscosity;
+    double dt, t_end;
+    int64_t pressure_max_iters;
+    double pressure_tolerance;
+    int32_t convection;            /* FL_UPWIND / FL_CENTRAL */
+    int32_t stop_rule;             /* FL_STOP_* */
+    int32_t initial_guess;         /* FL_GUESS_* */
+    int32_t adaptive;              /* TimeStepSettings::adaptive */
+    int32_t bc_kind[4];            /* per side, FL_SIDE_* order */
+    double bc_u[4], bc_v[4];       /* Velocity sides */
+    double outlet_pressure[4];     /* PressureOutlet sides */
+    /* TimeStepSettings (incompressible.rs:35-56) */
+    double min_dt, max_dt, target_cfl, target_diffusion_number, max_growth_factor;
+} fl_grid2d_config;
+
+/* SolverStep (incompressible.rs:196-214), flow-only fields. */
+typedef struct {
+    int64_t iteration;
+    double time, time_step;
+    double continuity_residual;    /* max divergence */
+    double pressure_residual;
+    int64_t pressure_iterations;
+    double momentum_residual;      /* velocity change */
+    double max_speed, momentum_cfl, viscous_diffusion_number;
+    int32_t pressure_converged;
+    int32_t reached_end_time;
+} fl_grid2d_step_report;
+
+typedef struct fl_grid2d fl_grid2d;
+
+/* IncompressibleSolver::new (incompressible.rs:320-366): validates, uploads
+ * the mask (nx*ny bytes, non-zero = solid; NULL = none) and initialises the
+ * velocity (zero field, blocked faces 0, boundary values at t = 0). */
+fl_status fl_grid2d_create(fl_ctx *ctx, const fl_grid2d_config *cfg, const uint8_t *solid,
+                           fl_grid2d **out);
+void fl_grid2d_destroy(fl_grid2d *grid);
+/* advance (incompressible.rs:523-555) without file output: one projection
+ * step with the pressure solved by Jacobi-PCG on the device. */
+fl_status fl_grid2d_step(fl_grid2d *grid, fl_grid2d_step_report *report);
+/* solve_pressure_pcg (incompressible.rs:1048-1214) on host arrays: rhs as
+ * build_pressure_rhs produces it; p_inout is the warm start. */
+fl_status fl_grid2d_pressure_solve(fl_grid2d *grid, const double *rhs, double *p_inout,
+                                   fl_solve_report *report);
+fl_status fl_grid2d_get_fields(fl_grid2d *grid, double *u, double *v, double *p);
+fl_status fl_grid2d_set_fields(fl_grid2d *grid, const double *u, const double *v,
+                               const double *p);
+/* field_update (incompressible.rs:563-586): cell-centred u, v, speed and
+ * vorticity (compute_cell_fields :1381-1439); any pointer may be NULL. */
+fl_status fl_grid2d_cell_fields(fl_grid2d *grid, double *u_cell, double *v_cell, double *speed,
+                                double *vorticity);
+
+/* Batched 2D cavities of uneven sizes: `count` independent problems advance
+ * together, every kernel covering all of them and the pressure solves of all
+ * of them running in one persistent cooperative kernel (no host round trip
+ * inside a step).  Each problem keeps its own dt (adaptive if requested). */
+typedef struct fl_batch2d fl_batch2d;
+fl_status fl_batch2d_create(fl_ctx *ctx, int32_t count, const fl_grid2d_config *cfgs,
+                            fl_batch2d **out);
+void fl_batch2d_destroy(fl_batch2d *batch);
+/* One step of every problem that has not reached its t_end; reports[count]
+ * may be NULL. */
+fl_status fl_batch2d_step(fl_batch2d *batch, fl_grid2d_step_report *reports);
+fl_status fl_batch2d_get_fields(fl_batch2d *batch, int32_t index, double *u, double *v,
+                                double *p);
 
 /* ---------------------------------------------------------------------- */
 /* CSR linear algebra: replaces CsrMatrix (src/linear/csr.rs:4-125) and the

@@ -234,3 +234,87 @@ def test_mixed_steps_match_oracle(gpu_ctx):
         assert np.abs(f[name] - o.field(name)).max() < 1e-7 * uscale, name
     assert rel(f["p"], o.field("p")) < 1e-6
     assert rep.max_divergence < 1e-5
+
+
+# ------------------------------------------- multigrid-preconditioned FCG
+def _mg(ctx, n, **kw):
+    kw.setdefault("pressure_tolerance", 1e-8)
+    kw.setdefault("stop_rule", fa.STOP_REL_L2)
+    kw.setdefault("initial_guess", fa.GUESS_ZERO)
+    kw.setdefault("pressure_max_iters", 20000)
+    return make(ctx, n if isinstance(n, int) else n[0], n if isinstance(n, int) else n[1],
+                n if isinstance(n, int) else n[2], precision=fa.PRECISION_MIXED,
+                preconditioner=fa.PRECOND_MULTIGRID, **kw)
+
+
+@pytest.mark.parametrize("shape", [16, 32, (32, 16, 24), 64])
+def test_multigrid_fcg_meets_fp64_tolerance(gpu_ctx, shape):
+    print(f"[gpu] MG-FCG {shape}", flush=True)
+    dims = (shape,) * 3 if isinstance(shape, int) else shape
+    nx, ny, nz = dims
+    rng = np.random.default_rng(9)
+    rhs = rng.standard_normal((nz, ny, nx)) * 10.0
+    rhs[0, 0, 0] = 0.0
+    s = _mg(gpu_ctx, dims)
+    p = np.zeros_like(rhs)
+    rep = s.pressure_solve(rhs, p)
+    assert rep.converged and not rep.fallback
+    assert rep.iterations < 60, rep.iterations
+    o = O.Cavity3D(nx, ny, nz)
+    b = -rhs.ravel()
+    b[0] = 0.0
+    true = np.linalg.norm(b - o.apply_operator(p.ravel())) / np.linalg.norm(b)
+    assert true <= 1.5e-8
+    assert p[0, 0, 0] == 0.0
+    ref = make(gpu_ctx, nx, ny, nz, pressure_tolerance=1e-12, stop_rule=fa.STOP_REL_L2,
+               initial_guess=fa.GUESS_ZERO, pressure_max_iters=20000)
+    pr = np.zeros_like(rhs)
+    ref.pressure_solve(rhs, pr)
+    assert rel(p, pr) < 1e-6
+
+
+def test_multigrid_fcg_matches_dct(gpu_ctx):
+    d = np.load(os.path.join(GOLD, "neumann_dct.npz"))
+    s = _mg(gpu_ctx, 16, pressure_tolerance=1e-10)
+    p = np.zeros((16, 16, 16))
+    rep = s.pressure_solve(d["rhs3d_16x16x16"], p)
+    assert rep.converged and rel(p, d["p3d_16x16x16"]) < 1e-8
+
+
+def test_multigrid_steps_match_oracle(gpu_ctx):
+    n = 32
+    kw = dict(reynolds=1000.0, dt=5e-4, pressure_tolerance=1e-8, stop_rule=fa.STOP_REL_L2,
+              initial_guess=fa.GUESS_ZERO, pressure_max_iters=5000)
+    o = O.Cavity3D(n, **kw)
+    s = _mg(gpu_ctx, n, reynolds=1000.0, dt=5e-4)
+    for _ in range(4):
+        o.step()
+        rep = s.advance_one_step()
+        assert rep.pressure_converged and not rep.pressure_fallback
+        assert rep.pressure_iterations < 60
+    f = s.fields()
+    uscale = max(np.abs(o.field(c)).max() for c in ("u", "v", "w"))
+    for name in ("u", "v", "w"):
+        assert np.abs(f[name] - o.field(name)).max() < 1e-7 * uscale, name
+    assert rel(f["p"], o.field("p")) < 1e-6
+
+
+def test_multigrid_fallback_to_plain_pcg(gpu_ctx):
+    """An inner budget too small to converge trips the fallback; the plain
+    FP64 PCG then meets the same rel-res tolerance (the judge)."""
+    n = 32
+    rng = np.random.default_rng(2)
+    rhs = rng.standard_normal((n, n, n))
+    rhs[0, 0, 0] = 0.0
+    s = _mg(gpu_ctx, n, inner_tolerance=1e-12)  # FP32 cannot reach 1e-12: stagnates
+    p = np.zeros_like(rhs)
+    rep = s.pressure_solve(rhs, p)
+    assert rep.fallback == 1 and rep.converged
+    assert rep.residual_l2 <= 1e-8 * rep.initial_l2 * 1.0000001
+
+
+def test_multigrid_rejects_unsupported_grids(gpu_ctx):
+    with pytest.raises(fa.FlursysError, match="halve"):
+        _mg(gpu_ctx, (33, 20, 9))
+    with pytest.raises(fa.FlursysError, match="mixed-precision"):
+        make(gpu_ctx, 16, 16, 16, preconditioner=fa.PRECOND_MULTIGRID)

@@ -1,0 +1,214 @@
+"""GPU parity of the 2D projection path (IncompressibleSolver, batched cavities)
+through the C ABI against the oracle.
+
+Bars: pressure solves within 1 iteration of the oracle and p within 1e-9 of
+max|p| (the device applies the reference's zero-mean projections through the
+reductions, so the two differ only by round-off); whole steps within 1e-9 of
+the velocity scale; the DCT closed form to 1e-10; the reference's smoke
+thresholds and error texts.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+import flursys_amd as fa
+import flursys_amd.incompressible as I
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def oracle_for(cfg: I.SimulationConfig, solid=None, threads=0):
+    ts = cfg.time_step
+    return O.Solver2D(cfg.nx, cfg.ny, length=cfg.length, height=cfg.height,
+                      density=cfg.density, nu=cfg.kinematic_viscosity, dt=cfg.dt,
+                      t_end=cfg.t_end, pressure_max_iters=cfg.pressure_max_iters,
+                      pressure_tolerance=cfg.pressure_tolerance, convection=cfg.convection,
+                      stop_rule=cfg.stop_rule, initial_guess=cfg.initial_guess,
+                      threads=threads, bc_kind=cfg.bc_kind, bc_u=cfg.bc_u, bc_v=cfg.bc_v,
+                      outlet_pressure=cfg.outlet_pressure, adaptive=ts.adaptive,
+                      min_dt=ts.min_dt, max_dt=ts.max_dt, target_cfl=ts.target_cfl,
+                      target_diffusion=ts.target_diffusion_number,
+                      max_growth=ts.max_growth_factor, solid=solid)
+
+
+def channel_cfg(nx, ny, **kw):
+    """Velocity inlet left (u = 1), pressure outlet right, walls top/bottom."""
+    return I.SimulationConfig(nx=nx, ny=ny, length=nx / ny, height=1.0,
+                              bc_kind=(I.BC_VELOCITY, I.BC_PRESSURE_OUTLET, I.BC_VELOCITY,
+                                       I.BC_VELOCITY),
+                              bc_u=(1.0, 0.0, 0.0, 0.0), **kw)
+
+
+def block_mask(nx, ny):
+    m = np.zeros((ny, nx), np.uint8)
+    m[ny // 3: ny // 2, nx // 4: nx // 3] = 1
+    return m
+
+
+@pytest.mark.parametrize("key", ["16x16", "32x32", "24x40"])
+def test_pressure_solve_matches_dct(gpu_ctx, key):
+    d = np.load(os.path.join(GOLD, "neumann_dct.npz"))
+    ny, nx = map(int, key.split("x"))
+    cfg = I.SimulationConfig(nx=nx, ny=ny, pressure_tolerance=1e-13, stop_rule=fa.STOP_REL_L2,
+                             initial_guess=fa.GUESS_ZERO, pressure_max_iters=10000)
+    s = I.IncompressibleSolver(cfg, ctx=gpu_ctx)
+    p = np.zeros((ny, nx))
+    rep = s.pressure_solve(-d[f"b2d_{key}"], p)
+    assert rep.converged
+    assert rel(p, d[f"p2d_{key}"]) < 1e-10
+    assert abs(p.mean()) < 1e-12
+
+
+@pytest.mark.parametrize("case", ["cavity", "obstacle", "channel"])
+@pytest.mark.parametrize("stop,tol,guess", [(fa.STOP_ABS_INF, 1e-6, fa.GUESS_PREVIOUS),
+                                            (fa.STOP_REL_L2, 1e-8, fa.GUESS_ZERO)])
+def test_pressure_solve_matches_oracle(gpu_ctx, case, stop, tol, guess):
+    print(f"[gpu] 2D pressure solve {case} stop={stop}", flush=True)
+    nx, ny = 40, 24
+    kw = dict(pressure_tolerance=tol, stop_rule=stop, initial_guess=guess,
+              pressure_max_iters=5000)
+    solid = None
+    if case == "channel":
+        cfg = channel_cfg(nx, ny, **kw)
+    else:
+        cfg = I.SimulationConfig(nx=nx, ny=ny, **kw)
+        if case == "obstacle":
+            solid = block_mask(nx, ny)
+    rng = np.random.default_rng(3)
+    rhs = rng.standard_normal((ny, nx)) * 30.0
+    p0 = rng.standard_normal((ny, nx)) * 0.01
+    if solid is not None:
+        rhs[solid.astype(bool)] = 0.0
+    if case == "channel":
+        rhs[:, -1] = 0.0
+    o = oracle_for(cfg, solid)
+    o.field("rhs")[:] = rhs
+    o.field("p")[:] = p0
+    orep = o.solve_pressure()
+    s = I.IncompressibleSolver(cfg, solid=solid, ctx=gpu_ctx)
+    p = p0.copy()
+    rep = s.pressure_solve(rhs, p)
+    assert rep.converged and orep.converged
+    assert abs(rep.iterations - orep.iterations) <= 1
+    assert rel(p, o.field("p")) < 1e-9
+
+
+@pytest.mark.parametrize("scheme", [I.FIRST_ORDER_UPWIND, I.CENTRAL])
+@pytest.mark.parametrize("case", ["cavity", "obstacle", "channel"])
+def test_steps_match_oracle(gpu_ctx, scheme, case):
+    print(f"[gpu] 2D steps {case} scheme={scheme}", flush=True)
+    nx, ny = 32, 24
+    kw = dict(dt=2e-3, convection=scheme, pressure_tolerance=1e-8, stop_rule=fa.STOP_REL_L2,
+              initial_guess=fa.GUESS_ZERO, pressure_max_iters=5000, kinematic_viscosity=0.01)
+    solid = None
+    if case == "channel":
+        cfg = channel_cfg(nx, ny, **kw)
+    else:
+        cfg = I.SimulationConfig(nx=nx, ny=ny, **kw)
+        if case == "obstacle":
+            solid = block_mask(nx, ny)
+    o = oracle_for(cfg, solid)
+    s = I.IncompressibleSolver(cfg, solid=solid, ctx=gpu_ctx)
+    for _ in range(5):
+        d, orep = o.step()
+        st = s.advance()
+        assert st.pressure_converged
+        assert abs(st.pressure_iterations - orep.iterations) <= 1
+    f = s.fields()
+    uscale = max(np.abs(o.field("u")).max(), np.abs(o.field("v")).max())
+    assert np.abs(f["u"] - o.field("u")).max() < 1e-9 * uscale
+    assert np.abs(f["v"] - o.field("v")).max() < 1e-9 * uscale
+    assert rel(f["p"], o.field("p")) < 1e-8
+    assert abs(st.continuity_residual - d.max_divergence) <= 1e-9 * uscale / cfg.length * cfg.nx
+    assert abs(st.momentum_residual - d.velocity_change) <= 1e-9 * uscale
+    assert abs(st.max_speed - d.max_speed) <= 1e-9 * uscale
+    fu = s.field_update()
+    assert np.abs(fu["speed"] - o.field("speed")).max() < 1e-9 * uscale
+    assert np.abs(fu["vorticity"] - o.field("vort")).max() < 1e-7 * max(np.abs(o.field("vort")).max(), 1)
+
+
+def test_smoke_threshold_and_t_end(gpu_ctx):
+    """tests/smoke.rs:57-69: 16^2 cavity, one step, divergence < 1e-5; a
+    second step past t_end is an error (incompressible.rs:623-625)."""
+    cfg = I.SimulationConfig(nx=16, ny=16, dt=1e-3, t_end=1e-3, pressure_max_iters=800,
+                             pressure_tolerance=1e-6)
+    s = I.IncompressibleSolver(cfg, ctx=gpu_ctx)
+    st = s.advance()
+    assert st.continuity_residual < 1e-5 and st.reached_end_time
+    assert abs(st.time - 1e-3) < 1e-12
+    with pytest.raises(fa.FlursysError, match="already reached t_end"):
+        s.advance()
+
+
+def test_adaptive_time_stepping_matches_oracle(gpu_ctx):
+    cfg = I.SimulationConfig(nx=32, ny=32, dt=1e-3, kinematic_viscosity=0.01,
+                             time_step=I.TimeStepSettings(adaptive=True, min_dt=1e-6,
+                                                          max_dt=1e-2, target_cfl=0.5,
+                                                          target_diffusion_number=0.25,
+                                                          max_growth_factor=1.1))
+    o = oracle_for(cfg)
+    s = I.IncompressibleSolver(cfg, ctx=gpu_ctx)
+    for _ in range(6):
+        d, _ = o.step()
+        st = s.advance()
+        assert abs(st.time_step - d.time_step) <= 1e-12 * d.time_step
+    assert abs(st.time - o.time) < 1e-12
+
+
+def test_config1_as_written_blows_up(gpu_ctx):
+    """128^2, Re 100, dt = 0.25 h: viscous number 0.64 > 0.5 -> non-finite."""
+    cfg = I.SimulationConfig.cavity(128, reynolds=100.0, dt=0.25 / 128, pressure_tolerance=1e-8,
+                                    stop_rule=fa.STOP_REL_L2, initial_guess=fa.GUESS_ZERO,
+                                    pressure_max_iters=5000)
+    s = I.IncompressibleSolver(cfg, ctx=gpu_ctx)
+    with pytest.raises(fa.FlursysError, match="Non-finite"):
+        for _ in range(200):
+            st = s.advance()
+            assert abs(st.viscous_diffusion_number - 0.64) < 1e-12
+
+
+def test_config_validation_texts(gpu_ctx):
+    with pytest.raises(fa.FlursysError, match="nx and ny must be at least 4"):
+        I.IncompressibleSolver(I.SimulationConfig(nx=3, ny=8), ctx=gpu_ctx)
+    with pytest.raises(fa.FlursysError, match="t_end must be positive for projection coupling"):
+        I.IncompressibleSolver(I.SimulationConfig(t_end=0.0), ctx=gpu_ctx)
+    bad = I.SimulationConfig(dt=1.0, time_step=I.TimeStepSettings(adaptive=True))
+    with pytest.raises(fa.FlursysError, match="adaptive initial dt must lie between"):
+        I.IncompressibleSolver(bad, ctx=gpu_ctx)
+
+
+def test_batch_matches_individual_runs(gpu_ctx):
+    cfgs = [I.SimulationConfig.cavity(n, reynolds=re, dt=dt, pressure_tolerance=1e-8,
+                                      stop_rule=fa.STOP_REL_L2, initial_guess=fa.GUESS_ZERO,
+                                      pressure_max_iters=5000)
+            for n, re, dt in ((16, 100.0, 2e-3), (40, 400.0, 1e-3), (27, 1000.0, 1.5e-3))]
+    b = I.CavityBatch(cfgs, ctx=gpu_ctx)
+    oracles = [oracle_for(c) for c in cfgs]
+    for _ in range(4):
+        reps = b.step()
+        for o, r in zip(oracles, reps):
+            d, orep = o.step()
+            assert abs(r.pressure_iterations - orep.iterations) <= 1
+    for k, o in enumerate(oracles):
+        f = b.fields(k)
+        uscale = max(np.abs(o.field("u")).max(), np.abs(o.field("v")).max())
+        assert np.abs(f["u"] - o.field("u")).max() < 1e-9 * uscale
+        assert rel(f["p"], o.field("p")) < 1e-8
+
+
+def test_batch_respects_each_t_end(gpu_ctx):
+    cfgs = [I.SimulationConfig.cavity(16, dt=1e-3, t_end=2e-3),
+            I.SimulationConfig.cavity(20, dt=1e-3, t_end=4e-3)]
+    b = I.CavityBatch(cfgs, ctx=gpu_ctx)
+    for _ in range(4):
+        reps = b.step()
+    assert abs(reps[1].time - 4e-3) < 1e-12
+    assert reps[1].iteration == 4
