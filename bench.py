@@ -72,51 +72,59 @@ def cpu_baseline(n, re, dt, iters_per_step, sample_iters):
     }
 
 
-def run_variant(fa, ctx, name, precision, args):
+def run_variant(fa, ctx, name, precision, precond, args):
     n = args.n
     cfg = fa.LidDrivenCavity3DConfig(nx=n, ny=n, nz=n, reynolds=args.reynolds, dt=args.dt,
                                      pressure_tolerance=args.tol, stop_rule=fa.STOP_REL_L2,
                                      initial_guess=fa.GUESS_ZERO, pressure_max_iters=50000,
-                                     precision=precision, inner_tolerance=args.inner_tol)
+                                     precision=precision, preconditioner=precond,
+                                     inner_tolerance=args.inner_tol)
     log(f"[bench] {name}: {n}^3 Re={args.reynolds} dt={args.dt} tol={args.tol}")
     s = fa.LidDrivenCavity3DSolver(cfg, ctx)
+    # Warmup steps time every kernel (untimed region): their breakdown names
+    # the dominant kernel, which alone is event-timed in the timed region.
+    s.set_timing(True)
     for k in range(args.warmup):
         rep = s.advance_one_step()
-        log(f"[bench] {name} warmup {k + 1}: {rep.pressure_iterations} iterations")
-    s.set_timing(True)
-    iters, refs = [], []
+        log(f"[bench] {name} warmup {k + 1}: {rep.pressure_iterations} iterations, "
+            f"{rep.pressure_refinements} refinements, fallback {rep.pressure_fallback}")
+    breakdown = s.kernel_stats() if args.warmup else {}
+    dom = max(breakdown, key=lambda k: breakdown[k]["total_ms"]) if breakdown else None
+    s.set_timing(True, dom)
+    iters, refs, fbs = [], [], []
     t0 = time.perf_counter()
     for k in range(args.steps):
         rep = s.advance_one_step()  # returns after its stream has drained
         iters.append(rep.pressure_iterations)
         refs.append(rep.pressure_refinements)
+        fbs.append(rep.pressure_fallback)
         if not rep.pressure_converged:
             raise SystemExit(f"{name}: pressure solve did not converge")
         log(f"[bench] {name} step {k + 1}/{args.steps}: {rep.pressure_iterations} iterations, "
-            f"{rep.pressure_refinements} refinements, div {rep.max_divergence:.3e}")
+            f"{rep.pressure_refinements} refinements, fallback {rep.pressure_fallback}, "
+            f"div {rep.max_divergence:.3e}")
     elapsed = time.perf_counter() - t0
-    kt = s.kernel_times()
+    timed = s.kernel_stats()
     s.close()
-    cells = n ** 3
-    word = 4 if precision == fa.PRECISION_MIXED else 8
-    # Algorithmic HBM bytes per launch: dirapply reads r, d and writes d', q
-    # (4 passes); update reads x, d', r, q and writes x, r (6 passes).
-    kernels = {
-        "k_cg_update": (6 * word * cells, kt["ms_update"]),
-        "k_cg_dirapply": (4 * word * cells, kt["ms_apply"]),
-    }
+    if dom is None and timed:
+        dom = max(timed, key=lambda k: timed[k]["total_ms"])
+    step_ms = elapsed * 1e3 / args.steps
     return {
         "steps_per_s": args.steps / elapsed,
-        "ms_per_step": elapsed * 1e3 / args.steps,
+        "ms_per_step": step_ms,
         "cg_iters_per_step": sum(iters) / len(iters),
         "cg_iters_per_step_all": iters,
         "refinements_per_step": sum(refs) / len(refs),
-        "ms_per_cg_iteration": kt["ms_per_iteration"],
-        "cg_iteration_GBps": (10 * word * cells / (kt["ms_per_iteration"] * 1e-3) / 1e9
-                              if kt["ms_per_iteration"] else None),
-        "kernels": {k: {"ms": v[1], "bytes": v[0],
-                        "GBps": v[0] / (v[1] * 1e-3) / 1e9 if v[1] else None}
-                    for k, v in kernels.items()},
+        "fallback_steps": sum(1 for f in fbs if f),
+        "dominant_kernel": dom,
+        "dominant_timed": timed.get(dom),
+        # per-kernel breakdown of the warmup steps (all kernels event-timed)
+        "warmup_breakdown": {k: {"launches": v["launches"], "ms": v["ms"],
+                                 "share": v["total_ms"] / sum(x["total_ms"] for x in
+                                                              breakdown.values()),
+                                 "GBps": v["GBps"]}
+                             for k, v in sorted(breakdown.items(),
+                                                key=lambda kv: -kv[1]["total_ms"])},
         "max_divergence": rep.max_divergence,
         "dtype": "f32+f64" if precision == fa.PRECISION_MIXED else "f64",
     }
@@ -132,7 +140,7 @@ def main():
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=0.0)
-    ap.add_argument("--variants", default="plain_fp64_pcg,mixed_ir")
+    ap.add_argument("--variants", default="plain_fp64_pcg,mixed_ir,mg_fcg")
     ap.add_argument("--cpu-sample-iters", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-copy-probe", action="store_true")
@@ -148,14 +156,16 @@ def main():
     from flursys_amd import _lib
 
     ctx = fa.Context(0)
-    precisions = {"plain_fp64_pcg": fa.PRECISION_FP64, "mixed_ir": fa.PRECISION_MIXED}
+    modes = {"plain_fp64_pcg": (fa.PRECISION_FP64, fa.PRECOND_JACOBI),
+             "mixed_ir": (fa.PRECISION_MIXED, fa.PRECOND_JACOBI),
+             "mg_fcg": (fa.PRECISION_MIXED, fa.PRECOND_MULTIGRID)}
     results = {}
     for name in args.variants.split(","):
-        results[name] = run_variant(fa, ctx, name, precisions[name], args)
+        results[name] = run_variant(fa, ctx, name, *modes[name], args)
     best = max(results, key=lambda k: results[k]["steps_per_s"])
     b = results[best]
-    dom = max(b["kernels"], key=lambda k: b["kernels"][k]["ms"])
-    dk = b["kernels"][dom]
+    dom = b["dominant_kernel"]
+    dk = b["dominant_timed"] or {}
     copy = None
     if not args.no_copy_probe:
         import ctypes as C
@@ -191,14 +201,15 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"{dom} ({b['dtype']})",
-            "achieved": dk["GBps"],
+            "kernel": dom,
+            "achieved": dk.get("GBps"),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": dk["GBps"] / HBM_PEAK_GBS if dk["GBps"] else None,
+            "frac": dk["GBps"] / HBM_PEAK_GBS if dk.get("GBps") else None,
             "traffic": None,
-            "algorithmic_bytes_per_launch": dk["bytes"],
-            "ms_per_launch": dk["ms"],
+            "algorithmic_bytes_per_launch": dk.get("bytes_per_launch"),
+            "ms_per_launch": dk.get("ms"),
+            "timed_launches": dk.get("launches"),
         },
         "cpu_baseline": None,
     }

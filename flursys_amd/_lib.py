@@ -69,6 +69,11 @@ class Cavity3DStepReport(C.Structure):
     ]
 
 
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("launches", C.c_int64), ("total_ms", C.c_double),
+                ("bytes", C.c_double)]
+
+
 class LinOpts(C.Structure):
     _fields_ = [("absolute_tolerance", C.c_double), ("relative_tolerance", C.c_double),
                 ("max_iterations", C.c_int64)]
@@ -130,8 +135,8 @@ SIGNATURES = {
     "fl_cavity3d_get_fields": (C.c_int, [_P, _P, _P, _P, _P]),
     "fl_cavity3d_set_fields": (C.c_int, [_P, _P, _P, _P, _P]),
     "fl_cavity3d_get_intermediate": (C.c_int, [_P, _P, _P, _P, _P]),
-    "fl_cavity3d_set_timing": (C.c_int, [_P, C.c_int32]),
-    "fl_cavity3d_kernel_times": (C.c_int, [_P, _D, _D, _D, _D, C.POINTER(C.c_int64)]),
+    "fl_cavity3d_set_timing": (C.c_int, [_P, C.c_int32, C.c_char_p]),
+    "fl_cavity3d_kernel_stats": (C.c_int, [_P, C.c_int32, C.POINTER(KernelStat)]),
     "fl_grid2d_create": (C.c_int, [_P, C.POINTER(Grid2DConfig), _P, C.POINTER(_P)]),
     "fl_grid2d_destroy": (None, [_P]),
     "fl_grid2d_step": (C.c_int, [_P, C.POINTER(Grid2DStepReport)]),

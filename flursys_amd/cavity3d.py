@@ -155,13 +155,27 @@ class LidDrivenCavity3DSolver:
               self.ctx.handle)
         return out
 
-    def set_timing(self, enable: bool):
-        check(load().fl_cavity3d_set_timing(self.handle, int(enable)), self.ctx.handle)
+    def set_timing(self, enable: bool, kernel: str | None = None):
+        """Time every kernel (or only `kernel`) with HIP events from now on;
+        resets the statistics."""
+        name = kernel.encode() if kernel else None
+        check(load().fl_cavity3d_set_timing(self.handle, int(enable), name), self.ctx.handle)
 
-    def kernel_times(self):
-        a = [C.c_double() for _ in range(4)]
-        n = C.c_int64()
-        check(load().fl_cavity3d_kernel_times(self.handle, *[C.byref(x) for x in a],
-                                              C.byref(n)), self.ctx.handle)
-        return {"ms_per_iteration": a[0].value, "ms_apply": a[1].value,
-                "ms_update": a[2].value, "ms_direction": a[3].value, "iterations": n.value}
+    def kernel_stats(self) -> dict:
+        """{kernel name: {launches, total_ms, bytes, ms, GBps}} of the kernels
+        timed since set_timing; `ms` is the mean per launch and `GBps` the
+        algorithmic HBM bytes over device time."""
+        lib = load()
+        out = {}
+        k = 0
+        st = _lib.KernelStat()
+        while lib.fl_cavity3d_kernel_stats(self.handle, k, C.byref(st)) == _lib.OK:
+            if st.launches:
+                ms = st.total_ms / st.launches
+                out[st.name.decode()] = {
+                    "launches": st.launches, "total_ms": st.total_ms, "bytes": st.bytes,
+                    "ms": ms, "bytes_per_launch": st.bytes / st.launches,
+                    "GBps": st.bytes / (st.total_ms * 1e-3) / 1e9 if st.total_ms > 0 else None,
+                }
+            k += 1
+        return out

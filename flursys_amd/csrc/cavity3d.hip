@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "kprof.hpp"
 
 namespace fl {
 namespace c3 {
@@ -612,13 +613,18 @@ __global__ void __launch_bounds__(kBlock) k_fcg_dirapply(Grid g, const float *__
             cell_ijk(g, c, i, j, k);
             const Nbr o = neighbours(g, c, i, j, k);
             const uint32_t idx[7] = {c, o.e, o.w, o.n, o.s, o.f, o.b};
+            // all loads issued before any arithmetic so they are in flight together
+            float zv[7], ov[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int a = 0; a < 7; ++a) zv[a] = zt[idx[a]];
+            if (!first) {
+#pragma unroll
+                for (int a = 0; a < 7; ++a) ov[a] = pold[idx[a]];
+            }
             float pv[7];
 #pragma unroll
-            for (int a = 0; a < 7; ++a) {
-                // the pinned cell's p stays 0
-                const float z = zt[idx[a]] - z0;
-                pv[a] = idx[a] == 0 ? 0.0f : (first ? z : z - tb * pold[idx[a]]);
-            }
+            for (int a = 0; a < 7; ++a)  // the pinned cell's p stays 0
+                pv[a] = idx[a] == 0 ? 0.0f : (zv[a] - z0) - tb * ov[a];
             pc = pv[0];
             qc = cx * ((pv[0] - pv[1]) + (pv[0] - pv[2])) + cy * ((pv[0] - pv[3]) + (pv[0] - pv[4])) +
                  cz * ((pv[0] - pv[5]) + (pv[0] - pv[6]));
@@ -869,6 +875,28 @@ __global__ void __launch_bounds__(kBlock) k_finite(const double *__restrict__ a0
 #undef V3
 #undef W3
 
+// Kernel kinds of the per-kernel profiler, named as the kernels are (with
+// their template arguments).
+enum Kind {
+    K_PREDICT, K_RHS, K_CORRECT, K_DIAG, K_FINITE,
+    K_RESIDUAL_D, K_RESIDUAL_F, K_FOLD_OUTER, K_ADD_CORRECTION,
+    K_DIRAPPLY_D, K_UPDATE_D, K_DIRAPPLY_F, K_UPDATE_F,
+    K_MG_CHECK, K_MG_SWEEP_L0, K_MG_SWEEP_L0_PROLONG, K_MG_SWEEP_L0_PUBLISH,
+    K_MG_SWEEP_FIRST, K_MG_SWEEP, K_MG_SWEEP_PROLONG, K_MG_RESTRICT_L0, K_MG_RESTRICT,
+    K_MG_TAIL, K_FCG_DIRAPPLY, K_FCG_UPDATE, K_COUNT
+};
+const char *const kKindNames[K_COUNT] = {
+    "k_predict", "k_rhs", "k_correct", "k_diag", "k_finite",
+    "k_residual<double>", "k_residual<float>", "k_fold_outer", "k_add_correction",
+    "k_cg_dirapply<double>", "k_cg_update<double, double>",
+    "k_cg_dirapply<float>", "k_cg_update<float, float>",
+    "k_mg_check", "k_mg_sweep<false, false, true, false>",
+    "k_mg_sweep<false, true, true, false>", "k_mg_sweep<false, false, true, true>",
+    "k_mg_sweep<true, false, false, false>", "k_mg_sweep<false, false, false, false>",
+    "k_mg_sweep<false, true, false, false>", "k_mg_restrict<true>", "k_mg_restrict<false>",
+    "k_mg_tail", "k_fcg_dirapply", "k_fcg_update",
+};
+
 }  // namespace c3
 }  // namespace fl
 
@@ -911,11 +939,7 @@ struct fl_cavity3d {
     double *outer = nullptr;  // folded outer residual: r.z, r.r, max|r|
     uint32_t blocks = 1, ew_blocks = 1;
     int64_t step = 0;
-    // per-kernel timing (HIP events on the solver's stream)
-    bool timing = false;
-    std::vector<hipEvent_t> ev;
-    double t_apply = 0, t_update = 0;
-    int64_t t_iters = 0;
+    KProf prof;  // per-kernel timing (HIP events on the solver's stream)
 };
 
 namespace {
@@ -955,8 +979,7 @@ void free_all(fl_cavity3d *s) {
         if (s->meb[l]) (void)hipFree(s->meb[l]);
     }
     if (s->host) (void)hipHostFree(s->host);
-    for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
-    s->ev.clear();
+    s->prof.destroy();
 }
 
 // Multigrid levels: halve every axis while all are even and at least 8;
@@ -1057,38 +1080,31 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->sa, &s->host->a, sizeof(CgStateA), hipMemcpyHostToDevice, stm));
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->sb, &s->host->b, sizeof(CgStateB), hipMemcpyHostToDevice, stm));
     const uint32_t G = s->blocks;
+    const bool fp64 = sizeof(T) == 8;
+    const int kd = fp64 ? K_DIRAPPLY_D : K_DIRAPPLY_F, ku = fp64 ? K_UPDATE_D : K_UPDATE_F;
+    // algorithmic bytes: dirapply reads r, d and writes d', q; update reads
+    // x, d', r, q and writes x, r
+    const double bd = 4.0 * sizeof(T) * s->n, bu = (2.0 * sizeof(TX) + 4.0 * sizeof(T)) * s->n;
     int batch = 8;
-    long long launched = 0, done_iters = 0;
+    long long launched = 0;
     for (;;) {
         for (int b = 0; b < batch; ++b) {
             // iteration `launched` reads d[launched % 2] and writes the other
             T *dold = dbuf[launched & 1], *dnew = dbuf[(launched + 1) & 1];
-            if (s->timing) FL_HIP_TRY(ctx, hipEventRecord(s->ev[3 * b], stm));
-            k_cg_dirapply<T><<<G, kBlock, 0, stm>>>(s->g, prm, r, dold, dnew, q, s->pb, s->pa,
-                                                    s->sa, s->sb);
-            if (s->timing) FL_HIP_TRY(ctx, hipEventRecord(s->ev[3 * b + 1], stm));
-            k_cg_update<T, TX><<<G, kBlock, 0, stm>>>(s->g, x, r, dnew, q, s->pa, s->pb, s->sa,
-                                                      s->sb);
-            if (s->timing) FL_HIP_TRY(ctx, hipEventRecord(s->ev[3 * b + 2], stm));
+            FL_TIMED(s, kd, launched, bd,
+                     k_cg_dirapply<T><<<G, kBlock, 0, stm>>>(s->g, prm, r, dold, dnew, q, s->pb,
+                                                             s->pa, s->sa, s->sb));
+            FL_TIMED(s, ku, launched, bu,
+                     k_cg_update<T, TX><<<G, kBlock, 0, stm>>>(s->g, x, r, dnew, q, s->pa, s->pb,
+                                                               s->sa, s->sb));
             ++launched;
         }
         FL_HIP_TRY(ctx, hipGetLastError());
         FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->a, s->sa, sizeof(CgStateA), hipMemcpyDeviceToHost, stm));
         FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->b, s->sb, sizeof(CgStateB), hipMemcpyDeviceToHost, stm));
         FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
-        if (s->timing) {
-            // only iterations whose update ran count (later launches exit early)
-            const long long active = std::min<long long>(batch, s->host->b.iters - done_iters);
-            for (long long b = 0; b < active; ++b) {
-                float ta = 0, tu = 0;
-                FL_HIP_TRY(ctx, hipEventElapsedTime(&ta, s->ev[3 * b], s->ev[3 * b + 1]));
-                FL_HIP_TRY(ctx, hipEventElapsedTime(&tu, s->ev[3 * b + 1], s->ev[3 * b + 2]));
-                s->t_apply += ta;
-                s->t_update += tu;
-                s->t_iters += 1;
-            }
-        }
-        done_iters = s->host->b.iters;
+        // only iterations whose update ran count (later launches exit early)
+        FL_HIP_TRY(ctx, s->prof.flush(s->host->b.iters));
         if (s->host->a.done || s->host->b.done) break;
         batch = std::min(batch * 2, kMaxBatch);
     }
@@ -1098,7 +1114,8 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
 fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep, double rr0_ref = 0.0) {
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
-    k_residual<double><<<s->blocks, kBlock, 0, stm>>>(s->g, s->rhs, s->p, s->r, s->pb);
+    FL_TIMED(s, K_RESIDUAL_D, -1, 24.0 * s->n,
+             k_residual<double><<<s->blocks, kBlock, 0, stm>>>(s->g, s->rhs, s->p, s->r, s->pb));
     FL_HIP_TRY(ctx, hipGetLastError());
     const CgParams prm{s->cfg.pressure_tolerance, s->cfg.pressure_max_iters, s->cfg.stop_rule,
                        rr0_ref};
@@ -1124,41 +1141,68 @@ fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep, double rr0_ref = 0.0)
 
 // One multigrid-preconditioned FCG iteration: the V-cycle on r (levels
 // 0..tail-1 here, the rest in k_mg_tail), then the direction and update.
-void launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k) {
-    hipStream_t stm = s->ctx->stream;
+// Algorithmic HBM bytes per launch (FP32 words): a sweep reads f and e_in
+// and writes e_out (the first sweep of a level reads no e_in); prolongation
+// adds the coarse level once, the publishing sweep q_old; a restriction
+// reads f and e of the fine level and writes the coarse f.
+fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k) {
+    fl_ctx *ctx = s->ctx;
+    hipStream_t stm = ctx->stream;
     const int t = s->tail;
     const Level *L = s->lv;
     float *pold = s->d32[k & 1], *pnew = s->d32[(k + 1) & 1];
     const uint32_t G = s->blocks;
+    const double n0 = L[0].n, n1 = L[1].n;
     // down
-    k_mg_check<<<G, kBlock, 0, stm>>>(L[0], prm, s->r32, s->mea[0], s->pb, s->fa, s->fb);
-    k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], nullptr, nullptr, s->fa, s->fb);
-    k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb);
-    k_mg_restrict<true><<<s->lblocks[1], kBlock, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->mf[1], s->fa, s->fb);
+    FL_TIMED(s, K_MG_CHECK, k, 8.0 * n0,
+             k_mg_check<<<G, kBlock, 0, stm>>>(L[0], prm, s->r32, s->mea[0], s->pb, s->fa, s->fb));
+    FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
+             k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], nullptr, nullptr, s->fa, s->fb));
+    FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
+             k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb));
+    FL_TIMED(s, K_MG_RESTRICT_L0, k, 8.0 * n0 + 4.0 * n1,
+             k_mg_restrict<true><<<s->lblocks[1], kBlock, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->mf[1], s->fa, s->fb));
     for (int l = 1; l < t; ++l) {
         const uint32_t g = s->lblocks[l];
-        k_mg_sweep<true, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], nullptr, nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb);
-        k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb);
-        k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb);
-        k_mg_restrict<false><<<s->lblocks[l + 1], kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->mf[l + 1], s->fa, s->fb);
+        const double nl = L[l].n, nc = L[l + 1].n;
+        FL_TIMED(s, K_MG_SWEEP_FIRST, k, 8.0 * nl,
+                 k_mg_sweep<true, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], nullptr, nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb));
+        FL_TIMED(s, K_MG_SWEEP, k, 12.0 * nl,
+                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb));
+        FL_TIMED(s, K_MG_SWEEP, k, 12.0 * nl,
+                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb));
+        FL_TIMED(s, K_MG_RESTRICT, k, 8.0 * nl + 4.0 * nc,
+                 k_mg_restrict<false><<<s->lblocks[l + 1], kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->mf[l + 1], s->fa, s->fb));
     }
     TailLevels T{};
     T.count = s->nlev - t;
     for (int l = 0; l < T.count; ++l) T.lv[l] = L[t + l];
-    k_mg_tail<<<1, 1024, 0, stm>>>(T, s->mf[t], s->meb[t], s->fa, s->fb);
+    FL_TIMED(s, K_MG_TAIL, k, 8.0 * L[t].n,
+             k_mg_tail<<<1, 1024, 0, stm>>>(T, s->mf[t], s->meb[t], s->fa, s->fb));
     // up: the solution of level l ends in meb[l]
     for (int l = t - 1; l >= 1; --l) {
         const uint32_t g = s->lblocks[l];
-        k_mg_sweep<false, true, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->meb[l + 1], s->meb[l], nullptr, nullptr, s->fa, s->fb);
-        k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb);
-        k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb);
+        const double nl = L[l].n, nc = L[l + 1].n;
+        FL_TIMED(s, K_MG_SWEEP_PROLONG, k, 12.0 * nl + 4.0 * nc,
+                 k_mg_sweep<false, true, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->meb[l + 1], s->meb[l], nullptr, nullptr, s->fa, s->fb));
+        FL_TIMED(s, K_MG_SWEEP, k, 12.0 * nl,
+                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb));
+        FL_TIMED(s, K_MG_SWEEP, k, 12.0 * nl,
+                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb));
     }
-    k_mg_sweep<false, true, true, false><<<G, kBlock, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->meb[1], s->meb[0], nullptr, nullptr, s->fa, s->fb);
-    k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb);
-    k_mg_sweep<false, false, true, true><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], s->q32, s->pz, s->fa, s->fb);
-    // FCG direction and update
-    k_fcg_dirapply<<<G, kBlock, 0, stm>>>(s->g, s->meb[0], pold, pnew, s->q32, s->r32, s->pz, s->pa, s->fa, s->fb);
-    k_fcg_update<<<G, kBlock, 0, stm>>>((uint32_t)s->n, s->e32, s->r32, pnew, s->q32, s->pa, s->pb, s->fa, s->fb);
+    FL_TIMED(s, K_MG_SWEEP_L0_PROLONG, k, 12.0 * n0 + 4.0 * n1,
+             k_mg_sweep<false, true, true, false><<<G, kBlock, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->meb[1], s->meb[0], nullptr, nullptr, s->fa, s->fb));
+    FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
+             k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb));
+    FL_TIMED(s, K_MG_SWEEP_L0_PUBLISH, k, 16.0 * n0,
+             k_mg_sweep<false, false, true, true><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], s->q32, s->pz, s->fa, s->fb));
+    // FCG direction (reads z~, p_old, r; writes p, q) and update (x, r
+    // read and written, p and q read)
+    FL_TIMED(s, K_FCG_DIRAPPLY, k, 20.0 * n0,
+             k_fcg_dirapply<<<G, kBlock, 0, stm>>>(s->g, s->meb[0], pold, pnew, s->q32, s->r32, s->pz, s->pa, s->fa, s->fb));
+    FL_TIMED(s, K_FCG_UPDATE, k, 24.0 * n0,
+             k_fcg_update<<<G, kBlock, 0, stm>>>((uint32_t)s->n, s->e32, s->r32, pnew, s->q32, s->pa, s->pb, s->fa, s->fb));
+    return FL_OK;
 }
 
 // Inner FCG on A e = r32 (e32 = 0 on entry, r32 and its partials in pb from
@@ -1173,34 +1217,41 @@ fl_status run_fcg(fl_cavity3d *s, const CgParams &prm, bool *stagnated) {
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->fa, &s->host->fa, sizeof(FcgA), hipMemcpyHostToDevice, stm));
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->fb, &s->host->fb, sizeof(FcgB), hipMemcpyHostToDevice, stm));
     *stagnated = false;
-    long long launched = 0, iters_before = 0;
+    // Batches are sized from the observed contraction so that few launches
+    // land after the device-side stop test: the check of iteration k sees the
+    // residual after k updates, so after launching iterations 0..L-1 the
+    // host knows rr_{L-1} and rr_0.
+    long long launched = 0, at_prev = 0;
     int batch = 4;
     double rr_prev = -1.0;
     for (;;) {
-        if (s->timing) FL_HIP_TRY(ctx, hipEventRecord(s->ev[0], stm));
-        for (int b = 0; b < batch; ++b) launch_fcg_iteration(s, prm, launched++);
-        if (s->timing) FL_HIP_TRY(ctx, hipEventRecord(s->ev[1], stm));
+        for (int b = 0; b < batch; ++b) {
+            fl_status st = launch_fcg_iteration(s, prm, launched++);
+            if (st != FL_OK) return st;
+        }
         FL_HIP_TRY(ctx, hipGetLastError());
         FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->fa, s->fa, sizeof(FcgA), hipMemcpyDeviceToHost, stm));
         FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->fb, s->fb, sizeof(FcgB), hipMemcpyDeviceToHost, stm));
         FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
-        if (s->timing) {
-            // whole FCG iterations (V-cycle + direction + update); launches
-            // after the stop test exit at once and add only their launch gaps
-            float ms = 0.0f;
-            FL_HIP_TRY(ctx, hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
-            s->t_apply += ms;
-            s->t_iters += s->host->fb.iters - iters_before;
-        }
-        iters_before = s->host->fb.iters;
+        FL_HIP_TRY(ctx, s->prof.flush(s->host->fb.iters));
         if (s->host->fa.done || s->host->fb.done) break;
-        const double rr = s->host->fa.rr;
-        if (rr_prev >= 0.0 && batch >= 8 && !(rr < 0.25 * rr_prev)) {
+        const double rr = s->host->fa.rr, rr0 = s->host->fa.rr0;
+        const long long at = launched - 1;
+        if (rr_prev < 0.0) rr_prev = rr0;
+        const long long span = at - at_prev;
+        const double f = (span > 0 && rr_prev > 0.0) ? std::pow(rr / rr_prev, 1.0 / (double)span) : 0.5;
+        // stagnation: rr fails to halve over 4 iterations (0.84^4 = 0.5)
+        if (span >= 4 && !(f < 0.84)) {
             *stagnated = true;
             break;
         }
+        long long need = 16;
+        const double target = prm.tol * prm.tol * rr0;
+        if (f > 0.0 && f < 1.0 && rr > 0.0 && target > 0.0)
+            need = (long long)std::ceil(std::log(target / rr) / std::log(f));
+        batch = (int)std::min<long long>(std::max<long long>(need, 1), 16);
         rr_prev = rr;
-        batch = std::min(batch * 2, 16);
+        at_prev = at;
     }
     return FL_OK;
 }
@@ -1220,11 +1271,14 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
     bool converged = false, fallback = false;
     double rr = 0.0, rmax = 0.0;
     for (;;) {
-        k_residual<float><<<s->blocks, kBlock, 0, stm>>>(s->g, s->rhs, s->p, s->r32, s->pb);
-        k_fold_outer<<<1, kBlock, 0, stm>>>(s->pb, s->blocks, s->outer);
+        FL_TIMED(s, K_RESIDUAL_F, -1, 20.0 * s->n,
+                 k_residual<float><<<s->blocks, kBlock, 0, stm>>>(s->g, s->rhs, s->p, s->r32, s->pb));
+        FL_TIMED(s, K_FOLD_OUTER, -1, 32.0 * s->blocks,
+                 k_fold_outer<<<1, kBlock, 0, stm>>>(s->pb, s->blocks, s->outer));
         FL_HIP_TRY(ctx, hipGetLastError());
         FL_HIP_TRY(ctx, hipMemcpyAsync(s->host->outer, s->outer, 3 * sizeof(double), hipMemcpyDeviceToHost, stm));
         FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
+        FL_HIP_TRY(ctx, s->prof.flush());
         rr = s->host->outer[1];
         rmax = s->host->outer[2];
         if (sweeps == 0) rr0 = rr;
@@ -1258,7 +1312,8 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
         total += it;
         sweeps += 1;
         if (status == ST_OK && it > 0)
-            k_add_correction<<<s->ew_blocks, kBlock, 0, stm>>>((uint32_t)s->n, s->p, s->e32);
+            FL_TIMED(s, K_ADD_CORRECTION, -1, 20.0 * s->n,
+                     k_add_correction<<<s->ew_blocks, kBlock, 0, stm>>>((uint32_t)s->n, s->p, s->e32));
         if (mg && (status != ST_OK || stagnated || budget_hit)) {
             fallback = true;
             break;
@@ -1307,14 +1362,18 @@ fl_status run_diag(fl_cavity3d *s, fl_cavity3d_step_report *rep, bool check_fini
     for (int f = 0; f < 4; ++f) init.first_bad[f] = ~0ull;
     s->host->dg = init;
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->dg, &s->host->dg, sizeof(DiagState), hipMemcpyHostToDevice, stm));
-    k_diag<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->u, s->v, s->w, s->partials, s->dg);
+    const double faces = 8.0 * (double)(s->nu + s->nv + s->nw);
+    FL_TIMED(s, K_DIAG, -1, faces,
+             k_diag<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->u, s->v, s->w, s->partials, s->dg));
     if (check_finite)
-        k_finite<<<s->ew_blocks, kBlock, 0, stm>>>(s->p, (uint32_t)s->n, s->u, (uint32_t)s->nu,
-                                                   s->v, (uint32_t)s->nv, s->w, (uint32_t)s->nw,
-                                                   s->dg);
+        FL_TIMED(s, K_FINITE, -1, faces + 8.0 * s->n,
+                 k_finite<<<s->ew_blocks, kBlock, 0, stm>>>(s->p, (uint32_t)s->n, s->u,
+                                                            (uint32_t)s->nu, s->v, (uint32_t)s->nv,
+                                                            s->w, (uint32_t)s->nw, s->dg));
     FL_HIP_TRY(ctx, hipGetLastError());
     FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->dg, s->dg, sizeof(DiagState), hipMemcpyDeviceToHost, stm));
     FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
+    FL_HIP_TRY(ctx, s->prof.flush());
     const DiagState &h = s->host->dg;
     if (rep) {
         rep->max_divergence = h.max_div;
@@ -1339,8 +1398,12 @@ fl_status run_diag(fl_cavity3d *s, fl_cavity3d_step_report *rep, bool check_fini
 fl_status do_step(fl_cavity3d *s, fl_cavity3d_step_report *rep) {
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
-    k_predict<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->u, s->v, s->w, s->us, s->vs, s->ws);
-    k_rhs<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->us, s->vs, s->ws, s->rhs);
+    // algorithmic bytes: every face array read or written once, cells once
+    const double faces = 8.0 * (double)(s->nu + s->nv + s->nw), cells = 8.0 * s->n;
+    FL_TIMED(s, K_PREDICT, -1, 2.0 * faces,
+             k_predict<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->u, s->v, s->w, s->us, s->vs, s->ws));
+    FL_TIMED(s, K_RHS, -1, faces + cells,
+             k_rhs<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->us, s->vs, s->ws, s->rhs));
     FL_HIP_TRY(ctx, hipGetLastError());
     fl_solve_report sr{};
     fl_status st = solve_pressure(s, &sr);
@@ -1352,7 +1415,8 @@ fl_status do_step(fl_cavity3d *s, fl_cavity3d_step_report *rep) {
         rep->pressure_converged = sr.converged;
     }
     if (st != FL_OK) return st;
-    k_correct<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->p, s->us, s->vs, s->ws, s->u, s->v, s->w);
+    FL_TIMED(s, K_CORRECT, -1, 2.0 * faces + cells,
+             k_correct<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->p, s->us, s->vs, s->ws, s->u, s->v, s->w));
     FL_HIP_TRY(ctx, hipGetLastError());
     s->step += 1;
     st = run_diag(s, rep, true);
@@ -1377,6 +1441,7 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
     if (!s) return set_error(ctx, FL_ERR_OUT_OF_MEMORY, "host allocation failed");
     s->ctx = ctx;
     s->cfg = *cfg;
+    s->prof.init(kKindNames, K_COUNT);
     const uint32_t nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
     const double dx = cfg->length / nx, dy = cfg->height / ny, dz = cfg->depth / nz;
     s->g = Grid{nx, ny, nz, nx * ny, nx * ny * nz, 1.0 / (dx * dx), 1.0 / (dy * dy),
@@ -1484,6 +1549,7 @@ fl_status fl_cavity3d_pressure_solve(fl_cavity3d *s, const double *rhs, double *
     fl_status st = solve_pressure(s, rep);
     FL_HIP_TRY(ctx, hipMemcpyAsync(p_inout, s->p, s->n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     FL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    FL_HIP_TRY(ctx, s->prof.flush());
     return st;
 }
 
@@ -1528,30 +1594,32 @@ fl_status fl_cavity3d_get_intermediate(fl_cavity3d *s, double *u_star, double *v
     return FL_OK;
 }
 
-fl_status fl_cavity3d_set_timing(fl_cavity3d *s, int32_t enable) {
+fl_status fl_cavity3d_set_timing(fl_cavity3d *s, int32_t enable, const char *kernel) {
     if (!s) return FL_ERR_INVALID_ARGUMENT;
     fl_ctx *ctx = s->ctx;
-    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
-    if (enable && s->ev.empty()) {
-        s->ev.resize(3 * kMaxBatch);
-        for (auto &e : s->ev) FL_HIP_TRY(ctx, hipEventCreate(&e));
+    int only = -1;
+    if (kernel && kernel[0]) {
+        only = s->prof.find(kernel);
+        if (only < 0) return set_error(ctx, FL_ERR_INVALID_ARGUMENT, "unknown kernel '%s'", kernel);
     }
-    s->timing = enable != 0;
-    s->t_apply = s->t_update = 0.0;
-    s->t_iters = 0;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    FL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    s->prof.on = enable != 0;
+    s->prof.only = only;
+    s->prof.clear();
     return FL_OK;
 }
 
-fl_status fl_cavity3d_kernel_times(fl_cavity3d *s, double *ms_per_iteration, double *ms_apply,
-                                   double *ms_update, double *ms_direction,
-                                   int64_t *timed_iterations) {
-    if (!s) return FL_ERR_INVALID_ARGUMENT;
-    const double n = s->t_iters > 0 ? (double)s->t_iters : 1.0;
-    if (ms_apply) *ms_apply = s->t_apply / n;
-    if (ms_update) *ms_update = s->t_update / n;
-    if (ms_direction) *ms_direction = 0.0;  // fused into the apply kernel
-    if (ms_per_iteration) *ms_per_iteration = (s->t_apply + s->t_update) / n;
-    if (timed_iterations) *timed_iterations = s->t_iters;
+fl_status fl_cavity3d_kernel_stats(fl_cavity3d *s, int32_t index, fl_kernel_stat *out) {
+    if (!s || !out) return FL_ERR_INVALID_ARGUMENT;
+    if (index < 0 || index >= s->prof.kinds)
+        return set_error(s->ctx, FL_ERR_INVALID_ARGUMENT, "kernel index %d out of range [0, %d)",
+                         index, s->prof.kinds);
+    std::memset(out, 0, sizeof *out);
+    std::snprintf(out->name, sizeof out->name, "%s", s->prof.names[index]);
+    out->launches = s->prof.launches[index];
+    out->total_ms = s->prof.ms[index];
+    out->bytes = s->prof.bytes[index];
     return FL_OK;
 }
 

@@ -154,13 +154,22 @@ fl_status fl_cavity3d_set_fields(fl_cavity3d *solver, const double *u, const dou
 /* The predicted velocities u*, v*, w* and the pressure RHS of the last step. */
 fl_status fl_cavity3d_get_intermediate(fl_cavity3d *solver, double *u_star, double *v_star,
                                        double *w_star, double *rhs);
-/* Average device time (ms) of the FP64 (or FP32) PCG kernels of the last
- * solve, measured with HIP events on the solver's stream; zero when timing
- * is off.  Enables timing when `enable` is non-zero. */
-fl_status fl_cavity3d_set_timing(fl_cavity3d *solver, int32_t enable);
-fl_status fl_cavity3d_kernel_times(fl_cavity3d *solver, double *ms_per_iteration,
-                                   double *ms_apply, double *ms_update, double *ms_direction,
-                                   int64_t *timed_iterations);
+/* Per-kernel device timing (no counterpart in the reference; for
+ * benchmarking).  With `enable` non-zero, launches of the kernel named
+ * `kernel` -- or of every kernel when it is NULL or "" -- are bracketed by
+ * HIP events on the solver's stream; launches issued after a device-side
+ * stop test has fired (they exit at once) are not counted.  Resets the
+ * statistics.  Unknown names are FL_ERR_INVALID_ARGUMENT. */
+typedef struct {
+    char name[64];    /* kernel name with template arguments, as profilers show it */
+    int64_t launches; /* timed launches */
+    double total_ms;  /* summed device time of those launches */
+    double bytes;     /* summed algorithmic HBM bytes of those launches */
+} fl_kernel_stat;
+fl_status fl_cavity3d_set_timing(fl_cavity3d *solver, int32_t enable, const char *kernel);
+/* Statistics of kernel `index` (0-based, in a fixed order); past the last
+ * kernel FL_ERR_INVALID_ARGUMENT. */
+fl_status fl_cavity3d_kernel_stats(fl_cavity3d *solver, int32_t index, fl_kernel_stat *out);
 
 /* ---------------------------------------------------------------------- */
 /* 2D structured projection solver: IncompressibleSolver with projection
