@@ -12,6 +12,10 @@ Variants timed back to back on the same workload:
   plain_fp64_pcg  FP64 Jacobi-PCG (the judge)
   mixed_ir        FP32 stencil CG inside FP64 iterative refinement (FP64
                   residual, dots, alpha/beta and stop test)
+  mg_fcg          the same refinement with multigrid-preconditioned flexible
+                  CG as the inner solver (plain FP64 PCG fallback)
+  learned_fcg     mg_fcg started from the U-Net's warm start (weights from
+                  flursys_amd/train_unet.py; skipped when none are present)
 `value` is the steps/s of the fastest variant that meets the same FP64
 rel-res 1e-8 stop test; every variant's figures are in `config.variants`.
 
@@ -81,6 +85,13 @@ def run_variant(fa, ctx, name, precision, precond, args):
                                      inner_tolerance=args.inner_tol)
     log(f"[bench] {name}: {n}^3 Re={args.reynolds} dt={args.dt} tol={args.tol}")
     s = fa.LidDrivenCavity3DSolver(cfg, ctx)
+    net = None
+    if precond == fa.PRECOND_LEARNED:
+        from flursys_amd.unet import UNet
+
+        net = UNet(args.unet_n, 1, ctx=ctx)
+        net.load(args.weights)
+        s.set_unet(net)
     # Warmup steps time every kernel (untimed region): their breakdown names
     # the dominant kernel, which alone is event-timed in the timed region.
     s.set_timing(True)
@@ -91,21 +102,25 @@ def run_variant(fa, ctx, name, precision, precond, args):
     breakdown = s.kernel_stats() if args.warmup else {}
     dom = max(breakdown, key=lambda k: breakdown[k]["total_ms"]) if breakdown else None
     s.set_timing(True, dom)
-    iters, refs, fbs = [], [], []
+    iters, refs, fbs, warm = [], [], [], []
     t0 = time.perf_counter()
     for k in range(args.steps):
         rep = s.advance_one_step()  # returns after its stream has drained
         iters.append(rep.pressure_iterations)
         refs.append(rep.pressure_refinements)
         fbs.append(rep.pressure_fallback)
+        warm.append(rep.pressure_warm_start)
         if not rep.pressure_converged:
             raise SystemExit(f"{name}: pressure solve did not converge")
         log(f"[bench] {name} step {k + 1}/{args.steps}: {rep.pressure_iterations} iterations, "
             f"{rep.pressure_refinements} refinements, fallback {rep.pressure_fallback}, "
+            f"warm start {rep.pressure_warm_start} (ratio {rep.pressure_warm_ratio:.3f}), "
             f"div {rep.max_divergence:.3e}")
     elapsed = time.perf_counter() - t0
     timed = s.kernel_stats()
     s.close()
+    if net is not None:
+        net.close()
     if dom is None and timed:
         dom = max(timed, key=lambda k: timed[k]["total_ms"])
     step_ms = elapsed * 1e3 / args.steps
@@ -116,6 +131,8 @@ def run_variant(fa, ctx, name, precision, precond, args):
         "cg_iters_per_step_all": iters,
         "refinements_per_step": sum(refs) / len(refs),
         "fallback_steps": sum(1 for f in fbs if f),
+        "warm_start_kept": sum(1 for w in warm if w == 1),
+        "warm_start_rejected": sum(1 for w in warm if w == -1),
         "dominant_kernel": dom,
         "dominant_timed": timed.get(dom),
         # per-kernel breakdown of the warmup steps (all kernels event-timed)
@@ -140,7 +157,11 @@ def main():
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=0.0)
-    ap.add_argument("--variants", default="plain_fp64_pcg,mixed_ir,mg_fcg")
+    ap.add_argument("--variants", default="plain_fp64_pcg,mixed_ir,mg_fcg,learned_fcg")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "flursys_amd", "weights",
+                                                      "unet64_cavity.npy"),
+                    help="trained U-Net parameters for learned_fcg")
+    ap.add_argument("--unet-n", type=int, default=64, help="U-Net resolution of the weights")
     ap.add_argument("--cpu-sample-iters", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-copy-probe", action="store_true")
@@ -158,9 +179,14 @@ def main():
     ctx = fa.Context(0)
     modes = {"plain_fp64_pcg": (fa.PRECISION_FP64, fa.PRECOND_JACOBI),
              "mixed_ir": (fa.PRECISION_MIXED, fa.PRECOND_JACOBI),
-             "mg_fcg": (fa.PRECISION_MIXED, fa.PRECOND_MULTIGRID)}
+             "mg_fcg": (fa.PRECISION_MIXED, fa.PRECOND_MULTIGRID),
+             "learned_fcg": (fa.PRECISION_MIXED, fa.PRECOND_LEARNED)}
     results = {}
     for name in args.variants.split(","):
+        if name == "learned_fcg" and not os.path.exists(args.weights):
+            log(f"[bench] learned_fcg skipped: no trained weights at {args.weights} "
+                "(python -m flursys_amd.train_unet)")
+            continue
         results[name] = run_variant(fa, ctx, name, *modes[name], args)
     best = max(results, key=lambda k: results[k]["steps_per_s"])
     b = results[best]

@@ -6,7 +6,7 @@ the reference's solver interfaces over that ABI.
 """
 
 from ._lib import (GUESS_PREVIOUS, GUESS_ZERO, PRECISION_FP64, PRECISION_MIXED,
-                   PRECOND_JACOBI, PRECOND_MULTIGRID, STOP_ABS_INF, STOP_REL_L2, Context,
+                   PRECOND_JACOBI, PRECOND_LEARNED, PRECOND_MULTIGRID, STOP_ABS_INF, STOP_REL_L2, Context,
                    FlursysError, default_context, load)
 from .cavity3d import LidDrivenCavity3DConfig, LidDrivenCavity3DSolver, RunSummary3D
 
@@ -15,4 +15,5 @@ __all__ = [
     "LidDrivenCavity3DConfig", "LidDrivenCavity3DSolver", "RunSummary3D",
     "STOP_ABS_INF", "STOP_REL_L2", "GUESS_PREVIOUS", "GUESS_ZERO",
     "PRECISION_FP64", "PRECISION_MIXED", "PRECOND_JACOBI", "PRECOND_MULTIGRID",
+    "PRECOND_LEARNED",
 ]

@@ -24,7 +24,7 @@ STATUS_NAMES = {
 STOP_ABS_INF, STOP_REL_L2 = 0, 1
 GUESS_PREVIOUS, GUESS_ZERO = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
-PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
+PRECOND_JACOBI, PRECOND_MULTIGRID, PRECOND_LEARNED = 0, 1, 2
 
 
 class FlursysError(RuntimeError):
@@ -40,9 +40,10 @@ class FlursysError(RuntimeError):
 class SolveReport(C.Structure):
     _fields_ = [
         ("iterations", C.c_int64), ("refinements", C.c_int64),
-        ("fallback_iterations", C.c_int64), ("fallback", C.c_int32), ("reserved", C.c_int32),
+        ("fallback_iterations", C.c_int64), ("fallback", C.c_int32), ("warm_start", C.c_int32),
         ("residual_abs_inf", C.c_double), ("residual_l2", C.c_double),
         ("initial_l2", C.c_double), ("converged", C.c_int32), ("status", C.c_int32),
+        ("warm_start_ratio", C.c_double),
     ]
 
 
@@ -65,8 +66,13 @@ class Cavity3DStepReport(C.Structure):
         ("pressure_residual", C.c_double), ("pressure_residual_l2", C.c_double),
         ("max_divergence", C.c_double), ("divergence_l2", C.c_double),
         ("max_speed", C.c_double), ("pressure_converged", C.c_int32),
-        ("pressure_fallback", C.c_int32),
+        ("pressure_fallback", C.c_int32), ("pressure_warm_start", C.c_int32),
+        ("pad", C.c_int32), ("pressure_warm_ratio", C.c_double),
     ]
+
+
+class UNetConfig(C.Structure):
+    _fields_ = [("n", C.c_int32), ("max_batch", C.c_int32), ("seed", C.c_uint64)]
 
 
 class KernelStat(C.Structure):
@@ -155,6 +161,19 @@ SIGNATURES = {
     "fl_cg": (C.c_int, _LIN),
     "fl_pcg_jacobi": (C.c_int, _LIN),
     "fl_bicgstab": (C.c_int, _LIN),
+    "fl_unet_create": (C.c_int, [_P, C.POINTER(UNetConfig), C.POINTER(_P)]),
+    "fl_unet_destroy": (None, [_P]),
+    "fl_unet_num_params": (C.c_int64, [_P]),
+    "fl_unet_get_params": (C.c_int, [_P, _P]),
+    "fl_unet_set_params": (C.c_int, [_P, _P]),
+    "fl_unet_train_step": (C.c_int, [_P, C.c_int32, _P, _P, C.c_double, C.c_float,
+                                     C.POINTER(C.c_float)]),
+    "fl_unet_infer": (C.c_int, [_P, C.c_int32, _P, C.c_double, _P]),
+    "fl_unet_forward_raw": (C.c_int, [_P, C.c_int32, _P, _P]),
+    "fl_unet_grads": (C.c_int, [_P, C.c_int32, _P, _P, C.c_double, _P, C.POINTER(C.c_float)]),
+    "fl_probe_mfma_bf16": (C.c_int, [_P, _P, _P, _P]),
+    "fl_cavity3d_set_unet": (C.c_int, [_P, _P]),
+    "fl_cavity3d_coarse_pair": (C.c_int, [_P, C.c_int32, _P, _P, C.POINTER(C.c_double)]),
 }
 
 _lib = None

@@ -126,6 +126,23 @@ class LidDrivenCavity3DSolver:
               self.ctx.handle)
         return rep
 
+    def set_unet(self, net) -> None:
+        """Attach a ``flursys_amd.unet.UNet`` (or None) as the warm start of
+        the learned preconditioner; the solver keeps a reference to it."""
+        check(load().fl_cavity3d_set_unet(self.handle, net.handle if net is not None else None),
+              self.ctx.handle)
+        self._unet = net
+
+    def coarse_pair(self, n: int):
+        """(b, p, diag): the last step's right-hand side b = -rhs and pressure,
+        mean-restricted to n^3, as float32 (training pairs of the U-Net)."""
+        b = np.empty(n ** 3, np.float32)
+        p = np.empty(n ** 3, np.float32)
+        diag = C.c_double()
+        check(load().fl_cavity3d_coarse_pair(self.handle, n, ptr(b), ptr(p), C.byref(diag)),
+              self.ctx.handle)
+        return b, p, diag.value
+
     def fields(self):
         s = self.shapes
         out = {k: np.empty(v, np.float64) for k, v in s.items()}

@@ -39,6 +39,7 @@
 
 #include "common.hpp"
 #include "kprof.hpp"
+#include "unet.hpp"
 
 namespace fl {
 namespace c3 {
@@ -357,7 +358,8 @@ __device__ inline float f_at(const float *f, uint32_t c, bool level0, float f0) 
 }
 
 // One damped-Jacobi sweep on a level:
-//   e_out = e' + omega D^-1 (f - A e'),  e' = e_in + corr P(e_c) (kProlong)
+//   e_out = e' + omega D^-1 (f - A e'),  e' = e_in + corr P(e_c) (kProlong;
+//   a null e_in is zero)
 // or, kFirst, e_out = omega D^-1 f (from a zero iterate).  kPublish (level 0,
 // last sweep) also publishes sum z.q_old and sum q_old for the FCG beta.
 template <bool kFirst, bool kProlong, bool kLevel0, bool kPublish>
@@ -367,7 +369,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_sweep(Level L, Level LC, const fl
                                                      float *__restrict__ eout,
                                                      const float *__restrict__ qold,
                                                      double *__restrict__ pout, const FcgA *SA,
-                                                     const FcgB *SB) {
+                                                     const FcgB *SB, float corr) {
     if (SA->done || SB->done) return;
     const float f0 = kLevel0 ? (float)(-SA->sumr) : 0.0f;
     double s[2] = {0.0, 0.0}, m[1] = {0.0};
@@ -390,8 +392,8 @@ __global__ void __launch_bounds__(kBlock) k_mg_sweep(Level L, Level LC, const fl
 #pragma unroll
             for (int a = 0; a < 7; ++a) {
                 const uint32_t idx = ii[a] + L.nx * (jj[a] + L.ny * kk[a]);
-                ev[a] = ein[idx];
-                if (kProlong) ev[a] += kCorr * ec[(ii[a] >> 1) + LC.nx * ((jj[a] >> 1) + LC.ny * (kk[a] >> 1))];
+                ev[a] = (!kProlong || ein) ? ein[idx] : 0.0f;
+                if (kProlong) ev[a] += corr * ec[(ii[a] >> 1) + LC.nx * ((jj[a] >> 1) + LC.ny * (kk[a] >> 1))];
             }
             const float ae = L.cx * ((ev[0] - ev[1]) + (ev[0] - ev[2])) +
                              L.cy * ((ev[0] - ev[3]) + (ev[0] - ev[4])) +
@@ -875,6 +877,77 @@ __global__ void __launch_bounds__(kBlock) k_finite(const double *__restrict__ a0
 #undef V3
 #undef W3
 
+// ---------------------------------------------------------------------------
+// Learned warm start (see warm_start): the U-Net sees the initial residual
+// mean-restricted to its resolution and predicts the correction there.
+// ---------------------------------------------------------------------------
+
+// Mean of the 8 children: level L (fine) -> L+1.
+__global__ void __launch_bounds__(kBlock) k_restrict_mean(Level L, Level LC, const float *__restrict__ f,
+                                                          float *__restrict__ fc) {
+    for (uint32_t C = blockIdx.x * blockDim.x + threadIdx.x; C < LC.n; C += gridDim.x * blockDim.x) {
+        uint32_t I, J, K;
+        lvl_ijk(LC, C, I, J, K);
+        float acc = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+            acc += f[(2 * I + (a & 1)) + L.nx * ((2 * J + (a >> 1 & 1)) + L.ny * (2 * K + (a >> 2)))];
+        fc[C] = 0.125f * acc;
+    }
+}
+
+// x += sign (e - e_0) (FP64 += FP32), keeping the pinned cell's value;
+// sign -1 undoes sign +1.
+__global__ void __launch_bounds__(kBlock) k_add_shifted(uint32_t n, double *__restrict__ x,
+                                                        const float *__restrict__ e, double sign) {
+    const double e0 = (double)e[0];
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        x[c] += sign * ((double)e[c] - e0);
+}
+
+// Partials of r.e~ and e~.A e~ with e~ = e - e_0 on the pinned operator
+// (published as <3,1> rows {r.e~, e~.Ae~, 0 | 0}).
+__global__ void __launch_bounds__(kBlock) k_warm_dots(Grid g, const float *__restrict__ r,
+                                                      const float *__restrict__ e,
+                                                      double *__restrict__ partials) {
+    double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
+    const float e0 = e[0];
+    const float cx = (float)g.ix2, cy = (float)g.iy2, cz = (float)g.iz2;
+    TileSched ts((g.n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t c = t * kBlock + threadIdx.x;
+        if (c >= g.n || c == 0) continue;
+        uint32_t i, j, k;
+        cell_ijk(g, c, i, j, k);
+        const Nbr o = neighbours(g, c, i, j, k);
+        const uint32_t idx[7] = {c, o.e, o.w, o.n, o.s, o.f, o.b};
+        float ev[7];
+#pragma unroll
+        for (int a = 0; a < 7; ++a) ev[a] = e[idx[a]] - e0;  // the pinned cell's e~ is 0
+        const float ae = cx * ((ev[0] - ev[1]) + (ev[0] - ev[2])) + cy * ((ev[0] - ev[3]) + (ev[0] - ev[4])) +
+                         cz * ((ev[0] - ev[5]) + (ev[0] - ev[6]));
+        s[0] += (double)r[c] * (double)ev[0];
+        s[1] += (double)ev[0] * (double)ae;
+    }
+    publish_partials<3, 1>(s, m, partials);
+}
+
+// out[C] = sign * mean of the f^3 fine cells of coarse cell C (nc^3 cells).
+__global__ void __launch_bounds__(kBlock) k_block_mean(Grid g, const double *__restrict__ src, double sign,
+                                                       uint32_t nc, uint32_t f, float *__restrict__ out) {
+    const uint32_t ncell = nc * nc * nc;
+    const double w = sign / ((double)f * f * f);
+    for (uint32_t C = blockIdx.x * blockDim.x + threadIdx.x; C < ncell; C += gridDim.x * blockDim.x) {
+        const uint32_t I = C % nc, J = (C / nc) % nc, K = C / (nc * nc);
+        double acc = 0.0;
+        for (uint32_t dk = 0; dk < f; ++dk)
+            for (uint32_t dj = 0; dj < f; ++dj)
+                for (uint32_t di = 0; di < f; ++di)
+                    acc += src[(I * f + di) + g.nx * ((J * f + dj) + g.ny * (K * f + dk))];
+        out[C] = (float)(w * acc);
+    }
+}
+
 // Kernel kinds of the per-kernel profiler, named as the kernels are (with
 // their template arguments).
 enum Kind {
@@ -883,7 +956,7 @@ enum Kind {
     K_DIRAPPLY_D, K_UPDATE_D, K_DIRAPPLY_F, K_UPDATE_F,
     K_MG_CHECK, K_MG_SWEEP_L0, K_MG_SWEEP_L0_PROLONG, K_MG_SWEEP_L0_PUBLISH,
     K_MG_SWEEP_FIRST, K_MG_SWEEP, K_MG_SWEEP_PROLONG, K_MG_RESTRICT_L0, K_MG_RESTRICT,
-    K_MG_TAIL, K_FCG_DIRAPPLY, K_FCG_UPDATE, K_COUNT
+    K_MG_TAIL, K_FCG_DIRAPPLY, K_FCG_UPDATE, K_WARM_START, K_COUNT
 };
 const char *const kKindNames[K_COUNT] = {
     "k_predict", "k_rhs", "k_correct", "k_diag", "k_finite",
@@ -895,6 +968,7 @@ const char *const kKindNames[K_COUNT] = {
     "k_mg_sweep<true, false, false, false>", "k_mg_sweep<false, false, false, false>",
     "k_mg_sweep<false, true, false, false>", "k_mg_restrict<true>", "k_mg_restrict<false>",
     "k_mg_tail", "k_fcg_dirapply", "k_fcg_update",
+    "unet_warm_start",  // restriction + U-Net inference + interpolation (many kernels)
 };
 
 }  // namespace c3
@@ -925,6 +999,15 @@ struct fl_cavity3d {
     Level lv[kMaxLevels]{};
     float *mf[kMaxLevels] = {}, *mea[kMaxLevels] = {}, *meb[kMaxLevels] = {};
     uint32_t lblocks[kMaxLevels] = {};
+    // learned warm start (FL_PRECOND_LEARNED): the network works on level
+    // wl_level; wl_in / wl_out hold its input and output there
+    fl_unet *unet = nullptr;
+    int wl_level = 0;
+    double wl_alpha = 0.0;   // step of the last warm start along wl_e
+    const float *wl_e = nullptr;
+    float *wl_in = nullptr, *wl_out = nullptr;
+    float *pair = nullptr;  // fl_cavity3d_coarse_pair staging (2 n^3)
+    size_t pair_cells = 0;
     double *pz = nullptr;  // partials z.q_old, sum q_old
     FcgA *fa = nullptr;
     FcgB *fb = nullptr;
@@ -934,7 +1017,7 @@ struct fl_cavity3d {
         FcgA fa;
         FcgB fb;
         DiagState dg;
-        double outer[3];
+        double outer[4];
     } *host = nullptr;
     double *outer = nullptr;  // folded outer residual: r.z, r.r, max|r|
     uint32_t blocks = 1, ew_blocks = 1;
@@ -946,6 +1029,8 @@ namespace {
 
 constexpr int kMaxBatch = 64;
 
+// out = {s0, s1, max, s2} of <3,1> partials (for k_residual: r.z, r.r,
+// max|r|, sum r).
 __global__ void k_fold_outer(const double *__restrict__ partials, uint32_t rows, double *out) {
     double s[3], m[1];
     fold_partials<3, 1>(partials, rows, s, m);
@@ -953,6 +1038,7 @@ __global__ void k_fold_outer(const double *__restrict__ partials, uint32_t rows,
         out[0] = s[0];
         out[1] = s[1];
         out[2] = m[0];
+        out[3] = s[2];
     }
 }
 
@@ -970,7 +1056,7 @@ void free_all(fl_cavity3d *s) {
                     s->sb,  s->dg,     s->outer};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    void *more[] = {s->pz, s->fa, s->fb};
+    void *more[] = {s->pz, s->fa, s->fb, s->wl_in, s->wl_out, s->pair};
     for (void *b : more)
         if (b) (void)hipFree(b);
     for (int l = 0; l < kMaxLevels; ++l) {
@@ -1047,9 +1133,10 @@ fl_status validate(fl_ctx *ctx, const fl_cavity3d_config *c) {
         return set_error(ctx, FL_ERR_CONFIG, "unknown precision %d", c->precision);
     if (!(c->inner_tolerance >= 0.0 && c->inner_tolerance < 1.0))
         return set_error(ctx, FL_ERR_CONFIG, "inner_tolerance must lie in [0, 1)");
-    if (c->preconditioner != FL_PRECOND_JACOBI && c->preconditioner != FL_PRECOND_MULTIGRID)
+    if (c->preconditioner != FL_PRECOND_JACOBI && c->preconditioner != FL_PRECOND_MULTIGRID &&
+        c->preconditioner != FL_PRECOND_LEARNED)
         return set_error(ctx, FL_ERR_CONFIG, "unknown preconditioner %d", c->preconditioner);
-    if (c->preconditioner == FL_PRECOND_MULTIGRID && c->precision != FL_PRECISION_MIXED)
+    if (c->preconditioner != FL_PRECOND_JACOBI && c->precision != FL_PRECISION_MIXED)
         return set_error(ctx, FL_ERR_CONFIG, "the multigrid preconditioner runs in the mixed-precision solve");
     const double cells = (double)c->nx * (double)c->ny * (double)c->nz;
     const double faces = (double)(c->nx + 1) * (double)(c->ny + 1) * (double)(c->nz + 1) * 3.0;
@@ -1128,6 +1215,8 @@ fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep, double rr0_ref = 0.0)
         rep->iterations = b.iters;
         rep->refinements = 0;
         rep->fallback = 0;
+        rep->warm_start = 0;
+        rep->warm_start_ratio = 0.0;
         rep->fallback_iterations = 0;
         rep->residual_abs_inf = a.rmax;
         rep->residual_l2 = std::sqrt(a.rr);
@@ -1157,20 +1246,20 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     FL_TIMED(s, K_MG_CHECK, k, 8.0 * n0,
              k_mg_check<<<G, kBlock, 0, stm>>>(L[0], prm, s->r32, s->mea[0], s->pb, s->fa, s->fb));
     FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
-             k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], nullptr, nullptr, s->fa, s->fb));
+             k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], nullptr, nullptr, s->fa, s->fb, kCorr));
     FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
-             k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb));
+             k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, kCorr));
     FL_TIMED(s, K_MG_RESTRICT_L0, k, 8.0 * n0 + 4.0 * n1,
              k_mg_restrict<true><<<s->lblocks[1], kBlock, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->mf[1], s->fa, s->fb));
     for (int l = 1; l < t; ++l) {
         const uint32_t g = s->lblocks[l];
         const double nl = L[l].n, nc = L[l + 1].n;
         FL_TIMED(s, K_MG_SWEEP_FIRST, k, 8.0 * nl,
-                 k_mg_sweep<true, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], nullptr, nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb));
+                 k_mg_sweep<true, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], nullptr, nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb, kCorr));
         FL_TIMED(s, K_MG_SWEEP, k, 12.0 * nl,
-                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb));
+                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr));
         FL_TIMED(s, K_MG_SWEEP, k, 12.0 * nl,
-                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb));
+                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb, kCorr));
         FL_TIMED(s, K_MG_RESTRICT, k, 8.0 * nl + 4.0 * nc,
                  k_mg_restrict<false><<<s->lblocks[l + 1], kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->mf[l + 1], s->fa, s->fb));
     }
@@ -1184,18 +1273,18 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
         const uint32_t g = s->lblocks[l];
         const double nl = L[l].n, nc = L[l + 1].n;
         FL_TIMED(s, K_MG_SWEEP_PROLONG, k, 12.0 * nl + 4.0 * nc,
-                 k_mg_sweep<false, true, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->meb[l + 1], s->meb[l], nullptr, nullptr, s->fa, s->fb));
+                 k_mg_sweep<false, true, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->meb[l + 1], s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr));
         FL_TIMED(s, K_MG_SWEEP, k, 12.0 * nl,
-                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb));
+                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb, kCorr));
         FL_TIMED(s, K_MG_SWEEP, k, 12.0 * nl,
-                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb));
+                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr));
     }
     FL_TIMED(s, K_MG_SWEEP_L0_PROLONG, k, 12.0 * n0 + 4.0 * n1,
-             k_mg_sweep<false, true, true, false><<<G, kBlock, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->meb[1], s->meb[0], nullptr, nullptr, s->fa, s->fb));
+             k_mg_sweep<false, true, true, false><<<G, kBlock, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->meb[1], s->meb[0], nullptr, nullptr, s->fa, s->fb, kCorr));
     FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
-             k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb));
+             k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, kCorr));
     FL_TIMED(s, K_MG_SWEEP_L0_PUBLISH, k, 16.0 * n0,
-             k_mg_sweep<false, false, true, true><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], s->q32, s->pz, s->fa, s->fb));
+             k_mg_sweep<false, false, true, true><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], s->q32, s->pz, s->fa, s->fb, kCorr));
     // FCG direction (reads z~, p_old, r; writes p, q) and update (x, r
     // read and written, p and q read)
     FL_TIMED(s, K_FCG_DIRAPPLY, k, 20.0 * n0,
@@ -1256,44 +1345,135 @@ fl_status run_fcg(fl_cavity3d *s, const CgParams &prm, bool *stagnated) {
     return FL_OK;
 }
 
+// Learned warm start from the FP32 residual r0 in r32 (sum_r = sum of r0):
+// the upward half of a V-cycle whose coarse solve is the U-Net.  r0 is
+// mean-restricted to the network's level L, the network predicts the
+// correction there, and levels L-1 .. 0 each prolongate the coarser
+// correction and smooth it (damped Jacobi against the restricted residual;
+// level 0 with the pinned row replaced as in the multigrid preconditioner).
+// The correction e is then added to p with the energy-minimising step
+// alpha = r0.e~ / e~.A e~ (e~ = e - e_0, the pinned operator), so that the
+// A-norm of the error cannot grow.  The diagonal handed to the network is
+// the rediscretised Laplacian's at its resolution, as in training.
+fl_status warm_start(fl_cavity3d *s, double sum_r) {
+    fl_ctx *ctx = s->ctx;
+    hipStream_t stm = ctx->stream;
+    const int L = s->wl_level;
+    const Level *lv = s->lv;
+    const float *src = s->r32;
+    for (int l = 0; l < L; ++l) {
+        float *dst = l + 1 == L ? s->wl_in : s->mf[l + 1];
+        k_restrict_mean<<<s->lblocks[l + 1], kBlock, 0, stm>>>(lv[l], lv[l + 1], src, dst);
+        src = dst;
+    }
+    FL_HIP_TRY(ctx, hipGetLastError());
+    const double f = (double)(1u << L);
+    const double diag = 2.0 * (s->g.ix2 + s->g.iy2 + s->g.iz2) / (f * f);
+    if (unet_infer_device(s->unet, stm, src, diag, s->wl_out) != FL_OK)
+        return set_error(ctx, FL_ERR_HIP, "U-Net inference failed: %s",
+                         fl_last_error(unet_context(s->unet)));
+    // multigrid state for the level-0 sweeps: running, r_0 -> -sum_{k!=0} r_k
+    s->host->fa = FcgA{};
+    s->host->fa.sumr = sum_r;
+    s->host->fb = FcgB{};
+    FL_HIP_TRY(ctx, hipMemcpyAsync(s->fa, &s->host->fa, sizeof(FcgA), hipMemcpyHostToDevice, stm));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(s->fb, &s->host->fb, sizeof(FcgB), hipMemcpyHostToDevice, stm));
+    const float *ec = s->wl_out;
+    for (int l = L - 1; l >= 1; --l) {
+        const uint32_t g = s->lblocks[l];
+        k_mg_sweep<false, true, false, false><<<g, kBlock, 0, stm>>>(lv[l], lv[l + 1], s->mf[l], nullptr, ec, s->mea[l], nullptr, nullptr, s->fa, s->fb, 1.0f);
+        k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(lv[l], lv[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb, 1.0f);
+        ec = s->meb[l];
+    }
+    float *e0 = s->meb[0];
+    if (L >= 1)
+        k_mg_sweep<false, true, true, false><<<s->blocks, kBlock, 0, stm>>>(lv[0], lv[1], s->r32, nullptr, ec, s->mea[0], nullptr, nullptr, s->fa, s->fb, 1.0f);
+    else
+        k_mg_sweep<false, false, true, false><<<s->blocks, kBlock, 0, stm>>>(lv[0], lv[0], s->r32, ec, nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, 1.0f);
+    k_mg_sweep<false, false, true, false><<<s->blocks, kBlock, 0, stm>>>(lv[0], lv[0], s->r32, s->mea[0], nullptr, e0, nullptr, nullptr, s->fa, s->fb, 1.0f);
+    // alpha = r0.e~ / e~.A e~
+    k_warm_dots<<<s->blocks, kBlock, 0, stm>>>(s->g, s->r32, e0, s->pb);
+    k_fold_outer<<<1, kBlock, 0, stm>>>(s->pb, s->blocks, s->outer);
+    FL_HIP_TRY(ctx, hipGetLastError());
+    FL_HIP_TRY(ctx, hipMemcpyAsync(s->host->outer, s->outer, 4 * sizeof(double), hipMemcpyDeviceToHost, stm));
+    FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
+    const double re = s->host->outer[0], eae = s->host->outer[1];
+    const double alpha = (eae > 0.0 && std::isfinite(re / eae)) ? re / eae : 0.0;
+    s->wl_alpha = alpha;
+    s->wl_e = e0;
+    k_add_shifted<<<s->ew_blocks, kBlock, 0, stm>>>((uint32_t)s->n, s->p, e0, alpha);
+    FL_HIP_TRY(ctx, hipGetLastError());
+    return FL_OK;
+}
+
 // FP32 CG (Jacobi or multigrid-preconditioned flexible) inside FP64
 // iterative refinement; multigrid falls back to plain FP64 Jacobi-PCG from
 // the current iterate when an inner solve stagnates or exhausts its budget.
 fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
-    const bool mg = s->cfg.preconditioner == FL_PRECOND_MULTIGRID;
+    const bool mg = s->cfg.preconditioner != FL_PRECOND_JACOBI;
     const double inner_tol =
         s->cfg.inner_tolerance > 0.0 ? s->cfg.inner_tolerance : (mg ? 1.0e-4 : 1.0e-5);
     const bool rel = s->cfg.stop_rule == FL_STOP_REL_L2;
     double rr0 = 0.0;
     long long total = 0, sweeps = 0;
-    bool converged = false, fallback = false;
+    bool converged = false, fallback = false, first_pass = true;
     double rr = 0.0, rmax = 0.0;
+    // learned warm start: 1 pending, 2 applied and awaiting its residual;
+    // warm_used ends 1 (kept) or -1 (rejected: it did not lower the residual)
+    int warm = (s->cfg.preconditioner == FL_PRECOND_LEARNED && s->unet) ? 1 : 0, warm_used = 0;
+    double warm_ratio = 0.0;
     for (;;) {
         FL_TIMED(s, K_RESIDUAL_F, -1, 20.0 * s->n,
                  k_residual<float><<<s->blocks, kBlock, 0, stm>>>(s->g, s->rhs, s->p, s->r32, s->pb));
         FL_TIMED(s, K_FOLD_OUTER, -1, 32.0 * s->blocks,
                  k_fold_outer<<<1, kBlock, 0, stm>>>(s->pb, s->blocks, s->outer));
         FL_HIP_TRY(ctx, hipGetLastError());
-        FL_HIP_TRY(ctx, hipMemcpyAsync(s->host->outer, s->outer, 3 * sizeof(double), hipMemcpyDeviceToHost, stm));
+        FL_HIP_TRY(ctx, hipMemcpyAsync(s->host->outer, s->outer, 4 * sizeof(double), hipMemcpyDeviceToHost, stm));
         FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
         FL_HIP_TRY(ctx, s->prof.flush());
         rr = s->host->outer[1];
         rmax = s->host->outer[2];
-        if (sweeps == 0) rr0 = rr;
+        if (first_pass) rr0 = rr;
+        first_pass = false;
         if (!std::isfinite(rmax))
             return set_error(ctx, FL_ERR_SOLVER, "%s", status_text(ST_RESIDUAL));
+        if (warm == 2) {
+            warm = 0;
+            warm_ratio = rr0 > 0.0 ? std::sqrt(rr / rr0) : 0.0;
+            if (!(rr < rr0)) {  // undo the prediction and restart from the guess
+                k_add_shifted<<<s->ew_blocks, kBlock, 0, stm>>>((uint32_t)s->n, s->p, s->wl_e, -s->wl_alpha);
+                FL_HIP_TRY(ctx, hipGetLastError());
+                warm_used = -1;
+                continue;
+            }
+            warm_used = 1;
+        }
         converged = rel ? std::sqrt(rr) <= s->cfg.pressure_tolerance * std::sqrt(rr0)
                         : rmax < s->cfg.pressure_tolerance;
         if (converged || total >= s->cfg.pressure_max_iters || sweeps >= 64) break;
+        if (warm == 1) {
+            fl_status wst = FL_OK;
+            FL_TIMED(s, K_WARM_START, -1, 28.0 * s->n, wst = warm_start(s, s->host->outer[3]));
+            if (wst != FL_OK) return wst;
+            warm = 2;
+            continue;
+        }
         FL_HIP_TRY(ctx, hipMemsetAsync(s->e32, 0, s->n * sizeof(float), stm));
         long long it = 0;
         int status = ST_OK;
         bool budget_hit = false, stagnated = false;
+        // With the default inner tolerance under the rel-L2 rule an inner
+        // solve need not go below half of the reduction still missing: the
+        // last sweep stops near the target instead of overshooting it by up
+        // to the inner tolerance.  An explicit inner tolerance is kept as is.
+        double itol = inner_tol;
+        if (rel && rr > 0.0 && !(s->cfg.inner_tolerance > 0.0))
+            itol = std::max(inner_tol, 0.5 * s->cfg.pressure_tolerance * std::sqrt(rr0 / rr));
         if (mg) {
             const long long budget = std::min<long long>(s->cfg.pressure_max_iters - total, 100);
-            const CgParams prm{inner_tol, budget, FL_STOP_REL_L2, 0.0};
+            const CgParams prm{itol, budget, FL_STOP_REL_L2, 0.0};
             fl_status st = run_fcg(s, prm, &stagnated);
             if (st != FL_OK) return st;
             const FcgB &b = s->host->fb;
@@ -1301,7 +1481,7 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
             it = b.iters;
             budget_hit = !s->host->fa.converged && !stagnated && status == ST_OK;
         } else {
-            const CgParams prm{inner_tol, s->cfg.pressure_max_iters - total, FL_STOP_REL_L2, 0.0};
+            const CgParams prm{itol, s->cfg.pressure_max_iters - total, FL_STOP_REL_L2, 0.0};
             fl_status st = run_cg<float, float>(s, prm, s->e32, s->r32, s->d32, s->q32);
             if (st != FL_OK) return st;
             const CgStateB &b = s->host->b;
@@ -1333,6 +1513,8 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
         rep->iterations = total;
         rep->refinements = sweeps;
         rep->fallback = fallback ? 1 : 0;
+        rep->warm_start = warm_used;
+        rep->warm_start_ratio = warm_ratio;
         rep->fallback_iterations = fb.iterations;
         rep->residual_abs_inf = rmax;
         rep->residual_l2 = std::sqrt(rr);
@@ -1413,6 +1595,9 @@ fl_status do_step(fl_cavity3d *s, fl_cavity3d_step_report *rep) {
         rep->pressure_residual = sr.residual_abs_inf;
         rep->pressure_residual_l2 = sr.residual_l2;
         rep->pressure_converged = sr.converged;
+        rep->pressure_fallback = sr.fallback;
+        rep->pressure_warm_start = sr.warm_start;
+        rep->pressure_warm_ratio = sr.warm_start_ratio;
     }
     if (st != FL_OK) return st;
     FL_TIMED(s, K_CORRECT, -1, 2.0 * faces + cells,
@@ -1479,13 +1664,13 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
             (st = alloc(s, &s->q32, s->n)))
             goto fail;
     }
-    if (!mixed || cfg->preconditioner == FL_PRECOND_MULTIGRID) {
+    if (!mixed || cfg->preconditioner != FL_PRECOND_JACOBI) {
         // FP64 PCG vectors (the multigrid path keeps them for its fallback)
         if ((st = alloc(s, &s->r, s->n)) || (st = alloc(s, &s->d[0], s->n)) ||
             (st = alloc(s, &s->d[1], s->n)) || (st = alloc(s, &s->q, s->n)))
             goto fail;
     }
-    if (cfg->preconditioner == FL_PRECOND_MULTIGRID) {
+    if (cfg->preconditioner != FL_PRECOND_JACOBI) {
         if ((st = build_hierarchy(s)) != FL_OK) goto fail;
         if ((st = alloc(s, &s->pz, prow)) || (st = alloc(s, &s->fa, 1)) || (st = alloc(s, &s->fb, 1)))
             goto fail;
@@ -1607,6 +1792,72 @@ fl_status fl_cavity3d_set_timing(fl_cavity3d *s, int32_t enable, const char *ker
     s->prof.on = enable != 0;
     s->prof.only = only;
     s->prof.clear();
+    return FL_OK;
+}
+
+fl_status fl_cavity3d_set_unet(fl_cavity3d *s, fl_unet *net) {
+    if (!s) return FL_ERR_INVALID_ARGUMENT;
+    fl_ctx *ctx = s->ctx;
+    if (!net) {
+        s->unet = nullptr;
+        return FL_OK;
+    }
+    if (s->cfg.preconditioner != FL_PRECOND_LEARNED)
+        return set_error(ctx, FL_ERR_CONFIG, "the solver was not created with the learned preconditioner");
+    if (unet_context(net)->device != ctx->device)
+        return set_error(ctx, FL_ERR_CONFIG, "the U-Net lives on device %d, the solver on %d",
+                         unet_context(net)->device, ctx->device);
+    if (s->g.nx != s->g.ny || s->g.ny != s->g.nz)
+        return set_error(ctx, FL_ERR_CONFIG, "the learned warm start needs a cubic grid");
+    const uint32_t nn = (uint32_t)unet_resolution(net);
+    int L = -1;
+    for (int l = 0; l <= s->tail && l < s->nlev; ++l)
+        if (s->lv[l].nx == nn) L = l;
+    if (L < 0)
+        return set_error(ctx, FL_ERR_CONFIG,
+                         "no multigrid level of the %u^3 grid has the U-Net's resolution %u^3", s->g.nx, nn);
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t cells = (size_t)nn * nn * nn;
+    if (!s->wl_in || s->lv[s->wl_level].n != cells) {
+        if (s->wl_in) (void)hipFree(s->wl_in);
+        if (s->wl_out) (void)hipFree(s->wl_out);
+        s->wl_in = s->wl_out = nullptr;
+        fl_status st = alloc(s, &s->wl_in, cells);
+        if (st == FL_OK) st = alloc(s, &s->wl_out, cells);
+        if (st != FL_OK) return st;
+    }
+    s->unet = net;
+    s->wl_level = L;
+    return FL_OK;
+}
+
+fl_status fl_cavity3d_coarse_pair(fl_cavity3d *s, int32_t n, float *rhs_c, float *p_c, double *diag) {
+    if (!s || !rhs_c || !p_c) return set_error(s ? s->ctx : nullptr, FL_ERR_INVALID_ARGUMENT, "null argument");
+    fl_ctx *ctx = s->ctx;
+    if (s->g.nx != s->g.ny || s->g.ny != s->g.nz || n < 1 || s->g.nx % (uint32_t)n != 0)
+        return set_error(ctx, FL_ERR_INVALID_ARGUMENT,
+                         "coarse pairs need a cubic grid whose size %u is a multiple of n = %d", s->g.nx, n);
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t stm = ctx->stream;
+    const size_t cells = (size_t)n * n * n;
+    if (s->pair_cells < cells) {
+        if (s->pair) (void)hipFree(s->pair);
+        s->pair = nullptr;
+        s->pair_cells = 0;
+        fl_status st = alloc(s, &s->pair, 2 * cells);
+        if (st != FL_OK) return st;
+        s->pair_cells = cells;
+    }
+    const uint32_t f = s->g.nx / (uint32_t)n;
+    const unsigned grid = (unsigned)std::min<size_t>((cells + kBlock - 1) / kBlock, 4096);
+    // b = -rhs, the right-hand side of the solver's A p = b
+    k_block_mean<<<grid, kBlock, 0, stm>>>(s->g, s->rhs, -1.0, (uint32_t)n, f, s->pair);
+    k_block_mean<<<grid, kBlock, 0, stm>>>(s->g, s->p, 1.0, (uint32_t)n, f, s->pair + cells);
+    FL_HIP_TRY(ctx, hipGetLastError());
+    FL_HIP_TRY(ctx, hipMemcpyAsync(rhs_c, s->pair, cells * sizeof(float), hipMemcpyDeviceToHost, stm));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(p_c, s->pair + cells, cells * sizeof(float), hipMemcpyDeviceToHost, stm));
+    FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
+    if (diag) *diag = 2.0 * (s->g.ix2 + s->g.iy2 + s->g.iz2) / ((double)f * f);
     return FL_OK;
 }
 

@@ -1242,6 +1242,8 @@ fl_status fl_grid2d_pressure_solve(fl_grid2d *g, const double *rhs, double *p_in
         rep->iterations = B->h_sb[0].iters;
         rep->refinements = 0;
         rep->fallback = 0;
+        rep->warm_start = 0;
+        rep->warm_start_ratio = 0.0;
         rep->fallback_iterations = 0;
         rep->residual_abs_inf = B->h_sa[0].rmax;
         rep->residual_l2 = std::sqrt(B->h_sa[0].rr);

@@ -8,8 +8,9 @@
 
 namespace fl {
 
-// Resolution per axis of the network's cubic input.
+// Resolution per axis of the network's cubic input, and its context.
 int unet_resolution(const fl_unet *net);
+fl_ctx *unet_context(const fl_unet *net);
 // One inference on device memory: `in` and `out` hold n^3 FP32 values at the
 // network's resolution (x fastest); out = the predicted solution of the
 // rediscretised Poisson problem with diagonal `diag`, zero mean.  Runs on

@@ -60,10 +60,13 @@ enum { FL_PRECISION_FP64 = 0, FL_PRECISION_MIXED = 1 };
  * cell-centred geometric-multigrid V-cycle (3+3 damped-Jacobi sweeps per
  * level, piecewise-constant transfers with a 1.8 coarse-correction factor);
  * when an inner solve exceeds its budget or stagnates, the solve falls back
- * to plain FP64 Jacobi-PCG from the current iterate (counted in the report). */
-enum { FL_PRECOND_JACOBI = 0, FL_PRECOND_MULTIGRID = 1 };
+ * to plain FP64 Jacobi-PCG from the current iterate (counted in the report).
+ * LEARNED (no reference counterpart) is MULTIGRID started from the warm
+ * start of a U-Net attached with fl_cavity3d_set_unet. */
+enum { FL_PRECOND_JACOBI = 0, FL_PRECOND_MULTIGRID = 1, FL_PRECOND_LEARNED = 2 };
 
 typedef struct fl_ctx fl_ctx;
+typedef struct fl_unet fl_unet;
 
 /* Creates a context bound to HIP device `device` with its own stream. */
 fl_status fl_create(int device, fl_ctx **out);
@@ -88,12 +91,14 @@ typedef struct {
     int64_t refinements;       /* outer FP64 refinement sweeps (mixed only) */
     int64_t fallback_iterations; /* plain FP64 PCG iterations after a fallback */
     int32_t fallback;          /* 1 when the plain PCG fallback ran */
-    int32_t reserved;
+    int32_t warm_start;        /* learned warm start: 1 kept, -1 rejected (it did not
+                                  lower the residual and was undone), 0 none */
     double residual_abs_inf;   /* max|r| of the recurrence residual at exit */
     double residual_l2;        /* ||r||_2 at exit */
     double initial_l2;         /* ||r_0||_2 */
     int32_t converged;         /* 1 when the stop rule fired */
     int32_t status;            /* fl_status of the solve */
+    double warm_start_ratio;   /* ||r||_2 after the warm start / ||r_0||_2 (0: none) */
 } fl_solve_report;
 
 /* ---------------------------------------------------------------------- */
@@ -127,6 +132,9 @@ typedef struct {
     double max_speed;              /* incompressible_3d.rs:519-532 */
     int32_t pressure_converged;
     int32_t pressure_fallback;     /* 1 when the plain PCG fallback ran */
+    int32_t pressure_warm_start;   /* fl_solve_report.warm_start of the step's solve */
+    int32_t pad;
+    double pressure_warm_ratio;    /* fl_solve_report.warm_start_ratio */
 } fl_cavity3d_step_report;
 
 typedef struct fl_cavity3d fl_cavity3d;
@@ -314,6 +322,55 @@ fl_status fl_pcg_jacobi(fl_csr *matrix, const double *rhs, double *x_inout,
 /* bicgstab (src/linear/solver.rs:155-256). */
 fl_status fl_bicgstab(fl_csr *matrix, const double *rhs, double *x_inout,
                       const fl_lin_opts *opts, fl_lin_report *report, fl_err_detail *detail);
+
+/* ---------------------------------------------------------------------- */
+/* Learned warm start (no counterpart in the reference; the pressure path's
+ * learned FCG).  A 3D U-Net (4 levels, 16/32/48/64 channels, 3x3x3 BF16
+ * MFMA convolutions, ReLU, no biases) maps the right-hand side of a Poisson
+ * problem on an n^3 grid to its zero-mean solution.  Inputs are normalised
+ * by their RMS and targets by diag / (RMS G) with the network's output gain
+ * G, where `diag` is the Laplacian's diagonal 2 (1/hx^2 + 1/hy^2 + 1/hz^2) at
+ * the network's resolution.  Host arrays are [batch][n^3] FP32, x fastest. */
+typedef struct {
+    int32_t n;         /* resolution per axis: a power of two in [32, 256] */
+    int32_t max_batch; /* largest training / inference batch, 1..64 */
+    uint64_t seed;     /* weight initialisation stream */
+} fl_unet_config;
+fl_status fl_unet_create(fl_ctx *ctx, const fl_unet_config *cfg, fl_unet **out);
+void fl_unet_destroy(fl_unet *net);
+/* Parameters as FP32, layer by layer, each [cout][K padded to 32] with
+ * k = tap * cin + c and tap = 9 (dz+1) + 3 (dy+1) + (dx+1), followed by the
+ * output gain G (not trained; default n^2 / 2): targets are normalised by
+ * diag / (RMS G), predictions are p = (o - mean o) RMS G / diag. */
+int64_t fl_unet_num_params(const fl_unet *net);
+fl_status fl_unet_get_params(fl_unet *net, float *params);
+/* Loads parameters (entries that must be zero are zeroed) and resets Adam. */
+fl_status fl_unet_set_params(fl_unet *net, const float *params);
+/* One Adam step (beta 0.9 / 0.999) on the mean-squared error of the
+ * zero-mean normalised prediction; `loss` receives the batch mean. */
+fl_status fl_unet_train_step(fl_unet *net, int32_t batch, const float *rhs, const float *p,
+                             double diag, float lr, float *loss);
+/* Predicted zero-mean solutions. */
+fl_status fl_unet_infer(fl_unet *net, int32_t batch, const float *rhs, double diag, float *p_out);
+/* Diagnostics for numerical checks: the raw 16-channel output
+ * ([batch][n^3][16]) of the normalised input, and the parameter gradient
+ * of the training loss (no update). */
+fl_status fl_unet_forward_raw(fl_unet *net, int32_t batch, const float *rhs, float *out16);
+fl_status fl_unet_grads(fl_unet *net, int32_t batch, const float *rhs, const float *p, double diag,
+                        float *grad, float *loss);
+/* MFMA operand-layout check: d (16x16) = a (16x32) b (32x16), row-major,
+ * operands rounded to BF16. */
+fl_status fl_probe_mfma_bf16(fl_ctx *ctx, const float *a, const float *b, float *d);
+/* Attaches `net` (may be NULL) to a solver created with FL_PRECOND_LEARNED:
+ * the grid must be a cube of n * 2^L cells per axis with L a multigrid level
+ * that keeps the full domain.  The network must outlive its use by the
+ * solver and must not be used concurrently. */
+fl_status fl_cavity3d_set_unet(fl_cavity3d *solver, fl_unet *net);
+/* Training pairs: the last step's pressure right-hand side b = -rhs and
+ * pressure p, mean-restricted to n^3 (n = nx / 2^L); `diag` receives the
+ * Laplacian diagonal at that resolution. */
+fl_status fl_cavity3d_coarse_pair(fl_cavity3d *solver, int32_t n, float *rhs_c, float *p_c,
+                                  double *diag);
 
 #ifdef __cplusplus
 }
