@@ -162,7 +162,8 @@ def main():
                                                       "unet64_cavity.npy"),
                     help="trained U-Net parameters for learned_fcg")
     ap.add_argument("--unet-n", type=int, default=64, help="U-Net resolution of the weights")
-    ap.add_argument("--cpu-sample-iters", type=int, default=150)
+    ap.add_argument("--cpu-sample-iters", type=int, default=1500,
+                    help="PCG iterations of the CPU baseline sample (about 15 s at 256^3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-copy-probe", action="store_true")
     args = ap.parse_args()
@@ -239,6 +240,15 @@ def main():
         },
         "cpu_baseline": None,
     }
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
+    # this workload (calibrated against the copy probe), when profiled
+    tpath = os.path.join(ROOT, "profiles", f"traffic_cavity{n}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        if tr.get("workload") == out["config"]["workload"] and dom in tr.get("kernels", {}):
+            out["roofline"]["traffic"] = tr["kernels"][dom]["traffic_bytes"]
+            out["roofline"]["traffic_source"] = tr["source"]
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle) sample")
         plain = results.get("plain_fp64_pcg", b)
