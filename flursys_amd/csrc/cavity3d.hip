@@ -357,10 +357,29 @@ __device__ inline float f_at(const float *f, uint32_t c, bool level0, float f0) 
     return (level0 && c == 0) ? f0 : f[c];
 }
 
+// The first two damped-Jacobi sweeps from a zero iterate at cell c of a
+// level: e1 = omega D^-1 f is pointwise, so e2 = e1 + omega D^-1 (f - A e1)
+// needs only f and D^-1 on the 7-point stencil (a clamped neighbour is the
+// cell itself and contributes nothing, as in the sweep).
+__device__ inline float two_sweeps_from_zero(const Level &L, const float *f, uint32_t i, uint32_t j,
+                                             uint32_t k, bool level0, float f0) {
+    const uint32_t ii[7] = {i, i + 1 < L.nx ? i + 1 : i, i > 0 ? i - 1 : i, i, i, i, i};
+    const uint32_t jj[7] = {j, j, j, j + 1 < L.ny ? j + 1 : j, j > 0 ? j - 1 : j, j, j};
+    const uint32_t kk[7] = {k, k, k, k, k, k + 1 < L.nz ? k + 1 : k, k > 0 ? k - 1 : k};
+    float fv[7], e1[7];
+#pragma unroll
+    for (int a = 0; a < 7; ++a) fv[a] = f_at(f, ii[a] + L.nx * (jj[a] + L.ny * kk[a]), level0, f0);
+#pragma unroll
+    for (int a = 0; a < 7; ++a) e1[a] = kOmega * L.inv[lvl_didx(L, ii[a], jj[a], kk[a])] * fv[a];
+    const float ae = L.cx * ((e1[0] - e1[1]) + (e1[0] - e1[2])) + L.cy * ((e1[0] - e1[3]) + (e1[0] - e1[4])) +
+                     L.cz * ((e1[0] - e1[5]) + (e1[0] - e1[6]));
+    return e1[0] + kOmega * L.inv[lvl_didx(L, i, j, k)] * (fv[0] - ae);
+}
+
 // One damped-Jacobi sweep on a level:
 //   e_out = e' + omega D^-1 (f - A e'),  e' = e_in + corr P(e_c) (kProlong;
 //   a null e_in is zero)
-// or, kFirst, e_out = omega D^-1 f (from a zero iterate).  kPublish (level 0,
+// or, kFirst, the first two sweeps from a zero iterate.  kPublish (level 0,
 // last sweep) also publishes sum z.q_old and sum q_old for the FCG beta.
 template <bool kFirst, bool kProlong, bool kLevel0, bool kPublish>
 __global__ void __launch_bounds__(kBlock) k_mg_sweep(Level L, Level LC, const float *__restrict__ f,
@@ -383,7 +402,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_sweep(Level L, Level LC, const fl
         const float inv = L.inv[lvl_didx(L, i, j, k)];
         float out;
         if (kFirst) {
-            out = kOmega * inv * fc;
+            out = two_sweeps_from_zero(L, f, i, j, k, kLevel0, f0);
         } else {
             const uint32_t ii[7] = {i, i + 1 < L.nx ? i + 1 : i, i > 0 ? i - 1 : i, i, i, i, i};
             const uint32_t jj[7] = {j, j, j, j + 1 < L.ny ? j + 1 : j, j > 0 ? j - 1 : j, j, j};
@@ -545,8 +564,9 @@ __global__ void __launch_bounds__(1024) k_mg_tail(TailLevels T, const float *__r
     for (uint32_t c = threadIdx.x; c < T.lv[0].n; c += blockDim.x) eout[c] = E[0][c];
 }
 
-// Stop test of the inner FCG (folds the update partials), then the first
-// pre-sweep of the cycle on level 0.
+// Stop test of the inner FCG (folds the update partials), then the first two
+// pre-sweeps of the cycle on level 0 (from a zero iterate, in one pass).
+
 __global__ void __launch_bounds__(kBlock) k_mg_check(Level L, CgParams prm, const float *__restrict__ r,
                                                      float *__restrict__ eout,
                                                      const double *__restrict__ pin, FcgA *SA,
@@ -583,7 +603,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_check(Level L, CgParams prm, cons
         if (c >= L.n) continue;
         uint32_t i, j, k;
         lvl_ijk(L, c, i, j, k);
-        eout[c] = kOmega * L.inv[lvl_didx(L, i, j, k)] * f_at(r, c, true, f0);
+        eout[c] = two_sweeps_from_zero(L, r, i, j, k, true, f0);
     }
 }
 
@@ -728,8 +748,12 @@ __global__ void __launch_bounds__(kBlock) k_predict(StepParams P, const double *
     const uint32_t nu = (P.nx + 1) * P.ny * P.nz, nv = P.nx * (P.ny + 1) * P.nz,
                    nw = P.nx * P.ny * (P.nz + 1);
     const double dx2 = P.dx * P.dx, dy2 = P.dy * P.dy, dz2 = P.dz * P.dz;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nu + nv + nw;
-         t += gridDim.x * blockDim.x) {
+    // u, v and w faces in turn, each split into per-XCD z-slabs
+    for (int a = 0; a < 3; ++a) {
+    const uint32_t base = a == 0 ? 0 : (a == 1 ? nu : nu + nv);
+    const XcdRange r(a == 0 ? nu : (a == 1 ? nv : nw));
+    for (uint32_t l = r.first; l < r.end; l += r.step) {
+        const uint32_t t = base + l;
         if (t < nu) {
             const int i = t % (P.nx + 1), j = (t / (P.nx + 1)) % P.ny, k = t / ((P.nx + 1) * P.ny);
             if (i == 0 || i == (int)P.nx) { us[t] = 0.0; continue; }
@@ -779,6 +803,7 @@ __global__ void __launch_bounds__(kBlock) k_predict(StepParams P, const double *
                                         (f - 2.0 * c + bk) / dz2));
         }
     }
+    }
 }
 
 // build_pressure_rhs (incompressible_3d.rs:275-291), rhs(0,0,0) = 0.
@@ -788,7 +813,8 @@ __global__ void __launch_bounds__(kBlock) k_rhs(StepParams P, const double *__re
                                                 double *__restrict__ rhs) {
     const uint32_t n = P.nx * P.ny * P.nz;
     const double scale = P.rho / P.dt;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const XcdRange r(n);
+    for (uint32_t c = r.first; c < r.end; c += r.step) {
         const int i = c % P.nx, j = (c / P.nx) % P.ny, k = c / (P.nx * P.ny);
         rhs[c] = c == 0 ? 0.0
                         : scale * (us[U3(i + 1, j, k)] - us[U3(i, j, k)]) / P.dx +
@@ -808,8 +834,11 @@ __global__ void __launch_bounds__(kBlock) k_correct(StepParams P, const double *
     const uint32_t nu = (P.nx + 1) * P.ny * P.nz, nv = P.nx * (P.ny + 1) * P.nz,
                    nw = P.nx * P.ny * (P.nz + 1);
     const double scale = P.dt / P.rho;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nu + nv + nw;
-         t += gridDim.x * blockDim.x) {
+    for (int a = 0; a < 3; ++a) {  // u, v and w faces, per-XCD z-slabs
+    const uint32_t base = a == 0 ? 0 : (a == 1 ? nu : nu + nv);
+    const XcdRange r(a == 0 ? nu : (a == 1 ? nv : nw));
+    for (uint32_t l = r.first; l < r.end; l += r.step) {
+        const uint32_t t = base + l;
         if (t < nu) {
             const int i = t % (P.nx + 1), j = (t / (P.nx + 1)) % P.ny, k = t / ((P.nx + 1) * P.ny);
             if (i == 0 || i == (int)P.nx) continue;
@@ -826,6 +855,7 @@ __global__ void __launch_bounds__(kBlock) k_correct(StepParams P, const double *
             w[s] = ws[s] - scale * (p[W3(i, j, k)] - p[W3(i, j, k - 1)]) / P.dz;
         }
     }
+    }
 }
 
 struct DiagState {
@@ -841,7 +871,8 @@ __global__ void __launch_bounds__(kBlock) k_diag(StepParams P, const double *__r
                                                  double *partials, DiagState *dg) {
     const uint32_t n = P.nx * P.ny * P.nz;
     double s[1] = {0.0}, m[2] = {0.0, 0.0};
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const XcdRange r(n);
+    for (uint32_t c = r.first; c < r.end; c += r.step) {
         const int i = c % P.nx, j = (c / P.nx) % P.ny, k = c / (P.nx * P.ny);
         const double u0 = u[U3(i, j, k)], u1 = u[U3(i + 1, j, k)];
         const double v0 = v[V3(i, j, k)], v1 = v[V3(i, j + 1, k)];
@@ -1242,11 +1273,10 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     float *pold = s->d32[k & 1], *pnew = s->d32[(k + 1) & 1];
     const uint32_t G = s->blocks;
     const double n0 = L[0].n, n1 = L[1].n;
-    // down
+    // down: the first two pre-sweeps in one pass (into meb), the third into
+    // mea, which the restriction and the up pass read
     FL_TIMED(s, K_MG_CHECK, k, 8.0 * n0,
-             k_mg_check<<<G, kBlock, 0, stm>>>(L[0], prm, s->r32, s->mea[0], s->pb, s->fa, s->fb));
-    FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
-             k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], nullptr, nullptr, s->fa, s->fb, kCorr));
+             k_mg_check<<<G, kBlock, 0, stm>>>(L[0], prm, s->r32, s->meb[0], s->pb, s->fa, s->fb));
     FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
              k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, kCorr));
     FL_TIMED(s, K_MG_RESTRICT_L0, k, 8.0 * n0 + 4.0 * n1,
@@ -1255,9 +1285,7 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
         const uint32_t g = s->lblocks[l];
         const double nl = L[l].n, nc = L[l + 1].n;
         FL_TIMED(s, K_MG_SWEEP_FIRST, k, 8.0 * nl,
-                 k_mg_sweep<true, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], nullptr, nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb, kCorr));
-        FL_TIMED(s, K_MG_SWEEP, k, 12.0 * nl,
-                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr));
+                 k_mg_sweep<true, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], nullptr, nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr));
         FL_TIMED(s, K_MG_SWEEP, k, 12.0 * nl,
                  k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb, kCorr));
         FL_TIMED(s, K_MG_RESTRICT, k, 8.0 * nl + 4.0 * nc,

@@ -247,6 +247,27 @@ struct TileSched {
     }
 };
 
+// Element-wise form of the same idea for index spaces without tiles: XCD x
+// owns the contiguous range [n x / 8, n (x + 1) / 8) and its blocks stride
+// through it, so stencil neighbours (the next row or plane) are read by the
+// same XCD and hit its L2:  for (i = r.first; i < r.end; i += r.step).
+struct XcdRange {
+    uint32_t first, end, step;
+    __device__ inline explicit XcdRange(uint32_t n) {
+        if ((gridDim.x % kXcds) == 0 && gridDim.x >= 8 * kXcds) {
+            const uint32_t xcd = blockIdx.x % kXcds, lb = blockIdx.x / kXcds;
+            const uint32_t begin = (uint32_t)(((uint64_t)n * xcd) / kXcds);
+            end = (uint32_t)(((uint64_t)n * (xcd + 1)) / kXcds);
+            first = begin + lb * blockDim.x + threadIdx.x;
+            step = (gridDim.x / kXcds) * blockDim.x;
+        } else {
+            first = blockIdx.x * blockDim.x + threadIdx.x;
+            end = n;
+            step = gridDim.x * blockDim.x;
+        }
+    }
+};
+
 inline uint32_t grid_blocks(const fl_ctx *ctx, uint64_t ntiles, int per_cu = 8) {
     uint64_t g = (uint64_t)ctx->num_cu * per_cu;
     if (ntiles < g) g = ntiles;
