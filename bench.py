@@ -147,6 +147,94 @@ def run_variant(fa, ctx, name, precision, precond, args):
     }
 
 
+def batch2d_configs(fa, count=64, seed=1):
+    """64 2D lid-driven cavities, n = round(64 * 8^u) with u ~ U[0, 1) (log-
+    uniform 64-512), Re ~ U[100, 1000] (both from numpy's PCG64, seed 1), the
+    reference's adaptive dt rule (target CFL 0.5, diffusion number 0.25) and
+    its solver defaults (first-order upwind, Jacobi-PCG from the previous
+    pressure to max|r| < 1e-5, 1200 iterations at most)."""
+    import numpy as np
+
+    import flursys_amd.incompressible as I
+
+    rng = np.random.default_rng(seed)
+    u = rng.random(count)
+    re = 100.0 + 900.0 * rng.random(count)
+    ts = I.TimeStepSettings(adaptive=True, target_cfl=0.5, target_diffusion_number=0.25)
+    return [I.SimulationConfig.cavity(int(round(64 * 8.0 ** a)), reynolds=float(r), time_step=ts)
+            for a, r in zip(u, re)]
+
+
+def run_batch2d(fa, ctx, args):
+    import numpy as np
+
+    import flursys_amd.incompressible as I
+
+    cfgs = batch2d_configs(fa)
+    cells = sum(c.nx * c.ny for c in cfgs)
+    log(f"[bench] batch2d: {len(cfgs)} cavities, {cells} cells, n "
+        f"{min(c.nx for c in cfgs)}-{max(c.nx for c in cfgs)}")
+    out = {}
+    # the graph-captured run is the measurement; a shorter eager run (the
+    # same launches issued one by one) is timed beside it
+    for mode, graphs, total in (("graphs", True, args.batch_steps),
+                                ("eager", False, min(args.batch_steps, 50))):
+        b = I.CavityBatch(cfgs, ctx=ctx)
+        blocks, tune_ms, table = b.autotune(20)
+        log(f"[bench] batch2d {mode}: autotune (blocks: ms per 20 iterations) {table}")
+        b.run(5, use_graphs=graphs)  # warmup (captures the graphs)
+        chunk, left, ms, iters = 25, total, [], 0
+        t0 = time.perf_counter()
+        while left > 0:
+            k = min(chunk, left)
+            reps, step_ms = b.run(k, use_graphs=graphs, step_times=True)
+            ms.extend(step_ms.tolist())
+            iters += sum(r.pressure_iterations for r in reps)
+            left -= k
+            log(f"[bench] batch2d {mode}: {total - left}/{total} steps, "
+                f"{iters / (len(cfgs) * (total - left)):.1f} PCG iterations per cavity-step, "
+                f"{np.mean(ms):.2f} ms/step")
+        elapsed = time.perf_counter() - t0
+        b.close()
+        out[mode] = {"elapsed_s": elapsed, "steps": total, "step_ms": ms, "blocks": blocks,
+                     "autotune_ms": tune_ms, "autotune_table": table, "pressure_iterations": iters}
+    g = out["graphs"]
+    ms = np.array(g["step_ms"])
+    steps = args.batch_steps
+    return {
+        "metric": "aggregate cavity-steps/s, 64 uneven 2D cavities (64^2-512^2) advanced together",
+        "value": len(cfgs) * steps / g["elapsed_s"],
+        "unit": "cavity-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 5,
+        "ms_per_step": g["elapsed_s"] * 1e3 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: cavities started from rest (no dataset)",
+        "config": {
+            "workload": f"{len(cfgs)} 2D lid-driven cavities, n = round(64*8^u), Re U[100,1000] "
+                        "(seed 1), adaptive dt (CFL 0.5, diffusion 0.25), upwind, Jacobi-PCG "
+                        "warm start max|r| < 1e-5",
+            "cells": cells,
+            "step_latency_ms": {"p50": float(np.percentile(ms, 50)),
+                                "p99": float(np.percentile(ms, 99)),
+                                "max": float(ms.max())},
+            "pcg_iterations_per_cavity_step": g["pressure_iterations"] / (len(cfgs) * steps),
+            "autotuned_pcg_blocks": g["blocks"],
+            "autotune_ms_per_20_iterations": {str(k): v for k, v in g["autotune_table"].items()},
+            "eager_cavity_steps_per_s": len(cfgs) * out["eager"]["steps"] / out["eager"]["elapsed_s"],
+            "eager_steps": out["eager"]["steps"],
+            "eager_step_latency_ms_p50": float(np.percentile(out["eager"]["step_ms"], 50)),
+            "parallelism": "single GPU",
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +254,11 @@ def main():
                     help="PCG iterations of the CPU baseline sample (about 15 s at 256^3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-copy-probe", action="store_true")
+    ap.add_argument("--config", default="cavity3d", choices=["cavity3d", "batch2d"],
+                    help="cavity3d: the headline 256^3 benchmark; batch2d: 64 uneven 2D "
+                         "cavities advanced together (a separate JSON line)")
+    ap.add_argument("--batch-steps", type=int, default=1000,
+                    help="steps of every cavity in the batch2d configuration")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -178,6 +271,10 @@ def main():
     from flursys_amd import _lib
 
     ctx = fa.Context(0)
+    if args.config == "batch2d":
+        print(json.dumps(run_batch2d(fa, ctx, args)), flush=True)
+        ctx.close()
+        return
     modes = {"plain_fp64_pcg": (fa.PRECISION_FP64, fa.PRECOND_JACOBI),
              "mixed_ir": (fa.PRECISION_MIXED, fa.PRECOND_JACOBI),
              "mg_fcg": (fa.PRECISION_MIXED, fa.PRECOND_MULTIGRID),

@@ -71,6 +71,13 @@ class Cavity3DStepReport(C.Structure):
     ]
 
 
+class Batch2DRunReport(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("time", C.c_double), ("pressure_iterations", C.c_int64),
+                ("last_pressure_residual", C.c_double), ("max_divergence", C.c_double),
+                ("max_speed", C.c_double), ("reached_end_time", C.c_int32),
+                ("status", C.c_int32)]
+
+
 class UNetConfig(C.Structure):
     _fields_ = [("n", C.c_int32), ("max_batch", C.c_int32), ("seed", C.c_uint64)]
 
@@ -154,6 +161,9 @@ SIGNATURES = {
     "fl_batch2d_destroy": (None, [_P]),
     "fl_batch2d_step": (C.c_int, [_P, C.POINTER(Grid2DStepReport)]),
     "fl_batch2d_get_fields": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
+    "fl_batch2d_run": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(Batch2DRunReport), _P]),
+    "fl_batch2d_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _D, C.c_int32,
+                                      C.POINTER(C.c_int32), _D, C.POINTER(C.c_int32)]),
     "fl_csr_upload": (C.c_int,[_P, C.c_int64, C.c_int64, _P, _P, _P, C.c_int64,
                                 C.POINTER(_P), C.POINTER(ErrDetail)]),
     "fl_csr_destroy": (None, [_P]),

@@ -397,11 +397,8 @@ struct PcgArgs {
     const int *active;  // grids taking part in this solve
     int *fin;  // fin[parity * nprob + grid]: grid finished in that loop (the
                // host sets fin[1][g] = !active[g], the "loop -1" verdicts)
+    const uint32_t *bt0;  // first tile of each block, gridDim.x + 1 entries
 };
-
-__host__ __device__ inline uint32_t range_lo(uint32_t b, uint32_t T, uint32_t G) {
-    return (uint32_t)(((uint64_t)b * T) / G);
-}
 
 // Jacobi inverse diagonal from the open-neighbour bits.
 __device__ inline double inv_diag(const Prob &P, uint8_t f) {
@@ -423,8 +420,10 @@ __device__ inline void fold_tiles(const double *part, uint32_t tile0, uint32_t n
 
 __global__ void __launch_bounds__(kT) k_pcg(PcgArgs A) {
     cgrp::grid_group gg = cgrp::this_grid();
-    const uint32_t G = gridDim.x, b = blockIdx.x;
-    const uint32_t t_lo = range_lo(b, A.ntiles, G), t_hi = range_lo(b + 1, A.ntiles, G);
+    const uint32_t b = blockIdx.x;
+    // block b owns tiles [bt0[b], bt0[b+1]): contiguous ranges of equal
+    // expected work (see set_pcg_blocks)
+    const uint32_t t_lo = A.bt0[b], t_hi = A.bt0[b + 1];
     const uint32_t tid = threadIdx.x;
     // ---- init 0: sums of x0 and b = -rhs over the gauge cells
     for (uint32_t t = t_lo; t < t_hi; ++t) {
@@ -713,6 +712,126 @@ __global__ void __launch_bounds__(kT) k_pcg(PcgArgs A) {
     }
 }
 
+// ------------------------------------------------ device-side step control
+// Runs of many steps (fl_batch2d_run) keep every decision on the device: the
+// time-step rule, t_end, the step and time counters and the first error of
+// each grid, so that a step is a fixed launch sequence (capturable in a HIP
+// graph) and the host synchronises once per run.
+struct StepCtl {
+    // TimeStepSettings / t_end (incompressible.rs:35-90)
+    double dt0, t_end, min_dt, max_dt, target_cfl, target_diff, growth;
+    int32_t adaptive, pad0;
+    // state
+    double time, current_dt, max_u, max_v;
+    long long step, iters;
+    double last_residual, max_div, max_speed;
+    // first error: RE_MIN_DT (vals = stable, cfl, diffusion dt), RE_PCG
+    // (which = PCG status), RE_NONFINITE (which = field, index)
+    int32_t error, which;
+    unsigned long long index;
+    double vals[3];
+};
+enum { RE_NONE = 0, RE_MIN_DT = 1, RE_PCG = 2, RE_NONFINITE = 3 };
+
+__host__ __device__ inline bool ctl_reached_end(double time, double t_end) {
+    const double tol = 2.220446049250313e-16 * fmax(fabs(t_end), 1.0) * 16.0;
+    return time >= t_end - tol;
+}
+
+// Step setup, one thread per grid: dt (fixed, or adaptive_time_step
+// incompressible.rs:2308-2336) clamped to t_end, the activity flag, the
+// PCG's per-solve state and the diagnostics reset.
+__global__ void k_run_setup(const Prob *probs, StepCtl *ctl, int count, int *active, double *dts,
+                            int *fin, PcgB *sb, Diag *diag) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= count) return;
+    StepCtl &c = ctl[p];
+    const Prob &P = probs[p];
+    Diag d{};
+    for (int f = 0; f < 4; ++f) d.bad[f] = ~0ull;
+    diag[p] = d;
+    int act = 0;
+    double dt = c.dt0;
+    if (c.error == RE_NONE && !ctl_reached_end(c.time, c.t_end)) {
+        bool ok = true;
+        if (c.adaptive) {
+            const double rate = c.max_u / P.dx + c.max_v / P.dy;
+            const double cfl_dt = rate > 2.220446049250313e-16 ? c.target_cfl / rate : c.max_dt;
+            const double inv = 1.0 / (P.dx * P.dx) + 1.0 / (P.dy * P.dy);
+            const double diff_dt = c.target_diff / (P.nu * inv);
+            const double stable = fmin(fmin(c.max_dt, cfl_dt), diff_dt);
+            if (stable < c.min_dt) {
+                c.error = RE_MIN_DT;
+                c.vals[0] = stable;
+                c.vals[1] = cfl_dt;
+                c.vals[2] = diff_dt;
+                ok = false;
+            } else {
+                const double g = fmin(stable, c.current_dt * c.growth);
+                dt = fmin(fmax(g, c.min_dt), c.max_dt);
+            }
+        }
+        if (ok) {
+            dt = fmin(dt, c.t_end - c.time);
+            if (dt > 0.0) {
+                act = 1;
+                if (c.adaptive) c.current_dt = dt;
+            }
+        }
+    }
+    active[p] = act;
+    dts[p] = dt;
+    fin[p] = 0;
+    fin[count + p] = act ? 0 : 1;
+    PcgB b{};
+    b.first = 1;
+    sb[p] = b;
+}
+
+// Step finish, one thread per active grid: the PCG verdict, counters, the
+// ensure_finite check (incompressible.rs:1707-1742; pressure, u, v, speed)
+// and the maxima the next step's dt rule reads.
+__global__ void k_run_finish(StepCtl *ctl, int count, const int *active, const double *dts,
+                             const Diag *diag, const PcgA *sa, const PcgB *sb) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= count || !active[p]) return;
+    StepCtl &c = ctl[p];
+    const int st = sb[p].status != ST_OK ? sb[p].status : sa[p].status;
+    if (st != ST_OK) {
+        c.error = RE_PCG;
+        c.which = st;
+        return;
+    }
+    c.step += 1;
+    c.time += dts[p];
+    const Diag &d = diag[p];
+    for (int f = 0; f < 4; ++f)
+        if (d.bad[f] != ~0ull) {
+            c.error = RE_NONFINITE;
+            c.which = f;
+            c.index = d.bad[f];
+            return;
+        }
+    c.max_u = d.max_u;
+    c.max_v = d.max_v;
+    c.iters += sb[p].iters;
+    c.last_residual = sa[p].rmax;
+    c.max_div = d.max_div;
+    c.max_speed = d.max_speed;
+}
+
+// Autotuner input: a fixed pseudo-random right-hand side (splitmix64 of the
+// cell index), zero on padding.
+__global__ void k_fill_rhs(double *rhs, size_t n, uint64_t seed) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint64_t z = seed + 0x9e3779b97f4a7c15ull * (i + 1);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        rhs[i] = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+}
+
 }  // namespace g2
 }  // namespace fl
 
@@ -750,8 +869,17 @@ struct Batch2 {
     // host-side solver state per grid
     std::vector<int64_t> step;
     std::vector<double> time, current_dt, max_u, max_v;
-    uint32_t pcg_blocks = 1;
+    uint32_t pcg_blocks = 1, resident = 1;
+    uint32_t *bt0 = nullptr;  // persistent PCG: first tile of each block
+    std::vector<uint32_t> h_bt0;
     long long kmax = 1;
+    // device-driven runs: per-grid control state, the captured step halves
+    // around the cooperative PCG launch, per-step events
+    StepCtl *ctl = nullptr, *h_ctl = nullptr;
+    long long kmax_all = 1;
+    hipGraph_t gr_pre = nullptr, gr_post = nullptr;
+    hipGraphExec_t gx_pre = nullptr, gx_post = nullptr;
+    std::vector<hipEvent_t> ev;
 };
 
 template <typename T>
@@ -775,9 +903,16 @@ void free_batch(Batch2 *B) {
                  B->sb,    B->fin,       B->active, B->dts, B->diag};
     for (void *p : d)
         if (p) (void)hipFree(p);
-    void *h[] = {B->h_active, B->h_fin, B->h_dts, B->h_sa, B->h_sb, B->h_diag};
+    void *h[] = {B->h_active, B->h_fin, B->h_dts, B->h_sa, B->h_sb, B->h_diag, B->h_ctl};
     for (void *p : h)
         if (p) (void)hipHostFree(p);
+    if (B->ctl) (void)hipFree(B->ctl);
+    if (B->bt0) (void)hipFree(B->bt0);
+    if (B->gx_pre) (void)hipGraphExecDestroy(B->gx_pre);
+    if (B->gx_post) (void)hipGraphExecDestroy(B->gx_post);
+    if (B->gr_pre) (void)hipGraphDestroy(B->gr_pre);
+    if (B->gr_post) (void)hipGraphDestroy(B->gr_post);
+    for (hipEvent_t e : B->ev) (void)hipEventDestroy(e);
 }
 
 // SimulationConfig::validate (incompressible.rs:93-191), flow-only parts.
@@ -853,6 +988,45 @@ bool reached_end(const Batch2 *B, int p) {
     const double t_end = B->cfg[p].t_end;
     const double tol = 2.220446049250313e-16 * std::max(std::fabs(t_end), 1.0) * 16.0;
     return B->time[p] >= t_end - tol;
+}
+
+// The persistent PCG's grid size and its tile partition: contiguous tile
+// ranges of equal expected work.  A tile weighs its grid's size per axis
+// (Jacobi-PCG iteration counts grow linearly with it) plus a fixed per-tile
+// cost, so the blocks holding the grids that iterate longest hold fewer
+// tiles and the last iterations, when only those grids are left, still run
+// on many blocks.  A grid's owner block is the one holding its first tile.
+fl_status set_pcg_blocks(Batch2 *B, uint32_t G) {
+    G = std::max<uint32_t>(1, std::min(G, B->resident));
+    B->pcg_blocks = G;
+    std::vector<double> w(B->ntiles, 0.0);
+    double W = 0.0;
+    for (int p = 0; p < B->count; ++p) {
+        const Prob &P = B->hp[p];
+        const double wt = (double)std::max(P.nx, P.ny) + 16.0;
+        for (uint32_t t = 0; t < P.ntiles; ++t) w[P.tile0 + t] = wt;
+        W += wt * P.ntiles;
+    }
+    B->h_bt0.assign(G + 1, B->ntiles);
+    B->h_bt0[0] = 0;
+    double acc = 0.0;
+    uint32_t t = 0;
+    for (uint32_t b = 1; b < G; ++b) {
+        const double target = W * (double)b / (double)G;
+        while (t < B->ntiles && acc + 0.5 * w[t] < target) acc += w[t++];
+        B->h_bt0[b] = t;
+    }
+    for (int p = 0; p < B->count; ++p) {
+        Prob &P = B->hp[p];
+        for (uint32_t b = 0; b < G; ++b)
+            if (B->h_bt0[b] <= P.tile0 && P.tile0 < B->h_bt0[b + 1]) P.owner = b;
+    }
+    hipStream_t s = B->ctx->stream;
+    FL_HIP_TRY(B->ctx, hipMemcpyAsync(B->probs, B->hp.data(), B->count * sizeof(Prob),
+                                      hipMemcpyHostToDevice, s));
+    FL_HIP_TRY(B->ctx, hipMemcpyAsync(B->bt0, B->h_bt0.data(), (G + 1) * sizeof(uint32_t),
+                                      hipMemcpyHostToDevice, s));
+    return FL_OK;
 }
 
 fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
@@ -933,17 +1107,9 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
     // persistent-kernel grid: all blocks co-resident (cooperative launch)
     int per_cu = 0;
     FL_HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg, kT, 0));
-    uint64_t G = (uint64_t)std::max(per_cu, 1) * ctx->num_cu;
-    G = std::min<uint64_t>(G, (uint64_t)ctx->num_cu * 2);
+    B->resident = (uint32_t)std::max(per_cu, 1) * (uint32_t)ctx->num_cu;
+    uint64_t G = std::min<uint64_t>(B->resident, (uint64_t)ctx->num_cu * 2);
     G = std::min<uint64_t>(G, B->ntiles);
-    B->pcg_blocks = (uint32_t)std::max<uint64_t>(G, 1);
-    for (int p = 0; p < count; ++p) {
-        Prob &P = B->hp[p];
-        uint32_t owner = 0;
-        for (uint32_t b = 0; b < B->pcg_blocks; ++b)
-            if (range_lo(b, B->ntiles, B->pcg_blocks) <= P.tile0) owner = b;
-        P.owner = owner;
-    }
     if ((st = dalloc(B, &B->probs, count)) || (st = dalloc(B, &B->tile_prob, B->ntiles)) ||
         (st = dalloc(B, &B->flags, B->ncell)) || (st = dalloc(B, &B->x, B->ncell)) ||
         (st = dalloc(B, &B->rhs, B->ncell)) || (st = dalloc(B, &B->r, B->ncell)) ||
@@ -959,16 +1125,21 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
         (st = dalloc(B, &B->dpart, (size_t)B->ntiles * 6)) || (st = dalloc(B, &B->sa, count)) ||
         (st = dalloc(B, &B->sb, count)) || (st = dalloc(B, &B->fin, 2 * count)) ||
         (st = dalloc(B, &B->active, count)) || (st = dalloc(B, &B->dts, count)) ||
-        (st = dalloc(B, &B->diag, count)))
+        (st = dalloc(B, &B->diag, count)) || (st = dalloc(B, &B->ctl, count)) ||
+        (st = dalloc(B, &B->bt0, (size_t)B->resident + 1)))
         return st;
     if ((st = halloc(B, &B->h_active, count)) || (st = halloc(B, &B->h_fin, 2 * count)) ||
         (st = halloc(B, &B->h_dts, count)) || (st = halloc(B, &B->h_sa, count)) ||
-        (st = halloc(B, &B->h_sb, count)) || (st = halloc(B, &B->h_diag, count)))
+        (st = halloc(B, &B->h_sb, count)) || (st = halloc(B, &B->h_diag, count)) ||
+        (st = halloc(B, &B->h_ctl, count)))
         return st;
+    for (int p = 0; p < count; ++p)
+        B->kmax_all = std::max<long long>(B->kmax_all, B->hp[p].max_iter + 2);
     hipStream_t s = ctx->stream;
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->probs, B->hp.data(), count * sizeof(Prob), hipMemcpyHostToDevice, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->tile_prob, htp.data(), htp.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->flags, hflags.data(), hflags.size(), hipMemcpyHostToDevice, s));
+    if ((st = set_pcg_blocks(B, (uint32_t)std::max<uint64_t>(G, 1))) != FL_OK) return st;
     B->step.assign(count, 0);
     B->time.assign(count, 0.0);
     B->current_dt.resize(count);
@@ -1023,7 +1194,7 @@ fl_status run_pcg(Batch2 *B) {
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->sb, B->h_sb, B->count * sizeof(PcgB), hipMemcpyHostToDevice, s));
     PcgArgs a{B->probs, B->tile_prob, B->ntiles, (uint32_t)B->count, kmax, B->flags, B->rhs,
               B->x, B->r, B->d0, B->d1, B->q, B->pa, B->pb, B->p0, B->sa, B->sb,
-              B->active, B->fin};
+              B->active, B->fin, B->bt0};
     void *args[] = {&a};
     FL_HIP_TRY(ctx, hipLaunchCooperativeKernel((const void *)k_pcg, dim3(B->pcg_blocks), dim3(kT),
                                                args, 0, s));
@@ -1135,6 +1306,247 @@ fl_status step_batch(Batch2 *B, fl_grid2d_step_report *reps, bool strict) {
             r.reached_end_time = reached_end(B, p);
         }
     }
+    return FL_OK;
+}
+
+// ------------------------------------------------ device-driven runs
+// The step without host decisions: setup, predictor, RHS (before the PCG)
+// and correction, cell fields, diagnostics, finish (after it).
+fl_status launch_pre(Batch2 *B, hipStream_t s) {
+    fl_ctx *ctx = B->ctx;
+    const uint32_t G = ew_blocks(B);
+    k_run_setup<<<(B->count + 63) / 64, 64, 0, s>>>(B->probs, B->ctl, B->count, B->active, B->dts,
+                                                    B->fin, B->sb, B->diag);
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->uo, B->u, B->nuf * sizeof(double), hipMemcpyDeviceToDevice, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->vo, B->v, B->nvf * sizeof(double), hipMemcpyDeviceToDevice, s));
+    k_predict<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->dts, B->flags,
+                               B->u, B->v, B->us, B->vs);
+    k_boundaries<<<B->count, kT, 0, s>>>(B->probs, B->active, B->us, B->vs);
+    k_rhs<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->dts, B->flags, B->us,
+                           B->vs, B->rhs);
+    FL_HIP_TRY(ctx, hipGetLastError());
+    return FL_OK;
+}
+fl_status launch_post(Batch2 *B, hipStream_t s) {
+    fl_ctx *ctx = B->ctx;
+    const uint32_t G = ew_blocks(B);
+    k_correct<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->dts, B->flags, B->x,
+                               B->us, B->vs, B->u, B->v);
+    k_boundaries<<<B->count, kT, 0, s>>>(B->probs, B->active, B->u, B->v);
+    k_cells<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->flags, B->x, B->u,
+                             B->v, B->uo, B->vo, B->uc, B->vc, B->sp, B->vort, B->dpart, B->diag);
+    k_fold_diag<<<B->count, kT, 0, s>>>(B->probs, B->active, B->dpart, B->diag);
+    k_run_finish<<<(B->count + 63) / 64, 64, 0, s>>>(B->ctl, B->count, B->active, B->dts, B->diag,
+                                                     B->sa, B->sb);
+    FL_HIP_TRY(ctx, hipGetLastError());
+    return FL_OK;
+}
+fl_status launch_pcg_coop(Batch2 *B, hipStream_t s, const double *rhs, double *x, long long kmax) {
+    PcgArgs a{B->probs, B->tile_prob, B->ntiles, (uint32_t)B->count, kmax, B->flags, rhs,
+              x, B->r, B->d0, B->d1, B->q, B->pa, B->pb, B->p0, B->sa, B->sb,
+              B->active, B->fin, B->bt0};
+    void *args[] = {&a};
+    FL_HIP_TRY(B->ctx, hipLaunchCooperativeKernel((const void *)k_pcg, dim3(B->pcg_blocks), dim3(kT),
+                                                  args, 0, s));
+    return FL_OK;
+}
+
+fl_status capture(Batch2 *B, bool pre, hipGraph_t *g, hipGraphExec_t *x) {
+    fl_ctx *ctx = B->ctx;
+    hipStream_t s = ctx->stream;
+    FL_HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    fl_status st = pre ? launch_pre(B, s) : launch_post(B, s);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (st != FL_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess)
+        return set_error(ctx, FL_ERR_HIP, "stream capture failed: %s", hipGetErrorString(e));
+    FL_HIP_TRY(ctx, hipGraphInstantiate(x, graph, nullptr, nullptr, 0));
+    *g = graph;
+    return FL_OK;
+}
+
+fl_status run_device(Batch2 *B, int64_t steps, bool graph, fl_batch2d_run_report *reps,
+                     double *step_ms) {
+    fl_ctx *ctx = B->ctx;
+    hipStream_t s = ctx->stream;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    for (int p = 0; p < B->count; ++p) {
+        const fl_grid2d_config &c = B->cfg[p];
+        StepCtl &k = B->h_ctl[p];
+        k = StepCtl{};
+        k.dt0 = c.dt;
+        k.t_end = c.t_end;
+        k.min_dt = c.min_dt;
+        k.max_dt = c.max_dt;
+        k.target_cfl = c.target_cfl;
+        k.target_diff = c.target_diffusion_number;
+        k.growth = c.max_growth_factor;
+        k.adaptive = c.adaptive;
+        k.time = B->time[p];
+        k.current_dt = B->current_dt[p];
+        k.max_u = B->max_u[p];
+        k.max_v = B->max_v[p];
+        k.step = B->step[p];
+    }
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->ctl, B->h_ctl, B->count * sizeof(StepCtl), hipMemcpyHostToDevice, s));
+    fl_status st;
+    if (graph && !B->gx_pre) {
+        if ((st = capture(B, true, &B->gr_pre, &B->gx_pre)) != FL_OK) return st;
+        if ((st = capture(B, false, &B->gr_post, &B->gx_post)) != FL_OK) return st;
+    }
+    if (step_ms)
+        while ((int64_t)B->ev.size() < 2 * steps) {
+            hipEvent_t e;
+            FL_HIP_TRY(ctx, hipEventCreate(&e));
+            B->ev.push_back(e);
+        }
+    for (int64_t k = 0; k < steps; ++k) {
+        if (step_ms) FL_HIP_TRY(ctx, hipEventRecord(B->ev[2 * k], s));
+        if (graph) FL_HIP_TRY(ctx, hipGraphLaunch(B->gx_pre, s));
+        else if ((st = launch_pre(B, s)) != FL_OK) return st;
+        if ((st = launch_pcg_coop(B, s, B->rhs, B->x, B->kmax_all)) != FL_OK) return st;
+        if (graph) FL_HIP_TRY(ctx, hipGraphLaunch(B->gx_post, s));
+        else if ((st = launch_post(B, s)) != FL_OK) return st;
+        if (step_ms) FL_HIP_TRY(ctx, hipEventRecord(B->ev[2 * k + 1], s));
+    }
+    FL_HIP_TRY(ctx, hipMemcpyAsync(B->h_ctl, B->ctl, B->count * sizeof(StepCtl), hipMemcpyDeviceToHost, s));
+    FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (step_ms)
+        for (int64_t k = 0; k < steps; ++k) {
+            float ms = 0.0f;
+            FL_HIP_TRY(ctx, hipEventElapsedTime(&ms, B->ev[2 * k], B->ev[2 * k + 1]));
+            step_ms[k] = ms;
+        }
+    int first_bad = -1;
+    const int64_t nsteps = steps;
+    for (int p = 0; p < B->count; ++p) {
+        const StepCtl &k = B->h_ctl[p];
+        const int64_t taken = k.step - B->step[p];
+        B->step[p] = k.step;
+        B->time[p] = k.time;
+        B->current_dt[p] = k.current_dt;
+        B->max_u[p] = k.max_u;
+        B->max_v[p] = k.max_v;
+        if (k.error != RE_NONE && first_bad < 0) first_bad = p;
+        if (reps) {
+            fl_batch2d_run_report &r = reps[p];
+            r.steps = taken;
+            r.time = k.time;
+            r.pressure_iterations = k.iters;
+            r.last_pressure_residual = k.last_residual;
+            r.max_divergence = k.max_div;
+            r.max_speed = k.max_speed;
+            r.reached_end_time = ctl_reached_end(k.time, k.t_end) ? 1 : 0;
+            r.status = k.error == RE_NONE ? FL_OK : FL_ERR_SOLVER;
+        }
+    }
+    (void)nsteps;
+    if (first_bad < 0) return FL_OK;
+    const StepCtl &k = B->h_ctl[first_bad];
+    if (k.error == RE_MIN_DT)
+        return set_error(ctx, FL_ERR_SOLVER,
+                         "required stable time step %.3e is below configured min_dt %.3e; CFL "
+                         "constraint=%.3e, diffusion constraint=%.3e",
+                         k.vals[0], k.min_dt, k.vals[1], k.vals[2]);
+    if (k.error == RE_PCG) return set_error(ctx, FL_ERR_SOLVER, "%s", pcg_text(k.which));
+    const Prob &P = B->hp[first_bad];
+    const char *names[4] = {"pressure", "u", "v", "speed"};
+    const double *arrs[4] = {B->x + (size_t)P.tile0 * kT, B->u + P.uoff, B->v + P.voff,
+                             B->sp + (size_t)P.tile0 * kT};
+    double val = 0.0;
+    FL_HIP_TRY(ctx, hipMemcpy(&val, arrs[k.which] + k.index, sizeof(double), hipMemcpyDeviceToHost));
+    return set_error(ctx, FL_ERR_SOLVER, "Non-finite %s value at flat index %llu: %g (step %lld)",
+                     names[k.which], k.index, val, (long long)k.step);
+}
+
+// Chooses the persistent PCG's grid size: every co-resident candidate runs
+// `iters` iterations on a fixed pseudo-random right-hand side over all grids
+// (scratch x and rhs; the solver state is untouched), the fastest of three
+// timed runs wins.
+fl_status autotune(Batch2 *B, int iters, int32_t *chosen, double *best_ms, int32_t cap,
+                   int32_t *tb, double *tms, int32_t *tlen) {
+    fl_ctx *ctx = B->ctx;
+    hipStream_t s = ctx->stream;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int per_cu = 0;
+    FL_HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg, kT, 0));
+    const uint32_t resident = (uint32_t)std::max(per_cu, 1) * (uint32_t)ctx->num_cu;
+    std::vector<uint32_t> cand;
+    for (uint32_t m : {1u, 2u, 4u, 8u})
+        for (uint32_t G : {(uint32_t)ctx->num_cu * m / 2, (uint32_t)ctx->num_cu * m})
+            if (G >= 1 && G <= resident && G <= B->ntiles &&
+                std::find(cand.begin(), cand.end(), G) == cand.end())
+                cand.push_back(G);
+    if (cand.empty()) cand.push_back(std::min<uint32_t>(B->ntiles, resident));
+    double *rhs = nullptr, *x = nullptr;
+    FL_HIP_TRY(ctx, hipMalloc(&rhs, B->ncell * sizeof(double)));
+    if (hipMalloc(&x, B->ncell * sizeof(double)) != hipSuccess) {
+        (void)hipFree(rhs);
+        return set_error(ctx, FL_ERR_OUT_OF_MEMORY, "autotune scratch allocation failed");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    fl_status st = FL_OK;
+    if (tlen) *tlen = 0;
+    uint32_t best = B->pcg_blocks;
+    double best_t = 1e300;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        st = set_error(ctx, FL_ERR_HIP, "event creation failed");
+    if (st == FL_OK) {
+        k_fill_rhs<<<256, 256, 0, s>>>(rhs, B->ncell, 12345);
+        for (int p = 0; p < B->count; ++p) {
+            B->h_active[p] = 1;
+            B->h_fin[p] = 0;
+            B->h_fin[B->count + p] = 0;
+            B->h_sb[p] = PcgB{};
+            B->h_sb[p].first = 1;
+        }
+        (void)hipMemcpyAsync(B->active, B->h_active, B->count * sizeof(int), hipMemcpyHostToDevice, s);
+    }
+    for (uint32_t G : cand) {
+        if (st != FL_OK) break;
+        if ((st = set_pcg_blocks(B, G)) != FL_OK) break;
+        double tmin = 1e300;
+        for (int rep = 0; rep < 4 && st == FL_OK; ++rep) {
+            if (hipMemsetAsync(x, 0, B->ncell * sizeof(double), s) != hipSuccess ||
+                hipMemcpyAsync(B->fin, B->h_fin, 2 * B->count * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(B->sb, B->h_sb, B->count * sizeof(PcgB), hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipEventRecord(e0, s) != hipSuccess) {
+                st = set_error(ctx, FL_ERR_HIP, "autotune setup failed");
+                break;
+            }
+            if ((st = launch_pcg_coop(B, s, rhs, x, iters)) != FL_OK) break;
+            float ms = 0.0f;
+            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                st = set_error(ctx, FL_ERR_HIP, "autotune timing failed");
+                break;
+            }
+            if (rep > 0) tmin = std::min(tmin, (double)ms);  // the first run warms up
+        }
+        if (st == FL_OK && tb && tms && tlen && *tlen < cap) {
+            tb[*tlen] = (int32_t)G;
+            tms[*tlen] = tmin;
+            *tlen += 1;
+        }
+        if (st == FL_OK && tmin < best_t) {
+            best_t = tmin;
+            best = G;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(rhs);
+    (void)hipFree(x);
+    if (st != FL_OK) return st;
+    if ((st = set_pcg_blocks(B, best)) != FL_OK) return st;
+    FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (chosen) *chosen = (int32_t)best;
+    if (best_ms) *best_ms = best_t;
     return FL_OK;
 }
 
@@ -1328,6 +1740,22 @@ fl_status fl_batch2d_step(fl_batch2d *b, fl_grid2d_step_report *reps) {
 fl_status fl_batch2d_get_fields(fl_batch2d *b, int32_t index, double *u, double *v, double *p) {
     if (!b || index < 0 || index >= b->b->count) return FL_ERR_INVALID_ARGUMENT;
     return get_fields(b->b, index, u, v, p);
+}
+
+fl_status fl_batch2d_run(fl_batch2d *b, int64_t steps, int32_t use_graphs,
+                         fl_batch2d_run_report *reports, double *step_ms) {
+    if (!b) return FL_ERR_INVALID_ARGUMENT;
+    if (steps < 0) return set_error(b->b->ctx, FL_ERR_INVALID_ARGUMENT, "negative step count");
+    return run_device(b->b, steps, use_graphs != 0, reports, step_ms);
+}
+
+fl_status fl_batch2d_autotune(fl_batch2d *b, int32_t iterations, int32_t *blocks, double *ms,
+                              int32_t table_cap, int32_t *table_blocks, double *table_ms,
+                              int32_t *table_len) {
+    if (!b) return FL_ERR_INVALID_ARGUMENT;
+    if (iterations < 1)
+        return set_error(b->b->ctx, FL_ERR_INVALID_ARGUMENT, "autotune needs at least one iteration");
+    return autotune(b->b, iterations, blocks, ms, table_cap, table_blocks, table_ms, table_len);
 }
 
 }  // extern "C"

@@ -213,3 +213,27 @@ class CavityBatch:
         check(load().fl_batch2d_get_fields(self.handle, index, ptr(u), ptr(v), ptr(p)),
               self.ctx.handle)
         return {"u": u, "v": v, "p": p}
+
+    def run(self, steps: int, use_graphs: bool = True, step_times: bool = False):
+        """`steps` device-driven steps (one host synchronisation).  Returns
+        (per-grid run reports, per-step device milliseconds or None); raises
+        FlursysError with the first failing grid's message."""
+        reps = (_lib.Batch2DRunReport * len(self.cfgs))()
+        ms = np.zeros(max(steps, 1)) if step_times else None
+        st = load().fl_batch2d_run(self.handle, steps, int(use_graphs), reps,
+                                   ptr(ms) if step_times else None)
+        self.last_run = list(reps)
+        check(st, self.ctx.handle)
+        return self.last_run, (ms[:steps] if step_times else None)
+
+    def autotune(self, iterations: int = 20):
+        """Chooses the persistent PCG's grid size; returns (blocks, ms,
+        {candidate blocks: ms})."""
+        g = C.c_int32()
+        ms = C.c_double()
+        tb = (C.c_int32 * 16)()
+        tm = (C.c_double * 16)()
+        n = C.c_int32()
+        check(load().fl_batch2d_autotune(self.handle, iterations, C.byref(g), C.byref(ms), 16,
+                                         tb, tm, C.byref(n)), self.ctx.handle)
+        return g.value, ms.value, {tb[i]: tm[i] for i in range(n.value)}

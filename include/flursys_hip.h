@@ -259,6 +259,35 @@ void fl_batch2d_destroy(fl_batch2d *batch);
 fl_status fl_batch2d_step(fl_batch2d *batch, fl_grid2d_step_report *reports);
 fl_status fl_batch2d_get_fields(fl_batch2d *batch, int32_t index, double *u, double *v,
                                 double *p);
+/* `steps` steps driven entirely on the device (no reference counterpart; the
+ * same numerics as fl_batch2d_step): the dt rule, t_end, counters and error
+ * checks run in kernels, each step is the captured pre-solve half, the
+ * cooperative persistent PCG and the captured post-solve half (HIP graphs
+ * when `use_graphs`), and the host synchronises once.  A grid that fails
+ * stops (its report status is FL_ERR_SOLVER) while the others go on; the
+ * call then returns FL_ERR_SOLVER with the lowest-index grid's message.
+ * reports[count] and step_ms[steps] (device time of each batch step) may be
+ * NULL. */
+typedef struct {
+    int64_t steps;                 /* steps taken in this run */
+    double time;                   /* simulated time after the run */
+    int64_t pressure_iterations;   /* summed over the run */
+    double last_pressure_residual; /* max|r| of the last solve */
+    double max_divergence;         /* of the last step */
+    double max_speed;
+    int32_t reached_end_time;
+    int32_t status;                /* FL_OK, or FL_ERR_SOLVER when the grid failed */
+} fl_batch2d_run_report;
+fl_status fl_batch2d_run(fl_batch2d *batch, int64_t steps, int32_t use_graphs,
+                         fl_batch2d_run_report *reports, double *step_ms);
+/* Autotuner of the persistent PCG's launch shape (no reference counterpart):
+ * times `iterations` PCG iterations over all grids for every co-resident
+ * grid size, keeps the fastest and returns it and its time.  When
+ * `table_blocks` / `table_ms` (capacity `table_cap`) are given they receive
+ * every candidate and its time, `*table_len` their number. */
+fl_status fl_batch2d_autotune(fl_batch2d *batch, int32_t iterations, int32_t *blocks,
+                              double *ms, int32_t table_cap, int32_t *table_blocks,
+                              double *table_ms, int32_t *table_len);
 
 /* ---------------------------------------------------------------------- */
 /* CSR linear algebra: replaces CsrMatrix (src/linear/csr.rs:4-125) and the

@@ -212,3 +212,74 @@ def test_batch_respects_each_t_end(gpu_ctx):
         reps = b.step()
     assert abs(reps[1].time - 4e-3) < 1e-12
     assert reps[1].iteration == 4
+
+
+def uneven_cfgs():
+    """Uneven cavities with the reference's adaptive dt rule and defaults
+    (warm start, max|r| < 1e-5), as in the batched benchmark."""
+    ts = I.TimeStepSettings(adaptive=True, target_cfl=0.5, target_diffusion_number=0.25)
+    return [I.SimulationConfig.cavity(n, reynolds=re, time_step=ts)
+            for n, re in ((16, 100.0), (37, 400.0), (64, 1000.0), (24, 250.0))]
+
+
+def all_fields(b):
+    return [b.fields(k) for k in range(len(b.cfgs))]
+
+
+def test_device_run_matches_host_steps(gpu_ctx):
+    print("[gpu] batch device run vs host steps", flush=True)
+    host = I.CavityBatch(uneven_cfgs(), ctx=gpu_ctx)
+    for _ in range(6):
+        host.step()
+    dev = I.CavityBatch(uneven_cfgs(), ctx=gpu_ctx)
+    reps, ms = dev.run(6, use_graphs=False, step_times=True)
+    assert all(r.steps == 6 and r.status == 0 for r in reps)
+    assert len(ms) == 6 and np.all(ms > 0)
+    for fh, fd in zip(all_fields(host), all_fields(dev)):
+        for name in ("u", "v", "p"):
+            assert rel(fd[name], fh[name]) < 1e-12, name
+
+
+def test_graph_replays_match_eager_runs(gpu_ctx):
+    """Captured step halves replayed with eager work and host reads between
+    the replays give the eager run's fields after every replay."""
+    print("[gpu] batch graph replays", flush=True)
+    eager = I.CavityBatch(uneven_cfgs(), ctx=gpu_ctx)
+    graph = I.CavityBatch(uneven_cfgs(), ctx=gpu_ctx)
+    for rnd in range(3):
+        eager.run(2, use_graphs=False)
+        graph.run(2, use_graphs=True)
+        for fe, fg in zip(all_fields(eager), all_fields(graph)):
+            for name in ("u", "v", "p"):
+                assert np.array_equal(fe[name], fg[name]), (rnd, name)
+    # eager host-driven steps in between keep both in step
+    eager.step()
+    graph.step()
+    eager.run(1, use_graphs=False)
+    graph.run(1, use_graphs=True)
+    for fe, fg in zip(all_fields(eager), all_fields(graph)):
+        assert np.array_equal(fe["p"], fg["p"])
+
+
+def test_device_run_reports_the_first_error_and_continues(gpu_ctx):
+    ts_bad = I.TimeStepSettings(adaptive=True, min_dt=0.5, max_dt=1.0)  # stable dt < min_dt
+    cfgs = uneven_cfgs()[:2] + [I.SimulationConfig.cavity(20, dt=0.6, time_step=ts_bad)]
+    b = I.CavityBatch(cfgs, ctx=gpu_ctx)
+    with pytest.raises(fa.FlursysError) as e:
+        b.run(3)
+    assert "below configured min_dt" in e.value.message
+    reps = b.last_run
+    assert reps[2].status != 0 and reps[2].steps == 0
+    assert reps[0].status == 0 and reps[0].steps == 3
+
+
+def test_autotune_keeps_results(gpu_ctx):
+    print("[gpu] batch autotune", flush=True)
+    ref = I.CavityBatch(uneven_cfgs(), ctx=gpu_ctx)
+    ref.run(4)
+    b = I.CavityBatch(uneven_cfgs(), ctx=gpu_ctx)
+    blocks, ms, table = b.autotune(10)
+    assert blocks >= 1 and ms > 0 and table[blocks] == ms and min(table.values()) == ms
+    b.run(4)
+    for fr, fb in zip(all_fields(ref), all_fields(b)):
+        assert np.array_equal(fr["p"], fb["p"])
