@@ -175,13 +175,20 @@ def run_batch2d(fa, ctx, args):
     log(f"[bench] batch2d: {len(cfgs)} cavities, {cells} cells, n "
         f"{min(c.nx for c in cfgs)}-{max(c.nx for c in cfgs)}")
     out = {}
+    chosen, table = None, []
     # the graph-captured run is the measurement; a shorter eager run (the
     # same launches issued one by one) is timed beside it
     for mode, graphs, total in (("graphs", True, args.batch_steps),
                                 ("eager", False, min(args.batch_steps, 50))):
         b = I.CavityBatch(cfgs, ctx=ctx)
-        blocks, tune_ms, table = b.autotune(20)
-        log(f"[bench] batch2d {mode}: autotune (blocks: ms per 20 iterations) {table}")
+        if chosen is None:
+            chosen, table = b.autotune(args.tune_iterations)
+            log(f"[bench] batch2d: autotune over (threads, blocks, partition), solves capped at "
+                f"{args.tune_iterations} iterations: chosen {chosen}")
+            for t in table:
+                log(f"[bench]   {t}")
+        else:
+            b.set_launch(chosen["threads"], chosen["blocks"], chosen["partition"])
         b.run(5, use_graphs=graphs)  # warmup (captures the graphs)
         chunk, left, ms, iters = 25, total, [], 0
         t0 = time.perf_counter()
@@ -196,8 +203,8 @@ def run_batch2d(fa, ctx, args):
                 f"{np.mean(ms):.2f} ms/step")
         elapsed = time.perf_counter() - t0
         b.close()
-        out[mode] = {"elapsed_s": elapsed, "steps": total, "step_ms": ms, "blocks": blocks,
-                     "autotune_ms": tune_ms, "autotune_table": table, "pressure_iterations": iters}
+        out[mode] = {"elapsed_s": elapsed, "steps": total, "step_ms": ms,
+                     "pressure_iterations": iters}
     g = out["graphs"]
     ms = np.array(g["step_ms"])
     steps = args.batch_steps
@@ -223,8 +230,9 @@ def run_batch2d(fa, ctx, args):
                                 "p99": float(np.percentile(ms, 99)),
                                 "max": float(ms.max())},
             "pcg_iterations_per_cavity_step": g["pressure_iterations"] / (len(cfgs) * steps),
-            "autotuned_pcg_blocks": g["blocks"],
-            "autotune_ms_per_20_iterations": {str(k): v for k, v in g["autotune_table"].items()},
+            "autotuned_pcg_launch": chosen,
+            "autotune_solve_ms": {f"{t['threads']}x{t['blocks']}/{'work' if t['partition'] else 'tiles'}":
+                                  round(t["ms"], 3) for t in table},
             "eager_cavity_steps_per_s": len(cfgs) * out["eager"]["steps"] / out["eager"]["elapsed_s"],
             "eager_steps": out["eager"]["steps"],
             "eager_step_latency_ms_p50": float(np.percentile(out["eager"]["step_ms"], 50)),
@@ -245,6 +253,8 @@ def main():
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=0.0)
+    ap.add_argument("--tune-iterations", type=int, default=1300,
+                    help="iteration cap of the batch autotuner's solves")
     ap.add_argument("--variants", default="plain_fp64_pcg,mixed_ir,mg_fcg,learned_fcg")
     ap.add_argument("--weights", default=os.path.join(ROOT, "flursys_amd", "weights",
                                                       "unet64_cavity.npy"),

@@ -78,6 +78,14 @@ class Batch2DRunReport(C.Structure):
                 ("status", C.c_int32)]
 
 
+PARTITION_TILES, PARTITION_WORK = 0, 1
+
+
+class Batch2DTuning(C.Structure):
+    _fields_ = [("threads", C.c_int32), ("blocks", C.c_int32), ("partition", C.c_int32),
+                ("reserved", C.c_int32), ("ms", C.c_double)]
+
+
 class UNetConfig(C.Structure):
     _fields_ = [("n", C.c_int32), ("max_batch", C.c_int32), ("seed", C.c_uint64)]
 
@@ -162,8 +170,10 @@ SIGNATURES = {
     "fl_batch2d_step": (C.c_int, [_P, C.POINTER(Grid2DStepReport)]),
     "fl_batch2d_get_fields": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
     "fl_batch2d_run": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(Batch2DRunReport), _P]),
-    "fl_batch2d_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _D, C.c_int32,
-                                      C.POINTER(C.c_int32), _D, C.POINTER(C.c_int32)]),
+    "fl_batch2d_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(Batch2DTuning), C.c_int32,
+                                      C.POINTER(Batch2DTuning), C.POINTER(C.c_int32)]),
+    "fl_batch2d_set_launch": (C.c_int, [_P, C.POINTER(Batch2DTuning)]),
+    "fl_batch2d_get_launch": (C.c_int, [_P, C.POINTER(Batch2DTuning)]),
     "fl_csr_upload": (C.c_int,[_P, C.c_int64, C.c_int64, _P, _P, _P, C.c_int64,
                                 C.POINTER(_P), C.POINTER(ErrDetail)]),
     "fl_csr_destroy": (None, [_P]),

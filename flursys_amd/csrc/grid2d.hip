@@ -78,7 +78,7 @@ struct PcgB {  // written by the owner block in phase B (and the host)
     long long iters;
     int first, status;
 };
-enum { ST_OK = 0, ST_SINGULAR = 1, ST_ALPHA = 2, ST_RESIDUAL = 3 };
+enum { ST_OK = 0, ST_SINGULAR = 1, ST_ALPHA = 2, ST_RESIDUAL = 3, ST_BARRIER = 4 };
 
 struct Diag {
     double max_div, max_speed, max_u, max_v, du, dv;
@@ -383,6 +383,21 @@ __global__ void __launch_bounds__(kT) k_fold_diag(const Prob *probs, const int *
 }
 
 // ------------------------------------------------------ persistent PCG
+// Work layout: a block of NT threads owns the contiguous tile range
+// [bt0[b], bt0[b+1]); its waves stream through the 64-cell sub-tiles of that
+// range independently (sub-tile s goes to wave s mod NT/64) and write one
+// partial row per sub-tile.  Every block folds the partial rows of the grids
+// its range touches with groups of 256 threads: thread u sums rows u,
+// u + 256, ... in order, each wave combines its lanes by a fixed butterfly
+// and the group adds its four wave totals in order.  So every block gets
+// the same scalars, and results depend neither on the number of blocks nor
+// on their size.  Each block keeps the per-grid solver state (rz, the
+// previous rz, rr0, iteration count) in LDS, computing it exactly as every
+// other block does; a grid's owner block also writes it out for the host.
+constexpr int kSub = kT / 64;        // sub-tiles per tile
+constexpr int kMaxBlockGrids = 256;  // grids one PCG block may touch
+constexpr uint32_t kFoldT = 256;     // threads folding one grid's rows
+
 struct PcgArgs {
     const Prob *probs;
     const uint32_t *tile_prob;
@@ -391,325 +406,448 @@ struct PcgArgs {
     const uint8_t *flags;
     const double *rhs;
     double *x, *r, *d0, *d1, *q;
-    double *pa, *pb, *p0;  // per-tile partials
+    double *pa, *pb, *p0;  // per-sub-tile partial rows: 1, 5 and 2 columns
     PcgA *sa;
     PcgB *sb;
     const int *active;  // grids taking part in this solve
     int *fin;  // fin[parity * nprob + grid]: grid finished in that loop (the
                // host sets fin[1][g] = !active[g], the "loop -1" verdicts)
     const uint32_t *bt0;  // first tile of each block, gridDim.x + 1 entries
+    unsigned *bar;        // grid barrier: arrival count [0], generation [32], abort [64]
 };
 
+// Grid-wide barrier for a co-resident (cooperative) grid.  The last block to
+// arrive resets the arrival count and then advances the generation the
+// others wait on, so the count is zero again after every barrier and between
+// launches.  Thread 0's device-scope fences publish the block's writes
+// before it arrives and make the other blocks' writes visible after.  The
+// wait is bounded: a block that waits about four seconds (far beyond any
+// phase of the solve) raises the abort word, every barrier after that
+// returns false at once, and the kernel ends with the abort word set for
+// the host to report.
+constexpr unsigned kBarrierSpins = 1u << 22;
+__device__ inline bool grid_barrier(unsigned *bar) {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned *count = bar, *gen = bar + 32, *abort = bar + 64;
+        int ok = __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+        if (ok) {
+            const unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            if (atomicAdd(count, 1u) == gridDim.x - 1) {
+                __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __threadfence();
+                __hip_atomic_store(gen, g + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                unsigned spins = 0;
+                while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                    if (++spins > kBarrierSpins ||
+                        __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                        __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            __threadfence();
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
 // Jacobi inverse diagonal from the open-neighbour bits.
-__device__ inline double inv_diag(const Prob &P, uint8_t f) {
-    const double dg = P.cx * (double)(((f & F_E) != 0) + ((f & F_W) != 0)) +
-                      P.cy * (double)(((f & F_N) != 0) + ((f & F_S) != 0));
+__device__ inline double inv_diag(double cx, double cy, uint8_t f) {
+    const double dg = cx * (double)(((f & F_E) != 0) + ((f & F_W) != 0)) +
+                      cy * (double)(((f & F_N) != 0) + ((f & F_S) != 0));
     return dg > 0.0 ? 1.0 / dg : 1.0;
 }
 
-template <int K>
-__device__ inline void fold_tiles(const double *part, uint32_t tile0, uint32_t nt, int nsum,
-                                  double (&out)[K]) {
-    double s[K - 1 > 0 ? K - 1 : 1], m[1];
-    (void)nsum;
-    fold_partials<K - 1, 1>(part + (size_t)tile0 * K, nt, s, m);
-#pragma unroll
-    for (int a = 0; a < K - 1; ++a) out[a] = s[a];
-    out[K - 1] = m[0];
+// A grid as one PCG block sees it: geometry, the scalars of the current
+// phase and the solver state.
+struct GridL {
+    uint32_t tile0, tend, n, nx;
+    uint32_t fxm, fxs, owner;  // x = c mod nx via FastDiv's multiplier and shift
+    int32_t active;
+    double cx, cy;
+    double a, b;  // phase A: beta, mean of z; phase B: alpha; setup: mean of b / x
+    double rz, rz_prev, rr0;
+    long long iters;
+    int32_t done, first;
+};
+
+__device__ inline uint32_t grid_col(const GridL &G, uint32_t c) {
+    const uint32_t t = __umulhi(c, G.fxm);
+    return c - ((t + ((c - t) >> 1)) >> G.fxs) * G.nx;
 }
 
-__global__ void __launch_bounds__(kT) k_pcg(PcgArgs A) {
-    cgrp::grid_group gg = cgrp::this_grid();
-    const uint32_t b = blockIdx.x;
-    // block b owns tiles [bt0[b], bt0[b+1]): contiguous ranges of equal
-    // expected work (see set_pcg_blocks)
-    const uint32_t t_lo = A.bt0[b], t_hi = A.bt0[b + 1];
-    const uint32_t tid = threadIdx.x;
-    // ---- init 0: sums of x0 and b = -rhs over the gauge cells
-    for (uint32_t t = t_lo; t < t_hi; ++t) {
-        const uint32_t pi = A.tile_prob[t];
-        double s[2] = {0.0, 0.0}, m[1] = {0.0};
-        if (A.active[pi]) {
-            const Prob &P = A.probs[pi];
-            const uint32_t c = (t - P.tile0) * kT + tid;
-            const size_t base = (size_t)P.tile0 * kT;
-            if (c < P.n && !(A.flags[base + c] & F_ID)) {
-                s[0] = P.guess == FL_GUESS_ZERO ? 0.0 : A.x[base + c];
-                s[1] = -A.rhs[base + c];
+// Calls f(s, j, c) for each of this wave's sub-tiles s of [s_lo, s_hi): j is
+// the block-local grid, c this lane's cell in that grid.
+template <uint32_t kW, typename F>
+__device__ inline void for_subtiles(const GridL *gl, uint32_t s_lo, uint32_t s_hi, F &&f) {
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t j = 0;
+    for (uint32_t s = s_lo + wave; s < s_hi; s += kW) {
+        const uint32_t t = s / kSub;
+        while (t >= gl[j].tend) ++j;
+        f(s, j, (t - gl[j].tile0) * kT + (s % kSub) * 64 + lane);
+    }
+}
+
+// Folds the partial rows (NS sums, then NM maxima per row) of every grid j
+// of the block with want(j), then calls use(j, sums, maxima) in one thread.
+// Ends with a block barrier, so what use() wrote to LDS is visible.
+template <int NS, int NM, int NT, typename W, typename U>
+__device__ inline void fold_grids(const double *__restrict__ part, const GridL *gl, uint32_t ng,
+                                  double (*red)[8], W &&want, U &&use) {
+    static_assert(NS + NM <= 8, "at most 8 columns");
+    constexpr int K = NS + NM;
+    constexpr uint32_t kG = NT / kFoldT;
+    const uint32_t q = threadIdx.x / kFoldT, u = threadIdx.x % kFoldT;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (uint32_t j0 = 0; j0 < ng; j0 += kG) {
+        const uint32_t j = j0 + q;
+        const bool on = j < ng && want(j);
+        double v[K];
+#pragma unroll
+        for (int a = 0; a < K; ++a) v[a] = 0.0;
+        if (on) {
+            const uint32_t row0 = gl[j].tile0 * kSub, rows = (gl[j].tend - gl[j].tile0) * kSub;
+#pragma unroll 4
+            for (uint32_t i = u; i < rows; i += kFoldT) {
+                const double *p = part + (size_t)(row0 + i) * K;
+#pragma unroll
+                for (int a = 0; a < NS; ++a) v[a] += p[a];
+#pragma unroll
+                for (int a = NS; a < K; ++a) v[a] = fmax(v[a], p[a]);
             }
         }
-        block_reduce<2, 1>(s, m);
-        if (tid == 0) {
-            A.p0[(size_t)t * 3] = s[0];
-            A.p0[(size_t)t * 3 + 1] = s[1];
-            A.p0[(size_t)t * 3 + 2] = 0.0;
+#pragma unroll
+        for (int a = 0; a < NS; ++a) v[a] = wave_sum(v[a]);
+#pragma unroll
+        for (int a = NS; a < K; ++a) v[a] = wave_max(v[a]);
+        if (lane == 0)
+#pragma unroll
+            for (int a = 0; a < K; ++a) red[wave][a] = v[a];
+        __syncthreads();
+        if (on && u == 0) {
+            double sm[NS > 0 ? NS : 1], mx[NM > 0 ? NM : 1];
+            const uint32_t w0 = q * (kFoldT / 64);
+#pragma unroll
+            for (int a = 0; a < NS; ++a) {
+                double t = red[w0][a];
+                for (uint32_t w = 1; w < kFoldT / 64; ++w) t += red[w0 + w][a];
+                sm[a] = t;
+            }
+#pragma unroll
+            for (int a = 0; a < NM; ++a) {
+                double t = red[w0][NS + a];
+                for (uint32_t w = 1; w < kFoldT / 64; ++w) t = fmax(t, red[w0 + w][NS + a]);
+                mx[a] = t;
+            }
+            use(j, sm, mx);
         }
         __syncthreads();
     }
-    gg.sync();
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
+    constexpr uint32_t kW = NT / 64;
+    __shared__ GridL gl[kMaxBlockGrids];
+    __shared__ double red[kW][8];
+    const uint32_t b = blockIdx.x, lane = threadIdx.x & 63;
+    const uint32_t t_lo = A.bt0[b], t_hi = A.bt0[b + 1];
+    const uint32_t s_lo = t_lo * kSub, s_hi = t_hi * kSub;
+    // the block's tiles belong to grids [g_lo, g_lo + ng) (tiles are grid-major)
+    const uint32_t g_lo = t_lo < t_hi ? A.tile_prob[t_lo] : 0u;
+    const uint32_t ng = t_lo < t_hi ? A.tile_prob[t_hi - 1] - g_lo + 1 : 0u;
+    for (uint32_t j = threadIdx.x; j < ng; j += NT) {
+        const Prob &P = A.probs[g_lo + j];
+        const PcgB sb = A.sb[g_lo + j];
+        GridL &G = gl[j];
+        G.tile0 = P.tile0;
+        G.tend = P.tile0 + P.ntiles;
+        G.n = P.n;
+        G.nx = P.nx;
+        G.fxm = P.fx.m;
+        G.fxs = P.fx.s;
+        G.owner = P.owner;
+        G.active = A.active[g_lo + j];
+        G.cx = P.cx;
+        G.cy = P.cy;
+        G.rz_prev = sb.rz_prev;
+        G.iters = sb.iters;
+        G.first = sb.first;
+        G.rr0 = 0.0;
+        G.rz = 0.0;
+    }
+    __syncthreads();
+    // ---- init 0: sums of x0 and b = -rhs over the gauge cells
+    for_subtiles<kW>(gl, s_lo, s_hi, [&](uint32_t s, uint32_t j, uint32_t c) {
+        const GridL &G = gl[j];
+        const size_t base = (size_t)G.tile0 * kT;
+        double v0 = 0.0, v1 = 0.0;
+        if (G.active && c < G.n && !(A.flags[base + c] & F_ID)) {
+            v0 = A.probs[g_lo + j].guess == FL_GUESS_ZERO ? 0.0 : A.x[base + c];
+            v1 = -A.rhs[base + c];
+        }
+        v0 = wave_sum(v0);
+        v1 = wave_sum(v1);
+        if (lane == 0) {
+            A.p0[(size_t)s * 2] = v0;
+            A.p0[(size_t)s * 2 + 1] = v1;
+        }
+    });
+    if (!grid_barrier(A.bar)) return;
     // ---- init 1: r = P(b) - A x0; identity rows x = value, r = 0
-    {
-        uint32_t cur = ~0u;
-        double mean_b = 0.0;
-        for (uint32_t t = t_lo; t < t_hi; ++t) {
-            const uint32_t pi = A.tile_prob[t];
-            const bool skip = !A.active[pi];
-            const Prob &P = A.probs[pi];
-            if (!skip && pi != cur) {
-                cur = pi;
-                double f[3];
-                fold_tiles<3>(A.p0, P.tile0, P.ntiles, 2, f);
-                mean_b = P.zm ? f[1] * P.inv_count : 0.0;
+    fold_grids<2, 0, NT>(A.p0, gl, ng, red, [&](uint32_t j) { return gl[j].active != 0; },
+                         [&](uint32_t j, const double *sm, const double *) {
+                             const Prob &P = A.probs[g_lo + j];
+                             gl[j].b = P.zm ? sm[1] * P.inv_count : 0.0;
+                         });
+    for_subtiles<kW>(gl, s_lo, s_hi, [&](uint32_t s, uint32_t j, uint32_t c) {
+        const GridL &G = gl[j];
+        const size_t base = (size_t)G.tile0 * kT;
+        double v[4] = {0.0, 0.0, 0.0, 0.0}, mx = 0.0;
+        if (G.active && c < G.n) {
+            const Prob &P = A.probs[g_lo + j];
+            const uint8_t *F = A.flags + base;
+            double *x = A.x + base;
+            const uint8_t f = F[c];
+            double rc = 0.0, z = 0.0;
+            if (f & F_ID) {
+                x[c] = (f & F_OUTLET) ? P.pout : 0.0;
+            } else {
+                const bool zero = P.guess == FL_GUESS_ZERO;
+                auto xv = [&](uint32_t k) -> double {
+                    const uint8_t g = F[k];
+                    if (g & F_ID) return (g & F_OUTLET) ? P.pout : 0.0;
+                    return zero ? 0.0 : x[k];
+                };
+                const double xc = xv(c);
+                double dg = 0.0, sum = 0.0;
+                if (f & F_E) { dg += G.cx; sum += G.cx * xv(c + 1); }
+                if (f & F_W) { dg += G.cx; sum += G.cx * xv(c - 1); }
+                if (f & F_N) { dg += G.cy; sum += G.cy * xv(c + G.nx); }
+                if (f & F_S) { dg += G.cy; sum += G.cy * xv(c - G.nx); }
+                rc = (-A.rhs[base + c] - G.b) - (dg * xc - sum);
+                z = rc * inv_diag(G.cx, G.cy, f);
+                v[1] = z;
+                v[2] = rc;
             }
-            double s[4] = {0, 0, 0, 0}, m[1] = {0.0};
-            if (!skip) {
-                const uint32_t c = (t - P.tile0) * kT + tid;
-                const size_t base = (size_t)P.tile0 * kT;
-                if (c < P.n) {
-                    const uint8_t f = A.flags[base + c];
-                    double *x = A.x + base;
-                    double rc = 0.0, z = 0.0;
-                    if (f & F_ID) {
-                        x[c] = (f & F_OUTLET) ? P.pout : 0.0;
-                    } else {
-                        const bool zero = P.guess == FL_GUESS_ZERO;
-                        auto xv = [&](uint32_t k) -> double {
-                            const uint8_t g = A.flags[base + k];
-                            if (g & F_ID) return (g & F_OUTLET) ? P.pout : 0.0;
-                            return zero ? 0.0 : x[k];
-                        };
-                        const double xc = xv(c);
-                        double dg = 0.0, sum = 0.0;
-                        if (f & F_E) { dg += P.cx; sum += P.cx * xv(c + 1); }
-                        if (f & F_W) { dg += P.cx; sum += P.cx * xv(c - 1); }
-                        if (f & F_N) { dg += P.cy; sum += P.cy * xv(c + P.nx); }
-                        if (f & F_S) { dg += P.cy; sum += P.cy * xv(c - P.nx); }
-                        rc = (-A.rhs[base + c] - mean_b) - (dg * xc - sum);
-                        z = rc * inv_diag(P, f);
-                        s[1] += z;
-                        s[2] += rc;
-                    }
-                    A.r[base + c] = rc;
-                    s[0] += rc * z;
-                    s[3] += rc * rc;
-                    m[0] = fmax(m[0], fabs(rc));
-                }
-            }
-            block_reduce<4, 1>(s, m);
-            if (tid == 0) {
-                double *pb = A.pb + (size_t)t * 5;
-                pb[0] = s[0]; pb[1] = s[1]; pb[2] = s[2]; pb[3] = s[3]; pb[4] = m[0];
-            }
-            __syncthreads();
+            A.r[base + c] = rc;
+            v[0] = rc * z;
+            v[3] = rc * rc;
+            mx = fabs(rc);
         }
-    }
-    gg.sync();
-    // zero-guess problems start from x = 0 on the fluid cells
-    for (uint32_t t = t_lo; t < t_hi; ++t) {
-        const uint32_t pi = A.tile_prob[t];
-        const Prob &P = A.probs[pi];
-        if (!A.active[pi] || P.guess != FL_GUESS_ZERO) continue;
-        const uint32_t c = (t - P.tile0) * kT + tid;
-        const size_t base = (size_t)P.tile0 * kT;
-        if (c < P.n && !(A.flags[base + c] & F_ID)) A.x[base + c] = 0.0;
-    }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) v[a] = wave_sum(v[a]);
+        mx = wave_max(mx);
+        if (lane == 0) {
+            double *pb = A.pb + (size_t)s * 5;
+            pb[0] = v[0]; pb[1] = v[1]; pb[2] = v[2]; pb[3] = v[3]; pb[4] = mx;
+        }
+    });
+    if (!grid_barrier(A.bar)) return;
+    // zero-guess grids start from x = 0 on the fluid cells
+    for_subtiles<kW>(gl, s_lo, s_hi, [&](uint32_t, uint32_t j, uint32_t c) {
+        const GridL &G = gl[j];
+        const size_t base = (size_t)G.tile0 * kT;
+        if (G.active && A.probs[g_lo + j].guess == FL_GUESS_ZERO && c < G.n &&
+            !(A.flags[base + c] & F_ID))
+            A.x[base + c] = 0.0;
+    });
     // ---- CG loop
-    long long k = 0;
-    for (; k < A.kmax; ++k) {
+    for (long long k = 0; k < A.kmax; ++k) {
         const int par = (int)(k & 1), prev = par ^ 1;
-        double *dold = (k & 1) ? A.d1 : A.d0, *dnew = (k & 1) ? A.d0 : A.d1;
-        // phase A
-        {
-            uint32_t cur = ~0u;
-            bool done = false, first = false;
-            double beta = 0.0, mz = 0.0;
-            for (uint32_t t = t_lo; t < t_hi; ++t) {
-                const uint32_t pi = A.tile_prob[t];
-                const Prob &P = A.probs[pi];
-                const bool was = A.fin[prev * A.nprob + pi];
-                if (pi != cur) {
-                    cur = pi;
-                    done = was;
-                    if (!was) {
-                        double f[5];
-                        fold_tiles<5>(A.pb, P.tile0, P.ntiles, 4, f);
-                        const PcgB sb = A.sb[pi];
-                        first = sb.first != 0;
-                        const double rr = f[3], rmax = f[4];
-                        const double rz = P.zm ? f[0] - f[2] * f[1] * P.inv_count : f[0];
-                        mz = P.zm ? f[1] * P.inv_count : 0.0;
-                        const double rr0 = first ? rr : A.sa[pi].rr0;
-                        int status = ST_OK, conv = 0;
-                        beta = 0.0;
-                        if (!first && !isfinite(rmax)) {
-                            status = ST_RESIDUAL;
-                            done = true;
-                        } else if (P.stop_rule == FL_STOP_REL_L2 ? sqrt(rr) <= P.tol * sqrt(rr0)
-                                                                : rmax < P.tol) {
-                            conv = 1;
-                            done = true;
-                        } else if (!first) {
-                            if (fabs(sb.rz_prev) < 1.0e-30) done = true;
-                            else if (sb.iters >= P.max_iter) done = true;
-                            else beta = rz / sb.rz_prev;
-                        }
-                        if (b == P.owner && tid == 0) {
-                            PcgA &sa = A.sa[pi];
-                            sa.rz = rz;
-                            sa.beta = beta;
-                            sa.mz = mz;
-                            sa.rr0 = rr0;
-                            sa.rr = rr;
-                            sa.rmax = rmax;
-                            sa.converged = conv;
-                            sa.status = status;
-                        }
-                    }
-                    if (b == P.owner && tid == 0) A.fin[par * A.nprob + pi] = done ? 1 : 0;
-                }
-                double s[1] = {0.0}, m[1] = {0.0};
-                if (!done) {
-                    const uint32_t c = (t - P.tile0) * kT + tid;
-                    const size_t base = (size_t)P.tile0 * kT;
-                    if (c < P.n) {
-                        const uint8_t f = A.flags[base + c];
-                        const double *r = A.r + base, *dv = dold + base;
-                        auto dn = [&](uint32_t kk) -> double {
-                            const uint8_t g = A.flags[base + kk];
-                            const double z = (g & F_ID) ? r[kk] : r[kk] * inv_diag(P, g) - mz;
-                            return first ? z : z + beta * dv[kk];
-                        };
-                        const double dc = dn(c);
-                        double qc = dc;
-                        if (!(f & F_ID)) {
-                            double dg = 0.0, sum = 0.0;
-                            if (f & F_E) { dg += P.cx; sum += P.cx * dn(c + 1); }
-                            if (f & F_W) { dg += P.cx; sum += P.cx * dn(c - 1); }
-                            if (f & F_N) { dg += P.cy; sum += P.cy * dn(c + P.nx); }
-                            if (f & F_S) { dg += P.cy; sum += P.cy * dn(c - P.nx); }
-                            qc = dg * dc - sum;
-                        }
-                        dnew[base + c] = dc;
-                        A.q[base + c] = qc;
-                        s[0] = dc * qc;
-                    }
-                }
-                block_reduce<1, 1>(s, m);
-                if (tid == 0) {
-                    A.pa[(size_t)t * 2] = s[0];
-                    A.pa[(size_t)t * 2 + 1] = 0.0;
-                }
-                __syncthreads();
-            }
+        const double *dold = (k & 1) ? A.d1 : A.d0;
+        double *dnew = (k & 1) ? A.d0 : A.d1;
+        // phase A scalars: stop test, beta, mean of z
+        for (uint32_t j = threadIdx.x; j < ng; j += NT) {
+            const bool was = A.fin[prev * A.nprob + g_lo + j] != 0;
+            gl[j].done = was ? 1 : 0;
+            if (was && b == gl[j].owner) A.fin[par * A.nprob + g_lo + j] = 1;
         }
-        gg.sync();
-        // phase B
-        {
-            uint32_t cur = ~0u;
-            bool skip = false;
+        __syncthreads();
+        fold_grids<4, 1, NT>(A.pb, gl, ng, red, [&](uint32_t j) { return gl[j].done == 0; },
+                             [&](uint32_t j, const double *sm, const double *mx) {
+            const uint32_t pi = g_lo + j;
+            const Prob &P = A.probs[pi];
+            GridL &G = gl[j];
+            const bool first = G.first != 0;
+            const double rr = sm[3], rmax = mx[0];
+            const double rz = P.zm ? sm[0] - sm[2] * sm[1] * P.inv_count : sm[0];
+            const double mz = P.zm ? sm[1] * P.inv_count : 0.0;
+            const double rr0 = first ? rr : G.rr0;
+            int status = ST_OK, conv = 0;
+            bool done = false;
+            double beta = 0.0;
+            if (!first && !isfinite(rmax)) {
+                status = ST_RESIDUAL;
+                done = true;
+            } else if (P.stop_rule == FL_STOP_REL_L2 ? sqrt(rr) <= P.tol * sqrt(rr0)
+                                                    : rmax < P.tol) {
+                conv = 1;
+                done = true;
+            } else if (!first) {
+                if (fabs(G.rz_prev) < 1.0e-30) done = true;
+                else if (G.iters >= P.max_iter) done = true;
+                else beta = rz / G.rz_prev;
+            }
+            G.a = beta;
+            G.b = mz;
+            G.rz = rz;
+            G.rr0 = rr0;
+            G.done = done ? 1 : 0;
+            if (b == G.owner) {
+                PcgA &sa = A.sa[pi];
+                sa.rz = rz;
+                sa.beta = beta;
+                sa.mz = mz;
+                sa.rr0 = rr0;
+                sa.rr = rr;
+                sa.rmax = rmax;
+                sa.converged = conv;
+                sa.status = status;
+                A.fin[par * A.nprob + pi] = done ? 1 : 0;
+            }
+        });
+        // phase A: d' = P(D^-1 r) + beta d at the stencil points, q = A d'
+        for_subtiles<kW>(gl, s_lo, s_hi, [&](uint32_t s, uint32_t j, uint32_t c) {
+            const GridL &G = gl[j];
+            double v = 0.0;
+            if (!G.done && c < G.n) {
+                const size_t base = (size_t)G.tile0 * kT;
+                const uint8_t *F = A.flags + base;
+                const double *r = A.r + base, *dv = dold + base;
+                // neighbours from the geometry (a closed side still reads a
+                // cell of the grid), so every load is independent of the flags
+                const uint32_t i = grid_col(G, c);
+                const uint32_t ce = i + 1 < G.nx ? c + 1 : c, cw = i > 0 ? c - 1 : c;
+                const uint32_t cn = c + G.nx < G.n ? c + G.nx : c, cs = c >= G.nx ? c - G.nx : c;
+                const uint8_t g[5] = {F[c], F[ce], F[cw], F[cn], F[cs]};
+                const double rv[5] = {r[c], r[ce], r[cw], r[cn], r[cs]};
+                const double dd[5] = {dv[c], dv[ce], dv[cw], dv[cn], dv[cs]};
+                double dn[5];
+#pragma unroll
+                for (int a = 0; a < 5; ++a) {
+                    const double z = (g[a] & F_ID) ? rv[a] : rv[a] * inv_diag(G.cx, G.cy, g[a]) - G.b;
+                    dn[a] = G.first ? z : z + G.a * dd[a];
+                }
+                const uint8_t f = g[0];
+                double qc = dn[0];
+                if (!(f & F_ID)) {
+                    double dg = 0.0, sum = 0.0;
+                    if (f & F_E) { dg += G.cx; sum += G.cx * dn[1]; }
+                    if (f & F_W) { dg += G.cx; sum += G.cx * dn[2]; }
+                    if (f & F_N) { dg += G.cy; sum += G.cy * dn[3]; }
+                    if (f & F_S) { dg += G.cy; sum += G.cy * dn[4]; }
+                    qc = dg * dn[0] - sum;
+                }
+                dnew[base + c] = dn[0];
+                A.q[base + c] = qc;
+                v = dn[0] * qc;
+            }
+            v = wave_sum(v);
+            if (lane == 0) A.pa[s] = v;
+        });
+        if (!grid_barrier(A.bar)) return;
+        // phase B scalars: alpha
+        for (uint32_t j = threadIdx.x; j < ng; j += NT)
+            gl[j].done = A.fin[par * A.nprob + g_lo + j] != 0 ? 1 : 0;
+        __syncthreads();
+        fold_grids<1, 0, NT>(A.pa, gl, ng, red, [&](uint32_t j) { return gl[j].done == 0; },
+                             [&](uint32_t j, const double *sm, const double *) {
+            const uint32_t pi = g_lo + j;
+            GridL &G = gl[j];
+            const double den = sm[0];
+            int status = ST_OK;
             double alpha = 0.0;
-            for (uint32_t t = t_lo; t < t_hi; ++t) {
-                const uint32_t pi = A.tile_prob[t];
-                const Prob &P = A.probs[pi];
-                if (pi != cur) {
-                    cur = pi;
-                    skip = A.fin[par * A.nprob + pi] != 0;
-                    if (!skip) {
-                        double s[1], m[1];
-                        fold_partials<1, 1>(A.pa + (size_t)P.tile0 * 2, P.ntiles, s, m);
-                        const double den = s[0];
-                        int status = ST_OK;
-                        alpha = 0.0;
-                        if (fabs(den) < 1.0e-30 || !isfinite(den)) status = ST_SINGULAR;
-                        else {
-                            alpha = A.sa[pi].rz / den;
-                            if (!isfinite(alpha)) status = ST_ALPHA;
-                        }
-                        if (b == P.owner && tid == 0) {
-                            PcgB &sb = A.sb[pi];
-                            sb.rz_prev = A.sa[pi].rz;
-                            sb.alpha = alpha;
-                            sb.first = 0;
-                            sb.status = status;
-                            if (status == ST_OK) sb.iters += 1;
-                            else A.fin[par * A.nprob + pi] = 1;
-                        }
-                        skip = status != ST_OK;
-                    }
-                }
-                double s[4] = {0, 0, 0, 0}, m[1] = {0.0};
-                if (!skip) {
-                    const uint32_t c = (t - P.tile0) * kT + tid;
-                    const size_t base = (size_t)P.tile0 * kT;
-                    if (c < P.n) {
-                        const uint8_t f = A.flags[base + c];
-                        A.x[base + c] += alpha * dnew[base + c];
-                        const double rc = A.r[base + c] - alpha * A.q[base + c];
-                        A.r[base + c] = rc;
-                        if (!(f & F_ID)) {
-                            const double z = rc * inv_diag(P, f);
-                            s[0] += rc * z;
-                            s[1] += z;
-                            s[2] += rc;
-                        } else {
-                            s[0] += rc * rc;
-                        }
-                        s[3] += rc * rc;
-                        m[0] = fmax(m[0], fabs(rc));
-                    }
-                }
-                block_reduce<4, 1>(s, m);
-                if (tid == 0) {
-                    double *pb = A.pb + (size_t)t * 5;
-                    pb[0] = s[0]; pb[1] = s[1]; pb[2] = s[2]; pb[3] = s[3]; pb[4] = m[0];
-                }
-                __syncthreads();
+            if (fabs(den) < 1.0e-30 || !isfinite(den)) status = ST_SINGULAR;
+            else {
+                alpha = G.rz / den;
+                if (!isfinite(alpha)) status = ST_ALPHA;
             }
-        }
-        gg.sync();
-        bool all = true;
-        for (uint32_t p = 0; p < A.nprob; ++p) all = all && A.fin[par * A.nprob + p];
-        if (all) break;
+            G.rz_prev = G.rz;
+            G.first = 0;
+            if (status == ST_OK) G.iters += 1;
+            G.a = alpha;
+            G.done = status != ST_OK ? 1 : 0;
+            if (b == G.owner) {
+                PcgB &sb = A.sb[pi];
+                sb.rz_prev = G.rz;
+                sb.alpha = alpha;
+                sb.first = 0;
+                sb.status = status;
+                sb.iters = G.iters;
+                if (status != ST_OK) A.fin[par * A.nprob + pi] = 1;
+            }
+        });
+        // phase B: x += alpha d', r -= alpha q, partials of the next test
+        for_subtiles<kW>(gl, s_lo, s_hi, [&](uint32_t s, uint32_t j, uint32_t c) {
+            const GridL &G = gl[j];
+            double v[4] = {0.0, 0.0, 0.0, 0.0}, mx = 0.0;
+            if (!G.done && c < G.n) {
+                const size_t i = (size_t)G.tile0 * kT + c;
+                const uint8_t f = A.flags[i];
+                const double xv = A.x[i], dv = dnew[i], r0 = A.r[i], qv = A.q[i];
+                A.x[i] = xv + G.a * dv;
+                const double rc = r0 - G.a * qv;
+                A.r[i] = rc;
+                if (!(f & F_ID)) {
+                    const double z = rc * inv_diag(G.cx, G.cy, f);
+                    v[0] = rc * z;
+                    v[1] = z;
+                    v[2] = rc;
+                } else {
+                    v[0] = rc * rc;
+                }
+                v[3] = rc * rc;
+                mx = fabs(rc);
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a) v[a] = wave_sum(v[a]);
+            mx = wave_max(mx);
+            if (lane == 0) {
+                double *pb = A.pb + (size_t)s * 5;
+                pb[0] = v[0]; pb[1] = v[1]; pb[2] = v[2]; pb[3] = v[3]; pb[4] = mx;
+            }
+        });
+        if (!grid_barrier(A.bar)) return;
+        int open = 0;
+        for (uint32_t p = threadIdx.x; p < A.nprob; p += NT) open |= A.fin[par * A.nprob + p] == 0;
+        if (!__syncthreads_or(open)) break;
     }
     // ---- final: project x to the zero-mean gauge (or write identity values)
-    for (uint32_t t = t_lo; t < t_hi; ++t) {
-        const uint32_t pi = A.tile_prob[t];
-        double s[2] = {0.0, 0.0}, m[1] = {0.0};
-        if (A.active[pi]) {
-            const Prob &P = A.probs[pi];
-            const uint32_t c = (t - P.tile0) * kT + tid;
-            const size_t base = (size_t)P.tile0 * kT;
-            if (c < P.n && !(A.flags[base + c] & F_ID)) s[0] = A.x[base + c];
+    for_subtiles<kW>(gl, s_lo, s_hi, [&](uint32_t s, uint32_t j, uint32_t c) {
+        const GridL &G = gl[j];
+        const size_t base = (size_t)G.tile0 * kT;
+        double v = 0.0;
+        if (G.active && c < G.n && !(A.flags[base + c] & F_ID)) v = A.x[base + c];
+        v = wave_sum(v);
+        if (lane == 0) {
+            A.p0[(size_t)s * 2] = v;
+            A.p0[(size_t)s * 2 + 1] = 0.0;
         }
-        block_reduce<2, 1>(s, m);
-        if (tid == 0) {
-            A.p0[(size_t)t * 3] = s[0];
-            A.p0[(size_t)t * 3 + 1] = 0.0;
-            A.p0[(size_t)t * 3 + 2] = 0.0;
-        }
-        __syncthreads();
-    }
-    gg.sync();
-    uint32_t cur = ~0u;
-    double mean_x = 0.0;
-    for (uint32_t t = t_lo; t < t_hi; ++t) {
-        const uint32_t pi = A.tile_prob[t];
-        const Prob &P = A.probs[pi];
-        if (!A.active[pi]) continue;
-        if (pi != cur) {
-            cur = pi;
-            double f[3];
-            fold_tiles<3>(A.p0, P.tile0, P.ntiles, 2, f);
-            mean_x = P.zm ? f[0] * P.inv_count : 0.0;
-        }
-        const uint32_t c = (t - P.tile0) * kT + tid;
-        const size_t base = (size_t)P.tile0 * kT;
-        if (c < P.n) {
-            const uint8_t f = A.flags[base + c];
-            if (f & F_ID) A.x[base + c] = (f & F_OUTLET) ? P.pout : 0.0;
-            else A.x[base + c] -= mean_x;
-        }
-    }
+    });
+    if (!grid_barrier(A.bar)) return;
+    fold_grids<2, 0, NT>(A.p0, gl, ng, red, [&](uint32_t j) { return gl[j].active != 0; },
+                         [&](uint32_t j, const double *sm, const double *) {
+                             const Prob &P = A.probs[g_lo + j];
+                             gl[j].b = P.zm ? sm[0] * P.inv_count : 0.0;
+                         });
+    for_subtiles<kW>(gl, s_lo, s_hi, [&](uint32_t, uint32_t j, uint32_t c) {
+        const GridL &G = gl[j];
+        if (!G.active || c >= G.n) return;
+        const size_t i = (size_t)G.tile0 * kT + c;
+        const uint8_t f = A.flags[i];
+        if (f & F_ID) A.x[i] = (f & F_OUTLET) ? A.probs[g_lo + j].pout : 0.0;
+        else A.x[i] -= G.b;
+    });
 }
 
 // ------------------------------------------------ device-side step control
@@ -792,11 +930,12 @@ __global__ void k_run_setup(const Prob *probs, StepCtl *ctl, int count, int *act
 // ensure_finite check (incompressible.rs:1707-1742; pressure, u, v, speed)
 // and the maxima the next step's dt rule reads.
 __global__ void k_run_finish(StepCtl *ctl, int count, const int *active, const double *dts,
-                             const Diag *diag, const PcgA *sa, const PcgB *sb) {
+                             const Diag *diag, const PcgA *sa, const PcgB *sb,
+                             const unsigned *bar) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= count || !active[p]) return;
     StepCtl &c = ctl[p];
-    const int st = sb[p].status != ST_OK ? sb[p].status : sa[p].status;
+    const int st = bar[64] != 0 ? ST_BARRIER : sb[p].status != ST_OK ? sb[p].status : sa[p].status;
     if (st != ST_OK) {
         c.error = RE_PCG;
         c.which = st;
@@ -869,7 +1008,9 @@ struct Batch2 {
     // host-side solver state per grid
     std::vector<int64_t> step;
     std::vector<double> time, current_dt, max_u, max_v;
-    uint32_t pcg_blocks = 1, resident = 1;
+    uint32_t pcg_blocks = 1, pcg_threads = 1024;
+    int32_t pcg_partition = FL_PARTITION_TILES;
+    unsigned *bar = nullptr;  // the persistent PCG's grid barrier
     uint32_t *bt0 = nullptr;  // persistent PCG: first tile of each block
     std::vector<uint32_t> h_bt0;
     long long kmax = 1;
@@ -908,6 +1049,7 @@ void free_batch(Batch2 *B) {
         if (p) (void)hipHostFree(p);
     if (B->ctl) (void)hipFree(B->ctl);
     if (B->bt0) (void)hipFree(B->bt0);
+    if (B->bar) (void)hipFree(B->bar);
     if (B->gx_pre) (void)hipGraphExecDestroy(B->gx_pre);
     if (B->gx_post) (void)hipGraphExecDestroy(B->gx_post);
     if (B->gr_pre) (void)hipGraphDestroy(B->gr_pre);
@@ -990,20 +1132,60 @@ bool reached_end(const Batch2 *B, int p) {
     return B->time[p] >= t_end - tol;
 }
 
-// The persistent PCG's grid size and its tile partition: contiguous tile
-// ranges of equal expected work.  A tile weighs its grid's size per axis
-// (Jacobi-PCG iteration counts grow linearly with it) plus a fixed per-tile
-// cost, so the blocks holding the grids that iterate longest hold fewer
-// tiles and the last iterations, when only those grids are left, still run
-// on many blocks.  A grid's owner block is the one holding its first tile.
-fl_status set_pcg_blocks(Batch2 *B, uint32_t G) {
-    G = std::max<uint32_t>(1, std::min(G, B->resident));
-    B->pcg_blocks = G;
+// Grid of tile t (tiles are grid-major).
+uint32_t tile_grid(const Batch2 *B, uint32_t t) {
+    uint32_t lo = 0, hi = (uint32_t)B->hp.size();
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (B->hp[mid].tile0 <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The persistent PCG's kernel for a block size.
+const void *pcg_kernel(uint32_t threads) {
+    switch (threads) {
+        case 256: return (const void *)k_pcg<256>;
+        case 512: return (const void *)k_pcg<512>;
+        default: return (const void *)k_pcg<1024>;
+    }
+}
+constexpr uint32_t kPcgThreads[3] = {256, 512, 1024};
+
+// Blocks of `threads` threads that fit on the device at once.
+fl_status pcg_resident(Batch2 *B, uint32_t threads, uint32_t *out) {
+    int per_cu = 0;
+    FL_HIP_TRY(B->ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pcg_kernel(threads),
+                                                                     (int)threads, 0));
+    *out = (uint32_t)std::max(per_cu, 0) * (uint32_t)B->ctx->num_cu;
+    if (*out == 0)
+        return set_error(B->ctx, FL_ERR_HIP, "the persistent PCG does not fit on the device");
+    return FL_OK;
+}
+
+// The persistent PCG's launch shape and its tile partition: G blocks of
+// `threads` threads, each holding a contiguous tile range.  A tile weighs 1
+// under FL_PARTITION_TILES (balanced while every grid iterates); under
+// FL_PARTITION_WORK it weighs its grid's size per axis (Jacobi-PCG
+// iteration counts grow linearly with it) plus a fixed cost, so the blocks
+// holding the grids that iterate longest hold fewer tiles and the last
+// iterations, when only those grids are left, run on more blocks.  A grid's
+// owner block is the one holding its first tile.
+fl_status set_pcg_blocks(Batch2 *B, uint32_t G, uint32_t threads, int32_t partition) {
+    if (threads != 256 && threads != 512 && threads != 1024)
+        return set_error(B->ctx, FL_ERR_INVALID_ARGUMENT, "PCG blocks hold 256, 512 or 1024 threads");
+    if (partition != FL_PARTITION_TILES && partition != FL_PARTITION_WORK)
+        return set_error(B->ctx, FL_ERR_INVALID_ARGUMENT, "unknown tile partition %d", partition);
+    uint32_t resident = 0;
+    fl_status st = pcg_resident(B, threads, &resident);
+    if (st != FL_OK) return st;
+    G = std::max<uint32_t>(1, std::min(G, std::min(resident, B->ntiles)));
     std::vector<double> w(B->ntiles, 0.0);
     double W = 0.0;
     for (int p = 0; p < B->count; ++p) {
         const Prob &P = B->hp[p];
-        const double wt = (double)std::max(P.nx, P.ny) + 16.0;
+        const double wt = partition == FL_PARTITION_WORK ? (double)std::max(P.nx, P.ny) + 16.0 : 1.0;
         for (uint32_t t = 0; t < P.ntiles; ++t) w[P.tile0 + t] = wt;
         W += wt * P.ntiles;
     }
@@ -1016,11 +1198,23 @@ fl_status set_pcg_blocks(Batch2 *B, uint32_t G) {
         while (t < B->ntiles && acc + 0.5 * w[t] < target) acc += w[t++];
         B->h_bt0[b] = t;
     }
+    for (uint32_t b = 0; b < G; ++b)
+        if (B->h_bt0[b] < B->h_bt0[b + 1]) {
+            const uint32_t g0 = B->hp.size() > 1 ? tile_grid(B, B->h_bt0[b]) : 0;
+            const uint32_t g1 = B->hp.size() > 1 ? tile_grid(B, B->h_bt0[b + 1] - 1) : 0;
+            if (g1 - g0 + 1 > (uint32_t)kMaxBlockGrids)
+                return set_error(B->ctx, FL_ERR_CONFIG,
+                                 "a persistent PCG block would hold %u grids (at most %d); use more blocks",
+                                 g1 - g0 + 1, kMaxBlockGrids);
+        }
     for (int p = 0; p < B->count; ++p) {
         Prob &P = B->hp[p];
         for (uint32_t b = 0; b < G; ++b)
             if (B->h_bt0[b] <= P.tile0 && P.tile0 < B->h_bt0[b + 1]) P.owner = b;
     }
+    B->pcg_blocks = G;
+    B->pcg_threads = threads;
+    B->pcg_partition = partition;
     hipStream_t s = B->ctx->stream;
     FL_HIP_TRY(B->ctx, hipMemcpyAsync(B->probs, B->hp.data(), B->count * sizeof(Prob),
                                       hipMemcpyHostToDevice, s));
@@ -1105,11 +1299,11 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
     B->nuf = uo;
     B->nvf = vo;
     // persistent-kernel grid: all blocks co-resident (cooperative launch)
-    int per_cu = 0;
-    FL_HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg, kT, 0));
-    B->resident = (uint32_t)std::max(per_cu, 1) * (uint32_t)ctx->num_cu;
-    uint64_t G = std::min<uint64_t>(B->resident, (uint64_t)ctx->num_cu * 2);
-    G = std::min<uint64_t>(G, B->ntiles);
+    uint32_t resident = 0, max_resident = 0;
+    for (uint32_t th : kPcgThreads) {
+        if ((st = pcg_resident(B, th, &resident)) != FL_OK) return st;
+        max_resident = std::max(max_resident, resident);
+    }
     if ((st = dalloc(B, &B->probs, count)) || (st = dalloc(B, &B->tile_prob, B->ntiles)) ||
         (st = dalloc(B, &B->flags, B->ncell)) || (st = dalloc(B, &B->x, B->ncell)) ||
         (st = dalloc(B, &B->rhs, B->ncell)) || (st = dalloc(B, &B->r, B->ncell)) ||
@@ -1119,14 +1313,14 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
         (st = dalloc(B, &B->vort, B->ncell)) || (st = dalloc(B, &B->u, B->nuf)) ||
         (st = dalloc(B, &B->v, B->nvf)) || (st = dalloc(B, &B->us, B->nuf)) ||
         (st = dalloc(B, &B->vs, B->nvf)) || (st = dalloc(B, &B->uo, B->nuf)) ||
-        (st = dalloc(B, &B->vo, B->nvf)) || (st = dalloc(B, &B->pa, (size_t)B->ntiles * 2)) ||
-        (st = dalloc(B, &B->pb, (size_t)B->ntiles * 5)) ||
-        (st = dalloc(B, &B->p0, (size_t)B->ntiles * 3)) ||
+        (st = dalloc(B, &B->vo, B->nvf)) || (st = dalloc(B, &B->pa, (size_t)B->ntiles * kSub)) ||
+        (st = dalloc(B, &B->pb, (size_t)B->ntiles * kSub * 5)) ||
+        (st = dalloc(B, &B->p0, (size_t)B->ntiles * kSub * 2)) ||
         (st = dalloc(B, &B->dpart, (size_t)B->ntiles * 6)) || (st = dalloc(B, &B->sa, count)) ||
         (st = dalloc(B, &B->sb, count)) || (st = dalloc(B, &B->fin, 2 * count)) ||
         (st = dalloc(B, &B->active, count)) || (st = dalloc(B, &B->dts, count)) ||
         (st = dalloc(B, &B->diag, count)) || (st = dalloc(B, &B->ctl, count)) ||
-        (st = dalloc(B, &B->bt0, (size_t)B->resident + 1)))
+        (st = dalloc(B, &B->bt0, (size_t)max_resident + 1)) || (st = dalloc(B, &B->bar, 128)))
         return st;
     if ((st = halloc(B, &B->h_active, count)) || (st = halloc(B, &B->h_fin, 2 * count)) ||
         (st = halloc(B, &B->h_dts, count)) || (st = halloc(B, &B->h_sa, count)) ||
@@ -1139,7 +1333,7 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->probs, B->hp.data(), count * sizeof(Prob), hipMemcpyHostToDevice, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->tile_prob, htp.data(), htp.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->flags, hflags.data(), hflags.size(), hipMemcpyHostToDevice, s));
-    if ((st = set_pcg_blocks(B, (uint32_t)std::max<uint64_t>(G, 1))) != FL_OK) return st;
+    if ((st = set_pcg_blocks(B, ctx->num_cu, 1024, FL_PARTITION_TILES)) != FL_OK) return st;
     B->step.assign(count, 0);
     B->time.assign(count, 0.0);
     B->current_dt.resize(count);
@@ -1194,10 +1388,10 @@ fl_status run_pcg(Batch2 *B) {
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->sb, B->h_sb, B->count * sizeof(PcgB), hipMemcpyHostToDevice, s));
     PcgArgs a{B->probs, B->tile_prob, B->ntiles, (uint32_t)B->count, kmax, B->flags, B->rhs,
               B->x, B->r, B->d0, B->d1, B->q, B->pa, B->pb, B->p0, B->sa, B->sb,
-              B->active, B->fin, B->bt0};
+              B->active, B->fin, B->bt0, B->bar};
     void *args[] = {&a};
-    FL_HIP_TRY(ctx, hipLaunchCooperativeKernel((const void *)k_pcg, dim3(B->pcg_blocks), dim3(kT),
-                                               args, 0, s));
+    FL_HIP_TRY(ctx, hipLaunchCooperativeKernel(pcg_kernel(B->pcg_threads), dim3(B->pcg_blocks),
+                                               dim3(B->pcg_threads), args, 0, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->h_sa, B->sa, B->count * sizeof(PcgA), hipMemcpyDeviceToHost, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->h_sb, B->sb, B->count * sizeof(PcgB), hipMemcpyDeviceToHost, s));
     FL_HIP_TRY(ctx, hipStreamSynchronize(s));
@@ -1209,8 +1403,19 @@ const char *pcg_text(int status) {
         case ST_SINGULAR: return "PCG pressure solver encountered a singular search direction";
         case ST_ALPHA: return "PCG pressure solver produced a non-finite step length";
         case ST_RESIDUAL: return "PCG pressure solver produced a non-finite residual";
+        case ST_BARRIER: return "the persistent PCG's grid barrier timed out";
         default: return "";
     }
+}
+
+// After a synchronise: a persistent PCG whose barrier timed out leaves the
+// abort word set; clear the barrier state and report it.
+fl_status barrier_check(Batch2 *B) {
+    unsigned word = 0;
+    FL_HIP_TRY(B->ctx, hipMemcpy(&word, B->bar + 64, sizeof(unsigned), hipMemcpyDeviceToHost));
+    if (word == 0) return FL_OK;
+    FL_HIP_TRY(B->ctx, hipMemset(B->bar, 0, 128 * sizeof(unsigned)));
+    return set_error(B->ctx, FL_ERR_SOLVER, "%s", pcg_text(ST_BARRIER));
 }
 
 fl_status pcg_status(Batch2 *B, int p) {
@@ -1262,7 +1467,7 @@ fl_status step_batch(Batch2 *B, fl_grid2d_step_report *reps, bool strict) {
                            B->vs, B->rhs);
     FL_HIP_TRY(ctx, hipGetLastError());
     fl_status st = run_pcg(B);
-    if (st != FL_OK) return st;
+    if (st != FL_OK || (st = barrier_check(B)) != FL_OK) return st;
     for (int p = 0; p < B->count; ++p)
         if (B->h_active[p] && (st = pcg_status(B, p)) != FL_OK) return st;
     k_correct<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->dts, B->flags, B->x,
@@ -1337,17 +1542,17 @@ fl_status launch_post(Batch2 *B, hipStream_t s) {
                              B->v, B->uo, B->vo, B->uc, B->vc, B->sp, B->vort, B->dpart, B->diag);
     k_fold_diag<<<B->count, kT, 0, s>>>(B->probs, B->active, B->dpart, B->diag);
     k_run_finish<<<(B->count + 63) / 64, 64, 0, s>>>(B->ctl, B->count, B->active, B->dts, B->diag,
-                                                     B->sa, B->sb);
+                                                     B->sa, B->sb, B->bar);
     FL_HIP_TRY(ctx, hipGetLastError());
     return FL_OK;
 }
 fl_status launch_pcg_coop(Batch2 *B, hipStream_t s, const double *rhs, double *x, long long kmax) {
     PcgArgs a{B->probs, B->tile_prob, B->ntiles, (uint32_t)B->count, kmax, B->flags, rhs,
               x, B->r, B->d0, B->d1, B->q, B->pa, B->pb, B->p0, B->sa, B->sb,
-              B->active, B->fin, B->bt0};
+              B->active, B->fin, B->bt0, B->bar};
     void *args[] = {&a};
-    FL_HIP_TRY(B->ctx, hipLaunchCooperativeKernel((const void *)k_pcg, dim3(B->pcg_blocks), dim3(kT),
-                                                  args, 0, s));
+    FL_HIP_TRY(B->ctx, hipLaunchCooperativeKernel(pcg_kernel(B->pcg_threads), dim3(B->pcg_blocks),
+                                                  dim3(B->pcg_threads), args, 0, s));
     return FL_OK;
 }
 
@@ -1445,6 +1650,7 @@ fl_status run_device(Batch2 *B, int64_t steps, bool graph, fl_batch2d_run_report
         }
     }
     (void)nsteps;
+    if ((st = barrier_check(B)) != FL_OK) return st;
     if (first_bad < 0) return FL_OK;
     const StepCtl &k = B->h_ctl[first_bad];
     if (k.error == RE_MIN_DT)
@@ -1463,25 +1669,39 @@ fl_status run_device(Batch2 *B, int64_t steps, bool graph, fl_batch2d_run_report
                      names[k.which], k.index, val, (long long)k.step);
 }
 
-// Chooses the persistent PCG's grid size: every co-resident candidate runs
-// `iters` iterations on a fixed pseudo-random right-hand side over all grids
-// (scratch x and rhs; the solver state is untouched), the fastest of three
-// timed runs wins.
-fl_status autotune(Batch2 *B, int iters, int32_t *chosen, double *best_ms, int32_t cap,
-                   int32_t *tb, double *tms, int32_t *tlen) {
+// Chooses the persistent PCG's launch shape and tile partition: every
+// candidate (threads per block, co-resident block count, partition) solves
+// the same pseudo-random right-hand sides from zero with each grid's own
+// tolerance, capped at `iters` iterations, into scratch x and rhs (the
+// solver's fields are untouched), and the fastest is kept.  Solver results
+// do not depend on the choice.
+fl_status autotune(Batch2 *B, int iters, fl_batch2d_tuning *chosen, int32_t cap,
+                   fl_batch2d_tuning *table, int32_t *tlen) {
     fl_ctx *ctx = B->ctx;
     hipStream_t s = ctx->stream;
     FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
-    int per_cu = 0;
-    FL_HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg, kT, 0));
-    const uint32_t resident = (uint32_t)std::max(per_cu, 1) * (uint32_t)ctx->num_cu;
-    std::vector<uint32_t> cand;
-    for (uint32_t m : {1u, 2u, 4u, 8u})
-        for (uint32_t G : {(uint32_t)ctx->num_cu * m / 2, (uint32_t)ctx->num_cu * m})
-            if (G >= 1 && G <= resident && G <= B->ntiles &&
-                std::find(cand.begin(), cand.end(), G) == cand.end())
-                cand.push_back(G);
-    if (cand.empty()) cand.push_back(std::min<uint32_t>(B->ntiles, resident));
+    std::vector<fl_batch2d_tuning> cand;
+    fl_status st;
+    for (uint32_t th : kPcgThreads) {
+        uint32_t resident = 0;
+        if ((st = pcg_resident(B, th, &resident)) != FL_OK) return st;
+        std::vector<uint32_t> gs;
+        for (uint32_t m : {1u, 2u, 4u, 8u})
+            for (uint32_t G : {(uint32_t)ctx->num_cu * m / 2, (uint32_t)ctx->num_cu * m})
+                if (G >= 1 && G <= resident && G <= B->ntiles &&
+                    std::find(gs.begin(), gs.end(), G) == gs.end())
+                    gs.push_back(G);
+        if (gs.empty()) gs.push_back(std::min<uint32_t>(B->ntiles, resident));
+        for (uint32_t G : gs)
+            for (int32_t part : {(int32_t)FL_PARTITION_TILES, (int32_t)FL_PARTITION_WORK}) {
+                fl_batch2d_tuning c{};
+                c.threads = (int32_t)th;
+                c.blocks = (int32_t)G;
+                c.partition = part;
+                cand.push_back(c);
+            }
+    }
+    const fl_batch2d_tuning keep{(int32_t)B->pcg_threads, (int32_t)B->pcg_blocks, B->pcg_partition, 0, 0.0};
     double *rhs = nullptr, *x = nullptr;
     FL_HIP_TRY(ctx, hipMalloc(&rhs, B->ncell * sizeof(double)));
     if (hipMalloc(&x, B->ncell * sizeof(double)) != hipSuccess) {
@@ -1489,10 +1709,10 @@ fl_status autotune(Batch2 *B, int iters, int32_t *chosen, double *best_ms, int32
         return set_error(ctx, FL_ERR_OUT_OF_MEMORY, "autotune scratch allocation failed");
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    fl_status st = FL_OK;
+    st = FL_OK;
     if (tlen) *tlen = 0;
-    uint32_t best = B->pcg_blocks;
-    double best_t = 1e300;
+    fl_batch2d_tuning best = keep;
+    best.ms = 1e300;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         st = set_error(ctx, FL_ERR_HIP, "event creation failed");
     if (st == FL_OK) {
@@ -1504,49 +1724,48 @@ fl_status autotune(Batch2 *B, int iters, int32_t *chosen, double *best_ms, int32
             B->h_sb[p] = PcgB{};
             B->h_sb[p].first = 1;
         }
-        (void)hipMemcpyAsync(B->active, B->h_active, B->count * sizeof(int), hipMemcpyHostToDevice, s);
+        if (hipMemcpyAsync(B->active, B->h_active, B->count * sizeof(int), hipMemcpyHostToDevice, s) !=
+            hipSuccess)
+            st = set_error(ctx, FL_ERR_HIP, "autotune setup failed");
     }
-    for (uint32_t G : cand) {
-        if (st != FL_OK) break;
-        if ((st = set_pcg_blocks(B, G)) != FL_OK) break;
-        double tmin = 1e300;
-        for (int rep = 0; rep < 4 && st == FL_OK; ++rep) {
-            if (hipMemsetAsync(x, 0, B->ncell * sizeof(double), s) != hipSuccess ||
-                hipMemcpyAsync(B->fin, B->h_fin, 2 * B->count * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
-                hipMemcpyAsync(B->sb, B->h_sb, B->count * sizeof(PcgB), hipMemcpyHostToDevice, s) != hipSuccess ||
-                hipEventRecord(e0, s) != hipSuccess) {
-                st = set_error(ctx, FL_ERR_HIP, "autotune setup failed");
-                break;
-            }
-            if ((st = launch_pcg_coop(B, s, rhs, x, iters)) != FL_OK) break;
-            float ms = 0.0f;
-            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-                st = set_error(ctx, FL_ERR_HIP, "autotune timing failed");
-                break;
-            }
-            if (rep > 0) tmin = std::min(tmin, (double)ms);  // the first run warms up
+    // an untimed solve with the first candidate, then one timed solve each
+    for (size_t ci = 0; ci <= cand.size() && st == FL_OK; ++ci) {
+        fl_batch2d_tuning &c = cand[ci == 0 ? 0 : ci - 1];
+        if ((st = set_pcg_blocks(B, (uint32_t)c.blocks, (uint32_t)c.threads, c.partition)) != FL_OK) break;
+        c.blocks = (int32_t)B->pcg_blocks;
+        if (hipMemsetAsync(x, 0, B->ncell * sizeof(double), s) != hipSuccess ||
+            hipMemcpyAsync(B->fin, B->h_fin, 2 * B->count * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(B->sb, B->h_sb, B->count * sizeof(PcgB), hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipEventRecord(e0, s) != hipSuccess) {
+            st = set_error(ctx, FL_ERR_HIP, "autotune setup failed");
+            break;
         }
-        if (st == FL_OK && tb && tms && tlen && *tlen < cap) {
-            tb[*tlen] = (int32_t)G;
-            tms[*tlen] = tmin;
-            *tlen += 1;
+        if ((st = launch_pcg_coop(B, s, rhs, x, iters)) != FL_OK) break;
+        float ms = 0.0f;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            st = set_error(ctx, FL_ERR_HIP, "autotune timing failed");
+            break;
         }
-        if (st == FL_OK && tmin < best_t) {
-            best_t = tmin;
-            best = G;
-        }
+        if (ci == 0) continue;
+        c.ms = ms;
+        if (table && tlen && *tlen < cap) table[(*tlen)++] = c;
+        if (c.ms < best.ms) best = c;
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipStreamSynchronize(s);
     (void)hipFree(rhs);
     (void)hipFree(x);
-    if (st != FL_OK) return st;
-    if ((st = set_pcg_blocks(B, best)) != FL_OK) return st;
+    if (st == FL_OK) st = barrier_check(B);
+    if (st != FL_OK) {
+        (void)set_pcg_blocks(B, (uint32_t)keep.blocks, (uint32_t)keep.threads, keep.partition);
+        return st;
+    }
+    if ((st = set_pcg_blocks(B, (uint32_t)best.blocks, (uint32_t)best.threads, best.partition)) != FL_OK)
+        return st;
     FL_HIP_TRY(ctx, hipStreamSynchronize(s));
-    if (chosen) *chosen = (int32_t)best;
-    if (best_ms) *best_ms = best_t;
+    if (chosen) *chosen = best;
     return FL_OK;
 }
 
@@ -1647,7 +1866,7 @@ fl_status fl_grid2d_pressure_solve(fl_grid2d *g, const double *rhs, double *p_in
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->rhs, rhs, (size_t)P.n * 8, hipMemcpyHostToDevice, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->x, p_inout, (size_t)P.n * 8, hipMemcpyHostToDevice, s));
     fl_status st = run_pcg(B);
-    if (st != FL_OK) return st;
+    if (st != FL_OK || (st = barrier_check(B)) != FL_OK) return st;
     FL_HIP_TRY(ctx, hipMemcpyAsync(p_inout, B->x, (size_t)P.n * 8, hipMemcpyDeviceToHost, s));
     FL_HIP_TRY(ctx, hipStreamSynchronize(s));
     if (rep) {
@@ -1749,13 +1968,30 @@ fl_status fl_batch2d_run(fl_batch2d *b, int64_t steps, int32_t use_graphs,
     return run_device(b->b, steps, use_graphs != 0, reports, step_ms);
 }
 
-fl_status fl_batch2d_autotune(fl_batch2d *b, int32_t iterations, int32_t *blocks, double *ms,
-                              int32_t table_cap, int32_t *table_blocks, double *table_ms,
-                              int32_t *table_len) {
+fl_status fl_batch2d_autotune(fl_batch2d *b, int32_t iterations, fl_batch2d_tuning *chosen,
+                              int32_t table_cap, fl_batch2d_tuning *table, int32_t *table_len) {
     if (!b) return FL_ERR_INVALID_ARGUMENT;
     if (iterations < 1)
         return set_error(b->b->ctx, FL_ERR_INVALID_ARGUMENT, "autotune needs at least one iteration");
-    return autotune(b->b, iterations, blocks, ms, table_cap, table_blocks, table_ms, table_len);
+    return autotune(b->b, iterations, chosen, table_cap, table, table_len);
+}
+
+fl_status fl_batch2d_set_launch(fl_batch2d *b, const fl_batch2d_tuning *launch) {
+    if (!b || !launch) return FL_ERR_INVALID_ARGUMENT;
+    Batch2 *B = b->b;
+    FL_HIP_TRY(B->ctx, hipSetDevice(B->ctx->device));
+    fl_status st = set_pcg_blocks(B, (uint32_t)std::max(launch->blocks, 1), (uint32_t)launch->threads,
+                                  launch->partition);
+    if (st != FL_OK) return st;
+    FL_HIP_TRY(B->ctx, hipStreamSynchronize(B->ctx->stream));
+    return FL_OK;
+}
+
+fl_status fl_batch2d_get_launch(const fl_batch2d *b, fl_batch2d_tuning *launch) {
+    if (!b || !launch) return FL_ERR_INVALID_ARGUMENT;
+    const Batch2 *B = b->b;
+    *launch = fl_batch2d_tuning{(int32_t)B->pcg_threads, (int32_t)B->pcg_blocks, B->pcg_partition, 0, 0.0};
+    return FL_OK;
 }
 
 }  // extern "C"

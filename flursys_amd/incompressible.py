@@ -227,13 +227,26 @@ class CavityBatch:
         return self.last_run, (ms[:steps] if step_times else None)
 
     def autotune(self, iterations: int = 20):
-        """Chooses the persistent PCG's grid size; returns (blocks, ms,
-        {candidate blocks: ms})."""
-        g = C.c_int32()
-        ms = C.c_double()
-        tb = (C.c_int32 * 16)()
-        tm = (C.c_double * 16)()
+        """Chooses the persistent PCG's launch shape and tile partition;
+        returns (chosen, table), each entry a dict with threads, blocks,
+        partition and ms."""
+        best = _lib.Batch2DTuning()
+        tab = (_lib.Batch2DTuning * 64)()
         n = C.c_int32()
-        check(load().fl_batch2d_autotune(self.handle, iterations, C.byref(g), C.byref(ms), 16,
-                                         tb, tm, C.byref(n)), self.ctx.handle)
-        return g.value, ms.value, {tb[i]: tm[i] for i in range(n.value)}
+        check(load().fl_batch2d_autotune(self.handle, iterations, C.byref(best), 64, tab,
+                                         C.byref(n)), self.ctx.handle)
+        return _tuning(best), [_tuning(tab[i]) for i in range(n.value)]
+
+    def launch(self):
+        """The persistent PCG's current launch shape."""
+        t = _lib.Batch2DTuning()
+        check(load().fl_batch2d_get_launch(self.handle, C.byref(t)), self.ctx.handle)
+        return _tuning(t)
+
+    def set_launch(self, threads: int, blocks: int, partition: int = _lib.PARTITION_TILES):
+        t = _lib.Batch2DTuning(threads, blocks, partition, 0, 0.0)
+        check(load().fl_batch2d_set_launch(self.handle, C.byref(t)), self.ctx.handle)
+
+
+def _tuning(t) -> dict:
+    return {"threads": t.threads, "blocks": t.blocks, "partition": t.partition, "ms": t.ms}

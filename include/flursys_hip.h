@@ -280,14 +280,30 @@ typedef struct {
 } fl_batch2d_run_report;
 fl_status fl_batch2d_run(fl_batch2d *batch, int64_t steps, int32_t use_graphs,
                          fl_batch2d_run_report *reports, double *step_ms);
-/* Autotuner of the persistent PCG's launch shape (no reference counterpart):
- * times `iterations` PCG iterations over all grids for every co-resident
- * grid size, keeps the fastest and returns it and its time.  When
- * `table_blocks` / `table_ms` (capacity `table_cap`) are given they receive
- * every candidate and its time, `*table_len` their number. */
-fl_status fl_batch2d_autotune(fl_batch2d *batch, int32_t iterations, int32_t *blocks,
-                              double *ms, int32_t table_cap, int32_t *table_blocks,
-                              double *table_ms, int32_t *table_len);
+/* Launch shape of the batch's persistent PCG (no reference counterpart):
+ * `threads` per block (256, 512 or 1024), `blocks` (clamped to what is
+ * co-resident and to the tile count) and how the grids' 256-cell tiles are
+ * split into the blocks' contiguous ranges.  Results do not depend on it. */
+enum { FL_PARTITION_TILES = 0, /* equal tile counts */
+       FL_PARTITION_WORK = 1   /* tiles weighted by their grid's size per axis */ };
+typedef struct fl_batch2d_tuning {
+    int32_t threads;
+    int32_t blocks;
+    int32_t partition;
+    int32_t reserved;
+    double ms; /* autotuner: time of the candidate's solve */
+} fl_batch2d_tuning;
+fl_status fl_batch2d_set_launch(fl_batch2d *batch, const fl_batch2d_tuning *launch);
+fl_status fl_batch2d_get_launch(const fl_batch2d *batch, fl_batch2d_tuning *launch);
+/* Autotuner of the launch shape (no reference counterpart): every candidate
+ * (threads, co-resident block count, partition) solves the same
+ * pseudo-random right-hand sides from zero with each grid's tolerance,
+ * capped at `iterations` PCG iterations, in scratch memory; the fastest is
+ * kept and returned in `*chosen`.  When `table` (capacity `table_cap`) is
+ * given it receives every candidate with its time, `*table_len` their
+ * number. */
+fl_status fl_batch2d_autotune(fl_batch2d *batch, int32_t iterations, fl_batch2d_tuning *chosen,
+                              int32_t table_cap, fl_batch2d_tuning *table, int32_t *table_len);
 
 /* ---------------------------------------------------------------------- */
 /* CSR linear algebra: replaces CsrMatrix (src/linear/csr.rs:4-125) and the

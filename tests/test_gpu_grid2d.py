@@ -278,8 +278,37 @@ def test_autotune_keeps_results(gpu_ctx):
     ref = I.CavityBatch(uneven_cfgs(), ctx=gpu_ctx)
     ref.run(4)
     b = I.CavityBatch(uneven_cfgs(), ctx=gpu_ctx)
-    blocks, ms, table = b.autotune(10)
-    assert blocks >= 1 and ms > 0 and table[blocks] == ms and min(table.values()) == ms
+    chosen, table = b.autotune(10)
+    assert chosen["blocks"] >= 1 and chosen["ms"] > 0
+    assert chosen in table and min(t["ms"] for t in table) == chosen["ms"]
+    assert {t["threads"] for t in table} == {256, 512, 1024}
+    assert b.launch() == dict(chosen, ms=0.0)
     b.run(4)
     for fr, fb in zip(all_fields(ref), all_fields(b)):
         assert np.array_equal(fr["p"], fb["p"])
+
+
+def test_launch_shapes_give_identical_results(gpu_ctx):
+    """Partials are per 64-cell sub-tile and folded in a fixed order, so the
+    persistent PCG's block size, block count and partition change nothing."""
+    print("[gpu] batch launch shapes", flush=True)
+    cfgs = uneven_cfgs() + [I.SimulationConfig.cavity(96, reynolds=700.0)]
+    ref = I.CavityBatch(cfgs, ctx=gpu_ctx)
+    ref.run(3)
+    for threads, blocks, part in ((256, 1, 0), (512, 7, 1), (1024, 3, 0), (256, 40, 1)):
+        b = I.CavityBatch(cfgs, ctx=gpu_ctx)
+        b.set_launch(threads, blocks, part)
+        assert b.launch()["threads"] == threads
+        b.run(3)
+        for fr, fb in zip(all_fields(ref), all_fields(b)):
+            for name in ("u", "v", "p"):
+                assert np.array_equal(fr[name], fb[name]), (threads, blocks, part, name)
+        b.close()
+
+
+def test_launch_shape_validation(gpu_ctx):
+    b = I.CavityBatch(uneven_cfgs()[:1], ctx=gpu_ctx)
+    with pytest.raises(fa.FlursysError, match="256, 512 or 1024 threads"):
+        b.set_launch(128, 4)
+    with pytest.raises(fa.FlursysError, match="unknown tile partition"):
+        b.set_launch(256, 4, 7)
