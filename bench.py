@@ -183,12 +183,12 @@ def run_batch2d(fa, ctx, args):
         b = I.CavityBatch(cfgs, ctx=ctx)
         if chosen is None:
             chosen, table = b.autotune(args.tune_iterations)
-            log(f"[bench] batch2d: autotune over (threads, blocks, partition), solves capped at "
+            log(f"[bench] batch2d: autotune over (threads, blocks), solves capped at "
                 f"{args.tune_iterations} iterations: chosen {chosen}")
             for t in table:
                 log(f"[bench]   {t}")
         else:
-            b.set_launch(chosen["threads"], chosen["blocks"], chosen["partition"])
+            b.set_launch(chosen["threads"], chosen["blocks"])
         b.run(5, use_graphs=graphs)  # warmup (captures the graphs)
         chunk, left, ms, iters = 25, total, [], 0
         t0 = time.perf_counter()
@@ -231,8 +231,8 @@ def run_batch2d(fa, ctx, args):
                                 "max": float(ms.max())},
             "pcg_iterations_per_cavity_step": g["pressure_iterations"] / (len(cfgs) * steps),
             "autotuned_pcg_launch": chosen,
-            "autotune_solve_ms": {f"{t['threads']}x{t['blocks']}/{'work' if t['partition'] else 'tiles'}":
-                                  round(t["ms"], 3) for t in table},
+            "autotune_solve_ms": {f"{t['threads']}x{t['blocks']}": round(t["ms"], 3)
+                                  for t in table},
             "eager_cavity_steps_per_s": len(cfgs) * out["eager"]["steps"] / out["eager"]["elapsed_s"],
             "eager_steps": out["eager"]["steps"],
             "eager_step_latency_ms_p50": float(np.percentile(out["eager"]["step_ms"], 50)),

@@ -78,12 +78,9 @@ class Batch2DRunReport(C.Structure):
                 ("status", C.c_int32)]
 
 
-PARTITION_TILES, PARTITION_WORK = 0, 1
-
-
 class Batch2DTuning(C.Structure):
-    _fields_ = [("threads", C.c_int32), ("blocks", C.c_int32), ("partition", C.c_int32),
-                ("reserved", C.c_int32), ("ms", C.c_double)]
+    _fields_ = [("threads", C.c_int32), ("blocks", C.c_int32), ("reserved", C.c_int32 * 2),
+                ("ms", C.c_double)]
 
 
 class UNetConfig(C.Structure):

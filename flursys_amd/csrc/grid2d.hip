@@ -55,7 +55,6 @@ struct Prob {
     uint32_t nx, ny, n;
     uint32_t tile0, ntiles;
     uint32_t uoff, voff;
-    uint32_t owner;  // block of the PCG kernel that records this grid's state
     double dx, dy, cx, cy, nu, rho;
     int32_t scheme;
     int32_t kind[4];
@@ -69,11 +68,11 @@ struct Prob {
     FastDiv fx;
 };
 
-struct PcgA {  // written by the owner block in phase A
+struct PcgA {  // a grid's PCG state after phase A
     double rz, beta, mz, rr0, rr, rmax;
     int converged, status;
 };
-struct PcgB {  // written by the owner block in phase B (and the host)
+struct PcgB {  // a grid's PCG state after phase B (set up by the host)
     double rz_prev, alpha;
     long long iters;
     int first, status;
@@ -383,20 +382,23 @@ __global__ void __launch_bounds__(kT) k_fold_diag(const Prob *probs, const int *
 }
 
 // ------------------------------------------------------ persistent PCG
-// Work layout: a block of NT threads owns the contiguous tile range
-// [bt0[b], bt0[b+1]); its waves stream through the 64-cell sub-tiles of that
-// range independently (sub-tile s goes to wave s mod NT/64) and write one
-// partial row per sub-tile.  Every block folds the partial rows of the grids
-// its range touches with groups of 256 threads: thread u sums rows u,
-// u + 256, ... in order, each wave combines its lanes by a fixed butterfly
-// and the group adds its four wave totals in order.  So every block gets
-// the same scalars, and results depend neither on the number of blocks nor
-// on their size.  Each block keeps the per-grid solver state (rz, the
-// previous rz, rr0, iteration count) in LDS, computing it exactly as every
-// other block does; a grid's owner block also writes it out for the host.
-constexpr int kSub = kT / 64;        // sub-tiles per tile
-constexpr int kMaxBlockGrids = 256;  // grids one PCG block may touch
-constexpr uint32_t kFoldT = 256;     // threads folding one grid's rows
+// Work layout.  The 256-cell tiles of every grid are cut into 64-cell
+// sub-tiles (one per wave).  At the start of each iteration every block
+// lists the sub-tiles of the grids still iterating, in grid order, and takes
+// an equal contiguous share of that list, so the work stays balanced over
+// all blocks as grids finish.  A wave streams through its block's share and
+// writes one partial row per sub-tile (rows are stored by the sub-tile's
+// place in its grid, not by who computed them).  Every block then folds the
+// rows of the grids its share touches, with groups of 256 threads: thread u
+// sums rows u, u + 256, ... in order, each wave combines its lanes by a
+// fixed butterfly and the group adds its four wave totals in order.  So
+// every block gets the same scalars, and results depend neither on the
+// number of blocks, nor on their size, nor on the share a block gets.  The
+// block whose share holds a grid's first sub-tile writes the grid's state
+// out (for the host and for the next iteration's blocks).
+constexpr int kSub = kT / 64;     // sub-tiles per tile
+constexpr int kMaxGrids = 256;    // grids per batch
+constexpr uint32_t kFoldT = 256;  // threads folding one grid's rows
 
 struct PcgArgs {
     const Prob *probs;
@@ -412,8 +414,7 @@ struct PcgArgs {
     const int *active;  // grids taking part in this solve
     int *fin;  // fin[parity * nprob + grid]: grid finished in that loop (the
                // host sets fin[1][g] = !active[g], the "loop -1" verdicts)
-    const uint32_t *bt0;  // first tile of each block, gridDim.x + 1 entries
-    unsigned *bar;        // grid barrier: arrival count [0], generation [32], abort [64]
+    unsigned *bar;  // grid barrier: arrival count [0], generation [32], abort [64]
 };
 
 // Grid-wide barrier for a co-resident (cooperative) grid.  The last block to
@@ -466,12 +467,13 @@ __device__ inline double inv_diag(double cx, double cy, uint8_t f) {
     return dg > 0.0 ? 1.0 / dg : 1.0;
 }
 
-// A grid as one PCG block sees it: geometry, the scalars of the current
-// phase and the solver state.
+// A grid as every PCG block sees it: geometry, its place in the iteration's
+// sub-tile list, the scalars of the current phase and the solver state.
 struct GridL {
-    uint32_t tile0, tend, n, nx;
-    uint32_t fxm, fxs, owner;  // x = c mod nx via FastDiv's multiplier and shift
-    int32_t active;
+    uint32_t tile0, nsub, n, nx;
+    uint32_t fxm, fxs;  // x = c mod nx via FastDiv's multiplier and shift
+    uint32_t off;       // first list index (sub-tiles of earlier iterating grids)
+    int32_t act;        // iterating in this loop
     double cx, cy;
     double a, b;  // phase A: beta, mean of z; phase B: alpha; setup: mean of b / x
     double rz, rz_prev, rr0;
@@ -479,44 +481,119 @@ struct GridL {
     int32_t done, first;
 };
 
-__device__ inline uint32_t grid_col(const GridL &G, uint32_t c) {
+// Wave-uniform copies (scalar registers) of values read from LDS.
+__device__ inline uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline double uni(double v) {
+    const uint64_t u = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// A grid's fields the cell loops use, in scalar registers.
+struct GridU {
+    uint32_t tile0, n, nx, fxm, fxs;
+    double cx, cy, a, b;
+    bool first, done;
+};
+__device__ inline GridU grid_uni(const GridL &G) {
+    return {uni(G.tile0), uni(G.n), uni(G.nx), uni(G.fxm), uni(G.fxs), uni(G.cx), uni(G.cy),
+            uni(G.a), uni(G.b), uni((uint32_t)G.first) != 0, uni((uint32_t)G.done) != 0};
+}
+__device__ inline uint32_t grid_col(const GridU &G, uint32_t c) {
     const uint32_t t = __umulhi(c, G.fxm);
     return c - ((t + ((c - t) >> 1)) >> G.fxs) * G.nx;
 }
 
-// Calls f(s, j, c) for each of this wave's sub-tiles s of [s_lo, s_hi): j is
-// the block-local grid, c this lane's cell in that grid.
+// Element i of a grid's array at a 32-bit byte offset from a scalar base
+// (grids hold fewer than 2^28 cells).
+__device__ inline double ld(const double *base, uint32_t i) {
+    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + (i << 3));
+}
+__device__ inline void st(double *base, uint32_t i, double v) {
+    *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + (i << 3)) = v;
+}
+
+// The iteration's sub-tile list: offsets of the iterating grids (one wave
+// scans gl[].act in grid order), this block's share [lo, hi) and the grids
+// [g0, g1] that share touches.  Ends with a block barrier.
+struct Share {
+    uint32_t lo, hi, g0, g1, ng;
+};
+__device__ inline Share make_share(GridL *gl, uint32_t nprob) {
+    __shared__ uint32_t s_total, s_g0, s_g1;
+    if (threadIdx.x < 64) {
+        uint32_t carry = 0;
+        for (uint32_t c0 = 0; c0 < nprob; c0 += 64) {
+            const uint32_t g = c0 + threadIdx.x;
+            const uint32_t v = g < nprob && gl[g].act ? gl[g].nsub : 0u;
+            uint32_t incl = v;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(incl, o, 64);
+                if ((int)threadIdx.x >= o) incl += t;
+            }
+            if (g < nprob) gl[g].off = carry + incl - v;
+            carry += __shfl(incl, 63, 64);
+        }
+        if (threadIdx.x == 0) {
+            s_total = carry;
+            s_g0 = 1u;
+            s_g1 = 0u;
+        }
+    }
+    __syncthreads();
+    const uint64_t S = s_total;
+    Share sh;
+    sh.lo = (uint32_t)(S * blockIdx.x / gridDim.x);
+    sh.hi = (uint32_t)(S * (blockIdx.x + 1) / gridDim.x);
+    for (uint32_t g = threadIdx.x; g < nprob; g += blockDim.x) {
+        const GridL &G = gl[g];
+        if (!G.act || sh.lo >= sh.hi) continue;
+        if (G.off <= sh.lo && sh.lo < G.off + G.nsub) s_g0 = g;
+        if (G.off < sh.hi && sh.hi <= G.off + G.nsub) s_g1 = g;
+    }
+    __syncthreads();
+    sh.g0 = s_g0;
+    sh.g1 = s_g1;
+    sh.ng = nprob;
+    __syncthreads();
+    return sh;
+}
+
+// Calls f(s, g, c) for each of this wave's sub-tiles in the block's share:
+// s is the sub-tile's partial row, g its grid, c this lane's cell.
 template <uint32_t kW, typename F>
-__device__ inline void for_subtiles(const GridL *gl, uint32_t s_lo, uint32_t s_hi, F &&f) {
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint32_t j = 0;
-    for (uint32_t s = s_lo + wave; s < s_hi; s += kW) {
-        const uint32_t t = s / kSub;
-        while (t >= gl[j].tend) ++j;
-        f(s, j, (t - gl[j].tile0) * kT + (s % kSub) * 64 + lane);
+__device__ inline void for_share(const GridL *gl, const Share &sh, F &&f) {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    uint32_t g = sh.g0;
+    for (uint32_t i = sh.lo + wave; i < sh.hi; i += kW) {
+        while (g + 1 < uni(sh.ng) && (!uni((uint32_t)gl[g].act) || i >= uni(gl[g].off) + uni(gl[g].nsub))) ++g;
+        const uint32_t ls = i - uni(gl[g].off);
+        f(uni(gl[g].tile0) * kSub + ls, g, ls * 64 + lane);
     }
 }
 
-// Folds the partial rows (NS sums, then NM maxima per row) of every grid j
-// of the block with want(j), then calls use(j, sums, maxima) in one thread.
+// Folds the partial rows (NS sums, then NM maxima per row) of every grid g
+// in [g0, g1] with want(g), then calls use(g, sums, maxima) in one thread.
 // Ends with a block barrier, so what use() wrote to LDS is visible.
 template <int NS, int NM, int NT, typename W, typename U>
-__device__ inline void fold_grids(const double *__restrict__ part, const GridL *gl, uint32_t ng,
-                                  double (*red)[8], W &&want, U &&use) {
+__device__ inline void fold_grids(const double *__restrict__ part, const GridL *gl, uint32_t g0,
+                                  uint32_t g1, double (*red)[8], W &&want, U &&use) {
     static_assert(NS + NM <= 8, "at most 8 columns");
     constexpr int K = NS + NM;
     constexpr uint32_t kG = NT / kFoldT;
     const uint32_t q = threadIdx.x / kFoldT, u = threadIdx.x % kFoldT;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (uint32_t j0 = 0; j0 < ng; j0 += kG) {
-        const uint32_t j = j0 + q;
-        const bool on = j < ng && want(j);
+    for (uint32_t j0 = g0; j0 <= g1; j0 += kG) {
+        const uint32_t g = j0 + q;
+        const bool on = g <= g1 && want(g);
         double v[K];
 #pragma unroll
         for (int a = 0; a < K; ++a) v[a] = 0.0;
         if (on) {
-            const uint32_t row0 = gl[j].tile0 * kSub, rows = (gl[j].tend - gl[j].tile0) * kSub;
-#pragma unroll 4
+            const uint32_t row0 = gl[g].tile0 * kSub, rows = gl[g].nsub;
+#pragma unroll 2
             for (uint32_t i = u; i < rows; i += kFoldT) {
                 const double *p = part + (size_t)(row0 + i) * K;
 #pragma unroll
@@ -548,7 +625,7 @@ __device__ inline void fold_grids(const double *__restrict__ part, const GridL *
                 for (uint32_t w = 1; w < kFoldT / 64; ++w) t = fmax(t, red[w0 + w][NS + a]);
                 mx[a] = t;
             }
-            use(j, sm, mx);
+            use(g, sm, mx);
         }
         __syncthreads();
     }
@@ -557,42 +634,39 @@ __device__ inline void fold_grids(const double *__restrict__ part, const GridL *
 template <int NT>
 __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
     constexpr uint32_t kW = NT / 64;
-    __shared__ GridL gl[kMaxBlockGrids];
+    __shared__ GridL gl[kMaxGrids];
     __shared__ double red[kW][8];
-    const uint32_t b = blockIdx.x, lane = threadIdx.x & 63;
-    const uint32_t t_lo = A.bt0[b], t_hi = A.bt0[b + 1];
-    const uint32_t s_lo = t_lo * kSub, s_hi = t_hi * kSub;
-    // the block's tiles belong to grids [g_lo, g_lo + ng) (tiles are grid-major)
-    const uint32_t g_lo = t_lo < t_hi ? A.tile_prob[t_lo] : 0u;
-    const uint32_t ng = t_lo < t_hi ? A.tile_prob[t_hi - 1] - g_lo + 1 : 0u;
-    for (uint32_t j = threadIdx.x; j < ng; j += NT) {
-        const Prob &P = A.probs[g_lo + j];
-        const PcgB sb = A.sb[g_lo + j];
-        GridL &G = gl[j];
+    const uint32_t lane = threadIdx.x & 63, nprob = A.nprob;
+    for (uint32_t g = threadIdx.x; g < nprob; g += NT) {
+        const Prob &P = A.probs[g];
+        const PcgB sb = A.sb[g];
+        GridL &G = gl[g];
         G.tile0 = P.tile0;
-        G.tend = P.tile0 + P.ntiles;
+        G.nsub = P.ntiles * kSub;
         G.n = P.n;
         G.nx = P.nx;
         G.fxm = P.fx.m;
         G.fxs = P.fx.s;
-        G.owner = P.owner;
-        G.active = A.active[g_lo + j];
+        G.act = A.active[g] != 0;
         G.cx = P.cx;
         G.cy = P.cy;
         G.rz_prev = sb.rz_prev;
         G.iters = sb.iters;
         G.first = sb.first;
-        G.rr0 = 0.0;
+        G.rr0 = A.sa[g].rr0;
         G.rz = 0.0;
+        G.done = 0;
     }
     __syncthreads();
+    // the set-up and final passes share out every active grid
+    const Share all = make_share(gl, nprob);
     // ---- init 0: sums of x0 and b = -rhs over the gauge cells
-    for_subtiles<kW>(gl, s_lo, s_hi, [&](uint32_t s, uint32_t j, uint32_t c) {
-        const GridL &G = gl[j];
+    for_share<kW>(gl, all, [=](uint32_t s, uint32_t g, uint32_t c) {
+        const GridL &G = gl[g];
         const size_t base = (size_t)G.tile0 * kT;
         double v0 = 0.0, v1 = 0.0;
-        if (G.active && c < G.n && !(A.flags[base + c] & F_ID)) {
-            v0 = A.probs[g_lo + j].guess == FL_GUESS_ZERO ? 0.0 : A.x[base + c];
+        if (c < G.n && !(A.flags[base + c] & F_ID)) {
+            v0 = A.probs[g].guess == FL_GUESS_ZERO ? 0.0 : A.x[base + c];
             v1 = -A.rhs[base + c];
         }
         v0 = wave_sum(v0);
@@ -604,17 +678,17 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
     });
     if (!grid_barrier(A.bar)) return;
     // ---- init 1: r = P(b) - A x0; identity rows x = value, r = 0
-    fold_grids<2, 0, NT>(A.p0, gl, ng, red, [&](uint32_t j) { return gl[j].active != 0; },
-                         [&](uint32_t j, const double *sm, const double *) {
-                             const Prob &P = A.probs[g_lo + j];
-                             gl[j].b = P.zm ? sm[1] * P.inv_count : 0.0;
+    fold_grids<2, 0, NT>(A.p0, gl, all.g0, all.g1, red, [=](uint32_t g) { return gl[g].act != 0; },
+                         [=](uint32_t g, const double *sm, const double *) {
+                             const Prob &P = A.probs[g];
+                             gl[g].b = P.zm ? sm[1] * P.inv_count : 0.0;
                          });
-    for_subtiles<kW>(gl, s_lo, s_hi, [&](uint32_t s, uint32_t j, uint32_t c) {
-        const GridL &G = gl[j];
+    for_share<kW>(gl, all, [=](uint32_t s, uint32_t g, uint32_t c) {
+        const GridL &G = gl[g];
         const size_t base = (size_t)G.tile0 * kT;
         double v[4] = {0.0, 0.0, 0.0, 0.0}, mx = 0.0;
-        if (G.active && c < G.n) {
-            const Prob &P = A.probs[g_lo + j];
+        if (c < G.n) {
+            const Prob &P = A.probs[g];
             const uint8_t *F = A.flags + base;
             double *x = A.x + base;
             const uint8_t f = F[c];
@@ -624,8 +698,8 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
             } else {
                 const bool zero = P.guess == FL_GUESS_ZERO;
                 auto xv = [&](uint32_t k) -> double {
-                    const uint8_t g = F[k];
-                    if (g & F_ID) return (g & F_OUTLET) ? P.pout : 0.0;
+                    const uint8_t h = F[k];
+                    if (h & F_ID) return (h & F_OUTLET) ? P.pout : 0.0;
                     return zero ? 0.0 : x[k];
                 };
                 const double xc = xv(c);
@@ -653,31 +727,27 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
         }
     });
     if (!grid_barrier(A.bar)) return;
-    // zero-guess grids start from x = 0 on the fluid cells
-    for_subtiles<kW>(gl, s_lo, s_hi, [&](uint32_t, uint32_t j, uint32_t c) {
-        const GridL &G = gl[j];
+    // zero-guess grids start from x = 0 on the fluid cells (read next in
+    // phase B, after a barrier)
+    for_share<kW>(gl, all, [=](uint32_t, uint32_t g, uint32_t c) {
+        const GridL &G = gl[g];
         const size_t base = (size_t)G.tile0 * kT;
-        if (G.active && A.probs[g_lo + j].guess == FL_GUESS_ZERO && c < G.n &&
-            !(A.flags[base + c] & F_ID))
+        if (A.probs[g].guess == FL_GUESS_ZERO && c < G.n && !(A.flags[base + c] & F_ID))
             A.x[base + c] = 0.0;
     });
-    // ---- CG loop
+    // ---- CG loop; the host's fin[1] verdicts say which grids iterate
+    for (uint32_t g = threadIdx.x; g < nprob; g += NT) gl[g].act = A.fin[A.nprob + g] == 0;
+    __syncthreads();
+    Share sh = make_share(gl, nprob);
     for (long long k = 0; k < A.kmax; ++k) {
-        const int par = (int)(k & 1), prev = par ^ 1;
+        const int par = (int)(k & 1);
         const double *dold = (k & 1) ? A.d1 : A.d0;
         double *dnew = (k & 1) ? A.d0 : A.d1;
         // phase A scalars: stop test, beta, mean of z
-        for (uint32_t j = threadIdx.x; j < ng; j += NT) {
-            const bool was = A.fin[prev * A.nprob + g_lo + j] != 0;
-            gl[j].done = was ? 1 : 0;
-            if (was && b == gl[j].owner) A.fin[par * A.nprob + g_lo + j] = 1;
-        }
-        __syncthreads();
-        fold_grids<4, 1, NT>(A.pb, gl, ng, red, [&](uint32_t j) { return gl[j].done == 0; },
-                             [&](uint32_t j, const double *sm, const double *mx) {
-            const uint32_t pi = g_lo + j;
-            const Prob &P = A.probs[pi];
-            GridL &G = gl[j];
+        fold_grids<4, 1, NT>(A.pb, gl, sh.g0, sh.g1, red, [=](uint32_t g) { return gl[g].act != 0; },
+                             [=](uint32_t g, const double *sm, const double *mx) {
+            const Prob &P = A.probs[g];
+            GridL &G = gl[g];
             const bool first = G.first != 0;
             const double rr = sm[3], rmax = mx[0];
             const double rz = P.zm ? sm[0] - sm[2] * sm[1] * P.inv_count : sm[0];
@@ -703,8 +773,8 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
             G.rz = rz;
             G.rr0 = rr0;
             G.done = done ? 1 : 0;
-            if (b == G.owner) {
-                PcgA &sa = A.sa[pi];
+            if (G.off >= sh.lo && G.off < sh.hi) {  // this block holds the first sub-tile
+                PcgA &sa = A.sa[g];
                 sa.rz = rz;
                 sa.beta = beta;
                 sa.mz = mz;
@@ -713,32 +783,38 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
                 sa.rmax = rmax;
                 sa.converged = conv;
                 sa.status = status;
-                A.fin[par * A.nprob + pi] = done ? 1 : 0;
+                A.fin[par * A.nprob + g] = done ? 1 : 0;
             }
         });
         // phase A: d' = P(D^-1 r) + beta d at the stencil points, q = A d'
-        for_subtiles<kW>(gl, s_lo, s_hi, [&](uint32_t s, uint32_t j, uint32_t c) {
-            const GridL &G = gl[j];
+        for_share<kW>(gl, sh, [=](uint32_t s, uint32_t g, uint32_t c) {
+            const GridU G = grid_uni(gl[g]);
+            if (G.done) return;
             double v = 0.0;
-            if (!G.done && c < G.n) {
+            if (c < G.n) {
                 const size_t base = (size_t)G.tile0 * kT;
                 const uint8_t *F = A.flags + base;
                 const double *r = A.r + base, *dv = dold + base;
                 // neighbours from the geometry (a closed side still reads a
-                // cell of the grid), so every load is independent of the flags
+                // cell of the grid), so no load waits for the flags
                 const uint32_t i = grid_col(G, c);
                 const uint32_t ce = i + 1 < G.nx ? c + 1 : c, cw = i > 0 ? c - 1 : c;
                 const uint32_t cn = c + G.nx < G.n ? c + G.nx : c, cs = c >= G.nx ? c - G.nx : c;
-                const uint8_t g[5] = {F[c], F[ce], F[cw], F[cn], F[cs]};
-                const double rv[5] = {r[c], r[ce], r[cw], r[cn], r[cs]};
-                const double dd[5] = {dv[c], dv[ce], dv[cw], dv[cn], dv[cs]};
-                double dn[5];
+                const uint32_t k5[5] = {c, ce, cw, cn, cs};
+                uint8_t h[5];
+                double rv[5], dd[5], dn[5];
 #pragma unroll
                 for (int a = 0; a < 5; ++a) {
-                    const double z = (g[a] & F_ID) ? rv[a] : rv[a] * inv_diag(G.cx, G.cy, g[a]) - G.b;
+                    h[a] = F[k5[a]];
+                    rv[a] = ld(r, k5[a]);
+                    dd[a] = ld(dv, k5[a]);
+                }
+#pragma unroll
+                for (int a = 0; a < 5; ++a) {
+                    const double z = (h[a] & F_ID) ? rv[a] : rv[a] * inv_diag(G.cx, G.cy, h[a]) - G.b;
                     dn[a] = G.first ? z : z + G.a * dd[a];
                 }
-                const uint8_t f = g[0];
+                const uint8_t f = h[0];
                 double qc = dn[0];
                 if (!(f & F_ID)) {
                     double dg = 0.0, sum = 0.0;
@@ -748,8 +824,8 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
                     if (f & F_S) { dg += G.cy; sum += G.cy * dn[4]; }
                     qc = dg * dn[0] - sum;
                 }
-                dnew[base + c] = dn[0];
-                A.q[base + c] = qc;
+                st(dnew + base, c, dn[0]);
+                st(A.q + base, c, qc);
                 v = dn[0] * qc;
             }
             v = wave_sum(v);
@@ -757,13 +833,10 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
         });
         if (!grid_barrier(A.bar)) return;
         // phase B scalars: alpha
-        for (uint32_t j = threadIdx.x; j < ng; j += NT)
-            gl[j].done = A.fin[par * A.nprob + g_lo + j] != 0 ? 1 : 0;
-        __syncthreads();
-        fold_grids<1, 0, NT>(A.pa, gl, ng, red, [&](uint32_t j) { return gl[j].done == 0; },
-                             [&](uint32_t j, const double *sm, const double *) {
-            const uint32_t pi = g_lo + j;
-            GridL &G = gl[j];
+        fold_grids<1, 0, NT>(A.pa, gl, sh.g0, sh.g1, red,
+                             [=](uint32_t g) { return gl[g].act != 0 && gl[g].done == 0; },
+                             [=](uint32_t g, const double *sm, const double *) {
+            GridL &G = gl[g];
             const double den = sm[0];
             int status = ST_OK;
             double alpha = 0.0;
@@ -772,32 +845,31 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
                 alpha = G.rz / den;
                 if (!isfinite(alpha)) status = ST_ALPHA;
             }
-            G.rz_prev = G.rz;
-            G.first = 0;
-            if (status == ST_OK) G.iters += 1;
             G.a = alpha;
             G.done = status != ST_OK ? 1 : 0;
-            if (b == G.owner) {
-                PcgB &sb = A.sb[pi];
+            if (G.off >= sh.lo && G.off < sh.hi) {
+                PcgB &sb = A.sb[g];
                 sb.rz_prev = G.rz;
                 sb.alpha = alpha;
                 sb.first = 0;
                 sb.status = status;
-                sb.iters = G.iters;
-                if (status != ST_OK) A.fin[par * A.nprob + pi] = 1;
+                sb.iters = G.iters + (status == ST_OK ? 1 : 0);
+                if (status != ST_OK) A.fin[par * A.nprob + g] = 1;
             }
         });
         // phase B: x += alpha d', r -= alpha q, partials of the next test
-        for_subtiles<kW>(gl, s_lo, s_hi, [&](uint32_t s, uint32_t j, uint32_t c) {
-            const GridL &G = gl[j];
+        for_share<kW>(gl, sh, [=](uint32_t s, uint32_t g, uint32_t c) {
+            const GridU G = grid_uni(gl[g]);
+            if (G.done) return;
             double v[4] = {0.0, 0.0, 0.0, 0.0}, mx = 0.0;
-            if (!G.done && c < G.n) {
-                const size_t i = (size_t)G.tile0 * kT + c;
-                const uint8_t f = A.flags[i];
-                const double xv = A.x[i], dv = dnew[i], r0 = A.r[i], qv = A.q[i];
-                A.x[i] = xv + G.a * dv;
+            if (c < G.n) {
+                const size_t base = (size_t)G.tile0 * kT;
+                const uint8_t f = A.flags[base + c];
+                const double xv = ld(A.x + base, c), dv = ld(dnew + base, c);
+                const double r0 = ld(A.r + base, c), qv = ld(A.q + base, c);
+                st(A.x + base, c, xv + G.a * dv);
                 const double rc = r0 - G.a * qv;
-                A.r[i] = rc;
+                st(A.r + base, c, rc);
                 if (!(f & F_ID)) {
                     const double z = rc * inv_diag(G.cx, G.cy, f);
                     v[0] = rc * z;
@@ -818,16 +890,34 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
             }
         });
         if (!grid_barrier(A.bar)) return;
+        // next loop: which grids still iterate, and their state as the
+        // blocks that held their first sub-tiles wrote it
         int open = 0;
-        for (uint32_t p = threadIdx.x; p < A.nprob; p += NT) open |= A.fin[par * A.nprob + p] == 0;
+        for (uint32_t g = threadIdx.x; g < nprob; g += NT) {
+            GridL &G = gl[g];
+            if (!G.act) continue;
+            const int f = A.fin[par * A.nprob + g];
+            const PcgB sb = A.sb[g];
+            G.act = f == 0;
+            G.rz_prev = sb.rz_prev;
+            G.iters = sb.iters;
+            G.first = sb.first;
+            G.rr0 = A.sa[g].rr0;
+            G.done = 0;
+            open |= G.act;
+        }
         if (!__syncthreads_or(open)) break;
+        sh = make_share(gl, nprob);
     }
     // ---- final: project x to the zero-mean gauge (or write identity values)
-    for_subtiles<kW>(gl, s_lo, s_hi, [&](uint32_t s, uint32_t j, uint32_t c) {
-        const GridL &G = gl[j];
+    for (uint32_t g = threadIdx.x; g < nprob; g += NT) gl[g].act = A.active[g] != 0;
+    __syncthreads();
+    const Share fin_sh = make_share(gl, nprob);
+    for_share<kW>(gl, fin_sh, [=](uint32_t s, uint32_t g, uint32_t c) {
+        const GridL &G = gl[g];
         const size_t base = (size_t)G.tile0 * kT;
         double v = 0.0;
-        if (G.active && c < G.n && !(A.flags[base + c] & F_ID)) v = A.x[base + c];
+        if (c < G.n && !(A.flags[base + c] & F_ID)) v = A.x[base + c];
         v = wave_sum(v);
         if (lane == 0) {
             A.p0[(size_t)s * 2] = v;
@@ -835,17 +925,17 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
         }
     });
     if (!grid_barrier(A.bar)) return;
-    fold_grids<2, 0, NT>(A.p0, gl, ng, red, [&](uint32_t j) { return gl[j].active != 0; },
-                         [&](uint32_t j, const double *sm, const double *) {
-                             const Prob &P = A.probs[g_lo + j];
-                             gl[j].b = P.zm ? sm[0] * P.inv_count : 0.0;
+    fold_grids<2, 0, NT>(A.p0, gl, fin_sh.g0, fin_sh.g1, red, [=](uint32_t g) { return gl[g].act != 0; },
+                         [=](uint32_t g, const double *sm, const double *) {
+                             const Prob &P = A.probs[g];
+                             gl[g].b = P.zm ? sm[0] * P.inv_count : 0.0;
                          });
-    for_subtiles<kW>(gl, s_lo, s_hi, [&](uint32_t, uint32_t j, uint32_t c) {
-        const GridL &G = gl[j];
-        if (!G.active || c >= G.n) return;
+    for_share<kW>(gl, fin_sh, [=](uint32_t, uint32_t g, uint32_t c) {
+        const GridL &G = gl[g];
+        if (c >= G.n) return;
         const size_t i = (size_t)G.tile0 * kT + c;
         const uint8_t f = A.flags[i];
-        if (f & F_ID) A.x[i] = (f & F_OUTLET) ? A.probs[g_lo + j].pout : 0.0;
+        if (f & F_ID) A.x[i] = (f & F_OUTLET) ? A.probs[g].pout : 0.0;
         else A.x[i] -= G.b;
     });
 }
@@ -1009,10 +1099,7 @@ struct Batch2 {
     std::vector<int64_t> step;
     std::vector<double> time, current_dt, max_u, max_v;
     uint32_t pcg_blocks = 1, pcg_threads = 1024;
-    int32_t pcg_partition = FL_PARTITION_TILES;
     unsigned *bar = nullptr;  // the persistent PCG's grid barrier
-    uint32_t *bt0 = nullptr;  // persistent PCG: first tile of each block
-    std::vector<uint32_t> h_bt0;
     long long kmax = 1;
     // device-driven runs: per-grid control state, the captured step halves
     // around the cooperative PCG launch, per-step events
@@ -1048,7 +1135,6 @@ void free_batch(Batch2 *B) {
     for (void *p : h)
         if (p) (void)hipHostFree(p);
     if (B->ctl) (void)hipFree(B->ctl);
-    if (B->bt0) (void)hipFree(B->bt0);
     if (B->bar) (void)hipFree(B->bar);
     if (B->gx_pre) (void)hipGraphExecDestroy(B->gx_pre);
     if (B->gx_post) (void)hipGraphExecDestroy(B->gx_post);
@@ -1103,8 +1189,8 @@ fl_status validate2(fl_ctx *ctx, const fl_grid2d_config &c) {
         if (!std::isfinite(c.max_growth_factor) || c.max_growth_factor < 1.0)
             return set_error(ctx, FL_ERR_CONFIG, "adaptive max_growth_factor must be finite and at least 1");
     }
-    if ((double)c.nx * (double)c.ny > 1.0e9)
-        return set_error(ctx, FL_ERR_CONFIG, "grid too large");
+    if ((double)c.nx * (double)c.ny > 268435456.0)
+        return set_error(ctx, FL_ERR_CONFIG, "grid too large (at most 2^28 cells)");
     return FL_OK;
 }
 
@@ -1132,17 +1218,6 @@ bool reached_end(const Batch2 *B, int p) {
     return B->time[p] >= t_end - tol;
 }
 
-// Grid of tile t (tiles are grid-major).
-uint32_t tile_grid(const Batch2 *B, uint32_t t) {
-    uint32_t lo = 0, hi = (uint32_t)B->hp.size();
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (B->hp[mid].tile0 <= t) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // The persistent PCG's kernel for a block size.
 const void *pcg_kernel(uint32_t threads) {
     switch (threads) {
@@ -1164,62 +1239,16 @@ fl_status pcg_resident(Batch2 *B, uint32_t threads, uint32_t *out) {
     return FL_OK;
 }
 
-// The persistent PCG's launch shape and its tile partition: G blocks of
-// `threads` threads, each holding a contiguous tile range.  A tile weighs 1
-// under FL_PARTITION_TILES (balanced while every grid iterates); under
-// FL_PARTITION_WORK it weighs its grid's size per axis (Jacobi-PCG
-// iteration counts grow linearly with it) plus a fixed cost, so the blocks
-// holding the grids that iterate longest hold fewer tiles and the last
-// iterations, when only those grids are left, run on more blocks.  A grid's
-// owner block is the one holding its first tile.
-fl_status set_pcg_blocks(Batch2 *B, uint32_t G, uint32_t threads, int32_t partition) {
+// The persistent PCG's launch shape: G co-resident blocks of `threads`
+// threads (the blocks share the work out among themselves every iteration).
+fl_status set_pcg_blocks(Batch2 *B, uint32_t G, uint32_t threads) {
     if (threads != 256 && threads != 512 && threads != 1024)
         return set_error(B->ctx, FL_ERR_INVALID_ARGUMENT, "PCG blocks hold 256, 512 or 1024 threads");
-    if (partition != FL_PARTITION_TILES && partition != FL_PARTITION_WORK)
-        return set_error(B->ctx, FL_ERR_INVALID_ARGUMENT, "unknown tile partition %d", partition);
     uint32_t resident = 0;
     fl_status st = pcg_resident(B, threads, &resident);
     if (st != FL_OK) return st;
-    G = std::max<uint32_t>(1, std::min(G, std::min(resident, B->ntiles)));
-    std::vector<double> w(B->ntiles, 0.0);
-    double W = 0.0;
-    for (int p = 0; p < B->count; ++p) {
-        const Prob &P = B->hp[p];
-        const double wt = partition == FL_PARTITION_WORK ? (double)std::max(P.nx, P.ny) + 16.0 : 1.0;
-        for (uint32_t t = 0; t < P.ntiles; ++t) w[P.tile0 + t] = wt;
-        W += wt * P.ntiles;
-    }
-    B->h_bt0.assign(G + 1, B->ntiles);
-    B->h_bt0[0] = 0;
-    double acc = 0.0;
-    uint32_t t = 0;
-    for (uint32_t b = 1; b < G; ++b) {
-        const double target = W * (double)b / (double)G;
-        while (t < B->ntiles && acc + 0.5 * w[t] < target) acc += w[t++];
-        B->h_bt0[b] = t;
-    }
-    for (uint32_t b = 0; b < G; ++b)
-        if (B->h_bt0[b] < B->h_bt0[b + 1]) {
-            const uint32_t g0 = B->hp.size() > 1 ? tile_grid(B, B->h_bt0[b]) : 0;
-            const uint32_t g1 = B->hp.size() > 1 ? tile_grid(B, B->h_bt0[b + 1] - 1) : 0;
-            if (g1 - g0 + 1 > (uint32_t)kMaxBlockGrids)
-                return set_error(B->ctx, FL_ERR_CONFIG,
-                                 "a persistent PCG block would hold %u grids (at most %d); use more blocks",
-                                 g1 - g0 + 1, kMaxBlockGrids);
-        }
-    for (int p = 0; p < B->count; ++p) {
-        Prob &P = B->hp[p];
-        for (uint32_t b = 0; b < G; ++b)
-            if (B->h_bt0[b] <= P.tile0 && P.tile0 < B->h_bt0[b + 1]) P.owner = b;
-    }
-    B->pcg_blocks = G;
+    B->pcg_blocks = std::max<uint32_t>(1, std::min(G, std::min(resident, B->ntiles)));
     B->pcg_threads = threads;
-    B->pcg_partition = partition;
-    hipStream_t s = B->ctx->stream;
-    FL_HIP_TRY(B->ctx, hipMemcpyAsync(B->probs, B->hp.data(), B->count * sizeof(Prob),
-                                      hipMemcpyHostToDevice, s));
-    FL_HIP_TRY(B->ctx, hipMemcpyAsync(B->bt0, B->h_bt0.data(), (G + 1) * sizeof(uint32_t),
-                                      hipMemcpyHostToDevice, s));
     return FL_OK;
 }
 
@@ -1299,11 +1328,8 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
     B->nuf = uo;
     B->nvf = vo;
     // persistent-kernel grid: all blocks co-resident (cooperative launch)
-    uint32_t resident = 0, max_resident = 0;
-    for (uint32_t th : kPcgThreads) {
-        if ((st = pcg_resident(B, th, &resident)) != FL_OK) return st;
-        max_resident = std::max(max_resident, resident);
-    }
+    if (count > kMaxGrids)
+        return set_error(ctx, FL_ERR_CONFIG, "a batch holds at most %d grids", kMaxGrids);
     if ((st = dalloc(B, &B->probs, count)) || (st = dalloc(B, &B->tile_prob, B->ntiles)) ||
         (st = dalloc(B, &B->flags, B->ncell)) || (st = dalloc(B, &B->x, B->ncell)) ||
         (st = dalloc(B, &B->rhs, B->ncell)) || (st = dalloc(B, &B->r, B->ncell)) ||
@@ -1320,7 +1346,7 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
         (st = dalloc(B, &B->sb, count)) || (st = dalloc(B, &B->fin, 2 * count)) ||
         (st = dalloc(B, &B->active, count)) || (st = dalloc(B, &B->dts, count)) ||
         (st = dalloc(B, &B->diag, count)) || (st = dalloc(B, &B->ctl, count)) ||
-        (st = dalloc(B, &B->bt0, (size_t)max_resident + 1)) || (st = dalloc(B, &B->bar, 128)))
+        (st = dalloc(B, &B->bar, 128)))
         return st;
     if ((st = halloc(B, &B->h_active, count)) || (st = halloc(B, &B->h_fin, 2 * count)) ||
         (st = halloc(B, &B->h_dts, count)) || (st = halloc(B, &B->h_sa, count)) ||
@@ -1333,7 +1359,7 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->probs, B->hp.data(), count * sizeof(Prob), hipMemcpyHostToDevice, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->tile_prob, htp.data(), htp.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->flags, hflags.data(), hflags.size(), hipMemcpyHostToDevice, s));
-    if ((st = set_pcg_blocks(B, ctx->num_cu, 1024, FL_PARTITION_TILES)) != FL_OK) return st;
+    if ((st = set_pcg_blocks(B, ctx->num_cu, 1024)) != FL_OK) return st;
     B->step.assign(count, 0);
     B->time.assign(count, 0.0);
     B->current_dt.resize(count);
@@ -1388,7 +1414,7 @@ fl_status run_pcg(Batch2 *B) {
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->sb, B->h_sb, B->count * sizeof(PcgB), hipMemcpyHostToDevice, s));
     PcgArgs a{B->probs, B->tile_prob, B->ntiles, (uint32_t)B->count, kmax, B->flags, B->rhs,
               B->x, B->r, B->d0, B->d1, B->q, B->pa, B->pb, B->p0, B->sa, B->sb,
-              B->active, B->fin, B->bt0, B->bar};
+              B->active, B->fin, B->bar};
     void *args[] = {&a};
     FL_HIP_TRY(ctx, hipLaunchCooperativeKernel(pcg_kernel(B->pcg_threads), dim3(B->pcg_blocks),
                                                dim3(B->pcg_threads), args, 0, s));
@@ -1549,7 +1575,7 @@ fl_status launch_post(Batch2 *B, hipStream_t s) {
 fl_status launch_pcg_coop(Batch2 *B, hipStream_t s, const double *rhs, double *x, long long kmax) {
     PcgArgs a{B->probs, B->tile_prob, B->ntiles, (uint32_t)B->count, kmax, B->flags, rhs,
               x, B->r, B->d0, B->d1, B->q, B->pa, B->pb, B->p0, B->sa, B->sb,
-              B->active, B->fin, B->bt0, B->bar};
+              B->active, B->fin, B->bar};
     void *args[] = {&a};
     FL_HIP_TRY(B->ctx, hipLaunchCooperativeKernel(pcg_kernel(B->pcg_threads), dim3(B->pcg_blocks),
                                                   dim3(B->pcg_threads), args, 0, s));
@@ -1669,8 +1695,8 @@ fl_status run_device(Batch2 *B, int64_t steps, bool graph, fl_batch2d_run_report
                      names[k.which], k.index, val, (long long)k.step);
 }
 
-// Chooses the persistent PCG's launch shape and tile partition: every
-// candidate (threads per block, co-resident block count, partition) solves
+// Chooses the persistent PCG's launch shape: every candidate (threads per
+// block, co-resident block count) solves
 // the same pseudo-random right-hand sides from zero with each grid's own
 // tolerance, capped at `iters` iterations, into scratch x and rhs (the
 // solver's fields are untouched), and the fastest is kept.  Solver results
@@ -1692,16 +1718,14 @@ fl_status autotune(Batch2 *B, int iters, fl_batch2d_tuning *chosen, int32_t cap,
                     std::find(gs.begin(), gs.end(), G) == gs.end())
                     gs.push_back(G);
         if (gs.empty()) gs.push_back(std::min<uint32_t>(B->ntiles, resident));
-        for (uint32_t G : gs)
-            for (int32_t part : {(int32_t)FL_PARTITION_TILES, (int32_t)FL_PARTITION_WORK}) {
-                fl_batch2d_tuning c{};
-                c.threads = (int32_t)th;
-                c.blocks = (int32_t)G;
-                c.partition = part;
-                cand.push_back(c);
-            }
+        for (uint32_t G : gs) {
+            fl_batch2d_tuning c{};
+            c.threads = (int32_t)th;
+            c.blocks = (int32_t)G;
+            cand.push_back(c);
+        }
     }
-    const fl_batch2d_tuning keep{(int32_t)B->pcg_threads, (int32_t)B->pcg_blocks, B->pcg_partition, 0, 0.0};
+    const fl_batch2d_tuning keep{(int32_t)B->pcg_threads, (int32_t)B->pcg_blocks, {0, 0}, 0.0};
     double *rhs = nullptr, *x = nullptr;
     FL_HIP_TRY(ctx, hipMalloc(&rhs, B->ncell * sizeof(double)));
     if (hipMalloc(&x, B->ncell * sizeof(double)) != hipSuccess) {
@@ -1731,7 +1755,7 @@ fl_status autotune(Batch2 *B, int iters, fl_batch2d_tuning *chosen, int32_t cap,
     // an untimed solve with the first candidate, then one timed solve each
     for (size_t ci = 0; ci <= cand.size() && st == FL_OK; ++ci) {
         fl_batch2d_tuning &c = cand[ci == 0 ? 0 : ci - 1];
-        if ((st = set_pcg_blocks(B, (uint32_t)c.blocks, (uint32_t)c.threads, c.partition)) != FL_OK) break;
+        if ((st = set_pcg_blocks(B, (uint32_t)c.blocks, (uint32_t)c.threads)) != FL_OK) break;
         c.blocks = (int32_t)B->pcg_blocks;
         if (hipMemsetAsync(x, 0, B->ncell * sizeof(double), s) != hipSuccess ||
             hipMemcpyAsync(B->fin, B->h_fin, 2 * B->count * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -1759,10 +1783,10 @@ fl_status autotune(Batch2 *B, int iters, fl_batch2d_tuning *chosen, int32_t cap,
     (void)hipFree(x);
     if (st == FL_OK) st = barrier_check(B);
     if (st != FL_OK) {
-        (void)set_pcg_blocks(B, (uint32_t)keep.blocks, (uint32_t)keep.threads, keep.partition);
+        (void)set_pcg_blocks(B, (uint32_t)keep.blocks, (uint32_t)keep.threads);
         return st;
     }
-    if ((st = set_pcg_blocks(B, (uint32_t)best.blocks, (uint32_t)best.threads, best.partition)) != FL_OK)
+    if ((st = set_pcg_blocks(B, (uint32_t)best.blocks, (uint32_t)best.threads)) != FL_OK)
         return st;
     FL_HIP_TRY(ctx, hipStreamSynchronize(s));
     if (chosen) *chosen = best;
@@ -1980,8 +2004,7 @@ fl_status fl_batch2d_set_launch(fl_batch2d *b, const fl_batch2d_tuning *launch) 
     if (!b || !launch) return FL_ERR_INVALID_ARGUMENT;
     Batch2 *B = b->b;
     FL_HIP_TRY(B->ctx, hipSetDevice(B->ctx->device));
-    fl_status st = set_pcg_blocks(B, (uint32_t)std::max(launch->blocks, 1), (uint32_t)launch->threads,
-                                  launch->partition);
+    fl_status st = set_pcg_blocks(B, (uint32_t)std::max(launch->blocks, 1), (uint32_t)launch->threads);
     if (st != FL_OK) return st;
     FL_HIP_TRY(B->ctx, hipStreamSynchronize(B->ctx->stream));
     return FL_OK;
@@ -1990,7 +2013,7 @@ fl_status fl_batch2d_set_launch(fl_batch2d *b, const fl_batch2d_tuning *launch) 
 fl_status fl_batch2d_get_launch(const fl_batch2d *b, fl_batch2d_tuning *launch) {
     if (!b || !launch) return FL_ERR_INVALID_ARGUMENT;
     const Batch2 *B = b->b;
-    *launch = fl_batch2d_tuning{(int32_t)B->pcg_threads, (int32_t)B->pcg_blocks, B->pcg_partition, 0, 0.0};
+    *launch = fl_batch2d_tuning{(int32_t)B->pcg_threads, (int32_t)B->pcg_blocks, {0, 0}, 0.0};
     return FL_OK;
 }
 

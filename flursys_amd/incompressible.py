@@ -227,9 +227,8 @@ class CavityBatch:
         return self.last_run, (ms[:steps] if step_times else None)
 
     def autotune(self, iterations: int = 20):
-        """Chooses the persistent PCG's launch shape and tile partition;
-        returns (chosen, table), each entry a dict with threads, blocks,
-        partition and ms."""
+        """Chooses the persistent PCG's launch shape; returns (chosen, table),
+        each entry a dict with threads, blocks and ms."""
         best = _lib.Batch2DTuning()
         tab = (_lib.Batch2DTuning * 64)()
         n = C.c_int32()
@@ -243,10 +242,10 @@ class CavityBatch:
         check(load().fl_batch2d_get_launch(self.handle, C.byref(t)), self.ctx.handle)
         return _tuning(t)
 
-    def set_launch(self, threads: int, blocks: int, partition: int = _lib.PARTITION_TILES):
-        t = _lib.Batch2DTuning(threads, blocks, partition, 0, 0.0)
+    def set_launch(self, threads: int, blocks: int):
+        t = _lib.Batch2DTuning(threads, blocks)
         check(load().fl_batch2d_set_launch(self.handle, C.byref(t)), self.ctx.handle)
 
 
 def _tuning(t) -> dict:
-    return {"threads": t.threads, "blocks": t.blocks, "partition": t.partition, "ms": t.ms}
+    return {"threads": t.threads, "blocks": t.blocks, "ms": t.ms}

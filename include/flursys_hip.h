@@ -281,22 +281,20 @@ typedef struct {
 fl_status fl_batch2d_run(fl_batch2d *batch, int64_t steps, int32_t use_graphs,
                          fl_batch2d_run_report *reports, double *step_ms);
 /* Launch shape of the batch's persistent PCG (no reference counterpart):
- * `threads` per block (256, 512 or 1024), `blocks` (clamped to what is
- * co-resident and to the tile count) and how the grids' 256-cell tiles are
- * split into the blocks' contiguous ranges.  Results do not depend on it. */
-enum { FL_PARTITION_TILES = 0, /* equal tile counts */
-       FL_PARTITION_WORK = 1   /* tiles weighted by their grid's size per axis */ };
+ * `threads` per block (256, 512 or 1024) and `blocks` (clamped to what is
+ * co-resident and to the tile count).  The blocks share the iterating
+ * grids' cells out equally every iteration; results do not depend on the
+ * shape.  A batch holds at most 256 grids. */
 typedef struct fl_batch2d_tuning {
     int32_t threads;
     int32_t blocks;
-    int32_t partition;
-    int32_t reserved;
+    int32_t reserved[2];
     double ms; /* autotuner: time of the candidate's solve */
 } fl_batch2d_tuning;
 fl_status fl_batch2d_set_launch(fl_batch2d *batch, const fl_batch2d_tuning *launch);
 fl_status fl_batch2d_get_launch(const fl_batch2d *batch, fl_batch2d_tuning *launch);
 /* Autotuner of the launch shape (no reference counterpart): every candidate
- * (threads, co-resident block count, partition) solves the same
+ * (threads, co-resident block count) solves the same
  * pseudo-random right-hand sides from zero with each grid's tolerance,
  * capped at `iterations` PCG iterations, in scratch memory; the fastest is
  * kept and returned in `*chosen`.  When `table` (capacity `table_cap`) is

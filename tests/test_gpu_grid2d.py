@@ -290,19 +290,19 @@ def test_autotune_keeps_results(gpu_ctx):
 
 def test_launch_shapes_give_identical_results(gpu_ctx):
     """Partials are per 64-cell sub-tile and folded in a fixed order, so the
-    persistent PCG's block size, block count and partition change nothing."""
+    persistent PCG's block size and block count change nothing."""
     print("[gpu] batch launch shapes", flush=True)
     cfgs = uneven_cfgs() + [I.SimulationConfig.cavity(96, reynolds=700.0)]
     ref = I.CavityBatch(cfgs, ctx=gpu_ctx)
     ref.run(3)
-    for threads, blocks, part in ((256, 1, 0), (512, 7, 1), (1024, 3, 0), (256, 40, 1)):
+    for threads, blocks in ((256, 1), (512, 7), (1024, 3), (256, 40)):
         b = I.CavityBatch(cfgs, ctx=gpu_ctx)
-        b.set_launch(threads, blocks, part)
+        b.set_launch(threads, blocks)
         assert b.launch()["threads"] == threads
         b.run(3)
         for fr, fb in zip(all_fields(ref), all_fields(b)):
             for name in ("u", "v", "p"):
-                assert np.array_equal(fr[name], fb[name]), (threads, blocks, part, name)
+                assert np.array_equal(fr[name], fb[name]), (threads, blocks, name)
         b.close()
 
 
@@ -310,5 +310,6 @@ def test_launch_shape_validation(gpu_ctx):
     b = I.CavityBatch(uneven_cfgs()[:1], ctx=gpu_ctx)
     with pytest.raises(fa.FlursysError, match="256, 512 or 1024 threads"):
         b.set_launch(128, 4)
-    with pytest.raises(fa.FlursysError, match="unknown tile partition"):
-        b.set_launch(256, 4, 7)
+    too_many = [I.SimulationConfig.cavity(8)] * 257
+    with pytest.raises(fa.FlursysError, match="at most 256 grids"):
+        I.CavityBatch(too_many, ctx=gpu_ctx)
