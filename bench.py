@@ -147,7 +147,7 @@ def run_variant(fa, ctx, name, precision, precond, args):
     }
 
 
-def batch2d_configs(fa, count=64, seed=1):
+def batch2d_configs(fa, count=64, seed=1, precision=None):
     """64 2D lid-driven cavities, n = round(64 * 8^u) with u ~ U[0, 1) (log-
     uniform 64-512), Re ~ U[100, 1000] (both from numpy's PCG64, seed 1), the
     reference's adaptive dt rule (target CFL 0.5, diffusion number 0.25) and
@@ -161,8 +161,12 @@ def batch2d_configs(fa, count=64, seed=1):
     u = rng.random(count)
     re = 100.0 + 900.0 * rng.random(count)
     ts = I.TimeStepSettings(adaptive=True, target_cfl=0.5, target_diffusion_number=0.25)
-    return [I.SimulationConfig.cavity(int(round(64 * 8.0 ** a)), reynolds=float(r), time_step=ts)
+    cfgs = [I.SimulationConfig.cavity(int(round(64 * 8.0 ** a)), reynolds=float(r), time_step=ts)
             for a, r in zip(u, re)]
+    if precision is not None:
+        for c in cfgs:
+            c.precision = precision
+    return cfgs
 
 
 def run_batch2d(fa, ctx, args):
@@ -170,7 +174,8 @@ def run_batch2d(fa, ctx, args):
 
     import flursys_amd.incompressible as I
 
-    cfgs = batch2d_configs(fa)
+    prec = fa.PRECISION_MIXED if args.batch_precision == "mixed" else fa.PRECISION_FP64
+    cfgs = batch2d_configs(fa, precision=prec)
     cells = sum(c.nx * c.ny for c in cfgs)
     log(f"[bench] batch2d: {len(cfgs)} cavities, {cells} cells, n "
         f"{min(c.nx for c in cfgs)}-{max(c.nx for c in cfgs)}")
@@ -219,12 +224,13 @@ def run_batch2d(fa, ctx, args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 CG in f64 refinement" if args.batch_precision == "mixed" else "f64",
         "data": "synthetic: cavities started from rest (no dataset)",
         "config": {
             "workload": f"{len(cfgs)} 2D lid-driven cavities, n = round(64*8^u), Re U[100,1000] "
-                        "(seed 1), adaptive dt (CFL 0.5, diffusion 0.25), upwind, Jacobi-PCG "
-                        "warm start max|r| < 1e-5",
+                        "(seed 1), adaptive dt (CFL 0.5, diffusion 0.25), upwind, Jacobi-"
+                        + ("CG cycles (FP32) in FP64 refinement" if args.batch_precision == "mixed"
+                           else "PCG (FP64)") + ", previous pressure, true max|r| < 1e-5",
             "cells": cells,
             "step_latency_ms": {"p50": float(np.percentile(ms, 50)),
                                 "p99": float(np.percentile(ms, 99)),
@@ -253,6 +259,8 @@ def main():
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=0.0)
+    ap.add_argument("--batch-precision", choices=("fp64", "mixed"), default="mixed",
+                    help="batch pressure solves: FP64 PCG, or FP32 CG in FP64 refinement")
     ap.add_argument("--tune-iterations", type=int, default=1300,
                     help="iteration cap of the batch autotuner's solves")
     ap.add_argument("--variants", default="plain_fp64_pcg,mixed_ir,mg_fcg,learned_fcg")

@@ -119,7 +119,8 @@ class Grid2DConfig(C.Structure):
         ("bc_kind", C.c_int32 * 4), ("bc_u", C.c_double * 4), ("bc_v", C.c_double * 4),
         ("outlet_pressure", C.c_double * 4), ("min_dt", C.c_double), ("max_dt", C.c_double),
         ("target_cfl", C.c_double), ("target_diffusion_number", C.c_double),
-        ("max_growth_factor", C.c_double),
+        ("max_growth_factor", C.c_double), ("precision", C.c_int32), ("reserved", C.c_int32),
+        ("inner_tolerance", C.c_double),
     ]
 
 

@@ -36,6 +36,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -64,18 +65,21 @@ struct Prob {
     double inv_count;
     double tol;
     long long max_iter;
+    double eta;  // mixed path: relative tolerance of each inner CG cycle
     int32_t stop_rule, guess;
     FastDiv fx;
 };
 
 struct PcgA {  // a grid's PCG state after phase A
     double rz, beta, mz, rr0, rr, rmax;
+    double rr0o, mb;  // r.r of the first true residual; mean of b
     int converged, status;
 };
 struct PcgB {  // a grid's PCG state after phase B (set up by the host)
     double rz_prev, alpha;
     long long iters;
     int first, status;
+    int outer, pad;  // mixed path: the last pass was a true residual
 };
 enum { ST_OK = 0, ST_SINGULAR = 1, ST_ALPHA = 2, ST_RESIDUAL = 3, ST_BARRIER = 4 };
 
@@ -382,33 +386,39 @@ __global__ void __launch_bounds__(kT) k_fold_diag(const Prob *probs, const int *
 }
 
 // ------------------------------------------------------ persistent PCG
-// Work layout.  The 256-cell tiles of every grid are cut into 64-cell
-// sub-tiles (one per wave).  At the start of each iteration every block
-// lists the sub-tiles of the grids still iterating, in grid order, and takes
-// an equal contiguous share of that list, so the work stays balanced over
-// all blocks as grids finish.  A wave streams through its block's share and
-// writes one partial row per sub-tile (rows are stored by the sub-tile's
-// place in its grid, not by who computed them).  Every block then folds the
-// rows of the grids its share touches, with groups of 256 threads: thread u
-// sums rows u, u + 256, ... in order, each wave combines its lanes by a
-// fixed butterfly and the group adds its four wave totals in order.  So
-// every block gets the same scalars, and results depend neither on the
-// number of blocks, nor on their size, nor on the share a block gets.  The
-// block whose share holds a grid's first sub-tile writes the grid's state
-// out (for the host and for the next iteration's blocks).
-constexpr int kSub = kT / 64;     // sub-tiles per tile
+// Work layout.  At the start of each iteration every block lists the
+// 256-cell tiles of the grids still iterating, in grid order, and takes an
+// equal contiguous share of that list, so the work stays balanced over all
+// blocks as grids finish.  A wave takes whole tiles of its block's share
+// (tile i goes to wave i mod NT/64); each lane holds four cells of the tile
+// (lane, lane + 64, ...), so every lane has four cells' loads in flight,
+// and the wave writes one partial row per tile: the lane sums its cells in
+// order and the wave combines its lanes by a fixed butterfly.  Every block
+// then folds the rows of the grids its share touches, with groups of 256
+// threads: thread u sums rows u, u + 256, ... in order, each wave combines
+// its lanes by the butterfly and the group adds its four wave totals in
+// order.  So every block gets the same scalars, and results depend neither
+// on the number of blocks, nor on their size, nor on the share a block
+// gets.  The block whose share holds a grid's first tile writes the grid's
+// state out (for the host and for the next iteration's blocks).
+constexpr int kCells = kT / 64;   // cells per lane in a tile
 constexpr int kMaxGrids = 256;    // grids per batch
 constexpr uint32_t kFoldT = 256;  // threads folding one grid's rows
 
+// T is the storage type of the CG vectors: double (the reference's FP64
+// PCG) or float (FP32 CG cycles on the correction e inside FP64 iterative
+// refinement: x and every true residual stay FP64).
+template <typename T>
 struct PcgArgs {
     const Prob *probs;
     const uint32_t *tile_prob;
     uint32_t ntiles, nprob;
-    long long kmax;  // hard bound on the loop (max over grids of max_iter + 2)
+    long long kmax;  // hard bound on the loop
     const uint8_t *flags;
     const double *rhs;
-    double *x, *r, *d0, *d1, *q;
-    double *pa, *pb, *p0;  // per-sub-tile partial rows: 1, 5 and 2 columns
+    double *x;
+    T *r, *d0, *d1, *q, *e;  // FP64: e is x
+    double *pa, *pb, *p0;  // per-tile partial rows: 1, 5 and 2 columns
     PcgA *sa;
     PcgB *sb;
     const int *active;  // grids taking part in this solve
@@ -470,16 +480,35 @@ __device__ inline double inv_diag(double cx, double cy, uint8_t f) {
 // A grid as every PCG block sees it: geometry, its place in the iteration's
 // sub-tile list, the scalars of the current phase and the solver state.
 struct GridL {
-    uint32_t tile0, nsub, n, nx;
+    uint32_t tile0, ntiles, n, nx;
     uint32_t fxm, fxs;  // x = c mod nx via FastDiv's multiplier and shift
-    uint32_t off;       // first list index (sub-tiles of earlier iterating grids)
+    uint32_t off;       // first list index (tiles of earlier iterating grids)
     int32_t act;        // iterating in this loop
     double cx, cy;
     double a, b;  // phase A: beta, mean of z; phase B: alpha; setup: mean of b / x
     double rz, rz_prev, rr0;
+    double rr0o, mb;  // r.r of the first true residual; mean of b
     long long iters;
     int32_t done, first;
+    int32_t outer;    // mixed: the partials come from a true residual
+    int32_t skip_a;   // no CG work in phase A
+    int32_t apply;    // mixed: x += e, e = 0 in phase A
+    int32_t outer_b;  // mixed: phase B is a true-residual pass
 };
+
+// Adds one cell's residual to the stop test's partial sums: r.z (r.r on
+// identity rows), z, r (both on the gauge cells only) and r.r.
+__device__ inline void residual_partials(uint8_t f, double r, double inv_d, double (&v)[4]) {
+    if (!(f & F_ID)) {
+        const double z = r * inv_d;
+        v[0] += r * z;
+        v[1] += z;
+        v[2] += r;
+    } else {
+        v[0] += r * r;
+    }
+    v[3] += r * r;
+}
 
 // Wave-uniform copies (scalar registers) of values read from LDS.
 __device__ inline uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -494,11 +523,12 @@ __device__ inline double uni(double v) {
 struct GridU {
     uint32_t tile0, n, nx, fxm, fxs;
     double cx, cy, a, b;
-    bool first, done;
+    bool first, done, skip_a, apply;
 };
 __device__ inline GridU grid_uni(const GridL &G) {
     return {uni(G.tile0), uni(G.n), uni(G.nx), uni(G.fxm), uni(G.fxs), uni(G.cx), uni(G.cy),
-            uni(G.a), uni(G.b), uni((uint32_t)G.first) != 0, uni((uint32_t)G.done) != 0};
+            uni(G.a), uni(G.b), uni((uint32_t)G.first) != 0, uni((uint32_t)G.done) != 0,
+            uni((uint32_t)G.skip_a) != 0, uni((uint32_t)G.apply) != 0};
 }
 __device__ inline uint32_t grid_col(const GridU &G, uint32_t c) {
     const uint32_t t = __umulhi(c, G.fxm);
@@ -507,14 +537,16 @@ __device__ inline uint32_t grid_col(const GridU &G, uint32_t c) {
 
 // Element i of a grid's array at a 32-bit byte offset from a scalar base
 // (grids hold fewer than 2^28 cells).
-__device__ inline double ld(const double *base, uint32_t i) {
-    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + (i << 3));
+template <typename T>
+__device__ inline T ld(const T *base, uint32_t i) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * (uint32_t)sizeof(T));
 }
-__device__ inline void st(double *base, uint32_t i, double v) {
-    *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + (i << 3)) = v;
+template <typename T>
+__device__ inline void st(T *base, uint32_t i, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(T)) = v;
 }
 
-// The iteration's sub-tile list: offsets of the iterating grids (one wave
+// The iteration's tile list: offsets of the iterating grids (one wave
 // scans gl[].act in grid order), this block's share [lo, hi) and the grids
 // [g0, g1] that share touches.  Ends with a block barrier.
 struct Share {
@@ -526,7 +558,7 @@ __device__ inline Share make_share(GridL *gl, uint32_t nprob) {
         uint32_t carry = 0;
         for (uint32_t c0 = 0; c0 < nprob; c0 += 64) {
             const uint32_t g = c0 + threadIdx.x;
-            const uint32_t v = g < nprob && gl[g].act ? gl[g].nsub : 0u;
+            const uint32_t v = g < nprob && gl[g].act ? gl[g].ntiles : 0u;
             uint32_t incl = v;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
@@ -550,8 +582,8 @@ __device__ inline Share make_share(GridL *gl, uint32_t nprob) {
     for (uint32_t g = threadIdx.x; g < nprob; g += blockDim.x) {
         const GridL &G = gl[g];
         if (!G.act || sh.lo >= sh.hi) continue;
-        if (G.off <= sh.lo && sh.lo < G.off + G.nsub) s_g0 = g;
-        if (G.off < sh.hi && sh.hi <= G.off + G.nsub) s_g1 = g;
+        if (G.off <= sh.lo && sh.lo < G.off + G.ntiles) s_g0 = g;
+        if (G.off < sh.hi && sh.hi <= G.off + G.ntiles) s_g1 = g;
     }
     __syncthreads();
     sh.g0 = s_g0;
@@ -561,16 +593,17 @@ __device__ inline Share make_share(GridL *gl, uint32_t nprob) {
     return sh;
 }
 
-// Calls f(s, g, c) for each of this wave's sub-tiles in the block's share:
-// s is the sub-tile's partial row, g its grid, c this lane's cell.
+// Calls f(t, g, c0) for each of this wave's tiles in the block's share: t
+// is the tile (its partial row), g its grid, c0 this lane's first cell (the
+// others follow at +64, +128, +192).
 template <uint32_t kW, typename F>
 __device__ inline void for_share(const GridL *gl, const Share &sh, F &&f) {
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     uint32_t g = sh.g0;
     for (uint32_t i = sh.lo + wave; i < sh.hi; i += kW) {
-        while (g + 1 < uni(sh.ng) && (!uni((uint32_t)gl[g].act) || i >= uni(gl[g].off) + uni(gl[g].nsub))) ++g;
-        const uint32_t ls = i - uni(gl[g].off);
-        f(uni(gl[g].tile0) * kSub + ls, g, ls * 64 + lane);
+        while (g + 1 < uni(sh.ng) && (!uni((uint32_t)gl[g].act) || i >= uni(gl[g].off) + uni(gl[g].ntiles))) ++g;
+        const uint32_t lt = i - uni(gl[g].off);
+        f(uni(gl[g].tile0) + lt, g, lt * kT + lane);
     }
 }
 
@@ -592,7 +625,7 @@ __device__ inline void fold_grids(const double *__restrict__ part, const GridL *
 #pragma unroll
         for (int a = 0; a < K; ++a) v[a] = 0.0;
         if (on) {
-            const uint32_t row0 = gl[g].tile0 * kSub, rows = gl[g].nsub;
+            const uint32_t row0 = gl[g].tile0, rows = gl[g].ntiles;
 #pragma unroll 2
             for (uint32_t i = u; i < rows; i += kFoldT) {
                 const double *p = part + (size_t)(row0 + i) * K;
@@ -631,8 +664,11 @@ __device__ inline void fold_grids(const double *__restrict__ part, const GridL *
     }
 }
 
-template <int NT>
-__global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
+template <int NT, typename T>
+__global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
+    constexpr bool kMixed = !std::is_same<T, double>::value;
+    // cells per lane whose phase-A loads are in flight together
+    constexpr int kBatchA = 2;
     constexpr uint32_t kW = NT / 64;
     __shared__ GridL gl[kMaxGrids];
     __shared__ double red[kW][8];
@@ -642,7 +678,7 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
         const PcgB sb = A.sb[g];
         GridL &G = gl[g];
         G.tile0 = P.tile0;
-        G.nsub = P.ntiles * kSub;
+        G.ntiles = P.ntiles;
         G.n = P.n;
         G.nx = P.nx;
         G.fxm = P.fx.m;
@@ -653,50 +689,62 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
         G.rz_prev = sb.rz_prev;
         G.iters = sb.iters;
         G.first = sb.first;
-        G.rr0 = A.sa[g].rr0;
+        G.outer = kMixed ? 1 : 0;  // the set-up residual is a true residual
+        G.rr0 = 0.0;
+        G.rr0o = -1.0;  // not yet known
         G.rz = 0.0;
-        G.done = 0;
+        G.done = G.skip_a = G.apply = G.outer_b = 0;
     }
     __syncthreads();
     // the set-up and final passes share out every active grid
     const Share all = make_share(gl, nprob);
     // ---- init 0: sums of x0 and b = -rhs over the gauge cells
-    for_share<kW>(gl, all, [=](uint32_t s, uint32_t g, uint32_t c) {
+    for_share<kW>(gl, all, [=](uint32_t t, uint32_t g, uint32_t c0) {
         const GridL &G = gl[g];
         const size_t base = (size_t)G.tile0 * kT;
+        const bool zero = A.probs[g].guess == FL_GUESS_ZERO;
         double v0 = 0.0, v1 = 0.0;
-        if (c < G.n && !(A.flags[base + c] & F_ID)) {
-            v0 = A.probs[g].guess == FL_GUESS_ZERO ? 0.0 : A.x[base + c];
-            v1 = -A.rhs[base + c];
+#pragma unroll
+        for (int u = 0; u < kCells; ++u) {
+            const uint32_t c = c0 + 64 * u;
+            if (c < G.n && !(A.flags[base + c] & F_ID)) {
+                v0 += zero ? 0.0 : A.x[base + c];
+                v1 += -A.rhs[base + c];
+            }
         }
         v0 = wave_sum(v0);
         v1 = wave_sum(v1);
         if (lane == 0) {
-            A.p0[(size_t)s * 2] = v0;
-            A.p0[(size_t)s * 2 + 1] = v1;
+            A.p0[(size_t)t * 2] = v0;
+            A.p0[(size_t)t * 2 + 1] = v1;
         }
     });
     if (!grid_barrier(A.bar)) return;
-    // ---- init 1: r = P(b) - A x0; identity rows x = value, r = 0
+    // ---- init 1: r = P(b) - A x0; identity rows x = value, r = 0 (the
+    // mixed path stores r in T and clears the correction e)
     fold_grids<2, 0, NT>(A.p0, gl, all.g0, all.g1, red, [=](uint32_t g) { return gl[g].act != 0; },
                          [=](uint32_t g, const double *sm, const double *) {
                              const Prob &P = A.probs[g];
-                             gl[g].b = P.zm ? sm[1] * P.inv_count : 0.0;
+                             gl[g].mb = P.zm ? sm[1] * P.inv_count : 0.0;
+                             if (gl[g].off >= all.lo && gl[g].off < all.hi) A.sa[g].mb = gl[g].mb;
                          });
-    for_share<kW>(gl, all, [=](uint32_t s, uint32_t g, uint32_t c) {
+    for_share<kW>(gl, all, [=](uint32_t t, uint32_t g, uint32_t c0) {
         const GridL &G = gl[g];
+        const Prob &P = A.probs[g];
         const size_t base = (size_t)G.tile0 * kT;
+        const uint8_t *F = A.flags + base;
+        double *x = A.x + base;
+        const bool zero = P.guess == FL_GUESS_ZERO;
         double v[4] = {0.0, 0.0, 0.0, 0.0}, mx = 0.0;
-        if (c < G.n) {
-            const Prob &P = A.probs[g];
-            const uint8_t *F = A.flags + base;
-            double *x = A.x + base;
+#pragma unroll
+        for (int u = 0; u < kCells; ++u) {
+            const uint32_t c = c0 + 64 * u;
+            if (c >= G.n) continue;
             const uint8_t f = F[c];
-            double rc = 0.0, z = 0.0;
+            double rc = 0.0;
             if (f & F_ID) {
                 x[c] = (f & F_OUTLET) ? P.pout : 0.0;
             } else {
-                const bool zero = P.guess == FL_GUESS_ZERO;
                 auto xv = [&](uint32_t k) -> double {
                     const uint8_t h = F[k];
                     if (h & F_ID) return (h & F_OUTLET) ? P.pout : 0.0;
@@ -708,133 +756,206 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
                 if (f & F_W) { dg += G.cx; sum += G.cx * xv(c - 1); }
                 if (f & F_N) { dg += G.cy; sum += G.cy * xv(c + G.nx); }
                 if (f & F_S) { dg += G.cy; sum += G.cy * xv(c - G.nx); }
-                rc = (-A.rhs[base + c] - G.b) - (dg * xc - sum);
-                z = rc * inv_diag(G.cx, G.cy, f);
-                v[1] = z;
-                v[2] = rc;
+                rc = (-A.rhs[base + c] - G.mb) - (dg * xc - sum);
             }
-            A.r[base + c] = rc;
-            v[0] = rc * z;
-            v[3] = rc * rc;
-            mx = fabs(rc);
+            const T rt = (T)rc;
+            A.r[base + c] = rt;
+            if (kMixed) A.e[base + c] = (T)0;
+            residual_partials(f, (double)rt, inv_diag(G.cx, G.cy, f), v);
+            mx = fmax(mx, fabs(rc));
         }
 #pragma unroll
         for (int a = 0; a < 4; ++a) v[a] = wave_sum(v[a]);
         mx = wave_max(mx);
         if (lane == 0) {
-            double *pb = A.pb + (size_t)s * 5;
+            double *pb = A.pb + (size_t)t * 5;
             pb[0] = v[0]; pb[1] = v[1]; pb[2] = v[2]; pb[3] = v[3]; pb[4] = mx;
         }
     });
     if (!grid_barrier(A.bar)) return;
     // zero-guess grids start from x = 0 on the fluid cells (read next in
     // phase B, after a barrier)
-    for_share<kW>(gl, all, [=](uint32_t, uint32_t g, uint32_t c) {
+    for_share<kW>(gl, all, [=](uint32_t, uint32_t g, uint32_t c0) {
         const GridL &G = gl[g];
         const size_t base = (size_t)G.tile0 * kT;
-        if (A.probs[g].guess == FL_GUESS_ZERO && c < G.n && !(A.flags[base + c] & F_ID))
-            A.x[base + c] = 0.0;
+        if (A.probs[g].guess != FL_GUESS_ZERO) return;
+#pragma unroll
+        for (int u = 0; u < kCells; ++u) {
+            const uint32_t c = c0 + 64 * u;
+            if (c < G.n && !(A.flags[base + c] & F_ID)) A.x[base + c] = 0.0;
+        }
     });
     // ---- CG loop; the host's fin[1] verdicts say which grids iterate
-    for (uint32_t g = threadIdx.x; g < nprob; g += NT) gl[g].act = A.fin[A.nprob + g] == 0;
+    for (uint32_t g = threadIdx.x; g < nprob; g += NT) {
+        gl[g].act = A.fin[A.nprob + g] == 0;
+        gl[g].mb = A.sa[g].mb;
+    }
     __syncthreads();
     Share sh = make_share(gl, nprob);
     for (long long k = 0; k < A.kmax; ++k) {
         const int par = (int)(k & 1);
-        const double *dold = (k & 1) ? A.d1 : A.d0;
-        double *dnew = (k & 1) ? A.d0 : A.d1;
-        // phase A scalars: stop test, beta, mean of z
+        const T *dold = (k & 1) ? A.d1 : A.d0;
+        T *dnew = (k & 1) ? A.d0 : A.d1;
+        // phase A scalars: stop test, beta, mean of z; the mixed path also
+        // decides between a CG step and a true-residual pass
         fold_grids<4, 1, NT>(A.pb, gl, sh.g0, sh.g1, red, [=](uint32_t g) { return gl[g].act != 0; },
                              [=](uint32_t g, const double *sm, const double *mx) {
             const Prob &P = A.probs[g];
             GridL &G = gl[g];
-            const bool first = G.first != 0;
             const double rr = sm[3], rmax = mx[0];
             const double rz = P.zm ? sm[0] - sm[2] * sm[1] * P.inv_count : sm[0];
             const double mz = P.zm ? sm[1] * P.inv_count : 0.0;
-            const double rr0 = first ? rr : G.rr0;
             int status = ST_OK, conv = 0;
-            bool done = false;
-            double beta = 0.0;
-            if (!first && !isfinite(rmax)) {
-                status = ST_RESIDUAL;
-                done = true;
-            } else if (P.stop_rule == FL_STOP_REL_L2 ? sqrt(rr) <= P.tol * sqrt(rr0)
-                                                    : rmax < P.tol) {
-                conv = 1;
-                done = true;
-            } else if (!first) {
-                if (fabs(G.rz_prev) < 1.0e-30) done = true;
-                else if (G.iters >= P.max_iter) done = true;
-                else beta = rz / G.rz_prev;
+            bool done = false, skip_a = false, outer_b = false, first = G.first != 0;
+            double beta = 0.0, rr0 = G.rr0, rr0o = G.rr0o;
+            const bool owner = G.off >= sh.lo && G.off < sh.hi;
+            if (!kMixed) {
+                rr0 = first ? rr : G.rr0;
+                if (!first && !isfinite(rmax)) {
+                    status = ST_RESIDUAL;
+                    done = true;
+                } else if (P.stop_rule == FL_STOP_REL_L2 ? sqrt(rr) <= P.tol * sqrt(rr0)
+                                                        : rmax < P.tol) {
+                    conv = 1;
+                    done = true;
+                } else if (!first) {
+                    if (fabs(G.rz_prev) < 1.0e-30) done = true;
+                    else if (G.iters >= P.max_iter) done = true;
+                    else beta = rz / G.rz_prev;
+                }
+            } else if (G.outer) {
+                // a true residual: the reference's stop test, else a new
+                // inner CG from it
+                if (rr0o < 0.0) rr0o = rr;
+                if (!isfinite(rmax)) {
+                    status = ST_RESIDUAL;
+                    done = true;
+                } else if (P.stop_rule == FL_STOP_REL_L2 ? sqrt(rr) <= P.tol * sqrt(rr0o)
+                                                        : rmax < P.tol) {
+                    conv = 1;
+                    done = true;
+                } else if (G.iters >= P.max_iter) {
+                    done = true;
+                } else {
+                    first = true;
+                    rr0 = rr;
+                }
+            } else {
+                // inner CG: end the cycle (true-residual pass next) when the
+                // recurrence residual meets half the outer tolerance, has
+                // dropped by the inner tolerance, or the budget is spent
+                const bool met = P.stop_rule == FL_STOP_REL_L2 ? sqrt(rr) <= 0.5 * P.tol * sqrt(rr0o)
+                                                               : rmax <= 0.5 * P.tol;
+                if (!isfinite(rmax) || met || sqrt(rr) <= P.eta * sqrt(rr0) ||
+                    G.iters >= P.max_iter || fabs(G.rz_prev) < 1.0e-30) {
+                    skip_a = true;
+                    outer_b = true;
+                } else {
+                    beta = rz / G.rz_prev;
+                }
             }
+            G.apply = kMixed && G.outer ? 1 : 0;  // x += e after a true-residual pass
             G.a = beta;
             G.b = mz;
             G.rz = rz;
             G.rr0 = rr0;
+            G.rr0o = rr0o;
+            G.first = first ? 1 : 0;
             G.done = done ? 1 : 0;
-            if (G.off >= sh.lo && G.off < sh.hi) {  // this block holds the first sub-tile
+            G.skip_a = done || skip_a ? 1 : 0;
+            G.outer_b = outer_b ? 1 : 0;
+            if (owner) {
                 PcgA &sa = A.sa[g];
                 sa.rz = rz;
                 sa.beta = beta;
                 sa.mz = mz;
                 sa.rr0 = rr0;
-                sa.rr = rr;
-                sa.rmax = rmax;
-                sa.converged = conv;
-                sa.status = status;
+                sa.rr0o = rr0o;
+                if (!kMixed || G.outer) {
+                    sa.rr = rr;
+                    sa.rmax = rmax;
+                    sa.converged = conv;
+                    sa.status = status;
+                }
                 A.fin[par * A.nprob + g] = done ? 1 : 0;
             }
         });
-        // phase A: d' = P(D^-1 r) + beta d at the stencil points, q = A d'
-        for_share<kW>(gl, sh, [=](uint32_t s, uint32_t g, uint32_t c) {
+        // phase A: (mixed, after a true-residual pass) x += e, e = 0; then
+        // d' = P(D^-1 r) + beta d at the stencil points, q = A d'
+        for_share<kW>(gl, sh, [=](uint32_t t, uint32_t g, uint32_t c0) {
             const GridU G = grid_uni(gl[g]);
-            if (G.done) return;
+            const size_t base = (size_t)G.tile0 * kT;
+            if (kMixed && G.apply) {
+#pragma unroll
+                for (int u = 0; u < kCells; ++u) {
+                    const uint32_t c = c0 + 64 * u;
+                    if (c >= G.n) continue;
+                    const double ev = (double)ld(A.e + base, c);
+                    st(A.x + base, c, ld(A.x + base, c) + ev);
+                    st(A.e + base, c, (T)0);
+                }
+            }
+            if (G.skip_a) return;
+            const uint8_t *F = A.flags + base;
+            const T *r = A.r + base, *dv = dold + base;
             double v = 0.0;
-            if (c < G.n) {
-                const size_t base = (size_t)G.tile0 * kT;
-                const uint8_t *F = A.flags + base;
-                const double *r = A.r + base, *dv = dold + base;
-                // neighbours from the geometry (a closed side still reads a
-                // cell of the grid), so no load waits for the flags
-                const uint32_t i = grid_col(G, c);
-                const uint32_t ce = i + 1 < G.nx ? c + 1 : c, cw = i > 0 ? c - 1 : c;
-                const uint32_t cn = c + G.nx < G.n ? c + G.nx : c, cs = c >= G.nx ? c - G.nx : c;
-                const uint32_t k5[5] = {c, ce, cw, cn, cs};
-                uint8_t h[5];
-                double rv[5], dd[5], dn[5];
 #pragma unroll
-                for (int a = 0; a < 5; ++a) {
-                    h[a] = F[k5[a]];
-                    rv[a] = ld(r, k5[a]);
-                    dd[a] = ld(dv, k5[a]);
+            for (int u0 = 0; u0 < kCells; u0 += kBatchA) {
+                // loads of kBatchA cells first; padding lanes read the grid's
+                // last cell, and neighbours come from the geometry (a closed
+                // side still reads a cell of the grid), so no load waits for
+                // the flags
+                uint8_t h[kBatchA][5];
+                T rv[kBatchA][5], dd[kBatchA][5];
+#pragma unroll
+                for (int u = 0; u < kBatchA; ++u) {
+                    const uint32_t c = min(c0 + 64 * (u0 + u), G.n - 1);
+                    const uint32_t i = grid_col(G, c);
+                    const uint32_t ce = i + 1 < G.nx ? c + 1 : c, cw = i > 0 ? c - 1 : c;
+                    const uint32_t cn = c + G.nx < G.n ? c + G.nx : c, cs = c >= G.nx ? c - G.nx : c;
+                    const uint32_t k5[5] = {c, ce, cw, cn, cs};
+#pragma unroll
+                    for (int a = 0; a < 5; ++a) {
+                        h[u][a] = F[k5[a]];
+                        rv[u][a] = ld(r, k5[a]);
+                        dd[u][a] = G.first ? (T)0 : ld(dv, k5[a]);
+                    }
                 }
 #pragma unroll
-                for (int a = 0; a < 5; ++a) {
-                    const double z = (h[a] & F_ID) ? rv[a] : rv[a] * inv_diag(G.cx, G.cy, h[a]) - G.b;
-                    dn[a] = G.first ? z : z + G.a * dd[a];
+                for (int u = 0; u < kBatchA; ++u) {
+                    const uint32_t c = c0 + 64 * (u0 + u);
+                    if (c >= G.n) continue;
+                    double dn[5];
+#pragma unroll
+                    for (int a = 0; a < 5; ++a) {
+                        const double ra = (double)rv[u][a];
+                        const double z = (h[u][a] & F_ID) ? ra : ra * inv_diag(G.cx, G.cy, h[u][a]) - G.b;
+                        dn[a] = G.first ? z : z + G.a * (double)dd[u][a];
+                    }
+                    const uint8_t f = h[u][0];
+                    double qc = dn[0];
+                    if (!(f & F_ID)) {
+                        double dg = 0.0, sum = 0.0;
+                        if (f & F_E) { dg += G.cx; sum += G.cx * dn[1]; }
+                        if (f & F_W) { dg += G.cx; sum += G.cx * dn[2]; }
+                        if (f & F_N) { dg += G.cy; sum += G.cy * dn[3]; }
+                        if (f & F_S) { dg += G.cy; sum += G.cy * dn[4]; }
+                        qc = dg * dn[0] - sum;
+                    }
+                    // d'.q from the stored (for the mixed path rounded) values
+                    const T dt = (T)dn[0], qt = (T)qc;
+                    st(dnew + base, c, dt);
+                    st(A.q + base, c, qt);
+                    v += (double)dt * (double)qt;
                 }
-                const uint8_t f = h[0];
-                double qc = dn[0];
-                if (!(f & F_ID)) {
-                    double dg = 0.0, sum = 0.0;
-                    if (f & F_E) { dg += G.cx; sum += G.cx * dn[1]; }
-                    if (f & F_W) { dg += G.cx; sum += G.cx * dn[2]; }
-                    if (f & F_N) { dg += G.cy; sum += G.cy * dn[3]; }
-                    if (f & F_S) { dg += G.cy; sum += G.cy * dn[4]; }
-                    qc = dg * dn[0] - sum;
-                }
-                st(dnew + base, c, dn[0]);
-                st(A.q + base, c, qc);
-                v = dn[0] * qc;
             }
             v = wave_sum(v);
-            if (lane == 0) A.pa[s] = v;
+            if (lane == 0) A.pa[t] = v;
         });
         if (!grid_barrier(A.bar)) return;
         // phase B scalars: alpha
         fold_grids<1, 0, NT>(A.pa, gl, sh.g0, sh.g1, red,
-                             [=](uint32_t g) { return gl[g].act != 0 && gl[g].done == 0; },
+                             [=](uint32_t g) { return gl[g].act != 0 && gl[g].skip_a == 0; },
                              [=](uint32_t g, const double *sm, const double *) {
             GridL &G = gl[g];
             const double den = sm[0];
@@ -852,58 +973,103 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
                 sb.rz_prev = G.rz;
                 sb.alpha = alpha;
                 sb.first = 0;
+                sb.outer = 0;
                 sb.status = status;
                 sb.iters = G.iters + (status == ST_OK ? 1 : 0);
                 if (status != ST_OK) A.fin[par * A.nprob + g] = 1;
             }
         });
-        // phase B: x += alpha d', r -= alpha q, partials of the next test
-        for_share<kW>(gl, sh, [=](uint32_t s, uint32_t g, uint32_t c) {
+        // phase B: x (mixed: e) += alpha d', r -= alpha q, partials of the
+        // next test; the mixed path's true-residual pass r = P(b) - A (x + e)
+        for_share<kW>(gl, sh, [=](uint32_t t, uint32_t g, uint32_t c0) {
             const GridU G = grid_uni(gl[g]);
-            if (G.done) return;
+            const bool outer = kMixed && uni((uint32_t)gl[g].outer_b) != 0;
+            if (G.done || (G.skip_a && !outer)) return;
+            const size_t base = (size_t)G.tile0 * kT;
+            const uint8_t *F = A.flags + base;
             double v[4] = {0.0, 0.0, 0.0, 0.0}, mx = 0.0;
-            if (c < G.n) {
-                const size_t base = (size_t)G.tile0 * kT;
-                const uint8_t f = A.flags[base + c];
-                const double xv = ld(A.x + base, c), dv = ld(dnew + base, c);
-                const double r0 = ld(A.r + base, c), qv = ld(A.q + base, c);
-                st(A.x + base, c, xv + G.a * dv);
-                const double rc = r0 - G.a * qv;
-                st(A.r + base, c, rc);
-                if (!(f & F_ID)) {
-                    const double z = rc * inv_diag(G.cx, G.cy, f);
-                    v[0] = rc * z;
-                    v[1] = z;
-                    v[2] = rc;
-                } else {
-                    v[0] = rc * rc;
+            if (outer) {
+                const double mb = uni(gl[g].mb);
+#pragma unroll
+                for (int u = 0; u < kCells; ++u) {
+                    const uint32_t c = c0 + 64 * u;
+                    if (c >= G.n) continue;
+                    const uint8_t f = F[c];
+                    double rc = 0.0;
+                    if (!(f & F_ID)) {
+                        const uint32_t i = grid_col(G, c);
+                        const uint32_t ce = i + 1 < G.nx ? c + 1 : c, cw = i > 0 ? c - 1 : c;
+                        const uint32_t cn = c + G.nx < G.n ? c + G.nx : c, cs = c >= G.nx ? c - G.nx : c;
+                        const uint32_t k5[5] = {c, ce, cw, cn, cs};
+                        double xs[5];
+#pragma unroll
+                        for (int a = 0; a < 5; ++a)
+                            xs[a] = ld(A.x + base, k5[a]) + (double)ld(A.e + base, k5[a]);
+                        // identity rows hold their value in x and 0 in e
+                        double dg = 0.0, sum = 0.0;
+                        if (f & F_E) { dg += G.cx; sum += G.cx * xs[1]; }
+                        if (f & F_W) { dg += G.cx; sum += G.cx * xs[2]; }
+                        if (f & F_N) { dg += G.cy; sum += G.cy * xs[3]; }
+                        if (f & F_S) { dg += G.cy; sum += G.cy * xs[4]; }
+                        rc = (-A.rhs[base + c] - mb) - (dg * xs[0] - sum);
+                    }
+                    const T rt = (T)rc;
+                    st(A.r + base, c, rt);
+                    residual_partials(f, (double)rt, inv_diag(G.cx, G.cy, f), v);
+                    mx = fmax(mx, fabs(rc));
                 }
-                v[3] = rc * rc;
-                mx = fabs(rc);
+            } else {
+                uint8_t f[kCells];
+                T ev[kCells], dv[kCells], r0[kCells], qv[kCells];
+#pragma unroll
+                for (int u = 0; u < kCells; ++u) {
+                    const uint32_t c = min(c0 + 64 * u, G.n - 1);
+                    f[u] = F[c];
+                    ev[u] = ld(A.e + base, c);
+                    dv[u] = ld(dnew + base, c);
+                    r0[u] = ld(A.r + base, c);
+                    qv[u] = ld(A.q + base, c);
+                }
+#pragma unroll
+                for (int u = 0; u < kCells; ++u) {
+                    const uint32_t c = c0 + 64 * u;
+                    if (c >= G.n) continue;
+                    st(A.e + base, c, (T)((double)ev[u] + G.a * (double)dv[u]));
+                    const T rt = (T)((double)r0[u] - G.a * (double)qv[u]);
+                    st(A.r + base, c, rt);
+                    residual_partials(f[u], (double)rt, inv_diag(G.cx, G.cy, f[u]), v);
+                    mx = fmax(mx, fabs((double)rt));
+                }
             }
 #pragma unroll
             for (int a = 0; a < 4; ++a) v[a] = wave_sum(v[a]);
             mx = wave_max(mx);
             if (lane == 0) {
-                double *pb = A.pb + (size_t)s * 5;
+                double *pb = A.pb + (size_t)t * 5;
                 pb[0] = v[0]; pb[1] = v[1]; pb[2] = v[2]; pb[3] = v[3]; pb[4] = mx;
             }
         });
+        if (kMixed)  // the true-residual pass's owner marks the next test
+            for (uint32_t g = sh.g0 + threadIdx.x; g <= sh.g1; g += NT)
+                if (gl[g].act && gl[g].outer_b && gl[g].off >= sh.lo && gl[g].off < sh.hi) A.sb[g].outer = 1;
         if (!grid_barrier(A.bar)) return;
         // next loop: which grids still iterate, and their state as the
-        // blocks that held their first sub-tiles wrote it
+        // blocks that held their first tiles wrote it
         int open = 0;
         for (uint32_t g = threadIdx.x; g < nprob; g += NT) {
             GridL &G = gl[g];
             if (!G.act) continue;
             const int f = A.fin[par * A.nprob + g];
             const PcgB sb = A.sb[g];
+            const PcgA &sa = A.sa[g];
             G.act = f == 0;
             G.rz_prev = sb.rz_prev;
             G.iters = sb.iters;
             G.first = sb.first;
-            G.rr0 = A.sa[g].rr0;
-            G.done = 0;
+            G.outer = sb.outer;
+            G.rr0 = sa.rr0;
+            G.rr0o = sa.rr0o;
+            G.done = G.skip_a = G.apply = G.outer_b = 0;
             open |= G.act;
         }
         if (!__syncthreads_or(open)) break;
@@ -913,15 +1079,19 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
     for (uint32_t g = threadIdx.x; g < nprob; g += NT) gl[g].act = A.active[g] != 0;
     __syncthreads();
     const Share fin_sh = make_share(gl, nprob);
-    for_share<kW>(gl, fin_sh, [=](uint32_t s, uint32_t g, uint32_t c) {
+    for_share<kW>(gl, fin_sh, [=](uint32_t t, uint32_t g, uint32_t c0) {
         const GridL &G = gl[g];
         const size_t base = (size_t)G.tile0 * kT;
         double v = 0.0;
-        if (c < G.n && !(A.flags[base + c] & F_ID)) v = A.x[base + c];
+#pragma unroll
+        for (int u = 0; u < kCells; ++u) {
+            const uint32_t c = c0 + 64 * u;
+            if (c < G.n && !(A.flags[base + c] & F_ID)) v += A.x[base + c];
+        }
         v = wave_sum(v);
         if (lane == 0) {
-            A.p0[(size_t)s * 2] = v;
-            A.p0[(size_t)s * 2 + 1] = 0.0;
+            A.p0[(size_t)t * 2] = v;
+            A.p0[(size_t)t * 2 + 1] = 0.0;
         }
     });
     if (!grid_barrier(A.bar)) return;
@@ -930,13 +1100,18 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs A) {
                              const Prob &P = A.probs[g];
                              gl[g].b = P.zm ? sm[0] * P.inv_count : 0.0;
                          });
-    for_share<kW>(gl, fin_sh, [=](uint32_t, uint32_t g, uint32_t c) {
+    for_share<kW>(gl, fin_sh, [=](uint32_t, uint32_t g, uint32_t c0) {
         const GridL &G = gl[g];
-        if (c >= G.n) return;
-        const size_t i = (size_t)G.tile0 * kT + c;
-        const uint8_t f = A.flags[i];
-        if (f & F_ID) A.x[i] = (f & F_OUTLET) ? A.probs[g].pout : 0.0;
-        else A.x[i] -= G.b;
+        const size_t base = (size_t)G.tile0 * kT;
+#pragma unroll
+        for (int u = 0; u < kCells; ++u) {
+            const uint32_t c = c0 + 64 * u;
+            if (c >= G.n) continue;
+            const size_t i = base + c;
+            const uint8_t f = A.flags[i];
+            if (f & F_ID) A.x[i] = (f & F_OUTLET) ? A.probs[g].pout : 0.0;
+            else A.x[i] -= G.b;
+        }
     });
 }
 
@@ -1099,6 +1274,8 @@ struct Batch2 {
     std::vector<int64_t> step;
     std::vector<double> time, current_dt, max_u, max_v;
     uint32_t pcg_blocks = 1, pcg_threads = 1024;
+    bool mixed = false;       // FP32 CG cycles inside FP64 refinement
+    float *e32 = nullptr;     // mixed: the FP32 correction
     unsigned *bar = nullptr;  // the persistent PCG's grid barrier
     long long kmax = 1;
     // device-driven runs: per-grid control state, the captured step halves
@@ -1136,6 +1313,7 @@ void free_batch(Batch2 *B) {
         if (p) (void)hipHostFree(p);
     if (B->ctl) (void)hipFree(B->ctl);
     if (B->bar) (void)hipFree(B->bar);
+    if (B->e32) (void)hipFree(B->e32);
     if (B->gx_pre) (void)hipGraphExecDestroy(B->gx_pre);
     if (B->gx_post) (void)hipGraphExecDestroy(B->gx_post);
     if (B->gr_pre) (void)hipGraphDestroy(B->gr_pre);
@@ -1164,6 +1342,10 @@ fl_status validate2(fl_ctx *ctx, const fl_grid2d_config &c) {
                          "fluid density and kinematic viscosity must be finite and positive");
     if (c.convection != FL_UPWIND && c.convection != FL_CENTRAL)
         return set_error(ctx, FL_ERR_CONFIG, "unknown convection scheme %d", c.convection);
+    if (c.precision != FL_PRECISION_FP64 && c.precision != FL_PRECISION_MIXED)
+        return set_error(ctx, FL_ERR_CONFIG, "unknown precision %d", c.precision);
+    if (!(std::isfinite(c.inner_tolerance) && c.inner_tolerance >= 0.0 && c.inner_tolerance < 1.0))
+        return set_error(ctx, FL_ERR_CONFIG, "inner_tolerance must lie in [0, 1)");
     if (c.stop_rule != FL_STOP_ABS_INF && c.stop_rule != FL_STOP_REL_L2)
         return set_error(ctx, FL_ERR_CONFIG, "unknown stop rule %d", c.stop_rule);
     if (c.initial_guess != FL_GUESS_PREVIOUS && c.initial_guess != FL_GUESS_ZERO)
@@ -1219,19 +1401,22 @@ bool reached_end(const Batch2 *B, int p) {
 }
 
 // The persistent PCG's kernel for a block size.
-const void *pcg_kernel(uint32_t threads) {
+const void *pcg_kernel(uint32_t threads, bool mixed) {
     switch (threads) {
-        case 256: return (const void *)k_pcg<256>;
-        case 512: return (const void *)k_pcg<512>;
-        default: return (const void *)k_pcg<1024>;
+        case 256: return mixed ? (const void *)k_pcg<256, float> : (const void *)k_pcg<256, double>;
+        case 512: return mixed ? (const void *)k_pcg<512, float> : (const void *)k_pcg<512, double>;
+        default: return mixed ? (const void *)k_pcg<1024, float> : (const void *)k_pcg<1024, double>;
     }
 }
+// Loop bound of a grid's solve: FP64 iterations, or (mixed) CG iterations
+// plus at most as many true-residual passes.
+long long pcg_kmax(const Prob &P, bool mixed) { return mixed ? 2 * P.max_iter + 4 : P.max_iter + 2; }
 constexpr uint32_t kPcgThreads[3] = {256, 512, 1024};
 
 // Blocks of `threads` threads that fit on the device at once.
 fl_status pcg_resident(Batch2 *B, uint32_t threads, uint32_t *out) {
     int per_cu = 0;
-    FL_HIP_TRY(B->ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pcg_kernel(threads),
+    FL_HIP_TRY(B->ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pcg_kernel(threads, B->mixed),
                                                                      (int)threads, 0));
     *out = (uint32_t)std::max(per_cu, 0) * (uint32_t)B->ctx->num_cu;
     if (*out == 0)
@@ -1296,6 +1481,7 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
         P.zm = outlet ? 0 : 1;
         P.tol = c.pressure_tolerance;
         P.max_iter = c.pressure_max_iters;
+        P.eta = c.inner_tolerance > 0.0 ? c.inner_tolerance : 1.0e-5;
         P.stop_rule = c.stop_rule;
         P.guess = c.initial_guess;
         P.fx = FastDiv(P.nx);
@@ -1330,6 +1516,10 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
     // persistent-kernel grid: all blocks co-resident (cooperative launch)
     if (count > kMaxGrids)
         return set_error(ctx, FL_ERR_CONFIG, "a batch holds at most %d grids", kMaxGrids);
+    B->mixed = cfgs[0].precision == FL_PRECISION_MIXED;
+    for (int p = 1; p < count; ++p)
+        if (cfgs[p].precision != cfgs[0].precision)
+            return set_error(ctx, FL_ERR_CONFIG, "all grids of a batch share one precision");
     if ((st = dalloc(B, &B->probs, count)) || (st = dalloc(B, &B->tile_prob, B->ntiles)) ||
         (st = dalloc(B, &B->flags, B->ncell)) || (st = dalloc(B, &B->x, B->ncell)) ||
         (st = dalloc(B, &B->rhs, B->ncell)) || (st = dalloc(B, &B->r, B->ncell)) ||
@@ -1339,14 +1529,14 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
         (st = dalloc(B, &B->vort, B->ncell)) || (st = dalloc(B, &B->u, B->nuf)) ||
         (st = dalloc(B, &B->v, B->nvf)) || (st = dalloc(B, &B->us, B->nuf)) ||
         (st = dalloc(B, &B->vs, B->nvf)) || (st = dalloc(B, &B->uo, B->nuf)) ||
-        (st = dalloc(B, &B->vo, B->nvf)) || (st = dalloc(B, &B->pa, (size_t)B->ntiles * kSub)) ||
-        (st = dalloc(B, &B->pb, (size_t)B->ntiles * kSub * 5)) ||
-        (st = dalloc(B, &B->p0, (size_t)B->ntiles * kSub * 2)) ||
+        (st = dalloc(B, &B->vo, B->nvf)) || (st = dalloc(B, &B->pa, (size_t)B->ntiles)) ||
+        (st = dalloc(B, &B->pb, (size_t)B->ntiles * 5)) ||
+        (st = dalloc(B, &B->p0, (size_t)B->ntiles * 2)) ||
         (st = dalloc(B, &B->dpart, (size_t)B->ntiles * 6)) || (st = dalloc(B, &B->sa, count)) ||
         (st = dalloc(B, &B->sb, count)) || (st = dalloc(B, &B->fin, 2 * count)) ||
         (st = dalloc(B, &B->active, count)) || (st = dalloc(B, &B->dts, count)) ||
         (st = dalloc(B, &B->diag, count)) || (st = dalloc(B, &B->ctl, count)) ||
-        (st = dalloc(B, &B->bar, 128)))
+        (st = dalloc(B, &B->bar, 128)) || (st = dalloc(B, &B->e32, B->mixed ? B->ncell : 1)))
         return st;
     if ((st = halloc(B, &B->h_active, count)) || (st = halloc(B, &B->h_fin, 2 * count)) ||
         (st = halloc(B, &B->h_dts, count)) || (st = halloc(B, &B->h_sa, count)) ||
@@ -1354,7 +1544,7 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
         (st = halloc(B, &B->h_ctl, count)))
         return st;
     for (int p = 0; p < count; ++p)
-        B->kmax_all = std::max<long long>(B->kmax_all, B->hp[p].max_iter + 2);
+        B->kmax_all = std::max<long long>(B->kmax_all, pcg_kmax(B->hp[p], B->mixed));
     hipStream_t s = ctx->stream;
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->probs, B->hp.data(), count * sizeof(Prob), hipMemcpyHostToDevice, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->tile_prob, htp.data(), htp.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
@@ -1398,6 +1588,24 @@ fl_status run_cells(Batch2 *B) {
     return FL_OK;
 }
 
+// One cooperative launch of the persistent PCG on rhs / x (the batch's own
+// or scratch vectors).
+fl_status launch_pcg_coop(Batch2 *B, hipStream_t s, const double *rhs, double *x, long long kmax) {
+    void *args[1];
+    PcgArgs<double> a64{B->probs, B->tile_prob, B->ntiles, (uint32_t)B->count, kmax, B->flags, rhs,
+                        x, B->r, B->d0, B->d1, B->q, x, B->pa, B->pb, B->p0, B->sa, B->sb,
+                        B->active, B->fin, B->bar};
+    // the mixed path keeps its FP32 vectors in the FP64 vectors' memory
+    PcgArgs<float> a32{B->probs, B->tile_prob, B->ntiles, (uint32_t)B->count, kmax, B->flags, rhs,
+                       x, reinterpret_cast<float *>(B->r), reinterpret_cast<float *>(B->d0),
+                       reinterpret_cast<float *>(B->d1), reinterpret_cast<float *>(B->q), B->e32,
+                       B->pa, B->pb, B->p0, B->sa, B->sb, B->active, B->fin, B->bar};
+    args[0] = B->mixed ? (void *)&a32 : (void *)&a64;
+    FL_HIP_TRY(B->ctx, hipLaunchCooperativeKernel(pcg_kernel(B->pcg_threads, B->mixed), dim3(B->pcg_blocks),
+                                                  dim3(B->pcg_threads), args, 0, s));
+    return FL_OK;
+}
+
 // The persistent PCG over the active grids; states land in h_sa / h_sb.
 fl_status run_pcg(Batch2 *B) {
     fl_ctx *ctx = B->ctx;
@@ -1408,16 +1616,12 @@ fl_status run_pcg(Batch2 *B) {
         B->h_fin[B->count + p] = B->h_active[p] ? 0 : 1;
         B->h_sb[p] = PcgB{};
         B->h_sb[p].first = 1;
-        if (B->h_active[p]) kmax = std::max<long long>(kmax, B->hp[p].max_iter + 2);
+        if (B->h_active[p]) kmax = std::max<long long>(kmax, pcg_kmax(B->hp[p], B->mixed));
     }
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->fin, B->h_fin, 2 * B->count * sizeof(int), hipMemcpyHostToDevice, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->sb, B->h_sb, B->count * sizeof(PcgB), hipMemcpyHostToDevice, s));
-    PcgArgs a{B->probs, B->tile_prob, B->ntiles, (uint32_t)B->count, kmax, B->flags, B->rhs,
-              B->x, B->r, B->d0, B->d1, B->q, B->pa, B->pb, B->p0, B->sa, B->sb,
-              B->active, B->fin, B->bar};
-    void *args[] = {&a};
-    FL_HIP_TRY(ctx, hipLaunchCooperativeKernel(pcg_kernel(B->pcg_threads), dim3(B->pcg_blocks),
-                                               dim3(B->pcg_threads), args, 0, s));
+    fl_status st = launch_pcg_coop(B, s, B->rhs, B->x, kmax);
+    if (st != FL_OK) return st;
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->h_sa, B->sa, B->count * sizeof(PcgA), hipMemcpyDeviceToHost, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->h_sb, B->sb, B->count * sizeof(PcgB), hipMemcpyDeviceToHost, s));
     FL_HIP_TRY(ctx, hipStreamSynchronize(s));
@@ -1570,15 +1774,6 @@ fl_status launch_post(Batch2 *B, hipStream_t s) {
     k_run_finish<<<(B->count + 63) / 64, 64, 0, s>>>(B->ctl, B->count, B->active, B->dts, B->diag,
                                                      B->sa, B->sb, B->bar);
     FL_HIP_TRY(ctx, hipGetLastError());
-    return FL_OK;
-}
-fl_status launch_pcg_coop(Batch2 *B, hipStream_t s, const double *rhs, double *x, long long kmax) {
-    PcgArgs a{B->probs, B->tile_prob, B->ntiles, (uint32_t)B->count, kmax, B->flags, rhs,
-              x, B->r, B->d0, B->d1, B->q, B->pa, B->pb, B->p0, B->sa, B->sb,
-              B->active, B->fin, B->bar};
-    void *args[] = {&a};
-    FL_HIP_TRY(B->ctx, hipLaunchCooperativeKernel(pcg_kernel(B->pcg_threads), dim3(B->pcg_blocks),
-                                                  dim3(B->pcg_threads), args, 0, s));
     return FL_OK;
 }
 

@@ -62,6 +62,10 @@ class SimulationConfig:
     bc_u: tuple = (0.0, 0.0, 0.0, 1.0)
     bc_v: tuple = (0.0, 0.0, 0.0, 0.0)
     outlet_pressure: tuple = (0.0, 0.0, 0.0, 0.0)
+    # GPU path: PRECISION_FP64 (the reference's PCG) or PRECISION_MIXED
+    # (FP32 CG cycles inside FP64 iterative refinement)
+    precision: int = _lib.PRECISION_FP64
+    inner_tolerance: float = 0.0
 
     @staticmethod
     def cavity(n, reynolds=100.0, lid_velocity=1.0, **kw):
@@ -87,6 +91,7 @@ class SimulationConfig:
         c.min_dt, c.max_dt, c.target_cfl = ts.min_dt, ts.max_dt, ts.target_cfl
         c.target_diffusion_number, c.max_growth_factor = (ts.target_diffusion_number,
                                                           ts.max_growth_factor)
+        c.precision, c.inner_tolerance = self.precision, self.inner_tolerance
         return c
 
 

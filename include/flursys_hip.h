@@ -208,6 +208,15 @@ typedef struct {
     double outlet_pressure[4];     /* PressureOutlet sides */
     /* TimeStepSettings (incompressible.rs:35-56) */
     double min_dt, max_dt, target_cfl, target_diffusion_number, max_growth_factor;
+    /* GPU path: FL_PRECISION_FP64 is the reference's Jacobi-PCG;
+     * FL_PRECISION_MIXED runs FP32 Jacobi-CG cycles on the correction inside
+     * FP64 iterative refinement (x, every true residual, the dots and the
+     * stop test stay FP64).  A batch's grids share one precision. */
+    int32_t precision;
+    int32_t reserved;
+    /* FL_PRECISION_MIXED: relative residual drop that ends an inner cycle
+     * (0 selects 1e-5).  pressure_max_iters caps the inner total. */
+    double inner_tolerance;
 } fl_grid2d_config;
 
 /* SolverStep (incompressible.rs:196-214), flow-only fields. */

@@ -313,3 +313,78 @@ def test_launch_shape_validation(gpu_ctx):
     too_many = [I.SimulationConfig.cavity(8)] * 257
     with pytest.raises(fa.FlursysError, match="at most 256 grids"):
         I.CavityBatch(too_many, ctx=gpu_ctx)
+
+
+def mixed(cfg, **kw):
+    """The same case with FP32 CG cycles inside FP64 iterative refinement."""
+    cfg.precision = fa.PRECISION_MIXED
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def test_mixed_batch_meets_the_tolerance_and_matches_the_oracle(gpu_ctx):
+    """Mixed precision: every accepted pressure meets the same FP64 relative
+    residual (the stop test runs on true FP64 residuals), so the fields
+    agree with the FP64 oracle to the level that tolerance allows."""
+    print("[gpu] mixed batch vs oracle", flush=True)
+    base = dict(pressure_tolerance=1e-8, stop_rule=fa.STOP_REL_L2, initial_guess=fa.GUESS_ZERO,
+                pressure_max_iters=5000)
+    cfgs = [mixed(I.SimulationConfig.cavity(n, reynolds=re, dt=dt, **base))
+            for n, re, dt in ((16, 100.0, 2e-3), (40, 400.0, 1e-3), (27, 1000.0, 1.5e-3))]
+    b = I.CavityBatch(cfgs, ctx=gpu_ctx)
+    oracles = [oracle_for(c) for c in cfgs]
+    for _ in range(4):
+        reps = b.step()
+        for o, r in zip(oracles, reps):
+            _, orep = o.step()
+            assert r.pressure_converged
+            assert r.pressure_iterations <= 2 * orep.iterations + 10
+    for k, o in enumerate(oracles):
+        f = b.fields(k)
+        uscale = max(np.abs(o.field("u")).max(), np.abs(o.field("v")).max())
+        assert np.abs(f["u"] - o.field("u")).max() < 1e-7 * uscale
+        assert rel(f["p"], o.field("p")) < 1e-6
+
+
+def test_mixed_reference_defaults_meet_max_residual(gpu_ctx):
+    """The reference's defaults (previous pressure, max|r| < 1e-5): mixed
+    steps report a true FP64 residual under the tolerance and stay close to
+    the FP64 batch."""
+    print("[gpu] mixed batch, reference defaults", flush=True)
+    f64 = I.CavityBatch(uneven_cfgs(), ctx=gpu_ctx)
+    mix = I.CavityBatch([mixed(c) for c in uneven_cfgs()], ctx=gpu_ctx)
+    for _ in range(5):
+        r64, rmx = f64.step(), mix.step()
+        for a, m in zip(r64, rmx):
+            assert m.pressure_converged and m.pressure_residual < 1e-5
+            assert m.pressure_iterations <= 2 * a.pressure_iterations + 10
+    for fa64, fmx in zip(all_fields(f64), all_fields(mix)):
+        uscale = max(np.abs(fa64["u"]).max(), np.abs(fa64["v"]).max())
+        assert np.abs(fmx["u"] - fa64["u"]).max() < 1e-4 * uscale
+
+
+def test_mixed_launch_shapes_and_graphs_give_identical_results(gpu_ctx):
+    print("[gpu] mixed batch launch shapes and graphs", flush=True)
+    cfgs = [mixed(c) for c in uneven_cfgs()]
+    ref = I.CavityBatch(cfgs, ctx=gpu_ctx)
+    ref.run(3, use_graphs=False)
+    for threads, blocks, graphs in ((256, 5, True), (1024, 2, False)):
+        b = I.CavityBatch(cfgs, ctx=gpu_ctx)
+        b.set_launch(threads, blocks)
+        b.run(3, use_graphs=graphs)
+        for fr, fb in zip(all_fields(ref), all_fields(b)):
+            assert np.array_equal(fr["p"], fb["p"])
+        b.close()
+
+
+def test_precision_validation(gpu_ctx):
+    bad = I.SimulationConfig.cavity(16)
+    bad.precision = 7
+    with pytest.raises(fa.FlursysError, match="unknown precision"):
+        I.CavityBatch([bad], ctx=gpu_ctx)
+    with pytest.raises(fa.FlursysError, match="share one precision"):
+        I.CavityBatch([I.SimulationConfig.cavity(16), mixed(I.SimulationConfig.cavity(16))],
+                      ctx=gpu_ctx)
+    with pytest.raises(fa.FlursysError, match="inner_tolerance"):
+        I.CavityBatch([mixed(I.SimulationConfig.cavity(16), inner_tolerance=2.0)], ctx=gpu_ctx)
