@@ -417,7 +417,7 @@ struct PcgArgs {
     const uint8_t *flags;
     const double *rhs;
     double *x;
-    T *r, *d0, *d1, *q, *e;  // FP64: e is x
+    T *r, *z, *d0, *d1, *q, *e;  // z = D^-1 r (identity rows: r); FP64: e is x
     double *pa, *pb, *p0;  // per-tile partial rows: 1, 5 and 2 columns
     PcgA *sa;
     PcgB *sb;
@@ -497,10 +497,12 @@ struct GridL {
 };
 
 // Adds one cell's residual to the stop test's partial sums: r.z (r.r on
-// identity rows), z, r (both on the gauge cells only) and r.r.
-__device__ inline void residual_partials(uint8_t f, double r, double inv_d, double (&v)[4]) {
+// identity rows), z, r (both on the gauge cells only) and r.r.  Returns the
+// cell's Jacobi-preconditioned residual z = D^-1 r (identity rows: r).
+__device__ inline double residual_partials(uint8_t f, double r, double inv_d, double (&v)[4]) {
+    double z = r;
     if (!(f & F_ID)) {
-        const double z = r * inv_d;
+        z = r * inv_d;
         v[0] += r * z;
         v[1] += z;
         v[2] += r;
@@ -508,6 +510,7 @@ __device__ inline void residual_partials(uint8_t f, double r, double inv_d, doub
         v[0] += r * r;
     }
     v[3] += r * r;
+    return z;
 }
 
 // Wave-uniform copies (scalar registers) of values read from LDS.
@@ -668,7 +671,7 @@ template <int NT, typename T>
 __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
     constexpr bool kMixed = !std::is_same<T, double>::value;
     // cells per lane whose phase-A loads are in flight together
-    constexpr int kBatchA = 2;
+    constexpr int kBatchA = kMixed ? 4 : 2;
     constexpr uint32_t kW = NT / 64;
     __shared__ GridL gl[kMaxGrids];
     __shared__ double red[kW][8];
@@ -761,7 +764,7 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
             const T rt = (T)rc;
             A.r[base + c] = rt;
             if (kMixed) A.e[base + c] = (T)0;
-            residual_partials(f, (double)rt, inv_diag(G.cx, G.cy, f), v);
+            A.z[base + c] = (T)residual_partials(f, (double)rt, inv_diag(G.cx, G.cy, f), v);
             mx = fmax(mx, fabs(rc));
         }
 #pragma unroll
@@ -897,7 +900,7 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
             }
             if (G.skip_a) return;
             const uint8_t *F = A.flags + base;
-            const T *r = A.r + base, *dv = dold + base;
+            const T *z = A.z + base, *dv = dold + base;
             double v = 0.0;
 #pragma unroll
             for (int u0 = 0; u0 < kCells; u0 += kBatchA) {
@@ -905,8 +908,8 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
                 // last cell, and neighbours come from the geometry (a closed
                 // side still reads a cell of the grid), so no load waits for
                 // the flags
-                uint8_t h[kBatchA][5];
-                T rv[kBatchA][5], dd[kBatchA][5];
+                uint8_t f[kBatchA];
+                T zv[kBatchA][5], dd[kBatchA][5];
 #pragma unroll
                 for (int u = 0; u < kBatchA; ++u) {
                     const uint32_t c = min(c0 + 64 * (u0 + u), G.n - 1);
@@ -914,10 +917,10 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
                     const uint32_t ce = i + 1 < G.nx ? c + 1 : c, cw = i > 0 ? c - 1 : c;
                     const uint32_t cn = c + G.nx < G.n ? c + G.nx : c, cs = c >= G.nx ? c - G.nx : c;
                     const uint32_t k5[5] = {c, ce, cw, cn, cs};
+                    f[u] = F[c];
 #pragma unroll
                     for (int a = 0; a < 5; ++a) {
-                        h[u][a] = F[k5[a]];
-                        rv[u][a] = ld(r, k5[a]);
+                        zv[u][a] = ld(z, k5[a]);
                         dd[u][a] = G.first ? (T)0 : ld(dv, k5[a]);
                     }
                 }
@@ -925,25 +928,27 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
                 for (int u = 0; u < kBatchA; ++u) {
                     const uint32_t c = c0 + 64 * (u0 + u);
                     if (c >= G.n) continue;
+                    // the projection P(z) = z - mean(z) on the gauge cells is
+                    // applied to the stored d' only: A annihilates the
+                    // constant on the gauge cells (their open neighbours are
+                    // gauge cells, identity rows only exist without a gauge)
                     double dn[5];
 #pragma unroll
-                    for (int a = 0; a < 5; ++a) {
-                        const double ra = (double)rv[u][a];
-                        const double z = (h[u][a] & F_ID) ? ra : ra * inv_diag(G.cx, G.cy, h[u][a]) - G.b;
-                        dn[a] = G.first ? z : z + G.a * (double)dd[u][a];
-                    }
-                    const uint8_t f = h[u][0];
-                    double qc = dn[0];
-                    if (!(f & F_ID)) {
+                    for (int a = 0; a < 5; ++a)
+                        dn[a] = G.first ? (double)zv[u][a] : (double)zv[u][a] + G.a * (double)dd[u][a];
+                    double qc = dn[0], dc = dn[0];
+                    if (!(f[u] & F_ID)) {
                         double dg = 0.0, sum = 0.0;
-                        if (f & F_E) { dg += G.cx; sum += G.cx * dn[1]; }
-                        if (f & F_W) { dg += G.cx; sum += G.cx * dn[2]; }
-                        if (f & F_N) { dg += G.cy; sum += G.cy * dn[3]; }
-                        if (f & F_S) { dg += G.cy; sum += G.cy * dn[4]; }
+                        if (f[u] & F_E) { dg += G.cx; sum += G.cx * dn[1]; }
+                        if (f[u] & F_W) { dg += G.cx; sum += G.cx * dn[2]; }
+                        if (f[u] & F_N) { dg += G.cy; sum += G.cy * dn[3]; }
+                        if (f[u] & F_S) { dg += G.cy; sum += G.cy * dn[4]; }
                         qc = dg * dn[0] - sum;
+                        const double zs = (double)zv[u][0] - G.b;
+                        dc = G.first ? zs : zs + G.a * (double)dd[u][0];
                     }
                     // d'.q from the stored (for the mixed path rounded) values
-                    const T dt = (T)dn[0], qt = (T)qc;
+                    const T dt = (T)dc, qt = (T)qc;
                     st(dnew + base, c, dt);
                     st(A.q + base, c, qt);
                     v += (double)dt * (double)qt;
@@ -1015,7 +1020,7 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
                     }
                     const T rt = (T)rc;
                     st(A.r + base, c, rt);
-                    residual_partials(f, (double)rt, inv_diag(G.cx, G.cy, f), v);
+                    st(A.z + base, c, (T)residual_partials(f, (double)rt, inv_diag(G.cx, G.cy, f), v));
                     mx = fmax(mx, fabs(rc));
                 }
             } else {
@@ -1037,7 +1042,7 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
                     st(A.e + base, c, (T)((double)ev[u] + G.a * (double)dv[u]));
                     const T rt = (T)((double)r0[u] - G.a * (double)qv[u]);
                     st(A.r + base, c, rt);
-                    residual_partials(f[u], (double)rt, inv_diag(G.cx, G.cy, f[u]), v);
+                    st(A.z + base, c, (T)residual_partials(f[u], (double)rt, inv_diag(G.cx, G.cy, f[u]), v));
                     mx = fmax(mx, fabs((double)rt));
                 }
             }
@@ -1256,6 +1261,7 @@ struct Batch2 {
     uint32_t *tile_prob = nullptr;
     uint8_t *flags = nullptr;
     double *x = nullptr, *rhs = nullptr, *r = nullptr, *d0 = nullptr, *d1 = nullptr, *q = nullptr;
+    double *zv = nullptr;  // Jacobi-preconditioned residual
     double *uc = nullptr, *vc = nullptr, *sp = nullptr, *vort = nullptr;
     double *u = nullptr, *v = nullptr, *us = nullptr, *vs = nullptr, *uo = nullptr, *vo = nullptr;
     double *pa = nullptr, *pb = nullptr, *p0 = nullptr, *dpart = nullptr;
@@ -1314,6 +1320,7 @@ void free_batch(Batch2 *B) {
     if (B->ctl) (void)hipFree(B->ctl);
     if (B->bar) (void)hipFree(B->bar);
     if (B->e32) (void)hipFree(B->e32);
+    if (B->zv) (void)hipFree(B->zv);
     if (B->gx_pre) (void)hipGraphExecDestroy(B->gx_pre);
     if (B->gx_post) (void)hipGraphExecDestroy(B->gx_post);
     if (B->gr_pre) (void)hipGraphDestroy(B->gr_pre);
@@ -1524,6 +1531,7 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
         (st = dalloc(B, &B->flags, B->ncell)) || (st = dalloc(B, &B->x, B->ncell)) ||
         (st = dalloc(B, &B->rhs, B->ncell)) || (st = dalloc(B, &B->r, B->ncell)) ||
         (st = dalloc(B, &B->d0, B->ncell)) || (st = dalloc(B, &B->d1, B->ncell)) ||
+        (st = dalloc(B, &B->zv, B->ncell)) ||
         (st = dalloc(B, &B->q, B->ncell)) || (st = dalloc(B, &B->uc, B->ncell)) ||
         (st = dalloc(B, &B->vc, B->ncell)) || (st = dalloc(B, &B->sp, B->ncell)) ||
         (st = dalloc(B, &B->vort, B->ncell)) || (st = dalloc(B, &B->u, B->nuf)) ||
@@ -1593,11 +1601,12 @@ fl_status run_cells(Batch2 *B) {
 fl_status launch_pcg_coop(Batch2 *B, hipStream_t s, const double *rhs, double *x, long long kmax) {
     void *args[1];
     PcgArgs<double> a64{B->probs, B->tile_prob, B->ntiles, (uint32_t)B->count, kmax, B->flags, rhs,
-                        x, B->r, B->d0, B->d1, B->q, x, B->pa, B->pb, B->p0, B->sa, B->sb,
+                        x, B->r, B->zv, B->d0, B->d1, B->q, x, B->pa, B->pb, B->p0, B->sa, B->sb,
                         B->active, B->fin, B->bar};
     // the mixed path keeps its FP32 vectors in the FP64 vectors' memory
     PcgArgs<float> a32{B->probs, B->tile_prob, B->ntiles, (uint32_t)B->count, kmax, B->flags, rhs,
-                       x, reinterpret_cast<float *>(B->r), reinterpret_cast<float *>(B->d0),
+                       x, reinterpret_cast<float *>(B->r), reinterpret_cast<float *>(B->zv),
+                       reinterpret_cast<float *>(B->d0),
                        reinterpret_cast<float *>(B->d1), reinterpret_cast<float *>(B->q), B->e32,
                        B->pa, B->pb, B->p0, B->sa, B->sb, B->active, B->fin, B->bar};
     args[0] = B->mixed ? (void *)&a32 : (void *)&a64;
