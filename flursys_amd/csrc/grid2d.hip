@@ -488,7 +488,9 @@ struct GridL {
     double a, b;  // phase A: beta, mean of z; phase B: alpha; setup: mean of b / x
     double rz, rz_prev, rr0;
     double rr0o, mb;  // r.r of the first true residual; mean of b
-    long long iters;
+    double tol, eta, inv_count;  // stop test (copied from Prob)
+    long long iters, max_iter;
+    int32_t stop_rule, zm;
     int32_t done, first;
     int32_t outer;    // mixed: the partials come from a true residual
     int32_t skip_a;   // no CG work in phase A
@@ -667,6 +669,71 @@ __device__ inline void fold_grids(const double *__restrict__ part, const GridL *
     }
 }
 
+// Phase A over one tile (this lane's cells c0, c0 + 64, ...): the search
+// direction d' = P(z) + beta d and q = A d'; returns the lane's sum of d'.q.
+// kFirst: d' = P(z) (no previous direction is read).
+template <bool kFirst, int kBatchA, typename T>
+__device__ inline double search_dir(const PcgArgs<T> &A, const GridU &G, const T *dold, T *dnew,
+                                    uint32_t c0) {
+    const size_t base = (size_t)G.tile0 * kT;
+    const uint8_t *F = A.flags + base;
+    const T *z = A.z + base, *dv = dold + base;
+    double v = 0.0;
+#pragma unroll
+    for (int u0 = 0; u0 < kCells; u0 += kBatchA) {
+        // loads of kBatchA cells first; padding lanes read the grid's
+        // last cell, and neighbours come from the geometry (a closed
+        // side still reads a cell of the grid), so no load waits for
+        // the flags
+        uint8_t f[kBatchA];
+        T zv[kBatchA][5], dd[kBatchA][kFirst ? 1 : 5];
+#pragma unroll
+        for (int u = 0; u < kBatchA; ++u) {
+            const uint32_t c = min(c0 + 64 * (u0 + u), G.n - 1);
+            const uint32_t i = grid_col(G, c);
+            const uint32_t ce = i + 1 < G.nx ? c + 1 : c, cw = i > 0 ? c - 1 : c;
+            const uint32_t cn = c + G.nx < G.n ? c + G.nx : c, cs = c >= G.nx ? c - G.nx : c;
+            const uint32_t k5[5] = {c, ce, cw, cn, cs};
+            f[u] = F[c];
+#pragma unroll
+            for (int a = 0; a < 5; ++a) {
+                zv[u][a] = ld(z, k5[a]);
+                if (!kFirst) dd[u][a] = ld(dv, k5[a]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBatchA; ++u) {
+            const uint32_t c = c0 + 64 * (u0 + u);
+            if (c >= G.n) continue;
+            // the projection P(z) = z - mean(z) on the gauge cells is
+            // applied to the stored d' only: A annihilates the
+            // constant on the gauge cells (their open neighbours are
+            // gauge cells, identity rows only exist without a gauge)
+            double dn[5];
+#pragma unroll
+            for (int a = 0; a < 5; ++a)
+                dn[a] = kFirst ? (double)zv[u][a] : (double)zv[u][a] + G.a * (double)dd[u][kFirst ? 0 : a];
+            double qc = dn[0], dc = dn[0];
+            if (!(f[u] & F_ID)) {
+                double dg = 0.0, sum = 0.0;
+                if (f[u] & F_E) { dg += G.cx; sum += G.cx * dn[1]; }
+                if (f[u] & F_W) { dg += G.cx; sum += G.cx * dn[2]; }
+                if (f[u] & F_N) { dg += G.cy; sum += G.cy * dn[3]; }
+                if (f[u] & F_S) { dg += G.cy; sum += G.cy * dn[4]; }
+                qc = dg * dn[0] - sum;
+                const double zs = (double)zv[u][0] - G.b;
+                dc = kFirst ? zs : zs + G.a * (double)dd[u][0];
+            }
+            // d'.q from the stored (for the mixed path rounded) values
+            const T dt = (T)dc, qt = (T)qc;
+            st(dnew + base, c, dt);
+            st(A.q + base, c, qt);
+            v += (double)dt * (double)qt;
+        }
+    }
+    return v;
+}
+
 template <int NT, typename T>
 __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
     constexpr bool kMixed = !std::is_same<T, double>::value;
@@ -689,6 +756,12 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
         G.act = A.active[g] != 0;
         G.cx = P.cx;
         G.cy = P.cy;
+        G.tol = P.tol;
+        G.eta = P.eta;
+        G.inv_count = P.inv_count;
+        G.max_iter = P.max_iter;
+        G.stop_rule = P.stop_rule;
+        G.zm = P.zm;
         G.rz_prev = sb.rz_prev;
         G.iters = sb.iters;
         G.first = sb.first;
@@ -803,11 +876,10 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
         // decides between a CG step and a true-residual pass
         fold_grids<4, 1, NT>(A.pb, gl, sh.g0, sh.g1, red, [=](uint32_t g) { return gl[g].act != 0; },
                              [=](uint32_t g, const double *sm, const double *mx) {
-            const Prob &P = A.probs[g];
             GridL &G = gl[g];
             const double rr = sm[3], rmax = mx[0];
-            const double rz = P.zm ? sm[0] - sm[2] * sm[1] * P.inv_count : sm[0];
-            const double mz = P.zm ? sm[1] * P.inv_count : 0.0;
+            const double rz = G.zm ? sm[0] - sm[2] * sm[1] * G.inv_count : sm[0];
+            const double mz = G.zm ? sm[1] * G.inv_count : 0.0;
             int status = ST_OK, conv = 0;
             bool done = false, skip_a = false, outer_b = false, first = G.first != 0;
             double beta = 0.0, rr0 = G.rr0, rr0o = G.rr0o;
@@ -817,13 +889,13 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
                 if (!first && !isfinite(rmax)) {
                     status = ST_RESIDUAL;
                     done = true;
-                } else if (P.stop_rule == FL_STOP_REL_L2 ? sqrt(rr) <= P.tol * sqrt(rr0)
-                                                        : rmax < P.tol) {
+                } else if (G.stop_rule == FL_STOP_REL_L2 ? sqrt(rr) <= G.tol * sqrt(rr0)
+                                                        : rmax < G.tol) {
                     conv = 1;
                     done = true;
                 } else if (!first) {
                     if (fabs(G.rz_prev) < 1.0e-30) done = true;
-                    else if (G.iters >= P.max_iter) done = true;
+                    else if (G.iters >= G.max_iter) done = true;
                     else beta = rz / G.rz_prev;
                 }
             } else if (G.outer) {
@@ -833,11 +905,11 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
                 if (!isfinite(rmax)) {
                     status = ST_RESIDUAL;
                     done = true;
-                } else if (P.stop_rule == FL_STOP_REL_L2 ? sqrt(rr) <= P.tol * sqrt(rr0o)
-                                                        : rmax < P.tol) {
+                } else if (G.stop_rule == FL_STOP_REL_L2 ? sqrt(rr) <= G.tol * sqrt(rr0o)
+                                                        : rmax < G.tol) {
                     conv = 1;
                     done = true;
-                } else if (G.iters >= P.max_iter) {
+                } else if (G.iters >= G.max_iter) {
                     done = true;
                 } else {
                     first = true;
@@ -847,10 +919,10 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
                 // inner CG: end the cycle (true-residual pass next) when the
                 // recurrence residual meets half the outer tolerance, has
                 // dropped by the inner tolerance, or the budget is spent
-                const bool met = P.stop_rule == FL_STOP_REL_L2 ? sqrt(rr) <= 0.5 * P.tol * sqrt(rr0o)
-                                                               : rmax <= 0.5 * P.tol;
-                if (!isfinite(rmax) || met || sqrt(rr) <= P.eta * sqrt(rr0) ||
-                    G.iters >= P.max_iter || fabs(G.rz_prev) < 1.0e-30) {
+                const bool met = G.stop_rule == FL_STOP_REL_L2 ? sqrt(rr) <= 0.5 * G.tol * sqrt(rr0o)
+                                                               : rmax <= 0.5 * G.tol;
+                if (!isfinite(rmax) || met || sqrt(rr) <= G.eta * sqrt(rr0) ||
+                    G.iters >= G.max_iter || fabs(G.rz_prev) < 1.0e-30) {
                     skip_a = true;
                     outer_b = true;
                 } else {
@@ -899,63 +971,10 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
                 }
             }
             if (G.skip_a) return;
-            const uint8_t *F = A.flags + base;
-            const T *z = A.z + base, *dv = dold + base;
-            double v = 0.0;
-#pragma unroll
-            for (int u0 = 0; u0 < kCells; u0 += kBatchA) {
-                // loads of kBatchA cells first; padding lanes read the grid's
-                // last cell, and neighbours come from the geometry (a closed
-                // side still reads a cell of the grid), so no load waits for
-                // the flags
-                uint8_t f[kBatchA];
-                T zv[kBatchA][5], dd[kBatchA][5];
-#pragma unroll
-                for (int u = 0; u < kBatchA; ++u) {
-                    const uint32_t c = min(c0 + 64 * (u0 + u), G.n - 1);
-                    const uint32_t i = grid_col(G, c);
-                    const uint32_t ce = i + 1 < G.nx ? c + 1 : c, cw = i > 0 ? c - 1 : c;
-                    const uint32_t cn = c + G.nx < G.n ? c + G.nx : c, cs = c >= G.nx ? c - G.nx : c;
-                    const uint32_t k5[5] = {c, ce, cw, cn, cs};
-                    f[u] = F[c];
-#pragma unroll
-                    for (int a = 0; a < 5; ++a) {
-                        zv[u][a] = ld(z, k5[a]);
-                        dd[u][a] = G.first ? (T)0 : ld(dv, k5[a]);
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < kBatchA; ++u) {
-                    const uint32_t c = c0 + 64 * (u0 + u);
-                    if (c >= G.n) continue;
-                    // the projection P(z) = z - mean(z) on the gauge cells is
-                    // applied to the stored d' only: A annihilates the
-                    // constant on the gauge cells (their open neighbours are
-                    // gauge cells, identity rows only exist without a gauge)
-                    double dn[5];
-#pragma unroll
-                    for (int a = 0; a < 5; ++a)
-                        dn[a] = G.first ? (double)zv[u][a] : (double)zv[u][a] + G.a * (double)dd[u][a];
-                    double qc = dn[0], dc = dn[0];
-                    if (!(f[u] & F_ID)) {
-                        double dg = 0.0, sum = 0.0;
-                        if (f[u] & F_E) { dg += G.cx; sum += G.cx * dn[1]; }
-                        if (f[u] & F_W) { dg += G.cx; sum += G.cx * dn[2]; }
-                        if (f[u] & F_N) { dg += G.cy; sum += G.cy * dn[3]; }
-                        if (f[u] & F_S) { dg += G.cy; sum += G.cy * dn[4]; }
-                        qc = dg * dn[0] - sum;
-                        const double zs = (double)zv[u][0] - G.b;
-                        dc = G.first ? zs : zs + G.a * (double)dd[u][0];
-                    }
-                    // d'.q from the stored (for the mixed path rounded) values
-                    const T dt = (T)dc, qt = (T)qc;
-                    st(dnew + base, c, dt);
-                    st(A.q + base, c, qt);
-                    v += (double)dt * (double)qt;
-                }
-            }
-            v = wave_sum(v);
-            if (lane == 0) A.pa[t] = v;
+            const double v = G.first ? search_dir<true, kBatchA>(A, G, dold, dnew, c0)
+                                     : search_dir<false, kBatchA>(A, G, dold, dnew, c0);
+            const double vs = wave_sum(v);
+            if (lane == 0) A.pa[t] = vs;
         });
         if (!grid_barrier(A.bar)) return;
         // phase B scalars: alpha
