@@ -207,9 +207,12 @@ def run_batch2d(fa, ctx, args):
                 f"{iters / (len(cfgs) * (total - left)):.1f} PCG iterations per cavity-step, "
                 f"{np.mean(ms):.2f} ms/step")
         elapsed = time.perf_counter() - t0
+        gmode = b.graph_mode()
         b.close()
         out[mode] = {"elapsed_s": elapsed, "steps": total, "step_ms": ms,
-                     "pressure_iterations": iters}
+                     "pressure_iterations": iters,
+                     "graph_mode": {0: "none", 1: "whole step (cooperative PCG kernel node)",
+                                    2: "split (graphs around a cooperative launch)"}[gmode]}
     g = out["graphs"]
     ms = np.array(g["step_ms"])
     steps = args.batch_steps
@@ -236,6 +239,7 @@ def run_batch2d(fa, ctx, args):
                                 "p99": float(np.percentile(ms, 99)),
                                 "max": float(ms.max())},
             "pcg_iterations_per_cavity_step": g["pressure_iterations"] / (len(cfgs) * steps),
+            "graph_mode": g["graph_mode"],
             "autotuned_pcg_launch": chosen,
             "autotune_solve_ms": {f"{t['threads']}x{t['blocks']}": round(t["ms"], 3)
                                   for t in table},

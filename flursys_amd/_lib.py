@@ -78,6 +78,9 @@ class Batch2DRunReport(C.Structure):
                 ("status", C.c_int32)]
 
 
+GRAPH_NONE, GRAPH_WHOLE_STEP, GRAPH_SPLIT = 0, 1, 2
+
+
 class Batch2DTuning(C.Structure):
     _fields_ = [("threads", C.c_int32), ("blocks", C.c_int32), ("reserved", C.c_int32 * 2),
                 ("ms", C.c_double)]
@@ -171,6 +174,7 @@ SIGNATURES = {
     "fl_batch2d_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(Batch2DTuning), C.c_int32,
                                       C.POINTER(Batch2DTuning), C.POINTER(C.c_int32)]),
     "fl_batch2d_set_launch": (C.c_int, [_P, C.POINTER(Batch2DTuning)]),
+    "fl_batch2d_graph_mode": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "fl_batch2d_get_launch": (C.c_int, [_P, C.POINTER(Batch2DTuning)]),
     "fl_csr_upload": (C.c_int,[_P, C.c_int64, C.c_int64, _P, _P, _P, C.c_int64,
                                 C.POINTER(_P), C.POINTER(ErrDetail)]),

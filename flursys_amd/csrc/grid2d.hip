@@ -1303,12 +1303,14 @@ struct Batch2 {
     float *e32 = nullptr;     // mixed: the FP32 correction
     unsigned *bar = nullptr;  // the persistent PCG's grid barrier
     long long kmax = 1;
-    // device-driven runs: per-grid control state, the captured step halves
-    // around the cooperative PCG launch, per-step events
+    // device-driven runs: per-grid control state, the captured step (one
+    // graph holding the cooperative PCG as a kernel node, or, where the
+    // runtime cannot capture that, the two halves around it), per-step events
     StepCtl *ctl = nullptr, *h_ctl = nullptr;
     long long kmax_all = 1;
-    hipGraph_t gr_pre = nullptr, gr_post = nullptr;
-    hipGraphExec_t gx_pre = nullptr, gx_post = nullptr;
+    int32_t graph_mode = 0;  // FL_GRAPH_* of the captured step
+    hipGraph_t gr_step = nullptr, gr_pre = nullptr, gr_post = nullptr;
+    hipGraphExec_t gx_step = nullptr, gx_pre = nullptr, gx_post = nullptr;
     std::vector<hipEvent_t> ev;
 };
 
@@ -1326,6 +1328,23 @@ fl_status halloc(Batch2 *B, T **p, size_t n) {
     return FL_OK;
 }
 
+// Releases the captured step graphs (a launch-shape change re-captures).
+void drop_graphs(Batch2 *B) {
+    hipGraphExec_t *x[] = {&B->gx_step, &B->gx_pre, &B->gx_post};
+    hipGraph_t *g[] = {&B->gr_step, &B->gr_pre, &B->gr_post};
+    for (hipGraphExec_t *e : x)
+        if (*e) {
+            (void)hipGraphExecDestroy(*e);
+            *e = nullptr;
+        }
+    for (hipGraph_t *e : g)
+        if (*e) {
+            (void)hipGraphDestroy(*e);
+            *e = nullptr;
+        }
+    B->graph_mode = FL_GRAPH_NONE;
+}
+
 void free_batch(Batch2 *B) {
     void *d[] = {B->probs, B->tile_prob, B->flags, B->x,  B->rhs, B->r,  B->d0,    B->d1,
                  B->q,     B->uc,        B->vc,    B->sp, B->vort, B->u, B->v,     B->us,
@@ -1340,10 +1359,7 @@ void free_batch(Batch2 *B) {
     if (B->bar) (void)hipFree(B->bar);
     if (B->e32) (void)hipFree(B->e32);
     if (B->zv) (void)hipFree(B->zv);
-    if (B->gx_pre) (void)hipGraphExecDestroy(B->gx_pre);
-    if (B->gx_post) (void)hipGraphExecDestroy(B->gx_post);
-    if (B->gr_pre) (void)hipGraphDestroy(B->gr_pre);
-    if (B->gr_post) (void)hipGraphDestroy(B->gr_post);
+    drop_graphs(B);
     for (hipEvent_t e : B->ev) (void)hipEventDestroy(e);
 }
 
@@ -1458,7 +1474,9 @@ fl_status set_pcg_blocks(Batch2 *B, uint32_t G, uint32_t threads) {
     uint32_t resident = 0;
     fl_status st = pcg_resident(B, threads, &resident);
     if (st != FL_OK) return st;
-    B->pcg_blocks = std::max<uint32_t>(1, std::min(G, std::min(resident, B->ntiles)));
+    G = std::max<uint32_t>(1, std::min(G, std::min(resident, B->ntiles)));
+    if (G != B->pcg_blocks || threads != B->pcg_threads) drop_graphs(B);
+    B->pcg_blocks = G;
     B->pcg_threads = threads;
     return FL_OK;
 }
@@ -1617,7 +1635,8 @@ fl_status run_cells(Batch2 *B) {
 
 // One cooperative launch of the persistent PCG on rhs / x (the batch's own
 // or scratch vectors).
-fl_status launch_pcg_coop(Batch2 *B, hipStream_t s, const double *rhs, double *x, long long kmax) {
+fl_status launch_pcg_coop(Batch2 *B, hipStream_t s, const double *rhs, double *x, long long kmax,
+                          bool capturable = false) {
     void *args[1];
     PcgArgs<double> a64{B->probs, B->tile_prob, B->ntiles, (uint32_t)B->count, kmax, B->flags, rhs,
                         x, B->r, B->zv, B->d0, B->d1, B->q, x, B->pa, B->pb, B->p0, B->sa, B->sb,
@@ -1629,6 +1648,22 @@ fl_status launch_pcg_coop(Batch2 *B, hipStream_t s, const double *rhs, double *x
                        reinterpret_cast<float *>(B->d1), reinterpret_cast<float *>(B->q), B->e32,
                        B->pa, B->pb, B->p0, B->sa, B->sb, B->active, B->fin, B->bar};
     args[0] = B->mixed ? (void *)&a32 : (void *)&a64;
+    if (capturable) {
+        // the extensible launch with the cooperative attribute, which a
+        // stream capture records as a cooperative kernel node
+        hipLaunchAttribute attr{};
+        attr.id = hipLaunchAttributeCooperative;
+        attr.val.cooperative = 1;
+        hipLaunchConfig_t cfg{};
+        cfg.gridDim = dim3(B->pcg_blocks);
+        cfg.blockDim = dim3(B->pcg_threads);
+        cfg.dynamicSmemBytes = 0;
+        cfg.stream = s;
+        cfg.attrs = &attr;
+        cfg.numAttrs = 1;
+        FL_HIP_TRY(B->ctx, hipLaunchKernelExC(&cfg, pcg_kernel(B->pcg_threads, B->mixed), args));
+        return FL_OK;
+    }
     FL_HIP_TRY(B->ctx, hipLaunchCooperativeKernel(pcg_kernel(B->pcg_threads, B->mixed), dim3(B->pcg_blocks),
                                                   dim3(B->pcg_threads), args, 0, s));
     return FL_OK;
@@ -1805,21 +1840,43 @@ fl_status launch_post(Batch2 *B, hipStream_t s) {
     return FL_OK;
 }
 
-fl_status capture(Batch2 *B, bool pre, hipGraph_t *g, hipGraphExec_t *x) {
+// Captures part of a step: 0 the part before the PCG, 1 the part after it,
+// 2 the whole step with the PCG as a cooperative kernel node.
+fl_status capture(Batch2 *B, int part, hipGraph_t *g, hipGraphExec_t *x) {
     fl_ctx *ctx = B->ctx;
     hipStream_t s = ctx->stream;
     FL_HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    fl_status st = pre ? launch_pre(B, s) : launch_post(B, s);
+    fl_status st = FL_OK;
+    if (part != 1) st = launch_pre(B, s);
+    if (st == FL_OK && part == 2) st = launch_pcg_coop(B, s, B->rhs, B->x, B->kmax_all, true);
+    if (st == FL_OK && part != 0) st = launch_post(B, s);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (st == FL_OK && e != hipSuccess)
+        st = set_error(ctx, FL_ERR_HIP, "stream capture failed: %s", hipGetErrorString(e));
+    if (st == FL_OK && hipGraphInstantiate(x, graph, nullptr, nullptr, 0) != hipSuccess)
+        st = set_error(ctx, FL_ERR_HIP, "graph instantiation failed");
     if (st != FL_OK) {
         if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
         return st;
     }
-    if (e != hipSuccess)
-        return set_error(ctx, FL_ERR_HIP, "stream capture failed: %s", hipGetErrorString(e));
-    FL_HIP_TRY(ctx, hipGraphInstantiate(x, graph, nullptr, nullptr, 0));
     *g = graph;
+    return FL_OK;
+}
+
+// The step graph: one graph for the whole step where the runtime records the
+// cooperative launch, else graphs of the two halves around it.
+fl_status capture_step(Batch2 *B) {
+    if (B->graph_mode != FL_GRAPH_NONE) return FL_OK;
+    if (capture(B, 2, &B->gr_step, &B->gx_step) == FL_OK) {
+        B->graph_mode = FL_GRAPH_WHOLE_STEP;
+        return FL_OK;
+    }
+    fl_status st;
+    if ((st = capture(B, 0, &B->gr_pre, &B->gx_pre)) != FL_OK) return st;
+    if ((st = capture(B, 1, &B->gr_post, &B->gx_post)) != FL_OK) return st;
+    B->graph_mode = FL_GRAPH_SPLIT;
     return FL_OK;
 }
 
@@ -1848,10 +1905,7 @@ fl_status run_device(Batch2 *B, int64_t steps, bool graph, fl_batch2d_run_report
     }
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->ctl, B->h_ctl, B->count * sizeof(StepCtl), hipMemcpyHostToDevice, s));
     fl_status st;
-    if (graph && !B->gx_pre) {
-        if ((st = capture(B, true, &B->gr_pre, &B->gx_pre)) != FL_OK) return st;
-        if ((st = capture(B, false, &B->gr_post, &B->gx_post)) != FL_OK) return st;
-    }
+    if (graph && (st = capture_step(B)) != FL_OK) return st;
     if (step_ms)
         while ((int64_t)B->ev.size() < 2 * steps) {
             hipEvent_t e;
@@ -1860,11 +1914,15 @@ fl_status run_device(Batch2 *B, int64_t steps, bool graph, fl_batch2d_run_report
         }
     for (int64_t k = 0; k < steps; ++k) {
         if (step_ms) FL_HIP_TRY(ctx, hipEventRecord(B->ev[2 * k], s));
-        if (graph) FL_HIP_TRY(ctx, hipGraphLaunch(B->gx_pre, s));
-        else if ((st = launch_pre(B, s)) != FL_OK) return st;
-        if ((st = launch_pcg_coop(B, s, B->rhs, B->x, B->kmax_all)) != FL_OK) return st;
-        if (graph) FL_HIP_TRY(ctx, hipGraphLaunch(B->gx_post, s));
-        else if ((st = launch_post(B, s)) != FL_OK) return st;
+        if (graph && B->graph_mode == FL_GRAPH_WHOLE_STEP) {
+            FL_HIP_TRY(ctx, hipGraphLaunch(B->gx_step, s));
+        } else {
+            if (graph) FL_HIP_TRY(ctx, hipGraphLaunch(B->gx_pre, s));
+            else if ((st = launch_pre(B, s)) != FL_OK) return st;
+            if ((st = launch_pcg_coop(B, s, B->rhs, B->x, B->kmax_all)) != FL_OK) return st;
+            if (graph) FL_HIP_TRY(ctx, hipGraphLaunch(B->gx_post, s));
+            else if ((st = launch_post(B, s)) != FL_OK) return st;
+        }
         if (step_ms) FL_HIP_TRY(ctx, hipEventRecord(B->ev[2 * k + 1], s));
     }
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->h_ctl, B->ctl, B->count * sizeof(StepCtl), hipMemcpyDeviceToHost, s));
@@ -2230,6 +2288,12 @@ fl_status fl_batch2d_set_launch(fl_batch2d *b, const fl_batch2d_tuning *launch) 
     fl_status st = set_pcg_blocks(B, (uint32_t)std::max(launch->blocks, 1), (uint32_t)launch->threads);
     if (st != FL_OK) return st;
     FL_HIP_TRY(B->ctx, hipStreamSynchronize(B->ctx->stream));
+    return FL_OK;
+}
+
+fl_status fl_batch2d_graph_mode(const fl_batch2d *b, int32_t *mode) {
+    if (!b || !mode) return FL_ERR_INVALID_ARGUMENT;
+    *mode = b->b->graph_mode;
     return FL_OK;
 }
 

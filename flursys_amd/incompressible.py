@@ -241,6 +241,14 @@ class CavityBatch:
                                          C.byref(n)), self.ctx.handle)
         return _tuning(best), [_tuning(tab[i]) for i in range(n.value)]
 
+    def graph_mode(self) -> int:
+        """How graph runs replay a step: _lib.GRAPH_WHOLE_STEP (one graph with
+        the persistent PCG as a cooperative kernel node), GRAPH_SPLIT (graphs
+        around a cooperative launch) or GRAPH_NONE (not captured yet)."""
+        m = C.c_int32()
+        check(load().fl_batch2d_graph_mode(self.handle, C.byref(m)), self.ctx.handle)
+        return m.value
+
     def launch(self):
         """The persistent PCG's current launch shape."""
         t = _lib.Batch2DTuning()

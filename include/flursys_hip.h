@@ -301,6 +301,14 @@ typedef struct fl_batch2d_tuning {
     double ms; /* autotuner: time of the candidate's solve */
 } fl_batch2d_tuning;
 fl_status fl_batch2d_set_launch(fl_batch2d *batch, const fl_batch2d_tuning *launch);
+/* How fl_batch2d_run(use_graphs = 1) replays a step (no reference
+ * counterpart): FL_GRAPH_WHOLE_STEP, one captured graph holding the
+ * persistent PCG as a cooperative kernel node; FL_GRAPH_SPLIT, captured
+ * graphs of the step's two halves around a cooperative launch (where the
+ * runtime cannot capture that launch); FL_GRAPH_NONE before the first
+ * graph run or after a launch-shape change. */
+enum { FL_GRAPH_NONE = 0, FL_GRAPH_WHOLE_STEP = 1, FL_GRAPH_SPLIT = 2 };
+fl_status fl_batch2d_graph_mode(const fl_batch2d *batch, int32_t *mode);
 fl_status fl_batch2d_get_launch(const fl_batch2d *batch, fl_batch2d_tuning *launch);
 /* Autotuner of the launch shape (no reference counterpart): every candidate
  * (threads, co-resident block count) solves the same

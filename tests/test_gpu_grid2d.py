@@ -246,15 +246,25 @@ def test_graph_replays_match_eager_runs(gpu_ctx):
     print("[gpu] batch graph replays", flush=True)
     eager = I.CavityBatch(uneven_cfgs(), ctx=gpu_ctx)
     graph = I.CavityBatch(uneven_cfgs(), ctx=gpu_ctx)
+    assert graph.graph_mode() == fa._lib.GRAPH_NONE
     for rnd in range(3):
         eager.run(2, use_graphs=False)
         graph.run(2, use_graphs=True)
+        print(f"[gpu] graph mode {graph.graph_mode()}", flush=True)
+        assert graph.graph_mode() in (fa._lib.GRAPH_WHOLE_STEP, fa._lib.GRAPH_SPLIT)
         for fe, fg in zip(all_fields(eager), all_fields(graph)):
             for name in ("u", "v", "p"):
                 assert np.array_equal(fe[name], fg[name]), (rnd, name)
     # eager host-driven steps in between keep both in step
     eager.step()
     graph.step()
+    eager.run(1, use_graphs=False)
+    graph.run(1, use_graphs=True)
+    for fe, fg in zip(all_fields(eager), all_fields(graph)):
+        assert np.array_equal(fe["p"], fg["p"])
+    # a launch-shape change re-captures the step
+    graph.set_launch(256, 3)
+    assert graph.graph_mode() == fa._lib.GRAPH_NONE
     eager.run(1, use_graphs=False)
     graph.run(1, use_graphs=True)
     for fe, fg in zip(all_fields(eager), all_fields(graph)):
