@@ -186,7 +186,11 @@ def run_batch2d(fa, ctx, args):
     for mode, graphs, total in (("graphs", True, args.batch_steps),
                                 ("eager", False, min(args.batch_steps, 50))):
         b = I.CavityBatch(cfgs, ctx=ctx)
-        if chosen is None:
+        if chosen is None and args.batch_launch:
+            th, bl = (int(v) for v in args.batch_launch.split("x"))
+            b.set_launch(th, bl)
+            chosen = dict(b.launch(), source="--batch-launch")
+        elif chosen is None:
             chosen, table = b.autotune(args.tune_iterations)
             log(f"[bench] batch2d: autotune over (threads, blocks), solves capped at "
                 f"{args.tune_iterations} iterations: chosen {chosen}")
@@ -195,13 +199,15 @@ def run_batch2d(fa, ctx, args):
         else:
             b.set_launch(chosen["threads"], chosen["blocks"])
         b.run(5, use_graphs=graphs)  # warmup (captures the graphs)
-        chunk, left, ms, iters = 25, total, [], 0
+        chunk, left, ms, pms, iters, cell_its = 25, total, [], [], 0, 0
         t0 = time.perf_counter()
         while left > 0:
             k = min(chunk, left)
-            reps, step_ms = b.run(k, use_graphs=graphs, step_times=True)
+            reps, step_ms, pcg_ms = b.run(k, use_graphs=graphs, step_times=True, pcg_times=True)
             ms.extend(step_ms.tolist())
+            pms.extend(pcg_ms.tolist())
             iters += sum(r.pressure_iterations for r in reps)
+            cell_its += sum(r.pressure_iterations * c.nx * c.ny for r, c in zip(reps, cfgs))
             left -= k
             log(f"[bench] batch2d {mode}: {total - left}/{total} steps, "
                 f"{iters / (len(cfgs) * (total - left)):.1f} PCG iterations per cavity-step, "
@@ -209,13 +215,29 @@ def run_batch2d(fa, ctx, args):
         elapsed = time.perf_counter() - t0
         gmode = b.graph_mode()
         b.close()
-        out[mode] = {"elapsed_s": elapsed, "steps": total, "step_ms": ms,
-                     "pressure_iterations": iters,
+        out[mode] = {"elapsed_s": elapsed, "steps": total, "step_ms": ms, "pcg_ms": pms,
+                     "pressure_iterations": iters, "cell_iterations": cell_its,
                      "graph_mode": {0: "none", 1: "whole step (cooperative PCG kernel node)",
                                     2: "split (graphs around a cooperative launch)"}[gmode]}
     g = out["graphs"]
     ms = np.array(g["step_ms"])
     steps = args.batch_steps
+    # roofline of the persistent PCG from the eager run, where each launch is
+    # timed with events on its stream: algorithmic bytes per cell-iteration
+    # (phase A reads flags, z, d and writes d, q; phase B reads flags, x or
+    # e, d, r, q and writes x or e, r, z) times the cell-iterations, over the
+    # launches' time
+    e = out["eager"]
+    wb = 4 if args.batch_precision == "mixed" else 8
+    bytes_per = (1 + 4 * wb) + (1 + 7 * wb)
+    pcg_s = sum(t for t in e["pcg_ms"] if t > 0) / 1e3
+    achieved = e["cell_iterations"] * bytes_per / pcg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                "frac": achieved / 8000.0, "traffic": None,
+                "kernel": "k_pcg (persistent, cooperative)",
+                "bytes_per_cell_iteration": bytes_per,
+                "pcg_ms_per_step": pcg_s * 1e3 / e["steps"]}
+    cpu = None if args.no_cpu else batch2d_cpu_baseline(cfgs)
     return {
         "metric": "aggregate cavity-steps/s, 64 uneven 2D cavities (64^2-512^2) advanced together",
         "value": len(cfgs) * steps / g["elapsed_s"],
@@ -248,9 +270,33 @@ def run_batch2d(fa, ctx, args):
             "eager_step_latency_ms_p50": float(np.percentile(out["eager"]["step_ms"], 50)),
             "parallelism": "single GPU",
         },
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
     }
+
+
+def batch2d_cpu_baseline(cfgs, stride=2):
+    """The CPU oracle (OpenMP, FP64 Jacobi-PCG) on a bounded sample: the first
+    step from rest of every `stride`-th cavity, as cavity-steps/s."""
+    import oracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    sample = cfgs[::stride]
+    t0 = time.perf_counter()
+    for c in sample:
+        ts = c.time_step
+        s = O.Solver2D(c.nx, c.ny, nu=c.kinematic_viscosity, dt=c.dt,
+                       pressure_max_iters=c.pressure_max_iters,
+                       pressure_tolerance=c.pressure_tolerance, threads=threads,
+                       adaptive=ts.adaptive, min_dt=ts.min_dt, max_dt=ts.max_dt,
+                       target_cfl=ts.target_cfl, target_diffusion=ts.target_diffusion_number,
+                       max_growth=ts.max_growth_factor, bc_u=c.bc_u)
+        s.step()
+    dt = time.perf_counter() - t0
+    log(f"[bench] batch2d CPU oracle: {len(sample)} cavity-steps in {dt:.1f} s")
+    return {"value": len(sample) / dt, "unit": "cavity-steps/s", "cores": threads, "kind": "port",
+            "sample": f"first step from rest of {len(sample)} of the 64 cavities (every "
+                      f"{stride}nd), FP64 Jacobi-PCG, {threads} OpenMP threads"}
 
 
 def main():
@@ -265,6 +311,8 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--batch-precision", choices=("fp64", "mixed"), default="mixed",
                     help="batch pressure solves: FP64 PCG, or FP32 CG in FP64 refinement")
+    ap.add_argument("--batch-launch", default="",
+                    help="batch2d: THREADSxBLOCKS for the persistent PCG instead of autotuning")
     ap.add_argument("--tune-iterations", type=int, default=1300,
                     help="iteration cap of the batch autotuner's solves")
     ap.add_argument("--variants", default="plain_fp64_pcg,mixed_ir,mg_fcg,learned_fcg")

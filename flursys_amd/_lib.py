@@ -170,7 +170,7 @@ SIGNATURES = {
     "fl_batch2d_destroy": (None, [_P]),
     "fl_batch2d_step": (C.c_int, [_P, C.POINTER(Grid2DStepReport)]),
     "fl_batch2d_get_fields": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
-    "fl_batch2d_run": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(Batch2DRunReport), _P]),
+    "fl_batch2d_run": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(Batch2DRunReport), _P, _P]),
     "fl_batch2d_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(Batch2DTuning), C.c_int32,
                                       C.POINTER(Batch2DTuning), C.POINTER(C.c_int32)]),
     "fl_batch2d_set_launch": (C.c_int, [_P, C.POINTER(Batch2DTuning)]),

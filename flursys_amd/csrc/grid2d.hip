@@ -1881,7 +1881,7 @@ fl_status capture_step(Batch2 *B) {
 }
 
 fl_status run_device(Batch2 *B, int64_t steps, bool graph, fl_batch2d_run_report *reps,
-                     double *step_ms) {
+                     double *step_ms, double *pcg_ms) {
     fl_ctx *ctx = B->ctx;
     hipStream_t s = ctx->stream;
     FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -1906,33 +1906,44 @@ fl_status run_device(Batch2 *B, int64_t steps, bool graph, fl_batch2d_run_report
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->ctl, B->h_ctl, B->count * sizeof(StepCtl), hipMemcpyHostToDevice, s));
     fl_status st;
     if (graph && (st = capture_step(B)) != FL_OK) return st;
-    if (step_ms)
-        while ((int64_t)B->ev.size() < 2 * steps) {
+    const bool whole = graph && B->graph_mode == FL_GRAPH_WHOLE_STEP;
+    const bool timed = step_ms || (pcg_ms && !whole);
+    if (timed)
+        while ((int64_t)B->ev.size() < 4 * steps) {
             hipEvent_t e;
             FL_HIP_TRY(ctx, hipEventCreate(&e));
             B->ev.push_back(e);
         }
     for (int64_t k = 0; k < steps; ++k) {
-        if (step_ms) FL_HIP_TRY(ctx, hipEventRecord(B->ev[2 * k], s));
-        if (graph && B->graph_mode == FL_GRAPH_WHOLE_STEP) {
+        if (timed) FL_HIP_TRY(ctx, hipEventRecord(B->ev[4 * k], s));
+        if (whole) {
             FL_HIP_TRY(ctx, hipGraphLaunch(B->gx_step, s));
         } else {
             if (graph) FL_HIP_TRY(ctx, hipGraphLaunch(B->gx_pre, s));
             else if ((st = launch_pre(B, s)) != FL_OK) return st;
+            if (timed) FL_HIP_TRY(ctx, hipEventRecord(B->ev[4 * k + 1], s));
             if ((st = launch_pcg_coop(B, s, B->rhs, B->x, B->kmax_all)) != FL_OK) return st;
+            if (timed) FL_HIP_TRY(ctx, hipEventRecord(B->ev[4 * k + 2], s));
             if (graph) FL_HIP_TRY(ctx, hipGraphLaunch(B->gx_post, s));
             else if ((st = launch_post(B, s)) != FL_OK) return st;
         }
-        if (step_ms) FL_HIP_TRY(ctx, hipEventRecord(B->ev[2 * k + 1], s));
+        if (timed) FL_HIP_TRY(ctx, hipEventRecord(B->ev[4 * k + 3], s));
     }
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->h_ctl, B->ctl, B->count * sizeof(StepCtl), hipMemcpyDeviceToHost, s));
     FL_HIP_TRY(ctx, hipStreamSynchronize(s));
-    if (step_ms)
-        for (int64_t k = 0; k < steps; ++k) {
-            float ms = 0.0f;
-            FL_HIP_TRY(ctx, hipEventElapsedTime(&ms, B->ev[2 * k], B->ev[2 * k + 1]));
+    for (int64_t k = 0; k < steps && timed; ++k) {
+        float ms = 0.0f;
+        if (step_ms) {
+            FL_HIP_TRY(ctx, hipEventElapsedTime(&ms, B->ev[4 * k], B->ev[4 * k + 3]));
             step_ms[k] = ms;
         }
+        if (pcg_ms && !whole) {
+            FL_HIP_TRY(ctx, hipEventElapsedTime(&ms, B->ev[4 * k + 1], B->ev[4 * k + 2]));
+            pcg_ms[k] = ms;
+        }
+    }
+    if (pcg_ms && whole)
+        for (int64_t k = 0; k < steps; ++k) pcg_ms[k] = -1.0;
     int first_bad = -1;
     const int64_t nsteps = steps;
     for (int p = 0; p < B->count; ++p) {
@@ -2267,10 +2278,10 @@ fl_status fl_batch2d_get_fields(fl_batch2d *b, int32_t index, double *u, double 
 }
 
 fl_status fl_batch2d_run(fl_batch2d *b, int64_t steps, int32_t use_graphs,
-                         fl_batch2d_run_report *reports, double *step_ms) {
+                         fl_batch2d_run_report *reports, double *step_ms, double *pcg_ms) {
     if (!b) return FL_ERR_INVALID_ARGUMENT;
     if (steps < 0) return set_error(b->b->ctx, FL_ERR_INVALID_ARGUMENT, "negative step count");
-    return run_device(b->b, steps, use_graphs != 0, reports, step_ms);
+    return run_device(b->b, steps, use_graphs != 0, reports, step_ms, pcg_ms);
 }
 
 fl_status fl_batch2d_autotune(fl_batch2d *b, int32_t iterations, fl_batch2d_tuning *chosen,

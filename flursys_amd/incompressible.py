@@ -219,17 +219,23 @@ class CavityBatch:
               self.ctx.handle)
         return {"u": u, "v": v, "p": p}
 
-    def run(self, steps: int, use_graphs: bool = True, step_times: bool = False):
+    def run(self, steps: int, use_graphs: bool = True, step_times: bool = False,
+            pcg_times: bool = False):
         """`steps` device-driven steps (one host synchronisation).  Returns
-        (per-grid run reports, per-step device milliseconds or None); raises
-        FlursysError with the first failing grid's message."""
+        (per-grid run reports, per-step device milliseconds or None), plus
+        the per-step persistent-PCG milliseconds when `pcg_times` (-1 where
+        the step replays as one graph); raises FlursysError with the first
+        failing grid's message."""
         reps = (_lib.Batch2DRunReport * len(self.cfgs))()
         ms = np.zeros(max(steps, 1)) if step_times else None
+        pms = np.zeros(max(steps, 1)) if pcg_times else None
         st = load().fl_batch2d_run(self.handle, steps, int(use_graphs), reps,
-                                   ptr(ms) if step_times else None)
+                                   ptr(ms) if step_times else None,
+                                   ptr(pms) if pcg_times else None)
         self.last_run = list(reps)
         check(st, self.ctx.handle)
-        return self.last_run, (ms[:steps] if step_times else None)
+        out = (self.last_run, (ms[:steps] if step_times else None))
+        return out + (pms[:steps],) if pcg_times else out
 
     def autotune(self, iterations: int = 20):
         """Chooses the persistent PCG's launch shape; returns (chosen, table),

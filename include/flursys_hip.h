@@ -287,8 +287,10 @@ typedef struct {
     int32_t reached_end_time;
     int32_t status;                /* FL_OK, or FL_ERR_SOLVER when the grid failed */
 } fl_batch2d_run_report;
+/* step_ms / pcg_ms (optional, `steps` entries): device time of each step and
+ * of its persistent PCG launch (-1 when the step replays as one graph). */
 fl_status fl_batch2d_run(fl_batch2d *batch, int64_t steps, int32_t use_graphs,
-                         fl_batch2d_run_report *reports, double *step_ms);
+                         fl_batch2d_run_report *reports, double *step_ms, double *pcg_ms);
 /* Launch shape of the batch's persistent PCG (no reference counterpart):
  * `threads` per block (256, 512 or 1024) and `blocks` (clamped to what is
  * co-resident and to the tile count).  The blocks share the iterating

@@ -398,3 +398,13 @@ def test_precision_validation(gpu_ctx):
                       ctx=gpu_ctx)
     with pytest.raises(fa.FlursysError, match="inner_tolerance"):
         I.CavityBatch([mixed(I.SimulationConfig.cavity(16), inner_tolerance=2.0)], ctx=gpu_ctx)
+
+
+def test_run_step_and_pcg_times(gpu_ctx):
+    b = I.CavityBatch(uneven_cfgs(), ctx=gpu_ctx)
+    _, ms, pms = b.run(2, use_graphs=False, step_times=True, pcg_times=True)
+    assert np.all(ms > 0) and np.all(pms > 0) and np.all(pms <= ms)
+    _, ms, pms = b.run(2, use_graphs=True, step_times=True, pcg_times=True)
+    assert np.all(ms > 0)
+    if b.graph_mode() == fa._lib.GRAPH_WHOLE_STEP:
+        assert np.all(pms == -1.0)
