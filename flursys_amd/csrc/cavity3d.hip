@@ -73,6 +73,17 @@ template <typename T> __device__ inline T inv_diag(const Grid &g, uint32_t idx);
 template <> __device__ inline double inv_diag<double>(const Grid &g, uint32_t idx) { return g.inv[idx]; }
 template <> __device__ inline float inv_diag<float>(const Grid &g, uint32_t idx) { return g.invf[idx]; }
 
+// The 8 reciprocal diagonals in LDS: indexed per lane, the kernel-argument
+// copy would be a vector load from the argument segment per lookup.  Every
+// thread of the block must call it (it ends with a block barrier).
+template <typename T>
+__device__ inline const T *inv_table(const Grid &g) {
+    __shared__ T tab[8];
+    if (threadIdx.x < 8) tab[threadIdx.x] = inv_diag<T>(g, threadIdx.x);
+    __syncthreads();
+    return tab;
+}
+
 // Neighbour offsets clamped at the walls (a missing neighbour reads the
 // centre, contributing c * (x_c - x_c) = 0).
 struct Nbr {
@@ -119,6 +130,7 @@ template <typename TR>
 __global__ void __launch_bounds__(kBlock) k_residual(Grid g, const double *__restrict__ rhs,
                                                      const double *__restrict__ x,
                                                      TR *__restrict__ r, double *partials) {
+    const TR *inv_tab = inv_table<TR>(g);
     double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
     TileSched ts((g.n + kBlock - 1) / kBlock);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
@@ -135,7 +147,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(Grid g, const double *__res
                               g.iy2 * ((xc - x[o.n]) + (xc - x[o.s])) +
                               g.iz2 * ((xc - x[o.f]) + (xc - x[o.b]));
             rc = (TR)(-rhs[c] - ax);
-            inv = (double)inv_diag<TR>(g, diag_index(g, i, j, k));
+            inv = (double)inv_tab[diag_index(g, i, j, k)];
         }
         r[c] = rc;
         const double rd = (double)rc;
@@ -156,6 +168,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_dirapply(
     T *__restrict__ dnew, T *__restrict__ q, const double *__restrict__ pin,
     double *__restrict__ pout, CgStateA *SA, const CgStateB *SB) {
     if (SA->done || SB->done) return;
+    const T *inv_tab = inv_table<T>(g);
     double f[3], fm[1];
     fold_partials<3, 1>(pin, gridDim.x, f, fm);
     const double rz = f[0], rr = f[1], rmax = fm[0];
@@ -228,7 +241,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_dirapply(
         for (int a = 0; a < 7; ++a) {
             // a clamped neighbour is the centre itself; the pinned cell's row
             // is the identity
-            const T inv = idx[a] == 0 ? (T)1 : inv_diag<T>(g, (a > 0 && idx[a] == c) ? ic : di[a]);
+            const T inv = idx[a] == 0 ? (T)1 : inv_tab[(a > 0 && idx[a] == c) ? ic : di[a]];
             const T z = rv[a] * inv;
             dn[a] = first ? z : z + tb * dv[a];
         }
@@ -252,6 +265,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x
                                                       double *__restrict__ pout,
                                                       const CgStateA *SA, CgStateB *SB) {
     if (SA->done || SB->done) return;
+    const T *inv_tab = inv_table<T>(g);
     double f[1], fm[1];
     fold_partials<1, 1>(pin, gridDim.x, f, fm);
     const double den = f[0];
@@ -287,7 +301,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x
         if (c != 0) {
             uint32_t i, j, k;
             cell_ijk(g, c, i, j, k);
-            z = rc * inv_diag<T>(g, diag_index(g, i, j, k));
+            z = rc * inv_tab[diag_index(g, i, j, k)];
         }
         const double rd = (double)rc;
         s[0] += rd * (double)z;
