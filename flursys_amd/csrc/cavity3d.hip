@@ -366,6 +366,16 @@ __device__ inline uint32_t lvl_didx(const Level &L, uint32_t i, uint32_t j, uint
            ((uint32_t)(k > 0 && k + 1 < L.nz) << 2);
 }
 
+// A level's 8 reciprocal diagonals in LDS (per-lane lookups of the
+// kernel-argument copy are vector loads from the argument segment).  Every
+// thread of the block must call it (it ends with a block barrier).
+__device__ inline const float *lvl_inv_table(const Level &L) {
+    __shared__ float tab[8];
+    if (threadIdx.x < 8) tab[threadIdx.x] = L.inv[threadIdx.x];
+    __syncthreads();
+    return tab;
+}
+
 // f on level 0: the residual with the compatibility value at cell 0.
 __device__ inline float f_at(const float *f, uint32_t c, bool level0, float f0) {
     return (level0 && c == 0) ? f0 : f[c];
@@ -375,8 +385,8 @@ __device__ inline float f_at(const float *f, uint32_t c, bool level0, float f0) 
 // level: e1 = omega D^-1 f is pointwise, so e2 = e1 + omega D^-1 (f - A e1)
 // needs only f and D^-1 on the 7-point stencil (a clamped neighbour is the
 // cell itself and contributes nothing, as in the sweep).
-__device__ inline float two_sweeps_from_zero(const Level &L, const float *f, uint32_t i, uint32_t j,
-                                             uint32_t k, bool level0, float f0) {
+__device__ inline float two_sweeps_from_zero(const Level &L, const float *inv, const float *f,
+                                             uint32_t i, uint32_t j, uint32_t k, bool level0, float f0) {
     const uint32_t ii[7] = {i, i + 1 < L.nx ? i + 1 : i, i > 0 ? i - 1 : i, i, i, i, i};
     const uint32_t jj[7] = {j, j, j, j + 1 < L.ny ? j + 1 : j, j > 0 ? j - 1 : j, j, j};
     const uint32_t kk[7] = {k, k, k, k, k, k + 1 < L.nz ? k + 1 : k, k > 0 ? k - 1 : k};
@@ -384,10 +394,10 @@ __device__ inline float two_sweeps_from_zero(const Level &L, const float *f, uin
 #pragma unroll
     for (int a = 0; a < 7; ++a) fv[a] = f_at(f, ii[a] + L.nx * (jj[a] + L.ny * kk[a]), level0, f0);
 #pragma unroll
-    for (int a = 0; a < 7; ++a) e1[a] = kOmega * L.inv[lvl_didx(L, ii[a], jj[a], kk[a])] * fv[a];
+    for (int a = 0; a < 7; ++a) e1[a] = kOmega * inv[lvl_didx(L, ii[a], jj[a], kk[a])] * fv[a];
     const float ae = L.cx * ((e1[0] - e1[1]) + (e1[0] - e1[2])) + L.cy * ((e1[0] - e1[3]) + (e1[0] - e1[4])) +
                      L.cz * ((e1[0] - e1[5]) + (e1[0] - e1[6]));
-    return e1[0] + kOmega * L.inv[lvl_didx(L, i, j, k)] * (fv[0] - ae);
+    return e1[0] + kOmega * inv[lvl_didx(L, i, j, k)] * (fv[0] - ae);
 }
 
 // One damped-Jacobi sweep on a level:
@@ -404,6 +414,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_sweep(Level L, Level LC, const fl
                                                      double *__restrict__ pout, const FcgA *SA,
                                                      const FcgB *SB, float corr) {
     if (SA->done || SB->done) return;
+    const float *inv_tab = lvl_inv_table(L);
     const float f0 = kLevel0 ? (float)(-SA->sumr) : 0.0f;
     double s[2] = {0.0, 0.0}, m[1] = {0.0};
     TileSched ts((L.n + kBlock - 1) / kBlock);
@@ -413,10 +424,10 @@ __global__ void __launch_bounds__(kBlock) k_mg_sweep(Level L, Level LC, const fl
         uint32_t i, j, k;
         lvl_ijk(L, c, i, j, k);
         const float fc = f_at(f, c, kLevel0, f0);
-        const float inv = L.inv[lvl_didx(L, i, j, k)];
+        const float inv = inv_tab[lvl_didx(L, i, j, k)];
         float out;
         if (kFirst) {
-            out = two_sweeps_from_zero(L, f, i, j, k, kLevel0, f0);
+            out = two_sweeps_from_zero(L, inv_tab, f, i, j, k, kLevel0, f0);
         } else {
             const uint32_t ii[7] = {i, i + 1 < L.nx ? i + 1 : i, i > 0 ? i - 1 : i, i, i, i, i};
             const uint32_t jj[7] = {j, j, j, j + 1 < L.ny ? j + 1 : j, j > 0 ? j - 1 : j, j, j};
@@ -610,6 +621,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_check(Level L, CgParams prm, cons
         SA->done = done;
     }
     if (done) return;
+    const float *inv_tab = lvl_inv_table(L);
     const float f0 = (float)(-sumr);
     TileSched ts((L.n + kBlock - 1) / kBlock);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
@@ -617,7 +629,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_check(Level L, CgParams prm, cons
         if (c >= L.n) continue;
         uint32_t i, j, k;
         lvl_ijk(L, c, i, j, k);
-        eout[c] = two_sweeps_from_zero(L, r, i, j, k, true, f0);
+        eout[c] = two_sweeps_from_zero(L, inv_tab, r, i, j, k, true, f0);
     }
 }
 
