@@ -159,115 +159,18 @@ __global__ void __launch_bounds__(kBlock) k_residual(Grid g, const double *__res
     publish_partials<3, 1>(s, m, partials);
 }
 
-// d' = z + beta d (z = D^-1 r), q = A d', partial d'.q.  Folds the update
-// partials first: stop test, breakdown test and beta, in the reference's
-// order (incompressible.rs:1154-1185).
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_cg_dirapply(
-    Grid g, CgParams prm, const T *__restrict__ r, const T *__restrict__ dold,
-    T *__restrict__ dnew, T *__restrict__ q, const double *__restrict__ pin,
-    double *__restrict__ pout, CgStateA *SA, const CgStateB *SB) {
-    if (SA->done || SB->done) return;
-    const T *inv_tab = inv_table<T>(g);
-    double f[3], fm[1];
-    fold_partials<3, 1>(pin, gridDim.x, f, fm);
-    const double rz = f[0], rr = f[1], rmax = fm[0];
-    const bool first = SB->first != 0;
-    const double rr0 = prm.rr0_ref > 0.0 ? prm.rr0_ref : (first ? rr : SA->rr0);
-    const long long iters = SB->iters;
-    int done = 0, converged = 0, status = ST_OK;
-    double beta = 0.0;
-    if (!first && !isfinite(rmax)) {
-        status = ST_RESIDUAL;
-        done = 1;
-    } else {
-        const bool conv = prm.stop_rule == FL_STOP_REL_L2 ? (sqrt(rr) <= prm.tol * sqrt(rr0))
-                                                          : (rmax < prm.tol);
-        if (conv) {
-            converged = done = 1;
-        } else if (!first) {
-            if (fabs(SB->rz_prev) < 1.0e-30) done = 1;  // the silent break (:1171-1173)
-            else if (iters >= prm.max_iter) done = 1;   // MaxIterations
-            else beta = rz / SB->rz_prev;
-        }
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        SA->rz = rz;
-        SA->beta = beta;
-        SA->rr0 = rr0;
-        SA->rr = rr;
-        SA->rmax = rmax;
-        SA->converged = converged;
-        SA->status = status;
-        SA->done = done;
-    }
-    if (done) return;
-    const T tb = (T)beta;
-    double s[1] = {0.0}, m[1] = {0.0};
-    TileSched ts((g.n + kBlock - 1) / kBlock);
-    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
-        const uint32_t c = t * kBlock + threadIdx.x;
-        if (c >= g.n) continue;
-        if (c == 0) {  // pinned row: identity; r_0 = d_0 = 0 throughout
-            const T d0 = first ? r[0] : (T)(r[0] + tb * dold[0]);
-            dnew[0] = d0;
-            q[0] = d0;
-            s[0] += (double)d0 * (double)d0;
-            continue;
-        }
-        uint32_t i, j, k;
-        cell_ijk(g, c, i, j, k);
-        const Nbr o = neighbours(g, c, i, j, k);
-        const uint32_t idx[7] = {c, o.e, o.w, o.n, o.s, o.f, o.b};
-        T rv[7], dv[7] = {0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-        for (int a = 0; a < 7; ++a) rv[a] = r[idx[a]];
-        if (!first) {
-#pragma unroll
-            for (int a = 0; a < 7; ++a) dv[a] = dold[idx[a]];
-        }
-        // reciprocal diagonals of the centre and of the six neighbours: bit
-        // a of the index is "two neighbours along axis a"
-        const uint32_t bx = i > 0 && i + 1 < g.nx, by = j > 0 && j + 1 < g.ny,
-                       bz = k > 0 && k + 1 < g.nz;
-        const uint32_t ic = bx | by << 1 | bz << 2;
-        const uint32_t ixe = (i + 2 < g.nx) | by << 1 | bz << 2, ixw = (i > 1) | by << 1 | bz << 2;
-        const uint32_t iyn = bx | (j + 2 < g.ny) << 1 | bz << 2, iys = bx | (j > 1) << 1 | bz << 2;
-        const uint32_t izf = bx | by << 1 | (k + 2 < g.nz) << 2, izb = bx | by << 1 | (k > 1) << 2;
-        const uint32_t di[7] = {ic, ixe, ixw, iyn, iys, izf, izb};
-        const T cx = (T)g.ix2, cy = (T)g.iy2, cz = (T)g.iz2;
-        T dn[7];
-#pragma unroll
-        for (int a = 0; a < 7; ++a) {
-            // a clamped neighbour is the centre itself; the pinned cell's row
-            // is the identity
-            const T inv = idx[a] == 0 ? (T)1 : inv_tab[(a > 0 && idx[a] == c) ? ic : di[a]];
-            const T z = rv[a] * inv;
-            dn[a] = first ? z : z + tb * dv[a];
-        }
-        const T qc = cx * ((dn[0] - dn[1]) + (dn[0] - dn[2])) +
-                     cy * ((dn[0] - dn[3]) + (dn[0] - dn[4])) +
-                     cz * ((dn[0] - dn[5]) + (dn[0] - dn[6]));
-        dnew[c] = dn[0];
-        q[c] = qc;
-        s[0] += (double)dn[0] * (double)qc;
-    }
-    publish_partials<1, 1>(s, m, pout);
-}
-
-// alpha = r.z / d'.q; x += alpha d', r -= alpha q; partials r.z, r.r, max|r|.
 template <typename T, typename TX>
 __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x,
                                                       T *__restrict__ r,
                                                       const T *__restrict__ d,
                                                       const T *__restrict__ q,
                                                       const double *__restrict__ pin,
-                                                      double *__restrict__ pout,
+                                                      uint32_t rows, double *__restrict__ pout,
                                                       const CgStateA *SA, CgStateB *SB) {
     if (SA->done || SB->done) return;
     const T *inv_tab = inv_table<T>(g);
     double f[1], fm[1];
-    fold_partials<1, 1>(pin, gridDim.x, f, fm);
+    fold_partials<1, 1>(pin, rows, f, fm);
     const double den = f[0];
     int status = ST_OK;
     double alpha = 0.0;
@@ -589,16 +492,119 @@ __global__ void __launch_bounds__(1024) k_mg_tail(TailLevels T, const float *__r
     for (uint32_t c = threadIdx.x; c < T.lv[0].n; c += blockDim.x) eout[c] = E[0][c];
 }
 
-// Stop test of the inner FCG (folds the update partials), then the first two
-// pre-sweeps of the cycle on level 0 (from a zero iterate, in one pass).
+// ------------------------------------------------ column-marching kernels
+// The level-0 stencil passes of the solvers as column marches: a block of
+// kZX x kZY threads owns as many (x, y) columns over kZK planes; a thread
+// walks its column up in z keeping the operand at k-1, k and k+1 in
+// registers, so each cell loads only the next plane's centre and its four
+// in-plane neighbours (the per-cell form loads all seven, the two z
+// neighbours a plane apart).  Each block publishes one partial row
+// (blocks in x, y, z order); consumers fold ZGrid::count rows.
+constexpr uint32_t kZX = 128, kZY = 2, kZK = 64, kZT = kZX * kZY;
 
-__global__ void __launch_bounds__(kBlock) k_mg_check(Level L, CgParams prm, const float *__restrict__ r,
+struct ZGrid {
+    uint32_t bx, by, bz, count;
+};
+inline ZGrid zgrid(uint32_t nx, uint32_t ny, uint32_t nz) {
+    ZGrid z{(nx + kZX - 1) / kZX, (ny + kZY - 1) / kZY, (nz + kZK - 1) / kZK, 0};
+    z.count = z.bx * z.by * z.bz;
+    return z;
+}
+
+// Walks this thread's column.  load(c, i, j, k) fetches the raw values of a cell,
+// make(raw, c, i, j, k) turns them into the stencil operand; every load of a
+// plane step is issued before any operand is formed.  body(i, j, k, c, v)
+// gets v = {centre, +x, -x, +y, -y, +z, -z} (neighbours clamped at the walls
+// to the cell itself).  Threads past the grid return.
+template <typename Ld, typename Mk, typename Body>
+__device__ inline void zmarch(uint32_t nx, uint32_t ny, uint32_t nz, Ld &&load, Mk &&make, Body &&body) {
+    const uint32_t i = blockIdx.x * kZX + threadIdx.x % kZX, j = blockIdx.y * kZY + threadIdx.x / kZX;
+    if (i >= nx || j >= ny) return;
+    const uint32_t k0 = blockIdx.z * kZK, k1 = min(k0 + kZK, nz), nxy = nx * ny;
+    const uint32_t ie = i + 1 < nx ? i + 1 : i, iw = i > 0 ? i - 1 : i;
+    const uint32_t jn = j + 1 < ny ? j + 1 : j, js = j > 0 ? j - 1 : j;
+    const uint32_t de = ie - i, dw = i - iw, dn = (jn - j) * nx, ds = (j - js) * nx;
+    uint32_t c = i + nx * (j + ny * k0);
+    const uint32_t cb = k0 > 0 ? c - nxy : c;
+    auto em = make(load(cb, i, j, k0 > 0 ? k0 - 1 : k0), cb, i, j, k0 > 0 ? k0 - 1 : k0);
+    auto ec = make(load(c, i, j, k0), c, i, j, k0);
+    for (uint32_t k = k0; k < k1; ++k, c += nxy) {
+        const uint32_t kp = k + 1 < nz ? k + 1 : k, cp = c + (kp - k) * nxy;
+        const auto rp = load(cp, i, j, kp), re = load(c + de, ie, j, k), rw = load(c - dw, iw, j, k),
+                   rn = load(c + dn, i, jn, k), rs = load(c - ds, i, js, k);
+        const auto ep = make(rp, cp, i, j, kp);
+        const decltype(ec) v[7] = {ec, make(re, c + de, ie, j, k), make(rw, c - dw, iw, j, k),
+                                   make(rn, c + dn, i, jn, k), make(rs, c - ds, i, js, k), ep, em};
+        body(i, j, k, c, v);
+        em = ec;
+        ec = ep;
+    }
+}
+
+// Block reduction of NS sums and NM maxima into this block's partial row.
+template <int NS, int NM>
+__device__ inline void publish_row(double (&s)[NS], double (&m)[NM], double *pout) {
+    block_reduce<NS, NM>(s, m);
+    if (threadIdx.x == 0) {
+        const size_t row = blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
+#pragma unroll
+        for (int a = 0; a < NS; ++a) pout[row * (NS + NM) + a] = s[a];
+#pragma unroll
+        for (int a = 0; a < NM; ++a) pout[row * (NS + NM) + NS + a] = m[a];
+    }
+}
+
+// Level-0 damped-Jacobi sweep, e_out = e' + omega D^-1 (f - A e') with
+// e' = e_in (+ corr P(e_c), kProlong); kPublish also publishes
+// sum e_out.q_old and sum q_old for the FCG beta.
+template <bool kProlong, bool kPublish>
+__global__ void __launch_bounds__(kZT) k_mg_sweep0_z(Level L, Level LC, const float *__restrict__ f,
+                                                     const float *__restrict__ ein,
+                                                     const float *__restrict__ ec,
                                                      float *__restrict__ eout,
-                                                     const double *__restrict__ pin, FcgA *SA,
-                                                     const FcgB *SB) {
+                                                     const float *__restrict__ qold,
+                                                     double *__restrict__ pout, const FcgA *SA,
+                                                     const FcgB *SB, float corr) {
+    if (SA->done || SB->done) return;
+    const float *inv = lvl_inv_table(L);
+    const float f0 = (float)(-SA->sumr);
+    const bool publish = kPublish && !SB->first;
+    double s[2] = {0.0, 0.0}, m[1] = {0.0};
+    struct Raw {
+        float e, c;
+    };
+    zmarch(
+        L.nx, L.ny, L.nz,
+        [&](uint32_t c, uint32_t i, uint32_t j, uint32_t k) -> Raw {
+            return Raw{ein[c], kProlong ? ec[(i >> 1) + LC.nx * ((j >> 1) + LC.ny * (k >> 1))] : 0.0f};
+        },
+        [&](const Raw &r, uint32_t, uint32_t, uint32_t, uint32_t) -> float {
+            return kProlong ? r.e + corr * r.c : r.e;
+        },
+        [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const float (&v)[7]) {
+            const float ae = L.cx * ((v[0] - v[1]) + (v[0] - v[2])) + L.cy * ((v[0] - v[3]) + (v[0] - v[4])) +
+                             L.cz * ((v[0] - v[5]) + (v[0] - v[6]));
+            const float out = v[0] + kOmega * inv[lvl_didx(L, i, j, k)] * (f_at(f, c, true, f0) - ae);
+            eout[c] = out;
+            if (publish) {
+                const double qo = (double)qold[c];
+                s[0] += (double)out * qo;
+                s[1] += qo;
+            }
+        });
+    if (kPublish) publish_row<2, 1>(s, m, pout);
+}
+
+// Stop test of the inner FCG (folds the `rows` update partials), then the
+// first two pre-sweeps of the cycle on level 0 from a zero iterate:
+// e1 = omega D^-1 f is pointwise and marched, e2 = e1 + omega D^-1 (f - A e1).
+__global__ void __launch_bounds__(kZT) k_mg_check_z(Level L, CgParams prm, const float *__restrict__ r,
+                                                    float *__restrict__ eout,
+                                                    const double *__restrict__ pin, uint32_t rows,
+                                                    FcgA *SA, const FcgB *SB) {
     if (SA->done || SB->done) return;
     double fs[3], fm[1];
-    fold_partials<3, 1>(pin, gridDim.x, fs, fm);
+    fold_partials<3, 1>(pin, rows, fs, fm);
     const double rr = fs[1], sumr = fs[2], rmax = fm[0];
     const bool first = SB->first != 0;
     const double rr0 = first ? rr : SA->rr0;
@@ -611,7 +617,8 @@ __global__ void __launch_bounds__(kBlock) k_mg_check(Level L, CgParams prm, cons
     } else if (SB->iters >= prm.max_iter) {
         done = 1;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const bool lead = blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0;
+    if (lead && threadIdx.x == 0) {
         SA->rr0 = rr0;
         SA->rr = rr;
         SA->rmax = rmax;
@@ -621,31 +628,108 @@ __global__ void __launch_bounds__(kBlock) k_mg_check(Level L, CgParams prm, cons
         SA->done = done;
     }
     if (done) return;
-    const float *inv_tab = lvl_inv_table(L);
+    const float *inv = lvl_inv_table(L);
     const float f0 = (float)(-sumr);
-    TileSched ts((L.n + kBlock - 1) / kBlock);
-    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
-        const uint32_t c = t * kBlock + threadIdx.x;
-        if (c >= L.n) continue;
-        uint32_t i, j, k;
-        lvl_ijk(L, c, i, j, k);
-        eout[c] = two_sweeps_from_zero(L, inv_tab, r, i, j, k, true, f0);
-    }
+    zmarch(
+        L.nx, L.ny, L.nz, [&](uint32_t c, uint32_t, uint32_t, uint32_t) -> float { return f_at(r, c, true, f0); },
+        [&](float fv, uint32_t, uint32_t i, uint32_t j, uint32_t k) -> float {
+            return kOmega * inv[lvl_didx(L, i, j, k)] * fv;
+        },
+        [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const float (&v)[7]) {
+            const float ae = L.cx * ((v[0] - v[1]) + (v[0] - v[2])) + L.cy * ((v[0] - v[3]) + (v[0] - v[4])) +
+                             L.cz * ((v[0] - v[5]) + (v[0] - v[6]));
+            eout[c] = v[0] + kOmega * inv[lvl_didx(L, i, j, k)] * (f_at(r, c, true, f0) - ae);
+        });
 }
 
-// p = (z~ - z~_0) - beta p_old, q = A p (pinned), partials p.q and r.p.
+// Jacobi-PCG direction: d' = D^-1 r + beta d (the pinned row: r_0 + beta d_0),
+// q = A d' (q_0 = d'_0), partial d'.q.  Folds the `rows` update partials.
+template <bool kFirst, typename T>
+__device__ inline void cg_direction(const Grid &g, const T *inv, const T *r, const T *dold, T *dnew,
+                                    T *q, T tb, double (&s)[1]) {
+    const T cx = (T)g.ix2, cy = (T)g.iy2, cz = (T)g.iz2;
+    struct Raw {
+        T r, d;
+    };
+    zmarch(
+        g.nx, g.ny, g.nz,
+        [&](uint32_t c, uint32_t, uint32_t, uint32_t) -> Raw { return Raw{r[c], kFirst ? (T)0 : dold[c]}; },
+        [&](const Raw &v, uint32_t c, uint32_t i, uint32_t j, uint32_t k) -> T {
+            const T z = c == 0 ? v.r : v.r * inv[diag_index(g, i, j, k)];
+            return kFirst ? z : z + tb * v.d;
+        },
+        [&](uint32_t, uint32_t, uint32_t, uint32_t c, const T (&v)[7]) {
+            const T qc = c == 0 ? v[0]
+                                : cx * ((v[0] - v[1]) + (v[0] - v[2])) + cy * ((v[0] - v[3]) + (v[0] - v[4])) +
+                                      cz * ((v[0] - v[5]) + (v[0] - v[6]));
+            dnew[c] = v[0];
+            q[c] = qc;
+            s[0] += (double)v[0] * (double)qc;
+        });
+}
+template <typename T>
+__global__ void __launch_bounds__(kZT) k_cg_dirapply_z(Grid g, CgParams prm, const T *__restrict__ r,
+                                                       const T *__restrict__ dold, T *__restrict__ dnew,
+                                                       T *__restrict__ q, const double *__restrict__ pin,
+                                                       uint32_t rows, double *__restrict__ pout,
+                                                       CgStateA *SA, const CgStateB *SB) {
+    if (SA->done || SB->done) return;
+    const T *inv = inv_table<T>(g);
+    double f[3], fm[1];
+    fold_partials<3, 1>(pin, rows, f, fm);
+    const double rz = f[0], rr = f[1], rmax = fm[0];
+    const bool first = SB->first != 0;
+    const double rr0 = prm.rr0_ref > 0.0 ? prm.rr0_ref : (first ? rr : SA->rr0);
+    const long long iters = SB->iters;
+    int done = 0, converged = 0, status = ST_OK;
+    double beta = 0.0;
+    if (!first && !isfinite(rmax)) {
+        status = ST_RESIDUAL;
+        done = 1;
+    } else {
+        const bool conv = prm.stop_rule == FL_STOP_REL_L2 ? (sqrt(rr) <= prm.tol * sqrt(rr0))
+                                                          : (rmax < prm.tol);
+        if (conv) {
+            converged = done = 1;
+        } else if (!first) {
+            if (fabs(SB->rz_prev) < 1.0e-30) done = 1;  // the silent break (:1171-1173)
+            else if (iters >= prm.max_iter) done = 1;   // MaxIterations
+            else beta = rz / SB->rz_prev;
+        }
+    }
+    const bool lead = blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0;
+    if (lead && threadIdx.x == 0) {
+        SA->rz = rz;
+        SA->beta = beta;
+        SA->rr0 = rr0;
+        SA->rr = rr;
+        SA->rmax = rmax;
+        SA->converged = converged;
+        SA->status = status;
+        SA->done = done;
+    }
+    if (done) return;
+    double s[1] = {0.0}, m[1] = {0.0};
+    if (first) cg_direction<true, T>(g, inv, r, dold, dnew, q, (T)0, s);
+    else cg_direction<false, T>(g, inv, r, dold, dnew, q, (T)beta, s);
+    publish_row<1, 1>(s, m, pout);
+}
+
+// p = (z~ - z~_0) - beta p_old, q = A p (pinned), partials p.q and r.p;
+// folds the `rows` partials of the publishing sweep.  (A column march was
+// slower here: two operands per point, and p is formed at every load.)
 __global__ void __launch_bounds__(kBlock) k_fcg_dirapply(Grid g, const float *__restrict__ zt,
                                                          const float *__restrict__ pold,
                                                          float *__restrict__ pnew,
                                                          float *__restrict__ q,
                                                          const float *__restrict__ r,
                                                          const double *__restrict__ pz,
-                                                         double *__restrict__ pout,
+                                                         uint32_t rows, double *__restrict__ pout,
                                                          const FcgA *SA, const FcgB *SB) {
     if (SA->done || SB->done) return;
     const bool first = SB->first != 0;
     double fz[2], fm[1];
-    fold_partials<2, 1>(pz, gridDim.x, fz, fm);
+    fold_partials<2, 1>(pz, rows, fz, fm);
     const float z0 = zt[0];
     const double beta = first ? 0.0 : (fz[0] - (double)z0 * fz[1]) / SB->pq;
     const float tb = (float)beta;
@@ -690,12 +774,12 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update(uint32_t n, float *__rest
                                                        float *__restrict__ r,
                                                        const float *__restrict__ p,
                                                        const float *__restrict__ q,
-                                                       const double *__restrict__ pin,
+                                                       const double *__restrict__ pin, uint32_t rows,
                                                        double *__restrict__ pout, const FcgA *SA,
                                                        FcgB *SB) {
     if (SA->done || SB->done) return;
     double f[2], fm[1];
-    fold_partials<2, 1>(pin, gridDim.x, f, fm);
+    fold_partials<2, 1>(pin, rows, f, fm);
     const double pq = f[0], rp = f[1];
     int status = ST_OK;
     double alpha = 0.0;
@@ -1018,10 +1102,10 @@ enum Kind {
 const char *const kKindNames[K_COUNT] = {
     "k_predict", "k_rhs", "k_correct", "k_diag", "k_finite",
     "k_residual<double>", "k_residual<float>", "k_fold_outer", "k_add_correction",
-    "k_cg_dirapply<double>", "k_cg_update<double, double>",
-    "k_cg_dirapply<float>", "k_cg_update<float, float>",
-    "k_mg_check", "k_mg_sweep<false, false, true, false>",
-    "k_mg_sweep<false, true, true, false>", "k_mg_sweep<false, false, true, true>",
+    "k_cg_dirapply_z<double>", "k_cg_update<double, double>",
+    "k_cg_dirapply_z<float>", "k_cg_update<float, float>",
+    "k_mg_check_z", "k_mg_sweep0_z<false, false>",
+    "k_mg_sweep0_z<true, false>", "k_mg_sweep0_z<false, true>",
     "k_mg_sweep<true, false, false, false>", "k_mg_sweep<false, false, false, false>",
     "k_mg_sweep<false, true, false, false>", "k_mg_restrict<true>", "k_mg_restrict<false>",
     "k_mg_tail", "k_fcg_dirapply", "k_fcg_update",
@@ -1078,6 +1162,7 @@ struct fl_cavity3d {
     } *host = nullptr;
     double *outer = nullptr;  // folded outer residual: r.z, r.r, max|r|
     uint32_t blocks = 1, ew_blocks = 1;
+    ZGrid zg{1, 1, 1, 1};  // column-marching launch of the level-0 stencils
     int64_t step = 0;
     KProf prof;  // per-kernel timing (HIP events on the solver's stream)
 };
@@ -1224,6 +1309,7 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->sa, &s->host->a, sizeof(CgStateA), hipMemcpyHostToDevice, stm));
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->sb, &s->host->b, sizeof(CgStateB), hipMemcpyHostToDevice, stm));
     const uint32_t G = s->blocks;
+    const dim3 zg(s->zg.bx, s->zg.by, s->zg.bz);
     const bool fp64 = sizeof(T) == 8;
     const int kd = fp64 ? K_DIRAPPLY_D : K_DIRAPPLY_F, ku = fp64 ? K_UPDATE_D : K_UPDATE_F;
     // algorithmic bytes: dirapply reads r, d and writes d', q; update reads
@@ -1236,11 +1322,11 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
             // iteration `launched` reads d[launched % 2] and writes the other
             T *dold = dbuf[launched & 1], *dnew = dbuf[(launched + 1) & 1];
             FL_TIMED(s, kd, launched, bd,
-                     k_cg_dirapply<T><<<G, kBlock, 0, stm>>>(s->g, prm, r, dold, dnew, q, s->pb,
-                                                             s->pa, s->sa, s->sb));
+                     k_cg_dirapply_z<T><<<zg, kZT, 0, stm>>>(s->g, prm, r, dold, dnew, q, s->pb, G,
+                                                              s->pa, s->sa, s->sb));
             FL_TIMED(s, ku, launched, bu,
-                     k_cg_update<T, TX><<<G, kBlock, 0, stm>>>(s->g, x, r, dnew, q, s->pa, s->pb,
-                                                               s->sa, s->sb));
+                     k_cg_update<T, TX><<<G, kBlock, 0, stm>>>(s->g, x, r, dnew, q, s->pa, s->zg.count,
+                                                               s->pb, s->sa, s->sb));
             ++launched;
         }
         FL_HIP_TRY(ctx, hipGetLastError());
@@ -1298,13 +1384,14 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     const Level *L = s->lv;
     float *pold = s->d32[k & 1], *pnew = s->d32[(k + 1) & 1];
     const uint32_t G = s->blocks;
+    const dim3 zg(s->zg.bx, s->zg.by, s->zg.bz);
     const double n0 = L[0].n, n1 = L[1].n;
     // down: the first two pre-sweeps in one pass (into meb), the third into
     // mea, which the restriction and the up pass read
     FL_TIMED(s, K_MG_CHECK, k, 8.0 * n0,
-             k_mg_check<<<G, kBlock, 0, stm>>>(L[0], prm, s->r32, s->meb[0], s->pb, s->fa, s->fb));
+             k_mg_check_z<<<zg, kZT, 0, stm>>>(L[0], prm, s->r32, s->meb[0], s->pb, G, s->fa, s->fb));
     FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
-             k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, kCorr));
+             k_mg_sweep0_z<false, false><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, kCorr));
     FL_TIMED(s, K_MG_RESTRICT_L0, k, 8.0 * n0 + 4.0 * n1,
              k_mg_restrict<true><<<s->lblocks[1], kBlock, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->mf[1], s->fa, s->fb));
     for (int l = 1; l < t; ++l) {
@@ -1334,17 +1421,17 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
                  k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr));
     }
     FL_TIMED(s, K_MG_SWEEP_L0_PROLONG, k, 12.0 * n0 + 4.0 * n1,
-             k_mg_sweep<false, true, true, false><<<G, kBlock, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->meb[1], s->meb[0], nullptr, nullptr, s->fa, s->fb, kCorr));
+             k_mg_sweep0_z<true, false><<<zg, kZT, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->meb[1], s->meb[0], nullptr, nullptr, s->fa, s->fb, kCorr));
     FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
-             k_mg_sweep<false, false, true, false><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, kCorr));
+             k_mg_sweep0_z<false, false><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, kCorr));
     FL_TIMED(s, K_MG_SWEEP_L0_PUBLISH, k, 16.0 * n0,
-             k_mg_sweep<false, false, true, true><<<G, kBlock, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], s->q32, s->pz, s->fa, s->fb, kCorr));
+             k_mg_sweep0_z<false, true><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], s->q32, s->pz, s->fa, s->fb, kCorr));
     // FCG direction (reads z~, p_old, r; writes p, q) and update (x, r
     // read and written, p and q read)
     FL_TIMED(s, K_FCG_DIRAPPLY, k, 20.0 * n0,
-             k_fcg_dirapply<<<G, kBlock, 0, stm>>>(s->g, s->meb[0], pold, pnew, s->q32, s->r32, s->pz, s->pa, s->fa, s->fb));
+             k_fcg_dirapply<<<G, kBlock, 0, stm>>>(s->g, s->meb[0], pold, pnew, s->q32, s->r32, s->pz, s->zg.count, s->pa, s->fa, s->fb));
     FL_TIMED(s, K_FCG_UPDATE, k, 24.0 * n0,
-             k_fcg_update<<<G, kBlock, 0, stm>>>((uint32_t)s->n, s->e32, s->r32, pnew, s->q32, s->pa, s->pb, s->fa, s->fb));
+             k_fcg_update<<<G, kBlock, 0, stm>>>((uint32_t)s->n, s->e32, s->r32, pnew, s->q32, s->pa, G, s->pb, s->fa, s->fb));
     return FL_OK;
 }
 
@@ -1703,7 +1790,8 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
     s->blocks = grid_blocks(ctx, ntiles, 4);
     s->ew_blocks = grid_blocks(ctx, (s->nu + s->nv + s->nw + kBlock - 1) / kBlock);
     const bool mixed = cfg->precision == FL_PRECISION_MIXED;
-    const size_t prow = (size_t)std::max(s->blocks, s->ew_blocks) * 4;
+    s->zg = zgrid(nx, ny, nz);
+    const size_t prow = (size_t)std::max(std::max(s->blocks, s->ew_blocks), s->zg.count) * 4;
     if ((st = alloc(s, &s->u, s->nu)) || (st = alloc(s, &s->v, s->nv)) ||
         (st = alloc(s, &s->w, s->nw)) || (st = alloc(s, &s->us, s->nu)) ||
         (st = alloc(s, &s->vs, s->nv)) || (st = alloc(s, &s->ws, s->nw)) ||
