@@ -494,33 +494,54 @@ __global__ void __launch_bounds__(1024) k_mg_tail(TailLevels T, const float *__r
 
 // ------------------------------------------------ column-marching kernels
 // The level-0 stencil passes of the solvers as column marches: a block of
-// kZX x kZY threads owns as many (x, y) columns over kZK planes; a thread
-// walks its column up in z keeping the operand at k-1, k and k+1 in
-// registers, so each cell loads only the next plane's centre and its four
-// in-plane neighbours (the per-cell form loads all seven, the two z
-// neighbours a plane apart).  Each block publishes one partial row
-// (blocks in x, y, z order); consumers fold ZGrid::count rows.
-constexpr uint32_t kZX = 128, kZY = 2, kZK = 64, kZT = kZX * kZY;
+// kZX x kZY threads owns as many (x, y) columns over nz / gridDim.z planes
+// (rounded up); a thread walks its column up in z keeping the operand at
+// k-1, k and k+1 in registers, so each cell loads only the next plane's
+// centre and its four in-plane neighbours (the per-cell form loads all
+// seven, the two z neighbours a plane apart).  Each block publishes one
+// partial row (blocks in x, y, z order); consumers fold ZGrid::count rows.
+// Marches of kZK planes for the one-operand sweeps (twice the blocks of
+// kZKD: more waves in flight), kZKD for the two-operand search directions
+// (fewer chunk-edge planes read twice, fewer partial rows to fold).
+constexpr uint32_t kZX = 128, kZY = 2, kZK = 32, kZKD = 64, kZT = kZX * kZY;
 
 struct ZGrid {
     uint32_t bx, by, bz, count;
 };
-inline ZGrid zgrid(uint32_t nx, uint32_t ny, uint32_t nz) {
-    ZGrid z{(nx + kZX - 1) / kZX, (ny + kZY - 1) / kZY, (nz + kZK - 1) / kZK, 0};
+
+// The dispatcher deals workgroups to the XCDs round-robin by linear id, and
+// each XCD has its own L2.  This maps the hardware id to a logical block so
+// that each XCD gets one contiguous run of logical blocks: the y-neighbours
+// that read each other's edge rows then share an L2 instead of fetching
+// those rows from HBM twice.
+__device__ inline uint3 zblock() {
+    const uint32_t nb = gridDim.x * gridDim.y * gridDim.z;
+    const uint32_t hw = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const uint32_t q = nb / kXcds, r = nb % kXcds, x = hw % kXcds, slot = hw / kXcds;
+    const uint32_t lid = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + slot;
+    return make_uint3(lid % gridDim.x, (lid / gridDim.x) % gridDim.y, lid / (gridDim.x * gridDim.y));
+}
+inline ZGrid zgrid(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t kz) {
+    ZGrid z{(nx + kZX - 1) / kZX, (ny + kZY - 1) / kZY, (nz + kz - 1) / kz, 0};
     z.count = z.bx * z.by * z.bz;
     return z;
 }
 
 // Walks this thread's column.  load(c, i, j, k) fetches the raw values of a cell,
-// make(raw, c, i, j, k) turns them into the stencil operand; every load of a
-// plane step is issued before any operand is formed.  body(i, j, k, c, v)
+// make(raw, c, i, j, k) turns them into the stencil operand; side(c) fetches
+// centre-only values of the current cell (no neighbours); every load of a
+// plane step is issued before any operand is formed.  body(i, j, k, c, v, sd)
 // gets v = {centre, +x, -x, +y, -y, +z, -z} (neighbours clamped at the walls
-// to the cell itself).  Threads past the grid return.
-template <typename Ld, typename Mk, typename Body>
-__device__ inline void zmarch(uint32_t nx, uint32_t ny, uint32_t nz, Ld &&load, Mk &&make, Body &&body) {
-    const uint32_t i = blockIdx.x * kZX + threadIdx.x % kZX, j = blockIdx.y * kZY + threadIdx.x / kZX;
+// to the cell itself) and side's value.  Threads past the grid return.
+template <typename Ld, typename Mk, typename Sd, typename Body>
+__device__ inline void zmarch_side(uint32_t nx, uint32_t ny, uint32_t nz, Ld &&load, Mk &&make, Sd &&side,
+                                   Body &&body) {
+    const uint3 b = zblock();
+    const uint32_t i = b.x * kZX + threadIdx.x % kZX, j = b.y * kZY + threadIdx.x / kZX;
     if (i >= nx || j >= ny) return;
-    const uint32_t k0 = blockIdx.z * kZK, k1 = min(k0 + kZK, nz), nxy = nx * ny;
+    const uint32_t kz = (nz + gridDim.z - 1) / gridDim.z, k0 = b.z * kz;
+    if (k0 >= nz) return;
+    const uint32_t k1 = min(k0 + kz, nz), nxy = nx * ny;
     const uint32_t ie = i + 1 < nx ? i + 1 : i, iw = i > 0 ? i - 1 : i;
     const uint32_t jn = j + 1 < ny ? j + 1 : j, js = j > 0 ? j - 1 : j;
     const uint32_t de = ie - i, dw = i - iw, dn = (jn - j) * nx, ds = (j - js) * nx;
@@ -532,13 +553,20 @@ __device__ inline void zmarch(uint32_t nx, uint32_t ny, uint32_t nz, Ld &&load, 
         const uint32_t kp = k + 1 < nz ? k + 1 : k, cp = c + (kp - k) * nxy;
         const auto rp = load(cp, i, j, kp), re = load(c + de, ie, j, k), rw = load(c - dw, iw, j, k),
                    rn = load(c + dn, i, jn, k), rs = load(c - ds, i, js, k);
+        const auto sd = side(c);
         const auto ep = make(rp, cp, i, j, kp);
         const decltype(ec) v[7] = {ec, make(re, c + de, ie, j, k), make(rw, c - dw, iw, j, k),
                                    make(rn, c + dn, i, jn, k), make(rs, c - ds, i, js, k), ep, em};
-        body(i, j, k, c, v);
+        body(i, j, k, c, v, sd);
         em = ec;
         ec = ep;
     }
+}
+template <typename Ld, typename Mk, typename Body>
+__device__ inline void zmarch(uint32_t nx, uint32_t ny, uint32_t nz, Ld &&load, Mk &&make, Body &&body) {
+    zmarch_side(
+        nx, ny, nz, load, make, [](uint32_t) { return 0; },
+        [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const auto &v, int) { body(i, j, k, c, v); });
 }
 
 // Block reduction of NS sums and NM maxima into this block's partial row.
@@ -716,16 +744,15 @@ __global__ void __launch_bounds__(kZT) k_cg_dirapply_z(Grid g, CgParams prm, con
 }
 
 // p = (z~ - z~_0) - beta p_old, q = A p (pinned), partials p.q and r.p;
-// folds the `rows` partials of the publishing sweep.  (A column march was
-// slower here: two operands per point, and p is formed at every load.)
-__global__ void __launch_bounds__(kBlock) k_fcg_dirapply(Grid g, const float *__restrict__ zt,
-                                                         const float *__restrict__ pold,
-                                                         float *__restrict__ pnew,
-                                                         float *__restrict__ q,
-                                                         const float *__restrict__ r,
-                                                         const double *__restrict__ pz,
-                                                         uint32_t rows, double *__restrict__ pout,
-                                                         const FcgA *SA, const FcgB *SB) {
+// folds the `rows` partials of the publishing sweep.  A column march: each
+// block publishes one row, ZGrid::count rows in all.
+__global__ void __launch_bounds__(kZT) k_fcg_dirapply_z(Grid g, const float *__restrict__ zt,
+                                                        const float *__restrict__ pold,
+                                                        float *__restrict__ pnew, float *__restrict__ q,
+                                                        const float *__restrict__ r,
+                                                        const double *__restrict__ pz, uint32_t rows,
+                                                        double *__restrict__ pout, const FcgA *SA,
+                                                        const FcgB *SB) {
     if (SA->done || SB->done) return;
     const bool first = SB->first != 0;
     double fz[2], fm[1];
@@ -735,38 +762,26 @@ __global__ void __launch_bounds__(kBlock) k_fcg_dirapply(Grid g, const float *__
     const float tb = (float)beta;
     const float cx = (float)g.ix2, cy = (float)g.iy2, cz = (float)g.iz2;
     double s[2] = {0.0, 0.0}, m[1] = {0.0};
-    TileSched ts((g.n + kBlock - 1) / kBlock);
-    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
-        const uint32_t c = t * kBlock + threadIdx.x;
-        if (c >= g.n) continue;
-        float pc = 0.0f, qc = 0.0f;
-        if (c != 0) {
-            uint32_t i, j, k;
-            cell_ijk(g, c, i, j, k);
-            const Nbr o = neighbours(g, c, i, j, k);
-            const uint32_t idx[7] = {c, o.e, o.w, o.n, o.s, o.f, o.b};
-            // all loads issued before any arithmetic so they are in flight together
-            float zv[7], ov[7] = {0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-            for (int a = 0; a < 7; ++a) zv[a] = zt[idx[a]];
-            if (!first) {
-#pragma unroll
-                for (int a = 0; a < 7; ++a) ov[a] = pold[idx[a]];
-            }
-            float pv[7];
-#pragma unroll
-            for (int a = 0; a < 7; ++a)  // the pinned cell's p stays 0
-                pv[a] = idx[a] == 0 ? 0.0f : (zv[a] - z0) - tb * ov[a];
-            pc = pv[0];
-            qc = cx * ((pv[0] - pv[1]) + (pv[0] - pv[2])) + cy * ((pv[0] - pv[3]) + (pv[0] - pv[4])) +
-                 cz * ((pv[0] - pv[5]) + (pv[0] - pv[6]));
-        }
-        pnew[c] = pc;
-        q[c] = qc;
-        s[0] += (double)pc * (double)qc;
-        s[1] += (double)r[c] * (double)pc;
-    }
-    publish_partials<2, 1>(s, m, pout);
+    struct Raw {
+        float z, p;
+    };
+    zmarch_side(
+        g.nx, g.ny, g.nz,
+        [&](uint32_t c, uint32_t, uint32_t, uint32_t) -> Raw { return Raw{zt[c], first ? 0.0f : pold[c]}; },
+        [&](const Raw &v, uint32_t c, uint32_t, uint32_t, uint32_t) -> float {
+            return c == 0 ? 0.0f : (v.z - z0) - tb * v.p;  // the pinned cell's p stays 0
+        },
+        [&](uint32_t c) -> float { return r[c]; },
+        [&](uint32_t, uint32_t, uint32_t, uint32_t c, const float (&v)[7], float rc) {
+            const float qc = c == 0 ? 0.0f
+                                    : cx * ((v[0] - v[1]) + (v[0] - v[2])) + cy * ((v[0] - v[3]) + (v[0] - v[4])) +
+                                          cz * ((v[0] - v[5]) + (v[0] - v[6]));
+            pnew[c] = v[0];
+            q[c] = qc;
+            s[0] += (double)v[0] * (double)qc;
+            s[1] += (double)rc * (double)v[0];
+        });
+    publish_row<2, 1>(s, m, pout);
 }
 
 // alpha = r.p / p.q; x += alpha p, r -= alpha q; partials r.r, sum r, max|r|.
@@ -1108,7 +1123,7 @@ const char *const kKindNames[K_COUNT] = {
     "k_mg_sweep0_z<true, false>", "k_mg_sweep0_z<false, true>",
     "k_mg_sweep<true, false, false, false>", "k_mg_sweep<false, false, false, false>",
     "k_mg_sweep<false, true, false, false>", "k_mg_restrict<true>", "k_mg_restrict<false>",
-    "k_mg_tail", "k_fcg_dirapply", "k_fcg_update",
+    "k_mg_tail", "k_fcg_dirapply_z", "k_fcg_update",
     "unet_warm_start",  // restriction + U-Net inference + interpolation (many kernels)
 };
 
@@ -1162,7 +1177,8 @@ struct fl_cavity3d {
     } *host = nullptr;
     double *outer = nullptr;  // folded outer residual: r.z, r.r, max|r|
     uint32_t blocks = 1, ew_blocks = 1;
-    ZGrid zg{1, 1, 1, 1};  // column-marching launch of the level-0 stencils
+    ZGrid zg{1, 1, 1, 1};   // column-marching launch of the level-0 sweeps
+    ZGrid zgd{1, 1, 1, 1};  // and of the search directions
     int64_t step = 0;
     KProf prof;  // per-kernel timing (HIP events on the solver's stream)
 };
@@ -1309,7 +1325,7 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->sa, &s->host->a, sizeof(CgStateA), hipMemcpyHostToDevice, stm));
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->sb, &s->host->b, sizeof(CgStateB), hipMemcpyHostToDevice, stm));
     const uint32_t G = s->blocks;
-    const dim3 zg(s->zg.bx, s->zg.by, s->zg.bz);
+    const dim3 zg(s->zgd.bx, s->zgd.by, s->zgd.bz);
     const bool fp64 = sizeof(T) == 8;
     const int kd = fp64 ? K_DIRAPPLY_D : K_DIRAPPLY_F, ku = fp64 ? K_UPDATE_D : K_UPDATE_F;
     // algorithmic bytes: dirapply reads r, d and writes d', q; update reads
@@ -1325,7 +1341,7 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
                      k_cg_dirapply_z<T><<<zg, kZT, 0, stm>>>(s->g, prm, r, dold, dnew, q, s->pb, G,
                                                               s->pa, s->sa, s->sb));
             FL_TIMED(s, ku, launched, bu,
-                     k_cg_update<T, TX><<<G, kBlock, 0, stm>>>(s->g, x, r, dnew, q, s->pa, s->zg.count,
+                     k_cg_update<T, TX><<<G, kBlock, 0, stm>>>(s->g, x, r, dnew, q, s->pa, s->zgd.count,
                                                                s->pb, s->sa, s->sb));
             ++launched;
         }
@@ -1429,9 +1445,9 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     // FCG direction (reads z~, p_old, r; writes p, q) and update (x, r
     // read and written, p and q read)
     FL_TIMED(s, K_FCG_DIRAPPLY, k, 20.0 * n0,
-             k_fcg_dirapply<<<G, kBlock, 0, stm>>>(s->g, s->meb[0], pold, pnew, s->q32, s->r32, s->pz, s->zg.count, s->pa, s->fa, s->fb));
+             k_fcg_dirapply_z<<<dim3(s->zgd.bx, s->zgd.by, s->zgd.bz), kZT, 0, stm>>>(s->g, s->meb[0], pold, pnew, s->q32, s->r32, s->pz, s->zg.count, s->pa, s->fa, s->fb));
     FL_TIMED(s, K_FCG_UPDATE, k, 24.0 * n0,
-             k_fcg_update<<<G, kBlock, 0, stm>>>((uint32_t)s->n, s->e32, s->r32, pnew, s->q32, s->pa, G, s->pb, s->fa, s->fb));
+             k_fcg_update<<<G, kBlock, 0, stm>>>((uint32_t)s->n, s->e32, s->r32, pnew, s->q32, s->pa, s->zgd.count, s->pb, s->fa, s->fb));
     return FL_OK;
 }
 
@@ -1790,8 +1806,10 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
     s->blocks = grid_blocks(ctx, ntiles, 4);
     s->ew_blocks = grid_blocks(ctx, (s->nu + s->nv + s->nw + kBlock - 1) / kBlock);
     const bool mixed = cfg->precision == FL_PRECISION_MIXED;
-    s->zg = zgrid(nx, ny, nz);
-    const size_t prow = (size_t)std::max(std::max(s->blocks, s->ew_blocks), s->zg.count) * 4;
+    s->zg = zgrid(nx, ny, nz, kZK);
+    s->zgd = zgrid(nx, ny, nz, kZKD);
+    const size_t prow =
+        (size_t)std::max(std::max(s->blocks, s->ew_blocks), std::max(s->zg.count, s->zgd.count)) * 4;
     if ((st = alloc(s, &s->u, s->nu)) || (st = alloc(s, &s->v, s->nv)) ||
         (st = alloc(s, &s->w, s->nw)) || (st = alloc(s, &s->us, s->nu)) ||
         (st = alloc(s, &s->vs, s->nv)) || (st = alloc(s, &s->ws, s->nw)) ||
