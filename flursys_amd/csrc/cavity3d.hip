@@ -33,6 +33,7 @@
 // x += e in FP64, until the outer FP64 stop test holds.
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -357,14 +358,13 @@ __global__ void __launch_bounds__(kBlock) k_mg_sweep(Level L, Level LC, const fl
     if (kPublish) publish_partials<2, 1>(s, m, pout);
 }
 
-// Restriction of the residual: f_c(C) = mean over the 8 children of f - A e.
-template <bool kLevel0>
+// Restriction of the residual on a coarse level (level 0 has its column
+// march, k_mg_restrict0_z): f_c(C) = mean over the 8 children of f - A e.
 __global__ void __launch_bounds__(kBlock) k_mg_restrict(Level L, Level LC, const float *__restrict__ f,
                                                         const float *__restrict__ e,
                                                         float *__restrict__ fcoarse, const FcgA *SA,
                                                         const FcgB *SB) {
     if (SA->done || SB->done) return;
-    const float f0 = kLevel0 ? (float)(-SA->sumr) : 0.0f;
     for (uint32_t C = blockIdx.x * blockDim.x + threadIdx.x; C < LC.n; C += gridDim.x * blockDim.x) {
         uint32_t I, J, K;
         lvl_ijk(LC, C, I, J, K);
@@ -379,26 +379,34 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Level L, Level LC, const
             const float ef = e[k + 1 < L.nz ? c + L.nxy : c], eb = e[k > 0 ? c - L.nxy : c];
             const float ae = L.cx * ((ec - ee) + (ec - ew)) + L.cy * ((ec - en) + (ec - es)) +
                              L.cz * ((ec - ef) + (ec - eb));
-            acc += f_at(f, c, kLevel0, f0) - ae;
+            acc += f[c] - ae;
         }
         fcoarse[C] = 0.125f * acc;
     }
 }
 
 // The levels with <= 4096 cells: a whole V-cycle (the same sweeps and
-// transfers) in one workgroup out of LDS; the coarsest level gets 60 sweeps
-// and its mean removed.  `lv` holds levels t..nl-1; f is level t's right-hand
-// side and e receives its solution.
+// transfers) in one workgroup out of LDS.  `lv` holds levels t..nl-1 (at
+// most three: a level of <= 4096 cells halves only from 16^3, to 8^3 and
+// then 4^3); f is level t's right-hand side and e receives its solution.
+// The coarsest level is solved exactly when `pinv` holds its dense
+// pseudo-inverse (n x n, symmetric), else by 60 sweeps with the mean removed
+// (the Neumann null space).
+constexpr int kTailMax = 3;
+constexpr uint32_t kDirectCells = 128;
 struct TailLevels {
-    Level lv[kMaxLevels];
+    Level lv[kTailMax];
     int count;
+    const float *pinv;
 };
-__device__ inline void tail_sweep(const Level &L, const float *f, const float *ein, float *eout,
-                                  const float *ec, const Level *LC, bool first) {
+// `itab` is the level's reciprocal-diagonal table in LDS (a per-lane index
+// into the kernel-argument copy would put the whole argument in scratch).
+__device__ inline void tail_sweep(const Level &L, const float *itab, const float *f, const float *ein,
+                                  float *eout, const float *ec, const Level *LC, bool first) {
     for (uint32_t c = threadIdx.x; c < L.n; c += blockDim.x) {
         uint32_t i, j, k;
         lvl_ijk(L, c, i, j, k);
-        const float inv = L.inv[lvl_didx(L, i, j, k)];
+        const float inv = itab[lvl_didx(L, i, j, k)];
         if (first) {
             eout[c] = kOmega * inv * f[c];
             continue;
@@ -407,6 +415,7 @@ __device__ inline void tail_sweep(const Level &L, const float *f, const float *e
         const uint32_t jj[7] = {j, j, j, j + 1 < L.ny ? j + 1 : j, j > 0 ? j - 1 : j, j, j};
         const uint32_t kk[7] = {k, k, k, k, k, k + 1 < L.nz ? k + 1 : k, k > 0 ? k - 1 : k};
         float ev[7];
+#pragma unroll
         for (int a = 0; a < 7; ++a) {
             ev[a] = ein[ii[a] + L.nx * (jj[a] + L.ny * kk[a])];
             if (ec) ev[a] += kCorr * ec[(ii[a] >> 1) + LC->nx * ((jj[a] >> 1) + LC->ny * (kk[a] >> 1))];
@@ -418,78 +427,272 @@ __device__ inline void tail_sweep(const Level &L, const float *f, const float *e
     }
     __syncthreads();
 }
-__global__ void __launch_bounds__(1024) k_mg_tail(TailLevels T, const float *__restrict__ fin,
-                                                  float *__restrict__ eout, const FcgA *SA,
-                                                  const FcgB *SB) {
-    if (SA->done || SB->done) return;
-    // LDS: per level f, e, w (ping-pong); levels shrink by 8 so 3 * 1.15 * 4096 floats suffice
-    __shared__ float pool[3 * 4800];
-    float *F[kMaxLevels], *E[kMaxLevels], *W[kMaxLevels];
+// The V-cycle over C tail levels; every loop over levels is unrolled so the
+// per-level LDS pointers stay in registers.
+template <int C>
+__device__ inline void tail_vcycle(const TailLevels &T, float *pool, float *red, float *itab,
+                                   const float *__restrict__ fin, float *__restrict__ eout) {
+    float *F[C], *E[C], *W[C], *I[C];
     uint32_t off = 0;
-    for (int l = 0; l < T.count; ++l) {
+#pragma unroll
+    for (int l = 0; l < C; ++l) {
         F[l] = pool + off; off += T.lv[l].n;
         E[l] = pool + off; off += T.lv[l].n;
         W[l] = pool + off; off += T.lv[l].n;
+        I[l] = itab + 8 * l;
+        if (threadIdx.x < 8) I[l][threadIdx.x] = T.lv[l].inv[threadIdx.x];
     }
     for (uint32_t c = threadIdx.x; c < T.lv[0].n; c += blockDim.x) F[0][c] = fin[c];
     __syncthreads();
-    const int last = T.count - 1;
+    constexpr int last = C - 1;
     // down: pre-smooth (result in E), restrict into F[l+1]
+#pragma unroll
     for (int l = 0; l < last; ++l) {
         const Level &L = T.lv[l];
-        tail_sweep(L, F[l], nullptr, E[l], nullptr, nullptr, true);
-        tail_sweep(L, F[l], E[l], W[l], nullptr, nullptr, false);
-        tail_sweep(L, F[l], W[l], E[l], nullptr, nullptr, false);
+        tail_sweep(L, I[l], F[l], nullptr, E[l], nullptr, nullptr, true);
+        tail_sweep(L, I[l], F[l], E[l], W[l], nullptr, nullptr, false);
+        tail_sweep(L, I[l], F[l], W[l], E[l], nullptr, nullptr, false);
         const Level &LC = T.lv[l + 1];
-        for (uint32_t C = threadIdx.x; C < LC.n; C += blockDim.x) {
+        const float *e = E[l];
+        for (uint32_t cc = threadIdx.x; cc < LC.n; cc += blockDim.x) {
             uint32_t I, J, K;
-            lvl_ijk(LC, C, I, J, K);
+            lvl_ijk(LC, cc, I, J, K);
             float acc = 0.0f;
+#pragma unroll
             for (int a = 0; a < 8; ++a) {
                 const uint32_t i = 2 * I + (a & 1), j = 2 * J + (a >> 1 & 1), k = 2 * K + (a >> 2);
                 const uint32_t c = i + L.nx * (j + L.ny * k);
-                const float *e = E[l];
                 const float ec = e[c];
                 const float ae = L.cx * ((ec - e[i + 1 < L.nx ? c + 1 : c]) + (ec - e[i > 0 ? c - 1 : c])) +
                                  L.cy * ((ec - e[j + 1 < L.ny ? c + L.nx : c]) + (ec - e[j > 0 ? c - L.nx : c])) +
                                  L.cz * ((ec - e[k + 1 < L.nz ? c + L.nxy : c]) + (ec - e[k > 0 ? c - L.nxy : c]));
                 acc += F[l][c] - ae;
             }
-            F[l + 1][C] = 0.125f * acc;
+            F[l + 1][cc] = 0.125f * acc;
         }
         __syncthreads();
     }
-    // coarsest: 60 sweeps from zero, then remove the mean (Neumann null space)
-    {
-        const Level &L = T.lv[last];
-        tail_sweep(L, F[last], nullptr, E[last], nullptr, nullptr, true);
-        for (int it = 0; it < 59; it += 2) {
-            tail_sweep(L, F[last], E[last], W[last], nullptr, nullptr, false);
-            tail_sweep(L, F[last], W[last], E[last], nullptr, nullptr, false);
+    // coarsest level, solution in E[last]
+    const Level &LL = T.lv[last];
+    if (T.pinv) {
+        for (uint32_t c = threadIdx.x; c < LL.n; c += blockDim.x) {
+            float acc = 0.0f;
+            for (uint32_t j = 0; j < LL.n; ++j) acc += T.pinv[(size_t)j * LL.n + c] * F[last][j];
+            E[last][c] = acc;
         }
-        __shared__ float red[1024];
+        __syncthreads();
+    } else {
+        tail_sweep(LL, I[last], F[last], nullptr, E[last], nullptr, nullptr, true);
+        for (int it = 0; it < 59; it += 2) {
+            tail_sweep(LL, I[last], F[last], E[last], W[last], nullptr, nullptr, false);
+            tail_sweep(LL, I[last], F[last], W[last], E[last], nullptr, nullptr, false);
+        }
         float acc = 0.0f;
-        for (uint32_t c = threadIdx.x; c < L.n; c += blockDim.x) acc += E[last][c];
+        for (uint32_t c = threadIdx.x; c < LL.n; c += blockDim.x) acc += E[last][c];
         red[threadIdx.x] = acc;
         __syncthreads();
         for (int w = blockDim.x / 2; w > 0; w >>= 1) {
             if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
             __syncthreads();
         }
-        const float mean = red[0] / (float)L.n;
-        for (uint32_t c = threadIdx.x; c < L.n; c += blockDim.x) E[last][c] -= mean;
+        const float mean = red[0] / (float)LL.n;
+        for (uint32_t c = threadIdx.x; c < LL.n; c += blockDim.x) E[last][c] -= mean;
         __syncthreads();
     }
-    // up: prolongate-correct fused into the first post-sweep; result in E
+    // up: prolongate-correct fused into the first post-sweep; a level's
+    // result ends in W
+    const float *sol = E[last];
+#pragma unroll
     for (int l = last - 1; l >= 0; --l) {
         const Level &L = T.lv[l];
-        tail_sweep(L, F[l], E[l], W[l], E[l + 1], &T.lv[l + 1], false);
-        tail_sweep(L, F[l], W[l], E[l], nullptr, nullptr, false);
-        tail_sweep(L, F[l], E[l], W[l], nullptr, nullptr, false);
-        for (uint32_t c = threadIdx.x; c < L.n; c += blockDim.x) E[l][c] = W[l][c];
-        __syncthreads();
+        tail_sweep(L, I[l], F[l], E[l], W[l], sol, &T.lv[l + 1], false);
+        tail_sweep(L, I[l], F[l], W[l], E[l], nullptr, nullptr, false);
+        tail_sweep(L, I[l], F[l], E[l], W[l], nullptr, nullptr, false);
+        sol = W[l];
     }
-    for (uint32_t c = threadIdx.x; c < T.lv[0].n; c += blockDim.x) eout[c] = E[0][c];
+    for (uint32_t c = threadIdx.x; c < T.lv[0].n; c += blockDim.x) eout[c] = sol[c];
+}
+__global__ void __launch_bounds__(1024) k_mg_tail(TailLevels T, const float *__restrict__ fin,
+                                                  float *__restrict__ eout, const FcgA *SA,
+                                                  const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    // LDS: per level f, e, w (ping-pong); levels shrink by 8 so 3 * 1.15 * 4096 floats suffice
+    __shared__ float pool[3 * 4800];
+    __shared__ float red[1024];
+    __shared__ float itab[8 * kTailMax];
+    if (T.count == 1) tail_vcycle<1>(T, pool, red, itab, fin, eout);
+    else if (T.count == 2) tail_vcycle<2>(T, pool, red, itab, fin, eout);
+    else tail_vcycle<3>(T, pool, red, itab, fin, eout);
+}
+
+// ------------------------------------------------ fused coarse-level passes
+// A level's down pass (two sweeps from zero, a third sweep, the residual
+// restriction) and its up pass (prolongate-correct and three sweeps) as one
+// launch each.  A block owns a kFT^3 tile of fine cells and recomputes, in
+// LDS, each intermediate iterate over the halo the later stages read: the
+// down pass stages f over a halo of 3, the second iterate over 2, the third
+// over 1; the up pass likewise.  Every array shares one (kFT + 6)^3 frame, so
+// a neighbour is +-1, +-kFH, +-kFH^2 away.  Neighbours outside the domain
+// are the cell itself (the Neumann clamp), so they drop out of A e.
+constexpr int kFT = 8, kFH = kFT + 6, kFF = kFH * kFH * kFH;
+constexpr uint32_t kFB = 1024;
+
+struct FTile {
+    int x0, y0, z0;  // global coordinates of frame cell (0, 0, 0)
+};
+__device__ inline FTile ftile(const Level &L) {
+    const uint32_t tx = (L.nx + kFT - 1) / kFT, ty = (L.ny + kFT - 1) / kFT;
+    // XCD-contiguous tile order (the dispatcher deals blocks round-robin)
+    const uint32_t nb = gridDim.x, q = nb / kXcds, r = nb % kXcds, x = blockIdx.x % kXcds,
+                   slot = blockIdx.x / kXcds;
+    const uint32_t t = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + slot;
+    return FTile{(int)(t % tx) * kFT - 3, (int)((t / tx) % ty) * kFT - 3, (int)(t / (tx * ty)) * kFT - 3};
+}
+// Visits the frame cells of the tile grown by halo h that lie in the domain:
+// fn(frame index, x, y, z).
+template <int H, typename Fn>
+__device__ inline void fvisit(const Level &L, const FTile &T, Fn &&fn) {
+    constexpr int E = kFT + 2 * H, N = E * E * E, o = 3 - H;
+    for (int t = threadIdx.x; t < N; t += kFB) {
+        const int a = t % E + o, b = (t / E) % E + o, c = t / (E * E) + o;
+        const int x = T.x0 + a, y = T.y0 + b, z = T.z0 + c;
+        if (x < 0 || y < 0 || z < 0 || x >= (int)L.nx || y >= (int)L.ny || z >= (int)L.nz) continue;
+        fn(a + kFH * (b + kFH * c), x, y, z);
+    }
+}
+// Staging form of fvisit over the full halo of 3: ld(global index, x, y, z) runs for
+// every cell of this thread first (coordinates clamped into the domain, so
+// all of them are issued together), then st(frame index, x, y, z, value)
+// for the cells that lie in the domain.
+template <typename Ld, typename St>
+__device__ inline void fstage(const Level &L, const FTile &T, Ld &&ld, St &&st) {
+    constexpr int E = kFH, N = kFF, R = (N + kFB - 1) / kFB;
+    decltype(ld(0u, 0, 0, 0)) v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int t = min((int)threadIdx.x + q * (int)kFB, N - 1);
+        const int x = min(max(T.x0 + t % E, 0), (int)L.nx - 1);
+        const int y = min(max(T.y0 + (t / E) % E, 0), (int)L.ny - 1);
+        const int z = min(max(T.z0 + t / (E * E), 0), (int)L.nz - 1);
+        v[q] = ld((uint32_t)x + L.nx * ((uint32_t)y + L.ny * (uint32_t)z), x, y, z);
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int t = (int)threadIdx.x + q * (int)kFB;
+        if (t >= N) break;
+        const int x = T.x0 + t % E, y = T.y0 + (t / E) % E, z = T.z0 + t / (E * E);
+        if (x < 0 || y < 0 || z < 0 || x >= (int)L.nx || y >= (int)L.ny || z >= (int)L.nz) continue;
+        st(t, x, y, z, v[q]);
+    }
+}
+struct FNbr {
+    bool e, w, n, s, f, b;
+};
+__device__ inline FNbr fnbr(const Level &L, int x, int y, int z) {
+    return FNbr{x + 1 < (int)L.nx, x > 0, y + 1 < (int)L.ny, y > 0, z + 1 < (int)L.nz, z > 0};
+}
+__device__ inline uint32_t fdidx(const FNbr &m) {
+    return (uint32_t)(m.e && m.w) | ((uint32_t)(m.n && m.s) << 1) | ((uint32_t)(m.f && m.b) << 2);
+}
+// (A e)(idx) over the frame array e, clamped neighbours dropped
+__device__ inline float flap(const Level &L, const float *e, int idx, const FNbr &m) {
+    const float c = e[idx];
+    return L.cx * ((m.e ? c - e[idx + 1] : 0.0f) + (m.w ? c - e[idx - 1] : 0.0f)) +
+           L.cy * ((m.n ? c - e[idx + kFH] : 0.0f) + (m.s ? c - e[idx - kFH] : 0.0f)) +
+           L.cz * ((m.f ? c - e[idx + kFH * kFH] : 0.0f) + (m.b ? c - e[idx - kFH * kFH] : 0.0f));
+}
+__device__ inline uint32_t fglobal(const Level &L, int x, int y, int z) {
+    return (uint32_t)x + L.nx * ((uint32_t)y + L.ny * (uint32_t)z);
+}
+
+// Down pass of level L: e3 (three damped-Jacobi sweeps from zero on f) into
+// e_out, the restricted residual mean_8(f - A e3) into the next level's f.
+__global__ void __launch_bounds__(kFB) k_mg_down(Level L, Level LC, const float *__restrict__ f,
+                                                 float *__restrict__ eout, float *__restrict__ fcoarse,
+                                                 const FcgA *SA, const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    __shared__ float sf[kFF], s2[kFF], s3[kFF];
+    const float *inv = lvl_inv_table(L);
+    const FTile T = ftile(L);
+    // f, and e1 = omega D^-1 f (pointwise), over the halo of 3
+    fstage(L, T, [&](uint32_t g, int, int, int) { return f[g]; },
+           [&](int id, int x, int y, int z, float v) {
+               sf[id] = v;
+               s3[id] = kOmega * inv[fdidx(fnbr(L, x, y, z))] * v;
+           });
+    __syncthreads();
+    // e2 = e1 + omega D^-1 (f - A e1) over 2, e3 = e2 + omega D^-1 (f - A e2) over 1
+    fvisit<2>(L, T, [&](int id, int x, int y, int z) {
+        const FNbr m = fnbr(L, x, y, z);
+        s2[id] = s3[id] + kOmega * inv[fdidx(m)] * (sf[id] - flap(L, s3, id, m));
+    });
+    __syncthreads();
+    fvisit<1>(L, T, [&](int id, int x, int y, int z) {
+        const FNbr m = fnbr(L, x, y, z);
+        s3[id] = s2[id] + kOmega * inv[fdidx(m)] * (sf[id] - flap(L, s2, id, m));
+    });
+    __syncthreads();
+    // residual of e3 on the tile (into s2), e3 out
+    fvisit<0>(L, T, [&](int id, int x, int y, int z) {
+        const FNbr m = fnbr(L, x, y, z);
+        eout[fglobal(L, x, y, z)] = s3[id];
+        s2[id] = sf[id] - flap(L, s3, id, m);
+    });
+    __syncthreads();
+    if (threadIdx.x < (kFT / 2) * (kFT / 2) * (kFT / 2)) {
+        constexpr int h = kFT / 2;
+        const int a = 3 + 2 * (threadIdx.x % h), b = 3 + 2 * ((threadIdx.x / h) % h), c = 3 + 2 * (threadIdx.x / (h * h));
+        const int x = T.x0 + a, y = T.y0 + b, z = T.z0 + c;
+        if (x < (int)L.nx && y < (int)L.ny && z < (int)L.nz) {
+            const int id = a + kFH * (b + kFH * c);
+            float acc = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc += s2[id + (q & 1) + kFH * ((q >> 1) & 1) + kFH * kFH * (q >> 2)];
+            fcoarse[(uint32_t)(x >> 1) + LC.nx * ((uint32_t)(y >> 1) + LC.ny * (uint32_t)(z >> 1))] = 0.125f * acc;
+        }
+    }
+}
+
+// Up pass of level L: e' = e3 + corr P(e_c), then three sweeps; the result
+// into e_out.
+__global__ void __launch_bounds__(kFB) k_mg_up(Level L, Level LC, const float *__restrict__ f,
+                                               const float *__restrict__ e3, const float *__restrict__ ec,
+                                               float *__restrict__ eout, const FcgA *SA, const FcgB *SB,
+                                               float corr) {
+    if (SA->done || SB->done) return;
+    __shared__ float sf[kFF], sa[kFF], sb[kFF];
+    const float *inv = lvl_inv_table(L);
+    const FTile T = ftile(L);
+    struct Raw {
+        float e, c, f;
+    };
+    fstage(L, T,
+           [&](uint32_t g, int x, int y, int z) {
+               return Raw{e3[g], ec[(uint32_t)(x >> 1) + LC.nx * ((uint32_t)(y >> 1) + LC.ny * (uint32_t)(z >> 1))],
+                          f[g]};
+           },
+           [&](int id, int, int, int, const Raw &v) {
+               sa[id] = v.e + corr * v.c;
+               sf[id] = v.f;
+           });
+    __syncthreads();
+    fvisit<2>(L, T, [&](int id, int x, int y, int z) {
+        const FNbr m = fnbr(L, x, y, z);
+        sb[id] = sa[id] + kOmega * inv[fdidx(m)] * (sf[id] - flap(L, sa, id, m));
+    });
+    __syncthreads();
+    fvisit<1>(L, T, [&](int id, int x, int y, int z) {
+        const FNbr m = fnbr(L, x, y, z);
+        sa[id] = sb[id] + kOmega * inv[fdidx(m)] * (sf[id] - flap(L, sb, id, m));
+    });
+    __syncthreads();
+    fvisit<0>(L, T, [&](int id, int x, int y, int z) {
+        const FNbr m = fnbr(L, x, y, z);
+        eout[fglobal(L, x, y, z)] = sa[id] + kOmega * inv[fdidx(m)] * (sf[id] - flap(L, sa, id, m));
+    });
+}
+inline uint32_t ftiles(const Level &L) {
+    return ((L.nx + kFT - 1) / kFT) * ((L.ny + kFT - 1) / kFT) * ((L.nz + kFT - 1) / kFT);
 }
 
 // ------------------------------------------------ column-marching kernels
@@ -567,6 +770,54 @@ __device__ inline void zmarch(uint32_t nx, uint32_t ny, uint32_t nz, Ld &&load, 
     zmarch_side(
         nx, ny, nz, load, make, [](uint32_t) { return 0; },
         [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const auto &v, int) { body(i, j, k, c, v); });
+}
+
+// Level-0 residual restriction, f_c = mean over the 8 children of f - A e,
+// as a column march: a thread owns the column pair (i, j), (i, j + 1) with
+// j even and walks an even number of planes, so the children's y pair sums
+// in registers, the x pair across neighbouring lanes and the z pair across
+// two plane steps.  Reads f and e once, writes f_c.
+constexpr uint32_t kRX = 128;
+__global__ void __launch_bounds__(kRX) k_mg_restrict0_z(Level L, Level LC, const float *__restrict__ f,
+                                                        const float *__restrict__ e,
+                                                        float *__restrict__ fcoarse, const FcgA *SA,
+                                                        const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    const float f0 = (float)(-SA->sumr);
+    const uint3 b = zblock();
+    const uint32_t nx = L.nx, ny = L.ny, nz = L.nz, nxy = L.nxy;
+    const uint32_t kz = (((nz + gridDim.z - 1) / gridDim.z) + 1) & ~1u, k0 = b.z * kz;
+    if (k0 >= nz) return;  // uniform over the block
+    const uint32_t k1 = min(k0 + kz, nz);
+    const uint32_t iraw = b.x * kRX + threadIdx.x, j = 2 * b.y;
+    const bool live = iraw < nx;  // nx is even, so a live lane's x partner is live
+    const uint32_t i = live ? iraw : nx - 1;
+    const uint32_t de = i + 1 < nx ? 1 : 0, dw = i > 0 ? 1 : 0;
+    const uint32_t ds = j > 0 ? nx : 0, dn = j + 2 < ny ? nx : 0;  // row j+1 always exists
+    uint32_t c = i + nx * (j + ny * k0);  // row j; row j+1 is c + nx
+    const uint32_t cb = k0 > 0 ? c - nxy : c;
+    float am = e[cb], bm = e[cb + nx], ac = e[c], bc = e[c + nx];
+    const float cx = L.cx, cy = L.cy, cz = L.cz;
+    float acc = 0.0f;
+    for (uint32_t k = k0; k < k1; ++k, c += nxy) {
+        const uint32_t cp = k + 1 < nz ? c + nxy : c;
+        const float ap = e[cp], bp = e[cp + nx];
+        const float ae = e[c + de], aw = e[c - dw], be = e[c + nx + de], bw = e[c + nx - dw];
+        const float as = e[c - ds], bn = e[c + nx + dn];
+        const float fa = (c == 0) ? f0 : f[c], fb = f[c + nx];
+        const float ra = fa - (cx * ((ac - ae) + (ac - aw)) + cy * ((ac - bc) + (ac - as)) +
+                               cz * ((ac - ap) + (ac - am)));
+        const float rb = fb - (cx * ((bc - be) + (bc - bw)) + cy * ((bc - bn) + (bc - ac)) +
+                               cz * ((bc - bp) + (bc - bm)));
+        float sum = ra + rb;
+        sum += __shfl_xor(sum, 1);
+        if (((k - k0) & 1) == 0) {
+            acc = sum;
+        } else if (live && (i & 1) == 0) {
+            fcoarse[(i >> 1) + LC.nx * ((j >> 1) + LC.ny * (k >> 1))] = 0.125f * (acc + sum);
+        }
+        am = ac; ac = ap; bm = bc; bc = bp;
+    }
 }
 
 // Block reduction of NS sums and NM maxima into this block's partial row.
@@ -1111,8 +1362,8 @@ enum Kind {
     K_RESIDUAL_D, K_RESIDUAL_F, K_FOLD_OUTER, K_ADD_CORRECTION,
     K_DIRAPPLY_D, K_UPDATE_D, K_DIRAPPLY_F, K_UPDATE_F,
     K_MG_CHECK, K_MG_SWEEP_L0, K_MG_SWEEP_L0_PROLONG, K_MG_SWEEP_L0_PUBLISH,
-    K_MG_SWEEP_FIRST, K_MG_SWEEP, K_MG_SWEEP_PROLONG, K_MG_RESTRICT_L0, K_MG_RESTRICT,
-    K_MG_TAIL, K_FCG_DIRAPPLY, K_FCG_UPDATE, K_WARM_START, K_COUNT
+    K_MG_RESTRICT_L0, K_MG_DOWN, K_MG_TAIL, K_MG_UP, K_FCG_DIRAPPLY, K_FCG_UPDATE, K_WARM_START,
+    K_COUNT
 };
 const char *const kKindNames[K_COUNT] = {
     "k_predict", "k_rhs", "k_correct", "k_diag", "k_finite",
@@ -1121,9 +1372,7 @@ const char *const kKindNames[K_COUNT] = {
     "k_cg_dirapply_z<float>", "k_cg_update<float, float>",
     "k_mg_check_z", "k_mg_sweep0_z<false, false>",
     "k_mg_sweep0_z<true, false>", "k_mg_sweep0_z<false, true>",
-    "k_mg_sweep<true, false, false, false>", "k_mg_sweep<false, false, false, false>",
-    "k_mg_sweep<false, true, false, false>", "k_mg_restrict<true>", "k_mg_restrict<false>",
-    "k_mg_tail", "k_fcg_dirapply_z", "k_fcg_update",
+    "k_mg_restrict0_z", "k_mg_down", "k_mg_tail", "k_mg_up", "k_fcg_dirapply_z", "k_fcg_update",
     "unet_warm_start",  // restriction + U-Net inference + interpolation (many kernels)
 };
 
@@ -1155,6 +1404,7 @@ struct fl_cavity3d {
     Level lv[kMaxLevels]{};
     float *mf[kMaxLevels] = {}, *mea[kMaxLevels] = {}, *meb[kMaxLevels] = {};
     uint32_t lblocks[kMaxLevels] = {};
+    float *pinv = nullptr;  // coarsest level's pseudo-inverse (<= kDirectCells cells)
     // learned warm start (FL_PRECOND_LEARNED): the network works on level
     // wl_level; wl_in / wl_out hold its input and output there
     fl_unet *unet = nullptr;
@@ -1214,7 +1464,7 @@ void free_all(fl_cavity3d *s) {
                     s->sb,  s->dg,     s->outer};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    void *more[] = {s->pz, s->fa, s->fb, s->wl_in, s->wl_out, s->pair};
+    void *more[] = {s->pz, s->fa, s->fb, s->wl_in, s->wl_out, s->pair, s->pinv};
     for (void *b : more)
         if (b) (void)hipFree(b);
     for (int l = 0; l < kMaxLevels; ++l) {
@@ -1224,6 +1474,62 @@ void free_all(fl_cavity3d *s) {
     }
     if (s->host) (void)hipHostFree(s->host);
     s->prof.destroy();
+}
+
+// Dense pseudo-inverse of a level's operator (the 7-point Neumann Laplacian
+// with weights cx, cy, cz; null space the constants), for the exact solve of
+// the coarsest level:  P = M^-1 - 11^T / (s n) with M = A + (s / n) 11^T,
+// s = mean diagonal.  M is SPD, M^-1 maps the mean-free vectors to
+// themselves, so P b is the mean-free solution of A e = b - mean(b).
+// Gauss-Jordan elimination in double; n <= kDirectCells.
+std::vector<float> coarse_pseudo_inverse(const Level &L) {
+    const uint32_t n = L.n;
+    std::vector<double> M((size_t)n * n, 0.0), I((size_t)n * n, 0.0);
+    const double w[3] = {L.cx, L.cy, L.cz};
+    double diag = 0.0;
+    for (uint32_t c = 0; c < n; ++c) {
+        const uint32_t i = c % L.nx, j = (c / L.nx) % L.ny, k = c / L.nxy;
+        const int64_t cc = c;
+        const int64_t nb[6] = {i + 1 < L.nx ? cc + 1 : -1,     i > 0 ? cc - 1 : -1,
+                               j + 1 < L.ny ? cc + L.nx : -1,  j > 0 ? cc - L.nx : -1,
+                               k + 1 < L.nz ? cc + L.nxy : -1, k > 0 ? cc - L.nxy : -1};
+        for (int a = 0; a < 6; ++a) {
+            if (nb[a] < 0) continue;
+            M[(size_t)c * n + c] += w[a / 2];
+            M[(size_t)c * n + nb[a]] -= w[a / 2];
+        }
+        diag += M[(size_t)c * n + c];
+        I[(size_t)c * n + c] = 1.0;
+    }
+    const double sc = diag / n;
+    for (size_t a = 0; a < (size_t)n * n; ++a) M[a] += sc / n;
+    for (uint32_t col = 0; col < n; ++col) {
+        uint32_t piv = col;
+        for (uint32_t r = col + 1; r < n; ++r)
+            if (std::fabs(M[(size_t)r * n + col]) > std::fabs(M[(size_t)piv * n + col])) piv = r;
+        if (piv != col)
+            for (uint32_t a = 0; a < n; ++a) {
+                std::swap(M[(size_t)piv * n + a], M[(size_t)col * n + a]);
+                std::swap(I[(size_t)piv * n + a], I[(size_t)col * n + a]);
+            }
+        const double d = 1.0 / M[(size_t)col * n + col];
+        for (uint32_t a = 0; a < n; ++a) {
+            M[(size_t)col * n + a] *= d;
+            I[(size_t)col * n + a] *= d;
+        }
+        for (uint32_t r = 0; r < n; ++r) {
+            if (r == col) continue;
+            const double f = M[(size_t)r * n + col];
+            if (f == 0.0) continue;
+            for (uint32_t a = 0; a < n; ++a) {
+                M[(size_t)r * n + a] -= f * M[(size_t)col * n + a];
+                I[(size_t)r * n + a] -= f * I[(size_t)col * n + a];
+            }
+        }
+    }
+    std::vector<float> P((size_t)n * n);
+    for (size_t a = 0; a < (size_t)n * n; ++a) P[a] = (float)(I[a] - 1.0 / (sc * n));
+    return P;
 }
 
 // Multigrid levels: halve every axis while all are even and at least 8;
@@ -1259,6 +1565,9 @@ fl_status build_hierarchy(fl_cavity3d *s) {
     int t = 1;
     while (s->lv[t].n > kTailCells) ++t;
     s->tail = t;
+    if (nl - t > kTailMax)
+        return set_error(s->ctx, FL_ERR_CONFIG, "multigrid tail of %d levels (at most %d)", nl - t,
+                         kTailMax);
     for (int l = 0; l < nl; ++l)
         s->lblocks[l] = grid_blocks(s->ctx, (s->lv[l].n + kBlock - 1) / kBlock, 4);
     return FL_OK;
@@ -1393,6 +1702,10 @@ fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep, double rr0_ref = 0.0)
 // and writes e_out (the first sweep of a level reads no e_in); prolongation
 // adds the coarse level once, the publishing sweep q_old; a restriction
 // reads f and e of the fine level and writes the coarse f.
+// Levels from kFusedFrom on run their down and up passes as one fused launch
+// each (k_mg_down / k_mg_up); finer ones as separate sweep and restriction
+// launches.
+constexpr int kFusedFrom = 2;
 fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k) {
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
@@ -1409,32 +1722,39 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
              k_mg_sweep0_z<false, false><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, kCorr));
     FL_TIMED(s, K_MG_RESTRICT_L0, k, 8.0 * n0 + 4.0 * n1,
-             k_mg_restrict<true><<<s->lblocks[1], kBlock, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->mf[1], s->fa, s->fb));
+             k_mg_restrict0_z<<<dim3((L[0].nx + kRX - 1) / kRX, L[0].ny / 2, s->zg.bz), kRX, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->mf[1], s->fa, s->fb));
     for (int l = 1; l < t; ++l) {
-        const uint32_t g = s->lblocks[l];
         const double nl = L[l].n, nc = L[l + 1].n;
-        FL_TIMED(s, K_MG_SWEEP_FIRST, k, 8.0 * nl,
-                 k_mg_sweep<true, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], nullptr, nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr));
-        FL_TIMED(s, K_MG_SWEEP, k, 12.0 * nl,
-                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb, kCorr));
-        FL_TIMED(s, K_MG_RESTRICT, k, 8.0 * nl + 4.0 * nc,
-                 k_mg_restrict<false><<<s->lblocks[l + 1], kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->mf[l + 1], s->fa, s->fb));
+        if (l >= kFusedFrom) {
+            FL_TIMED(s, K_MG_DOWN, k, 8.0 * nl + 4.0 * nc,
+                     k_mg_down<<<ftiles(L[l]), kFB, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->mf[l + 1], s->fa, s->fb));
+        } else {
+            const uint32_t g = s->lblocks[l];
+            FL_TIMED(s, K_MG_DOWN, k, 8.0 * nl + 4.0 * nc,
+                     k_mg_sweep<true, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], nullptr, nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr);
+                     k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb, kCorr);
+                     k_mg_restrict<<<s->lblocks[l + 1], kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->mf[l + 1], s->fa, s->fb));
+        }
     }
     TailLevels T{};
     T.count = s->nlev - t;
     for (int l = 0; l < T.count; ++l) T.lv[l] = L[t + l];
+    T.pinv = s->pinv;
     FL_TIMED(s, K_MG_TAIL, k, 8.0 * L[t].n,
              k_mg_tail<<<1, 1024, 0, stm>>>(T, s->mf[t], s->meb[t], s->fa, s->fb));
     // up: the solution of level l ends in meb[l]
     for (int l = t - 1; l >= 1; --l) {
-        const uint32_t g = s->lblocks[l];
         const double nl = L[l].n, nc = L[l + 1].n;
-        FL_TIMED(s, K_MG_SWEEP_PROLONG, k, 12.0 * nl + 4.0 * nc,
-                 k_mg_sweep<false, true, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->meb[l + 1], s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr));
-        FL_TIMED(s, K_MG_SWEEP, k, 12.0 * nl,
-                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb, kCorr));
-        FL_TIMED(s, K_MG_SWEEP, k, 12.0 * nl,
-                 k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr));
+        if (l >= kFusedFrom) {
+            FL_TIMED(s, K_MG_UP, k, 12.0 * nl + 4.0 * nc,
+                     k_mg_up<<<ftiles(L[l]), kFB, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->meb[l + 1], s->meb[l], s->fa, s->fb, kCorr));
+        } else {
+            const uint32_t g = s->lblocks[l];
+            FL_TIMED(s, K_MG_UP, k, 12.0 * nl + 4.0 * nc,
+                     k_mg_sweep<false, true, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->meb[l + 1], s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr);
+                     k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb, kCorr);
+                     k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr));
+        }
     }
     FL_TIMED(s, K_MG_SWEEP_L0_PROLONG, k, 12.0 * n0 + 4.0 * n1,
              k_mg_sweep0_z<true, false><<<zg, kZT, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->meb[1], s->meb[0], nullptr, nullptr, s->fa, s->fb, kCorr));
@@ -1839,6 +2159,12 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
             if (l > 0 && (st = alloc(s, &s->mf[l], n))) goto fail;
             if (l < s->tail && (st = alloc(s, &s->mea[l], n))) goto fail;
             if ((st = alloc(s, &s->meb[l], n))) goto fail;
+        }
+        if (s->lv[s->nlev - 1].n <= kDirectCells) {
+            const std::vector<float> P = coarse_pseudo_inverse(s->lv[s->nlev - 1]);
+            if ((st = alloc(s, &s->pinv, P.size()))) goto fail;
+            FL_HIP_TRY(ctx, hipMemcpyAsync(s->pinv, P.data(), P.size() * sizeof(float),
+                                           hipMemcpyHostToDevice, ctx->stream));
         }
     }
     if (hipHostMalloc(&s->host, sizeof(fl_cavity3d::Host), hipHostMallocDefault) != hipSuccess ||
