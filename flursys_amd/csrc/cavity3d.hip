@@ -1,14 +1,14 @@
 // 3D lid-driven cavity projection step on gfx950.
 //
 // Replaces LidDrivenCavity3DSolver (src/solver/incompressible_3d.rs):
-//   predict_momentum :174-273   -> k_predict
+//   predict_momentum :174-273   -> k_predict_z<0, 1, 2>
 //   build_pressure_rhs :275-291 -> k_rhs
 //   solve_pressure_poisson :293-332 -> pinned Jacobi-PCG; the reference only
 //       has SOR here, the PCG loop is the 2D solver's
 //       (src/solver/incompressible.rs:1048-1214) on the 7-point operator with
 //       cell (0,0,0) eliminated (src/poisson.rs:303-337 style)
 //   correct_velocity :334-363   -> k_correct
-//   max_divergence / max_speed / ensure_finite :504-554 -> k_diag, k_finite
+//   max_divergence / max_speed / ensure_finite :504-554 -> k_diag_z (+ k_diag_fold)
 //
 // Data layout in HBM: the reference's own (src/field.rs:104-107), one flat
 // array per field, x fastest.  The PCG vectors x (= p), r, q and two
@@ -1113,73 +1113,129 @@ __device__ inline double w_nb(const StepParams &P, const double *w, int i, int j
     return -c;
 }
 
-// predict_momentum (incompressible_3d.rs:174-273): central, advective form.
-// Boundary-normal faces are written 0 (the reference zero-fills u*, v*, w*).
-__global__ void __launch_bounds__(kBlock) k_predict(StepParams P, const double *__restrict__ u,
-                                                    const double *__restrict__ v,
-                                                    const double *__restrict__ w,
-                                                    double *__restrict__ us,
-                                                    double *__restrict__ vs,
-                                                    double *__restrict__ ws) {
-    const uint32_t nu = (P.nx + 1) * P.ny * P.nz, nv = P.nx * (P.ny + 1) * P.nz,
-                   nw = P.nx * P.ny * (P.nz + 1);
-    const double dx2 = P.dx * P.dx, dy2 = P.dy * P.dy, dz2 = P.dz * P.dz;
-    // u, v and w faces in turn, each split into per-XCD z-slabs
-    for (int a = 0; a < 3; ++a) {
-    const uint32_t base = a == 0 ? 0 : (a == 1 ? nu : nu + nv);
-    const XcdRange r(a == 0 ? nu : (a == 1 ? nv : nw));
-    for (uint32_t l = r.first; l < r.end; l += r.step) {
-        const uint32_t t = base + l;
-        if (t < nu) {
-            const int i = t % (P.nx + 1), j = (t / (P.nx + 1)) % P.ny, k = t / ((P.nx + 1) * P.ny);
-            if (i == 0 || i == (int)P.nx) { us[t] = 0.0; continue; }
-            const double c = u[t];
-            const double e = u_nb(P, u, i, j, k, 1, 0, 0, c), wv = u_nb(P, u, i, j, k, -1, 0, 0, c);
-            const double n = u_nb(P, u, i, j, k, 0, 1, 0, c), so = u_nb(P, u, i, j, k, 0, -1, 0, c);
-            const double f = u_nb(P, u, i, j, k, 0, 0, 1, c), bk = u_nb(P, u, i, j, k, 0, 0, -1, c);
-            const double vv = 0.25 * (v[V3(i - 1, j, k)] + v[V3(i, j, k)] + v[V3(i - 1, j + 1, k)] +
-                                      v[V3(i, j + 1, k)]);
-            const double ww = 0.25 * (w[W3(i - 1, j, k)] + w[W3(i, j, k)] + w[W3(i - 1, j, k + 1)] +
-                                      w[W3(i, j, k + 1)]);
-            us[t] = c + P.dt * (-c * (e - wv) / (2.0 * P.dx) - vv * (n - so) / (2.0 * P.dy) -
-                                ww * (f - bk) / (2.0 * P.dz) +
-                                P.nu * ((e - 2.0 * c + wv) / dx2 + (n - 2.0 * c + so) / dy2 +
-                                        (f - 2.0 * c + bk) / dz2));
-        } else if (t < nu + nv) {
-            const uint32_t s = t - nu;
-            const int i = s % P.nx, j = (s / P.nx) % (P.ny + 1), k = s / (P.nx * (P.ny + 1));
-            if (j == 0 || j == (int)P.ny) { vs[s] = 0.0; continue; }
-            const double c = v[s];
-            const double e = v_nb(P, v, i, j, k, 1, 0, 0, c), wv = v_nb(P, v, i, j, k, -1, 0, 0, c);
-            const double n = v_nb(P, v, i, j, k, 0, 1, 0, c), so = v_nb(P, v, i, j, k, 0, -1, 0, c);
-            const double f = v_nb(P, v, i, j, k, 0, 0, 1, c), bk = v_nb(P, v, i, j, k, 0, 0, -1, c);
-            const double uu = 0.25 * (u[U3(i, j - 1, k)] + u[U3(i + 1, j - 1, k)] + u[U3(i, j, k)] +
-                                      u[U3(i + 1, j, k)]);
-            const double ww = 0.25 * (w[W3(i, j - 1, k)] + w[W3(i, j, k)] + w[W3(i, j - 1, k + 1)] +
-                                      w[W3(i, j, k + 1)]);
-            vs[s] = c + P.dt * (-uu * (e - wv) / (2.0 * P.dx) - c * (n - so) / (2.0 * P.dy) -
-                                ww * (f - bk) / (2.0 * P.dz) +
-                                P.nu * ((e - 2.0 * c + wv) / dx2 + (n - 2.0 * c + so) / dy2 +
-                                        (f - 2.0 * c + bk) / dz2));
+// predict_momentum (incompressible_3d.rs:174-273): central, advective form,
+// one launch per face family (A = 0, 1, 2: u, v, w) as a column march over
+// that family's faces: a thread walks face column (i, j) up in z, keeping
+// the family's value at k-1, k, k+1 and the other families' planes it
+// shares with the next face in registers.  Boundary-normal faces are
+// written 0 (the reference zero-fills u*, v*, w*); neighbours outside the
+// domain follow u/v/w_neighbor (:366-447): the mirror -c, or 2 lid - c
+// above the lid for u.  The divisions by 2 dx and dx^2 are multiplications
+// by reciprocals computed once (within an ulp of the quotient).
+template <int A>
+__global__ void __launch_bounds__(kZT) k_predict_z(StepParams P, const double *__restrict__ u,
+                                                   const double *__restrict__ v,
+                                                   const double *__restrict__ w,
+                                                   double *__restrict__ out) {
+    const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
+    const int FX = nx + (A == 0), FY = ny + (A == 1), FZ = nz + (A == 2);
+    const uint3 b = zblock();
+    const int i = (int)(b.x * kZX + threadIdx.x % kZX), j = (int)(b.y * kZY + threadIdx.x / kZX);
+    if (i >= FX || j >= FY) return;
+    const int kz = (FZ + (int)gridDim.z - 1) / (int)gridDim.z, k0 = (int)b.z * kz;
+    if (k0 >= FZ) return;
+    const int k1 = min(k0 + kz, FZ);
+    const size_t sx = 1, sy = (size_t)FX, sz = (size_t)FX * FY;  // this family's strides
+    const size_t uy = (size_t)nx + 1, uz = (size_t)(nx + 1) * ny;  // u's
+    const size_t vy = (size_t)nx, vz = (size_t)nx * (ny + 1);      // v's
+    const size_t wy = (size_t)nx, wz = (size_t)nx * ny;            // w's
+    const double hx = 1.0 / (2.0 * P.dx), hy = 1.0 / (2.0 * P.dy), hz = 1.0 / (2.0 * P.dz);
+    const double ix2 = 1.0 / (P.dx * P.dx), iy2 = 1.0 / (P.dy * P.dy), iz2 = 1.0 / (P.dz * P.dz);
+    size_t c0 = (size_t)i + sy * j + sz * k0;
+    if ((A == 0 && (i == 0 || i == nx)) || (A == 1 && (j == 0 || j == ny))) {
+        for (int k = k0; k < k1; ++k, c0 += sz) out[c0] = 0.0;
+        return;
+    }
+    // the family's own value below (k-1) and at k
+    double cm = k0 > 0 ? (A == 0 ? u : (A == 1 ? v : w))[c0 - sz] : 0.0;
+    double cc = (A == 0 ? u : (A == 1 ? v : w))[c0];
+    const double *self = A == 0 ? u : (A == 1 ? v : w);
+    // cross-family pairs on plane k (u faces: w; v faces: w) or k-1 (w faces: u and v)
+    double xa = 0.0, xb = 0.0, ya = 0.0, yb = 0.0;
+    if (A == 0) {
+        xa = w[(size_t)(i - 1) + wy * j + wz * k0];
+        xb = w[(size_t)i + wy * j + wz * k0];
+    } else if (A == 1) {
+        xa = w[(size_t)i + wy * (j - 1) + wz * k0];
+        xb = w[(size_t)i + wy * j + wz * k0];
+    } else if (k0 > 0) {
+        xa = u[(size_t)i + uy * j + uz * (k0 - 1)];
+        xb = u[(size_t)(i + 1) + uy * j + uz * (k0 - 1)];
+        ya = v[(size_t)i + vy * j + vz * (k0 - 1)];
+        yb = v[(size_t)i + vy * (j + 1) + vz * (k0 - 1)];
+    }
+    for (int k = k0; k < k1; ++k, c0 += sz) {
+        const double c = cc;
+        if (A == 0) {
+            const double up = k + 1 < nz ? self[c0 + sz] : 0.0;
+            const double e = self[c0 + sx], wv = self[c0 - sx];
+            const double n = j + 1 < ny ? self[c0 + sy] : 2.0 * P.lid - c;
+            const double so = j > 0 ? self[c0 - sy] : -c;
+            const double v0 = v[(size_t)(i - 1) + vy * j + vz * k], v1 = v[(size_t)i + vy * j + vz * k];
+            const double v2 = v[(size_t)(i - 1) + vy * (j + 1) + vz * k], v3 = v[(size_t)i + vy * (j + 1) + vz * k];
+            const double wa1 = w[(size_t)(i - 1) + wy * j + wz * (k + 1)], wb1 = w[(size_t)i + wy * j + wz * (k + 1)];
+            const double f = k + 1 < nz ? up : -c, bk = k > 0 ? cm : -c;
+            const double vv = 0.25 * (v0 + v1 + v2 + v3);
+            const double ww = 0.25 * (xa + xb + wa1 + wb1);
+            out[c0] = c + P.dt * (-c * (e - wv) * hx - vv * (n - so) * hy -
+                                  ww * (f - bk) * hz +
+                                  P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
+                                          (f - 2.0 * c + bk) * iz2));
+            cm = cc;
+            cc = up;
+            xa = wa1;
+            xb = wb1;
+        } else if (A == 1) {
+            const double up = k + 1 < nz ? self[c0 + sz] : 0.0;
+            const double e = i + 1 < nx ? self[c0 + sx] : -c, wv = i > 0 ? self[c0 - sx] : -c;
+            const double n = self[c0 + sy], so = self[c0 - sy];
+            const double u0 = u[(size_t)i + uy * (j - 1) + uz * k], u1 = u[(size_t)(i + 1) + uy * (j - 1) + uz * k];
+            const double u2 = u[(size_t)i + uy * j + uz * k], u3 = u[(size_t)(i + 1) + uy * j + uz * k];
+            const double wa1 = w[(size_t)i + wy * (j - 1) + wz * (k + 1)], wb1 = w[(size_t)i + wy * j + wz * (k + 1)];
+            const double f = k + 1 < nz ? up : -c, bk = k > 0 ? cm : -c;
+            const double uu = 0.25 * (u0 + u1 + u2 + u3);
+            const double ww = 0.25 * (xa + xb + wa1 + wb1);
+            out[c0] = c + P.dt * (-uu * (e - wv) * hx - c * (n - so) * hy -
+                                  ww * (f - bk) * hz +
+                                  P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
+                                          (f - 2.0 * c + bk) * iz2));
+            cm = cc;
+            cc = up;
+            xa = wa1;
+            xb = wb1;
         } else {
-            const uint32_t s = t - nu - nv;
-            const int i = s % P.nx, j = (s / P.nx) % P.ny, k = s / (P.nx * P.ny);
-            if (k == 0 || k == (int)P.nz) { ws[s] = 0.0; continue; }
-            const double c = w[s];
-            const double e = w_nb(P, w, i, j, k, 1, 0, 0, c), wv = w_nb(P, w, i, j, k, -1, 0, 0, c);
-            const double n = w_nb(P, w, i, j, k, 0, 1, 0, c), so = w_nb(P, w, i, j, k, 0, -1, 0, c);
-            const double f = w_nb(P, w, i, j, k, 0, 0, 1, c), bk = w_nb(P, w, i, j, k, 0, 0, -1, c);
-            const double uu = 0.25 * (u[U3(i, j, k - 1)] + u[U3(i + 1, j, k - 1)] + u[U3(i, j, k)] +
-                                      u[U3(i + 1, j, k)]);
-            const double vv = 0.25 * (v[V3(i, j, k - 1)] + v[V3(i, j + 1, k - 1)] + v[V3(i, j, k)] +
-                                      v[V3(i, j + 1, k)]);
-            ws[s] = c + P.dt * (-uu * (e - wv) / (2.0 * P.dx) - vv * (n - so) / (2.0 * P.dy) -
-                                c * (f - bk) / (2.0 * P.dz) +
-                                P.nu * ((e - 2.0 * c + wv) / dx2 + (n - 2.0 * c + so) / dy2 +
-                                        (f - 2.0 * c + bk) / dz2));
+            // w faces: z runs over 0..nz; faces 0 and nz are boundary-normal
+            const double up = k + 1 <= nz ? self[c0 + sz] : 0.0;
+            const double ua1 = k < nz ? u[(size_t)i + uy * j + uz * k] : 0.0;
+            const double ub1 = k < nz ? u[(size_t)(i + 1) + uy * j + uz * k] : 0.0;
+            const double va1 = k < nz ? v[(size_t)i + vy * j + vz * k] : 0.0;
+            const double vb1 = k < nz ? v[(size_t)i + vy * (j + 1) + vz * k] : 0.0;
+            if (k == 0 || k == nz) {
+                out[c0] = 0.0;
+            } else {
+                const double e = i + 1 < nx ? self[c0 + sx] : -c, wv = i > 0 ? self[c0 - sx] : -c;
+                const double n = j + 1 < ny ? self[c0 + sy] : -c, so = j > 0 ? self[c0 - sy] : -c;
+                const double f = up, bk = cm;
+                const double uu = 0.25 * (xa + xb + ua1 + ub1);
+                const double vv = 0.25 * (ya + yb + va1 + vb1);
+                out[c0] = c + P.dt * (-uu * (e - wv) * hx - vv * (n - so) * hy -
+                                      c * (f - bk) * hz +
+                                      P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
+                                              (f - 2.0 * c + bk) * iz2));
+            }
+            cm = cc;
+            cc = up;
+            xa = ua1;
+            xb = ub1;
+            ya = va1;
+            yb = vb1;
         }
     }
-    }
+}
+
+inline dim3 predict_grid(const StepParams &P, int a) {
+    const uint32_t fx = P.nx + (a == 0), fy = P.ny + (a == 1), fz = P.nz + (a == 2);
+    return dim3((fx + kZX - 1) / kZX, (fy + kZY - 1) / kZY, (fz + kZK - 1) / kZK);
 }
 
 // build_pressure_rhs (incompressible_3d.rs:275-291), rhs(0,0,0) = 0.
@@ -1240,44 +1296,59 @@ struct DiagState {
     unsigned counter;
 };
 
-// max_divergence (:504-517), max_speed (:519-532) and sum(div^2).
-__global__ void __launch_bounds__(kBlock) k_diag(StepParams P, const double *__restrict__ u,
-                                                 const double *__restrict__ v,
-                                                 const double *__restrict__ w,
-                                                 double *partials, DiagState *dg) {
-    const uint32_t n = P.nx * P.ny * P.nz;
+// max_divergence (:504-517), max_speed (:519-532) and sum(div^2) over the
+// cells as a column march (w's top face of one cell is the next cell's
+// bottom face, carried in a register), with ensure_finite's scan (:534-554)
+// of p, u, v and w folded in when `check`: a face is checked by the cell on
+// its low side, the top boundary faces by the last cell of their row.  Each
+// block publishes one row {sum div^2 | max |div|, max speed}; k_diag_fold
+// folds them.
+__global__ void __launch_bounds__(kZT) k_diag_z(StepParams P, const double *__restrict__ p,
+                                                const double *__restrict__ u,
+                                                const double *__restrict__ v,
+                                                const double *__restrict__ w,
+                                                double *__restrict__ pout, DiagState *dg, int check) {
+    const uint32_t nx = P.nx, ny = P.ny, nz = P.nz;
     double s[1] = {0.0}, m[2] = {0.0, 0.0};
-    const XcdRange r(n);
-    for (uint32_t c = r.first; c < r.end; c += r.step) {
-        const int i = c % P.nx, j = (c / P.nx) % P.ny, k = c / (P.nx * P.ny);
-        const double u0 = u[U3(i, j, k)], u1 = u[U3(i + 1, j, k)];
-        const double v0 = v[V3(i, j, k)], v1 = v[V3(i, j + 1, k)];
-        const double w0 = w[W3(i, j, k)], w1 = w[W3(i, j, k + 1)];
-        const double div = (u1 - u0) / P.dx + (v1 - v0) / P.dy + (w1 - w0) / P.dz;
-        s[0] += div * div;
-        m[0] = fmax(m[0], fabs(div));
-        const double a = 0.5 * (u0 + u1), b = 0.5 * (v0 + v1), cc = 0.5 * (w0 + w1);
-        m[1] = fmax(m[1], sqrt(a * a + b * b + cc * cc));
+    const uint3 b = zblock();
+    const uint32_t i = b.x * kZX + threadIdx.x % kZX, j = b.y * kZY + threadIdx.x / kZX;
+    const uint32_t kz = (nz + gridDim.z - 1) / gridDim.z, k0 = b.z * kz, k1 = min(k0 + kz, nz);
+    if (i < nx && j < ny && k0 < nz) {
+        const size_t uy = nx + 1, uz = (size_t)(nx + 1) * ny, vy = nx, vz = (size_t)nx * (ny + 1),
+                     wy = nx, wz = (size_t)nx * ny;
+        double w0 = w[i + wy * j + wz * k0];
+        for (uint32_t k = k0; k < k1; ++k) {
+            const size_t cu = i + uy * j + uz * k, cv = i + vy * j + vz * k, cw = i + wy * j + wz * k;
+            const double u0 = u[cu], u1 = u[cu + 1], v0 = v[cv], v1 = v[cv + vy], w1 = w[cw + wz];
+            const double div = (u1 - u0) / P.dx + (v1 - v0) / P.dy + (w1 - w0) / P.dz;
+            s[0] += div * div;
+            m[0] = fmax(m[0], fabs(div));
+            const double a = 0.5 * (u0 + u1), bb = 0.5 * (v0 + v1), cc = 0.5 * (w0 + w1);
+            m[1] = fmax(m[1], sqrt(a * a + bb * bb + cc * cc));
+            if (check) {
+                const double pc = p[cw];  // cells and w faces share the layout up to nz
+                if (!isfinite(pc)) atomicMin(&dg->first_bad[0], (unsigned long long)cw);
+                if (!isfinite(u0)) atomicMin(&dg->first_bad[1], (unsigned long long)cu);
+                if (i + 1 == nx && !isfinite(u1)) atomicMin(&dg->first_bad[1], (unsigned long long)(cu + 1));
+                if (!isfinite(v0)) atomicMin(&dg->first_bad[2], (unsigned long long)cv);
+                if (j + 1 == ny && !isfinite(v1)) atomicMin(&dg->first_bad[2], (unsigned long long)(cv + vy));
+                if (!isfinite(w0)) atomicMin(&dg->first_bad[3], (unsigned long long)cw);
+                if (k + 1 == nz && !isfinite(w1)) atomicMin(&dg->first_bad[3], (unsigned long long)(cw + wz));
+            }
+            w0 = w1;
+        }
     }
-    if (grid_reduce<1, 2>(s, m, partials, &dg->counter) && threadIdx.x == 0) {
+    publish_row<1, 2>(s, m, pout);
+}
+__global__ void __launch_bounds__(kBlock) k_diag_fold(const double *__restrict__ pin, uint32_t rows,
+                                                      DiagState *dg) {
+    double s[1], m[2];
+    fold_partials<1, 2>(pin, rows, s, m);
+    if (threadIdx.x == 0) {
         dg->div_l2sq = s[0];
         dg->max_div = m[0];
         dg->max_speed = m[1];
     }
-}
-
-// ensure_finite (:534-554): first non-finite index of p, u, v, w.
-__global__ void __launch_bounds__(kBlock) k_finite(const double *__restrict__ a0, uint32_t n0,
-                                                   const double *__restrict__ a1, uint32_t n1,
-                                                   const double *__restrict__ a2, uint32_t n2,
-                                                   const double *__restrict__ a3, uint32_t n3,
-                                                   DiagState *dg) {
-    const double *arr[4] = {a0, a1, a2, a3};
-    const uint32_t len[4] = {n0, n1, n2, n3};
-    for (int f = 0; f < 4; ++f)
-        for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < len[f];
-             c += gridDim.x * blockDim.x)
-            if (!isfinite(arr[f][c])) atomicMin(&dg->first_bad[f], (unsigned long long)c);
 }
 
 #undef U3
@@ -1358,7 +1429,7 @@ __global__ void __launch_bounds__(kBlock) k_block_mean(Grid g, const double *__r
 // Kernel kinds of the per-kernel profiler, named as the kernels are (with
 // their template arguments).
 enum Kind {
-    K_PREDICT, K_RHS, K_CORRECT, K_DIAG, K_FINITE,
+    K_PREDICT, K_RHS, K_CORRECT, K_DIAG,
     K_RESIDUAL_D, K_RESIDUAL_F, K_FOLD_OUTER, K_ADD_CORRECTION,
     K_DIRAPPLY_D, K_UPDATE_D, K_DIRAPPLY_F, K_UPDATE_F,
     K_MG_CHECK, K_MG_SWEEP_L0, K_MG_SWEEP_L0_PROLONG, K_MG_SWEEP_L0_PUBLISH,
@@ -1366,7 +1437,7 @@ enum Kind {
     K_COUNT
 };
 const char *const kKindNames[K_COUNT] = {
-    "k_predict", "k_rhs", "k_correct", "k_diag", "k_finite",
+    "k_predict_z", "k_rhs", "k_correct", "k_diag_z",
     "k_residual<double>", "k_residual<float>", "k_fold_outer", "k_add_correction",
     "k_cg_dirapply_z<double>", "k_cg_update<double, double>",
     "k_cg_dirapply_z<float>", "k_cg_update<float, float>",
@@ -2022,13 +2093,10 @@ fl_status run_diag(fl_cavity3d *s, fl_cavity3d_step_report *rep, bool check_fini
     s->host->dg = init;
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->dg, &s->host->dg, sizeof(DiagState), hipMemcpyHostToDevice, stm));
     const double faces = 8.0 * (double)(s->nu + s->nv + s->nw);
-    FL_TIMED(s, K_DIAG, -1, faces,
-             k_diag<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->u, s->v, s->w, s->partials, s->dg));
-    if (check_finite)
-        FL_TIMED(s, K_FINITE, -1, faces + 8.0 * s->n,
-                 k_finite<<<s->ew_blocks, kBlock, 0, stm>>>(s->p, (uint32_t)s->n, s->u,
-                                                            (uint32_t)s->nu, s->v, (uint32_t)s->nv,
-                                                            s->w, (uint32_t)s->nw, s->dg));
+    const dim3 zg(s->zg.bx, s->zg.by, s->zg.bz);
+    FL_TIMED(s, K_DIAG, -1, faces + (check_finite ? 8.0 * s->n : 0.0),
+             k_diag_z<<<zg, kZT, 0, stm>>>(s->P, s->p, s->u, s->v, s->w, s->partials, s->dg, check_finite ? 1 : 0);
+             k_diag_fold<<<1, kBlock, 0, stm>>>(s->partials, s->zg.count, s->dg));
     FL_HIP_TRY(ctx, hipGetLastError());
     FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->dg, s->dg, sizeof(DiagState), hipMemcpyDeviceToHost, stm));
     FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
@@ -2060,7 +2128,9 @@ fl_status do_step(fl_cavity3d *s, fl_cavity3d_step_report *rep) {
     // algorithmic bytes: every face array read or written once, cells once
     const double faces = 8.0 * (double)(s->nu + s->nv + s->nw), cells = 8.0 * s->n;
     FL_TIMED(s, K_PREDICT, -1, 2.0 * faces,
-             k_predict<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->u, s->v, s->w, s->us, s->vs, s->ws));
+             k_predict_z<0><<<predict_grid(s->P, 0), kZT, 0, stm>>>(s->P, s->u, s->v, s->w, s->us);
+             k_predict_z<1><<<predict_grid(s->P, 1), kZT, 0, stm>>>(s->P, s->u, s->v, s->w, s->vs);
+             k_predict_z<2><<<predict_grid(s->P, 2), kZT, 0, stm>>>(s->P, s->u, s->v, s->w, s->ws));
     FL_TIMED(s, K_RHS, -1, faces + cells,
              k_rhs<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->us, s->vs, s->ws, s->rhs));
     FL_HIP_TRY(ctx, hipGetLastError());
