@@ -1,7 +1,7 @@
 // 3D lid-driven cavity projection step on gfx950.
 //
 // Replaces LidDrivenCavity3DSolver (src/solver/incompressible_3d.rs):
-//   predict_momentum :174-273   -> k_predict_z<0, 1, 2>
+//   predict_momentum :174-273   -> k_predict3_z
 //   build_pressure_rhs :275-291 -> k_rhs
 //   solve_pressure_poisson :293-332 -> pinned Jacobi-PCG; the reference only
 //       has SOR here, the PCG loop is the 2D solver's
@@ -1114,128 +1114,105 @@ __device__ inline double w_nb(const StepParams &P, const double *w, int i, int j
 }
 
 // predict_momentum (incompressible_3d.rs:174-273): central, advective form,
-// one launch per face family (A = 0, 1, 2: u, v, w) as a column march over
-// that family's faces: a thread walks face column (i, j) up in z, keeping
-// the family's value at k-1, k, k+1 and the other families' planes it
-// shares with the next face in registers.  Boundary-normal faces are
-// written 0 (the reference zero-fills u*, v*, w*); neighbours outside the
-// domain follow u/v/w_neighbor (:366-447): the mirror -c, or 2 lid - c
-// above the lid for u.  The divisions by 2 dx and dx^2 are multiplications
-// by reciprocals computed once (within an ulp of the quotient).
-template <int A>
-__global__ void __launch_bounds__(kZT) k_predict_z(StepParams P, const double *__restrict__ u,
-                                                   const double *__restrict__ v,
-                                                   const double *__restrict__ w,
-                                                   double *__restrict__ out) {
+// all three face families in one column march over the cells: thread (i, j)
+// walks k and writes u(i, j, k), v(i, j, k) and w(i, j, k) (the last column,
+// row and plane also write the boundary faces u(nx, .), v(., ny),
+// w(., ., nz)).  Boundary-normal faces are written 0 (the reference
+// zero-fills u*, v*, w*); neighbours outside the domain follow
+// u/v/w_neighbor (:366-447): the mirror -c, or 2 lid - c above the lid for
+// u.  Each velocity array is streamed once: the plane-k+1 values the next
+// step needs (u, v, w at (i, j) and w at (i-1, j), (i, j-1)) and the
+// plane-k-1 values (u, v, w at (i, j), u at (i+1, j), v at (i, j+1)) stay in
+// registers; the other in-plane neighbours are loads that hit in L1/L2.  The
+// divisions by 2 dx and dx^2 are multiplications by reciprocals computed
+// once (within an ulp of the quotient).
+__global__ void __launch_bounds__(kZT) k_predict3_z(StepParams P, const double *__restrict__ u,
+                                                    const double *__restrict__ v,
+                                                    const double *__restrict__ w,
+                                                    double *__restrict__ us, double *__restrict__ vs,
+                                                    double *__restrict__ ws) {
     const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
-    const int FX = nx + (A == 0), FY = ny + (A == 1), FZ = nz + (A == 2);
     const uint3 b = zblock();
     const int i = (int)(b.x * kZX + threadIdx.x % kZX), j = (int)(b.y * kZY + threadIdx.x / kZX);
-    if (i >= FX || j >= FY) return;
-    const int kz = (FZ + (int)gridDim.z - 1) / (int)gridDim.z, k0 = (int)b.z * kz;
-    if (k0 >= FZ) return;
-    const int k1 = min(k0 + kz, FZ);
-    const size_t sx = 1, sy = (size_t)FX, sz = (size_t)FX * FY;  // this family's strides
-    const size_t uy = (size_t)nx + 1, uz = (size_t)(nx + 1) * ny;  // u's
-    const size_t vy = (size_t)nx, vz = (size_t)nx * (ny + 1);      // v's
-    const size_t wy = (size_t)nx, wz = (size_t)nx * ny;            // w's
+    if (i >= nx || j >= ny) return;
+    const int kz = (nz + (int)gridDim.z - 1) / (int)gridDim.z, k0 = (int)b.z * kz;
+    if (k0 >= nz) return;
+    const int k1 = min(k0 + kz, nz);
+    const size_t uy = (size_t)nx + 1, uz = (size_t)(nx + 1) * ny;  // u strides
+    const size_t vy = (size_t)nx, vz = (size_t)nx * (ny + 1);      // v strides
+    const size_t wy = (size_t)nx, wz = (size_t)nx * ny;            // w strides
     const double hx = 1.0 / (2.0 * P.dx), hy = 1.0 / (2.0 * P.dy), hz = 1.0 / (2.0 * P.dz);
     const double ix2 = 1.0 / (P.dx * P.dx), iy2 = 1.0 / (P.dy * P.dy), iz2 = 1.0 / (P.dz * P.dz);
-    size_t c0 = (size_t)i + sy * j + sz * k0;
-    if ((A == 0 && (i == 0 || i == nx)) || (A == 1 && (j == 0 || j == ny))) {
-        for (int k = k0; k < k1; ++k, c0 += sz) out[c0] = 0.0;
-        return;
+    const bool xw = i > 0, xe = i + 1 < nx, ys = j > 0, yn = j + 1 < ny;
+    size_t cu = (size_t)i + uy * j + uz * k0, cv = (size_t)i + vy * j + vz * k0,
+           cw = (size_t)i + wy * j + wz * k0;
+    // carried: plane k (u0, v0, w0, w at (i-1, j) and (i, j-1)) and plane k-1
+    double u0 = u[cu], v0 = v[cv], w0 = w[cw];
+    double ww0 = xw ? w[cw - 1] : 0.0, ws0 = ys ? w[cw - wy] : 0.0;
+    double um = 0.0, vm = 0.0, wm = 0.0, uem = 0.0, vnm = 0.0;
+    if (k0 > 0) {
+        um = u[cu - uz];
+        vm = v[cv - vz];
+        wm = w[cw - wz];
+        uem = u[cu - uz + 1];
+        vnm = v[cv - vz + vy];
     }
-    // the family's own value below (k-1) and at k
-    double cm = k0 > 0 ? (A == 0 ? u : (A == 1 ? v : w))[c0 - sz] : 0.0;
-    double cc = (A == 0 ? u : (A == 1 ? v : w))[c0];
-    const double *self = A == 0 ? u : (A == 1 ? v : w);
-    // cross-family pairs on plane k (u faces: w; v faces: w) or k-1 (w faces: u and v)
-    double xa = 0.0, xb = 0.0, ya = 0.0, yb = 0.0;
-    if (A == 0) {
-        xa = w[(size_t)(i - 1) + wy * j + wz * k0];
-        xb = w[(size_t)i + wy * j + wz * k0];
-    } else if (A == 1) {
-        xa = w[(size_t)i + wy * (j - 1) + wz * k0];
-        xb = w[(size_t)i + wy * j + wz * k0];
-    } else if (k0 > 0) {
-        xa = u[(size_t)i + uy * j + uz * (k0 - 1)];
-        xb = u[(size_t)(i + 1) + uy * j + uz * (k0 - 1)];
-        ya = v[(size_t)i + vy * j + vz * (k0 - 1)];
-        yb = v[(size_t)i + vy * (j + 1) + vz * (k0 - 1)];
-    }
-    for (int k = k0; k < k1; ++k, c0 += sz) {
-        const double c = cc;
-        if (A == 0) {
-            const double up = k + 1 < nz ? self[c0 + sz] : 0.0;
-            const double e = self[c0 + sx], wv = self[c0 - sx];
-            const double n = j + 1 < ny ? self[c0 + sy] : 2.0 * P.lid - c;
-            const double so = j > 0 ? self[c0 - sy] : -c;
-            const double v0 = v[(size_t)(i - 1) + vy * j + vz * k], v1 = v[(size_t)i + vy * j + vz * k];
-            const double v2 = v[(size_t)(i - 1) + vy * (j + 1) + vz * k], v3 = v[(size_t)i + vy * (j + 1) + vz * k];
-            const double wa1 = w[(size_t)(i - 1) + wy * j + wz * (k + 1)], wb1 = w[(size_t)i + wy * j + wz * (k + 1)];
-            const double f = k + 1 < nz ? up : -c, bk = k > 0 ? cm : -c;
-            const double vv = 0.25 * (v0 + v1 + v2 + v3);
-            const double ww = 0.25 * (xa + xb + wa1 + wb1);
-            out[c0] = c + P.dt * (-c * (e - wv) * hx - vv * (n - so) * hy -
-                                  ww * (f - bk) * hz +
-                                  P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
-                                          (f - 2.0 * c + bk) * iz2));
-            cm = cc;
-            cc = up;
-            xa = wa1;
-            xb = wb1;
-        } else if (A == 1) {
-            const double up = k + 1 < nz ? self[c0 + sz] : 0.0;
-            const double e = i + 1 < nx ? self[c0 + sx] : -c, wv = i > 0 ? self[c0 - sx] : -c;
-            const double n = self[c0 + sy], so = self[c0 - sy];
-            const double u0 = u[(size_t)i + uy * (j - 1) + uz * k], u1 = u[(size_t)(i + 1) + uy * (j - 1) + uz * k];
-            const double u2 = u[(size_t)i + uy * j + uz * k], u3 = u[(size_t)(i + 1) + uy * j + uz * k];
-            const double wa1 = w[(size_t)i + wy * (j - 1) + wz * (k + 1)], wb1 = w[(size_t)i + wy * j + wz * (k + 1)];
-            const double f = k + 1 < nz ? up : -c, bk = k > 0 ? cm : -c;
-            const double uu = 0.25 * (u0 + u1 + u2 + u3);
-            const double ww = 0.25 * (xa + xb + wa1 + wb1);
-            out[c0] = c + P.dt * (-uu * (e - wv) * hx - c * (n - so) * hy -
-                                  ww * (f - bk) * hz +
-                                  P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
-                                          (f - 2.0 * c + bk) * iz2));
-            cm = cc;
-            cc = up;
-            xa = wa1;
-            xb = wb1;
+    for (int k = k0; k < k1; ++k, cu += uz, cv += vz, cw += wz) {
+        const bool zt = k + 1 < nz;
+        // plane k+1 (carried to the next step)
+        const double up = zt ? u[cu + uz] : 0.0, vp = zt ? v[cv + vz] : 0.0, wp = w[cw + wz];
+        const double wwp = xw ? w[cw + wz - 1] : 0.0, wsp = ys ? w[cw + wz - wy] : 0.0;
+        // plane k neighbours
+        const double ue = u[cu + 1], uw = xw ? u[cu - 1] : 0.0, un = yn ? u[cu + uy] : 0.0;
+        const double usn = ys ? u[cu - uy] : 0.0, use = ys ? u[cu - uy + 1] : 0.0;
+        const double ve = xe ? v[cv + 1] : 0.0, vw = xw ? v[cv - 1] : 0.0, vn = v[cv + vy];
+        const double vsn = ys ? v[cv - vy] : 0.0, vnw = xw ? v[cv + vy - 1] : 0.0;
+        const double we = xe ? w[cw + 1] : 0.0, wn = yn ? w[cw + wy] : 0.0;
+        // u(i, j, k): boundary-normal at i = 0 (and i = nx, written below)
+        if (i == 0) {
+            us[cu] = 0.0;
         } else {
-            // w faces: z runs over 0..nz; faces 0 and nz are boundary-normal
-            const double up = k + 1 <= nz ? self[c0 + sz] : 0.0;
-            const double ua1 = k < nz ? u[(size_t)i + uy * j + uz * k] : 0.0;
-            const double ub1 = k < nz ? u[(size_t)(i + 1) + uy * j + uz * k] : 0.0;
-            const double va1 = k < nz ? v[(size_t)i + vy * j + vz * k] : 0.0;
-            const double vb1 = k < nz ? v[(size_t)i + vy * (j + 1) + vz * k] : 0.0;
-            if (k == 0 || k == nz) {
-                out[c0] = 0.0;
-            } else {
-                const double e = i + 1 < nx ? self[c0 + sx] : -c, wv = i > 0 ? self[c0 - sx] : -c;
-                const double n = j + 1 < ny ? self[c0 + sy] : -c, so = j > 0 ? self[c0 - sy] : -c;
-                const double f = up, bk = cm;
-                const double uu = 0.25 * (xa + xb + ua1 + ub1);
-                const double vv = 0.25 * (ya + yb + va1 + vb1);
-                out[c0] = c + P.dt * (-uu * (e - wv) * hx - vv * (n - so) * hy -
-                                      c * (f - bk) * hz +
-                                      P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
-                                              (f - 2.0 * c + bk) * iz2));
-            }
-            cm = cc;
-            cc = up;
-            xa = ua1;
-            xb = ub1;
-            ya = va1;
-            yb = vb1;
+            const double c = u0, e = ue, wv = uw;
+            const double n = yn ? un : 2.0 * P.lid - c, so = ys ? usn : -c;
+            const double f = zt ? up : -c, bk = k > 0 ? um : -c;
+            const double vv = 0.25 * (vw + v0 + vnw + vn);
+            const double wwv = 0.25 * (ww0 + w0 + wwp + wp);
+            us[cu] = c + P.dt * (-c * (e - wv) * hx - vv * (n - so) * hy - wwv * (f - bk) * hz +
+                                 P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
+                                         (f - 2.0 * c + bk) * iz2));
         }
+        if (i + 1 == nx) us[cu + 1] = 0.0;
+        // v(i, j, k): boundary-normal at j = 0 (and j = ny, written below)
+        if (j == 0) {
+            vs[cv] = 0.0;
+        } else {
+            const double c = v0;
+            const double e = xe ? ve : -c, wv = xw ? vw : -c, n = vn, so = vsn;
+            const double f = zt ? vp : -c, bk = k > 0 ? vm : -c;
+            const double uu = 0.25 * (usn + use + u0 + ue);
+            const double wwv = 0.25 * (ws0 + w0 + wsp + wp);
+            vs[cv] = c + P.dt * (-uu * (e - wv) * hx - c * (n - so) * hy - wwv * (f - bk) * hz +
+                                 P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
+                                         (f - 2.0 * c + bk) * iz2));
+        }
+        if (j + 1 == ny) vs[cv + vy] = 0.0;
+        // w(i, j, k): boundary-normal at k = 0 (and k = nz, written below)
+        if (k == 0) {
+            ws[cw] = 0.0;
+        } else {
+            const double c = w0;
+            const double e = xe ? we : -c, wv = xw ? ww0 : -c, n = yn ? wn : -c, so = ys ? ws0 : -c;
+            const double f = wp, bk = wm;
+            const double uu = 0.25 * (um + uem + u0 + ue);
+            const double vv = 0.25 * (vm + vnm + v0 + vn);
+            ws[cw] = c + P.dt * (-uu * (e - wv) * hx - vv * (n - so) * hy - c * (f - bk) * hz +
+                                 P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
+                                         (f - 2.0 * c + bk) * iz2));
+        }
+        if (k + 1 == nz) ws[cw + wz] = 0.0;
+        um = u0; vm = v0; wm = w0; uem = ue; vnm = vn;
+        u0 = up; v0 = vp; w0 = wp; ww0 = wwp; ws0 = wsp;
     }
-}
-
-inline dim3 predict_grid(const StepParams &P, int a) {
-    const uint32_t fx = P.nx + (a == 0), fy = P.ny + (a == 1), fz = P.nz + (a == 2);
-    return dim3((fx + kZX - 1) / kZX, (fy + kZY - 1) / kZY, (fz + kZK - 1) / kZK);
 }
 
 // build_pressure_rhs (incompressible_3d.rs:275-291), rhs(0,0,0) = 0.
@@ -1437,7 +1414,7 @@ enum Kind {
     K_COUNT
 };
 const char *const kKindNames[K_COUNT] = {
-    "k_predict_z", "k_rhs", "k_correct", "k_diag_z",
+    "k_predict3_z", "k_rhs", "k_correct", "k_diag_z",
     "k_residual<double>", "k_residual<float>", "k_fold_outer", "k_add_correction",
     "k_cg_dirapply_z<double>", "k_cg_update<double, double>",
     "k_cg_dirapply_z<float>", "k_cg_update<float, float>",
@@ -2128,9 +2105,7 @@ fl_status do_step(fl_cavity3d *s, fl_cavity3d_step_report *rep) {
     // algorithmic bytes: every face array read or written once, cells once
     const double faces = 8.0 * (double)(s->nu + s->nv + s->nw), cells = 8.0 * s->n;
     FL_TIMED(s, K_PREDICT, -1, 2.0 * faces,
-             k_predict_z<0><<<predict_grid(s->P, 0), kZT, 0, stm>>>(s->P, s->u, s->v, s->w, s->us);
-             k_predict_z<1><<<predict_grid(s->P, 1), kZT, 0, stm>>>(s->P, s->u, s->v, s->w, s->vs);
-             k_predict_z<2><<<predict_grid(s->P, 2), kZT, 0, stm>>>(s->P, s->u, s->v, s->w, s->ws));
+             k_predict3_z<<<dim3(s->zg.bx, s->zg.by, s->zg.bz), kZT, 0, stm>>>(s->P, s->u, s->v, s->w, s->us, s->vs, s->ws));
     FL_TIMED(s, K_RHS, -1, faces + cells,
              k_rhs<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->us, s->vs, s->ws, s->rhs));
     FL_HIP_TRY(ctx, hipGetLastError());
