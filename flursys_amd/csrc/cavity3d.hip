@@ -7,8 +7,8 @@
 //       has SOR here, the PCG loop is the 2D solver's
 //       (src/solver/incompressible.rs:1048-1214) on the 7-point operator with
 //       cell (0,0,0) eliminated (src/poisson.rs:303-337 style)
-//   correct_velocity :334-363   -> k_correct
-//   max_divergence / max_speed / ensure_finite :504-554 -> k_diag_z (+ k_diag_fold)
+//   correct_velocity :334-363 and
+//   max_divergence / max_speed / ensure_finite :504-554 -> k_correct_diag_z (+ k_diag_fold)
 //
 // Data layout in HBM: the reference's own (src/field.rs:104-107), one flat
 // array per field, x fastest.  The PCG vectors x (= p), r, q and two
@@ -1232,87 +1232,79 @@ __global__ void __launch_bounds__(kBlock) k_rhs(StepParams P, const double *__re
     }
 }
 
-// correct_velocity (incompressible_3d.rs:334-363): interior faces only.
-__global__ void __launch_bounds__(kBlock) k_correct(StepParams P, const double *__restrict__ p,
-                                                    const double *__restrict__ us,
-                                                    const double *__restrict__ vs,
-                                                    const double *__restrict__ ws,
-                                                    double *__restrict__ u,
-                                                    double *__restrict__ v,
-                                                    double *__restrict__ w) {
-    const uint32_t nu = (P.nx + 1) * P.ny * P.nz, nv = P.nx * (P.ny + 1) * P.nz,
-                   nw = P.nx * P.ny * (P.nz + 1);
-    const double scale = P.dt / P.rho;
-    for (int a = 0; a < 3; ++a) {  // u, v and w faces, per-XCD z-slabs
-    const uint32_t base = a == 0 ? 0 : (a == 1 ? nu : nu + nv);
-    const XcdRange r(a == 0 ? nu : (a == 1 ? nv : nw));
-    for (uint32_t l = r.first; l < r.end; l += r.step) {
-        const uint32_t t = base + l;
-        if (t < nu) {
-            const int i = t % (P.nx + 1), j = (t / (P.nx + 1)) % P.ny, k = t / ((P.nx + 1) * P.ny);
-            if (i == 0 || i == (int)P.nx) continue;
-            u[t] = us[t] - scale * (p[W3(i, j, k)] - p[W3(i - 1, j, k)]) / P.dx;
-        } else if (t < nu + nv) {
-            const uint32_t s = t - nu;
-            const int i = s % P.nx, j = (s / P.nx) % (P.ny + 1), k = s / (P.nx * (P.ny + 1));
-            if (j == 0 || j == (int)P.ny) continue;
-            v[s] = vs[s] - scale * (p[W3(i, j, k)] - p[W3(i, j - 1, k)]) / P.dy;
-        } else {
-            const uint32_t s = t - nu - nv;
-            const int i = s % P.nx, j = (s / P.nx) % P.ny, k = s / (P.nx * P.ny);
-            if (k == 0 || k == (int)P.nz) continue;
-            w[s] = ws[s] - scale * (p[W3(i, j, k)] - p[W3(i, j, k - 1)]) / P.dz;
-        }
-    }
-    }
-}
-
 struct DiagState {
     double div_l2sq, max_div, max_speed;
     unsigned long long first_bad[4];  // first non-finite flat index per field
     unsigned counter;
 };
 
-// max_divergence (:504-517), max_speed (:519-532) and sum(div^2) over the
-// cells as a column march (w's top face of one cell is the next cell's
-// bottom face, carried in a register), with ensure_finite's scan (:534-554)
-// of p, u, v and w folded in when `check`: a face is checked by the cell on
-// its low side, the top boundary faces by the last cell of their row.  Each
-// block publishes one row {sum div^2 | max |div|, max speed}; k_diag_fold
-// folds them.
-__global__ void __launch_bounds__(kZT) k_diag_z(StepParams P, const double *__restrict__ p,
-                                                const double *__restrict__ u,
-                                                const double *__restrict__ v,
-                                                const double *__restrict__ w,
-                                                double *__restrict__ pout, DiagState *dg, int check) {
-    const uint32_t nx = P.nx, ny = P.ny, nz = P.nz;
+// correct_velocity (:334-363) and, on the corrected field, max_divergence
+// (:504-517), max_speed (:519-532), sum(div^2) and ensure_finite's scan
+// (:534-554) of p, u, v, w, as one column march over the cells.  Thread
+// (i, j) at plane k corrects its low faces u(i, j, k), v(i, j, k) (interior
+// faces only, as the reference) and the top face w(i, j, k+1) (carried up as
+// the next cell's bottom face; a chunk's first bottom face is recomputed,
+// not read, since the chunk below writes it).  The cell's high faces
+// u(i+1, .), v(., j+1) are recomputed from us, vs and p with the same
+// arithmetic their owners use, boundary faces are read (the correction
+// leaves them).  A face is checked for finiteness by the cell on its low
+// side, the top boundary faces by the last cell of their row.  Each block
+// publishes one row {sum div^2 | max |div|, max speed}; k_diag_fold folds
+// them.
+__global__ void __launch_bounds__(kZT) k_correct_diag_z(StepParams P, const double *__restrict__ p,
+                                                        const double *__restrict__ us,
+                                                        const double *__restrict__ vs,
+                                                        const double *__restrict__ ws,
+                                                        double *__restrict__ u, double *__restrict__ v,
+                                                        double *__restrict__ w,
+                                                        double *__restrict__ pout, DiagState *dg) {
+    const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
     double s[1] = {0.0}, m[2] = {0.0, 0.0};
     const uint3 b = zblock();
-    const uint32_t i = b.x * kZX + threadIdx.x % kZX, j = b.y * kZY + threadIdx.x / kZX;
-    const uint32_t kz = (nz + gridDim.z - 1) / gridDim.z, k0 = b.z * kz, k1 = min(k0 + kz, nz);
+    const int i = (int)(b.x * kZX + threadIdx.x % kZX), j = (int)(b.y * kZY + threadIdx.x / kZX);
+    const int kz = (nz + (int)gridDim.z - 1) / (int)gridDim.z, k0 = (int)b.z * kz, k1 = min(k0 + kz, nz);
     if (i < nx && j < ny && k0 < nz) {
-        const size_t uy = nx + 1, uz = (size_t)(nx + 1) * ny, vy = nx, vz = (size_t)nx * (ny + 1),
+        const double scale = P.dt / P.rho;
+        const size_t uy = (size_t)nx + 1, uz = (size_t)(nx + 1) * ny, vy = nx, vz = (size_t)nx * (ny + 1),
                      wy = nx, wz = (size_t)nx * ny;
-        double w0 = w[i + wy * j + wz * k0];
-        for (uint32_t k = k0; k < k1; ++k) {
-            const size_t cu = i + uy * j + uz * k, cv = i + vy * j + vz * k, cw = i + wy * j + wz * k;
-            const double u0 = u[cu], u1 = u[cu + 1], v0 = v[cv], v1 = v[cv + vy], w1 = w[cw + wz];
+        size_t cu = (size_t)i + uy * j + uz * k0, cv = (size_t)i + vy * j + vz * k0,
+               cw = (size_t)i + wy * j + wz * k0;
+        double pc = p[cw];  // cells share w's layout below nz
+        // bottom face of the first cell
+        double w0 = k0 == 0 ? w[cw] : ws[cw] - scale * (pc - p[cw - wz]) / P.dz;
+        for (int k = k0; k < k1; ++k, cu += uz, cv += vz, cw += wz) {
+            const bool zt = k + 1 < nz;
+            const double pp = zt ? p[cw + wz] : 0.0;
+            const double pw = i > 0 ? p[cw - 1] : 0.0, pe = i + 1 < nx ? p[cw + 1] : 0.0;
+            const double psn = j > 0 ? p[cw - wy] : 0.0, pn = j + 1 < ny ? p[cw + wy] : 0.0;
+            const double us0 = i > 0 ? us[cu] : 0.0, us1 = i + 1 < nx ? us[cu + 1] : 0.0;
+            const double vs0 = j > 0 ? vs[cv] : 0.0, vs1 = j + 1 < ny ? vs[cv + vy] : 0.0;
+            const double ws1 = zt ? ws[cw + wz] : 0.0;
+            const double ub0 = i == 0 ? u[cu] : 0.0, ub1 = i + 1 == nx ? u[cu + 1] : 0.0;
+            const double vb0 = j == 0 ? v[cv] : 0.0, vb1 = j + 1 == ny ? v[cv + vy] : 0.0;
+            const double wb1 = zt ? 0.0 : w[cw + wz];
+            const double u0 = i > 0 ? us0 - scale * (pc - pw) / P.dx : ub0;
+            const double u1 = i + 1 < nx ? us1 - scale * (pe - pc) / P.dx : ub1;
+            const double v0 = j > 0 ? vs0 - scale * (pc - psn) / P.dy : vb0;
+            const double v1 = j + 1 < ny ? vs1 - scale * (pn - pc) / P.dy : vb1;
+            const double w1 = zt ? ws1 - scale * (pp - pc) / P.dz : wb1;
+            if (i > 0) u[cu] = u0;
+            if (j > 0) v[cv] = v0;
+            if (zt) w[cw + wz] = w1;
             const double div = (u1 - u0) / P.dx + (v1 - v0) / P.dy + (w1 - w0) / P.dz;
             s[0] += div * div;
             m[0] = fmax(m[0], fabs(div));
             const double a = 0.5 * (u0 + u1), bb = 0.5 * (v0 + v1), cc = 0.5 * (w0 + w1);
             m[1] = fmax(m[1], sqrt(a * a + bb * bb + cc * cc));
-            if (check) {
-                const double pc = p[cw];  // cells and w faces share the layout up to nz
-                if (!isfinite(pc)) atomicMin(&dg->first_bad[0], (unsigned long long)cw);
-                if (!isfinite(u0)) atomicMin(&dg->first_bad[1], (unsigned long long)cu);
-                if (i + 1 == nx && !isfinite(u1)) atomicMin(&dg->first_bad[1], (unsigned long long)(cu + 1));
-                if (!isfinite(v0)) atomicMin(&dg->first_bad[2], (unsigned long long)cv);
-                if (j + 1 == ny && !isfinite(v1)) atomicMin(&dg->first_bad[2], (unsigned long long)(cv + vy));
-                if (!isfinite(w0)) atomicMin(&dg->first_bad[3], (unsigned long long)cw);
-                if (k + 1 == nz && !isfinite(w1)) atomicMin(&dg->first_bad[3], (unsigned long long)(cw + wz));
-            }
+            if (!isfinite(pc)) atomicMin(&dg->first_bad[0], (unsigned long long)cw);
+            if (!isfinite(u0)) atomicMin(&dg->first_bad[1], (unsigned long long)cu);
+            if (i + 1 == nx && !isfinite(u1)) atomicMin(&dg->first_bad[1], (unsigned long long)(cu + 1));
+            if (!isfinite(v0)) atomicMin(&dg->first_bad[2], (unsigned long long)cv);
+            if (j + 1 == ny && !isfinite(v1)) atomicMin(&dg->first_bad[2], (unsigned long long)(cv + vy));
+            if (!isfinite(w0)) atomicMin(&dg->first_bad[3], (unsigned long long)cw);
+            if (!zt && !isfinite(w1)) atomicMin(&dg->first_bad[3], (unsigned long long)(cw + wz));
             w0 = w1;
+            pc = pp;
         }
     }
     publish_row<1, 2>(s, m, pout);
@@ -1406,7 +1398,7 @@ __global__ void __launch_bounds__(kBlock) k_block_mean(Grid g, const double *__r
 // Kernel kinds of the per-kernel profiler, named as the kernels are (with
 // their template arguments).
 enum Kind {
-    K_PREDICT, K_RHS, K_CORRECT, K_DIAG,
+    K_PREDICT, K_RHS, K_CORRECT,
     K_RESIDUAL_D, K_RESIDUAL_F, K_FOLD_OUTER, K_ADD_CORRECTION,
     K_DIRAPPLY_D, K_UPDATE_D, K_DIRAPPLY_F, K_UPDATE_F,
     K_MG_CHECK, K_MG_SWEEP_L0, K_MG_SWEEP_L0_PROLONG, K_MG_SWEEP_L0_PUBLISH,
@@ -1414,7 +1406,7 @@ enum Kind {
     K_COUNT
 };
 const char *const kKindNames[K_COUNT] = {
-    "k_predict3_z", "k_rhs", "k_correct", "k_diag_z",
+    "k_predict3_z", "k_rhs", "k_correct_diag_z",
     "k_residual<double>", "k_residual<float>", "k_fold_outer", "k_add_correction",
     "k_cg_dirapply_z<double>", "k_cg_update<double, double>",
     "k_cg_dirapply_z<float>", "k_cg_update<float, float>",
@@ -2062,17 +2054,20 @@ fl_status solve_pressure(fl_cavity3d *s, fl_solve_report *rep) {
     return s->cfg.precision == FL_PRECISION_MIXED ? solve_mixed(s, rep) : solve_fp64(s, rep);
 }
 
-fl_status run_diag(fl_cavity3d *s, fl_cavity3d_step_report *rep, bool check_finite) {
+// The correction and the step's diagnostics (one fused launch + the fold),
+// then the finite check's verdict.
+fl_status correct_and_diag(fl_cavity3d *s, fl_cavity3d_step_report *rep) {
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
     DiagState init{};
     for (int f = 0; f < 4; ++f) init.first_bad[f] = ~0ull;
     s->host->dg = init;
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->dg, &s->host->dg, sizeof(DiagState), hipMemcpyHostToDevice, stm));
-    const double faces = 8.0 * (double)(s->nu + s->nv + s->nw);
+    // algorithmic bytes: us, vs, ws read, u, v, w written, p read
+    const double faces = 8.0 * (double)(s->nu + s->nv + s->nw), cells = 8.0 * s->n;
     const dim3 zg(s->zg.bx, s->zg.by, s->zg.bz);
-    FL_TIMED(s, K_DIAG, -1, faces + (check_finite ? 8.0 * s->n : 0.0),
-             k_diag_z<<<zg, kZT, 0, stm>>>(s->P, s->p, s->u, s->v, s->w, s->partials, s->dg, check_finite ? 1 : 0);
+    FL_TIMED(s, K_CORRECT, -1, 2.0 * faces + cells,
+             k_correct_diag_z<<<zg, kZT, 0, stm>>>(s->P, s->p, s->us, s->vs, s->ws, s->u, s->v, s->w, s->partials, s->dg);
              k_diag_fold<<<1, kBlock, 0, stm>>>(s->partials, s->zg.count, s->dg));
     FL_HIP_TRY(ctx, hipGetLastError());
     FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->dg, s->dg, sizeof(DiagState), hipMemcpyDeviceToHost, stm));
@@ -2084,7 +2079,7 @@ fl_status run_diag(fl_cavity3d *s, fl_cavity3d_step_report *rep, bool check_fini
         rep->divergence_l2 = std::sqrt(h.div_l2sq);
         rep->max_speed = h.max_speed;
     }
-    if (check_finite) {
+    {
         const char *names[4] = {"pressure", "u", "v", "w"};
         double *arrs[4] = {s->p, s->u, s->v, s->w};
         for (int f = 0; f < 4; ++f)
@@ -2122,11 +2117,8 @@ fl_status do_step(fl_cavity3d *s, fl_cavity3d_step_report *rep) {
         rep->pressure_warm_ratio = sr.warm_start_ratio;
     }
     if (st != FL_OK) return st;
-    FL_TIMED(s, K_CORRECT, -1, 2.0 * faces + cells,
-             k_correct<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->p, s->us, s->vs, s->ws, s->u, s->v, s->w));
-    FL_HIP_TRY(ctx, hipGetLastError());
     s->step += 1;
-    st = run_diag(s, rep, true);
+    st = correct_and_diag(s, rep);
     if (rep) {
         rep->step = s->step;
         rep->time = (double)s->step * s->cfg.dt;
