@@ -765,6 +765,7 @@ __device__ inline void zmarch_side(uint32_t nx, uint32_t ny, uint32_t nz, Ld &&l
         ec = ep;
     }
 }
+
 template <typename Ld, typename Mk, typename Body>
 __device__ inline void zmarch(uint32_t nx, uint32_t ny, uint32_t nz, Ld &&load, Mk &&make, Body &&body) {
     zmarch_side(
@@ -852,7 +853,11 @@ __global__ void __launch_bounds__(kZT) k_mg_sweep0_z(Level L, Level LC, const fl
     struct Raw {
         float e, c;
     };
-    zmarch(
+    struct Side {
+        float f, q;
+    };
+    // f (and q_old) of the cell are fetched with the plane's stencil loads
+    zmarch_side(
         L.nx, L.ny, L.nz,
         [&](uint32_t c, uint32_t i, uint32_t j, uint32_t k) -> Raw {
             return Raw{ein[c], kProlong ? ec[(i >> 1) + LC.nx * ((j >> 1) + LC.ny * (k >> 1))] : 0.0f};
@@ -860,13 +865,14 @@ __global__ void __launch_bounds__(kZT) k_mg_sweep0_z(Level L, Level LC, const fl
         [&](const Raw &r, uint32_t, uint32_t, uint32_t, uint32_t) -> float {
             return kProlong ? r.e + corr * r.c : r.e;
         },
-        [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const float (&v)[7]) {
+        [&](uint32_t c) -> Side { return Side{f_at(f, c, true, f0), publish ? qold[c] : 0.0f}; },
+        [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const float (&v)[7], const Side &sd) {
             const float ae = L.cx * ((v[0] - v[1]) + (v[0] - v[2])) + L.cy * ((v[0] - v[3]) + (v[0] - v[4])) +
                              L.cz * ((v[0] - v[5]) + (v[0] - v[6]));
-            const float out = v[0] + kOmega * inv[lvl_didx(L, i, j, k)] * (f_at(f, c, true, f0) - ae);
+            const float out = v[0] + kOmega * inv[lvl_didx(L, i, j, k)] * (sd.f - ae);
             eout[c] = out;
             if (publish) {
-                const double qo = (double)qold[c];
+                const double qo = (double)sd.q;
                 s[0] += (double)out * qo;
                 s[1] += qo;
             }
@@ -909,15 +915,16 @@ __global__ void __launch_bounds__(kZT) k_mg_check_z(Level L, CgParams prm, const
     if (done) return;
     const float *inv = lvl_inv_table(L);
     const float f0 = (float)(-sumr);
-    zmarch(
+    zmarch_side(
         L.nx, L.ny, L.nz, [&](uint32_t c, uint32_t, uint32_t, uint32_t) -> float { return f_at(r, c, true, f0); },
         [&](float fv, uint32_t, uint32_t i, uint32_t j, uint32_t k) -> float {
             return kOmega * inv[lvl_didx(L, i, j, k)] * fv;
         },
-        [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const float (&v)[7]) {
+        [&](uint32_t c) -> float { return f_at(r, c, true, f0); },
+        [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const float (&v)[7], float fc) {
             const float ae = L.cx * ((v[0] - v[1]) + (v[0] - v[2])) + L.cy * ((v[0] - v[3]) + (v[0] - v[4])) +
                              L.cz * ((v[0] - v[5]) + (v[0] - v[6]));
-            eout[c] = v[0] + kOmega * inv[lvl_didx(L, i, j, k)] * (f_at(r, c, true, f0) - ae);
+            eout[c] = v[0] + kOmega * inv[lvl_didx(L, i, j, k)] * (fc - ae);
         });
 }
 
