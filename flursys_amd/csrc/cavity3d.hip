@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(kBlock) k_add_correction(uint32_t n, double *_
 //   p = z - (z.q_old / p_old.q_old) p_old,  q = A p,  alpha = r.p / p.q.
 // ---------------------------------------------------------------------------
 constexpr float kOmega = 6.0f / 7.0f;
-constexpr float kCorr = 1.8f;
+constexpr float kCorr = 2.0f;
 constexpr int kMaxLevels = 12;
 constexpr uint32_t kTailCells = 4096;
 
@@ -1753,6 +1753,8 @@ fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep, double rr0_ref = 0.0)
 // each (k_mg_down / k_mg_up); finer ones as separate sweep and restriction
 // launches.
 constexpr int kFusedFrom = 2;
+// Damped-Jacobi sweeps on level 0 before and after the coarse correction.
+constexpr int kL0Smooth = 3;
 fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k) {
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
@@ -1766,10 +1768,15 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     // mea, which the restriction and the up pass read
     FL_TIMED(s, K_MG_CHECK, k, 8.0 * n0,
              k_mg_check_z<<<zg, kZT, 0, stm>>>(L[0], prm, s->r32, s->meb[0], s->pb, G, s->fa, s->fb));
-    FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
-             k_mg_sweep0_z<false, false><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, kCorr));
+    float *pre0 = s->meb[0], *post0 = s->mea[0];  // level 0's pre-smoothed iterate, and the other buffer
+    if (kL0Smooth == 3) {
+        FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
+                 k_mg_sweep0_z<false, false><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, kCorr));
+        pre0 = s->mea[0];
+        post0 = s->meb[0];
+    }
     FL_TIMED(s, K_MG_RESTRICT_L0, k, 8.0 * n0 + 4.0 * n1,
-             k_mg_restrict0_z<<<dim3((L[0].nx + kRX - 1) / kRX, L[0].ny / 2, s->zg.bz), kRX, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->mf[1], s->fa, s->fb));
+             k_mg_restrict0_z<<<dim3((L[0].nx + kRX - 1) / kRX, L[0].ny / 2, s->zg.bz), kRX, 0, stm>>>(L[0], L[1], s->r32, pre0, s->mf[1], s->fa, s->fb));
     for (int l = 1; l < t; ++l) {
         const double nl = L[l].n, nc = L[l + 1].n;
         if (l >= kFusedFrom) {
@@ -1803,10 +1810,14 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
                      k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr));
         }
     }
+    // up on level 0: prolongate-correct sweep into post0 (kL0Smooth == 2:
+    // mea) or into meb (3, then one more sweep into mea); the publishing
+    // sweep leaves z~ in meb
     FL_TIMED(s, K_MG_SWEEP_L0_PROLONG, k, 12.0 * n0 + 4.0 * n1,
-             k_mg_sweep0_z<true, false><<<zg, kZT, 0, stm>>>(L[0], L[1], s->r32, s->mea[0], s->meb[1], s->meb[0], nullptr, nullptr, s->fa, s->fb, kCorr));
-    FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
-             k_mg_sweep0_z<false, false><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, kCorr));
+             k_mg_sweep0_z<true, false><<<zg, kZT, 0, stm>>>(L[0], L[1], s->r32, pre0, s->meb[1], post0, nullptr, nullptr, s->fa, s->fb, kCorr));
+    if (kL0Smooth == 3)
+        FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
+                 k_mg_sweep0_z<false, false><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, kCorr));
     FL_TIMED(s, K_MG_SWEEP_L0_PUBLISH, k, 16.0 * n0,
              k_mg_sweep0_z<false, true><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], s->q32, s->pz, s->fa, s->fb, kCorr));
     // FCG direction (reads z~, p_old, r; writes p, q) and update (x, r
