@@ -393,6 +393,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Level L, Level LC, const
 // pseudo-inverse (n x n, symmetric), else by 60 sweeps with the mean removed
 // (the Neumann null space).
 constexpr int kTailMax = 3;
+constexpr int kFcgHints = 8;
 constexpr uint32_t kDirectCells = 128;
 struct TailLevels {
     Level lv[kTailMax];
@@ -880,17 +881,15 @@ __global__ void __launch_bounds__(kZT) k_mg_sweep0_z(Level L, Level LC, const fl
     if (kPublish) publish_row<2, 1>(s, m, pout);
 }
 
-// Stop test of the inner FCG (folds the `rows` update partials), then the
-// first two pre-sweeps of the cycle on level 0 from a zero iterate:
-// e1 = omega D^-1 f is pointwise and marched, e2 = e1 + omega D^-1 (f - A e1).
-__global__ void __launch_bounds__(kZT) k_mg_check_z(Level L, CgParams prm, const float *__restrict__ r,
-                                                    float *__restrict__ eout,
-                                                    const double *__restrict__ pin, uint32_t rows,
-                                                    FcgA *SA, const FcgB *SB) {
-    if (SA->done || SB->done) return;
+// Stop test of the inner FCG: folds the `rows` update partials (r.r, sum r,
+// max|r| after the last update), and the lead thread records the verdict.
+// Returns done; *sumr gets sum r.
+__device__ inline int fcg_stop_test(const CgParams &prm, const double *__restrict__ pin, uint32_t rows,
+                                    FcgA *SA, const FcgB *SB, bool lead, double *sumr) {
     double fs[3], fm[1];
     fold_partials<3, 1>(pin, rows, fs, fm);
-    const double rr = fs[1], sumr = fs[2], rmax = fm[0];
+    const double rr = fs[1], rmax = fm[0];
+    *sumr = fs[2];
     const bool first = SB->first != 0;
     const double rr0 = first ? rr : SA->rr0;
     int done = 0, conv = 0, status = ST_OK;
@@ -902,16 +901,38 @@ __global__ void __launch_bounds__(kZT) k_mg_check_z(Level L, CgParams prm, const
     } else if (SB->iters >= prm.max_iter) {
         done = 1;
     }
-    const bool lead = blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0;
     if (lead && threadIdx.x == 0) {
         SA->rr0 = rr0;
         SA->rr = rr;
         SA->rmax = rmax;
-        SA->sumr = sumr;
+        SA->sumr = *sumr;
         SA->converged = conv;
         SA->status = status;
         SA->done = done;
     }
+    return done;
+}
+
+// The stop test alone (one block), launched after a batch of iterations so
+// the host learns the verdict without launching a whole further iteration.
+__global__ void __launch_bounds__(kBlock) k_fcg_check(CgParams prm, const double *__restrict__ pin,
+                                                      uint32_t rows, FcgA *SA, const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    double sumr;
+    (void)fcg_stop_test(prm, pin, rows, SA, SB, true, &sumr);
+}
+
+// Stop test of the inner FCG, then the first two pre-sweeps of the cycle on
+// level 0 from a zero iterate: e1 = omega D^-1 f is pointwise and marched,
+// e2 = e1 + omega D^-1 (f - A e1).
+__global__ void __launch_bounds__(kZT) k_mg_check_z(Level L, CgParams prm, const float *__restrict__ r,
+                                                    float *__restrict__ eout,
+                                                    const double *__restrict__ pin, uint32_t rows,
+                                                    FcgA *SA, const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    double sumr;
+    const bool lead = blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0;
+    const int done = fcg_stop_test(prm, pin, rows, SA, SB, lead, &sumr);
     if (done) return;
     const float *inv = lvl_inv_table(L);
     const float f0 = (float)(-sumr);
@@ -1452,6 +1473,7 @@ struct fl_cavity3d {
     float *mf[kMaxLevels] = {}, *mea[kMaxLevels] = {}, *meb[kMaxLevels] = {};
     uint32_t lblocks[kMaxLevels] = {};
     float *pinv = nullptr;  // coarsest level's pseudo-inverse (<= kDirectCells cells)
+    int fcg_hint[kFcgHints] = {};  // inner FCG updates of the last step, per refinement round
     // learned warm start (FL_PRECOND_LEARNED): the network works on level
     // wl_level; wl_in / wl_out hold its input and output there
     fl_unet *unet = nullptr;
@@ -1832,7 +1854,8 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
 // Inner FCG on A e = r32 (e32 = 0 on entry, r32 and its partials in pb from
 // k_residual).  Returns with the device state in host->fa / host->fb;
 // *stagnated is set when the residual fails to halve over a whole batch.
-fl_status run_fcg(fl_cavity3d *s, const CgParams &prm, bool *stagnated) {
+// `hint` is the update count this solve needed last time (0: unknown).
+fl_status run_fcg(fl_cavity3d *s, const CgParams &prm, int hint, bool *stagnated) {
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
     s->host->fa = FcgA{};
@@ -1841,18 +1864,21 @@ fl_status run_fcg(fl_cavity3d *s, const CgParams &prm, bool *stagnated) {
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->fa, &s->host->fa, sizeof(FcgA), hipMemcpyHostToDevice, stm));
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->fb, &s->host->fb, sizeof(FcgB), hipMemcpyHostToDevice, stm));
     *stagnated = false;
-    // Batches are sized from the observed contraction so that few launches
-    // land after the device-side stop test: the check of iteration k sees the
-    // residual after k updates, so after launching iterations 0..L-1 the
-    // host knows rr_{L-1} and rr_0.
+    // A batch of iterations is followed by the stop test alone (k_fcg_check),
+    // so after launching iterations 0..L-1 the host knows rr_L, the residual
+    // after L updates.  The first batch is the hint (a solve in a time loop
+    // needs about what it needed last step), later ones are sized from the
+    // observed contraction, so few iterations land after the device-side
+    // stop test.
     long long launched = 0, at_prev = 0;
-    int batch = 4;
+    int batch = hint > 0 ? std::min(hint, 16) : 4;
     double rr_prev = -1.0;
     for (;;) {
         for (int b = 0; b < batch; ++b) {
             fl_status st = launch_fcg_iteration(s, prm, launched++);
             if (st != FL_OK) return st;
         }
+        k_fcg_check<<<1, kBlock, 0, stm>>>(prm, s->pb, s->blocks, s->fa, s->fb);
         FL_HIP_TRY(ctx, hipGetLastError());
         FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->fa, s->fa, sizeof(FcgA), hipMemcpyDeviceToHost, stm));
         FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->fb, s->fb, sizeof(FcgB), hipMemcpyDeviceToHost, stm));
@@ -1860,7 +1886,7 @@ fl_status run_fcg(fl_cavity3d *s, const CgParams &prm, bool *stagnated) {
         FL_HIP_TRY(ctx, s->prof.flush(s->host->fb.iters));
         if (s->host->fa.done || s->host->fb.done) break;
         const double rr = s->host->fa.rr, rr0 = s->host->fa.rr0;
-        const long long at = launched - 1;
+        const long long at = launched;
         if (rr_prev < 0.0) rr_prev = rr0;
         const long long span = at - at_prev;
         const double f = (span > 0 && rr_prev > 0.0) ? std::pow(rr / rr_prev, 1.0 / (double)span) : 0.5;
@@ -2009,11 +2035,13 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
         if (mg) {
             const long long budget = std::min<long long>(s->cfg.pressure_max_iters - total, 100);
             const CgParams prm{itol, budget, FL_STOP_REL_L2, 0.0};
-            fl_status st = run_fcg(s, prm, &stagnated);
+            const int slot = (int)std::min<long long>(sweeps, kFcgHints - 1);
+            fl_status st = run_fcg(s, prm, s->fcg_hint[slot], &stagnated);
             if (st != FL_OK) return st;
             const FcgB &b = s->host->fb;
             status = b.status != ST_OK ? b.status : s->host->fa.status;
             it = b.iters;
+            s->fcg_hint[slot] = (int)std::min<long long>(it, 16);
             budget_hit = !s->host->fa.converged && !stagnated && status == ST_OK;
         } else {
             const CgParams prm{itol, s->cfg.pressure_max_iters - total, FL_STOP_REL_L2, 0.0};
