@@ -29,7 +29,7 @@
 // Mixed precision (FL_PRECISION_MIXED): the same two kernels run on FP32
 // vectors (dots, alpha, beta and the stop test stay FP64) as the inner
 // solver of FP64 iterative refinement: r = b - A x is recomputed in FP64
-// (k_residual), the inner CG solves A e = r to a relative tolerance, and
+// (k_residual_z), the inner CG solves A e = r to a relative tolerance, and
 // x += e in FP64, until the outer FP64 stop test holds.
 
 #include <algorithm>
@@ -122,43 +122,6 @@ struct CgParams {
     double rr0_ref;  // > 0: the rel-L2 test is against this ||r0||^2
 };
 enum { ST_OK = 0, ST_SINGULAR = 1, ST_ALPHA = 2, ST_RESIDUAL = 3 };
-
-// r = b - A x with b = -rhs (b_0 = 0, row 0 the identity of the pin);
-// publishes r.z, r.r, max|r| as the "update" partials of iteration 0.  For
-// the mixed path the residual is computed in FP64 and stored as FP32 for the
-// inner solver, with the partials taken from the stored values.
-template <typename TR>
-__global__ void __launch_bounds__(kBlock) k_residual(Grid g, const double *__restrict__ rhs,
-                                                     const double *__restrict__ x,
-                                                     TR *__restrict__ r, double *partials) {
-    const TR *inv_tab = inv_table<TR>(g);
-    double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
-    TileSched ts((g.n + kBlock - 1) / kBlock);
-    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
-        const uint32_t c = t * kBlock + threadIdx.x;
-        if (c >= g.n) continue;
-        TR rc = 0;
-        double inv = 1.0;
-        if (c != 0) {
-            uint32_t i, j, k;
-            cell_ijk(g, c, i, j, k);
-            const Nbr o = neighbours(g, c, i, j, k);
-            const double xc = x[c];
-            const double ax = g.ix2 * ((xc - x[o.e]) + (xc - x[o.w])) +
-                              g.iy2 * ((xc - x[o.n]) + (xc - x[o.s])) +
-                              g.iz2 * ((xc - x[o.f]) + (xc - x[o.b]));
-            rc = (TR)(-rhs[c] - ax);
-            inv = (double)inv_tab[diag_index(g, i, j, k)];
-        }
-        r[c] = rc;
-        const double rd = (double)rc;
-        s[0] += rd * (rd * inv);
-        s[1] += rd * rd;
-        s[2] += rd;
-        m[0] = fmax(m[0], fabs(rd));
-    }
-    publish_partials<3, 1>(s, m, partials);
-}
 
 template <typename T, typename TX>
 __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x,
@@ -835,6 +798,41 @@ __device__ inline void publish_row(double (&s)[NS], double (&m)[NM], double *pou
     }
 }
 
+// r = b - A x with b = -rhs (b_0 = 0, row 0 the identity of the pin) as a
+// column march over x; each block publishes one row {r.z, r.r, sum r |
+// max|r|} (z = D^-1 r), the "update" partials the first iteration of an
+// inner solve folds.  For the mixed path the residual is computed in FP64
+// and stored as FP32, with the partials taken from the stored values.
+template <typename TR>
+__global__ void __launch_bounds__(kZT) k_residual_z(Grid g, const double *__restrict__ rhs,
+                                                    const double *__restrict__ x, TR *__restrict__ r,
+                                                    double *__restrict__ pout) {
+    const TR *inv_tab = inv_table<TR>(g);
+    double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
+    zmarch_side(
+        g.nx, g.ny, g.nz, [&](uint32_t c, uint32_t, uint32_t, uint32_t) -> double { return x[c]; },
+        [&](double v, uint32_t, uint32_t, uint32_t, uint32_t) -> double { return v; },
+        [&](uint32_t c) -> double { return rhs[c]; },
+        [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const double (&v)[7], double rh) {
+            TR rc = 0;
+            double inv = 1.0;
+            if (c != 0) {
+                const double ax = g.ix2 * ((v[0] - v[1]) + (v[0] - v[2])) + g.iy2 * ((v[0] - v[3]) + (v[0] - v[4])) +
+                                  g.iz2 * ((v[0] - v[5]) + (v[0] - v[6]));
+                rc = (TR)(-rh - ax);
+                inv = (double)inv_tab[diag_index(g, i, j, k)];
+            }
+            r[c] = rc;
+            const double rd = (double)rc;
+            s[0] += rd * (rd * inv);
+            s[1] += rd * rd;
+            s[2] += rd;
+            m[0] = fmax(m[0], fabs(rd));
+        });
+    publish_row<3, 1>(s, m, pout);
+}
+
+
 // Level-0 damped-Jacobi sweep, e_out = e' + omega D^-1 (f - A e') with
 // e' = e_in (+ corr P(e_c), kProlong); kPublish also publishes
 // sum e_out.q_old and sum q_old for the FCG beta.
@@ -1435,7 +1433,7 @@ enum Kind {
 };
 const char *const kKindNames[K_COUNT] = {
     "k_predict3_z", "k_rhs", "k_correct_diag_z",
-    "k_residual<double>", "k_residual<float>", "k_fold_outer", "k_add_correction",
+    "k_residual_z<double>", "k_residual_z<float>", "k_fold_outer", "k_add_correction",
     "k_cg_dirapply_z<double>", "k_cg_update<double, double>",
     "k_cg_dirapply_z<float>", "k_cg_update<float, float>",
     "k_mg_check_z", "k_mg_sweep0_z<false, false>",
@@ -1506,7 +1504,7 @@ namespace {
 
 constexpr int kMaxBatch = 64;
 
-// out = {s0, s1, max, s2} of <3,1> partials (for k_residual: r.z, r.r,
+// out = {s0, s1, max, s2} of <3,1> partials (for k_residual_z: r.z, r.r,
 // max|r|, sum r).
 __global__ void k_fold_outer(const double *__restrict__ partials, uint32_t rows, double *out) {
     double s[3], m[1];
@@ -1692,7 +1690,7 @@ const char *status_text(int status) {
 
 // Runs CG iterations (dirapply + update pairs) on vectors of type T until
 // the device state says done.  The "update" partials of iteration 0 must
-// already be in s->pb (from k_residual).
+// already be in s->pb (from k_residual_z, ZGrid zgd's rows).
 template <typename T, typename TX>
 fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf[2], T *q) {
     fl_ctx *ctx = s->ctx;
@@ -1716,8 +1714,9 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
             // iteration `launched` reads d[launched % 2] and writes the other
             T *dold = dbuf[launched & 1], *dnew = dbuf[(launched + 1) & 1];
             FL_TIMED(s, kd, launched, bd,
-                     k_cg_dirapply_z<T><<<zg, kZT, 0, stm>>>(s->g, prm, r, dold, dnew, q, s->pb, G,
-                                                              s->pa, s->sa, s->sb));
+                     k_cg_dirapply_z<T><<<zg, kZT, 0, stm>>>(s->g, prm, r, dold, dnew, q, s->pb,
+                                                              launched == 0 ? s->zgd.count : G, s->pa,
+                                                              s->sa, s->sb));
             FL_TIMED(s, ku, launched, bu,
                      k_cg_update<T, TX><<<G, kBlock, 0, stm>>>(s->g, x, r, dnew, q, s->pa, s->zgd.count,
                                                                s->pb, s->sa, s->sb));
@@ -1739,7 +1738,7 @@ fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep, double rr0_ref = 0.0)
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
     FL_TIMED(s, K_RESIDUAL_D, -1, 24.0 * s->n,
-             k_residual<double><<<s->blocks, kBlock, 0, stm>>>(s->g, s->rhs, s->p, s->r, s->pb));
+             k_residual_z<double><<<dim3(s->zgd.bx, s->zgd.by, s->zgd.bz), kZT, 0, stm>>>(s->g, s->rhs, s->p, s->r, s->pb));
     FL_HIP_TRY(ctx, hipGetLastError());
     const CgParams prm{s->cfg.pressure_tolerance, s->cfg.pressure_max_iters, s->cfg.stop_rule,
                        rr0_ref};
@@ -1789,7 +1788,7 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     // down: the first two pre-sweeps in one pass (into meb), the third into
     // mea, which the restriction and the up pass read
     FL_TIMED(s, K_MG_CHECK, k, 8.0 * n0,
-             k_mg_check_z<<<zg, kZT, 0, stm>>>(L[0], prm, s->r32, s->meb[0], s->pb, G, s->fa, s->fb));
+             k_mg_check_z<<<zg, kZT, 0, stm>>>(L[0], prm, s->r32, s->meb[0], s->pb, k == 0 ? s->zgd.count : G, s->fa, s->fb));
     float *pre0 = s->meb[0], *post0 = s->mea[0];  // level 0's pre-smoothed iterate, and the other buffer
     if (kL0Smooth == 3) {
         FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
@@ -1852,7 +1851,7 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
 }
 
 // Inner FCG on A e = r32 (e32 = 0 on entry, r32 and its partials in pb from
-// k_residual).  Returns with the device state in host->fa / host->fb;
+// k_residual_z, zgd.count rows).  Returns with the device state in host->fa / host->fb;
 // *stagnated is set when the residual fails to halve over a whole batch.
 // `hint` is the update count this solve needed last time (0: unknown).
 fl_status run_fcg(fl_cavity3d *s, const CgParams &prm, int hint, bool *stagnated) {
@@ -1987,9 +1986,9 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
     double warm_ratio = 0.0;
     for (;;) {
         FL_TIMED(s, K_RESIDUAL_F, -1, 20.0 * s->n,
-                 k_residual<float><<<s->blocks, kBlock, 0, stm>>>(s->g, s->rhs, s->p, s->r32, s->pb));
-        FL_TIMED(s, K_FOLD_OUTER, -1, 32.0 * s->blocks,
-                 k_fold_outer<<<1, kBlock, 0, stm>>>(s->pb, s->blocks, s->outer));
+                 k_residual_z<float><<<dim3(s->zgd.bx, s->zgd.by, s->zgd.bz), kZT, 0, stm>>>(s->g, s->rhs, s->p, s->r32, s->pb));
+        FL_TIMED(s, K_FOLD_OUTER, -1, 32.0 * s->zgd.count,
+                 k_fold_outer<<<1, kBlock, 0, stm>>>(s->pb, s->zgd.count, s->outer));
         FL_HIP_TRY(ctx, hipGetLastError());
         FL_HIP_TRY(ctx, hipMemcpyAsync(s->host->outer, s->outer, 4 * sizeof(double), hipMemcpyDeviceToHost, stm));
         FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
