@@ -120,6 +120,7 @@ struct CgParams {
     long long max_iter;
     int stop_rule;
     double rr0_ref;  // > 0: the rel-L2 test is against this ||r0||^2
+    double e_bound;  // multigrid FCG: bound of |A^-1 r| / max|r| (L_max^2 / pi^2)
 };
 enum { ST_OK = 0, ST_SINGULAR = 1, ST_ALPHA = 2, ST_RESIDUAL = 3 };
 
@@ -214,7 +215,8 @@ struct Level {
 
 struct FcgA {  // written by block 0 of k_mg_check (the stop test)
     double rr0, rr, rmax, sumr;
-    int done, converged, status, pad;
+    int done, converged, status;
+    float escale;  // storage scale of the level-0 iterates this iteration
 };
 struct FcgB {  // written by block 0 of k_fcg_update (and the host)
     double pq;
@@ -357,6 +359,26 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Level L, Level LC, const
 // (the Neumann null space).
 constexpr int kTailMax = 3;
 constexpr int kFcgHints = 8;
+
+// Level-0 multigrid iterates are stored as FP16 scaled by a power of two,
+// halving the bytes of the level-0 sweeps' iterate streams: they only steer
+// the flexible CG, whose vectors stay FP32 and whose dots and stop tests
+// stay FP64.  Arithmetic is FP32; scaling by a power of two is exact.  The
+// iterates of a V-cycle on r are bounded by |A^-1 r| <= max|r| L_max^2 / pi^2
+// (the operator's smallest non-zero eigenvalue is at least pi^2 / L_max^2)
+// and the smoothest ones a sweep makes are ~max|r| h^2 / 12; the scale
+// S = 2^(10 - e), with max|r| L_max^2 / pi^2 in [2^(e-1), 2^e), maps the
+// largest below 2^10 and leaves about n^2 of range above FP16's smallest
+// normal (6e-5).  An overflow (a diverging solve) makes the inner solve
+// stagnate, which hands over to the FP64 PCG.
+typedef _Float16 e16;
+__device__ inline float e_ld(const e16 *p, size_t i, float inv_scale) { return (float)p[i] * inv_scale; }
+__device__ inline void e_st(e16 *p, size_t i, float v, float scale) { p[i] = (e16)(v * scale); }
+__device__ inline float e_scale_for(double rmax, double e_bound) {
+    int ex = 0;
+    (void)frexp(rmax * e_bound, &ex);
+    return ldexpf(1.0f, min(max(10 - ex, -120), 120));
+}
 constexpr uint32_t kDirectCells = 128;
 struct TailLevels {
     Level lv[kTailMax];
@@ -744,10 +766,11 @@ __device__ inline void zmarch(uint32_t nx, uint32_t ny, uint32_t nz, Ld &&load, 
 // two plane steps.  Reads f and e once, writes f_c.
 constexpr uint32_t kRX = 128;
 __global__ void __launch_bounds__(kRX) k_mg_restrict0_z(Level L, Level LC, const float *__restrict__ f,
-                                                        const float *__restrict__ e,
+                                                        const e16 *__restrict__ e,
                                                         float *__restrict__ fcoarse, const FcgA *SA,
                                                         const FcgB *SB) {
     if (SA->done || SB->done) return;
+    const float iS = 1.0f / SA->escale;
     const float f0 = (float)(-SA->sumr);
     const uint3 b = zblock();
     const uint32_t nx = L.nx, ny = L.ny, nz = L.nz, nxy = L.nxy;
@@ -761,14 +784,14 @@ __global__ void __launch_bounds__(kRX) k_mg_restrict0_z(Level L, Level LC, const
     const uint32_t ds = j > 0 ? nx : 0, dn = j + 2 < ny ? nx : 0;  // row j+1 always exists
     uint32_t c = i + nx * (j + ny * k0);  // row j; row j+1 is c + nx
     const uint32_t cb = k0 > 0 ? c - nxy : c;
-    float am = e[cb], bm = e[cb + nx], ac = e[c], bc = e[c + nx];
+    float am = e_ld(e, cb, iS), bm = e_ld(e, cb + nx, iS), ac = e_ld(e, c, iS), bc = e_ld(e, c + nx, iS);
     const float cx = L.cx, cy = L.cy, cz = L.cz;
     float acc = 0.0f;
     for (uint32_t k = k0; k < k1; ++k, c += nxy) {
         const uint32_t cp = k + 1 < nz ? c + nxy : c;
-        const float ap = e[cp], bp = e[cp + nx];
-        const float ae = e[c + de], aw = e[c - dw], be = e[c + nx + de], bw = e[c + nx - dw];
-        const float as = e[c - ds], bn = e[c + nx + dn];
+        const float ap = e_ld(e, cp, iS), bp = e_ld(e, cp + nx, iS);
+        const float ae = e_ld(e, c + de, iS), aw = e_ld(e, c - dw, iS), be = e_ld(e, c + nx + de, iS), bw = e_ld(e, c + nx - dw, iS);
+        const float as = e_ld(e, c - ds, iS), bn = e_ld(e, c + nx + dn, iS);
         const float fa = (c == 0) ? f0 : f[c], fb = f[c + nx];
         const float ra = fa - (cx * ((ac - ae) + (ac - aw)) + cy * ((ac - bc) + (ac - as)) +
                                cz * ((ac - ap) + (ac - am)));
@@ -838,13 +861,14 @@ __global__ void __launch_bounds__(kZT) k_residual_z(Grid g, const double *__rest
 // sum e_out.q_old and sum q_old for the FCG beta.
 template <bool kProlong, bool kPublish>
 __global__ void __launch_bounds__(kZT) k_mg_sweep0_z(Level L, Level LC, const float *__restrict__ f,
-                                                     const float *__restrict__ ein,
+                                                     const e16 *__restrict__ ein,
                                                      const float *__restrict__ ec,
-                                                     float *__restrict__ eout,
+                                                     e16 *__restrict__ eout,
                                                      const float *__restrict__ qold,
                                                      double *__restrict__ pout, const FcgA *SA,
                                                      const FcgB *SB, float corr) {
     if (SA->done || SB->done) return;
+    const float S = SA->escale, iS = 1.0f / S;
     const float *inv = lvl_inv_table(L);
     const float f0 = (float)(-SA->sumr);
     const bool publish = kPublish && !SB->first;
@@ -859,7 +883,7 @@ __global__ void __launch_bounds__(kZT) k_mg_sweep0_z(Level L, Level LC, const fl
     zmarch_side(
         L.nx, L.ny, L.nz,
         [&](uint32_t c, uint32_t i, uint32_t j, uint32_t k) -> Raw {
-            return Raw{ein[c], kProlong ? ec[(i >> 1) + LC.nx * ((j >> 1) + LC.ny * (k >> 1))] : 0.0f};
+            return Raw{e_ld(ein, c, iS), kProlong ? ec[(i >> 1) + LC.nx * ((j >> 1) + LC.ny * (k >> 1))] : 0.0f};
         },
         [&](const Raw &r, uint32_t, uint32_t, uint32_t, uint32_t) -> float {
             return kProlong ? r.e + corr * r.c : r.e;
@@ -869,7 +893,7 @@ __global__ void __launch_bounds__(kZT) k_mg_sweep0_z(Level L, Level LC, const fl
             const float ae = L.cx * ((v[0] - v[1]) + (v[0] - v[2])) + L.cy * ((v[0] - v[3]) + (v[0] - v[4])) +
                              L.cz * ((v[0] - v[5]) + (v[0] - v[6]));
             const float out = v[0] + kOmega * inv[lvl_didx(L, i, j, k)] * (sd.f - ae);
-            eout[c] = out;
+            e_st(eout, c, out, S);
             if (publish) {
                 const double qo = (double)sd.q;
                 s[0] += (double)out * qo;
@@ -883,11 +907,12 @@ __global__ void __launch_bounds__(kZT) k_mg_sweep0_z(Level L, Level LC, const fl
 // max|r| after the last update), and the lead thread records the verdict.
 // Returns done; *sumr gets sum r.
 __device__ inline int fcg_stop_test(const CgParams &prm, const double *__restrict__ pin, uint32_t rows,
-                                    FcgA *SA, const FcgB *SB, bool lead, double *sumr) {
+                                    FcgA *SA, const FcgB *SB, bool lead, double *sumr, double *rmax_out) {
     double fs[3], fm[1];
     fold_partials<3, 1>(pin, rows, fs, fm);
     const double rr = fs[1], rmax = fm[0];
     *sumr = fs[2];
+    *rmax_out = rmax;
     const bool first = SB->first != 0;
     const double rr0 = first ? rr : SA->rr0;
     int done = 0, conv = 0, status = ST_OK;
@@ -916,22 +941,24 @@ __device__ inline int fcg_stop_test(const CgParams &prm, const double *__restric
 __global__ void __launch_bounds__(kBlock) k_fcg_check(CgParams prm, const double *__restrict__ pin,
                                                       uint32_t rows, FcgA *SA, const FcgB *SB) {
     if (SA->done || SB->done) return;
-    double sumr;
-    (void)fcg_stop_test(prm, pin, rows, SA, SB, true, &sumr);
+    double sumr, rmax;
+    (void)fcg_stop_test(prm, pin, rows, SA, SB, true, &sumr, &rmax);
 }
 
 // Stop test of the inner FCG, then the first two pre-sweeps of the cycle on
 // level 0 from a zero iterate: e1 = omega D^-1 f is pointwise and marched,
 // e2 = e1 + omega D^-1 (f - A e1).
 __global__ void __launch_bounds__(kZT) k_mg_check_z(Level L, CgParams prm, const float *__restrict__ r,
-                                                    float *__restrict__ eout,
+                                                    e16 *__restrict__ eout,
                                                     const double *__restrict__ pin, uint32_t rows,
                                                     FcgA *SA, const FcgB *SB) {
     if (SA->done || SB->done) return;
-    double sumr;
+    double sumr, rmax;
     const bool lead = blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0;
-    const int done = fcg_stop_test(prm, pin, rows, SA, SB, lead, &sumr);
+    const int done = fcg_stop_test(prm, pin, rows, SA, SB, lead, &sumr, &rmax);
     if (done) return;
+    const float S = e_scale_for(rmax, prm.e_bound);
+    if (lead && threadIdx.x == 0) SA->escale = S;
     const float *inv = lvl_inv_table(L);
     const float f0 = (float)(-sumr);
     zmarch_side(
@@ -943,7 +970,7 @@ __global__ void __launch_bounds__(kZT) k_mg_check_z(Level L, CgParams prm, const
         [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const float (&v)[7], float fc) {
             const float ae = L.cx * ((v[0] - v[1]) + (v[0] - v[2])) + L.cy * ((v[0] - v[3]) + (v[0] - v[4])) +
                              L.cz * ((v[0] - v[5]) + (v[0] - v[6]));
-            eout[c] = v[0] + kOmega * inv[lvl_didx(L, i, j, k)] * (fc - ae);
+            e_st(eout, c, v[0] + kOmega * inv[lvl_didx(L, i, j, k)] * (fc - ae), S);
         });
 }
 
@@ -1023,7 +1050,7 @@ __global__ void __launch_bounds__(kZT) k_cg_dirapply_z(Grid g, CgParams prm, con
 // p = (z~ - z~_0) - beta p_old, q = A p (pinned), partials p.q and r.p;
 // folds the `rows` partials of the publishing sweep.  A column march: each
 // block publishes one row, ZGrid::count rows in all.
-__global__ void __launch_bounds__(kZT) k_fcg_dirapply_z(Grid g, const float *__restrict__ zt,
+__global__ void __launch_bounds__(kZT) k_fcg_dirapply_z(Grid g, const e16 *__restrict__ zt,
                                                         const float *__restrict__ pold,
                                                         float *__restrict__ pnew, float *__restrict__ q,
                                                         const float *__restrict__ r,
@@ -1031,10 +1058,11 @@ __global__ void __launch_bounds__(kZT) k_fcg_dirapply_z(Grid g, const float *__r
                                                         double *__restrict__ pout, const FcgA *SA,
                                                         const FcgB *SB) {
     if (SA->done || SB->done) return;
+    const float iS = 1.0f / SA->escale;
     const bool first = SB->first != 0;
     double fz[2], fm[1];
     fold_partials<2, 1>(pz, rows, fz, fm);
-    const float z0 = zt[0];
+    const float z0 = e_ld(zt, 0, iS);
     const double beta = first ? 0.0 : (fz[0] - (double)z0 * fz[1]) / SB->pq;
     const float tb = (float)beta;
     const float cx = (float)g.ix2, cy = (float)g.iy2, cz = (float)g.iz2;
@@ -1044,7 +1072,7 @@ __global__ void __launch_bounds__(kZT) k_fcg_dirapply_z(Grid g, const float *__r
     };
     zmarch_side(
         g.nx, g.ny, g.nz,
-        [&](uint32_t c, uint32_t, uint32_t, uint32_t) -> Raw { return Raw{zt[c], first ? 0.0f : pold[c]}; },
+        [&](uint32_t c, uint32_t, uint32_t, uint32_t) -> Raw { return Raw{e_ld(zt, c, iS), first ? 0.0f : pold[c]}; },
         [&](const Raw &v, uint32_t c, uint32_t, uint32_t, uint32_t) -> float {
             return c == 0 ? 0.0f : (v.z - z0) - tb * v.p;  // the pinned cell's p stays 0
         },
@@ -1468,7 +1496,9 @@ struct fl_cavity3d {
     // memory, levels tail..nlev-1 inside k_mg_tail
     int nlev = 0, tail = 0;
     Level lv[kMaxLevels]{};
-    float *mf[kMaxLevels] = {}, *mea[kMaxLevels] = {}, *meb[kMaxLevels] = {};
+    float *mf[kMaxLevels] = {}, *mea[kMaxLevels] = {}, *meb[kMaxLevels] = {};  // levels >= 1
+    e16 *m0a = nullptr, *m0b = nullptr;  // level 0's iterates
+    double e_bound = 1.0;                 // L_max^2 / pi^2 (see e_scale_for)
     uint32_t lblocks[kMaxLevels] = {};
     float *pinv = nullptr;  // coarsest level's pseudo-inverse (<= kDirectCells cells)
     int fcg_hint[kFcgHints] = {};  // inner FCG updates of the last step, per refinement round
@@ -1531,7 +1561,7 @@ void free_all(fl_cavity3d *s) {
                     s->sb,  s->dg,     s->outer};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    void *more[] = {s->pz, s->fa, s->fb, s->wl_in, s->wl_out, s->pair, s->pinv};
+    void *more[] = {s->pz, s->fa, s->fb, s->wl_in, s->wl_out, s->pair, s->pinv, s->m0a, s->m0b};
     for (void *b : more)
         if (b) (void)hipFree(b);
     for (int l = 0; l < kMaxLevels; ++l) {
@@ -1632,6 +1662,11 @@ fl_status build_hierarchy(fl_cavity3d *s) {
     int t = 1;
     while (s->lv[t].n > kTailCells) ++t;
     s->tail = t;
+    // the longest side from the level-0 coefficients (cx = 1 / hx^2)
+    const double lx = s->g.nx / std::sqrt(s->g.ix2), ly = s->g.ny / std::sqrt(s->g.iy2),
+                 lz = s->g.nz / std::sqrt(s->g.iz2);
+    const double lmax = std::max(lx, std::max(ly, lz));
+    s->e_bound = lmax * lmax / (M_PI * M_PI);
     if (nl - t > kTailMax)
         return set_error(s->ctx, FL_ERR_CONFIG, "multigrid tail of %d levels (at most %d)", nl - t,
                          kTailMax);
@@ -1741,7 +1776,7 @@ fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep, double rr0_ref = 0.0)
              k_residual_z<double><<<dim3(s->zgd.bx, s->zgd.by, s->zgd.bz), kZT, 0, stm>>>(s->g, s->rhs, s->p, s->r, s->pb));
     FL_HIP_TRY(ctx, hipGetLastError());
     const CgParams prm{s->cfg.pressure_tolerance, s->cfg.pressure_max_iters, s->cfg.stop_rule,
-                       rr0_ref};
+                       rr0_ref, 0.0};
     fl_status st = run_cg<double, double>(s, prm, s->p, s->r, s->d, s->q);
     if (st != FL_OK) return st;
     const CgStateA &a = s->host->a;
@@ -1766,10 +1801,11 @@ fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep, double rr0_ref = 0.0)
 
 // One multigrid-preconditioned FCG iteration: the V-cycle on r (levels
 // 0..tail-1 here, the rest in k_mg_tail), then the direction and update.
-// Algorithmic HBM bytes per launch (FP32 words): a sweep reads f and e_in
-// and writes e_out (the first sweep of a level reads no e_in); prolongation
-// adds the coarse level once, the publishing sweep q_old; a restriction
-// reads f and e of the fine level and writes the coarse f.
+// Algorithmic HBM bytes per launch (FP32 words, level 0's iterates FP16): a
+// sweep reads f and e_in and writes e_out (the first sweep of a level reads
+// no e_in); prolongation adds the coarse level once, the publishing sweep
+// q_old; a restriction reads f and e of the fine level and writes the
+// coarse f.
 // Levels from kFusedFrom on run their down and up passes as one fused launch
 // each (k_mg_down / k_mg_up); finer ones as separate sweep and restriction
 // launches.
@@ -1787,16 +1823,16 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     const double n0 = L[0].n, n1 = L[1].n;
     // down: the first two pre-sweeps in one pass (into meb), the third into
     // mea, which the restriction and the up pass read
-    FL_TIMED(s, K_MG_CHECK, k, 8.0 * n0,
-             k_mg_check_z<<<zg, kZT, 0, stm>>>(L[0], prm, s->r32, s->meb[0], s->pb, k == 0 ? s->zgd.count : G, s->fa, s->fb));
-    float *pre0 = s->meb[0], *post0 = s->mea[0];  // level 0's pre-smoothed iterate, and the other buffer
+    FL_TIMED(s, K_MG_CHECK, k, 6.0 * n0,
+             k_mg_check_z<<<zg, kZT, 0, stm>>>(L[0], prm, s->r32, s->m0b, s->pb, k == 0 ? s->zgd.count : G, s->fa, s->fb));
+    e16 *pre0 = s->m0b, *post0 = s->m0a;  // level 0's pre-smoothed iterate, and the other buffer
     if (kL0Smooth == 3) {
-        FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
-                 k_mg_sweep0_z<false, false><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, kCorr));
-        pre0 = s->mea[0];
-        post0 = s->meb[0];
+        FL_TIMED(s, K_MG_SWEEP_L0, k, 8.0 * n0,
+                 k_mg_sweep0_z<false, false><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->m0b, nullptr, s->m0a, nullptr, nullptr, s->fa, s->fb, kCorr));
+        pre0 = s->m0a;
+        post0 = s->m0b;
     }
-    FL_TIMED(s, K_MG_RESTRICT_L0, k, 8.0 * n0 + 4.0 * n1,
+    FL_TIMED(s, K_MG_RESTRICT_L0, k, 6.0 * n0 + 4.0 * n1,
              k_mg_restrict0_z<<<dim3((L[0].nx + kRX - 1) / kRX, L[0].ny / 2, s->zg.bz), kRX, 0, stm>>>(L[0], L[1], s->r32, pre0, s->mf[1], s->fa, s->fb));
     for (int l = 1; l < t; ++l) {
         const double nl = L[l].n, nc = L[l + 1].n;
@@ -1834,17 +1870,17 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     // up on level 0: prolongate-correct sweep into post0 (kL0Smooth == 2:
     // mea) or into meb (3, then one more sweep into mea); the publishing
     // sweep leaves z~ in meb
-    FL_TIMED(s, K_MG_SWEEP_L0_PROLONG, k, 12.0 * n0 + 4.0 * n1,
+    FL_TIMED(s, K_MG_SWEEP_L0_PROLONG, k, 8.0 * n0 + 4.0 * n1,
              k_mg_sweep0_z<true, false><<<zg, kZT, 0, stm>>>(L[0], L[1], s->r32, pre0, s->meb[1], post0, nullptr, nullptr, s->fa, s->fb, kCorr));
     if (kL0Smooth == 3)
-        FL_TIMED(s, K_MG_SWEEP_L0, k, 12.0 * n0,
-                 k_mg_sweep0_z<false, false><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->meb[0], nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, kCorr));
-    FL_TIMED(s, K_MG_SWEEP_L0_PUBLISH, k, 16.0 * n0,
-             k_mg_sweep0_z<false, true><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->mea[0], nullptr, s->meb[0], s->q32, s->pz, s->fa, s->fb, kCorr));
+        FL_TIMED(s, K_MG_SWEEP_L0, k, 8.0 * n0,
+                 k_mg_sweep0_z<false, false><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->m0b, nullptr, s->m0a, nullptr, nullptr, s->fa, s->fb, kCorr));
+    FL_TIMED(s, K_MG_SWEEP_L0_PUBLISH, k, 12.0 * n0,
+             k_mg_sweep0_z<false, true><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->m0a, nullptr, s->m0b, s->q32, s->pz, s->fa, s->fb, kCorr));
     // FCG direction (reads z~, p_old, r; writes p, q) and update (x, r
     // read and written, p and q read)
-    FL_TIMED(s, K_FCG_DIRAPPLY, k, 20.0 * n0,
-             k_fcg_dirapply_z<<<dim3(s->zgd.bx, s->zgd.by, s->zgd.bz), kZT, 0, stm>>>(s->g, s->meb[0], pold, pnew, s->q32, s->r32, s->pz, s->zg.count, s->pa, s->fa, s->fb));
+    FL_TIMED(s, K_FCG_DIRAPPLY, k, 18.0 * n0,
+             k_fcg_dirapply_z<<<dim3(s->zgd.bx, s->zgd.by, s->zgd.bz), kZT, 0, stm>>>(s->g, s->m0b, pold, pnew, s->q32, s->r32, s->pz, s->zg.count, s->pa, s->fa, s->fb));
     FL_TIMED(s, K_FCG_UPDATE, k, 24.0 * n0,
              k_fcg_update<<<G, kBlock, 0, stm>>>((uint32_t)s->n, s->e32, s->r32, pnew, s->q32, s->pa, s->zgd.count, s->pb, s->fa, s->fb));
     return FL_OK;
@@ -1945,12 +1981,14 @@ fl_status warm_start(fl_cavity3d *s, double sum_r) {
         k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(lv[l], lv[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb, 1.0f);
         ec = s->meb[l];
     }
-    float *e0 = s->meb[0];
+    // level 0 in FP32 scratch: the inner solve's direction buffers, free until
+    // it starts (e0 must outlive the residual check that may undo it)
+    float *e0 = s->d32[1], *e1 = s->d32[0];
     if (L >= 1)
-        k_mg_sweep<false, true, true, false><<<s->blocks, kBlock, 0, stm>>>(lv[0], lv[1], s->r32, nullptr, ec, s->mea[0], nullptr, nullptr, s->fa, s->fb, 1.0f);
+        k_mg_sweep<false, true, true, false><<<s->blocks, kBlock, 0, stm>>>(lv[0], lv[1], s->r32, nullptr, ec, e1, nullptr, nullptr, s->fa, s->fb, 1.0f);
     else
-        k_mg_sweep<false, false, true, false><<<s->blocks, kBlock, 0, stm>>>(lv[0], lv[0], s->r32, ec, nullptr, s->mea[0], nullptr, nullptr, s->fa, s->fb, 1.0f);
-    k_mg_sweep<false, false, true, false><<<s->blocks, kBlock, 0, stm>>>(lv[0], lv[0], s->r32, s->mea[0], nullptr, e0, nullptr, nullptr, s->fa, s->fb, 1.0f);
+        k_mg_sweep<false, false, true, false><<<s->blocks, kBlock, 0, stm>>>(lv[0], lv[0], s->r32, ec, nullptr, e1, nullptr, nullptr, s->fa, s->fb, 1.0f);
+    k_mg_sweep<false, false, true, false><<<s->blocks, kBlock, 0, stm>>>(lv[0], lv[0], s->r32, e1, nullptr, e0, nullptr, nullptr, s->fa, s->fb, 1.0f);
     // alpha = r0.e~ / e~.A e~
     k_warm_dots<<<s->blocks, kBlock, 0, stm>>>(s->g, s->r32, e0, s->pb);
     k_fold_outer<<<1, kBlock, 0, stm>>>(s->pb, s->blocks, s->outer);
@@ -2033,7 +2071,7 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
             itol = std::max(inner_tol, 0.5 * s->cfg.pressure_tolerance * std::sqrt(rr0 / rr));
         if (mg) {
             const long long budget = std::min<long long>(s->cfg.pressure_max_iters - total, 100);
-            const CgParams prm{itol, budget, FL_STOP_REL_L2, 0.0};
+            const CgParams prm{itol, budget, FL_STOP_REL_L2, 0.0, s->e_bound};
             const int slot = (int)std::min<long long>(sweeps, kFcgHints - 1);
             fl_status st = run_fcg(s, prm, s->fcg_hint[slot], &stagnated);
             if (st != FL_OK) return st;
@@ -2043,7 +2081,7 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
             s->fcg_hint[slot] = (int)std::min<long long>(it, 16);
             budget_hit = !s->host->fa.converged && !stagnated && status == ST_OK;
         } else {
-            const CgParams prm{itol, s->cfg.pressure_max_iters - total, FL_STOP_REL_L2, 0.0};
+            const CgParams prm{itol, s->cfg.pressure_max_iters - total, FL_STOP_REL_L2, 0.0, 0.0};
             fl_status st = run_cg<float, float>(s, prm, s->e32, s->r32, s->d32, s->q32);
             if (st != FL_OK) return st;
             const CgStateB &b = s->host->b;
@@ -2238,7 +2276,11 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
             goto fail;
         for (int l = 0; l <= s->tail; ++l) {
             const size_t n = s->lv[l].n;
-            if (l > 0 && (st = alloc(s, &s->mf[l], n))) goto fail;
+            if (l == 0) {
+                if ((st = alloc(s, &s->m0a, n)) || (st = alloc(s, &s->m0b, n))) goto fail;
+                continue;
+            }
+            if ((st = alloc(s, &s->mf[l], n))) goto fail;
             if (l < s->tail && (st = alloc(s, &s->mea[l], n))) goto fail;
             if ((st = alloc(s, &s->meb[l], n))) goto fail;
         }

@@ -275,6 +275,32 @@ def test_multigrid_fcg_meets_fp64_tolerance(gpu_ctx, shape):
     assert rel(p, pr) < 1e-6
 
 
+@pytest.mark.parametrize("size,scale", [(40.0, 1.0), (0.02, 1.0), (1.0, 1e25), (1.0, 1e-6),
+                                        (3.0, 1e12)])
+def test_multigrid_fcg_scaled_problems(gpu_ctx, size, scale):
+    """The level-0 iterates are stored in FP16 scaled from max|r| and the
+    domain size: large and small domains and right-hand sides far from 1
+    still meet the FP64 tolerance without the FP64 fallback.  (Much smaller
+    right-hand sides trip the solvers' absolute 1e-30 breakdown thresholds,
+    the reference's, before any storage format matters.)"""
+    print(f"[gpu] MG-FCG scaled problem size={size} scale={scale}", flush=True)
+    n = 32
+    rng = np.random.default_rng(5)
+    rhs = rng.standard_normal((n, n, n)) * scale
+    rhs[0, 0, 0] = 0.0
+    s = _mg(gpu_ctx, n, length=size, height=size, depth=size)
+    p = np.zeros_like(rhs)
+    rep = s.pressure_solve(rhs, p)
+    assert rep.converged and not rep.fallback, (rep.iterations, rep.fallback)
+    assert rep.iterations < 60, rep.iterations
+    # the reference on the unscaled problem (the solution is linear in rhs)
+    ref = make(gpu_ctx, n, n, n, length=size, height=size, depth=size, pressure_tolerance=1e-12,
+               stop_rule=fa.STOP_REL_L2, initial_guess=fa.GUESS_ZERO, pressure_max_iters=20000)
+    pr = np.zeros_like(rhs)
+    ref.pressure_solve(rhs / scale, pr)
+    assert rel(p, pr * scale) < 1e-6
+
+
 def test_multigrid_fcg_matches_dct(gpu_ctx):
     d = np.load(os.path.join(GOLD, "neumann_dct.npz"))
     s = _mg(gpu_ctx, 16, pressure_tolerance=1e-10)
