@@ -855,6 +855,66 @@ __global__ void __launch_bounds__(kZT) k_residual_z(Grid g, const double *__rest
     publish_row<3, 1>(s, m, pout);
 }
 
+// The same residual with two cells per lane (x even): the centre, the +-y
+// and the +-z operands are 16-byte loads of the pair, so a cell costs half
+// the vector-memory instructions (the single-cell march is bound by the
+// texture-address unit); only the pair's outer x neighbours are 8-byte
+// loads.  Arithmetic, in the same order as k_residual_z, gives the same r.
+template <typename T2> struct Pair;
+template <> struct Pair<double> { typedef double2 type; };
+template <> struct Pair<float> { typedef float2 type; };
+template <typename TR>
+__global__ void __launch_bounds__(kZT) k_residual_z2(Grid g, const double *__restrict__ rhs,
+                                                     const double *__restrict__ x, TR *__restrict__ r,
+                                                     double *__restrict__ pout) {
+    typedef typename Pair<TR>::type TR2;
+    const TR *inv_tab = inv_table<TR>(g);
+    double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
+    const uint3 b = zblock();
+    const uint32_t nx = g.nx, ny = g.ny, nz = g.nz, nxy = nx * ny;
+    const uint32_t i0 = 2 * (b.x * kZX + threadIdx.x % kZX), j = b.y * kZY + threadIdx.x / kZX;
+    const uint32_t kz = (nz + gridDim.z - 1) / gridDim.z, k0 = b.z * kz;
+    if (i0 < nx && j < ny && k0 < nz) {  // nx is even, so i0 + 1 < nx
+        const uint32_t k1 = min(k0 + kz, nz);
+        const bool xw = i0 > 0, xe = i0 + 2 < nx, ys = j > 0, yn = j + 1 < ny;
+        const double ix2 = g.ix2, iy2 = g.iy2, iz2 = g.iz2;
+        uint32_t c = i0 + nx * (j + ny * k0);
+        double2 xc = *(const double2 *)(x + c);
+        double2 xm = k0 > 0 ? *(const double2 *)(x + c - nxy) : xc;
+        for (uint32_t k = k0; k < k1; ++k, c += nxy) {
+            const double2 xp = k + 1 < nz ? *(const double2 *)(x + c + nxy) : xc;
+            const double2 xn = yn ? *(const double2 *)(x + c + nx) : xc;
+            const double2 xs = ys ? *(const double2 *)(x + c - nx) : xc;
+            const double xe1 = xe ? x[c + 2] : xc.y, xw0 = xw ? x[c - 1] : xc.x;
+            const double2 rh = *(const double2 *)(rhs + c);
+            const double ax0 = ix2 * ((xc.x - xc.y) + (xc.x - xw0)) + iy2 * ((xc.x - xn.x) + (xc.x - xs.x)) +
+                               iz2 * ((xc.x - xp.x) + (xc.x - xm.x));
+            const double ax1 = ix2 * ((xc.y - xe1) + (xc.y - xc.x)) + iy2 * ((xc.y - xn.y) + (xc.y - xs.y)) +
+                               iz2 * ((xc.y - xp.y) + (xc.y - xm.y));
+            TR r0 = 0, r1 = (TR)(-rh.y - ax1);
+            double inv0 = 1.0;
+            if (c != 0) {
+                r0 = (TR)(-rh.x - ax0);
+                inv0 = (double)inv_tab[diag_index(g, i0, j, k)];
+            }
+            const double inv1 = (double)inv_tab[diag_index(g, i0 + 1, j, k)];
+            TR2 rv;
+            rv.x = r0;
+            rv.y = r1;
+            *(TR2 *)(r + c) = rv;
+            const double d0 = (double)r0, d1 = (double)r1;
+            s[0] += d0 * (d0 * inv0) + d1 * (d1 * inv1);
+            s[1] += d0 * d0 + d1 * d1;
+            s[2] += d0 + d1;
+            m[0] = fmax(m[0], fmax(fabs(d0), fabs(d1)));
+            xm = xc;
+            xc = xp;
+        }
+    }
+    publish_row<3, 1>(s, m, pout);
+}
+
+
 
 // Level-0 damped-Jacobi sweep, e_out = e' + omega D^-1 (f - A e') with
 // e' = e_in (+ corr P(e_c), kProlong); kPublish also publishes
@@ -1526,6 +1586,7 @@ struct fl_cavity3d {
     uint32_t blocks = 1, ew_blocks = 1;
     ZGrid zg{1, 1, 1, 1};   // column-marching launch of the level-0 sweeps
     ZGrid zgd{1, 1, 1, 1};  // and of the search directions
+    ZGrid zr{1, 1, 1, 1};   // of the residual (two cells per lane when nx is even)
     int64_t step = 0;
     KProf prof;  // per-kernel timing (HIP events on the solver's stream)
 };
@@ -1750,7 +1811,7 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
             T *dold = dbuf[launched & 1], *dnew = dbuf[(launched + 1) & 1];
             FL_TIMED(s, kd, launched, bd,
                      k_cg_dirapply_z<T><<<zg, kZT, 0, stm>>>(s->g, prm, r, dold, dnew, q, s->pb,
-                                                              launched == 0 ? s->zgd.count : G, s->pa,
+                                                              launched == 0 ? s->zr.count : G, s->pa,
                                                               s->sa, s->sb));
             FL_TIMED(s, ku, launched, bu,
                      k_cg_update<T, TX><<<G, kBlock, 0, stm>>>(s->g, x, r, dnew, q, s->pa, s->zgd.count,
@@ -1769,11 +1830,20 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
     return FL_OK;
 }
 
+// r = b - A p into r, partials (s->zr.count rows) into pb.
+template <typename TR>
+void launch_residual(fl_cavity3d *s, TR *r) {
+    const dim3 grid(s->zr.bx, s->zr.by, s->zr.bz);
+    if (s->g.nx % 2 == 0)
+        k_residual_z2<TR><<<grid, kZT, 0, s->ctx->stream>>>(s->g, s->rhs, s->p, r, s->pb);
+    else
+        k_residual_z<TR><<<grid, kZT, 0, s->ctx->stream>>>(s->g, s->rhs, s->p, r, s->pb);
+}
+
 fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep, double rr0_ref = 0.0) {
     fl_ctx *ctx = s->ctx;
-    hipStream_t stm = ctx->stream;
     FL_TIMED(s, K_RESIDUAL_D, -1, 24.0 * s->n,
-             k_residual_z<double><<<dim3(s->zgd.bx, s->zgd.by, s->zgd.bz), kZT, 0, stm>>>(s->g, s->rhs, s->p, s->r, s->pb));
+             launch_residual<double>(s, s->r));
     FL_HIP_TRY(ctx, hipGetLastError());
     const CgParams prm{s->cfg.pressure_tolerance, s->cfg.pressure_max_iters, s->cfg.stop_rule,
                        rr0_ref, 0.0};
@@ -1824,7 +1894,7 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     // down: the first two pre-sweeps in one pass (into meb), the third into
     // mea, which the restriction and the up pass read
     FL_TIMED(s, K_MG_CHECK, k, 6.0 * n0,
-             k_mg_check_z<<<zg, kZT, 0, stm>>>(L[0], prm, s->r32, s->m0b, s->pb, k == 0 ? s->zgd.count : G, s->fa, s->fb));
+             k_mg_check_z<<<zg, kZT, 0, stm>>>(L[0], prm, s->r32, s->m0b, s->pb, k == 0 ? s->zr.count : G, s->fa, s->fb));
     e16 *pre0 = s->m0b, *post0 = s->m0a;  // level 0's pre-smoothed iterate, and the other buffer
     if (kL0Smooth == 3) {
         FL_TIMED(s, K_MG_SWEEP_L0, k, 8.0 * n0,
@@ -2024,9 +2094,9 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
     double warm_ratio = 0.0;
     for (;;) {
         FL_TIMED(s, K_RESIDUAL_F, -1, 20.0 * s->n,
-                 k_residual_z<float><<<dim3(s->zgd.bx, s->zgd.by, s->zgd.bz), kZT, 0, stm>>>(s->g, s->rhs, s->p, s->r32, s->pb));
-        FL_TIMED(s, K_FOLD_OUTER, -1, 32.0 * s->zgd.count,
-                 k_fold_outer<<<1, kBlock, 0, stm>>>(s->pb, s->zgd.count, s->outer));
+                 launch_residual<float>(s, s->r32));
+        FL_TIMED(s, K_FOLD_OUTER, -1, 32.0 * s->zr.count,
+                 k_fold_outer<<<1, kBlock, 0, stm>>>(s->pb, s->zr.count, s->outer));
         FL_HIP_TRY(ctx, hipGetLastError());
         FL_HIP_TRY(ctx, hipMemcpyAsync(s->host->outer, s->outer, 4 * sizeof(double), hipMemcpyDeviceToHost, stm));
         FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
@@ -2248,6 +2318,11 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
     const bool mixed = cfg->precision == FL_PRECISION_MIXED;
     s->zg = zgrid(nx, ny, nz, kZK);
     s->zgd = zgrid(nx, ny, nz, kZKD);
+    s->zr = s->zgd;
+    if (nx % 2 == 0) {
+        s->zr.bx = (nx / 2 + kZX - 1) / kZX;
+        s->zr.count = s->zr.bx * s->zr.by * s->zr.bz;
+    }
     const size_t prow =
         (size_t)std::max(std::max(s->blocks, s->ew_blocks), std::max(s->zg.count, s->zgd.count)) * 4;
     if ((st = alloc(s, &s->u, s->nu)) || (st = alloc(s, &s->v, s->nv)) ||
