@@ -1059,6 +1059,65 @@ __device__ inline void cg_direction(const Grid &g, const T *inv, const T *r, con
             s[0] += (double)v[0] * (double)qc;
         });
 }
+// cg_direction with two cells per lane (x even; launched on ZGrid zr): the
+// centre, +-y and +-z operands are 16-byte loads of the pair, only the pair's
+// outer x neighbours are single loads.  Same arithmetic, same d' and q.
+// Used for FP64 (185 -> 175 us at 256^3); the FP32 direction was slower so.
+template <bool kFirst, typename T>
+__device__ inline void cg_direction2(const Grid &g, const T *inv, const T *r, const T *dold, T *dnew, T *q,
+                                     T tb, double (&s)[1]) {
+    typedef typename Pair<T>::type T2;
+    const T cx = (T)g.ix2, cy = (T)g.iy2, cz = (T)g.iz2;
+    const uint3 b = zblock();
+    const uint32_t nx = g.nx, ny = g.ny, nz = g.nz, nxy = nx * ny;
+    const uint32_t i0 = 2 * (b.x * kZX + threadIdx.x % kZX), j = b.y * kZY + threadIdx.x / kZX;
+    const uint32_t kz = (nz + gridDim.z - 1) / gridDim.z, k0 = b.z * kz;
+    if (i0 >= nx || j >= ny || k0 >= nz) return;
+    const uint32_t k1 = min(k0 + kz, nz);
+    const bool xw = i0 > 0, xe = i0 + 2 < nx, ys = j > 0, yn = j + 1 < ny;
+    // d' of one cell from its raw r, d
+    auto mk = [&](T rv, T dv, uint32_t c, uint32_t i, uint32_t jj, uint32_t k) -> T {
+        const T z = c == 0 ? rv : rv * inv[diag_index(g, i, jj, k)];
+        return kFirst ? z : z + tb * dv;
+    };
+    auto mk2 = [&](uint32_t c, uint32_t jj, uint32_t k) -> T2 {
+        const T2 rv = *(const T2 *)(r + c);
+        T2 dv;
+        dv.x = 0;
+        dv.y = 0;
+        if (!kFirst) dv = *(const T2 *)(dold + c);
+        T2 o;
+        o.x = mk(rv.x, dv.x, c, i0, jj, k);
+        o.y = mk(rv.y, dv.y, c + 1, i0 + 1, jj, k);
+        return o;
+    };
+    auto mk1 = [&](uint32_t c, uint32_t i, uint32_t k) -> T {
+        return mk(r[c], kFirst ? (T)0 : dold[c], c, i, j, k);
+    };
+    uint32_t c = i0 + nx * (j + ny * k0);
+    T2 ec = mk2(c, j, k0);
+    T2 em = k0 > 0 ? mk2(c - nxy, j, k0 - 1) : ec;
+    for (uint32_t k = k0; k < k1; ++k, c += nxy) {
+        const T2 ep = k + 1 < nz ? mk2(c + nxy, j, k + 1) : ec;
+        const T2 en = yn ? mk2(c + nx, j + 1, k) : ec;
+        const T2 es = ys ? mk2(c - nx, j - 1, k) : ec;
+        const T we = xw ? mk1(c - 1, i0 - 1, k) : ec.x;
+        const T ee = xe ? mk1(c + 2, i0 + 2, k) : ec.y;
+        const T qa = c == 0 ? ec.x
+                            : cx * ((ec.x - ec.y) + (ec.x - we)) + cy * ((ec.x - en.x) + (ec.x - es.x)) +
+                                  cz * ((ec.x - ep.x) + (ec.x - em.x));
+        const T qb = cx * ((ec.y - ee) + (ec.y - ec.x)) + cy * ((ec.y - en.y) + (ec.y - es.y)) +
+                     cz * ((ec.y - ep.y) + (ec.y - em.y));
+        *(T2 *)(dnew + c) = ec;
+        T2 qv;
+        qv.x = qa;
+        qv.y = qb;
+        *(T2 *)(q + c) = qv;
+        s[0] += (double)ec.x * (double)qa + (double)ec.y * (double)qb;
+        em = ec;
+        ec = ep;
+    }
+}
 template <typename T>
 __global__ void __launch_bounds__(kZT) k_cg_dirapply_z(Grid g, CgParams prm, const T *__restrict__ r,
                                                        const T *__restrict__ dold, T *__restrict__ dnew,
@@ -1102,8 +1161,13 @@ __global__ void __launch_bounds__(kZT) k_cg_dirapply_z(Grid g, CgParams prm, con
     }
     if (done) return;
     double s[1] = {0.0}, m[1] = {0.0};
-    if (first) cg_direction<true, T>(g, inv, r, dold, dnew, q, (T)0, s);
-    else cg_direction<false, T>(g, inv, r, dold, dnew, q, (T)beta, s);
+    if (sizeof(T) == 8 && g.nx % 2 == 0) {  // FP64 on even x: launched on ZGrid zr
+        if (first) cg_direction2<true, T>(g, inv, r, dold, dnew, q, (T)0, s);
+        else cg_direction2<false, T>(g, inv, r, dold, dnew, q, (T)beta, s);
+    } else {
+        if (first) cg_direction<true, T>(g, inv, r, dold, dnew, q, (T)0, s);
+        else cg_direction<false, T>(g, inv, r, dold, dnew, q, (T)beta, s);
+    }
     publish_row<1, 1>(s, m, pout);
 }
 
@@ -1797,8 +1861,11 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->sa, &s->host->a, sizeof(CgStateA), hipMemcpyHostToDevice, stm));
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->sb, &s->host->b, sizeof(CgStateB), hipMemcpyHostToDevice, stm));
     const uint32_t G = s->blocks;
-    const dim3 zg(s->zgd.bx, s->zgd.by, s->zgd.bz);
     const bool fp64 = sizeof(T) == 8;
+    // the FP64 direction marches two cells per lane on even-x grids (zr); the
+    // FP32 one is faster one cell per lane (zgd)
+    const ZGrid &zd = fp64 ? s->zr : s->zgd;
+    const dim3 zg(zd.bx, zd.by, zd.bz);
     const int kd = fp64 ? K_DIRAPPLY_D : K_DIRAPPLY_F, ku = fp64 ? K_UPDATE_D : K_UPDATE_F;
     // algorithmic bytes: dirapply reads r, d and writes d', q; update reads
     // x, d', r, q and writes x, r
@@ -1814,7 +1881,7 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
                                                               launched == 0 ? s->zr.count : G, s->pa,
                                                               s->sa, s->sb));
             FL_TIMED(s, ku, launched, bu,
-                     k_cg_update<T, TX><<<G, kBlock, 0, stm>>>(s->g, x, r, dnew, q, s->pa, s->zgd.count,
+                     k_cg_update<T, TX><<<G, kBlock, 0, stm>>>(s->g, x, r, dnew, q, s->pa, zd.count,
                                                                s->pb, s->sa, s->sb));
             ++launched;
         }
