@@ -1059,10 +1059,10 @@ __device__ inline void cg_direction(const Grid &g, const T *inv, const T *r, con
             s[0] += (double)v[0] * (double)qc;
         });
 }
-// cg_direction with two cells per lane (x even; launched on ZGrid zr): the
+// cg_direction with two cells per lane (x even; launched on ZGrid zp): the
 // centre, +-y and +-z operands are 16-byte loads of the pair, only the pair's
 // outer x neighbours are single loads.  Same arithmetic, same d' and q.
-// Used for FP64 (185 -> 175 us at 256^3); the FP32 direction was slower so.
+// Used for FP64 (185 -> 168 us at 256^3); the FP32 direction was slower so.
 template <bool kFirst, typename T>
 __device__ inline void cg_direction2(const Grid &g, const T *inv, const T *r, const T *dold, T *dnew, T *q,
                                      T tb, double (&s)[1]) {
@@ -1161,7 +1161,7 @@ __global__ void __launch_bounds__(kZT) k_cg_dirapply_z(Grid g, CgParams prm, con
     }
     if (done) return;
     double s[1] = {0.0}, m[1] = {0.0};
-    if (sizeof(T) == 8 && g.nx % 2 == 0) {  // FP64 on even x: launched on ZGrid zr
+    if (sizeof(T) == 8 && g.nx % 2 == 0) {  // FP64 on even x: launched on ZGrid zp
         if (first) cg_direction2<true, T>(g, inv, r, dold, dnew, q, (T)0, s);
         else cg_direction2<false, T>(g, inv, r, dold, dnew, q, (T)beta, s);
     } else {
@@ -1651,6 +1651,7 @@ struct fl_cavity3d {
     ZGrid zg{1, 1, 1, 1};   // column-marching launch of the level-0 sweeps
     ZGrid zgd{1, 1, 1, 1};  // and of the search directions
     ZGrid zr{1, 1, 1, 1};   // of the residual (two cells per lane when nx is even)
+    ZGrid zp{1, 1, 1, 1};   // of the FP64 search direction (likewise, 32-plane marches)
     int64_t step = 0;
     KProf prof;  // per-kernel timing (HIP events on the solver's stream)
 };
@@ -1862,9 +1863,9 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->sb, &s->host->b, sizeof(CgStateB), hipMemcpyHostToDevice, stm));
     const uint32_t G = s->blocks;
     const bool fp64 = sizeof(T) == 8;
-    // the FP64 direction marches two cells per lane on even-x grids (zr); the
+    // the FP64 direction marches two cells per lane on even-x grids (zp); the
     // FP32 one is faster one cell per lane (zgd)
-    const ZGrid &zd = fp64 ? s->zr : s->zgd;
+    const ZGrid &zd = fp64 ? s->zp : s->zgd;
     const dim3 zg(zd.bx, zd.by, zd.bz);
     const int kd = fp64 ? K_DIRAPPLY_D : K_DIRAPPLY_F, ku = fp64 ? K_UPDATE_D : K_UPDATE_F;
     // algorithmic bytes: dirapply reads r, d and writes d', q; update reads
@@ -2386,9 +2387,13 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
     s->zg = zgrid(nx, ny, nz, kZK);
     s->zgd = zgrid(nx, ny, nz, kZKD);
     s->zr = s->zgd;
+    s->zp = s->zgd;
     if (nx % 2 == 0) {
         s->zr.bx = (nx / 2 + kZX - 1) / kZX;
         s->zr.count = s->zr.bx * s->zr.by * s->zr.bz;
+        s->zp = s->zg;
+        s->zp.bx = s->zr.bx;
+        s->zp.count = s->zp.bx * s->zp.by * s->zp.bz;
     }
     const size_t prow =
         (size_t)std::max(std::max(s->blocks, s->ew_blocks), std::max(s->zg.count, s->zgd.count)) * 4;
