@@ -58,6 +58,11 @@ struct Grid {
     float invf[8];
 };
 
+// Two-cell vector types (marches and streams with two cells per lane).
+template <typename T> struct Pair;
+template <> struct Pair<double> { typedef double2 type; };
+template <> struct Pair<float> { typedef float2 type; };
+
 __device__ inline void cell_ijk(const Grid &g, uint32_t c, uint32_t &i, uint32_t &j,
                                 uint32_t &k) {
     k = g.fxy.div(c);
@@ -124,6 +129,7 @@ struct CgParams {
 };
 enum { ST_OK = 0, ST_SINGULAR = 1, ST_ALPHA = 2, ST_RESIDUAL = 3 };
 
+constexpr bool kUpdatePairs = true;
 template <typename T, typename TX>
 __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x,
                                                       T *__restrict__ r,
@@ -156,15 +162,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x
     if (status != ST_OK) return;
     const T ta = (T)alpha;
     double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
-    TileSched ts((g.n + kBlock - 1) / kBlock);
-    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
-        const uint32_t c = t * kBlock + threadIdx.x;
-        if (c >= g.n) continue;
-        const TX xc = x[c];
-        const T dc = d[c], rc0 = r[c], qc = q[c];
-        x[c] = xc + (TX)(ta * dc);
+    auto cell = [&](uint32_t c, TX xc, T dc, T rc0, T qc, TX &xo, T &ro) {
+        xo = xc + (TX)(ta * dc);
         const T rc = rc0 - ta * qc;
-        r[c] = rc;
+        ro = rc;
         T z = rc;  // the same z = D^-1 r the direction kernel forms
         if (c != 0) {
             uint32_t i, j, k;
@@ -176,6 +177,33 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x
         s[1] += rd * rd;
         s[2] += rd;
         m[0] = fmax(m[0], fabs(rd));
+    };
+    if (kUpdatePairs && sizeof(T) == 8 && sizeof(TX) == 8 && g.n % 2 == 0) {
+        // FP64: two cells per lane (16-byte loads and stores), 512-cell tiles
+        typedef typename Pair<T>::type T2;
+        TileSched ts((g.n / 2 + kBlock - 1) / kBlock);
+        for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+            const uint32_t c = 2 * (t * kBlock + threadIdx.x);
+            if (c >= g.n) continue;
+            const T2 xc = *(const T2 *)(x + c), dc = *(const T2 *)(d + c), rc = *(const T2 *)(r + c),
+                     qc = *(const T2 *)(q + c);
+            T2 xo, ro;
+            cell(c, xc.x, dc.x, rc.x, qc.x, xo.x, ro.x);
+            cell(c + 1, xc.y, dc.y, rc.y, qc.y, xo.y, ro.y);
+            *(T2 *)(x + c) = xo;
+            *(T2 *)(r + c) = ro;
+        }
+    } else {
+        TileSched ts((g.n + kBlock - 1) / kBlock);
+        for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+            const uint32_t c = t * kBlock + threadIdx.x;
+            if (c >= g.n) continue;
+            TX xo;
+            T ro;
+            cell(c, x[c], d[c], r[c], q[c], xo, ro);
+            x[c] = xo;
+            r[c] = ro;
+        }
     }
     publish_partials<3, 1>(s, m, pout);
 }
@@ -860,9 +888,6 @@ __global__ void __launch_bounds__(kZT) k_residual_z(Grid g, const double *__rest
 // the vector-memory instructions (the single-cell march is bound by the
 // texture-address unit); only the pair's outer x neighbours are 8-byte
 // loads.  Arithmetic, in the same order as k_residual_z, gives the same r.
-template <typename T2> struct Pair;
-template <> struct Pair<double> { typedef double2 type; };
-template <> struct Pair<float> { typedef float2 type; };
 template <typename TR>
 __global__ void __launch_bounds__(kZT) k_residual_z2(Grid g, const double *__restrict__ rhs,
                                                      const double *__restrict__ x, TR *__restrict__ r,
