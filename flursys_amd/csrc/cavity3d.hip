@@ -1239,6 +1239,7 @@ __global__ void __launch_bounds__(kZT) k_fcg_dirapply_z(Grid g, const e16 *__res
 }
 
 // alpha = r.p / p.q; x += alpha p, r -= alpha q; partials r.r, sum r, max|r|.
+constexpr bool kFcgUpdatePairs = true;
 __global__ void __launch_bounds__(kBlock) k_fcg_update(uint32_t n, float *__restrict__ x,
                                                        float *__restrict__ r,
                                                        const float *__restrict__ p,
@@ -1267,16 +1268,37 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update(uint32_t n, float *__rest
     if (status != ST_OK) return;
     const float ta = (float)alpha;
     double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
-    TileSched ts((n + kBlock - 1) / kBlock);
-    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
-        const uint32_t c = t * kBlock + threadIdx.x;
-        if (c >= n) continue;
-        x[c] += ta * p[c];
-        const float rc = r[c] - ta * q[c];
-        r[c] = rc;
+    auto acc = [&](float rc) {
         s[1] += (double)rc * (double)rc;
         s[2] += (double)rc;
         m[0] = fmax(m[0], fabs((double)rc));
+    };
+    if (kFcgUpdatePairs && n % 2 == 0) {  // two cells per lane, 8-byte loads and stores
+        TileSched ts((n / 2 + kBlock - 1) / kBlock);
+        for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+            const uint32_t c = 2 * (t * kBlock + threadIdx.x);
+            if (c >= n) continue;
+            float2 xc = *(const float2 *)(x + c), rc = *(const float2 *)(r + c);
+            const float2 pc = *(const float2 *)(p + c), qc = *(const float2 *)(q + c);
+            xc.x += ta * pc.x;
+            xc.y += ta * pc.y;
+            rc.x -= ta * qc.x;
+            rc.y -= ta * qc.y;
+            *(float2 *)(x + c) = xc;
+            *(float2 *)(r + c) = rc;
+            acc(rc.x);
+            acc(rc.y);
+        }
+    } else {
+        TileSched ts((n + kBlock - 1) / kBlock);
+        for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+            const uint32_t c = t * kBlock + threadIdx.x;
+            if (c >= n) continue;
+            x[c] += ta * p[c];
+            const float rc = r[c] - ta * q[c];
+            r[c] = rc;
+            acc(rc);
+        }
     }
     publish_partials<3, 1>(s, m, pout);
 }
