@@ -129,7 +129,6 @@ struct CgParams {
 };
 enum { ST_OK = 0, ST_SINGULAR = 1, ST_ALPHA = 2, ST_RESIDUAL = 3 };
 
-constexpr bool kUpdatePairs = true;
 template <typename T, typename TX>
 __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x,
                                                       T *__restrict__ r,
@@ -178,7 +177,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x
         s[2] += rd;
         m[0] = fmax(m[0], fabs(rd));
     };
-    if (kUpdatePairs && sizeof(T) == 8 && sizeof(TX) == 8 && g.n % 2 == 0) {
+    if (sizeof(T) == 8 && sizeof(TX) == 8 && g.n % 2 == 0) {
         // FP64: two cells per lane (16-byte loads and stores), 512-cell tiles
         typedef typename Pair<T>::type T2;
         TileSched ts((g.n / 2 + kBlock - 1) / kBlock);
@@ -1239,7 +1238,6 @@ __global__ void __launch_bounds__(kZT) k_fcg_dirapply_z(Grid g, const e16 *__res
 }
 
 // alpha = r.p / p.q; x += alpha p, r -= alpha q; partials r.r, sum r, max|r|.
-constexpr bool kFcgUpdatePairs = true;
 __global__ void __launch_bounds__(kBlock) k_fcg_update(uint32_t n, float *__restrict__ x,
                                                        float *__restrict__ r,
                                                        const float *__restrict__ p,
@@ -1273,7 +1271,7 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update(uint32_t n, float *__rest
         s[2] += (double)rc;
         m[0] = fmax(m[0], fabs((double)rc));
     };
-    if (kFcgUpdatePairs && n % 2 == 0) {  // two cells per lane, 8-byte loads and stores
+    if (n % 2 == 0) {  // two cells per lane, 8-byte loads and stores
         TileSched ts((n / 2 + kBlock - 1) / kBlock);
         for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
             const uint32_t c = 2 * (t * kBlock + threadIdx.x);
