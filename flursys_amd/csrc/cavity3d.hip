@@ -350,8 +350,9 @@ __global__ void __launch_bounds__(kBlock) k_mg_sweep(Level L, Level LC, const fl
     if (kPublish) publish_partials<2, 1>(s, m, pout);
 }
 
-// Restriction of the residual on a coarse level (level 0 has its column
-// march, k_mg_restrict0_z): f_c(C) = mean over the 8 children of f - A e.
+// Restriction of the residual, f_c(C) = mean over the 8 children of f - A e,
+// one coarse cell per thread (the levels in global memory use the column
+// march k_mg_restrict_z; this one serves the learned warm start's levels).
 __global__ void __launch_bounds__(kBlock) k_mg_restrict(Level L, Level LC, const float *__restrict__ f,
                                                         const float *__restrict__ e,
                                                         float *__restrict__ fcoarse, const FcgA *SA,
@@ -400,6 +401,7 @@ constexpr int kFcgHints = 8;
 // stagnate, which hands over to the FP64 PCG.
 typedef _Float16 e16;
 __device__ inline float e_ld(const e16 *p, size_t i, float inv_scale) { return (float)p[i] * inv_scale; }
+__device__ inline float e_ld(const float *p, size_t i, float) { return p[i]; }  // coarse levels: FP32
 __device__ inline void e_st(e16 *p, size_t i, float v, float scale) { p[i] = (e16)(v * scale); }
 __device__ inline float e_scale_for(double rmax, double e_bound) {
     int ex = 0;
@@ -786,19 +788,22 @@ __device__ inline void zmarch(uint32_t nx, uint32_t ny, uint32_t nz, Ld &&load, 
         [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const auto &v, int) { body(i, j, k, c, v); });
 }
 
-// Level-0 residual restriction, f_c = mean over the 8 children of f - A e,
-// as a column march: a thread owns the column pair (i, j), (i, j + 1) with
-// j even and walks an even number of planes, so the children's y pair sums
-// in registers, the x pair across neighbouring lanes and the z pair across
-// two plane steps.  Reads f and e once, writes f_c.
+// Residual restriction, f_c = mean over the 8 children of f - A e, as a
+// column march: a thread owns the column pair (i, j), (i, j + 1) with j even
+// and walks an even number of planes, so the children's y pair sums in
+// registers, the x pair across neighbouring lanes and the z pair across two
+// plane steps.  Reads f and e once, writes f_c.  E = e16: level 0 (scaled
+// FP16 iterate, f pinned at cell 0); E = float: a coarser level.
 constexpr uint32_t kRX = 128;
-__global__ void __launch_bounds__(kRX) k_mg_restrict0_z(Level L, Level LC, const float *__restrict__ f,
-                                                        const e16 *__restrict__ e,
-                                                        float *__restrict__ fcoarse, const FcgA *SA,
-                                                        const FcgB *SB) {
+template <typename E>
+__global__ void __launch_bounds__(kRX) k_mg_restrict_z(Level L, Level LC, const float *__restrict__ f,
+                                                       const E *__restrict__ e,
+                                                       float *__restrict__ fcoarse, const FcgA *SA,
+                                                       const FcgB *SB) {
     if (SA->done || SB->done) return;
-    const float iS = 1.0f / SA->escale;
-    const float f0 = (float)(-SA->sumr);
+    constexpr bool kL0 = sizeof(E) == 2;
+    const float iS = kL0 ? 1.0f / SA->escale : 1.0f;
+    const float f0 = kL0 ? (float)(-SA->sumr) : 0.0f;
     const uint3 b = zblock();
     const uint32_t nx = L.nx, ny = L.ny, nz = L.nz, nxy = L.nxy;
     const uint32_t kz = (((nz + gridDim.z - 1) / gridDim.z) + 1) & ~1u, k0 = b.z * kz;
@@ -819,7 +824,7 @@ __global__ void __launch_bounds__(kRX) k_mg_restrict0_z(Level L, Level LC, const
         const float ap = e_ld(e, cp, iS), bp = e_ld(e, cp + nx, iS);
         const float ae = e_ld(e, c + de, iS), aw = e_ld(e, c - dw, iS), be = e_ld(e, c + nx + de, iS), bw = e_ld(e, c + nx - dw, iS);
         const float as = e_ld(e, c - ds, iS), bn = e_ld(e, c + nx + dn, iS);
-        const float fa = (c == 0) ? f0 : f[c], fb = f[c + nx];
+        const float fa = (kL0 && c == 0) ? f0 : f[c], fb = f[c + nx];
         const float ra = fa - (cx * ((ac - ae) + (ac - aw)) + cy * ((ac - bc) + (ac - as)) +
                                cz * ((ac - ap) + (ac - am)));
         const float rb = fb - (cx * ((bc - be) + (bc - bw)) + cy * ((bc - bn) + (bc - ac)) +
@@ -985,6 +990,41 @@ __global__ void __launch_bounds__(kZT) k_mg_sweep0_z(Level L, Level LC, const fl
             }
         });
     if (kPublish) publish_row<2, 1>(s, m, pout);
+}
+
+// Damped-Jacobi sweep on a coarse level (FP32 iterates) as a column march,
+// k_mg_sweep's arithmetic: kFirst gives the first two sweeps from a zero
+// iterate (e1 = omega D^-1 f marched, e2 = e1 + omega D^-1 (f - A e1)),
+// otherwise e_out = e' + omega D^-1 (f - A e') with e' = e_in
+// (+ corr P(e_c), kProlong; a null e_in is zero).
+template <bool kFirst, bool kProlong>
+__global__ void __launch_bounds__(kZT) k_mg_sweep_z(Level L, Level LC, const float *__restrict__ f,
+                                                    const float *__restrict__ ein,
+                                                    const float *__restrict__ ec,
+                                                    float *__restrict__ eout, const FcgA *SA,
+                                                    const FcgB *SB, float corr) {
+    if (SA->done || SB->done) return;
+    const float *inv = lvl_inv_table(L);
+    struct Raw {
+        float e, c;
+    };
+    zmarch_side(
+        L.nx, L.ny, L.nz,
+        [&](uint32_t c, uint32_t i, uint32_t j, uint32_t k) -> Raw {
+            if (kFirst) return Raw{f[c], 0.0f};
+            return Raw{ein ? ein[c] : 0.0f,
+                       kProlong ? ec[(i >> 1) + LC.nx * ((j >> 1) + LC.ny * (k >> 1))] : 0.0f};
+        },
+        [&](const Raw &r, uint32_t, uint32_t i, uint32_t j, uint32_t k) -> float {
+            if (kFirst) return kOmega * inv[lvl_didx(L, i, j, k)] * r.e;
+            return kProlong ? r.e + corr * r.c : r.e;
+        },
+        [&](uint32_t c) -> float { return f[c]; },
+        [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const float (&v)[7], float fc) {
+            const float ae = L.cx * ((v[0] - v[1]) + (v[0] - v[2])) + L.cy * ((v[0] - v[3]) + (v[0] - v[4])) +
+                             L.cz * ((v[0] - v[5]) + (v[0] - v[6]));
+            eout[c] = v[0] + kOmega * inv[lvl_didx(L, i, j, k)] * (fc - ae);
+        });
 }
 
 // Stop test of the inner FCG: folds the `rows` update partials (r.r, sum r,
@@ -1635,7 +1675,7 @@ const char *const kKindNames[K_COUNT] = {
     "k_cg_dirapply_z<float>", "k_cg_update<float, float>",
     "k_mg_check_z", "k_mg_sweep0_z<false, false>",
     "k_mg_sweep0_z<true, false>", "k_mg_sweep0_z<false, true>",
-    "k_mg_restrict0_z", "k_mg_down", "k_mg_tail", "k_mg_up", "k_fcg_dirapply_z", "k_fcg_update",
+    "k_mg_restrict_z<e16>", "k_mg_down", "k_mg_tail", "k_mg_up", "k_fcg_dirapply_z", "k_fcg_update",
     "unet_warm_start",  // restriction + U-Net inference + interpolation (many kernels)
 };
 
@@ -1669,6 +1709,7 @@ struct fl_cavity3d {
     e16 *m0a = nullptr, *m0b = nullptr;  // level 0's iterates
     double e_bound = 1.0;                 // L_max^2 / pi^2 (see e_scale_for)
     uint32_t lblocks[kMaxLevels] = {};
+    ZGrid zl[kMaxLevels] = {};  // column-marching launch of levels 1..kFusedFrom-1
     float *pinv = nullptr;  // coarsest level's pseudo-inverse (<= kDirectCells cells)
     int fcg_hint[kFcgHints] = {};  // inner FCG updates of the last step, per refinement round
     // learned warm start (FL_PRECOND_LEARNED): the network works on level
@@ -1841,8 +1882,15 @@ fl_status build_hierarchy(fl_cavity3d *s) {
     if (nl - t > kTailMax)
         return set_error(s->ctx, FL_ERR_CONFIG, "multigrid tail of %d levels (at most %d)", nl - t,
                          kTailMax);
-    for (int l = 0; l < nl; ++l)
-        s->lblocks[l] = grid_blocks(s->ctx, (s->lv[l].n + kBlock - 1) / kBlock, 4);
+    for (int l = 0; l < nl; ++l) {
+        const Level &L = s->lv[l];
+        s->lblocks[l] = grid_blocks(s->ctx, (L.n + kBlock - 1) / kBlock, 4);
+        // an even march length (the restriction pairs planes) short enough
+        // for about 1024 workgroups
+        uint32_t kz = 4;
+        while (kz < kZK && L.n / ((uint64_t)kZT * kz * 2) >= 1024) kz *= 2;
+        s->zl[l] = zgrid(L.nx, L.ny, L.nz, kz);
+    }
     return FL_OK;
 }
 
@@ -2016,18 +2064,19 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
         post0 = s->m0b;
     }
     FL_TIMED(s, K_MG_RESTRICT_L0, k, 6.0 * n0 + 4.0 * n1,
-             k_mg_restrict0_z<<<dim3((L[0].nx + kRX - 1) / kRX, L[0].ny / 2, s->zg.bz), kRX, 0, stm>>>(L[0], L[1], s->r32, pre0, s->mf[1], s->fa, s->fb));
+             k_mg_restrict_z<e16><<<dim3((L[0].nx + kRX - 1) / kRX, L[0].ny / 2, s->zg.bz), kRX, 0, stm>>>(L[0], L[1], s->r32, pre0, s->mf[1], s->fa, s->fb));
     for (int l = 1; l < t; ++l) {
         const double nl = L[l].n, nc = L[l + 1].n;
         if (l >= kFusedFrom) {
             FL_TIMED(s, K_MG_DOWN, k, 8.0 * nl + 4.0 * nc,
                      k_mg_down<<<ftiles(L[l]), kFB, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->mf[l + 1], s->fa, s->fb));
         } else {
-            const uint32_t g = s->lblocks[l];
+            const dim3 g(s->zl[l].bx, s->zl[l].by, s->zl[l].bz);
+            const dim3 gr((L[l].nx + kRX - 1) / kRX, L[l].ny / 2, s->zl[l].bz);
             FL_TIMED(s, K_MG_DOWN, k, 8.0 * nl + 4.0 * nc,
-                     k_mg_sweep<true, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], nullptr, nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr);
-                     k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb, kCorr);
-                     k_mg_restrict<<<s->lblocks[l + 1], kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->mf[l + 1], s->fa, s->fb));
+                     k_mg_sweep_z<true, false><<<g, kZT, 0, stm>>>(L[l], L[l], s->mf[l], nullptr, nullptr, s->meb[l], s->fa, s->fb, kCorr);
+                     k_mg_sweep_z<false, false><<<g, kZT, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], s->fa, s->fb, kCorr);
+                     k_mg_restrict_z<float><<<gr, kRX, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->mf[l + 1], s->fa, s->fb));
         }
     }
     TailLevels T{};
@@ -2043,11 +2092,11 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
             FL_TIMED(s, K_MG_UP, k, 12.0 * nl + 4.0 * nc,
                      k_mg_up<<<ftiles(L[l]), kFB, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->meb[l + 1], s->meb[l], s->fa, s->fb, kCorr));
         } else {
-            const uint32_t g = s->lblocks[l];
+            const dim3 g(s->zl[l].bx, s->zl[l].by, s->zl[l].bz);
             FL_TIMED(s, K_MG_UP, k, 12.0 * nl + 4.0 * nc,
-                     k_mg_sweep<false, true, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->meb[l + 1], s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr);
-                     k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], nullptr, nullptr, s->fa, s->fb, kCorr);
-                     k_mg_sweep<false, false, false, false><<<g, kBlock, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], nullptr, nullptr, s->fa, s->fb, kCorr));
+                     k_mg_sweep_z<false, true><<<g, kZT, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->meb[l + 1], s->meb[l], s->fa, s->fb, kCorr);
+                     k_mg_sweep_z<false, false><<<g, kZT, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], s->fa, s->fb, kCorr);
+                     k_mg_sweep_z<false, false><<<g, kZT, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], s->fa, s->fb, kCorr));
         }
     }
     // up on level 0: prolongate-correct sweep into post0 (kL0Smooth == 2:
