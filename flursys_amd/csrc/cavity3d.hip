@@ -1085,17 +1085,49 @@ __global__ void __launch_bounds__(kZT) k_mg_check_z(Level L, CgParams prm, const
     if (lead && threadIdx.x == 0) SA->escale = S;
     const float *inv = lvl_inv_table(L);
     const float f0 = (float)(-sumr);
-    zmarch_side(
-        L.nx, L.ny, L.nz, [&](uint32_t c, uint32_t, uint32_t, uint32_t) -> float { return f_at(r, c, true, f0); },
-        [&](float fv, uint32_t, uint32_t i, uint32_t j, uint32_t k) -> float {
-            return kOmega * inv[lvl_didx(L, i, j, k)] * fv;
-        },
-        [&](uint32_t c) -> float { return f_at(r, c, true, f0); },
-        [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const float (&v)[7], float fc) {
-            const float ae = L.cx * ((v[0] - v[1]) + (v[0] - v[2])) + L.cy * ((v[0] - v[3]) + (v[0] - v[4])) +
-                             L.cz * ((v[0] - v[5]) + (v[0] - v[6]));
-            e_st(eout, c, v[0] + kOmega * inv[lvl_didx(L, i, j, k)] * (fc - ae), S);
-        });
+    // The march written out: the x and y wall bits of the column and of its
+    // four side neighbours are fixed, so each one's omega D^-1 for a z-interior
+    // and a z-wall plane sits in registers; the raw f of the +z plane is kept
+    // as the next plane's centre.
+    const uint3 b = zblock();
+    const uint32_t nx = L.nx, ny = L.ny, nz = L.nz, nxy = L.nxy;
+    const uint32_t i = b.x * kZX + threadIdx.x % kZX, j = b.y * kZY + threadIdx.x / kZX;
+    if (i >= nx || j >= ny) return;
+    const uint32_t kz = (nz + gridDim.z - 1) / gridDim.z, k0 = b.z * kz;
+    if (k0 >= nz) return;
+    const uint32_t k1 = min(k0 + kz, nz);
+    const uint32_t ie = i + 1 < nx ? i + 1 : i, iw = i > 0 ? i - 1 : i;
+    const uint32_t jn = j + 1 < ny ? j + 1 : j, js = j > 0 ? j - 1 : j;
+    const uint32_t de = ie - i, dw = i - iw, dn = (jn - j) * nx, ds = (j - js) * nx;
+    auto xyb = [&](uint32_t ii, uint32_t jj) -> uint32_t {
+        return (uint32_t)(ii > 0 && ii + 1 < nx) | ((uint32_t)(jj > 0 && jj + 1 < ny) << 1);
+    };
+    const uint32_t bc = xyb(i, j), be = xyb(ie, j), bw = xyb(iw, j), bn = xyb(i, jn), bs = xyb(i, js);
+    const float wc0 = kOmega * inv[bc], wc1 = kOmega * inv[bc | 4], we0 = kOmega * inv[be],
+                we1 = kOmega * inv[be | 4], ww0 = kOmega * inv[bw], ww1 = kOmega * inv[bw | 4],
+                wn0 = kOmega * inv[bn], wn1 = kOmega * inv[bn | 4], ws0 = kOmega * inv[bs],
+                ws1 = kOmega * inv[bs | 4];
+    auto zin = [&](uint32_t k) { return k > 0 && k + 1 < nz; };
+    uint32_t c = i + nx * (j + ny * k0);
+    const uint32_t km = k0 > 0 ? k0 - 1 : k0;
+    float fc = f_at(r, c, true, f0);
+    float em = (zin(km) ? wc1 : wc0) * f_at(r, c - (k0 - km) * nxy, true, f0);
+    float ec = (zin(k0) ? wc1 : wc0) * fc;
+    for (uint32_t k = k0; k < k1; ++k, c += nxy) {
+        const uint32_t kp = k + 1 < nz ? k + 1 : k, cp = c + (kp - k) * nxy;
+        const float fp = f_at(r, cp, true, f0), fe = f_at(r, c + de, true, f0), fw = f_at(r, c - dw, true, f0),
+                    fn = f_at(r, c + dn, true, f0), fs = f_at(r, c - ds, true, f0);
+        const bool zk = zin(k);
+        const float ep = (zin(kp) ? wc1 : wc0) * fp;
+        const float v1 = (zk ? we1 : we0) * fe, v2 = (zk ? ww1 : ww0) * fw, v3 = (zk ? wn1 : wn0) * fn,
+                    v4 = (zk ? ws1 : ws0) * fs;
+        const float ae = L.cx * ((ec - v1) + (ec - v2)) + L.cy * ((ec - v3) + (ec - v4)) +
+                         L.cz * ((ec - ep) + (ec - em));
+        e_st(eout, c, ec + (zk ? wc1 : wc0) * (fc - ae), S);
+        em = ec;
+        ec = ep;
+        fc = fp;
+    }
 }
 
 // Jacobi-PCG direction: d' = D^-1 r + beta d (the pinned row: r_0 + beta d_0),
