@@ -945,38 +945,29 @@ __global__ void __launch_bounds__(kZT) k_residual_z2(Grid g, const double *__res
 
 
 
-// Level-0 damped-Jacobi sweep, e_out = e' + omega D^-1 (f - A e') with
-// e' = e_in (+ corr P(e_c), kProlong); kPublish also publishes
+// Level-0 damped-Jacobi sweep, e_out = e_in + omega D^-1 (f - A e_in)
+// (the prolongating one is k_mg_prolong0_z); kPublish also publishes
 // sum e_out.q_old and sum q_old for the FCG beta.
-template <bool kProlong, bool kPublish>
-__global__ void __launch_bounds__(kZT) k_mg_sweep0_z(Level L, Level LC, const float *__restrict__ f,
-                                                     const e16 *__restrict__ ein,
-                                                     const float *__restrict__ ec,
-                                                     e16 *__restrict__ eout,
+template <bool kPublish>
+__global__ void __launch_bounds__(kZT) k_mg_sweep0_z(Level L, const float *__restrict__ f,
+                                                     const e16 *__restrict__ ein, e16 *__restrict__ eout,
                                                      const float *__restrict__ qold,
                                                      double *__restrict__ pout, const FcgA *SA,
-                                                     const FcgB *SB, float corr) {
+                                                     const FcgB *SB) {
     if (SA->done || SB->done) return;
     const float S = SA->escale, iS = 1.0f / S;
     const float *inv = lvl_inv_table(L);
     const float f0 = (float)(-SA->sumr);
     const bool publish = kPublish && !SB->first;
     double s[2] = {0.0, 0.0}, m[1] = {0.0};
-    struct Raw {
-        float e, c;
-    };
     struct Side {
         float f, q;
     };
     // f (and q_old) of the cell are fetched with the plane's stencil loads
     zmarch_side(
         L.nx, L.ny, L.nz,
-        [&](uint32_t c, uint32_t i, uint32_t j, uint32_t k) -> Raw {
-            return Raw{e_ld(ein, c, iS), kProlong ? ec[(i >> 1) + LC.nx * ((j >> 1) + LC.ny * (k >> 1))] : 0.0f};
-        },
-        [&](const Raw &r, uint32_t, uint32_t, uint32_t, uint32_t) -> float {
-            return kProlong ? r.e + corr * r.c : r.e;
-        },
+        [&](uint32_t c, uint32_t, uint32_t, uint32_t) -> float { return e_ld(ein, c, iS); },
+        [&](float e, uint32_t, uint32_t, uint32_t, uint32_t) -> float { return e; },
         [&](uint32_t c) -> Side { return Side{f_at(f, c, true, f0), publish ? qold[c] : 0.0f}; },
         [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const float (&v)[7], const Side &sd) {
             const float ae = L.cx * ((v[0] - v[1]) + (v[0] - v[2])) + L.cy * ((v[0] - v[3]) + (v[0] - v[4])) +
@@ -990,6 +981,65 @@ __global__ void __launch_bounds__(kZT) k_mg_sweep0_z(Level L, Level LC, const fl
             }
         });
     if (kPublish) publish_row<2, 1>(s, m, pout);
+}
+
+// The level-0 prolongate-correct sweep, e_out = e' + omega D^-1 (f - A e')
+// with e' = e_in + corr P(e_c), as an explicit march.  P is constant on the
+// 2x2x2 children of a coarse cell, so a plane step needs the coarse values of
+// the column's own coarse cell and of the one x and one y neighbour outside
+// it; they sit in registers and are refetched every second plane, instead of
+// one coarse load per stencil operand.
+__global__ void __launch_bounds__(kZT) k_mg_prolong0_z(Level L, Level LC, const float *__restrict__ f,
+                                                       const e16 *__restrict__ ein,
+                                                       const float *__restrict__ ecoarse,
+                                                       e16 *__restrict__ eout, const FcgA *SA,
+                                                       const FcgB *SB, float corr) {
+    if (SA->done || SB->done) return;
+    const float S = SA->escale, iS = 1.0f / S;
+    const float *inv = lvl_inv_table(L);
+    const float f0 = (float)(-SA->sumr);
+    const uint3 b = zblock();
+    const uint32_t nx = L.nx, ny = L.ny, nz = L.nz, nxy = L.nxy, ncxy = LC.nxy;
+    const uint32_t i = b.x * kZX + threadIdx.x % kZX, j = b.y * kZY + threadIdx.x / kZX;
+    if (i >= nx || j >= ny) return;
+    const uint32_t kz = (nz + gridDim.z - 1) / gridDim.z, k0 = b.z * kz;
+    if (k0 >= nz) return;
+    const uint32_t k1 = min(k0 + kz, nz);
+    const uint32_t ie = i + 1 < nx ? i + 1 : i, iw = i > 0 ? i - 1 : i;
+    const uint32_t jn = j + 1 < ny ? j + 1 : j, js = j > 0 ? j - 1 : j;
+    const uint32_t de = ie - i, dw = i - iw, dn = (jn - j) * nx, ds = (j - js) * nx;
+    // coarse columns: own, and of the x / y neighbour outside the own cell
+    // (the own column again where that neighbour is clamped at a wall)
+    const bool xodd = (i & 1) != 0, yodd = (j & 1) != 0;
+    const uint32_t cc = (i >> 1) + LC.nx * (j >> 1);
+    const uint32_t cx = ((xodd ? ie : iw) >> 1) + LC.nx * (j >> 1);
+    const uint32_t cy = (i >> 1) + LC.nx * ((yodd ? jn : js) >> 1);
+    uint32_t c = i + nx * (j + ny * k0);
+    const uint32_t km = k0 > 0 ? k0 - 1 : k0;
+    float Cc = ecoarse[cc + ncxy * (k0 >> 1)];
+    float Cx = ecoarse[cx + ncxy * (k0 >> 1)], Cy = ecoarse[cy + ncxy * (k0 >> 1)];
+    float em = e_ld(ein, c - (k0 - km) * nxy, iS) + corr * ecoarse[cc + ncxy * (km >> 1)];
+    float ev = e_ld(ein, c, iS) + corr * Cc;
+    for (uint32_t k = k0; k < k1; ++k, c += nxy) {
+        const uint32_t kp = k + 1 < nz ? k + 1 : k, cp = c + (kp - k) * nxy;
+        if ((k & 1) == 0 && k != k0) {  // a new coarse plane (Cc came with the last +z operand)
+            Cx = ecoarse[cx + ncxy * (k >> 1)];
+            Cy = ecoarse[cy + ncxy * (k >> 1)];
+        }
+        const float Cn = (kp >> 1) == (k >> 1) ? Cc : ecoarse[cc + ncxy * (kp >> 1)];
+        const float rp = e_ld(ein, cp, iS), re = e_ld(ein, c + de, iS), rw = e_ld(ein, c - dw, iS),
+                    rn = e_ld(ein, c + dn, iS), rs = e_ld(ein, c - ds, iS);
+        const float fc = f_at(f, c, true, f0);
+        const float ep = rp + corr * Cn;
+        const float v1 = re + corr * (xodd ? Cx : Cc), v2 = rw + corr * (xodd ? Cc : Cx);
+        const float v3 = rn + corr * (yodd ? Cy : Cc), v4 = rs + corr * (yodd ? Cc : Cy);
+        const float ae = L.cx * ((ev - v1) + (ev - v2)) + L.cy * ((ev - v3) + (ev - v4)) +
+                         L.cz * ((ev - ep) + (ev - em));
+        e_st(eout, c, ev + kOmega * inv[lvl_didx(L, i, j, k)] * (fc - ae), S);
+        em = ev;
+        ev = ep;
+        Cc = Cn;
+    }
 }
 
 // Damped-Jacobi sweep on a coarse level (FP32 iterates) as a column march,
@@ -1705,8 +1755,8 @@ const char *const kKindNames[K_COUNT] = {
     "k_residual_z<double>", "k_residual_z<float>", "k_fold_outer", "k_add_correction",
     "k_cg_dirapply_z<double>", "k_cg_update<double, double>",
     "k_cg_dirapply_z<float>", "k_cg_update<float, float>",
-    "k_mg_check_z", "k_mg_sweep0_z<false, false>",
-    "k_mg_sweep0_z<true, false>", "k_mg_sweep0_z<false, true>",
+    "k_mg_check_z", "k_mg_sweep0_z<false>",
+    "k_mg_prolong0_z", "k_mg_sweep0_z<true>",
     "k_mg_restrict_z<e16>", "k_mg_down", "k_mg_tail", "k_mg_up", "k_fcg_dirapply_z", "k_fcg_update",
     "unet_warm_start",  // restriction + U-Net inference + interpolation (many kernels)
 };
@@ -2091,7 +2141,7 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     e16 *pre0 = s->m0b, *post0 = s->m0a;  // level 0's pre-smoothed iterate, and the other buffer
     if (kL0Smooth == 3) {
         FL_TIMED(s, K_MG_SWEEP_L0, k, 8.0 * n0,
-                 k_mg_sweep0_z<false, false><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->m0b, nullptr, s->m0a, nullptr, nullptr, s->fa, s->fb, kCorr));
+                 k_mg_sweep0_z<false><<<zg, kZT, 0, stm>>>(L[0], s->r32, s->m0b, s->m0a, nullptr, nullptr, s->fa, s->fb));
         pre0 = s->m0a;
         post0 = s->m0b;
     }
@@ -2135,12 +2185,12 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     // mea) or into meb (3, then one more sweep into mea); the publishing
     // sweep leaves z~ in meb
     FL_TIMED(s, K_MG_SWEEP_L0_PROLONG, k, 8.0 * n0 + 4.0 * n1,
-             k_mg_sweep0_z<true, false><<<zg, kZT, 0, stm>>>(L[0], L[1], s->r32, pre0, s->meb[1], post0, nullptr, nullptr, s->fa, s->fb, kCorr));
+             k_mg_prolong0_z<<<zg, kZT, 0, stm>>>(L[0], L[1], s->r32, pre0, s->meb[1], post0, s->fa, s->fb, kCorr));
     if (kL0Smooth == 3)
         FL_TIMED(s, K_MG_SWEEP_L0, k, 8.0 * n0,
-                 k_mg_sweep0_z<false, false><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->m0b, nullptr, s->m0a, nullptr, nullptr, s->fa, s->fb, kCorr));
+                 k_mg_sweep0_z<false><<<zg, kZT, 0, stm>>>(L[0], s->r32, s->m0b, s->m0a, nullptr, nullptr, s->fa, s->fb));
     FL_TIMED(s, K_MG_SWEEP_L0_PUBLISH, k, 12.0 * n0,
-             k_mg_sweep0_z<false, true><<<zg, kZT, 0, stm>>>(L[0], L[0], s->r32, s->m0a, nullptr, s->m0b, s->q32, s->pz, s->fa, s->fb, kCorr));
+             k_mg_sweep0_z<true><<<zg, kZT, 0, stm>>>(L[0], s->r32, s->m0a, s->m0b, s->q32, s->pz, s->fa, s->fb));
     // FCG direction (reads z~, p_old, r; writes p, q) and update (x, r
     // read and written, p and q read)
     FL_TIMED(s, K_FCG_DIRAPPLY, k, 18.0 * n0,
