@@ -458,8 +458,22 @@ __device__ inline void tail_vcycle(const TailLevels &T, float *pool, float *red,
         if (threadIdx.x < 8) I[l][threadIdx.x] = T.lv[l].inv[threadIdx.x];
     }
     for (uint32_t c = threadIdx.x; c < T.lv[0].n; c += blockDim.x) F[0][c] = fin[c];
-    __syncthreads();
     constexpr int last = C - 1;
+    // The exact coarsest solve splits each cell's column of the
+    // pseudo-inverse over kPL lanes; the column is fetched into registers
+    // here, so its global loads overlap the down pass instead of stalling
+    // the solve.
+    constexpr uint32_t kPL = 8, kPM = kDirectCells / kPL;
+    static_assert(kPL * kDirectCells <= 1024 && kDirectCells % kPL == 0, "one block of 1024 covers the solve");
+    const Level &LL = T.lv[last];
+    const uint32_t pc = threadIdx.x / kPL, pl = threadIdx.x % kPL;
+    float prow[kPM];
+#pragma unroll
+    for (uint32_t m = 0; m < kPM; ++m) {
+        const uint32_t j = pl + kPL * m;
+        prow[m] = (T.pinv && pc < LL.n && j < LL.n) ? T.pinv[(size_t)j * LL.n + pc] : 0.0f;
+    }
+    __syncthreads();
     // down: pre-smooth (result in E), restrict into F[l+1]
 #pragma unroll
     for (int l = 0; l < last; ++l) {
@@ -488,13 +502,17 @@ __device__ inline void tail_vcycle(const TailLevels &T, float *pool, float *red,
         __syncthreads();
     }
     // coarsest level, solution in E[last]
-    const Level &LL = T.lv[last];
     if (T.pinv) {
-        for (uint32_t c = threadIdx.x; c < LL.n; c += blockDim.x) {
-            float acc = 0.0f;
-            for (uint32_t j = 0; j < LL.n; ++j) acc += T.pinv[(size_t)j * LL.n + c] * F[last][j];
-            E[last][c] = acc;
+        float acc = 0.0f;
+#pragma unroll
+        for (uint32_t m = 0; m < kPM; ++m) {
+            const uint32_t j = pl + kPL * m;
+            if (pc < LL.n && j < LL.n) acc += prow[m] * F[last][j];
         }
+        acc += __shfl_xor(acc, 1);
+        acc += __shfl_xor(acc, 2);
+        acc += __shfl_xor(acc, 4);
+        if (pc < LL.n && pl == 0) E[last][pc] = acc;
         __syncthreads();
     } else {
         tail_sweep(LL, I[last], F[last], nullptr, E[last], nullptr, nullptr, true);
