@@ -249,7 +249,7 @@ def _mg(ctx, n, **kw):
                 preconditioner=fa.PRECOND_MULTIGRID, **kw)
 
 
-@pytest.mark.parametrize("shape", [16, 32, (32, 16, 24), 64])
+@pytest.mark.parametrize("shape", [16, 32, (32, 16, 24), 64, (96, 64, 80)])
 def test_multigrid_fcg_meets_fp64_tolerance(gpu_ctx, shape):
     print(f"[gpu] MG-FCG {shape}", flush=True)
     dims = (shape,) * 3 if isinstance(shape, int) else shape
