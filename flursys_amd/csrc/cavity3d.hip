@@ -2372,6 +2372,20 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
         FL_HIP_TRY(ctx, s->prof.flush());
         rr = s->host->outer[1];
         rmax = s->host->outer[2];
+        if (!mg) {
+            // The eliminated row 0 of the Neumann system has the residual
+            // -sum_{k!=0} r_k (the columns of A sum to zero and the closed
+            // cavity's right-hand side telescopes).  An FP32 Jacobi-CG leaves
+            // error along the pinned operator's lowest mode, whose residual is
+            // spread almost evenly over all cells: small in ||r||_2, yet
+            // sqrt(n) times larger in that sum, which is the divergence left
+            // at cell (0,0,0).  The stop test is therefore taken on the full
+            // Neumann residual (rows k != 0 and row 0).  The multigrid path
+            // hands row 0 to its Neumann cycle in compatible form instead.
+            const double r0 = s->host->outer[3];
+            rr += r0 * r0;
+            rmax = std::max(rmax, std::fabs(r0));
+        }
         if (first_pass) rr0 = rr;
         first_pass = false;
         if (!std::isfinite(rmax))
@@ -2635,8 +2649,15 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
         if (s->lv[s->nlev - 1].n <= kDirectCells) {
             const std::vector<float> P = coarse_pseudo_inverse(s->lv[s->nlev - 1]);
             if ((st = alloc(s, &s->pinv, P.size()))) goto fail;
-            FL_HIP_TRY(ctx, hipMemcpyAsync(s->pinv, P.data(), P.size() * sizeof(float),
-                                           hipMemcpyHostToDevice, ctx->stream));
+            const hipError_t e = hipMemcpyAsync(s->pinv, P.data(), P.size() * sizeof(float),
+                                                hipMemcpyHostToDevice, ctx->stream);
+            // the host vector P must stay alive until the copy has finished
+            const hipError_t e2 = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                st = set_error(ctx, FL_ERR_HIP, "pseudo-inverse upload failed: %s",
+                               hipGetErrorString(e != hipSuccess ? e : e2));
+                goto fail;
+            }
         }
     }
     if (hipHostMalloc(&s->host, sizeof(fl_cavity3d::Host), hipHostMallocDefault) != hipSuccess ||

@@ -58,7 +58,7 @@ enum { FL_PRECISION_FP64 = 0, FL_PRECISION_MIXED = 1 };
 /* Preconditioner of the inner solve.  JACOBI is the reference's.  MULTIGRID
  * (no reference counterpart) runs flexible CG preconditioned by one
  * cell-centred geometric-multigrid V-cycle (3+3 damped-Jacobi sweeps per
- * level, piecewise-constant transfers with a 1.8 coarse-correction factor);
+ * level, piecewise-constant transfers with a 2.0 coarse-correction factor);
  * when an inner solve exceeds its budget or stagnates, the solve falls back
  * to plain FP64 Jacobi-PCG from the current iterate (counted in the report).
  * LEARNED (no reference counterpart) is MULTIGRID started from the warm

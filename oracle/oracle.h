@@ -179,6 +179,58 @@ int or_bicgstab(int64_t n, const int64_t *offsets, const int64_t *cols, const do
                 const double *rhs, double *x, double atol, double rtol, int64_t max_iter,
                 or_lin_report *rep);
 
+/* ============ 2D unstructured mesh + two-point Poisson assembly ============ */
+/* UnstructuredMesh (src/mesh.rs:317-423, 533-609), 2D polygons only. */
+typedef struct {
+    int64_t ncells, nfaces;
+    double *cx, *cy, *vol;        /* cell centroid and area */
+    int64_t *owner, *neigh;       /* neigh = -1 on boundary faces */
+    double *fcx, *fcy, *sx, *sy;  /* face centre and area vector (out of the owner) */
+    int64_t *fv0, *fv1;           /* oriented face vertices */
+} or_mesh2d;
+/* MeshError kinds (src/mesh.rs:270-310), the ones 2D polygons can raise */
+enum {
+    OR_MESH_EMPTY = 20,
+    OR_MESH_NON_FINITE_POINT = 21,
+    OR_MESH_INVALID_CELL = 22,
+    OR_MESH_INVALID_POINT_INDEX = 23,
+    OR_MESH_DEGENERATE_FACE = 24,
+    OR_MESH_NON_MANIFOLD = 25,
+    OR_MESH_ZERO_VOLUME = 26,
+    OR_MESH_INCONSISTENT_ORIENTATION = 27,
+};
+/* ScalarBoundaryCondition (the two the Poisson assembly uses) */
+enum { OR_SCALAR_ZERO_GRADIENT = 0, OR_SCALAR_FIXED_VALUE = 1 };
+/* PoissonError kinds (src/poisson.rs:78-88) and the NumericsError cases of
+ * assemble_poisson */
+enum {
+    OR_POISSON_MISSING_REFERENCE = 30,
+    OR_POISSON_INCOMPATIBLE_SOURCE = 31,
+    OR_POISSON_INVALID_REFERENCE_CELL = 32,
+    OR_POISSON_INVALID_REFERENCE_VALUE = 33,
+    OR_POISSON_NON_FINITE_SOURCE = 34,
+    OR_POISSON_DEGENERATE_DISTANCE = 35,
+    OR_POISSON_INVALID_GEOMETRY = 36,
+    OR_POISSON_DEGENERATE_FACE_DISTANCE = 37,
+    OR_POISSON_MISSING_BOUNDARY = 38,
+    OR_POISSON_NUMERICS = 39,
+    OR_POISSON_LINEAR = 40,
+};
+/* Cells are polygons: vertex lists cell_vtx[cell_off[c] .. cell_off[c+1]). */
+int or_mesh2d_from_cells(int64_t npts, const double *xy, int64_t ncells, const int64_t *cell_off,
+                         const int64_t *cell_vtx, or_mesh2d *m, char *message, int cap);
+void or_mesh2d_free(or_mesh2d *m);
+/* assemble_poisson with Diffusivity::Constant(gamma), optional cell source,
+ * per-face boundary conditions (bc_kind/bc_value indexed by face; interior
+ * entries ignored) and an optional reference.  Outputs are malloc'ed (free
+ * with or_free); `detail` receives the integrated source of an
+ * IncompatiblePureNeumannSource error. */
+int or_assemble_poisson(const or_mesh2d *m, double gamma, const double *source,
+                        const int32_t *bc_kind, const double *bc_value, int32_t has_reference,
+                        int64_t ref_cell, double ref_value, int64_t *nnz_out,
+                        int64_t **offsets_out, int64_t **cols_out, double **vals_out,
+                        double **rhs_out, double *detail, char *message, int cap);
+
 /* ============ unstructured triangle-mesh Poisson (config-3 producer) ============ */
 /* Builds the jittered structured triangulation of the unit square with m x m
  * squares (2*m*m triangles), assembles the two-point Poisson matrix with

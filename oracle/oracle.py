@@ -123,6 +123,14 @@ class LinReport(C.Structure):
     ]
 
 
+class Mesh2DStruct(C.Structure):
+    _fields_ = [("ncells", C.c_int64), ("nfaces", C.c_int64)] + [
+        (name, C.POINTER(C.c_double)) for name in ("cx", "cy", "vol")] + [
+        (name, C.POINTER(C.c_int64)) for name in ("owner", "neigh")] + [
+        (name, C.POINTER(C.c_double)) for name in ("fcx", "fcy", "sx", "sy")] + [
+        (name, C.POINTER(C.c_int64)) for name in ("fv0", "fv1")]
+
+
 def build() -> None:
     """Compile liboracle.so from the C sources (make, in-tree)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -183,6 +191,13 @@ def lib():
             C.c_int64, C.c_double, C.c_uint64, C.c_int64, C.POINTER(C.c_int64),
             C.POINTER(C.c_int64), P64, P64, PD, PD, C.POINTER(C.c_int64), P64, P64, PD,
             C.c_char_p, C.c_int]
+        L.or_mesh2d_from_cells.argtypes = [C.c_int64, _f64p, C.c_int64, _i64p, _i64p,
+                                           C.POINTER(Mesh2DStruct), C.c_char_p, C.c_int]
+        L.or_mesh2d_free.argtypes = [C.POINTER(Mesh2DStruct)]
+        L.or_assemble_poisson.argtypes = [
+            C.POINTER(Mesh2DStruct), C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+            C.c_int64, C.c_double, C.POINTER(C.c_int64), P64, P64, PD, PD,
+            C.POINTER(C.c_double), C.c_char_p, C.c_int]
         L.or_free.argtypes = [C.c_void_p]
         L.or_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
         L.or_splitmix64.restype = C.c_uint64
@@ -339,6 +354,18 @@ class Solver2D:
         if st != 0:
             raise RuntimeError(rep.message.decode())
         return rep
+
+    def predict(self, dt):
+        """predict_momentum (projection coupling): u, v -> us, vs."""
+        lib().or2d_predict(C.byref(self.s), dt)
+
+    def apply_boundaries(self):
+        """apply_velocity_boundaries on the current u, v."""
+        lib().or2d_apply_velocity_boundaries(C.byref(self.s), self.s.u, self.s.v)
+
+    def cell_fields(self):
+        """compute_cell_fields: ucell, vcell, speed, vort from u, v."""
+        lib().or2d_cell_fields(C.byref(self.s))
 
     def apply_operator(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64).ravel()
@@ -504,6 +531,103 @@ def mesh_poisson(m, jitter=0.3, seed=0, ref_cell=0):
         L.or_free(p)
     out["A"] = Csr(n, n, out["offsets"], out["cols"], out["vals"], validate=False)
     return out
+
+
+_MESH_KINDS = {20: "EmptyMesh", 21: "NonFinitePoint", 22: "InvalidCellDefinition",
+               23: "InvalidPointIndex", 24: "DegenerateFace", 25: "NonManifoldFace",
+               26: "ZeroVolumeCell", 27: "InconsistentFaceOrientation"}
+_POISSON_KINDS = {30: "MissingReferenceForPureNeumann", 31: "IncompatiblePureNeumannSource",
+                  32: "InvalidReferenceCell", 33: "InvalidReferenceValue",
+                  34: "NonFiniteSource", 35: "DegenerateOwnerNeighbourDistance",
+                  36: "InvalidNonOrthogonalGeometry", 37: "DegenerateOwnerFaceDistance",
+                  38: "MissingBoundaryCondition", 39: "Numerics", 40: "Linear"}
+
+
+class MeshError(Exception):
+    def __init__(self, kind, message):
+        self.kind = kind
+        super().__init__(f"{kind}: {message}")
+
+
+class PoissonError(Exception):
+    def __init__(self, kind, message, detail=0.0):
+        self.kind = kind
+        self.detail = detail
+        super().__init__(f"{kind}: {message}")
+
+
+ZERO_GRADIENT, FIXED_VALUE = 0, 1
+
+
+class Mesh2D:
+    """UnstructuredMesh::from_cells for 2D polygons (oracle/fvmesh.c)."""
+
+    def __init__(self, points, cells):
+        xy = np.ascontiguousarray(np.asarray(points, dtype=np.float64)[:, :2]).ravel()
+        off = np.zeros(len(cells) + 1, np.int64)
+        off[1:] = np.cumsum([len(c) for c in cells])
+        vtx = np.ascontiguousarray(np.concatenate([np.asarray(c, np.int64) for c in cells])
+                                   if cells else np.zeros(1, np.int64))
+        self.m = Mesh2DStruct()
+        msg = C.create_string_buffer(256)
+        st = lib().or_mesh2d_from_cells(len(xy) // 2, xy, len(cells), off, vtx,
+                                        C.byref(self.m), msg, 256)
+        if st != 0:
+            raise MeshError(_MESH_KINDS.get(st, str(st)), msg.value.decode())
+        nc, nf = self.m.ncells, self.m.nfaces
+        self.centers = np.stack([_arr(self.m.cx, nc), _arr(self.m.cy, nc)], 1).copy()
+        self.volumes = _arr(self.m.vol, nc).copy()
+        self.owner = _arr(self.m.owner, nf).copy()
+        self.neighbour = _arr(self.m.neigh, nf).copy()
+        self.face_centers = np.stack([_arr(self.m.fcx, nf), _arr(self.m.fcy, nf)], 1).copy()
+        self.area_vectors = np.stack([_arr(self.m.sx, nf), _arr(self.m.sy, nf)], 1).copy()
+
+    def __del__(self):
+        try:
+            lib().or_mesh2d_free(C.byref(self.m))
+        except Exception:
+            pass
+
+    @property
+    def boundary_faces(self):
+        return np.nonzero(self.neighbour < 0)[0]
+
+
+def assemble_poisson(mesh, gamma=1.0, source=None, boundary=None, reference=None):
+    """assemble_poisson (src/poisson.rs:146-263) + apply_reference (:303-337).
+
+    `boundary` maps a boundary face index to ("FixedValue", value) or
+    "ZeroGradient" (default for every face: ZeroGradient); `reference` is
+    (cell, value) or None.  Returns (Csr, rhs)."""
+    L = lib()
+    nf = mesh.m.nfaces
+    kind = np.zeros(nf, np.int32)
+    value = np.zeros(nf, np.float64)
+    for f, bc in (boundary or {}).items():
+        if bc == "ZeroGradient":
+            continue
+        kind[f], value[f] = FIXED_VALUE, bc[1]
+    src = None if source is None else np.ascontiguousarray(source, dtype=np.float64)
+    nnz = C.c_int64()
+    po, pc = C.POINTER(C.c_int64)(), C.POINTER(C.c_int64)()
+    pv, pr = C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+    detail = C.c_double()
+    msg = C.create_string_buffer(256)
+    has_ref = reference is not None
+    rc, rv = (reference if has_ref else (0, 0.0))
+    st = L.or_assemble_poisson(C.byref(mesh.m), gamma,
+                               None if src is None else src.ctypes.data, kind.ctypes.data,
+                               value.ctypes.data, int(has_ref), rc, rv, C.byref(nnz),
+                               C.byref(po), C.byref(pc), C.byref(pv), C.byref(pr),
+                               C.byref(detail), msg, 256)
+    if st != 0:
+        raise PoissonError(_POISSON_KINDS.get(st, str(st)), msg.value.decode(), detail.value)
+    n, z = mesh.m.ncells, nnz.value
+    off, cols = _arr(po, n + 1).copy(), _arr(pc, z).copy()
+    vals, rhs = _arr(pv, z).copy(), _arr(pr, n).copy()
+    for p in (po, pc, pv, pr):
+        L.or_free(p)
+    return Csr(n, n, off, cols, vals, validate=False), rhs
 
 
 class Rng:
