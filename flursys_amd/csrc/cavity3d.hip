@@ -126,6 +126,9 @@ struct CgParams {
     int stop_rule;
     double rr0_ref;  // > 0: the rel-L2 test is against this ||r0||^2
     double e_bound;  // multigrid FCG: bound of |A^-1 r| / max|r| (L_max^2 / pi^2)
+    // Jacobi CG on the Neumann operator (row 0 a regular row) instead of the
+    // pinned one; the right-hand side must be compatible (sum r = 0)
+    int neumann;
 };
 enum { ST_OK = 0, ST_SINGULAR = 1, ST_ALPHA = 2, ST_RESIDUAL = 3 };
 
@@ -136,7 +139,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x
                                                       const T *__restrict__ q,
                                                       const double *__restrict__ pin,
                                                       uint32_t rows, double *__restrict__ pout,
-                                                      const CgStateA *SA, CgStateB *SB) {
+                                                      const CgStateA *SA, CgStateB *SB, int neumann) {
     if (SA->done || SB->done) return;
     const T *inv_tab = inv_table<T>(g);
     double f[1], fm[1];
@@ -166,7 +169,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x
         const T rc = rc0 - ta * qc;
         ro = rc;
         T z = rc;  // the same z = D^-1 r the direction kernel forms
-        if (c != 0) {
+        if (c != 0 || neumann) {
             uint32_t i, j, k;
             cell_ijk(g, c, i, j, k);
             z = rc * inv_tab[diag_index(g, i, j, k)];
@@ -207,11 +210,31 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x
     publish_partials<3, 1>(s, m, pout);
 }
 
-// x += e (FP64 += FP32): the refinement step of the mixed path.
+// x += e (FP64 += FP32): the refinement step of the mixed path.  A
+// correction solved on the Neumann operator is defined up to a constant:
+// with `shift` it is taken as e - e_0, the one that keeps the pin x_0 = 0.
 __global__ void __launch_bounds__(kBlock) k_add_correction(uint32_t n, double *__restrict__ x,
-                                                           const float *__restrict__ e) {
+                                                           const float *__restrict__ e, int shift) {
+    const double e0 = shift ? (double)e[0] : 0.0;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
-        x[c] += (double)e[c];
+        x[c] += (double)e[c] - e0;
+}
+
+// The compatible Neumann form of the pinned residual for the inner Jacobi
+// CG: row 0 gets r_0 = -sum_{k!=0} r_k (outer[3], folded from the residual
+// kernel's partials), stored in FP32, and its contribution {r.z, r.r, r,
+// |r|} is written as one extra partial row after the residual's rows.
+__global__ void k_row0_compat(Grid g, const double *__restrict__ outer, float *__restrict__ r,
+                              double *__restrict__ row) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const float rf = (float)(-outer[3]);
+    r[0] = rf;
+    const float z = rf * g.invf[diag_index(g, 0, 0, 0)];
+    const double rd = (double)rf;
+    row[0] = rd * (double)z;
+    row[1] = rd * rd;
+    row[2] = rd;
+    row[3] = fabs(rd);
 }
 
 // ---------------------------------------------------------------------------
@@ -220,7 +243,7 @@ __global__ void __launch_bounds__(kBlock) k_add_correction(uint32_t n, double *_
 // The preconditioner is one V-cycle of cell-centred geometric multigrid for
 // the Neumann operator: 3 damped-Jacobi sweeps (omega = 6/7) before and after
 // the coarse correction, restriction = mean of the 8 children, prolongation =
-// piecewise constant scaled by 1.8, Galerkin-consistent coarse coefficients
+// piecewise constant scaled by 2.0, Galerkin-consistent coarse coefficients
 // (each axis coefficient halves per level).  Levels down to <= 4096 cells run
 // in one single-workgroup kernel out of LDS.  The pinned system is handed to
 // the Neumann cycle in its exact equivalent form: r_0 is replaced by
@@ -1205,9 +1228,10 @@ __global__ void __launch_bounds__(kZT) k_mg_check_z(Level L, CgParams prm, const
 
 // Jacobi-PCG direction: d' = D^-1 r + beta d (the pinned row: r_0 + beta d_0),
 // q = A d' (q_0 = d'_0), partial d'.q.  Folds the `rows` update partials.
+// With `neu` row 0 is a regular row of the Neumann operator.
 template <bool kFirst, typename T>
 __device__ inline void cg_direction(const Grid &g, const T *inv, const T *r, const T *dold, T *dnew,
-                                    T *q, T tb, double (&s)[1]) {
+                                    T *q, T tb, double (&s)[1], bool neu) {
     const T cx = (T)g.ix2, cy = (T)g.iy2, cz = (T)g.iz2;
     struct Raw {
         T r, d;
@@ -1216,11 +1240,11 @@ __device__ inline void cg_direction(const Grid &g, const T *inv, const T *r, con
         g.nx, g.ny, g.nz,
         [&](uint32_t c, uint32_t, uint32_t, uint32_t) -> Raw { return Raw{r[c], kFirst ? (T)0 : dold[c]}; },
         [&](const Raw &v, uint32_t c, uint32_t i, uint32_t j, uint32_t k) -> T {
-            const T z = c == 0 ? v.r : v.r * inv[diag_index(g, i, j, k)];
+            const T z = (c == 0 && !neu) ? v.r : v.r * inv[diag_index(g, i, j, k)];
             return kFirst ? z : z + tb * v.d;
         },
         [&](uint32_t, uint32_t, uint32_t, uint32_t c, const T (&v)[7]) {
-            const T qc = c == 0 ? v[0]
+            const T qc = (c == 0 && !neu) ? v[0]
                                 : cx * ((v[0] - v[1]) + (v[0] - v[2])) + cy * ((v[0] - v[3]) + (v[0] - v[4])) +
                                       cz * ((v[0] - v[5]) + (v[0] - v[6]));
             dnew[c] = v[0];
@@ -1334,8 +1358,8 @@ __global__ void __launch_bounds__(kZT) k_cg_dirapply_z(Grid g, CgParams prm, con
         if (first) cg_direction2<true, T>(g, inv, r, dold, dnew, q, (T)0, s);
         else cg_direction2<false, T>(g, inv, r, dold, dnew, q, (T)beta, s);
     } else {
-        if (first) cg_direction<true, T>(g, inv, r, dold, dnew, q, (T)0, s);
-        else cg_direction<false, T>(g, inv, r, dold, dnew, q, (T)beta, s);
+        if (first) cg_direction<true, T>(g, inv, r, dold, dnew, q, (T)0, s, prm.neumann != 0);
+        else cg_direction<false, T>(g, inv, r, dold, dnew, q, (T)beta, s, prm.neumann != 0);
     }
     publish_row<1, 1>(s, m, pout);
 }
@@ -2051,9 +2075,12 @@ const char *status_text(int status) {
 
 // Runs CG iterations (dirapply + update pairs) on vectors of type T until
 // the device state says done.  The "update" partials of iteration 0 must
-// already be in s->pb (from k_residual_z, ZGrid zgd's rows).
+// already be in s->pb (from k_residual_z: `rows0` rows, by default ZGrid
+// zr's count).
 template <typename T, typename TX>
-fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf[2], T *q) {
+fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf[2], T *q,
+                 uint32_t rows0 = 0) {
+    if (rows0 == 0) rows0 = s->zr.count;
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
     s->host->a = CgStateA{};
@@ -2079,11 +2106,11 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
             T *dold = dbuf[launched & 1], *dnew = dbuf[(launched + 1) & 1];
             FL_TIMED(s, kd, launched, bd,
                      k_cg_dirapply_z<T><<<zg, kZT, 0, stm>>>(s->g, prm, r, dold, dnew, q, s->pb,
-                                                              launched == 0 ? s->zr.count : G, s->pa,
+                                                              launched == 0 ? rows0 : G, s->pa,
                                                               s->sa, s->sb));
             FL_TIMED(s, ku, launched, bu,
                      k_cg_update<T, TX><<<G, kBlock, 0, stm>>>(s->g, x, r, dnew, q, s->pa, zd.count,
-                                                               s->pb, s->sa, s->sb));
+                                                               s->pb, s->sa, s->sb, prm.neumann));
             ++launched;
         }
         FL_HIP_TRY(ctx, hipGetLastError());
@@ -2434,8 +2461,13 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
             s->fcg_hint[slot] = (int)std::min<long long>(it, 16);
             budget_hit = !s->host->fa.converged && !stagnated && status == ST_OK;
         } else {
-            const CgParams prm{itol, s->cfg.pressure_max_iters - total, FL_STOP_REL_L2, 0.0, 0.0};
-            fl_status st = run_cg<float, float>(s, prm, s->e32, s->r32, s->d32, s->q32);
+            // the inner Jacobi-CG solves the compatible Neumann form of the
+            // pinned correction equation (row 0 with r_0 = -sum_{k!=0} r_k);
+            // its solution, shifted to e_0 = 0, is the pinned correction, and
+            // the Neumann operator lacks the pinned operator's near-null mode
+            k_row0_compat<<<1, 64, 0, stm>>>(s->g, s->outer, s->r32, s->pb + 4 * (size_t)s->zr.count);
+            const CgParams prm{itol, s->cfg.pressure_max_iters - total, FL_STOP_REL_L2, 0.0, 0.0, 1};
+            fl_status st = run_cg<float, float>(s, prm, s->e32, s->r32, s->d32, s->q32, s->zr.count + 1);
             if (st != FL_OK) return st;
             const CgStateB &b = s->host->b;
             status = b.status != ST_OK ? b.status : s->host->a.status;
@@ -2446,7 +2478,7 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
         sweeps += 1;
         if (status == ST_OK && it > 0)
             FL_TIMED(s, K_ADD_CORRECTION, -1, 20.0 * s->n,
-                     k_add_correction<<<s->ew_blocks, kBlock, 0, stm>>>((uint32_t)s->n, s->p, s->e32));
+                     k_add_correction<<<s->ew_blocks, kBlock, 0, stm>>>((uint32_t)s->n, s->p, s->e32, mg ? 0 : 1));
         if (mg && (status != ST_OK || stagnated || budget_hit)) {
             fallback = true;
             break;
@@ -2617,7 +2649,7 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
         (st = alloc(s, &s->vs, s->nv)) || (st = alloc(s, &s->ws, s->nw)) ||
         (st = alloc(s, &s->p, s->n)) || (st = alloc(s, &s->rhs, s->n)) ||
         (st = alloc(s, &s->partials, prow)) || (st = alloc(s, &s->pa, prow)) ||
-        (st = alloc(s, &s->pb, prow)) || (st = alloc(s, &s->outer, 4)) ||
+        (st = alloc(s, &s->pb, prow + 4)) || (st = alloc(s, &s->outer, 4)) ||
         (st = alloc(s, &s->sa, 1)) || (st = alloc(s, &s->sb, 1)) || (st = alloc(s, &s->dg, 1)))
         goto fail;
     if (mixed) {

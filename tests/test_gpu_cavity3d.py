@@ -238,6 +238,53 @@ def test_mixed_steps_match_oracle(gpu_ctx):
     assert rep.max_divergence < 1e-5
 
 
+def _c2_sample(f):
+    return np.concatenate([f[0::8, 0::8, 0::8].ravel(), f[5::8, 5::8, 5::8].ravel()])
+
+
+@pytest.mark.parametrize("prec", [fa.PRECISION_FP64, fa.PRECISION_MIXED])
+def test_cavity128_re400_20_steps_match_oracle(gpu_ctx, prec):
+    """The 3D cavity at 128^3, Re 400, dt 5e-4, rel-res 1e-8 from a zero
+    guess, 20 steps from rest, against the FP64 oracle's run of the same
+    configuration (tests/golden/cavity128_re400.npz: every field on two
+    stride-8 lattices, the full plane next to the lid, norms, per-step
+    iterations and divergence).  Stated tolerances: FP64 u, v, w within
+    1e-9 of the lid velocity and p within 1e-8 of max|p|; FP32-in-FP64
+    refinement u, v, w within 1e-7 and p within 1e-6; max divergence below
+    the reference's smoke bar 1e-5 (tests/smoke.rs:127) at every step."""
+    g = np.load(os.path.join(GOLD, "cavity128_re400.npz"))
+    n, steps = int(g["n"]), int(g["steps"])
+    print(f"[gpu] 128^3 Re 400 {steps} steps precision={prec}", flush=True)
+    s = make(gpu_ctx, n, n, n, reynolds=400.0, dt=5e-4, pressure_tolerance=1e-8,
+             stop_rule=fa.STOP_REL_L2, initial_guess=fa.GUESS_ZERO,
+             pressure_max_iters=20000, precision=prec)
+    divs = []
+    for k in range(steps):
+        rep = s.advance_one_step()
+        assert rep.pressure_converged
+        divs.append(rep.max_divergence)
+        if prec == fa.PRECISION_FP64:
+            assert abs(rep.pressure_iterations - int(g["iterations"][k])) <= 1
+    f = s.fields()
+    vel_tol, p_tol = (1e-9, 1e-8) if prec == fa.PRECISION_FP64 else (1e-7, 1e-6)
+    errs = {}
+    for name in ("u", "v", "w", "p"):
+        scale = 1.0 if name != "p" else float(g["p_max"])
+        errs[name] = float(np.abs(_c2_sample(f[name]) - g[f"{name}_sample"]).max()) / scale
+    errs["u_lid"] = float(np.abs(f["u"][:, n - 1, :] - g["u_lidplane"]).max())
+    errs["p_lid"] = float(np.abs(f["p"][:, n - 1, :] - g["p_lidplane"]).max()) / float(g["p_max"])
+    print(f"[gpu] 128^3 errors vs oracle: {errs}, max div {max(divs):.3e}", flush=True)
+    for name in ("u", "v", "w", "u_lid"):
+        assert errs[name] < vel_tol, name
+    for name in ("p", "p_lid"):
+        assert errs[name] < p_tol, name
+    for name in ("u", "v", "w", "p"):
+        assert abs(np.linalg.norm(f[name]) - float(g[f"{name}_l2"])) <= (
+            vel_tol if name != "p" else p_tol) * float(g[f"{name}_l2"]), name
+    assert max(divs) < 1e-5
+    assert abs(rep.max_speed - float(g["max_speed"][-1])) <= vel_tol
+
+
 # ------------------------------------------- multigrid-preconditioned FCG
 def _mg(ctx, n, **kw):
     kw.setdefault("pressure_tolerance", 1e-8)
