@@ -183,6 +183,7 @@ SIGNATURES = {
     "fl_cg": (C.c_int, _LIN),
     "fl_pcg_jacobi": (C.c_int, _LIN),
     "fl_bicgstab": (C.c_int, _LIN),
+    "fl_pcg_jacobi_noalloc": (C.c_int, _LIN),
     "fl_unet_create": (C.c_int, [_P, C.POINTER(UNetConfig), C.POINTER(_P)]),
     "fl_unet_destroy": (None, [_P]),
     "fl_unet_num_params": (C.c_int64, [_P]),

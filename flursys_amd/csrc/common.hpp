@@ -27,6 +27,7 @@ struct fl_ctx {
 namespace fl {
 
 void ctx_release(fl_ctx *ctx);  // drops one reference (context.hip)
+void noalloc_init(int device);   // per-device address of the allocation-free solve buffers (csr.hip)
 inline void ctx_retain(fl_ctx *ctx) { ctx->refs += 1; }
 
 constexpr int kBlock = 256;   // threads per workgroup: 4 wave64s

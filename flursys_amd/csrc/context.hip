@@ -72,6 +72,9 @@ fl_status fl_create(int device, fl_ctx **out) {
         delete ctx;
         return FL_ERR_HIP;
     }
+    // With the device current, look up its copy of the allocation-free
+    // solve buffers (fl_pcg_jacobi_noalloc retries if this lookup failed).
+    fl::noalloc_init(device);
     *out = ctx;
     return FL_OK;
 }

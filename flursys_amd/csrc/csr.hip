@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -459,7 +460,52 @@ __global__ void __launch_bounds__(kBlock) k_bi_update(uint32_t n, double *__rest
     }
 }
 
+// ------------------------------------------- allocation-free solve buffers
+// fl_pcg_jacobi_noalloc keeps every device buffer of a solve -- the five
+// work vectors x, b, r, d, q, the per-block partial rows and the solver
+// state -- in this file-scope array instead of library allocations.  Each
+// GPU the library runs on holds its own copy (42 MiB) until the process
+// ends.  Host code never passes `scratch` itself to a kernel or copy (in host
+// code the name is not a device address); it uses the per-device address
+// from hipGetSymbolAddress, looked up once per device.
+constexpr size_t kNoAllocMaxRows = (size_t)1 << 20;  // largest matrix dimension
+constexpr size_t kNoAllocStride = kNoAllocMaxRows + 32;  // one vector, 256-byte aligned
+constexpr uint32_t kNoAllocMaxBlocks = 4096;
+constexpr size_t kNoAllocPartials = (size_t)kNoAllocMaxBlocks * 4;
+constexpr size_t kNoAllocState = 64;  // doubles reserved for LinState
+constexpr size_t kScratchDoubles = 5 * kNoAllocStride + kNoAllocPartials + kNoAllocState;
+static_assert(sizeof(LinState) <= kNoAllocState * sizeof(double), "LinState outgrew its slot");
+static_assert(kNoAllocStride % 32 == 0, "vectors must stay 256-byte aligned");
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (the current device), looked
+// up on the first call for that device.
+hipError_t scratch_address(int device, double **out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+
 }  // namespace csr
+
+// Called by fl_create with the context's device current.
+void noalloc_init(int device) {
+    double *p = nullptr;
+    (void)csr::scratch_address(device, &p);
+}
+
 }  // namespace fl
 
 using namespace fl;
@@ -546,11 +592,19 @@ fl_status state_error(fl_csr *m, const LinState &h, fl_err_detail *d) {
 
 enum Kind { KIND_CG = 0, KIND_PCG = 1, KIND_BICG = 2 };
 
-fl_status solve(fl_csr *m, int kind, const double *rhs, double *x_inout, const fl_lin_opts *opts,
-                fl_lin_report *rep, fl_err_detail *d) {
+// Device buffers of one solve: vectors x, b, r, d/p, q/v (+ t, rhat for
+// BiCGSTAB), the per-block partial rows and the solver state.
+struct Work {
+    double *vec[7];
+    double *partials;
+    LinState *st;
+};
+
+// The checks every solve entry point makes before touching the device.
+fl_status check_solve(fl_csr *m, int kind, const double *rhs, const double *x_inout,
+                      const fl_lin_opts *opts, fl_err_detail *d) {
     fl_ctx *ctx = m->ctx;
     if (!rhs || !x_inout || !opts) return err(ctx, d, FL_ERR_INVALID_ARGUMENT, "null argument");
-    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
     fl_status st = validate_inputs(m, rhs, x_inout, opts, d);
     if (st != FL_OK) return st;
     if (kind == KIND_PCG && m->inv_status != FL_OK) {
@@ -558,11 +612,18 @@ fl_status solve(fl_csr *m, int kind, const double *rhs, double *x_inout, const f
         return set_error(ctx, (fl_status)m->inv_status, "Jacobi preconditioner: %s (row %lld)",
                          m->inv_ctx, (long long)m->inv_row);
     }
-    if ((st = ensure_vectors(m, kind == KIND_BICG ? 7 : 5)) != FL_OK) return st;
+    return FL_OK;
+}
+
+// The solve proper on caller-provided work buffers (check_solve passed and
+// the context's device is current).
+fl_status solve_on(fl_csr *m, int kind, const double *rhs, double *x_inout, const fl_lin_opts *opts,
+                   fl_lin_report *rep, fl_err_detail *d, const Work &w) {
+    fl_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
     const size_t bytes = (size_t)m->n * sizeof(double);
-    double *x = m->vec[0], *b = m->vec[1], *r = m->vec[2], *dp = m->vec[3], *qv = m->vec[4];
-    double *tv = m->vec[5], *rhat = m->vec[6];
+    double *x = w.vec[0], *b = w.vec[1], *r = w.vec[2], *dp = w.vec[3], *qv = w.vec[4];
+    double *tv = w.vec[5], *rhat = w.vec[6];
     FL_HIP_TRY(ctx, hipMemcpyAsync(x, x_inout, bytes, hipMemcpyHostToDevice, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(b, rhs, bytes, hipMemcpyHostToDevice, s));
     LinState init{};
@@ -571,35 +632,37 @@ fl_status solve(fl_csr *m, int kind, const double *rhs, double *x_inout, const f
     init.max_iter = opts->max_iterations;
     init.bad_row = ~0ull;
     init.bad_z = ~0ull;
-    *m->st_host = init;
-    FL_HIP_TRY(ctx, hipMemcpyAsync(m->st, m->st_host, sizeof(LinState), hipMemcpyHostToDevice, s));
+    *m->st_host = init;  // also clears the reduction counters of the state on the device
+    LinState *const dst = w.st;
+    double *const partials = w.partials;
+    FL_HIP_TRY(ctx, hipMemcpyAsync(dst, m->st_host, sizeof(LinState), hipMemcpyHostToDevice, s));
     const Mat A{m->off, m->col, m->val, m->n};
     const double *inv = kind == KIND_PCG ? m->inv : nullptr;
     const uint32_t G = m->blocks;
     FL_HIP_TRY(ctx, hipEventRecord(m->e0, s));
-    k_spmv<<<G, kBlock, 0, s>>>(A, x, qv, m->st);
-    k_cg_init_r<<<G, kBlock, 0, s>>>(m->n, b, qv, r, m->partials, m->st);
+    k_spmv<<<G, kBlock, 0, s>>>(A, x, qv, dst);
+    k_cg_init_r<<<G, kBlock, 0, s>>>(m->n, b, qv, r, partials, dst);
     if (kind == KIND_BICG)
-        k_bi_init<<<G, kBlock, 0, s>>>(m->n, r, rhat, dp, qv, m->partials, m->st);
+        k_bi_init<<<G, kBlock, 0, s>>>(m->n, r, rhat, dp, qv, partials, dst);
     else
-        k_cg_init_d<<<G, kBlock, 0, s>>>(m->n, r, inv, dp, m->partials, m->st);
+        k_cg_init_d<<<G, kBlock, 0, s>>>(m->n, r, inv, dp, partials, dst);
     FL_HIP_TRY(ctx, hipGetLastError());
     int batch = 4;
     for (;;) {
         for (int it = 0; it < batch; ++it) {
             if (kind == KIND_BICG) {
-                k_bi_p<<<G, kBlock, 0, s>>>(m->n, r, dp, qv, m->st);
-                k_bi_spmv_v<<<G, kBlock, 0, s>>>(A, dp, rhat, qv, m->partials, m->st);
-                k_bi_spmv_t<<<G, kBlock, 0, s>>>(A, r, qv, tv, m->partials, m->st);
-                k_bi_update<<<G, kBlock, 0, s>>>(m->n, x, r, dp, qv, tv, rhat, m->partials, m->st);
+                k_bi_p<<<G, kBlock, 0, s>>>(m->n, r, dp, qv, dst);
+                k_bi_spmv_v<<<G, kBlock, 0, s>>>(A, dp, rhat, qv, partials, dst);
+                k_bi_spmv_t<<<G, kBlock, 0, s>>>(A, r, qv, tv, partials, dst);
+                k_bi_update<<<G, kBlock, 0, s>>>(m->n, x, r, dp, qv, tv, rhat, partials, dst);
             } else {
-                k_cg_spmv<<<G, kBlock, 0, s>>>(A, dp, qv, m->partials, m->st);
-                k_cg_update<<<G, kBlock, 0, s>>>(m->n, x, r, dp, qv, inv, m->partials, m->st);
-                k_cg_dir<<<G, kBlock, 0, s>>>(m->n, r, inv, dp, m->st);
+                k_cg_spmv<<<G, kBlock, 0, s>>>(A, dp, qv, partials, dst);
+                k_cg_update<<<G, kBlock, 0, s>>>(m->n, x, r, dp, qv, inv, partials, dst);
+                k_cg_dir<<<G, kBlock, 0, s>>>(m->n, r, inv, dp, dst);
             }
         }
         FL_HIP_TRY(ctx, hipGetLastError());
-        FL_HIP_TRY(ctx, hipMemcpyAsync(m->st_host, m->st, sizeof(LinState), hipMemcpyDeviceToHost, s));
+        FL_HIP_TRY(ctx, hipMemcpyAsync(m->st_host, dst, sizeof(LinState), hipMemcpyDeviceToHost, s));
         FL_HIP_TRY(ctx, hipStreamSynchronize(s));
         if (m->st_host->done) break;
         batch = std::min(batch * 2, 64);
@@ -624,6 +687,22 @@ fl_status solve(fl_csr *m, int kind, const double *rhs, double *x_inout, const f
     }
     if (d) set_detail(d, FL_OK, "");
     return FL_OK;
+}
+
+// The library-allocated path: work vectors allocated on first use and kept
+// with the matrix.
+fl_status solve(fl_csr *m, int kind, const double *rhs, double *x_inout, const fl_lin_opts *opts,
+                fl_lin_report *rep, fl_err_detail *d) {
+    fl_ctx *ctx = m->ctx;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    fl_status st = check_solve(m, kind, rhs, x_inout, opts, d);
+    if (st != FL_OK) return st;
+    if ((st = ensure_vectors(m, kind == KIND_BICG ? 7 : 5)) != FL_OK) return st;
+    Work w{};
+    for (int k = 0; k < 7; ++k) w.vec[k] = m->vec[k];
+    w.partials = m->partials;
+    w.st = m->st;
+    return solve_on(m, kind, rhs, x_inout, opts, rep, d, w);
 }
 
 }  // namespace
@@ -793,6 +872,33 @@ fl_status fl_bicgstab(fl_csr *m, const double *rhs, double *x, const fl_lin_opts
                       fl_lin_report *rep, fl_err_detail *d) {
     if (!m) return FL_ERR_INVALID_ARGUMENT;
     return solve(m, KIND_BICG, rhs, x, o, rep, d);
+}
+
+fl_status fl_pcg_jacobi_noalloc(fl_csr *m, const double *rhs, double *x, const fl_lin_opts *o,
+                                fl_lin_report *rep, fl_err_detail *d) {
+    if (!m) return FL_ERR_INVALID_ARGUMENT;
+    fl_ctx *ctx = m->ctx;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if ((size_t)m->n > kNoAllocMaxRows || (size_t)m->ncols > kNoAllocMaxRows)
+        return err(ctx, d, FL_ERR_INVALID_ARGUMENT,
+                   "matrix exceeds the allocation-free solve's limit of 1048576 rows and columns",
+                   (int64_t)std::max(m->n, m->ncols));
+    if (m->blocks > kNoAllocMaxBlocks)
+        return err(ctx, d, FL_ERR_INVALID_ARGUMENT,
+                   "launch needs more partial rows than the allocation-free buffers hold",
+                   (int64_t)m->blocks);
+    fl_status st = check_solve(m, KIND_PCG, rhs, x, o, d);
+    if (st != FL_OK) return st;
+    double *base = nullptr;
+    const hipError_t e = scratch_address(ctx->device, &base);
+    if (e != hipSuccess)
+        return set_error(ctx, FL_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    Work w{};
+    for (int k = 0; k < 5; ++k) w.vec[k] = base + k * kNoAllocStride;
+    w.vec[5] = w.vec[6] = nullptr;  // BiCGSTAB only
+    w.partials = base + 5 * kNoAllocStride;
+    w.st = reinterpret_cast<LinState *>(w.partials + kNoAllocPartials);
+    return solve_on(m, KIND_PCG, rhs, x, o, rep, d, w);
 }
 
 }  // extern "C"

@@ -146,3 +146,10 @@ def pcg(matrix, rhs, solution, options=None) -> LinearSolveReport:
 def bicgstab(matrix, rhs, solution, options=None) -> LinearSolveReport:
     """bicgstab (solver.rs:155-256)."""
     return _solve("fl_bicgstab", matrix, rhs, solution, options)
+
+
+def pcg_noalloc(matrix, rhs, solution, options=None) -> LinearSolveReport:
+    """pcg on the library's static per-GPU buffers: no device allocation
+    (no reference counterpart; at most 1048576 rows; calls that overlap on
+    one GPU race)."""
+    return _solve("fl_pcg_jacobi_noalloc", matrix, rhs, solution, options)

@@ -384,6 +384,18 @@ fl_status fl_pcg_jacobi(fl_csr *matrix, const double *rhs, double *x_inout,
 /* bicgstab (src/linear/solver.rs:155-256). */
 fl_status fl_bicgstab(fl_csr *matrix, const double *rhs, double *x_inout,
                       const fl_lin_opts *opts, fl_lin_report *report, fl_err_detail *detail);
+/* No counterpart in the reference.  fl_pcg_jacobi for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops): the same kernels, results bitwise identical, but every device
+ * buffer of the solve (work vectors, partial sums, solver state) lives in a
+ * library-static array that each GPU holds (42 MiB) until the process ends;
+ * the call makes no hipMalloc/hipFree.  Matrices of more than 1048576 rows
+ * or columns are rejected (FL_ERR_INVALID_ARGUMENT).  All calls on one GPU
+ * share that array: calls that overlap on one GPU (from several threads or
+ * contexts) race and must be serialised by the caller. */
+fl_status fl_pcg_jacobi_noalloc(fl_csr *matrix, const double *rhs, double *x_inout,
+                                const fl_lin_opts *opts, fl_lin_report *report,
+                                fl_err_detail *detail);
 
 /* ---------------------------------------------------------------------- */
 /* Learned warm start (no counterpart in the reference; the pressure path's
