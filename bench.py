@@ -14,10 +14,16 @@ Variants timed back to back on the same workload:
                   residual, dots, alpha/beta and stop test)
   mg_fcg          the same refinement with multigrid-preconditioned flexible
                   CG as the inner solver (plain FP64 PCG fallback)
-  learned_fcg     mg_fcg started from the U-Net's warm start (weights from
-                  flursys_amd/train_unet.py; skipped when none are present)
-`value` is the steps/s of the fastest variant that meets the same FP64
-rel-res 1e-8 stop test; every variant's figures are in `config.variants`.
+  learned_fcg     the same flexible CG whose V-cycle takes the U-Net's
+                  prediction as its coarse solve on the 64^3 level (weights
+                  from flursys_amd/train_unet.py; skipped when none are
+                  present)
+`value` is the steps/s of learned_fcg, the metric's method; every variant
+meets the same FP64 rel-res 1e-8 stop test, and each one's steps/s and CG
+iterations/step are top-level fields beside it (`config.variants` holds the
+details).  Supporting configurations, one record each: C1 (2D 128^2), C2 (3D
+128^3, both precisions, with the per-field error against the FP64 oracle)
+and C3 (2M-cell CSR CG / PCG / BiCGSTAB).
 
 Output: one JSON line with steps/s, CG iterations/step, the HBM roofline of
 the dominant kernel (timed live with HIP events on the library's stream) and
@@ -102,7 +108,9 @@ def run_variant(fa, ctx, name, precision, precond, args):
         log(f"[bench] {name} warmup {k + 1}: {rep.pressure_iterations} iterations, "
             f"{rep.pressure_refinements} refinements, fallback {rep.pressure_fallback}")
     breakdown = s.kernel_stats() if args.warmup else {}
-    dom = max(breakdown, key=lambda k: breakdown[k]["total_ms"]) if breakdown else None
+    # the dominant single kernel (the U-Net entries time groups of kernels)
+    single = {k: v for k, v in breakdown.items() if not k.startswith("unet_")}
+    dom = max(single, key=lambda k: single[k]["total_ms"]) if single else None
     s.set_timing(True, dom)
     iters, refs, fbs, warm = [], [], [], []
     t0 = time.perf_counter()
@@ -124,7 +132,8 @@ def run_variant(fa, ctx, name, precision, precond, args):
     if net is not None:
         net.close()
     if dom is None and timed:
-        dom = max(timed, key=lambda k: timed[k]["total_ms"])
+        dom = max((k for k in timed if not k.startswith("unet_")), key=lambda k: timed[k]["total_ms"],
+                  default=None)
     step_ms = elapsed * 1e3 / args.steps
     return {
         "steps_per_s": args.steps / elapsed,
@@ -301,6 +310,180 @@ def batch2d_cpu_baseline(cfgs, stride=2):
                       f"{stride}nd), FP64 Jacobi-PCG, {threads} OpenMP threads"}
 
 
+def unet_flops(n):
+    """Algorithmic FLOPs of one U-Net inference at n^3: 2 * 27 * cin * cout
+    per output voxel and layer (the first layer's input and the head's
+    output have one channel that counts)."""
+    from flursys_amd.unet import layer_table
+
+    table = layer_table()
+    total = 0
+    for i, (c1, c2, cin, cout, kpad, level, relu, off) in enumerate(table):
+        vox = (n >> level) ** 3
+        total += 2 * 27 * (1 if i == 0 else cin) * (1 if i == len(table) - 1 else cout) * vox
+    return total
+
+
+def run_c2(fa, ctx, args):
+    """The 3D cavity at 128^3, Re 400, dt 5e-4, rel-res 1e-8 from a zero guess,
+    `args.c2_steps` steps from rest in both precisions, against the FP64 oracle's
+    run of the same configuration (tests/golden/cavity128_re400.npz, 20 steps)."""
+    import numpy as np
+
+    g = np.load(os.path.join(ROOT, "tests", "golden", "cavity128_re400.npz"))
+    n, steps = int(g["n"]), int(g["steps"])
+
+    def sample(f):
+        return np.concatenate([f[0::8, 0::8, 0::8].ravel(), f[5::8, 5::8, 5::8].ravel()])
+
+    out = {"workload": f"3D cavity {n}^3, Re 400, dt 5e-4, rel-res 1e-8, zero guess, "
+                       f"{steps} steps from rest",
+           "oracle_sample": "FP64 CPU oracle fields after the same steps on two stride-8 "
+                            "lattices and the full plane next to the lid"}
+    for name, prec in (("fp64_pcg", fa.PRECISION_FP64), ("mixed_ir", fa.PRECISION_MIXED)):
+        cfg = fa.LidDrivenCavity3DConfig(nx=n, ny=n, nz=n, reynolds=400.0, dt=5e-4,
+                                         pressure_tolerance=1e-8, stop_rule=fa.STOP_REL_L2,
+                                         initial_guess=fa.GUESS_ZERO, pressure_max_iters=50000,
+                                         precision=prec)
+        s = fa.LidDrivenCavity3DSolver(cfg, ctx)
+        its, divs, dl2 = [], [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            rep = s.advance_one_step()
+            its.append(rep.pressure_iterations)
+            divs.append(rep.max_divergence)
+            dl2.append(rep.divergence_l2)
+        el = time.perf_counter() - t0
+        f = s.fields()
+        s.close()
+        err = {}
+        for k in ("u", "v", "w", "p"):
+            scale = 1.0 if k != "p" else float(g["p_max"])
+            err[k] = float(np.abs(sample(f[k]) - g[f"{k}_sample"]).max()) / scale
+        err["u_lid_plane"] = float(np.abs(f["u"][:, n - 1, :] - g["u_lidplane"]).max())
+        err["p_lid_plane"] = float(np.abs(f["p"][:, n - 1, :] - g["p_lidplane"]).max()) / float(g["p_max"])
+        out[name] = {"steps_per_s": steps / el, "cg_iters_per_step": sum(its) / len(its),
+                     "max_error_vs_oracle": err,
+                     "error_scales": "u, v, w: lid velocity 1; p: max|p| of the oracle",
+                     "max_divergence": max(divs), "divergence_l2_last": dl2[-1],
+                     "oracle_cg_iters_per_step": float(np.mean(g["iterations"]))}
+        log(f"[bench] C2 {name}: {out[name]['steps_per_s']:.1f} steps/s, "
+            f"{out[name]['cg_iters_per_step']:.0f} its/step, errors {err}")
+    return out
+
+
+def run_c1(fa, ctx, args):
+    """2D cavity 128^2, Re 100, first-order upwind, Jacobi-PCG to rel-res 1e-8
+    from a zero guess.  dt = 0.25 h as written breaks the explicit diffusion
+    limit (viscous number 0.64 > 0.5) and diverges; the reference's adaptive
+    time step rule (target CFL 0.5, diffusion number 0.25) is run instead."""
+    import numpy as np
+
+    import flursys_amd.incompressible as I
+    import oracle as O
+
+    n, re = 128, 100.0
+    ts = I.TimeStepSettings(adaptive=True, target_cfl=0.5, target_diffusion_number=0.25)
+    cfg = I.SimulationConfig.cavity(n, reynolds=re, dt=0.25 / n, time_step=ts,
+                                    stop_rule=fa.STOP_REL_L2, initial_guess=fa.GUESS_ZERO,
+                                    pressure_tolerance=1e-8, pressure_max_iters=5000)
+    steps = args.c1_steps
+    out = {"workload": f"2D cavity {n}^2, Re 100, upwind, rel-res 1e-8 from zero, adaptive dt "
+                       "(the written dt = 0.25 h is unstable: viscous number 0.64)",
+           "steps": steps}
+    b = I.CavityBatch([cfg], ctx=ctx)
+    b.run(3, use_graphs=True)
+    t0 = time.perf_counter()
+    reps, ms = b.run(steps, use_graphs=True, step_times=True)
+    el = time.perf_counter() - t0
+    its = reps[0].pressure_iterations / steps
+    out["gpu"] = {"steps_per_s": steps / el, "cg_iters_per_step": its,
+                  "step_ms_p50": float(np.percentile(ms, 50)), "dt_last": None}
+    # the persistent PCG's own launch time (eager steps, HIP events)
+    reps, ms, pms = b.run(50, use_graphs=False, step_times=True, pcg_times=True)
+    pcg_s = float(np.sum(pms[pms > 0])) / 1e3
+    cell_its = reps[0].pressure_iterations * n * n
+    out["gpu"]["k_pcg_GBps"] = cell_its * 90.0 / pcg_s / 1e9
+    out["gpu"]["k_pcg_bytes_per_cell_iteration"] = 90
+    out["gpu"]["k_pcg_ms_per_step"] = pcg_s * 1e3 / 50
+    b.close()
+    log(f"[bench] C1 GPU: {out['gpu']['steps_per_s']:.1f} steps/s, {its:.0f} its/step")
+    if not args.no_cpu:
+        cores = min(16, os.cpu_count() or 1)
+        for threads in (1, cores):
+            s = O.Solver2D(n, nu=1.0 / re, dt=0.25 / n, adaptive=True, target_cfl=0.5,
+                           target_diffusion=0.25, stop_rule=O.STOP_REL_L2,
+                           initial_guess=O.GUESS_ZERO, pressure_tolerance=1e-8,
+                           pressure_max_iters=5000, threads=threads)
+            k, cit = 0, 0
+            t0 = time.perf_counter()
+            while time.perf_counter() - t0 < args.c1_cpu_seconds:
+                d, rep = s.step()
+                cit += rep.iterations
+                k += 1
+            el = time.perf_counter() - t0
+            out[f"cpu_oracle_{threads}_threads"] = {
+                "steps_per_s": k / el, "cg_iters_per_step": cit / k, "steps_timed": k,
+                "kind": "port (C oracle, OpenMP FP64 restatement of the reference path)"}
+            log(f"[bench] C1 CPU oracle {threads} threads: {k / el:.2f} steps/s")
+    return out
+
+
+def run_c3(fa, ctx, args):
+    """Unstructured pressure system: the pinned two-point Poisson matrix of a
+    1000 x 1000 jittered triangulation (2,000,000 cells), CG and Jacobi-PCG
+    to rel-res 1e-10, BiCGSTAB on the matrix plus a +-0.1 antisymmetric
+    perturbation.  The written jitter +-0.3 h makes the reference's mesh
+    builder reject the mesh (inconsistent face orientation); +-0.25 h is
+    used."""
+    import numpy as np
+
+    import flursys_amd.linear as FL
+    from flursys_amd import meshgen as M
+    import oracle as O
+
+    m = args.c3_m
+    t0 = time.perf_counter()
+    A, area, faces = M.poisson_system(m, 0.25, 0, 0)
+    rhs = M.zero_mean_source(area, 0, 0)
+    B = M.skew_perturbation(A, faces, 0.1, 0, 0)
+    n, nnz = A.shape[0], A.nnz
+    log(f"[bench] C3 system {n} cells, nnz {nnz} built in {time.perf_counter() - t0:.1f} s")
+    spmv = nnz * 12 + (n + 1) * 4
+    bytes_iter = {"cg": spmv + 11 * n * 8, "pcg": spmv + 13 * n * 8,
+                  "bicgstab": 2 * (B.nnz * 12 + (n + 1) * 4) + 18 * n * 8}
+    out = {"workload": f"{m}x{m} jittered (+-0.25 h) triangulation, {n} cells, nnz {nnz}, "
+                       "rel-res 1e-10 from zero",
+           "bytes_per_iteration": "SpMV nnz*12 + (n+1)*4, plus 8 B per vector pass "
+                                  "(CG 11, Jacobi-PCG 13, BiCGSTAB 18 passes and 2 SpMVs)"}
+    opts = FL.LinearSolverOptions(0.0, 1e-10, 50000)
+    for kind, M_ in (("cg", A), ("pcg", A), ("bicgstab", B)):
+        dev = FL.CsrMatrix(n, n, M_.indptr, M_.indices, M_.data, ctx)
+        x = np.zeros(n)
+        getattr(FL, kind)(dev, rhs, x, FL.LinearSolverOptions(0.0, 1e-10, 5))  # warm-up
+        x = np.zeros(n)
+        rep = getattr(FL, kind)(dev, rhs, x, opts)
+        dev.close()
+        true = float(np.linalg.norm(rhs - M_ @ x) / np.linalg.norm(rhs))
+        s = rep.device_ms / 1e3
+        out[kind] = {"s_per_solve": s, "iterations": rep.iterations, "converged": rep.converged(),
+                     "true_rel_residual": true,
+                     "GBps": rep.iterations * bytes_iter[kind] / s / 1e9 if s > 0 else None}
+        out[kind]["frac_of_8TBps"] = (out[kind]["GBps"] or 0.0) / HBM_PEAK_GBS
+        log(f"[bench] C3 {kind}: {rep.iterations} its, {s:.3f} s, {out[kind]['GBps']:.0f} GB/s")
+    if not args.no_cpu:
+        Ao = O.Csr(n, n, A.indptr.astype(np.int64), A.indices.astype(np.int64), A.data, validate=False)
+        k = 30
+        t0 = time.perf_counter()
+        O.cg(Ao, rhs, np.zeros(n), 0.0, 1e-30, k)
+        per = (time.perf_counter() - t0) / k
+        out["cpu_oracle_cg"] = {"s_per_iteration": per,
+                                "s_per_solve_extrapolated": per * out["cg"]["iterations"],
+                                "cores": 1, "kind": "port (C oracle, serial like src/linear/)",
+                                "sample": f"{k} CG iterations on the same matrix"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -331,6 +514,11 @@ def main():
                          "cavities advanced together (a separate JSON line)")
     ap.add_argument("--batch-steps", type=int, default=1000,
                     help="steps of every cavity in the batch2d configuration")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the supporting configurations (C1 2D 128^2, C2 3D 128^3, C3 CSR)")
+    ap.add_argument("--c1-steps", type=int, default=300)
+    ap.add_argument("--c1-cpu-seconds", type=float, default=4.0)
+    ap.add_argument("--c3-m", type=int, default=1000, help="C3 triangulation: m x m squares")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -358,7 +546,10 @@ def main():
                 "(python -m flursys_amd.train_unet)")
             continue
         results[name] = run_variant(fa, ctx, name, *modes[name], args)
-    best = max(results, key=lambda k: results[k]["steps_per_s"])
+    # the headline is the learned flexible CG (the metric's method); the
+    # other variants are reported beside it
+    best = "learned_fcg" if "learned_fcg" in results else max(
+        results, key=lambda k: results[k]["steps_per_s"])
     b = results[best]
     dom = b["dominant_kernel"]
     dk = b["dominant_timed"] or {}
@@ -387,7 +578,8 @@ def main():
         "config": {
             "workload": f"3D lid-driven cavity {n}^3, Re {args.reynolds:g}, dt {args.dt:g}, "
                         f"pressure to rel-res {args.tol:g} (FP64 test) from a zero guess, "
-                        "p(0,0,0) pinned",
+                        "p(0,0,0) pinned; value = learned flexible CG (U-Net coarse solve in "
+                        "the multigrid V-cycle, FP32 inner solve in FP64 refinement)",
             "grid": [n, n, n],
             "variant": best,
             "cg_iters_per_step": b["cg_iters_per_step"],
@@ -418,6 +610,31 @@ def main():
         if tr.get("workload") == out["config"]["workload"] and dom in tr.get("kernels", {}):
             out["roofline"]["traffic"] = tr["kernels"][dom]["traffic_bytes"]
             out["roofline"]["traffic_source"] = tr["source"]
+    # every variant's steps/s and CG iterations/step as top-level scalars
+    for name, r in results.items():
+        out[f"{name}_steps_per_s"] = r["steps_per_s"]
+        out[f"{name}_cg_iters_per_step"] = r["cg_iters_per_step"]
+    if "plain_fp64_pcg" in results:
+        out["vs_plain_fp64_pcg"] = b["steps_per_s"] / results["plain_fp64_pcg"]["steps_per_s"]
+        if "mg_fcg" in results:
+            out["mg_fcg_vs_plain_fp64_pcg"] = (results["mg_fcg"]["steps_per_s"] /
+                                               results["plain_fp64_pcg"]["steps_per_s"])
+    lr = results.get("learned_fcg")
+    if lr and "unet_coarse_solve" in lr["warmup_breakdown"]:
+        u = lr["warmup_breakdown"]["unet_coarse_solve"]
+        fl = unet_flops(args.unet_n)
+        out["unet"] = {"resolution": args.unet_n, "flops_per_inference": fl,
+                       "ms_per_coarse_solve": u["ms"], "launches_per_step": u["launches"],
+                       "mfma_TFLOPs": fl / (u["ms"] * 1e-3) / 1e12,
+                       "mfma_peak_TFLOPs": 2500.0,
+                       "mfma_frac": fl / (u["ms"] * 1e-3) / 2.5e15,
+                       "note": "time of the coarse solve = inference (about 20 kernels) plus two "
+                               "damped-Jacobi sweeps on 64^3; FLOPs count the useful "
+                               "convolution work (2*27*cin*cout per voxel and layer)"}
+    if not args.no_extras:
+        out["c2"] = run_c2(fa, ctx, args)
+        out["c1"] = run_c1(fa, ctx, args)
+        out["c3"] = run_c3(fa, ctx, args)
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle) sample")
         plain = results.get("plain_fp64_pcg", b)

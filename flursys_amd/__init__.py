@@ -6,8 +6,8 @@ the reference's solver interfaces over that ABI.
 """
 
 from ._lib import (GUESS_PREVIOUS, GUESS_ZERO, PRECISION_FP64, PRECISION_MIXED,
-                   PRECOND_JACOBI, PRECOND_LEARNED, PRECOND_MULTIGRID, STOP_ABS_INF, STOP_REL_L2, Context,
-                   FlursysError, default_context, load)
+                   PRECOND_JACOBI, PRECOND_LEARNED, PRECOND_MULTIGRID, SOLVER_PCG, SOLVER_SOR,
+                   STOP_ABS_INF, STOP_REL_L2, Context, FlursysError, default_context, load)
 from .cavity3d import LidDrivenCavity3DConfig, LidDrivenCavity3DSolver, RunSummary3D
 
 __all__ = [
@@ -15,5 +15,5 @@ __all__ = [
     "LidDrivenCavity3DConfig", "LidDrivenCavity3DSolver", "RunSummary3D",
     "STOP_ABS_INF", "STOP_REL_L2", "GUESS_PREVIOUS", "GUESS_ZERO",
     "PRECISION_FP64", "PRECISION_MIXED", "PRECOND_JACOBI", "PRECOND_MULTIGRID",
-    "PRECOND_LEARNED",
+    "PRECOND_LEARNED", "SOLVER_PCG", "SOLVER_SOR",
 ]

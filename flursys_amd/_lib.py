@@ -25,6 +25,7 @@ STOP_ABS_INF, STOP_REL_L2 = 0, 1
 GUESS_PREVIOUS, GUESS_ZERO = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 PRECOND_JACOBI, PRECOND_MULTIGRID, PRECOND_LEARNED = 0, 1, 2
+SOLVER_PCG, SOLVER_SOR = 0, 1
 
 
 class FlursysError(RuntimeError):
@@ -56,6 +57,7 @@ class Cavity3DConfig(C.Structure):
         ("pressure_max_iters", C.c_int64), ("pressure_tolerance", C.c_double),
         ("stop_rule", C.c_int32), ("initial_guess", C.c_int32), ("precision", C.c_int32),
         ("preconditioner", C.c_int32), ("inner_tolerance", C.c_double),
+        ("pressure_solver", C.c_int32), ("reserved", C.c_int32), ("pressure_omega", C.c_double),
     ]
 
 
@@ -123,7 +125,9 @@ class Grid2DConfig(C.Structure):
         ("outlet_pressure", C.c_double * 4), ("min_dt", C.c_double), ("max_dt", C.c_double),
         ("target_cfl", C.c_double), ("target_diffusion_number", C.c_double),
         ("max_growth_factor", C.c_double), ("precision", C.c_int32), ("reserved", C.c_int32),
-        ("inner_tolerance", C.c_double),
+        ("inner_tolerance", C.c_double), ("pressure_solver", C.c_int32),
+        ("inlet_profile", C.c_int32), ("pressure_omega", C.c_double),
+        ("inlet_step_height", C.c_double),
     ]
 
 
@@ -197,6 +201,8 @@ SIGNATURES = {
     "fl_probe_mfma_bf16": (C.c_int, [_P, _P, _P, _P]),
     "fl_cavity3d_set_unet": (C.c_int, [_P, _P]),
     "fl_cavity3d_coarse_pair": (C.c_int, [_P, C.c_int32, _P, _P, C.POINTER(C.c_double)]),
+    "fl_cavity3d_capture_coarse": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "fl_cavity3d_captured_rhs": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
 }
 
 _lib = None

@@ -41,13 +41,16 @@ class LidDrivenCavity3DConfig:
     precision: int = PRECISION_FP64
     preconditioner: int = PRECOND_JACOBI
     inner_tolerance: float = 0.0
+    pressure_solver: int = 0  # SOLVER_PCG; SOLVER_SOR is the reference's solver
+    pressure_omega: float = 1.7
 
     def to_c(self) -> Cavity3DConfig:
         return Cavity3DConfig(self.nx, self.ny, self.nz, self.length, self.height, self.depth,
                               self.density, self.lid_velocity, self.reynolds, self.dt,
                               self.pressure_max_iters, self.pressure_tolerance, self.stop_rule,
                               self.initial_guess, self.precision,
-                              self.preconditioner, self.inner_tolerance)
+                              self.preconditioner, self.inner_tolerance,
+                              self.pressure_solver, 0, self.pressure_omega)
 
     def viscosity(self) -> float:
         """viscosity (incompressible_3d.rs:86-88)."""
@@ -142,6 +145,22 @@ class LidDrivenCavity3DSolver:
         check(load().fl_cavity3d_coarse_pair(self.handle, n, ptr(b), ptr(p), C.byref(diag)),
               self.ctx.handle)
         return b, p, diag.value
+
+    def capture_coarse(self, n: int, iteration: int = 0) -> None:
+        """Arm (n > 0) or stop (0) the capture of the right-hand side V-cycle
+        `iteration` of each pressure solve's first inner FCG solve receives on
+        the level of n^3 cells."""
+        self._capture_n = n
+        check(load().fl_cavity3d_capture_coarse(self.handle, n, iteration), self.ctx.handle)
+
+    def captured_rhs(self):
+        """(f, diag): the last captured coarse right-hand side (float32, n^3)
+        and the rediscretised Laplacian diagonal at that resolution."""
+        n = self._capture_n
+        f = np.empty(n ** 3, np.float32)
+        diag = C.c_double()
+        check(load().fl_cavity3d_captured_rhs(self.handle, ptr(f), C.byref(diag)), self.ctx.handle)
+        return f, diag.value
 
     def fields(self):
         s = self.shapes

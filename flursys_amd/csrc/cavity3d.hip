@@ -1226,6 +1226,65 @@ __global__ void __launch_bounds__(kZT) k_mg_check_z(Level L, CgParams prm, const
     }
 }
 
+// ---------------------------------------------------------------------------
+// SOR (PressureSolverKind::Sor, the reference's 3D solver,
+// incompressible_3d.rs:293-332) in red-black order.  Each half sweep updates
+// the cells of one colour (i + j + k) & 1 from the other colour's values:
+// candidate = (sum_nb c_nb p_nb - rhs) / (2 (ix2 + iy2 + iz2)) with a missing
+// neighbour reading the centre (pressure_neighbor :450-474), p += omega
+// (candidate - p); p(0,0,0) stays 0.  After every sweep the max residual
+// |Lap p - rhs| over the cells other than (0,0,0) (pressure_residual
+// :476-502, k_residual_z with the pinned row) decides the stop.
+struct SorState {
+    double res;
+    long long iters;
+    int done, converged, status, pad;
+};
+
+__global__ void __launch_bounds__(kBlock) k_sor3(Grid g, const double *__restrict__ rhs,
+                                                 double *__restrict__ p, double omega, uint32_t color,
+                                                 const SorState *st) {
+    if (st->done) return;
+    const uint32_t nx = g.nx, ny = g.ny, nxy = g.nxy;
+    const uint32_t nh = (nx + 1) / 2, total = nh * ny * g.nz;
+    const double diagonal = 2.0 * (g.ix2 + g.iy2 + g.iz2);
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const uint32_t row = t / nh, i2 = t - row * nh;
+        const uint32_t k = row / ny, j = row - k * ny;
+        const uint32_t i = 2 * i2 + ((color + j + k) & 1);
+        if (i >= nx) continue;
+        const uint32_t c = i + nx * (j + ny * k);
+        if (c == 0) {
+            p[0] = 0.0;
+            continue;
+        }
+        const double ctr = p[c];
+        const double e = i + 1 < nx ? p[c + 1] : ctr, w = i > 0 ? p[c - 1] : ctr;
+        const double n = j + 1 < ny ? p[c + nx] : ctr, s = j > 0 ? p[c - nx] : ctr;
+        const double f = k + 1 < g.nz ? p[c + nxy] : ctr, b = k > 0 ? p[c - nxy] : ctr;
+        const double sum = g.ix2 * (e + w) + g.iy2 * (n + s) + g.iz2 * (f + b);
+        const double candidate = (sum - rhs[c]) / diagonal;
+        p[c] = ctr + omega * (candidate - ctr);
+    }
+}
+
+// The sweep's verdict from the folded max residual (outer[2]).
+__global__ void k_sor_check(const double *__restrict__ outer, double tol, long long max_iter, SorState *st) {
+    if (blockIdx.x != 0 || threadIdx.x != 0 || st->done) return;
+    const double res = outer[2];
+    st->res = res;
+    st->iters += 1;
+    if (!isfinite(res)) {
+        st->status = ST_RESIDUAL;
+        st->done = 1;
+    } else if (res < tol) {
+        st->converged = 1;
+        st->done = 1;
+    } else if (st->iters >= max_iter) {
+        st->done = 1;
+    }
+}
+
 // Jacobi-PCG direction: d' = D^-1 r + beta d (the pinned row: r_0 + beta d_0),
 // q = A d' (q_0 = d'_0), partial d'.q.  Folds the `rows` update partials.
 // With `neu` row 0 is a regular row of the Neumann operator.
@@ -1795,6 +1854,7 @@ enum Kind {
     K_DIRAPPLY_D, K_UPDATE_D, K_DIRAPPLY_F, K_UPDATE_F,
     K_MG_CHECK, K_MG_SWEEP_L0, K_MG_SWEEP_L0_PROLONG, K_MG_SWEEP_L0_PUBLISH,
     K_MG_RESTRICT_L0, K_MG_DOWN, K_MG_TAIL, K_MG_UP, K_FCG_DIRAPPLY, K_FCG_UPDATE, K_WARM_START,
+    K_UNET_COARSE,
     K_COUNT
 };
 const char *const kKindNames[K_COUNT] = {
@@ -1806,6 +1866,7 @@ const char *const kKindNames[K_COUNT] = {
     "k_mg_prolong0_z", "k_mg_sweep0_z<true>",
     "k_mg_restrict_z<e16>", "k_mg_down", "k_mg_tail", "k_mg_up", "k_fcg_dirapply_z", "k_fcg_update",
     "unet_warm_start",  // restriction + U-Net inference + interpolation (many kernels)
+    "unet_coarse_solve",  // U-Net inference + two sweeps on its level (many kernels)
 };
 
 }  // namespace c3
@@ -1850,6 +1911,11 @@ struct fl_cavity3d {
     float *wl_in = nullptr, *wl_out = nullptr;
     float *pair = nullptr;  // fl_cavity3d_coarse_pair staging (2 n^3)
     size_t pair_cells = 0;
+    // training capture: the right-hand side of V-cycle cap_iter of the first
+    // inner FCG solve of each pressure solve, on level cap_level (0: off)
+    int cap_level = 0, cap_iter = 0;
+    bool cap_taken = false;
+    float *cap = nullptr;
     double *pz = nullptr;  // partials z.q_old, sum q_old
     FcgA *fa = nullptr;
     FcgB *fb = nullptr;
@@ -1862,6 +1928,7 @@ struct fl_cavity3d {
         double outer[4];
     } *host = nullptr;
     double *outer = nullptr;  // folded outer residual: r.z, r.r, max|r|
+    SorState *sor = nullptr;  // SOR iteration state (FL_SOLVER_SOR)
     uint32_t blocks = 1, ew_blocks = 1;
     ZGrid zg{1, 1, 1, 1};   // column-marching launch of the level-0 sweeps
     ZGrid zgd{1, 1, 1, 1};  // and of the search directions
@@ -1902,7 +1969,8 @@ void free_all(fl_cavity3d *s) {
                     s->sb,  s->dg,     s->outer};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    void *more[] = {s->pz, s->fa, s->fb, s->wl_in, s->wl_out, s->pair, s->pinv, s->m0a, s->m0b};
+    void *more[] = {s->pz, s->fa, s->fb, s->wl_in, s->wl_out, s->pair, s->pinv, s->m0a, s->m0b, s->cap,
+                    s->sor};
     for (void *b : more)
         if (b) (void)hipFree(b);
     for (int l = 0; l < kMaxLevels; ++l) {
@@ -2057,6 +2125,16 @@ fl_status validate(fl_ctx *ctx, const fl_cavity3d_config *c) {
         return set_error(ctx, FL_ERR_CONFIG, "unknown preconditioner %d", c->preconditioner);
     if (c->preconditioner != FL_PRECOND_JACOBI && c->precision != FL_PRECISION_MIXED)
         return set_error(ctx, FL_ERR_CONFIG, "the multigrid preconditioner runs in the mixed-precision solve");
+    if (c->pressure_solver != FL_SOLVER_PCG && c->pressure_solver != FL_SOLVER_SOR)
+        return set_error(ctx, FL_ERR_CONFIG, "unknown pressure solver %d", c->pressure_solver);
+    if (c->pressure_solver == FL_SOLVER_SOR) {
+        if (!(std::isfinite(c->pressure_omega) && c->pressure_omega > 0.0 && c->pressure_omega < 2.0))
+            return set_error(ctx, FL_ERR_CONFIG, "pressure_omega must lie between 0 and 2");
+        if (c->precision != FL_PRECISION_FP64 || c->preconditioner != FL_PRECOND_JACOBI ||
+            c->stop_rule != FL_STOP_ABS_INF)
+            return set_error(ctx, FL_ERR_CONFIG,
+                             "the SOR pressure solver runs in FP64 with the reference's max-residual stop rule");
+    }
     const double cells = (double)c->nx * (double)c->ny * (double)c->nz;
     const double faces = (double)(c->nx + 1) * (double)(c->ny + 1) * (double)(c->nz + 1) * 3.0;
     if (cells + faces >= 2147483647.0)
@@ -2177,6 +2255,13 @@ fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep, double rr0_ref = 0.0)
 constexpr int kFusedFrom = 2;
 // Damped-Jacobi sweeps on level 0 before and after the coarse correction.
 constexpr int kL0Smooth = 3;
+// The multigrid level whose coarse solve is the U-Net (FL_PRECOND_LEARNED
+// with a network attached at a level below the finest), or 0 for none.
+int learned_level(const fl_cavity3d *s) {
+    if (s->cfg.preconditioner != FL_PRECOND_LEARNED || !s->unet) return 0;
+    return s->wl_level >= 1 && s->wl_level <= s->tail ? s->wl_level : 0;
+}
+
 fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k) {
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
@@ -2199,7 +2284,12 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     }
     FL_TIMED(s, K_MG_RESTRICT_L0, k, 6.0 * n0 + 4.0 * n1,
              k_mg_restrict_z<e16><<<dim3((L[0].nx + kRX - 1) / kRX, L[0].ny / 2, s->zg.bz), kRX, 0, stm>>>(L[0], L[1], s->r32, pre0, s->mf[1], s->fa, s->fb));
-    for (int l = 1; l < t; ++l) {
+    // The learned coarse solve: from the network's level Lw down, the cycle
+    // is replaced by the U-Net's prediction of A_Lw^-1 f_Lw (the level's
+    // Galerkin diagonal sets its scale) and two damped-Jacobi sweeps.
+    const int Lw = learned_level(s);
+    const int down_end = Lw > 0 ? Lw : t;
+    for (int l = 1; l < down_end; ++l) {
         const double nl = L[l].n, nc = L[l + 1].n;
         if (l >= kFusedFrom) {
             FL_TIMED(s, K_MG_DOWN, k, 8.0 * nl + 4.0 * nc,
@@ -2213,14 +2303,32 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
                      k_mg_restrict_z<float><<<gr, kRX, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->mf[l + 1], s->fa, s->fb));
         }
     }
-    TailLevels T{};
-    T.count = s->nlev - t;
-    for (int l = 0; l < T.count; ++l) T.lv[l] = L[t + l];
-    T.pinv = s->pinv;
-    FL_TIMED(s, K_MG_TAIL, k, 8.0 * L[t].n,
-             k_mg_tail<<<1, 1024, 0, stm>>>(T, s->mf[t], s->meb[t], s->fa, s->fb));
+    if (s->cap_level > 0 && !s->cap_taken && k == s->cap_iter && s->cap_level <= down_end) {
+        FL_HIP_TRY(ctx, hipMemcpyAsync(s->cap, s->mf[s->cap_level], L[s->cap_level].n * sizeof(float),
+                                       hipMemcpyDeviceToDevice, stm));
+        s->cap_taken = true;
+    }
+    if (Lw > 0) {
+        const Level &LL = L[Lw];
+        const double diag = 2.0 * ((double)LL.cx + (double)LL.cy + (double)LL.cz);
+        const uint32_t gl = s->lblocks[Lw];
+        fl_status ust = FL_OK;
+        FL_TIMED(s, K_UNET_COARSE, k, 0.0,
+                 ust = unet_infer_device(s->unet, stm, s->mf[Lw], diag, s->wl_out);
+                 k_mg_sweep<false, false, false, false><<<gl, kBlock, 0, stm>>>(LL, LL, s->mf[Lw], s->wl_out, nullptr, s->wl_in, nullptr, nullptr, s->fa, s->fb, 1.0f);
+                 k_mg_sweep<false, false, false, false><<<gl, kBlock, 0, stm>>>(LL, LL, s->mf[Lw], s->wl_in, nullptr, s->meb[Lw], nullptr, nullptr, s->fa, s->fb, 1.0f));
+        if (ust != FL_OK)
+            return set_error(ctx, FL_ERR_HIP, "U-Net inference failed: %s", fl_last_error(unet_context(s->unet)));
+    } else {
+        TailLevels T{};
+        T.count = s->nlev - t;
+        for (int l = 0; l < T.count; ++l) T.lv[l] = L[t + l];
+        T.pinv = s->pinv;
+        FL_TIMED(s, K_MG_TAIL, k, 8.0 * L[t].n,
+                 k_mg_tail<<<1, 1024, 0, stm>>>(T, s->mf[t], s->meb[t], s->fa, s->fb));
+    }
     // up: the solution of level l ends in meb[l]
-    for (int l = t - 1; l >= 1; --l) {
+    for (int l = down_end - 1; l >= 1; --l) {
         const double nl = L[l].n, nc = L[l + 1].n;
         if (l >= kFusedFrom) {
             FL_TIMED(s, K_MG_UP, k, 12.0 * nl + 4.0 * nc,
@@ -2385,8 +2493,12 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
     bool converged = false, fallback = false, first_pass = true;
     double rr = 0.0, rmax = 0.0;
     // learned warm start: 1 pending, 2 applied and awaiting its residual;
-    // warm_used ends 1 (kept) or -1 (rejected: it did not lower the residual)
-    int warm = (s->cfg.preconditioner == FL_PRECOND_LEARNED && s->unet) ? 1 : 0, warm_used = 0;
+    // warm_used ends 1 (kept) or -1 (rejected: it did not lower the residual).
+    // With the network inside the V-cycle (learned_level > 0) the first FCG
+    // step, alpha z_0 with z_0 from the learned cycle and the energy-
+    // minimising alpha, is that warm start, so none is taken separately.
+    int warm = (s->cfg.preconditioner == FL_PRECOND_LEARNED && s->unet && learned_level(s) == 0) ? 1 : 0,
+        warm_used = 0;
     double warm_ratio = 0.0;
     for (;;) {
         FL_TIMED(s, K_RESIDUAL_F, -1, 20.0 * s->n,
@@ -2510,6 +2622,43 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
     return FL_OK;
 }
 
+// SOR to max|Lap p - rhs| < tol (the reference's solver and stop rule),
+// batches of sweeps with the verdict kept on the device.
+fl_status solve_sor(fl_cavity3d *s, fl_solve_report *rep) {
+    fl_ctx *ctx = s->ctx;
+    hipStream_t stm = ctx->stream;
+    const SorState zero{};
+    FL_HIP_TRY(ctx, hipMemcpyAsync(s->sor, &zero, sizeof(SorState), hipMemcpyHostToDevice, stm));
+    const uint32_t half = ((s->g.nx + 1) / 2) * s->g.ny * s->g.nz;
+    const uint32_t G = grid_blocks(ctx, (half + kBlock - 1) / kBlock, 8);
+    SorState h{};
+    int batch = 4;
+    for (;;) {
+        for (int b = 0; b < batch; ++b) {
+            k_sor3<<<G, kBlock, 0, stm>>>(s->g, s->rhs, s->p, s->cfg.pressure_omega, 0u, s->sor);
+            k_sor3<<<G, kBlock, 0, stm>>>(s->g, s->rhs, s->p, s->cfg.pressure_omega, 1u, s->sor);
+            launch_residual<double>(s, s->r);
+            k_fold_outer<<<1, kBlock, 0, stm>>>(s->pb, s->zr.count, s->outer);
+            k_sor_check<<<1, 64, 0, stm>>>(s->outer, s->cfg.pressure_tolerance, s->cfg.pressure_max_iters, s->sor);
+        }
+        FL_HIP_TRY(ctx, hipGetLastError());
+        FL_HIP_TRY(ctx, hipMemcpyAsync(&h, s->sor, sizeof(SorState), hipMemcpyDeviceToHost, stm));
+        FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
+        if (h.done) break;
+        batch = std::min(batch * 2, kMaxBatch);
+    }
+    if (h.status != ST_OK)
+        return set_error(ctx, FL_ERR_SOLVER, "3D pressure solver produced a non-finite residual");
+    if (rep) {
+        *rep = fl_solve_report{};
+        rep->iterations = h.iters;
+        rep->residual_abs_inf = h.res;
+        rep->converged = h.converged;
+        rep->status = FL_OK;
+    }
+    return FL_OK;
+}
+
 // Pressure solve: p holds the guess on entry and the solution on exit; rhs
 // is the device right-hand side.
 fl_status solve_pressure(fl_cavity3d *s, fl_solve_report *rep) {
@@ -2519,6 +2668,8 @@ fl_status solve_pressure(fl_cavity3d *s, fl_solve_report *rep) {
         FL_HIP_TRY(ctx, hipMemsetAsync(s->p, 0, s->n * sizeof(double), stm));
     else
         FL_HIP_TRY(ctx, hipMemsetAsync(s->p, 0, sizeof(double), stm));  // pin p(0,0,0) = 0
+    s->cap_taken = false;
+    if (s->cfg.pressure_solver == FL_SOLVER_SOR) return solve_sor(s, rep);
     return s->cfg.precision == FL_PRECISION_MIXED ? solve_mixed(s, rep) : solve_fp64(s, rep);
 }
 
@@ -2650,7 +2801,8 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
         (st = alloc(s, &s->p, s->n)) || (st = alloc(s, &s->rhs, s->n)) ||
         (st = alloc(s, &s->partials, prow)) || (st = alloc(s, &s->pa, prow)) ||
         (st = alloc(s, &s->pb, prow + 4)) || (st = alloc(s, &s->outer, 4)) ||
-        (st = alloc(s, &s->sa, 1)) || (st = alloc(s, &s->sb, 1)) || (st = alloc(s, &s->dg, 1)))
+        (st = alloc(s, &s->sa, 1)) || (st = alloc(s, &s->sb, 1)) || (st = alloc(s, &s->dg, 1)) ||
+        (st = alloc(s, &s->sor, 1)))
         goto fail;
     if (mixed) {
         if ((st = alloc(s, &s->e32, s->n)) || (st = alloc(s, &s->r32, s->n)) ||
@@ -2839,6 +2991,45 @@ fl_status fl_cavity3d_set_unet(fl_cavity3d *s, fl_unet *net) {
     }
     s->unet = net;
     s->wl_level = L;
+    return FL_OK;
+}
+
+fl_status fl_cavity3d_capture_coarse(fl_cavity3d *s, int32_t n, int32_t iteration) {
+    if (!s || iteration < 0) return FL_ERR_INVALID_ARGUMENT;
+    s->cap_iter = iteration;
+    fl_ctx *ctx = s->ctx;
+    if (n == 0) {
+        s->cap_level = 0;
+        return FL_OK;
+    }
+    if (s->cfg.preconditioner == FL_PRECOND_JACOBI)
+        return set_error(ctx, FL_ERR_CONFIG, "coarse capture needs a multigrid preconditioner");
+    int L = -1;
+    for (int l = 1; l <= s->tail && l < s->nlev; ++l)
+        if (s->lv[l].nx == (uint32_t)n && s->lv[l].ny == (uint32_t)n && s->lv[l].nz == (uint32_t)n) L = l;
+    if (L < 0)
+        return set_error(ctx, FL_ERR_CONFIG, "no multigrid level below the finest has %d^3 cells", n);
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (s->cap) (void)hipFree(s->cap);
+    s->cap = nullptr;
+    fl_status st = alloc(s, &s->cap, s->lv[L].n);
+    if (st != FL_OK) return st;
+    s->cap_level = L;
+    return FL_OK;
+}
+
+fl_status fl_cavity3d_captured_rhs(fl_cavity3d *s, float *rhs_c, double *diag) {
+    if (!s || !rhs_c) return set_error(s ? s->ctx : nullptr, FL_ERR_INVALID_ARGUMENT, "null argument");
+    fl_ctx *ctx = s->ctx;
+    if (s->cap_level <= 0) return set_error(ctx, FL_ERR_CONFIG, "coarse capture is not armed");
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const Level &L = s->lv[s->cap_level];
+    FL_HIP_TRY(ctx, hipMemcpyAsync(rhs_c, s->cap, L.n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    FL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (diag) {
+        const double f = (double)(1u << s->cap_level);
+        *diag = 2.0 * (s->g.ix2 + s->g.iy2 + s->g.iz2) / (f * f);
+    }
     return FL_OK;
 }
 

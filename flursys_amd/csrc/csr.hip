@@ -5,8 +5,10 @@
 // LinearAlgebraError kinds and &'static str texts.
 //
 // Device layout: int32 row offsets and columns, FP64 values, FP64 vectors.
-// Rows are cut into 256-row tiles handed out XCD-slab-wise (common.hpp), one
-// thread per row (the unstructured Poisson rows hold ~4 entries).
+// Rows are cut into tiles handed out XCD-slab-wise (common.hpp); an SpMV
+// works each row with four adjacent lanes (the unstructured Poisson rows
+// hold ~4 entries, so a row's entries are one coalesced load per operand),
+// the vector kernels one element per lane.
 //
 // Per CG iteration (3 kernels, scalars never leave the device):
 //   k_cg_spmv    q = A d, d.q                    -> alpha
@@ -91,11 +93,37 @@ struct LinState {
     unsigned counters[8];
 };
 
-__device__ inline double row_dot(const Mat &A, uint32_t row, const double *__restrict__ x) {
-    const int b = A.off[row], e = A.off[row + 1];
-    double s = 0.0;
-    for (int k = b; k < e; ++k) s += A.val[k] * x[A.col[k]];
-    return s;
+// SpMV rows are worked by groups of kRowLanes adjacent lanes: per chunk of
+// four entries lane q loads entry q (the ~4 entries of a Poisson row are one
+// coalesced load per operand across the group) and forms its product; the
+// group then adds the chunk's products in entry order, so the row value is
+// the reference's left-to-right sum of rounded products (spmv_into,
+// csr.rs:89-106) and every lane of the group holds it.  A 256-thread block
+// covers a tile of 64 rows.
+constexpr uint32_t kRowLanes = 4, kRowsPerTile = kBlock / kRowLanes;
+struct RowSlot {
+    uint32_t row, q;
+    bool valid;
+};
+__device__ inline RowSlot row_slot(uint32_t tile, uint32_t n) {
+    RowSlot r;
+    r.row = tile * kRowsPerTile + threadIdx.x / kRowLanes;
+    r.q = threadIdx.x % kRowLanes;
+    r.valid = r.row < n;
+    return r;
+}
+template <typename F>
+__device__ inline double row_dot_lanes(const Mat &A, const RowSlot &r, F &&xval) {
+    double acc = 0.0;
+    const int b = r.valid ? A.off[r.row] : 0, e = r.valid ? A.off[r.row + 1] : 0;
+    const int lead = (int)(threadIdx.x & 63u) & ~(int)(kRowLanes - 1);
+    for (int k0 = b; k0 < e; k0 += kRowLanes) {  // uniform over the group
+        const int k = k0 + (int)r.q;
+        const double prod = k < e ? A.val[k] * xval(A.col[k]) : 0.0;
+#pragma unroll
+        for (int t = 0; t < (int)kRowLanes; ++t) acc += __shfl(prod, lead + t);
+    }
+    return acc;
 }
 
 __device__ inline void fail(LinState *st, int status, int ctx, long long index) {
@@ -112,13 +140,13 @@ __device__ inline unsigned long long atomic_read(unsigned long long *p) {
 // ------------------------------------------------------------------ SpMV
 __global__ void __launch_bounds__(kBlock) k_spmv(Mat A, const double *__restrict__ x,
                                                  double *__restrict__ y, LinState *st) {
-    TileSched ts((A.n + kBlock - 1) / kBlock);
+    TileSched ts((A.n + kRowsPerTile - 1) / kRowsPerTile);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
-        const uint32_t row = t * kBlock + threadIdx.x;
-        if (row >= A.n) continue;
-        const double v = row_dot(A, row, x);
-        if (!isfinite(v)) atomicMin(&st->bad_row, (unsigned long long)row);
-        y[row] = v;
+        const RowSlot r = row_slot(t, A.n);
+        const double v = row_dot_lanes(A, r, [&](int c) { return x[c]; });
+        if (!r.valid || r.q != 0) continue;
+        if (!isfinite(v)) atomicMin(&st->bad_row, (unsigned long long)r.row);
+        y[r.row] = v;
     }
 }
 
@@ -189,14 +217,14 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv(Mat A, const double *__restr
                                                     LinState *st) {
     if (st->done) return;
     double s[1] = {0.0}, m[1] = {0.0};
-    TileSched ts((A.n + kBlock - 1) / kBlock);
+    TileSched ts((A.n + kRowsPerTile - 1) / kRowsPerTile);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
-        const uint32_t row = t * kBlock + threadIdx.x;
-        if (row >= A.n) continue;
-        const double v = row_dot(A, row, d);
-        if (!isfinite(v)) atomicMin(&st->bad_row, (unsigned long long)row);
-        q[row] = v;
-        s[0] += d[row] * v;
+        const RowSlot r = row_slot(t, A.n);
+        const double v = row_dot_lanes(A, r, [&](int c) { return d[c]; });
+        if (!r.valid || r.q != 0) continue;
+        if (!isfinite(v)) atomicMin(&st->bad_row, (unsigned long long)r.row);
+        q[r.row] = v;
+        s[0] += d[r.row] * v;
     }
     if (grid_reduce<1, 1>(s, m, partials, &st->counters[2]) && threadIdx.x == 0) {
         const unsigned long long bad = atomic_read(&st->bad_row);
@@ -331,14 +359,14 @@ __global__ void __launch_bounds__(kBlock) k_bi_spmv_v(Mat A, const double *__res
                                                       LinState *st) {
     if (st->done) return;
     double s[1] = {0.0}, m[1] = {0.0};
-    TileSched ts((A.n + kBlock - 1) / kBlock);
+    TileSched ts((A.n + kRowsPerTile - 1) / kRowsPerTile);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
-        const uint32_t row = t * kBlock + threadIdx.x;
-        if (row >= A.n) continue;
-        const double vv = row_dot(A, row, p);
-        if (!isfinite(vv)) atomicMin(&st->bad_row, (unsigned long long)row);
-        v[row] = vv;
-        s[0] += rhat[row] * vv;
+        const RowSlot r = row_slot(t, A.n);
+        const double vv = row_dot_lanes(A, r, [&](int c) { return p[c]; });
+        if (!r.valid || r.q != 0) continue;
+        if (!isfinite(vv)) atomicMin(&st->bad_row, (unsigned long long)r.row);
+        v[r.row] = vv;
+        s[0] += rhat[r.row] * vv;
     }
     if (grid_reduce<1, 1>(s, m, partials, &st->counters[2]) && threadIdx.x == 0) {
         const unsigned long long bad = atomic_read(&st->bad_row);
@@ -360,16 +388,12 @@ __global__ void __launch_bounds__(kBlock) k_bi_spmv_t(Mat A, const double *__res
     if (st->done) return;
     const double alpha = st->alpha;
     double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
-    TileSched ts((A.n + kBlock - 1) / kBlock);
+    TileSched ts((A.n + kRowsPerTile - 1) / kRowsPerTile);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
-        const uint32_t row = t * kBlock + threadIdx.x;
-        if (row >= A.n) continue;
-        const int b = A.off[row], e = A.off[row + 1];
-        double acc = 0.0;
-        for (int k = b; k < e; ++k) {
-            const int c = A.col[k];
-            acc += A.val[k] * (r[c] - alpha * v[c]);
-        }
+        const RowSlot rs = row_slot(t, A.n);
+        const double acc = row_dot_lanes(A, rs, [&](int c) { return r[c] - alpha * v[c]; });
+        if (!rs.valid || rs.q != 0) continue;
+        const uint32_t row = rs.row;
         if (!isfinite(acc)) atomicMin(&st->bad_row, (unsigned long long)row);
         tvec[row] = acc;
         const double si = r[row] - alpha * v[row];

@@ -68,6 +68,10 @@ struct Prob {
     double eta;  // mixed path: relative tolerance of each inner CG cycle
     int32_t stop_rule, guess;
     FastDiv fx;
+    int32_t inlet;       // FL_INLET_* of a left Velocity side
+    double step_h, height;  // inlet profile: step height and domain height
+    double omega;        // SOR relaxation factor
+    uint32_t ref;        // SOR gauge: the first fluid cell (find_reference_cell)
 };
 
 struct PcgA {  // a grid's PCG state after phase A
@@ -131,6 +135,129 @@ __device__ inline double v_nb(const Prob &P, const uint8_t *F, const double *v, 
 }
 __device__ inline double transported(int scheme, double fv, double l, double r) {
     return scheme == FL_UPWIND ? (fv >= 0.0 ? l : r) : 0.5 * (l + r);
+}
+
+// ------------------------------------------------------------ SOR
+// solve_pressure_sor (incompressible.rs:979-1046) in red-black order.  One
+// sweep = the two colours (i + j) & 1, the gauge shift (the first fluid
+// cell's value subtracted from every non-solid cell when no outlet pins the
+// level) and the max residual of pressure_residual (:1532-1567); the verdict
+// is kept per grid on the device.
+struct SorState {
+    unsigned long long res_bits;  // max |Lap p - rhs| of the sweep (non-negative double bits)
+    double res, refval;
+    long long iters;
+    int done, converged, bad, pad;
+};
+
+__global__ void __launch_bounds__(kT) k_sor_sweep(const Prob *probs, const uint32_t *tile_prob,
+                                                  uint32_t ntiles, const int *active, const uint8_t *flags,
+                                                  const double *__restrict__ rhs, double *__restrict__ x,
+                                                  uint32_t color, const SorState *st) {
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t pi = tile_prob[t];
+        if (!active[pi] || st[pi].done) continue;
+        const Prob &P = probs[pi];
+        uint32_t c, i, j;
+        if (!cell_of(P, t, c, i, j) || ((i + j) & 1) != color) continue;
+        const size_t g = (size_t)P.tile0 * kT + c;
+        const uint8_t f = flags[g];
+        if (f & F_SOLID) {
+            x[g] = 0.0;
+            continue;
+        }
+        if (f & F_OUTLET) {
+            x[g] = P.pout;
+            continue;
+        }
+        double ap = 0.0, sum = 0.0;
+        if (f & F_E) { ap += P.cx; sum += P.cx * x[g + 1]; }
+        if (f & F_W) { ap += P.cx; sum += P.cx * x[g - 1]; }
+        if (f & F_N) { ap += P.cy; sum += P.cy * x[g + P.nx]; }
+        if (f & F_S) { ap += P.cy; sum += P.cy * x[g - P.nx]; }
+        if (ap > 0.0) {
+            const double candidate = (sum - rhs[g]) / ap;
+            const double old = x[g];
+            x[g] = old + P.omega * (candidate - old);
+        }
+    }
+}
+// The gauge value of each grid (one thread per grid).
+__global__ void k_sor_ref(const Prob *probs, const int *active, int count, const double *__restrict__ x,
+                          SorState *st) {
+    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= count || !active[pi] || st[pi].done) return;
+    const Prob &P = probs[pi];
+    st[pi].refval = P.zm ? x[(size_t)P.tile0 * kT + P.ref] : 0.0;
+}
+__global__ void __launch_bounds__(kT) k_sor_shift(const Prob *probs, const uint32_t *tile_prob,
+                                                  uint32_t ntiles, const int *active, const uint8_t *flags,
+                                                  double *__restrict__ x, const SorState *st) {
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t pi = tile_prob[t];
+        if (!active[pi] || st[pi].done || !probs[pi].zm) continue;
+        const Prob &P = probs[pi];
+        uint32_t c, i, j;
+        if (!cell_of(P, t, c, i, j)) continue;
+        const size_t g = (size_t)P.tile0 * kT + c;
+        if (!(flags[g] & F_SOLID)) x[g] -= st[pi].refval;
+    }
+}
+// max |Lap p - rhs| over fluid, non-outlet cells (a closed neighbour reads the
+// centre); one atomic max per tile on the non-negative double's bits.
+__global__ void __launch_bounds__(kT) k_sor_residual(const Prob *probs, const uint32_t *tile_prob,
+                                                     uint32_t ntiles, const int *active, const uint8_t *flags,
+                                                     const double *__restrict__ rhs,
+                                                     const double *__restrict__ x, SorState *st) {
+    __shared__ double red[kT / 64];
+    __shared__ int bad_any[1];
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t pi = tile_prob[t];
+        if (!active[pi] || st[pi].done) continue;  // uniform over the tile
+        const Prob &P = probs[pi];
+        if (threadIdx.x == 0) bad_any[0] = 0;
+        __syncthreads();
+        double r = 0.0;
+        uint32_t c, i, j;
+        if (cell_of(P, t, c, i, j)) {
+            const size_t g = (size_t)P.tile0 * kT + c;
+            const uint8_t f = flags[g];
+            if (!(f & F_ID)) {
+                const double p0 = x[g];
+                const double pe = (f & F_E) ? x[g + 1] : p0, pw = (f & F_W) ? x[g - 1] : p0;
+                const double pn = (f & F_N) ? x[g + P.nx] : p0, ps = (f & F_S) ? x[g - P.nx] : p0;
+                const double lap = (pe - 2.0 * p0 + pw) * P.cx + (pn - 2.0 * p0 + ps) * P.cy;
+                r = fabs(lap - rhs[g]);
+                if (!isfinite(r)) {
+                    bad_any[0] = 1;
+                    r = 0.0;
+                }
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_xor(r, o));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = red[0];
+            for (int w = 1; w < kT / 64; ++w) m = fmax(m, red[w]);
+            atomicMax(&st[pi].res_bits, (unsigned long long)__double_as_longlong(m));
+            if (bad_any[0]) atomicOr(&st[pi].bad, 1);
+        }
+        __syncthreads();
+    }
+}
+// The sweep's verdict per grid (one thread per grid).
+__global__ void k_sor_check(const Prob *probs, const int *active, int count, SorState *st) {
+    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= count || !active[pi] || st[pi].done) return;
+    SorState &S = st[pi];
+    const double res = __longlong_as_double((long long)S.res_bits);
+    S.res_bits = 0ull;
+    S.iters += 1;
+    S.res = S.bad ? (double)NAN : res;
+    if (S.bad) S.done = 1;
+    else if (res < probs[pi].tol) S.converged = S.done = 1;
+    else if (S.iters >= probs[pi].max_iter) S.done = 1;
 }
 
 // ------------------------------------------------------------ step kernels
@@ -205,7 +332,15 @@ __global__ void __launch_bounds__(kT) k_boundaries(const Prob *probs, const int 
     const uint32_t nx = P.nx, ny = P.ny;
     for (uint32_t j = threadIdx.x; j < ny; j += blockDim.x) {
         double &l = u[(nx + 1) * j], &r = u[nx + (nx + 1) * j];
-        l = P.kind[SL] == KV ? P.bu[SL] : (P.kind[SL] == KS ? 0.0 : u[1 + (nx + 1) * j]);
+        double ul = P.bu[SL];
+        if (P.inlet == FL_INLET_PARABOLIC) {
+            // Case::boundary_velocity's Channel / BackwardFacingStep inlet at the
+            // face centre y = (j + 1/2) dy (src/cases/mod.rs:141-170)
+            const double y = ((double)j + 0.5) * P.dy;
+            const double eta = fmin(fmax((y - P.step_h) / (P.height - P.step_h), 0.0), 1.0);
+            ul = y <= P.step_h ? 0.0 : 6.0 * P.bu[SL] * eta * (1.0 - eta);
+        }
+        l = P.kind[SL] == KV ? ul : (P.kind[SL] == KS ? 0.0 : u[1 + (nx + 1) * j]);
         r = P.kind[SR] == KV ? P.bu[SR] : (P.kind[SR] == KS ? 0.0 : u[nx - 1 + (nx + 1) * j]);
     }
     for (uint32_t i = threadIdx.x; i < nx; i += blockDim.x) {
@@ -1312,6 +1447,9 @@ struct Batch2 {
     hipGraph_t gr_step = nullptr, gr_pre = nullptr, gr_post = nullptr;
     hipGraphExec_t gx_step = nullptr, gx_pre = nullptr, gx_post = nullptr;
     std::vector<hipEvent_t> ev;
+    // SOR (FL_SOLVER_SOR): per-grid iteration state
+    bool sor = false;
+    SorState *sor_st = nullptr, *h_sor = nullptr;
 };
 
 template <typename T>
@@ -1359,6 +1497,8 @@ void free_batch(Batch2 *B) {
     if (B->bar) (void)hipFree(B->bar);
     if (B->e32) (void)hipFree(B->e32);
     if (B->zv) (void)hipFree(B->zv);
+    if (B->sor_st) (void)hipFree(B->sor_st);
+    if (B->h_sor) (void)hipHostFree(B->h_sor);
     drop_graphs(B);
     for (hipEvent_t e : B->ev) (void)hipEventDestroy(e);
 }
@@ -1395,6 +1535,20 @@ fl_status validate2(fl_ctx *ctx, const fl_grid2d_config &c) {
     for (int s = 0; s < 4; ++s)
         if (c.bc_kind[s] < FL_BC_VELOCITY || c.bc_kind[s] > FL_BC_OUTLET)
             return set_error(ctx, FL_ERR_CONFIG, "unknown boundary kind %d", c.bc_kind[s]);
+    if (c.pressure_solver != FL_SOLVER_PCG && c.pressure_solver != FL_SOLVER_SOR)
+        return set_error(ctx, FL_ERR_CONFIG, "unknown pressure solver %d", c.pressure_solver);
+    if (c.pressure_solver == FL_SOLVER_SOR) {
+        if (!std::isfinite(c.pressure_omega) || c.pressure_omega <= 0.0 || c.pressure_omega >= 2.0)
+            return set_error(ctx, FL_ERR_CONFIG, "pressure_omega must lie between 0 and 2");
+        if (c.precision != FL_PRECISION_FP64 || c.stop_rule != FL_STOP_ABS_INF)
+            return set_error(ctx, FL_ERR_CONFIG,
+                             "the SOR pressure solver runs in FP64 with the reference's max-residual stop rule");
+    }
+    if (c.inlet_profile != FL_INLET_UNIFORM && c.inlet_profile != FL_INLET_PARABOLIC)
+        return set_error(ctx, FL_ERR_CONFIG, "unknown inlet profile %d", c.inlet_profile);
+    if (c.inlet_profile == FL_INLET_PARABOLIC &&
+        !(std::isfinite(c.inlet_step_height) && c.inlet_step_height >= 0.0 && c.inlet_step_height < c.height))
+        return set_error(ctx, FL_ERR_CONFIG, "inlet_step_height must lie in [0, height)");
     if (c.adaptive) {
         if (!std::isfinite(c.min_dt) || c.min_dt <= 0.0)
             return set_error(ctx, FL_ERR_CONFIG, "adaptive min_dt must be finite and positive");
@@ -1529,10 +1683,15 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
         P.stop_rule = c.stop_rule;
         P.guess = c.initial_guess;
         P.fx = FastDiv(P.nx);
+        P.inlet = c.inlet_profile;
+        P.step_h = c.inlet_step_height;
+        P.height = c.height;
+        P.omega = c.pressure_omega;
         const uint8_t *sol = solids ? solids[p] : nullptr;
         auto solid = [&](uint32_t i, uint32_t j) { return sol && sol[i + P.nx * j]; };
         hflags.resize((size_t)(tile + P.ntiles) * kT, 0);
         uint64_t mean_cells = 0;
+        bool found_ref = false;  // find_reference_cell (:1614-1623): first non-solid cell
         for (uint32_t j = 0; j < P.ny; ++j)
             for (uint32_t i = 0; i < P.nx; ++i) {
                 uint8_t f = 0;
@@ -1545,6 +1704,10 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
                 if (s) f |= F_SOLID | F_ID;
                 if (out) f |= F_OUTLET | F_ID;
                 if (!(f & F_ID)) ++mean_cells;
+                if (!s && !found_ref) {
+                    P.ref = i + P.nx * j;
+                    found_ref = true;
+                }
                 hflags[(size_t)tile * kT + i + P.nx * j] = f;
             }
         P.inv_count = mean_cells ? 1.0 / (double)mean_cells : 0.0;
@@ -1561,9 +1724,13 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
     if (count > kMaxGrids)
         return set_error(ctx, FL_ERR_CONFIG, "a batch holds at most %d grids", kMaxGrids);
     B->mixed = cfgs[0].precision == FL_PRECISION_MIXED;
-    for (int p = 1; p < count; ++p)
+    B->sor = cfgs[0].pressure_solver == FL_SOLVER_SOR;
+    for (int p = 1; p < count; ++p) {
         if (cfgs[p].precision != cfgs[0].precision)
             return set_error(ctx, FL_ERR_CONFIG, "all grids of a batch share one precision");
+        if (cfgs[p].pressure_solver != cfgs[0].pressure_solver)
+            return set_error(ctx, FL_ERR_CONFIG, "all grids of a batch share one pressure solver");
+    }
     if ((st = dalloc(B, &B->probs, count)) || (st = dalloc(B, &B->tile_prob, B->ntiles)) ||
         (st = dalloc(B, &B->flags, B->ncell)) || (st = dalloc(B, &B->x, B->ncell)) ||
         (st = dalloc(B, &B->rhs, B->ncell)) || (st = dalloc(B, &B->r, B->ncell)) ||
@@ -1581,8 +1748,10 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
         (st = dalloc(B, &B->sb, count)) || (st = dalloc(B, &B->fin, 2 * count)) ||
         (st = dalloc(B, &B->active, count)) || (st = dalloc(B, &B->dts, count)) ||
         (st = dalloc(B, &B->diag, count)) || (st = dalloc(B, &B->ctl, count)) ||
-        (st = dalloc(B, &B->bar, 128)) || (st = dalloc(B, &B->e32, B->mixed ? B->ncell : 1)))
+        (st = dalloc(B, &B->bar, 128)) || (st = dalloc(B, &B->e32, B->mixed ? B->ncell : 1)) ||
+        (st = dalloc(B, &B->sor_st, count)))
         return st;
+    if ((st = halloc(B, &B->h_sor, count)) != FL_OK) return st;
     if ((st = halloc(B, &B->h_active, count)) || (st = halloc(B, &B->h_fin, 2 * count)) ||
         (st = halloc(B, &B->h_dts, count)) || (st = halloc(B, &B->h_sa, count)) ||
         (st = halloc(B, &B->h_sb, count)) || (st = halloc(B, &B->h_diag, count)) ||
@@ -1691,6 +1860,51 @@ fl_status run_pcg(Batch2 *B) {
     return FL_OK;
 }
 
+// SOR for every active grid (FL_SOLVER_SOR batches); the results land in
+// h_sa / h_sb like the PCG's for the step reports.
+fl_status run_sor(Batch2 *B) {
+    fl_ctx *ctx = B->ctx;
+    hipStream_t s = ctx->stream;
+    FL_HIP_TRY(ctx, hipMemsetAsync(B->sor_st, 0, B->count * sizeof(SorState), s));
+    for (int p = 0; p < B->count; ++p)
+        if (B->h_active[p] && B->hp[p].guess == FL_GUESS_ZERO)
+            FL_HIP_TRY(ctx, hipMemsetAsync(B->x + (size_t)B->hp[p].tile0 * kT, 0, (size_t)B->hp[p].n * 8, s));
+    const uint32_t G = ew_blocks(B);
+    const unsigned gb = (unsigned)(B->count + 63) / 64;
+    int batch = 4;
+    for (;;) {
+        for (int b = 0; b < batch; ++b) {
+            k_sor_sweep<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->flags, B->rhs, B->x, 0u, B->sor_st);
+            k_sor_sweep<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->flags, B->rhs, B->x, 1u, B->sor_st);
+            k_sor_ref<<<gb, 64, 0, s>>>(B->probs, B->active, B->count, B->x, B->sor_st);
+            k_sor_shift<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->flags, B->x, B->sor_st);
+            k_sor_residual<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->flags, B->rhs, B->x, B->sor_st);
+            k_sor_check<<<gb, 64, 0, s>>>(B->probs, B->active, B->count, B->sor_st);
+        }
+        FL_HIP_TRY(ctx, hipGetLastError());
+        FL_HIP_TRY(ctx, hipMemcpyAsync(B->h_sor, B->sor_st, B->count * sizeof(SorState), hipMemcpyDeviceToHost, s));
+        FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+        bool all = true;
+        for (int p = 0; p < B->count; ++p)
+            if (B->h_active[p] && !B->h_sor[p].done) all = false;
+        if (all) break;
+        batch = std::min(batch * 2, 256);
+    }
+    bool bad = false;
+    for (int p = 0; p < B->count; ++p) {
+        if (!B->h_active[p]) continue;
+        const SorState &S = B->h_sor[p];
+        B->h_sa[p] = PcgA{};
+        B->h_sb[p] = PcgB{};
+        B->h_sa[p].rmax = S.res;
+        B->h_sa[p].converged = S.converged;
+        B->h_sb[p].iters = S.iters;
+        bad = bad || S.bad;
+    }
+    if (bad) return set_error(ctx, FL_ERR_SOLVER, "Pressure solver produced a non-finite residual");
+    return FL_OK;
+}
+
 const char *pcg_text(int status) {
     switch (status) {
         case ST_SINGULAR: return "PCG pressure solver encountered a singular search direction";
@@ -1759,7 +1973,7 @@ fl_status step_batch(Batch2 *B, fl_grid2d_step_report *reps, bool strict) {
     k_rhs<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->dts, B->flags, B->us,
                            B->vs, B->rhs);
     FL_HIP_TRY(ctx, hipGetLastError());
-    fl_status st = run_pcg(B);
+    fl_status st = B->sor ? run_sor(B) : run_pcg(B);
     if (st != FL_OK || (st = barrier_check(B)) != FL_OK) return st;
     for (int p = 0; p < B->count; ++p)
         if (B->h_active[p] && (st = pcg_status(B, p)) != FL_OK) return st;
@@ -1885,6 +2099,8 @@ fl_status run_device(Batch2 *B, int64_t steps, bool graph, fl_batch2d_run_report
     fl_ctx *ctx = B->ctx;
     hipStream_t s = ctx->stream;
     FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (B->sor)
+        return set_error(ctx, FL_ERR_CONFIG, "device-driven batch runs use the PCG pressure solver");
     for (int p = 0; p < B->count; ++p) {
         const fl_grid2d_config &c = B->cfg[p];
         StepCtl &k = B->h_ctl[p];
@@ -2181,7 +2397,7 @@ fl_status fl_grid2d_pressure_solve(fl_grid2d *g, const double *rhs, double *p_in
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->active, B->h_active, sizeof(int), hipMemcpyHostToDevice, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->rhs, rhs, (size_t)P.n * 8, hipMemcpyHostToDevice, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->x, p_inout, (size_t)P.n * 8, hipMemcpyHostToDevice, s));
-    fl_status st = run_pcg(B);
+    fl_status st = B->sor ? run_sor(B) : run_pcg(B);
     if (st != FL_OK || (st = barrier_check(B)) != FL_OK) return st;
     FL_HIP_TRY(ctx, hipMemcpyAsync(p_inout, B->x, (size_t)P.n * 8, hipMemcpyDeviceToHost, s));
     FL_HIP_TRY(ctx, hipStreamSynchronize(s));

@@ -74,16 +74,25 @@ __device__ inline int venc(int x, int y, int z, int lx) { return (z << lx | y) <
 struct ConvArgs {
     const bf16 *x1, *x2, *mask, *w;
     bf16 *y1, *y2;
+    float *part;     // K-split partial sums [ksplit][batch * nvox][cout] (FP32); ksplit 1: unused
     int lx;          // log2 of the resolution of x1 and of the output
     int c1, c2;      // channels of x1 (full resolution) and x2 (half, upsampled)
     int cin, kpad;   // c1 + c2, padded 27 cin
     int cout, ysplit;  // output channels; channels >= ysplit go to y2
     int relu;
+    int nsplit;      // output-channel groups of NB blocks, one per workgroup
+    int ksplit;      // K ranges, one per workgroup (> 1: partial sums, then k_conv_reduce)
+    int batch;
 };
 
 // One 3x3x3 convolution (or, with flipped transposed weights and the ReLU
-// mask of the layer's output, its data gradient).  Workgroup = 4 waves, each
-// wave 16 output voxels x all output channels (NB blocks of 16).
+// mask of the layer's output, its data gradient) as an implicit GEMM.
+// Workgroup = 4 waves, each wave 16 output voxels x NB blocks of 16 output
+// channels over one K range; blockIdx.x = voxel tile + tiles * (channel
+// group + nsplit * K range), blockIdx.y = sample.  The coarse levels have few
+// voxels, so their output channels and K are split over workgroups until
+// the chip is full (k_conv_reduce sums the K ranges in a fixed order).  The
+// operands of the next K step are loaded while the current one multiplies.
 template <int NB>
 __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -91,59 +100,136 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     const int b = blockIdx.y;
     const int n = 1 << a.lx;
     const long long nvox = 1ll << (3 * a.lx), nvox2 = nvox >> 3;
-    const int vbase = (blockIdx.x * 4 + wave) * 16;
+    const int tiles = (int)(nvox >> 6);
+    const int vt = blockIdx.x % tiles, rest = blockIdx.x / tiles;
+    const int ns = rest % a.nsplit, ks = rest / a.nsplit;
+    const int vbase = (vt * 4 + wave) * 16;
     int x, y, z;
     vdec(vbase + m, a.lx, x, y, z);
     const bf16 *x1 = a.x1 + b * nvox * a.c1;
     const bf16 *mk = a.mask ? a.mask + b * nvox * a.c1 : nullptr;
     const bf16 *x2 = a.x2 ? a.x2 + b * nvox2 * a.c2 : nullptr;
+    const int K = 27 * a.cin;
+    const int steps = a.kpad >> 5, per = (steps + a.ksplit - 1) / a.ksplit;
+    const int kb = ks * per * 32, ke = min(steps, (ks + 1) * per) * 32;
     f32x4 acc[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int K = 27 * a.cin;
-    int tap = 0, c = g * 8;
-    while (c >= a.cin) {
-        c -= a.cin;
-        ++tap;
-    }
-    for (int k0 = 0; k0 < a.kpad; k0 += 32) {
+    // this lane's k = k0 + 8 g: tap and channel, advanced by 32 per step
+    int c = kb + g * 8, tap = c / a.cin;
+    c -= tap * a.cin;
+    auto load_a = [&](int k0, int tp, int ch) -> bf16x8 {
         bf16x8 av = zero8();
         if (k0 + g * 8 < K) {
-            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+            const int dz = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dx = tp % 3 - 1;
             const int xx = x + dx, yy = y + dy, zz = z + dz;
             if ((unsigned)xx < (unsigned)n && (unsigned)yy < (unsigned)n && (unsigned)zz < (unsigned)n) {
-                if (c < a.c1) {
-                    const long long off = (long long)venc(xx, yy, zz, a.lx) * a.c1 + c;
+                if (ch < a.c1) {
+                    const long long off = (long long)venc(xx, yy, zz, a.lx) * a.c1 + ch;
                     av = ld8(x1 + off);
                     if (mk) av = mask8(av, ld8(mk + off));
                 } else {
                     const long long off =
-                        (long long)venc(xx >> 1, yy >> 1, zz >> 1, a.lx - 1) * a.c2 + (c - a.c1);
+                        (long long)venc(xx >> 1, yy >> 1, zz >> 1, a.lx - 1) * a.c2 + (ch - a.c1);
                     av = ld8(x2 + off);
                 }
             }
         }
-        const bf16 *wp = a.w + (long long)m * a.kpad + k0 + g * 8;
+        return av;
+    };
+    const bf16 *wp = a.w + (long long)(ns * NB * 16 + m) * a.kpad + g * 8;
+    if (kb < ke) {
+        bf16x8 av = load_a(kb, tap, c), bv[NB];
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) acc[nb] = mfma(av, ld8(wp + (long long)nb * 16 * a.kpad), acc[nb]);
-        c += 32;
-        while (c >= a.cin) {
-            c -= a.cin;
-            ++tap;
+        for (int nb = 0; nb < NB; ++nb) bv[nb] = ld8(wp + (long long)nb * 16 * a.kpad + kb);
+        for (int k0 = kb; k0 < ke; k0 += 32) {
+            c += 32;
+            while (c >= a.cin) {
+                c -= a.cin;
+                ++tap;
+            }
+            bf16x8 an = zero8(), bn[NB];
+            const bool more = k0 + 32 < ke;
+            if (more) an = load_a(k0 + 32, tap, c);
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+                bn[nb] = more ? ld8(wp + (long long)nb * 16 * a.kpad + k0 + 32) : zero8();
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) acc[nb] = mfma(av, bv[nb], acc[nb]);
+            av = an;
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) bv[nb] = bn[nb];
         }
     }
     const int yc2 = a.cout - a.ysplit;
+    const long long rows = (long long)a.batch * nvox;
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-        const int ch = nb * 16 + m;
+        const int ch = (ns * NB + nb) * 16 + m;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const long long vo = b * nvox + vbase + 4 * g + j;
             float val = acc[nb][j];
+            if (a.ksplit > 1) {
+                a.part[((long long)ks * rows + vo) * a.cout + ch] = val;
+                continue;
+            }
             if (a.relu) val = fmaxf(val, 0.0f);
             if (ch < a.ysplit) a.y1[vo * a.ysplit + ch] = (bf16)val;
             else a.y2[vo * yc2 + (ch - a.ysplit)] = (bf16)val;
         }
+    }
+}
+
+// Sum of a K-split convolution's partial sums in K-range order, ReLU, BF16.
+__global__ void __launch_bounds__(256) k_conv_reduce(ConvArgs a, long long rows) {
+    const long long total = rows * a.cout;
+    const int yc2 = a.cout - a.ysplit;
+    for (long long e = blockIdx.x * 256ll + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+        float s = 0.0f;
+        for (int ks = 0; ks < a.ksplit; ++ks) s += a.part[(long long)ks * total + e];
+        if (a.relu) s = fmaxf(s, 0.0f);
+        const long long vo = e / a.cout;
+        const int ch = (int)(e - vo * a.cout);
+        if (ch < a.ysplit) a.y1[vo * a.ysplit + ch] = (bf16)s;
+        else a.y2[vo * yc2 + (ch - a.ysplit)] = (bf16)s;
+    }
+}
+
+// The first layer's convolution of the one-channel input (stored with a
+// channel stride of 16, channels 1..15 zero): K = 27 taps (one 32-wide MFMA
+// step) instead of 27 x 16.  Lane group g takes taps 8 g .. 8 g + 7, whose
+// weights sit at k = tap * 16 of the layer's [cout][K padded] rows.
+__global__ void __launch_bounds__(256) k_conv_in1(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    const int b = blockIdx.y;
+    const int n = 1 << a.lx;
+    const long long nvox = 1ll << (3 * a.lx);
+    const int vbase = (blockIdx.x * 4 + wave) * 16;
+    int x, y, z;
+    vdec(vbase + m, a.lx, x, y, z);
+    const bf16 *x1 = a.x1 + b * nvox * a.c1;
+    bf16x8 av = zero8(), bv = zero8();
+    const bf16 *wp = a.w + (long long)m * a.kpad;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int tp = 8 * g + j;
+        if (tp < 27) {
+            const int dz = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dx = tp % 3 - 1;
+            const int xx = x + dx, yy = y + dy, zz = z + dz;
+            if ((unsigned)xx < (unsigned)n && (unsigned)yy < (unsigned)n && (unsigned)zz < (unsigned)n)
+                av[j] = x1[(long long)venc(xx, yy, zz, a.lx) * a.c1];
+            bv[j] = wp[tp * a.cin];
+        }
+    }
+    const f32x4 acc = mfma(av, bv, f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const long long vo = b * nvox + vbase + 4 * g + j;
+        float val = acc[j];
+        if (a.relu) val = fmaxf(val, 0.0f);
+        a.y1[vo * a.cout + m] = (bf16)val;
     }
 }
 
@@ -544,6 +630,8 @@ struct fl_unet {
     float *part = nullptr, *part2 = nullptr;  // per-sample statistics rows
     float *partial = nullptr, *in = nullptr, *tgt = nullptr, *out = nullptr, *scale = nullptr,
           *loss = nullptr;
+    float *kpart = nullptr;  // K-split partial sums of the coarse levels' convolutions
+    size_t kpart_len = 0;
     float *host_loss = nullptr;  // pinned
     int wgroups = 64;
     long long step = 0;
@@ -564,7 +652,7 @@ void ufree(fl_unet *u) {
         if (u->grad[i]) (void)hipFree(u->grad[i]);
     }
     void *bufs[] = {u->tmp, u->w, u->m1, u->m2, u->gr, u->wb, u->wt, u->partial,
-                    u->in,  u->tgt, u->out, u->scale, u->loss, u->part, u->part2};
+                    u->in,  u->tgt, u->out, u->scale, u->loss, u->part, u->part2, u->kpart};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (u->host_loss) (void)hipHostFree(u->host_loss);
@@ -602,10 +690,42 @@ void launch_loss(fl_unet *u, hipStream_t stm, double diag, int B) {
     k_loss_fold<<<1, 64, 0, stm>>>(u->part2, nv, B, u->loss);
 }
 
-fl_status launch_conv(fl_unet *u, hipStream_t stm, int B, const ConvArgs &a) {
+// Work split of a convolution with `cout` outputs over B * nvox voxels and
+// kpad K: whole output-channel groups per wave while that fills the chip
+// (8192 waves), else one 16-channel block per wave and, below 4096 waves,
+// K ranges of at least two MFMA steps.
+struct ConvSplit {
+    int nb, nsplit, ksplit;
+};
+ConvSplit conv_split(long long nvox, int B, int cout, int kpad) {
+    const long long base = B * nvox / 16;
+    const int blocks = cout / 16, steps = kpad / 32;
+    ConvSplit sp{blocks, 1, 1};
+    if (base * 1 >= 8192 || blocks == 1) {
+        sp = ConvSplit{blocks, 1, 1};
+    } else {
+        sp = ConvSplit{1, blocks, 1};
+    }
+    const long long waves = base * sp.nsplit;
+    if (waves < 4096) {
+        const long long want = (8192 + waves - 1) / waves;
+        sp.ksplit = (int)std::max<long long>(1, std::min<long long>(want, steps / 2));
+    }
+    return sp;
+}
+
+fl_status launch_conv(fl_unet *u, hipStream_t stm, int B, ConvArgs a) {
     const long long nvox = 1ll << (3 * a.lx);
-    const dim3 grid((unsigned)(nvox / 64), (unsigned)B);
-    switch (a.cout / 16) {
+    const ConvSplit sp = conv_split(nvox, B, a.cout, a.kpad);
+    a.nsplit = sp.nsplit;
+    a.ksplit = sp.ksplit;
+    a.batch = B;
+    a.part = u->kpart;
+    const long long rows = (long long)B * nvox;
+    if (sp.ksplit > 1 && (size_t)(sp.ksplit * rows * a.cout) > u->kpart_len)
+        return set_error(u->ctx, FL_ERR_INVALID_ARGUMENT, "K-split buffer too small");
+    const dim3 grid((unsigned)(nvox / 64 * sp.nsplit * sp.ksplit), (unsigned)B);
+    switch (sp.nb) {
         case 1: k_conv<1><<<grid, 256, 0, stm>>>(a); break;
         case 2: k_conv<2><<<grid, 256, 0, stm>>>(a); break;
         case 3: k_conv<3><<<grid, 256, 0, stm>>>(a); break;
@@ -614,6 +734,10 @@ fl_status launch_conv(fl_unet *u, hipStream_t stm, int B, const ConvArgs &a) {
         case 6: k_conv<6><<<grid, 256, 0, stm>>>(a); break;
         case 7: k_conv<7><<<grid, 256, 0, stm>>>(a); break;
         default: return set_error(u->ctx, FL_ERR_INVALID_ARGUMENT, "unsupported channel count %d", a.cout);
+    }
+    if (sp.ksplit > 1) {
+        const long long total = rows * a.cout;
+        k_conv_reduce<<<(unsigned)std::min<long long>((total + 255) / 256, 8192), 256, 0, stm>>>(a, rows);
     }
     FL_HIP_TRY(u->ctx, hipGetLastError());
     return FL_OK;
@@ -650,6 +774,12 @@ fl_status forward(fl_unet *u, hipStream_t stm, int B) {
         a.cout = L.cout;
         a.ysplit = L.cout;
         a.relu = ls.relu;
+        if (op == 0) {  // the one-channel input: 27 taps, not 27 x 16
+            a.batch = B;
+            k_conv_in1<<<dim3((unsigned)(nvox_at(u, 0) / 64), (unsigned)B), 256, 0, stm>>>(a);
+            FL_HIP_TRY(u->ctx, hipGetLastError());
+            continue;
+        }
         fl_status st = launch_conv(u, stm, B, a);
         if (st != FL_OK) return st;
     }
@@ -840,6 +970,16 @@ fl_status fl_unet_create(fl_ctx *ctx, const fl_unet_config *cfg, fl_unet **out) 
     u->P = P;
     u->Pt = Pt;
     const size_t B = (size_t)u->maxb;
+    // K-split buffer: the largest split forward or data-gradient convolution
+    // over every batch size up to max_batch
+    for (int bb = 1; bb <= u->maxb; ++bb)
+        for (int l = 0; l < kLayers; ++l) {
+            const fl_unet::L &L = u->layer[l];
+            const long long nv = 1ll << (3 * L.lx);
+            const ConvSplit f = conv_split(nv, bb, L.cout, L.kpad), t = conv_split(nv, bb, L.cin, L.kpad_t);
+            if (f.ksplit > 1) u->kpart_len = std::max(u->kpart_len, (size_t)(f.ksplit * bb * nv * L.cout));
+            if (t.ksplit > 1) u->kpart_len = std::max(u->kpart_len, (size_t)(t.ksplit * bb * nv * L.cin));
+        }
     for (int i = 0; i < kActs && st == FL_OK; ++i) {
         const size_t bytes = B * (size_t)nvox_at(u, kActSpec[i].level) * kActSpec[i].ch * sizeof(bf16);
         if ((st = ualloc(u, (void **)&u->act[i], bytes)) == FL_OK)
@@ -857,7 +997,8 @@ fl_status fl_unet_create(fl_ctx *ctx, const fl_unet_config *cfg, fl_unet **out) 
          (st = ualloc(u, (void **)&u->out, B * nv0 * 4)) ||
          (st = ualloc(u, (void **)&u->scale, B * 4)) || (st = ualloc(u, (void **)&u->loss, B * 4)) ||
          (st = ualloc(u, (void **)&u->part, B * kSG * 2 * 4)) ||
-         (st = ualloc(u, (void **)&u->part2, B * kSG * 2 * 4))))
+         (st = ualloc(u, (void **)&u->part2, B * kSG * 2 * 4)) ||
+         (st = ualloc(u, (void **)&u->kpart, std::max<size_t>(u->kpart_len, 1) * 4))))
         ;
     if (st == FL_OK && hipHostMalloc(&u->host_loss, B * sizeof(float), hipHostMallocDefault) != hipSuccess)
         st = set_error(ctx, FL_ERR_OUT_OF_MEMORY, "pinned host allocation failed");

@@ -23,6 +23,8 @@ from ._lib import (GUESS_PREVIOUS, STOP_ABS_INF, Grid2DConfig, Grid2DStepReport,
 FIRST_ORDER_UPWIND, CENTRAL = 0, 1
 BC_VELOCITY, BC_SYMMETRY, BC_PRESSURE_OUTLET = 0, 1, 2
 LEFT, RIGHT, BOTTOM, TOP = 0, 1, 2, 3
+INLET_UNIFORM, INLET_PARABOLIC = 0, 1
+SOLVER_PCG, SOLVER_SOR = _lib.SOLVER_PCG, _lib.SOLVER_SOR
 
 
 @dataclass
@@ -66,6 +68,14 @@ class SimulationConfig:
     # (FP32 CG cycles inside FP64 iterative refinement)
     precision: int = _lib.PRECISION_FP64
     inner_tolerance: float = 0.0
+    # PressureSolverKind: SOLVER_PCG (default) or SOLVER_SOR with pressure_omega
+    pressure_solver: int = _lib.SOLVER_PCG
+    pressure_omega: float = 1.7
+    # left Velocity side: INLET_UNIFORM (bc_u) or INLET_PARABOLIC (the Channel /
+    # BackwardFacingStep inlet profile, mean bc_u[LEFT], zero below
+    # inlet_step_height)
+    inlet_profile: int = 0
+    inlet_step_height: float = 0.0
 
     @staticmethod
     def cavity(n, reynolds=100.0, lid_velocity=1.0, **kw):
@@ -92,6 +102,8 @@ class SimulationConfig:
         c.target_diffusion_number, c.max_growth_factor = (ts.target_diffusion_number,
                                                           ts.max_growth_factor)
         c.precision, c.inner_tolerance = self.precision, self.inner_tolerance
+        c.pressure_solver, c.pressure_omega = self.pressure_solver, self.pressure_omega
+        c.inlet_profile, c.inlet_step_height = self.inlet_profile, self.inlet_step_height
         return c
 
 

@@ -50,8 +50,12 @@ enum { FL_STOP_ABS_INF = 0, FL_STOP_REL_L2 = 1 };
 enum { FL_GUESS_PREVIOUS = 0, FL_GUESS_ZERO = 1 };
 /* Pressure solver.  PCG is the reference's Jacobi-PCG; on the 3D cavity it is
  * the same algorithm on the pinned 7-point operator (the reference 3D solver
- * only has SOR, incompressible_3d.rs:293-332). */
-enum { FL_SOLVER_PCG = 0 };
+ * only has SOR, incompressible_3d.rs:293-332).  SOR is PressureSolverKind::Sor
+ * (2D incompressible.rs:979-1046, 3D incompressible_3d.rs:293-332) in
+ * red-black order: the same update, relaxation factor, gauge and max-residual
+ * stop test after every sweep, reaching the same fixed point as the
+ * reference's lexicographic sweep (iterates and counts differ; FP64 only). */
+enum { FL_SOLVER_PCG = 0, FL_SOLVER_SOR = 1 };
 /* Arithmetic of the pressure solve: FP64 throughout, or an FP32 stencil CG
  * inside FP64 iterative refinement (FP64 residual, dots, alpha/beta, test). */
 enum { FL_PRECISION_FP64 = 0, FL_PRECISION_MIXED = 1 };
@@ -61,8 +65,9 @@ enum { FL_PRECISION_FP64 = 0, FL_PRECISION_MIXED = 1 };
  * level, piecewise-constant transfers with a 2.0 coarse-correction factor);
  * when an inner solve exceeds its budget or stagnates, the solve falls back
  * to plain FP64 Jacobi-PCG from the current iterate (counted in the report).
- * LEARNED (no reference counterpart) is MULTIGRID started from the warm
- * start of a U-Net attached with fl_cavity3d_set_unet. */
+ * LEARNED (no reference counterpart) is MULTIGRID with a U-Net attached
+ * with fl_cavity3d_set_unet as the V-cycle's coarse solve (or, where the grid
+ * has the network's resolution, as a warm start). */
 enum { FL_PRECOND_JACOBI = 0, FL_PRECOND_MULTIGRID = 1, FL_PRECOND_LEARNED = 2 };
 
 typedef struct fl_ctx fl_ctx;
@@ -118,6 +123,9 @@ typedef struct {
      * (0 selects 1e-5 for Jacobi-CG, 1e-4 for multigrid FCG).
      * pressure_max_iters caps the inner total. */
     double inner_tolerance;
+    int32_t pressure_solver;       /* FL_SOLVER_* (SOR: FP64, Jacobi, AbsInf rule) */
+    int32_t reserved;
+    double pressure_omega;         /* SOR relaxation factor in (0, 2); reference default 1.7 */
 } fl_cavity3d_config;
 
 typedef struct {
@@ -217,7 +225,21 @@ typedef struct {
     /* FL_PRECISION_MIXED: relative residual drop that ends an inner cycle
      * (0 selects 1e-5).  pressure_max_iters caps the inner total. */
     double inner_tolerance;
+    /* PressureSolverKind (incompressible.rs:20-24): FL_SOLVER_PCG, or
+     * FL_SOLVER_SOR (FP64, the max-residual rule, relaxation pressure_omega in
+     * (0, 2); single solvers and host-stepped batches). */
+    int32_t pressure_solver;
+    /* Normal velocity of a left Velocity side: FL_INLET_UNIFORM, u = bc_u[LEFT];
+     * FL_INLET_PARABOLIC, u(y) = 6 ubar eta (1 - eta) with eta = (y - s) /
+     * (H - s) clamped to [0, 1], u = 0 for y <= s, ubar = bc_u[LEFT] and s =
+     * inlet_step_height -- Case::boundary_velocity's Channel (s = 0) and
+     * BackwardFacingStep inlets (src/cases/mod.rs:141-170), which apply when
+     * the side has no velocity override. */
+    int32_t inlet_profile;
+    double pressure_omega;
+    double inlet_step_height;
 } fl_grid2d_config;
+enum { FL_INLET_UNIFORM = 0, FL_INLET_PARABOLIC = 1 };
 
 /* SolverStep (incompressible.rs:196-214), flow-only fields. */
 typedef struct {
@@ -437,14 +459,27 @@ fl_status fl_unet_grads(fl_unet *net, int32_t batch, const float *rhs, const flo
 fl_status fl_probe_mfma_bf16(fl_ctx *ctx, const float *a, const float *b, float *d);
 /* Attaches `net` (may be NULL) to a solver created with FL_PRECOND_LEARNED:
  * the grid must be a cube of n * 2^L cells per axis with L a multigrid level
- * that keeps the full domain.  The network must outlive its use by the
- * solver and must not be used concurrently. */
+ * that keeps the full domain.  With L >= 1 the network is the coarse solve
+ * of every V-cycle of the flexible CG (levels L and below are replaced by
+ * its prediction of A_L^-1 f_L and two damped-Jacobi sweeps on level L; the
+ * first FCG step is then the learned warm start); with L = 0 it gives a
+ * separate warm start ahead of the multigrid FCG.  The network must outlive
+ * its use by the solver and must not be used concurrently. */
 fl_status fl_cavity3d_set_unet(fl_cavity3d *solver, fl_unet *net);
 /* Training pairs: the last step's pressure right-hand side b = -rhs and
  * pressure p, mean-restricted to n^3 (n = nx / 2^L); `diag` receives the
  * Laplacian diagonal at that resolution. */
 fl_status fl_cavity3d_coarse_pair(fl_cavity3d *solver, int32_t n, float *rhs_c, float *p_c,
                                   double *diag);
+/* Training utility (no reference counterpart): with n > 0, each later
+ * multigrid or learned pressure solve copies the right-hand side that
+ * V-cycle number `iteration` (0-based) of its first inner FCG solve receives
+ * on the multigrid level of n^3 cells (a level below the finest) aside;
+ * n = 0 stops the capture.  fl_cavity3d_captured_rhs returns the last copy
+ * (n^3 floats, x fastest) and the rediscretised Laplacian diagonal at that
+ * resolution, the inputs of coarse-solve training pairs. */
+fl_status fl_cavity3d_capture_coarse(fl_cavity3d *solver, int32_t n, int32_t iteration);
+fl_status fl_cavity3d_captured_rhs(fl_cavity3d *solver, float *rhs_c, double *diag);
 
 #ifdef __cplusplus
 }

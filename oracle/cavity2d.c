@@ -117,8 +117,17 @@ void or2d_apply_velocity_boundaries(or2d_state *s, double *u, double *v) {
     const int64_t nx = s->cfg.nx, ny = s->cfg.ny;
     const int32_t *kind = s->cfg.bc_kind;
     for (int64_t j = 0; j < ny; ++j) {
+        /* Case::boundary_velocity (src/cases/mod.rs:141-170): a Channel or
+         * BackwardFacingStep inlet is parabolic in y = u_face_y(j) */
+        double ul = s->cfg.bc_u[OR_SIDE_LEFT];
+        if (s->cfg.inlet_profile == 1) {
+            const double y = ((double)j + 0.5) * s->dy, sh = s->cfg.inlet_step_height;
+            double eta = (y - sh) / (s->cfg.height - sh);
+            eta = eta < 0.0 ? 0.0 : (eta > 1.0 ? 1.0 : eta);
+            ul = y <= sh ? 0.0 : 6.0 * s->cfg.bc_u[OR_SIDE_LEFT] * eta * (1.0 - eta);
+        }
         switch (kind[OR_SIDE_LEFT]) {
-            case OR_BC_VELOCITY: u[UI(0, j)] = s->cfg.bc_u[OR_SIDE_LEFT]; break;
+            case OR_BC_VELOCITY: u[UI(0, j)] = ul; break;
             case OR_BC_SYMMETRY: u[UI(0, j)] = 0.0; break;
             default: u[UI(0, j)] = u[UI(1, j)]; break;
         }

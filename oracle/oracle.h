@@ -114,6 +114,11 @@ typedef struct {
     int32_t adaptive;
     double min_dt, max_dt, target_cfl, target_diffusion, max_growth;
     double t_end;
+    /* left Velocity side: 0 uniform bc_u, 1 the parabolic Channel /
+     * BackwardFacingStep inlet (src/cases/mod.rs:141-170) of mean bc_u[LEFT]
+     * above inlet_step_height */
+    int32_t inlet_profile;
+    double inlet_step_height;
 } or2d_config;
 
 typedef struct {

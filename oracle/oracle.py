@@ -87,6 +87,7 @@ class Config2D(C.Structure):
         ("min_dt", C.c_double), ("max_dt", C.c_double), ("target_cfl", C.c_double),
         ("target_diffusion", C.c_double), ("max_growth", C.c_double),
         ("t_end", C.c_double),
+        ("inlet_profile", C.c_int32), ("inlet_step_height", C.c_double),
     ]
 
 
@@ -301,7 +302,8 @@ class Solver2D:
                  stop_rule=STOP_ABS_INF, initial_guess=GUESS_PREVIOUS, threads=0,
                  bc_kind=(BC_VELOCITY,) * 4, bc_u=(0.0, 0.0, 0.0, 1.0), bc_v=(0.0,) * 4,
                  outlet_pressure=(0.0,) * 4, adaptive=False, min_dt=1.0e-6, max_dt=1.0e-2,
-                 target_cfl=0.5, target_diffusion=0.25, max_growth=1.1, solid=None):
+                 target_cfl=0.5, target_diffusion=0.25, max_growth=1.1, solid=None,
+                 inlet_profile=0, inlet_step_height=0.0):
         ny = nx if ny is None else ny
         cfg = Config2D()
         cfg.nx, cfg.ny, cfg.length, cfg.height = nx, ny, length, height
@@ -317,6 +319,7 @@ class Solver2D:
         cfg.adaptive = int(adaptive)
         cfg.min_dt, cfg.max_dt, cfg.target_cfl = min_dt, max_dt, target_cfl
         cfg.target_diffusion, cfg.max_growth, cfg.t_end = target_diffusion, max_growth, t_end
+        cfg.inlet_profile, cfg.inlet_step_height = inlet_profile, inlet_step_height
         self.cfg = cfg
         self.s = State2D()
         self._solid = None

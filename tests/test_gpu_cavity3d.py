@@ -128,6 +128,57 @@ def test_gpu_matches_reference_sor_solution(gpu_ctx):
     assert rel(f["p"], o.field("p")) < 1e-8
 
 
+@pytest.mark.parametrize("key", ["8x8x8", "12x10x14"])
+def test_sor_matches_dct_and_reference_sor(gpu_ctx, key):
+    """PressureSolverKind::Sor on the GPU (red-black order) reaches the
+    reference solver's fixed point: the DCT closed form of the pinned
+    system, and the oracle's lexicographic SOR at the same tolerance (the two
+    orders differ in their iterates and counts, not in the solution).  Bar:
+    1e-9 of max|p| at max-residual tolerance 1e-11."""
+    d = np.load(os.path.join(GOLD, "neumann_dct.npz"))
+    nz, ny, nx = map(int, key.split("x"))
+    print(f"[gpu] SOR pressure solve {key}", flush=True)
+    rhs = d[f"rhs3d_{key}"]
+    s = make(gpu_ctx, nx, ny, nz, pressure_solver=fa.SOLVER_SOR, pressure_omega=1.7,
+             pressure_tolerance=1e-11, pressure_max_iters=50000)
+    p = np.zeros((nz, ny, nx))
+    rep = s.pressure_solve(rhs, p)
+    assert rep.converged and rep.residual_abs_inf < 1e-11
+    assert rel(p, d[f"p3d_{key}"]) < 1e-9
+    assert p[0, 0, 0] == 0.0
+    o = O.Cavity3D(nx, ny, nz, solver=O.SOLVER_SOR, pressure_omega=1.7, pressure_tolerance=1e-11,
+                   pressure_max_iters=50000)
+    o.field("rhs")[:] = rhs
+    orep = o.solve_pressure()
+    assert orep.converged
+    assert rel(p, o.field("p")) < 1e-9
+    print(f"[gpu] SOR sweeps: red-black {rep.iterations}, lexicographic {orep.iterations}", flush=True)
+
+
+def test_sor_steps_match_reference_sor(gpu_ctx):
+    """Two cavity steps (tests/smoke.rs:113-130 setting, 8^3) with SOR on both
+    sides to max residual 1e-10: fields agree to 1e-8 of their scale."""
+    kw = dict(dt=5e-4, pressure_tolerance=1e-10, pressure_max_iters=50000)
+    o = O.Cavity3D(8, solver=O.SOLVER_SOR, pressure_omega=1.7, **kw)
+    s = make(gpu_ctx, 8, 8, 8, pressure_solver=fa.SOLVER_SOR, pressure_omega=1.7, **kw)
+    for _ in range(2):
+        o.step()
+        rep = s.advance_one_step()
+        assert rep.pressure_converged and rep.max_divergence < 1e-5
+    f = s.fields()
+    uscale = max(np.abs(o.field(c)).max() for c in ("u", "v", "w"))
+    for name in ("u", "v", "w"):
+        assert np.abs(f[name] - o.field(name)).max() < 1e-8 * uscale, name
+    assert rel(f["p"], o.field("p")) < 1e-8
+
+
+def test_sor_config_errors(gpu_ctx):
+    with pytest.raises(fa.FlursysError, match="pressure_omega must lie between 0 and 2"):
+        make(gpu_ctx, 8, 8, 8, pressure_solver=fa.SOLVER_SOR, pressure_omega=2.0)
+    with pytest.raises(fa.FlursysError, match="max-residual stop rule"):
+        make(gpu_ctx, 8, 8, 8, pressure_solver=fa.SOLVER_SOR, stop_rule=fa.STOP_REL_L2)
+
+
 def test_repeatable_bitwise(gpu_ctx):
     out = []
     for _ in range(2):

@@ -181,12 +181,25 @@ def test_mesh_systems_match_oracle(gpu_ctx, m, kind):
     assert rg.converged() and ro.converged
     # The device dots sum in a fixed tree, the oracle (like the reference's
     # serial linear/ module) left to right.  On the small mesh the counts
-    # agree to one iteration; on the 20000-cell mesh CG and BiCGSTAB run
-    # hundreds of iterations, where round-off in the recurrences shifts the
-    # count by a few per cent, so the bar there is the count within 5% and
-    # the solution itself (true residual and agreement with the oracle's x).
-    slack = 1 if m <= 20 else max(1, int(0.05 * ro.iterations))
-    assert abs(rg.iterations - ro.iterations) <= slack, (rg.iterations, ro.iterations)
+    # agree to one iteration; on the 20000-cell mesh CG runs ~1300
+    # iterations, where round-off in the recurrences shifts the count by a
+    # few per cent, so the bar there is the count within 5% and the solution
+    # itself (true residual and agreement with the oracle's x).  BiCGSTAB's
+    # count on the skew system is chaotic in round-off: the oracle itself
+    # needs 303-397 iterations when the right-hand side is perturbed by
+    # 1e-15 relative, so its bar is the spread of the oracle's own counts
+    # under such perturbations (widened by 5%).
+    if m > 20 and kind == "bicgstab":
+        rng = np.random.default_rng(0)
+        its = [ro.iterations]
+        for _ in range(5):
+            xp = np.zeros(n)
+            its.append(O.bicgstab(Ao, rhs * (1 + 1e-15 * rng.standard_normal(n)), xp, 0.0,
+                                  1e-10, 20000).iterations)
+        assert 0.95 * min(its) <= rg.iterations <= 1.05 * max(its), (rg.iterations, its)
+    else:
+        slack = 1 if m <= 20 else max(1, int(0.05 * ro.iterations))
+        assert abs(rg.iterations - ro.iterations) <= slack, (rg.iterations, ro.iterations)
     B = sp.csr_matrix((Ao.vals, Ao.cols, Ao.offsets), shape=(n, n))
     assert np.linalg.norm(rhs - B @ x) <= 5e-10 * np.linalg.norm(rhs)
     xbar = 1e-8 if m <= 20 else 1e-6

@@ -28,6 +28,9 @@ def rel(a, b):
 def oracle_for(cfg: I.SimulationConfig, solid=None, threads=0):
     ts = cfg.time_step
     return O.Solver2D(cfg.nx, cfg.ny, length=cfg.length, height=cfg.height,
+                      solver=O.SOLVER_SOR if cfg.pressure_solver == I.SOLVER_SOR else O.SOLVER_PCG,
+                      pressure_omega=cfg.pressure_omega, inlet_profile=cfg.inlet_profile,
+                      inlet_step_height=cfg.inlet_step_height,
                       density=cfg.density, nu=cfg.kinematic_viscosity, dt=cfg.dt,
                       t_end=cfg.t_end, pressure_max_iters=cfg.pressure_max_iters,
                       pressure_tolerance=cfg.pressure_tolerance, convection=cfg.convection,
@@ -408,3 +411,124 @@ def test_run_step_and_pcg_times(gpu_ctx):
     assert np.all(ms > 0)
     if b.graph_mode() == fa._lib.GRAPH_WHOLE_STEP:
         assert np.all(pms == -1.0)
+
+
+# ------------------------------------------------------------------ SOR
+@pytest.mark.parametrize("case", ["cavity", "obstacle", "channel"])
+def test_sor_pressure_solve_matches_oracle_sor(gpu_ctx, case):
+    """PressureSolverKind::Sor (incompressible.rs:979-1046) on the GPU in
+    red-black order against the oracle's lexicographic SOR, both to max
+    residual 1e-11: the same fixed point (the first fluid cell's gauge, or
+    the outlet pressure) to 1e-8 of max|p|.  Sweep counts differ by order."""
+    nx, ny = 24, 20
+    solid = block_mask(nx, ny) if case == "obstacle" else None
+    kw = dict(pressure_solver=I.SOLVER_SOR, pressure_omega=1.7, pressure_tolerance=1e-11,
+              pressure_max_iters=50000)
+    cfg = channel_cfg(nx, ny, **kw) if case == "channel" else I.SimulationConfig(nx=nx, ny=ny, **kw)
+    print(f"[gpu] 2D SOR {case}", flush=True)
+    rng = np.random.default_rng(3)
+    rhs = rng.standard_normal((ny, nx)) * 20.0
+    if case != "channel":
+        fluid = np.ones((ny, nx), bool) if solid is None else solid == 0
+        rhs[fluid] -= rhs[fluid].mean()  # compatible Neumann right-hand side
+    if solid is not None:
+        rhs[solid == 1] = 0.0
+    o = oracle_for(cfg, solid)
+    o.field("rhs")[:] = rhs
+    orep = o.solve_pressure()
+    s = I.IncompressibleSolver(cfg, solid=solid, ctx=gpu_ctx)
+    p = np.zeros((ny, nx))
+    rep = s.pressure_solve(rhs, p)
+    assert rep.converged and orep.converged
+    assert rep.residual_abs_inf < 1e-11
+    assert rel(p, o.field("p")) < 1e-8
+    print(f"[gpu] SOR sweeps: red-black {rep.iterations}, lexicographic {orep.iterations}", flush=True)
+
+
+def test_sor_steps_match_oracle(gpu_ctx):
+    """Whole cavity steps (tests/smoke.rs:57-69 setting, 16^2) with SOR to max
+    residual 1e-10 on both sides: fields within 1e-8 of the velocity scale,
+    and the smoke threshold."""
+    cfg = I.SimulationConfig.cavity(16, dt=1e-3, pressure_solver=I.SOLVER_SOR,
+                                    pressure_omega=1.7, pressure_tolerance=1e-10,
+                                    pressure_max_iters=50000)
+    o = oracle_for(cfg)
+    s = I.IncompressibleSolver(cfg, ctx=gpu_ctx)
+    for _ in range(3):
+        od, _ = o.step()
+        r = s.advance()
+        assert r.pressure_converged
+    assert r.continuity_residual < 1e-5
+    f = s.fields()
+    for name in ("u", "v"):
+        assert np.abs(f[name] - o.field(name)).max() < 1e-8, name
+    assert rel(f["p"], o.field("p")) < 1e-7
+
+
+# ---------------------------------------------- Symmetry sides, inlet profile
+@pytest.mark.parametrize("kinds", [
+    (I.BC_SYMMETRY, I.BC_SYMMETRY, I.BC_VELOCITY, I.BC_VELOCITY),
+    (I.BC_VELOCITY, I.BC_PRESSURE_OUTLET, I.BC_SYMMETRY, I.BC_SYMMETRY)])
+def test_symmetry_sides_match_oracle(gpu_ctx, kinds):
+    """Symmetry sides (apply_velocity_boundaries :1996-2023: zero normal face;
+    tangential ghost = centre, :2176, :2197) on the GPU against the oracle:
+    a lid-driven box with symmetric side walls, and a channel with symmetric
+    top and bottom (the CylinderRe100 case's sides, src/cases/mod.rs:121-127).
+    The channel takes the parabolic inlet: with a uniform one, uniform flow is
+    the exact solution, the second step's right-hand side is round-off and
+    the PCG breaks down on both sides."""
+    nx, ny = 28, 20
+    box = kinds[0] == I.BC_SYMMETRY
+    bc_u = (0.0, 0.0, 0.0, 1.0) if box else (1.0, 0.0, 0.0, 0.0)
+    cfg = I.SimulationConfig(nx=nx, ny=ny, length=1.4, height=1.0, bc_kind=kinds, bc_u=bc_u,
+                             inlet_profile=I.INLET_UNIFORM if box else I.INLET_PARABOLIC,
+                             dt=1e-3, pressure_tolerance=1e-8, stop_rule=fa.STOP_REL_L2,
+                             initial_guess=fa.GUESS_ZERO, pressure_max_iters=5000,
+                             convection=I.CENTRAL)
+    o = oracle_for(cfg)
+    s = I.IncompressibleSolver(cfg, ctx=gpu_ctx)
+    for _ in range(4):
+        od, orep = o.step()
+        r = s.advance()
+        assert r.pressure_converged
+        assert abs(r.pressure_iterations - orep.iterations) <= 1
+    f = s.fields()
+    for name in ("u", "v"):
+        assert np.abs(f[name] - o.field(name)).max() < 1e-9, name
+    assert rel(f["p"], o.field("p")) < 1e-9
+    # the symmetric sides' normal faces are zero
+    if kinds[0] == I.BC_SYMMETRY:
+        assert not f["u"][:, 0].any() and not f["u"][:, nx].any()
+    else:
+        assert not f["v"][0].any() and not f["v"][ny].any()
+
+
+@pytest.mark.parametrize("step_h", [0.0, 0.5])
+def test_parabolic_inlet_matches_oracle(gpu_ctx, step_h):
+    """The Channel (step 0) and BackwardFacingStep (step 0.5) inlets of
+    Case::boundary_velocity (src/cases/mod.rs:141-170): u(y) = 6 ubar eta
+    (1 - eta) above the step, 0 below; the inlet faces against the hand
+    formula and whole steps against the oracle."""
+    nx, ny, ubar = 32, 16, 0.8
+    cfg = I.SimulationConfig(nx=nx, ny=ny, length=2.0, height=1.0,
+                             bc_kind=(I.BC_VELOCITY, I.BC_PRESSURE_OUTLET, I.BC_VELOCITY,
+                                      I.BC_VELOCITY),
+                             bc_u=(ubar, 0.0, 0.0, 0.0), inlet_profile=I.INLET_PARABOLIC,
+                             inlet_step_height=step_h, dt=1e-3, pressure_tolerance=1e-8,
+                             stop_rule=fa.STOP_REL_L2, initial_guess=fa.GUESS_ZERO,
+                             pressure_max_iters=5000)
+    o = oracle_for(cfg)
+    s = I.IncompressibleSolver(cfg, ctx=gpu_ctx)
+    y = (np.arange(ny) + 0.5) / ny
+    eta = np.clip((y - step_h) / (1.0 - step_h), 0.0, 1.0)
+    inlet = np.where(y <= step_h, 0.0, 6.0 * ubar * eta * (1.0 - eta))
+    assert np.abs(s.fields()["u"][:, 0] - inlet).max() < 1e-15
+    for _ in range(3):
+        o.step()
+        r = s.advance()
+        assert r.pressure_converged
+    f = s.fields()
+    assert np.abs(f["u"][:, 0] - inlet).max() < 1e-15
+    for name in ("u", "v"):
+        assert np.abs(f[name] - o.field(name)).max() < 1e-9, name
+    assert rel(f["p"], o.field("p")) < 1e-9

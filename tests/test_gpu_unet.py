@@ -201,7 +201,10 @@ def test_train_step_validates_arguments(gpu_ctx):
 @pytest.mark.parametrize("grid,netn", [(32, 32), (64, 32)])
 def test_learned_steps_match_oracle(gpu_ctx, grid, netn):
     """Whatever the (untrained) network predicts, the solve meets the same
-    FP64 stop test and the fields match the oracle."""
+    FP64 stop test and the fields match the oracle: on a grid of the
+    network's resolution as a warm start (kept or undone), on a finer grid as
+    the coarse solve inside the V-cycle (no separate warm start; a cycle that
+    stagnates hands over to the plain FP64 PCG)."""
     print(f"[gpu] learned FCG cavity {grid}^3 with a {netn}^3 U-Net", flush=True)
     kw = dict(reynolds=400.0, dt=5e-4, pressure_tolerance=1e-10, stop_rule=fa.STOP_REL_L2,
               initial_guess=fa.GUESS_ZERO, pressure_max_iters=5000)
@@ -217,7 +220,10 @@ def test_learned_steps_match_oracle(gpu_ctx, grid, netn):
         o.step()
         assert rep.pressure_converged
         states.add(rep.pressure_warm_start)
-    assert states <= {1, -1} and states
+    if grid == netn:
+        assert states <= {1, -1} and states
+    else:
+        assert states == {0}
     f = s.fields()
     uscale = max(np.abs(o.field(c)).max() for c in ("u", "v", "w"))
     for c in ("u", "v", "w"):
@@ -254,3 +260,57 @@ def test_coarse_pairs_are_block_means(gpu_ctx):
     assert rel(b, blocks(-inter)) < 1e-6
     assert rel(p, blocks(pf)) < 1e-6
     assert diag == pytest.approx(6.0 * 32 * 32)
+
+
+def test_trained_network_gain_in_fcg(gpu_ctx):
+    """What the trained network gains: the learned flexible CG at 128^3 on the
+    benchmark's held-out flow (Re 1000) with the committed weights
+    (flursys_amd/weights/unet64_cavity.npy) as the V-cycle's 64^3 coarse
+    solve, against the same network untrained and against the multigrid
+    coarse levels it replaces.  Iterations and ms/step are printed; the
+    trained network must converge every step without the plain-PCG fallback
+    and need fewer FCG iterations than the untrained one."""
+    import os
+    import time
+
+    wpath = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                         "flursys_amd", "weights", "unet64_cavity.npy")
+    kw = dict(nx=128, ny=128, nz=128, reynolds=1000.0, dt=5e-4, pressure_tolerance=1e-8,
+              stop_rule=fa.STOP_REL_L2, initial_guess=fa.GUESS_ZERO, pressure_max_iters=20000,
+              precision=fa.PRECISION_MIXED)
+    out = {}
+    for name in ("trained", "untrained", "multigrid"):
+        learned = name != "multigrid"
+        cfg = fa.LidDrivenCavity3DConfig(
+            preconditioner=fa.PRECOND_LEARNED if learned else fa.PRECOND_MULTIGRID, **kw)
+        s = fa.LidDrivenCavity3DSolver(cfg, gpu_ctx)
+        net = None
+        if learned:
+            net = UNet(64, 1, seed=1, ctx=gpu_ctx)
+            trained = UNet(64, 1, ctx=gpu_ctx)
+            trained.load(wpath)
+            if name == "trained":
+                net.set_params(trained.params())
+            else:  # same architecture and output gain, initial weights
+                net.set_gain(trained.gain)
+            trained.close()
+            s.set_unet(net)
+        s.advance_one_step()  # from rest: warm-up
+        its, fbs = [], []
+        t0 = time.perf_counter()
+        for _ in range(4):
+            rep = s.advance_one_step()
+            assert rep.pressure_converged
+            its.append(rep.pressure_iterations)
+            fbs.append(rep.pressure_fallback)
+        ms = (time.perf_counter() - t0) * 250.0
+        out[name] = (its, fbs, ms)
+        print(f"[gpu] learned FCG 128^3 Re 1000, {name}: FCG iterations {its}, "
+              f"fallbacks {sum(fbs)}, {ms:.2f} ms/step", flush=True)
+        s.close()
+        if net is not None:
+            net.close()
+    its_t, fb_t, _ = out["trained"]
+    its_u, fb_u, _ = out["untrained"]
+    assert sum(fb_t) == 0
+    assert sum(fb_u) > 0 or sum(its_t) < sum(its_u)
