@@ -1463,97 +1463,126 @@ template <typename T> constexpr uint32_t tile_lds_len() {
 // kStep false: partial d'.q into s[0].  kStep true: the step, partials
 // {r'.z', r'.r', sum r' | max|r'|} (z' = D^-1 r', the pinned row z'_0 = r'_0).
 template <typename T, bool kFirst, bool kStep>
-__device__ inline void cg_tile_pass(const Grid &g, const T *inv, T *lds, const T *__restrict__ r,
-                                    const T *__restrict__ dold, T tb, T ta, bool neu, T *__restrict__ x,
-                                    T *__restrict__ rnew, T *__restrict__ dnew, double (&s)[3], double (&m)[1]) {
+__device__ inline void cg_tile_pass(const Grid &g, T *lds, const T *__restrict__ r, const T *__restrict__ dold,
+                                    T tb, T ta, bool neu, T *__restrict__ x, T *__restrict__ rnew,
+                                    T *__restrict__ dnew, double (&s)[3], double (&m)[1]) {
     constexpr int V = 16 / sizeof(T);
-    constexpr int W = 64 * V, TY = kTileRows, OFF = V, S = W + 2 * V;
+    constexpr int W = 64 * V, TY = kTileRows, OFF = V, S = W + 2 * V, BUF = (TY + 2) * S;
     typedef typename Vec16<T>::type TV;
     typedef Lanes<T> RV;
-    auto slot = [&](int bf, int row, int col) -> T * { return lds + ((size_t)bf * (TY + 2) + row) * S + col; };
     const T cx = (T)g.ix2, cy = (T)g.iy2, cz = (T)g.iz2;
     const uint3 b = zblock();
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint32_t nx = g.nx, ny = g.ny, nz = g.nz;
-    const uint32_t tx0 = b.x * W, ty0 = b.y * TY, i0 = tx0 + lane * V;
+    const uint32_t nx = g.nx, ny = g.ny, nz = g.nz, nxy = nx * ny;
+    const uint32_t tx0 = b.x * W, ty0 = b.y * TY, i0 = tx0 + lane * V, j = ty0 + wv;
     const uint32_t kz = (nz + gridDim.z - 1) / gridDim.z, k0 = b.z * kz;
     if (k0 >= nz) return;  // block-uniform; the caller's reduction follows
     const uint32_t k1 = min(k0 + kz, nz);
-    const uint32_t j = ty0 + wv;
     const bool own = i0 < nx && j < ny;
-    // halo rows: wave 0 loads the row below the tile, wave 1 the row above
+    // halo rows: wave 0 forms the row below the tile, wave 1 the row above
     const uint32_t jh = wv == 0 ? ty0 - 1 : ty0 + TY;
     const bool hrow = wv < 2 && i0 < nx && (wv == 0 ? ty0 > 0 : ty0 + TY < ny);
-    const int hslot = wv == 0 ? 0 : TY + 1;
     // halo columns: thread t < TY the cell west of row t, TY <= t < 2 TY east of row t - TY
     const uint32_t t = threadIdx.x;
     const bool east = t >= TY;
     const uint32_t jx = ty0 + (east ? t - TY : t), ix = east ? tx0 + W : tx0 - 1;
     const bool xh = t < 2 * TY && jx < ny && (east ? tx0 + W < nx : tx0 > 0);
-    const int xslot = east ? OFF + W : OFF - 1, xrow = 1 + (int)(east ? t - TY : t);
-    auto cell = [&](uint32_t jj, uint32_t k) { return i0 + nx * (jj + ny * k); };
-    auto dprime = [&](T rv, T dv, uint32_t c, uint32_t i, uint32_t jj, uint32_t k) -> T {
-        const T z = (c == 0 && !neu) ? rv : rv * inv[diag_index(g, i, jj, k)];
-        return kFirst ? z : z + tb * dv;
+    // Per-lane constants of the march.  Plane-0 offsets of the three loads:
+    // a lane with nothing to load reads its own (or cell 0's) values instead,
+    // so every lane issues the same loads and the compiler's wait counts on
+    // them are static.
+    const uint32_t co = own ? i0 + nx * j : 0u, ch = hrow ? i0 + nx * jh : co, cxh = xh ? ix + nx * jx : co;
+    // reciprocal diagonals: [0] on an interior plane, [1] on a wall plane
+    auto inv_of = [&](uint32_t i, uint32_t jj, int wall) -> T {
+        const uint32_t a = (uint32_t)(i > 0 && i + 1 < nx) | ((uint32_t)(jj > 0 && jj + 1 < ny) << 1) |
+                           ((uint32_t)(!wall) << 2);
+        return inv_diag<T>(g, a);
     };
-    auto form_vec = [&](const TV &rv, const TV &dv, uint32_t c, uint32_t jj, uint32_t k) -> RV {
-        RV o;
-        const T *rp = (const T *)&rv, *dp = (const T *)&dv;
+    T io[2][V], ih[2][V], ixv[2];
 #pragma unroll
-        for (int v = 0; v < V; ++v) o.a[v] = dprime(rp[v], kFirst ? (T)0 : dp[v], c + v, i0 + v, jj, k);
-        return o;
-    };
+    for (int zw = 0; zw < 2; ++zw) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            io[zw][v] = inv_of(i0 + v, j, zw);
+            ih[zw][v] = inv_of(i0 + v, jh, zw);
+        }
+        ixv[zw] = inv_of(ix, jx, zw);
+    }
+    auto wall_plane = [&](uint32_t k) { return (k == 0 || k + 1 == nz) ? 1 : 0; };
+    const bool pinlane = !neu && i0 == 0 && j == 0;  // owns cell 0 (plane 0, v = 0)
+    T *const own_slot = lds + (1 + wv) * S + OFF + lane * V;
+    T *const h_slot = lds + (wv == 0 ? 0 : TY + 1) * S + OFF + lane * V;
+    T *const x_slot = lds + (1 + (east ? t - TY : t)) * S + (east ? OFF + W : OFF - 1);
+    const bool wl = i0 == 0, el = i0 + V == nx, sl = j == 0, nl = j + 1 == ny;
     TV rr{}, dd{}, xx{}, hr{}, hd{};
     T xr = 0, xd = 0;
     auto load = [&](uint32_t k) {
-        if (own) {
-            const uint32_t c = cell(j, k);
-            rr = *(const TV *)(r + c);
-            if (!kFirst) dd = *(const TV *)(dold + c);
-            if (kStep) xx = *(const TV *)(x + c);
+        const uint32_t pk = nxy * k;
+        const uint32_t o = own ? pk : 0u, h = hrow ? pk : o, q = xh ? pk : o;
+        rr = *(const TV *)(r + co + o);
+        if (!kFirst) dd = *(const TV *)(dold + co + o);
+        if (kStep) xx = *(const TV *)(x + co + o);
+        hr = *(const TV *)(r + ch + h);
+        if (!kFirst) hd = *(const TV *)(dold + ch + h);
+        xr = r[cxh + q];
+        if (!kFirst) xd = dold[cxh + q];
+    };
+    // d' of the own cells of plane k from the loaded r, d
+    auto own_dprime = [&](uint32_t k) -> RV {
+        const int zw = wall_plane(k);
+        const bool pin = pinlane && k == 0;
+        const T *rv = (const T *)&rr, *dv = (const T *)&dd;
+        RV o;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const T z = (v == 0 && pin) ? rv[v] : rv[v] * io[zw][v];
+            o.a[v] = kFirst ? z : z + tb * dv[v];
         }
-        if (hrow) {
-            const uint32_t c = cell(jh, k);
-            hr = *(const TV *)(r + c);
-            if (!kFirst) hd = *(const TV *)(dold + c);
-        }
-        if (xh) {
-            const uint32_t c = ix + nx * (jx + ny * k);
-            xr = r[c];
-            if (!kFirst) xd = dold[c];
-        }
+        return o;
     };
     RV em{}, ec{}, ep{};
     TV rc{}, rp{}, xc{}, xp{};
-    // d' of plane k from the loaded values: into `out` (own cells) and LDS buffer bf
+    // d' of plane k into `out` (own cells) and LDS buffer bf (own cells, halos,
+    // and at a wall the edge cell's own value as its clamped neighbour)
     auto form = [&](uint32_t k, int bf, RV &out, TV &rs, TV &xs) {
+        const int zw = wall_plane(k);
         if (own) {
-            out = form_vec(rr, dd, cell(j, k), j, k);
+            out = own_dprime(k);
             rs = rr;
             if (kStep) xs = xx;
-            T *row = slot(bf, 1 + wv, OFF + lane * V);
-            *(TV *)row = *(const TV *)&out;
-            if (i0 == 0) row[-1] = out.a[0];
-            if (i0 + V == nx) row[V] = out.a[V - 1];
-            if (j == 0) *(TV *)(row - S) = *(const TV *)&out;
-            if (j + 1 == ny) *(TV *)(row + S) = *(const TV *)&out;
+            T *const ob = own_slot + bf * BUF;
+            *(TV *)ob = *(const TV *)&out;
+            if (wl) ob[-1] = out.a[0];
+            if (el) ob[V] = out.a[V - 1];
+            if (sl) *(TV *)(ob - S) = *(const TV *)&out;
+            if (nl) *(TV *)(ob + S) = *(const TV *)&out;
         }
         if (hrow) {
-            const RV h = form_vec(hr, hd, cell(jh, k), jh, k);
-            *(TV *)slot(bf, hslot, OFF + lane * V) = *(const TV *)&h;
+            const T *rv = (const T *)&hr, *dv = (const T *)&hd;
+            RV h;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const T z = rv[v] * ih[zw][v];
+                h.a[v] = kFirst ? z : z + tb * dv[v];
+            }
+            *(TV *)(h_slot + bf * BUF) = *(const TV *)&h;
         }
-        if (xh) *slot(bf, xrow, xslot) = dprime(xr, xd, ix + nx * (jx + ny * k), ix, jx, k);
+        if (xh) {
+            const T z = xr * ixv[zw];
+            x_slot[bf * BUF] = kFirst ? z : z + tb * xd;
+        }
     };
     if (k0 > 0) {
         load(k0 - 1);
-        if (own) em = form_vec(rr, dd, cell(j, k0 - 1), j, k0 - 1);
+        em = own_dprime(k0 - 1);
     }
     load(k0);
     form(k0, 0, ec, rc, xc);
     if (k0 == 0) em = ec;
     if (k0 + 1 < nz) load(k0 + 1);
     __syncthreads();
-    for (uint32_t k = k0; k < k1; ++k) {
+    uint32_t c = co + nxy * k0;
+    for (uint32_t k = k0; k < k1; ++k, c += nxy) {
         const int bf = (k - k0) & 1;
         if (k + 1 < nz) {
             form(k + 1, bf ^ 1, ep, rp, xp);
@@ -1562,8 +1591,9 @@ __device__ inline void cg_tile_pass(const Grid &g, const T *inv, T *lds, const T
             ep = ec;
         }
         if (own) {
-            const uint32_t c = cell(j, k);
-            const T *row = slot(bf, 1 + wv, OFF + lane * V);
+            const T *row = own_slot + bf * BUF;
+            const bool pin = pinlane && k == 0;
+            const int zw = wall_plane(k);
             T qv[V];
 #pragma unroll
             for (int v = 0; v < V; ++v) {
@@ -1571,9 +1601,9 @@ __device__ inline void cg_tile_pass(const Grid &g, const T *inv, T *lds, const T
                 const T ve = v + 1 < V ? ec.a[v + 1] : row[V];
                 const T vw = v > 0 ? ec.a[v - 1] : row[-1];
                 const T vn = row[S + v], vs = row[-S + v];
-                qv[v] = (c + v == 0 && !neu) ? v0
-                                             : cx * ((v0 - ve) + (v0 - vw)) + cy * ((v0 - vn) + (v0 - vs)) +
-                                                   cz * ((v0 - ep.a[v]) + (v0 - em.a[v]));
+                const T qf = cx * ((v0 - ve) + (v0 - vw)) + cy * ((v0 - vn) + (v0 - vs)) +
+                             cz * ((v0 - ep.a[v]) + (v0 - em.a[v]));
+                qv[v] = (v == 0 && pin) ? v0 : qf;
             }
             if (!kStep) {
 #pragma unroll
@@ -1586,7 +1616,7 @@ __device__ inline void cg_tile_pass(const Grid &g, const T *inv, T *lds, const T
                     xo[v] = xi[v] + (T)(ta * ec.a[v]);
                     const T rn = ri[v] - ta * qv[v];
                     ro[v] = rn;
-                    const T z = (c + v == 0 && !neu) ? rn : rn * inv[diag_index(g, i0 + v, j, k)];
+                    const T z = (v == 0 && pin) ? rn : rn * io[zw][v];
                     const double rd = (double)rn;
                     s[0] += rd * (double)z;
                     s[1] += rd * rd;
@@ -1614,7 +1644,6 @@ __global__ void __launch_bounds__(kBlock) k_cg_dot_t(Grid g, CgParams prm, const
                                                      uint32_t rows, double *__restrict__ pout, CgStateA *SA,
                                                      const CgStateB *SB) {
     if (SA->done || SB->done) return;
-    const T *inv = inv_table<T>(g);
     double f[3], fm[1];
     fold_partials<3, 1>(pin, rows, f, fm);
     const double rz = f[0], rr = f[1], rmax = fm[0];
@@ -1651,8 +1680,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_dot_t(Grid g, CgParams prm, const
     __shared__ __align__(16) T lds[tile_lds_len<T>()];
     double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
     const bool neu = prm.neumann != 0;
-    if (first) cg_tile_pass<T, true, false>(g, inv, lds, r, dold, (T)0, (T)0, neu, nullptr, nullptr, nullptr, s, m);
-    else cg_tile_pass<T, false, false>(g, inv, lds, r, dold, (T)beta, (T)0, neu, nullptr, nullptr, nullptr, s, m);
+    if (first) cg_tile_pass<T, true, false>(g, lds, r, dold, (T)0, (T)0, neu, nullptr, nullptr, nullptr, s, m);
+    else cg_tile_pass<T, false, false>(g, lds, r, dold, (T)beta, (T)0, neu, nullptr, nullptr, nullptr, s, m);
     double s1[1] = {s[0]};
     publish_row<1, 1>(s1, m, pout);
 }
@@ -1666,7 +1695,6 @@ __global__ void __launch_bounds__(kBlock) k_cg_step_t(Grid g, T *__restrict__ x,
                                                       uint32_t rows, double *__restrict__ pout,
                                                       const CgStateA *SA, CgStateB *SB, int neumann) {
     if (SA->done || SB->done) return;
-    const T *inv = inv_table<T>(g);
     double f[1], fm[1];
     fold_partials<1, 1>(pin, rows, f, fm);
     const double den = f[0];
@@ -1690,8 +1718,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_step_t(Grid g, T *__restrict__ x,
     __shared__ __align__(16) T lds[tile_lds_len<T>()];
     double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
     const bool neu = neumann != 0;
-    if (SA->first) cg_tile_pass<T, true, true>(g, inv, lds, r, dold, (T)0, (T)alpha, neu, x, rnew, dnew, s, m);
-    else cg_tile_pass<T, false, true>(g, inv, lds, r, dold, (T)SA->beta, (T)alpha, neu, x, rnew, dnew, s, m);
+    if (SA->first) cg_tile_pass<T, true, true>(g, lds, r, dold, (T)0, (T)alpha, neu, x, rnew, dnew, s, m);
+    else cg_tile_pass<T, false, true>(g, lds, r, dold, (T)SA->beta, (T)alpha, neu, x, rnew, dnew, s, m);
     publish_row<3, 1>(s, m, pout);
 }
 
@@ -3108,8 +3136,8 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
         s->zp.bx = s->zr.bx;
         s->zp.count = s->zp.bx * s->zp.by * s->zp.bz;
     }
-    // tile passes: 32-plane chunks in FP64, 16 in FP32 (fastest at 256^3)
-    if (nx % 2 == 0) s->zt = tile_grid<double>(nx, ny, nz, 32);
+    // tile passes: 16-plane chunks (as fast as 32 at 256^3, faster on small grids)
+    if (nx % 2 == 0) s->zt = tile_grid<double>(nx, ny, nz, 16);
     if (nx % 4 == 0) s->ztf = tile_grid<float>(nx, ny, nz, 16);
     const size_t prow =
         (size_t)std::max(std::max(std::max(s->blocks, s->ew_blocks), std::max(s->zg.count, s->zgd.count)),

@@ -49,7 +49,11 @@ def test_pressure_solve_matches_dct(gpu_ctx, key):
 
 @pytest.mark.parametrize("stop,tol,guess", [(fa.STOP_ABS_INF, 1e-6, fa.GUESS_PREVIOUS),
                                             (fa.STOP_REL_L2, 1e-8, fa.GUESS_ZERO)])
-@pytest.mark.parametrize("shape", [(16, 16, 16), (14, 10, 12), (33, 20, 9)])
+# (130, 7, 18) and (260, 6, 35): two and three x-tiles of the plane-tile passes
+# (128 FP64 cells wide) with a partial last tile, a partial 4-row y tile and a
+# partial 16-plane z chunk; (33, 20, 9) has odd x and takes the column march.
+@pytest.mark.parametrize("shape", [(16, 16, 16), (14, 10, 12), (33, 20, 9), (130, 7, 18),
+                                   (260, 6, 35)])
 def test_pressure_solve_matches_oracle(gpu_ctx, shape, stop, tol, guess):
     nx, ny, nz = shape
     print(f"[gpu] pressure solve vs oracle {shape} stop={stop}", flush=True)
@@ -245,14 +249,18 @@ def test_large_grid_properties(gpu_ctx):
 # outer FP64 stop test is the same rel-res 1e-8, so the pressure agrees with
 # the FP64 solution to ~cond(A) * 1e-8 relative; the bars below are 1e-6 for
 # p and 1e-7 of the velocity scale for u, v, w after 6 steps.
-def test_mixed_pressure_solve_meets_fp64_tolerance(gpu_ctx):
-    n = 32
+# (32, 32, 32) and (264, 9, 19) run the FP32 plane-tile passes (256 cells wide:
+# a partial second x-tile); (30, 10, 8) has nx % 4 != 0 and takes the column march.
+@pytest.mark.parametrize("shape", [(32, 32, 32), (264, 9, 19), (30, 10, 8)])
+def test_mixed_pressure_solve_meets_fp64_tolerance(gpu_ctx, shape):
+    nx, ny, nz = shape
+    print(f"[gpu] mixed pressure solve {shape}", flush=True)
     rng = np.random.default_rng(5)
-    rhs = rng.standard_normal((n, n, n)) * 100.0
+    rhs = rng.standard_normal((nz, ny, nx)) * 100.0
     rhs[0, 0, 0] = 0.0
     out = {}
     for prec in (fa.PRECISION_FP64, fa.PRECISION_MIXED):
-        s = make(gpu_ctx, n, n, n, pressure_tolerance=1e-8, stop_rule=fa.STOP_REL_L2,
+        s = make(gpu_ctx, nx, ny, nz, pressure_tolerance=1e-8, stop_rule=fa.STOP_REL_L2,
                  initial_guess=fa.GUESS_ZERO, pressure_max_iters=20000, precision=prec)
         p = np.zeros_like(rhs)
         rep = s.pressure_solve(rhs, p)
@@ -262,7 +270,7 @@ def test_mixed_pressure_solve_meets_fp64_tolerance(gpu_ctx):
     p32, r32 = out[fa.PRECISION_MIXED]
     assert r32.refinements >= 1
     assert r32.residual_l2 <= 1e-8 * r32.initial_l2
-    o = O.Cavity3D(n, n, n)
+    o = O.Cavity3D(nx, ny, nz)
     b = -rhs.ravel()
     b[0] = 0.0
     true = np.linalg.norm(b - o.apply_operator(p32.ravel())) / np.linalg.norm(b)
