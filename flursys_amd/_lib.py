@@ -26,6 +26,8 @@ GUESS_PREVIOUS, GUESS_ZERO = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 PRECOND_JACOBI, PRECOND_MULTIGRID, PRECOND_LEARNED = 0, 1, 2
 SOLVER_PCG, SOLVER_SOR = 0, 1
+# fl_solve_report.fallback_reason
+FALLBACK_NONE, FALLBACK_BUDGET, FALLBACK_STAGNATION, FALLBACK_BREAKDOWN = 0, 1, 2, 3
 
 
 class FlursysError(RuntimeError):
@@ -44,7 +46,8 @@ class SolveReport(C.Structure):
         ("fallback_iterations", C.c_int64), ("fallback", C.c_int32), ("warm_start", C.c_int32),
         ("residual_abs_inf", C.c_double), ("residual_l2", C.c_double),
         ("initial_l2", C.c_double), ("converged", C.c_int32), ("status", C.c_int32),
-        ("warm_start_ratio", C.c_double),
+        ("warm_start_ratio", C.c_double), ("fallback_reason", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -69,7 +72,7 @@ class Cavity3DStepReport(C.Structure):
         ("max_divergence", C.c_double), ("divergence_l2", C.c_double),
         ("max_speed", C.c_double), ("pressure_converged", C.c_int32),
         ("pressure_fallback", C.c_int32), ("pressure_warm_start", C.c_int32),
-        ("pad", C.c_int32), ("pressure_warm_ratio", C.c_double),
+        ("pressure_fallback_reason", C.c_int32), ("pressure_warm_ratio", C.c_double),
     ]
 
 

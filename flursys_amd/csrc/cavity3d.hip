@@ -2574,6 +2574,7 @@ fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep, double rr0_ref = 0.0)
         rep->fallback = 0;
         rep->warm_start = 0;
         rep->warm_start_ratio = 0.0;
+        rep->fallback_reason = FL_FALLBACK_NONE;
         rep->fallback_iterations = 0;
         rep->residual_abs_inf = a.rmax;
         rep->residual_l2 = std::sqrt(a.rr);
@@ -2834,6 +2835,7 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
     double rr0 = 0.0;
     long long total = 0, sweeps = 0;
     bool converged = false, fallback = false, first_pass = true;
+    int reason = FL_FALLBACK_NONE;
     double rr = 0.0, rmax = 0.0;
     // learned warm start: 1 pending, 2 applied and awaiting its residual;
     // warm_used ends 1 (kept) or -1 (rejected: it did not lower the residual).
@@ -2936,6 +2938,8 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
                      k_add_correction<<<s->ew_blocks, kBlock, 0, stm>>>((uint32_t)s->n, s->p, s->e32, mg ? 0 : 1));
         if (mg && (status != ST_OK || stagnated || budget_hit)) {
             fallback = true;
+            reason = status != ST_OK ? FL_FALLBACK_BREAKDOWN
+                                     : (stagnated ? FL_FALLBACK_STAGNATION : FL_FALLBACK_BUDGET);
             break;
         }
         if (it == 0) break;  // no progress possible in FP32
@@ -2953,6 +2957,7 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
         rep->iterations = total;
         rep->refinements = sweeps;
         rep->fallback = fallback ? 1 : 0;
+        rep->fallback_reason = fallback ? reason : FL_FALLBACK_NONE;
         rep->warm_start = warm_used;
         rep->warm_start_ratio = warm_ratio;
         rep->fallback_iterations = fb.iterations;
@@ -3075,6 +3080,7 @@ fl_status do_step(fl_cavity3d *s, fl_cavity3d_step_report *rep) {
         rep->pressure_residual_l2 = sr.residual_l2;
         rep->pressure_converged = sr.converged;
         rep->pressure_fallback = sr.fallback;
+        rep->pressure_fallback_reason = sr.fallback_reason;
         rep->pressure_warm_start = sr.warm_start;
         rep->pressure_warm_ratio = sr.warm_start_ratio;
     }

@@ -2406,6 +2406,7 @@ fl_status fl_grid2d_pressure_solve(fl_grid2d *g, const double *rhs, double *p_in
         rep->refinements = 0;
         rep->fallback = 0;
         rep->warm_start = 0;
+        rep->fallback_reason = FL_FALLBACK_NONE;
         rep->warm_start_ratio = 0.0;
         rep->fallback_iterations = 0;
         rep->residual_abs_inf = B->h_sa[0].rmax;

@@ -104,7 +104,18 @@ typedef struct {
     int32_t converged;         /* 1 when the stop rule fired */
     int32_t status;            /* fl_status of the solve */
     double warm_start_ratio;   /* ||r||_2 after the warm start / ||r_0||_2 (0: none) */
+    int32_t fallback_reason;   /* FL_FALLBACK_*: why the plain PCG fallback ran */
+    int32_t reserved;
 } fl_solve_report;
+
+/* Why a preconditioned (multigrid / learned) solve handed over to the plain
+ * FP64 Jacobi-PCG fallback. */
+enum {
+    FL_FALLBACK_NONE = 0,
+    FL_FALLBACK_BUDGET = 1,      /* the inner iteration budget ran out */
+    FL_FALLBACK_STAGNATION = 2,  /* ||r|| failed to halve over four iterations */
+    FL_FALLBACK_BREAKDOWN = 3    /* a singular search direction or non-finite scalar */
+};
 
 /* ---------------------------------------------------------------------- */
 /* 3D lid-driven cavity.  Mirrors LidDrivenCavity3DConfig
@@ -141,7 +152,7 @@ typedef struct {
     int32_t pressure_converged;
     int32_t pressure_fallback;     /* 1 when the plain PCG fallback ran */
     int32_t pressure_warm_start;   /* fl_solve_report.warm_start of the step's solve */
-    int32_t pad;
+    int32_t pressure_fallback_reason; /* fl_solve_report.fallback_reason */
     double pressure_warm_ratio;    /* fl_solve_report.warm_start_ratio */
 } fl_cavity3d_step_report;
 

@@ -444,7 +444,13 @@ def test_multigrid_fallback_to_plain_pcg(gpu_ctx):
     p = np.zeros_like(rhs)
     rep = s.pressure_solve(rhs, p)
     assert rep.fallback == 1 and rep.converged
+    assert rep.fallback_reason == fa.FALLBACK_STAGNATION
     assert rep.residual_l2 <= 1e-8 * rep.initial_l2 * 1.0000001
+    # the same solve at the default inner tolerance needs no fallback
+    s = _mg(gpu_ctx, n)
+    p = np.zeros_like(rhs)
+    rep = s.pressure_solve(rhs, p)
+    assert rep.fallback == 0 and rep.fallback_reason == fa.FALLBACK_NONE and rep.converged
 
 
 def test_multigrid_rejects_unsupported_grids(gpu_ctx):
