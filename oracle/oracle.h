@@ -236,6 +236,46 @@ int or_assemble_poisson(const or_mesh2d *m, double gamma, const double *source,
                         int64_t **offsets_out, int64_t **cols_out, double **vals_out,
                         double **rhs_out, double *detail, char *message, int cap);
 
+/* ============ caller-shaped systems: SIMPLE pressure correction, momentum ============ */
+/* SimpleError / MomentumError kinds (src/simple.rs, src/momentum.rs) */
+enum {
+    OR_SIMPLE_INVALID_REFERENCE = 50,
+    OR_SIMPLE_INVALID_FACE_COEF = 51,
+    OR_SIMPLE_DEGENERATE = 52,
+    OR_SIMPLE_LINEAR = 53,
+    OR_MOM_INVALID_RELAX = 60,
+    OR_MOM_INVALID_DT = 61,
+    OR_MOM_NONFINITE_SOURCE = 62,
+    OR_MOM_NONFINITE_FLUX = 63,
+    OR_MOM_DEGENERATE = 64,
+    OR_MOM_GEOMETRY = 65,
+    OR_MOM_MISSING_BOUNDARY = 66,
+    OR_MOM_BACKFLOW = 67,
+    OR_MOM_MISSING_RELAX_FIELD = 68,
+    OR_MOM_LINEAR = 69,
+    OR_MOM_FACE_DIFFUSIVITY = 70,
+};
+/* interior-face interpolation weights (0 on boundary faces) */
+int or_face_weights(const or_mesh2d *m, double *lambda, char *message, int cap);
+/* pressure_face_coefficients: d_f of every face (0 on boundary faces) */
+int or_pressure_face_coefficients(const or_mesh2d *m, const double *r_au, double *coef,
+                                  char *message, int cap);
+/* assemble_pressure_correction: CSR + rhs (caller frees with or_free) */
+int or_assemble_pressure_correction(const or_mesh2d *m, const double *flux, const double *coef,
+                                    int64_t ref_cell, int64_t *nnz_out, int64_t **off_out,
+                                    int64_t **col_out, double **val_out, double **rhs_out,
+                                    char *message, int cap);
+/* assemble_momentum_component with Diffusivity::Constant(nu), per-face
+ * boundary conditions (bc_kind/bc_value by face, OR_SCALAR_*), optional
+ * source, optional transient (old != NULL with dt) and equation
+ * under-relaxation alpha (relax_field needed when alpha < 1).  Returns the
+ * relaxed CSR, rhs and the unrelaxed diagonal (caller frees with or_free). */
+int or_assemble_momentum(const or_mesh2d *m, double nu, const double *flux, const int32_t *bc_kind,
+                         const double *bc_value, const double *source, const double *old, double dt,
+                         const double *relax_field, double alpha, int64_t *nnz_out, int64_t **off_out,
+                         int64_t **col_out, double **val_out, double **rhs_out, double **diag_out,
+                         char *message, int cap);
+
 /* ============ unstructured triangle-mesh Poisson (config-3 producer) ============ */
 /* Builds the jittered structured triangulation of the unit square with m x m
  * squares (2*m*m triangles), assembles the two-point Poisson matrix with

@@ -199,6 +199,16 @@ def lib():
             C.POINTER(Mesh2DStruct), C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
             C.c_int64, C.c_double, C.POINTER(C.c_int64), P64, P64, PD, PD,
             C.POINTER(C.c_double), C.c_char_p, C.c_int]
+        L.or_face_weights.argtypes = [C.POINTER(Mesh2DStruct), _f64p, C.c_char_p, C.c_int]
+        L.or_pressure_face_coefficients.argtypes = [C.POINTER(Mesh2DStruct), _f64p, _f64p,
+                                                    C.c_char_p, C.c_int]
+        L.or_assemble_pressure_correction.argtypes = [
+            C.POINTER(Mesh2DStruct), _f64p, _f64p, C.c_int64, C.POINTER(C.c_int64), P64, P64,
+            PD, PD, C.c_char_p, C.c_int]
+        L.or_assemble_momentum.argtypes = [
+            C.POINTER(Mesh2DStruct), C.c_double, _f64p, C.c_void_p, C.c_void_p, C.c_void_p,
+            C.c_void_p, C.c_double, C.c_void_p, C.c_double, C.POINTER(C.c_int64), P64, P64, PD,
+            PD, PD, C.c_char_p, C.c_int]
         L.or_free.argtypes = [C.c_void_p]
         L.or_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
         L.or_splitmix64.restype = C.c_uint64
@@ -647,3 +657,109 @@ class Rng:
 
     def normal(self, n):
         return np.array([lib().or_normal(C.byref(self.state)) for _ in range(n)])
+
+
+class CallerSystemError(RuntimeError):
+    """SimpleError / MomentumError of the caller-shaped assemblies (oracle/simple.c)."""
+
+    def __init__(self, code, message):
+        super().__init__(f"{code}: {message}")
+        self.code = code
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def face_weights(mesh):
+    """Interior-face interpolation weights (src/numerics.rs:294-303); 0 on boundary faces."""
+    out = np.zeros(mesh.m.nfaces)
+    msg = C.create_string_buffer(256)
+    st = lib().or_face_weights(C.byref(mesh.m), out, msg, 256)
+    if st != 0:
+        raise CallerSystemError(st, msg.value.decode())
+    return out
+
+
+def pressure_face_coefficients(mesh, r_au):
+    """pressure_face_coefficients (src/simple.rs:342-370): d_f per face."""
+    out = np.zeros(mesh.m.nfaces)
+    msg = C.create_string_buffer(256)
+    st = lib().or_pressure_face_coefficients(C.byref(mesh.m), _f64(r_au), out, msg, 256)
+    if st != 0:
+        raise CallerSystemError(st, msg.value.decode())
+    return out
+
+
+def _take_csr(n, nnz, po, pc, pv, extra):
+    L = lib()
+    off, cols = _arr(po, n + 1).copy(), _arr(pc, nnz).copy()
+    vals = _arr(pv, nnz).copy()
+    outs = [_arr(p, n).copy() for p in extra]
+    for p in (po, pc, pv, *extra):
+        L.or_free(p)
+    return Csr(n, n, off, cols, vals, validate=False), outs
+
+
+def assemble_pressure_correction(mesh, flux, coef, reference_cell):
+    """assemble_pressure_correction (src/simple.rs:518-579): (Csr, rhs)."""
+    nnz = C.c_int64()
+    po, pc = C.POINTER(C.c_int64)(), C.POINTER(C.c_int64)()
+    pv, pr = C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+    msg = C.create_string_buffer(256)
+    st = lib().or_assemble_pressure_correction(C.byref(mesh.m), _f64(flux), _f64(coef),
+                                               reference_cell, C.byref(nnz), C.byref(po),
+                                               C.byref(pc), C.byref(pv), C.byref(pr), msg, 256)
+    if st != 0:
+        raise CallerSystemError(st, msg.value.decode())
+    A, (rhs,) = _take_csr(mesh.m.ncells, nnz.value, po, pc, pv, [pr])
+    return A, rhs
+
+
+def assemble_momentum(mesh, viscosity, flux, boundary, source=None, old=None, dt=0.0,
+                      relaxation_field=None, under_relaxation=1.0):
+    """assemble_momentum_component (src/momentum.rs:180-341) with a constant
+    viscosity.  `boundary` maps every boundary face to ("FixedValue", value)
+    or "ZeroGradient".  Returns (Csr, rhs, unrelaxed diagonal)."""
+    nf = mesh.m.nfaces
+    kind = np.full(nf, -1, np.int32)
+    value = np.zeros(nf)
+    for f, bc in boundary.items():
+        if bc == "ZeroGradient":
+            kind[f] = 0
+        else:
+            kind[f], value[f] = FIXED_VALUE, bc[1]
+    keep = [None if a is None else _f64(a) for a in (source, old, relaxation_field)]
+    ptr = [None if a is None else a.ctypes.data for a in keep]
+    nnz = C.c_int64()
+    po, pc = C.POINTER(C.c_int64)(), C.POINTER(C.c_int64)()
+    pv, pr, pd = C.POINTER(C.c_double)(), C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+    msg = C.create_string_buffer(256)
+    st = lib().or_assemble_momentum(C.byref(mesh.m), viscosity, _f64(flux), kind.ctypes.data,
+                                    value.ctypes.data, ptr[0], ptr[1], dt, ptr[2],
+                                    under_relaxation, C.byref(nnz), C.byref(po), C.byref(pc),
+                                    C.byref(pv), C.byref(pr), C.byref(pd), msg, 256)
+    if st != 0:
+        raise CallerSystemError(st, msg.value.decode())
+    A, (rhs, diag) = _take_csr(mesh.m.ncells, nnz.value, po, pc, pv, [pr, pd])
+    return A, rhs, diag
+
+
+def triangulated_square(n, jitter=0.2, seed=0):
+    """Mesh2D of the unit square: n x n squares, each cut into two triangles
+    along its rising diagonal, interior vertices moved uniformly by up to
+    `jitter` h in x and y (numpy generator `seed`)."""
+    rng = np.random.default_rng(seed)
+    xs = np.linspace(0.0, 1.0, n + 1)
+    X, Y = np.meshgrid(xs, xs)
+    inner = (X > 0) & (X < 1) & (Y > 0) & (Y < 1)
+    h = 1.0 / n
+    X = X + inner * rng.uniform(-jitter * h, jitter * h, X.shape)
+    Y = Y + inner * rng.uniform(-jitter * h, jitter * h, Y.shape)
+    pts = np.stack([X.ravel(), Y.ravel()], 1)
+    cells = []
+    for j in range(n):
+        for i in range(n):
+            a = i + (n + 1) * j
+            cells += [[a, a + 1, a + n + 2], [a, a + n + 2, a + n + 1]]
+    return Mesh2D(pts, cells)
