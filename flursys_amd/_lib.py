@@ -20,6 +20,7 @@ STATUS_NAMES = {
     1: "DimensionMismatch", 2: "InvalidCsr", 3: "IndexOutOfBounds", 4: "NonFiniteValue",
     5: "MissingDiagonal", 6: "ZeroDiagonal", 7: "Breakdown", 8: "SolverError",
     9: "ConfigError", 10: "InvalidArgument", 11: "HipError", 12: "OutOfMemory",
+    13: "IoError",
 }
 STOP_ABS_INF, STOP_REL_L2 = 0, 1
 GUESS_PREVIOUS, GUESS_ZERO = 0, 1
@@ -150,6 +151,18 @@ _D = C.POINTER(C.c_double)
 _LIN =[_P, _P, _P, C.POINTER(LinOpts), C.POINTER(LinReport), C.POINTER(ErrDetail)]
 
 # name -> (restype, argtypes); every symbol include/flursys_hip.h declares.
+class HistoryRow(C.Structure):
+    """fl_history_row: one history.csv row (incompressible.rs:1744-1767) plus
+    the GPU path's cg_iters, hbm_GBps, roofline_frac and fallback_count."""
+    _fields_ = [("step", C.c_int64), ("time", C.c_double), ("pressure_residual", C.c_double),
+                ("pressure_iterations", C.c_int64)] + [
+        (name, C.c_double) for name in (
+            "max_divergence", "velocity_change", "max_speed", "cd", "cl", "reattachment",
+            "momentum_cfl", "viscous_diffusion_number", "time_step")] + [
+        ("cg_iters", C.c_int64), ("hbm_GBps", C.c_double), ("roofline_frac", C.c_double),
+        ("fallback_count", C.c_int64)]
+
+
 SIGNATURES = {
     "fl_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "fl_destroy": (None, [_P]),
@@ -162,6 +175,20 @@ SIGNATURES = {
     "fl_cavity3d_run": (C.c_int, [_P, C.c_int64, C.POINTER(Cavity3DStepReport)]),
     "fl_cavity3d_pressure_solve": (C.c_int, [_P, _P, _P, C.POINTER(SolveReport)]),
     "fl_cavity3d_get_fields": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fl_cavity3d_cell_fields": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fl_write_field_csv": (C.c_int, [C.c_char_p, C.c_int64, C.c_int64, C.c_double, C.c_double,
+                                     _P, _P, _P, _P, _P, _P, C.c_char_p, C.c_int32]),
+    "fl_write_vtk_2d": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int64, C.c_int64, C.c_double,
+                                  C.c_double, _P, _P, _P, _P, _P, _P, C.c_char_p, C.c_int32]),
+    "fl_write_vtk_3d": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int64, C.c_int64, C.c_int64,
+                                  C.c_double, C.c_double, C.c_double, _P, _P, _P, _P,
+                                  C.c_char_p, C.c_int32]),
+    "fl_write_ppm": (C.c_int, [C.c_char_p, C.c_int64, C.c_int64, _P, _P, C.c_int32, C.c_char_p,
+                               C.c_int32]),
+    "fl_write_csv_columns": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, C.c_int64, _P,
+                                       C.c_char_p, C.c_char_p, C.c_int32]),
+    "fl_append_history": (C.c_int, [C.c_char_p, C.POINTER(HistoryRow), C.c_int32, C.c_char_p,
+                                    C.c_int32]),
     "fl_cavity3d_set_fields": (C.c_int, [_P, _P, _P, _P, _P]),
     "fl_cavity3d_get_intermediate": (C.c_int, [_P, _P, _P, _P, _P]),
     "fl_cavity3d_set_timing": (C.c_int, [_P, C.c_int32, C.c_char_p]),

@@ -1841,6 +1841,21 @@ struct StepParams {
 #define V3(i, j, k) ((i) + P.nx * ((j) + (P.ny + 1) * (k)))
 #define W3(i, j, k) ((i) + P.nx * ((j) + P.ny * (k)))
 
+// Cell-centred velocity for output: the mean of each cell's two faces per
+// component, 0.5 (u_i + u_{i+1}) etc. (write_legacy_vtk_3d, output.rs:170-180).
+__global__ void __launch_bounds__(kBlock) k_cell_velocity(StepParams P, const double *__restrict__ u,
+                                                          const double *__restrict__ v,
+                                                          const double *__restrict__ w,
+                                                          double *__restrict__ out) {
+    const size_t n = (size_t)P.nx * P.ny * P.nz;
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / ((size_t)P.nx * P.ny));
+        out[c] = 0.5 * (u[U3(i, j, k)] + u[U3(i + 1, j, k)]);
+        out[n + c] = 0.5 * (v[V3(i, j, k)] + v[V3(i, j + 1, k)]);
+        out[2 * n + c] = 0.5 * (w[W3(i, j, k)] + w[W3(i, j, k + 1)]);
+    }
+}
+
 // u/v/w_neighbor (incompressible_3d.rs:366-447).
 __device__ inline double u_nb(const StepParams &P, const double *u, int i, int j, int k, int di,
                               int dj, int dk, double c) {
@@ -2241,6 +2256,7 @@ struct fl_cavity3d {
     ZGrid zt{0, 0, 0, 0}, ztf{0, 0, 0, 0};
     int64_t step = 0;
     KProf prof;  // per-kernel timing (HIP events on the solver's stream)
+    double *cell_out = nullptr;  // output staging: cell-centred u, v, w (allocated on first use)
 };
 
 namespace {
@@ -2275,7 +2291,7 @@ void free_all(fl_cavity3d *s) {
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     void *more[] = {s->pz, s->fa, s->fb, s->wl_in, s->wl_out, s->pair, s->pinv, s->m0a, s->m0b, s->cap,
-                    s->sor};
+                    s->sor, s->cell_out};
     for (void *b : more)
         if (b) (void)hipFree(b);
     for (int l = 0; l < kMaxLevels; ++l) {
@@ -2945,7 +2961,8 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
         if (it == 0) break;  // no progress possible in FP32
     }
     fl_solve_report fb{};
-    if (fallback && total < s->cfg.pressure_max_iters) {
+    const bool fallback_runs = fallback && total < s->cfg.pressure_max_iters;
+    if (fallback_runs) {
         // the judge: plain FP64 Jacobi-PCG from the current iterate
         fl_status st = solve_fp64(s, &fb, rr0);
         if (st != FL_OK) return st;
@@ -2956,7 +2973,7 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
     if (rep) {
         rep->iterations = total;
         rep->refinements = sweeps;
-        rep->fallback = fallback ? 1 : 0;
+        rep->fallback = fallback_runs ? 1 : 0;
         rep->fallback_reason = fallback ? reason : FL_FALLBACK_NONE;
         rep->warm_start = warm_used;
         rep->warm_start_ratio = warm_ratio;
@@ -3262,6 +3279,29 @@ fl_status fl_cavity3d_get_fields(fl_cavity3d *s, double *u, double *v, double *w
     if (u) FL_HIP_TRY(ctx, hipMemcpyAsync(u, s->u, s->nu * 8, hipMemcpyDeviceToHost, stm));
     if (v) FL_HIP_TRY(ctx, hipMemcpyAsync(v, s->v, s->nv * 8, hipMemcpyDeviceToHost, stm));
     if (w) FL_HIP_TRY(ctx, hipMemcpyAsync(w, s->w, s->nw * 8, hipMemcpyDeviceToHost, stm));
+    if (p) FL_HIP_TRY(ctx, hipMemcpyAsync(p, s->p, s->n * 8, hipMemcpyDeviceToHost, stm));
+    FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
+    return FL_OK;
+}
+
+fl_status fl_cavity3d_cell_fields(fl_cavity3d *s, double *p, double *uc, double *vc, double *wc) {
+    if (!s) return FL_ERR_INVALID_ARGUMENT;
+    fl_ctx *ctx = s->ctx;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t stm = ctx->stream;
+    if (uc || vc || wc) {
+        if (!s->cell_out) {  // staging for the three components, kept for later snapshots
+            if (hipMalloc(&s->cell_out, 3 * s->n * sizeof(double)) != hipSuccess) {
+                s->cell_out = nullptr;
+                return set_error(ctx, FL_ERR_OUT_OF_MEMORY, "cannot allocate the output staging buffer");
+            }
+        }
+        k_cell_velocity<<<s->blocks, kBlock, 0, stm>>>(s->P, s->u, s->v, s->w, s->cell_out);
+        FL_HIP_TRY(ctx, hipGetLastError());
+        if (uc) FL_HIP_TRY(ctx, hipMemcpyAsync(uc, s->cell_out, s->n * 8, hipMemcpyDeviceToHost, stm));
+        if (vc) FL_HIP_TRY(ctx, hipMemcpyAsync(vc, s->cell_out + s->n, s->n * 8, hipMemcpyDeviceToHost, stm));
+        if (wc) FL_HIP_TRY(ctx, hipMemcpyAsync(wc, s->cell_out + 2 * s->n, s->n * 8, hipMemcpyDeviceToHost, stm));
+    }
     if (p) FL_HIP_TRY(ctx, hipMemcpyAsync(p, s->p, s->n * 8, hipMemcpyDeviceToHost, stm));
     FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
     return FL_OK;

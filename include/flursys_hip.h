@@ -40,7 +40,8 @@ typedef enum {
     FL_ERR_CONFIG = 9,
     FL_ERR_INVALID_ARGUMENT = 10,
     FL_ERR_HIP = 11,
-    FL_ERR_OUT_OF_MEMORY = 12
+    FL_ERR_OUT_OF_MEMORY = 12,
+    FL_ERR_IO = 13                 /* an output file could not be written */
 } fl_status;
 
 /* Stop rule: the reference's absolute max|r| < tol (incompressible.rs:
@@ -104,7 +105,9 @@ typedef struct {
     int32_t converged;         /* 1 when the stop rule fired */
     int32_t status;            /* fl_status of the solve */
     double warm_start_ratio;   /* ||r||_2 after the warm start / ||r_0||_2 (0: none) */
-    int32_t fallback_reason;   /* FL_FALLBACK_*: why the plain PCG fallback ran */
+    int32_t fallback_reason;   /* FL_FALLBACK_*: why the preconditioned solve handed over
+                                  (also when the iteration cap left the fallback no room:
+                                  fallback 0, converged 0) */
     int32_t reserved;
 } fl_solve_report;
 
@@ -491,6 +494,60 @@ fl_status fl_cavity3d_coarse_pair(fl_cavity3d *solver, int32_t n, float *rhs_c, 
  * resolution, the inputs of coarse-solve training pairs. */
 fl_status fl_cavity3d_capture_coarse(fl_cavity3d *solver, int32_t n, int32_t iteration);
 fl_status fl_cavity3d_captured_rhs(fl_cavity3d *solver, float *rhs_c, double *diag);
+
+/* ---------------------------------------------------------------------- */
+/* On-disk outputs (src/output.rs) in the reference's formats and number
+ * formatting (Rust {:.12} / {:.12e}), written from host arrays filled by the
+ * device (fl_grid2d_cell_fields, fl_cavity3d_cell_fields).  Host-only calls:
+ * they take no context.  `err` (may be NULL) receives the reference's
+ * "Cannot create <path>: <reason>" text on FL_ERR_IO.  2D arrays are nx*ny,
+ * x fastest; `solid` and `temperature` may be NULL. */
+/* write_field_csv (output.rs:31-67): field.csv */
+fl_status fl_write_field_csv(const char *path, int64_t nx, int64_t ny, double dx, double dy,
+                             const uint8_t *solid, const double *p, const double *u_cell,
+                             const double *v_cell, const double *vorticity,
+                             const double *temperature, char *err, int32_t errcap);
+/* write_legacy_vtk (output.rs:69-125): field.vtk of the 2D solver */
+fl_status fl_write_vtk_2d(const char *path, const char *title, int64_t nx, int64_t ny, double dx,
+                          double dy, const uint8_t *solid, const double *p, const double *u_cell,
+                          const double *v_cell, const double *vorticity, const double *temperature,
+                          char *err, int32_t errcap);
+/* write_legacy_vtk_3d (output.rs:127-183) with the face averages already
+ * taken (cell-centred u, v, w of fl_cavity3d_cell_fields) */
+fl_status fl_write_vtk_3d(const char *path, const char *title, int64_t nx, int64_t ny, int64_t nz,
+                          double dx, double dy, double dz, const double *p, const double *u_cell,
+                          const double *v_cell, const double *w_cell, char *err, int32_t errcap);
+/* write_ppm_frame (output.rs:201-255): P6 image, rows top to bottom */
+fl_status fl_write_ppm(const char *path, int64_t nx, int64_t ny, const uint8_t *solid,
+                       const double *field, int32_t symmetric, char *err, int32_t errcap);
+/* One history.csv row (append_histories, incompressible.rs:1744-1767);
+ * with `extended` the header and row add cg_iters, hbm_GBps, roofline_frac
+ * and fallback_count after the reference's thirteen columns. */
+typedef struct {
+    int64_t step;
+    double time, pressure_residual;
+    int64_t pressure_iterations;
+    double max_divergence, velocity_change, max_speed, cd, cl, reattachment;
+    double momentum_cfl, viscous_diffusion_number, time_step;
+    int64_t cg_iters;
+    double hbm_GBps, roofline_frac;
+    int64_t fallback_count;
+} fl_history_row;
+/* append_history (output.rs:185-199): the header is written when the file
+ * does not exist yet. */
+fl_status fl_append_history(const char *path, const fl_history_row *row, int32_t extended,
+                            char *err, int32_t errcap);
+/* The case-specific profile files of write_case_specific_profiles
+ * (incompressible.rs:1821-1913): a header line, then `nrows` rows of
+ * `ncols` columns, column c formatted by fmt[c] ('f': {:.12}, 'e': {:.12e}). */
+fl_status fl_write_csv_columns(const char *path, const char *header, int32_t ncols, int64_t nrows,
+                               const double *const *cols, const char *fmt, char *err,
+                               int32_t errcap);
+/* Cell-centred fields of the 3D solver for output (incompressible_3d.rs via
+ * output.rs:170-180): p and the averages of the two faces of each cell,
+ * formed on the device; any pointer may be NULL. */
+fl_status fl_cavity3d_cell_fields(fl_cavity3d *solver, double *p, double *u_cell, double *v_cell,
+                                  double *w_cell);
 
 #ifdef __cplusplus
 }
