@@ -20,7 +20,7 @@ STATUS_NAMES = {
     1: "DimensionMismatch", 2: "InvalidCsr", 3: "IndexOutOfBounds", 4: "NonFiniteValue",
     5: "MissingDiagonal", 6: "ZeroDiagonal", 7: "Breakdown", 8: "SolverError",
     9: "ConfigError", 10: "InvalidArgument", 11: "HipError", 12: "OutOfMemory",
-    13: "IoError",
+    13: "IoError", 14: "MeshError", 15: "PoissonError",
 }
 STOP_ABS_INF, STOP_REL_L2 = 0, 1
 GUESS_PREVIOUS, GUESS_ZERO = 0, 1
@@ -176,6 +176,17 @@ SIGNATURES = {
     "fl_cavity3d_pressure_solve": (C.c_int, [_P, _P, _P, C.POINTER(SolveReport)]),
     "fl_cavity3d_get_fields": (C.c_int, [_P, _P, _P, _P, _P]),
     "fl_cavity3d_cell_fields": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fl_mesh2d_create": (C.c_int, [_P, C.c_int64, _P, C.c_int64, _P, _P, C.POINTER(_P),
+                                   C.POINTER(C.c_int32)]),
+    "fl_mesh2d_destroy": (None, [_P]),
+    "fl_mesh2d_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "fl_mesh2d_get": (C.c_int, [_P] + [_P] * 9),
+    "fl_mesh2d_assemble_poisson": (C.c_int, [_P, C.c_double, _P, _P, _P, C.c_int32, C.c_int64,
+                                             C.c_double, C.POINTER(_P), _P,
+                                             C.POINTER(C.c_int32), _D]),
+    "fl_csr_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                              C.POINTER(C.c_int64)]),
+    "fl_csr_download": (C.c_int, [_P, _P, _P, _P]),
     "fl_write_field_csv": (C.c_int, [C.c_char_p, C.c_int64, C.c_int64, C.c_double, C.c_double,
                                      _P, _P, _P, _P, _P, _P, C.c_char_p, C.c_int32]),
     "fl_write_vtk_2d": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int64, C.c_int64, C.c_double,

@@ -28,6 +28,10 @@ namespace fl {
 
 void ctx_release(fl_ctx *ctx);  // drops one reference (context.hip)
 void noalloc_init(int device);   // per-device address of the allocation-free solve buffers (csr.hip)
+// Wraps device CSR arrays (int32 offsets and columns, FP64 values; sorted,
+// duplicate-free, finite) in a solver matrix, taking ownership (csr.hip).
+fl_status csr_adopt(fl_ctx *ctx, uint32_t n, uint32_t ncols, int64_t nnz, int *off, int *col,
+                    double *val, fl_csr **out);
 inline void ctx_retain(fl_ctx *ctx) { ctx->refs += 1; }
 
 constexpr int kBlock = 256;   // threads per workgroup: 4 wave64s

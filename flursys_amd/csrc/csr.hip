@@ -731,6 +731,89 @@ fl_status solve(fl_csr *m, int kind, const double *rhs, double *x_inout, const f
 
 }  // namespace
 
+namespace {
+// JacobiPreconditioner::new (preconditioner.rs:9-28) on the device: the
+// inverse diagonal of every row and, in `first`, the smallest failing row
+// packed with its kind (row << 2 | 1 missing, 2 zero, 3 non-finite inverse).
+__global__ void k_jacobi_inverse(uint32_t n, const int *__restrict__ off, const int *__restrict__ col,
+                                 const double *__restrict__ val, double *__restrict__ inv,
+                                 unsigned long long *first) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        int found = -1;
+        for (int e = off[r]; e < off[r + 1]; ++e)
+            if (col[e] == (int)r) found = e;
+        unsigned code = 0;
+        if (found < 0) {
+            code = 1;
+        } else if (val[found] == 0.0) {
+            code = 2;
+        } else {
+            inv[r] = 1.0 / val[found];
+            if (!isfinite(inv[r])) code = 3;
+        }
+        if (code) atomicMin(first, ((unsigned long long)r << 2) | code);
+    }
+}
+
+}  // namespace
+
+namespace fl {
+fl_status csr_adopt(fl_ctx *ctx, uint32_t n, uint32_t ncols, int64_t nnz, int *off, int *col, double *val,
+                    fl_csr **out) {
+    *out = nullptr;
+    fl_csr *m = new (std::nothrow) fl_csr();
+    if (!m) {
+        (void)hipFree(off);
+        (void)hipFree(col);
+        (void)hipFree(val);
+        return set_error(ctx, FL_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    m->ctx = ctx;
+    m->n = n;
+    m->ncols = ncols;
+    m->nnz = nnz;
+    m->off = off;
+    m->col = col;
+    m->val = val;
+    const int64_t tiles = (n + kBlock - 1) / kBlock;
+    m->blocks = grid_blocks(ctx, tiles > 0 ? tiles : 1);
+    hipStream_t s = ctx->stream;
+    unsigned long long *first = nullptr;
+    bool okay = hipMalloc(&m->inv, (n > 0 ? n : 1) * sizeof(double)) == hipSuccess &&
+                hipMalloc(&m->partials, (size_t)m->blocks * 4 * sizeof(double)) == hipSuccess &&
+                hipMalloc(&m->st, sizeof(LinState)) == hipSuccess &&
+                hipHostMalloc(&m->st_host, sizeof(LinState), hipHostMallocDefault) == hipSuccess &&
+                hipEventCreate(&m->e0) == hipSuccess && hipEventCreate(&m->e1) == hipSuccess &&
+                hipMalloc(&first, sizeof(unsigned long long)) == hipSuccess;
+    unsigned long long h = ~0ull;
+    okay = okay && hipMemcpyAsync(first, &h, sizeof h, hipMemcpyHostToDevice, s) == hipSuccess &&
+           hipMemsetAsync(m->st, 0, sizeof(LinState), s) == hipSuccess;
+    if (okay && n > 0) {
+        k_jacobi_inverse<<<m->blocks, kBlock, 0, s>>>(n, off, col, val, m->inv, first);
+        okay = hipGetLastError() == hipSuccess;
+    }
+    okay = okay && hipMemcpyAsync(&h, first, sizeof h, hipMemcpyDeviceToHost, s) == hipSuccess &&
+           hipStreamSynchronize(s) == hipSuccess;
+    if (first) (void)hipFree(first);
+    if (!okay) {
+        free_csr(m);
+        delete m;
+        return set_error(ctx, FL_ERR_OUT_OF_MEMORY, "device allocation failed");
+    }
+    if (h != ~0ull) {
+        m->inv_row = (int64_t)(h >> 2);
+        switch (h & 3) {
+            case 1: m->inv_status = FL_ERR_MISSING_DIAGONAL; m->inv_ctx = "missing diagonal entry"; break;
+            case 2: m->inv_status = FL_ERR_ZERO_DIAGONAL; m->inv_ctx = "zero diagonal entry"; break;
+            default: m->inv_status = FL_ERR_NON_FINITE; m->inv_ctx = "inverse diagonal"; break;
+        }
+    }
+    ctx_retain(ctx);
+    *out = m;
+    return FL_OK;
+}
+}  // namespace fl
+
 extern "C" {
 
 fl_status fl_csr_upload(fl_ctx *ctx, int64_t nrows, int64_t ncols, const int64_t *off,
@@ -923,6 +1006,31 @@ fl_status fl_pcg_jacobi_noalloc(fl_csr *m, const double *rhs, double *x, const f
     w.partials = base + 5 * kNoAllocStride;
     w.st = reinterpret_cast<LinState *>(w.partials + kNoAllocPartials);
     return solve_on(m, KIND_PCG, rhs, x, o, rep, d, w);
+}
+
+fl_status fl_csr_info(fl_csr *m, int64_t *nrows, int64_t *ncols, int64_t *nnz) {
+    if (!m) return FL_ERR_INVALID_ARGUMENT;
+    if (nrows) *nrows = m->n;
+    if (ncols) *ncols = m->ncols;
+    if (nnz) *nnz = m->nnz;
+    return FL_OK;
+}
+
+fl_status fl_csr_download(fl_csr *m, int64_t *off, int64_t *cols, double *vals) {
+    if (!m) return FL_ERR_INVALID_ARGUMENT;
+    fl_ctx *ctx = m->ctx;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    std::vector<int> ho(m->n + 1), hc(m->nnz > 0 ? m->nnz : 1);
+    FL_HIP_TRY(ctx, hipMemcpy(ho.data(), m->off, (m->n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    if (m->nnz > 0)
+        FL_HIP_TRY(ctx, hipMemcpy(hc.data(), m->col, m->nnz * sizeof(int), hipMemcpyDeviceToHost));
+    if (off)
+        for (uint32_t r = 0; r <= m->n; ++r) off[r] = ho[r];
+    if (cols)
+        for (int64_t e = 0; e < m->nnz; ++e) cols[e] = hc[e];
+    if (vals && m->nnz > 0)
+        FL_HIP_TRY(ctx, hipMemcpy(vals, m->val, m->nnz * sizeof(double), hipMemcpyDeviceToHost));
+    return FL_OK;
 }
 
 }  // extern "C"

@@ -87,6 +87,25 @@ class CsrMatrix:
         self.handle = h
         self.nrows, self.ncols, self.nnz = nrows, ncols, vals.size
 
+    @classmethod
+    def adopt(cls, handle, ctx):
+        """Wraps a matrix the device built (fl_mesh2d_assemble_poisson)."""
+        m = cls.__new__(cls)
+        m.ctx, m.handle = ctx, handle
+        n, nc, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        load().fl_csr_info(handle, C.byref(n), C.byref(nc), C.byref(nnz))
+        m.nrows, m.ncols, m.nnz = n.value, nc.value, nnz.value
+        return m
+
+    def arrays(self):
+        """Host copies (row_offsets, column_indices, values)."""
+        off = np.empty(self.nrows + 1, np.int64)
+        cols = np.empty(max(self.nnz, 1), np.int64)
+        vals = np.empty(max(self.nnz, 1), np.float64)
+        _lib.check(load().fl_csr_download(self.handle, ptr(off), ptr(cols), ptr(vals)),
+                   self.ctx.handle)
+        return off, cols[:self.nnz], vals[:self.nnz]
+
     def close(self):
         if getattr(self, "handle", None):
             load().fl_csr_destroy(self.handle)

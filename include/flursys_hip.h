@@ -41,7 +41,9 @@ typedef enum {
     FL_ERR_INVALID_ARGUMENT = 10,
     FL_ERR_HIP = 11,
     FL_ERR_OUT_OF_MEMORY = 12,
-    FL_ERR_IO = 13                 /* an output file could not be written */
+    FL_ERR_IO = 13,                /* an output file could not be written */
+    FL_ERR_MESH = 14,              /* MeshError of a mesh build (kind in FL_MESH_*) */
+    FL_ERR_POISSON = 15            /* PoissonError of an assembly (kind in FL_POISSON_*) */
 } fl_status;
 
 /* Stop rule: the reference's absolute max|r| < tol (incompressible.rs:
@@ -494,6 +496,58 @@ fl_status fl_cavity3d_coarse_pair(fl_cavity3d *solver, int32_t n, float *rhs_c, 
  * resolution, the inputs of coarse-solve training pairs. */
 fl_status fl_cavity3d_capture_coarse(fl_cavity3d *solver, int32_t n, int32_t iteration);
 fl_status fl_cavity3d_captured_rhs(fl_cavity3d *solver, float *rhs_c, double *diag);
+
+/* ---------------------------------------------------------------------- */
+/* Device assembly of unstructured finite-volume systems (no reference
+ * counterpart as a device path: it runs UnstructuredMesh::from_cells,
+ * src/mesh.rs:317-423, assemble_poisson + apply_reference,
+ * src/poisson.rs:146-263, 303-337, and CsrBuilder::finalize,
+ * src/linear/csr.rs:159-192, as GPU sorts and scans).  2D polygon cells of
+ * 3..16 vertices; faces are numbered in the reference's creation order and
+ * carry its geometry; duplicate CSR entries are summed in insertion order
+ * and exact zeros dropped. */
+typedef struct fl_mesh2d fl_mesh2d;
+enum {                          /* MeshError kinds (src/mesh.rs:270-310) */
+    FL_MESH_OK = 0, FL_MESH_EMPTY = 1, FL_MESH_NON_FINITE_POINT = 2, FL_MESH_INVALID_CELL = 3,
+    FL_MESH_INVALID_POINT_INDEX = 4, FL_MESH_DEGENERATE_FACE = 5, FL_MESH_NON_MANIFOLD = 6,
+    FL_MESH_ZERO_VOLUME = 7, FL_MESH_INCONSISTENT_ORIENTATION = 8
+};
+enum {                          /* PoissonError kinds (src/poisson.rs:78-88) */
+    FL_POISSON_OK = 0, FL_POISSON_MISSING_REFERENCE = 1, FL_POISSON_INCOMPATIBLE_SOURCE = 2,
+    FL_POISSON_INVALID_REFERENCE_CELL = 3, FL_POISSON_INVALID_REFERENCE_VALUE = 4,
+    FL_POISSON_NON_FINITE_SOURCE = 5, FL_POISSON_DEGENERATE_DISTANCE = 6,
+    FL_POISSON_INVALID_GEOMETRY = 7, FL_POISSON_DEGENERATE_FACE_DISTANCE = 8,
+    FL_POISSON_MISSING_BOUNDARY = 9, FL_POISSON_NUMERICS = 10, FL_POISSON_LINEAR = 11
+};
+/* UnstructuredMesh::from_cells for 2D polygons: points xy[2*npts], cell c's
+ * vertices cell_vtx[cell_off[c] .. cell_off[c+1]).  A MeshError returns
+ * FL_ERR_MESH with its kind in *mesh_error (may be NULL) and the text in
+ * fl_last_error. */
+fl_status fl_mesh2d_create(fl_ctx *ctx, int64_t npts, const double *xy, int64_t ncells,
+                           const int64_t *cell_off, const int64_t *cell_vtx, fl_mesh2d **out,
+                           int32_t *mesh_error);
+void fl_mesh2d_destroy(fl_mesh2d *mesh);
+fl_status fl_mesh2d_info(fl_mesh2d *mesh, int64_t *ncells, int64_t *nfaces);
+/* Cell centroids and areas, face owner / neighbour (-1 on the boundary),
+ * face centres and area vectors (out of the owner); any pointer may be NULL. */
+fl_status fl_mesh2d_get(fl_mesh2d *mesh, double *cx, double *cy, double *volume, int64_t *owner,
+                        int64_t *neighbour, double *fcx, double *fcy, double *sx, double *sy);
+/* assemble_poisson with Diffusivity::Constant(gamma) (Linear face
+ * interpolation, the default), optional cell source (host), per-face
+ * boundary kinds (0 ZeroGradient, 1 FixedValue with bc_value; NULL: every
+ * face ZeroGradient) and optional reference; the matrix stays on the device
+ * as a solver matrix for fl_cg / fl_pcg_jacobi / fl_bicgstab, the rhs comes
+ * back to the host.  A PoissonError returns FL_ERR_POISSON with its kind in
+ * *poisson_error and, for an incompatible pure-Neumann source, the
+ * integrated source in *detail (both may be NULL). */
+fl_status fl_mesh2d_assemble_poisson(fl_mesh2d *mesh, double gamma, const double *source,
+                                     const int32_t *bc_kind, const double *bc_value,
+                                     int32_t has_reference, int64_t ref_cell, double ref_value,
+                                     fl_csr **matrix, double *rhs, int32_t *poisson_error,
+                                     double *detail);
+/* Shape and host copies of a device CSR matrix (any pointer may be NULL). */
+fl_status fl_csr_info(fl_csr *matrix, int64_t *nrows, int64_t *ncols, int64_t *nnz);
+fl_status fl_csr_download(fl_csr *matrix, int64_t *row_offsets, int64_t *cols, double *vals);
 
 /* ---------------------------------------------------------------------- */
 /* On-disk outputs (src/output.rs) in the reference's formats and number

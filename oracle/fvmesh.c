@@ -16,9 +16,11 @@
  *                                         towards the owner's vertex average)
  *   cell_geometry (TwoD)                  src/mesh.rs:584-609  (shoelace centroid)
  *   orientation checks                    src/mesh.rs:396-408
- *   assemble_poisson                      src/poisson.rs:146-263 (constant Gamma:
- *                                         interpolate_diffusivity of a constant is
- *                                         that constant on every face)
+ *   assemble_poisson                      src/poisson.rs:146-263 (constant Gamma,
+ *                                         interpolated Linear to interior faces,
+ *                                         (1 - lambda) G + lambda G, as
+ *                                         interpolate_diffusivity does; the owner
+ *                                         value G on boundary faces)
  *   apply_reference                       src/poisson.rs:303-337
  *   CsrBuilder::finalize                  src/linear/csr.rs:159-192 (or_csr_finalize)
  */
@@ -217,7 +219,11 @@ int or_assemble_poisson(const or_mesh2d *m, double gamma, const double *source,
                           "degenerate owner-neighbour distance at face %lld", f);
                 break;
             }
-            const double coef = gamma * (m->sx[f] * dx + m->sy[f] * dy) / d2;
+            /* interpolate_diffusivity, Linear (numerics.rs:393-412): the
+             * constant interpolated with the face weight (numerics.rs:294-303) */
+            const double lambda = ((m->fcx[f] - m->cx[P]) * dx + (m->fcy[f] - m->cy[P]) * dy) / d2;
+            const double gamma_f = (1.0 - lambda) * gamma + lambda * gamma;
+            const double coef = gamma_f * (m->sx[f] * dx + m->sy[f] * dy) / d2;
             if (!isfinite(coef) || coef <= 0.0) {
                 rc = fail(message, cap, OR_POISSON_INVALID_GEOMETRY,
                           "invalid non-orthogonal geometry at face %lld", f);
