@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libflursys_hip.so")
 
 # status codes (fl_status)
 OK = 0
+ERR_CONFIG = 9
 STATUS_NAMES = {
     1: "DimensionMismatch", 2: "InvalidCsr", 3: "IndexOutOfBounds", 4: "NonFiniteValue",
     5: "MissingDiagonal", 6: "ZeroDiagonal", 7: "Breakdown", 8: "SolverError",
@@ -131,7 +132,8 @@ class Grid2DConfig(C.Structure):
         ("max_growth_factor", C.c_double), ("precision", C.c_int32), ("reserved", C.c_int32),
         ("inner_tolerance", C.c_double), ("pressure_solver", C.c_int32),
         ("inlet_profile", C.c_int32), ("pressure_omega", C.c_double),
-        ("inlet_step_height", C.c_double),
+        ("inlet_step_height", C.c_double), ("coupling", C.c_int32), ("reserved2", C.c_int32),
+        ("velocity_relaxation", C.c_double), ("pressure_relaxation", C.c_double),
     ]
 
 

@@ -72,6 +72,8 @@ struct Prob {
     double step_h, height;  // inlet profile: step height and domain height
     double omega;        // SOR relaxation factor
     uint32_t ref;        // SOR gauge: the first fluid cell (find_reference_cell)
+    int32_t simple;      // SIMPLE-style coupling (else projection)
+    double au, ap;       // velocity relaxation (1 for projection), pressure relaxation
 };
 
 struct PcgA {  // a grid's PCG state after phase A
@@ -264,6 +266,7 @@ __global__ void k_sor_check(const Prob *probs, const int *active, int count, Sor
 __global__ void __launch_bounds__(kT) k_predict(const Prob *probs, const uint32_t *tile_prob,
                                                 uint32_t ntiles, const int *active,
                                                 const double *dts, const uint8_t *flags,
+                                                const double *__restrict__ Pp,
                                                 const double *__restrict__ U,
                                                 const double *__restrict__ V,
                                                 double *__restrict__ US, double *__restrict__ VS) {
@@ -277,6 +280,7 @@ __global__ void __launch_bounds__(kT) k_predict(const Prob *probs, const uint32_
         const uint8_t *F = flags + (size_t)P.tile0 * kT;
         const double *u = U + P.uoff, *v = V + P.voff;
         double *us = US + P.uoff, *vs = VS + P.voff;
+        const double *p = Pp + (size_t)P.tile0 * kT;
         const double dt = dts[pi], dx = P.dx, dy = P.dy;
         if (i >= 1) {
             double val = 0.0;
@@ -291,7 +295,9 @@ __global__ void __launch_bounds__(kT) k_predict(const Prob *probs, const uint32_
                 const double pn = transported(P.scheme, vn, up, un_nb), ps = transported(P.scheme, vsv, us_nb, up);
                 const double conv = (ue * pe - uw * pw) / dx + (vn * pn - vsv * ps) / dy;
                 const double diff = (ue_nb - 2.0 * up + uw_nb) / (dx * dx) + (un_nb - 2.0 * up + us_nb) / (dy * dy);
-                val = up + dt * (-conv + P.nu * diff);
+                // SIMPLE: the relaxed step with the current pressure's gradient
+                val = P.simple ? up + P.au * dt * (-conv + P.nu * diff - (p[c] - p[c - 1]) / dx / P.rho)
+                               : up + dt * (-conv + P.nu * diff);
             }
             us[i + (nx + 1) * j] = val;
         } else {
@@ -311,7 +317,8 @@ __global__ void __launch_bounds__(kT) k_predict(const Prob *probs, const uint32_
                 const double pn = transported(P.scheme, vn, vp, vn_nb), ps = transported(P.scheme, vsv, vs_nb, vp);
                 const double conv = (ue * pe - uw * pw) / dx + (vn * pn - vsv * ps) / dy;
                 const double diff = (ve_nb - 2.0 * vp + vw_nb) / (dx * dx) + (vn_nb - 2.0 * vp + vs_nb) / (dy * dy);
-                val = vp + dt * (-conv + P.nu * diff);
+                val = P.simple ? vp + P.au * dt * (-conv + P.nu * diff - (p[c] - p[c - nx]) / dy / P.rho)
+                               : vp + dt * (-conv + P.nu * diff);
             }
             vs[i + nx * j] = val;
         } else {
@@ -350,7 +357,8 @@ __global__ void __launch_bounds__(kT) k_boundaries(const Prob *probs, const int 
     }
 }
 
-// build_pressure_rhs (:924-970): zero on solid and outlet cells.
+// build_pressure_rhs (:924-970): zero on solid and outlet cells; SIMPLE
+// scales by the velocity relaxation (:936-941).
 __global__ void __launch_bounds__(kT) k_rhs(const Prob *probs, const uint32_t *tile_prob,
                                             uint32_t ntiles, const int *active, const double *dts,
                                             const uint8_t *flags, const double *__restrict__ US,
@@ -370,12 +378,13 @@ __global__ void __launch_bounds__(kT) k_rhs(const Prob *probs, const uint32_t *t
         const uint32_t nx = P.nx;
         const double div = (us[i + 1 + (nx + 1) * j] - us[i + (nx + 1) * j]) / P.dx +
                            (vs[i + nx * (j + 1)] - vs[i + nx * j]) / P.dy;
-        RHS[base + c] = P.rho / dts[pi] * div;
+        RHS[base + c] = P.rho / (P.au * dts[pi]) * div;
     }
 }
 
-// correct_velocity (:1322-1379): u = u* - dt/rho grad p on open interior
-// faces, 0 on blocked ones, boundary faces copied from u*.
+// correct_velocity (:1322-1379): u = u* - a dt/rho grad p on open interior
+// faces (a = the velocity relaxation under SIMPLE, else 1), 0 on blocked
+// ones, boundary faces copied from u*.
 __global__ void __launch_bounds__(kT) k_correct(const Prob *probs, const uint32_t *tile_prob,
                                                 uint32_t ntiles, const int *active,
                                                 const double *dts, const uint8_t *flags,
@@ -394,7 +403,7 @@ __global__ void __launch_bounds__(kT) k_correct(const Prob *probs, const uint32_
         const double *p = Pp + (size_t)P.tile0 * kT;
         const double *us = US + P.uoff, *vs = VS + P.voff;
         double *u = U + P.uoff, *v = V + P.voff;
-        const double dor = dts[pi] / P.rho;
+        const double dor = P.au * dts[pi] / P.rho;
         if (i >= 1)
             u[i + (nx + 1) * j] = uopen(P, F, i, j)
                                       ? us[i + (nx + 1) * j] - dor * ((p[c] - p[c - 1]) / P.dx)
@@ -408,6 +417,32 @@ __global__ void __launch_bounds__(kT) k_correct(const Prob *probs, const uint32_
         else
             v[i] = vs[i];
         if (j == ny - 1) v[i + nx * ny] = vs[i + nx * ny];
+    }
+}
+
+// SIMPLE's pressure split (incompressible.rs:669-673, 683-694): before the
+// solve the pressure moves to `pold` and the solve starts from p' = 0; after
+// the correction p = p_old + alpha_p p'.  Projection grids are untouched.
+__global__ void __launch_bounds__(kT) k_simple_save(const Prob *probs, const uint32_t *tile_prob,
+                                                    uint32_t ntiles, const int *active,
+                                                    double *__restrict__ X, double *__restrict__ PO) {
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t pi = tile_prob[t];
+        if (!active[pi] || !probs[pi].simple) continue;
+        const size_t k = (size_t)t * kT + threadIdx.x;
+        PO[k] = X[k];
+        X[k] = 0.0;
+    }
+}
+__global__ void __launch_bounds__(kT) k_simple_combine(const Prob *probs, const uint32_t *tile_prob,
+                                                       uint32_t ntiles, const int *active,
+                                                       double *__restrict__ X,
+                                                       const double *__restrict__ PO) {
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t pi = tile_prob[t];
+        if (!active[pi] || !probs[pi].simple) continue;
+        const size_t k = (size_t)t * kT + threadIdx.x;
+        X[k] = PO[k] + probs[pi].ap * X[k];
     }
 }
 
@@ -1282,7 +1317,7 @@ __global__ void __launch_bounds__(NT) k_pcg(PcgArgs<T> A) {
 struct StepCtl {
     // TimeStepSettings / t_end (incompressible.rs:35-90)
     double dt0, t_end, min_dt, max_dt, target_cfl, target_diff, growth;
-    int32_t adaptive, pad0;
+    int32_t adaptive, simple;
     // state
     double time, current_dt, max_u, max_v;
     long long step, iters;
@@ -1314,7 +1349,7 @@ __global__ void k_run_setup(const Prob *probs, StepCtl *ctl, int count, int *act
     diag[p] = d;
     int act = 0;
     double dt = c.dt0;
-    if (c.error == RE_NONE && !ctl_reached_end(c.time, c.t_end)) {
+    if (c.error == RE_NONE && (c.simple || !ctl_reached_end(c.time, c.t_end))) {
         bool ok = true;
         if (c.adaptive) {
             const double rate = c.max_u / P.dx + c.max_v / P.dy;
@@ -1334,7 +1369,7 @@ __global__ void k_run_setup(const Prob *probs, StepCtl *ctl, int count, int *act
             }
         }
         if (ok) {
-            dt = fmin(dt, c.t_end - c.time);
+            if (!c.simple) dt = fmin(dt, c.t_end - c.time);
             if (dt > 0.0) {
                 act = 1;
                 if (c.adaptive) c.current_dt = dt;
@@ -1366,7 +1401,7 @@ __global__ void k_run_finish(StepCtl *ctl, int count, const int *active, const d
         return;
     }
     c.step += 1;
-    c.time += dts[p];
+    c.time = c.simple ? (double)c.step * c.dt0 : c.time + dts[p];
     const Diag &d = diag[p];
     for (int f = 0; f < 4; ++f)
         if (d.bad[f] != ~0ull) {
@@ -1450,6 +1485,9 @@ struct Batch2 {
     // SOR (FL_SOLVER_SOR): per-grid iteration state
     bool sor = false;
     SorState *sor_st = nullptr, *h_sor = nullptr;
+    // SIMPLE-style coupling on any grid: the pressure before the p' solve
+    bool simple = false;
+    double *pold = nullptr;
 };
 
 template <typename T>
@@ -1487,7 +1525,7 @@ void free_batch(Batch2 *B) {
     void *d[] = {B->probs, B->tile_prob, B->flags, B->x,  B->rhs, B->r,  B->d0,    B->d1,
                  B->q,     B->uc,        B->vc,    B->sp, B->vort, B->u, B->v,     B->us,
                  B->vs,    B->uo,        B->vo,    B->pa, B->pb,   B->p0, B->dpart, B->sa,
-                 B->sb,    B->fin,       B->active, B->dts, B->diag};
+                 B->sb,    B->fin,       B->active, B->dts, B->diag, B->pold};
     for (void *p : d)
         if (p) (void)hipFree(p);
     void *h[] = {B->h_active, B->h_fin, B->h_dts, B->h_sa, B->h_sb, B->h_diag, B->h_ctl};
@@ -1503,6 +1541,9 @@ void free_batch(Batch2 *B) {
     for (hipEvent_t e : B->ev) (void)hipEventDestroy(e);
 }
 
+// A relaxation factor of the config: 0 selects the reference's default.
+double relaxation(double v, double dflt) { return v == 0.0 ? dflt : v; }
+
 // SimulationConfig::validate (incompressible.rs:93-191), flow-only parts.
 fl_status validate2(fl_ctx *ctx, const fl_grid2d_config &c) {
     if (c.nx < 4 || c.ny < 4) return set_error(ctx, FL_ERR_CONFIG, "nx and ny must be at least 4");
@@ -1512,10 +1553,17 @@ fl_status validate2(fl_ctx *ctx, const fl_grid2d_config &c) {
         return set_error(ctx, FL_ERR_CONFIG, "dt must be finite and positive");
     if (!std::isfinite(c.t_end) || c.t_end < 0.0)
         return set_error(ctx, FL_ERR_CONFIG, "t_end must be finite and non-negative");
-    if (c.t_end == 0.0)
+    if (c.coupling != FL_COUPLING_PROJECTION && c.coupling != FL_COUPLING_SIMPLE)
+        return set_error(ctx, FL_ERR_CONFIG, "unknown coupling %d", c.coupling);
+    if (c.coupling == FL_COUPLING_PROJECTION && c.t_end == 0.0)
         return set_error(ctx, FL_ERR_CONFIG, "t_end must be positive for projection coupling");
     if (c.pressure_max_iters <= 0)
         return set_error(ctx, FL_ERR_CONFIG, "pressure_max_iters must be positive");
+    const double vr = relaxation(c.velocity_relaxation, 0.7), pr = relaxation(c.pressure_relaxation, 0.3);
+    if (!std::isfinite(vr) || vr <= 0.0 || vr > 1.0)
+        return set_error(ctx, FL_ERR_CONFIG, "velocity_relaxation must lie between 0 (exclusive) and 1");
+    if (!std::isfinite(pr) || pr <= 0.0 || pr > 1.0)
+        return set_error(ctx, FL_ERR_CONFIG, "pressure_relaxation must lie between 0 (exclusive) and 1");
     if (!(std::isfinite(c.pressure_tolerance) && c.pressure_tolerance > 0.0))
         return set_error(ctx, FL_ERR_CONFIG, "pressure_tolerance must be finite and positive");
     if (!(std::isfinite(c.density) && c.density > 0.0 && std::isfinite(c.kinematic_viscosity) &&
@@ -1550,6 +1598,8 @@ fl_status validate2(fl_ctx *ctx, const fl_grid2d_config &c) {
         !(std::isfinite(c.inlet_step_height) && c.inlet_step_height >= 0.0 && c.inlet_step_height < c.height))
         return set_error(ctx, FL_ERR_CONFIG, "inlet_step_height must lie in [0, height)");
     if (c.adaptive) {
+        if (c.coupling != FL_COUPLING_PROJECTION)
+            return set_error(ctx, FL_ERR_CONFIG, "adaptive time stepping requires transient projection coupling");
         if (!std::isfinite(c.min_dt) || c.min_dt <= 0.0)
             return set_error(ctx, FL_ERR_CONFIG, "adaptive min_dt must be finite and positive");
         if (!std::isfinite(c.max_dt) || c.max_dt <= 0.0)
@@ -1590,7 +1640,9 @@ fl_status adaptive_dt(fl_ctx *ctx, const fl_grid2d_config &c, double current, do
     return FL_OK;
 }
 
+// reached_end_time (incompressible.rs:557-561): SIMPLE never ends on time.
 bool reached_end(const Batch2 *B, int p) {
+    if (B->hp[p].simple) return false;
     const double t_end = B->cfg[p].t_end;
     const double tol = 2.220446049250313e-16 * std::max(std::fabs(t_end), 1.0) * 16.0;
     return B->time[p] >= t_end - tol;
@@ -1687,6 +1739,10 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
         P.step_h = c.inlet_step_height;
         P.height = c.height;
         P.omega = c.pressure_omega;
+        P.simple = c.coupling == FL_COUPLING_SIMPLE;
+        P.au = P.simple ? relaxation(c.velocity_relaxation, 0.7) : 1.0;
+        P.ap = relaxation(c.pressure_relaxation, 0.3);
+        if (P.simple) B->simple = true;
         const uint8_t *sol = solids ? solids[p] : nullptr;
         auto solid = [&](uint32_t i, uint32_t j) { return sol && sol[i + P.nx * j]; };
         hflags.resize((size_t)(tile + P.ntiles) * kT, 0);
@@ -1749,7 +1805,7 @@ fl_status build(Batch2 *B, fl_ctx *ctx, int count, const fl_grid2d_config *cfgs,
         (st = dalloc(B, &B->active, count)) || (st = dalloc(B, &B->dts, count)) ||
         (st = dalloc(B, &B->diag, count)) || (st = dalloc(B, &B->ctl, count)) ||
         (st = dalloc(B, &B->bar, 128)) || (st = dalloc(B, &B->e32, B->mixed ? B->ncell : 1)) ||
-        (st = dalloc(B, &B->sor_st, count)))
+        (st = dalloc(B, &B->sor_st, count)) || (st = dalloc(B, &B->pold, B->simple ? B->ncell : 1)))
         return st;
     if ((st = halloc(B, &B->h_sor, count)) != FL_OK) return st;
     if ((st = halloc(B, &B->h_active, count)) || (st = halloc(B, &B->h_fin, 2 * count)) ||
@@ -1951,10 +2007,12 @@ fl_status step_batch(Batch2 *B, fl_grid2d_step_report *reps, bool strict) {
                                        B->hp[p].dx, B->hp[p].dy, &dt);
             if (st != FL_OK) return st;
         }
-        dt = std::min(dt, c.t_end - B->time[p]);
-        if (dt <= 0.0) {
-            if (strict) return set_error(ctx, FL_ERR_SOLVER, "projection simulation has already reached t_end");
-            continue;
+        if (!B->hp[p].simple) {
+            dt = std::min(dt, c.t_end - B->time[p]);
+            if (dt <= 0.0) {
+                if (strict) return set_error(ctx, FL_ERR_SOLVER, "projection simulation has already reached t_end");
+                continue;
+            }
         }
         if (c.adaptive) B->current_dt[p] = dt;
         B->h_dts[p] = dt;
@@ -1968,10 +2026,11 @@ fl_status step_batch(Batch2 *B, fl_grid2d_step_report *reps, bool strict) {
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->vo, B->v, B->nvf * sizeof(double), hipMemcpyDeviceToDevice, s));
     const uint32_t G = ew_blocks(B);
     k_predict<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->dts, B->flags,
-                               B->u, B->v, B->us, B->vs);
+                               B->x, B->u, B->v, B->us, B->vs);
     k_boundaries<<<B->count, kT, 0, s>>>(B->probs, B->active, B->us, B->vs);
     k_rhs<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->dts, B->flags, B->us,
                            B->vs, B->rhs);
+    if (B->simple) k_simple_save<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->x, B->pold);
     FL_HIP_TRY(ctx, hipGetLastError());
     fl_status st = B->sor ? run_sor(B) : run_pcg(B);
     if (st != FL_OK || (st = barrier_check(B)) != FL_OK) return st;
@@ -1979,6 +2038,7 @@ fl_status step_batch(Batch2 *B, fl_grid2d_step_report *reps, bool strict) {
         if (B->h_active[p] && (st = pcg_status(B, p)) != FL_OK) return st;
     k_correct<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->dts, B->flags, B->x,
                                B->us, B->vs, B->u, B->v);
+    if (B->simple) k_simple_combine<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->x, B->pold);
     k_boundaries<<<B->count, kT, 0, s>>>(B->probs, B->active, B->u, B->v);
     FL_HIP_TRY(ctx, hipGetLastError());
     if ((st = run_cells(B)) != FL_OK) return st;
@@ -1987,7 +2047,7 @@ fl_status step_batch(Batch2 *B, fl_grid2d_step_report *reps, bool strict) {
         const Prob &P = B->hp[p];
         const Diag &dg = B->h_diag[p];
         B->step[p] += 1;
-        B->time[p] += B->h_dts[p];
+        B->time[p] = P.simple ? (double)B->step[p] * B->cfg[p].dt : B->time[p] + B->h_dts[p];
         // ensure_finite (:1707-1742): pressure, u, v, speed in that order
         const char *names[4] = {"pressure", "u", "v", "speed"};
         double *arrs[4] = {B->x + (size_t)P.tile0 * kT, B->u + P.uoff, B->v + P.voff,
@@ -2032,10 +2092,11 @@ fl_status launch_pre(Batch2 *B, hipStream_t s) {
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->uo, B->u, B->nuf * sizeof(double), hipMemcpyDeviceToDevice, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(B->vo, B->v, B->nvf * sizeof(double), hipMemcpyDeviceToDevice, s));
     k_predict<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->dts, B->flags,
-                               B->u, B->v, B->us, B->vs);
+                               B->x, B->u, B->v, B->us, B->vs);
     k_boundaries<<<B->count, kT, 0, s>>>(B->probs, B->active, B->us, B->vs);
     k_rhs<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->dts, B->flags, B->us,
                            B->vs, B->rhs);
+    if (B->simple) k_simple_save<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->x, B->pold);
     FL_HIP_TRY(ctx, hipGetLastError());
     return FL_OK;
 }
@@ -2044,6 +2105,7 @@ fl_status launch_post(Batch2 *B, hipStream_t s) {
     const uint32_t G = ew_blocks(B);
     k_correct<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->dts, B->flags, B->x,
                                B->us, B->vs, B->u, B->v);
+    if (B->simple) k_simple_combine<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->x, B->pold);
     k_boundaries<<<B->count, kT, 0, s>>>(B->probs, B->active, B->u, B->v);
     k_cells<<<G, kT, 0, s>>>(B->probs, B->tile_prob, B->ntiles, B->active, B->flags, B->x, B->u,
                              B->v, B->uo, B->vo, B->uc, B->vc, B->sp, B->vort, B->dpart, B->diag);
@@ -2113,6 +2175,7 @@ fl_status run_device(Batch2 *B, int64_t steps, bool graph, fl_batch2d_run_report
         k.target_diff = c.target_diffusion_number;
         k.growth = c.max_growth_factor;
         k.adaptive = c.adaptive;
+        k.simple = B->hp[p].simple;
         k.time = B->time[p];
         k.current_dt = B->current_dt[p];
         k.max_u = B->max_u[p];
@@ -2179,7 +2242,7 @@ fl_status run_device(Batch2 *B, int64_t steps, bool graph, fl_batch2d_run_report
             r.last_pressure_residual = k.last_residual;
             r.max_divergence = k.max_div;
             r.max_speed = k.max_speed;
-            r.reached_end_time = ctl_reached_end(k.time, k.t_end) ? 1 : 0;
+            r.reached_end_time = !k.simple && ctl_reached_end(k.time, k.t_end) ? 1 : 0;
             r.status = k.error == RE_NONE ? FL_OK : FL_ERR_SOLVER;
         }
     }

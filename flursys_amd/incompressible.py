@@ -1,5 +1,5 @@
-"""Host-side mirror of the 2D IncompressibleSolver (projection coupling) over
-the C ABI, plus the batched uneven-cavity driver.
+"""Host-side mirror of the 2D IncompressibleSolver (projection or SIMPLE-style
+coupling) over the C ABI, plus the batched uneven-cavity driver.
 
 Names follow src/solver/incompressible.rs: ``SimulationConfig`` (:58-90, the
 flow-only knobs), ``TimeStepSettings`` (:35-56), ``ConvectionScheme``
@@ -25,6 +25,8 @@ BC_VELOCITY, BC_SYMMETRY, BC_PRESSURE_OUTLET = 0, 1, 2
 LEFT, RIGHT, BOTTOM, TOP = 0, 1, 2, 3
 INLET_UNIFORM, INLET_PARABOLIC = 0, 1
 SOLVER_PCG, SOLVER_SOR = _lib.SOLVER_PCG, _lib.SOLVER_SOR
+# PressureVelocityCoupling (incompressible.rs:26-33)
+PROJECTION, SIMPLE = 0, 1
 
 
 @dataclass
@@ -41,7 +43,7 @@ class TimeStepSettings:
 
 @dataclass
 class SimulationConfig:
-    """SimulationConfig (incompressible.rs:58-90), projection coupling.
+    """SimulationConfig (incompressible.rs:58-90), flow-only knobs.
 
     The default case is the lid-driven cavity (src/cases/cavity.rs): unit
     square, lid u = 1 on top, nu = U L / Re."""
@@ -76,6 +78,11 @@ class SimulationConfig:
     # inlet_step_height)
     inlet_profile: int = 0
     inlet_step_height: float = 0.0
+    # PROJECTION (transient) or SIMPLE (steady pseudo-steps of size dt with
+    # the relaxation factors below, incompressible.rs:659-712)
+    coupling: int = PROJECTION
+    velocity_relaxation: float = 0.7
+    pressure_relaxation: float = 0.3
 
     @staticmethod
     def cavity(n, reynolds=100.0, lid_velocity=1.0, **kw):
@@ -104,6 +111,14 @@ class SimulationConfig:
         c.precision, c.inner_tolerance = self.precision, self.inner_tolerance
         c.pressure_solver, c.pressure_omega = self.pressure_solver, self.pressure_omega
         c.inlet_profile, c.inlet_step_height = self.inlet_profile, self.inlet_step_height
+        # the C struct reads 0 as "the default"; the reference rejects 0
+        for name in ("velocity_relaxation", "pressure_relaxation"):
+            if getattr(self, name) == 0.0:
+                raise _lib.FlursysError(_lib.ERR_CONFIG,
+                                        f"{name} must lie between 0 (exclusive) and 1")
+        c.coupling = self.coupling
+        c.velocity_relaxation, c.pressure_relaxation = (self.velocity_relaxation,
+                                                        self.pressure_relaxation)
         return c
 
 
@@ -133,7 +148,8 @@ class SolverStep:
 
 
 class IncompressibleSolver:
-    """GPU IncompressibleSolver (projection coupling, PCG pressure)."""
+    """GPU IncompressibleSolver (projection or SIMPLE coupling; PCG or SOR
+    pressure)."""
 
     def __init__(self, cfg: SimulationConfig, solid=None, ctx: _lib.Context | None = None):
         self.cfg = cfg

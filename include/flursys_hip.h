@@ -254,8 +254,20 @@ typedef struct {
     int32_t inlet_profile;
     double pressure_omega;
     double inlet_step_height;
+    /* PressureVelocityCoupling (incompressible.rs:26-33): FL_COUPLING_PROJECTION
+     * (transient, the default) or FL_COUPLING_SIMPLE (steady SIMPLE-style
+     * pseudo-steps, incompressible.rs:659-712: the predictor adds the relaxed
+     * pressure gradient, the solve is for a correction p' from zero, the
+     * velocity correction and the right-hand side carry velocity_relaxation,
+     * p += pressure_relaxation p'; time = step * dt and no end time; fixed dt
+     * only).  The relaxation factors lie in (0, 1]; 0 selects the reference's
+     * defaults, 0.7 and 0.3. */
+    int32_t coupling;
+    int32_t reserved2;
+    double velocity_relaxation, pressure_relaxation;
 } fl_grid2d_config;
 enum { FL_INLET_UNIFORM = 0, FL_INLET_PARABOLIC = 1 };
+enum { FL_COUPLING_PROJECTION = 0, FL_COUPLING_SIMPLE = 1 };
 
 /* SolverStep (incompressible.rs:196-214), flow-only fields. */
 typedef struct {

@@ -76,7 +76,17 @@ int or2d_init(or2d_state *s, const or2d_config *cfg, const uint8_t *solid) {
     /* SimulationConfig::validate (incompressible.rs:93-191), the parts this path uses */
     if (cfg->nx < 4 || cfg->ny < 4) return OR_ERR_CONFIG;
     if (!(isfinite(cfg->dt) && cfg->dt > 0)) return OR_ERR_CONFIG;
-    if (!(isfinite(cfg->t_end) && cfg->t_end > 0)) return OR_ERR_CONFIG;
+    if (!(isfinite(cfg->t_end) && cfg->t_end >= 0)) return OR_ERR_CONFIG;
+    const int simple = cfg->coupling == OR_COUPLING_SIMPLE;
+    if (cfg->coupling != OR_COUPLING_PROJECTION && !simple) return OR_ERR_CONFIG;
+    if (!simple && cfg->t_end == 0) return OR_ERR_CONFIG;
+    if (!(isfinite(cfg->velocity_relaxation) && cfg->velocity_relaxation > 0 &&
+          cfg->velocity_relaxation <= 1))
+        return OR_ERR_CONFIG;
+    if (!(isfinite(cfg->pressure_relaxation) && cfg->pressure_relaxation > 0 &&
+          cfg->pressure_relaxation <= 1))
+        return OR_ERR_CONFIG;
+    if (simple && cfg->adaptive) return OR_ERR_CONFIG;
     if (cfg->pressure_max_iters <= 0) return OR_ERR_CONFIG;
     if (!(cfg->pressure_omega > 0 && cfg->pressure_omega < 2)) return OR_ERR_CONFIG;
     if (!(isfinite(cfg->density) && cfg->density > 0 && isfinite(cfg->nu) && cfg->nu > 0))
@@ -97,6 +107,7 @@ int or2d_init(or2d_state *s, const or2d_config *cfg, const uint8_t *solid) {
     s->speed = calloc(s->n, 8); s->vort = calloc(s->n, 8);
     s->x = calloc(s->n, 8); s->b = calloc(s->n, 8); s->r = calloc(s->n, 8);
     s->z = calloc(s->n, 8); s->d = calloc(s->n, 8); s->q = calloc(s->n, 8);
+    s->pold = calloc(s->n, 8);
     s->current_dt = cfg->dt;
     /* initialize_velocity (incompressible.rs:588-620) with the cavity's zero
      * initial field: blocked faces 0, then the boundary velocities at t = 0. */
@@ -107,7 +118,7 @@ int or2d_init(or2d_state *s, const or2d_config *cfg, const uint8_t *solid) {
 
 void or2d_free(or2d_state *s) {
     void *f[] = {s->solid, s->p, s->rhs, s->u, s->v, s->us, s->vs, s->uold, s->vold, s->ucell,
-                 s->vcell, s->speed, s->vort, s->x, s->b, s->r, s->z, s->d, s->q};
+                 s->vcell, s->speed, s->vort, s->x, s->b, s->r, s->z, s->d, s->q, s->pold};
     for (size_t i = 0; i < sizeof(f) / sizeof(f[0]); ++i) free(f[i]);
     memset(s, 0, sizeof(*s));
 }
@@ -191,12 +202,16 @@ static inline double transported(int scheme, double fv, double left, double righ
 
 /* predict_momentum, incompressible.rs:748-922 (projection: no pressure
  * gradient, velocity relaxation 1, no buoyancy). */
+/* predict_momentum, incompressible.rs:748-922; SIMPLE adds the relaxed
+ * pressure gradient of the current pressure (:822-826, :848-853). */
 void or2d_predict(or2d_state *s, double dt) {
     set_threads(s->cfg.threads);
     const int64_t nx = s->cfg.nx, ny = s->cfg.ny;
     const double dx = s->dx, dy = s->dy, nu = s->cfg.nu;
     const int sc = s->cfg.convection;
-    const double *u = s->u, *v = s->v;
+    const double *u = s->u, *v = s->v, *p = s->p;
+    const int simple = s->cfg.coupling == OR_COUPLING_SIMPLE;
+    const double rho = s->cfg.density, au = s->cfg.velocity_relaxation;
     memset(s->us, 0, sizeof(double) * (nx + 1) * ny);
     memset(s->vs, 0, sizeof(double) * nx * (ny + 1));
 #pragma omp parallel for schedule(static)
@@ -213,7 +228,9 @@ void or2d_predict(or2d_state *s, double dt) {
             double pn = transported(sc, vn, up, un_nb), ps = transported(sc, vs, us_nb, up);
             double conv = (ue * pe - uw * pw) / dx + (vn * pn - vs * ps) / dy;
             double diff = (ue_nb - 2.0 * up + uw_nb) / (dx * dx) + (un_nb - 2.0 * up + us_nb) / (dy * dy);
-            s->us[UI(i, j)] = up + dt * (-conv + nu * diff);
+            s->us[UI(i, j)] =
+                simple ? up + au * dt * (-conv + nu * diff - (p[PI_(i, j)] - p[PI_(i - 1, j)]) / dx / rho)
+                       : up + dt * (-conv + nu * diff);
         }
 #pragma omp parallel for schedule(static)
     for (int64_t j = 1; j < ny; ++j)
@@ -229,14 +246,17 @@ void or2d_predict(or2d_state *s, double dt) {
             double pn = transported(sc, vn, vp, vn_nb), ps = transported(sc, vs, vs_nb, vp);
             double conv = (ue * pe - uw * pw) / dx + (vn * pn - vs * ps) / dy;
             double diff = (ve_nb - 2.0 * vp + vw_nb) / (dx * dx) + (vn_nb - 2.0 * vp + vs_nb) / (dy * dy);
-            s->vs[VI(i, j)] = vp + dt * (-conv + nu * diff);
+            s->vs[VI(i, j)] =
+                simple ? vp + au * dt * (-conv + nu * diff - (p[PI_(i, j)] - p[PI_(i, j - 1)]) / dy / rho)
+                       : vp + dt * (-conv + nu * diff);
         }
 }
 
 /* build_pressure_rhs, incompressible.rs:924-970 */
 void or2d_build_rhs(or2d_state *s, double dt) {
     const int64_t nx = s->cfg.nx, ny = s->cfg.ny;
-    const double rod = s->cfg.density / (1.0 * dt);
+    const double scale = s->cfg.coupling == OR_COUPLING_SIMPLE ? s->cfg.velocity_relaxation : 1.0;
+    const double rod = s->cfg.density / (scale * dt);
 #pragma omp parallel for schedule(static)
     for (int64_t j = 0; j < ny; ++j)
         for (int64_t i = 0; i < nx; ++i) {
@@ -491,7 +511,8 @@ int or2d_solve_pressure(or2d_state *s, or_solve_report *rep) {
 /* correct_velocity, incompressible.rs:1322-1379 (projection: scale 1) */
 void or2d_correct(or2d_state *s, double dt) {
     const int64_t nx = s->cfg.nx, ny = s->cfg.ny;
-    const double dor = 1.0 * dt / s->cfg.density;
+    const double scale = s->cfg.coupling == OR_COUPLING_SIMPLE ? s->cfg.velocity_relaxation : 1.0;
+    const double dor = scale * dt / s->cfg.density;
     memcpy(s->u, s->us, 8 * (nx + 1) * ny);
     memcpy(s->v, s->vs, 8 * nx * (ny + 1));
 #pragma omp parallel for schedule(static)
@@ -604,14 +625,21 @@ static double maxdiff(const double *a, const double *b, int64_t n) {
     return m;
 }
 
-/* advance_one_step, incompressible.rs:622-746 (projection coupling). */
+/* reached_end_time, incompressible.rs:557-561: SIMPLE never ends on time. */
+int or2d_reached_end(const or2d_state *s) {
+    const double tend = s->cfg.t_end;
+    const double ttol = 2.220446049250313e-16 * fmax(fabs(tend), 1.0) * 16.0;
+    return s->cfg.coupling == OR_COUPLING_PROJECTION && s->time >= tend - ttol;
+}
+
+/* advance_one_step, incompressible.rs:622-746, both couplings. */
 int or2d_step(or2d_state *s, or2d_diag *diag, or_solve_report *rep) {
     memset(rep, 0, sizeof(*rep));
     const int64_t nx = s->cfg.nx, ny = s->cfg.ny;
     const int64_t nu_ = (nx + 1) * ny, nv_ = nx * (ny + 1);
     const double tend = s->cfg.t_end;
-    const double ttol = 2.220446049250313e-16 * fmax(fabs(tend), 1.0) * 16.0;
-    if (s->time >= tend - ttol) {
+    const int simple = s->cfg.coupling == OR_COUPLING_SIMPLE;
+    if (or2d_reached_end(s)) {
         rep->status = OR_ERR_SOLVER;
         snprintf(rep->message, sizeof rep->message, "projection simulation has already reached t_end");
         return rep->status;
@@ -624,25 +652,40 @@ int or2d_step(or2d_state *s, or2d_diag *diag, or_solve_report *rep) {
                                   &dt, rep->message, sizeof rep->message);
         if (st) { rep->status = st; return st; }
     }
-    double remaining = tend - s->time;
-    dt = fmin(dt, remaining);
-    if (dt <= 0.0) {
-        rep->status = OR_ERR_SOLVER;
-        snprintf(rep->message, sizeof rep->message, "projection simulation has already reached t_end");
-        return rep->status;
+    if (!simple) {
+        double remaining = tend - s->time;
+        dt = fmin(dt, remaining);
+        if (dt <= 0.0) {
+            rep->status = OR_ERR_SOLVER;
+            snprintf(rep->message, sizeof rep->message, "projection simulation has already reached t_end");
+            return rep->status;
+        }
     }
     if (s->cfg.adaptive) s->current_dt = dt;
     memcpy(s->uold, s->u, 8 * nu_);
     memcpy(s->vold, s->v, 8 * nv_);
     or2d_predict(s, dt);
     or2d_apply_velocity_boundaries(s, s->us, s->vs);
+    if (simple) {
+        /* the solve is for p', not the absolute pressure (:669-673) */
+        memcpy(s->pold, s->p, 8 * s->n);
+        memset(s->p, 0, 8 * s->n);
+    }
     or2d_build_rhs(s, dt);
     int st = or2d_solve_pressure(s, rep);
     if (st) return st;
     or2d_correct(s, dt);
+    if (simple) {
+        /* p = p_old + alpha_p p' (:683-694) */
+        const double ap = s->cfg.pressure_relaxation;
+        for (int64_t k = 0; k < s->n; ++k) s->p[k] = s->pold[k] + ap * s->p[k];
+    }
     or2d_apply_velocity_boundaries(s, s->u, s->v);
     s->step += 1;
-    s->time += dt;
+    if (simple)
+        s->time = (double)s->step * s->cfg.dt;
+    else
+        s->time += dt;
     or2d_cell_fields(s);
     /* ensure_finite, incompressible.rs:1707-1742 */
     struct { const char *name; const double *f; int64_t n; } fs[] = {

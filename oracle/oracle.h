@@ -119,7 +119,12 @@ typedef struct {
      * above inlet_step_height */
     int32_t inlet_profile;
     double inlet_step_height;
+    /* PressureVelocityCoupling (incompressible.rs:26-33): OR_COUPLING_PROJECTION
+     * or OR_COUPLING_SIMPLE with its relaxation factors in (0, 1] */
+    int32_t coupling;
+    double velocity_relaxation, pressure_relaxation;
 } or2d_config;
+enum { OR_COUPLING_PROJECTION = 0, OR_COUPLING_SIMPLE = 1 };
 
 typedef struct {
     double time_step, pressure_residual, max_divergence, max_speed;
@@ -135,6 +140,7 @@ typedef struct {
     double *p, *rhs, *u, *v, *us, *vs, *uold, *vold;
     double *ucell, *vcell, *speed, *vort;
     double *x, *b, *r, *z, *d, *q;
+    double *pold;                /* SIMPLE: the pressure before the p' solve */
     int64_t step;
     double time, current_dt;
 } or2d_state;
@@ -143,6 +149,7 @@ int or2d_init(or2d_state *s, const or2d_config *cfg, const uint8_t *solid);
 void or2d_free(or2d_state *s);
 void or2d_apply_velocity_boundaries(or2d_state *s, double *u, double *v);
 void or2d_predict(or2d_state *s, double dt);
+int or2d_reached_end(const or2d_state *s);
 void or2d_build_rhs(or2d_state *s, double dt);
 int or2d_solve_pressure(or2d_state *s, or_solve_report *rep);
 void or2d_correct(or2d_state *s, double dt);

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 STOP_ABS_INF, STOP_REL_L2 = 0, 1
 GUESS_PREVIOUS, GUESS_ZERO = 0, 1
 SOLVER_PCG, SOLVER_SOR = 0, 1
+COUPLING_PROJECTION, COUPLING_SIMPLE = 0, 1
 UPWIND, CENTRAL = 0, 1
 BC_VELOCITY, BC_SYMMETRY, BC_OUTLET = 0, 1, 2
 LEFT, RIGHT, BOTTOM, TOP = 0, 1, 2, 3
@@ -88,6 +89,8 @@ class Config2D(C.Structure):
         ("target_diffusion", C.c_double), ("max_growth", C.c_double),
         ("t_end", C.c_double),
         ("inlet_profile", C.c_int32), ("inlet_step_height", C.c_double),
+        ("coupling", C.c_int32), ("velocity_relaxation", C.c_double),
+        ("pressure_relaxation", C.c_double),
     ]
 
 
@@ -108,7 +111,7 @@ class State2D(C.Structure):
         ("solid", C.POINTER(C.c_uint8)),
     ] + [(name, C.POINTER(C.c_double)) for name in (
         "p", "rhs", "u", "v", "us", "vs", "uold", "vold", "ucell", "vcell", "speed", "vort",
-        "x", "b", "r", "z", "d", "q")] + [
+        "x", "b", "r", "z", "d", "q", "pold")] + [
         ("step", C.c_int64), ("time", C.c_double), ("current_dt", C.c_double),
     ]
 
@@ -159,6 +162,7 @@ def lib():
         L.or2d_init.argtypes = [C.POINTER(State2D), C.POINTER(Config2D), C.c_void_p]
         L.or2d_free.argtypes = [C.POINTER(State2D)]
         L.or2d_step.argtypes = [C.POINTER(State2D), C.POINTER(Diag2D), C.POINTER(SolveReport)]
+        L.or2d_reached_end.argtypes = [C.POINTER(State2D)]
         L.or2d_solve_pressure.argtypes = [C.POINTER(State2D), C.POINTER(SolveReport)]
         L.or2d_predict.argtypes = [C.POINTER(State2D), C.c_double]
         L.or2d_build_rhs.argtypes = [C.POINTER(State2D), C.c_double]
@@ -304,7 +308,8 @@ class Cavity3D:
 # 2D solver
 # ----------------------------------------------------------------------------
 class Solver2D:
-    """Oracle mirror of IncompressibleSolver (src/solver/incompressible.rs), projection."""
+    """Oracle mirror of IncompressibleSolver (src/solver/incompressible.rs),
+    projection or SIMPLE-style coupling."""
 
     def __init__(self, nx, ny=None, *, length=1.0, height=1.0, density=1.0, nu=0.01,
                  dt=1.0e-3, t_end=1.0e30, pressure_max_iters=800, pressure_tolerance=1.0e-6,
@@ -313,7 +318,8 @@ class Solver2D:
                  bc_kind=(BC_VELOCITY,) * 4, bc_u=(0.0, 0.0, 0.0, 1.0), bc_v=(0.0,) * 4,
                  outlet_pressure=(0.0,) * 4, adaptive=False, min_dt=1.0e-6, max_dt=1.0e-2,
                  target_cfl=0.5, target_diffusion=0.25, max_growth=1.1, solid=None,
-                 inlet_profile=0, inlet_step_height=0.0):
+                 inlet_profile=0, inlet_step_height=0.0, coupling=COUPLING_PROJECTION,
+                 velocity_relaxation=0.7, pressure_relaxation=0.3):
         ny = nx if ny is None else ny
         cfg = Config2D()
         cfg.nx, cfg.ny, cfg.length, cfg.height = nx, ny, length, height
@@ -330,6 +336,8 @@ class Solver2D:
         cfg.min_dt, cfg.max_dt, cfg.target_cfl = min_dt, max_dt, target_cfl
         cfg.target_diffusion, cfg.max_growth, cfg.t_end = target_diffusion, max_growth, t_end
         cfg.inlet_profile, cfg.inlet_step_height = inlet_profile, inlet_step_height
+        cfg.coupling, cfg.velocity_relaxation = coupling, velocity_relaxation
+        cfg.pressure_relaxation = pressure_relaxation
         self.cfg = cfg
         self.s = State2D()
         self._solid = None
@@ -395,6 +403,9 @@ class Solver2D:
     @property
     def time(self):
         return self.s.time
+
+    def reached_end(self):
+        return bool(lib().or2d_reached_end(C.byref(self.s)))
 
 
 def stability_numbers(dt, nu, dx, dy, umax, vmax):

@@ -1,5 +1,5 @@
-"""GPU parity of the 2D projection path (IncompressibleSolver, batched cavities)
-through the C ABI against the oracle.
+"""GPU parity of the 2D path (IncompressibleSolver with projection or
+SIMPLE-style coupling, batched cavities) through the C ABI against the oracle.
 
 Bars: pressure solves within 1 iteration of the oracle and p within 1e-9 of
 max|p| (the device applies the reference's zero-mean projections through the
@@ -39,7 +39,9 @@ def oracle_for(cfg: I.SimulationConfig, solid=None, threads=0):
                       outlet_pressure=cfg.outlet_pressure, adaptive=ts.adaptive,
                       min_dt=ts.min_dt, max_dt=ts.max_dt, target_cfl=ts.target_cfl,
                       target_diffusion=ts.target_diffusion_number,
-                      max_growth=ts.max_growth_factor, solid=solid)
+                      max_growth=ts.max_growth_factor, solid=solid, coupling=cfg.coupling,
+                      velocity_relaxation=cfg.velocity_relaxation,
+                      pressure_relaxation=cfg.pressure_relaxation)
 
 
 def channel_cfg(nx, ny, **kw):
@@ -532,3 +534,92 @@ def test_parabolic_inlet_matches_oracle(gpu_ctx, step_h):
     for name in ("u", "v"):
         assert np.abs(f[name] - o.field(name)).max() < 1e-9, name
     assert rel(f["p"], o.field("p")) < 1e-9
+
+
+# ------------------------------------------------ SIMPLE-style coupling
+@pytest.mark.parametrize("case,guess", [("obstacle", fa.GUESS_ZERO), ("obstacle", fa.GUESS_PREVIOUS),
+                                        ("channel", fa.GUESS_PREVIOUS)])
+def test_simple_steps_match_oracle(gpu_ctx, case, guess):
+    """SIMPLE pseudo-steps (incompressible.rs:659-712) with the reference's
+    relaxation defaults: the p' solve starts from zero whatever the warm-start
+    setting, fields within 1e-9 of the scale, time = step * dt, no end time
+    (t_end = 0 is valid under SIMPLE)."""
+    print(f"[gpu] 2D SIMPLE {case} guess={guess}", flush=True)
+    nx, ny = 36, 28
+    kw = dict(dt=4e-3, pressure_tolerance=1e-10, stop_rule=fa.STOP_REL_L2, initial_guess=guess,
+              pressure_max_iters=5000, kinematic_viscosity=0.01, coupling=I.SIMPLE, t_end=0.0)
+    solid = None
+    if case == "channel":
+        cfg = channel_cfg(nx, ny, inlet_profile=I.INLET_PARABOLIC, **kw)
+    else:
+        cfg = I.SimulationConfig(nx=nx, ny=ny, **kw)
+        solid = block_mask(nx, ny)
+    o = oracle_for(cfg, solid)
+    s = I.IncompressibleSolver(cfg, solid=solid, ctx=gpu_ctx)
+    for k in range(1, 16):
+        d, orep = o.step()
+        st = s.advance()
+        assert st.pressure_converged and abs(st.pressure_iterations - orep.iterations) <= 1
+        assert st.iteration == k and st.time == k * cfg.dt and not st.reached_end_time
+        assert st.time_step == cfg.dt
+    f = s.fields()
+    uscale = max(np.abs(o.field("u")).max(), np.abs(o.field("v")).max())
+    assert np.abs(f["u"] - o.field("u")).max() < 1e-9 * uscale
+    assert np.abs(f["v"] - o.field("v")).max() < 1e-9 * uscale
+    assert rel(f["p"], o.field("p")) < 1e-8
+    assert abs(st.momentum_residual - d.velocity_change) <= 1e-9 * uscale
+
+
+def test_simple_sor_steps_match_oracle(gpu_ctx):
+    cfg = I.SimulationConfig.cavity(16, dt=2e-3, pressure_solver=I.SOLVER_SOR, pressure_omega=1.7,
+                                    pressure_tolerance=1e-10, pressure_max_iters=50000,
+                                    coupling=I.SIMPLE, velocity_relaxation=0.8,
+                                    pressure_relaxation=0.5)
+    o = oracle_for(cfg)
+    s = I.IncompressibleSolver(cfg, ctx=gpu_ctx)
+    for _ in range(4):
+        o.step()
+        assert s.advance().pressure_converged
+    f = s.fields()
+    for name in ("u", "v"):
+        assert np.abs(f[name] - o.field(name)).max() < 1e-8, name
+    assert rel(f["p"], o.field("p")) < 1e-7
+
+
+def test_simple_and_projection_grids_in_one_device_run(gpu_ctx):
+    """A batch holding SIMPLE and projection grids, stepped on the device
+    (captured graphs) and on the host: each grid equals its oracle run."""
+    print("[gpu] SIMPLE + projection batch device run", flush=True)
+    common = dict(pressure_tolerance=1e-10, stop_rule=fa.STOP_REL_L2, initial_guess=fa.GUESS_ZERO,
+                  pressure_max_iters=5000)
+    cfgs = [I.SimulationConfig.cavity(24, reynolds=100.0, dt=3e-3, coupling=I.SIMPLE, **common),
+            I.SimulationConfig.cavity(20, reynolds=400.0, dt=1e-3, t_end=5e-3, **common),
+            I.SimulationConfig.cavity(30, reynolds=200.0, dt=2e-3, coupling=I.SIMPLE,
+                                      velocity_relaxation=0.5, pressure_relaxation=0.9, **common)]
+    dev = I.CavityBatch(cfgs, ctx=gpu_ctx)
+    reps, _ = dev.run(8, use_graphs=True)
+    host = I.CavityBatch(cfgs, ctx=gpu_ctx)
+    for _ in range(8):
+        host.step()
+    assert [r.steps for r in reps] == [8, 5, 8]
+    assert not reps[0].reached_end_time and reps[1].reached_end_time
+    assert reps[0].time == 8 * 3e-3 and reps[2].time == 8 * 2e-3
+    for k, c in enumerate(cfgs):
+        o = oracle_for(c)
+        for _ in range(reps[k].steps):
+            o.step()
+        for f in (dev.fields(k), host.fields(k)):
+            uscale = max(np.abs(o.field("u")).max(), np.abs(o.field("v")).max())
+            assert np.abs(f["u"] - o.field("u")).max() < 1e-9 * uscale
+            assert rel(f["p"], o.field("p")) < 1e-8
+
+
+def test_simple_validation_texts(gpu_ctx):
+    bad = I.SimulationConfig(coupling=I.SIMPLE, time_step=I.TimeStepSettings(adaptive=True))
+    with pytest.raises(fa.FlursysError, match="adaptive time stepping requires transient projection coupling"):
+        I.IncompressibleSolver(bad, ctx=gpu_ctx)
+    for name, val in (("velocity_relaxation", 0.0), ("velocity_relaxation", 1.5),
+                      ("pressure_relaxation", -0.1), ("pressure_relaxation", float("nan"))):
+        cfg = I.SimulationConfig(coupling=I.SIMPLE, **{name: val})
+        with pytest.raises(fa.FlursysError, match=f"{name} must lie between 0 \\(exclusive\\) and 1"):
+            I.IncompressibleSolver(cfg, ctx=gpu_ctx)
