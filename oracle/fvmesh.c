@@ -41,7 +41,7 @@ static int fail(char *message, int cap, int code, const char *fmt, long long a) 
 
 void or_mesh2d_free(or_mesh2d *m) {
     void *f[] = {m->cx, m->cy, m->vol, m->owner, m->neigh, m->fcx, m->fcy, m->sx, m->sy,
-                 m->fv0, m->fv1};
+                 m->fv0, m->fv1, m->cell_off, m->cell_faces};
     for (size_t i = 0; i < sizeof(f) / sizeof(f[0]); ++i) free(f[i]);
     memset(m, 0, sizeof(*m));
 }
@@ -81,6 +81,9 @@ int or_mesh2d_from_cells(int64_t npts, const double *xy, int64_t ncells, const i
     m->sy = malloc(sizeof(double) * nedge_max);
     m->fv0 = malloc(sizeof(int64_t) * nedge_max);
     m->fv1 = malloc(sizeof(int64_t) * nedge_max);
+    m->cell_off = malloc(sizeof(int64_t) * (ncells + 1));
+    m->cell_faces = malloc(sizeof(int64_t) * (nedge_max ? nedge_max : 1));
+    memcpy(m->cell_off, cell_off, sizeof(int64_t) * (ncells + 1));
     m->ncells = ncells;
     int rc = OR_OK;
     int64_t nf = 0;
@@ -134,6 +137,7 @@ int or_mesh2d_from_cells(int64_t npts, const double *xy, int64_t ncells, const i
                     break;
                 }
                 m->neigh[f] = c;
+                m->cell_faces[o + t] = f;
             } else {
                 tab[h].a = ka;
                 tab[h].b = kb;
@@ -146,6 +150,7 @@ int or_mesh2d_from_cells(int64_t npts, const double *xy, int64_t ncells, const i
                 m->sy[nf] = sy;
                 m->fv0[nf] = a;
                 m->fv1[nf] = b;
+                m->cell_faces[o + t] = nf;
                 nf++;
             }
         }

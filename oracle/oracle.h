@@ -199,6 +199,7 @@ typedef struct {
     int64_t *owner, *neigh;       /* neigh = -1 on boundary faces */
     double *fcx, *fcy, *sx, *sy;  /* face centre and area vector (out of the owner) */
     int64_t *fv0, *fv1;           /* oriented face vertices */
+    int64_t *cell_off, *cell_faces; /* Cell::faces: the faces of cell c in its edge order */
 } or_mesh2d;
 /* MeshError kinds (src/mesh.rs:270-310), the ones 2D polygons can raise */
 enum {
@@ -282,6 +283,83 @@ int or_assemble_momentum(const or_mesh2d *m, double nu, const double *flux, cons
                          const double *relax_field, double alpha, int64_t *nnz_out, int64_t **off_out,
                          int64_t **col_out, double **val_out, double **rhs_out, double **diag_out,
                          char *message, int cap);
+
+/* ============ collocated unstructured SIMPLE (simple2d.c) ============ */
+enum {
+    OR_SIMPLE_INVALID_MOM_DIAG = 54,
+    OR_SIMPLE_INVALID_RELAX = 55,
+    OR_SIMPLE_INVALID_OUTER = 56,
+    OR_SIMPLE_MOM_NOT_CONVERGED = 57,
+    OR_SIMPLE_P_NOT_CONVERGED = 58,
+    /* NumericsError kinds of the least-squares stencil */
+    OR_NUM_SINGULAR_LSQ = 80,
+    OR_NUM_DEGENERATE_ON = 81,
+    OR_NUM_DEGENERATE_OF = 82,
+    OR_NUM_MISSING_BOUNDARY = 83,
+};
+enum { OR_LSQ_SKIP = 0, OR_LSQ_NEIGHBOUR = 1, OR_LSQ_FIXED = 2 };
+/* LeastSquaresGradientStencil for 2D meshes: one equation slot per cell face
+ * (the mesh's cell_faces order); ZeroGradient faces are skipped. */
+typedef struct {
+    int64_t ncells;
+    int32_t *kind;       /* OR_LSQ_* */
+    int64_t *nb;         /* the neighbouring cell (OR_LSQ_NEIGHBOUR) */
+    double *val;         /* the boundary value (OR_LSQ_FIXED) */
+    double *rcx, *rcy;   /* rhs coefficient offset / |offset|^2 */
+    double *inv;         /* per cell: the inverse 2x2 normal matrix, row-major */
+} or_lsq2d;
+int or_lsq2d_build(const or_mesh2d *m, const int32_t *bc_kind, const double *bc_value,
+                   or_lsq2d *st, int64_t *bad, char *message, int cap);
+void or_lsq2d_free(or_lsq2d *st);
+void or_lsq2d_gradient(const or_mesh2d *m, const or_lsq2d *st, const double *values, double *gx,
+                       double *gy);
+int or_initial_face_flux(const or_mesh2d *m, const double *ux, const double *uy,
+                         const double *wall_flux, double *flux, int64_t *bad, char *message, int cap);
+double or_continuity_rms(const or_mesh2d *m, const double *flux);
+int or_rhie_chow_flux(const or_mesh2d *m, const or_lsq2d *pst, const double *usx, const double *usy,
+                      const double *p, const double *r_au, const double *wall_flux, double *flux,
+                      int64_t *bad, double *badv, char *message, int cap);
+int or_correct_face_flux(const or_mesh2d *m, const double *predicted, const double *coef,
+                         const double *pc, const int32_t *corr_kind, const double *corr_value,
+                         double *out, int64_t *bad, double *badv, char *message, int cap);
+int or_correct_cell_velocity(int64_t nc, const double *usx, const double *usy, const double *r_au,
+                             const double *gx, const double *gy, double *ux, double *uy, int64_t *bad,
+                             double *badv, char *message, int cap);
+/* SimpleOptions (simple.rs:93-131) with a constant viscosity; the velocity
+ * boundary is per boundary face (OR_SCALAR_* kind, x and y values) and the
+ * pressure-correction boundary, when given (corr_kind != NULL), anchors the
+ * correction instead of reference_cell. */
+typedef struct {
+    double viscosity;
+    int64_t reference_cell;
+    int64_t max_outer_iterations;
+    double continuity_tol;
+    double mom_atol, mom_rtol;
+    int64_t mom_max;
+    double p_atol, p_rtol;
+    int64_t p_max;
+    double alpha_u, alpha_p;
+} or_simple_opts;
+/* SimpleReport (simple.rs:133-142); the z momentum report is 0 iterations */
+typedef struct {
+    int32_t converged, pad;
+    int64_t outer_iterations;
+    double initial_rms, final_rms;
+    int64_t mom_iters[3];
+    int32_t mom_converged[3], p_converged;
+    double mom_final[3];
+    int64_t p_iters;
+    double p_final;
+    int64_t err_index;
+    double err_value;
+} or_simple_report;
+/* solve_simple: advances (ux, uy, p, flux) in place; history[0 .. iterations]
+ * receives the continuity RMS (capacity max_outer_iterations + 1). */
+int or_simple2d_solve(const or_mesh2d *m, const or_lsq2d *pst, const int32_t *vel_kind,
+                      const double *vel_x, const double *vel_y, const int32_t *corr_kind,
+                      const double *corr_value, const or_simple_opts *o, double *ux, double *uy,
+                      double *p, double *flux, double *history, or_simple_report *rep,
+                      char *message, int cap);
 
 /* ============ unstructured triangle-mesh Poisson (config-3 producer) ============ */
 /* Builds the jittered structured triangulation of the unit square with m x m

@@ -132,7 +132,30 @@ class Mesh2DStruct(C.Structure):
         (name, C.POINTER(C.c_double)) for name in ("cx", "cy", "vol")] + [
         (name, C.POINTER(C.c_int64)) for name in ("owner", "neigh")] + [
         (name, C.POINTER(C.c_double)) for name in ("fcx", "fcy", "sx", "sy")] + [
-        (name, C.POINTER(C.c_int64)) for name in ("fv0", "fv1")]
+        (name, C.POINTER(C.c_int64)) for name in ("fv0", "fv1", "cell_off", "cell_faces")]
+
+
+class Lsq2DStruct(C.Structure):
+    _fields_ = [("ncells", C.c_int64), ("kind", C.POINTER(C.c_int32)),
+                ("nb", C.POINTER(C.c_int64))] + [
+        (name, C.POINTER(C.c_double)) for name in ("val", "rcx", "rcy", "inv")]
+
+
+class SimpleOpts(C.Structure):
+    _fields_ = [("viscosity", C.c_double), ("reference_cell", C.c_int64),
+                ("max_outer_iterations", C.c_int64), ("continuity_tol", C.c_double),
+                ("mom_atol", C.c_double), ("mom_rtol", C.c_double), ("mom_max", C.c_int64),
+                ("p_atol", C.c_double), ("p_rtol", C.c_double), ("p_max", C.c_int64),
+                ("alpha_u", C.c_double), ("alpha_p", C.c_double)]
+
+
+class SimpleReportC(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("pad", C.c_int32), ("outer_iterations", C.c_int64),
+                ("initial_rms", C.c_double), ("final_rms", C.c_double),
+                ("mom_iters", C.c_int64 * 3), ("mom_converged", C.c_int32 * 3),
+                ("p_converged", C.c_int32), ("mom_final", C.c_double * 3),
+                ("p_iters", C.c_int64), ("p_final", C.c_double), ("err_index", C.c_int64),
+                ("err_value", C.c_double)]
 
 
 def build() -> None:
@@ -213,6 +236,29 @@ def lib():
             C.POINTER(Mesh2DStruct), C.c_double, _f64p, C.c_void_p, C.c_void_p, C.c_void_p,
             C.c_void_p, C.c_double, C.c_void_p, C.c_double, C.POINTER(C.c_int64), P64, P64, PD,
             PD, PD, C.c_char_p, C.c_int]
+        L.or_lsq2d_build.argtypes = [C.POINTER(Mesh2DStruct), C.c_void_p, C.c_void_p,
+                                     C.POINTER(Lsq2DStruct), C.POINTER(C.c_int64), C.c_char_p,
+                                     C.c_int]
+        L.or_lsq2d_free.argtypes = [C.POINTER(Lsq2DStruct)]
+        L.or_lsq2d_gradient.argtypes = [C.POINTER(Mesh2DStruct), C.POINTER(Lsq2DStruct), _f64p,
+                                        _f64p, _f64p]
+        L.or_initial_face_flux.argtypes = [C.POINTER(Mesh2DStruct), _f64p, _f64p, _f64p, _f64p,
+                                           C.POINTER(C.c_int64), C.c_char_p, C.c_int]
+        L.or_continuity_rms.argtypes = [C.POINTER(Mesh2DStruct), _f64p]
+        L.or_continuity_rms.restype = C.c_double
+        L.or_rhie_chow_flux.argtypes = [C.POINTER(Mesh2DStruct), C.POINTER(Lsq2DStruct), _f64p,
+                                        _f64p, _f64p, _f64p, _f64p, _f64p, C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_double), C.c_char_p, C.c_int]
+        L.or_correct_face_flux.argtypes = [C.POINTER(Mesh2DStruct), _f64p, _f64p, _f64p,
+                                           C.c_void_p, C.c_void_p, _f64p, C.POINTER(C.c_int64),
+                                           C.POINTER(C.c_double), C.c_char_p, C.c_int]
+        L.or_correct_cell_velocity.argtypes = [C.c_int64, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                               _f64p, _f64p, C.POINTER(C.c_int64),
+                                               C.POINTER(C.c_double), C.c_char_p, C.c_int]
+        L.or_simple2d_solve.argtypes = [
+            C.POINTER(Mesh2DStruct), C.POINTER(Lsq2DStruct), C.c_void_p, C.c_void_p, C.c_void_p,
+            C.c_void_p, C.c_void_p, C.POINTER(SimpleOpts), _f64p, _f64p, _f64p, _f64p, _f64p,
+            C.POINTER(SimpleReportC), C.c_char_p, C.c_int]
         L.or_free.argtypes = [C.c_void_p]
         L.or_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
         L.or_splitmix64.restype = C.c_uint64
@@ -774,3 +820,173 @@ def triangulated_square(n, jitter=0.2, seed=0):
             a = i + (n + 1) * j
             cells += [[a, a + 1, a + n + 2], [a, a + n + 2, a + n + 1]]
     return Mesh2D(pts, cells)
+
+
+# ----------------------------------------------------------------------------
+# collocated unstructured SIMPLE (oracle/simple2d.c)
+# ----------------------------------------------------------------------------
+SIMPLE_ERRORS = {50: "InvalidPressureReference", 51: "InvalidPressureFaceCoefficient",
+                 53: "Linear", 54: "InvalidMomentumDiagonal", 55: "InvalidPressureRelaxation",
+                 56: "InvalidOuterIterations", 57: "MomentumLinearDidNotConverge",
+                 58: "PressureLinearDidNotConverge", 80: "SingularLeastSquaresStencil",
+                 81: "DegenerateOwnerNeighbourDistance", 82: "DegenerateOwnerFaceDistance",
+                 83: "MissingBoundaryCondition"}
+
+
+class SimpleError(RuntimeError):
+    """SimpleError / NumericsError of the SIMPLE oracle (kind, index, value)."""
+
+    def __init__(self, code, message, index=0, value=0.0):
+        self.code, self.kind = code, SIMPLE_ERRORS.get(code, str(code))
+        self.index, self.value = index, value
+        super().__init__(f"{self.kind}: {message}")
+
+
+def _scalar_boundary(mesh, boundary):
+    """Per-face (kind, value) arrays of a {face: "ZeroGradient" | ("FixedValue", v)}
+    map; faces not in the map are -1 (missing)."""
+    nf = mesh.m.nfaces
+    kind = np.full(nf, -1, np.int32)
+    value = np.zeros(nf)
+    for f, bc in boundary.items():
+        if bc == "ZeroGradient":
+            kind[f] = ZERO_GRADIENT
+        else:
+            kind[f], value[f] = FIXED_VALUE, bc[1]
+    return kind, value
+
+
+class LsqStencil:
+    """LeastSquaresGradientStencil::new (src/numerics.rs:106-200) on a Mesh2D
+    with a {face: "ZeroGradient" | ("FixedValue", v)} boundary."""
+
+    def __init__(self, mesh, boundary):
+        self.mesh = mesh
+        self.s = Lsq2DStruct()
+        kind, value = _scalar_boundary(mesh, boundary)
+        bad = C.c_int64()
+        msg = C.create_string_buffer(256)
+        st = lib().or_lsq2d_build(C.byref(mesh.m), kind.ctypes.data, value.ctypes.data,
+                                  C.byref(self.s), C.byref(bad), msg, 256)
+        if st != 0:
+            raise SimpleError(st, msg.value.decode(), bad.value)
+
+    def __del__(self):
+        try:
+            lib().or_lsq2d_free(C.byref(self.s))
+        except Exception:
+            pass
+
+    def gradient(self, values):
+        nc = self.mesh.m.ncells
+        gx, gy = np.zeros(nc), np.zeros(nc)
+        lib().or_lsq2d_gradient(C.byref(self.mesh.m), C.byref(self.s), _f64(values), gx, gy)
+        return np.stack([gx, gy], 1)
+
+
+def initial_face_flux(mesh, velocity, wall_flux):
+    """initial_face_flux (src/simple.rs:719-744); velocity is (ncells, 2)."""
+    v = _f64(velocity)
+    out = np.zeros(mesh.m.nfaces)
+    bad = C.c_int64()
+    msg = C.create_string_buffer(256)
+    st = lib().or_initial_face_flux(C.byref(mesh.m), _f64(v[:, 0]), _f64(v[:, 1]),
+                                    _f64(wall_flux), out, C.byref(bad), msg, 256)
+    if st != 0:
+        raise SimpleError(st, msg.value.decode(), bad.value)
+    return out
+
+
+def continuity_rms(mesh, flux):
+    """continuity_rms (src/simple.rs:703-717)."""
+    return lib().or_continuity_rms(C.byref(mesh.m), _f64(flux))
+
+
+def rhie_chow_flux(mesh, stencil, predictor, pressure, r_au, wall_flux):
+    """rhie_chow_predicted_flux (src/simple.rs:746-805)."""
+    v = _f64(predictor)
+    out = np.zeros(mesh.m.nfaces)
+    bad, badv = C.c_int64(), C.c_double()
+    msg = C.create_string_buffer(256)
+    st = lib().or_rhie_chow_flux(C.byref(mesh.m), C.byref(stencil.s), _f64(v[:, 0]),
+                                 _f64(v[:, 1]), _f64(pressure), _f64(r_au), _f64(wall_flux), out,
+                                 C.byref(bad), C.byref(badv), msg, 256)
+    if st != 0:
+        raise SimpleError(st, msg.value.decode(), bad.value, badv.value)
+    return out
+
+
+def correct_face_flux(mesh, predicted, coefficients, correction, correction_boundary=None):
+    """correct_face_flux(_with_boundary) (src/simple.rs:407-484)."""
+    out = np.zeros(mesh.m.nfaces)
+    kind = value = None
+    if correction_boundary is not None:
+        kind, value = _scalar_boundary(mesh, correction_boundary)
+    bad, badv = C.c_int64(), C.c_double()
+    msg = C.create_string_buffer(256)
+    st = lib().or_correct_face_flux(C.byref(mesh.m), _f64(predicted), _f64(coefficients),
+                                    _f64(correction), None if kind is None else kind.ctypes.data,
+                                    None if value is None else value.ctypes.data, out,
+                                    C.byref(bad), C.byref(badv), msg, 256)
+    if st != 0:
+        raise SimpleError(st, msg.value.decode(), bad.value, badv.value)
+    return out
+
+
+def correct_cell_velocity(predictor, r_au, gradient):
+    """correct_cell_velocity_from_gradient (src/simple.rs:670-701)."""
+    v, g = _f64(predictor), _f64(gradient)
+    n = len(v)
+    ux, uy = np.zeros(n), np.zeros(n)
+    bad, badv = C.c_int64(), C.c_double()
+    msg = C.create_string_buffer(256)
+    st = lib().or_correct_cell_velocity(n, _f64(v[:, 0]), _f64(v[:, 1]), _f64(r_au),
+                                        _f64(g[:, 0]), _f64(g[:, 1]), ux, uy, C.byref(bad),
+                                        C.byref(badv), msg, 256)
+    if st != 0:
+        raise SimpleError(st, msg.value.decode(), bad.value, badv.value)
+    return np.stack([ux, uy], 1)
+
+
+def solve_simple(mesh, stencil, velocity_boundary, velocity, pressure, flux, viscosity,
+                 correction_boundary=None, reference_cell=0, max_outer_iterations=200,
+                 continuity_tol=1.0e-10, momentum_solver=(1.0e-10, 1.0e-8, 1000),
+                 pressure_solver=(1.0e-10, 1.0e-8, 1000), velocity_relaxation=0.7,
+                 pressure_relaxation=0.3):
+    """solve_simple (src/simple.rs:148-311) with SimpleOptions::steady defaults.
+    `velocity_boundary` maps every boundary face to (ux, uy) (FixedVelocity)
+    or "ZeroGradient".  Returns (report dict, velocity, pressure, flux,
+    continuity history)."""
+    nf = mesh.m.nfaces
+    vk = np.full(nf, -1, np.int32)
+    vx, vy = np.zeros(nf), np.zeros(nf)
+    for f, bc in velocity_boundary.items():
+        if bc == "ZeroGradient":
+            vk[f] = ZERO_GRADIENT
+        else:
+            vk[f], vx[f], vy[f] = FIXED_VALUE, bc[0], bc[1]
+    ck = cv = None
+    if correction_boundary is not None:
+        ck, cv = _scalar_boundary(mesh, correction_boundary)
+    o = SimpleOpts(viscosity, reference_cell, max_outer_iterations, continuity_tol,
+                   *momentum_solver, *pressure_solver, velocity_relaxation, pressure_relaxation)
+    v = _f64(velocity).copy()
+    ux, uy = _f64(v[:, 0]).copy(), _f64(v[:, 1]).copy()
+    p, phi = _f64(pressure).copy(), _f64(flux).copy()
+    hist = np.zeros(max(max_outer_iterations, 0) + 1)
+    rep = SimpleReportC()
+    msg = C.create_string_buffer(256)
+    st = lib().or_simple2d_solve(C.byref(mesh.m), C.byref(stencil.s), vk.ctypes.data,
+                                 vx.ctypes.data, vy.ctypes.data,
+                                 None if ck is None else ck.ctypes.data,
+                                 None if cv is None else cv.ctypes.data, C.byref(o), ux, uy, p,
+                                 phi, hist, C.byref(rep), msg, 256)
+    if st != 0:
+        raise SimpleError(st, msg.value.decode(), rep.err_index, rep.err_value)
+    report = {"converged": bool(rep.converged), "outer_iterations": rep.outer_iterations,
+              "initial_continuity_rms": rep.initial_rms, "final_continuity_rms": rep.final_rms,
+              "momentum_iterations": list(rep.mom_iters),
+              "momentum_converged": [bool(c) for c in rep.mom_converged],
+              "pressure_iterations": rep.p_iters, "pressure_converged": bool(rep.p_converged)}
+    done = rep.outer_iterations if rep.converged else max_outer_iterations
+    return report, np.stack([ux, uy], 1), p, phi, hist[:done + 1]
