@@ -21,7 +21,7 @@ STATUS_NAMES = {
     1: "DimensionMismatch", 2: "InvalidCsr", 3: "IndexOutOfBounds", 4: "NonFiniteValue",
     5: "MissingDiagonal", 6: "ZeroDiagonal", 7: "Breakdown", 8: "SolverError",
     9: "ConfigError", 10: "InvalidArgument", 11: "HipError", 12: "OutOfMemory",
-    13: "IoError", 14: "MeshError", 15: "PoissonError",
+    13: "IoError", 14: "MeshError", 15: "PoissonError", 16: "SimpleError",
 }
 STOP_ABS_INF, STOP_REL_L2 = 0, 1
 GUESS_PREVIOUS, GUESS_ZERO = 0, 1
@@ -119,6 +119,23 @@ class ErrDetail(C.Structure):
                 ("column", C.c_int64), ("context", C.c_char * 96)]
 
 
+class Simple2DOptions(C.Structure):
+    """fl_simple2d_options: SimpleOptions (src/simple.rs:93-131)."""
+    _fields_ = [("viscosity", C.c_double), ("reference_cell", C.c_int64),
+                ("max_outer_iterations", C.c_int64), ("continuity_absolute_tolerance", C.c_double),
+                ("momentum_solver", LinOpts), ("pressure_solver", LinOpts),
+                ("velocity_relaxation", C.c_double), ("pressure_relaxation", C.c_double)]
+
+
+class Simple2DReport(C.Structure):
+    """fl_simple2d_report: SimpleReport (src/simple.rs:133-142) and the error item."""
+    _fields_ = [("converged", C.c_int32), ("error_kind", C.c_int32),
+                ("outer_iterations", C.c_int64), ("initial_continuity_rms", C.c_double),
+                ("final_continuity_rms", C.c_double), ("momentum", LinReport * 3),
+                ("pressure", LinReport), ("error_index", C.c_int64), ("error_value", C.c_double),
+                ("device_ms", C.c_double)]
+
+
 class Grid2DConfig(C.Structure):
     _fields_ = [
         ("nx", C.c_int64), ("ny", C.c_int64), ("length", C.c_double), ("height", C.c_double),
@@ -189,6 +206,12 @@ SIGNATURES = {
     "fl_csr_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                               C.POINTER(C.c_int64)]),
     "fl_csr_download": (C.c_int, [_P, _P, _P, _P]),
+    "fl_simple2d_create": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(_P), _P]),
+    "fl_simple2d_destroy": (None, [_P]),
+    "fl_simple2d_initial_face_flux": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fl_simple2d_set_state": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fl_simple2d_get_state": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fl_simple2d_solve": (C.c_int, [_P, _P, _P, _P]),
     "fl_write_field_csv": (C.c_int, [C.c_char_p, C.c_int64, C.c_int64, C.c_double, C.c_double,
                                      _P, _P, _P, _P, _P, _P, C.c_char_p, C.c_int32]),
     "fl_write_vtk_2d": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int64, C.c_int64, C.c_double,

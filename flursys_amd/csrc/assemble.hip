@@ -31,6 +31,7 @@
 #include <rocprim/rocprim.hpp>
 
 #include "common.hpp"
+#include "mesh2d.hpp"
 
 // No fused multiply-adds in this file: the reference's geometry and
 // coefficients (Rust, which never contracts) are products and sums rounded
@@ -394,19 +395,6 @@ struct Scratch {
 using namespace fl;
 using namespace fl::asm2;
 
-struct fl_mesh2d {
-    fl_ctx *ctx = nullptr;
-    int64_t npts = 0, ncells = 0, nfaces = 0, nemit = 0;
-    double *cx = nullptr, *cy = nullptr, *vol = nullptr;
-    int *owner = nullptr, *neigh = nullptr;
-    double *fcx = nullptr, *fcy = nullptr, *sx = nullptr, *sy = nullptr;
-    int *off = nullptr, *emit_face = nullptr;
-    void release() {
-        void *b[] = {cx, cy, vol, owner, neigh, fcx, fcy, sx, sy, off, emit_face};
-        for (void *p : b)
-            if (p) (void)hipFree(p);
-    }
-};
 
 namespace {
 

@@ -32,6 +32,15 @@ void noalloc_init(int device);   // per-device address of the allocation-free so
 // duplicate-free, finite) in a solver matrix, taking ownership (csr.hip).
 fl_status csr_adopt(fl_ctx *ctx, uint32_t n, uint32_t ncols, int64_t nnz, int *off, int *col,
                     double *val, fl_csr **out);
+// The matrix's device values (same sparsity, rewritten by the caller's kernels).
+double *csr_values(fl_csr *m);
+// Recomputes the Jacobi inverse diagonal after the values changed (csr.hip).
+fl_status csr_refresh_jacobi(fl_csr *m);
+// A CG (kind 0), Jacobi-PCG (1) or BiCGSTAB (2) solve on device vectors: rhs
+// and x_inout are device arrays of n doubles, produced by the library's own
+// kernels (their finiteness is not re-checked on the host) (csr.hip).
+fl_status csr_solve_device(fl_csr *m, int kind, const double *rhs, double *x_inout, const fl_lin_opts *opts,
+                           fl_lin_report *rep, fl_err_detail *d);
 inline void ctx_retain(fl_ctx *ctx) { ctx->refs += 1; }
 
 constexpr int kBlock = 256;   // threads per workgroup: 4 wave64s

@@ -550,6 +550,7 @@ struct fl_csr {
     LinState *st = nullptr, *st_host = nullptr;
     uint32_t blocks = 1;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    unsigned long long *first = nullptr;  // Jacobi refresh: the first failing row
 };
 
 namespace {
@@ -574,7 +575,7 @@ fl_status err(fl_ctx *ctx, fl_err_detail *d, fl_status st, const char *text, int
 }
 
 void free_csr(fl_csr *m) {
-    void *bufs[] = {m->off, m->col, m->val, m->inv, m->partials, m->st};
+    void *bufs[] = {m->off, m->col, m->val, m->inv, m->partials, m->st, m->first};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (double *v : m->vec)
@@ -642,14 +643,16 @@ fl_status check_solve(fl_csr *m, int kind, const double *rhs, const double *x_in
 // The solve proper on caller-provided work buffers (check_solve passed and
 // the context's device is current).
 fl_status solve_on(fl_csr *m, int kind, const double *rhs, double *x_inout, const fl_lin_opts *opts,
-                   fl_lin_report *rep, fl_err_detail *d, const Work &w) {
+                   fl_lin_report *rep, fl_err_detail *d, const Work &w, bool device_io = false) {
     fl_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
     const size_t bytes = (size_t)m->n * sizeof(double);
     double *x = w.vec[0], *b = w.vec[1], *r = w.vec[2], *dp = w.vec[3], *qv = w.vec[4];
     double *tv = w.vec[5], *rhat = w.vec[6];
-    FL_HIP_TRY(ctx, hipMemcpyAsync(x, x_inout, bytes, hipMemcpyHostToDevice, s));
-    FL_HIP_TRY(ctx, hipMemcpyAsync(b, rhs, bytes, hipMemcpyHostToDevice, s));
+    const hipMemcpyKind in = device_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind out = device_io ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    FL_HIP_TRY(ctx, hipMemcpyAsync(x, x_inout, bytes, in, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(b, rhs, bytes, in, s));
     LinState init{};
     init.threshold = opts->absolute_tolerance;
     init.alpha = opts->relative_tolerance;  // consumed by k_cg_init_r
@@ -697,7 +700,7 @@ fl_status solve_on(fl_csr *m, int kind, const double *rhs, double *x_inout, cons
         FL_HIP_TRY(ctx, hipStreamSynchronize(s));
         return state_error(m, h, d);
     }
-    FL_HIP_TRY(ctx, hipMemcpyAsync(x_inout, x, bytes, hipMemcpyDeviceToHost, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(x_inout, x, bytes, out, s));
     FL_HIP_TRY(ctx, hipStreamSynchronize(s));
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, m->e0, m->e1);
@@ -757,7 +760,62 @@ __global__ void k_jacobi_inverse(uint32_t n, const int *__restrict__ off, const 
 
 }  // namespace
 
+namespace {
+// JacobiPreconditioner::new on the matrix's current values: the inverse
+// diagonal and the outcome (missing, zero or non-finite row) in inv_status.
+fl_status refresh_jacobi(fl_csr *m, unsigned long long *first) {
+    hipStream_t s = m->ctx->stream;
+    unsigned long long h = ~0ull;
+    FL_HIP_TRY(m->ctx, hipMemcpyAsync(first, &h, sizeof h, hipMemcpyHostToDevice, s));
+    if (m->n > 0) k_jacobi_inverse<<<m->blocks, kBlock, 0, s>>>(m->n, m->off, m->col, m->val, m->inv, first);
+    FL_HIP_TRY(m->ctx, hipGetLastError());
+    FL_HIP_TRY(m->ctx, hipMemcpyAsync(&h, first, sizeof h, hipMemcpyDeviceToHost, s));
+    FL_HIP_TRY(m->ctx, hipStreamSynchronize(s));
+    m->inv_status = FL_OK;
+    m->inv_row = 0;
+    m->inv_ctx = "";
+    if (h != ~0ull) {
+        m->inv_row = (int64_t)(h >> 2);
+        switch (h & 3) {
+            case 1: m->inv_status = FL_ERR_MISSING_DIAGONAL; m->inv_ctx = "missing diagonal entry"; break;
+            case 2: m->inv_status = FL_ERR_ZERO_DIAGONAL; m->inv_ctx = "zero diagonal entry"; break;
+            default: m->inv_status = FL_ERR_NON_FINITE; m->inv_ctx = "inverse diagonal"; break;
+        }
+    }
+    return FL_OK;
+}
+}  // namespace
+
 namespace fl {
+double *csr_values(fl_csr *m) { return m->val; }
+
+fl_status csr_refresh_jacobi(fl_csr *m) {
+    if (!m->first) FL_HIP_TRY(m->ctx, hipMalloc(&m->first, sizeof(unsigned long long)));
+    return refresh_jacobi(m, m->first);
+}
+
+fl_status csr_solve_device(fl_csr *m, int kind, const double *rhs, double *x_inout, const fl_lin_opts *opts,
+                           fl_lin_report *rep, fl_err_detail *d) {
+    fl_ctx *ctx = m->ctx;
+    FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (!rhs || !x_inout || !opts) return err(ctx, d, FL_ERR_INVALID_ARGUMENT, "null argument");
+    if (!std::isfinite(opts->absolute_tolerance) || opts->absolute_tolerance < 0.0 ||
+        !std::isfinite(opts->relative_tolerance) || opts->relative_tolerance < 0.0)
+        return err(ctx, d, FL_ERR_BREAKDOWN, "invalid solver tolerance");
+    if (kind == KIND_PCG && m->inv_status != FL_OK) {
+        set_detail(d, m->inv_status, m->inv_ctx, m->inv_row, 0, 0, m->inv_row);
+        return set_error(ctx, (fl_status)m->inv_status, "Jacobi preconditioner: %s (row %lld)", m->inv_ctx,
+                         (long long)m->inv_row);
+    }
+    fl_status st = ensure_vectors(m, kind == KIND_BICG ? 7 : 5);
+    if (st != FL_OK) return st;
+    Work w{};
+    for (int k = 0; k < 7; ++k) w.vec[k] = m->vec[k];
+    w.partials = m->partials;
+    w.st = m->st;
+    return solve_on(m, kind, rhs, x_inout, opts, rep, d, w, true);
+}
+
 fl_status csr_adopt(fl_ctx *ctx, uint32_t n, uint32_t ncols, int64_t nnz, int *off, int *col, double *val,
                     fl_csr **out) {
     *out = nullptr;
@@ -785,28 +843,13 @@ fl_status csr_adopt(fl_ctx *ctx, uint32_t n, uint32_t ncols, int64_t nnz, int *o
                 hipHostMalloc(&m->st_host, sizeof(LinState), hipHostMallocDefault) == hipSuccess &&
                 hipEventCreate(&m->e0) == hipSuccess && hipEventCreate(&m->e1) == hipSuccess &&
                 hipMalloc(&first, sizeof(unsigned long long)) == hipSuccess;
-    unsigned long long h = ~0ull;
-    okay = okay && hipMemcpyAsync(first, &h, sizeof h, hipMemcpyHostToDevice, s) == hipSuccess &&
-           hipMemsetAsync(m->st, 0, sizeof(LinState), s) == hipSuccess;
-    if (okay && n > 0) {
-        k_jacobi_inverse<<<m->blocks, kBlock, 0, s>>>(n, off, col, val, m->inv, first);
-        okay = hipGetLastError() == hipSuccess;
-    }
-    okay = okay && hipMemcpyAsync(&h, first, sizeof h, hipMemcpyDeviceToHost, s) == hipSuccess &&
-           hipStreamSynchronize(s) == hipSuccess;
+    okay = okay && hipMemsetAsync(m->st, 0, sizeof(LinState), s) == hipSuccess;
+    okay = okay && refresh_jacobi(m, first) == FL_OK;
     if (first) (void)hipFree(first);
     if (!okay) {
         free_csr(m);
         delete m;
         return set_error(ctx, FL_ERR_OUT_OF_MEMORY, "device allocation failed");
-    }
-    if (h != ~0ull) {
-        m->inv_row = (int64_t)(h >> 2);
-        switch (h & 3) {
-            case 1: m->inv_status = FL_ERR_MISSING_DIAGONAL; m->inv_ctx = "missing diagonal entry"; break;
-            case 2: m->inv_status = FL_ERR_ZERO_DIAGONAL; m->inv_ctx = "zero diagonal entry"; break;
-            default: m->inv_status = FL_ERR_NON_FINITE; m->inv_ctx = "inverse diagonal"; break;
-        }
     }
     ctx_retain(ctx);
     *out = m;

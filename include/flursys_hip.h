@@ -43,7 +43,8 @@ typedef enum {
     FL_ERR_OUT_OF_MEMORY = 12,
     FL_ERR_IO = 13,                /* an output file could not be written */
     FL_ERR_MESH = 14,              /* MeshError of a mesh build (kind in FL_MESH_*) */
-    FL_ERR_POISSON = 15            /* PoissonError of an assembly (kind in FL_POISSON_*) */
+    FL_ERR_POISSON = 15,           /* PoissonError of an assembly (kind in FL_POISSON_*) */
+    FL_ERR_SIMPLE = 16             /* SimpleError of the unstructured SIMPLE (kind in FL_SIMPLE_*) */
 } fl_status;
 
 /* Stop rule: the reference's absolute max|r| < tol (incompressible.rs:
@@ -560,6 +561,91 @@ fl_status fl_mesh2d_assemble_poisson(fl_mesh2d *mesh, double gamma, const double
 /* Shape and host copies of a device CSR matrix (any pointer may be NULL). */
 fl_status fl_csr_info(fl_csr *matrix, int64_t *nrows, int64_t *ncols, int64_t *nnz);
 fl_status fl_csr_download(fl_csr *matrix, int64_t *row_offsets, int64_t *cols, double *vals);
+
+/* ---------------------------------------------------------------------- */
+/* Collocated unstructured SIMPLE on the device: solve_simple (src/simple.rs:
+ * 148-311) for steady, constant-density flow on a 2D polygon mesh, with
+ * every outer iteration's pieces as kernels -- the least-squares pressure
+ * gradient (numerics.rs:650-682) and its source, the momentum CSR assembly
+ * (momentum.rs:180-341: first-order upwind convection, two-point diffusion,
+ * equation relaxation), BiCGSTAB on the x and y systems (momentum.rs:
+ * 445-475), rAU, the Rhie-Chow predicted flux (simple.rs:746-805), the
+ * pressure-correction CSR (simple.rs:518-638) and its Jacobi-PCG, the
+ * pressure, velocity (LSQ gradient of p') and face-flux corrections and the
+ * continuity RMS (simple.rs:703-717) -- with no host round trip of a field.
+ * The velocity is (x, y): the z system of a 2D mesh has a zero right-hand
+ * side and zero guess, and its report is 0 iterations, converged, as the
+ * reference's BiCGSTAB returns it.  The pressure-correction matrix keeps the
+ * momentum matrix's sparsity: with a reference cell its row and column hold
+ * explicit zeros (and 1 on the diagonal) instead of being dropped, which
+ * leaves every product and therefore the solve unchanged. */
+typedef struct fl_simple2d fl_simple2d;
+/* SimpleError kinds (src/simple.rs:17-29) and the NumericsError /
+ * MomentumError cases the device path detects */
+enum {
+    FL_SIMPLE_INVALID_MOMENTUM_DIAGONAL = 1,
+    FL_SIMPLE_INVALID_PRESSURE_FACE_COEFFICIENT = 2,
+    FL_SIMPLE_INVALID_PRESSURE_REFERENCE = 3,
+    FL_SIMPLE_INVALID_PRESSURE_RELAXATION = 4,
+    FL_SIMPLE_INVALID_OUTER_ITERATIONS = 5,
+    FL_SIMPLE_MOMENTUM_DID_NOT_CONVERGE = 6,
+    FL_SIMPLE_PRESSURE_DID_NOT_CONVERGE = 7,
+    FL_SIMPLE_SINGULAR_STENCIL = 8,         /* NumericsError::SingularLeastSquaresStencil */
+    FL_SIMPLE_DEGENERATE_DISTANCE = 9,      /* owner-neighbour / owner-face distance */
+    FL_SIMPLE_MISSING_BOUNDARY = 10,        /* a boundary face without a condition */
+    FL_SIMPLE_BACKFLOW = 11,                /* MomentumError: backflow on a ZeroGradient face */
+    FL_SIMPLE_NONFINITE_FLUX = 12,          /* MomentumError: non-finite face flux */
+    FL_SIMPLE_LINEAR = 13,                  /* a LinearAlgebraError of a solve */
+    FL_SIMPLE_INVALID_OPTIONS = 14          /* viscosity or velocity relaxation out of range */
+};
+/* SimpleOptions (simple.rs:93-131) with Diffusivity::Constant(viscosity). */
+typedef struct {
+    double viscosity;
+    int64_t reference_cell;          /* used when no correction boundary anchors p' */
+    int64_t max_outer_iterations;
+    double continuity_absolute_tolerance;
+    fl_lin_opts momentum_solver, pressure_solver;
+    double velocity_relaxation, pressure_relaxation;
+} fl_simple2d_options;
+/* SimpleReport (simple.rs:133-142) plus the failing item of an error. */
+typedef struct {
+    int32_t converged;
+    int32_t error_kind;              /* FL_SIMPLE_* when the call returned FL_ERR_SIMPLE */
+    int64_t outer_iterations;
+    double initial_continuity_rms, final_continuity_rms;
+    fl_lin_report momentum[3];       /* last outer iteration's x, y, z solves */
+    fl_lin_report pressure;          /* last pressure-correction solve */
+    int64_t error_index;             /* the cell or face of error_kind */
+    double error_value;
+    double device_ms;                /* wall time of the outer iterations */
+} fl_simple2d_report;
+/* Per boundary face (arrays of nfaces, interior entries ignored):
+ * velocity_kind 1 FixedValue (velocity_x/y, a wall or inlet velocity) or 0
+ * ZeroGradient; pressure_kind / pressure_value, the LSQ stencil's boundary
+ * (LeastSquaresGradientStencil::new, numerics.rs:106-200); correction_kind /
+ * correction_value (NULL: none, the reference cell pins p'), the
+ * pressure-correction boundary whose FixedValue faces anchor p' (an outlet).
+ * The mesh must outlive the solver. */
+fl_status fl_simple2d_create(fl_mesh2d *mesh, const int32_t *velocity_kind, const double *velocity_x,
+                             const double *velocity_y, const int32_t *pressure_kind,
+                             const double *pressure_value, const int32_t *correction_kind,
+                             const double *correction_value, fl_simple2d **out,
+                             fl_simple2d_report *error);
+void fl_simple2d_destroy(fl_simple2d *simple);
+/* initial_face_flux (simple.rs:719-744): interior faces from the linearly
+ * interpolated cell velocity, boundary faces from wall_flux (host arrays). */
+fl_status fl_simple2d_initial_face_flux(fl_simple2d *simple, const double *ux, const double *uy,
+                                        const double *wall_flux, double *face_flux);
+/* SimpleState: cell velocity (x, y), pressure and the authoritative face flux. */
+fl_status fl_simple2d_set_state(fl_simple2d *simple, const double *ux, const double *uy,
+                                const double *p, const double *face_flux);
+fl_status fl_simple2d_get_state(fl_simple2d *simple, double *ux, double *uy, double *p,
+                                double *face_flux);
+/* solve_simple from the current state; history (may be NULL) receives the
+ * continuity RMS before the first and after every outer iteration
+ * (max_outer_iterations + 1 entries). */
+fl_status fl_simple2d_solve(fl_simple2d *simple, const fl_simple2d_options *options,
+                            fl_simple2d_report *report, double *history);
 
 /* ---------------------------------------------------------------------- */
 /* On-disk outputs (src/output.rs) in the reference's formats and number
