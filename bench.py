@@ -82,7 +82,7 @@ def cpu_baseline(n, re, dt, iters_per_step, sample_iters):
     }
 
 
-def run_variant(fa, ctx, name, precision, precond, args):
+def run_variant(fa, ctx, name, precision, precond, args, learned_mode=None):
     n = args.n
     cfg = fa.LidDrivenCavity3DConfig(nx=n, ny=n, nz=n, reynolds=args.reynolds, dt=args.dt,
                                      pressure_tolerance=args.tol, stop_rule=fa.STOP_REL_L2,
@@ -98,6 +98,8 @@ def run_variant(fa, ctx, name, precision, precond, args):
         net = UNet(args.unet_n, 1, ctx=ctx)
         net.load(args.weights)
         s.set_unet(net)
+        if learned_mode is not None:
+            s.set_learned_mode(learned_mode)
     # The last warmup step times every kernel (untimed region; the first
     # step's launches pay one-time costs): its breakdown names the dominant
     # kernel, which alone is event-timed in the timed region.
@@ -484,6 +486,77 @@ def run_c3(fa, ctx, args):
     return out
 
 
+def run_f1(fa, ctx, args):
+    """Collocated unstructured SIMPLE (solve_simple, src/simple.rs:148-311)
+    on the device: a lid-driven cavity of side n on an n x n jittered mesh
+    of quads and triangle pairs, Re 20 (nu = n / 20), the reference's
+    SimpleOptions::steady relaxation (0.7 / 0.3), momentum BiCGSTAB and
+    pressure-correction PCG to atol 1e-10 / rtol 1e-8, a fixed number of
+    outer iterations (continuity tolerance 0).  The CPU oracle (serial, like
+    the reference's src/linear/) runs one outer iteration of the smaller
+    mesh."""
+    import numpy as np
+
+    from flursys_amd.linear import LinearSolverOptions
+    from flursys_amd.mesh import DeviceMesh2D
+    from flursys_amd.meshgen import quad_triangle_grid
+    from flursys_amd.simple import DeviceSimple2D, SimpleOptions
+
+    out = {"workload": "lid-driven cavity, jittered quads + triangle pairs, Re 20, relaxation "
+                       "0.7/0.3, solves to atol 1e-10 rtol 1e-8, fixed outer iterations"}
+
+    def setup(n):
+        pts, cells = quad_triangle_grid(n, 0.2, 9)
+        t0 = time.perf_counter()
+        dm = DeviceMesh2D(pts, cells, ctx)
+        g = dm.geometry()
+        bnd = np.nonzero(g["neighbour"] < 0)[0]
+        vel = {int(f): ((1.0, 0.0) if abs(g["fcy"][f] - n) < 1e-9 else (0.0, 0.0)) for f in bnd}
+        s = DeviceSimple2D(dm, vel, {int(f): "ZeroGradient" for f in bnd})
+        return pts, cells, dm, s, vel, bnd, time.perf_counter() - t0
+
+    def opts(n, iters):
+        return SimpleOptions(n / 20.0, max_outer_iterations=iters, continuity_absolute_tolerance=0.0,
+                             momentum_solver=LinearSolverOptions(1e-10, 1e-8, 2000),
+                             pressure_solver=LinearSolverOptions(1e-10, 1e-8, 20000))
+
+    for n, iters in ((args.f1_cpu_n, 3), (args.f1_n, 5)):
+        pts, cells, dm, s, vel, bnd, t_setup = setup(n)
+        s.solve(opts(n, 1))  # warm-up (allocates the solver vectors)
+        s.set_state(np.zeros((dm.ncells, 2)), np.zeros(dm.ncells), np.zeros(dm.nfaces))
+        t0 = time.perf_counter()
+        rep = s.solve(opts(n, iters))
+        dt = time.perf_counter() - t0
+        out[f"{n}x{n}"] = {"cells": dm.ncells, "faces": dm.nfaces, "outer_iterations": iters,
+                           "s_per_outer_iteration": dt / iters,
+                           "mesh_and_setup_s": t_setup,
+                           "last_pressure_pcg_iterations": rep.pressure_report.iterations,
+                           "last_momentum_bicgstab_iterations":
+                               [rep.momentum_reports[0].iterations,
+                                rep.momentum_reports[1].iterations],
+                           "final_continuity_rms": rep.final_continuity_rms}
+        log(f"[bench] f1 SIMPLE {dm.ncells} cells: {dt / iters * 1e3:.1f} ms per outer iteration "
+            f"(PCG {rep.pressure_report.iterations} its)")
+        s.close()
+        dm.close()
+        if n == args.f1_cpu_n and not args.no_cpu:
+            import oracle as O
+
+            om = O.Mesh2D(pts, cells)
+            st = O.LsqStencil(om, {int(f): "ZeroGradient" for f in bnd})
+            t0 = time.perf_counter()
+            O.solve_simple(om, st, vel, np.zeros((om.m.ncells, 2)), np.zeros(om.m.ncells),
+                           np.zeros(om.m.nfaces), n / 20.0, max_outer_iterations=1,
+                           continuity_tol=0.0, momentum_solver=(1e-10, 1e-8, 2000),
+                           pressure_solver=(1e-10, 1e-8, 20000))
+            cpu = time.perf_counter() - t0
+            out["cpu_oracle"] = {"s_per_outer_iteration": cpu, "cores": 1, "kind": "port",
+                                 "sample": f"the first outer iteration of the {n}x{n} mesh"}
+            out["gpu_vs_cpu_first_iteration_mesh"] = f"{n}x{n}"
+            log(f"[bench] f1 CPU oracle {om.m.ncells} cells: {cpu:.2f} s for the first outer iteration")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -500,7 +573,8 @@ def main():
                     help="batch2d: THREADSxBLOCKS for the persistent PCG instead of autotuning")
     ap.add_argument("--tune-iterations", type=int, default=1300,
                     help="iteration cap of the batch autotuner's solves")
-    ap.add_argument("--variants", default="plain_fp64_pcg,mixed_ir,mg_fcg,learned_fcg")
+    ap.add_argument("--variants",
+                    default="plain_fp64_pcg,mixed_ir,mg_fcg,learned_fcg,learned_fcg_replace")
     ap.add_argument("--weights", default=os.path.join(ROOT, "flursys_amd", "weights",
                                                       "unet64_cavity.npy"),
                     help="trained U-Net parameters for learned_fcg")
@@ -515,10 +589,14 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1000,
                     help="steps of every cavity in the batch2d configuration")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the supporting configurations (C1 2D 128^2, C2 3D 128^3, C3 CSR)")
+                    help="skip the supporting configurations (C1 2D 128^2, C2 3D 128^3, C3 CSR, "
+                         "f1 unstructured SIMPLE)")
     ap.add_argument("--c1-steps", type=int, default=300)
     ap.add_argument("--c1-cpu-seconds", type=float, default=4.0)
     ap.add_argument("--c3-m", type=int, default=1000, help="C3 triangulation: m x m squares")
+    ap.add_argument("--f1-n", type=int, default=600, help="unstructured SIMPLE mesh: n x n")
+    ap.add_argument("--f1-cpu-n", type=int, default=200,
+                    help="unstructured SIMPLE mesh timed on the CPU oracle too")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -535,17 +613,22 @@ def main():
         print(json.dumps(run_batch2d(fa, ctx, args)), flush=True)
         ctx.close()
         return
-    modes = {"plain_fp64_pcg": (fa.PRECISION_FP64, fa.PRECOND_JACOBI),
-             "mixed_ir": (fa.PRECISION_MIXED, fa.PRECOND_JACOBI),
-             "mg_fcg": (fa.PRECISION_MIXED, fa.PRECOND_MULTIGRID),
-             "learned_fcg": (fa.PRECISION_MIXED, fa.PRECOND_LEARNED)}
+    # learned_fcg: the network's coarse prediction corrected by the V-cycle
+    # below its level; learned_fcg_replace: the network and two sweeps in
+    # place of the coarse levels
+    modes = {"plain_fp64_pcg": (fa.PRECISION_FP64, fa.PRECOND_JACOBI, None),
+             "mixed_ir": (fa.PRECISION_MIXED, fa.PRECOND_JACOBI, None),
+             "mg_fcg": (fa.PRECISION_MIXED, fa.PRECOND_MULTIGRID, None),
+             "learned_fcg": (fa.PRECISION_MIXED, fa.PRECOND_LEARNED, fa.LEARNED_CORRECTED),
+             "learned_fcg_replace": (fa.PRECISION_MIXED, fa.PRECOND_LEARNED, fa.LEARNED_REPLACE)}
     results = {}
     for name in args.variants.split(","):
-        if name == "learned_fcg" and not os.path.exists(args.weights):
-            log(f"[bench] learned_fcg skipped: no trained weights at {args.weights} "
+        if name.startswith("learned_fcg") and not os.path.exists(args.weights):
+            log(f"[bench] {name} skipped: no trained weights at {args.weights} "
                 "(python -m flursys_amd.train_unet)")
             continue
-        results[name] = run_variant(fa, ctx, name, *modes[name], args)
+        precision, precond, lmode = modes[name]
+        results[name] = run_variant(fa, ctx, name, precision, precond, args, lmode)
     # the headline is the learned flexible CG (the metric's method); the
     # other variants are reported beside it
     best = "learned_fcg" if "learned_fcg" in results else max(
@@ -578,8 +661,9 @@ def main():
         "config": {
             "workload": f"3D lid-driven cavity {n}^3, Re {args.reynolds:g}, dt {args.dt:g}, "
                         f"pressure to rel-res {args.tol:g} (FP64 test) from a zero guess, "
-                        "p(0,0,0) pinned; value = learned flexible CG (U-Net coarse solve in "
-                        "the multigrid V-cycle, FP32 inner solve in FP64 refinement)",
+                        "p(0,0,0) pinned; value = learned flexible CG (the U-Net's 64^3 coarse "
+                        "prediction corrected by the V-cycle below it, FP32 inner solve in FP64 "
+                        "refinement)",
             "grid": [n, n, n],
             "variant": best,
             "cg_iters_per_step": b["cg_iters_per_step"],
@@ -628,13 +712,14 @@ def main():
                        "mfma_TFLOPs": fl / (u["ms"] * 1e-3) / 1e12,
                        "mfma_peak_TFLOPs": 2500.0,
                        "mfma_frac": fl / (u["ms"] * 1e-3) / 2.5e15,
-                       "note": "time of the coarse solve = inference (about 20 kernels) plus two "
-                               "damped-Jacobi sweeps on 64^3; FLOPs count the useful "
-                               "convolution work (2*27*cin*cout per voxel and layer)"}
+                       "note": "time of the coarse solve = inference (about 20 kernels) plus the "
+                               "residual f - A x on 64^3; FLOPs count the useful convolution "
+                               "work (2*27*cin*cout per voxel and layer)"}
     if not args.no_extras:
         out["c2"] = run_c2(fa, ctx, args)
         out["c1"] = run_c1(fa, ctx, args)
         out["c3"] = run_c3(fa, ctx, args)
+        out["f1"] = run_f1(fa, ctx, args)
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle) sample")
         plain = results.get("plain_fp64_pcg", b)

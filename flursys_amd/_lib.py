@@ -28,6 +28,7 @@ GUESS_PREVIOUS, GUESS_ZERO = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 PRECOND_JACOBI, PRECOND_MULTIGRID, PRECOND_LEARNED = 0, 1, 2
 SOLVER_PCG, SOLVER_SOR = 0, 1
+LEARNED_CORRECTED, LEARNED_REPLACE = 0, 1
 # fl_solve_report.fallback_reason
 FALLBACK_NONE, FALLBACK_BUDGET, FALLBACK_STAGNATION, FALLBACK_BREAKDOWN = 0, 1, 2, 3
 
@@ -266,6 +267,7 @@ SIGNATURES = {
     "fl_unet_grads": (C.c_int, [_P, C.c_int32, _P, _P, C.c_double, _P, C.POINTER(C.c_float)]),
     "fl_probe_mfma_bf16": (C.c_int, [_P, _P, _P, _P]),
     "fl_cavity3d_set_unet": (C.c_int, [_P, _P]),
+    "fl_cavity3d_set_learned_mode": (C.c_int, [_P, C.c_int32]),
     "fl_cavity3d_coarse_pair": (C.c_int, [_P, C.c_int32, _P, _P, C.POINTER(C.c_double)]),
     "fl_cavity3d_capture_coarse": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "fl_cavity3d_captured_rhs": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),

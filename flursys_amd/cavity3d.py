@@ -136,6 +136,12 @@ class LidDrivenCavity3DSolver:
               self.ctx.handle)
         self._unet = net
 
+    def set_learned_mode(self, mode: int) -> None:
+        """_lib.LEARNED_CORRECTED (default: the V-cycle corrects the network's
+        coarse prediction) or _lib.LEARNED_REPLACE (the prediction and two
+        damped-Jacobi sweeps replace the coarse levels)."""
+        check(load().fl_cavity3d_set_learned_mode(self.handle, mode), self.ctx.handle)
+
     def coarse_pair(self, n: int):
         """(b, p, diag): the last step's right-hand side b = -rhs and pressure,
         mean-restricted to n^3, as float32 (training pairs of the U-Net)."""

@@ -325,6 +325,37 @@ __device__ inline float two_sweeps_from_zero(const Level &L, const float *inv, c
     return e1[0] + kOmega * inv[lvl_didx(L, i, j, k)] * (fv[0] - ae);
 }
 
+// The learned coarse solve's residual on a level: f <- f - A x (in place;
+// each cell reads only its own f), and the correction's sum a <- a + b.
+__global__ void __launch_bounds__(kBlock) k_lvl_residual(Level L, float *__restrict__ f,
+                                                         const float *__restrict__ x, const FcgA *SA,
+                                                         const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    TileSched ts((L.n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t c = t * kBlock + threadIdx.x;
+        if (c >= L.n) continue;
+        uint32_t i, j, k;
+        lvl_ijk(L, c, i, j, k);
+        const uint32_t ii[7] = {i, i + 1 < L.nx ? i + 1 : i, i > 0 ? i - 1 : i, i, i, i, i};
+        const uint32_t jj[7] = {j, j, j, j + 1 < L.ny ? j + 1 : j, j > 0 ? j - 1 : j, j, j};
+        const uint32_t kk[7] = {k, k, k, k, k, k + 1 < L.nz ? k + 1 : k, k > 0 ? k - 1 : k};
+        float ev[7];
+#pragma unroll
+        for (int a = 0; a < 7; ++a) ev[a] = x[ii[a] + L.nx * (jj[a] + L.ny * kk[a])];
+        const float ae = L.cx * ((ev[0] - ev[1]) + (ev[0] - ev[2])) +
+                         L.cy * ((ev[0] - ev[3]) + (ev[0] - ev[4])) +
+                         L.cz * ((ev[0] - ev[5]) + (ev[0] - ev[6]));
+        f[c] = f[c] - ae;
+    }
+}
+__global__ void __launch_bounds__(kBlock) k_lvl_add(uint32_t n, float *__restrict__ a,
+                                                    const float *__restrict__ b, const FcgA *SA,
+                                                    const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock) a[c] += b[c];
+}
+
 // One damped-Jacobi sweep on a level:
 //   e_out = e' + omega D^-1 (f - A e'),  e' = e_in + corr P(e_c) (kProlong;
 //   a null e_in is zero)
@@ -2182,7 +2213,7 @@ const char *const kKindNames[K_COUNT] = {
     "k_mg_prolong0_z", "k_mg_sweep0_z<true>",
     "k_mg_restrict_z<e16>", "k_mg_down", "k_mg_tail", "k_mg_up", "k_fcg_dirapply_z", "k_fcg_update",
     "unet_warm_start",  // restriction + U-Net inference + interpolation (many kernels)
-    "unet_coarse_solve",  // U-Net inference + two sweeps on its level (many kernels)
+    "unet_coarse_solve",  // U-Net inference + the residual on its level (many kernels)
     "k_cg_dot_t<double>", "k_cg_step_t<double>", "k_cg_dot_t<float>", "k_cg_step_t<float>",
 };
 
@@ -2223,6 +2254,7 @@ struct fl_cavity3d {
     // wl_level; wl_in / wl_out hold its input and output there
     fl_unet *unet = nullptr;
     int wl_level = 0;
+    int learned_mode = FL_LEARNED_CORRECTED;  // FL_LEARNED_*
     double wl_alpha = 0.0;   // step of the last warm start along wl_e
     const float *wl_e = nullptr;
     float *wl_in = nullptr, *wl_out = nullptr;
@@ -2644,12 +2676,43 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     }
     FL_TIMED(s, K_MG_RESTRICT_L0, k, 6.0 * n0 + 4.0 * n1,
              k_mg_restrict_z<e16><<<dim3((L[0].nx + kRX - 1) / kRX, L[0].ny / 2, s->zg.bz), kRX, 0, stm>>>(L[0], L[1], s->r32, pre0, s->mf[1], s->fa, s->fb));
-    // The learned coarse solve: from the network's level Lw down, the cycle
-    // is replaced by the U-Net's prediction of A_Lw^-1 f_Lw (the level's
-    // Galerkin diagonal sets its scale) and two damped-Jacobi sweeps.
+    // The learned coarse solve: on the network's level Lw the U-Net predicts
+    // x = A_Lw^-1 f_Lw (the level's Galerkin diagonal sets its scale); the
+    // cycle goes on below Lw with the residual f_Lw - A x in place of f_Lw,
+    // and x is added back to level Lw's result on the way up, so the coarse
+    // solve is the multigrid correction of the network's prediction (its
+    // error is the cycle's contraction of the network's error).
     const int Lw = learned_level(s);
-    const int down_end = Lw > 0 ? Lw : t;
+    auto capture = [&](int l) -> fl_status {
+        if (s->cap_level == l && !s->cap_taken && k == s->cap_iter) {
+            FL_HIP_TRY(ctx, hipMemcpyAsync(s->cap, s->mf[l], L[l].n * sizeof(float), hipMemcpyDeviceToDevice, stm));
+            s->cap_taken = true;
+        }
+        return FL_OK;
+    };
+    auto learned = [&]() -> fl_status {
+        const Level &LL = L[Lw];
+        const double diag = 2.0 * ((double)LL.cx + (double)LL.cy + (double)LL.cz);
+        const uint32_t gl = s->lblocks[Lw];
+        fl_status ust = FL_OK;
+        FL_TIMED(s, K_UNET_COARSE, k, 0.0,
+                 ust = unet_infer_device(s->unet, stm, s->mf[Lw], diag, s->wl_out);
+                 k_lvl_residual<<<gl, kBlock, 0, stm>>>(LL, s->mf[Lw], s->wl_out, s->fa, s->fb));
+        if (ust != FL_OK)
+            return set_error(ctx, FL_ERR_HIP, "U-Net inference failed: %s", fl_last_error(unet_context(s->unet)));
+        return FL_OK;
+    };
+    auto learned_add = [&]() {
+        k_lvl_add<<<s->lblocks[Lw], kBlock, 0, stm>>>(L[Lw].n, s->meb[Lw], s->wl_out, s->fa, s->fb);
+    };
+    // FL_LEARNED_REPLACE: levels Lw and below are replaced by the prediction
+    // and two damped-Jacobi sweeps on level Lw
+    const bool replace = Lw > 0 && s->learned_mode == FL_LEARNED_REPLACE;
+    const int down_end = replace ? Lw : t;
     for (int l = 1; l < down_end; ++l) {
+        fl_status lst = capture(l);
+        if (lst == FL_OK && l == Lw) lst = learned();
+        if (lst != FL_OK) return lst;
         const double nl = L[l].n, nc = L[l + 1].n;
         if (l >= kFusedFrom) {
             FL_TIMED(s, K_MG_DOWN, k, 8.0 * nl + 4.0 * nc,
@@ -2663,12 +2726,9 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
                      k_mg_restrict_z<float><<<gr, kRX, 0, stm>>>(L[l], L[l + 1], s->mf[l], s->mea[l], s->mf[l + 1], s->fa, s->fb));
         }
     }
-    if (s->cap_level > 0 && !s->cap_taken && k == s->cap_iter && s->cap_level <= down_end) {
-        FL_HIP_TRY(ctx, hipMemcpyAsync(s->cap, s->mf[s->cap_level], L[s->cap_level].n * sizeof(float),
-                                       hipMemcpyDeviceToDevice, stm));
-        s->cap_taken = true;
-    }
-    if (Lw > 0) {
+    if (replace) {
+        fl_status lst = capture(Lw);
+        if (lst != FL_OK) return lst;
         const Level &LL = L[Lw];
         const double diag = 2.0 * ((double)LL.cx + (double)LL.cy + (double)LL.cz);
         const uint32_t gl = s->lblocks[Lw];
@@ -2680,12 +2740,16 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
         if (ust != FL_OK)
             return set_error(ctx, FL_ERR_HIP, "U-Net inference failed: %s", fl_last_error(unet_context(s->unet)));
     } else {
+        fl_status lst = capture(t);
+        if (lst == FL_OK && Lw == t) lst = learned();
+        if (lst != FL_OK) return lst;
         TailLevels T{};
         T.count = s->nlev - t;
         for (int l = 0; l < T.count; ++l) T.lv[l] = L[t + l];
         T.pinv = s->pinv;
         FL_TIMED(s, K_MG_TAIL, k, 8.0 * L[t].n,
                  k_mg_tail<<<1, 1024, 0, stm>>>(T, s->mf[t], s->meb[t], s->fa, s->fb));
+        if (Lw == t) learned_add();
     }
     // up: the solution of level l ends in meb[l]
     for (int l = down_end - 1; l >= 1; --l) {
@@ -2700,6 +2764,7 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
                      k_mg_sweep_z<false, false><<<g, kZT, 0, stm>>>(L[l], L[l], s->mf[l], s->meb[l], nullptr, s->mea[l], s->fa, s->fb, kCorr);
                      k_mg_sweep_z<false, false><<<g, kZT, 0, stm>>>(L[l], L[l], s->mf[l], s->mea[l], nullptr, s->meb[l], s->fa, s->fb, kCorr));
         }
+        if (!replace && l == Lw) learned_add();
     }
     // up on level 0: prolongate-correct sweep into post0 (kL0Smooth == 2:
     // mea) or into meb (3, then one more sweep into mea); the publishing
@@ -3348,6 +3413,14 @@ fl_status fl_cavity3d_set_timing(fl_cavity3d *s, int32_t enable, const char *ker
     s->prof.on = enable != 0;
     s->prof.only = only;
     s->prof.clear();
+    return FL_OK;
+}
+
+fl_status fl_cavity3d_set_learned_mode(fl_cavity3d *s, int32_t mode) {
+    if (!s) return FL_ERR_INVALID_ARGUMENT;
+    if (mode != FL_LEARNED_CORRECTED && mode != FL_LEARNED_REPLACE)
+        return set_error(s->ctx, FL_ERR_CONFIG, "unknown learned coarse-solve mode %d", mode);
+    s->learned_mode = mode;
     return FL_OK;
 }
 

@@ -133,3 +133,28 @@ def skew_perturbation(A, faces, amplitude: float = 0.1, seed: int = 0, ref: int 
     out = (A + D - D.T).tocsr()
     out.sort_indices()
     return out
+
+
+def quad_triangle_grid(n: int, jitter: float = 0.2, seed: int = 9):
+    """An n x n square of side n: quads, the squares with x + y even split
+    into two triangles (v00, v10, v11), (v00, v11, v01); interior vertices
+    moved by U(-jitter, +jitter) in x and y (numpy generator `seed`).  Every
+    cell keeps at least two face neighbours, so the least-squares stencil of
+    all-ZeroGradient walls is regular.  Returns (points (N, 2), polygon list)."""
+    rng = np.random.default_rng(seed)
+    ys, xs = np.meshgrid(np.arange(n + 1, dtype=np.float64), np.arange(n + 1, dtype=np.float64),
+                         indexing="ij")
+    inner = (xs > 0) & (xs < n) & (ys > 0) & (ys < n)
+    jx = rng.uniform(-jitter, jitter, xs.shape)
+    jy = rng.uniform(-jitter, jitter, xs.shape)
+    pts = np.stack([(xs + inner * jx).ravel(), (ys + inner * jy).ravel()], 1)
+    cells = []
+    for y in range(n):
+        for x in range(n):
+            a = x + (n + 1) * y
+            b, c, d = a + 1, a + n + 2, a + n + 1
+            if (x + y) % 2 == 0:
+                cells += [[a, b, c], [a, c, d]]
+            else:
+                cells.append([a, b, c, d])
+    return pts, cells

@@ -489,12 +489,19 @@ fl_status fl_probe_mfma_bf16(fl_ctx *ctx, const float *a, const float *b, float 
 /* Attaches `net` (may be NULL) to a solver created with FL_PRECOND_LEARNED:
  * the grid must be a cube of n * 2^L cells per axis with L a multigrid level
  * that keeps the full domain.  With L >= 1 the network is the coarse solve
- * of every V-cycle of the flexible CG (levels L and below are replaced by
- * its prediction of A_L^-1 f_L and two damped-Jacobi sweeps on level L; the
- * first FCG step is then the learned warm start); with L = 0 it gives a
- * separate warm start ahead of the multigrid FCG.  The network must outlive
- * its use by the solver and must not be used concurrently. */
+ * of every V-cycle of the flexible CG, as set by fl_cavity3d_set_learned_mode;
+ * with L = 0 it gives a separate warm start ahead of the multigrid FCG.  The
+ * network must outlive its use by the solver and must not be used
+ * concurrently. */
 fl_status fl_cavity3d_set_unet(fl_cavity3d *solver, fl_unet *net);
+/* How the network serves as the coarse solve on level L (no reference
+ * counterpart).  FL_LEARNED_CORRECTED (the default): the network predicts
+ * x = A_L^-1 f_L, the V-cycle continues below L on f_L - A_L x and adds x
+ * back on the way up (the multigrid correction of the prediction).
+ * FL_LEARNED_REPLACE: levels L and below are replaced by the prediction and
+ * two damped-Jacobi sweeps on level L. */
+enum { FL_LEARNED_CORRECTED = 0, FL_LEARNED_REPLACE = 1 };
+fl_status fl_cavity3d_set_learned_mode(fl_cavity3d *solver, int32_t mode);
 /* Training pairs: the last step's pressure right-hand side b = -rhs and
  * pressure p, mean-restricted to n^3 (n = nx / 2^L); `diag` receives the
  * Laplacian diagonal at that resolution. */
