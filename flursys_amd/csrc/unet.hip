@@ -181,6 +181,106 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     }
 }
 
+// The forward convolution of the fine levels with its operands staged in
+// LDS.  A workgroup owns a 16 x R x 4 block of output voxels (wave w: plane
+// z0 + w, rows y0 .. y0 + R - 1, one 16-voxel MFMA row each) and NB
+// 16-channel output blocks.  Input channels are taken 16 at a time: per
+// chunk the block stages (one batch of 16-byte global loads, then the LDS
+// stores) the (16 + 2) x (R + 2) x 6 halo of those channels (zero outside
+// the domain; channels past c1 from the half-resolution x2, the decoder's
+// nearest upsampling) as two 8-channel halves, and the chunk's weights as
+// [tap][half][output channel] units, so the 16 lanes of an MFMA row read
+// 256 contiguous bytes of either.  K runs over the 27 taps in pairs: lanes
+// g = 0, 1 take tap 2 q, channels 0-7 / 8-15, lanes g = 2, 3 tap 2 q + 1
+// (a 28th zero tap completes the last pair).  Each staged input value serves
+// all 27 taps and NB output blocks, each weight fragment R rows.
+template <int R, int NB>
+__global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
+    constexpr int TZ = 4, HX = 18, HY = R + 2, HZ = TZ + 2, NU = HX * HY * HZ;
+    constexpr int UA = 2 * NU, UB = 28 * 2 * NB * 16, UT = UA + UB, ITER = (UT + 255) / 256;
+    __shared__ bf16x8 lds[UT];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    const int b = blockIdx.y;
+    const int n = 1 << a.lx;
+    const long long nvox = 1ll << (3 * a.lx), nvox2 = nvox >> 3;
+    const int tx = n >> 4, ty = n / R, tz = n / TZ;
+    const int tiles = tx * ty * tz;
+    const int tile = blockIdx.x % tiles, ns = blockIdx.x / tiles;
+    const int x0 = (tile % tx) * 16, y0 = ((tile / tx) % ty) * R, z0 = (tile / (tx * ty)) * TZ;
+    const bf16 *x1 = a.x1 + b * nvox * a.c1;
+    const bf16 *x2 = a.x2 ? a.x2 + b * nvox2 * a.c2 : nullptr;
+    const bf16 *w0 = a.w + (long long)(ns * NB * 16) * a.kpad;
+    f32x4 acc[R][NB];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[r][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int h = g & 1;
+    for (int c0 = 0; c0 < a.cin; c0 += 16) {
+        bf16x8 st[ITER];
+#pragma unroll
+        for (int i = 0; i < ITER; ++i) {
+            const int u = threadIdx.x + 256 * i;
+            bf16x8 val = zero8();
+            if (u < UA) {
+                const int hh = u / NU, v = u - hh * NU;
+                const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
+                const int xx = x0 - 1 + hx, yy = y0 - 1 + hy, zz = z0 - 1 + hz;
+                if ((unsigned)xx < (unsigned)n && (unsigned)yy < (unsigned)n && (unsigned)zz < (unsigned)n) {
+                    const int ch = c0 + hh * 8;
+                    if (ch < a.c1)
+                        val = ld8(x1 + (long long)venc(xx, yy, zz, a.lx) * a.c1 + ch);
+                    else
+                        val = ld8(x2 + (long long)venc(xx >> 1, yy >> 1, zz >> 1, a.lx - 1) * a.c2 + (ch - a.c1));
+                }
+            } else if (u < UT) {
+                const int e = u - UA;
+                const int oc = e % (NB * 16), hh = (e / (NB * 16)) & 1, tap = e / (NB * 32);
+                if (tap < 27) val = ld8(w0 + (long long)oc * a.kpad + (long long)tap * a.cin + c0 + hh * 8);
+            }
+            st[i] = val;
+        }
+        __syncthreads();  // the previous chunk's MFMA reads are done
+#pragma unroll
+        for (int i = 0; i < ITER; ++i) {
+            const int u = threadIdx.x + 256 * i;
+            if (u < UT) lds[u] = st[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 14; ++q) {
+            const int tap = 2 * q + (g >> 1);      // 27: the zero tap
+            const int tp = tap < 27 ? tap : 26;   // (its input address: any valid one)
+            const int dz = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dx = tp % 3 - 1;
+            const int base = h * NU + ((wave + 1 + dz) * HY + (1 + dy)) * HX + (m + 1 + dx);
+            bf16x8 bv[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) bv[nb] = lds[UA + (tap * 2 + h) * (NB * 16) + nb * 16 + m];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const bf16x8 av = lds[base + r * HX];
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) acc[r][nb] = mfma(av, bv[nb], acc[r][nb]);
+            }
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+        const int ch = (ns * NB + nb) * 16 + m;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long vo = b * nvox + venc(x0 + 4 * g + j, y0 + r, z0 + wave, a.lx);
+                float val = acc[r][nb][j];
+                if (a.relu) val = fmaxf(val, 0.0f);
+                a.y1[vo * a.cout + ch] = (bf16)val;
+            }
+        }
+    }
+}
+
 // Sum of a K-split convolution's partial sums in K-range order, ReLU, BF16.
 __global__ void __launch_bounds__(256) k_conv_reduce(ConvArgs a, long long rows) {
     const long long total = rows * a.cout;
@@ -716,6 +816,22 @@ ConvSplit conv_split(long long nvox, int B, int cout, int kpad) {
 
 fl_status launch_conv(fl_unet *u, hipStream_t stm, int B, ConvArgs a) {
     const long long nvox = 1ll << (3 * a.lx);
+    // forward convolutions of the 64^3 and 32^3 levels: the LDS-staged kernel
+    if (!a.mask && a.lx >= 5 && a.cin % 16 == 0 && a.c1 % 16 == 0 && a.cout % 16 == 0 &&
+        a.ysplit == a.cout) {
+        // 64^3 and finer: 16 x 4 x 4 voxels and one output block per
+        // workgroup; 32^3: 16 x 1 x 4 voxels and two output blocks (at
+        // least 512 workgroups either way)
+        const int n = 1 << a.lx, R = a.lx >= 6 ? 4 : 1;
+        const int NB = (a.lx >= 6 || a.cout % 32) ? 1 : 2;
+        const unsigned tiles = (unsigned)((n / 16) * (n / R) * (n / 4));
+        const dim3 grid(tiles * (unsigned)(a.cout / (16 * NB)), (unsigned)B);
+        if (R == 4) k_conv_lds<4, 1><<<grid, 256, 0, stm>>>(a);
+        else if (NB == 2) k_conv_lds<1, 2><<<grid, 256, 0, stm>>>(a);
+        else k_conv_lds<1, 1><<<grid, 256, 0, stm>>>(a);
+        FL_HIP_TRY(u->ctx, hipGetLastError());
+        return FL_OK;
+    }
     const ConvSplit sp = conv_split(nvox, B, a.cout, a.kpad);
     a.nsplit = sp.nsplit;
     a.ksplit = sp.ksplit;
