@@ -567,6 +567,7 @@ __global__ void __launch_bounds__(256) k_sample_partials(const float *__restrict
 // channel 0, channels 1..15 zero.
 __global__ void __launch_bounds__(256) k_pack(const float *__restrict__ in, int nvox,
                                               const float *__restrict__ part, bf16 *__restrict__ x0,
+                                              bf16 *__restrict__ x0c, int full,
                                               float *__restrict__ scale) {
     const int b = blockIdx.y;
     float ss, unused;
@@ -575,11 +576,15 @@ __global__ void __launch_bounds__(256) k_pack(const float *__restrict__ in, int 
     if (blockIdx.x == 0 && threadIdx.x == 0) scale[b] = s;
     const float inv = 1.0f / s;
     for (int v = blockIdx.x * 256 + threadIdx.x; v < nvox; v += gridDim.x * 256) {
-        bf16x8 lo = zero8();
-        lo[0] = (bf16)(in[(long long)b * nvox + v] * inv);
-        bf16 *dst = x0 + ((long long)b * nvox + v) * 16;
-        st8(dst, lo);
-        st8(dst + 8, zero8());
+        const bf16 val = (bf16)(in[(long long)b * nvox + v] * inv);
+        x0c[(long long)b * nvox + v] = val;
+        if (full) {  // the 16-channel copy the first layer's weight gradient reads
+            bf16x8 lo = zero8();
+            lo[0] = val;
+            bf16 *dst = x0 + ((long long)b * nvox + v) * 16;
+            st8(dst, lo);
+            st8(dst + 8, zero8());
+        }
     }
 }
 
@@ -725,6 +730,7 @@ struct fl_unet {
     size_t P = 0, Pt = 0;  // parameter counts (padded), transposed copies
     bf16 *act[kActs] = {}, *grad[kActs] = {};
     bf16 *tmp = nullptr;           // upsample part of a concat layer's data gradient
+    bf16 *x0c = nullptr;           // the normalised input, one channel
     float *w = nullptr, *m1 = nullptr, *m2 = nullptr, *gr = nullptr;
     bf16 *wb = nullptr, *wt = nullptr;
     float *part = nullptr, *part2 = nullptr;  // per-sample statistics rows
@@ -751,7 +757,7 @@ void ufree(fl_unet *u) {
         if (u->act[i]) (void)hipFree(u->act[i]);
         if (u->grad[i]) (void)hipFree(u->grad[i]);
     }
-    void *bufs[] = {u->tmp, u->w, u->m1, u->m2, u->gr, u->wb, u->wt, u->partial,
+    void *bufs[] = {u->tmp, u->x0c, u->w, u->m1, u->m2, u->gr, u->wb, u->wt, u->partial,
                     u->in,  u->tgt, u->out, u->scale, u->loss, u->part, u->part2, u->kpart};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -767,11 +773,11 @@ long long nvox_at(const fl_unet *u, int level) { return 1ll << (3 * (u->lx - lev
 float tscale(const fl_unet *u, double diag) { return (float)(diag / (double)u->gain); }
 
 // x0 <- normalised input (and scale[b] <- rms)
-void launch_pack(fl_unet *u, hipStream_t stm, const float *in, int B) {
+void launch_pack(fl_unet *u, hipStream_t stm, const float *in, int B, bool full = true) {
     const int nv = (int)nvox_at(u, 0);
     const dim3 grid(kSG, (unsigned)B);
     k_sample_partials<0><<<grid, 256, 0, stm>>>(in, nullptr, nullptr, 0.0f, nv, u->part);
-    k_pack<<<grid, 256, 0, stm>>>(in, nv, u->part, u->act[X0], u->scale);
+    k_pack<<<grid, 256, 0, stm>>>(in, nv, u->part, u->act[X0], u->x0c, full ? 1 : 0, u->scale);
 }
 // out <- denormalised zero-mean prediction
 void launch_unpack(fl_unet *u, hipStream_t stm, double diag, float *out, int B) {
@@ -816,12 +822,13 @@ ConvSplit conv_split(long long nvox, int B, int cout, int kpad) {
 
 fl_status launch_conv(fl_unet *u, hipStream_t stm, int B, ConvArgs a) {
     const long long nvox = 1ll << (3 * a.lx);
-    // forward convolutions of the 64^3 and 32^3 levels: the LDS-staged kernel
-    if (!a.mask && a.lx >= 5 && a.cin % 16 == 0 && a.c1 % 16 == 0 && a.cout % 16 == 0 &&
+    // forward convolutions of the 64^3 to 16^3 levels: the LDS-staged kernel
+    if (!a.mask && a.lx >= 4 && a.cin % 16 == 0 && a.c1 % 16 == 0 && a.cout % 16 == 0 &&
         a.ysplit == a.cout) {
         // 64^3 and finer: 16 x 4 x 4 voxels and one output block per
-        // workgroup; 32^3: 16 x 1 x 4 voxels and two output blocks (at
-        // least 512 workgroups either way)
+        // workgroup; 32^3 and 16^3: 16 x 1 x 4 voxels and two output blocks
+        // where the channels pair up (512 workgroups at 64^3 and 32^3, 64 per
+        // output block at 16^3, where the old kernel needed K splits)
         const int n = 1 << a.lx, R = a.lx >= 6 ? 4 : 1;
         const int NB = (a.lx >= 6 || a.cout % 32) ? 1 : 2;
         const unsigned tiles = (unsigned)((n / 16) * (n / R) * (n / 4));
@@ -892,6 +899,8 @@ fl_status forward(fl_unet *u, hipStream_t stm, int B) {
         a.relu = ls.relu;
         if (op == 0) {  // the one-channel input: 27 taps, not 27 x 16
             a.batch = B;
+            a.x1 = u->x0c;  // the compact one-channel copy
+            a.c1 = 1;
             k_conv_in1<<<dim3((unsigned)(nvox_at(u, 0) / 64), (unsigned)B), 256, 0, stm>>>(a);
             FL_HIP_TRY(u->ctx, hipGetLastError());
             continue;
@@ -1033,7 +1042,7 @@ int unet_resolution(const fl_unet *net) { return net ? net->n : 0; }
 fl_ctx *unet_context(const fl_unet *net) { return net ? net->ctx : nullptr; }
 
 fl_status unet_infer_device(fl_unet *u, hipStream_t stm, const float *in, double diag, float *out) {
-    launch_pack(u, stm, in, 1);
+    launch_pack(u, stm, in, 1, false);
     FL_HIP_TRY(u->ctx, hipGetLastError());
     fl_status st = forward(u, stm, 1);
     if (st != FL_OK) return st;
@@ -1109,6 +1118,7 @@ fl_status fl_unet_create(fl_ctx *ctx, const fl_unet_config *cfg, fl_unet **out) 
          (st = ualloc(u, (void **)&u->wb, P * 2)) || (st = ualloc(u, (void **)&u->wt, Pt * 2)) ||
          (st = ualloc(u, (void **)&u->partial, (size_t)u->wgroups * max_partial * 4)) ||
          (st = ualloc(u, (void **)&u->in, B * nv0 * 4)) ||
+         (st = ualloc(u, (void **)&u->x0c, B * nv0 * 2)) ||
          (st = ualloc(u, (void **)&u->tgt, B * nv0 * 4)) ||
          (st = ualloc(u, (void **)&u->out, B * nv0 * 4)) ||
          (st = ualloc(u, (void **)&u->scale, B * 4)) || (st = ualloc(u, (void **)&u->loss, B * 4)) ||
@@ -1231,7 +1241,7 @@ fl_status fl_unet_infer(fl_unet *u, int32_t batch, const float *rhs, double diag
     hipStream_t stm = ctx->stream;
     const long long nv = nvox_at(u, 0);
     FL_HIP_TRY(ctx, hipMemcpyAsync(u->in, rhs, batch * nv * sizeof(float), hipMemcpyHostToDevice, stm));
-    launch_pack(u, stm, u->in, batch);
+    launch_pack(u, stm, u->in, batch, false);
     FL_HIP_TRY(ctx, hipGetLastError());
     fl_status st = forward(u, stm, batch);
     if (st != FL_OK) return st;
