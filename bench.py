@@ -691,7 +691,10 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if tr.get("workload") == out["config"]["workload"] and dom in tr.get("kernels", {}):
+        # the profiled workload names the grid, Re, dt and stop test; the
+        # headline workload string continues with the variant's description
+        if (out["config"]["workload"].startswith(tr.get("workload", "\0"))
+                and dom in tr.get("kernels", {})):
             out["roofline"]["traffic"] = tr["kernels"][dom]["traffic_bytes"]
             out["roofline"]["traffic_source"] = tr["source"]
     # every variant's steps/s and CG iterations/step as top-level scalars

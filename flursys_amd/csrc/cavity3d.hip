@@ -1545,24 +1545,29 @@ __device__ inline void cg_tile_pass(const Grid &g, T *lds, const T *__restrict__
     T *const h_slot = lds + (wv == 0 ? 0 : TY + 1) * S + OFF + lane * V;
     T *const x_slot = lds + (1 + (east ? t - TY : t)) * S + (east ? OFF + W : OFF - 1);
     const bool wl = i0 == 0, el = i0 + V == nx, sl = j == 0, nl = j + 1 == ny;
-    TV rr{}, dd{}, xx{}, hr{}, hd{};
-    T xr = 0, xd = 0;
-    auto load = [&](uint32_t k) {
+    // the raw loads of one plane (those of plane k + 1 are in flight while
+    // plane k is computed)
+    struct Ld {
+        TV rr, dd, xx, hr, hd;
+        T xr, xd;
+    };
+    Ld LA{};
+    auto load = [&](Ld &L, uint32_t k) {
         const uint32_t pk = nxy * k;
         const uint32_t o = own ? pk : 0u, h = hrow ? pk : o, q = xh ? pk : o;
-        rr = *(const TV *)(r + co + o);
-        if (!kFirst) dd = *(const TV *)(dold + co + o);
-        if (kStep) xx = *(const TV *)(x + co + o);
-        hr = *(const TV *)(r + ch + h);
-        if (!kFirst) hd = *(const TV *)(dold + ch + h);
-        xr = r[cxh + q];
-        if (!kFirst) xd = dold[cxh + q];
+        L.rr = *(const TV *)(r + co + o);
+        if (!kFirst) L.dd = *(const TV *)(dold + co + o);
+        if (kStep) L.xx = *(const TV *)(x + co + o);
+        L.hr = *(const TV *)(r + ch + h);
+        if (!kFirst) L.hd = *(const TV *)(dold + ch + h);
+        L.xr = r[cxh + q];
+        if (!kFirst) L.xd = dold[cxh + q];
     };
     // d' of the own cells of plane k from the loaded r, d
-    auto own_dprime = [&](uint32_t k) -> RV {
+    auto own_dprime = [&](const Ld &L, uint32_t k) -> RV {
         const int zw = wall_plane(k);
         const bool pin = pinlane && k == 0;
-        const T *rv = (const T *)&rr, *dv = (const T *)&dd;
+        const T *rv = (const T *)&L.rr, *dv = (const T *)&L.dd;
         RV o;
 #pragma unroll
         for (int v = 0; v < V; ++v) {
@@ -1575,12 +1580,12 @@ __device__ inline void cg_tile_pass(const Grid &g, T *lds, const T *__restrict__
     TV rc{}, rp{}, xc{}, xp{};
     // d' of plane k into `out` (own cells) and LDS buffer bf (own cells, halos,
     // and at a wall the edge cell's own value as its clamped neighbour)
-    auto form = [&](uint32_t k, int bf, RV &out, TV &rs, TV &xs) {
+    auto form = [&](const Ld &L, uint32_t k, int bf, RV &out, TV &rs, TV &xs) {
         const int zw = wall_plane(k);
         if (own) {
-            out = own_dprime(k);
-            rs = rr;
-            if (kStep) xs = xx;
+            out = own_dprime(L, k);
+            rs = L.rr;
+            if (kStep) xs = L.xx;
             T *const ob = own_slot + bf * BUF;
             *(TV *)ob = *(const TV *)&out;
             if (wl) ob[-1] = out.a[0];
@@ -1589,7 +1594,7 @@ __device__ inline void cg_tile_pass(const Grid &g, T *lds, const T *__restrict__
             if (nl) *(TV *)(ob + S) = *(const TV *)&out;
         }
         if (hrow) {
-            const T *rv = (const T *)&hr, *dv = (const T *)&hd;
+            const T *rv = (const T *)&L.hr, *dv = (const T *)&L.hd;
             RV h;
 #pragma unroll
             for (int v = 0; v < V; ++v) {
@@ -1599,28 +1604,14 @@ __device__ inline void cg_tile_pass(const Grid &g, T *lds, const T *__restrict__
             *(TV *)(h_slot + bf * BUF) = *(const TV *)&h;
         }
         if (xh) {
-            const T z = xr * ixv[zw];
-            x_slot[bf * BUF] = kFirst ? z : z + tb * xd;
+            const T z = L.xr * ixv[zw];
+            x_slot[bf * BUF] = kFirst ? z : z + tb * L.xd;
         }
     };
-    if (k0 > 0) {
-        load(k0 - 1);
-        em = own_dprime(k0 - 1);
-    }
-    load(k0);
-    form(k0, 0, ec, rc, xc);
-    if (k0 == 0) em = ec;
-    if (k0 + 1 < nz) load(k0 + 1);
-    __syncthreads();
+    // plane k from LDS buffer bf and the registers (em, ec, ep); the step
+    // pass stores x, r', d' of its cells
     uint32_t c = co + nxy * k0;
-    for (uint32_t k = k0; k < k1; ++k, c += nxy) {
-        const int bf = (k - k0) & 1;
-        if (k + 1 < nz) {
-            form(k + 1, bf ^ 1, ep, rp, xp);
-            if (k + 2 < nz && k + 1 < k1) load(k + 2);
-        } else {
-            ep = ec;
-        }
+    auto compute = [&](uint32_t k, int bf) {
         if (own) {
             const T *row = own_slot + bf * BUF;
             const bool pin = pinlane && k == 0;
@@ -1659,11 +1650,34 @@ __device__ inline void cg_tile_pass(const Grid &g, T *lds, const T *__restrict__
                 *(TV *)(dnew + c) = *(const TV *)&ec;
             }
         }
-        __syncthreads();
+    };
+    auto advance = [&]() {
         em = ec;
         ec = ep;
         rc = rp;
         if (kStep) xc = xp;
+        c += nxy;
+    };
+    if (k0 > 0) {
+        load(LA, k0 - 1);
+        em = own_dprime(LA, k0 - 1);
+    }
+    load(LA, k0);
+    form(LA, k0, 0, ec, rc, xc);
+    if (k0 == 0) em = ec;
+    if (k0 + 1 < nz) load(LA, k0 + 1);
+    __syncthreads();
+    for (uint32_t k = k0; k < k1; ++k) {
+        const int bf = (k - k0) & 1;
+        if (k + 1 < nz) {
+            form(LA, k + 1, bf ^ 1, ep, rp, xp);
+            if (k + 2 < nz && k + 1 < k1) load(LA, k + 2);
+        } else {
+            ep = ec;
+        }
+        compute(k, bf);
+        __syncthreads();
+        advance();
     }
 }
 
@@ -1793,6 +1807,149 @@ __global__ void __launch_bounds__(kZT) k_fcg_dirapply_z(Grid g, const e16 *__res
             s[0] += (double)v[0] * (double)qc;
             s[1] += (double)rc * (double)v[0];
         });
+    publish_row<2, 1>(s, m, pout);
+}
+
+// The same direction as a plane-tile pass (FP32, four cells per lane; nx a
+// multiple of 4): a block owns a 256 x 4 (x, y) tile over a chunk of planes,
+// forms p of the next plane into an LDS plane buffer with a one-cell halo
+// (rows above and below by waves 0 and 1, the cells west and east of each
+// row by threads 0-7; at a wall the edge cell's own p, the clamped
+// neighbour), takes the in-plane neighbours from LDS and the z neighbours
+// from registers.  Two LDS buffers alternate, one barrier per plane.  The
+// stencil arithmetic is k_fcg_dirapply_z's, in the same order.
+__global__ void __launch_bounds__(kBlock) k_fcg_dirapply_t(Grid g, const e16 *__restrict__ zt,
+                                                        const float *__restrict__ pold,
+                                                        float *__restrict__ pnew, float *__restrict__ q,
+                                                        const float *__restrict__ r,
+                                                        const double *__restrict__ pz, uint32_t rows,
+                                                        double *__restrict__ pout, const FcgA *SA,
+                                                        const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    const float iS = 1.0f / SA->escale;
+    const bool first = SB->first != 0;
+    double fz[2], fm[1];
+    fold_partials<2, 1>(pz, rows, fz, fm);
+    const float z0 = e_ld(zt, 0, iS);
+    const double beta = first ? 0.0 : (fz[0] - (double)z0 * fz[1]) / SB->pq;
+    const float tb = (float)beta;
+    const float cx = (float)g.ix2, cy = (float)g.iy2, cz = (float)g.iz2;
+    double s[2] = {0.0, 0.0}, m[1] = {0.0};
+    __shared__ __align__(16) float lds[tile_lds_len<float>()];
+    constexpr int V = 4;
+    constexpr int W = 64 * V, TY = kTileRows, OFF = V, S = W + 2 * V, BUF = (TY + 2) * S;
+    typedef e16 Z4 __attribute__((ext_vector_type(4)));  // 8-byte loads of 4 iterates
+    struct F4 {
+        float a[4];
+    };
+    const uint3 b = zblock();
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t nx = g.nx, ny = g.ny, nz = g.nz, nxy = nx * ny;
+    const uint32_t tx0 = b.x * W, ty0 = b.y * TY, i0 = tx0 + lane * V, j = ty0 + wv;
+    const uint32_t kz = (nz + gridDim.z - 1) / gridDim.z, k0 = b.z * kz;
+    if (k0 < nz) {
+        const uint32_t k1 = min(k0 + kz, nz);
+        const bool own = i0 < nx && j < ny;
+        const uint32_t jh = wv == 0 ? ty0 - 1 : ty0 + TY;
+        const bool hrow = wv < 2 && i0 < nx && (wv == 0 ? ty0 > 0 : ty0 + TY < ny);
+        const uint32_t t = threadIdx.x;
+        const bool east = t >= TY;
+        const uint32_t jx = ty0 + (east ? t - TY : t), ix = east ? tx0 + W : tx0 - 1;
+        const bool xh = t < 2 * TY && jx < ny && (east ? tx0 + W < nx : tx0 > 0);
+        // every lane issues the same loads (a lane with nothing to load reads
+        // its own or cell 0's values), so the loads' wait counts are static
+        const uint32_t co = own ? i0 + nx * j : 0u, ch = hrow ? i0 + nx * jh : co, cxh = xh ? ix + nx * jx : co;
+        const bool pin0 = i0 == 0 && j == 0;  // owns cell 0 (plane 0, v = 0): p_0 = q_0 = 0
+        float *const own_slot = lds + (1 + wv) * S + OFF + lane * V;
+        float *const h_slot = lds + (wv == 0 ? 0 : TY + 1) * S + OFF + lane * V;
+        float *const x_slot = lds + (1 + (east ? t - TY : t)) * S + (east ? OFF + W : OFF - 1);
+        const bool wl = i0 == 0, el = i0 + V == nx, sl = j == 0, nl = j + 1 == ny;
+        Z4 oz{}, hz{};
+        float4 op{}, hp{};
+        e16 xz = 0;
+        float xp = 0.0f;
+        auto load = [&](uint32_t k) {
+            const uint32_t pk = nxy * k;
+            const uint32_t o = own ? pk : 0u, h = hrow ? pk : o, qq = xh ? pk : o;
+            oz = *(const Z4 *)(zt + co + o);
+            hz = *(const Z4 *)(zt + ch + h);
+            xz = zt[cxh + qq];
+            if (!first) {
+                op = *(const float4 *)(pold + co + o);
+                hp = *(const float4 *)(pold + ch + h);
+                xp = pold[cxh + qq];
+            }
+        };
+        auto pval = [&](e16 z, float po) { return ((float)z * iS - z0) - tb * po; };
+        F4 pm{}, pc{}, pn{};
+        // p of plane k: own cells into `out`, and into LDS buffer bf
+        auto form = [&](uint32_t k, int bf, F4 &out) {
+            if (own) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) out.a[v] = pval(oz[v], first ? 0.0f : op[v]);
+                if (pin0 && k == 0) out.a[0] = 0.0f;
+                float *const ob = own_slot + bf * BUF;
+                *(float4 *)ob = *(const float4 *)&out;
+                if (wl) ob[-1] = out.a[0];
+                if (el) ob[V] = out.a[V - 1];
+                if (sl) *(float4 *)(ob - S) = *(const float4 *)&out;
+                if (nl) *(float4 *)(ob + S) = *(const float4 *)&out;
+            }
+            if (hrow) {
+                F4 h;
+#pragma unroll
+                for (int v = 0; v < V; ++v) h.a[v] = pval(hz[v], first ? 0.0f : hp[v]);
+                *(float4 *)(h_slot + bf * BUF) = *(const float4 *)&h;
+            }
+            if (xh) x_slot[bf * BUF] = pval(xz, first ? 0.0f : xp);
+        };
+        // the plane k0 - 1 below the chunk (or plane k0 itself at the bottom wall)
+        if (k0 > 0) {
+            load(k0 - 1);
+            if (own) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) pm.a[v] = pval(oz[v], first ? 0.0f : op[v]);
+                if (pin0 && k0 == 1) pm.a[0] = 0.0f;
+            }
+        }
+        load(k0);
+        form(k0, 0, pc);
+        if (k0 == 0) pm = pc;
+        if (k0 + 1 < nz) load(k0 + 1);
+        __syncthreads();
+        uint32_t c = co + nxy * k0;
+        for (uint32_t k = k0; k < k1; ++k, c += nxy) {
+            const int bf = (k - k0) & 1;
+            const float4 rc = *(const float4 *)(r + c);  // (cell (0, 0, k)'s for a lane without cells)
+            if (k + 1 < nz) {
+                form(k + 1, bf ^ 1, pn);
+                if (k + 2 < nz && k + 1 < k1) load(k + 2);
+            } else {
+                pn = pc;
+            }
+            if (own) {
+                const float *row = own_slot + bf * BUF;
+                F4 qo;
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const float v0 = pc.a[v];
+                    const float ve = v + 1 < V ? pc.a[v + 1] : row[V];
+                    const float vw = v > 0 ? pc.a[v - 1] : row[-1];
+                    const float vn = row[S + v], vs = row[-S + v];
+                    const float qf = cx * ((v0 - ve) + (v0 - vw)) + cy * ((v0 - vn) + (v0 - vs)) +
+                                     cz * ((v0 - pn.a[v]) + (v0 - pm.a[v]));
+                    qo.a[v] = (v == 0 && pin0 && k == 0) ? 0.0f : qf;
+                    s[0] += (double)v0 * (double)qo.a[v];
+                    s[1] += (double)rc[v] * (double)v0;
+                }
+                *(float4 *)(pnew + c) = *(const float4 *)&pc;
+                *(float4 *)(q + c) = *(const float4 *)&qo;
+            }
+            __syncthreads();
+            pm = pc;
+            pc = pn;
+        }
+    }
     publish_row<2, 1>(s, m, pout);
 }
 
@@ -2201,7 +2358,7 @@ enum Kind {
     K_MG_CHECK, K_MG_SWEEP_L0, K_MG_SWEEP_L0_PROLONG, K_MG_SWEEP_L0_PUBLISH,
     K_MG_RESTRICT_L0, K_MG_DOWN, K_MG_TAIL, K_MG_UP, K_FCG_DIRAPPLY, K_FCG_UPDATE, K_WARM_START,
     K_UNET_COARSE,
-    K_CGDOT_D, K_CGSTEP_D, K_CGDOT_F, K_CGSTEP_F,
+    K_CGDOT_D, K_CGSTEP_D, K_CGDOT_F, K_CGSTEP_F, K_FCG_DIRAPPLY_T,
     K_COUNT
 };
 const char *const kKindNames[K_COUNT] = {
@@ -2215,6 +2372,7 @@ const char *const kKindNames[K_COUNT] = {
     "unet_warm_start",  // restriction + U-Net inference + interpolation (many kernels)
     "unet_coarse_solve",  // U-Net inference + the residual on its level (many kernels)
     "k_cg_dot_t<double>", "k_cg_step_t<double>", "k_cg_dot_t<float>", "k_cg_step_t<float>",
+    "k_fcg_dirapply_t",
 };
 
 }  // namespace c3
@@ -2283,9 +2441,13 @@ struct fl_cavity3d {
     ZGrid zgd{1, 1, 1, 1};  // and of the search directions
     ZGrid zr{1, 1, 1, 1};   // of the residual (two cells per lane when nx is even)
     ZGrid zp{1, 1, 1, 1};   // of the FP64 search direction (likewise, 32-plane marches)
-    // plane-tile Jacobi-PCG passes (FP64, FP32); count 0 where nx is not a
-    // multiple of the lane width (2 FP64, 4 FP32 cells)
-    ZGrid zt{0, 0, 0, 0}, ztf{0, 0, 0, 0};
+    // plane-tile Jacobi-PCG passes (FP64, FP32): the dot pass's and the step
+    // pass's launches; count 0 where nx is not a multiple of the lane width
+    // (2 FP64, 4 FP32 cells)
+    ZGrid zt{0, 0, 0, 0}, ztf{0, 0, 0, 0}, zts{0, 0, 0, 0}, ztfs{0, 0, 0, 0};
+    // plane-tile FCG search direction (count 0: nx not a multiple of 4, the
+    // column march runs instead)
+    ZGrid ztd{0, 0, 0, 0};
     int64_t step = 0;
     KProf prof;  // per-kernel timing (HIP events on the solver's stream)
     double *cell_out = nullptr;  // output staging: cell-centred u, v, w (allocated on first use)
@@ -2521,7 +2683,7 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->sb, &s->host->b, sizeof(CgStateB), hipMemcpyHostToDevice, stm));
     const uint32_t G = s->blocks;
     const bool fp64 = sizeof(T) == 8;
-    const ZGrid &zt = fp64 ? s->zt : s->ztf;
+    const ZGrid &zt = fp64 ? s->zt : s->ztf, &zts = fp64 ? s->zts : s->ztfs;
     auto aligned = [](const void *p) { return ((uintptr_t)p & 15u) == 0; };
     if (sizeof(TX) == sizeof(T) && zt.count > 0 && aligned(x) && aligned(r) && aligned(q) &&
         aligned(dbuf[0]) && aligned(dbuf[1])) {
@@ -2529,7 +2691,7 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
         // second residual buffer: iteration i reads r, d from slot i % 2 and
         // writes r', d' to the other.  The final residual is scratch.
         T *const rbuf[2] = {r, q};
-        const dim3 tg(zt.bx, zt.by, zt.bz);
+        const dim3 tg(zt.bx, zt.by, zt.bz), tgs(zts.bx, zts.by, zts.bz);
         const int kd = fp64 ? K_CGDOT_D : K_CGDOT_F, ks = fp64 ? K_CGSTEP_D : K_CGSTEP_F;
         // algorithmic bytes: the dot pass reads r, d; the step reads x, r, d
         // and writes x, r', d'
@@ -2541,12 +2703,12 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
                 const int a = (int)(launched & 1);
                 FL_TIMED(s, kd, launched, bd,
                          k_cg_dot_t<T><<<tg, kBlock, 0, stm>>>(s->g, prm, rbuf[a], dbuf[a], s->pb,
-                                                               launched == 0 ? rows0 : zt.count, s->pa,
+                                                               launched == 0 ? rows0 : zts.count, s->pa,
                                                                s->sa, s->sb));
                 FL_TIMED(s, ks, launched, bs,
-                         k_cg_step_t<T><<<tg, kBlock, 0, stm>>>(s->g, (T *)x, rbuf[a], dbuf[a], rbuf[a ^ 1],
-                                                                dbuf[a ^ 1], s->pa, zt.count, s->pb, s->sa,
-                                                                s->sb, prm.neumann));
+                         k_cg_step_t<T><<<tgs, kBlock, 0, stm>>>(s->g, (T *)x, rbuf[a], dbuf[a], rbuf[a ^ 1],
+                                                                 dbuf[a ^ 1], s->pa, zt.count, s->pb, s->sa,
+                                                                 s->sb, prm.neumann));
                 ++launched;
             }
             FL_HIP_TRY(ctx, hipGetLastError());
@@ -2778,10 +2940,17 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
              k_mg_sweep0_z<true><<<zg, kZT, 0, stm>>>(L[0], s->r32, s->m0a, s->m0b, s->q32, s->pz, s->fa, s->fb));
     // FCG direction (reads z~, p_old, r; writes p, q) and update (x, r
     // read and written, p and q read)
-    FL_TIMED(s, K_FCG_DIRAPPLY, k, 18.0 * n0,
-             k_fcg_dirapply_z<<<dim3(s->zgd.bx, s->zgd.by, s->zgd.bz), kZT, 0, stm>>>(s->g, s->m0b, pold, pnew, s->q32, s->r32, s->pz, s->zg.count, s->pa, s->fa, s->fb));
+    uint32_t drows = s->zgd.count;
+    if (s->ztd.count > 0) {
+        drows = s->ztd.count;
+        FL_TIMED(s, K_FCG_DIRAPPLY_T, k, 18.0 * n0,
+                 k_fcg_dirapply_t<<<dim3(s->ztd.bx, s->ztd.by, s->ztd.bz), kBlock, 0, stm>>>(s->g, s->m0b, pold, pnew, s->q32, s->r32, s->pz, s->zg.count, s->pa, s->fa, s->fb));
+    } else {
+        FL_TIMED(s, K_FCG_DIRAPPLY, k, 18.0 * n0,
+                 k_fcg_dirapply_z<<<dim3(s->zgd.bx, s->zgd.by, s->zgd.bz), kZT, 0, stm>>>(s->g, s->m0b, pold, pnew, s->q32, s->r32, s->pz, s->zg.count, s->pa, s->fa, s->fb));
+    }
     FL_TIMED(s, K_FCG_UPDATE, k, 24.0 * n0,
-             k_fcg_update<<<G, kBlock, 0, stm>>>((uint32_t)s->n, s->e32, s->r32, pnew, s->q32, s->pa, s->zgd.count, s->pb, s->fa, s->fb));
+             k_fcg_update<<<G, kBlock, 0, stm>>>((uint32_t)s->n, s->e32, s->r32, pnew, s->q32, s->pa, drows, s->pb, s->fa, s->fb));
     return FL_OK;
 }
 
@@ -3176,6 +3345,28 @@ fl_status do_step(fl_cavity3d *s, fl_cavity3d_step_report *rep) {
     return st;
 }
 
+// Launch of a plane-tile pass with at most `per_cu` blocks per CU (0: as
+// many as the kernel's registers and LDS let one CU hold), all resident at
+// once: the z chunks are as long as needed, at least 8 planes.  A pass's
+// block marches its chunk bound by the latency of the next plane's loads, so
+// one round of resident blocks beats a second, partly idle one (measured at
+// 256^3: FP64 dot pass 87 -> 81 us against 16-plane chunks).  The FP64 step
+// pass is fastest at 2 blocks per CU (64-plane chunks at 256^3: 146 -> 140
+// us), the FP32 passes at full occupancy.
+template <typename T>
+ZGrid tile_grid_resident(fl_ctx *ctx, const void *kernel, uint32_t nx, uint32_t ny, uint32_t nz,
+                         int per_cu = 0) {
+    ZGrid z = tile_grid<T>(nx, ny, nz, nz);
+    int fit = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, kernel, kBlock, 0) != hipSuccess || fit < 1) fit = 1;
+    if (per_cu > 0) fit = std::min(fit, per_cu);
+    const uint64_t slots = (uint64_t)fit * (uint64_t)std::max(ctx->num_cu, 1);
+    const uint64_t layer = (uint64_t)z.bx * z.by;
+    uint32_t bz = (uint32_t)std::max<uint64_t>(1, slots / layer);
+    bz = std::min(bz, std::max(1u, nz / 8));
+    return tile_grid<T>(nx, ny, nz, (nz + bz - 1) / bz);
+}
+
 }  // namespace
 
 extern "C" {
@@ -3224,12 +3415,21 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
         s->zp.bx = s->zr.bx;
         s->zp.count = s->zp.bx * s->zp.by * s->zp.bz;
     }
-    // tile passes: 16-plane chunks (as fast as 32 at 256^3, faster on small grids)
-    if (nx % 2 == 0) s->zt = tile_grid<double>(nx, ny, nz, 16);
-    if (nx % 4 == 0) s->ztf = tile_grid<float>(nx, ny, nz, 16);
+    if (nx % 2 == 0) {
+        s->zt = tile_grid_resident<double>(ctx, (const void *)k_cg_dot_t<double>, nx, ny, nz);
+        s->zts = tile_grid_resident<double>(ctx, (const void *)k_cg_step_t<double>, nx, ny, nz, 2);
+    }
+    if (nx % 4 == 0) {
+        s->ztf = tile_grid_resident<float>(ctx, (const void *)k_cg_dot_t<float>, nx, ny, nz);
+        s->ztfs = tile_grid_resident<float>(ctx, (const void *)k_cg_step_t<float>, nx, ny, nz);
+    }
+    // the FCG direction's tile pass (16-byte float and 8-byte FP16 lane loads)
+    if (nx % 4 == 0 && cfg->preconditioner != FL_PRECOND_JACOBI)
+        s->ztd = tile_grid_resident<float>(ctx, (const void *)k_fcg_dirapply_t, nx, ny, nz);
     const size_t prow =
         (size_t)std::max(std::max(std::max(s->blocks, s->ew_blocks), std::max(s->zg.count, s->zgd.count)),
-                         std::max(s->zt.count, s->ztf.count)) * 4;
+                         std::max(std::max(std::max(s->zt.count, s->ztf.count), std::max(s->zts.count, s->ztfs.count)),
+                                  s->ztd.count)) * 4;
     if ((st = alloc(s, &s->u, s->nu)) || (st = alloc(s, &s->v, s->nv)) ||
         (st = alloc(s, &s->w, s->nw)) || (st = alloc(s, &s->us, s->nu)) ||
         (st = alloc(s, &s->vs, s->nv)) || (st = alloc(s, &s->ws, s->nw)) ||
