@@ -583,9 +583,10 @@ def main():
                     help="PCG iterations of the CPU baseline sample (about 15 s at 256^3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-copy-probe", action="store_true")
-    ap.add_argument("--config", default="cavity3d", choices=["cavity3d", "batch2d"],
+    ap.add_argument("--config", default="cavity3d", choices=["cavity3d", "batch2d", "csr"],
                     help="cavity3d: the headline 256^3 benchmark; batch2d: 64 uneven 2D "
-                         "cavities advanced together (a separate JSON line)")
+                         "cavities advanced together; csr: the 2M-cell CSR CG / PCG / "
+                         "BiCGSTAB record alone (each a separate JSON line)")
     ap.add_argument("--batch-steps", type=int, default=1000,
                     help="steps of every cavity in the batch2d configuration")
     ap.add_argument("--no-extras", action="store_true",
@@ -611,6 +612,10 @@ def main():
     ctx = fa.Context(0)
     if args.config == "batch2d":
         print(json.dumps(run_batch2d(fa, ctx, args)), flush=True)
+        ctx.close()
+        return
+    if args.config == "csr":
+        print(json.dumps(run_c3(fa, ctx, args)), flush=True)
         ctx.close()
         return
     # learned_fcg: the network's coarse prediction corrected by the V-cycle
