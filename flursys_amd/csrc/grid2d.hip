@@ -1522,6 +1522,8 @@ void drop_graphs(Batch2 *B) {
 }
 
 void free_batch(Batch2 *B) {
+    // the graph execs first (the stream has drained): none refers to freed memory
+    drop_graphs(B);
     void *d[] = {B->probs, B->tile_prob, B->flags, B->x,  B->rhs, B->r,  B->d0,    B->d1,
                  B->q,     B->uc,        B->vc,    B->sp, B->vort, B->u, B->v,     B->us,
                  B->vs,    B->uo,        B->vo,    B->pa, B->pb,   B->p0, B->dpart, B->sa,
@@ -1537,7 +1539,6 @@ void free_batch(Batch2 *B) {
     if (B->zv) (void)hipFree(B->zv);
     if (B->sor_st) (void)hipFree(B->sor_st);
     if (B->h_sor) (void)hipHostFree(B->h_sor);
-    drop_graphs(B);
     for (hipEvent_t e : B->ev) (void)hipEventDestroy(e);
 }
 
