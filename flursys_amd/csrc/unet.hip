@@ -83,6 +83,8 @@ struct ConvArgs {
     int nsplit;      // output-channel groups of NB blocks, one per workgroup
     int ksplit;      // K ranges, one per workgroup (> 1: partial sums, then k_conv_reduce)
     int batch;
+    float *hout;     // inference head (k_conv_lds<R, 1, true>): output channel 0, FP32 [batch][voxel]
+    float *hsum;     // and its per-workgroup sums [batch][tile]
 };
 
 // One 3x3x3 convolution (or, with flipped transposed weights and the ReLU
@@ -193,8 +195,16 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
 // 256 contiguous bytes of either.  K runs over the 27 taps in pairs: lanes
 // g = 0, 1 take tap 2 q, channels 0-7 / 8-15, lanes g = 2, 3 tap 2 q + 1
 // (a 28th zero tap completes the last pair).  Each staged input value serves
-// all 27 taps and NB output blocks, each weight fragment R rows.
-template <int R, int NB>
+// all 27 taps and NB output blocks, each weight fragment R rows.  The MFMA
+// takes the weight fragment as A (rows: output channels) and the voxel row
+// as B (columns: voxels), so a lane's result is four consecutive channels of
+// one voxel, stored as 8 contiguous bytes.
+//
+// kHead (inference): the network's head.  Only output channel 0 is used, so
+// the lanes that hold it store it compactly as FP32 (the BF16 value the full
+// head would store) and the workgroup's sum of it goes to hsum[sample][tile];
+// the arithmetic is the full head's.
+template <int R, int NB, bool kHead = false>
 __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
     constexpr int TZ = 4, HX = 18, HY = R + 2, HZ = TZ + 2, NU = HX * HY * HZ;
     constexpr int UA = 2 * NU, UB = 28 * 2 * NB * 16, UT = UA + UB, ITER = (UT + 255) / 256;
@@ -261,22 +271,47 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
             for (int r = 0; r < R; ++r) {
                 const bf16x8 av = lds[base + r * HX];
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb) acc[r][nb] = mfma(av, bv[nb], acc[r][nb]);
+                for (int nb = 0; nb < NB; ++nb) acc[r][nb] = mfma(bv[nb], av, acc[r][nb]);
             }
         }
     }
+    // D[4g + j][m]: channels 4g .. 4g + 3 of the block of voxel (x0 + m, y0 + r, z0 + wave)
+    if (kHead) {
+        __shared__ float hs[4];
+        float sum = 0.0f;
+        if (g == 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float val = acc[r][0][0];
+                if (a.relu) val = fmaxf(val, 0.0f);
+                val = (float)(bf16)val;
+                a.hout[b * nvox + venc(x0 + m, y0 + r, z0 + wave, a.lx)] = val;
+                sum += val;
+            }
+        }
+        // lanes 0-15 of each wave hold the sums: fixed-order reduction
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+        if (lane == 0) hs[wave] = sum;
+        __syncthreads();
+        if (threadIdx.x == 0) a.hsum[(long long)b * tiles + tile] = (hs[0] + hs[1]) + (hs[2] + hs[3]);
+        return;
+    }
+    typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-        const int ch = (ns * NB + nb) * 16 + m;
+        const int ch = (ns * NB + nb) * 16 + 4 * g;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
+            const long long vo = b * nvox + venc(x0 + m, y0 + r, z0 + wave, a.lx);
+            bf16x4 o;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const long long vo = b * nvox + venc(x0 + 4 * g + j, y0 + r, z0 + wave, a.lx);
                 float val = acc[r][nb][j];
                 if (a.relu) val = fmaxf(val, 0.0f);
-                a.y1[vo * a.cout + ch] = (bf16)val;
+                o[j] = (bf16)val;
             }
+            *reinterpret_cast<bf16x4 *>(a.y1 + vo * a.cout + ch) = o;
         }
     }
 }
@@ -286,50 +321,22 @@ __global__ void __launch_bounds__(256) k_conv_reduce(ConvArgs a, long long rows)
     const long long total = rows * a.cout;
     const int yc2 = a.cout - a.ysplit;
     for (long long e = blockIdx.x * 256ll + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+        // the K ranges summed in order; eight loads in flight at a time
         float s = 0.0f;
-        for (int ks = 0; ks < a.ksplit; ++ks) s += a.part[(long long)ks * total + e];
+        int ks = 0;
+        for (; ks + 8 <= a.ksplit; ks += 8) {
+            float v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = a.part[(long long)(ks + i) * total + e];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) s += v[i];
+        }
+        for (; ks < a.ksplit; ++ks) s += a.part[(long long)ks * total + e];
         if (a.relu) s = fmaxf(s, 0.0f);
         const long long vo = e / a.cout;
         const int ch = (int)(e - vo * a.cout);
         if (ch < a.ysplit) a.y1[vo * a.ysplit + ch] = (bf16)s;
         else a.y2[vo * yc2 + (ch - a.ysplit)] = (bf16)s;
-    }
-}
-
-// The first layer's convolution of the one-channel input (stored with a
-// channel stride of 16, channels 1..15 zero): K = 27 taps (one 32-wide MFMA
-// step) instead of 27 x 16.  Lane group g takes taps 8 g .. 8 g + 7, whose
-// weights sit at k = tap * 16 of the layer's [cout][K padded] rows.
-__global__ void __launch_bounds__(256) k_conv_in1(ConvArgs a) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int m = lane & 15, g = lane >> 4;
-    const int b = blockIdx.y;
-    const int n = 1 << a.lx;
-    const long long nvox = 1ll << (3 * a.lx);
-    const int vbase = (blockIdx.x * 4 + wave) * 16;
-    int x, y, z;
-    vdec(vbase + m, a.lx, x, y, z);
-    const bf16 *x1 = a.x1 + b * nvox * a.c1;
-    bf16x8 av = zero8(), bv = zero8();
-    const bf16 *wp = a.w + (long long)m * a.kpad;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const int tp = 8 * g + j;
-        if (tp < 27) {
-            const int dz = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dx = tp % 3 - 1;
-            const int xx = x + dx, yy = y + dy, zz = z + dz;
-            if ((unsigned)xx < (unsigned)n && (unsigned)yy < (unsigned)n && (unsigned)zz < (unsigned)n)
-                av[j] = x1[(long long)venc(xx, yy, zz, a.lx) * a.c1];
-            bv[j] = wp[tp * a.cin];
-        }
-    }
-    const f32x4 acc = mfma(av, bv, f32x4{0.f, 0.f, 0.f, 0.f});
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const long long vo = b * nvox + vbase + 4 * g + j;
-        float val = acc[j];
-        if (a.relu) val = fmaxf(val, 0.0f);
-        a.y1[vo * a.cout + m] = (bf16)val;
     }
 }
 
@@ -507,10 +514,10 @@ __global__ void k_upsample_back(const bf16 *__restrict__ gf, bf16 *__restrict__ 
 }
 
 // Per-sample statistics are two-pass: a partials kernel on grid (kSG,
-// batch) writes one row per workgroup, and the elementwise kernel that
-// needs the statistic folds its sample's kSG rows in index order
-// (deterministic).  Blocks are 256 threads.
-constexpr int kSG = 64;
+// batch) writes one row per workgroup, and every block of the kernel that
+// needs the statistic folds its sample's kSG rows with the same fixed tree
+// (deterministic, one row per thread).  Blocks are 256 threads.
+constexpr int kSG = 256;
 
 // Sum over a 256-thread workgroup of two values; valid in thread 0.
 __device__ inline void wg_sum2(float &a, float &b) {
@@ -528,12 +535,92 @@ __device__ inline void wg_sum2(float &a, float &b) {
     __syncthreads();
     a = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
     b = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+    __syncthreads();  // sh may be reused by the caller's next wg_sum2
 }
+// The sample's kSG rows of two sums, folded by the whole workgroup (every
+// thread must call it): thread t loads row t, then wg_sum2's fixed tree.
 __device__ inline void fold2(const float *part, int b, float &a, float &c) {
-    a = c = 0.0f;
-    for (int i = 0; i < kSG; ++i) {
-        a += part[(b * kSG + i) * 2];
-        c += part[(b * kSG + i) * 2 + 1];
+    a = part[(b * kSG + threadIdx.x) * 2];
+    c = part[(b * kSG + threadIdx.x) * 2 + 1];
+    wg_sum2(a, c);
+}
+
+// The first layer fused with the input normalisation: x0 = bf16(in / s_b)
+// with s_b the rms of the sample (1 when zero; k_pack's rule, from the
+// k_sample_partials<0> rows), then the 1 -> 16 convolution of that one
+// channel (27 taps, one 32-wide MFMA step) and ReLU.  A block owns a
+// 16 x 4 x 4 voxel tile and stages the normalised input with a one-voxel halo
+// in LDS (zero outside the domain); wave w takes plane z0 + w, one MFMA per
+// 16-voxel row.  The MFMA takes the weights as A (rows: output channels; the
+// layer's weight of tap t sits at k = t * cin of its row) and the taps as B
+// (columns: voxels), so a lane's D holds four consecutive channels of one
+// voxel and stores 8 contiguous bytes.
+__global__ void __launch_bounds__(256) k_in1(ConvArgs a, const float *__restrict__ in,
+                                             const float *__restrict__ part, float *__restrict__ scale) {
+    constexpr int HX = 18, HY = 6, HZ = 6, NU = HX * HY * HZ;
+    __shared__ bf16 tile[NU];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    const int b = blockIdx.y;
+    const int n = 1 << a.lx;
+    const long long nvox = 1ll << (3 * a.lx);
+    const int tx = n >> 4, ty = n >> 2;
+    const int t = blockIdx.x;
+    const int x0 = (t % tx) * 16, y0 = ((t / tx) % ty) * 4, z0 = (t / (tx * ty)) * 4;
+    // the raw input values of this thread's halo slots and its weights are
+    // loaded first; the statistics fold overlaps them
+    const float *inb = in + b * nvox;
+    constexpr int IT = (NU + 255) / 256;
+    float raw[IT];
+    bool inside[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int u = threadIdx.x + 256 * i;
+        const int hx = u % HX, hy = (u / HX) % HY, hz = u / (HX * HY);
+        const int xx = x0 - 1 + hx, yy = y0 - 1 + hy, zz = z0 - 1 + hz;
+        inside[i] = u < NU && (unsigned)xx < (unsigned)n && (unsigned)yy < (unsigned)n && (unsigned)zz < (unsigned)n;
+        raw[i] = inside[i] ? inb[venc(xx, yy, zz, a.lx)] : 0.0f;
+    }
+    bf16x8 wv = zero8();
+    const bf16 *wp = a.w + (long long)m * a.kpad;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int tp = 8 * g + j;
+        if (tp < 27) wv[j] = wp[tp * a.cin];
+    }
+    float ss, unused;
+    fold2(part, b, ss, unused);
+    const float s = ss > 0.0f ? sqrtf(ss / (float)nvox) : 1.0f;
+    if (blockIdx.x == 0 && threadIdx.x == 0) scale[b] = s;
+    const float inv = 1.0f / s;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int u = threadIdx.x + 256 * i;
+        if (u < NU) tile[u] = inside[i] ? (bf16)(raw[i] * inv) : (bf16)0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        bf16x8 xv = zero8();
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int tp = 8 * g + j;
+            if (tp < 27) {
+                const int dz = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dx = tp % 3 - 1;
+                xv[j] = tile[((wave + 1 + dz) * HY + (r + 1 + dy)) * HX + (m + 1 + dx)];
+            }
+        }
+        const f32x4 acc = mfma(wv, xv, f32x4{0.f, 0.f, 0.f, 0.f});
+        typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        bf16x4 o;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float val = acc[j];
+            if (a.relu) val = fmaxf(val, 0.0f);
+            o[j] = (bf16)val;
+        }
+        const long long vo = b * nvox + venc(x0 + m, y0 + r, z0 + wave, a.lx);
+        *reinterpret_cast<bf16x4 *>(a.y1 + vo * a.cout + 4 * g) = o;
     }
 }
 
@@ -547,6 +634,7 @@ __global__ void __launch_bounds__(256) k_sample_partials(const float *__restrict
     const int b = blockIdx.y;
     float a = 0.0f, c = 0.0f;
     const float ts = (MODE == 1 && x) ? d / scale[b] : 0.0f;
+#pragma unroll 4
     for (int v = blockIdx.x * 256 + threadIdx.x; v < nvox; v += kSG * 256) {
         const long long i = (long long)b * nvox + v;
         if (MODE == 0) {
@@ -563,28 +651,24 @@ __global__ void __launch_bounds__(256) k_sample_partials(const float *__restrict
     }
 }
 
-// Input packing: s_b = rms of the sample (1 when zero), x0 = in / s_b in
-// channel 0, channels 1..15 zero.
+// Input packing for training: s_b = rms of the sample (1 when zero), x0 =
+// in / s_b in channel 0 and channels 1..15 zero -- the 16-channel input the
+// first layer's weight gradient reads (the forward pass normalises the input
+// itself, k_in1).
 __global__ void __launch_bounds__(256) k_pack(const float *__restrict__ in, int nvox,
-                                              const float *__restrict__ part, bf16 *__restrict__ x0,
-                                              bf16 *__restrict__ x0c, int full,
-                                              float *__restrict__ scale) {
+                                              const float *__restrict__ part, bf16 *__restrict__ x0) {
     const int b = blockIdx.y;
     float ss, unused;
     fold2(part, b, ss, unused);
     const float s = ss > 0.0f ? sqrtf(ss / (float)nvox) : 1.0f;
-    if (blockIdx.x == 0 && threadIdx.x == 0) scale[b] = s;
     const float inv = 1.0f / s;
     for (int v = blockIdx.x * 256 + threadIdx.x; v < nvox; v += gridDim.x * 256) {
         const bf16 val = (bf16)(in[(long long)b * nvox + v] * inv);
-        x0c[(long long)b * nvox + v] = val;
-        if (full) {  // the 16-channel copy the first layer's weight gradient reads
-            bf16x8 lo = zero8();
-            lo[0] = val;
-            bf16 *dst = x0 + ((long long)b * nvox + v) * 16;
-            st8(dst, lo);
-            st8(dst + 8, zero8());
-        }
+        bf16x8 lo = zero8();
+        lo[0] = val;
+        bf16 *dst = x0 + ((long long)b * nvox + v) * 16;
+        st8(dst, lo);
+        st8(dst + 8, zero8());
     }
 }
 
@@ -618,26 +702,35 @@ __global__ void __launch_bounds__(256) k_loss(const bf16 *__restrict__ out, cons
     }
 }
 
-// loss_b = mean(e^2), one thread per sample.
-__global__ void k_loss_fold(const float *__restrict__ part2, int nvox, int batch, float *__restrict__ loss) {
-    const int b = threadIdx.x;
-    if (b >= batch) return;
+// loss_b = mean(e^2), one workgroup per sample.
+__global__ void __launch_bounds__(256) k_loss_fold(const float *__restrict__ part2, int nvox,
+                                                   float *__restrict__ loss) {
+    const int b = blockIdx.x;
     float se, unused;
     fold2(part2, b, se, unused);
-    loss[b] = se / (float)nvox;
+    if (threadIdx.x == 0) loss[b] = se / (float)nvox;
 }
 
-// Output unpacking: p = (o - mean o) s_b / d.
-__global__ void __launch_bounds__(256) k_unpack(const bf16 *__restrict__ out,
-                                                const float *__restrict__ scale, float d, int nvox,
-                                                const float *__restrict__ part, float *__restrict__ p) {
+// Output unpacking from the inference head's compact channel 0: mean o from
+// the head's `rows` workgroup sums of the sample (every block folds them in
+// the same fixed order), then p = (o - mean o) s_b / d.
+__global__ void __launch_bounds__(256) k_unpack_c(const float *__restrict__ o, const float *__restrict__ hsum,
+                                                  int rows, const float *__restrict__ scale, float d, int nvox,
+                                                  float *__restrict__ p) {
+    __shared__ float sh[4];
     const int b = blockIdx.y;
-    float so, unused;
-    fold2(part, b, so, unused);
-    const float mo = so / (float)nvox, f = scale[b] / d;
-    for (int v = blockIdx.x * 256 + threadIdx.x; v < nvox; v += gridDim.x * 256) {
+    float a = 0.0f;
+    for (int i = threadIdx.x; i < rows; i += 256) a += hsum[(long long)b * rows + i];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+    __syncthreads();
+    const float mo = ((sh[0] + sh[1]) + (sh[2] + sh[3])) / (float)nvox, f = scale[b] / d;
+    // four voxels per thread (nvox is a multiple of 1024)
+    for (int v = 4 * (blockIdx.x * 256 + threadIdx.x); v < nvox; v += 4 * gridDim.x * 256) {
         const long long i = (long long)b * nvox + v;
-        p[i] = ((float)out[i * 16] - mo) * f;
+        const float4 ov = *reinterpret_cast<const float4 *>(o + i);
+        *reinterpret_cast<float4 *>(p + i) = make_float4((ov.x - mo) * f, (ov.y - mo) * f, (ov.z - mo) * f,
+                                                         (ov.w - mo) * f);
     }
 }
 
@@ -730,7 +823,9 @@ struct fl_unet {
     size_t P = 0, Pt = 0;  // parameter counts (padded), transposed copies
     bf16 *act[kActs] = {}, *grad[kActs] = {};
     bf16 *tmp = nullptr;           // upsample part of a concat layer's data gradient
-    bf16 *x0c = nullptr;           // the normalised input, one channel
+    float *hout = nullptr;         // inference head: output channel 0 [batch][voxel]
+    float *hsum = nullptr;         // and its workgroup sums [batch][head_rows]
+    int head_rows = 0;
     float *w = nullptr, *m1 = nullptr, *m2 = nullptr, *gr = nullptr;
     bf16 *wb = nullptr, *wt = nullptr;
     float *part = nullptr, *part2 = nullptr;  // per-sample statistics rows
@@ -757,7 +852,7 @@ void ufree(fl_unet *u) {
         if (u->act[i]) (void)hipFree(u->act[i]);
         if (u->grad[i]) (void)hipFree(u->grad[i]);
     }
-    void *bufs[] = {u->tmp, u->x0c, u->w, u->m1, u->m2, u->gr, u->wb, u->wt, u->partial,
+    void *bufs[] = {u->tmp, u->hout, u->hsum, u->w, u->m1, u->m2, u->gr, u->wb, u->wt, u->partial,
                     u->in,  u->tgt, u->out, u->scale, u->loss, u->part, u->part2, u->kpart};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -772,19 +867,20 @@ long long nvox_at(const fl_unet *u, int level) { return 1ll << (3 * (u->lx - lev
 // data's typical rms(p) diag / rms(b) so that targets are O(1).
 float tscale(const fl_unet *u, double diag) { return (float)(diag / (double)u->gain); }
 
-// x0 <- normalised input (and scale[b] <- rms)
+// The input's per-sample statistics (the first layer normalises with them);
+// for training (full) also the 16-channel normalised input X0.
 void launch_pack(fl_unet *u, hipStream_t stm, const float *in, int B, bool full = true) {
     const int nv = (int)nvox_at(u, 0);
     const dim3 grid(kSG, (unsigned)B);
     k_sample_partials<0><<<grid, 256, 0, stm>>>(in, nullptr, nullptr, 0.0f, nv, u->part);
-    k_pack<<<grid, 256, 0, stm>>>(in, nv, u->part, u->act[X0], u->x0c, full ? 1 : 0, u->scale);
+    if (full) k_pack<<<grid, 256, 0, stm>>>(in, nv, u->part, u->act[X0]);
 }
-// out <- denormalised zero-mean prediction
+// out <- denormalised zero-mean prediction, from the inference head's
+// compact output and workgroup sums
 void launch_unpack(fl_unet *u, hipStream_t stm, double diag, float *out, int B) {
     const int nv = (int)nvox_at(u, 0);
-    const dim3 grid(kSG, (unsigned)B);
-    k_sample_partials<1><<<grid, 256, 0, stm>>>(nullptr, u->act[OUT], u->scale, tscale(u, diag), nv, u->part);
-    k_unpack<<<grid, 256, 0, stm>>>(u->act[OUT], u->scale, tscale(u, diag), nv, u->part, out);
+    const dim3 grid((unsigned)(nv / 1024), (unsigned)B);
+    k_unpack_c<<<grid, 256, 0, stm>>>(u->hout, u->hsum, u->head_rows, u->scale, tscale(u, diag), nv, out);
 }
 // grad[OUT] <- loss gradient, loss[b] <- per-sample loss (targets in u->tgt)
 void launch_loss(fl_unet *u, hipStream_t stm, double diag, int B) {
@@ -793,7 +889,7 @@ void launch_loss(fl_unet *u, hipStream_t stm, double diag, int B) {
     k_sample_partials<1><<<grid, 256, 0, stm>>>(u->tgt, u->act[OUT], u->scale, tscale(u, diag), nv, u->part);
     k_loss<<<grid, 256, 0, stm>>>(u->act[OUT], u->tgt, u->scale, tscale(u, diag), nv, B, u->part,
                                   u->grad[OUT], u->part2);
-    k_loss_fold<<<1, 64, 0, stm>>>(u->part2, nv, B, u->loss);
+    k_loss_fold<<<(unsigned)B, 256, 0, stm>>>(u->part2, nv, u->loss);
 }
 
 // Work split of a convolution with `cout` outputs over B * nvox voxels and
@@ -820,8 +916,17 @@ ConvSplit conv_split(long long nvox, int B, int cout, int kpad) {
     return sp;
 }
 
-fl_status launch_conv(fl_unet *u, hipStream_t stm, int B, ConvArgs a) {
+fl_status launch_conv(fl_unet *u, hipStream_t stm, int B, ConvArgs a, bool head = false) {
     const long long nvox = 1ll << (3 * a.lx);
+    if (head) {  // the inference head: compact channel 0 (cout 16, 32^3 and finer)
+        const int n = 1 << a.lx, R = a.lx >= 6 ? 4 : 1;
+        const unsigned tiles = (unsigned)((n / 16) * (n / R) * (n / 4));
+        const dim3 grid(tiles, (unsigned)B);
+        if (R == 4) k_conv_lds<4, 1, true><<<grid, 256, 0, stm>>>(a);
+        else k_conv_lds<1, 1, true><<<grid, 256, 0, stm>>>(a);
+        FL_HIP_TRY(u->ctx, hipGetLastError());
+        return FL_OK;
+    }
     // forward convolutions of the 64^3 to 16^3 levels: the LDS-staged kernel
     if (!a.mask && a.lx >= 4 && a.cin % 16 == 0 && a.c1 % 16 == 0 && a.cout % 16 == 0 &&
         a.ysplit == a.cout) {
@@ -870,7 +975,11 @@ unsigned ew_grid(long long items) {
     return (unsigned)std::max<long long>(1, std::min<long long>((items + 255) / 256, 8192));
 }
 
-fl_status forward(fl_unet *u, hipStream_t stm, int B) {
+// The forward pass of B samples of the FP32 input `in` (their statistics
+// already in u->part).  infer: the head writes only channel 0, compactly
+// (u->hout, u->hsum); otherwise the full 16-channel head output act[OUT]
+// that training and fl_unet_forward_raw read.
+fl_status forward(fl_unet *u, hipStream_t stm, int B, const float *in, bool infer) {
     for (int op : kSchedule) {
         if (op < 0) {
             const Pool &p = kPools[-1 - op];
@@ -897,15 +1006,18 @@ fl_status forward(fl_unet *u, hipStream_t stm, int B) {
         a.cout = L.cout;
         a.ysplit = L.cout;
         a.relu = ls.relu;
-        if (op == 0) {  // the one-channel input: 27 taps, not 27 x 16
+        if (op == 0) {  // the one-channel input: 27 taps, not 27 x 16; normalised on load
             a.batch = B;
-            a.x1 = u->x0c;  // the compact one-channel copy
-            a.c1 = 1;
-            k_conv_in1<<<dim3((unsigned)(nvox_at(u, 0) / 64), (unsigned)B), 256, 0, stm>>>(a);
+            k_in1<<<dim3((unsigned)(nvox_at(u, 0) / 256), (unsigned)B), 256, 0, stm>>>(a, in, u->part, u->scale);
             FL_HIP_TRY(u->ctx, hipGetLastError());
             continue;
         }
-        fl_status st = launch_conv(u, stm, B, a);
+        const bool head = infer && op == kLayers - 1;
+        if (head) {
+            a.hout = u->hout;
+            a.hsum = u->hsum;
+        }
+        fl_status st = launch_conv(u, stm, B, a, head);
         if (st != FL_OK) return st;
     }
     return FL_OK;
@@ -1044,7 +1156,7 @@ fl_ctx *unet_context(const fl_unet *net) { return net ? net->ctx : nullptr; }
 fl_status unet_infer_device(fl_unet *u, hipStream_t stm, const float *in, double diag, float *out) {
     launch_pack(u, stm, in, 1, false);
     FL_HIP_TRY(u->ctx, hipGetLastError());
-    fl_status st = forward(u, stm, 1);
+    fl_status st = forward(u, stm, 1, in, true);
     if (st != FL_OK) return st;
     launch_unpack(u, stm, diag, out, 1);
     FL_HIP_TRY(u->ctx, hipGetLastError());
@@ -1111,6 +1223,10 @@ fl_status fl_unet_create(fl_ctx *ctx, const fl_unet_config *cfg, fl_unet **out) 
             st = ualloc(u, (void **)&u->grad[i], bytes);
     }
     const size_t nv0 = (size_t)nvox_at(u, 0);
+    {  // workgroups of the inference head (k_conv_lds tiles: 16 x R x 4 voxels)
+        const int R = lx >= 6 ? 4 : 1;
+        u->head_rows = (cfg->n / 16) * (cfg->n / R) * (cfg->n / 4);
+    }
     if (st == FL_OK &&
         ((st = ualloc(u, (void **)&u->tmp, B * max_tmp * sizeof(bf16))) ||
          (st = ualloc(u, (void **)&u->w, P * 4)) || (st = ualloc(u, (void **)&u->m1, P * 4)) ||
@@ -1118,7 +1234,8 @@ fl_status fl_unet_create(fl_ctx *ctx, const fl_unet_config *cfg, fl_unet **out) 
          (st = ualloc(u, (void **)&u->wb, P * 2)) || (st = ualloc(u, (void **)&u->wt, Pt * 2)) ||
          (st = ualloc(u, (void **)&u->partial, (size_t)u->wgroups * max_partial * 4)) ||
          (st = ualloc(u, (void **)&u->in, B * nv0 * 4)) ||
-         (st = ualloc(u, (void **)&u->x0c, B * nv0 * 2)) ||
+         (st = ualloc(u, (void **)&u->hout, B * nv0 * 4)) ||
+         (st = ualloc(u, (void **)&u->hsum, B * (size_t)u->head_rows * 4)) ||
          (st = ualloc(u, (void **)&u->tgt, B * nv0 * 4)) ||
          (st = ualloc(u, (void **)&u->out, B * nv0 * 4)) ||
          (st = ualloc(u, (void **)&u->scale, B * 4)) || (st = ualloc(u, (void **)&u->loss, B * 4)) ||
@@ -1209,7 +1326,7 @@ fl_status fl_unet_train_step(fl_unet *u, int32_t batch, const float *rhs, const 
     FL_HIP_TRY(ctx, hipMemcpyAsync(u->tgt, p, batch * nv * sizeof(float), hipMemcpyHostToDevice, stm));
     launch_pack(u, stm, u->in, batch);
     FL_HIP_TRY(ctx, hipGetLastError());
-    fl_status st = forward(u, stm, batch);
+    fl_status st = forward(u, stm, batch, u->in, false);
     if (st != FL_OK) return st;
     launch_loss(u, stm, diag, batch);
     FL_HIP_TRY(ctx, hipGetLastError());
@@ -1243,7 +1360,7 @@ fl_status fl_unet_infer(fl_unet *u, int32_t batch, const float *rhs, double diag
     FL_HIP_TRY(ctx, hipMemcpyAsync(u->in, rhs, batch * nv * sizeof(float), hipMemcpyHostToDevice, stm));
     launch_pack(u, stm, u->in, batch, false);
     FL_HIP_TRY(ctx, hipGetLastError());
-    fl_status st = forward(u, stm, batch);
+    fl_status st = forward(u, stm, batch, u->in, true);
     if (st != FL_OK) return st;
     launch_unpack(u, stm, diag, u->out, batch);
     FL_HIP_TRY(ctx, hipGetLastError());
@@ -1262,7 +1379,7 @@ fl_status fl_unet_forward_raw(fl_unet *u, int32_t batch, const float *rhs, float
     const long long nv = nvox_at(u, 0);
     FL_HIP_TRY(ctx, hipMemcpyAsync(u->in, rhs, batch * nv * sizeof(float), hipMemcpyHostToDevice, stm));
     launch_pack(u, stm, u->in, batch);
-    fl_status st = forward(u, stm, batch);
+    fl_status st = forward(u, stm, batch, u->in, false);
     if (st != FL_OK) return st;
     std::vector<uint16_t> raw((size_t)batch * nv * 16);
     FL_HIP_TRY(ctx, hipMemcpyAsync(raw.data(), u->act[OUT], raw.size() * 2, hipMemcpyDeviceToHost, stm));
@@ -1287,7 +1404,7 @@ fl_status fl_unet_grads(fl_unet *u, int32_t batch, const float *rhs, const float
     FL_HIP_TRY(ctx, hipMemcpyAsync(u->in, rhs, batch * nv * sizeof(float), hipMemcpyHostToDevice, stm));
     FL_HIP_TRY(ctx, hipMemcpyAsync(u->tgt, p, batch * nv * sizeof(float), hipMemcpyHostToDevice, stm));
     launch_pack(u, stm, u->in, batch);
-    fl_status st = forward(u, stm, batch);
+    fl_status st = forward(u, stm, batch, u->in, false);
     if (st != FL_OK) return st;
     launch_loss(u, stm, diag, batch);
     if ((st = backward(u, stm, batch)) != FL_OK) return st;
