@@ -117,62 +117,10 @@ __device__ inline void block_reduce(double (&s)[NS], double (&m)[NM]) {
     }
 }
 
-// One-pass grid reduction: every block publishes its partials, the last block
-// to arrive (device-scope counter) folds them in block order and returns true
-// with the totals in thread 0.  `partials` holds gridDim.x * (NS + NM)
-// doubles; `counter` must be zero on entry and is reset by the last block.
-// NS and NM must both be at least 1 (pass a dummy where one is unused).
-template <int NS, int NM>
-__device__ inline bool grid_reduce(double (&s)[NS], double (&m)[NM], double *partials,
-                                   unsigned *counter) {
-    static_assert(NS >= 1 && NM >= 1, "grid_reduce needs at least one sum and one max");
-    constexpr int K = NS + NM;
-    __shared__ bool is_last;
-    block_reduce<NS, NM>(s, m);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int a = 0; a < NS; ++a) partials[blockIdx.x * K + a] = s[a];
-#pragma unroll
-        for (int a = 0; a < NM; ++a) partials[blockIdx.x * K + NS + a] = m[a];
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        is_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    // Fold the partials: each thread sums a strided subset in index order,
-    // then the block tree combines the threads in a fixed order.
-    double ts[NS], tm[NM];
-#pragma unroll
-    for (int a = 0; a < NS; ++a) ts[a] = 0.0;
-#pragma unroll
-    for (int a = 0; a < NM; ++a) tm[a] = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-        const volatile double *pb = partials + (size_t)b * K;
-#pragma unroll
-        for (int a = 0; a < NS; ++a) ts[a] += pb[a];
-#pragma unroll
-        for (int a = 0; a < NM; ++a) tm[a] = fmax(tm[a], pb[NS + a]);
-    }
-    __syncthreads();
-    block_reduce<NS, NM>(ts, tm);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int a = 0; a < NS; ++a) s[a] = ts[a];
-#pragma unroll
-        for (int a = 0; a < NM; ++a) m[a] = tm[a];
-        *counter = 0u;
-    }
-    return true;
-}
-
-// Fence-free reductions.  A producer kernel publishes one row of partials
-// per block (plain stores by thread 0); the consumer kernel -- which starts
-// after the producer has finished, so the stores are visible -- folds all
-// rows in block order in every block.  Every block therefore derives the same
-// scalars (alpha, beta, the stop test) without atomics, device-scope fences
-// or L2 write-backs, and the sums are bitwise repeatable.
+// Reductions across a grid without fences or atomics: each block publishes
+// its partial row (publish_partials: `partials` holds gridDim.x * (NS + NM)
+// doubles of a 1D grid), and every block of the next kernel folds all rows
+// in block order (fold_partials), so every block derives the same totals.
 template <int NS, int NM>
 __device__ inline void publish_partials(double (&s)[NS], double (&m)[NM], double *partials) {
     block_reduce<NS, NM>(s, m);

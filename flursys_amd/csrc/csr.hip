@@ -11,16 +11,27 @@
 // the vector kernels one element per lane.
 //
 // Per CG iteration (3 kernels, scalars never leave the device):
-//   k_cg_spmv    q = A d, d.q                    -> alpha
-//   k_cg_update  x += a d, r -= a q, r.r [, z = r/diag, r.z] -> stop test, beta
-//   k_cg_dir     d = z + beta d
+//   k_cg_spmv    q = A d, d.q
+//   k_cg_update  alpha; x += a d, r -= a q, r.r [, z = r/diag], r.z
+//   k_cg_dir     stop test, beta; d = z + beta d
 // Per BiCGSTAB iteration (4 kernels): s = r - alpha v is never stored; the
 // second SpMV gathers r and v and forms s on the fly, and the final update
 // recomputes it:
-//   k_bi_p       p = r + beta (p - omega v)
-//   k_bi_spmv_v  v = A p, rhat.v                  -> alpha
-//   k_bi_spmv_t  t = A s, s.s, t.t, t.s          -> s-test, omega
-//   k_bi_update  x += a p + w s, r = s - w t, r.r, rhat.r -> stop test, rho, beta
+//   k_bi_p       stop test of the previous iteration, rho, beta;
+//                p = r + beta (p - omega v)
+//   k_bi_spmv_v  v = A p, rhat.v
+//   k_bi_spmv_t  alpha; t = A s, s.s, t.t, t.s
+//   k_bi_update  s-test, omega; x += a p + w s, r = s - w t, r.r, rhat.r
+//
+// Reductions without fences: a kernel publishes one partial row per block
+// and the next kernel folds all rows in every block in a fixed order, so
+// every block derives the same scalars and verdicts; block 0 records them in
+// the state for the host and later kernels.  Scalars a kernel reads while
+// its block 0 writes the next value (rho, and BiCGSTAB's alpha and omega)
+// are kept per iteration parity.  Every launch carries a sequence number:
+// the launch that ends the solve stores its own in done_seq, and launches
+// with a larger number return at once, while the deciding launch's blocks
+// all finish their share (the BiCGSTAB early exit updates x there).
 
 #include <algorithm>
 #include <cstring>
@@ -83,15 +94,20 @@ struct Mat {
 };
 
 struct LinState {
-    double rho, rho_prev, alpha, beta, omega;
+    double rho[2];              // r.z (CG) or rhat.r (BiCGSTAB) by iteration parity
+    double alpha[2], omega[2];  // BiCGSTAB, by iteration parity
+    double atol, rtol;          // the options (threshold = atol + rtol ||r0||)
     double threshold, initial, rnorm;
     long long iter, max_iter;
-    int done, converged, s_conv, status, ctx;
+    long long done_seq;         // sequence number of the launch that ended the solve
+    int converged, status, ctx, pad;
     long long err_index;
-    unsigned long long bad_row;  // first non-finite row of the pending SpMV
-    unsigned long long bad_z;    // first non-finite preconditioned entry
-    unsigned counters[8];
+    unsigned long long bad_row;   // first non-finite row of the (first) SpMV of an iteration
+    unsigned long long bad_row2;  // of BiCGSTAB's second SpMV
+    unsigned long long bad_r;     // first non-finite initial residual entry
+    unsigned long long bad_z;     // first non-finite preconditioned entry
 };
+constexpr long long kRunning = 0x7fffffffffffffffll;  // done_seq of a solve still running
 
 // SpMV rows are worked by groups of kRowLanes adjacent lanes: per chunk of
 // four entries lane q loads entry q (the ~4 entries of a Poisson row are one
@@ -126,15 +142,24 @@ __device__ inline double row_dot_lanes(const Mat &A, const RowSlot &r, F &&xval)
     return acc;
 }
 
-__device__ inline void fail(LinState *st, int status, int ctx, long long index) {
+__device__ inline bool lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
+// The solve ends in this launch with an error (every block returns; block 0
+// records it).
+__device__ inline void fail(LinState *st, long long seq, int status, int ctx, long long index) {
+    if (!lead()) return;
     st->status = status;
     st->ctx = ctx;
     st->err_index = index;
-    st->done = 1;
+    st->done_seq = seq;
 }
-
-__device__ inline unsigned long long atomic_read(unsigned long long *p) {
-    return atomicAdd(p, 0ull);
+// The solve ends in this launch without an error (converged or
+// MaxIterations, recorded by block 0).
+__device__ inline void finish(LinState *st, long long seq, int converged, long long iter) {
+    if (!lead()) return;
+    st->converged = converged;
+    st->iter = iter;
+    st->done_seq = seq;
 }
 
 // ------------------------------------------------------------------ SpMV
@@ -151,10 +176,10 @@ __global__ void __launch_bounds__(kBlock) k_spmv(Mat A, const double *__restrict
 }
 
 // --------------------------------------------------------- CG / PCG init
-// r = b - q (q = A x from k_spmv), ||r||; stop test at iteration 0.
+// r = b - q (q = A x from k_spmv), partial r.r.
 __global__ void __launch_bounds__(kBlock) k_cg_init_r(uint32_t n, const double *__restrict__ b,
                                                       const double *__restrict__ q,
-                                                      double *__restrict__ r, double *partials,
+                                                      double *__restrict__ r, double *__restrict__ pout,
                                                       LinState *st) {
     double s[1] = {0.0}, m[1] = {0.0};
     TileSched ts((n + kBlock - 1) / kBlock);
@@ -162,35 +187,41 @@ __global__ void __launch_bounds__(kBlock) k_cg_init_r(uint32_t n, const double *
         const uint32_t i = t * kBlock + threadIdx.x;
         if (i >= n) continue;
         const double v = b[i] - q[i];
-        if (!isfinite(v)) atomicMin(&st->bad_z, (unsigned long long)i);
+        if (!isfinite(v)) atomicMin(&st->bad_r, (unsigned long long)i);
         r[i] = v;
         s[0] += v * v;
     }
-    if (grid_reduce<1, 1>(s, m, partials, &st->counters[0]) && threadIdx.x == 0) {
-        const unsigned long long bad_row = atomic_read(&st->bad_row);
-        if (bad_row != ~0ull) return fail(st, FL_ERR_NON_FINITE, C_SPMV_OUTPUT, bad_row);
-        const unsigned long long bad = atomic_read(&st->bad_z);
-        if (bad != ~0ull) return fail(st, FL_ERR_NON_FINITE, C_RESIDUAL, bad);
-        if (!isfinite(s[0])) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
-        const double init = sqrt(s[0]);
-        st->initial = init;
-        st->rnorm = init;
-        st->threshold = st->threshold + st->alpha * init;  // atol + rtol * ||r0|| (packed on entry)
-        st->alpha = 0.0;
-        st->iter = 0;
-        if (init <= st->threshold) {
-            st->converged = 1;
-            st->done = 1;
-        }
-    }
+    publish_partials<1, 1>(s, m, pout);
 }
 
-// z = D^-1 r (or r), d = z, rho = r.z.
+// The checks of residual_into and norm_l2 on the initial residual and the
+// stop test at iteration 0 (solver.rs:82-88), from k_cg_init_r's rows.
+// Returns false when the solve ends here.
+__device__ inline bool initial_tests(const double *__restrict__ pin, uint32_t rows, LinState *st,
+                                     long long seq) {
+    double f[1], fm[1];
+    fold_partials<1, 1>(pin, rows, f, fm);
+    if (st->bad_row != ~0ull) return fail(st, seq, FL_ERR_NON_FINITE, C_SPMV_OUTPUT, st->bad_row), false;
+    if (st->bad_r != ~0ull) return fail(st, seq, FL_ERR_NON_FINITE, C_RESIDUAL, st->bad_r), false;
+    if (!isfinite(f[0])) return fail(st, seq, FL_ERR_NON_FINITE, C_DOT, 0), false;
+    const double init = sqrt(f[0]);
+    const double threshold = st->atol + st->rtol * init;
+    if (lead()) {
+        st->initial = init;
+        st->rnorm = init;
+        st->threshold = threshold;
+    }
+    if (init <= threshold) return finish(st, seq, 1, 0), false;
+    return true;
+}
+
+// Initial tests; z = D^-1 r (or r), d = z, partial r.z.
 __global__ void __launch_bounds__(kBlock) k_cg_init_d(uint32_t n, const double *__restrict__ r,
                                                       const double *__restrict__ inv,
-                                                      double *__restrict__ d, double *partials,
-                                                      LinState *st) {
-    if (st->done) return;
+                                                      double *__restrict__ d, const double *__restrict__ pin,
+                                                      uint32_t rows, double *__restrict__ pout, LinState *st,
+                                                      long long seq) {
+    if (!initial_tests(pin, rows, st, seq)) return;
     double s[1] = {0.0}, m[1] = {0.0};
     TileSched ts((n + kBlock - 1) / kBlock);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
@@ -201,21 +232,26 @@ __global__ void __launch_bounds__(kBlock) k_cg_init_d(uint32_t n, const double *
         d[i] = z;
         s[0] += r[i] * z;
     }
-    if (grid_reduce<1, 1>(s, m, partials, &st->counters[1]) && threadIdx.x == 0) {
-        const unsigned long long bad = atomic_read(&st->bad_z);
-        if (bad != ~0ull) return fail(st, FL_ERR_NON_FINITE, C_PRECOND, bad);
-        if (!isfinite(s[0])) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
-        st->rho = s[0];
-        st->iter = 1;
-        if (st->max_iter < 1) st->done = 1;  // the loop 1..=0 never runs
-    }
+    publish_partials<1, 1>(s, m, pout);
+}
+
+// rho = r.z for iteration 1 (one workgroup).
+__global__ void __launch_bounds__(kBlock) k_cg_init_rho(const double *__restrict__ pin, uint32_t rows,
+                                                        LinState *st, long long seq) {
+    if (st->done_seq < seq) return;
+    double f[1], fm[1];
+    fold_partials<1, 1>(pin, rows, f, fm);
+    if (st->bad_z != ~0ull) return fail(st, seq, FL_ERR_NON_FINITE, C_PRECOND, st->bad_z);
+    if (!isfinite(f[0])) return fail(st, seq, FL_ERR_NON_FINITE, C_DOT, 0);
+    if (lead()) st->rho[1] = f[0];
+    if (st->max_iter < 1) return finish(st, seq, 0, 0);  // the loop 1..=0 never runs
 }
 
 // ------------------------------------------------------------ CG loop
 __global__ void __launch_bounds__(kBlock) k_cg_spmv(Mat A, const double *__restrict__ d,
-                                                    double *__restrict__ q, double *partials,
-                                                    LinState *st) {
-    if (st->done) return;
+                                                    double *__restrict__ q, double *__restrict__ pout,
+                                                    LinState *st, long long seq) {
+    if (st->done_seq < seq) return;
     double s[1] = {0.0}, m[1] = {0.0};
     TileSched ts((A.n + kRowsPerTile - 1) / kRowsPerTile);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
@@ -226,26 +262,28 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv(Mat A, const double *__restr
         q[r.row] = v;
         s[0] += d[r.row] * v;
     }
-    if (grid_reduce<1, 1>(s, m, partials, &st->counters[2]) && threadIdx.x == 0) {
-        const unsigned long long bad = atomic_read(&st->bad_row);
-        if (bad != ~0ull) return fail(st, FL_ERR_NON_FINITE, C_SPMV_OUTPUT, bad);
-        const double den = s[0];
-        if (!isfinite(den)) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
-        if (fabs(den) <= kBreakdown) return fail(st, FL_ERR_BREAKDOWN, C_CG_DEN, 0);
-        const double alpha = st->rho / den;
-        if (!isfinite(alpha)) return fail(st, FL_ERR_BREAKDOWN, C_CG_STEP, 0);
-        st->alpha = alpha;
-    }
+    publish_partials<1, 1>(s, m, pout);
 }
 
+// alpha = rho / d.q (solver.rs:98-110); x += alpha d, r -= alpha q, partials
+// r.r and r.z.
 __global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t n, double *__restrict__ x,
                                                       double *__restrict__ r,
                                                       const double *__restrict__ d,
                                                       const double *__restrict__ q,
                                                       const double *__restrict__ inv,
-                                                      double *partials, LinState *st) {
-    if (st->done) return;
-    const double alpha = st->alpha;
+                                                      const double *__restrict__ pin, uint32_t rows,
+                                                      double *__restrict__ pout, LinState *st,
+                                                      long long it, long long seq) {
+    if (st->done_seq < seq) return;
+    double f[1], fm[1];
+    fold_partials<1, 1>(pin, rows, f, fm);
+    if (st->bad_row != ~0ull) return fail(st, seq, FL_ERR_NON_FINITE, C_SPMV_OUTPUT, st->bad_row);
+    const double den = f[0];
+    if (!isfinite(den)) return fail(st, seq, FL_ERR_NON_FINITE, C_DOT, 0);
+    if (fabs(den) <= kBreakdown) return fail(st, seq, FL_ERR_BREAKDOWN, C_CG_DEN, 0);
+    const double alpha = st->rho[it & 1] / den;
+    if (!isfinite(alpha)) return fail(st, seq, FL_ERR_BREAKDOWN, C_CG_STEP, 0);
     double s[2] = {0.0, 0.0}, m[1] = {0.0};
     TileSched ts((n + kBlock - 1) / kBlock);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
@@ -259,38 +297,33 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t n, double *__rest
         if (!isfinite(z)) atomicMin(&st->bad_z, (unsigned long long)i);
         s[1] += ri * z;
     }
-    if (grid_reduce<2, 1>(s, m, partials, &st->counters[3]) && threadIdx.x == 0) {
-        // norm_l2 (vector.rs:20-32) then the stop test (solver.rs:126-133)
-        if (!isfinite(s[0])) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
-        const double rn = sqrt(s[0]);
-        st->rnorm = rn;
-        if (rn <= st->threshold) {
-            st->converged = 1;
-            st->done = 1;
-            return;
-        }
-        const unsigned long long bad = atomic_read(&st->bad_z);
-        if (bad != ~0ull) return fail(st, FL_ERR_NON_FINITE, C_PRECOND, bad);
-        const double rho_new = s[1];
-        if (!isfinite(rho_new)) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
-        if (fabs(st->rho) <= kBreakdown) return fail(st, FL_ERR_BREAKDOWN, C_CG_REC_DEN, 0);
-        const double beta = rho_new / st->rho;
-        if (!isfinite(beta)) return fail(st, FL_ERR_BREAKDOWN, C_CG_REC, 0);
-        st->beta = beta;
-        st->rho = rho_new;
-        if (st->iter >= st->max_iter) {
-            st->done = 1;  // MaxIterations: a status, not an error
-            return;
-        }
-        st->iter += 1;
-    }
+    publish_partials<2, 1>(s, m, pout);
 }
 
+// The stop test (norm_l2, solver.rs:120-133), rho_new = r.z and beta
+// (:134-148), MaxIterations after the last iteration; d = z + beta d.
 __global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, const double *__restrict__ r,
                                                    const double *__restrict__ inv,
-                                                   double *__restrict__ d, const LinState *st) {
-    if (st->done) return;
-    const double beta = st->beta;
+                                                   double *__restrict__ d, const double *__restrict__ pin,
+                                                   uint32_t rows, LinState *st, long long it, long long seq) {
+    if (st->done_seq < seq) return;
+    double f[2], fm[1];
+    fold_partials<2, 1>(pin, rows, f, fm);
+    if (!isfinite(f[0])) return fail(st, seq, FL_ERR_NON_FINITE, C_DOT, 0);
+    const double rn = sqrt(f[0]);
+    if (lead()) st->rnorm = rn;
+    if (rn <= st->threshold) return finish(st, seq, 1, it);
+    if (st->bad_z != ~0ull) return fail(st, seq, FL_ERR_NON_FINITE, C_PRECOND, st->bad_z);
+    const double rho_new = f[1], rho = st->rho[it & 1];
+    if (!isfinite(rho_new)) return fail(st, seq, FL_ERR_NON_FINITE, C_DOT, 0);
+    if (fabs(rho) <= kBreakdown) return fail(st, seq, FL_ERR_BREAKDOWN, C_CG_REC_DEN, 0);
+    const double beta = rho_new / rho;
+    if (!isfinite(beta)) return fail(st, seq, FL_ERR_BREAKDOWN, C_CG_REC, 0);
+    if (lead()) {
+        st->rho[(it + 1) & 1] = rho_new;
+        st->iter = it;
+    }
+    if (it >= st->max_iter) return finish(st, seq, 0, it);  // MaxIterations: a status, not an error
     TileSched ts((n + kBlock - 1) / kBlock);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const uint32_t i = t * kBlock + threadIdx.x;
@@ -301,15 +334,22 @@ __global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, const double *__r
 }
 
 // ------------------------------------------------------------ BiCGSTAB
-// Initial r = b - A x as for CG (k_spmv + k_cg_init_r); then rhat = r,
-// p = v = 0 and the first rho = rhat.r = r.r.
+// Initial r = b - A x as for CG (k_spmv + k_cg_init_r); then the initial
+// tests, rhat = r, p = v = 0, the iteration-0 scalars rho = alpha = omega
+// = 1 (solver.rs:175-177) and partials {r.r, rhat.r}.
 __global__ void __launch_bounds__(kBlock) k_bi_init(uint32_t n, const double *__restrict__ r,
-                                                    double *__restrict__ rhat,
-                                                    double *__restrict__ p,
-                                                    double *__restrict__ v, double *partials,
-                                                    LinState *st) {
-    if (st->done) return;
-    double s[1] = {0.0}, m[1] = {0.0};
+                                                    double *__restrict__ rhat, double *__restrict__ p,
+                                                    double *__restrict__ v, const double *__restrict__ pin,
+                                                    uint32_t rows, double *__restrict__ pout, LinState *st,
+                                                    long long seq) {
+    if (!initial_tests(pin, rows, st, seq)) return;
+    if (lead()) {
+        st->rho[0] = 1.0;
+        st->alpha[0] = 1.0;
+        st->omega[0] = 1.0;
+    }
+    if (st->max_iter < 1) return finish(st, seq, 0, 0);
+    double s[2] = {0.0, 0.0}, m[1] = {0.0};
     TileSched ts((n + kBlock - 1) / kBlock);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const uint32_t i = t * kBlock + threadIdx.x;
@@ -319,32 +359,39 @@ __global__ void __launch_bounds__(kBlock) k_bi_init(uint32_t n, const double *__
         p[i] = 0.0;
         v[i] = 0.0;
         s[0] += ri * ri;
+        s[1] += ri * ri;
     }
-    if (grid_reduce<1, 1>(s, m, partials, &st->counters[1]) && threadIdx.x == 0) {
-        st->rho_prev = 1.0;
-        st->alpha = 1.0;
-        st->omega = 1.0;
-        st->iter = 1;
-        if (st->max_iter < 1) {
-            st->done = 1;
-            return;
-        }
-        const double rho = s[0];
-        if (!isfinite(rho)) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
-        if (fabs(rho) <= kBreakdown) return fail(st, FL_ERR_BREAKDOWN, C_BI_RHO, 0);
-        const double beta = (rho / st->rho_prev) * (st->alpha / st->omega);
-        if (!isfinite(beta)) return fail(st, FL_ERR_BREAKDOWN, C_BI_BETA, 0);
-        st->rho = rho;
-        st->beta = beta;
-    }
+    publish_partials<2, 1>(s, m, pout);
 }
 
+// The end of iteration it - 1 (its residual norm test, solver.rs:236-245,
+// and MaxIterations), then iteration it's rho = rhat.r and beta (:179-190);
+// p = r + beta (p - omega v).
 __global__ void __launch_bounds__(kBlock) k_bi_p(uint32_t n, const double *__restrict__ r,
-                                                 double *__restrict__ p,
-                                                 const double *__restrict__ v,
-                                                 const LinState *st) {
-    if (st->done) return;
-    const double beta = st->beta, omega = st->omega;
+                                                 double *__restrict__ p, const double *__restrict__ v,
+                                                 const double *__restrict__ pin, uint32_t rows, LinState *st,
+                                                 long long it, long long seq) {
+    if (st->done_seq < seq) return;
+    double f[2], fm[1];
+    fold_partials<2, 1>(pin, rows, f, fm);
+    if (it > 1) {
+        if (!isfinite(f[0])) return fail(st, seq, FL_ERR_NON_FINITE, C_DOT, 0);
+        const double rn = sqrt(f[0]);
+        if (lead()) st->rnorm = rn;
+        if (rn <= st->threshold) return finish(st, seq, 1, it - 1);
+        if (it - 1 >= st->max_iter) return finish(st, seq, 0, it - 1);
+    }
+    const int c = (int)(it & 1), pv = c ^ 1;
+    const double rho = f[1];
+    if (!isfinite(rho)) return fail(st, seq, FL_ERR_NON_FINITE, C_DOT, 0);
+    if (fabs(rho) <= kBreakdown) return fail(st, seq, FL_ERR_BREAKDOWN, C_BI_RHO, 0);
+    const double omega = st->omega[pv];
+    const double beta = (rho / st->rho[pv]) * (st->alpha[pv] / omega);
+    if (!isfinite(beta)) return fail(st, seq, FL_ERR_BREAKDOWN, C_BI_BETA, 0);
+    if (lead()) {
+        st->rho[c] = rho;
+        st->iter = it;
+    }
     TileSched ts((n + kBlock - 1) / kBlock);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const uint32_t i = t * kBlock + threadIdx.x;
@@ -355,9 +402,9 @@ __global__ void __launch_bounds__(kBlock) k_bi_p(uint32_t n, const double *__res
 
 __global__ void __launch_bounds__(kBlock) k_bi_spmv_v(Mat A, const double *__restrict__ p,
                                                       const double *__restrict__ rhat,
-                                                      double *__restrict__ v, double *partials,
-                                                      LinState *st) {
-    if (st->done) return;
+                                                      double *__restrict__ v, double *__restrict__ pout,
+                                                      LinState *st, long long seq) {
+    if (st->done_seq < seq) return;
     double s[1] = {0.0}, m[1] = {0.0};
     TileSched ts((A.n + kRowsPerTile - 1) / kRowsPerTile);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
@@ -368,25 +415,27 @@ __global__ void __launch_bounds__(kBlock) k_bi_spmv_v(Mat A, const double *__res
         v[r.row] = vv;
         s[0] += rhat[r.row] * vv;
     }
-    if (grid_reduce<1, 1>(s, m, partials, &st->counters[2]) && threadIdx.x == 0) {
-        const unsigned long long bad = atomic_read(&st->bad_row);
-        if (bad != ~0ull) return fail(st, FL_ERR_NON_FINITE, C_SPMV_OUTPUT, bad);
-        const double den = s[0];
-        if (!isfinite(den)) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
-        if (fabs(den) <= kBreakdown) return fail(st, FL_ERR_BREAKDOWN, C_BI_ALPHA_DEN, 0);
-        const double alpha = st->rho / den;
-        if (!isfinite(alpha)) return fail(st, FL_ERR_BREAKDOWN, C_BI_ALPHA, 0);
-        st->alpha = alpha;
-    }
+    publish_partials<1, 1>(s, m, pout);
 }
 
-// t = A s with s = r - alpha v formed at each gathered column.
+// alpha = rho / rhat.v (solver.rs:192-203); t = A s with s = r - alpha v
+// formed at each gathered column; partials s.s, t.t, t.s.
 __global__ void __launch_bounds__(kBlock) k_bi_spmv_t(Mat A, const double *__restrict__ r,
                                                       const double *__restrict__ v,
                                                       double *__restrict__ tvec,
-                                                      double *partials, LinState *st) {
-    if (st->done) return;
-    const double alpha = st->alpha;
+                                                      const double *__restrict__ pin, uint32_t rows,
+                                                      double *__restrict__ pout, LinState *st, long long it,
+                                                      long long seq) {
+    if (st->done_seq < seq) return;
+    double f[1], fm[1];
+    fold_partials<1, 1>(pin, rows, f, fm);
+    if (st->bad_row != ~0ull) return fail(st, seq, FL_ERR_NON_FINITE, C_SPMV_OUTPUT, st->bad_row);
+    const double den = f[0];
+    if (!isfinite(den)) return fail(st, seq, FL_ERR_NON_FINITE, C_DOT, 0);
+    if (fabs(den) <= kBreakdown) return fail(st, seq, FL_ERR_BREAKDOWN, C_BI_ALPHA_DEN, 0);
+    const double alpha = st->rho[it & 1] / den;
+    if (!isfinite(alpha)) return fail(st, seq, FL_ERR_BREAKDOWN, C_BI_ALPHA, 0);
+    if (lead()) st->alpha[it & 1] = alpha;
     double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
     TileSched ts((A.n + kRowsPerTile - 1) / kRowsPerTile);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
@@ -394,59 +443,51 @@ __global__ void __launch_bounds__(kBlock) k_bi_spmv_t(Mat A, const double *__res
         const double acc = row_dot_lanes(A, rs, [&](int c) { return r[c] - alpha * v[c]; });
         if (!rs.valid || rs.q != 0) continue;
         const uint32_t row = rs.row;
-        if (!isfinite(acc)) atomicMin(&st->bad_row, (unsigned long long)row);
+        if (!isfinite(acc)) atomicMin(&st->bad_row2, (unsigned long long)row);
         tvec[row] = acc;
         const double si = r[row] - alpha * v[row];
         s[0] += si * si;
         s[1] += acc * acc;
         s[2] += acc * si;
     }
-    if (grid_reduce<3, 1>(s, m, partials, &st->counters[3]) && threadIdx.x == 0) {
-        if (!isfinite(s[0])) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
-        const double sn = sqrt(s[0]);
-        if (sn <= st->threshold) {  // early exit on ||s|| (solver.rs:206-218)
-            st->rnorm = sn;
-            st->s_conv = 1;
-            return;
-        }
-        const unsigned long long bad = atomic_read(&st->bad_row);
-        if (bad != ~0ull) return fail(st, FL_ERR_NON_FINITE, C_SPMV_OUTPUT, bad);
-        const double tt = s[1];
-        if (!isfinite(tt)) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
-        if (fabs(tt) <= kBreakdown) return fail(st, FL_ERR_BREAKDOWN, C_BI_OMEGA_DEN, 0);
-        if (!isfinite(s[2])) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
-        const double omega = s[2] / tt;
-        if (!isfinite(omega) || fabs(omega) <= kBreakdown)
-            return fail(st, FL_ERR_BREAKDOWN, C_BI_OMEGA, 0);
-        st->omega = omega;
-    }
+    publish_partials<3, 1>(s, m, pout);
 }
 
+// The s-test (early exit with x += alpha p, solver.rs:206-218), omega =
+// t.s / t.t (:220-231); x += alpha p + omega s, r = s - omega t, partials
+// r.r and rhat.r.
 __global__ void __launch_bounds__(kBlock) k_bi_update(uint32_t n, double *__restrict__ x,
                                                       double *__restrict__ r,
                                                       const double *__restrict__ p,
                                                       const double *__restrict__ v,
                                                       const double *__restrict__ tvec,
                                                       const double *__restrict__ rhat,
-                                                      double *partials, LinState *st) {
-    if (st->done) return;
-    const double alpha = st->alpha;
-    if (st->s_conv) {  // solution += alpha p and return Converged
+                                                      const double *__restrict__ pin, uint32_t rows,
+                                                      double *__restrict__ pout, LinState *st, long long it,
+                                                      long long seq) {
+    if (st->done_seq < seq) return;
+    double f[3], fm[1];
+    fold_partials<3, 1>(pin, rows, f, fm);
+    const double alpha = st->alpha[it & 1];
+    if (!isfinite(f[0])) return fail(st, seq, FL_ERR_NON_FINITE, C_DOT, 0);
+    const double sn = sqrt(f[0]);
+    if (sn <= st->threshold) {  // solution += alpha p and return Converged
         TileSched ts((n + kBlock - 1) / kBlock);
         for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
             const uint32_t i = t * kBlock + threadIdx.x;
             if (i < n) x[i] += alpha * p[i];
         }
-        // Only the last block to arrive raises `done`: every block has
-        // passed the early-exit test above by then.
-        double s0[1] = {0.0}, m0[1] = {0.0};
-        if (grid_reduce<1, 1>(s0, m0, partials, &st->counters[4]) && threadIdx.x == 0) {
-            st->converged = 1;
-            st->done = 1;
-        }
-        return;
+        if (lead()) st->rnorm = sn;
+        return finish(st, seq, 1, it);
     }
-    const double omega = st->omega;
+    if (st->bad_row2 != ~0ull) return fail(st, seq, FL_ERR_NON_FINITE, C_SPMV_OUTPUT, st->bad_row2);
+    const double tt = f[1];
+    if (!isfinite(tt)) return fail(st, seq, FL_ERR_NON_FINITE, C_DOT, 0);
+    if (fabs(tt) <= kBreakdown) return fail(st, seq, FL_ERR_BREAKDOWN, C_BI_OMEGA_DEN, 0);
+    if (!isfinite(f[2])) return fail(st, seq, FL_ERR_NON_FINITE, C_DOT, 0);
+    const double omega = f[2] / tt;
+    if (!isfinite(omega) || fabs(omega) <= kBreakdown) return fail(st, seq, FL_ERR_BREAKDOWN, C_BI_OMEGA, 0);
+    if (lead()) st->omega[it & 1] = omega;
     double s[2] = {0.0, 0.0}, m[1] = {0.0};
     TileSched ts((n + kBlock - 1) / kBlock);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
@@ -459,29 +500,7 @@ __global__ void __launch_bounds__(kBlock) k_bi_update(uint32_t n, double *__rest
         s[0] += ri * ri;
         s[1] += rhat[i] * ri;
     }
-    if (grid_reduce<2, 1>(s, m, partials, &st->counters[4]) && threadIdx.x == 0) {
-        if (!isfinite(s[0])) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
-        const double rn = sqrt(s[0]);
-        st->rnorm = rn;
-        if (rn <= st->threshold) {
-            st->converged = 1;
-            st->done = 1;
-            return;
-        }
-        if (st->iter >= st->max_iter) {
-            st->done = 1;
-            return;
-        }
-        st->iter += 1;
-        st->rho_prev = st->rho;
-        const double rho = s[1];
-        if (!isfinite(rho)) return fail(st, FL_ERR_NON_FINITE, C_DOT, 0);
-        if (fabs(rho) <= kBreakdown) return fail(st, FL_ERR_BREAKDOWN, C_BI_RHO, 0);
-        const double beta = (rho / st->rho_prev) * (st->alpha / st->omega);
-        if (!isfinite(beta)) return fail(st, FL_ERR_BREAKDOWN, C_BI_BETA, 0);
-        st->rho = rho;
-        st->beta = beta;
-    }
+    publish_partials<2, 1>(s, m, pout);
 }
 
 // ------------------------------------------- allocation-free solve buffers
@@ -495,7 +514,7 @@ __global__ void __launch_bounds__(kBlock) k_bi_update(uint32_t n, double *__rest
 constexpr size_t kNoAllocMaxRows = (size_t)1 << 20;  // largest matrix dimension
 constexpr size_t kNoAllocStride = kNoAllocMaxRows + 32;  // one vector, 256-byte aligned
 constexpr uint32_t kNoAllocMaxBlocks = 4096;
-constexpr size_t kNoAllocPartials = (size_t)kNoAllocMaxBlocks * 4;
+constexpr size_t kNoAllocPartials = (size_t)kNoAllocMaxBlocks * 8;  // two buffers of 4-double rows
 constexpr size_t kNoAllocState = 64;  // doubles reserved for LinState
 constexpr size_t kScratchDoubles = 5 * kNoAllocStride + kNoAllocPartials + kNoAllocState;
 static_assert(sizeof(LinState) <= kNoAllocState * sizeof(double), "LinState outgrew its slot");
@@ -618,7 +637,8 @@ fl_status state_error(fl_csr *m, const LinState &h, fl_err_detail *d) {
 enum Kind { KIND_CG = 0, KIND_PCG = 1, KIND_BICG = 2 };
 
 // Device buffers of one solve: vectors x, b, r, d/p, q/v (+ t, rhat for
-// BiCGSTAB), the per-block partial rows and the solver state.
+// BiCGSTAB), two buffers of per-block partial rows (blocks x 4 doubles
+// each, the second right after the first) and the solver state.
 struct Work {
     double *vec[7];
     double *partials;
@@ -654,44 +674,50 @@ fl_status solve_on(fl_csr *m, int kind, const double *rhs, double *x_inout, cons
     FL_HIP_TRY(ctx, hipMemcpyAsync(x, x_inout, bytes, in, s));
     FL_HIP_TRY(ctx, hipMemcpyAsync(b, rhs, bytes, in, s));
     LinState init{};
-    init.threshold = opts->absolute_tolerance;
-    init.alpha = opts->relative_tolerance;  // consumed by k_cg_init_r
+    init.atol = opts->absolute_tolerance;
+    init.rtol = opts->relative_tolerance;
     init.max_iter = opts->max_iterations;
-    init.bad_row = ~0ull;
-    init.bad_z = ~0ull;
-    *m->st_host = init;  // also clears the reduction counters of the state on the device
+    init.done_seq = kRunning;
+    init.bad_row = init.bad_row2 = init.bad_r = init.bad_z = ~0ull;
+    *m->st_host = init;
     LinState *const dst = w.st;
-    double *const partials = w.partials;
     FL_HIP_TRY(ctx, hipMemcpyAsync(dst, m->st_host, sizeof(LinState), hipMemcpyHostToDevice, s));
     const Mat A{m->off, m->col, m->val, m->n};
     const double *inv = kind == KIND_PCG ? m->inv : nullptr;
     const uint32_t G = m->blocks;
+    // two partial-row buffers: a kernel folds one and publishes into the other
+    double *const pa = w.partials, *const pb = w.partials + (size_t)G * 4;
+    long long seq = 0;  // launch sequence number (k_spmv and k_cg_init_r: 0, 1)
     FL_HIP_TRY(ctx, hipEventRecord(m->e0, s));
     k_spmv<<<G, kBlock, 0, s>>>(A, x, qv, dst);
-    k_cg_init_r<<<G, kBlock, 0, s>>>(m->n, b, qv, r, partials, dst);
-    if (kind == KIND_BICG)
-        k_bi_init<<<G, kBlock, 0, s>>>(m->n, r, rhat, dp, qv, partials, dst);
-    else
-        k_cg_init_d<<<G, kBlock, 0, s>>>(m->n, r, inv, dp, partials, dst);
+    k_cg_init_r<<<G, kBlock, 0, s>>>(m->n, b, qv, r, pb, dst);
+    seq = 2;
+    if (kind == KIND_BICG) {
+        k_bi_init<<<G, kBlock, 0, s>>>(m->n, r, rhat, dp, qv, pb, G, pa, dst, seq++);
+    } else {
+        k_cg_init_d<<<G, kBlock, 0, s>>>(m->n, r, inv, dp, pb, G, pa, dst, seq++);
+        k_cg_init_rho<<<1, kBlock, 0, s>>>(pa, G, dst, seq++);
+    }
     FL_HIP_TRY(ctx, hipGetLastError());
     int batch = 4;
+    long long it = 1;
     for (;;) {
-        for (int it = 0; it < batch; ++it) {
+        for (int k = 0; k < batch; ++k, ++it) {
             if (kind == KIND_BICG) {
-                k_bi_p<<<G, kBlock, 0, s>>>(m->n, r, dp, qv, dst);
-                k_bi_spmv_v<<<G, kBlock, 0, s>>>(A, dp, rhat, qv, partials, dst);
-                k_bi_spmv_t<<<G, kBlock, 0, s>>>(A, r, qv, tv, partials, dst);
-                k_bi_update<<<G, kBlock, 0, s>>>(m->n, x, r, dp, qv, tv, rhat, partials, dst);
+                k_bi_p<<<G, kBlock, 0, s>>>(m->n, r, dp, qv, pa, G, dst, it, seq++);
+                k_bi_spmv_v<<<G, kBlock, 0, s>>>(A, dp, rhat, qv, pa, dst, seq++);
+                k_bi_spmv_t<<<G, kBlock, 0, s>>>(A, r, qv, tv, pa, G, pb, dst, it, seq++);
+                k_bi_update<<<G, kBlock, 0, s>>>(m->n, x, r, dp, qv, tv, rhat, pb, G, pa, dst, it, seq++);
             } else {
-                k_cg_spmv<<<G, kBlock, 0, s>>>(A, dp, qv, partials, dst);
-                k_cg_update<<<G, kBlock, 0, s>>>(m->n, x, r, dp, qv, inv, partials, dst);
-                k_cg_dir<<<G, kBlock, 0, s>>>(m->n, r, inv, dp, dst);
+                k_cg_spmv<<<G, kBlock, 0, s>>>(A, dp, qv, pa, dst, seq++);
+                k_cg_update<<<G, kBlock, 0, s>>>(m->n, x, r, dp, qv, inv, pa, G, pb, dst, it, seq++);
+                k_cg_dir<<<G, kBlock, 0, s>>>(m->n, r, inv, dp, pb, G, dst, it, seq++);
             }
         }
         FL_HIP_TRY(ctx, hipGetLastError());
         FL_HIP_TRY(ctx, hipMemcpyAsync(m->st_host, dst, sizeof(LinState), hipMemcpyDeviceToHost, s));
         FL_HIP_TRY(ctx, hipStreamSynchronize(s));
-        if (m->st_host->done) break;
+        if (m->st_host->done_seq != kRunning) break;
         batch = std::min(batch * 2, 64);
     }
     FL_HIP_TRY(ctx, hipEventRecord(m->e1, s));
@@ -838,7 +864,7 @@ fl_status csr_adopt(fl_ctx *ctx, uint32_t n, uint32_t ncols, int64_t nnz, int *o
     hipStream_t s = ctx->stream;
     unsigned long long *first = nullptr;
     bool okay = hipMalloc(&m->inv, (n > 0 ? n : 1) * sizeof(double)) == hipSuccess &&
-                hipMalloc(&m->partials, (size_t)m->blocks * 4 * sizeof(double)) == hipSuccess &&
+                hipMalloc(&m->partials, (size_t)m->blocks * 8 * sizeof(double)) == hipSuccess &&
                 hipMalloc(&m->st, sizeof(LinState)) == hipSuccess &&
                 hipHostMalloc(&m->st_host, sizeof(LinState), hipHostMallocDefault) == hipSuccess &&
                 hipEventCreate(&m->e0) == hipSuccess && hipEventCreate(&m->e1) == hipSuccess &&
@@ -938,7 +964,7 @@ fl_status fl_csr_upload(fl_ctx *ctx, int64_t nrows, int64_t ncols, const int64_t
                 hipMalloc(&m->col, (nnz > 0 ? nnz : 1) * sizeof(int)) == hipSuccess &&
                 hipMalloc(&m->val, (nnz > 0 ? nnz : 1) * sizeof(double)) == hipSuccess &&
                 hipMalloc(&m->inv, (nrows > 0 ? nrows : 1) * sizeof(double)) == hipSuccess &&
-                hipMalloc(&m->partials, (size_t)m->blocks * 4 * sizeof(double)) == hipSuccess &&
+                hipMalloc(&m->partials, (size_t)m->blocks * 8 * sizeof(double)) == hipSuccess &&
                 hipMalloc(&m->st, sizeof(LinState)) == hipSuccess &&
                 hipHostMalloc(&m->st_host, sizeof(LinState), hipHostMallocDefault) == hipSuccess &&
                 hipEventCreate(&m->e0) == hipSuccess && hipEventCreate(&m->e1) == hipSuccess;
