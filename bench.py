@@ -460,19 +460,31 @@ def run_c3(fa, ctx, args):
                                   "(CG 11, Jacobi-PCG 13, BiCGSTAB 18 passes and 2 SpMVs)"}
     opts = FL.LinearSolverOptions(0.0, 1e-10, 50000)
     for kind, M_ in (("cg", A), ("pcg", A), ("bicgstab", B)):
-        dev = FL.CsrMatrix(n, n, M_.indptr, M_.indices, M_.data, ctx)
-        x = np.zeros(n)
-        getattr(FL, kind)(dev, rhs, x, FL.LinearSolverOptions(0.0, 1e-10, 5))  # warm-up
-        x = np.zeros(n)
-        rep = getattr(FL, kind)(dev, rhs, x, opts)
-        dev.close()
-        true = float(np.linalg.norm(rhs - M_ @ x) / np.linalg.norm(rhs))
-        s = rep.device_ms / 1e3
-        out[kind] = {"s_per_solve": s, "iterations": rep.iterations, "converged": rep.converged(),
-                     "true_rel_residual": true,
-                     "GBps": rep.iterations * bytes_iter[kind] / s / 1e9 if s > 0 else None}
-        out[kind]["frac_of_8TBps"] = (out[kind]["GBps"] or 0.0) / HBM_PEAK_GBS
-        log(f"[bench] C3 {kind}: {rep.iterations} its, {s:.3f} s, {out[kind]['GBps']:.0f} GB/s")
+        for mixed in (False, True):
+            dev = FL.CsrMatrix(n, n, M_.indptr, M_.indices, M_.data, ctx)
+            if mixed:
+                dev.set_precision(fa.PRECISION_MIXED)
+            x = np.zeros(n)
+            getattr(FL, kind)(dev, rhs, x, FL.LinearSolverOptions(0.0, 1e-10, 5))  # warm-up
+            x = np.zeros(n)
+            rep = getattr(FL, kind)(dev, rhs, x, opts)
+            dev.close()
+            true = float(np.linalg.norm(rhs - M_ @ x) / np.linalg.norm(rhs))
+            s = rep.device_ms / 1e3
+            key = kind + ("_mixed" if mixed else "")
+            out[key] = {"s_per_solve": s, "iterations": rep.iterations, "converged": rep.converged(),
+                        "true_rel_residual": true}
+            if mixed:
+                # FP32 inner iterations (4-byte values and vectors) plus one FP64
+                # residual per round; GB/s is not comparable, time is
+                out[key].update({"refinements": rep.refinements, "fallback": rep.fallback,
+                                 "speedup_vs_fp64": out[kind]["s_per_solve"] / s if s > 0 else None})
+                log(f"[bench] C3 {key}: {rep.iterations} its in {rep.refinements} rounds, {s:.3f} s, "
+                    f"true rel res {true:.2e}")
+            else:
+                out[key]["GBps"] = rep.iterations * bytes_iter[kind] / s / 1e9 if s > 0 else None
+                out[key]["frac_of_8TBps"] = (out[key]["GBps"] or 0.0) / HBM_PEAK_GBS
+                log(f"[bench] C3 {kind}: {rep.iterations} its, {s:.3f} s, {out[key]['GBps']:.0f} GB/s")
     if not args.no_cpu:
         Ao = O.Csr(n, n, A.indptr.astype(np.int64), A.indices.astype(np.int64), A.data, validate=False)
         k = 30

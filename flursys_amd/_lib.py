@@ -109,9 +109,10 @@ class LinOpts(C.Structure):
 
 
 class LinReport(C.Structure):
-    _fields_ = [("converged", C.c_int32), ("reserved", C.c_int32), ("iterations", C.c_int64),
+    _fields_ = [("converged", C.c_int32), ("refinements", C.c_int32), ("iterations", C.c_int64),
                 ("initial_residual", C.c_double), ("final_residual", C.c_double),
-                ("relative_residual", C.c_double), ("device_ms", C.c_double)]
+                ("relative_residual", C.c_double), ("device_ms", C.c_double),
+                ("fallback", C.c_int32), ("reserved", C.c_int32)]
 
 
 class ErrDetail(C.Structure):
@@ -255,6 +256,7 @@ SIGNATURES = {
     "fl_pcg_jacobi": (C.c_int, _LIN),
     "fl_bicgstab": (C.c_int, _LIN),
     "fl_pcg_jacobi_noalloc": (C.c_int, _LIN),
+    "fl_csr_set_precision": (C.c_int, [_P, C.c_int32, C.c_double]),
     "fl_unet_create": (C.c_int, [_P, C.POINTER(UNetConfig), C.POINTER(_P)]),
     "fl_unet_destroy": (None, [_P]),
     "fl_unet_num_params": (C.c_int64, [_P]),

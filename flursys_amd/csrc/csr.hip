@@ -64,6 +64,7 @@ enum Ctx : int {
     C_BI_ALPHA,
     C_BI_OMEGA_DEN,
     C_BI_OMEGA,
+    C_DIVERGED,
 };
 const char *ctx_text(int c) {
     switch (c) {
@@ -82,21 +83,25 @@ const char *ctx_text(int c) {
         case C_BI_ALPHA: return "non-finite BiCGSTAB alpha";
         case C_BI_OMEGA_DEN: return "zero BiCGSTAB omega denominator";
         case C_BI_OMEGA: return "zero BiCGSTAB omega";
+        case C_DIVERGED: return "diverging inner solve";  // internal to the mixed path
         default: return "";
     }
 }
 
-struct Mat {
+template <typename T> struct MatT {
     const int *off;
     const int *col;
-    const double *val;
+    const T *val;
     uint32_t n;
 };
+typedef MatT<double> Mat;
 
 struct LinState {
     double rho[2];              // r.z (CG) or rhat.r (BiCGSTAB) by iteration parity
     double alpha[2], omega[2];  // BiCGSTAB, by iteration parity
     double atol, rtol;          // the options (threshold = atol + rtol ||r0||)
+    double in_scale;            // 1, or (mixed inner solve) 1 / ||r|| of the round's residual
+    double diverge;             // mixed inner solve: end it when ||r|| exceeds this (0: never)
     double threshold, initial, rnorm;
     long long iter, max_iter;
     long long done_seq;         // sequence number of the launch that ended the solve
@@ -128,14 +133,14 @@ __device__ inline RowSlot row_slot(uint32_t tile, uint32_t n) {
     r.valid = r.row < n;
     return r;
 }
-template <typename F>
-__device__ inline double row_dot_lanes(const Mat &A, const RowSlot &r, F &&xval) {
-    double acc = 0.0;
+template <typename T, typename F>
+__device__ inline T row_dot_lanes(const MatT<T> &A, const RowSlot &r, F &&xval) {
+    T acc = 0;
     const int b = r.valid ? A.off[r.row] : 0, e = r.valid ? A.off[r.row + 1] : 0;
     const int lead = (int)(threadIdx.x & 63u) & ~(int)(kRowLanes - 1);
     for (int k0 = b; k0 < e; k0 += kRowLanes) {  // uniform over the group
         const int k = k0 + (int)r.q;
-        const double prod = k < e ? A.val[k] * xval(A.col[k]) : 0.0;
+        const T prod = k < e ? A.val[k] * xval(A.col[k]) : (T)0;
 #pragma unroll
         for (int t = 0; t < (int)kRowLanes; ++t) acc += __shfl(prod, lead + t);
     }
@@ -204,7 +209,7 @@ __device__ inline bool initial_tests(const double *__restrict__ pin, uint32_t ro
     if (st->bad_row != ~0ull) return fail(st, seq, FL_ERR_NON_FINITE, C_SPMV_OUTPUT, st->bad_row), false;
     if (st->bad_r != ~0ull) return fail(st, seq, FL_ERR_NON_FINITE, C_RESIDUAL, st->bad_r), false;
     if (!isfinite(f[0])) return fail(st, seq, FL_ERR_NON_FINITE, C_DOT, 0), false;
-    const double init = sqrt(f[0]);
+    const double init = sqrt(f[0]) * st->in_scale;
     const double threshold = st->atol + st->rtol * init;
     if (lead()) {
         st->initial = init;
@@ -216,21 +221,27 @@ __device__ inline bool initial_tests(const double *__restrict__ pin, uint32_t ro
 }
 
 // Initial tests; z = D^-1 r (or r), d = z, partial r.z.
-__global__ void __launch_bounds__(kBlock) k_cg_init_d(uint32_t n, const double *__restrict__ r,
-                                                      const double *__restrict__ inv,
-                                                      double *__restrict__ d, const double *__restrict__ pin,
+// (T = float: the inner solve of the mixed path, which starts from e = 0 on
+// the round's residual scaled to unit norm: r is scaled here by in_scale.)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cg_init_d(uint32_t n, T *__restrict__ r,
+                                                      const T *__restrict__ inv,
+                                                      T *__restrict__ d, const double *__restrict__ pin,
                                                       uint32_t rows, double *__restrict__ pout, LinState *st,
-                                                      long long seq) {
-    if (!initial_tests(pin, rows, st, seq)) return;
+                                                      long long seq, T *__restrict__ x0) {
+    if (st->done_seq < seq || !initial_tests(pin, rows, st, seq)) return;
+    const double sc = st->in_scale;
     double s[1] = {0.0}, m[1] = {0.0};
     TileSched ts((n + kBlock - 1) / kBlock);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const uint32_t i = t * kBlock + threadIdx.x;
         if (i >= n) continue;
-        const double z = inv ? r[i] * inv[i] : r[i];
+        if (sc != 1.0) r[i] *= (T)sc;
+        const T z = inv ? r[i] * inv[i] : r[i];
         if (!isfinite(z)) atomicMin(&st->bad_z, (unsigned long long)i);
         d[i] = z;
-        s[0] += r[i] * z;
+        if (x0) x0[i] = 0;
+        s[0] += (double)r[i] * (double)z;
     }
     publish_partials<1, 1>(s, m, pout);
 }
@@ -248,30 +259,32 @@ __global__ void __launch_bounds__(kBlock) k_cg_init_rho(const double *__restrict
 }
 
 // ------------------------------------------------------------ CG loop
-__global__ void __launch_bounds__(kBlock) k_cg_spmv(Mat A, const double *__restrict__ d,
-                                                    double *__restrict__ q, double *__restrict__ pout,
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cg_spmv(MatT<T> A, const T *__restrict__ d,
+                                                    T *__restrict__ q, double *__restrict__ pout,
                                                     LinState *st, long long seq) {
     if (st->done_seq < seq) return;
     double s[1] = {0.0}, m[1] = {0.0};
     TileSched ts((A.n + kRowsPerTile - 1) / kRowsPerTile);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const RowSlot r = row_slot(t, A.n);
-        const double v = row_dot_lanes(A, r, [&](int c) { return d[c]; });
+        const T v = row_dot_lanes(A, r, [&](int c) { return d[c]; });
         if (!r.valid || r.q != 0) continue;
         if (!isfinite(v)) atomicMin(&st->bad_row, (unsigned long long)r.row);
         q[r.row] = v;
-        s[0] += d[r.row] * v;
+        s[0] += (double)d[r.row] * (double)v;
     }
     publish_partials<1, 1>(s, m, pout);
 }
 
 // alpha = rho / d.q (solver.rs:98-110); x += alpha d, r -= alpha q, partials
 // r.r and r.z.
-__global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t n, double *__restrict__ x,
-                                                      double *__restrict__ r,
-                                                      const double *__restrict__ d,
-                                                      const double *__restrict__ q,
-                                                      const double *__restrict__ inv,
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t n, T *__restrict__ x,
+                                                      T *__restrict__ r,
+                                                      const T *__restrict__ d,
+                                                      const T *__restrict__ q,
+                                                      const T *__restrict__ inv,
                                                       const double *__restrict__ pin, uint32_t rows,
                                                       double *__restrict__ pout, LinState *st,
                                                       long long it, long long seq) {
@@ -289,22 +302,23 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t n, double *__rest
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const uint32_t i = t * kBlock + threadIdx.x;
         if (i >= n) continue;
-        x[i] += alpha * d[i];
-        const double ri = r[i] - alpha * q[i];
+        x[i] += (T)alpha * d[i];
+        const T ri = r[i] - (T)alpha * q[i];
         r[i] = ri;
-        s[0] += ri * ri;
-        const double z = inv ? ri * inv[i] : ri;
+        s[0] += (double)ri * (double)ri;
+        const T z = inv ? ri * inv[i] : ri;
         if (!isfinite(z)) atomicMin(&st->bad_z, (unsigned long long)i);
-        s[1] += ri * z;
+        s[1] += (double)ri * (double)z;
     }
     publish_partials<2, 1>(s, m, pout);
 }
 
 // The stop test (norm_l2, solver.rs:120-133), rho_new = r.z and beta
 // (:134-148), MaxIterations after the last iteration; d = z + beta d.
-__global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, const double *__restrict__ r,
-                                                   const double *__restrict__ inv,
-                                                   double *__restrict__ d, const double *__restrict__ pin,
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, const T *__restrict__ r,
+                                                   const T *__restrict__ inv,
+                                                   T *__restrict__ d, const double *__restrict__ pin,
                                                    uint32_t rows, LinState *st, long long it, long long seq) {
     if (st->done_seq < seq) return;
     double f[2], fm[1];
@@ -313,6 +327,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, const double *__r
     const double rn = sqrt(f[0]);
     if (lead()) st->rnorm = rn;
     if (rn <= st->threshold) return finish(st, seq, 1, it);
+    if (st->diverge > 0.0 && rn > st->diverge) return fail(st, seq, FL_ERR_BREAKDOWN, C_DIVERGED, 0);
     if (st->bad_z != ~0ull) return fail(st, seq, FL_ERR_NON_FINITE, C_PRECOND, st->bad_z);
     const double rho_new = f[1], rho = st->rho[it & 1];
     if (!isfinite(rho_new)) return fail(st, seq, FL_ERR_NON_FINITE, C_DOT, 0);
@@ -328,8 +343,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, const double *__r
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const uint32_t i = t * kBlock + threadIdx.x;
         if (i >= n) continue;
-        const double z = inv ? r[i] * inv[i] : r[i];
-        d[i] = z + beta * d[i];
+        const T z = inv ? r[i] * inv[i] : r[i];
+        d[i] = z + (T)beta * d[i];
     }
 }
 
@@ -337,29 +352,33 @@ __global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, const double *__r
 // Initial r = b - A x as for CG (k_spmv + k_cg_init_r); then the initial
 // tests, rhat = r, p = v = 0, the iteration-0 scalars rho = alpha = omega
 // = 1 (solver.rs:175-177) and partials {r.r, rhat.r}.
-__global__ void __launch_bounds__(kBlock) k_bi_init(uint32_t n, const double *__restrict__ r,
-                                                    double *__restrict__ rhat, double *__restrict__ p,
-                                                    double *__restrict__ v, const double *__restrict__ pin,
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bi_init(uint32_t n, T *__restrict__ r,
+                                                    T *__restrict__ rhat, T *__restrict__ p,
+                                                    T *__restrict__ v, const double *__restrict__ pin,
                                                     uint32_t rows, double *__restrict__ pout, LinState *st,
-                                                    long long seq) {
-    if (!initial_tests(pin, rows, st, seq)) return;
+                                                    long long seq, T *__restrict__ x0) {
+    if (st->done_seq < seq || !initial_tests(pin, rows, st, seq)) return;
     if (lead()) {
         st->rho[0] = 1.0;
         st->alpha[0] = 1.0;
         st->omega[0] = 1.0;
     }
     if (st->max_iter < 1) return finish(st, seq, 0, 0);
+    const double sc = st->in_scale;
     double s[2] = {0.0, 0.0}, m[1] = {0.0};
     TileSched ts((n + kBlock - 1) / kBlock);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const uint32_t i = t * kBlock + threadIdx.x;
         if (i >= n) continue;
-        const double ri = r[i];
+        if (sc != 1.0) r[i] *= (T)sc;
+        const T ri = r[i];
         rhat[i] = ri;
-        p[i] = 0.0;
-        v[i] = 0.0;
-        s[0] += ri * ri;
-        s[1] += ri * ri;
+        p[i] = 0;
+        v[i] = 0;
+        if (x0) x0[i] = 0;
+        s[0] += (double)ri * (double)ri;
+        s[1] += (double)ri * (double)ri;
     }
     publish_partials<2, 1>(s, m, pout);
 }
@@ -367,8 +386,9 @@ __global__ void __launch_bounds__(kBlock) k_bi_init(uint32_t n, const double *__
 // The end of iteration it - 1 (its residual norm test, solver.rs:236-245,
 // and MaxIterations), then iteration it's rho = rhat.r and beta (:179-190);
 // p = r + beta (p - omega v).
-__global__ void __launch_bounds__(kBlock) k_bi_p(uint32_t n, const double *__restrict__ r,
-                                                 double *__restrict__ p, const double *__restrict__ v,
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bi_p(uint32_t n, const T *__restrict__ r,
+                                                 T *__restrict__ p, const T *__restrict__ v,
                                                  const double *__restrict__ pin, uint32_t rows, LinState *st,
                                                  long long it, long long seq) {
     if (st->done_seq < seq) return;
@@ -379,6 +399,7 @@ __global__ void __launch_bounds__(kBlock) k_bi_p(uint32_t n, const double *__res
         const double rn = sqrt(f[0]);
         if (lead()) st->rnorm = rn;
         if (rn <= st->threshold) return finish(st, seq, 1, it - 1);
+        if (st->diverge > 0.0 && rn > st->diverge) return fail(st, seq, FL_ERR_BREAKDOWN, C_DIVERGED, 0);
         if (it - 1 >= st->max_iter) return finish(st, seq, 0, it - 1);
     }
     const int c = (int)(it & 1), pv = c ^ 1;
@@ -396,33 +417,35 @@ __global__ void __launch_bounds__(kBlock) k_bi_p(uint32_t n, const double *__res
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const uint32_t i = t * kBlock + threadIdx.x;
         if (i >= n) continue;
-        p[i] = r[i] + beta * (p[i] - omega * v[i]);
+        p[i] = r[i] + (T)beta * (p[i] - (T)omega * v[i]);
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_bi_spmv_v(Mat A, const double *__restrict__ p,
-                                                      const double *__restrict__ rhat,
-                                                      double *__restrict__ v, double *__restrict__ pout,
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bi_spmv_v(MatT<T> A, const T *__restrict__ p,
+                                                      const T *__restrict__ rhat,
+                                                      T *__restrict__ v, double *__restrict__ pout,
                                                       LinState *st, long long seq) {
     if (st->done_seq < seq) return;
     double s[1] = {0.0}, m[1] = {0.0};
     TileSched ts((A.n + kRowsPerTile - 1) / kRowsPerTile);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const RowSlot r = row_slot(t, A.n);
-        const double vv = row_dot_lanes(A, r, [&](int c) { return p[c]; });
+        const T vv = row_dot_lanes(A, r, [&](int c) { return p[c]; });
         if (!r.valid || r.q != 0) continue;
         if (!isfinite(vv)) atomicMin(&st->bad_row, (unsigned long long)r.row);
         v[r.row] = vv;
-        s[0] += rhat[r.row] * vv;
+        s[0] += (double)rhat[r.row] * (double)vv;
     }
     publish_partials<1, 1>(s, m, pout);
 }
 
 // alpha = rho / rhat.v (solver.rs:192-203); t = A s with s = r - alpha v
 // formed at each gathered column; partials s.s, t.t, t.s.
-__global__ void __launch_bounds__(kBlock) k_bi_spmv_t(Mat A, const double *__restrict__ r,
-                                                      const double *__restrict__ v,
-                                                      double *__restrict__ tvec,
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bi_spmv_t(MatT<T> A, const T *__restrict__ r,
+                                                      const T *__restrict__ v,
+                                                      T *__restrict__ tvec,
                                                       const double *__restrict__ pin, uint32_t rows,
                                                       double *__restrict__ pout, LinState *st, long long it,
                                                       long long seq) {
@@ -440,15 +463,16 @@ __global__ void __launch_bounds__(kBlock) k_bi_spmv_t(Mat A, const double *__res
     TileSched ts((A.n + kRowsPerTile - 1) / kRowsPerTile);
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const RowSlot rs = row_slot(t, A.n);
-        const double acc = row_dot_lanes(A, rs, [&](int c) { return r[c] - alpha * v[c]; });
+        const T ta = (T)alpha;
+        const T acc = row_dot_lanes(A, rs, [&](int c) { return r[c] - ta * v[c]; });
         if (!rs.valid || rs.q != 0) continue;
         const uint32_t row = rs.row;
         if (!isfinite(acc)) atomicMin(&st->bad_row2, (unsigned long long)row);
         tvec[row] = acc;
-        const double si = r[row] - alpha * v[row];
-        s[0] += si * si;
-        s[1] += acc * acc;
-        s[2] += acc * si;
+        const T si = r[row] - ta * v[row];
+        s[0] += (double)si * (double)si;
+        s[1] += (double)acc * (double)acc;
+        s[2] += (double)acc * (double)si;
     }
     publish_partials<3, 1>(s, m, pout);
 }
@@ -456,12 +480,13 @@ __global__ void __launch_bounds__(kBlock) k_bi_spmv_t(Mat A, const double *__res
 // The s-test (early exit with x += alpha p, solver.rs:206-218), omega =
 // t.s / t.t (:220-231); x += alpha p + omega s, r = s - omega t, partials
 // r.r and rhat.r.
-__global__ void __launch_bounds__(kBlock) k_bi_update(uint32_t n, double *__restrict__ x,
-                                                      double *__restrict__ r,
-                                                      const double *__restrict__ p,
-                                                      const double *__restrict__ v,
-                                                      const double *__restrict__ tvec,
-                                                      const double *__restrict__ rhat,
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bi_update(uint32_t n, T *__restrict__ x,
+                                                      T *__restrict__ r,
+                                                      const T *__restrict__ p,
+                                                      const T *__restrict__ v,
+                                                      const T *__restrict__ tvec,
+                                                      const T *__restrict__ rhat,
                                                       const double *__restrict__ pin, uint32_t rows,
                                                       double *__restrict__ pout, LinState *st, long long it,
                                                       long long seq) {
@@ -475,7 +500,7 @@ __global__ void __launch_bounds__(kBlock) k_bi_update(uint32_t n, double *__rest
         TileSched ts((n + kBlock - 1) / kBlock);
         for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
             const uint32_t i = t * kBlock + threadIdx.x;
-            if (i < n) x[i] += alpha * p[i];
+            if (i < n) x[i] += (T)alpha * p[i];
         }
         if (lead()) st->rnorm = sn;
         return finish(st, seq, 1, it);
@@ -493,14 +518,102 @@ __global__ void __launch_bounds__(kBlock) k_bi_update(uint32_t n, double *__rest
     for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
         const uint32_t i = t * kBlock + threadIdx.x;
         if (i >= n) continue;
-        const double si = r[i] - alpha * v[i];
-        x[i] += alpha * p[i] + omega * si;
-        const double ri = si - omega * tvec[i];
+        const T ta = (T)alpha, tw = (T)omega;
+        const T si = r[i] - ta * v[i];
+        x[i] += ta * p[i] + tw * si;
+        const T ri = si - tw * tvec[i];
         r[i] = ri;
-        s[0] += ri * ri;
-        s[1] += rhat[i] * ri;
+        s[0] += (double)ri * (double)ri;
+        s[1] += (double)rhat[i] * (double)ri;
     }
     publish_partials<2, 1>(s, m, pout);
+}
+
+// ------------------------------------------------- FP32 inner, FP64 refinement
+// The mixed path (fl_csr_set_precision, FL_PRECISION_MIXED): the solver
+// runs in FP32 (values, vectors, SpMV, axpys; dots, alpha/beta and tests in
+// FP64) on the correction e of A e = r inside FP64 iterative refinement.
+// Each round forms the true FP64 residual r = b - A x (k_spmv with the FP64
+// values, k_res_mixed), makes the reference's stop test on it against the
+// threshold fixed by the initial residual (k_outer_test), solves for e from
+// e = 0 and adds it, x += e (k_add_e).
+struct OuterState {
+    double atol, rtol;        // the caller's options
+    double threshold, initial, rnorm, prev;
+    long long round;
+    int converged, stalled, exhausted, pad;
+};
+
+// r = b - q in FP64 (q = A x), stored FP32; partial r.r of the FP64 values;
+// the round's correction e = 0.
+__global__ void __launch_bounds__(kBlock) k_res_mixed(uint32_t n, const double *__restrict__ b,
+                                                      const double *__restrict__ q, float *__restrict__ r32,
+                                                      float *__restrict__ e, double *__restrict__ pout,
+                                                      LinState *st) {
+    double s[1] = {0.0}, m[1] = {0.0};
+    TileSched ts((n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        if (i >= n) continue;
+        const double v = b[i] - q[i];
+        if (!isfinite(v)) atomicMin(&st->bad_r, (unsigned long long)i);
+        r32[i] = (float)v;
+        e[i] = 0.0f;  // the round's correction starts from zero
+        s[0] += v * v;
+    }
+    publish_partials<1, 1>(s, m, pout);
+}
+
+// The outer stop test of a round (one workgroup): ||r||_2 of the FP64
+// residual against atol + rtol ||r_0||_2 (solver.rs:82-88, 120-133); the
+// round's inner solve targets ||r_e|| <= threshold / 2 + inner_rel ||r||
+// and is skipped (done_seq 0) when the test passes, when the last round did
+// not halve the residual, or when the iteration budget is spent.
+__global__ void __launch_bounds__(kBlock) k_outer_test(const double *__restrict__ pin, uint32_t rows,
+                                                       OuterState *os, LinState *st, double inner_rel,
+                                                       int exhausted) {
+    double f[1], fm[1];
+    fold_partials<1, 1>(pin, rows, f, fm);
+    if (threadIdx.x != 0 || !isfinite(f[0])) return;  // (a non-finite residual: the inner tests report it)
+    const double rn = sqrt(f[0]);
+    if (os->round == 0) {
+        os->initial = rn;
+        os->threshold = os->atol + os->rtol * rn;
+    } else if (rn > 0.5 * os->prev) {
+        os->stalled = 1;  // a round must at least halve the residual
+    }
+    os->rnorm = rn;
+    os->prev = rn;
+    os->round += 1;
+    if (rn <= os->threshold) os->converged = 1;
+    if (exhausted) os->exhausted = 1;
+    if (os->converged || os->stalled || os->exhausted) {
+        st->done_seq = 0;
+        return;
+    }
+    // the inner solve works on r / ||r|| (unit norm): its absolute tests act
+    // as on a problem of ordinary scale however small the residual has become
+    st->in_scale = 1.0 / rn;
+    st->atol = 0.5 * os->threshold / rn;
+    st->rtol = inner_rel;
+    st->diverge = 1.0e4;  // (unit initial residual) an FP32 BiCGSTAB that blows up ends early
+}
+
+// x += ||r|| e (the inner solve's e solves the unit-norm system)
+__global__ void __launch_bounds__(kBlock) k_add_e(uint32_t n, double *__restrict__ x,
+                                                  const float *__restrict__ e, const OuterState *os) {
+    const double sc = os->rnorm;
+    TileSched ts((n + kBlock - 1) / kBlock);
+    for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        if (i < n) x[i] += (double)e[i] * sc;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_to_f32(long long n, const double *__restrict__ a,
+                                                   float *__restrict__ b) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        b[i] = (float)a[i];
 }
 
 // ------------------------------------------- allocation-free solve buffers
@@ -565,6 +678,11 @@ struct fl_csr {
     int64_t inv_row = 0;
     const char *inv_ctx = "";
     double *vec[8] = {};       // x, b, r, d/p, q/v, t, rhat, spare
+    int precision = FL_PRECISION_FP64;  // fl_csr_set_precision
+    double inner_rel = 1e-5;   // mixed: the inner solve's relative target
+    float *val32 = nullptr, *inv32 = nullptr;  // mixed: FP32 copies (made on first use)
+    float *vec32[6] = {};      // mixed: r, d/p, q/v, e, t, rhat
+    OuterState *os = nullptr, *os_host = nullptr;
     double *partials = nullptr;
     LinState *st = nullptr, *st_host = nullptr;
     uint32_t blocks = 1;
@@ -594,9 +712,12 @@ fl_status err(fl_ctx *ctx, fl_err_detail *d, fl_status st, const char *text, int
 }
 
 void free_csr(fl_csr *m) {
-    void *bufs[] = {m->off, m->col, m->val, m->inv, m->partials, m->st, m->first};
+    void *bufs[] = {m->off, m->col, m->val, m->inv, m->partials, m->st, m->first, m->val32, m->inv32, m->os};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    for (float *v : m->vec32)
+        if (v) (void)hipFree(v);
+    if (m->os_host) (void)hipHostFree(m->os_host);
     for (double *v : m->vec)
         if (v) (void)hipFree(v);
     if (m->st_host) (void)hipHostFree(m->st_host);
@@ -677,6 +798,7 @@ fl_status solve_on(fl_csr *m, int kind, const double *rhs, double *x_inout, cons
     init.atol = opts->absolute_tolerance;
     init.rtol = opts->relative_tolerance;
     init.max_iter = opts->max_iterations;
+    init.in_scale = 1.0;
     init.done_seq = kRunning;
     init.bad_row = init.bad_row2 = init.bad_r = init.bad_z = ~0ull;
     *m->st_host = init;
@@ -693,9 +815,9 @@ fl_status solve_on(fl_csr *m, int kind, const double *rhs, double *x_inout, cons
     k_cg_init_r<<<G, kBlock, 0, s>>>(m->n, b, qv, r, pb, dst);
     seq = 2;
     if (kind == KIND_BICG) {
-        k_bi_init<<<G, kBlock, 0, s>>>(m->n, r, rhat, dp, qv, pb, G, pa, dst, seq++);
+        k_bi_init<<<G, kBlock, 0, s>>>(m->n, r, rhat, dp, qv, pb, G, pa, dst, seq++, (double *)nullptr);
     } else {
-        k_cg_init_d<<<G, kBlock, 0, s>>>(m->n, r, inv, dp, pb, G, pa, dst, seq++);
+        k_cg_init_d<<<G, kBlock, 0, s>>>(m->n, r, inv, dp, pb, G, pa, dst, seq++, (double *)nullptr);
         k_cg_init_rho<<<1, kBlock, 0, s>>>(pa, G, dst, seq++);
     }
     FL_HIP_TRY(ctx, hipGetLastError());
@@ -737,10 +859,15 @@ fl_status solve_on(fl_csr *m, int kind, const double *rhs, double *x_inout, cons
         rep->final_residual = h.rnorm;
         rep->relative_residual = h.initial == 0.0 ? 0.0 : h.rnorm / h.initial;
         rep->device_ms = ms;
+        rep->refinements = 0;
+        rep->fallback = 0;
     }
     if (d) set_detail(d, FL_OK, "");
     return FL_OK;
 }
+
+fl_status solve_mixed(fl_csr *m, int kind, const double *rhs, double *x_inout, const fl_lin_opts *opts,
+                      fl_lin_report *rep, fl_err_detail *d);
 
 // The library-allocated path: work vectors allocated on first use and kept
 // with the matrix.
@@ -750,12 +877,179 @@ fl_status solve(fl_csr *m, int kind, const double *rhs, double *x_inout, const f
     FL_HIP_TRY(ctx, hipSetDevice(ctx->device));
     fl_status st = check_solve(m, kind, rhs, x_inout, opts, d);
     if (st != FL_OK) return st;
+    if (m->precision == FL_PRECISION_MIXED) return solve_mixed(m, kind, rhs, x_inout, opts, rep, d);
     if ((st = ensure_vectors(m, kind == KIND_BICG ? 7 : 5)) != FL_OK) return st;
     Work w{};
     for (int k = 0; k < 7; ++k) w.vec[k] = m->vec[k];
     w.partials = m->partials;
     w.st = m->st;
     return solve_on(m, kind, rhs, x_inout, opts, rep, d, w);
+}
+
+// FP32 copies of the values and the Jacobi inverse, and the FP32 vectors.
+fl_status ensure_mixed(fl_csr *m) {
+    fl_ctx *ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    const unsigned G = m->blocks;
+    if (!m->val32) {
+        FL_HIP_TRY(ctx, hipMalloc(&m->val32, (size_t)std::max<int64_t>(m->nnz, 1) * sizeof(float)));
+        k_to_f32<<<G, kBlock, 0, s>>>(m->nnz, m->val, m->val32);
+        FL_HIP_TRY(ctx, hipGetLastError());
+    }
+    if (!m->inv32) {
+        FL_HIP_TRY(ctx, hipMalloc(&m->inv32, (size_t)std::max<uint32_t>(m->n, 1) * sizeof(float)));
+        k_to_f32<<<G, kBlock, 0, s>>>(m->n, m->inv, m->inv32);
+        FL_HIP_TRY(ctx, hipGetLastError());
+    }
+    for (float *&v : m->vec32)
+        if (!v) FL_HIP_TRY(ctx, hipMalloc(&v, (size_t)m->n * sizeof(float) + 16));
+    if (!m->os) FL_HIP_TRY(ctx, hipMalloc(&m->os, sizeof(OuterState)));
+    if (!m->os_host && hipHostMalloc(&m->os_host, sizeof(OuterState), hipHostMallocDefault) != hipSuccess)
+        return set_error(ctx, FL_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
+    return FL_OK;
+}
+
+// The FP32 solver inside FP64 iterative refinement (see OuterState).  The
+// report counts the inner iterations of every round; an inner solve that
+// breaks down, or a round that does not lower the FP64 residual, hands the
+// solve over to the FP64 solver from the current x (fallback = 1) with the
+// same threshold and the rest of the iteration budget.
+fl_status solve_mixed(fl_csr *m, int kind, const double *rhs, double *x_inout, const fl_lin_opts *opts,
+                      fl_lin_report *rep, fl_err_detail *d) {
+    fl_ctx *ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    fl_status st = ensure_vectors(m, 3);
+    if (st == FL_OK) st = ensure_mixed(m);
+    if (st != FL_OK) return st;
+    const size_t bytes = (size_t)m->n * sizeof(double);
+    double *x = m->vec[0], *b = m->vec[1], *q64 = m->vec[2];
+    float *r = m->vec32[0], *dp = m->vec32[1], *qv = m->vec32[2], *e = m->vec32[3], *tv = m->vec32[4],
+          *rhat = m->vec32[5];
+    FL_HIP_TRY(ctx, hipMemcpyAsync(x, x_inout, bytes, hipMemcpyHostToDevice, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(b, rhs, bytes, hipMemcpyHostToDevice, s));
+    OuterState o{};
+    o.atol = opts->absolute_tolerance;
+    o.rtol = opts->relative_tolerance;
+    *m->os_host = o;
+    FL_HIP_TRY(ctx, hipMemcpyAsync(m->os, m->os_host, sizeof(OuterState), hipMemcpyHostToDevice, s));
+    const Mat A{m->off, m->col, m->val, m->n};
+    const MatT<float> A32{m->off, m->col, m->val32, m->n};
+    const float *inv = kind == KIND_PCG ? m->inv32 : nullptr;
+    const unsigned G = m->blocks;
+    double *const pa = m->partials, *const pb = m->partials + (size_t)G * 4;
+    LinState *const dst = m->st;
+    const long long budget = opts->max_iterations;
+    long long used = 0;
+    bool fallback = false;
+    FL_HIP_TRY(ctx, hipEventRecord(m->e0, s));
+    for (;;) {
+        LinState init{};
+        init.in_scale = 1.0;
+        init.max_iter = std::max<long long>(budget - used, 0);
+        init.done_seq = kRunning;
+        init.bad_row = init.bad_row2 = init.bad_r = init.bad_z = ~0ull;
+        *m->st_host = init;
+        FL_HIP_TRY(ctx, hipMemcpyAsync(dst, m->st_host, sizeof(LinState), hipMemcpyHostToDevice, s));
+        k_spmv<<<G, kBlock, 0, s>>>(A, x, q64, dst);
+        k_res_mixed<<<G, kBlock, 0, s>>>(m->n, b, q64, r, e, pb, dst);
+        k_outer_test<<<1, kBlock, 0, s>>>(pb, G, m->os, dst, m->inner_rel, used >= budget ? 1 : 0);
+        long long seq = 2;
+        if (kind == KIND_BICG) {
+            k_bi_init<<<G, kBlock, 0, s>>>(m->n, r, rhat, dp, qv, pb, G, pa, dst, seq++,
+                                           (float *)nullptr);
+        } else {
+            k_cg_init_d<<<G, kBlock, 0, s>>>(m->n, r, inv, dp, pb, G, pa, dst, seq++,
+                                             (float *)nullptr);
+            k_cg_init_rho<<<1, kBlock, 0, s>>>(pa, G, dst, seq++);
+        }
+        FL_HIP_TRY(ctx, hipGetLastError());
+        int batch = 4;
+        long long it = 1;
+        for (;;) {
+            FL_HIP_TRY(ctx, hipMemcpyAsync(m->st_host, dst, sizeof(LinState), hipMemcpyDeviceToHost, s));
+            FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+            if (m->st_host->done_seq != kRunning) break;
+            for (int k = 0; k < batch; ++k, ++it) {
+                if (kind == KIND_BICG) {
+                    k_bi_p<<<G, kBlock, 0, s>>>(m->n, (const float *)r, dp, qv, pa, G, dst, it, seq++);
+                    k_bi_spmv_v<<<G, kBlock, 0, s>>>(A32, dp, rhat, qv, pa, dst, seq++);
+                    k_bi_spmv_t<<<G, kBlock, 0, s>>>(A32, r, qv, tv, pa, G, pb, dst, it, seq++);
+                    k_bi_update<<<G, kBlock, 0, s>>>(m->n, e, r, dp, qv, tv, rhat, pb, G, pa, dst, it, seq++);
+                } else {
+                    k_cg_spmv<<<G, kBlock, 0, s>>>(A32, dp, qv, pa, dst, seq++);
+                    k_cg_update<<<G, kBlock, 0, s>>>(m->n, e, r, dp, qv, inv, pa, G, pb, dst, it, seq++);
+                    k_cg_dir<<<G, kBlock, 0, s>>>(m->n, (const float *)r, inv, dp, pb, G, dst, it, seq++);
+                }
+            }
+            FL_HIP_TRY(ctx, hipGetLastError());
+            batch = std::min(batch * 2, 64);
+        }
+        FL_HIP_TRY(ctx, hipMemcpyAsync(m->os_host, m->os, sizeof(OuterState), hipMemcpyDeviceToHost, s));
+        FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+        const LinState h = *m->st_host;
+        const OuterState oh = *m->os_host;
+        if (oh.converged || oh.exhausted) break;
+        if (oh.stalled) {
+            fallback = true;
+            break;
+        }
+        if (h.status != FL_OK) {
+            // the FP64 residual's own faults are the caller's errors (the
+            // initial tests, launch 2); an FP32 breakdown after the inner
+            // residual has fallen keeps its correction and restarts with a
+            // new round; anything else hands over to FP64
+            if (h.done_seq == 2) return state_error(m, h, d);
+            if (!(h.status == FL_ERR_BREAKDOWN && h.rnorm < h.initial)) {
+                fallback = true;
+                break;
+            }
+            used += std::max<long long>(h.iter, 1);
+        } else {
+            used += h.converged ? h.iter : h.max_iter;
+        }
+        k_add_e<<<G, kBlock, 0, s>>>(m->n, x, e, m->os);
+        FL_HIP_TRY(ctx, hipGetLastError());
+    }
+    FL_HIP_TRY(ctx, hipEventRecord(m->e1, s));
+    FL_HIP_TRY(ctx, hipMemcpyAsync(x_inout, x, bytes, hipMemcpyDeviceToHost, s));
+    FL_HIP_TRY(ctx, hipStreamSynchronize(s));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, m->e0, m->e1);
+    const OuterState oh = *m->os_host;
+    if (fallback) {
+        // FP64 from the current x, to the original threshold
+        fl_lin_opts o64{oh.threshold, 0.0, std::max<long long>(budget - used, 0)};
+        fl_lin_report r64{};
+        if ((st = ensure_vectors(m, kind == KIND_BICG ? 7 : 5)) != FL_OK) return st;
+        Work w{};
+        for (int k = 0; k < 7; ++k) w.vec[k] = m->vec[k];
+        w.partials = m->partials;
+        w.st = m->st;
+        if ((st = solve_on(m, kind, rhs, x_inout, &o64, &r64, d, w)) != FL_OK) return st;
+        if (rep) {
+            rep->converged = r64.converged;
+            rep->iterations = used + r64.iterations;
+            rep->initial_residual = oh.initial;
+            rep->final_residual = r64.final_residual;
+            rep->relative_residual = oh.initial == 0.0 ? 0.0 : r64.final_residual / oh.initial;
+            rep->device_ms = ms + r64.device_ms;
+            rep->refinements = (int32_t)oh.round;
+            rep->fallback = 1;
+        }
+        return FL_OK;
+    }
+    if (rep) {
+        rep->converged = oh.converged;
+        rep->iterations = oh.converged ? used : budget;
+        rep->initial_residual = oh.initial;
+        rep->final_residual = oh.rnorm;
+        rep->relative_residual = oh.initial == 0.0 ? 0.0 : oh.rnorm / oh.initial;
+        rep->device_ms = ms;
+        rep->refinements = (int32_t)oh.round;
+        rep->fallback = 0;
+    }
+    if (d) set_detail(d, FL_OK, "");
+    return FL_OK;
 }
 
 }  // namespace
@@ -985,6 +1279,17 @@ fl_status fl_csr_upload(fl_ctx *ctx, int64_t nrows, int64_t ncols, const int64_t
     ctx_retain(ctx);
     *out = m;
     if (d) set_detail(d, FL_OK, "");
+    return FL_OK;
+}
+
+fl_status fl_csr_set_precision(fl_csr *m, int32_t precision, double inner_tolerance) {
+    if (!m) return FL_ERR_INVALID_ARGUMENT;
+    if (precision != FL_PRECISION_FP64 && precision != FL_PRECISION_MIXED)
+        return set_error(m->ctx, FL_ERR_INVALID_ARGUMENT, "precision must be FL_PRECISION_FP64 or FL_PRECISION_MIXED");
+    if (!std::isfinite(inner_tolerance) || inner_tolerance < 0.0 || inner_tolerance >= 1.0)
+        return set_error(m->ctx, FL_ERR_INVALID_ARGUMENT, "inner tolerance must lie in [0, 1)");
+    m->precision = precision;
+    m->inner_rel = inner_tolerance > 0.0 ? inner_tolerance : 1e-5;
     return FL_OK;
 }
 

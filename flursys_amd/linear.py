@@ -56,6 +56,8 @@ class LinearSolveReport:
     final_residual: float
     relative_residual: float
     device_ms: float
+    refinements: int = 0  # mixed precision: FP64 residual rounds
+    fallback: bool = False  # mixed precision: finished on the FP64 solver
 
     def converged(self) -> bool:
         return self.status == "Converged"
@@ -117,6 +119,13 @@ class CsrMatrix:
         except Exception:
             pass
 
+    def set_precision(self, precision: int, inner_tolerance: float = 0.0):
+        """FL_PRECISION_FP64 (the reference's solvers) or FL_PRECISION_MIXED
+        (FP32 solver inside FP64 iterative refinement) for this matrix's
+        cg / pcg / bicgstab (no counterpart in the reference)."""
+        _lib.check(load().fl_csr_set_precision(self.handle, int(precision), float(inner_tolerance)),
+                   self.ctx.handle)
+
     def spmv(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
         if x.size != self.ncols:
@@ -149,7 +158,7 @@ def _solve(fn, matrix: CsrMatrix, rhs, solution, options):
         _raise(st, det, matrix.ctx.handle)
     return LinearSolveReport("Converged" if rep.converged else "MaxIterations", rep.iterations,
                              rep.initial_residual, rep.final_residual, rep.relative_residual,
-                             rep.device_ms)
+                             rep.device_ms, rep.refinements, bool(rep.fallback))
 
 
 def cg(matrix, rhs, solution, options=None) -> LinearSolveReport:

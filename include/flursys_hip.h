@@ -390,15 +390,19 @@ typedef struct {
 
 /* LinearSolveReport (src/linear/solver.rs:29-41): converged = 1 for
  * LinearSolverStatus::Converged, 0 for MaxIterations (a status, not an
- * error).  device_ms is the solve's device time (HIP events, no copies). */
+ * error).  device_ms is the solve's device time (HIP events, no copies).
+ * The mixed path (fl_csr_set_precision) reports its inner iterations over all
+ * rounds, the FP64 refinement rounds and whether it handed over to FP64. */
 typedef struct {
     int32_t converged;
-    int32_t reserved;
+    int32_t refinements;      /* FP64 residual rounds (mixed path; 0 for FP64) */
     int64_t iterations;
     double initial_residual;
     double final_residual;
     double relative_residual;
     double device_ms;
+    int32_t fallback;         /* 1: the mixed solve finished on the FP64 solver */
+    int32_t reserved;
 } fl_lin_report;
 
 /* The fields of LinearAlgebraError (src/linear/mod.rs:17-26) for the status
@@ -432,6 +436,22 @@ fl_status fl_cg(fl_csr *matrix, const double *rhs, double *x_inout, const fl_lin
  * src/linear/preconditioner.rs:9-28). */
 fl_status fl_pcg_jacobi(fl_csr *matrix, const double *rhs, double *x_inout,
                         const fl_lin_opts *opts, fl_lin_report *report, fl_err_detail *detail);
+/* No counterpart in the reference.  The arithmetic of the matrix's
+ * fl_cg / fl_pcg_jacobi / fl_bicgstab solves: FL_PRECISION_FP64 (the
+ * default, the reference's solvers) or FL_PRECISION_MIXED, the same solver in
+ * FP32 (values, vectors, SpMV and axpys; dots, alpha/beta and its tests in
+ * FP64) on the correction of A e = r inside FP64 iterative refinement: every
+ * round forms the true FP64 residual and makes the reference's stop test
+ * (||r||_2 <= atol + rtol ||r_0||_2) on it, so an accepted solution meets the
+ * FP64 solver's tolerance.  An inner solve ends when its residual is below
+ * half that threshold plus inner_tolerance times the round's ||r|| (0
+ * selects 1e-5; the round's system is scaled to a unit residual); an FP32
+ * breakdown after progress starts a new round, while a diverging or
+ * non-finite inner solve, or a round that does not halve the FP64 residual,
+ * hands the solve over to the FP64 solver from the current iterate
+ * (report.fallback).  fl_pcg_jacobi_noalloc and the device callers
+ * (SIMPLE) always solve in FP64. */
+fl_status fl_csr_set_precision(fl_csr *matrix, int32_t precision, double inner_tolerance);
 /* bicgstab (src/linear/solver.rs:155-256). */
 fl_status fl_bicgstab(fl_csr *matrix, const double *rhs, double *x_inout,
                       const fl_lin_opts *opts, fl_lin_report *report, fl_err_detail *detail);

@@ -289,3 +289,101 @@ def test_caller_shaped_systems_match_oracle(gpu_ctx):
         B = sp.csr_matrix((A.vals, A.cols, A.offsets), shape=(n, n))
         assert np.linalg.norm(b - B @ x) <= 2e-10 * np.linalg.norm(b)
         assert np.abs(x - x_o).max() <= 1e-8 * np.abs(x_o).max()
+
+
+# ------------------------------------- FP32 solver inside FP64 refinement
+@pytest.mark.parametrize("kind", ["cg", "pcg", "bicgstab"])
+def test_mixed_precision_meets_the_fp64_tolerance(gpu_ctx, kind):
+    """FL_PRECISION_MIXED on the 100x100 jittered triangulation (20,000
+    cells): the FP32 solver inside FP64 refinement converges to the same
+    stop test as the FP64 solver -- the true residual, recomputed with scipy,
+    meets rel 1e-10 -- without handing over to FP64; its solution agrees with
+    the FP64 solver's within 1e-8 of max|x| and it needs at most twice the
+    FP64 iterations."""
+    import flursys_amd as fa
+
+    print(f"[gpu] mixed-precision {kind} on the 100x100 mesh", flush=True)
+    mesh, rhs = mesh_system(100)
+    n = len(rhs)
+    if kind == "bicgstab":
+        B = skew(mesh)
+        off, cols, vals = B.indptr, B.indices, B.data
+    else:
+        off, cols, vals = mesh["offsets"], mesh["cols"], mesh["vals"]
+    S = sp.csr_matrix((vals, cols, off), shape=(n, n))
+    o = FL.LinearSolverOptions(0.0, 1e-10, 20000)
+    A64 = FL.CsrMatrix(n, n, off, cols, vals, gpu_ctx)
+    x64 = np.zeros(n)
+    r64 = getattr(FL, kind)(A64, rhs, x64, o)
+    A64.close()
+    A = FL.CsrMatrix(n, n, off, cols, vals, gpu_ctx)
+    A.set_precision(fa.PRECISION_MIXED)
+    x = np.zeros(n)
+    r = getattr(FL, kind)(A, rhs, x, o)
+    A.close()
+    print(f"[gpu]   FP64 {r64.iterations} its; mixed {r.iterations} its in {r.refinements} rounds, "
+          f"fallback {r.fallback}", flush=True)
+    assert r.converged() and not r.fallback and r.refinements >= 2
+    true = np.linalg.norm(rhs - S @ x) / np.linalg.norm(rhs)
+    assert true <= 1e-10 * (1 + 1e-6), true
+    assert r.final_residual <= 1e-10 * r.initial_residual
+    assert np.max(np.abs(x - x64)) <= 1e-8 * np.max(np.abs(x64))
+    assert r.iterations <= 2 * r64.iterations
+
+
+def test_mixed_precision_known_answers_and_errors(gpu_ctx, kat):
+    """The reference's exact small systems solve in mixed mode too, and the
+    errors keep their kinds: a NaN right-hand side is rejected before the
+    device, the zero matrix breaks down in FP32 and again in the FP64 solver
+    it hands over to ("zero CG search denominator")."""
+    import flursys_amd as fa
+
+    o = opts(kat)
+    a = np.array(kat["spd"]["a"])
+    ex = np.array(kat["spd"]["x"])
+    A = upload(gpu_ctx, O.Csr.from_dense(a))
+    A.set_precision(fa.PRECISION_MIXED)
+    for fn in (FL.cg, FL.pcg):
+        x = np.zeros(2)
+        r = fn(A, a @ ex, x, o)
+        assert r.converged() and np.allclose(x, ex, atol=1e-10, rtol=0)
+        assert r.final_residual <= o.absolute_tolerance + o.relative_tolerance * r.initial_residual
+    a3 = np.array(kat["nonsymmetric"]["a"])
+    ex3 = np.array(kat["nonsymmetric"]["x"])
+    B = upload(gpu_ctx, O.Csr.from_dense(a3))
+    B.set_precision(fa.PRECISION_MIXED)
+    x = np.zeros(3)
+    rb = FL.bicgstab(B, a3 @ ex3, x, o)
+    assert rb.converged() and np.allclose(x, ex3, atol=1e-10, rtol=0)
+    with pytest.raises(FL.LinearAlgebraError) as e:
+        FL.cg(A, [np.nan, 1.0], np.zeros(2))
+    assert e.value.kind == "NonFiniteValue"
+    with pytest.raises(fa.FlursysError):
+        A.set_precision(7)
+    Z = FL.CsrMatrix(1, 1, [0, 1], [0], [0.0], gpu_ctx)
+    Z.set_precision(fa.PRECISION_MIXED)
+    with pytest.raises(FL.LinearAlgebraError) as e:
+        FL.cg(Z, [1.0], np.zeros(1))
+    assert e.value.kind == "Breakdown" and e.value.context == "zero CG search denominator"
+
+
+def test_mixed_precision_hands_over_when_fp32_overflows(gpu_ctx):
+    """A system whose residual exceeds the FP32 range (entries 1e36, solution
+    entries up to 1e3): the inner FP32 solve cannot represent it, the solve
+    hands over to the FP64 solver (fallback) and meets the same tolerance as
+    the FP64 solver alone."""
+    import flursys_amd as fa
+
+    mesh, _ = mesh_system(20)
+    n = len(mesh["volumes"])
+    vals = mesh["vals"] * 1e36
+    S = sp.csr_matrix((vals, mesh["cols"], mesh["offsets"]), shape=(n, n))
+    xt = np.linspace(-1e3, 1e3, n)
+    xt[0] = 0.0
+    rhs = S @ xt
+    A = FL.CsrMatrix(n, n, mesh["offsets"], mesh["cols"], vals, gpu_ctx)
+    A.set_precision(fa.PRECISION_MIXED)
+    x = np.zeros(n)
+    r = FL.pcg(A, rhs, x, FL.LinearSolverOptions(0.0, 1e-10, 20000))
+    assert r.fallback and r.converged()
+    assert np.linalg.norm(rhs - S @ x) / np.linalg.norm(rhs) <= 1e-10 * (1 + 1e-6)
