@@ -936,14 +936,17 @@ __device__ inline void publish_row(double (&s)[NS], double (&m)[NM], double *pou
 // max|r|} (z = D^-1 r), the "update" partials the first iteration of an
 // inner solve folds.  For the mixed path the residual is computed in FP64
 // and stored as FP32, with the partials taken from the stored values.
-template <typename TR>
+//
+// kZeroX: x is known to be zero (a zero initial guess), so no x is read;
+// A 0 is formed from zeros in the same order, and r = -rhs bit for bit.
+template <typename TR, bool kZeroX>
 __global__ void __launch_bounds__(kZT) k_residual_z(Grid g, const double *__restrict__ rhs,
                                                     const double *__restrict__ x, TR *__restrict__ r,
                                                     double *__restrict__ pout) {
     const TR *inv_tab = inv_table<TR>(g);
     double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
     zmarch_side(
-        g.nx, g.ny, g.nz, [&](uint32_t c, uint32_t, uint32_t, uint32_t) -> double { return x[c]; },
+        g.nx, g.ny, g.nz, [&](uint32_t c, uint32_t, uint32_t, uint32_t) -> double { return kZeroX ? 0.0 : x[c]; },
         [&](double v, uint32_t, uint32_t, uint32_t, uint32_t) -> double { return v; },
         [&](uint32_t c) -> double { return rhs[c]; },
         [&](uint32_t i, uint32_t j, uint32_t k, uint32_t c, const double (&v)[7], double rh) {
@@ -970,7 +973,7 @@ __global__ void __launch_bounds__(kZT) k_residual_z(Grid g, const double *__rest
 // the vector-memory instructions (the single-cell march is bound by the
 // texture-address unit); only the pair's outer x neighbours are 8-byte
 // loads.  Arithmetic, in the same order as k_residual_z, gives the same r.
-template <typename TR>
+template <typename TR, bool kZeroX>
 __global__ void __launch_bounds__(kZT) k_residual_z2(Grid g, const double *__restrict__ rhs,
                                                      const double *__restrict__ x, TR *__restrict__ r,
                                                      double *__restrict__ pout) {
@@ -985,14 +988,17 @@ __global__ void __launch_bounds__(kZT) k_residual_z2(Grid g, const double *__res
         const uint32_t k1 = min(k0 + kz, nz);
         const bool xw = i0 > 0, xe = i0 + 2 < nx, ys = j > 0, yn = j + 1 < ny;
         const double ix2 = g.ix2, iy2 = g.iy2, iz2 = g.iz2;
+        const double2 z2 = make_double2(0.0, 0.0);
+        auto ld2 = [&](uint32_t a) -> double2 { return kZeroX ? z2 : *(const double2 *)(x + a); };
+        auto ld1 = [&](uint32_t a) -> double { return kZeroX ? 0.0 : x[a]; };
         uint32_t c = i0 + nx * (j + ny * k0);
-        double2 xc = *(const double2 *)(x + c);
-        double2 xm = k0 > 0 ? *(const double2 *)(x + c - nxy) : xc;
+        double2 xc = ld2(c);
+        double2 xm = k0 > 0 ? ld2(c - nxy) : xc;
         for (uint32_t k = k0; k < k1; ++k, c += nxy) {
-            const double2 xp = k + 1 < nz ? *(const double2 *)(x + c + nxy) : xc;
-            const double2 xn = yn ? *(const double2 *)(x + c + nx) : xc;
-            const double2 xs = ys ? *(const double2 *)(x + c - nx) : xc;
-            const double xe1 = xe ? x[c + 2] : xc.y, xw0 = xw ? x[c - 1] : xc.x;
+            const double2 xp = k + 1 < nz ? ld2(c + nxy) : xc;
+            const double2 xn = yn ? ld2(c + nx) : xc;
+            const double2 xs = ys ? ld2(c - nx) : xc;
+            const double xe1 = xe ? ld1(c + 2) : xc.y, xw0 = xw ? ld1(c - 1) : xc.x;
             const double2 rh = *(const double2 *)(rhs + c);
             const double ax0 = ix2 * ((xc.x - xc.y) + (xc.x - xw0)) + iy2 * ((xc.x - xn.x) + (xc.x - xs.x)) +
                                iz2 * ((xc.x - xp.x) + (xc.x - xm.x));
@@ -2422,6 +2428,9 @@ struct fl_cavity3d {
     // inner FCG solve of each pressure solve, on level cap_level (0: off)
     int cap_level = 0, cap_iter = 0;
     bool cap_taken = false;
+    // p is all zeros (a zero initial guess, nothing written since): the next
+    // residual is r = b and reads no p
+    bool p_zero = false;
     float *cap = nullptr;
     double *pz = nullptr;  // partials z.q_old, sum q_old
     FcgA *fa = nullptr;
@@ -2760,15 +2769,27 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
 template <typename TR>
 void launch_residual(fl_cavity3d *s, TR *r) {
     const dim3 grid(s->zr.bx, s->zr.by, s->zr.bz);
-    if (s->g.nx % 2 == 0)
-        k_residual_z2<TR><<<grid, kZT, 0, s->ctx->stream>>>(s->g, s->rhs, s->p, r, s->pb);
-    else
-        k_residual_z<TR><<<grid, kZT, 0, s->ctx->stream>>>(s->g, s->rhs, s->p, r, s->pb);
+    const bool z = s->p_zero;
+    s->p_zero = false;
+    hipStream_t stm = s->ctx->stream;
+    if (s->g.nx % 2 == 0) {
+        if (z) k_residual_z2<TR, true><<<grid, kZT, 0, stm>>>(s->g, s->rhs, s->p, r, s->pb);
+        else k_residual_z2<TR, false><<<grid, kZT, 0, stm>>>(s->g, s->rhs, s->p, r, s->pb);
+    } else {
+        if (z) k_residual_z<TR, true><<<grid, kZT, 0, stm>>>(s->g, s->rhs, s->p, r, s->pb);
+        else k_residual_z<TR, false><<<grid, kZT, 0, stm>>>(s->g, s->rhs, s->p, r, s->pb);
+    }
+}
+
+// Bytes of the next launch_residual: rhs, r and x unless x is known zero.
+template <typename TR>
+double residual_bytes(const fl_cavity3d *s) {
+    return (8.0 + sizeof(TR) + (s->p_zero ? 0.0 : 8.0)) * s->n;
 }
 
 fl_status solve_fp64(fl_cavity3d *s, fl_solve_report *rep, double rr0_ref = 0.0) {
     fl_ctx *ctx = s->ctx;
-    FL_TIMED(s, K_RESIDUAL_D, -1, 24.0 * s->n,
+    FL_TIMED(s, K_RESIDUAL_D, -1, residual_bytes<double>(s),
              launch_residual<double>(s, s->r));
     FL_HIP_TRY(ctx, hipGetLastError());
     const CgParams prm{s->cfg.pressure_tolerance, s->cfg.pressure_max_iters, s->cfg.stop_rule,
@@ -3096,7 +3117,7 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
         warm_used = 0;
     double warm_ratio = 0.0;
     for (;;) {
-        FL_TIMED(s, K_RESIDUAL_F, -1, 20.0 * s->n,
+        FL_TIMED(s, K_RESIDUAL_F, -1, residual_bytes<float>(s),
                  launch_residual<float>(s, s->r32));
         FL_TIMED(s, K_FOLD_OUTER, -1, 32.0 * s->zr.count,
                  k_fold_outer<<<1, kBlock, 0, stm>>>(s->pb, s->zr.count, s->outer));
@@ -3226,6 +3247,7 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
 fl_status solve_sor(fl_cavity3d *s, fl_solve_report *rep) {
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
+    s->p_zero = false;  // the sweeps write p before each residual
     const SorState zero{};
     FL_HIP_TRY(ctx, hipMemcpyAsync(s->sor, &zero, sizeof(SorState), hipMemcpyHostToDevice, stm));
     const uint32_t half = ((s->g.nx + 1) / 2) * s->g.ny * s->g.nz;
@@ -3268,6 +3290,7 @@ fl_status solve_pressure(fl_cavity3d *s, fl_solve_report *rep) {
     else
         FL_HIP_TRY(ctx, hipMemsetAsync(s->p, 0, sizeof(double), stm));  // pin p(0,0,0) = 0
     s->cap_taken = false;
+    s->p_zero = s->cfg.initial_guess == FL_GUESS_ZERO;
     if (s->cfg.pressure_solver == FL_SOLVER_SOR) return solve_sor(s, rep);
     return s->cfg.precision == FL_PRECISION_MIXED ? solve_mixed(s, rep) : solve_fp64(s, rep);
 }
