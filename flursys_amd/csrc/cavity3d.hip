@@ -302,8 +302,11 @@ __device__ inline const float *lvl_inv_table(const Level &L) {
 }
 
 // f on level 0: the residual with the compatibility value at cell 0.
+// The load is unconditional (then a select): a load under a branch would
+// make the marches wait for every outstanding access where the paths join.
 __device__ inline float f_at(const float *f, uint32_t c, bool level0, float f0) {
-    return (level0 && c == 0) ? f0 : f[c];
+    const float v = f[c];
+    return (level0 && c == 0) ? f0 : v;
 }
 
 // The first two damped-Jacobi sweeps from a zero iterate at cell c of a
@@ -902,7 +905,8 @@ __global__ void __launch_bounds__(kRX) k_mg_restrict_z(Level L, Level LC, const 
         const float ap = e_ld(e, cp, iS), bp = e_ld(e, cp + nx, iS);
         const float ae = e_ld(e, c + de, iS), aw = e_ld(e, c - dw, iS), be = e_ld(e, c + nx + de, iS), bw = e_ld(e, c + nx - dw, iS);
         const float as = e_ld(e, c - ds, iS), bn = e_ld(e, c + nx + dn, iS);
-        const float fa = (kL0 && c == 0) ? f0 : f[c], fb = f[c + nx];
+        const float fr = f[c], fb = f[c + nx];
+        const float fa = (kL0 && c == 0) ? f0 : fr;
         const float ra = fa - (cx * ((ac - ae) + (ac - aw)) + cy * ((ac - bc) + (ac - as)) +
                                cz * ((ac - ap) + (ac - am)));
         const float rb = fb - (cx * ((bc - be) + (bc - bw)) + cy * ((bc - bn) + (bc - ac)) +
@@ -1959,6 +1963,207 @@ __global__ void __launch_bounds__(kBlock) k_fcg_dirapply_t(Grid g, const e16 *__
     publish_row<2, 1>(s, m, pout);
 }
 
+// ------------------------------------------------ level-0 plane-tile marches
+// The level-0 stencil passes of the multigrid cycle on plane tiles (FP32
+// operand, four cells per lane, nx a multiple of 4), k_fcg_dirapply_t's
+// layout as a helper: a block of 4 waves owns a 256 x 4 (x, y) tile over a
+// chunk of planes; each plane's operand goes into an LDS buffer with a
+// one-cell halo (rows below and above by waves 0 and 1, the cells west and
+// east of each row by threads 0-7; at a wall the edge cell's own value, the
+// clamped neighbour), the in-plane neighbours come from LDS and the z
+// neighbours from registers.  Two buffers alternate, one barrier per plane;
+// the loads of plane k + 1 are in flight while plane k is computed.  A cell
+// then costs one load of each operand array instead of five.
+struct Quad {
+    float a[4];
+};
+struct Tile4 {
+    uint32_t i0, j;     // own quad: cells (i0 .. i0 + 3, j)
+    uint32_t jh;        // halo row (waves 0 and 1)
+    uint32_t ix, jx;    // halo cell (threads 0 .. 2 kTileRows - 1)
+    bool own, hrow, xh;
+    uint32_t co, ch, cxh;  // their plane-0 offsets (a lane without one reads its own, or cell 0)
+    uint32_t k0, k1;       // the chunk of planes; k0 >= nz: nothing to do
+};
+__device__ inline Tile4 tile4_geometry(uint32_t nx, uint32_t ny, uint32_t nz) {
+    constexpr uint32_t V = 4, W = 64 * V, TY = kTileRows;
+    const uint3 b = zblock();
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6, t = threadIdx.x;
+    const uint32_t tx0 = b.x * W, ty0 = b.y * TY;
+    Tile4 T;
+    T.i0 = tx0 + lane * V;
+    T.j = ty0 + wv;
+    const uint32_t kz = (nz + gridDim.z - 1) / gridDim.z;
+    T.k0 = b.z * kz;
+    T.k1 = min(T.k0 + kz, nz);
+    T.own = T.i0 < nx && T.j < ny;
+    T.jh = wv == 0 ? ty0 - 1 : ty0 + TY;
+    T.hrow = wv < 2 && T.i0 < nx && (wv == 0 ? ty0 > 0 : ty0 + TY < ny);
+    const bool east = t >= TY;
+    T.jx = ty0 + (east ? t - TY : t);
+    T.ix = east ? tx0 + W : tx0 - 1;
+    T.xh = t < 2 * TY && T.jx < ny && (east ? tx0 + W < nx : tx0 > 0);
+    T.co = T.own ? T.i0 + nx * T.j : 0u;
+    T.ch = T.hrow ? T.i0 + nx * T.jh : T.co;
+    T.cxh = T.xh ? T.ix + nx * T.jx : T.co;
+    return T;
+}
+// omega D^-1 of level 0 for the tile's cells, [0] on a wall plane, [1] on an
+// interior one (tab: the level's reciprocal diagonals in LDS)
+struct TileW {
+    float o[2][4], h[2][4], x[2];
+};
+__device__ inline TileW tile4_weights(const Level &L, const Tile4 &T, const float *tab) {
+    auto wgt = [&](uint32_t i, uint32_t j, int zin) -> float {
+        const uint32_t a = (uint32_t)(i > 0 && i + 1 < L.nx) | ((uint32_t)(j > 0 && j + 1 < L.ny) << 1) |
+                           ((uint32_t)zin << 2);
+        return kOmega * tab[a];
+    };
+    TileW w;
+#pragma unroll
+    for (int z = 0; z < 2; ++z) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            w.o[z][v] = wgt(T.i0 + v, T.j, z);
+            w.h[z][v] = wgt(T.i0 + v, T.jh, z);
+        }
+        w.x[z] = wgt(T.ix, T.jx, z);
+    }
+    return w;
+}
+// load(Ld &, own offset, halo-row offset, halo-cell offset) issues a plane's
+// loads; form(const Ld &, k, own Quad &, halo Quad &, halo float &, Sd &)
+// makes the operand of plane k (and side values of the own cells, kept for
+// the body); body(k, c, below, centre, above, row, side) computes plane k:
+// c is the own quad's first cell, `row` its LDS slot (row[-1], row[4] the x
+// neighbours outside the quad, row[-S + v], row[S + v] the y neighbours).
+template <typename Ld, typename Sd, typename LoadF, typename FormF, typename BodyF>
+__device__ inline void tile4_march(const Tile4 &T, uint32_t nx, uint32_t ny, uint32_t nz, float *lds, LoadF &&load,
+                                   FormF &&form, BodyF &&body) {
+    constexpr int V = 4, W = 64 * V, TY = kTileRows, OFF = V, S = W + 2 * V, BUF = (TY + 2) * S;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6, t = threadIdx.x, nxy = nx * ny;
+    const uint32_t k0 = T.k0, k1 = T.k1;
+    const bool east = t >= TY;
+    float *const own_slot = lds + (1 + wv) * S + OFF + lane * V;
+    float *const h_slot = lds + (wv == 0 ? 0 : TY + 1) * S + OFF + lane * V;
+    float *const x_slot = lds + (1 + (east ? t - TY : t)) * S + (east ? OFF + W : OFF - 1);
+    const bool wl = T.i0 == 0, el = T.i0 + V == nx, sl = T.j == 0, nl = T.j + 1 == ny;
+    Ld L{};
+    auto ld = [&](uint32_t k) {
+        const uint32_t pk = nxy * k;
+        const uint32_t o = T.own ? pk : 0u, h = T.hrow ? pk : o, q = T.xh ? pk : o;
+        load(L, T.co + o, T.ch + h, T.cxh + q);
+    };
+    auto put = [&](uint32_t k, int bf, Quad &out, Sd &sd) {
+        Quad hv;
+        float xv;
+        form(L, k, out, hv, xv, sd);
+        if (T.own) {
+            float *const ob = own_slot + bf * BUF;
+            *(float4 *)ob = *(const float4 *)&out;
+            if (wl) ob[-1] = out.a[0];
+            if (el) ob[V] = out.a[V - 1];
+            if (sl) *(float4 *)(ob - S) = *(const float4 *)&out;
+            if (nl) *(float4 *)(ob + S) = *(const float4 *)&out;
+        }
+        if (T.hrow) *(float4 *)(h_slot + bf * BUF) = *(const float4 *)&hv;
+        if (T.xh) x_slot[bf * BUF] = xv;
+    };
+    Quad pm{}, pc{}, pn{};
+    Sd sc{}, sn{};
+    if (k0 > 0) {  // the plane below the chunk: own values only
+        Quad hv;
+        float xv;
+        Sd sd;
+        ld(k0 - 1);
+        form(L, k0 - 1, pm, hv, xv, sd);
+    }
+    ld(k0);
+    put(k0, 0, pc, sc);
+    if (k0 == 0) pm = pc;
+    if (k0 + 1 < nz) ld(k0 + 1);
+    __syncthreads();
+    uint32_t c = T.co + nxy * k0;
+    for (uint32_t k = k0; k < k1; ++k, c += nxy) {
+        const int bf = (k - k0) & 1;
+        if (k + 1 < nz) {
+            put(k + 1, bf ^ 1, pn, sn);
+            if (k + 2 < nz && k + 1 < k1) ld(k + 2);
+        } else {
+            pn = pc;
+        }
+        if (T.own) body(k, c, pm, pc, pn, (const float *)(own_slot + bf * BUF), sc);
+        __syncthreads();
+        pm = pc;
+        pc = pn;
+        sc = sn;
+    }
+}
+constexpr int kTileS = 64 * 4 + 2 * 4;  // LDS row stride of the float tiles
+
+// k_mg_check_z as a plane-tile march: the stop test, then e1 = omega D^-1 f
+// formed per plane (f with the compatibility value at cell 0) and
+// e2 = e1 + omega D^-1 (f - A e1) from it, k_mg_check_z's arithmetic.
+__global__ void __launch_bounds__(kBlock) k_mg_check_t(Level L, CgParams prm, const float *__restrict__ r,
+                                                       e16 *__restrict__ eout, const double *__restrict__ pin,
+                                                       uint32_t rows, FcgA *SA, const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    double sumr, rmax;
+    const bool lead = blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0;
+    const int done = fcg_stop_test(prm, pin, rows, SA, SB, lead, &sumr, &rmax);
+    if (done) return;
+    const float S = e_scale_for(rmax, prm.e_bound);
+    if (lead && threadIdx.x == 0) SA->escale = S;
+    const float f0 = (float)(-sumr);
+    const float *tab = lvl_inv_table(L);
+    const uint32_t nx = L.nx, ny = L.ny, nz = L.nz;
+    const Tile4 T = tile4_geometry(nx, ny, nz);
+    if (T.k0 >= nz) return;
+    const TileW w = tile4_weights(L, T, tab);
+    __shared__ __align__(16) float lds[tile_lds_len<float>()];
+    const float cx = L.cx, cy = L.cy, cz = L.cz;
+    const bool pin0 = T.i0 == 0 && T.j == 0;
+    auto zin = [&](uint32_t k) { return (k > 0 && k + 1 < nz) ? 1 : 0; };
+    struct Ld {
+        float4 fo, fh;
+        float fx;
+    };
+    tile4_march<Ld, Quad>(
+        T, nx, ny, nz, lds,
+        [&](Ld &Q, uint32_t o, uint32_t h, uint32_t x) {
+            Q.fo = *(const float4 *)(r + o);
+            Q.fh = *(const float4 *)(r + h);
+            Q.fx = r[x];
+        },
+        [&](const Ld &Q, uint32_t k, Quad &out, Quad &hv, float &xv, Quad &f) {
+            const int z = zin(k);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                f.a[v] = (v == 0 && pin0 && k == 0) ? f0 : Q.fo[v];
+                out.a[v] = w.o[z][v] * f.a[v];
+                hv.a[v] = w.h[z][v] * Q.fh[v];
+            }
+            xv = w.x[z] * Q.fx;
+        },
+        [&](uint32_t k, uint32_t c, const Quad &em, const Quad &ec, const Quad &ep, const float *row,
+            const Quad &f) {
+            const int z = zin(k);
+            typedef e16 H4 __attribute__((ext_vector_type(4)));
+            H4 o;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float v0 = ec.a[v];
+                const float ve = v + 1 < 4 ? ec.a[v + 1] : row[4];
+                const float vw = v > 0 ? ec.a[v - 1] : row[-1];
+                const float vn = row[kTileS + v], vs = row[-kTileS + v];
+                const float ae = cx * ((v0 - ve) + (v0 - vw)) + cy * ((v0 - vn) + (v0 - vs)) +
+                                 cz * ((v0 - ep.a[v]) + (v0 - em.a[v]));
+                o[v] = (e16)((v0 + w.o[z][v] * (f.a[v] - ae)) * S);
+            }
+            *(H4 *)(eout + c) = o;
+        });
+}
+
 // alpha = r.p / p.q; x += alpha p, r -= alpha q; partials r.r, sum r, max|r|.
 __global__ void __launch_bounds__(kBlock) k_fcg_update(uint32_t n, float *__restrict__ x,
                                                        float *__restrict__ r,
@@ -2086,6 +2291,17 @@ __device__ inline double w_nb(const StepParams &P, const double *w, int i, int j
 // registers; the other in-plane neighbours are loads that hit in L1/L2.  The
 // divisions by 2 dx and dx^2 are multiplications by reciprocals computed
 // once (within an ulp of the quotient).
+//
+// The loads of plane k+1 are issued before plane k is computed and stored,
+// so a plane step waits for loads issued one step earlier, never for its
+// own stores.  Every load is unconditional: a neighbour outside the domain
+// reads an in-bounds cell whose value the selects below never use, so the
+// wait counts stay exact (a load under a branch makes the compiler wait for
+// every outstanding access where the branches join).
+struct PredLoads {
+    double up, vp, wp, wwp, wsp;                          // plane k+1
+    double ue, uw, un, usn, use, ve, vw, vn, vsn, vnw, we, wn;  // plane k
+};
 __global__ void __launch_bounds__(kZT) k_predict3_z(StepParams P, const double *__restrict__ u,
                                                     const double *__restrict__ v,
                                                     const double *__restrict__ w,
@@ -2098,80 +2314,108 @@ __global__ void __launch_bounds__(kZT) k_predict3_z(StepParams P, const double *
     const int kz = (nz + (int)gridDim.z - 1) / (int)gridDim.z, k0 = (int)b.z * kz;
     if (k0 >= nz) return;
     const int k1 = min(k0 + kz, nz);
-    const size_t uy = (size_t)nx + 1, uz = (size_t)(nx + 1) * ny;  // u strides
-    const size_t vy = (size_t)nx, vz = (size_t)nx * (ny + 1);      // v strides
-    const size_t wy = (size_t)nx, wz = (size_t)nx * ny;            // w strides
+    const uint32_t uy = (uint32_t)nx + 1, uz = (uint32_t)(nx + 1) * ny;  // u strides
+    const uint32_t vy = (uint32_t)nx, vz = (uint32_t)nx * (ny + 1);      // v strides
+    const uint32_t wy = (uint32_t)nx, wz = (uint32_t)nx * ny;            // w strides
     const double hx = 1.0 / (2.0 * P.dx), hy = 1.0 / (2.0 * P.dy), hz = 1.0 / (2.0 * P.dz);
     const double ix2 = 1.0 / (P.dx * P.dx), iy2 = 1.0 / (P.dy * P.dy), iz2 = 1.0 / (P.dz * P.dz);
     const bool xw = i > 0, xe = i + 1 < nx, ys = j > 0, yn = j + 1 < ny;
-    size_t cu = (size_t)i + uy * j + uz * k0, cv = (size_t)i + vy * j + vz * k0,
-           cw = (size_t)i + wy * j + wz * k0;
+    // neighbour steps, 0 where the neighbour is outside the domain
+    const uint32_t dw = xw ? 1u : 0u, de = xe ? 1u : 0u, dsu = ys ? uy : 0u, dnu = yn ? uy : 0u,
+                   dsv = ys ? vy : 0u, dsw = ys ? wy : 0u, dnw = yn ? wy : 0u;
+    const uint32_t iu = (uint32_t)i + uy * j, iv = (uint32_t)i + vy * j, iw = (uint32_t)i + wy * j;
+    // loads at 32-bit byte offsets from the array bases (each array is below
+    // 4 GiB, checked at set-up), so an address is one VGPR
+    auto ld = [](const double *base, uint32_t e) -> double {
+        return *(const double *)((const char *)base + (e << 3));
+    };
+    auto fetch = [&](int k) -> PredLoads {
+        const int kp = min(k + 1, nz - 1);  // u and v have nz planes, w nz + 1
+        const uint32_t cu = iu + uz * k, cv = iv + vz * k, cw = iw + wz * k, cwp = cw + wz;
+        PredLoads r;
+        r.up = ld(u, iu + uz * kp);
+        r.vp = ld(v, iv + vz * kp);
+        r.wp = ld(w, cwp);
+        r.wwp = ld(w, cwp - dw);
+        r.wsp = ld(w, cwp - dsw);
+        r.ue = ld(u, cu + 1);
+        r.uw = ld(u, cu - dw);
+        r.un = ld(u, cu + dnu);
+        r.usn = ld(u, cu - dsu);
+        r.use = ld(u, cu - dsu + 1);
+        r.ve = ld(v, cv + de);
+        r.vw = ld(v, cv - dw);
+        r.vn = ld(v, cv + vy);
+        r.vsn = ld(v, cv - dsv);
+        r.vnw = ld(v, cv + vy - dw);
+        r.we = ld(w, cw + de);
+        r.wn = ld(w, cw + dnw);
+        return r;
+    };
     // carried: plane k (u0, v0, w0, w at (i-1, j) and (i, j-1)) and plane k-1
-    double u0 = u[cu], v0 = v[cv], w0 = w[cw];
-    double ww0 = xw ? w[cw - 1] : 0.0, ws0 = ys ? w[cw - wy] : 0.0;
-    double um = 0.0, vm = 0.0, wm = 0.0, uem = 0.0, vnm = 0.0;
-    if (k0 > 0) {
-        um = u[cu - uz];
-        vm = v[cv - vz];
-        wm = w[cw - wz];
-        uem = u[cu - uz + 1];
-        vnm = v[cv - vz + vy];
-    }
-    for (int k = k0; k < k1; ++k, cu += uz, cv += vz, cw += wz) {
+    double u0 = ld(u, iu + uz * k0), v0 = ld(v, iv + vz * k0), w0 = ld(w, iw + wz * k0);
+    double ww0 = ld(w, iw + wz * k0 - dw), ws0 = ld(w, iw + wz * k0 - dsw);
+    const int km = k0 > 0 ? k0 - 1 : k0;
+    double um = ld(u, iu + uz * km), vm = ld(v, iv + vz * km), wm = ld(w, iw + wz * km),
+           uem = ld(u, iu + uz * km + 1), vnm = ld(v, iv + vz * km + vy);
+    auto plane = [&](int k, const PredLoads &cur) {
+        const uint32_t cu = iu + uz * k, cv = iv + vz * k, cw = iw + wz * k;
         const bool zt = k + 1 < nz;
-        // plane k+1 (carried to the next step)
-        const double up = zt ? u[cu + uz] : 0.0, vp = zt ? v[cv + vz] : 0.0, wp = w[cw + wz];
-        const double wwp = xw ? w[cw + wz - 1] : 0.0, wsp = ys ? w[cw + wz - wy] : 0.0;
-        // plane k neighbours
-        const double ue = u[cu + 1], uw = xw ? u[cu - 1] : 0.0, un = yn ? u[cu + uy] : 0.0;
-        const double usn = ys ? u[cu - uy] : 0.0, use = ys ? u[cu - uy + 1] : 0.0;
-        const double ve = xe ? v[cv + 1] : 0.0, vw = xw ? v[cv - 1] : 0.0, vn = v[cv + vy];
-        const double vsn = ys ? v[cv - vy] : 0.0, vnw = xw ? v[cv + vy - 1] : 0.0;
-        const double we = xe ? w[cw + 1] : 0.0, wn = yn ? w[cw + wy] : 0.0;
+        const double up = cur.up, vp = cur.vp, wp = cur.wp, wwp = cur.wwp, wsp = cur.wsp;
+        const double ue = cur.ue, uw = cur.uw, un = cur.un, usn = cur.usn, use = cur.use;
+        const double ve = cur.ve, vw = cur.vw, vn = cur.vn, vsn = cur.vsn, vnw = cur.vnw;
+        const double we = cur.we, wn = cur.wn;
         // u(i, j, k): boundary-normal at i = 0 (and i = nx, written below)
-        if (i == 0) {
-            us[cu] = 0.0;
-        } else {
+        {
             const double c = u0, e = ue, wv = uw;
             const double n = yn ? un : 2.0 * P.lid - c, so = ys ? usn : -c;
             const double f = zt ? up : -c, bk = k > 0 ? um : -c;
             const double vv = 0.25 * (vw + v0 + vnw + vn);
             const double wwv = 0.25 * (ww0 + w0 + wwp + wp);
-            us[cu] = c + P.dt * (-c * (e - wv) * hx - vv * (n - so) * hy - wwv * (f - bk) * hz +
-                                 P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
-                                         (f - 2.0 * c + bk) * iz2));
+            const double val = c + P.dt * (-c * (e - wv) * hx - vv * (n - so) * hy - wwv * (f - bk) * hz +
+                                           P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
+                                                   (f - 2.0 * c + bk) * iz2));
+            us[cu] = xw ? val : 0.0;
         }
-        if (i + 1 == nx) us[cu + 1] = 0.0;
+        if (!xe) us[cu + 1] = 0.0;
         // v(i, j, k): boundary-normal at j = 0 (and j = ny, written below)
-        if (j == 0) {
-            vs[cv] = 0.0;
-        } else {
+        {
             const double c = v0;
             const double e = xe ? ve : -c, wv = xw ? vw : -c, n = vn, so = vsn;
             const double f = zt ? vp : -c, bk = k > 0 ? vm : -c;
             const double uu = 0.25 * (usn + use + u0 + ue);
             const double wwv = 0.25 * (ws0 + w0 + wsp + wp);
-            vs[cv] = c + P.dt * (-uu * (e - wv) * hx - c * (n - so) * hy - wwv * (f - bk) * hz +
-                                 P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
-                                         (f - 2.0 * c + bk) * iz2));
+            const double val = c + P.dt * (-uu * (e - wv) * hx - c * (n - so) * hy - wwv * (f - bk) * hz +
+                                           P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
+                                                   (f - 2.0 * c + bk) * iz2));
+            vs[cv] = ys ? val : 0.0;
         }
-        if (j + 1 == ny) vs[cv + vy] = 0.0;
+        if (!yn) vs[cv + vy] = 0.0;
         // w(i, j, k): boundary-normal at k = 0 (and k = nz, written below)
-        if (k == 0) {
-            ws[cw] = 0.0;
-        } else {
+        {
             const double c = w0;
             const double e = xe ? we : -c, wv = xw ? ww0 : -c, n = yn ? wn : -c, so = ys ? ws0 : -c;
             const double f = wp, bk = wm;
             const double uu = 0.25 * (um + uem + u0 + ue);
             const double vv = 0.25 * (vm + vnm + v0 + vn);
-            ws[cw] = c + P.dt * (-uu * (e - wv) * hx - vv * (n - so) * hy - c * (f - bk) * hz +
-                                 P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
-                                         (f - 2.0 * c + bk) * iz2));
+            const double val = c + P.dt * (-uu * (e - wv) * hx - vv * (n - so) * hy - c * (f - bk) * hz +
+                                           P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
+                                                   (f - 2.0 * c + bk) * iz2));
+            ws[cw] = k > 0 ? val : 0.0;
         }
-        if (k + 1 == nz) ws[cw + wz] = 0.0;
+        if (!zt) ws[cw + wz] = 0.0;
         um = u0; vm = v0; wm = w0; uem = ue; vnm = vn;
         u0 = up; v0 = vp; w0 = wp; ww0 = wwp; ws0 = wsp;
+    };
+    // two planes per trip, so the prefetched set is loaded in place (a copy
+    // of it at the end of a trip would wait for its loads)
+    PredLoads a = fetch(k0);
+    for (int k = k0; k < k1; k += 2) {
+        const PredLoads bq = fetch(min(k + 1, nz - 1));
+        plane(k, a);
+        if (k + 1 >= k1) break;
+        a = fetch(min(k + 2, nz - 1));
+        plane(k + 1, bq);
     }
 }
 
@@ -2207,10 +2451,21 @@ struct DiagState {
 // not read, since the chunk below writes it).  The cell's high faces
 // u(i+1, .), v(., j+1) are recomputed from us, vs and p with the same
 // arithmetic their owners use, boundary faces are read (the correction
-// leaves them).  A face is checked for finiteness by the cell on its low
-// side, the top boundary faces by the last cell of their row.  Each block
+// leaves them, and the thread that owns one stores it back unchanged).  A
+// face is checked for finiteness by the cell on its low side, the top
+// boundary faces by the last cell of their row; each thread keeps the
+// lowest non-finite index per field and posts it once.  Each block
 // publishes one row {sum div^2 | max |div|, max speed}; k_diag_fold folds
 // them.
+//
+// As in k_predict3_z, plane k+1's loads are issued before plane k is
+// computed and stored, and no load is conditional: a missing neighbour
+// reads an in-bounds cell whose value the selects never use, and a boundary
+// face is read through a pointer select (the interior array at the same
+// index otherwise, a value this thread loads anyway).
+struct CorrLoads {
+    double pp, pw, pe, psn, pn, us0, us1, vs0, vs1, ws1, ub0, ub1, vb0, vb1, wb1;
+};
 __global__ void __launch_bounds__(kZT) k_correct_diag_z(StepParams P, const double *__restrict__ p,
                                                         const double *__restrict__ us,
                                                         const double *__restrict__ vs,
@@ -2225,47 +2480,82 @@ __global__ void __launch_bounds__(kZT) k_correct_diag_z(StepParams P, const doub
     const int kz = (nz + (int)gridDim.z - 1) / (int)gridDim.z, k0 = (int)b.z * kz, k1 = min(k0 + kz, nz);
     if (i < nx && j < ny && k0 < nz) {
         const double scale = P.dt / P.rho;
-        const size_t uy = (size_t)nx + 1, uz = (size_t)(nx + 1) * ny, vy = nx, vz = (size_t)nx * (ny + 1),
-                     wy = nx, wz = (size_t)nx * ny;
-        size_t cu = (size_t)i + uy * j + uz * k0, cv = (size_t)i + vy * j + vz * k0,
-               cw = (size_t)i + wy * j + wz * k0;
-        double pc = p[cw];  // cells share w's layout below nz
-        // bottom face of the first cell
-        double w0 = k0 == 0 ? w[cw] : ws[cw] - scale * (pc - p[cw - wz]) / P.dz;
-        for (int k = k0; k < k1; ++k, cu += uz, cv += vz, cw += wz) {
+        const uint32_t uy = (uint32_t)nx + 1, uz = (uint32_t)(nx + 1) * ny, vy = nx, vz = (uint32_t)nx * (ny + 1),
+                       wy = nx, wz = (uint32_t)nx * ny;
+        const bool xw = i > 0, xe = i + 1 < nx, ys = j > 0, yn = j + 1 < ny;
+        const uint32_t dw = xw ? 1u : 0u, de = xe ? 1u : 0u, dsw = ys ? wy : 0u, dnw = yn ? wy : 0u;
+        const uint32_t iu = (uint32_t)i + uy * j, iv = (uint32_t)i + vy * j, iw = (uint32_t)i + wy * j;
+        // boundary faces (read) or the interior array at the same index
+        const double *ub0p = xw ? us : u, *ub1p = xe ? us : u, *vb0p = ys ? vs : v, *vb1p = yn ? vs : v;
+        auto ld = [](const double *base, uint32_t e) -> double {
+            return *(const double *)((const char *)base + (e << 3));
+        };
+        auto fetch = [&](int k) -> CorrLoads {
             const bool zt = k + 1 < nz;
-            const double pp = zt ? p[cw + wz] : 0.0;
-            const double pw = i > 0 ? p[cw - 1] : 0.0, pe = i + 1 < nx ? p[cw + 1] : 0.0;
-            const double psn = j > 0 ? p[cw - wy] : 0.0, pn = j + 1 < ny ? p[cw + wy] : 0.0;
-            const double us0 = i > 0 ? us[cu] : 0.0, us1 = i + 1 < nx ? us[cu + 1] : 0.0;
-            const double vs0 = j > 0 ? vs[cv] : 0.0, vs1 = j + 1 < ny ? vs[cv + vy] : 0.0;
-            const double ws1 = zt ? ws[cw + wz] : 0.0;
-            const double ub0 = i == 0 ? u[cu] : 0.0, ub1 = i + 1 == nx ? u[cu + 1] : 0.0;
-            const double vb0 = j == 0 ? v[cv] : 0.0, vb1 = j + 1 == ny ? v[cv + vy] : 0.0;
-            const double wb1 = zt ? 0.0 : w[cw + wz];
-            const double u0 = i > 0 ? us0 - scale * (pc - pw) / P.dx : ub0;
-            const double u1 = i + 1 < nx ? us1 - scale * (pe - pc) / P.dx : ub1;
-            const double v0 = j > 0 ? vs0 - scale * (pc - psn) / P.dy : vb0;
-            const double v1 = j + 1 < ny ? vs1 - scale * (pn - pc) / P.dy : vb1;
-            const double w1 = zt ? ws1 - scale * (pp - pc) / P.dz : wb1;
-            if (i > 0) u[cu] = u0;
-            if (j > 0) v[cv] = v0;
-            if (zt) w[cw + wz] = w1;
+            const uint32_t cu = iu + uz * k, cv = iv + vz * k, cw = iw + wz * k;
+            CorrLoads r;
+            r.pp = ld(p, cw + (zt ? wz : 0u));
+            r.pw = ld(p, cw - dw);
+            r.pe = ld(p, cw + de);
+            r.psn = ld(p, cw - dsw);
+            r.pn = ld(p, cw + dnw);
+            r.us0 = ld(us, cu);
+            r.us1 = ld(us, cu + 1);
+            r.vs0 = ld(vs, cv);
+            r.vs1 = ld(vs, cv + vy);
+            r.ws1 = ld(ws, cw + wz);
+            r.ub0 = ld(ub0p, cu);
+            r.ub1 = ld(ub1p, cu + 1);
+            r.vb0 = ld(vb0p, cv);
+            r.vb1 = ld(vb1p, cv + vy);
+            r.wb1 = ld(zt ? ws : w, cw + wz);
+            return r;
+        };
+        unsigned long long bad[4] = {~0ull, ~0ull, ~0ull, ~0ull};
+        auto flag = [&](int f, bool finite, uint32_t idx) {
+            if (!finite) bad[f] = min(bad[f], (unsigned long long)idx);
+        };
+        double pc = ld(p, iw + wz * k0);  // cells share w's layout below nz
+        // bottom face of the first cell
+        double w0 = k0 == 0 ? ld(w, iw) : ld(ws, iw + wz * k0) - scale * (pc - ld(p, iw + wz * (k0 - 1))) / P.dz;
+        auto plane = [&](int k, const CorrLoads &L) {
+            const uint32_t cu = iu + uz * k, cv = iv + vz * k, cw = iw + wz * k;
+            const bool zt = k + 1 < nz;
+            const double u0 = xw ? L.us0 - scale * (pc - L.pw) / P.dx : L.ub0;
+            const double u1 = xe ? L.us1 - scale * (L.pe - pc) / P.dx : L.ub1;
+            const double v0 = ys ? L.vs0 - scale * (pc - L.psn) / P.dy : L.vb0;
+            const double v1 = yn ? L.vs1 - scale * (L.pn - pc) / P.dy : L.vb1;
+            const double w1 = zt ? L.ws1 - scale * (L.pp - pc) / P.dz : L.wb1;
+            *(double *)((char *)u + (cu << 3)) = u0;
+            *(double *)((char *)v + (cv << 3)) = v0;
+            *(double *)((char *)w + ((cw + wz) << 3)) = w1;
             const double div = (u1 - u0) / P.dx + (v1 - v0) / P.dy + (w1 - w0) / P.dz;
             s[0] += div * div;
             m[0] = fmax(m[0], fabs(div));
             const double a = 0.5 * (u0 + u1), bb = 0.5 * (v0 + v1), cc = 0.5 * (w0 + w1);
             m[1] = fmax(m[1], sqrt(a * a + bb * bb + cc * cc));
-            if (!isfinite(pc)) atomicMin(&dg->first_bad[0], (unsigned long long)cw);
-            if (!isfinite(u0)) atomicMin(&dg->first_bad[1], (unsigned long long)cu);
-            if (i + 1 == nx && !isfinite(u1)) atomicMin(&dg->first_bad[1], (unsigned long long)(cu + 1));
-            if (!isfinite(v0)) atomicMin(&dg->first_bad[2], (unsigned long long)cv);
-            if (j + 1 == ny && !isfinite(v1)) atomicMin(&dg->first_bad[2], (unsigned long long)(cv + vy));
-            if (!isfinite(w0)) atomicMin(&dg->first_bad[3], (unsigned long long)cw);
-            if (!zt && !isfinite(w1)) atomicMin(&dg->first_bad[3], (unsigned long long)(cw + wz));
+            flag(0, isfinite(pc), cw);
+            flag(1, isfinite(u0), cu);
+            flag(1, xe || isfinite(u1), cu + 1);
+            flag(2, isfinite(v0), cv);
+            flag(2, yn || isfinite(v1), cv + vy);
+            flag(3, isfinite(w0), cw);
+            flag(3, zt || isfinite(w1), cw + wz);
             w0 = w1;
-            pc = pp;
+            pc = L.pp;
+        };
+        // two planes per trip, so the prefetched set is loaded in place
+        CorrLoads a = fetch(k0);
+        for (int k = k0; k < k1; k += 2) {
+            const CorrLoads bq = fetch(min(k + 1, nz - 1));
+            plane(k, a);
+            if (k + 1 >= k1) break;
+            a = fetch(min(k + 2, nz - 1));
+            plane(k + 1, bq);
         }
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+            if (bad[f] != ~0ull) atomicMin(&dg->first_bad[f], bad[f]);
     }
     publish_row<1, 2>(s, m, pout);
 }
@@ -2364,7 +2654,7 @@ enum Kind {
     K_MG_CHECK, K_MG_SWEEP_L0, K_MG_SWEEP_L0_PROLONG, K_MG_SWEEP_L0_PUBLISH,
     K_MG_RESTRICT_L0, K_MG_DOWN, K_MG_TAIL, K_MG_UP, K_FCG_DIRAPPLY, K_FCG_UPDATE, K_WARM_START,
     K_UNET_COARSE,
-    K_CGDOT_D, K_CGSTEP_D, K_CGDOT_F, K_CGSTEP_F, K_FCG_DIRAPPLY_T,
+    K_CGDOT_D, K_CGSTEP_D, K_CGDOT_F, K_CGSTEP_F, K_FCG_DIRAPPLY_T, K_MG_CHECK_T,
     K_COUNT
 };
 const char *const kKindNames[K_COUNT] = {
@@ -2378,7 +2668,7 @@ const char *const kKindNames[K_COUNT] = {
     "unet_warm_start",  // restriction + U-Net inference + interpolation (many kernels)
     "unet_coarse_solve",  // U-Net inference + the residual on its level (many kernels)
     "k_cg_dot_t<double>", "k_cg_step_t<double>", "k_cg_dot_t<float>", "k_cg_step_t<float>",
-    "k_fcg_dirapply_t",
+    "k_fcg_dirapply_t", "k_mg_check_t",
 };
 
 }  // namespace c3
@@ -2457,6 +2747,8 @@ struct fl_cavity3d {
     // plane-tile FCG search direction (count 0: nx not a multiple of 4, the
     // column march runs instead)
     ZGrid ztd{0, 0, 0, 0};
+    // plane-tile level-0 cycle passes (count 0: the column marches run)
+    ZGrid ztc{0, 0, 0, 0};
     int64_t step = 0;
     KProf prof;  // per-kernel timing (HIP events on the solver's stream)
     double *cell_out = nullptr;  // output staging: cell-centred u, v, w (allocated on first use)
@@ -2663,6 +2955,11 @@ fl_status validate(fl_ctx *ctx, const fl_cavity3d_config *c) {
     const double faces = (double)(c->nx + 1) * (double)(c->ny + 1) * (double)(c->nz + 1) * 3.0;
     if (cells + faces >= 2147483647.0)
         return set_error(ctx, FL_ERR_CONFIG, "grid too large for 32-bit cell indexing");
+    // the step kernels address each FP64 field by a 32-bit byte offset
+    const double face_array = std::max((double)(c->nx + 1) * c->ny * c->nz,
+                                       std::max((double)c->nx * (c->ny + 1) * c->nz, (double)c->nx * c->ny * (c->nz + 1)));
+    if (face_array * 8.0 >= 4294967296.0)
+        return set_error(ctx, FL_ERR_CONFIG, "grid too large for 32-bit byte offsets (a field array reaches 4 GiB)");
     return FL_OK;
 }
 
@@ -2848,8 +3145,13 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     const double n0 = L[0].n, n1 = L[1].n;
     // down: the first two pre-sweeps in one pass (into meb), the third into
     // mea, which the restriction and the up pass read
-    FL_TIMED(s, K_MG_CHECK, k, 6.0 * n0,
-             k_mg_check_z<<<zg, kZT, 0, stm>>>(L[0], prm, s->r32, s->m0b, s->pb, k == 0 ? s->zr.count : G, s->fa, s->fb));
+    const uint32_t crows = k == 0 ? s->zr.count : G;
+    if (s->ztc.count > 0)
+        FL_TIMED(s, K_MG_CHECK_T, k, 6.0 * n0,
+                 k_mg_check_t<<<dim3(s->ztc.bx, s->ztc.by, s->ztc.bz), kBlock, 0, stm>>>(L[0], prm, s->r32, s->m0b, s->pb, crows, s->fa, s->fb));
+    else
+        FL_TIMED(s, K_MG_CHECK, k, 6.0 * n0,
+                 k_mg_check_z<<<zg, kZT, 0, stm>>>(L[0], prm, s->r32, s->m0b, s->pb, crows, s->fa, s->fb));
     e16 *pre0 = s->m0b, *post0 = s->m0a;  // level 0's pre-smoothed iterate, and the other buffer
     if (kL0Smooth == 3) {
         FL_TIMED(s, K_MG_SWEEP_L0, k, 8.0 * n0,
@@ -3447,8 +3749,10 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
         s->ztfs = tile_grid_resident<float>(ctx, (const void *)k_cg_step_t<float>, nx, ny, nz);
     }
     // the FCG direction's tile pass (16-byte float and 8-byte FP16 lane loads)
-    if (nx % 4 == 0 && cfg->preconditioner != FL_PRECOND_JACOBI)
+    if (nx % 4 == 0 && cfg->preconditioner != FL_PRECOND_JACOBI) {
         s->ztd = tile_grid_resident<float>(ctx, (const void *)k_fcg_dirapply_t, nx, ny, nz);
+        s->ztc = tile_grid_resident<float>(ctx, (const void *)k_mg_check_t, nx, ny, nz);
+    }
     const size_t prow =
         (size_t)std::max(std::max(std::max(s->blocks, s->ew_blocks), std::max(s->zg.count, s->zgd.count)),
                          std::max(std::max(std::max(s->zt.count, s->ztf.count), std::max(s->zts.count, s->ztfs.count)),

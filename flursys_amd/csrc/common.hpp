@@ -140,12 +140,28 @@ __device__ inline void fold_partials(const double *__restrict__ partials, uint32
     for (int a = 0; a < NS; ++a) s[a] = 0.0;
 #pragma unroll
     for (int a = 0; a < NM; ++a) m[a] = 0.0;
-    for (uint32_t b = threadIdx.x; b < rows; b += blockDim.x) {
-        const double *pb = partials + (size_t)b * (NS + NM);
+    // rows b = t, t + blockDim, ... summed in order; the loads of kU rows are
+    // issued together (a row past the end loads row 0 and is not added), so
+    // the fold waits for memory once per kU rows instead of once per row
+    constexpr int kU = 8, R = NS + NM;
+    const uint32_t bd = blockDim.x;
+    for (uint32_t b = threadIdx.x; b < rows; b += kU * bd) {
+        double v[kU][R];
 #pragma unroll
-        for (int a = 0; a < NS; ++a) s[a] += pb[a];
+        for (int u = 0; u < kU; ++u) {
+            const uint32_t bb = b + u * bd;
+            const size_t base = (size_t)(bb < rows ? bb : 0u) * R;
 #pragma unroll
-        for (int a = 0; a < NM; ++a) m[a] = fmax(m[a], pb[NS + a]);
+            for (int a = 0; a < R; ++a) v[u][a] = partials[base + a];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const bool ok = b + u * bd < rows;
+#pragma unroll
+            for (int a = 0; a < NS; ++a) s[a] = ok ? s[a] + v[u][a] : s[a];
+#pragma unroll
+            for (int a = 0; a < NM; ++a) m[a] = ok ? fmax(m[a], v[u][NS + a]) : m[a];
+        }
     }
     block_reduce<NS, NM>(s, m);
     if (threadIdx.x == 0) {
