@@ -2031,8 +2031,8 @@ __device__ inline TileW tile4_weights(const Level &L, const Tile4 &T, const floa
     }
     return w;
 }
-// load(Ld &, own offset, halo-row offset, halo-cell offset) issues a plane's
-// loads; form(const Ld &, k, own Quad &, halo Quad &, halo float &, Sd &)
+// load(Ld &, k, own offset, halo-row offset, halo-cell offset) issues plane
+// k's loads; form(const Ld &, k, own Quad &, halo Quad &, halo float &, Sd &)
 // makes the operand of plane k (and side values of the own cells, kept for
 // the body); body(k, c, below, centre, above, row, side) computes plane k:
 // c is the own quad's first cell, `row` its LDS slot (row[-1], row[4] the x
@@ -2052,7 +2052,7 @@ __device__ inline void tile4_march(const Tile4 &T, uint32_t nx, uint32_t ny, uin
     auto ld = [&](uint32_t k) {
         const uint32_t pk = nxy * k;
         const uint32_t o = T.own ? pk : 0u, h = T.hrow ? pk : o, q = T.xh ? pk : o;
-        load(L, T.co + o, T.ch + h, T.cxh + q);
+        load(L, k, T.co + o, T.ch + h, T.cxh + q);
     };
     auto put = [&](uint32_t k, int bf, Quad &out, Sd &sd) {
         Quad hv;
@@ -2130,7 +2130,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_check_t(Level L, CgParams prm, co
     };
     tile4_march<Ld, Quad>(
         T, nx, ny, nz, lds,
-        [&](Ld &Q, uint32_t o, uint32_t h, uint32_t x) {
+        [&](Ld &Q, uint32_t, uint32_t o, uint32_t h, uint32_t x) {
             Q.fo = *(const float4 *)(r + o);
             Q.fh = *(const float4 *)(r + h);
             Q.fx = r[x];
@@ -2159,6 +2159,155 @@ __global__ void __launch_bounds__(kBlock) k_mg_check_t(Level L, CgParams prm, co
                 const float ae = cx * ((v0 - ve) + (v0 - vw)) + cy * ((v0 - vn) + (v0 - vs)) +
                                  cz * ((v0 - ep.a[v]) + (v0 - em.a[v]));
                 o[v] = (e16)((v0 + w.o[z][v] * (f.a[v] - ae)) * S);
+            }
+            *(H4 *)(eout + c) = o;
+        });
+}
+
+// k_mg_sweep0_z as a plane-tile march: e_out = e_in + omega D^-1 (f - A e_in)
+// on the FP16 iterates; kPublish also publishes sum e_out.q_old and
+// sum q_old per block.
+template <bool kPublish>
+__global__ void __launch_bounds__(kBlock) k_mg_sweep0_t(Level L, const float *__restrict__ f,
+                                                        const e16 *__restrict__ ein, e16 *__restrict__ eout,
+                                                        const float *__restrict__ qold, double *__restrict__ pout,
+                                                        const FcgA *SA, const FcgB *SB) {
+    if (SA->done || SB->done) return;
+    const float S = SA->escale, iS = 1.0f / S;
+    const float f0 = (float)(-SA->sumr);
+    const bool publish = kPublish && !SB->first;
+    const float *tab = lvl_inv_table(L);
+    double s[2] = {0.0, 0.0}, m[1] = {0.0};
+    const uint32_t nx = L.nx, ny = L.ny, nz = L.nz;
+    const Tile4 T = tile4_geometry(nx, ny, nz);
+    if (T.k0 < nz) {
+        const TileW w = tile4_weights(L, T, tab);
+        __shared__ __align__(16) float lds[tile_lds_len<float>()];
+        const float cx = L.cx, cy = L.cy, cz = L.cz;
+        const bool pin0 = T.i0 == 0 && T.j == 0;
+        auto zin = [&](uint32_t k) { return (k > 0 && k + 1 < nz) ? 1 : 0; };
+        typedef e16 H4 __attribute__((ext_vector_type(4)));
+        struct Ld {
+            H4 eo, eh;
+            e16 ex;
+            float4 fo, qo;
+        };
+        struct Sd {
+            float f[4], q[4];
+        };
+        tile4_march<Ld, Sd>(
+            T, nx, ny, nz, lds,
+            [&](Ld &Q, uint32_t, uint32_t o, uint32_t h, uint32_t x) {
+                Q.eo = *(const H4 *)(ein + o);
+                Q.eh = *(const H4 *)(ein + h);
+                Q.ex = ein[x];
+                Q.fo = *(const float4 *)(f + o);
+                if (kPublish) Q.qo = *(const float4 *)(qold + o);  // (unused on the first iteration)
+            },
+            [&](const Ld &Q, uint32_t k, Quad &out, Quad &hv, float &xv, Sd &sd) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    out.a[v] = (float)Q.eo[v] * iS;
+                    hv.a[v] = (float)Q.eh[v] * iS;
+                    sd.f[v] = (v == 0 && pin0 && k == 0) ? f0 : Q.fo[v];
+                    sd.q[v] = kPublish ? Q.qo[v] : 0.0f;
+                }
+                xv = (float)Q.ex * iS;
+            },
+            [&](uint32_t k, uint32_t c, const Quad &em, const Quad &ec, const Quad &ep, const float *row,
+                const Sd &sd) {
+                const int z = zin(k);
+                H4 o;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float v0 = ec.a[v];
+                    const float ve = v + 1 < 4 ? ec.a[v + 1] : row[4];
+                    const float vw = v > 0 ? ec.a[v - 1] : row[-1];
+                    const float vn = row[kTileS + v], vs = row[-kTileS + v];
+                    const float ae = cx * ((v0 - ve) + (v0 - vw)) + cy * ((v0 - vn) + (v0 - vs)) +
+                                     cz * ((v0 - ep.a[v]) + (v0 - em.a[v]));
+                    const float out = v0 + w.o[z][v] * (sd.f[v] - ae);
+                    o[v] = (e16)(out * S);
+                    if (publish) {
+                        const double qo = (double)sd.q[v];
+                        s[0] += (double)out * qo;
+                        s[1] += qo;
+                    }
+                }
+                *(H4 *)(eout + c) = o;
+            });
+    }
+    if (kPublish) publish_row<2, 1>(s, m, pout);
+}
+
+// k_mg_prolong0_z as a plane-tile march: e' = e_in + corr P(e_c) is formed
+// per plane (P constant on the 2 x 2 x 2 children of a coarse cell, so a
+// quad's four cells take two coarse values), then
+// e_out = e' + omega D^-1 (f - A e').
+__global__ void __launch_bounds__(kBlock) k_mg_prolong0_t(Level L, Level LC, const float *__restrict__ f,
+                                                          const e16 *__restrict__ ein,
+                                                          const float *__restrict__ ecoarse,
+                                                          e16 *__restrict__ eout, const FcgA *SA,
+                                                          const FcgB *SB, float corr) {
+    if (SA->done || SB->done) return;
+    const float S = SA->escale, iS = 1.0f / S;
+    const float f0 = (float)(-SA->sumr);
+    const float *tab = lvl_inv_table(L);
+    const uint32_t nx = L.nx, ny = L.ny, nz = L.nz;
+    const Tile4 T = tile4_geometry(nx, ny, nz);
+    if (T.k0 >= nz) return;
+    const TileW w = tile4_weights(L, T, tab);
+    __shared__ __align__(16) float lds[tile_lds_len<float>()];
+    const float cx = L.cx, cy = L.cy, cz = L.cz;
+    const bool pin0 = T.i0 == 0 && T.j == 0;
+    auto zin = [&](uint32_t k) { return (k > 0 && k + 1 < nz) ? 1 : 0; };
+    // coarse columns of the own quad, the halo-row quad and the halo cell
+    // (a lane without one reads the own column, or column 0)
+    const uint32_t qo = T.own ? (T.i0 >> 1) + LC.nx * (T.j >> 1) : 0u;
+    const uint32_t qh = T.hrow ? (T.i0 >> 1) + LC.nx * (T.jh >> 1) : qo;
+    const uint32_t qx = T.xh ? (T.ix >> 1) + LC.nx * (T.jx >> 1) : qo;
+    typedef e16 H4 __attribute__((ext_vector_type(4)));
+    struct Ld {
+        H4 eo, eh;
+        e16 ex;
+        float2 co, ch;
+        float cxv;
+        float4 fo;
+    };
+    tile4_march<Ld, Quad>(
+        T, nx, ny, nz, lds,
+        [&](Ld &Q, uint32_t k, uint32_t o, uint32_t h, uint32_t x) {
+            const uint32_t pc = LC.nxy * (k >> 1);
+            Q.eo = *(const H4 *)(ein + o);
+            Q.eh = *(const H4 *)(ein + h);
+            Q.ex = ein[x];
+            Q.co = *(const float2 *)(ecoarse + qo + pc);
+            Q.ch = *(const float2 *)(ecoarse + qh + pc);
+            Q.cxv = ecoarse[qx + pc];
+            Q.fo = *(const float4 *)(f + o);
+        },
+        [&](const Ld &Q, uint32_t k, Quad &out, Quad &hv, float &xv, Quad &fq) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                out.a[v] = (float)Q.eo[v] * iS + corr * (v < 2 ? Q.co.x : Q.co.y);
+                hv.a[v] = (float)Q.eh[v] * iS + corr * (v < 2 ? Q.ch.x : Q.ch.y);
+                fq.a[v] = (v == 0 && pin0 && k == 0) ? f0 : Q.fo[v];
+            }
+            xv = (float)Q.ex * iS + corr * Q.cxv;
+        },
+        [&](uint32_t k, uint32_t c, const Quad &em, const Quad &ec, const Quad &ep, const float *row,
+            const Quad &fq) {
+            const int z = zin(k);
+            H4 o;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float v0 = ec.a[v];
+                const float ve = v + 1 < 4 ? ec.a[v + 1] : row[4];
+                const float vw = v > 0 ? ec.a[v - 1] : row[-1];
+                const float vn = row[kTileS + v], vs = row[-kTileS + v];
+                const float ae = cx * ((v0 - ve) + (v0 - vw)) + cy * ((v0 - vn) + (v0 - vs)) +
+                                 cz * ((v0 - ep.a[v]) + (v0 - em.a[v]));
+                o[v] = (e16)((v0 + w.o[z][v] * (fq.a[v] - ae)) * S);
             }
             *(H4 *)(eout + c) = o;
         });
@@ -2655,6 +2804,7 @@ enum Kind {
     K_MG_RESTRICT_L0, K_MG_DOWN, K_MG_TAIL, K_MG_UP, K_FCG_DIRAPPLY, K_FCG_UPDATE, K_WARM_START,
     K_UNET_COARSE,
     K_CGDOT_D, K_CGSTEP_D, K_CGDOT_F, K_CGSTEP_F, K_FCG_DIRAPPLY_T, K_MG_CHECK_T,
+    K_MG_SWEEP_L0_T, K_MG_PROLONG_L0_T, K_MG_PUBLISH_L0_T,
     K_COUNT
 };
 const char *const kKindNames[K_COUNT] = {
@@ -2669,6 +2819,7 @@ const char *const kKindNames[K_COUNT] = {
     "unet_coarse_solve",  // U-Net inference + the residual on its level (many kernels)
     "k_cg_dot_t<double>", "k_cg_step_t<double>", "k_cg_dot_t<float>", "k_cg_step_t<float>",
     "k_fcg_dirapply_t", "k_mg_check_t",
+    "k_mg_sweep0_t<false>", "k_mg_prolong0_t", "k_mg_sweep0_t<true>",
 };
 
 }  // namespace c3
@@ -2747,8 +2898,9 @@ struct fl_cavity3d {
     // plane-tile FCG search direction (count 0: nx not a multiple of 4, the
     // column march runs instead)
     ZGrid ztd{0, 0, 0, 0};
-    // plane-tile level-0 cycle passes (count 0: the column marches run)
-    ZGrid ztc{0, 0, 0, 0};
+    // plane-tile level-0 cycle passes: stop test and first sweeps, sweep,
+    // prolongation, publishing sweep (count 0: the column marches run)
+    ZGrid ztc{0, 0, 0, 0}, zts0{0, 0, 0, 0}, ztp0{0, 0, 0, 0}, zts1{0, 0, 0, 0};
     int64_t step = 0;
     KProf prof;  // per-kernel timing (HIP events on the solver's stream)
     double *cell_out = nullptr;  // output staging: cell-centred u, v, w (allocated on first use)
@@ -3153,9 +3305,18 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
         FL_TIMED(s, K_MG_CHECK, k, 6.0 * n0,
                  k_mg_check_z<<<zg, kZT, 0, stm>>>(L[0], prm, s->r32, s->m0b, s->pb, crows, s->fa, s->fb));
     e16 *pre0 = s->m0b, *post0 = s->m0a;  // level 0's pre-smoothed iterate, and the other buffer
+    const bool tiles = s->ztc.count > 0;
+    auto sweep0 = [&](const e16 *ein, e16 *eout) -> fl_status {
+        if (tiles)
+            FL_TIMED(s, K_MG_SWEEP_L0_T, k, 8.0 * n0,
+                     k_mg_sweep0_t<false><<<dim3(s->zts0.bx, s->zts0.by, s->zts0.bz), kBlock, 0, stm>>>(L[0], s->r32, ein, eout, nullptr, nullptr, s->fa, s->fb));
+        else
+            FL_TIMED(s, K_MG_SWEEP_L0, k, 8.0 * n0,
+                     k_mg_sweep0_z<false><<<zg, kZT, 0, stm>>>(L[0], s->r32, ein, eout, nullptr, nullptr, s->fa, s->fb));
+        return FL_OK;
+    };
     if (kL0Smooth == 3) {
-        FL_TIMED(s, K_MG_SWEEP_L0, k, 8.0 * n0,
-                 k_mg_sweep0_z<false><<<zg, kZT, 0, stm>>>(L[0], s->r32, s->m0b, s->m0a, nullptr, nullptr, s->fa, s->fb));
+        if (fl_status st = sweep0(s->m0b, s->m0a)) return st;
         pre0 = s->m0a;
         post0 = s->m0b;
     }
@@ -3254,23 +3415,34 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     // up on level 0: prolongate-correct sweep into post0 (kL0Smooth == 2:
     // mea) or into meb (3, then one more sweep into mea); the publishing
     // sweep leaves z~ in meb
-    FL_TIMED(s, K_MG_SWEEP_L0_PROLONG, k, 8.0 * n0 + 4.0 * n1,
-             k_mg_prolong0_z<<<zg, kZT, 0, stm>>>(L[0], L[1], s->r32, pre0, s->meb[1], post0, s->fa, s->fb, kCorr));
+    if (tiles)
+        FL_TIMED(s, K_MG_PROLONG_L0_T, k, 8.0 * n0 + 4.0 * n1,
+                 k_mg_prolong0_t<<<dim3(s->ztp0.bx, s->ztp0.by, s->ztp0.bz), kBlock, 0, stm>>>(L[0], L[1], s->r32, pre0, s->meb[1], post0, s->fa, s->fb, kCorr));
+    else
+        FL_TIMED(s, K_MG_SWEEP_L0_PROLONG, k, 8.0 * n0 + 4.0 * n1,
+                 k_mg_prolong0_z<<<zg, kZT, 0, stm>>>(L[0], L[1], s->r32, pre0, s->meb[1], post0, s->fa, s->fb, kCorr));
     if (kL0Smooth == 3)
-        FL_TIMED(s, K_MG_SWEEP_L0, k, 8.0 * n0,
-                 k_mg_sweep0_z<false><<<zg, kZT, 0, stm>>>(L[0], s->r32, s->m0b, s->m0a, nullptr, nullptr, s->fa, s->fb));
-    FL_TIMED(s, K_MG_SWEEP_L0_PUBLISH, k, 12.0 * n0,
-             k_mg_sweep0_z<true><<<zg, kZT, 0, stm>>>(L[0], s->r32, s->m0a, s->m0b, s->q32, s->pz, s->fa, s->fb));
+        if (fl_status st = sweep0(s->m0b, s->m0a)) return st;
+    // the publishing sweep's partial rows, folded by the direction pass
+    uint32_t zrows = s->zg.count;
+    if (tiles) {
+        zrows = s->zts1.count;
+        FL_TIMED(s, K_MG_PUBLISH_L0_T, k, 12.0 * n0,
+                 k_mg_sweep0_t<true><<<dim3(s->zts1.bx, s->zts1.by, s->zts1.bz), kBlock, 0, stm>>>(L[0], s->r32, s->m0a, s->m0b, s->q32, s->pz, s->fa, s->fb));
+    } else {
+        FL_TIMED(s, K_MG_SWEEP_L0_PUBLISH, k, 12.0 * n0,
+                 k_mg_sweep0_z<true><<<zg, kZT, 0, stm>>>(L[0], s->r32, s->m0a, s->m0b, s->q32, s->pz, s->fa, s->fb));
+    }
     // FCG direction (reads z~, p_old, r; writes p, q) and update (x, r
     // read and written, p and q read)
     uint32_t drows = s->zgd.count;
     if (s->ztd.count > 0) {
         drows = s->ztd.count;
         FL_TIMED(s, K_FCG_DIRAPPLY_T, k, 18.0 * n0,
-                 k_fcg_dirapply_t<<<dim3(s->ztd.bx, s->ztd.by, s->ztd.bz), kBlock, 0, stm>>>(s->g, s->m0b, pold, pnew, s->q32, s->r32, s->pz, s->zg.count, s->pa, s->fa, s->fb));
+                 k_fcg_dirapply_t<<<dim3(s->ztd.bx, s->ztd.by, s->ztd.bz), kBlock, 0, stm>>>(s->g, s->m0b, pold, pnew, s->q32, s->r32, s->pz, zrows, s->pa, s->fa, s->fb));
     } else {
         FL_TIMED(s, K_FCG_DIRAPPLY, k, 18.0 * n0,
-                 k_fcg_dirapply_z<<<dim3(s->zgd.bx, s->zgd.by, s->zgd.bz), kZT, 0, stm>>>(s->g, s->m0b, pold, pnew, s->q32, s->r32, s->pz, s->zg.count, s->pa, s->fa, s->fb));
+                 k_fcg_dirapply_z<<<dim3(s->zgd.bx, s->zgd.by, s->zgd.bz), kZT, 0, stm>>>(s->g, s->m0b, pold, pnew, s->q32, s->r32, s->pz, zrows, s->pa, s->fa, s->fb));
     }
     FL_TIMED(s, K_FCG_UPDATE, k, 24.0 * n0,
              k_fcg_update<<<G, kBlock, 0, stm>>>((uint32_t)s->n, s->e32, s->r32, pnew, s->q32, s->pa, drows, s->pb, s->fa, s->fb));
@@ -3752,11 +3924,14 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
     if (nx % 4 == 0 && cfg->preconditioner != FL_PRECOND_JACOBI) {
         s->ztd = tile_grid_resident<float>(ctx, (const void *)k_fcg_dirapply_t, nx, ny, nz);
         s->ztc = tile_grid_resident<float>(ctx, (const void *)k_mg_check_t, nx, ny, nz);
+        s->zts0 = tile_grid_resident<float>(ctx, (const void *)k_mg_sweep0_t<false>, nx, ny, nz);
+        s->ztp0 = tile_grid_resident<float>(ctx, (const void *)k_mg_prolong0_t, nx, ny, nz);
+        s->zts1 = tile_grid_resident<float>(ctx, (const void *)k_mg_sweep0_t<true>, nx, ny, nz);
     }
     const size_t prow =
         (size_t)std::max(std::max(std::max(s->blocks, s->ew_blocks), std::max(s->zg.count, s->zgd.count)),
                          std::max(std::max(std::max(s->zt.count, s->ztf.count), std::max(s->zts.count, s->ztfs.count)),
-                                  s->ztd.count)) * 4;
+                                  std::max(s->ztd.count, s->zts1.count))) * 4;
     if ((st = alloc(s, &s->u, s->nu)) || (st = alloc(s, &s->v, s->nv)) ||
         (st = alloc(s, &s->w, s->nw)) || (st = alloc(s, &s->us, s->nu)) ||
         (st = alloc(s, &s->vs, s->nv)) || (st = alloc(s, &s->ws, s->nw)) ||

@@ -187,8 +187,9 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
 // LDS.  A workgroup owns a 16 x R x 4 block of output voxels (wave w: plane
 // z0 + w, rows y0 .. y0 + R - 1, one 16-voxel MFMA row each) and NB
 // 16-channel output blocks.  Input channels are taken 16 at a time: per
-// chunk the block stages (one batch of 16-byte global loads, then the LDS
-// stores) the (16 + 2) x (R + 2) x 6 halo of those channels (zero outside
+// chunk the block stages (one batch of 16-byte global loads, on the coarse
+// levels issued while the previous chunk multiplies, then the LDS stores) the
+// (16 + 2) x (R + 2) x 6 halo of those channels (zero outside
 // the domain; channels past c1 from the half-resolution x2, the decoder's
 // nearest upsampling) as two 8-channel halves, and the chunk's weights as
 // [tap][half][output channel] units, so the 16 lanes of an MFMA row read
@@ -227,8 +228,9 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[r][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int h = g & 1;
-    for (int c0 = 0; c0 < a.cin; c0 += 16) {
-        bf16x8 st[ITER];
+    // one chunk of 16 input channels: its halo tile and weights into st
+    bf16x8 st[ITER];
+    auto stage = [&](int c0) {
 #pragma unroll
         for (int i = 0; i < ITER; ++i) {
             const int u = threadIdx.x + 256 * i;
@@ -251,6 +253,15 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
             }
             st[i] = val;
         }
+    };
+    // R == 1 (the coarse levels, a few workgroups per CU at most): the next
+    // chunk's loads are in flight while this chunk multiplies.  R == 4 (the
+    // finest level, one or two chunks) stages each chunk when it starts,
+    // which keeps st out of the MFMA loop's registers (occupancy).
+    constexpr bool kPipe = R == 1;
+    if (kPipe) stage(0);
+    for (int c0 = 0; c0 < a.cin; c0 += 16) {
+        if (!kPipe) stage(c0);
         __syncthreads();  // the previous chunk's MFMA reads are done
 #pragma unroll
         for (int i = 0; i < ITER; ++i) {
@@ -258,6 +269,7 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
             if (u < UT) lds[u] = st[i];
         }
         __syncthreads();
+        if (kPipe && c0 + 16 < a.cin) stage(c0 + 16);
 #pragma unroll
         for (int q = 0; q < 14; ++q) {
             const int tap = 2 * q + (g >> 1);      // 27: the zero tap
