@@ -1,14 +1,14 @@
 // 3D lid-driven cavity projection step on gfx950.
 //
 // Replaces LidDrivenCavity3DSolver (src/solver/incompressible_3d.rs):
-//   predict_momentum :174-273   -> k_predict3_z
+//   predict_momentum :174-273   -> k_predict3_t
 //   build_pressure_rhs :275-291 -> k_rhs
 //   solve_pressure_poisson :293-332 -> pinned Jacobi-PCG; the reference only
 //       has SOR here, the PCG loop is the 2D solver's
 //       (src/solver/incompressible.rs:1048-1214) on the 7-point operator with
 //       cell (0,0,0) eliminated (src/poisson.rs:303-337 style)
 //   correct_velocity :334-363 and
-//   max_divergence / max_speed / ensure_finite :504-554 -> k_correct_diag_z (+ k_diag_fold)
+//   max_divergence / max_speed / ensure_finite :504-554 -> k_correct_diag_t (+ k_diag_fold)
 //
 // Data layout in HBM: the reference's own (src/field.rs:104-107), one flat
 // array per field, x fastest.  The PCG vectors x (= p), r, q and two
@@ -2428,143 +2428,156 @@ __device__ inline double w_nb(const StepParams &P, const double *w, int i, int j
 }
 
 // predict_momentum (incompressible_3d.rs:174-273): central, advective form,
-// all three face families in one column march over the cells: thread (i, j)
-// walks k and writes u(i, j, k), v(i, j, k) and w(i, j, k) (the last column,
-// row and plane also write the boundary faces u(nx, .), v(., ny),
+// all three face families in one march over the cells: the thread of cell
+// (i, j) walks k and writes u(i, j, k), v(i, j, k) and w(i, j, k) (the last
+// column, row and plane also write the boundary faces u(nx, .), v(., ny),
 // w(., ., nz)).  Boundary-normal faces are written 0 (the reference
 // zero-fills u*, v*, w*); neighbours outside the domain follow
 // u/v/w_neighbor (:366-447): the mirror -c, or 2 lid - c above the lid for
-// u.  Each velocity array is streamed once: the plane-k+1 values the next
-// step needs (u, v, w at (i, j) and w at (i-1, j), (i, j-1)) and the
-// plane-k-1 values (u, v, w at (i, j), u at (i+1, j), v at (i, j+1)) stay in
-// registers; the other in-plane neighbours are loads that hit in L1/L2.  The
-// divisions by 2 dx and dx^2 are multiplications by reciprocals computed
-// once (within an ulp of the quotient).
+// u.  The divisions by 2 dx and dx^2 are multiplications by reciprocals
+// computed once (within an ulp of the quotient).
 //
-// The loads of plane k+1 are issued before plane k is computed and stored,
-// so a plane step waits for loads issued one step earlier, never for its
-// own stores.  Every load is unconditional: a neighbour outside the domain
-// reads an in-bounds cell whose value the selects below never use, so the
-// wait counts stay exact (a load under a branch makes the compiler wait for
-// every outstanding access where the branches join).
-struct PredLoads {
-    double up, vp, wp, wwp, wsp;                          // plane k+1
-    double ue, uw, un, usn, use, ve, vw, vn, vsn, vnw, we, wn;  // plane k
-};
-__global__ void __launch_bounds__(kZT) k_predict3_z(StepParams P, const double *__restrict__ u,
-                                                    const double *__restrict__ v,
-                                                    const double *__restrict__ w,
-                                                    double *__restrict__ us, double *__restrict__ vs,
-                                                    double *__restrict__ ws) {
+// The march runs on plane tiles.  A block of 4 waves owns a 64 x 4 (x, y) tile of cells over a
+// chunk of planes; per plane it stages u, v and w over the tile and a
+// one-cell frame (66 x 6 values each, frame cells outside an array read a
+// clamped in-bounds cell that the selects never use) into one of three LDS
+// buffers.  Computing plane k reads planes k and k + 1 from LDS and plane
+// k - 1's values from registers; plane k + 2 is written to the third buffer
+// after plane k is computed, from loads issued one plane earlier, so a
+// buffer is never written while a slower wave still reads it and one
+// barrier per plane suffices.  A cell costs about 4.6 global loads (its
+// share of the three frames) instead of 17.
+constexpr int kPTX = 64, kPTY = 4, kPFX = kPTX + 2, kPFY = kPTY + 2, kPF = kPFX * kPFY;
+__global__ void __launch_bounds__(kBlock) k_predict3_t(StepParams P, const double *__restrict__ u,
+                                                       const double *__restrict__ v,
+                                                       const double *__restrict__ w,
+                                                       double *__restrict__ us, double *__restrict__ vs,
+                                                       double *__restrict__ ws) {
+    __shared__ double tile[3][3][kPF];  // [buffer][u, v, w][frame cell]
     const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
     const uint3 b = zblock();
-    const int i = (int)(b.x * kZX + threadIdx.x % kZX), j = (int)(b.y * kZY + threadIdx.x / kZX);
-    if (i >= nx || j >= ny) return;
     const int kz = (nz + (int)gridDim.z - 1) / (int)gridDim.z, k0 = (int)b.z * kz;
-    if (k0 >= nz) return;
+    if (k0 >= nz) return;  // block-uniform
     const int k1 = min(k0 + kz, nz);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x0 = (int)b.x * kPTX, y0 = (int)b.y * kPTY;
+    const int i = x0 + lane, j = y0 + wave;
+    const bool own = i < nx && j < ny;
     const uint32_t uy = (uint32_t)nx + 1, uz = (uint32_t)(nx + 1) * ny;  // u strides
     const uint32_t vy = (uint32_t)nx, vz = (uint32_t)nx * (ny + 1);      // v strides
     const uint32_t wy = (uint32_t)nx, wz = (uint32_t)nx * ny;            // w strides
-    const double hx = 1.0 / (2.0 * P.dx), hy = 1.0 / (2.0 * P.dy), hz = 1.0 / (2.0 * P.dz);
-    const double ix2 = 1.0 / (P.dx * P.dx), iy2 = 1.0 / (P.dy * P.dy), iz2 = 1.0 / (P.dz * P.dz);
-    const bool xw = i > 0, xe = i + 1 < nx, ys = j > 0, yn = j + 1 < ny;
-    // neighbour steps, 0 where the neighbour is outside the domain
-    const uint32_t dw = xw ? 1u : 0u, de = xe ? 1u : 0u, dsu = ys ? uy : 0u, dnu = yn ? uy : 0u,
-                   dsv = ys ? vy : 0u, dsw = ys ? wy : 0u, dnw = yn ? wy : 0u;
-    const uint32_t iu = (uint32_t)i + uy * j, iv = (uint32_t)i + vy * j, iw = (uint32_t)i + wy * j;
-    // loads at 32-bit byte offsets from the array bases (each array is below
-    // 4 GiB, checked at set-up), so an address is one VGPR
     auto ld = [](const double *base, uint32_t e) -> double {
         return *(const double *)((const char *)base + (e << 3));
     };
-    auto fetch = [&](int k) -> PredLoads {
-        const int kp = min(k + 1, nz - 1);  // u and v have nz planes, w nz + 1
-        const uint32_t cu = iu + uz * k, cv = iv + vz * k, cw = iw + wz * k, cwp = cw + wz;
-        PredLoads r;
-        r.up = ld(u, iu + uz * kp);
-        r.vp = ld(v, iv + vz * kp);
-        r.wp = ld(w, cwp);
-        r.wwp = ld(w, cwp - dw);
-        r.wsp = ld(w, cwp - dsw);
-        r.ue = ld(u, cu + 1);
-        r.uw = ld(u, cu - dw);
-        r.un = ld(u, cu + dnu);
-        r.usn = ld(u, cu - dsu);
-        r.use = ld(u, cu - dsu + 1);
-        r.ve = ld(v, cv + de);
-        r.vw = ld(v, cv - dw);
-        r.vn = ld(v, cv + vy);
-        r.vsn = ld(v, cv - dsv);
-        r.vnw = ld(v, cv + vy - dw);
-        r.we = ld(w, cw + de);
-        r.wn = ld(w, cw + dnw);
-        return r;
+    // this thread's frame cells (f = t and t + 256) and their clamped
+    // in-plane offsets in u, v and w
+    constexpr int kQ = (kPF + kBlock - 1) / kBlock;
+    uint32_t fu[kQ], fv[kQ], fw[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const int f = min((int)threadIdx.x + q * (int)kBlock, kPF - 1);
+        const int gx = x0 - 1 + f % kPFX, gy = y0 - 1 + f / kPFX;
+        const int cxu = min(max(gx, 0), nx), cxc = min(max(gx, 0), nx - 1);
+        const int cyc = min(max(gy, 0), ny - 1), cyv = min(max(gy, 0), ny);
+        fu[q] = (uint32_t)cxu + uy * (uint32_t)cyc;
+        fv[q] = (uint32_t)cxc + vy * (uint32_t)cyv;
+        fw[q] = (uint32_t)cxc + wy * (uint32_t)cyc;
+    }
+    double su[kQ], sv[kQ], sw[kQ];
+    auto stage_load = [&](int k) {  // plane k of u, v (clamped to nz - 1) and w (k <= nz)
+        const uint32_t pu = uz * (uint32_t)min(k, nz - 1), pv = vz * (uint32_t)min(k, nz - 1),
+                       pw = wz * (uint32_t)min(k, nz);
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            su[q] = ld(u, fu[q] + pu);
+            sv[q] = ld(v, fv[q] + pv);
+            sw[q] = ld(w, fw[q] + pw);
+        }
     };
-    // carried: plane k (u0, v0, w0, w at (i-1, j) and (i, j-1)) and plane k-1
-    double u0 = ld(u, iu + uz * k0), v0 = ld(v, iv + vz * k0), w0 = ld(w, iw + wz * k0);
-    double ww0 = ld(w, iw + wz * k0 - dw), ws0 = ld(w, iw + wz * k0 - dsw);
+    auto stage_store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const int f = (int)threadIdx.x + q * (int)kBlock;
+            if (f < kPF) {
+                tile[buf][0][f] = su[q];
+                tile[buf][1][f] = sv[q];
+                tile[buf][2][f] = sw[q];
+            }
+        }
+    };
+    const double hx = 1.0 / (2.0 * P.dx), hy = 1.0 / (2.0 * P.dy), hz = 1.0 / (2.0 * P.dz);
+    const double ix2 = 1.0 / (P.dx * P.dx), iy2 = 1.0 / (P.dy * P.dy), iz2 = 1.0 / (P.dz * P.dz);
+    const bool xw = i > 0, xe = i + 1 < nx, ys = j > 0, yn = j + 1 < ny;
+    const int fc = (wave + 1) * kPFX + lane + 1;  // own frame cell
+    // plane k0 - 1 (own values; the bottom wall's are never used)
+    const uint32_t iu = (uint32_t)min(i, nx - 1) + uy * (uint32_t)min(j, ny - 1),
+                   iv = (uint32_t)min(i, nx - 1) + vy * (uint32_t)min(j, ny - 1),
+                   iw = (uint32_t)min(i, nx - 1) + wy * (uint32_t)min(j, ny - 1);
     const int km = k0 > 0 ? k0 - 1 : k0;
     double um = ld(u, iu + uz * km), vm = ld(v, iv + vz * km), wm = ld(w, iw + wz * km),
            uem = ld(u, iu + uz * km + 1), vnm = ld(v, iv + vz * km + vy);
-    auto plane = [&](int k, const PredLoads &cur) {
-        const uint32_t cu = iu + uz * k, cv = iv + vz * k, cw = iw + wz * k;
-        const bool zt = k + 1 < nz;
-        const double up = cur.up, vp = cur.vp, wp = cur.wp, wwp = cur.wwp, wsp = cur.wsp;
-        const double ue = cur.ue, uw = cur.uw, un = cur.un, usn = cur.usn, use = cur.use;
-        const double ve = cur.ve, vw = cur.vw, vn = cur.vn, vsn = cur.vsn, vnw = cur.vnw;
-        const double we = cur.we, wn = cur.wn;
-        // u(i, j, k): boundary-normal at i = 0 (and i = nx, written below)
-        {
-            const double c = u0, e = ue, wv = uw;
-            const double n = yn ? un : 2.0 * P.lid - c, so = ys ? usn : -c;
-            const double f = zt ? up : -c, bk = k > 0 ? um : -c;
-            const double vv = 0.25 * (vw + v0 + vnw + vn);
-            const double wwv = 0.25 * (ww0 + w0 + wwp + wp);
-            const double val = c + P.dt * (-c * (e - wv) * hx - vv * (n - so) * hy - wwv * (f - bk) * hz +
-                                           P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
-                                                   (f - 2.0 * c + bk) * iz2));
-            us[cu] = xw ? val : 0.0;
+    stage_load(k0);
+    stage_store(k0 % 3);
+    stage_load(k0 + 1);
+    stage_store((k0 + 1) % 3);
+    if (k0 + 1 < k1) stage_load(k0 + 2);
+    __syncthreads();
+    for (int k = k0; k < k1; ++k) {
+        const double *U = tile[k % 3][0], *V = tile[k % 3][1], *W = tile[k % 3][2];
+        const double *U1 = tile[(k + 1) % 3][0], *V1 = tile[(k + 1) % 3][1], *W1 = tile[(k + 1) % 3][2];
+        const double u0 = U[fc], ue = U[fc + 1], uw = U[fc - 1], un = U[fc + kPFX], usn = U[fc - kPFX],
+                     use = U[fc - kPFX + 1];
+        const double v0 = V[fc], ve = V[fc + 1], vw = V[fc - 1], vn = V[fc + kPFX], vsn = V[fc - kPFX],
+                     vnw = V[fc + kPFX - 1];
+        const double w0 = W[fc], we = W[fc + 1], wn = W[fc + kPFX], ww0 = W[fc - 1], ws0 = W[fc - kPFX];
+        const double up = U1[fc], vp = V1[fc], wp = W1[fc], wwp = W1[fc - 1], wsp = W1[fc - kPFX];
+        if (own) {
+            const uint32_t cu = (uint32_t)i + uy * j + uz * k, cv = (uint32_t)i + vy * j + vz * k,
+                           cw = (uint32_t)i + wy * j + wz * k;
+            const bool zt = k + 1 < nz;
+            {
+                const double c = u0, e = ue, wv = uw;
+                const double n = yn ? un : 2.0 * P.lid - c, so = ys ? usn : -c;
+                const double f = zt ? up : -c, bk = k > 0 ? um : -c;
+                const double vv = 0.25 * (vw + v0 + vnw + vn);
+                const double wwv = 0.25 * (ww0 + w0 + wwp + wp);
+                const double val = c + P.dt * (-c * (e - wv) * hx - vv * (n - so) * hy - wwv * (f - bk) * hz +
+                                               P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
+                                                       (f - 2.0 * c + bk) * iz2));
+                us[cu] = xw ? val : 0.0;
+            }
+            if (!xe) us[cu + 1] = 0.0;
+            {
+                const double c = v0;
+                const double e = xe ? ve : -c, wv = xw ? vw : -c, n = vn, so = vsn;
+                const double f = zt ? vp : -c, bk = k > 0 ? vm : -c;
+                const double uu = 0.25 * (usn + use + u0 + ue);
+                const double wwv = 0.25 * (ws0 + w0 + wsp + wp);
+                const double val = c + P.dt * (-uu * (e - wv) * hx - c * (n - so) * hy - wwv * (f - bk) * hz +
+                                               P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
+                                                       (f - 2.0 * c + bk) * iz2));
+                vs[cv] = ys ? val : 0.0;
+            }
+            if (!yn) vs[cv + vy] = 0.0;
+            {
+                const double c = w0;
+                const double e = xe ? we : -c, wv = xw ? ww0 : -c, n = yn ? wn : -c, so = ys ? ws0 : -c;
+                const double f = wp, bk = wm;
+                const double uu = 0.25 * (um + uem + u0 + ue);
+                const double vv = 0.25 * (vm + vnm + v0 + vn);
+                const double val = c + P.dt * (-uu * (e - wv) * hx - vv * (n - so) * hy - c * (f - bk) * hz +
+                                               P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
+                                                       (f - 2.0 * c + bk) * iz2));
+                ws[cw] = k > 0 ? val : 0.0;
+            }
+            if (!zt) ws[cw + wz] = 0.0;
         }
-        if (!xe) us[cu + 1] = 0.0;
-        // v(i, j, k): boundary-normal at j = 0 (and j = ny, written below)
-        {
-            const double c = v0;
-            const double e = xe ? ve : -c, wv = xw ? vw : -c, n = vn, so = vsn;
-            const double f = zt ? vp : -c, bk = k > 0 ? vm : -c;
-            const double uu = 0.25 * (usn + use + u0 + ue);
-            const double wwv = 0.25 * (ws0 + w0 + wsp + wp);
-            const double val = c + P.dt * (-uu * (e - wv) * hx - c * (n - so) * hy - wwv * (f - bk) * hz +
-                                           P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
-                                                   (f - 2.0 * c + bk) * iz2));
-            vs[cv] = ys ? val : 0.0;
-        }
-        if (!yn) vs[cv + vy] = 0.0;
-        // w(i, j, k): boundary-normal at k = 0 (and k = nz, written below)
-        {
-            const double c = w0;
-            const double e = xe ? we : -c, wv = xw ? ww0 : -c, n = yn ? wn : -c, so = ys ? ws0 : -c;
-            const double f = wp, bk = wm;
-            const double uu = 0.25 * (um + uem + u0 + ue);
-            const double vv = 0.25 * (vm + vnm + v0 + vn);
-            const double val = c + P.dt * (-uu * (e - wv) * hx - vv * (n - so) * hy - c * (f - bk) * hz +
-                                           P.nu * ((e - 2.0 * c + wv) * ix2 + (n - 2.0 * c + so) * iy2 +
-                                                   (f - 2.0 * c + bk) * iz2));
-            ws[cw] = k > 0 ? val : 0.0;
-        }
-        if (!zt) ws[cw + wz] = 0.0;
         um = u0; vm = v0; wm = w0; uem = ue; vnm = vn;
-        u0 = up; v0 = vp; w0 = wp; ww0 = wwp; ws0 = wsp;
-    };
-    // two planes per trip, so the prefetched set is loaded in place (a copy
-    // of it at the end of a trip would wait for its loads)
-    PredLoads a = fetch(k0);
-    for (int k = k0; k < k1; k += 2) {
-        const PredLoads bq = fetch(min(k + 1, nz - 1));
-        plane(k, a);
-        if (k + 1 >= k1) break;
-        a = fetch(min(k + 2, nz - 1));
-        plane(k + 1, bq);
+        if (k + 1 < k1) {
+            stage_store((k + 2) % 3);
+            if (k + 2 < k1) stage_load(k + 3);
+            __syncthreads();
+        }
     }
 }
 
@@ -2593,118 +2606,156 @@ struct DiagState {
 
 // correct_velocity (:334-363) and, on the corrected field, max_divergence
 // (:504-517), max_speed (:519-532), sum(div^2) and ensure_finite's scan
-// (:534-554) of p, u, v, w, as one column march over the cells.  Thread
+// (:534-554) of p, u, v, w, as one march over the cells.  The thread of cell
 // (i, j) at plane k corrects its low faces u(i, j, k), v(i, j, k) (interior
-// faces only, as the reference) and the top face w(i, j, k+1) (carried up as
-// the next cell's bottom face; a chunk's first bottom face is recomputed,
-// not read, since the chunk below writes it).  The cell's high faces
-// u(i+1, .), v(., j+1) are recomputed from us, vs and p with the same
-// arithmetic their owners use, boundary faces are read (the correction
-// leaves them, and the thread that owns one stores it back unchanged).  A
-// face is checked for finiteness by the cell on its low side, the top
+// faces only, as the reference; a boundary face is stored back unchanged)
+// and the top face w(i, j, k+1), carried up as the next cell's bottom face
+// (a chunk's first bottom face is recomputed, not read, since the chunk
+// below writes it).  The cell's high faces u(i+1, .), v(., j+1) are
+// recomputed from us, vs and p with the same arithmetic their owners use.
+// A face is checked for finiteness by the cell on its low side, the top
 // boundary faces by the last cell of their row; each thread keeps the
 // lowest non-finite index per field and posts it once.  Each block
 // publishes one row {sum div^2 | max |div|, max speed}; k_diag_fold folds
 // them.
 //
-// As in k_predict3_z, plane k+1's loads are issued before plane k is
-// computed and stored, and no load is conditional: a missing neighbour
-// reads an in-bounds cell whose value the selects never use, and a boundary
-// face is read through a pointer select (the interior array at the same
-// index otherwise, a value this thread loads anyway).
-struct CorrLoads {
-    double pp, pw, pe, psn, pn, us0, us1, vs0, vs1, ws1, ub0, ub1, vb0, vb1, wb1;
-};
-__global__ void __launch_bounds__(kZT) k_correct_diag_z(StepParams P, const double *__restrict__ p,
-                                                        const double *__restrict__ us,
-                                                        const double *__restrict__ vs,
-                                                        const double *__restrict__ ws,
-                                                        double *__restrict__ u, double *__restrict__ v,
-                                                        double *__restrict__ w,
-                                                        double *__restrict__ pout, DiagState *dg) {
+// The march runs on plane tiles.  A block of 4 waves owns a 64 x 4 (x, y) tile of cells over a
+// chunk of planes.  Per plane it stages p over the tile and a one-cell
+// frame (66 x 6), the x faces us over the tile's 65 faces per row and the
+// y faces vs over its 5 face rows; a boundary face is staged from the
+// corrected field itself (u at x = 0 and x = nx, v at y = 0 and y = ny),
+// which is the value the correction keeps there.  Cell (i, j, k) then reads
+// p at its four in-plane neighbours and both of its x and y faces from LDS,
+// p at k + 1 from the next plane's tile and its top face ws (w on the top
+// wall) from a per-lane load; its bottom face is the previous plane's top.
+// Three LDS buffers rotate as in k_predict3_t (one barrier per plane).
+constexpr int kCFU = kPTX + 1, kCFV = kPTY + 1;  // face-tile widths
+constexpr int kCU = kCFU * kPTY, kCV = kPTX * kCFV;
+__global__ void __launch_bounds__(kBlock) k_correct_diag_t(StepParams P, const double *__restrict__ p,
+                                                           const double *__restrict__ us,
+                                                           const double *__restrict__ vs,
+                                                           const double *__restrict__ ws,
+                                                           double *__restrict__ u, double *__restrict__ v,
+                                                           double *__restrict__ w,
+                                                           double *__restrict__ pout, DiagState *dg) {
+    __shared__ double tp[3][kPF], tu[3][kCU], tv[3][kCV];
     const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
     double s[1] = {0.0}, m[2] = {0.0, 0.0};
     const uint3 b = zblock();
-    const int i = (int)(b.x * kZX + threadIdx.x % kZX), j = (int)(b.y * kZY + threadIdx.x / kZX);
-    const int kz = (nz + (int)gridDim.z - 1) / (int)gridDim.z, k0 = (int)b.z * kz, k1 = min(k0 + kz, nz);
-    if (i < nx && j < ny && k0 < nz) {
+    const int kz = (nz + (int)gridDim.z - 1) / (int)gridDim.z, k0 = (int)b.z * kz;
+    const int k1 = min(k0 + kz, nz);
+    if (k0 < nz) {  // block-uniform
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int x0 = (int)b.x * kPTX, y0 = (int)b.y * kPTY;
+        const int i = x0 + lane, j = y0 + wave;
+        const bool own = i < nx && j < ny;
         const double scale = P.dt / P.rho;
         const uint32_t uy = (uint32_t)nx + 1, uz = (uint32_t)(nx + 1) * ny, vy = nx, vz = (uint32_t)nx * (ny + 1),
                        wy = nx, wz = (uint32_t)nx * ny;
-        const bool xw = i > 0, xe = i + 1 < nx, ys = j > 0, yn = j + 1 < ny;
-        const uint32_t dw = xw ? 1u : 0u, de = xe ? 1u : 0u, dsw = ys ? wy : 0u, dnw = yn ? wy : 0u;
-        const uint32_t iu = (uint32_t)i + uy * j, iv = (uint32_t)i + vy * j, iw = (uint32_t)i + wy * j;
-        // boundary faces (read) or the interior array at the same index
-        const double *ub0p = xw ? us : u, *ub1p = xe ? us : u, *vb0p = ys ? vs : v, *vb1p = yn ? vs : v;
         auto ld = [](const double *base, uint32_t e) -> double {
             return *(const double *)((const char *)base + (e << 3));
         };
-        auto fetch = [&](int k) -> CorrLoads {
-            const bool zt = k + 1 < nz;
-            const uint32_t cu = iu + uz * k, cv = iv + vz * k, cw = iw + wz * k;
-            CorrLoads r;
-            r.pp = ld(p, cw + (zt ? wz : 0u));
-            r.pw = ld(p, cw - dw);
-            r.pe = ld(p, cw + de);
-            r.psn = ld(p, cw - dsw);
-            r.pn = ld(p, cw + dnw);
-            r.us0 = ld(us, cu);
-            r.us1 = ld(us, cu + 1);
-            r.vs0 = ld(vs, cv);
-            r.vs1 = ld(vs, cv + vy);
-            r.ws1 = ld(ws, cw + wz);
-            r.ub0 = ld(ub0p, cu);
-            r.ub1 = ld(ub1p, cu + 1);
-            r.vb0 = ld(vb0p, cv);
-            r.vb1 = ld(vb1p, cv + vy);
-            r.wb1 = ld(zt ? ws : w, cw + wz);
-            return r;
+        // staging duties: p frame cells t, t + 256; x faces t, t + 256 (of 260);
+        // y faces t, t + 256 (of 320).  Offsets are clamped in-plane; a face on
+        // the domain boundary is read from the corrected field.
+        constexpr int kQ = 2;
+        uint32_t op[kQ], ou[kQ], ov[kQ];
+        bool bu[kQ], bv[kQ];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const int t = (int)threadIdx.x + q * (int)kBlock;
+            const int fp = min(t, kPF - 1), fu = min(t, kCU - 1), fv = min(t, kCV - 1);
+            const int px = min(max(x0 - 1 + fp % kPFX, 0), nx - 1), py = min(max(y0 - 1 + fp / kPFX, 0), ny - 1);
+            op[q] = (uint32_t)px + wy * (uint32_t)py;
+            const int ux = min(x0 + fu % kCFU, nx), uyy = min(y0 + fu / kCFU, ny - 1);
+            ou[q] = (uint32_t)ux + uy * (uint32_t)uyy;
+            bu[q] = ux == 0 || ux == nx;
+            const int vx = min(x0 + fv % kPTX, nx - 1), vyy = min(y0 + fv / kPTX, ny);
+            ov[q] = (uint32_t)vx + vy * (uint32_t)vyy;
+            bv[q] = vyy == 0 || vyy == ny;
+        }
+        double sp[kQ], su[kQ], sv[kQ];
+        auto stage_load = [&](int k) {  // plane k (p clamped to nz - 1)
+            const uint32_t kk = (uint32_t)min(k, nz - 1);
+            const uint32_t pp = wz * kk, pu = uz * kk, pv = vz * kk;
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                sp[q] = ld(p, op[q] + pp);
+                su[q] = ld(bu[q] ? u : us, ou[q] + pu);
+                sv[q] = ld(bv[q] ? v : vs, ov[q] + pv);
+            }
         };
+        auto stage_store = [&](int buf) {
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const int t = (int)threadIdx.x + q * (int)kBlock;
+                if (t < kPF) tp[buf][t] = sp[q];
+                if (t < kCU) tu[buf][t] = su[q];
+                if (t < kCV) tv[buf][t] = sv[q];
+            }
+        };
+        const bool xw = i > 0, xe = i + 1 < nx, ys = j > 0, yn = j + 1 < ny;
+        const int fcp = (wave + 1) * kPFX + lane + 1, fcu = wave * kCFU + lane, fcv = wave * kPTX + lane;
+        const uint32_t iw = (uint32_t)min(i, nx - 1) + wy * (uint32_t)min(j, ny - 1);
+        // the top face of plane k: ws, or w on the top wall
+        auto top_face = [&](int k) -> double { return ld(k + 1 < nz ? ws : w, iw + wz * (uint32_t)(k + 1)); };
         unsigned long long bad[4] = {~0ull, ~0ull, ~0ull, ~0ull};
         auto flag = [&](int f, bool finite, uint32_t idx) {
             if (!finite) bad[f] = min(bad[f], (unsigned long long)idx);
         };
-        double pc = ld(p, iw + wz * k0);  // cells share w's layout below nz
-        // bottom face of the first cell
-        double w0 = k0 == 0 ? ld(w, iw) : ld(ws, iw + wz * k0) - scale * (pc - ld(p, iw + wz * (k0 - 1))) / P.dz;
-        auto plane = [&](int k, const CorrLoads &L) {
-            const uint32_t cu = iu + uz * k, cv = iv + vz * k, cw = iw + wz * k;
-            const bool zt = k + 1 < nz;
-            const double u0 = xw ? L.us0 - scale * (pc - L.pw) / P.dx : L.ub0;
-            const double u1 = xe ? L.us1 - scale * (L.pe - pc) / P.dx : L.ub1;
-            const double v0 = ys ? L.vs0 - scale * (pc - L.psn) / P.dy : L.vb0;
-            const double v1 = yn ? L.vs1 - scale * (L.pn - pc) / P.dy : L.vb1;
-            const double w1 = zt ? L.ws1 - scale * (L.pp - pc) / P.dz : L.wb1;
-            *(double *)((char *)u + (cu << 3)) = u0;
-            *(double *)((char *)v + (cv << 3)) = v0;
-            *(double *)((char *)w + ((cw + wz) << 3)) = w1;
-            const double div = (u1 - u0) / P.dx + (v1 - v0) / P.dy + (w1 - w0) / P.dz;
-            s[0] += div * div;
-            m[0] = fmax(m[0], fabs(div));
-            const double a = 0.5 * (u0 + u1), bb = 0.5 * (v0 + v1), cc = 0.5 * (w0 + w1);
-            m[1] = fmax(m[1], sqrt(a * a + bb * bb + cc * cc));
-            flag(0, isfinite(pc), cw);
-            flag(1, isfinite(u0), cu);
-            flag(1, xe || isfinite(u1), cu + 1);
-            flag(2, isfinite(v0), cv);
-            flag(2, yn || isfinite(v1), cv + vy);
-            flag(3, isfinite(w0), cw);
-            flag(3, zt || isfinite(w1), cw + wz);
-            w0 = w1;
-            pc = L.pp;
-        };
-        // two planes per trip, so the prefetched set is loaded in place
-        CorrLoads a = fetch(k0);
-        for (int k = k0; k < k1; k += 2) {
-            const CorrLoads bq = fetch(min(k + 1, nz - 1));
-            plane(k, a);
-            if (k + 1 >= k1) break;
-            a = fetch(min(k + 2, nz - 1));
-            plane(k + 1, bq);
+        // bottom face of the first cell (the chunk below writes it)
+        double w0 = k0 == 0 ? ld(w, iw) : ld(ws, iw + wz * k0) - scale * (ld(p, iw + wz * k0) - ld(p, iw + wz * (k0 - 1))) / P.dz;
+        double wt = top_face(k0);
+        stage_load(k0);
+        stage_store(k0 % 3);
+        stage_load(k0 + 1);
+        stage_store((k0 + 1) % 3);
+        if (k0 + 1 < k1) stage_load(k0 + 2);
+        __syncthreads();
+        for (int k = k0; k < k1; ++k) {
+            const double wtn = top_face(min(k + 1, nz - 1));  // the next plane's, in flight meanwhile
+            const double *TP = tp[k % 3], *TU = tu[k % 3], *TV = tv[k % 3], *TP1 = tp[(k + 1) % 3];
+            const double pc = TP[fcp], pw = TP[fcp - 1], pe = TP[fcp + 1], psn = TP[fcp - kPFX],
+                         pn = TP[fcp + kPFX], pp = TP1[fcp];
+            const double us0 = TU[fcu], us1 = TU[fcu + 1], vs0 = TV[fcv], vs1 = TV[fcv + kPTX];
+            if (own) {
+                const uint32_t cu = (uint32_t)i + uy * j + uz * k, cv = (uint32_t)i + vy * j + vz * k,
+                               cw = (uint32_t)i + wy * j + wz * k;
+                const bool zt = k + 1 < nz;
+                const double u0 = xw ? us0 - scale * (pc - pw) / P.dx : us0;
+                const double u1 = xe ? us1 - scale * (pe - pc) / P.dx : us1;
+                const double v0 = ys ? vs0 - scale * (pc - psn) / P.dy : vs0;
+                const double v1 = yn ? vs1 - scale * (pn - pc) / P.dy : vs1;
+                const double w1 = zt ? wt - scale * (pp - pc) / P.dz : wt;
+                *(double *)((char *)u + (cu << 3)) = u0;
+                *(double *)((char *)v + (cv << 3)) = v0;
+                *(double *)((char *)w + ((cw + wz) << 3)) = w1;
+                const double div = (u1 - u0) / P.dx + (v1 - v0) / P.dy + (w1 - w0) / P.dz;
+                s[0] += div * div;
+                m[0] = fmax(m[0], fabs(div));
+                const double a = 0.5 * (u0 + u1), bb = 0.5 * (v0 + v1), cc = 0.5 * (w0 + w1);
+                m[1] = fmax(m[1], sqrt(a * a + bb * bb + cc * cc));
+                flag(0, isfinite(pc), cw);
+                flag(1, isfinite(u0), cu);
+                flag(1, xe || isfinite(u1), cu + 1);
+                flag(2, isfinite(v0), cv);
+                flag(2, yn || isfinite(v1), cv + vy);
+                flag(3, isfinite(w0), cw);
+                flag(3, zt || isfinite(w1), cw + wz);
+                w0 = w1;
+            }
+            wt = wtn;
+            if (k + 1 < k1) {
+                stage_store((k + 2) % 3);
+                if (k + 2 < k1) stage_load(k + 3);
+                __syncthreads();
+            }
         }
+        if (own) {
 #pragma unroll
-        for (int f = 0; f < 4; ++f)
-            if (bad[f] != ~0ull) atomicMin(&dg->first_bad[f], bad[f]);
+            for (int f = 0; f < 4; ++f)
+                if (bad[f] != ~0ull) atomicMin(&dg->first_bad[f], bad[f]);
+        }
     }
     publish_row<1, 2>(s, m, pout);
 }
@@ -2808,7 +2859,7 @@ enum Kind {
     K_COUNT
 };
 const char *const kKindNames[K_COUNT] = {
-    "k_predict3_z", "k_rhs", "k_correct_diag_z",
+    "k_predict3_t", "k_rhs", "k_correct_diag_t",
     "k_residual_z<double>", "k_residual_z<float>", "k_fold_outer", "k_add_correction",
     "k_cg_dirapply_z<double>", "k_cg_update<double, double>",
     "k_cg_dirapply_z<float>", "k_cg_update<float, float>",
@@ -2901,6 +2952,7 @@ struct fl_cavity3d {
     // plane-tile level-0 cycle passes: stop test and first sweeps, sweep,
     // prolongation, publishing sweep (count 0: the column marches run)
     ZGrid ztc{0, 0, 0, 0}, zts0{0, 0, 0, 0}, ztp0{0, 0, 0, 0}, zts1{0, 0, 0, 0};
+    ZGrid zpt{1, 1, 1, 1}, zct{1, 1, 1, 1};  // plane-tile predictor and correction
     int64_t step = 0;
     KProf prof;  // per-kernel timing (HIP events on the solver's stream)
     double *cell_out = nullptr;  // output staging: cell-centred u, v, w (allocated on first use)
@@ -3782,8 +3834,8 @@ fl_status correct_and_diag(fl_cavity3d *s, fl_cavity3d_step_report *rep) {
     const double faces = 8.0 * (double)(s->nu + s->nv + s->nw), cells = 8.0 * s->n;
     const dim3 zg(s->zg.bx, s->zg.by, s->zg.bz);
     FL_TIMED(s, K_CORRECT, -1, 2.0 * faces + cells,
-             k_correct_diag_z<<<zg, kZT, 0, stm>>>(s->P, s->p, s->us, s->vs, s->ws, s->u, s->v, s->w, s->partials, s->dg);
-             k_diag_fold<<<1, kBlock, 0, stm>>>(s->partials, s->zg.count, s->dg));
+             k_correct_diag_t<<<dim3(s->zct.bx, s->zct.by, s->zct.bz), kBlock, 0, stm>>>(s->P, s->p, s->us, s->vs, s->ws, s->u, s->v, s->w, s->partials, s->dg);
+             k_diag_fold<<<1, kBlock, 0, stm>>>(s->partials, s->zct.count, s->dg));
     FL_HIP_TRY(ctx, hipGetLastError());
     FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->dg, s->dg, sizeof(DiagState), hipMemcpyDeviceToHost, stm));
     FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
@@ -3815,7 +3867,7 @@ fl_status do_step(fl_cavity3d *s, fl_cavity3d_step_report *rep) {
     // algorithmic bytes: every face array read or written once, cells once
     const double faces = 8.0 * (double)(s->nu + s->nv + s->nw), cells = 8.0 * s->n;
     FL_TIMED(s, K_PREDICT, -1, 2.0 * faces,
-             k_predict3_z<<<dim3(s->zg.bx, s->zg.by, s->zg.bz), kZT, 0, stm>>>(s->P, s->u, s->v, s->w, s->us, s->vs, s->ws));
+             k_predict3_t<<<dim3(s->zpt.bx, s->zpt.by, s->zpt.bz), kBlock, 0, stm>>>(s->P, s->u, s->v, s->w, s->us, s->vs, s->ws));
     FL_TIMED(s, K_RHS, -1, faces + cells,
              k_rhs<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->us, s->vs, s->ws, s->rhs));
     FL_HIP_TRY(ctx, hipGetLastError());
@@ -3921,6 +3973,28 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
         s->ztfs = tile_grid_resident<float>(ctx, (const void *)k_cg_step_t<float>, nx, ny, nz);
     }
     // the FCG direction's tile pass (16-byte float and 8-byte FP16 lane loads)
+    {  // plane-tile predictor: 64 x 4 tiles, one round of resident blocks
+        int fit = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, (const void *)k_predict3_t, kBlock, 0) != hipSuccess ||
+            fit < 1)
+            fit = 1;
+        const uint32_t bx = (nx + kPTX - 1) / kPTX, by = (ny + kPTY - 1) / kPTY;
+        const uint64_t slots = (uint64_t)fit * (uint64_t)std::max(ctx->num_cu, 1);
+        uint32_t bz = (uint32_t)std::max<uint64_t>(1, slots / ((uint64_t)bx * by));
+        bz = std::min(bz, std::max(1u, nz / 4));
+        const uint32_t kz = (nz + bz - 1) / bz;
+        bz = (nz + kz - 1) / kz;
+        s->zpt = ZGrid{bx, by, bz, bx * by * bz};
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, (const void *)k_correct_diag_t, kBlock, 0) != hipSuccess ||
+            fit < 1)
+            fit = 1;
+        const uint64_t cslots = (uint64_t)fit * (uint64_t)std::max(ctx->num_cu, 1);
+        uint32_t cz = (uint32_t)std::max<uint64_t>(1, cslots / ((uint64_t)bx * by));
+        cz = std::min(cz, std::max(1u, nz / 4));
+        const uint32_t ckz = (nz + cz - 1) / cz;
+        cz = (nz + ckz - 1) / ckz;
+        s->zct = ZGrid{bx, by, cz, bx * by * cz};
+    }
     if (nx % 4 == 0 && cfg->preconditioner != FL_PRECOND_JACOBI) {
         s->ztd = tile_grid_resident<float>(ctx, (const void *)k_fcg_dirapply_t, nx, ny, nz);
         s->ztc = tile_grid_resident<float>(ctx, (const void *)k_mg_check_t, nx, ny, nz);
@@ -3929,7 +4003,8 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
         s->zts1 = tile_grid_resident<float>(ctx, (const void *)k_mg_sweep0_t<true>, nx, ny, nz);
     }
     const size_t prow =
-        (size_t)std::max(std::max(std::max(s->blocks, s->ew_blocks), std::max(s->zg.count, s->zgd.count)),
+        (size_t)std::max(std::max(std::max(std::max(s->blocks, s->ew_blocks), std::max(s->zg.count, s->zgd.count)),
+                                  std::max(s->zpt.count, s->zct.count)),
                          std::max(std::max(std::max(s->zt.count, s->ztf.count), std::max(s->zts.count, s->ztfs.count)),
                                   std::max(s->ztd.count, s->zts1.count))) * 4;
     if ((st = alloc(s, &s->u, s->nu)) || (st = alloc(s, &s->v, s->nv)) ||
