@@ -214,11 +214,18 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x
 // x += e (FP64 += FP32): the refinement step of the mixed path.  A
 // correction solved on the Neumann operator is defined up to a constant:
 // with `shift` it is taken as e - e_0, the one that keeps the pin x_0 = 0.
+// x += e - shift e_0.  The flexible-CG solution arrives in two parts: e
+// (valid after two updates or more) and a deferred last step ta * pend
+// (pend null: none).
 __global__ void __launch_bounds__(kBlock) k_add_correction(uint32_t n, double *__restrict__ x,
-                                                           const float *__restrict__ e, int shift) {
+                                                           const float *__restrict__ e, int shift, int e_valid,
+                                                           const float *__restrict__ pend, float ta) {
     const double e0 = shift ? (double)e[0] : 0.0;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
-        x[c] += (double)e[c] - e0;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        double d = e_valid ? (double)e[c] - e0 : 0.0;
+        if (pend) d += (double)(ta * pend[c]);
+        x[c] += d;
+    }
 }
 
 // The compatible Neumann form of the pinned residual for the inner Jacobi
@@ -277,7 +284,8 @@ struct FcgA {  // written by block 0 of k_mg_check (the stop test)
 struct FcgB {  // written by block 0 of k_fcg_update (and the host)
     double pq;
     long long iters;
-    int done, status, first, pad;
+    int done, status, first;
+    float alpha_even;  // alpha of the last even-numbered update (its x update is deferred)
 };
 
 __device__ inline void lvl_ijk(const Level &L, uint32_t c, uint32_t &i, uint32_t &j, uint32_t &k) {
@@ -2313,14 +2321,22 @@ __global__ void __launch_bounds__(kBlock) k_mg_prolong0_t(Level L, Level LC, con
         });
 }
 
-// alpha = r.p / p.q; x += alpha p, r -= alpha q; partials r.r, sum r, max|r|.
+// alpha = r.p / p.q; r -= alpha q; partials r.r, sum r, max|r|.  x is
+// updated every second iteration: update k even only keeps its alpha (in
+// SB->alpha_even, read by update k + 1, so no block reads what this launch
+// writes), update k odd adds both, x += alpha_{k-1} p_{k-1} + alpha_k p_k
+// (p_{k-1} is still in the other direction buffer), and update 1 writes x
+// outright, so x needs no zero fill.  This saves reading and writing x in
+// half of the iterations; a solve that ends after an even-numbered update
+// leaves alpha_even p_{k} for k_add_correction.
 __global__ void __launch_bounds__(kBlock) k_fcg_update(uint32_t n, float *__restrict__ x,
                                                        float *__restrict__ r,
                                                        const float *__restrict__ p,
+                                                       const float *__restrict__ pprev,
                                                        const float *__restrict__ q,
                                                        const double *__restrict__ pin, uint32_t rows,
                                                        double *__restrict__ pout, const FcgA *SA,
-                                                       FcgB *SB) {
+                                                       FcgB *SB, long long k) {
     if (SA->done || SB->done) return;
     double f[2], fm[1];
     fold_partials<2, 1>(pin, rows, f, fm);
@@ -2332,33 +2348,44 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update(uint32_t n, float *__rest
         alpha = rp / pq;
         if (!isfinite(alpha)) status = ST_ALPHA;
     }
+    const float ta = (float)alpha;
+    const bool odd = (k & 1) != 0;
+    float tprev = 0.0f;
+    if (odd) tprev = SB->alpha_even;  // written by update k - 1 (an even k writes, never reads it)
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         SB->pq = pq;
         SB->first = 0;
         SB->status = status;
+        if (!odd) SB->alpha_even = ta;
         if (status != ST_OK) SB->done = 1;
         else SB->iters += 1;
     }
     if (status != ST_OK) return;
-    const float ta = (float)alpha;
     double s[3] = {0.0, 0.0, 0.0}, m[1] = {0.0};
     auto acc = [&](float rc) {
         s[1] += (double)rc * (double)rc;
         s[2] += (double)rc;
         m[0] = fmax(m[0], fabs((double)rc));
     };
+    // x: 0 untouched (k even), 1 written (k == 1), 2 accumulated
+    const int xmode = !odd ? 0 : (k == 1 ? 1 : 2);
     if (n % 2 == 0) {  // two cells per lane, 8-byte loads and stores
         TileSched ts((n / 2 + kBlock - 1) / kBlock);
         for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
             const uint32_t c = 2 * (t * kBlock + threadIdx.x);
             if (c >= n) continue;
-            float2 xc = *(const float2 *)(x + c), rc = *(const float2 *)(r + c);
-            const float2 pc = *(const float2 *)(p + c), qc = *(const float2 *)(q + c);
-            xc.x += ta * pc.x;
-            xc.y += ta * pc.y;
+            float2 rc = *(const float2 *)(r + c);
+            const float2 qc = *(const float2 *)(q + c);
+            if (xmode != 0) {
+                const float2 pc = *(const float2 *)(p + c), pm = *(const float2 *)(pprev + c);
+                float2 xc = make_float2(0.0f, 0.0f);
+                if (xmode == 2) xc = *(const float2 *)(x + c);
+                xc.x += tprev * pm.x + ta * pc.x;
+                xc.y += tprev * pm.y + ta * pc.y;
+                *(float2 *)(x + c) = xc;
+            }
             rc.x -= ta * qc.x;
             rc.y -= ta * qc.y;
-            *(float2 *)(x + c) = xc;
             *(float2 *)(r + c) = rc;
             acc(rc.x);
             acc(rc.y);
@@ -2368,7 +2395,7 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update(uint32_t n, float *__rest
         for (uint32_t t = ts.first; t < ts.count; t += ts.step) {
             const uint32_t c = t * kBlock + threadIdx.x;
             if (c >= n) continue;
-            x[c] += ta * p[c];
+            if (xmode != 0) x[c] = (xmode == 2 ? x[c] : 0.0f) + (tprev * pprev[c] + ta * p[c]);
             const float rc = r[c] - ta * q[c];
             r[c] = rc;
             acc(rc);
@@ -3496,8 +3523,11 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
         FL_TIMED(s, K_FCG_DIRAPPLY, k, 18.0 * n0,
                  k_fcg_dirapply_z<<<dim3(s->zgd.bx, s->zgd.by, s->zgd.bz), kZT, 0, stm>>>(s->g, s->m0b, pold, pnew, s->q32, s->r32, s->pz, zrows, s->pa, s->fa, s->fb));
     }
-    FL_TIMED(s, K_FCG_UPDATE, k, 24.0 * n0,
-             k_fcg_update<<<G, kBlock, 0, stm>>>((uint32_t)s->n, s->e32, s->r32, pnew, s->q32, s->pa, drows, s->pb, s->fa, s->fb));
+    // bytes per cell: r, q read and r written; with the x update (k odd)
+    // also p and p_prev read and x written (and read, after the first pair)
+    const double ubytes = (k & 1) == 0 ? 12.0 : (k == 1 ? 24.0 : 28.0);
+    FL_TIMED(s, K_FCG_UPDATE, k, ubytes * n0,
+             k_fcg_update<<<G, kBlock, 0, stm>>>((uint32_t)s->n, s->e32, s->r32, pnew, pold, s->q32, s->pa, drows, s->pb, s->fa, s->fb, k));
     return FL_OK;
 }
 
@@ -3692,7 +3722,8 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
             warm = 2;
             continue;
         }
-        FL_HIP_TRY(ctx, hipMemsetAsync(s->e32, 0, s->n * sizeof(float), stm));
+        // the flexible CG writes its solution at its second update (no fill)
+        if (!mg) FL_HIP_TRY(ctx, hipMemsetAsync(s->e32, 0, s->n * sizeof(float), stm));
         long long it = 0;
         int status = ST_OK;
         bool budget_hit = false, stagnated = false;
@@ -3730,9 +3761,16 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
         if (status != ST_OK && !mg) return set_error(ctx, FL_ERR_SOLVER, "%s", status_text(status));
         total += it;
         sweeps += 1;
-        if (status == ST_OK && it > 0)
-            FL_TIMED(s, K_ADD_CORRECTION, -1, 20.0 * s->n,
-                     k_add_correction<<<s->ew_blocks, kBlock, 0, stm>>>((uint32_t)s->n, s->p, s->e32, mg ? 0 : 1));
+        if (status == ST_OK && it > 0) {
+            // flexible CG: e holds updates 0 .. 2 floor(it / 2) - 1; an odd
+            // count leaves alpha_even p_{it-1} (in d32[it & 1]) pending
+            const bool pend = mg && (it & 1);
+            const int e_valid = mg ? (it >= 2) : 1;
+            FL_TIMED(s, K_ADD_CORRECTION, -1, (pend ? 24.0 : 20.0) * s->n,
+                     k_add_correction<<<s->ew_blocks, kBlock, 0, stm>>>((uint32_t)s->n, s->p, s->e32, mg ? 0 : 1, e_valid,
+                                                                        pend ? s->d32[it & 1] : nullptr,
+                                                                        s->host->fb.alpha_even));
+        }
         if (mg && (status != ST_OK || stagnated || budget_hit)) {
             fallback = true;
             reason = status != ST_OK ? FL_FALLBACK_BREAKDOWN
