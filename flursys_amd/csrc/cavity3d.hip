@@ -33,6 +33,7 @@
 // x += e in FP64, until the outer FP64 stop test holds.
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -1204,6 +1205,14 @@ __device__ inline int fcg_stop_test(const CgParams &prm, const double *__restric
         SA->done = done;
     }
     return done;
+}
+
+// The state of a new inner flexible-CG solve: all zero, first iteration.
+__global__ void k_fcg_reset(FcgA *SA, FcgB *SB) {
+    *SA = FcgA{};
+    FcgB b{};
+    b.first = 1;
+    *SB = b;
 }
 
 // The stop test alone (one block), launched after a batch of iterations so
@@ -2786,6 +2795,12 @@ __global__ void __launch_bounds__(kBlock) k_correct_diag_t(StepParams P, const d
     }
     publish_row<1, 2>(s, m, pout);
 }
+// The diagnostics state of a step: sums and maxima zero, no non-finite value.
+__global__ void k_diag_reset(DiagState *dg) {
+    DiagState d{};
+    for (int f = 0; f < 4; ++f) d.first_bad[f] = ~0ull;
+    *dg = d;
+}
 __global__ void __launch_bounds__(kBlock) k_diag_fold(const double *__restrict__ pin, uint32_t rows,
                                                       DiagState *dg) {
     double s[1], m[2];
@@ -2952,6 +2967,8 @@ struct fl_cavity3d {
     bool p_zero = false;
     float *cap = nullptr;
     double *pz = nullptr;  // partials z.q_old, sum q_old
+    // the flexible-CG state: one allocation, fb directly after fa (as in
+    // Host), so one copy brings both to the host
     FcgA *fa = nullptr;
     FcgB *fb = nullptr;
     struct Host {
@@ -2962,6 +2979,8 @@ struct fl_cavity3d {
         DiagState dg;
         double outer[4];
     } *host = nullptr;
+    static_assert(offsetof(Host, fb) == offsetof(Host, fa) + sizeof(FcgA), "fa, fb adjacent in Host");
+    static_assert(sizeof(FcgA) % alignof(FcgB) == 0, "fb after fa is aligned");
     double *outer = nullptr;  // folded outer residual: r.z, r.r, max|r|
     SorState *sor = nullptr;  // SOR iteration state (FL_SOLVER_SOR)
     uint32_t blocks = 1, ew_blocks = 1;
@@ -3016,7 +3035,7 @@ void free_all(fl_cavity3d *s) {
                     s->sb,  s->dg,     s->outer};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    void *more[] = {s->pz, s->fa, s->fb, s->wl_in, s->wl_out, s->pair, s->pinv, s->m0a, s->m0b, s->cap,
+    void *more[] = {s->pz, s->fa, s->wl_in, s->wl_out, s->pair, s->pinv, s->m0a, s->m0b, s->cap,
                     s->sor, s->cell_out};
     for (void *b : more)
         if (b) (void)hipFree(b);
@@ -3541,8 +3560,7 @@ fl_status run_fcg(fl_cavity3d *s, const CgParams &prm, int hint, bool *stagnated
     s->host->fa = FcgA{};
     s->host->fb = FcgB{};
     s->host->fb.first = 1;
-    FL_HIP_TRY(ctx, hipMemcpyAsync(s->fa, &s->host->fa, sizeof(FcgA), hipMemcpyHostToDevice, stm));
-    FL_HIP_TRY(ctx, hipMemcpyAsync(s->fb, &s->host->fb, sizeof(FcgB), hipMemcpyHostToDevice, stm));
+    k_fcg_reset<<<1, 1, 0, stm>>>(s->fa, s->fb);  // the same state, without a copy
     *stagnated = false;
     // A batch of iterations is followed by the stop test alone (k_fcg_check),
     // so after launching iterations 0..L-1 the host knows rr_L, the residual
@@ -3560,8 +3578,7 @@ fl_status run_fcg(fl_cavity3d *s, const CgParams &prm, int hint, bool *stagnated
         }
         k_fcg_check<<<1, kBlock, 0, stm>>>(prm, s->pb, s->blocks, s->fa, s->fb);
         FL_HIP_TRY(ctx, hipGetLastError());
-        FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->fa, s->fa, sizeof(FcgA), hipMemcpyDeviceToHost, stm));
-        FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->fb, s->fb, sizeof(FcgB), hipMemcpyDeviceToHost, stm));
+        FL_HIP_TRY(ctx, hipMemcpyAsync(&s->host->fa, s->fa, sizeof(FcgA) + sizeof(FcgB), hipMemcpyDeviceToHost, stm));
         FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
         FL_HIP_TRY(ctx, s->prof.flush(s->host->fb.iters));
         if (s->host->fa.done || s->host->fb.done) break;
@@ -3864,10 +3881,7 @@ fl_status solve_pressure(fl_cavity3d *s, fl_solve_report *rep) {
 fl_status correct_and_diag(fl_cavity3d *s, fl_cavity3d_step_report *rep) {
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
-    DiagState init{};
-    for (int f = 0; f < 4; ++f) init.first_bad[f] = ~0ull;
-    s->host->dg = init;
-    FL_HIP_TRY(ctx, hipMemcpyAsync(s->dg, &s->host->dg, sizeof(DiagState), hipMemcpyHostToDevice, stm));
+    k_diag_reset<<<1, 1, 0, stm>>>(s->dg);  // (a kernel, not a host-to-device copy)
     // algorithmic bytes: us, vs, ws read, u, v, w written, p read
     const double faces = 8.0 * (double)(s->nu + s->nv + s->nw), cells = 8.0 * s->n;
     const dim3 zg(s->zg.bx, s->zg.by, s->zg.bz);
@@ -4068,8 +4082,9 @@ fl_status fl_cavity3d_create(fl_ctx *ctx, const fl_cavity3d_config *cfg, fl_cavi
     }
     if (cfg->preconditioner != FL_PRECOND_JACOBI) {
         if ((st = build_hierarchy(s)) != FL_OK) goto fail;
-        if ((st = alloc(s, &s->pz, prow)) || (st = alloc(s, &s->fa, 1)) || (st = alloc(s, &s->fb, 1)))
+        if ((st = alloc(s, &s->pz, prow)) || (st = alloc(s, &s->fa, 2)))
             goto fail;
+        s->fb = reinterpret_cast<FcgB *>(s->fa + 1);  // sizeof(FcgA) keeps FcgB's alignment
         for (int l = 0; l <= s->tail; ++l) {
             const size_t n = s->lv[l].n;
             if (l == 0) {
