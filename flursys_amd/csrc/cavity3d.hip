@@ -215,17 +215,18 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Grid g, TX *__restrict__ x
 // x += e (FP64 += FP32): the refinement step of the mixed path.  A
 // correction solved on the Neumann operator is defined up to a constant:
 // with `shift` it is taken as e - e_0, the one that keeps the pin x_0 = 0.
-// x += e - shift e_0.  The flexible-CG solution arrives in two parts: e
-// (valid after two updates or more) and a deferred last step ta * pend
-// (pend null: none).
+// x += e - shift e_0 (overwrite: x = e - shift e_0, x is zero and not
+// read).  The flexible-CG solution arrives in two parts: e (valid after two
+// updates or more) and a deferred last step ta * pend (pend null: none).
 __global__ void __launch_bounds__(kBlock) k_add_correction(uint32_t n, double *__restrict__ x,
                                                            const float *__restrict__ e, int shift, int e_valid,
-                                                           const float *__restrict__ pend, float ta) {
+                                                           const float *__restrict__ pend, float ta,
+                                                           int overwrite) {
     const double e0 = shift ? (double)e[0] : 0.0;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         double d = e_valid ? (double)e[c] - e0 : 0.0;
         if (pend) d += (double)(ta * pend[c]);
-        x[c] += d;
+        x[c] = overwrite ? d : x[c] + d;
     }
 }
 
@@ -2965,6 +2966,10 @@ struct fl_cavity3d {
     // p is all zeros (a zero initial guess, nothing written since): the next
     // residual is r = b and reads no p
     bool p_zero = false;
+    // p's contents stand for zero and were not cleared (a zero guess solved
+    // by the mixed multigrid path): the first correction writes p = e, and a
+    // solve that ends before any correction clears p then
+    bool p_stale = false;
     float *cap = nullptr;
     double *pz = nullptr;  // partials z.q_old, sum q_old
     // the flexible-CG state: one allocation, fb directly after fa (as in
@@ -3783,10 +3788,11 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
             // count leaves alpha_even p_{it-1} (in d32[it & 1]) pending
             const bool pend = mg && (it & 1);
             const int e_valid = mg ? (it >= 2) : 1;
-            FL_TIMED(s, K_ADD_CORRECTION, -1, (pend ? 24.0 : 20.0) * s->n,
+            FL_TIMED(s, K_ADD_CORRECTION, -1, ((pend ? 16.0 : 12.0) + (s->p_stale ? 0.0 : 8.0)) * s->n,
                      k_add_correction<<<s->ew_blocks, kBlock, 0, stm>>>((uint32_t)s->n, s->p, s->e32, mg ? 0 : 1, e_valid,
                                                                         pend ? s->d32[it & 1] : nullptr,
-                                                                        s->host->fb.alpha_even));
+                                                                        s->host->fb.alpha_even, s->p_stale ? 1 : 0));
+            s->p_stale = false;
         }
         if (mg && (status != ST_OK || stagnated || budget_hit)) {
             fallback = true;
@@ -3795,6 +3801,10 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
             break;
         }
         if (it == 0) break;  // no progress possible in FP32
+    }
+    if (s->p_stale) {  // no correction was added: the solution is the zero guess
+        FL_HIP_TRY(ctx, hipMemsetAsync(s->p, 0, s->n * sizeof(double), stm));
+        s->p_stale = false;
     }
     fl_solve_report fb{};
     const bool fallback_runs = fallback && total < s->cfg.pressure_max_iters;
@@ -3866,9 +3876,15 @@ fl_status solve_sor(fl_cavity3d *s, fl_solve_report *rep) {
 fl_status solve_pressure(fl_cavity3d *s, fl_solve_report *rep) {
     fl_ctx *ctx = s->ctx;
     hipStream_t stm = ctx->stream;
-    if (s->cfg.initial_guess == FL_GUESS_ZERO)
+    const bool zero = s->cfg.initial_guess == FL_GUESS_ZERO;
+    // the mixed multigrid path reads p only after its first correction
+    // (which writes it), unless a learned warm start adds to p first
+    const bool learned_warm = s->cfg.preconditioner == FL_PRECOND_LEARNED && s->unet && learned_level(s) == 0;
+    s->p_stale = zero && s->cfg.pressure_solver != FL_SOLVER_SOR && s->cfg.precision == FL_PRECISION_MIXED &&
+                 s->cfg.preconditioner != FL_PRECOND_JACOBI && !learned_warm;
+    if (zero && !s->p_stale)
         FL_HIP_TRY(ctx, hipMemsetAsync(s->p, 0, s->n * sizeof(double), stm));
-    else
+    else if (!zero)
         FL_HIP_TRY(ctx, hipMemsetAsync(s->p, 0, sizeof(double), stm));  // pin p(0,0,0) = 0
     s->cap_taken = false;
     s->p_zero = s->cfg.initial_guess == FL_GUESS_ZERO;
