@@ -2619,20 +2619,47 @@ __global__ void __launch_bounds__(kBlock) k_predict3_t(StepParams P, const doubl
 }
 
 // build_pressure_rhs (incompressible_3d.rs:275-291), rhs(0,0,0) = 0.
-__global__ void __launch_bounds__(kBlock) k_rhs(StepParams P, const double *__restrict__ us,
+//
+// kRes: the step's pressure solve starts from a zero guess in FP32-in-FP64
+// refinement, whose first residual is r = b = -rhs (b_0 = 0): it is written
+// here as FP32, with the residual kernel's partial row {r.z, r.r, sum r |
+// max|r|} per block, so that kernel (a second pass over rhs) is skipped.
+// The values are the residual kernel's bit for bit; only the summation
+// order of the partial sums differs.
+template <bool kRes>
+__global__ void __launch_bounds__(kBlock) k_rhs(StepParams P, Grid g, const double *__restrict__ us,
                                                 const double *__restrict__ vs,
                                                 const double *__restrict__ ws,
-                                                double *__restrict__ rhs) {
+                                                double *__restrict__ rhs, float *__restrict__ r32,
+                                                double *__restrict__ pout) {
     const uint32_t n = P.nx * P.ny * P.nz;
     const double scale = P.rho / P.dt;
+    const float *inv_tab = kRes ? inv_table<float>(g) : nullptr;
+    double sa[3] = {0.0, 0.0, 0.0}, ma[1] = {0.0};
     const XcdRange r(n);
     for (uint32_t c = r.first; c < r.end; c += r.step) {
         const int i = c % P.nx, j = (c / P.nx) % P.ny, k = c / (P.nx * P.ny);
-        rhs[c] = c == 0 ? 0.0
-                        : scale * (us[U3(i + 1, j, k)] - us[U3(i, j, k)]) / P.dx +
-                              scale * (vs[V3(i, j + 1, k)] - vs[V3(i, j, k)]) / P.dy +
-                              scale * (ws[W3(i, j, k + 1)] - ws[W3(i, j, k)]) / P.dz;
+        const double rv = c == 0 ? 0.0
+                                 : scale * (us[U3(i + 1, j, k)] - us[U3(i, j, k)]) / P.dx +
+                                       scale * (vs[V3(i, j + 1, k)] - vs[V3(i, j, k)]) / P.dy +
+                                       scale * (ws[W3(i, j, k + 1)] - ws[W3(i, j, k)]) / P.dz;
+        rhs[c] = rv;
+        if (kRes) {
+            float rf = 0.0f;
+            double inv = 1.0;
+            if (c != 0) {
+                rf = (float)(-rv);
+                inv = (double)inv_tab[diag_index(g, (uint32_t)i, (uint32_t)j, (uint32_t)k)];
+            }
+            r32[c] = rf;
+            const double rd = (double)rf;
+            sa[0] += rd * (rd * inv);
+            sa[1] += rd * rd;
+            sa[2] += rd;
+            ma[0] = fmax(ma[0], fabs(rd));
+        }
     }
+    if (kRes) publish_row<3, 1>(sa, ma, pout);
 }
 
 struct DiagState {
@@ -2970,6 +2997,10 @@ struct fl_cavity3d {
     // by the mixed multigrid path): the first correction writes p = e, and a
     // solve that ends before any correction clears p then
     bool p_stale = false;
+    // partial rows of the latest residual in pb; r_ready: the zero guess's
+    // FP32 residual is already in r32 (written with the right-hand side)
+    uint32_t rrows = 0;
+    bool r_ready = false;
     float *cap = nullptr;
     double *pz = nullptr;  // partials z.q_old, sum q_old
     // the flexible-CG state: one allocation, fb directly after fa (as in
@@ -3321,6 +3352,7 @@ fl_status run_cg(fl_cavity3d *s, const CgParams &prm, TX *x, T *r, T *const dbuf
 template <typename TR>
 void launch_residual(fl_cavity3d *s, TR *r) {
     const dim3 grid(s->zr.bx, s->zr.by, s->zr.bz);
+    s->rrows = s->zr.count;
     const bool z = s->p_zero;
     s->p_zero = false;
     hipStream_t stm = s->ctx->stream;
@@ -3400,7 +3432,7 @@ fl_status launch_fcg_iteration(fl_cavity3d *s, const CgParams &prm, long long k)
     const double n0 = L[0].n, n1 = L[1].n;
     // down: the first two pre-sweeps in one pass (into meb), the third into
     // mea, which the restriction and the up pass read
-    const uint32_t crows = k == 0 ? s->zr.count : G;
+    const uint32_t crows = k == 0 ? s->rrows : G;
     if (s->ztc.count > 0)
         FL_TIMED(s, K_MG_CHECK_T, k, 6.0 * n0,
                  k_mg_check_t<<<dim3(s->ztc.bx, s->ztc.by, s->ztc.bz), kBlock, 0, stm>>>(L[0], prm, s->r32, s->m0b, s->pb, crows, s->fa, s->fb));
@@ -3695,10 +3727,16 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
         warm_used = 0;
     double warm_ratio = 0.0;
     for (;;) {
-        FL_TIMED(s, K_RESIDUAL_F, -1, residual_bytes<float>(s),
-                 launch_residual<float>(s, s->r32));
-        FL_TIMED(s, K_FOLD_OUTER, -1, 32.0 * s->zr.count,
-                 k_fold_outer<<<1, kBlock, 0, stm>>>(s->pb, s->zr.count, s->outer));
+        if (s->r_ready) {  // r = b in r32 and its rows in pb, from the right-hand side kernel
+            s->r_ready = false;
+            s->p_zero = false;
+            s->rrows = s->ew_blocks;
+        } else {
+            FL_TIMED(s, K_RESIDUAL_F, -1, residual_bytes<float>(s),
+                     launch_residual<float>(s, s->r32));
+        }
+        FL_TIMED(s, K_FOLD_OUTER, -1, 32.0 * s->rrows,
+                 k_fold_outer<<<1, kBlock, 0, stm>>>(s->pb, s->rrows, s->outer));
         FL_HIP_TRY(ctx, hipGetLastError());
         FL_HIP_TRY(ctx, hipMemcpyAsync(s->host->outer, s->outer, 4 * sizeof(double), hipMemcpyDeviceToHost, stm));
         FL_HIP_TRY(ctx, hipStreamSynchronize(stm));
@@ -3772,9 +3810,9 @@ fl_status solve_mixed(fl_cavity3d *s, fl_solve_report *rep) {
             // pinned correction equation (row 0 with r_0 = -sum_{k!=0} r_k);
             // its solution, shifted to e_0 = 0, is the pinned correction, and
             // the Neumann operator lacks the pinned operator's near-null mode
-            k_row0_compat<<<1, 64, 0, stm>>>(s->g, s->outer, s->r32, s->pb + 4 * (size_t)s->zr.count);
+            k_row0_compat<<<1, 64, 0, stm>>>(s->g, s->outer, s->r32, s->pb + 4 * (size_t)s->rrows);
             const CgParams prm{itol, s->cfg.pressure_max_iters - total, FL_STOP_REL_L2, 0.0, 0.0, 1};
-            fl_status st = run_cg<float, float>(s, prm, s->e32, s->r32, s->d32, s->q32, s->zr.count + 1);
+            fl_status st = run_cg<float, float>(s, prm, s->e32, s->r32, s->d32, s->q32, s->rrows + 1);
             if (st != FL_OK) return st;
             const CgStateB &b = s->host->b;
             status = b.status != ST_OK ? b.status : s->host->a.status;
@@ -3936,8 +3974,15 @@ fl_status do_step(fl_cavity3d *s, fl_cavity3d_step_report *rep) {
     const double faces = 8.0 * (double)(s->nu + s->nv + s->nw), cells = 8.0 * s->n;
     FL_TIMED(s, K_PREDICT, -1, 2.0 * faces,
              k_predict3_t<<<dim3(s->zpt.bx, s->zpt.by, s->zpt.bz), kBlock, 0, stm>>>(s->P, s->u, s->v, s->w, s->us, s->vs, s->ws));
-    FL_TIMED(s, K_RHS, -1, faces + cells,
-             k_rhs<<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->us, s->vs, s->ws, s->rhs));
+    // the zero guess's FP32 residual with the right-hand side (see k_rhs)
+    s->r_ready = s->cfg.initial_guess == FL_GUESS_ZERO && s->cfg.precision == FL_PRECISION_MIXED &&
+                 s->cfg.pressure_solver != FL_SOLVER_SOR;
+    if (s->r_ready)
+        FL_TIMED(s, K_RHS, -1, faces + 1.5 * cells,
+                 k_rhs<true><<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->g, s->us, s->vs, s->ws, s->rhs, s->r32, s->pb));
+    else
+        FL_TIMED(s, K_RHS, -1, faces + cells,
+                 k_rhs<false><<<s->ew_blocks, kBlock, 0, stm>>>(s->P, s->g, s->us, s->vs, s->ws, s->rhs, nullptr, nullptr));
     FL_HIP_TRY(ctx, hipGetLastError());
     fl_solve_report sr{};
     fl_status st = solve_pressure(s, &sr);
