@@ -462,6 +462,29 @@ def test_multigrid_fallback_to_plain_pcg(gpu_ctx):
     assert rep.fallback == 0 and not rep.converged and rep.iterations == 2
 
 
+def test_multigrid_zero_guess_ignores_stale_pressure(gpu_ctx):
+    """A zero-guess solve on the mixed multigrid path does not clear p first
+    (its first correction writes p): a second solve on the same solver, with
+    the first solution still in device memory, gives the same bits, and a
+    zero right-hand side leaves p = 0 (no correction is ever added)."""
+    print("[gpu] MG-FCG stale p", flush=True)
+    n = 32
+    rng = np.random.default_rng(5)
+    rhs = rng.standard_normal((n, n, n))
+    rhs[0, 0, 0] = 0.0
+    s = _mg(gpu_ctx, n)
+    sols = []
+    for start in (np.zeros((n, n, n)), 1e3 * rng.standard_normal((n, n, n))):
+        p = start.copy()
+        rep = s.pressure_solve(rhs, p)
+        assert rep.converged and rep.fallback == 0
+        sols.append(p)
+    assert np.array_equal(sols[0], sols[1])
+    p = np.ones((n, n, n))
+    rep = s.pressure_solve(np.zeros((n, n, n)), p)
+    assert rep.iterations == 0 and rep.converged and np.all(p == 0.0)
+
+
 def test_multigrid_rejects_unsupported_grids(gpu_ctx):
     with pytest.raises(fa.FlursysError, match="halve"):
         _mg(gpu_ctx, (33, 20, 9))
