@@ -2638,7 +2638,9 @@ __global__ void __launch_bounds__(kBlock) k_rhs(StepParams P, Grid g, const doub
     double sa[3] = {0.0, 0.0, 0.0}, ma[1] = {0.0};
     const XcdRange r(n);
     for (uint32_t c = r.first; c < r.end; c += r.step) {
-        const int i = c % P.nx, j = (c / P.nx) % P.ny, k = c / (P.nx * P.ny);
+        uint32_t ui, uj, uk;
+        cell_ijk(g, c, ui, uj, uk);  // multiply-shift dividers, no runtime divides
+        const int i = (int)ui, j = (int)uj, k = (int)uk;
         const double rv = c == 0 ? 0.0
                                  : scale * (us[U3(i + 1, j, k)] - us[U3(i, j, k)]) / P.dx +
                                        scale * (vs[V3(i, j + 1, k)] - vs[V3(i, j, k)]) / P.dy +
